@@ -1,0 +1,213 @@
+/*
+ * talus_mi.h -- C ABI of libtalus_mi.so, the MI355X (gfx950) deformable tet
+ * step behind talus::World::step() and talus::PhysicalMaterial.
+ *
+ * Plain C: pointers and sizes only, no exceptions across the boundary.  Every
+ * call returns TALUS_MI_OK (0) or a negative status; talus_mi_last_error()
+ * gives the message of the calling thread's last failure.  One context owns
+ * one HIP stream and all device buffers; calls on a context must be
+ * serialized by the caller (src/talus README: "Access to a world must be
+ * serialized").  Results are bit-reproducible run to run: assembly sums in a
+ * fixed order and no floating-point atomics are used.
+ *
+ * Matrices are 3x3 row-major; node vectors are xyz-interleaved.
+ */
+#ifndef TALUS_MI_H
+#define TALUS_MI_H
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define TALUS_MI_OK 0
+#define TALUS_MI_ERR_ARGUMENT -1
+#define TALUS_MI_ERR_DEVICE -2
+#define TALUS_MI_ERR_STATE -3
+#define TALUS_MI_ERR_MEMORY -4
+
+/* Arithmetic of the element kernels and the inner solve. */
+#define TALUS_MI_FP64 0u  /* plain FP64 everywhere */
+#define TALUS_MI_MIXED 1u /* FP32 elements/PCG, FP64 Dm^-1, pivots, residuals */
+
+/* Preconditioner of the implicit solve. */
+#define TALUS_MI_BLOCK_JACOBI 0u /* node 3x3 diagonal inverse (solver.cc:1574) */
+#define TALUS_MI_FACTOR 1u       /* supernodal block Cholesky (factor.cc:549) */
+
+/* Step flags. */
+#define TALUS_MI_DETECT_TEARS 1u /* tear events and host topology update */
+#define TALUS_MI_RECORD_EVENTS 2u
+
+typedef struct talus_mi_ctx talus_mi_ctx;
+
+/* Same fields, order and meaning as talus::PhysicalMaterial
+ * (src/talus/material.h:12-29).  elastic_model: 0 Corotated, 1 NeoHookean.
+ * Lanes beyond the array use a default-constructed material
+ * (laneOf, src/talus/deformable/solver/solver.cc:30-37). */
+typedef struct talus_mi_material {
+  float friction;
+  float restitution;
+  float density;
+  float strength;
+  float youngs_modulus;
+  float poissons_ratio;
+  float stiffness_damping;
+  float yield_stress;
+  float hardening_modulus;
+  uint32_t elastic_model;
+  float hysteresis;
+  float damping_ratio;
+  float tear_strain;
+  float plastic_max;
+  float crush_stress;
+  float crush_limit;
+} talus_mi_material;
+
+/* Host SoA view of a tet PreparedMesh (src/talus/mesh.h:11-44).  Arrays are
+ * caller-owned and copied by talus_mi_body_add. */
+typedef struct talus_mi_mesh {
+  uint32_t nodes;
+  uint32_t tets;
+  float *rest_positions; /* 3*nodes */
+  float *node_mass;      /* nodes; NULL on input = rho*V/4 per tet in FP64 */
+  uint8_t *anchored;     /* nodes */
+  uint32_t *tet_nodes;   /* 4*tets, positively oriented */
+  float *rest_inverse;   /* 9*tets; output of the builder, not read on input */
+  float *volume;         /* tets;   output of the builder, not read on input */
+  uint32_t *tet_material;/* tets, material lane */
+} talus_mi_mesh;
+
+/* Structured lattice for the Kuhn tet builder (buildMesh, solver.cc:2185). */
+typedef struct talus_mi_lattice {
+  uint32_t resolution[3];   /* samples per axis = cells + 1 */
+  float voxel_size[3];
+  float origin[3];
+  const uint8_t *solid;     /* per cell, NULL = all solid (fill 1) */
+  const uint32_t *cell_lane;/* per cell, NULL = lane 0 */
+  const float *sample_positions; /* 3 per sample, NULL = origin+voxel*index */
+} talus_mi_lattice;
+
+/* SupportVolume (solver.h): anchored iff |world_to_support * p| <= half. */
+typedef struct talus_mi_support {
+  float world_to_support[12]; /* row-major 3x4 affine */
+  float half_extents[3];
+} talus_mi_support;
+
+/* Step parameters: deformable::Settings (solver.h:51-107) subset plus the
+ * IntegrationParameters dt and gravity (integrationparameters.h:9-33). */
+typedef struct talus_mi_step_params {
+  float dt;
+  float gravity[3];
+  uint32_t substeps;        /* default 1 */
+  uint32_t cg_iterations;   /* default 64 */
+  uint32_t factor_budget;   /* default 32 (cap with a factor) */
+  float tolerance;          /* default 1e-3, relative residual */
+  float spectral_radius;    /* default 0.5 */
+  uint32_t preconditioner;  /* TALUS_MI_BLOCK_JACOBI / TALUS_MI_FACTOR */
+  uint32_t flags;           /* TALUS_MI_DETECT_TEARS | TALUS_MI_RECORD_EVENTS */
+} talus_mi_step_params;
+
+/* deformable::StepReport subset (solver.h:211-311); filled per call. */
+typedef struct talus_mi_report {
+  uint32_t bodies;
+  uint32_t nodes;
+  uint32_t elements;
+  uint32_t substeps;        /* largest over bodies */
+  uint32_t cg_iterations;   /* summed over bodies and solves */
+  uint32_t cg_capped;
+  uint32_t solves;
+  uint32_t refactors;
+  double cg_worst_relative;
+  double elastic_energy;
+  double kinetic_energy;
+  uint32_t plastic_growth;
+  uint32_t yield_events;
+  uint32_t tear_events;
+  uint32_t split_nodes;
+  uint32_t non_finite;
+  uint32_t reserved;
+  double device_seconds;    /* hipEvent time of the whole step */
+} talus_mi_report;
+
+#define TALUS_MI_EVENT_YIELD 0u
+#define TALUS_MI_EVENT_TEAR 1u
+
+typedef struct talus_mi_event {
+  uint32_t kind;
+  uint32_t body;
+  uint32_t tet;
+  uint32_t step;
+} talus_mi_event;
+
+/* ---- context ---- */
+/* Creates a context on HIP device `device` computing in `precision`
+ * (TALUS_MI_FP64 / TALUS_MI_MIXED). */
+int talus_mi_create(talus_mi_ctx **out, int device, uint32_t precision);
+int talus_mi_destroy(talus_mi_ctx *ctx);
+const char *talus_mi_last_error(void);
+void talus_mi_default_params(talus_mi_step_params *params);
+
+/* ---- mesh (replaces deformable::buildMesh, solver.cc:2185-2418) ---- */
+/* Kuhn split of every solid cell into 6 tets, nodes numbered in first-touch
+ * z-y-x cell order; arrays are malloc'ed, release with talus_mi_mesh_free. */
+int talus_mi_build_lattice(const talus_mi_lattice *lattice,
+                           const talus_mi_material *materials, uint32_t lanes,
+                           const talus_mi_support *supports,
+                           uint32_t nsupports, talus_mi_mesh *out);
+void talus_mi_mesh_free(talus_mi_mesh *mesh);
+
+/* ---- bodies (the tet body set stepped inside World::step) ---- */
+int talus_mi_body_add(talus_mi_ctx *ctx, const talus_mi_mesh *mesh,
+                      const talus_mi_material *materials, uint32_t lanes,
+                      uint32_t *body);
+int talus_mi_body_info(talus_mi_ctx *ctx, uint32_t body, uint32_t *nodes,
+                       uint32_t *tets);
+/* DeformableState position/velocity/acceleration (mesh.h:56-62); NULL skips */
+int talus_mi_state_set(talus_mi_ctx *ctx, uint32_t body, const double *x,
+                       const double *v, const double *a);
+int talus_mi_state_get(talus_mi_ctx *ctx, uint32_t body, double *x, double *v,
+                       double *a);
+/* committed[e].rotation (9 per tet) and plastic points (10 per tet:
+ * corotated eps_p[9] + alpha, neo-Hookean (Fp^-1 - I)[9] + alpha) */
+int talus_mi_rotations_get(talus_mi_ctx *ctx, uint32_t body, double *R);
+int talus_mi_rotations_set(talus_mi_ctx *ctx, uint32_t body, const double *R);
+int talus_mi_plastic_get(talus_mi_ctx *ctx, uint32_t body, double *plastic);
+int talus_mi_plastic_set(talus_mi_ctx *ctx, uint32_t body,
+                         const double *plastic);
+/* Current connectivity, rest positions and masses (they change on tears). */
+int talus_mi_topology_get(talus_mi_ctx *ctx, uint32_t body,
+                          uint32_t *tet_nodes, double *rest, double *mass,
+                          uint8_t *anchored);
+
+/* ---- stepping (deformable::step, solver.cc:3472; World::step) ---- */
+int talus_mi_step(talus_mi_ctx *ctx, const talus_mi_step_params *params,
+                  talus_mi_report *report);
+/* Yield and tear events recorded since the last call (ascending tet order per
+ * body and step). */
+int talus_mi_events(talus_mi_ctx *ctx, talus_mi_event *out, uint32_t capacity,
+                    uint32_t *count);
+
+/* ---- kernel-level entry points (parity and roofline) ---- */
+/* Element kernel alone at positions x: per-tet corner forces (12 per tet),
+ * energies, assembled node forces, committed rotations (each may be NULL). */
+int talus_mi_eval_forces(talus_mi_ctx *ctx, uint32_t body, const double *x,
+                         double *tet_forces, double *tet_energy,
+                         double *node_forces, double *rotations);
+/* Matrix-free operator y = inertia*M*d + step^2 sum_e R K R^T d (applySystem,
+ * solver.cc:904-1048) with the committed rotations. */
+int talus_mi_apply_system(talus_mi_ctx *ctx, uint32_t body, double inertia,
+                          double step, const double *d, double *y);
+/* Times `repeats` launches of one kernel over every body at the current state
+ * with HIP events on the context stream: kind 0 element assembly, 1 operator
+ * element pass, 2 operator node gather.  avg_ms = mean launch duration. */
+int talus_mi_time_kernel(talus_mi_ctx *ctx, uint32_t kind, uint32_t repeats,
+                         double *avg_ms);
+/* The context's HIP stream (hipStream_t) for interop. */
+void *talus_mi_stream(talus_mi_ctx *ctx);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif

@@ -1,0 +1,371 @@
+"""FP64 CPU oracle for the Talus deformable tet step.
+
+TEST INFRASTRUCTURE ONLY.  Imported by tests/, by ``__graft_entry__.smoke()``
+and by the ``cpu_baseline`` leg of ``bench.py``; the product package
+``talus_amd`` never imports it.  The C library (``talus_oracle.c``) restates
+the reference algorithm; the file header there cites the reference lines each
+function follows.
+"""
+from __future__ import annotations
+
+import ctypes as C
+import os
+import subprocess
+
+import numpy as np
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(HERE, "liboracle.so")
+
+
+def build() -> None:
+    subprocess.run(["make", "-s", "-C", HERE], check=True)
+
+
+class Material(C.Structure):
+    """Field order of talus::PhysicalMaterial (src/talus/material.h:12-29)."""
+
+    _fields_ = [
+        ("friction", C.c_float), ("restitution", C.c_float),
+        ("density", C.c_float), ("strength", C.c_float),
+        ("youngs_modulus", C.c_float), ("poissons_ratio", C.c_float),
+        ("stiffness_damping", C.c_float), ("yield_stress", C.c_float),
+        ("hardening_modulus", C.c_float), ("elastic_model", C.c_uint32),
+        ("hysteresis", C.c_float), ("damping_ratio", C.c_float),
+        ("tear_strain", C.c_float), ("plastic_max", C.c_float),
+        ("crush_stress", C.c_float), ("crush_limit", C.c_float),
+    ]
+
+
+class Mesh(C.Structure):
+    _fields_ = [
+        ("nodes", C.c_uint32), ("tets", C.c_uint32),
+        ("rest", C.POINTER(C.c_double)), ("mass", C.POINTER(C.c_double)),
+        ("anchored", C.POINTER(C.c_uint8)), ("tet", C.POINTER(C.c_uint32)),
+        ("dminv", C.POINTER(C.c_double)), ("volume", C.POINTER(C.c_double)),
+        ("lane", C.POINTER(C.c_uint32)),
+    ]
+
+
+class Support(C.Structure):
+    _fields_ = [("world_to_support", C.c_float * 12), ("half", C.c_float * 3)]
+
+
+class Lattice(C.Structure):
+    _fields_ = [
+        ("res", C.c_uint32 * 3), ("voxel", C.c_float * 3),
+        ("origin", C.c_float * 3), ("solid", C.POINTER(C.c_uint8)),
+        ("cell_lane", C.POINTER(C.c_uint32)),
+        ("sample_pos", C.POINTER(C.c_float)),
+    ]
+
+
+class Settings(C.Structure):
+    _fields_ = [
+        ("dt", C.c_float), ("gravity", C.c_float * 3),
+        ("substeps", C.c_uint32), ("cg_iterations", C.c_uint32),
+        ("tolerance", C.c_float), ("spectral_radius", C.c_float),
+        ("threads", C.c_uint32),
+    ]
+
+
+class Report(C.Structure):
+    _fields_ = [
+        ("substeps", C.c_uint32), ("cg_iterations", C.c_uint32),
+        ("cg_capped", C.c_uint32), ("solves", C.c_uint32),
+        ("cg_worst_relative", C.c_double), ("elastic_energy", C.c_double),
+        ("kinetic_energy", C.c_double), ("plastic_growth", C.c_uint32),
+        ("yield_events", C.c_uint32), ("non_finite", C.c_uint32),
+    ]
+
+
+_lib = None
+
+
+def lib():
+    global _lib
+    if _lib is None:
+        if not os.path.exists(LIB_PATH):
+            build()
+        L = C.CDLL(LIB_PATH)
+        dp = C.POINTER(C.c_double)
+        L.orc_build_lattice.argtypes = [C.POINTER(Lattice), C.POINTER(Material), C.c_uint32,
+                                        C.POINTER(Support), C.c_uint32, C.POINTER(Mesh)]
+        L.orc_mesh_free.argtypes = [C.POINTER(Mesh)]
+        L.orc_lame.argtypes = [C.c_float, C.c_float, dp, dp]
+        L.orc_det3.argtypes = [dp]
+        L.orc_det3.restype = C.c_double
+        L.orc_polar.argtypes = [dp, dp]
+        L.orc_element_rotation.argtypes = [dp, dp]
+        L.orc_snh_tet.argtypes = [dp, dp, C.c_double, C.c_double, C.c_double, dp, dp, dp]
+        L.orc_corotated_tet.argtypes = [dp, dp, dp, C.c_double, C.c_double, C.c_double, dp,
+                                        C.c_double, C.c_double, dp, dp, dp, dp,
+                                        C.POINTER(C.c_uint32)]
+        L.orc_hencky_flow.argtypes = [dp, C.c_double, C.c_double, C.c_double, C.c_double, dp, dp]
+        L.orc_hencky_flow.restype = C.c_int
+        L.orc_stretch.argtypes = [dp, dp]
+        L.orc_stretch.restype = C.c_double
+        L.orc_eigen_sym3.argtypes = [dp, dp, dp]
+        L.orc_body_create.argtypes = [C.POINTER(Mesh), C.POINTER(Material), C.c_uint32]
+        L.orc_body_create.restype = C.c_void_p
+        L.orc_body_free.argtypes = [C.c_void_p]
+        L.orc_body_set_state.argtypes = [C.c_void_p, dp, dp, dp]
+        L.orc_body_get_state.argtypes = [C.c_void_p, dp, dp, dp]
+        L.orc_body_get_rotations.argtypes = [C.c_void_p, dp]
+        L.orc_body_set_rotations.argtypes = [C.c_void_p, dp]
+        L.orc_body_get_plastic.argtypes = [C.c_void_p, dp]
+        L.orc_body_set_plastic.argtypes = [C.c_void_p, dp]
+        L.orc_step.argtypes = [C.c_void_p, C.POINTER(Settings), C.POINTER(Report)]
+        L.orc_eval_forces.argtypes = [C.c_void_p, dp, C.c_uint32, dp, dp, dp, dp]
+        L.orc_apply_system.argtypes = [C.c_void_p, C.c_double, C.c_double, dp, dp, C.c_uint32]
+        L.orc_tet_stretch.argtypes = [C.c_void_p, dp]
+        L.orc_yield_flags.argtypes = [C.c_void_p, C.POINTER(C.c_uint8)]
+        _lib = L
+    return _lib
+
+
+def _dp(a):
+    if a is None:
+        return None
+    assert a.dtype == np.float64 and a.flags.c_contiguous
+    return a.ctypes.data_as(C.POINTER(C.c_double))
+
+
+def materials_array(materials):
+    """ctypes array of Material, or a pointer into a numpy record array with
+    the same 64-byte layout."""
+    if isinstance(materials, np.ndarray):
+        assert materials.dtype.itemsize == C.sizeof(Material)
+        arr = np.ascontiguousarray(materials)
+        ptr = arr.ctypes.data_as(C.POINTER(Material))
+        ptr._keep = arr
+        return ptr
+    arr = (Material * max(len(materials), 1))()
+    for i, m in enumerate(materials):
+        arr[i] = m
+    return arr
+
+
+def build_lattice(res, voxel, origin=(0.0, 0.0, 0.0), materials=(), supports=(),
+                  sample_pos=None, cell_lane=None, solid=None):
+    """Kuhn tet mesh of a structured lattice; returns a dict of numpy arrays."""
+    L = lib()
+    lat = Lattice()
+    lat.res[:] = [int(r) for r in res]
+    lat.voxel[:] = [float(v) for v in voxel]
+    lat.origin[:] = [float(v) for v in origin]
+    keep = []
+    if sample_pos is not None:
+        sp = np.ascontiguousarray(sample_pos, dtype=np.float32).reshape(-1)
+        keep.append(sp)
+        lat.sample_pos = sp.ctypes.data_as(C.POINTER(C.c_float))
+    if cell_lane is not None:
+        cl = np.ascontiguousarray(cell_lane, dtype=np.uint32)
+        keep.append(cl)
+        lat.cell_lane = cl.ctypes.data_as(C.POINTER(C.c_uint32))
+    if solid is not None:
+        so = np.ascontiguousarray(solid, dtype=np.uint8)
+        keep.append(so)
+        lat.solid = so.ctypes.data_as(C.POINTER(C.c_uint8))
+    mats = materials_array(materials if isinstance(materials, np.ndarray) else list(materials))
+    sups = (Support * max(len(supports), 1))()
+    for i, s in enumerate(supports):
+        sups[i] = s
+    out = Mesh()
+    rc = L.orc_build_lattice(C.byref(lat), mats, len(materials), sups, len(supports), C.byref(out))
+    if rc != 0:
+        raise RuntimeError("oracle mesh build failed")
+    n, t = out.nodes, out.tets
+    mesh = {
+        "rest": np.ctypeslib.as_array(out.rest, (3 * n,)).reshape(n, 3).copy(),
+        "mass": np.ctypeslib.as_array(out.mass, (n,)).copy(),
+        "anchored": np.ctypeslib.as_array(out.anchored, (n,)).copy(),
+        "tet": np.ctypeslib.as_array(out.tet, (4 * t,)).reshape(t, 4).copy(),
+        "dminv": np.ctypeslib.as_array(out.dminv, (9 * t,)).reshape(t, 3, 3).copy(),
+        "volume": np.ctypeslib.as_array(out.volume, (t,)).copy(),
+        "lane": np.ctypeslib.as_array(out.lane, (t,)).copy(),
+    }
+    L.orc_mesh_free(C.byref(out))
+    return mesh
+
+
+def _mesh_struct(mesh, keep):
+    m = Mesh()
+    arrs = {
+        "rest": np.ascontiguousarray(mesh["rest"], dtype=np.float64).reshape(-1),
+        "mass": np.ascontiguousarray(mesh["mass"], dtype=np.float64),
+        "anchored": np.ascontiguousarray(mesh["anchored"], dtype=np.uint8),
+        "tet": np.ascontiguousarray(mesh["tet"], dtype=np.uint32).reshape(-1),
+        "dminv": np.ascontiguousarray(mesh["dminv"], dtype=np.float64).reshape(-1),
+        "volume": np.ascontiguousarray(mesh["volume"], dtype=np.float64),
+        "lane": np.ascontiguousarray(mesh["lane"], dtype=np.uint32),
+    }
+    keep.append(arrs)
+    m.nodes = arrs["mass"].shape[0]
+    m.tets = arrs["volume"].shape[0]
+    m.rest = arrs["rest"].ctypes.data_as(C.POINTER(C.c_double))
+    m.mass = arrs["mass"].ctypes.data_as(C.POINTER(C.c_double))
+    m.anchored = arrs["anchored"].ctypes.data_as(C.POINTER(C.c_uint8))
+    m.tet = arrs["tet"].ctypes.data_as(C.POINTER(C.c_uint32))
+    m.dminv = arrs["dminv"].ctypes.data_as(C.POINTER(C.c_double))
+    m.volume = arrs["volume"].ctypes.data_as(C.POINTER(C.c_double))
+    m.lane = arrs["lane"].ctypes.data_as(C.POINTER(C.c_uint32))
+    return m
+
+
+class Body:
+    """One deformable tet body stepped by the oracle."""
+
+    def __init__(self, mesh, materials):
+        L = lib()
+        keep = []
+        ms = _mesh_struct(mesh, keep)
+        mats = materials_array(materials if isinstance(materials, np.ndarray) else list(materials))
+        self._h = L.orc_body_create(C.byref(ms), mats, len(materials))
+        self.nodes = int(ms.nodes)
+        self.tets = int(ms.tets)
+
+    def __del__(self):
+        if getattr(self, "_h", None):
+            lib().orc_body_free(self._h)
+            self._h = None
+
+    def set_state(self, x=None, v=None, a=None):
+        f = lambda z: None if z is None else np.ascontiguousarray(z, dtype=np.float64).reshape(-1)
+        x, v, a = f(x), f(v), f(a)
+        lib().orc_body_set_state(self._h, _dp(x), _dp(v), _dp(a))
+
+    def state(self):
+        x = np.empty(3 * self.nodes)
+        v = np.empty(3 * self.nodes)
+        a = np.empty(3 * self.nodes)
+        lib().orc_body_get_state(self._h, _dp(x), _dp(v), _dp(a))
+        return x.reshape(-1, 3), v.reshape(-1, 3), a.reshape(-1, 3)
+
+    def rotations(self):
+        R = np.empty(9 * self.tets)
+        lib().orc_body_get_rotations(self._h, _dp(R))
+        return R.reshape(-1, 3, 3)
+
+    def set_rotations(self, R):
+        R = np.ascontiguousarray(R, dtype=np.float64).reshape(-1)
+        lib().orc_body_set_rotations(self._h, _dp(R))
+
+    def plastic(self):
+        p = np.empty(10 * self.tets)
+        lib().orc_body_get_plastic(self._h, _dp(p))
+        return p.reshape(-1, 10)
+
+    def set_plastic(self, p):
+        p = np.ascontiguousarray(p, dtype=np.float64).reshape(-1)
+        lib().orc_body_set_plastic(self._h, _dp(p))
+
+    def step(self, dt=1.0 / 60.0, gravity=(0.0, -9.81, 0.0), substeps=1, cg_iterations=64,
+             tolerance=1e-3, spectral_radius=0.5, threads=1):
+        s = Settings()
+        s.dt = dt
+        s.gravity[:] = list(gravity)
+        s.substeps = substeps
+        s.cg_iterations = cg_iterations
+        s.tolerance = tolerance
+        s.spectral_radius = spectral_radius
+        s.threads = threads
+        r = Report()
+        lib().orc_step(self._h, C.byref(s), C.byref(r))
+        return {k: getattr(r, k) for k, _ in Report._fields_}
+
+    def eval_forces(self, x, threads=1):
+        x = np.ascontiguousarray(x, dtype=np.float64).reshape(-1)
+        tf = np.empty(12 * self.tets)
+        te = np.empty(self.tets)
+        nf = np.empty(3 * self.nodes)
+        R = np.empty(9 * self.tets)
+        lib().orc_eval_forces(self._h, _dp(x), threads, _dp(tf), _dp(te), _dp(nf), _dp(R))
+        return tf.reshape(-1, 4, 3), te, nf.reshape(-1, 3), R.reshape(-1, 3, 3)
+
+    def apply_system(self, inertia, step, d, threads=1):
+        d = np.ascontiguousarray(d, dtype=np.float64).reshape(-1)
+        y = np.empty_like(d)
+        lib().orc_apply_system(self._h, inertia, step, _dp(d), _dp(y), threads)
+        return y.reshape(-1, 3)
+
+    def stretch(self):
+        s = np.empty(self.tets)
+        lib().orc_tet_stretch(self._h, _dp(s))
+        return s
+
+    def yield_flags(self):
+        f = np.empty(self.tets, dtype=np.uint8)
+        lib().orc_yield_flags(self._h, f.ctypes.data_as(C.POINTER(C.c_uint8)))
+        return f
+
+
+# ---- per-element kernels, for closed-form and finite-difference checks ----
+
+def lame(E, nu):
+    lam, mu = C.c_double(), C.c_double()
+    lib().orc_lame(E, nu, C.byref(lam), C.byref(mu))
+    return lam.value, mu.value
+
+
+def snh_tet(u, dminv, volume, lam, mu, plastic_offset=None):
+    u = np.ascontiguousarray(u, dtype=np.float64).reshape(-1)
+    dm = np.ascontiguousarray(dminv, dtype=np.float64).reshape(-1)
+    ep = None if plastic_offset is None else np.ascontiguousarray(plastic_offset, dtype=np.float64).reshape(-1)
+    f = np.empty(12)
+    e = C.c_double()
+    lib().orc_snh_tet(_dp(u), _dp(dm), volume, lam, mu, _dp(ep), _dp(f), C.byref(e))
+    return f.reshape(4, 3), e.value
+
+
+def corotated_tet(x, X, dminv, volume, lam, mu, plastic=None, yield_stress=np.inf, hardening=0.0):
+    x = np.ascontiguousarray(x, dtype=np.float64).reshape(-1)
+    X = np.ascontiguousarray(X, dtype=np.float64).reshape(-1)
+    dm = np.ascontiguousarray(dminv, dtype=np.float64).reshape(-1)
+    pl = np.zeros(10) if plastic is None else np.ascontiguousarray(plastic, dtype=np.float64).reshape(-1)
+    R = np.empty(9)
+    f = np.empty(12)
+    e = C.c_double()
+    nxt = np.empty(10)
+    y = C.c_uint32()
+    lib().orc_corotated_tet(_dp(x), _dp(X), _dp(dm), volume, lam, mu, _dp(pl), yield_stress, hardening,
+                            _dp(R), _dp(f), C.byref(e), _dp(nxt), C.byref(y))
+    return {"R": R.reshape(3, 3), "force": f.reshape(4, 3), "energy": e.value,
+            "plastic": nxt, "yielding": y.value}
+
+
+def polar(F):
+    F = np.ascontiguousarray(F, dtype=np.float64).reshape(-1)
+    R = np.empty(9)
+    lib().orc_polar(_dp(F), _dp(R))
+    return R.reshape(3, 3)
+
+
+def element_rotation(F):
+    F = np.ascontiguousarray(F, dtype=np.float64).reshape(-1)
+    R = np.empty(9)
+    lib().orc_element_rotation(_dp(F), _dp(R))
+    return R.reshape(3, 3)
+
+
+def hencky_flow(F, mu, yield_stress, hardening, plastic_max=np.inf, offset=None, alpha=0.0):
+    F = np.ascontiguousarray(F, dtype=np.float64).reshape(-1)
+    ep = np.zeros(9) if offset is None else np.ascontiguousarray(offset, dtype=np.float64).reshape(-1).copy()
+    al = C.c_double(alpha)
+    flowed = lib().orc_hencky_flow(_dp(F), mu, yield_stress, hardening, plastic_max, _dp(ep), C.byref(al))
+    return bool(flowed), ep.reshape(3, 3), al.value
+
+
+def stretch(F):
+    F = np.ascontiguousarray(F, dtype=np.float64).reshape(-1)
+    d = np.empty(3)
+    s = lib().orc_stretch(_dp(F), _dp(d))
+    return s, d
+
+
+def eigen_sym3(A):
+    A = np.ascontiguousarray(A, dtype=np.float64).reshape(-1)
+    w = np.empty(3)
+    V = np.empty(9)
+    lib().orc_eigen_sym3(_dp(A), _dp(w), _dp(V))
+    return w, V.reshape(3, 3)

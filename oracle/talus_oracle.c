@@ -1,0 +1,1388 @@
+/*
+ * talus_oracle.c -- FP64 CPU restatement of the Talus deformable tet step.
+ *
+ * TEST INFRASTRUCTURE ONLY (the checker): loaded by tests/, by
+ * __graft_entry__.smoke() and by bench.py's cpu_baseline leg.  The product
+ * library never links or calls it.
+ *
+ * What each part restates (reference paths relative to the Talus tree):
+ *   mesh build      src/talus/deformable/solver/solver.cc:17-24 (KUHN),
+ *                   76-78 (cornerOffset), 163-180 (supported), 2185-2418
+ *   Lame moduli     src/talus/deformable/factor/factor.cc:21-30 (preciseLame)
+ *   polar rotation  src/talus/deformable/material/material.cc:168-197
+ *   inverted frame  material.cc:24-64 (elementRotation, Invertible mode)
+ *   corotated tet   material.cc:136-164, 236-265 (tetStrain, trial)
+ *   J2 return map   material.cc:594-630, 978-1031 (returnMap, tetPointTrial)
+ *   SNH law         src/talus/dynamics/neohookean/neohookean.cc:41-47,
+ *                   183-196; material.cc:886-946 (H-form first Piola)
+ *   Hencky flow     src/talus/dynamics/strainstep/plasticflow.cc:96-260
+ *   tear stretch    src/talus/dynamics/strainstep/tearing.cc:248-290, 469-483
+ *   operator        solver.cc:904-1048 (applySystem)
+ *   block Jacobi    material.cc:290-304, solver.cc:1574-1593
+ *   PCG             solver.cc:1904-2089 (conjugateGradient)
+ *   step            solver.cc:531-623, 3472-3950, 4351-4368, 5829-5849,
+ *                   1260-1346 (commitPlastic)
+ *
+ * Where the restatement deliberately differs from the reference, the
+ * difference is documented at the function.  Everything is FP64 and
+ * deterministic: element results go to per-(tet,corner) slots and nodes sum
+ * their slots in fixed incidence order, and dot products are summed in fixed
+ * blocks, so results do not depend on the thread count.
+ */
+#include "talus_oracle.h"
+
+#include <float.h>
+#include <math.h>
+#include <pthread.h>
+#include <stdlib.h>
+#include <string.h>
+
+/* ------------------------------------------------------------------ */
+/* 3x3 helpers, row-major                                               */
+/* ------------------------------------------------------------------ */
+
+double orc_det3(const double *a) {
+  return a[0] * (a[4] * a[8] - a[5] * a[7]) - a[1] * (a[3] * a[8] - a[5] * a[6]) +
+         a[2] * (a[3] * a[7] - a[4] * a[6]);
+}
+
+/* cofactor matrix: cof(A) = det(A) A^-T */
+static void cof3(const double *a, double *c) {
+  c[0] = a[4] * a[8] - a[5] * a[7];
+  c[1] = a[5] * a[6] - a[3] * a[8];
+  c[2] = a[3] * a[7] - a[4] * a[6];
+  c[3] = a[2] * a[7] - a[1] * a[8];
+  c[4] = a[0] * a[8] - a[2] * a[6];
+  c[5] = a[1] * a[6] - a[0] * a[7];
+  c[6] = a[1] * a[5] - a[2] * a[4];
+  c[7] = a[2] * a[3] - a[0] * a[5];
+  c[8] = a[0] * a[4] - a[1] * a[3];
+}
+
+static int inv3(const double *a, double *out) {
+  double c[9];
+  cof3(a, c);
+  const double d = a[0] * c[0] + a[1] * c[1] + a[2] * c[2];
+  if (d == 0.0) return 0;
+  const double s = 1.0 / d;
+  /* A^-1 = cof(A)^T / det */
+  out[0] = c[0] * s; out[1] = c[3] * s; out[2] = c[6] * s;
+  out[3] = c[1] * s; out[4] = c[4] * s; out[5] = c[7] * s;
+  out[6] = c[2] * s; out[7] = c[5] * s; out[8] = c[8] * s;
+  return 1;
+}
+
+static void mul3(const double *a, const double *b, double *c) {
+  double t[9];
+  for (int r = 0; r < 3; ++r)
+    for (int k = 0; k < 3; ++k)
+      t[r * 3 + k] = a[r * 3 + 0] * b[0 * 3 + k] + a[r * 3 + 1] * b[1 * 3 + k] +
+                     a[r * 3 + 2] * b[2 * 3 + k];
+  memcpy(c, t, sizeof t);
+}
+
+/* c = a^T b */
+static void mul3_tn(const double *a, const double *b, double *c) {
+  double t[9];
+  for (int r = 0; r < 3; ++r)
+    for (int k = 0; k < 3; ++k)
+      t[r * 3 + k] = a[0 * 3 + r] * b[0 * 3 + k] + a[1 * 3 + r] * b[1 * 3 + k] +
+                     a[2 * 3 + r] * b[2 * 3 + k];
+  memcpy(c, t, sizeof t);
+}
+
+/* c = a b^T */
+static void mul3_nt(const double *a, const double *b, double *c) {
+  double t[9];
+  for (int r = 0; r < 3; ++r)
+    for (int k = 0; k < 3; ++k)
+      t[r * 3 + k] = a[r * 3 + 0] * b[k * 3 + 0] + a[r * 3 + 1] * b[k * 3 + 1] +
+                     a[r * 3 + 2] * b[k * 3 + 2];
+  memcpy(c, t, sizeof t);
+}
+
+static void mv3(const double *a, const double *v, double *o) {
+  const double x = v[0], y = v[1], z = v[2];
+  o[0] = a[0] * x + a[1] * y + a[2] * z;
+  o[1] = a[3] * x + a[4] * y + a[5] * z;
+  o[2] = a[6] * x + a[7] * y + a[8] * z;
+}
+
+static void mtv3(const double *a, const double *v, double *o) {
+  const double x = v[0], y = v[1], z = v[2];
+  o[0] = a[0] * x + a[3] * y + a[6] * z;
+  o[1] = a[1] * x + a[4] * y + a[7] * z;
+  o[2] = a[2] * x + a[5] * y + a[8] * z;
+}
+
+static double trace3(const double *a) { return a[0] + a[4] + a[8]; }
+
+static double contract3(const double *a, const double *b) {
+  double s = 0.0;
+  for (int i = 0; i < 9; ++i) s += a[i] * b[i];
+  return s;
+}
+
+static void ident3(double *a) {
+  memset(a, 0, 9 * sizeof(double));
+  a[0] = a[4] = a[8] = 1.0;
+}
+
+/* shape gradients g0..g3 from the rows of Dm^-1 (material.cc:199-207) */
+static void shape_gradients(const double *dminv, double g[4][3]) {
+  for (int c = 0; c < 3; ++c) {
+    g[1][c] = dminv[0 * 3 + c];
+    g[2][c] = dminv[1 * 3 + c];
+    g[3][c] = dminv[2 * 3 + c];
+    g[0][c] = -(g[1][c] + g[2][c] + g[3][c]);
+  }
+}
+
+/* displacement gradient H = [u1-u0, u2-u0, u3-u0] Dm^-1 */
+static void displacement_gradient(const double *u, const double *dminv,
+                                  double *H) {
+  double d[3][3]; /* d[k] = u_{k+1} - u_0 */
+  for (int k = 0; k < 3; ++k)
+    for (int r = 0; r < 3; ++r) d[k][r] = u[(k + 1) * 3 + r] - u[r];
+  for (int r = 0; r < 3; ++r)
+    for (int c = 0; c < 3; ++c)
+      H[r * 3 + c] = d[0][r] * dminv[0 * 3 + c] + d[1][r] * dminv[1 * 3 + c] +
+                     d[2][r] * dminv[2 * 3 + c];
+}
+
+/* ------------------------------------------------------------------ */
+/* Symmetric 3x3 eigen decomposition: cyclic Jacobi to full precision.  */
+/* values ascending; vectors row-major with eigenvector i in column i.  */
+/* Stands in for Eigen::SelfAdjointEigenSolver (plasticflow.cc:97,111;  */
+/* tearing.cc:278), whose results are unique up to eigenvector sign.    */
+/* ------------------------------------------------------------------ */
+void orc_eigen_sym3(const double *A, double *values, double *vectors) {
+  double a[9];
+  memcpy(a, A, sizeof a);
+  a[1] = a[3] = 0.5 * (A[1] + A[3]);
+  a[2] = a[6] = 0.5 * (A[2] + A[6]);
+  a[5] = a[7] = 0.5 * (A[5] + A[7]);
+  double v[9];
+  ident3(v);
+  for (int sweep = 0; sweep < 64; ++sweep) {
+    const double off = a[1] * a[1] + a[2] * a[2] + a[5] * a[5];
+    const double diag = a[0] * a[0] + a[4] * a[4] + a[8] * a[8];
+    if (off <= 1e-36 * diag || off == 0.0) break;
+    static const int P[3] = {0, 0, 1}, Q[3] = {1, 2, 2};
+    for (int k = 0; k < 3; ++k) {
+      const int p = P[k], q = Q[k];
+      const double apq = a[p * 3 + q];
+      if (apq == 0.0) continue;
+      const double app = a[p * 3 + p], aqq = a[q * 3 + q];
+      const double theta = (aqq - app) / (2.0 * apq);
+      const double t = (theta >= 0.0 ? 1.0 : -1.0) /
+                       (fabs(theta) + sqrt(theta * theta + 1.0));
+      const double c = 1.0 / sqrt(t * t + 1.0), s = t * c;
+      /* A' = J^T A J with J the (p,q) rotation */
+      for (int r = 0; r < 3; ++r) {
+        const double arp = a[r * 3 + p], arq = a[r * 3 + q];
+        a[r * 3 + p] = c * arp - s * arq;
+        a[r * 3 + q] = s * arp + c * arq;
+      }
+      for (int r = 0; r < 3; ++r) {
+        const double apr = a[p * 3 + r], aqr = a[q * 3 + r];
+        a[p * 3 + r] = c * apr - s * aqr;
+        a[q * 3 + r] = s * apr + c * aqr;
+      }
+      for (int r = 0; r < 3; ++r) {
+        const double vrp = v[r * 3 + p], vrq = v[r * 3 + q];
+        v[r * 3 + p] = c * vrp - s * vrq;
+        v[r * 3 + q] = s * vrp + c * vrq;
+      }
+    }
+  }
+  double w[3] = {a[0], a[4], a[8]};
+  int order[3] = {0, 1, 2};
+  for (int i = 0; i < 3; ++i)
+    for (int j = i + 1; j < 3; ++j)
+      if (w[order[j]] < w[order[i]]) {
+        const int t = order[i];
+        order[i] = order[j];
+        order[j] = t;
+      }
+  for (int i = 0; i < 3; ++i) {
+    values[i] = w[order[i]];
+    for (int r = 0; r < 3; ++r) vectors[r * 3 + i] = v[r * 3 + order[i]];
+  }
+}
+
+/* ------------------------------------------------------------------ */
+/* Materials                                                            */
+/* ------------------------------------------------------------------ */
+
+/* preciseLame (factor.cc:21-30): nu clamped to [0, 0.49], FP64. */
+void orc_lame(float youngs, float poisson, double *lambda, double *mu) {
+  double nu = (double)poisson;
+  if (!(nu > 0.0)) nu = 0.0;
+  if (nu > 0.49) nu = 0.49;
+  const double e = (double)youngs;
+  *lambda = e * nu / ((1.0 + nu) * (1.0 - 2.0 * nu));
+  *mu = e / (2.0 * (1.0 + nu));
+}
+
+static const orc_material *lane_of(const orc_material *materials,
+                                   uint32_t lanes, uint32_t lane,
+                                   orc_material *fallback) {
+  /* laneOf (solver.cc:30-37): out-of-range lane -> default material */
+  if (lane < lanes) return &materials[lane];
+  memset(fallback, 0, sizeof *fallback);
+  fallback->friction = 0.5f;
+  fallback->density = 1000.0f;
+  fallback->strength = INFINITY;
+  fallback->youngs_modulus = 0.0f;
+  fallback->poissons_ratio = 0.3f;
+  fallback->yield_stress = INFINITY;
+  fallback->elastic_model = 0u;
+  fallback->damping_ratio = INFINITY;
+  fallback->tear_strain = INFINITY;
+  fallback->plastic_max = INFINITY;
+  fallback->crush_stress = INFINITY;
+  fallback->crush_limit = 1.0f;
+  return fallback;
+}
+
+/* yields (material.cc:752-754): finite and positive yield stress */
+static int yields(float yield) { return isfinite(yield) && yield > 0.0f; }
+
+/* ------------------------------------------------------------------ */
+/* Mesh build (buildMesh for fully solid lattice cells)                 */
+/* ------------------------------------------------------------------ */
+
+static const uint32_t KUHN[6][4] = {{0, 1, 3, 7}, {0, 3, 2, 7}, {0, 2, 6, 7},
+                                    {0, 6, 4, 7}, {0, 4, 5, 7}, {0, 5, 1, 7}};
+
+int orc_build_lattice(const orc_lattice *L, const orc_material *materials,
+                      uint32_t lanes, const orc_support *supports,
+                      uint32_t nsupports, orc_mesh *out) {
+  memset(out, 0, sizeof *out);
+  if (L->res[0] < 2 || L->res[1] < 2 || L->res[2] < 2) return -1;
+  const uint32_t cx = L->res[0] - 1, cy = L->res[1] - 1, cz = L->res[2] - 1;
+  const size_t samples = (size_t)L->res[0] * L->res[1] * L->res[2];
+  uint32_t *node_of = (uint32_t *)malloc(samples * sizeof(uint32_t));
+  for (size_t s = 0; s < samples; ++s) node_of[s] = UINT32_MAX;
+  uint32_t nodes = 0;
+  size_t solid_cells = 0;
+  /* first-touch node numbering in z, y, x cell scan, corners 0..7 */
+  for (uint32_t z = 0; z < cz; ++z)
+    for (uint32_t y = 0; y < cy; ++y)
+      for (uint32_t x = 0; x < cx; ++x) {
+        const size_t cell = x + (size_t)cx * (y + (size_t)cy * z);
+        if (L->solid && !L->solid[cell]) continue;
+        ++solid_cells;
+        for (uint32_t c = 0; c < 8; ++c) {
+          const uint32_t ax = x + (c & 1u), ay = y + ((c >> 1) & 1u),
+                         az = z + ((c >> 2) & 1u);
+          const size_t s = ax + (size_t)L->res[0] * (ay + (size_t)L->res[1] * az);
+          if (node_of[s] == UINT32_MAX) node_of[s] = nodes++;
+        }
+      }
+  if (nodes == 0) {
+    free(node_of);
+    return -1;
+  }
+  out->nodes = nodes;
+  out->rest = (double *)calloc(3 * (size_t)nodes, sizeof(double));
+  out->mass = (double *)calloc(nodes, sizeof(double));
+  out->anchored = (uint8_t *)calloc(nodes, 1);
+  out->tet = (uint32_t *)malloc(4 * 6 * solid_cells * sizeof(uint32_t));
+  out->dminv = (double *)malloc(9 * 6 * solid_cells * sizeof(double));
+  out->volume = (double *)malloc(6 * solid_cells * sizeof(double));
+  out->lane = (uint32_t *)malloc(6 * solid_cells * sizeof(uint32_t));
+  float *restf = (float *)malloc(3 * (size_t)nodes * sizeof(float));
+  for (uint32_t az = 0; az < L->res[2]; ++az)
+    for (uint32_t ay = 0; ay < L->res[1]; ++ay)
+      for (uint32_t ax = 0; ax < L->res[0]; ++ax) {
+        const size_t s = ax + (size_t)L->res[0] * (ay + (size_t)L->res[1] * az);
+        const uint32_t n = node_of[s];
+        if (n == UINT32_MAX) continue;
+        float p[3];
+        if (L->sample_pos) {
+          p[0] = L->sample_pos[3 * s + 0];
+          p[1] = L->sample_pos[3 * s + 1];
+          p[2] = L->sample_pos[3 * s + 2];
+        } else {
+          p[0] = L->origin[0] + L->voxel[0] * (float)ax;
+          p[1] = L->origin[1] + L->voxel[1] * (float)ay;
+          p[2] = L->origin[2] + L->voxel[2] * (float)az;
+        }
+        for (int k = 0; k < 3; ++k) {
+          restf[3 * n + k] = p[k];
+          out->rest[3 * n + k] = (double)p[k];
+        }
+      }
+  /* anchored: inside a support volume (solver.cc:163-180, 2265-2268) */
+  for (uint32_t n = 0; n < nodes; ++n) {
+    for (uint32_t v = 0; v < nsupports; ++v) {
+      const float *M = supports[v].world_to_support;
+      int inside = 1;
+      for (int r = 0; r < 3 && inside; ++r) {
+        const float l = M[r * 4 + 0] * restf[3 * n + 0] +
+                        M[r * 4 + 1] * restf[3 * n + 1] +
+                        M[r * 4 + 2] * restf[3 * n + 2] + M[r * 4 + 3];
+        if (!(fabsf(l) <= supports[v].half[r])) inside = 0;
+      }
+      if (inside) {
+        out->anchored[n] = 1;
+        break;
+      }
+    }
+  }
+  uint32_t tets = 0;
+  orc_material fb;
+  for (uint32_t z = 0; z < cz; ++z)
+    for (uint32_t y = 0; y < cy; ++y)
+      for (uint32_t x = 0; x < cx; ++x) {
+        const size_t cell = x + (size_t)cx * (y + (size_t)cy * z);
+        if (L->solid && !L->solid[cell]) continue;
+        uint32_t corners[8];
+        for (uint32_t c = 0; c < 8; ++c) {
+          const uint32_t ax = x + (c & 1u), ay = y + ((c >> 1) & 1u),
+                         az = z + ((c >> 2) & 1u);
+          corners[c] =
+              node_of[ax + (size_t)L->res[0] * (ay + (size_t)L->res[1] * az)];
+        }
+        const uint32_t lane = L->cell_lane ? L->cell_lane[cell] : 0u;
+        const double density =
+            (double)lane_of(materials, lanes, lane, &fb)->density;
+        for (int k = 0; k < 6; ++k) {
+          uint32_t tn[4] = {corners[KUHN[k][0]], corners[KUHN[k][1]],
+                            corners[KUHN[k][2]], corners[KUHN[k][3]]};
+          double shape[9];
+          for (int pass = 0; pass < 2; ++pass) {
+            for (int e = 0; e < 3; ++e)
+              for (int r = 0; r < 3; ++r)
+                shape[r * 3 + e] =
+                    out->rest[3 * tn[e + 1] + r] - out->rest[3 * tn[0] + r];
+            if (pass == 0 && orc_det3(shape) < 0.0) {
+              const uint32_t t = tn[1];
+              tn[1] = tn[2];
+              tn[2] = t;
+              continue;
+            }
+            break;
+          }
+          const double vol = orc_det3(shape) / 6.0;
+          if (!(vol > 0.0)) continue;
+          memcpy(&out->tet[4 * (size_t)tets], tn, sizeof tn);
+          inv3(shape, &out->dminv[9 * (size_t)tets]);
+          out->volume[tets] = vol;
+          out->lane[tets] = lane;
+          for (int c = 0; c < 4; ++c) out->mass[tn[c]] += density * vol * 0.25;
+          ++tets;
+        }
+      }
+  out->tets = tets;
+  free(restf);
+  free(node_of);
+  return tets ? 0 : -1;
+}
+
+void orc_mesh_free(orc_mesh *m) {
+  free(m->rest);
+  free(m->mass);
+  free(m->anchored);
+  free(m->tet);
+  free(m->dminv);
+  free(m->volume);
+  free(m->lane);
+  memset(m, 0, sizeof *m);
+}
+
+/* ------------------------------------------------------------------ */
+/* Rotations                                                            */
+/* ------------------------------------------------------------------ */
+
+/* polarRotation (material.cc:168-197): R <- (R + R^-T)/2 from R = F, at most
+ * 24 passes, stop when the largest entry change is below 1e-6; identity when
+ * |det F| < 1e-12. */
+void orc_polar(const double *F, double *R) {
+  if (fabs(orc_det3(F)) < 1.0e-12) {
+    ident3(R);
+    return;
+  }
+  double r[9];
+  memcpy(r, F, sizeof r);
+  for (int pass = 0; pass < 24; ++pass) {
+    double c[9];
+    cof3(r, c);
+    const double d = r[0] * c[0] + r[1] * c[1] + r[2] * c[2];
+    double largest = 0.0;
+    for (int i = 0; i < 9; ++i) {
+      const double next = 0.5 * (r[i] + c[i] / d); /* R^-T = cof(R)/det */
+      const double change = fabs(next - r[i]);
+      if (change > largest) largest = change;
+      r[i] = next;
+    }
+    if (largest < 1.0e-6) break;
+  }
+  memcpy(R, r, sizeof r);
+}
+
+/* elementRotation, Invertible mode (material.cc:24-64): polar when det F > 0,
+ * else the rotation of a sign-fixed SVD, U diag(1,1,det(UV^T)) V^T with the
+ * smallest singular value carrying the reflection.  Computed from the
+ * eigenvectors of F^T F; unique whenever sigma_2 > 0. */
+void orc_element_rotation(const double *F, double *R) {
+  if (orc_det3(F) > 0.0) {
+    orc_polar(F, R);
+    return;
+  }
+  double C[9], w[3], V[9];
+  mul3_tn(F, F, C);
+  orc_eigen_sym3(C, w, V); /* ascending: column 2 = largest */
+  /* v1 = largest, v2 = middle, v3 = smallest; make (v1,v2,v3) proper */
+  double v[3][3];
+  for (int r = 0; r < 3; ++r) {
+    v[0][r] = V[r * 3 + 2];
+    v[1][r] = V[r * 3 + 1];
+    v[2][r] = V[r * 3 + 0];
+  }
+  double cr[3] = {v[0][1] * v[1][2] - v[0][2] * v[1][1],
+                  v[0][2] * v[1][0] - v[0][0] * v[1][2],
+                  v[0][0] * v[1][1] - v[0][1] * v[1][0]};
+  for (int r = 0; r < 3; ++r) v[2][r] = cr[r];
+  double u[3][3];
+  for (int i = 0; i < 2; ++i) mv3(F, v[i], u[i]);
+  double n0 = sqrt(u[0][0] * u[0][0] + u[0][1] * u[0][1] + u[0][2] * u[0][2]);
+  if (!(n0 > 0.0)) {
+    ident3(R);
+    return;
+  }
+  for (int r = 0; r < 3; ++r) u[0][r] /= n0;
+  const double p = u[1][0] * u[0][0] + u[1][1] * u[0][1] + u[1][2] * u[0][2];
+  for (int r = 0; r < 3; ++r) u[1][r] -= p * u[0][r];
+  double n1 = sqrt(u[1][0] * u[1][0] + u[1][1] * u[1][1] + u[1][2] * u[1][2]);
+  if (!(n1 > 0.0)) {
+    ident3(R);
+    return;
+  }
+  for (int r = 0; r < 3; ++r) u[1][r] /= n1;
+  u[2][0] = u[0][1] * u[1][2] - u[0][2] * u[1][1];
+  u[2][1] = u[0][2] * u[1][0] - u[0][0] * u[1][2];
+  u[2][2] = u[0][0] * u[1][1] - u[0][1] * u[1][0];
+  for (int r = 0; r < 3; ++r)
+    for (int c = 0; c < 3; ++c)
+      R[r * 3 + c] = u[0][r] * v[0][c] + u[1][r] * v[1][c] + u[2][r] * v[2][c];
+}
+
+/* ------------------------------------------------------------------ */
+/* Stable neo-Hookean, one point per tet, displacement (H) form         */
+/* ------------------------------------------------------------------ */
+
+/* SNH energy density and first Piola stress from H = F - I
+ * (neohookean.cc:41-47, 125-147; material.cc:918-946):
+ *   J-1 = tr H + (tr^2 H - H:H^T)/2 + det H
+ *   psi = mu/2 |H|^2 - mu (J-1 - tr H) + (lambda+mu)/2 (J-1)^2
+ *   P   = mu (H + H^T - tr H I - cof H) + (lambda+mu)(J-1) cof F        */
+static double snh_stress(const double *H, double lambda, double mu,
+                         double *P) {
+  const double shifted = lambda + mu;
+  const double tr = trace3(H);
+  double Ht_contract = 0.0;
+  for (int r = 0; r < 3; ++r)
+    for (int c = 0; c < 3; ++c) Ht_contract += H[r * 3 + c] * H[c * 3 + r];
+  const double change = tr + 0.5 * (tr * tr - Ht_contract) + orc_det3(H);
+  double cofH[9], F[9], cofF[9];
+  cof3(H, cofH);
+  for (int i = 0; i < 9; ++i) F[i] = H[i];
+  F[0] += 1.0;
+  F[4] += 1.0;
+  F[8] += 1.0;
+  cof3(F, cofF);
+  for (int r = 0; r < 3; ++r)
+    for (int c = 0; c < 3; ++c)
+      P[r * 3 + c] = mu * (H[r * 3 + c] + H[c * 3 + r] - (r == c ? tr : 0.0) -
+                           cofH[r * 3 + c]) +
+                     shifted * change * cofF[r * 3 + c];
+  return 0.5 * mu * contract3(H, H) - mu * (change - tr) +
+         0.5 * shifted * change * change;
+}
+
+/* Forces f_i = -V P g_i, or with a plastic offset Ep = Fp^-1 - I the elastic
+ * gradient F_e = F (I + Ep) and f_i = -V P(F_e) (I+Ep)^T g_i. */
+void orc_snh_tet(const double *u, const double *dminv, double volume,
+                 double lambda, double mu, const double *ep, double *force,
+                 double *energy) {
+  double H[9], g[4][3], P[9];
+  displacement_gradient(u, dminv, H);
+  shape_gradients(dminv, g);
+  double He[9];
+  if (ep) {
+    double HE[9];
+    mul3(H, ep, HE);
+    for (int i = 0; i < 9; ++i) He[i] = H[i] + ep[i] + HE[i];
+  } else {
+    memcpy(He, H, sizeof He);
+  }
+  const double psi = snh_stress(He, lambda, mu, P);
+  double PF[9];
+  if (ep) {
+    double Fp[9]; /* (I + Ep) */
+    memcpy(Fp, ep, sizeof Fp);
+    Fp[0] += 1.0;
+    Fp[4] += 1.0;
+    Fp[8] += 1.0;
+    mul3_nt(P, Fp, PF); /* P (I+Ep)^T */
+  } else {
+    memcpy(PF, P, sizeof PF);
+  }
+  for (int i = 0; i < 4; ++i) {
+    double t[3];
+    mv3(PF, g[i], t);
+    for (int r = 0; r < 3; ++r) force[3 * i + r] = -volume * t[r];
+  }
+  if (energy) *energy = volume * psi;
+}
+
+/* ------------------------------------------------------------------ */
+/* Corotated tet with optional J2 return map                            */
+/* ------------------------------------------------------------------ */
+
+/* returnMap (material.cc:594-630) on the small strain in the corotated frame,
+ * plastic = {eps_p (9, row-major), alpha}.  Returns the stress. */
+static int j2_return(const double *strain, const double *plastic,
+                     double lambda, double mu, double yield, double hardening,
+                     double *stress, double *next) {
+  const double bulk = lambda + 2.0 * mu / 3.0;
+  double el[9];
+  for (int i = 0; i < 9; ++i) el[i] = strain[i] - (plastic ? plastic[i] : 0.0);
+  const double tr = trace3(el);
+  const double pressure = bulk * tr;
+  double trial[9];
+  for (int i = 0; i < 9; ++i) trial[i] = 2.0 * mu * el[i];
+  for (int d = 0; d < 3; ++d) trial[d * 4] -= 2.0 * mu * tr / 3.0;
+  const double norm = sqrt(contract3(trial, trial));
+  const double equivalent = sqrt(1.5) * norm;
+  const double alpha = plastic ? plastic[9] : 0.0;
+  const double radius = yield + hardening * alpha;
+  if (next && plastic) memcpy(next, plastic, 10 * sizeof(double));
+  if (next && !plastic) memset(next, 0, 10 * sizeof(double));
+  if (!(equivalent > radius) || !(norm > 0.0)) {
+    for (int i = 0; i < 9; ++i) stress[i] = trial[i];
+    for (int d = 0; d < 3; ++d) stress[d * 4] += pressure;
+    return 0;
+  }
+  const double gamma = (equivalent - radius) / (3.0 * mu + hardening);
+  const double theta = 1.0 - 3.0 * mu * gamma / equivalent;
+  for (int i = 0; i < 9; ++i) stress[i] = theta * trial[i];
+  for (int d = 0; d < 3; ++d) stress[d * 4] += pressure;
+  if (next) {
+    for (int i = 0; i < 9; ++i) next[i] += sqrt(1.5) * gamma * trial[i] / norm;
+    next[9] += gamma;
+  }
+  return 1;
+}
+
+/* tetStrain + trial / tetPointTrial (material.cc:136-164, 236-265,
+ * 978-1031): R = elementRotation(F); centred offsets
+ * o_i = R^T (x_i - xbar) - (X_i - Xbar); eps = sym(sum o_i g_i^T);
+ * sigma from the return map (elastic when the lane does not yield);
+ * f_i = -R V sigma g_i; energy V/2 sigma:(eps - eps_p_next). */
+void orc_corotated_tet(const double *x, const double *X, const double *dminv,
+                       double volume, double lambda, double mu,
+                       const double *plastic, double yield, double hardening,
+                       double *R, double *force, double *energy,
+                       double *plastic_next, uint32_t *yielding) {
+  double g[4][3], F[9], u[12];
+  shape_gradients(dminv, g);
+  for (int i = 0; i < 12; ++i) u[i] = x[i] - X[i];
+  displacement_gradient(u, dminv, F);
+  F[0] += 1.0;
+  F[4] += 1.0;
+  F[8] += 1.0;
+  orc_element_rotation(F, R);
+  double xc[3] = {0, 0, 0}, Xc[3] = {0, 0, 0};
+  for (int i = 0; i < 4; ++i)
+    for (int r = 0; r < 3; ++r) {
+      xc[r] += 0.25 * x[3 * i + r];
+      Xc[r] += 0.25 * X[3 * i + r];
+    }
+  double D[9] = {0};
+  for (int i = 0; i < 4; ++i) {
+    double arm[3], o[3];
+    for (int r = 0; r < 3; ++r) arm[r] = x[3 * i + r] - xc[r];
+    mtv3(R, arm, o);
+    for (int r = 0; r < 3; ++r) o[r] -= X[3 * i + r] - Xc[r];
+    for (int r = 0; r < 3; ++r)
+      for (int c = 0; c < 3; ++c) D[r * 3 + c] += o[r] * g[i][c];
+  }
+  double eps[9];
+  for (int r = 0; r < 3; ++r)
+    for (int c = 0; c < 3; ++c) eps[r * 3 + c] = 0.5 * (D[r * 3 + c] + D[c * 3 + r]);
+  double sigma[9], next[10];
+  const int plastic_now =
+      isfinite(yield) && yield > 0.0
+          ? j2_return(eps, plastic, lambda, mu, yield, hardening, sigma, next)
+          : j2_return(eps, NULL, lambda, mu, INFINITY, 0.0, sigma, next);
+  if (!(isfinite(yield) && yield > 0.0) && plastic)
+    memcpy(next, plastic, 10 * sizeof(double));
+  for (int i = 0; i < 4; ++i) {
+    double t[3], f[3];
+    mv3(sigma, g[i], t);
+    mv3(R, t, f);
+    for (int r = 0; r < 3; ++r) force[3 * i + r] = -volume * f[r];
+  }
+  double el[9];
+  for (int i = 0; i < 9; ++i) el[i] = eps[i] - next[i];
+  if (energy) *energy = volume * 0.5 * contract3(sigma, el);
+  if (plastic_next) memcpy(plastic_next, next, 10 * sizeof(double));
+  if (yielding) *yielding = (uint32_t)plastic_now;
+}
+
+/* ------------------------------------------------------------------ */
+/* Hencky return and plastic flow for SNH tets (plasticflow.cc:96-260)  */
+/* ------------------------------------------------------------------ */
+
+/* returnMap (plasticflow.cc:110-150): log principal strains of F_e F_e^T,
+ * radial return of the deviator to the yield surface.  Returns 1 and the
+ * returned F_e when the trial stress exceeds the yield stress. */
+static int hencky_return(const double *elastic, double mu, double yield,
+                         double *returned, double *trial_stress) {
+  double b[9], w[3], V[9];
+  mul3_nt(elastic, elastic, b);
+  orc_eigen_sym3(b, w, V);
+  if (!(w[0] > 0.0)) return 0;
+  double strain[3], dev[3];
+  for (int i = 0; i < 3; ++i) strain[i] = 0.5 * log(w[i]);
+  const double mean = (strain[0] + strain[1] + strain[2]) / 3.0;
+  for (int i = 0; i < 3; ++i) dev[i] = strain[i] - mean;
+  const double size = sqrt(dev[0] * dev[0] + dev[1] * dev[1] + dev[2] * dev[2]);
+  const double stress = sqrt(1.5) * 2.0 * mu * size;
+  if (trial_stress) *trial_stress = stress;
+  if (!(stress > yield) || size <= 1.0e-15) return 0;
+  double scale[3];
+  for (int i = 0; i < 3; ++i) scale[i] = exp(dev[i] * (yield / stress - 1.0));
+  double corr[9];
+  for (int r = 0; r < 3; ++r)
+    for (int c = 0; c < 3; ++c) {
+      double s = 0.0;
+      for (int i = 0; i < 3; ++i) s += V[r * 3 + i] * scale[i] * V[c * 3 + i];
+      corr[r * 3 + c] = s;
+    }
+  mul3(corr, elastic, returned);
+  return 1;
+}
+
+/* flowLattice body for one point (plasticflow.cc:170-255), applied to the
+ * single point of a tet.  plastic_offset = Fp^-1 - I (updated in place),
+ * alpha = accumulated hardening strain.  Returns 1 on a flow (yield) event.
+ * Crush compaction (crushStress) is not restated. */
+int orc_hencky_flow(const double *F, double mu, double yield, double hardening,
+                    double plastic_max, double *ep, double *alpha) {
+  if (!(orc_det3(F) > 0.0)) return 0;
+  double plastic[9], elastic[9];
+  memcpy(plastic, ep, sizeof plastic);
+  plastic[0] += 1.0;
+  plastic[4] += 1.0;
+  plastic[8] += 1.0;
+  mul3(F, plastic, elastic);
+  const double start = yield + hardening * (*alpha);
+  const double ratio = hardening / (3.0 * mu);
+  double after[9], sigma_trial = 0.0, returned[9];
+  const int trial = hencky_return(elastic, mu, start, returned, &sigma_trial);
+  if (!trial) return 0;
+  const double yield_now = (start + ratio * sigma_trial) / (1.0 + ratio);
+  double s2 = 0.0;
+  if (!hencky_return(elastic, mu, yield_now, after, &s2)) return 0;
+  double Finv[9], next[9];
+  if (!inv3(F, Finv)) return 0;
+  mul3(Finv, after, next);
+  const double kept = orc_det3(next);
+  if (isfinite(plastic_max) && kept > 0.0) {
+    double ninv[9];
+    inv3(next, ninv);
+    ninv[0] -= 1.0;
+    ninv[4] -= 1.0;
+    ninv[8] -= 1.0;
+    const double size = sqrt(contract3(ninv, ninv));
+    if (size > plastic_max) {
+      double capped[9], cinv[9];
+      for (int i = 0; i < 9; ++i) capped[i] = (plastic_max / size) * ninv[i];
+      capped[0] += 1.0;
+      capped[4] += 1.0;
+      capped[8] += 1.0;
+      const double cv = orc_det3(capped);
+      if (!(cv > 1.0e-6)) return 0;
+      inv3(capped, cinv);
+      const double s = cbrt(kept * cv);
+      for (int i = 0; i < 9; ++i) next[i] = s * cinv[i];
+    }
+  }
+  double change = 0.0;
+  for (int i = 0; i < 9; ++i) change += fabs(next[i] - plastic[i]);
+  if (!(change > 1.0e-12)) return 0;
+  for (int i = 0; i < 9; ++i) ep[i] = next[i] - (i % 4 == 0 ? 1.0 : 0.0);
+  *alpha += (sigma_trial - yield_now) / (3.0 * mu);
+  return 1;
+}
+
+/* stretchOf (tearing.cc:248-290) for a tet: the edge frame at every corner
+ * of a tet gives the same F, so s = sqrt(lambda_max(F^T F)); direction is the
+ * top eigenvector (material frame). */
+double orc_stretch(const double *F, double *direction) {
+  double C[9], w[3], V[9];
+  mul3_tn(F, F, C);
+  orc_eigen_sym3(C, w, V);
+  if (direction) {
+    double n = 0.0;
+    for (int r = 0; r < 3; ++r) n += V[r * 3 + 2] * V[r * 3 + 2];
+    n = sqrt(n);
+    for (int r = 0; r < 3; ++r) direction[r] = V[r * 3 + 2] / n;
+  }
+  return sqrt(w[2] > 0.0 ? w[2] : 0.0);
+}
+
+/* ------------------------------------------------------------------ */
+/* Threads: contiguous chunks; every output has one writer              */
+/* ------------------------------------------------------------------ */
+
+typedef void (*range_fn)(void *ctx, size_t begin, size_t end);
+typedef struct {
+  range_fn fn;
+  void *ctx;
+  size_t begin, end;
+} range_job;
+
+static void *range_main(void *p) {
+  range_job *j = (range_job *)p;
+  j->fn(j->ctx, j->begin, j->end);
+  return NULL;
+}
+
+static void parallel_for(size_t n, uint32_t threads, range_fn fn, void *ctx) {
+  if (threads <= 1 || n < 4096) {
+    fn(ctx, 0, n);
+    return;
+  }
+  if (threads > 256) threads = 256;
+  pthread_t tid[256];
+  range_job job[256];
+  const size_t chunk = (n + threads - 1) / threads;
+  uint32_t started = 0;
+  for (uint32_t t = 0; t < threads; ++t) {
+    const size_t b = (size_t)t * chunk;
+    if (b >= n) break;
+    job[t].fn = fn;
+    job[t].ctx = ctx;
+    job[t].begin = b;
+    job[t].end = b + chunk < n ? b + chunk : n;
+    if (t == 0) continue;
+    pthread_create(&tid[t], NULL, range_main, &job[t]);
+    ++started;
+  }
+  fn(ctx, job[0].begin, job[0].end);
+  for (uint32_t t = 1; t <= started; ++t) pthread_join(tid[t], NULL);
+}
+
+/* deterministic dot over n vec3: fixed 1024-node blocks summed in order */
+#define DOT_BLOCK 1024u
+typedef struct {
+  const double *a, *b;
+  const uint8_t *mask;
+  size_t n;
+  double *partial;
+} dot_ctx;
+
+static void dot_range(void *p, size_t begin, size_t end) {
+  dot_ctx *c = (dot_ctx *)p;
+  for (size_t blk = begin; blk < end; ++blk) {
+    double s = 0.0;
+    const size_t lo = blk * DOT_BLOCK;
+    const size_t hi = lo + DOT_BLOCK < c->n ? lo + DOT_BLOCK : c->n;
+    for (size_t i = lo; i < hi; ++i)
+      s += c->a[3 * i] * c->b[3 * i] + c->a[3 * i + 1] * c->b[3 * i + 1] +
+           c->a[3 * i + 2] * c->b[3 * i + 2];
+    c->partial[blk] = s;
+  }
+}
+
+/* ------------------------------------------------------------------ */
+/* Bodies                                                               */
+/* ------------------------------------------------------------------ */
+
+struct orc_body {
+  orc_mesh m;
+  double *lambda, *mu, *yield, *hard, *pmax, *tear;
+  uint8_t *snh, *plastic_lane;
+  double *x, *v, *a;
+  double *R;       /* 9 per tet, committed rotation */
+  double *plastic; /* 10 per tet */
+  uint8_t *yflag;
+  uint32_t *inc_start, *inc_slot;
+  /* scratch */
+  double *slot_f, *slot_d, *tet_energy;
+  double *force, *diag, *precond, *right, *weighted, *shifted, *carried,
+      *drive, *evaluated, *residual, *direction, *applied, *conditioned,
+      *increment, *unconstrained;
+  double *partial;
+  size_t nblocks;
+};
+
+orc_body *orc_body_create(const orc_mesh *mesh, const orc_material *materials,
+                          uint32_t lanes) {
+  orc_body *b = (orc_body *)calloc(1, sizeof *b);
+  const size_t N = mesh->nodes, T = mesh->tets;
+  b->m.nodes = mesh->nodes;
+  b->m.tets = mesh->tets;
+#define DUP(dst, src, count, type)                        \
+  do {                                                    \
+    dst = (type *)malloc((count) * sizeof(type) + 1);     \
+    memcpy(dst, src, (count) * sizeof(type));             \
+  } while (0)
+  DUP(b->m.rest, mesh->rest, 3 * N, double);
+  DUP(b->m.mass, mesh->mass, N, double);
+  DUP(b->m.anchored, mesh->anchored, N, uint8_t);
+  DUP(b->m.tet, mesh->tet, 4 * T, uint32_t);
+  DUP(b->m.dminv, mesh->dminv, 9 * T, double);
+  DUP(b->m.volume, mesh->volume, T, double);
+  DUP(b->m.lane, mesh->lane, T, uint32_t);
+#undef DUP
+  b->lambda = (double *)malloc(T * sizeof(double));
+  b->mu = (double *)malloc(T * sizeof(double));
+  b->yield = (double *)malloc(T * sizeof(double));
+  b->hard = (double *)malloc(T * sizeof(double));
+  b->pmax = (double *)malloc(T * sizeof(double));
+  b->tear = (double *)malloc(T * sizeof(double));
+  b->snh = (uint8_t *)malloc(T + 1);
+  b->plastic_lane = (uint8_t *)malloc(T + 1);
+  orc_material fb;
+  for (size_t e = 0; e < T; ++e) {
+    const orc_material *mt = lane_of(materials, lanes, mesh->lane[e], &fb);
+    orc_lame(mt->youngs_modulus, mt->poissons_ratio, &b->lambda[e], &b->mu[e]);
+    b->plastic_lane[e] = (uint8_t)yields(mt->yield_stress);
+    b->yield[e] = b->plastic_lane[e] ? (double)mt->yield_stress : INFINITY;
+    b->hard[e] = mt->hardening_modulus > 0.0f ? (double)mt->hardening_modulus : 0.0;
+    b->pmax[e] = (double)mt->plastic_max;
+    b->tear[e] = 1.0 + (double)mt->tear_strain;
+    b->snh[e] = mt->elastic_model == 1u ? 1u : 0u;
+  }
+  b->x = (double *)malloc(3 * N * sizeof(double));
+  memcpy(b->x, mesh->rest, 3 * N * sizeof(double));
+  b->v = (double *)calloc(3 * N, sizeof(double));
+  b->a = (double *)calloc(3 * N, sizeof(double));
+  b->R = (double *)malloc(9 * T * sizeof(double));
+  for (size_t e = 0; e < T; ++e) ident3(&b->R[9 * e]);
+  b->plastic = (double *)calloc(10 * T, sizeof(double));
+  b->yflag = (uint8_t *)calloc(T + 1, 1);
+  /* incidence: node -> slots (tet*4+corner) in ascending slot order
+   * (buildIncidence, solver.cc:1065-1101) */
+  b->inc_start = (uint32_t *)calloc(N + 1, sizeof(uint32_t));
+  for (size_t s = 0; s < 4 * T; ++s) ++b->inc_start[mesh->tet[s] + 1];
+  for (size_t n = 0; n < N; ++n) b->inc_start[n + 1] += b->inc_start[n];
+  b->inc_slot = (uint32_t *)malloc(4 * T * sizeof(uint32_t) + 4);
+  uint32_t *cursor = (uint32_t *)malloc(N * sizeof(uint32_t) + 4);
+  memcpy(cursor, b->inc_start, N * sizeof(uint32_t));
+  for (size_t s = 0; s < 4 * T; ++s) b->inc_slot[cursor[mesh->tet[s]]++] = (uint32_t)s;
+  free(cursor);
+  b->slot_f = (double *)malloc(12 * T * sizeof(double));
+  b->slot_d = (double *)malloc(36 * T * sizeof(double));
+  b->tet_energy = (double *)malloc(T * sizeof(double) + 8);
+#define VEC(p) p = (double *)calloc(3 * N, sizeof(double))
+  VEC(b->force);
+  VEC(b->right);
+  VEC(b->weighted);
+  VEC(b->shifted);
+  VEC(b->carried);
+  VEC(b->drive);
+  VEC(b->evaluated);
+  VEC(b->residual);
+  VEC(b->direction);
+  VEC(b->applied);
+  VEC(b->conditioned);
+  VEC(b->increment);
+  VEC(b->unconstrained);
+#undef VEC
+  b->diag = (double *)calloc(9 * N, sizeof(double));
+  b->precond = (double *)calloc(9 * N, sizeof(double));
+  b->nblocks = (N + DOT_BLOCK - 1) / DOT_BLOCK;
+  b->partial = (double *)calloc(b->nblocks + 1, sizeof(double));
+  return b;
+}
+
+void orc_body_free(orc_body *b) {
+  if (!b) return;
+  orc_mesh_free(&b->m);
+  void *ptrs[] = {b->lambda, b->mu, b->yield, b->hard, b->pmax, b->tear,
+                  b->snh, b->plastic_lane, b->x, b->v, b->a, b->R, b->plastic,
+                  b->yflag, b->inc_start, b->inc_slot, b->slot_f, b->slot_d,
+                  b->tet_energy, b->force, b->diag, b->precond, b->right,
+                  b->weighted, b->shifted, b->carried, b->drive, b->evaluated,
+                  b->residual, b->direction, b->applied, b->conditioned,
+                  b->increment, b->unconstrained, b->partial};
+  for (size_t i = 0; i < sizeof ptrs / sizeof ptrs[0]; ++i) free(ptrs[i]);
+  free(b);
+}
+
+void orc_body_set_state(orc_body *b, const double *x, const double *v,
+                        const double *a) {
+  const size_t n = 3 * (size_t)b->m.nodes * sizeof(double);
+  if (x) memcpy(b->x, x, n);
+  if (v) memcpy(b->v, v, n);
+  if (a) memcpy(b->a, a, n);
+}
+
+void orc_body_get_state(const orc_body *b, double *x, double *v, double *a) {
+  const size_t n = 3 * (size_t)b->m.nodes * sizeof(double);
+  if (x) memcpy(x, b->x, n);
+  if (v) memcpy(v, b->v, n);
+  if (a) memcpy(a, b->a, n);
+}
+
+void orc_body_get_rotations(const orc_body *b, double *R) {
+  memcpy(R, b->R, 9 * (size_t)b->m.tets * sizeof(double));
+}
+void orc_body_set_rotations(orc_body *b, const double *R) {
+  memcpy(b->R, R, 9 * (size_t)b->m.tets * sizeof(double));
+}
+void orc_body_get_plastic(const orc_body *b, double *p) {
+  memcpy(p, b->plastic, 10 * (size_t)b->m.tets * sizeof(double));
+}
+void orc_body_set_plastic(orc_body *b, const double *p) {
+  memcpy(b->plastic, p, 10 * (size_t)b->m.tets * sizeof(double));
+}
+void orc_yield_flags(const orc_body *b, uint8_t *flags) {
+  memcpy(flags, b->yflag, b->m.tets);
+}
+
+/* ---- element pass ---- */
+
+typedef struct {
+  orc_body *b;
+  const double *at;  /* positions */
+  int diag;          /* write block-Jacobi diagonal slots */
+} elem_ctx;
+
+static void gather4(const orc_body *b, const double *src, size_t e,
+                    double *out) {
+  for (int i = 0; i < 4; ++i)
+    for (int r = 0; r < 3; ++r)
+      out[3 * i + r] = src[3 * (size_t)b->m.tet[4 * e + i] + r];
+}
+
+/* One tet of assembleElastic (solver.cc:1382-1470): committed rotation,
+ * forces to slots, diagonal stiffness blocks R K_ii R^T to slots
+ * (stiffnessDiagonal, material.cc:290-304; closed form
+ * V((lambda+mu)(Rg)(Rg)^T + mu|g|^2 I)). */
+static void element_one(orc_body *b, const double *at, size_t e, int diag) {
+  double x[12], X[12], u[12], F[9], R[9];
+  gather4(b, at, e, x);
+  gather4(b, b->m.rest, e, X);
+  for (int i = 0; i < 12; ++i) u[i] = x[i] - X[i];
+  const double *dminv = &b->m.dminv[9 * e];
+  const double V = b->m.volume[e];
+  double energy = 0.0;
+  if (b->snh[e]) {
+    displacement_gradient(u, dminv, F);
+    F[0] += 1.0;
+    F[4] += 1.0;
+    F[8] += 1.0;
+    orc_element_rotation(F, R);
+    orc_snh_tet(u, dminv, V, b->lambda[e], b->mu[e],
+                b->plastic_lane[e] ? &b->plastic[10 * e] : NULL,
+                &b->slot_f[12 * e], &energy);
+  } else {
+    orc_corotated_tet(x, X, dminv, V, b->lambda[e], b->mu[e],
+                      &b->plastic[10 * e], b->yield[e], b->hard[e], R,
+                      &b->slot_f[12 * e], &energy, NULL, NULL);
+  }
+  memcpy(&b->R[9 * e], R, sizeof R);
+  b->tet_energy[e] = energy;
+  if (diag) {
+    double g[4][3];
+    shape_gradients(dminv, g);
+    const double lm = b->lambda[e] + b->mu[e];
+    for (int i = 0; i < 4; ++i) {
+      double rg[3];
+      mv3(R, g[i], rg);
+      const double gg = g[i][0] * g[i][0] + g[i][1] * g[i][1] + g[i][2] * g[i][2];
+      double *blk = &b->slot_d[36 * e + 9 * i];
+      for (int r = 0; r < 3; ++r)
+        for (int c = 0; c < 3; ++c)
+          blk[r * 3 + c] = V * (lm * rg[r] * rg[c] + (r == c ? b->mu[e] * gg : 0.0));
+    }
+  }
+}
+
+static void element_range(void *p, size_t begin, size_t end) {
+  elem_ctx *c = (elem_ctx *)p;
+  for (size_t e = begin; e < end; ++e) element_one(c->b, c->at, e, c->diag);
+}
+
+typedef struct {
+  orc_body *b;
+  double scale;
+  int diag;
+} gather_ctx;
+
+static void gather_range(void *p, size_t begin, size_t end) {
+  gather_ctx *c = (gather_ctx *)p;
+  orc_body *b = c->b;
+  for (size_t n = begin; n < end; ++n) {
+    double f[3] = {0, 0, 0}, d[9] = {0};
+    for (uint32_t k = b->inc_start[n]; k < b->inc_start[n + 1]; ++k) {
+      const uint32_t s = b->inc_slot[k];
+      for (int r = 0; r < 3; ++r) f[r] += b->slot_f[3 * (size_t)s + r];
+      if (c->diag)
+        for (int i = 0; i < 9; ++i) d[i] += c->scale * b->slot_d[9 * (size_t)s + i];
+    }
+    memcpy(&b->force[3 * n], f, sizeof f);
+    if (c->diag) memcpy(&b->diag[9 * n], d, sizeof d);
+  }
+}
+
+/* assembleElastic: returns the elastic energy */
+static double assemble(orc_body *b, const double *at, double step,
+                       uint32_t threads) {
+  elem_ctx ec = {b, at, 1};
+  parallel_for(b->m.tets, threads, element_range, &ec);
+  gather_ctx gc = {b, step * step, 1};
+  parallel_for(b->m.nodes, threads, gather_range, &gc);
+  double energy = 0.0;
+  for (size_t e = 0; e < b->m.tets; ++e) energy += b->tet_energy[e];
+  return energy;
+}
+
+void orc_eval_forces(orc_body *b, const double *x, uint32_t threads,
+                     double *tet_force, double *tet_energy, double *node_force,
+                     double *rotations) {
+  elem_ctx ec = {b, x, 0};
+  parallel_for(b->m.tets, threads, element_range, &ec);
+  gather_ctx gc = {b, 0.0, 0};
+  parallel_for(b->m.nodes, threads, gather_range, &gc);
+  if (tet_force) memcpy(tet_force, b->slot_f, 12 * (size_t)b->m.tets * sizeof(double));
+  if (tet_energy) memcpy(tet_energy, b->tet_energy, (size_t)b->m.tets * sizeof(double));
+  if (node_force) memcpy(node_force, b->force, 3 * (size_t)b->m.nodes * sizeof(double));
+  if (rotations) memcpy(rotations, b->R, 9 * (size_t)b->m.tets * sizeof(double));
+}
+
+/* ---- operator ---- */
+
+typedef struct {
+  orc_body *b;
+  const double *d;
+  double *y;
+  double inertia, scale;
+} op_ctx;
+
+/* Per tet: d'_j = R^T d_j, D = sum d'_j g_j^T, sigma = lambda tr(D) I +
+ * mu (D + D^T), y_i = R V sigma g_i -- equal to sum_j R K_ij R^T d_j with
+ * K_ij = V(lambda g_i g_j^T + mu g_j g_i^T + mu (g_i.g_j) I)
+ * (tetStiffness/block, material.cc:80-90, 209-234; applyStiffness 267-288). */
+static void op_element_range(void *p, size_t begin, size_t end) {
+  op_ctx *c = (op_ctx *)p;
+  orc_body *b = c->b;
+  for (size_t e = begin; e < end; ++e) {
+    double d[12], g[4][3], D[9] = {0};
+    gather4(b, c->d, e, d);
+    shape_gradients(&b->m.dminv[9 * e], g);
+    const double *R = &b->R[9 * e];
+    for (int j = 0; j < 4; ++j) {
+      double dp[3];
+      mtv3(R, &d[3 * j], dp);
+      for (int r = 0; r < 3; ++r)
+        for (int k = 0; k < 3; ++k) D[r * 3 + k] += dp[r] * g[j][k];
+    }
+    const double tr = trace3(D);
+    double s[9];
+    for (int r = 0; r < 3; ++r)
+      for (int k = 0; k < 3; ++k)
+        s[r * 3 + k] = b->mu[e] * (D[r * 3 + k] + D[k * 3 + r]) +
+                       (r == k ? b->lambda[e] * tr : 0.0);
+    const double V = b->m.volume[e];
+    for (int i = 0; i < 4; ++i) {
+      double t[3], y[3];
+      mv3(s, g[i], t);
+      mv3(R, t, y);
+      for (int r = 0; r < 3; ++r) b->slot_f[12 * e + 3 * i + r] = V * y[r];
+    }
+  }
+}
+
+static void op_gather_range(void *p, size_t begin, size_t end) {
+  op_ctx *c = (op_ctx *)p;
+  orc_body *b = c->b;
+  for (size_t n = begin; n < end; ++n) {
+    if (b->m.anchored[n]) {
+      c->y[3 * n] = c->y[3 * n + 1] = c->y[3 * n + 2] = 0.0;
+      continue;
+    }
+    double f[3] = {0, 0, 0};
+    for (uint32_t k = b->inc_start[n]; k < b->inc_start[n + 1]; ++k) {
+      const uint32_t s = b->inc_slot[k];
+      for (int r = 0; r < 3; ++r) f[r] += b->slot_f[3 * (size_t)s + r];
+    }
+    const double m = c->inertia * b->m.mass[n];
+    for (int r = 0; r < 3; ++r) c->y[3 * n + r] = m * c->d[3 * n + r] + c->scale * f[r];
+  }
+}
+
+void orc_apply_system(orc_body *b, double inertia, double step,
+                      const double *d, double *y, uint32_t threads) {
+  op_ctx c = {b, d, y, inertia, step * step};
+  parallel_for(b->m.tets, threads, op_element_range, &c);
+  parallel_for(b->m.nodes, threads, op_gather_range, &c);
+}
+
+static double dot(orc_body *b, const double *x, const double *y,
+                  uint32_t threads) {
+  dot_ctx c = {x, y, NULL, b->m.nodes, b->partial};
+  parallel_for(b->nblocks, threads > 1 && b->nblocks >= 64 ? threads : 1,
+               dot_range, &c);
+  double s = 0.0;
+  for (size_t i = 0; i < b->nblocks; ++i) s += b->partial[i];
+  return s;
+}
+
+static void precondition(orc_body *b) {
+  for (size_t n = 0; n < b->m.nodes; ++n)
+    mv3(&b->precond[9 * n], &b->residual[3 * n], &b->conditioned[3 * n]);
+}
+
+/* conjugateGradient (solver.cc:1904-2089) with the block-Jacobi
+ * preconditioner; returns iterations, sets capped / relative residual. */
+static uint32_t pcg(orc_body *b, double inertia, double step, uint32_t limit,
+                    double tolerance, double *solution, uint32_t threads,
+                    int *capped, double *relative) {
+  const size_t N = b->m.nodes;
+  const double reference = sqrt(dot(b, b->right, b->right, threads));
+  orc_apply_system(b, inertia, step, solution, b->applied, threads);
+  for (size_t n = 0; n < N; ++n)
+    for (int r = 0; r < 3; ++r)
+      b->residual[3 * n + r] =
+          b->m.anchored[n] ? 0.0 : b->right[3 * n + r] - b->applied[3 * n + r];
+  precondition(b);
+  memcpy(b->direction, b->conditioned, 3 * N * sizeof(double));
+  double squared = dot(b, b->residual, b->conditioned, threads);
+  const double floor_ref = reference > DBL_MIN ? reference : DBL_MIN;
+  const double target = tolerance * floor_ref;
+  uint32_t taken = 0;
+  for (uint32_t it = 0; it < limit; ++it) {
+    const double length = dot(b, b->residual, b->residual, threads);
+    if (sqrt(length) <= target) break;
+    orc_apply_system(b, inertia, step, b->direction, b->applied, threads);
+    const double curvature = dot(b, b->direction, b->applied, threads);
+    if (!(curvature > 0.0) || !(squared > 0.0)) break;
+    const double stride = squared / curvature;
+    for (size_t i = 0; i < 3 * N; ++i) {
+      solution[i] += stride * b->direction[i];
+      b->residual[i] -= stride * b->applied[i];
+    }
+    precondition(b);
+    const double next = dot(b, b->residual, b->conditioned, threads);
+    const double blend = next / squared;
+    squared = next;
+    for (size_t i = 0; i < 3 * N; ++i)
+      b->direction[i] = b->conditioned[i] + blend * b->direction[i];
+    ++taken;
+  }
+  const double residual = sqrt(dot(b, b->residual, b->residual, threads));
+  *capped = taken == limit && residual > target;
+  *relative = residual / floor_ref;
+  return taken;
+}
+
+/* integratorRule (solver.cc:531-545) and effectiveSystem (554-562) */
+typedef struct {
+  double am, af, beta, gamma;
+} rule_t;
+
+static rule_t rule_of(double spectral) {
+  double s = spectral < 0.0 ? 0.0 : (spectral > 1.0 ? 1.0 : spectral);
+  rule_t r;
+  r.am = (2.0 * s - 1.0) / (s + 1.0);
+  r.af = s / (s + 1.0);
+  r.gamma = 0.5 - r.am + r.af;
+  r.beta = 0.25 * (1.0 - r.am + r.af) * (1.0 - r.am + r.af);
+  return r;
+}
+
+/* balance (solver.cc:3746-3806): right-hand side, inertia on the diagonal,
+ * block-Jacobi inverse (guardPreconditioner, solver.cc:1574-1593). */
+static void balance(orc_body *b, const rule_t *rule, double inertia,
+                    const double *taken) {
+  for (size_t n = 0; n < b->m.nodes; ++n) {
+    double *P = &b->precond[9 * n];
+    if (b->m.anchored[n]) {
+      for (int r = 0; r < 3; ++r) {
+        b->v[3 * n + r] = 0.0;
+        b->a[3 * n + r] = 0.0;
+        b->right[3 * n + r] = 0.0;
+      }
+      ident3(P);
+      continue;
+    }
+    const double m = b->m.mass[n];
+    for (int r = 0; r < 3; ++r) {
+      double rhs = b->force[3 * n + r] + b->drive[3 * n + r] -
+                   rule->am * m * b->carried[3 * n + r];
+      if (taken) rhs -= inertia * m * taken[3 * n + r];
+      b->right[3 * n + r] = rhs;
+    }
+    double *D = &b->diag[9 * n];
+    D[0] += inertia * m;
+    D[4] += inertia * m;
+    D[8] += inertia * m;
+    if (!(orc_det3(D) > 0.0) || !inv3(D, P)) ident3(P);
+  }
+}
+
+/* predictorInverts (solver.cc:568-623) with FP64 determinants */
+static int predictor_inverts(orc_body *b, const rule_t *rule, double inner) {
+  const double carry = (0.5 - rule->beta) * inner * inner;
+  for (size_t i = 0; i < 3 * (size_t)b->m.nodes; ++i) {
+    const double shifted = b->x[i] + inner * b->v[i] + carry * b->a[i];
+    b->weighted[i] = (1.0 - rule->af) * shifted + rule->af * b->x[i];
+  }
+  for (size_t e = 0; e < b->m.tets; ++e) {
+    double w[12], X[12], u[12], F[9];
+    gather4(b, b->weighted, e, w);
+    gather4(b, b->m.rest, e, X);
+    for (int i = 0; i < 12; ++i) u[i] = w[i] - X[i];
+    displacement_gradient(u, &b->m.dminv[9 * e], F);
+    F[0] += 1.0;
+    F[4] += 1.0;
+    F[8] += 1.0;
+    if (orc_det3(F) <= 0.0) return 1;
+  }
+  return 0;
+}
+
+/* commitPlastic (solver.cc:1260-1346) for corotated lanes and the Hencky
+ * flow (plasticflow.cc:152-260) for neo-Hookean lanes, at final positions.
+ * Returns the number of tets whose plastic state grew. */
+static uint32_t commit_plastic(orc_body *b) {
+  uint32_t grew = 0;
+  for (size_t e = 0; e < b->m.tets; ++e) {
+    b->yflag[e] = 0;
+    if (!b->plastic_lane[e]) continue;
+    double x[12], X[12], u[12], F[9];
+    gather4(b, b->x, e, x);
+    gather4(b, b->m.rest, e, X);
+    for (int i = 0; i < 12; ++i) u[i] = x[i] - X[i];
+    double *pl = &b->plastic[10 * e];
+    if (b->snh[e]) {
+      displacement_gradient(u, &b->m.dminv[9 * e], F);
+      F[0] += 1.0;
+      F[4] += 1.0;
+      F[8] += 1.0;
+      if (orc_hencky_flow(F, b->mu[e], b->yield[e], b->hard[e], b->pmax[e], pl,
+                          &pl[9])) {
+        b->yflag[e] = 1;
+        ++grew;
+      }
+    } else {
+      double R[9], f[12], next[10];
+      orc_corotated_tet(x, X, &b->m.dminv[9 * e], b->m.volume[e], b->lambda[e],
+                        b->mu[e], pl, b->yield[e], b->hard[e], R, f, NULL, next,
+                        NULL);
+      if (next[9] > pl[9]) {
+        b->yflag[e] = 1;
+        ++grew;
+      }
+      memcpy(pl, next, sizeof next);
+    }
+  }
+  return grew;
+}
+
+/* One step of deformable::step for a non-floating tet body
+ * (solver.cc:3472-5901 without contacts, damping, floating frame; the
+ * elastic path 3895-3950 is used for every lane, see the header note). */
+int orc_step(orc_body *b, const orc_settings *s, orc_report *rep) {
+  memset(rep, 0, sizeof *rep);
+  const size_t N = b->m.nodes;
+  const rule_t rule = rule_of(s->spectral_radius);
+  const uint32_t configured = s->substeps > 1 ? s->substeps : 1;
+  const uint32_t threads = s->threads;
+  uint32_t substeps = configured;
+  if (predictor_inverts(b, &rule, (double)s->dt / configured))
+    substeps = configured > 4 ? configured : 4;
+  const double inner = (double)s->dt / substeps;
+  const double inertia = 1.0 - rule.am;
+  const double step = sqrt((1.0 - rule.af) * rule.beta) * inner;
+  const double carry = (0.5 - rule.beta) * inner * inner;
+  const double reach = step * step;
+  rep->substeps = substeps;
+  double elastic = 0.0;
+  for (uint32_t pass = 0; pass < substeps; ++pass) {
+    for (size_t n = 0; n < N; ++n)
+      for (int r = 0; r < 3; ++r) {
+        const size_t i = 3 * n + r;
+        b->carried[i] = b->a[i];
+        b->shifted[i] = b->x[i] + inner * b->v[i] + carry * b->a[i];
+        b->weighted[i] = (1.0 - rule.af) * b->shifted[i] + rule.af * b->x[i];
+        b->drive[i] = b->m.mass[n] * (double)s->gravity[r];
+      }
+    elastic = assemble(b, b->weighted, step, threads);
+    balance(b, &rule, inertia, NULL);
+    int capped = 0;
+    double rel = 0.0;
+    rep->cg_iterations += pcg(b, inertia, step, s->cg_iterations, s->tolerance,
+                              b->a, threads, &capped, &rel);
+    rep->cg_capped += (uint32_t)capped;
+    if (rel > rep->cg_worst_relative) rep->cg_worst_relative = rel;
+    ++rep->solves;
+    for (uint32_t read = 1; read < 3; ++read) {
+      for (size_t i = 0; i < 3 * N; ++i) b->evaluated[i] = b->weighted[i] + reach * b->a[i];
+      elastic = assemble(b, b->evaluated, step, threads);
+      balance(b, &rule, inertia, b->a);
+      memset(b->increment, 0, 3 * N * sizeof(double));
+      rep->cg_iterations += pcg(b, inertia, step, s->cg_iterations,
+                                s->tolerance, b->increment, threads, &capped,
+                                &rel);
+      rep->cg_capped += (uint32_t)capped;
+      if (rel > rep->cg_worst_relative) rep->cg_worst_relative = rel;
+      ++rep->solves;
+      for (size_t n = 0; n < N; ++n)
+        if (!b->m.anchored[n])
+          for (int r = 0; r < 3; ++r) b->a[3 * n + r] += b->increment[3 * n + r];
+    }
+    /* velocity (4351-4368) and position (5829-5849) */
+    for (size_t n = 0; n < N; ++n) {
+      if (b->m.anchored[n]) {
+        for (int r = 0; r < 3; ++r) b->v[3 * n + r] = b->a[3 * n + r] = 0.0;
+        continue;
+      }
+      for (int r = 0; r < 3; ++r) {
+        const size_t i = 3 * n + r;
+        b->v[i] += inner * ((1.0 - rule.gamma) * b->carried[i] + rule.gamma * b->a[i]);
+        b->x[i] = b->shifted[i] + rule.beta * inner * inner * b->a[i];
+      }
+    }
+    const uint32_t grew = commit_plastic(b);
+    rep->yield_events += grew;
+    rep->plastic_growth |= grew ? 1u : 0u;
+  }
+  double kinetic = 0.0;
+  for (size_t n = 0; n < N; ++n)
+    kinetic += 0.5 * b->m.mass[n] *
+               (b->v[3 * n] * b->v[3 * n] + b->v[3 * n + 1] * b->v[3 * n + 1] +
+                b->v[3 * n + 2] * b->v[3 * n + 2]);
+  rep->kinetic_energy = kinetic;
+  rep->elastic_energy = elastic;
+  for (size_t i = 0; i < 3 * N; ++i)
+    if (!isfinite(b->x[i]) || !isfinite(b->v[i])) {
+      rep->non_finite = 1;
+      break;
+    }
+  return 0;
+}
+
+void orc_tet_stretch(orc_body *b, double *stretch) {
+  for (size_t e = 0; e < b->m.tets; ++e) {
+    double x[12], X[12], u[12], F[9];
+    gather4(b, b->x, e, x);
+    gather4(b, b->m.rest, e, X);
+    for (int i = 0; i < 12; ++i) u[i] = x[i] - X[i];
+    displacement_gradient(u, &b->m.dminv[9 * e], F);
+    F[0] += 1.0;
+    F[4] += 1.0;
+    F[8] += 1.0;
+    stretch[e] = orc_stretch(F, NULL);
+  }
+}

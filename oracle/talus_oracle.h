@@ -1,0 +1,145 @@
+/*
+ * talus_oracle.h -- FP64 CPU restatement of the Talus deformable tet step.
+ *
+ * TEST INFRASTRUCTURE ONLY.  This library is the checker for the MI355X
+ * path: only tests/, __graft_entry__.smoke() and bench.py's cpu_baseline leg
+ * may load it.  The product (libtalus_mi.so) never links or calls it.
+ *
+ * Matrices are 3x3 row-major double[9].  Vectors of nodes are xyz-interleaved
+ * double[3*n].  The rest-shape inverse Dm^-1 is stored row-major, so its rows
+ * are the shape gradients g1, g2, g3 and g0 = -(g1+g2+g3)
+ * (reference: src/talus/deformable/material/material.cc:199-207).
+ */
+#ifndef TALUS_ORACLE_H
+#define TALUS_ORACLE_H
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* Same field order and meaning as talus::PhysicalMaterial
+ * (src/talus/material.h:12-29); elastic_model 0 = Corotated, 1 = NeoHookean. */
+typedef struct orc_material {
+  float friction, restitution, density, strength, youngs_modulus,
+      poissons_ratio, stiffness_damping, yield_stress, hardening_modulus;
+  uint32_t elastic_model;
+  float hysteresis, damping_ratio, tear_strain, plastic_max, crush_stress,
+      crush_limit;
+} orc_material;
+
+/* Tet mesh (PreparedMesh, src/talus/mesh.h:25-44, tets only), SoA. */
+typedef struct orc_mesh {
+  uint32_t nodes, tets;
+  double *rest;       /* 3*nodes, rest positions (from float lattice samples) */
+  double *mass;       /* nodes, lumped rho*V/4 per incident tet */
+  uint8_t *anchored;  /* nodes */
+  uint32_t *tet;      /* 4*tets */
+  double *dminv;      /* 9*tets, FP64 inverse of the rest edge matrix */
+  double *volume;     /* tets */
+  uint32_t *lane;     /* tets, material lane */
+} orc_mesh;
+
+/* Axis-aligned-or-not support volume (solver.h SupportVolume): a node is
+ * anchored when |world_to_support * rest| <= half on every axis. */
+typedef struct orc_support {
+  float world_to_support[12]; /* row-major 3x4 affine */
+  float half[3];
+} orc_support;
+
+/* Structured lattice input for the Kuhn mesh builder (buildMesh,
+ * solver.cc:2185-2418, for fully solid cells). */
+typedef struct orc_lattice {
+  uint32_t res[3];          /* samples per axis (cells + 1) */
+  float voxel[3];
+  float origin[3];
+  const uint8_t *solid;     /* cells, NULL = all solid */
+  const uint32_t *cell_lane;/* cells, NULL = lane 0 */
+  const float *sample_pos;  /* 3*samples or NULL = origin + voxel*index */
+} orc_lattice;
+
+typedef struct orc_settings {
+  float dt;
+  float gravity[3];
+  uint32_t substeps;
+  uint32_t cg_iterations;
+  float tolerance;
+  float spectral_radius;
+  uint32_t threads;         /* 0/1 = serial */
+} orc_settings;
+
+typedef struct orc_report {
+  uint32_t substeps;
+  uint32_t cg_iterations;
+  uint32_t cg_capped;
+  uint32_t solves;
+  double cg_worst_relative;
+  double elastic_energy;
+  double kinetic_energy;
+  uint32_t plastic_growth;  /* any yield this step */
+  uint32_t yield_events;    /* tets whose plastic state changed */
+  uint32_t non_finite;
+} orc_report;
+
+typedef struct orc_body orc_body;
+
+/* ---- mesh ---- */
+int orc_build_lattice(const orc_lattice *lattice, const orc_material *materials,
+                      uint32_t lanes, const orc_support *supports,
+                      uint32_t nsupports, orc_mesh *out);
+void orc_mesh_free(orc_mesh *mesh);
+
+/* ---- materials ---- */
+void orc_lame(float youngs, float poisson, double *lambda, double *mu);
+
+/* ---- element kernels (per tet) ---- */
+double orc_det3(const double *a);
+void orc_polar(const double *F, double *R);
+void orc_element_rotation(const double *F, double *R);
+void orc_snh_tet(const double *u, const double *dminv, double volume,
+                 double lambda, double mu, const double *plastic_offset,
+                 double *force, double *energy);
+void orc_corotated_tet(const double *x, const double *X, const double *dminv,
+                       double volume, double lambda, double mu,
+                       const double *plastic, double yield, double hardening,
+                       double *R, double *force, double *energy,
+                       double *plastic_next, uint32_t *yielding);
+int orc_hencky_flow(const double *F, double mu, double yield, double hardening,
+                    double plastic_max, double *plastic_offset,
+                    double *alpha);
+double orc_stretch(const double *F, double *direction);
+void orc_eigen_sym3(const double *A, double *values, double *vectors);
+
+/* ---- bodies ---- */
+orc_body *orc_body_create(const orc_mesh *mesh, const orc_material *materials,
+                          uint32_t lanes);
+void orc_body_free(orc_body *body);
+void orc_body_set_state(orc_body *body, const double *x, const double *v,
+                        const double *a);
+void orc_body_get_state(const orc_body *body, double *x, double *v, double *a);
+void orc_body_get_rotations(const orc_body *body, double *R);
+void orc_body_get_plastic(const orc_body *body, double *plastic);
+void orc_body_set_plastic(orc_body *body, const double *plastic);
+void orc_body_set_rotations(orc_body *body, const double *R);
+int orc_step(orc_body *body, const orc_settings *settings, orc_report *report);
+
+/* Per-tet forces (12 per tet) and energies at positions x, and the node
+ * force sum, using the current plastic state. */
+void orc_eval_forces(orc_body *body, const double *x, uint32_t threads,
+                     double *tet_force, double *tet_energy, double *node_force,
+                     double *rotations);
+/* y = inertia*M*d + h^2 * sum_e R_e K_e R_e^T d_e, anchored rows zeroed
+ * (applySystem, solver.cc:904-1048), with the body's committed rotations. */
+void orc_apply_system(orc_body *body, double inertia, double step,
+                      const double *d, double *y, uint32_t threads);
+/* Per-tet stretch sqrt(lambda_max(F^T F)) at the body's positions. */
+void orc_tet_stretch(orc_body *body, double *stretch);
+/* Yield flags of the last step (1 per tet). */
+void orc_yield_flags(const orc_body *body, uint8_t *flags);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif

@@ -1,0 +1,395 @@
+"""ctypes binding of libtalus_mi.so (the C ABI declared in include/talus_mi.h).
+
+Mirrors the host surface of the reference's deformable path: a
+``PhysicalMaterial`` record (src/talus/material.h:12-29), a tet
+``PreparedMesh`` built by ``build_lattice`` (deformable::buildMesh,
+src/talus/deformable/solver/solver.cc:2185), bodies with a
+``DeformableState`` (mesh.h:56-62) and ``Context.step`` for
+``deformable::step`` (solver.h:451-462).  Errors raise ``TalusError`` with the
+library's message; nothing here falls back to a CPU path.
+"""
+from __future__ import annotations
+
+import ctypes as C
+import math
+import os
+
+import numpy as np
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(HERE, "libtalus_mi.so")
+
+FP64 = 0
+MIXED = 1
+BLOCK_JACOBI = 0
+FACTOR = 1
+DETECT_TEARS = 1
+RECORD_EVENTS = 2
+EVENT_YIELD = 0
+EVENT_TEAR = 1
+COROTATED = 0
+NEO_HOOKEAN = 1
+
+
+class TalusError(RuntimeError):
+    pass
+
+
+class PhysicalMaterial(C.Structure):
+    """talus::PhysicalMaterial, same field order and defaults."""
+
+    _fields_ = [
+        ("friction", C.c_float), ("restitution", C.c_float),
+        ("density", C.c_float), ("strength", C.c_float),
+        ("youngs_modulus", C.c_float), ("poissons_ratio", C.c_float),
+        ("stiffness_damping", C.c_float), ("yield_stress", C.c_float),
+        ("hardening_modulus", C.c_float), ("elastic_model", C.c_uint32),
+        ("hysteresis", C.c_float), ("damping_ratio", C.c_float),
+        ("tear_strain", C.c_float), ("plastic_max", C.c_float),
+        ("crush_stress", C.c_float), ("crush_limit", C.c_float),
+    ]
+
+    def __init__(self, **kw):
+        super().__init__()
+        defaults = dict(friction=0.5, restitution=0.0, density=1000.0, strength=math.inf,
+                        youngs_modulus=0.0, poissons_ratio=0.3, stiffness_damping=0.0,
+                        yield_stress=math.inf, hardening_modulus=0.0, elastic_model=COROTATED,
+                        hysteresis=0.0, damping_ratio=math.inf, tear_strain=math.inf,
+                        plastic_max=math.inf, crush_stress=math.inf, crush_limit=1.0)
+        defaults.update(kw)
+        for k, v in defaults.items():
+            setattr(self, k, v)
+
+
+class _Mesh(C.Structure):
+    _fields_ = [
+        ("nodes", C.c_uint32), ("tets", C.c_uint32),
+        ("rest_positions", C.POINTER(C.c_float)), ("node_mass", C.POINTER(C.c_float)),
+        ("anchored", C.POINTER(C.c_uint8)), ("tet_nodes", C.POINTER(C.c_uint32)),
+        ("rest_inverse", C.POINTER(C.c_float)), ("volume", C.POINTER(C.c_float)),
+        ("tet_material", C.POINTER(C.c_uint32)),
+    ]
+
+
+class _Lattice(C.Structure):
+    _fields_ = [
+        ("resolution", C.c_uint32 * 3), ("voxel_size", C.c_float * 3),
+        ("origin", C.c_float * 3), ("solid", C.POINTER(C.c_uint8)),
+        ("cell_lane", C.POINTER(C.c_uint32)), ("sample_positions", C.POINTER(C.c_float)),
+    ]
+
+
+class Support(C.Structure):
+    """SupportVolume: anchored iff |world_to_support * p| <= half_extents."""
+
+    _fields_ = [("world_to_support", C.c_float * 12), ("half_extents", C.c_float * 3)]
+
+    @staticmethod
+    def box(center, half):
+        s = Support()
+        s.world_to_support[:] = [1, 0, 0, -center[0], 0, 1, 0, -center[1], 0, 0, 1, -center[2]]
+        s.half_extents[:] = list(half)
+        return s
+
+
+class StepParams(C.Structure):
+    _fields_ = [
+        ("dt", C.c_float), ("gravity", C.c_float * 3), ("substeps", C.c_uint32),
+        ("cg_iterations", C.c_uint32), ("factor_budget", C.c_uint32),
+        ("tolerance", C.c_float), ("spectral_radius", C.c_float),
+        ("preconditioner", C.c_uint32), ("flags", C.c_uint32),
+    ]
+
+
+class Report(C.Structure):
+    _fields_ = [
+        ("bodies", C.c_uint32), ("nodes", C.c_uint32), ("elements", C.c_uint32),
+        ("substeps", C.c_uint32), ("cg_iterations", C.c_uint32), ("cg_capped", C.c_uint32),
+        ("solves", C.c_uint32), ("refactors", C.c_uint32), ("cg_worst_relative", C.c_double),
+        ("elastic_energy", C.c_double), ("kinetic_energy", C.c_double),
+        ("plastic_growth", C.c_uint32), ("yield_events", C.c_uint32),
+        ("tear_events", C.c_uint32), ("split_nodes", C.c_uint32), ("non_finite", C.c_uint32),
+        ("reserved", C.c_uint32), ("device_seconds", C.c_double),
+    ]
+
+    def as_dict(self):
+        return {k: getattr(self, k) for k, _ in self._fields_ if k != "reserved"}
+
+
+class Event(C.Structure):
+    _fields_ = [("kind", C.c_uint32), ("body", C.c_uint32), ("tet", C.c_uint32),
+                ("step", C.c_uint32)]
+
+
+# every entry point of include/talus_mi.h with its ctypes signature
+_P = C.POINTER
+_SIGNATURES = {
+    "talus_mi_create": (C.c_int, [_P(C.c_void_p), C.c_int, C.c_uint32]),
+    "talus_mi_destroy": (C.c_int, [C.c_void_p]),
+    "talus_mi_last_error": (C.c_char_p, []),
+    "talus_mi_default_params": (None, [_P(StepParams)]),
+    "talus_mi_build_lattice": (C.c_int, [_P(_Lattice), _P(PhysicalMaterial), C.c_uint32,
+                                         _P(Support), C.c_uint32, _P(_Mesh)]),
+    "talus_mi_mesh_free": (None, [_P(_Mesh)]),
+    "talus_mi_body_add": (C.c_int, [C.c_void_p, _P(_Mesh), _P(PhysicalMaterial), C.c_uint32,
+                                    _P(C.c_uint32)]),
+    "talus_mi_body_info": (C.c_int, [C.c_void_p, C.c_uint32, _P(C.c_uint32), _P(C.c_uint32)]),
+    "talus_mi_state_set": (C.c_int, [C.c_void_p, C.c_uint32, _P(C.c_double), _P(C.c_double),
+                                     _P(C.c_double)]),
+    "talus_mi_state_get": (C.c_int, [C.c_void_p, C.c_uint32, _P(C.c_double), _P(C.c_double),
+                                     _P(C.c_double)]),
+    "talus_mi_rotations_get": (C.c_int, [C.c_void_p, C.c_uint32, _P(C.c_double)]),
+    "talus_mi_rotations_set": (C.c_int, [C.c_void_p, C.c_uint32, _P(C.c_double)]),
+    "talus_mi_plastic_get": (C.c_int, [C.c_void_p, C.c_uint32, _P(C.c_double)]),
+    "talus_mi_plastic_set": (C.c_int, [C.c_void_p, C.c_uint32, _P(C.c_double)]),
+    "talus_mi_topology_get": (C.c_int, [C.c_void_p, C.c_uint32, _P(C.c_uint32), _P(C.c_double),
+                                        _P(C.c_double), _P(C.c_uint8)]),
+    "talus_mi_step": (C.c_int, [C.c_void_p, _P(StepParams), _P(Report)]),
+    "talus_mi_events": (C.c_int, [C.c_void_p, _P(Event), C.c_uint32, _P(C.c_uint32)]),
+    "talus_mi_eval_forces": (C.c_int, [C.c_void_p, C.c_uint32, _P(C.c_double), _P(C.c_double),
+                                       _P(C.c_double), _P(C.c_double), _P(C.c_double)]),
+    "talus_mi_apply_system": (C.c_int, [C.c_void_p, C.c_uint32, C.c_double, C.c_double,
+                                        _P(C.c_double), _P(C.c_double)]),
+    "talus_mi_time_kernel": (C.c_int, [C.c_void_p, C.c_uint32, C.c_uint32, _P(C.c_double)]),
+    "talus_mi_stream": (C.c_void_p, [C.c_void_p]),
+}
+
+_lib = None
+
+
+def lib():
+    """Loads the in-tree libtalus_mi.so; raises if it has not been built."""
+    global _lib
+    if _lib is None:
+        if not os.path.exists(LIB_PATH):
+            raise TalusError(f"{LIB_PATH} is missing: run `python -m talus_amd.build`")
+        L = C.CDLL(LIB_PATH)
+        for name, (res, args) in _SIGNATURES.items():
+            fn = getattr(L, name)
+            fn.restype = res
+            fn.argtypes = args
+        _lib = L
+    return _lib
+
+
+def _check(rc):
+    if rc != 0:
+        msg = lib().talus_mi_last_error()
+        raise TalusError(f"status {rc}: {msg.decode() if msg else ''}")
+
+
+def _dp(a):
+    return None if a is None else a.ctypes.data_as(C.POINTER(C.c_double))
+
+
+def _f64(a, n=None):
+    if a is None:
+        return None
+    a = np.ascontiguousarray(a, dtype=np.float64).reshape(-1)
+    if n is not None and a.size != n:
+        raise ValueError(f"expected {n} values, got {a.size}")
+    return a
+
+
+def _materials(materials):
+    """ctypes array (or pointer into a numpy record array with the
+    PhysicalMaterial layout) of the material lanes."""
+    if isinstance(materials, np.ndarray):
+        if materials.dtype.itemsize != C.sizeof(PhysicalMaterial):
+            raise ValueError("material records must have the PhysicalMaterial layout")
+        arr = np.ascontiguousarray(materials)
+        ptr = arr.ctypes.data_as(C.POINTER(PhysicalMaterial))
+        ptr._keep = arr  # keep the buffer alive with the pointer
+        return ptr
+    arr = (PhysicalMaterial * max(len(materials), 1))()
+    for i, m in enumerate(materials):
+        arr[i] = m
+    return arr
+
+
+def build_lattice(resolution, voxel_size, origin=(0.0, 0.0, 0.0), materials=(), supports=(),
+                  sample_positions=None, cell_lane=None, solid=None):
+    """Kuhn tet mesh of a structured lattice; returns a dict of numpy arrays
+    (rest_positions, node_mass, anchored, tet_nodes, rest_inverse, volume, tet_material)."""
+    L = lib()
+    lat = _Lattice()
+    lat.resolution[:] = [int(r) for r in resolution]
+    lat.voxel_size[:] = [float(v) for v in voxel_size]
+    lat.origin[:] = [float(v) for v in origin]
+    keep = []
+    if sample_positions is not None:
+        sp = np.ascontiguousarray(sample_positions, dtype=np.float32).reshape(-1)
+        keep.append(sp)
+        lat.sample_positions = sp.ctypes.data_as(C.POINTER(C.c_float))
+    if cell_lane is not None:
+        cl = np.ascontiguousarray(cell_lane, dtype=np.uint32)
+        keep.append(cl)
+        lat.cell_lane = cl.ctypes.data_as(C.POINTER(C.c_uint32))
+    if solid is not None:
+        so = np.ascontiguousarray(solid, dtype=np.uint8)
+        keep.append(so)
+        lat.solid = so.ctypes.data_as(C.POINTER(C.c_uint8))
+    mats = _materials(materials if isinstance(materials, np.ndarray) else list(materials))
+    sups = (Support * max(len(supports), 1))()
+    for i, s in enumerate(supports):
+        sups[i] = s
+    out = _Mesh()
+    _check(L.talus_mi_build_lattice(C.byref(lat), mats, len(materials), sups, len(supports),
+                                    C.byref(out)))
+    n, t = out.nodes, out.tets
+    try:
+        mesh = {
+            "rest_positions": np.ctypeslib.as_array(out.rest_positions, (3 * n,)).reshape(n, 3).copy(),
+            "node_mass": np.ctypeslib.as_array(out.node_mass, (n,)).copy(),
+            "anchored": np.ctypeslib.as_array(out.anchored, (n,)).copy(),
+            "tet_nodes": np.ctypeslib.as_array(out.tet_nodes, (4 * t,)).reshape(t, 4).copy(),
+            "rest_inverse": np.ctypeslib.as_array(out.rest_inverse, (9 * t,)).reshape(t, 3, 3).copy(),
+            "volume": np.ctypeslib.as_array(out.volume, (t,)).copy(),
+            "tet_material": np.ctypeslib.as_array(out.tet_material, (t,)).copy(),
+        }
+    finally:
+        L.talus_mi_mesh_free(C.byref(out))
+    return mesh
+
+
+def default_params(**kw) -> StepParams:
+    p = StepParams()
+    lib().talus_mi_default_params(C.byref(p))
+    for k, v in kw.items():
+        if k == "gravity":
+            p.gravity[:] = list(v)
+        else:
+            setattr(p, k, v)
+    return p
+
+
+class Context:
+    """One device context: bodies, device buffers and a HIP stream."""
+
+    def __init__(self, device: int = 0, precision: int = FP64):
+        h = C.c_void_p()
+        _check(lib().talus_mi_create(C.byref(h), device, precision))
+        self._h = h
+        self.precision = precision
+        self.bodies = []  # (nodes, tets)
+
+    def close(self):
+        if getattr(self, "_h", None):
+            lib().talus_mi_destroy(self._h)
+            self._h = None
+
+    def __del__(self):
+        self.close()
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *exc):
+        self.close()
+
+    def add_body(self, mesh, materials, use_mesh_mass=False) -> int:
+        """Adds a tet body (copied); masses are rho V / 4 in FP64 unless
+        use_mesh_mass passes the mesh's node_mass."""
+        rest = np.ascontiguousarray(mesh["rest_positions"], dtype=np.float32).reshape(-1)
+        anchored = np.ascontiguousarray(mesh["anchored"], dtype=np.uint8)
+        tets = np.ascontiguousarray(mesh["tet_nodes"], dtype=np.uint32).reshape(-1)
+        lanes = np.ascontiguousarray(mesh["tet_material"], dtype=np.uint32)
+        m = _Mesh()
+        m.nodes = anchored.size
+        m.tets = lanes.size
+        m.rest_positions = rest.ctypes.data_as(C.POINTER(C.c_float))
+        m.anchored = anchored.ctypes.data_as(C.POINTER(C.c_uint8))
+        m.tet_nodes = tets.ctypes.data_as(C.POINTER(C.c_uint32))
+        m.tet_material = lanes.ctypes.data_as(C.POINTER(C.c_uint32))
+        if use_mesh_mass:
+            mass = np.ascontiguousarray(mesh["node_mass"], dtype=np.float32)
+            m.node_mass = mass.ctypes.data_as(C.POINTER(C.c_float))
+        mats = _materials(materials if isinstance(materials, np.ndarray) else list(materials))
+        body = C.c_uint32()
+        _check(lib().talus_mi_body_add(self._h, C.byref(m), mats, len(materials), C.byref(body)))
+        self.bodies.append((int(m.nodes), int(m.tets)))
+        return int(body.value)
+
+    def info(self, body):
+        n, t = C.c_uint32(), C.c_uint32()
+        _check(lib().talus_mi_body_info(self._h, body, C.byref(n), C.byref(t)))
+        return int(n.value), int(t.value)
+
+    def set_state(self, body, x=None, v=None, a=None):
+        n, _ = self.info(body)
+        x, v, a = _f64(x, 3 * n), _f64(v, 3 * n), _f64(a, 3 * n)
+        _check(lib().talus_mi_state_set(self._h, body, _dp(x), _dp(v), _dp(a)))
+
+    def state(self, body):
+        n, _ = self.info(body)
+        x, v, a = (np.empty(3 * n) for _ in range(3))
+        _check(lib().talus_mi_state_get(self._h, body, _dp(x), _dp(v), _dp(a)))
+        return x.reshape(n, 3), v.reshape(n, 3), a.reshape(n, 3)
+
+    def rotations(self, body):
+        _, t = self.info(body)
+        R = np.empty(9 * t)
+        _check(lib().talus_mi_rotations_get(self._h, body, _dp(R)))
+        return R.reshape(t, 3, 3)
+
+    def set_rotations(self, body, R):
+        _, t = self.info(body)
+        R = _f64(R, 9 * t)
+        _check(lib().talus_mi_rotations_set(self._h, body, _dp(R)))
+
+    def plastic(self, body):
+        _, t = self.info(body)
+        p = np.empty(10 * t)
+        _check(lib().talus_mi_plastic_get(self._h, body, _dp(p)))
+        return p.reshape(t, 10)
+
+    def set_plastic(self, body, p):
+        _, t = self.info(body)
+        p = _f64(p, 10 * t)
+        _check(lib().talus_mi_plastic_set(self._h, body, _dp(p)))
+
+    def topology(self, body):
+        n, t = self.info(body)
+        tets = np.empty(4 * t, dtype=np.uint32)
+        rest = np.empty(3 * n)
+        mass = np.empty(n)
+        anchored = np.empty(n, dtype=np.uint8)
+        _check(lib().talus_mi_topology_get(self._h, body, tets.ctypes.data_as(C.POINTER(C.c_uint32)),
+                                           _dp(rest), _dp(mass),
+                                           anchored.ctypes.data_as(C.POINTER(C.c_uint8))))
+        return {"tet_nodes": tets.reshape(t, 4), "rest": rest.reshape(n, 3), "mass": mass,
+                "anchored": anchored}
+
+    def step(self, params: StepParams | None = None, **kw) -> dict:
+        p = params if params is not None else default_params(**kw)
+        r = Report()
+        _check(lib().talus_mi_step(self._h, C.byref(p), C.byref(r)))
+        return r.as_dict()
+
+    def events(self, capacity=1 << 20):
+        buf = (Event * capacity)()
+        n = C.c_uint32()
+        _check(lib().talus_mi_events(self._h, buf, capacity, C.byref(n)))
+        return [(e.kind, e.body, e.tet, e.step) for e in buf[: n.value]]
+
+    def eval_forces(self, body, x):
+        n, t = self.info(body)
+        x = _f64(x, 3 * n)
+        tf, te, nf, R = np.empty(12 * t), np.empty(t), np.empty(3 * n), np.empty(9 * t)
+        _check(lib().talus_mi_eval_forces(self._h, body, _dp(x), _dp(tf), _dp(te), _dp(nf), _dp(R)))
+        return tf.reshape(t, 4, 3), te, nf.reshape(n, 3), R.reshape(t, 3, 3)
+
+    def apply_system(self, body, inertia, step, d):
+        n, _ = self.info(body)
+        d = _f64(d, 3 * n)
+        y = np.empty(3 * n)
+        _check(lib().talus_mi_apply_system(self._h, body, inertia, step, _dp(d), _dp(y)))
+        return y.reshape(n, 3)
+
+    def time_kernel(self, kind, repeats=20) -> float:
+        ms = C.c_double()
+        _check(lib().talus_mi_time_kernel(self._h, kind, repeats, C.byref(ms)))
+        return float(ms.value)
+
+    def stream(self) -> int:
+        return int(lib().talus_mi_stream(self._h) or 0)

@@ -1,0 +1,1007 @@
+// context.hip -- context, device buffers, step orchestration and the C ABI
+// of libtalus_mi.
+//
+// A context owns one HIP stream and every device buffer.  All bodies are
+// concatenated into one set of SoA arrays and stepped by the same launches
+// (one launch per kernel per substep pass, whatever the body count), each
+// body with its own substep count, PCG scalars and convergence.
+//
+// Step flow per body follows deformable::step for a non-floating tet body
+// (src/talus/deformable/solver/solver.cc:3472-5901): predictorInverts raises
+// the substep count to 4 (3574-3583); per substep the alpha-point predictor
+// (3680-3701), assembly + balance + PCG solve for the new acceleration, two
+// re-assembled correction solves (ANCHORED_READS = 3, 3895-3950), velocity
+// and position updates (4351-4368, 5829-5849) and commitPlastic (5851).
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <cmath>
+#include <cstring>
+#include <memory>
+#include <stdexcept>
+#include <string>
+#include <vector>
+
+#include "host.h"
+#include "kernels.h"
+#include "tmath.h"
+
+namespace tmi {
+
+namespace {
+thread_local std::string g_error;
+}
+
+int fail(int code, const std::string &message) {
+  g_error = message;
+  return code;
+}
+
+struct DeviceError : std::runtime_error {
+  using std::runtime_error::runtime_error;
+};
+
+#define HIPCHECK(expr)                                                                       \
+  do {                                                                                       \
+    const hipError_t status_ = (expr);                                                       \
+    if (status_ != hipSuccess)                                                               \
+      throw ::tmi::DeviceError(std::string(#expr) + " failed: " + hipGetErrorString(status_));      \
+  } while (0)
+
+int prepare_body(Body &b) {
+  const uint32_t lanes = uint32_t(b.materials.size());
+  b.lambda.resize(b.tets);
+  b.mu.resize(b.tets);
+  b.yield.resize(b.tets);
+  b.hardening.resize(b.tets);
+  b.plastic_max.resize(b.tets);
+  b.tear.resize(b.tets);
+  b.flags.resize(b.tets);
+  b.dminv.resize(9 * size_t(b.tets));
+  b.volume.resize(b.tets);
+  for (uint32_t e = 0; e < b.tets; ++e) {
+    for (int c = 0; c < 4; ++c)
+      if (b.tet[4 * size_t(e) + c] >= b.nodes) return fail(TALUS_MI_ERR_ARGUMENT, "tet node index out of range");
+    if (!rest_frame(b.rest.data(), &b.tet[4 * size_t(e)], &b.dminv[9 * size_t(e)], &b.volume[e]))
+      return fail(TALUS_MI_ERR_ARGUMENT, "tet " + std::to_string(e) + " is not positively oriented");
+    const talus_mi_material &m = lane_material(b.materials.data(), lanes, b.lane[e]);
+    precise_lame(m.youngs_modulus, m.poissons_ratio, &b.lambda[e], &b.mu[e]);
+    const bool plastic = std::isfinite(m.yield_stress) && m.yield_stress > 0.0f;
+    b.yield[e] = plastic ? double(m.yield_stress) : INFINITY;
+    b.hardening[e] = m.hardening_modulus > 0.0f ? double(m.hardening_modulus) : 0.0;
+    b.plastic_max[e] = double(m.plastic_max);
+    b.tear[e] = 1.0 + double(m.tear_strain);
+    uint8_t f = 0;
+    if (m.elastic_model == 1u) f |= FLAG_SNH;
+    if (plastic) f |= FLAG_PLASTIC;
+    if (std::isfinite(m.tear_strain)) f |= FLAG_TEAR;
+    b.flags[e] = f;
+  }
+  if (b.density_mass) {
+    b.mass.assign(b.nodes, 0.0);
+    for (uint32_t e = 0; e < b.tets; ++e) {
+      const double rho = double(lane_material(b.materials.data(), lanes, b.lane[e]).density);
+      for (int c = 0; c < 4; ++c) b.mass[b.tet[4 * size_t(e) + c]] += rho * b.volume[e] * 0.25;
+    }
+  }
+  return TALUS_MI_OK;
+}
+
+// ------------------------------------------------------------------ engine
+
+class EngineBase {
+ public:
+  virtual ~EngineBase() = default;
+  virtual void build(const std::vector<Body> &bodies) = 0;
+  virtual void download(std::vector<Body> &bodies) = 0;
+  virtual void step(const talus_mi_step_params &sp, talus_mi_report &rep,
+                    std::vector<talus_mi_event> &events, uint32_t step_index,
+                    std::vector<uint8_t> &torn) = 0;
+  virtual void state_set(uint32_t b, const double *x, const double *v, const double *a) = 0;
+  virtual void state_get(uint32_t b, double *x, double *v, double *a) = 0;
+  virtual void rotations_get(uint32_t b, double *R) = 0;
+  virtual void rotations_set(uint32_t b, const double *R) = 0;
+  virtual void plastic_get(uint32_t b, double *p) = 0;
+  virtual void plastic_set(uint32_t b, const double *p) = 0;
+  virtual void eval_forces(uint32_t b, const double *x, double *tf, double *te, double *nf,
+                           double *R) = 0;
+  virtual void apply_system(uint32_t b, double inertia, double step, const double *dv,
+                            double *y) = 0;
+  virtual double time_kernel(uint32_t kind, uint32_t repeats) = 0;
+};
+
+template <class Real>
+class Engine final : public EngineBase {
+ public:
+  explicit Engine(hipStream_t s) : stream_(s) {}
+  ~Engine() override { release(); }
+
+  void build(const std::vector<Body> &bodies) override;
+  void download(std::vector<Body> &bodies) override;
+  void step(const talus_mi_step_params &sp, talus_mi_report &rep,
+            std::vector<talus_mi_event> &events, uint32_t step_index,
+            std::vector<uint8_t> &torn) override;
+  void state_set(uint32_t b, const double *x, const double *v, const double *a) override;
+  void state_get(uint32_t b, double *x, double *v, double *a) override;
+  void rotations_get(uint32_t b, double *R) override;
+  void rotations_set(uint32_t b, const double *R) override;
+  void plastic_get(uint32_t b, double *p) override;
+  void plastic_set(uint32_t b, const double *p) override;
+  void eval_forces(uint32_t b, const double *x, double *tf, double *te, double *nf,
+                   double *R) override;
+  void apply_system(uint32_t b, double inertia, double step, const double *dv,
+                    double *y) override;
+  double time_kernel(uint32_t kind, uint32_t repeats) override;
+
+ private:
+  template <class T>
+  T *dev(size_t n) {
+    T *p = nullptr;
+    HIPCHECK(hipMalloc(&p, std::max<size_t>(n, 1) * sizeof(T)));
+    owned_.push_back(p);
+    return p;
+  }
+  template <class T>
+  T *dev_copy(const std::vector<T> &h) {
+    T *p = dev<T>(h.size());
+    if (!h.empty()) HIPCHECK(hipMemcpy(p, h.data(), h.size() * sizeof(T), hipMemcpyHostToDevice));
+    return p;
+  }
+  template <class T>
+  T *pinned(size_t n) {
+    T *p = nullptr;
+    HIPCHECK(hipHostMalloc(&p, std::max<size_t>(n, 1) * sizeof(T), hipHostMallocDefault));
+    pinned_.push_back(p);
+    return p;
+  }
+  void release() {
+    if (stream_) (void)hipStreamSynchronize(stream_);
+    for (void *p : owned_) (void)hipFree(p);
+    for (void *p : pinned_) (void)hipHostFree(p);
+    owned_.clear();
+    pinned_.clear();
+    built_ = false;
+  }
+  void upload_params(const std::vector<StepParams> &params, size_t slot);
+  void solve(bool first, uint32_t limit, uint32_t solve_index);
+  void all_active();
+
+  hipStream_t stream_;
+  std::vector<void *> owned_, pinned_;
+  bool built_ = false;
+  Dev<Real> d_{};
+  std::vector<uint32_t> n0_, t0_, nn_, tt_, nt0_, nt1_, tt0_, tt1_;
+  // pinned staging
+  StepParams *hparams_ = nullptr;
+  size_t hparams_slots_ = 0;
+  int *hflags_ = nullptr;
+  uint32_t *hiters_ = nullptr;
+  double *hrr_ = nullptr, *htarget_ = nullptr;
+  size_t hsolve_slots_ = 0;
+  double *hpartial_ = nullptr, *hepartial_ = nullptr;
+  uint32_t *hyields_ = nullptr;
+  Real *hvec_ = nullptr;  // 3N + 12T staging of Real
+  double *scratch64_ = nullptr;
+  bool any_plastic_ = false, any_tear_ = false;
+};
+
+template <class Real>
+void Engine<Real>::build(const std::vector<Body> &bodies) {
+  release();
+  const uint32_t B = uint32_t(bodies.size());
+  uint64_t N = 0, T = 0;
+  n0_.assign(B, 0);
+  t0_.assign(B, 0);
+  nn_.assign(B, 0);
+  tt_.assign(B, 0);
+  for (uint32_t b = 0; b < B; ++b) {
+    n0_[b] = uint32_t(N);
+    t0_[b] = uint32_t(T);
+    nn_[b] = bodies[b].nodes;
+    tt_[b] = bodies[b].tets;
+    N += bodies[b].nodes;
+    T += bodies[b].tets;
+  }
+  if (T >= (1ull << 30) || N >= (1ull << 31)) throw std::length_error("too many tets or nodes");
+  d_ = Dev<Real>{};
+  d_.N = uint32_t(N);
+  d_.T = uint32_t(T);
+  d_.B = B;
+  // tiles
+  std::vector<uint32_t> ntb, ntbeg, ntend, ttb, ttbeg, ttend;
+  nt0_.assign(B, 0);
+  nt1_.assign(B, 0);
+  tt0_.assign(B, 0);
+  tt1_.assign(B, 0);
+  for (uint32_t b = 0; b < B; ++b) {
+    nt0_[b] = uint32_t(ntb.size());
+    for (uint32_t s = 0; s < nn_[b]; s += TILE) {
+      ntb.push_back(b);
+      ntbeg.push_back(n0_[b] + s);
+      ntend.push_back(n0_[b] + std::min<uint32_t>(s + TILE, nn_[b]));
+    }
+    nt1_[b] = uint32_t(ntb.size());
+    tt0_[b] = uint32_t(ttb.size());
+    for (uint32_t s = 0; s < tt_[b]; s += TILE) {
+      ttb.push_back(b);
+      ttbeg.push_back(t0_[b] + s);
+      ttend.push_back(t0_[b] + std::min<uint32_t>(s + TILE, tt_[b]));
+    }
+    tt1_[b] = uint32_t(ttb.size());
+  }
+  d_.n_ntiles = uint32_t(ntb.size());
+  d_.n_ttiles = uint32_t(ttb.size());
+  d_.nt_body = dev_copy(ntb);
+  d_.nt_begin = dev_copy(ntbeg);
+  d_.nt_end = dev_copy(ntend);
+  d_.tt_body = dev_copy(ttb);
+  d_.tt_begin = dev_copy(ttbeg);
+  d_.tt_end = dev_copy(ttend);
+  d_.body_nt0 = dev_copy(nt0_);
+  d_.body_nt1 = dev_copy(nt1_);
+  // nodes
+  std::vector<double> X(3 * N), mass(N), x(3 * N), v(3 * N), a(3 * N);
+  std::vector<uint8_t> anchored(N);
+  std::vector<uint32_t> inc_start(N + 1, 0), inc_slot(4 * T);
+  std::vector<uint4> conn(T);
+  std::vector<double> dminv(9 * T), vol64(T), lam64(T), mu64(T), yield(T), hard(T), pmax(T),
+      tear(T), plastic(10 * T);
+  std::vector<Real> vol(T), lam(T), mu(T), R(9 * T);
+  std::vector<uint8_t> flags(T);
+  any_plastic_ = any_tear_ = false;
+  for (uint32_t b = 0; b < B; ++b) {
+    const Body &bd = bodies[b];
+    const size_t n0 = n0_[b], t0 = t0_[b];
+    std::copy(bd.rest.begin(), bd.rest.end(), X.begin() + 3 * n0);
+    std::copy(bd.mass.begin(), bd.mass.end(), mass.begin() + n0);
+    std::copy(bd.anchored.begin(), bd.anchored.end(), anchored.begin() + n0);
+    std::copy(bd.x.begin(), bd.x.end(), x.begin() + 3 * n0);
+    std::copy(bd.v.begin(), bd.v.end(), v.begin() + 3 * n0);
+    std::copy(bd.a.begin(), bd.a.end(), a.begin() + 3 * n0);
+    for (uint32_t e = 0; e < bd.tets; ++e) {
+      const size_t g = t0 + e;
+      const uint32_t *tn = &bd.tet[4 * size_t(e)];
+      conn[g] = make_uint4(uint32_t(n0 + tn[0]), uint32_t(n0 + tn[1]), uint32_t(n0 + tn[2]),
+                           uint32_t(n0 + tn[3]));
+      for (int c = 0; c < 4; ++c) ++inc_start[n0 + tn[c] + 1];
+      for (int k = 0; k < 9; ++k) {
+        dminv[size_t(k) * T + g] = bd.dminv[9 * size_t(e) + k];
+        R[size_t(k) * T + g] = Real(bd.R[9 * size_t(e) + k]);
+      }
+      for (int k = 0; k < 10; ++k) plastic[size_t(k) * T + g] = bd.plastic[10 * size_t(e) + k];
+      vol64[g] = bd.volume[e];
+      lam64[g] = bd.lambda[e];
+      mu64[g] = bd.mu[e];
+      vol[g] = Real(bd.volume[e]);
+      lam[g] = Real(bd.lambda[e]);
+      mu[g] = Real(bd.mu[e]);
+      yield[g] = bd.yield[e];
+      hard[g] = bd.hardening[e];
+      pmax[g] = bd.plastic_max[e];
+      tear[g] = bd.tear[e];
+      flags[g] = bd.flags[e];
+      any_plastic_ = any_plastic_ || (bd.flags[e] & FLAG_PLASTIC);
+      any_tear_ = any_tear_ || (bd.flags[e] & FLAG_TEAR);
+    }
+  }
+  for (size_t n = 0; n < N; ++n) inc_start[n + 1] += inc_start[n];
+  {
+    std::vector<uint32_t> cursor(inc_start.begin(), inc_start.end() - 1);
+    const uint32_t *cn = reinterpret_cast<const uint32_t *>(conn.data());
+    for (size_t s = 0; s < 4 * T; ++s) inc_slot[cursor[cn[s]]++] = uint32_t(s);
+  }
+  d_.X = dev_copy(X);
+  d_.mass = dev_copy(mass);
+  d_.anchored = dev_copy(anchored);
+  d_.inc_start = dev_copy(inc_start);
+  d_.inc_slot = dev_copy(inc_slot);
+  d_.x = dev_copy(x);
+  d_.v = dev_copy(v);
+  d_.a = dev_copy(a);
+  d_.carried = dev<double>(3 * N);
+  d_.shifted = dev<double>(3 * N);
+  d_.weighted = dev<double>(3 * N);
+  d_.right = dev<double>(3 * N);
+  d_.u_eval = dev<Real>(3 * N);
+  d_.P = dev<Real>(6 * N);
+  d_.x0 = dev<Real>(3 * N);
+  d_.dx = dev<Real>(3 * N);
+  d_.r = dev<Real>(3 * N);
+  d_.z = dev<Real>(3 * N);
+  d_.p = dev<Real>(3 * N);
+  d_.q = dev<Real>(3 * N);
+  d_.conn = dev_copy(conn);
+  d_.dminv = dev_copy(dminv);
+  d_.vol = dev_copy(vol);
+  d_.lam = dev_copy(lam);
+  d_.mu = dev_copy(mu);
+  d_.vol64 = dev_copy(vol64);
+  d_.lam64 = dev_copy(lam64);
+  d_.mu64 = dev_copy(mu64);
+  d_.flags = dev_copy(flags);
+  d_.yield = dev_copy(yield);
+  d_.hard = dev_copy(hard);
+  d_.pmax = dev_copy(pmax);
+  d_.tear = dev_copy(tear);
+  d_.R = dev_copy(R);
+  d_.plastic = dev_copy(plastic);
+  d_.slot_f = dev<Real>(12 * T);
+  d_.slot_d = dev<Real>(24 * T);
+  d_.energy = dev<Real>(T);
+  d_.yflag = dev<uint8_t>(T);
+  d_.tflag = dev<uint8_t>(T);
+  d_.partial = dev<double>(size_t(d_.n_ntiles) * NPART);
+  d_.epartial = dev<double>(d_.n_ttiles);
+  d_.sq = dev<double>(2 * size_t(B));
+  d_.done = dev<int>(2 * size_t(B));
+  d_.iters = dev<uint32_t>(B);
+  d_.target2 = dev<double>(B);
+  d_.rr_last = dev<double>(B);
+  d_.inverts = dev<int>(B);
+  d_.yields = dev<uint32_t>(B);
+  d_.params = dev<StepParams>(B);
+  HIPCHECK(hipMemset(d_.yflag, 0, std::max<size_t>(T, 1)));
+  HIPCHECK(hipMemset(d_.tflag, 0, std::max<size_t>(T, 1)));
+  HIPCHECK(hipMemset(d_.epartial, 0, std::max<size_t>(d_.n_ttiles, 1) * sizeof(double)));
+  scratch64_ = dev<double>(3 * N);
+  hparams_slots_ = 8;
+  hparams_ = pinned<StepParams>(hparams_slots_ * B);
+  hflags_ = pinned<int>(B);
+  hsolve_slots_ = 3 * 8;
+  hiters_ = pinned<uint32_t>(hsolve_slots_ * B);
+  hrr_ = pinned<double>(hsolve_slots_ * B);
+  htarget_ = pinned<double>(hsolve_slots_ * B);
+  hpartial_ = pinned<double>(size_t(d_.n_ntiles) * NPART);
+  hepartial_ = pinned<double>(d_.n_ttiles);
+  hyields_ = pinned<uint32_t>(B);
+  hvec_ = pinned<Real>(3 * N + 12 * T + 9 * T);
+  HIPCHECK(hipDeviceSynchronize());
+  built_ = true;
+}
+
+template <class Real>
+void Engine<Real>::download(std::vector<Body> &bodies) {
+  if (!built_) return;
+  for (uint32_t b = 0; b < bodies.size() && b < n0_.size(); ++b) {
+    Body &bd = bodies[b];
+    if (bd.nodes != nn_[b] || bd.tets != tt_[b]) continue;
+    bd.x.resize(3 * size_t(bd.nodes));
+    bd.v.resize(3 * size_t(bd.nodes));
+    bd.a.resize(3 * size_t(bd.nodes));
+    state_get(b, bd.x.data(), bd.v.data(), bd.a.data());
+    bd.R.resize(9 * size_t(bd.tets));
+    rotations_get(b, bd.R.data());
+    bd.plastic.resize(10 * size_t(bd.tets));
+    plastic_get(b, bd.plastic.data());
+  }
+}
+
+template <class Real>
+void Engine<Real>::upload_params(const std::vector<StepParams> &params, size_t slot) {
+  StepParams *h = hparams_ + (slot % hparams_slots_) * d_.B;
+  std::memcpy(h, params.data(), d_.B * sizeof(StepParams));
+  HIPCHECK(hipMemcpyAsync(const_cast<StepParams *>(d_.params), h, d_.B * sizeof(StepParams),
+                          hipMemcpyHostToDevice, stream_));
+}
+
+template <class Real>
+void Engine<Real>::all_active() {
+  std::vector<StepParams> p(d_.B);
+  for (auto &q : p) {
+    std::memset(&q, 0, sizeof q);
+    q.active = 1;
+  }
+  upload_params(p, hparams_slots_ - 1);
+}
+
+// conjugateGradient for every active body at once.  The acceleration (first
+// solve) or zero (corrections) is the starting iterate; the iterate change
+// accumulates in dx and is added to a by apply_increment.
+template <class Real>
+void Engine<Real>::solve(bool first, uint32_t limit, uint32_t solve_index) {
+  hipStream_t s = stream_;
+  if (first) {
+    Launch<Real>::op_element(d_, d_.x0, -1, s);
+    Launch<Real>::pcg_init(d_, true, s);
+  } else {
+    Launch<Real>::pcg_init(d_, false, s);
+  }
+  Launch<Real>::pcg_begin(d_, s);
+  constexpr uint32_t CHECK = 8;
+  for (uint32_t k = 0; k < limit; ++k) {
+    const int parity = int(k & 1u);
+    Launch<Real>::op_element(d_, d_.p, parity, s);
+    Launch<Real>::op_gather(d_, d_.p, d_.q, parity, s);
+    Launch<Real>::pcg_update(d_, parity, s);
+    Launch<Real>::pcg_direction(d_, parity, s);
+    if ((k + 1) % CHECK == 0 && k + 1 < limit) {
+      HIPCHECK(hipMemcpyAsync(hflags_, d_.done + size_t((k + 1) & 1u) * d_.B, d_.B * sizeof(int),
+                              hipMemcpyDeviceToHost, s));
+      HIPCHECK(hipStreamSynchronize(s));
+      bool all = true;
+      for (uint32_t b = 0; b < d_.B && all; ++b) all = hflags_[b] != 0;
+      if (all) break;
+    }
+  }
+  Launch<Real>::apply_increment(d_, first, s);
+  const size_t slot = (solve_index % hsolve_slots_) * d_.B;
+  HIPCHECK(hipMemcpyAsync(hiters_ + slot, d_.iters, d_.B * sizeof(uint32_t), hipMemcpyDeviceToHost, s));
+  HIPCHECK(hipMemcpyAsync(hrr_ + slot, d_.rr_last, d_.B * sizeof(double), hipMemcpyDeviceToHost, s));
+  HIPCHECK(hipMemcpyAsync(htarget_ + slot, d_.target2, d_.B * sizeof(double), hipMemcpyDeviceToHost, s));
+}
+
+template <class Real>
+void Engine<Real>::step(const talus_mi_step_params &sp, talus_mi_report &rep,
+                        std::vector<talus_mi_event> &events, uint32_t step_index,
+                        std::vector<uint8_t> &torn) {
+  const uint32_t B = d_.B;
+  hipStream_t s = stream_;
+  hipEvent_t began, ended;
+  HIPCHECK(hipEventCreate(&began));
+  HIPCHECK(hipEventCreate(&ended));
+  HIPCHECK(hipEventRecord(began, s));
+  // integratorRule (solver.cc:531-545), effectiveSystem (554-562)
+  double spectral = sp.spectral_radius;
+  spectral = spectral < 0.0 ? 0.0 : (spectral > 1.0 ? 1.0 : spectral);
+  const double am = (2.0 * spectral - 1.0) / (spectral + 1.0);
+  const double af = spectral / (spectral + 1.0);
+  const double gamma = 0.5 - am + af;
+  const double beta = 0.25 * (1.0 - am + af) * (1.0 - am + af);
+  const uint32_t configured = std::max(sp.substeps, 1u);
+  const bool factor = sp.preconditioner == TALUS_MI_FACTOR;
+  const uint32_t limit =
+      factor && sp.factor_budget ? std::min(sp.factor_budget, sp.cg_iterations) : sp.cg_iterations;
+  auto params_for = [&](uint32_t substeps, bool active) {
+    StepParams p;
+    std::memset(&p, 0, sizeof p);
+    const double inner = double(sp.dt) / substeps;
+    p.inner = inner;
+    p.carry = (0.5 - beta) * inner * inner;
+    const double heff = std::sqrt((1.0 - af) * beta) * inner;
+    p.reach = heff * heff;
+    p.inertia = 1.0 - am;
+    p.am = am;
+    p.af = af;
+    p.beta = beta;
+    p.gamma = gamma;
+    for (int k = 0; k < 3; ++k) p.g[k] = double(sp.gravity[k]);
+    p.tolerance = double(sp.tolerance);
+    p.active = active ? 1u : 0u;
+    p.limit = limit;
+    return p;
+  };
+  // predictorInverts at the configured substep
+  std::vector<StepParams> params(B);
+  for (uint32_t b = 0; b < B; ++b) params[b] = params_for(configured, true);
+  upload_params(params, 0);
+  HIPCHECK(hipMemsetAsync(d_.inverts, 0, B * sizeof(int), s));
+  HIPCHECK(hipMemsetAsync(d_.yields, 0, B * sizeof(uint32_t), s));
+  HIPCHECK(hipMemsetAsync(d_.yflag, 0, std::max<uint32_t>(d_.T, 1), s));
+  Launch<Real>::predictor_inverts(d_, s);
+  HIPCHECK(hipMemcpyAsync(hflags_, d_.inverts, B * sizeof(int), hipMemcpyDeviceToHost, s));
+  HIPCHECK(hipStreamSynchronize(s));
+  std::vector<uint32_t> substeps(B);
+  uint32_t passes = 0;
+  for (uint32_t b = 0; b < B; ++b) {
+    substeps[b] = hflags_[b] ? std::max(configured, 4u) : configured;
+    passes = std::max(passes, substeps[b]);
+  }
+  if (passes + 1 > hparams_slots_) {
+    // grow the pinned parameter ring (only for unusual substep counts)
+    HIPCHECK(hipStreamSynchronize(s));
+    hparams_slots_ = passes + 2;
+    hparams_ = pinned<StepParams>(hparams_slots_ * B);
+  }
+  if (3 * passes > hsolve_slots_) {
+    HIPCHECK(hipStreamSynchronize(s));
+    hsolve_slots_ = 3 * passes;
+    hiters_ = pinned<uint32_t>(hsolve_slots_ * B);
+    hrr_ = pinned<double>(hsolve_slots_ * B);
+    htarget_ = pinned<double>(hsolve_slots_ * B);
+  }
+  const bool diag = !factor || true;  // block-Jacobi preconditioner
+  uint32_t solve_index = 0;
+  for (uint32_t pass = 0; pass < passes; ++pass) {
+    for (uint32_t b = 0; b < B; ++b) params[b] = params_for(substeps[b], pass < substeps[b]);
+    upload_params(params, 1 + pass);
+    Launch<Real>::predict(d_, s);
+    Launch<Real>::assemble(d_, diag, false, s);
+    Launch<Real>::balance(d_, diag, false, s);
+    solve(true, limit, solve_index++);
+    for (uint32_t read = 1; read < 3; ++read) {
+      Launch<Real>::evaluate(d_, s);
+      Launch<Real>::assemble(d_, diag, read == 2, s);
+      Launch<Real>::balance(d_, diag, true, s);
+      solve(false, limit, solve_index++);
+    }
+    Launch<Real>::finish(d_, s);
+    if (any_plastic_) Launch<Real>::commit(d_, s);
+  }
+  // report reductions over every body
+  for (uint32_t b = 0; b < B; ++b) params[b] = params_for(substeps[b], true);
+  upload_params(params, 1 + passes);
+  Launch<Real>::kinetic(d_, s);
+  const bool tears = (sp.flags & TALUS_MI_DETECT_TEARS) && any_tear_;
+  if (tears) Launch<Real>::tear(d_, s);
+  HIPCHECK(hipEventRecord(ended, s));
+  HIPCHECK(hipMemcpyAsync(hpartial_, d_.partial, size_t(d_.n_ntiles) * NPART * sizeof(double),
+                          hipMemcpyDeviceToHost, s));
+  HIPCHECK(hipMemcpyAsync(hepartial_, d_.epartial, d_.n_ttiles * sizeof(double),
+                          hipMemcpyDeviceToHost, s));
+  HIPCHECK(hipMemcpyAsync(hyields_, d_.yields, B * sizeof(uint32_t), hipMemcpyDeviceToHost, s));
+  const bool record = (sp.flags & TALUS_MI_RECORD_EVENTS) != 0;
+  std::vector<uint8_t> yflag, tflag;
+  if (record && any_plastic_) {
+    yflag.resize(d_.T);
+    HIPCHECK(hipMemcpyAsync(yflag.data(), d_.yflag, d_.T, hipMemcpyDeviceToHost, s));
+  }
+  if (tears) {
+    tflag.resize(d_.T);
+    HIPCHECK(hipMemcpyAsync(tflag.data(), d_.tflag, d_.T, hipMemcpyDeviceToHost, s));
+  }
+  HIPCHECK(hipStreamSynchronize(s));
+  float ms = 0.0f;
+  HIPCHECK(hipEventElapsedTime(&ms, began, ended));
+  HIPCHECK(hipEventDestroy(began));
+  HIPCHECK(hipEventDestroy(ended));
+  // report (solver.h StepReport semantics, summed over bodies)
+  std::memset(&rep, 0, sizeof rep);
+  rep.bodies = B;
+  rep.nodes = d_.N;
+  rep.elements = d_.T;
+  rep.substeps = passes;
+  rep.device_seconds = ms * 1.0e-3;
+  for (uint32_t si = 0; si < solve_index; ++si) {
+    const uint32_t pass = si / 3;
+    const size_t slot = (si % hsolve_slots_) * B;
+    for (uint32_t b = 0; b < B; ++b) {
+      if (pass >= substeps[b]) continue;
+      rep.cg_iterations += hiters_[slot + b];
+      const bool capped = hiters_[slot + b] >= limit && !(hrr_[slot + b] <= htarget_[slot + b]);
+      rep.cg_capped += capped ? 1u : 0u;
+      const double reference2 = htarget_[slot + b] / (sp.tolerance * double(sp.tolerance));
+      const double rel = reference2 > 0.0 ? std::sqrt(hrr_[slot + b] / reference2) : 0.0;
+      rep.cg_worst_relative = std::max(rep.cg_worst_relative, rel);
+      ++rep.solves;
+    }
+  }
+  for (uint32_t b = 0; b < B; ++b) {
+    double kinetic = 0.0, bad = 0.0, elastic = 0.0;
+    for (uint32_t t = nt0_[b]; t < nt1_[b]; ++t) {
+      kinetic += hpartial_[t * NPART + 3];
+      bad += hpartial_[t * NPART + 1];
+    }
+    for (uint32_t t = tt0_[b]; t < tt1_[b]; ++t) elastic += hepartial_[t];
+    rep.kinetic_energy += kinetic;
+    rep.elastic_energy += elastic;
+    rep.non_finite |= bad > 0.0 ? 1u : 0u;
+    rep.yield_events += hyields_[b];
+  }
+  rep.plastic_growth = rep.yield_events ? 1u : 0u;
+  torn.assign(d_.T, 0);
+  if (tears) torn = tflag;
+  for (uint32_t b = 0; b < B; ++b) {
+    for (uint32_t e = 0; e < tt_[b]; ++e) {
+      const size_t g = t0_[b] + e;
+      if (record && !yflag.empty() && yflag[g])
+        events.push_back(talus_mi_event{TALUS_MI_EVENT_YIELD, b, e, step_index});
+      if (tears && tflag[g]) {
+        ++rep.tear_events;
+        if (record) events.push_back(talus_mi_event{TALUS_MI_EVENT_TEAR, b, e, step_index});
+      }
+    }
+  }
+}
+
+template <class Real>
+void Engine<Real>::state_set(uint32_t b, const double *x, const double *v, const double *a) {
+  const size_t off = 3 * size_t(n0_[b]), bytes = 3 * size_t(nn_[b]) * sizeof(double);
+  if (x) HIPCHECK(hipMemcpy(d_.x + off, x, bytes, hipMemcpyHostToDevice));
+  if (v) HIPCHECK(hipMemcpy(d_.v + off, v, bytes, hipMemcpyHostToDevice));
+  if (a) HIPCHECK(hipMemcpy(d_.a + off, a, bytes, hipMemcpyHostToDevice));
+}
+
+template <class Real>
+void Engine<Real>::state_get(uint32_t b, double *x, double *v, double *a) {
+  HIPCHECK(hipStreamSynchronize(stream_));
+  const size_t off = 3 * size_t(n0_[b]), bytes = 3 * size_t(nn_[b]) * sizeof(double);
+  if (x) HIPCHECK(hipMemcpy(x, d_.x + off, bytes, hipMemcpyDeviceToHost));
+  if (v) HIPCHECK(hipMemcpy(v, d_.v + off, bytes, hipMemcpyDeviceToHost));
+  if (a) HIPCHECK(hipMemcpy(a, d_.a + off, bytes, hipMemcpyDeviceToHost));
+}
+
+template <class Real>
+void Engine<Real>::rotations_get(uint32_t b, double *R) {
+  HIPCHECK(hipStreamSynchronize(stream_));
+  const size_t T = d_.T, t0 = t0_[b], n = tt_[b];
+  std::vector<Real> plane(n);
+  for (int k = 0; k < 9; ++k) {
+    HIPCHECK(hipMemcpy(plane.data(), d_.R + k * T + t0, n * sizeof(Real), hipMemcpyDeviceToHost));
+    for (size_t e = 0; e < n; ++e) R[9 * e + k] = double(plane[e]);
+  }
+}
+
+template <class Real>
+void Engine<Real>::rotations_set(uint32_t b, const double *R) {
+  const size_t T = d_.T, t0 = t0_[b], n = tt_[b];
+  std::vector<Real> plane(n);
+  for (int k = 0; k < 9; ++k) {
+    for (size_t e = 0; e < n; ++e) plane[e] = Real(R[9 * e + k]);
+    HIPCHECK(hipMemcpy(d_.R + k * T + t0, plane.data(), n * sizeof(Real), hipMemcpyHostToDevice));
+  }
+}
+
+template <class Real>
+void Engine<Real>::plastic_get(uint32_t b, double *p) {
+  HIPCHECK(hipStreamSynchronize(stream_));
+  const size_t T = d_.T, t0 = t0_[b], n = tt_[b];
+  std::vector<double> plane(n);
+  for (int k = 0; k < 10; ++k) {
+    HIPCHECK(hipMemcpy(plane.data(), d_.plastic + k * T + t0, n * sizeof(double), hipMemcpyDeviceToHost));
+    for (size_t e = 0; e < n; ++e) p[10 * e + k] = plane[e];
+  }
+}
+
+template <class Real>
+void Engine<Real>::plastic_set(uint32_t b, const double *p) {
+  const size_t T = d_.T, t0 = t0_[b], n = tt_[b];
+  std::vector<double> plane(n);
+  for (int k = 0; k < 10; ++k) {
+    for (size_t e = 0; e < n; ++e) plane[e] = p[10 * e + k];
+    HIPCHECK(hipMemcpy(d_.plastic + k * T + t0, plane.data(), n * sizeof(double), hipMemcpyHostToDevice));
+  }
+}
+
+template <class Real>
+void Engine<Real>::eval_forces(uint32_t b, const double *x, double *tf, double *te, double *nf,
+                               double *R) {
+  hipStream_t s = stream_;
+  const size_t n0 = n0_[b], nn = nn_[b], t0 = t0_[b], nt = tt_[b];
+  HIPCHECK(hipMemcpyAsync(scratch64_ + 3 * n0, x, 3 * nn * sizeof(double), hipMemcpyHostToDevice, s));
+  all_active();
+  Launch<Real>::displacement(d_, scratch64_, s);
+  Launch<Real>::assemble(d_, false, true, s);
+  Launch<Real>::gather_forces(d_, d_.q, s);
+  Real *h = hvec_;
+  HIPCHECK(hipMemcpyAsync(h, d_.slot_f + 12 * t0, 12 * nt * sizeof(Real), hipMemcpyDeviceToHost, s));
+  HIPCHECK(hipMemcpyAsync(h + 12 * nt, d_.q + 3 * n0, 3 * nn * sizeof(Real), hipMemcpyDeviceToHost, s));
+  HIPCHECK(hipStreamSynchronize(s));
+  if (tf)
+    for (size_t i = 0; i < 12 * nt; ++i) tf[i] = double(h[i]);
+  if (nf)
+    for (size_t i = 0; i < 3 * nn; ++i) nf[i] = double(h[12 * nt + i]);
+  if (te) {
+    std::vector<Real> en(nt);
+    HIPCHECK(hipMemcpy(en.data(), d_.energy + t0, nt * sizeof(Real), hipMemcpyDeviceToHost));
+    for (size_t e = 0; e < nt; ++e) te[e] = double(en[e]);
+  }
+  if (R) rotations_get(b, R);
+}
+
+template <class Real>
+void Engine<Real>::apply_system(uint32_t b, double inertia, double step, const double *dv,
+                                double *y) {
+  hipStream_t s = stream_;
+  const size_t n0 = n0_[b], nn = nn_[b];
+  std::vector<Real> in(3 * size_t(d_.N), Real(0));
+  for (size_t i = 0; i < 3 * nn; ++i) in[3 * n0 + i] = Real(dv[i]);
+  HIPCHECK(hipMemcpyAsync(d_.p, in.data(), in.size() * sizeof(Real), hipMemcpyHostToDevice, s));
+  Launch<Real>::op_element_plain(d_, d_.p, s);
+  Launch<Real>::apply_plain(d_, d_.p, d_.q, inertia, step * step, s);
+  std::vector<Real> out(3 * nn);
+  HIPCHECK(hipMemcpyAsync(out.data(), d_.q + 3 * n0, 3 * nn * sizeof(Real), hipMemcpyDeviceToHost, s));
+  HIPCHECK(hipStreamSynchronize(s));
+  for (size_t i = 0; i < 3 * nn; ++i) y[i] = double(out[i]);
+}
+
+template <class Real>
+double Engine<Real>::time_kernel(uint32_t kind, uint32_t repeats) {
+  hipStream_t s = stream_;
+  all_active();
+  // inputs at the current state
+  Launch<Real>::displacement(d_, d_.x, s);
+  HIPCHECK(hipMemsetAsync(d_.p, 0, 3 * size_t(d_.N) * sizeof(Real), s));
+  auto launch = [&]() {
+    if (kind == 0) Launch<Real>::assemble(d_, false, false, s);
+    else if (kind == 1) Launch<Real>::op_element(d_, d_.p, -1, s);
+    else Launch<Real>::op_gather(d_, d_.p, d_.q, -1, s);
+  };
+  launch();
+  hipEvent_t a, b;
+  HIPCHECK(hipEventCreate(&a));
+  HIPCHECK(hipEventCreate(&b));
+  HIPCHECK(hipEventRecord(a, s));
+  for (uint32_t i = 0; i < repeats; ++i) launch();
+  HIPCHECK(hipEventRecord(b, s));
+  HIPCHECK(hipEventSynchronize(b));
+  float ms = 0.0f;
+  HIPCHECK(hipEventElapsedTime(&ms, a, b));
+  HIPCHECK(hipEventDestroy(a));
+  HIPCHECK(hipEventDestroy(b));
+  return double(ms) / std::max(repeats, 1u);
+}
+
+}  // namespace tmi
+
+// ------------------------------------------------------------------ C ABI
+
+struct talus_mi_ctx {
+  int device = 0;
+  uint32_t precision = TALUS_MI_FP64;
+  hipStream_t stream = nullptr;
+  std::vector<tmi::Body> bodies;
+  std::unique_ptr<tmi::EngineBase> engine;
+  bool dirty = true;
+  uint32_t steps = 0;
+  std::vector<talus_mi_event> events;
+};
+
+namespace {
+
+using tmi::fail;
+
+void ensure(talus_mi_ctx *ctx) {
+  if (!ctx->dirty) return;
+  HIPCHECK(hipSetDevice(ctx->device));
+  ctx->engine->download(ctx->bodies);
+  ctx->engine->build(ctx->bodies);
+  ctx->dirty = false;
+}
+
+template <class F>
+int guarded(talus_mi_ctx *ctx, F &&f) {
+  if (!ctx) return fail(TALUS_MI_ERR_ARGUMENT, "null context");
+  try {
+    HIPCHECK(hipSetDevice(ctx->device));
+    return f();
+  } catch (const tmi::DeviceError &e) {
+    return fail(TALUS_MI_ERR_DEVICE, e.what());
+  } catch (const std::bad_alloc &) {
+    return fail(TALUS_MI_ERR_MEMORY, "out of host memory");
+  } catch (const std::exception &e) {
+    return fail(TALUS_MI_ERR_STATE, e.what());
+  }
+}
+
+bool body_ok(talus_mi_ctx *ctx, uint32_t body) { return body < ctx->bodies.size(); }
+
+}  // namespace
+
+extern "C" {
+
+const char *talus_mi_last_error(void) { return tmi::g_error.c_str(); }
+
+void talus_mi_default_params(talus_mi_step_params *p) {
+  if (!p) return;
+  std::memset(p, 0, sizeof *p);
+  p->dt = 1.0f / 60.0f;
+  p->gravity[1] = -9.81f;
+  p->substeps = 1;
+  p->cg_iterations = 64;
+  p->factor_budget = 32;
+  p->tolerance = 1.0e-3f;
+  p->spectral_radius = 0.5f;
+  p->preconditioner = TALUS_MI_BLOCK_JACOBI;
+  p->flags = 0;
+}
+
+int talus_mi_create(talus_mi_ctx **out, int device, uint32_t precision) {
+  if (!out) return fail(TALUS_MI_ERR_ARGUMENT, "null output pointer");
+  *out = nullptr;
+  if (precision != TALUS_MI_FP64 && precision != TALUS_MI_MIXED)
+    return fail(TALUS_MI_ERR_ARGUMENT, "unknown precision");
+  int count = 0;
+  if (hipGetDeviceCount(&count) != hipSuccess || count <= 0)
+    return fail(TALUS_MI_ERR_DEVICE, "no HIP device available");
+  if (device < 0 || device >= count) return fail(TALUS_MI_ERR_ARGUMENT, "device index out of range");
+  auto ctx = std::make_unique<talus_mi_ctx>();
+  ctx->device = device;
+  ctx->precision = precision;
+  try {
+    HIPCHECK(hipSetDevice(device));
+    HIPCHECK(hipStreamCreateWithFlags(&ctx->stream, hipStreamNonBlocking));
+    if (precision == TALUS_MI_MIXED)
+      ctx->engine = std::make_unique<tmi::Engine<float>>(ctx->stream);
+    else
+      ctx->engine = std::make_unique<tmi::Engine<double>>(ctx->stream);
+  } catch (const std::exception &e) {
+    return fail(TALUS_MI_ERR_DEVICE, e.what());
+  }
+  *out = ctx.release();
+  return TALUS_MI_OK;
+}
+
+int talus_mi_destroy(talus_mi_ctx *ctx) {
+  if (!ctx) return TALUS_MI_OK;
+  (void)hipSetDevice(ctx->device);
+  ctx->engine.reset();
+  if (ctx->stream) (void)hipStreamDestroy(ctx->stream);
+  delete ctx;
+  return TALUS_MI_OK;
+}
+
+int talus_mi_build_lattice(const talus_mi_lattice *lattice, const talus_mi_material *materials,
+                           uint32_t lanes, const talus_mi_support *supports, uint32_t nsupports,
+                           talus_mi_mesh *out) {
+  if (!lattice || !out) return fail(TALUS_MI_ERR_ARGUMENT, "null lattice or output");
+  if (nsupports && !supports) return fail(TALUS_MI_ERR_ARGUMENT, "null supports");
+  try {
+    return tmi::build_lattice(lattice, materials, lanes, supports, nsupports, out);
+  } catch (const std::bad_alloc &) {
+    return fail(TALUS_MI_ERR_MEMORY, "out of host memory");
+  }
+}
+
+void talus_mi_mesh_free(talus_mi_mesh *mesh) { tmi::mesh_free(mesh); }
+
+int talus_mi_body_add(talus_mi_ctx *ctx, const talus_mi_mesh *mesh,
+                      const talus_mi_material *materials, uint32_t lanes, uint32_t *body) {
+  if (!mesh || !mesh->rest_positions || !mesh->tet_nodes || !mesh->anchored)
+    return fail(TALUS_MI_ERR_ARGUMENT, "mesh is missing arrays");
+  if (mesh->nodes == 0 || mesh->tets == 0) return fail(TALUS_MI_ERR_ARGUMENT, "empty mesh");
+  if (lanes && !materials) return fail(TALUS_MI_ERR_ARGUMENT, "null materials");
+  return guarded(ctx, [&]() -> int {
+    tmi::Body b;
+    b.nodes = mesh->nodes;
+    b.tets = mesh->tets;
+    b.rest.resize(3 * size_t(b.nodes));
+    for (size_t i = 0; i < b.rest.size(); ++i) b.rest[i] = double(mesh->rest_positions[i]);
+    b.anchored.assign(mesh->anchored, mesh->anchored + b.nodes);
+    b.tet.assign(mesh->tet_nodes, mesh->tet_nodes + 4 * size_t(b.tets));
+    if (mesh->tet_material)
+      b.lane.assign(mesh->tet_material, mesh->tet_material + b.tets);
+    else
+      b.lane.assign(b.tets, 0u);
+    b.materials.assign(materials, materials + lanes);
+    b.density_mass = mesh->node_mass == nullptr;
+    if (!b.density_mass) {
+      b.mass.resize(b.nodes);
+      for (uint32_t n = 0; n < b.nodes; ++n) b.mass[n] = double(mesh->node_mass[n]);
+    }
+    const int rc = tmi::prepare_body(b);
+    if (rc != TALUS_MI_OK) return rc;
+    b.x = b.rest;
+    b.v.assign(3 * size_t(b.nodes), 0.0);
+    b.a.assign(3 * size_t(b.nodes), 0.0);
+    b.R.assign(9 * size_t(b.tets), 0.0);
+    for (uint32_t e = 0; e < b.tets; ++e) b.R[9 * size_t(e)] = b.R[9 * size_t(e) + 4] = b.R[9 * size_t(e) + 8] = 1.0;
+    b.plastic.assign(10 * size_t(b.tets), 0.0);
+    ctx->engine->download(ctx->bodies);
+    ctx->bodies.push_back(std::move(b));
+    ctx->dirty = true;
+    if (body) *body = uint32_t(ctx->bodies.size() - 1);
+    return TALUS_MI_OK;
+  });
+}
+
+int talus_mi_body_info(talus_mi_ctx *ctx, uint32_t body, uint32_t *nodes, uint32_t *tets) {
+  if (!ctx || !body_ok(ctx, body)) return fail(TALUS_MI_ERR_ARGUMENT, "unknown body");
+  if (nodes) *nodes = ctx->bodies[body].nodes;
+  if (tets) *tets = ctx->bodies[body].tets;
+  return TALUS_MI_OK;
+}
+
+int talus_mi_state_set(talus_mi_ctx *ctx, uint32_t body, const double *x, const double *v,
+                       const double *a) {
+  if (!ctx || !body_ok(ctx, body)) return fail(TALUS_MI_ERR_ARGUMENT, "unknown body");
+  return guarded(ctx, [&]() -> int {
+    ensure(ctx);
+    ctx->engine->state_set(body, x, v, a);
+    return TALUS_MI_OK;
+  });
+}
+
+int talus_mi_state_get(talus_mi_ctx *ctx, uint32_t body, double *x, double *v, double *a) {
+  if (!ctx || !body_ok(ctx, body)) return fail(TALUS_MI_ERR_ARGUMENT, "unknown body");
+  return guarded(ctx, [&]() -> int {
+    ensure(ctx);
+    ctx->engine->state_get(body, x, v, a);
+    return TALUS_MI_OK;
+  });
+}
+
+int talus_mi_rotations_get(talus_mi_ctx *ctx, uint32_t body, double *R) {
+  if (!ctx || !body_ok(ctx, body) || !R) return fail(TALUS_MI_ERR_ARGUMENT, "bad arguments");
+  return guarded(ctx, [&]() -> int {
+    ensure(ctx);
+    ctx->engine->rotations_get(body, R);
+    return TALUS_MI_OK;
+  });
+}
+
+int talus_mi_rotations_set(talus_mi_ctx *ctx, uint32_t body, const double *R) {
+  if (!ctx || !body_ok(ctx, body) || !R) return fail(TALUS_MI_ERR_ARGUMENT, "bad arguments");
+  return guarded(ctx, [&]() -> int {
+    ensure(ctx);
+    ctx->engine->rotations_set(body, R);
+    return TALUS_MI_OK;
+  });
+}
+
+int talus_mi_plastic_get(talus_mi_ctx *ctx, uint32_t body, double *p) {
+  if (!ctx || !body_ok(ctx, body) || !p) return fail(TALUS_MI_ERR_ARGUMENT, "bad arguments");
+  return guarded(ctx, [&]() -> int {
+    ensure(ctx);
+    ctx->engine->plastic_get(body, p);
+    return TALUS_MI_OK;
+  });
+}
+
+int talus_mi_plastic_set(talus_mi_ctx *ctx, uint32_t body, const double *p) {
+  if (!ctx || !body_ok(ctx, body) || !p) return fail(TALUS_MI_ERR_ARGUMENT, "bad arguments");
+  return guarded(ctx, [&]() -> int {
+    ensure(ctx);
+    ctx->engine->plastic_set(body, p);
+    return TALUS_MI_OK;
+  });
+}
+
+int talus_mi_topology_get(talus_mi_ctx *ctx, uint32_t body, uint32_t *tet_nodes, double *rest,
+                          double *mass, uint8_t *anchored) {
+  if (!ctx || !body_ok(ctx, body)) return fail(TALUS_MI_ERR_ARGUMENT, "unknown body");
+  const tmi::Body &b = ctx->bodies[body];
+  if (tet_nodes) std::memcpy(tet_nodes, b.tet.data(), b.tet.size() * sizeof(uint32_t));
+  if (rest) std::memcpy(rest, b.rest.data(), b.rest.size() * sizeof(double));
+  if (mass) std::memcpy(mass, b.mass.data(), b.mass.size() * sizeof(double));
+  if (anchored) std::memcpy(anchored, b.anchored.data(), b.anchored.size());
+  return TALUS_MI_OK;
+}
+
+int talus_mi_step(talus_mi_ctx *ctx, const talus_mi_step_params *params, talus_mi_report *report) {
+  if (!params || !report) return fail(TALUS_MI_ERR_ARGUMENT, "null params or report");
+  if (!(params->dt > 0.0f)) return fail(TALUS_MI_ERR_ARGUMENT, "dt must be positive");
+  if (params->preconditioner > TALUS_MI_FACTOR) return fail(TALUS_MI_ERR_ARGUMENT, "unknown preconditioner");
+  return guarded(ctx, [&]() -> int {
+    std::memset(report, 0, sizeof *report);
+    if (ctx->bodies.empty()) return TALUS_MI_OK;
+    ensure(ctx);
+    std::vector<uint8_t> torn;
+    ctx->engine->step(*params, *report, ctx->events, ctx->steps, torn);
+    ++ctx->steps;
+    return TALUS_MI_OK;
+  });
+}
+
+int talus_mi_events(talus_mi_ctx *ctx, talus_mi_event *out, uint32_t capacity, uint32_t *count) {
+  if (!ctx || !count) return fail(TALUS_MI_ERR_ARGUMENT, "null context or count");
+  const uint32_t n = uint32_t(std::min<size_t>(capacity, ctx->events.size()));
+  if (n && !out) return fail(TALUS_MI_ERR_ARGUMENT, "null output");
+  std::copy(ctx->events.begin(), ctx->events.begin() + n, out);
+  ctx->events.erase(ctx->events.begin(), ctx->events.begin() + n);
+  *count = n;
+  return TALUS_MI_OK;
+}
+
+int talus_mi_eval_forces(talus_mi_ctx *ctx, uint32_t body, const double *x, double *tet_forces,
+                         double *tet_energy, double *node_forces, double *rotations) {
+  if (!ctx || !body_ok(ctx, body) || !x) return fail(TALUS_MI_ERR_ARGUMENT, "bad arguments");
+  return guarded(ctx, [&]() -> int {
+    ensure(ctx);
+    ctx->engine->eval_forces(body, x, tet_forces, tet_energy, node_forces, rotations);
+    return TALUS_MI_OK;
+  });
+}
+
+int talus_mi_apply_system(talus_mi_ctx *ctx, uint32_t body, double inertia, double step,
+                          const double *d, double *y) {
+  if (!ctx || !body_ok(ctx, body) || !d || !y) return fail(TALUS_MI_ERR_ARGUMENT, "bad arguments");
+  return guarded(ctx, [&]() -> int {
+    ensure(ctx);
+    ctx->engine->apply_system(body, inertia, step, d, y);
+    return TALUS_MI_OK;
+  });
+}
+
+int talus_mi_time_kernel(talus_mi_ctx *ctx, uint32_t kind, uint32_t repeats, double *avg_ms) {
+  if (!avg_ms || kind > 2) return fail(TALUS_MI_ERR_ARGUMENT, "bad arguments");
+  return guarded(ctx, [&]() -> int {
+    if (ctx->bodies.empty()) return fail(TALUS_MI_ERR_STATE, "no bodies");
+    ensure(ctx);
+    *avg_ms = ctx->engine->time_kernel(kind, repeats);
+    return TALUS_MI_OK;
+  });
+}
+
+void *talus_mi_stream(talus_mi_ctx *ctx) { return ctx ? (void *)ctx->stream : nullptr; }
+
+}  // extern "C"

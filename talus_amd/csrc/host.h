@@ -1,0 +1,50 @@
+// host.h -- host-side types of libtalus_mi (not part of the C ABI).
+#pragma once
+
+#include <cstdint>
+#include <string>
+#include <vector>
+
+#include "../../include/talus_mi.h"
+
+namespace tmi {
+
+// Records the calling thread's error message and returns `code`.
+int fail(int code, const std::string &message);
+
+talus_mi_material default_material();
+const talus_mi_material &lane_material(const talus_mi_material *materials, uint32_t lanes,
+                                       uint32_t lane);
+void precise_lame(float youngs, float poisson, double *lambda, double *mu);
+bool rest_frame(const double *rest, const uint32_t *tet, double *dminv, double *volume);
+int build_lattice(const talus_mi_lattice *lattice, const talus_mi_material *materials,
+                  uint32_t lanes, const talus_mi_support *supports, uint32_t nsupports,
+                  talus_mi_mesh *out);
+void mesh_free(talus_mi_mesh *mesh);
+
+// One deformable tet body: the FP64 mesh data the device arrays are built
+// from, plus a host mirror of the state used across rebuilds.
+struct Body {
+  uint32_t nodes = 0, tets = 0;
+  std::vector<double> rest;       // 3*nodes
+  std::vector<double> mass;       // nodes
+  std::vector<uint8_t> anchored;  // nodes
+  std::vector<uint32_t> tet;      // 4*tets, body-local node indices
+  std::vector<double> dminv;      // 9*tets row-major
+  std::vector<double> volume;     // tets
+  std::vector<uint32_t> lane;     // tets
+  std::vector<talus_mi_material> materials;
+  // per-tet material data
+  std::vector<double> lambda, mu, yield, hardening, plastic_max, tear;
+  std::vector<uint8_t> flags;
+  // state mirror (valid when the device copy is not)
+  std::vector<double> x, v, a;
+  std::vector<double> R;        // 9*tets
+  std::vector<double> plastic;  // 10*tets
+  bool density_mass = true;     // masses follow rho V / 4 (recomputed on tears)
+};
+
+// Fills per-tet material data and FP64 rest frames; returns a status.
+int prepare_body(Body &body);
+
+}  // namespace tmi

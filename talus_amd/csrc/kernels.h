@@ -1,0 +1,111 @@
+// kernels.h -- device data layout and kernel launchers of libtalus_mi.
+//
+// Layout in HBM (all bodies concatenated; each body owns a contiguous node
+// range and tet range):
+//   node vectors      xyz-interleaved, FP64 master state (x, v, a, ...) and
+//                     Real working vectors (displacements, PCG vectors)
+//   tet connectivity  uint4 per tet (one 16-byte load)
+//   Dm^-1             FP64, 9 SoA planes dminv[k*T + e] (coalesced)
+//   V, lambda, mu     Real per tet
+//   rotations R       Real, 9 SoA planes
+//   force slots       Real, AoS per (tet, corner): slot_f[(4e+c)*3 + r], so a
+//                     node's gather reads 12 contiguous bytes per incidence
+//   incidence CSR     node -> slot codes 4e+c in ascending order, summed in
+//                     that fixed order (deterministic assembly, no atomics)
+// Work is cut into tiles of TILE items of one body (node tiles and tet
+// tiles); block b processes tile b, and per-body scalars (PCG alpha/beta,
+// convergence) are reduced from per-tile partial sums in a fixed order.
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include <cstdint>
+
+namespace tmi {
+
+constexpr int TILE = 256;
+constexpr int NPART = 4;  // partial sums per node tile
+
+// Per-body integrator constants for the current substep pass.
+struct StepParams {
+  double inner;    // substep h
+  double carry;    // (1/2 - beta) h^2
+  double reach;    // h_eff^2 = (1 - alpha_f) beta h^2
+  double inertia;  // 1 - alpha_m
+  double am, af, beta, gamma;
+  double g[3];
+  double tolerance;
+  uint32_t active;  // body takes part in this pass
+  uint32_t limit;   // CG iteration cap
+};
+
+template <class Real>
+struct Dev {
+  uint32_t N, T, B;
+  // tiles
+  const uint32_t *nt_body, *nt_begin, *nt_end;
+  uint32_t n_ntiles;
+  const uint32_t *tt_body, *tt_begin, *tt_end;
+  uint32_t n_ttiles;
+  const uint32_t *body_nt0, *body_nt1;
+  const StepParams *params;
+  // nodes
+  const double *X, *mass;
+  const uint8_t *anchored;
+  const uint32_t *inc_start, *inc_slot;
+  double *x, *v, *a, *carried, *shifted, *weighted, *right;
+  Real *u_eval, *P, *x0;
+  Real *dx, *r, *z, *p, *q;
+  // tets
+  const uint4 *conn;
+  const double *dminv;
+  const Real *vol, *lam, *mu;
+  const double *vol64, *lam64, *mu64;
+  const uint8_t *flags;  // bit0 neo-Hookean, bit1 plastic lane, bit2 tearable
+  const double *yield, *hard, *pmax, *tear;
+  Real *R;
+  double *plastic;  // 10 SoA planes
+  Real *slot_f, *slot_d, *energy;
+  uint8_t *yflag, *tflag;
+  // reductions
+  double *partial;   // NPART per node tile
+  double *epartial;  // 1 per tet tile
+  // PCG, per body
+  double *sq;        // [2][B]
+  int *done;         // [2][B]
+  uint32_t *iters;   // [B]
+  double *target2;   // [B]
+  double *rr_last;   // [B]
+  int *inverts;      // [B]
+  uint32_t *yields;  // [B]
+};
+
+enum : uint8_t { FLAG_SNH = 1, FLAG_PLASTIC = 2, FLAG_TEAR = 4 };
+
+template <class Real>
+struct Launch {
+  static void predict(const Dev<Real> &d, hipStream_t s);
+  static void predictor_inverts(const Dev<Real> &d, hipStream_t s);
+  static void evaluate(const Dev<Real> &d, hipStream_t s);
+  static void assemble(const Dev<Real> &d, bool diag, bool energy, hipStream_t s);
+  static void balance(const Dev<Real> &d, bool diag, bool taken, hipStream_t s);
+  static void pcg_init(const Dev<Real> &d, bool with_x0, hipStream_t s);
+  static void pcg_begin(const Dev<Real> &d, hipStream_t s);
+  static void op_element(const Dev<Real> &d, const Real *dir, int parity, hipStream_t s);
+  static void op_gather(const Dev<Real> &d, const Real *dir, Real *out, int parity, hipStream_t s);
+  static void pcg_update(const Dev<Real> &d, int parity, hipStream_t s);
+  static void pcg_direction(const Dev<Real> &d, int parity, hipStream_t s);
+  static void apply_increment(const Dev<Real> &d, bool from_x0, hipStream_t s);
+  static void finish(const Dev<Real> &d, hipStream_t s);
+  static void commit(const Dev<Real> &d, hipStream_t s);
+  static void tear(const Dev<Real> &d, hipStream_t s);
+  static void kinetic(const Dev<Real> &d, hipStream_t s);
+  // kernel-level entry points
+  static void displacement(const Dev<Real> &d, const double *xin, hipStream_t s);
+  static void apply_plain(const Dev<Real> &d, const Real *dir, Real *out, double inertia,
+                          double reach, hipStream_t s);
+  static void op_element_plain(const Dev<Real> &d, const Real *dir, hipStream_t s);
+  static void gather_forces(const Dev<Real> &d, Real *out, hipStream_t s);
+};
+
+}  // namespace tmi

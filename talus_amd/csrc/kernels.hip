@@ -1,0 +1,893 @@
+// kernels.hip -- gfx950 kernels of the deformable tet step.
+//
+// Element kernels (one lane per tet, tile-mapped) evaluate the constitutive
+// law and write per-(tet, corner) slots; node kernels (one lane per node)
+// sum their incident slots in CSR order, so assembly is deterministic and
+// atomic-free.  PCG keeps every scalar on the device: per-tile partial sums
+// are reduced per body in a fixed order by each consuming block, so no host
+// round trip is needed inside a solve.
+//
+// Reference mapping (paths relative to the Talus tree):
+//   k_predict          solver.cc:3680-3701 (predictor, alpha-point weights)
+//   k_predictor_inv    solver.cc:568-623 (predictorInverts)
+//   k_assemble         solver.cc:1354-1470 (assembleElastic) with
+//                      material.cc:978-1057 / neohookean.cc:125-196 per tet
+//   k_balance          solver.cc:3783-3806 (balance), 1574-1593 (guard)
+//   k_op_*             solver.cc:904-1048 (applySystem)
+//   k_pcg_*            solver.cc:1904-2089 (conjugateGradient)
+//   k_finish           solver.cc:4351-4368, 5829-5849
+//   k_commit           solver.cc:1260-1346 (commitPlastic) and
+//                      plasticflow.cc:152-260 (flowLattice, Hencky)
+//   k_tear             tearing.cc:469-483 (overstretched)
+#include <cfloat>
+
+#include "kernels.h"
+#include "tmath.h"
+
+namespace tmi {
+
+namespace {
+
+__device__ __forceinline__ double wave_sum(double v) {
+#pragma unroll
+  for (int off = 32; off > 0; off >>= 1) v += __shfl_down(v, off, 64);
+  return v;
+}
+
+// Fixed-order block sum; result valid in every thread.
+__device__ __forceinline__ double block_sum(double v, double *lds) {
+  v = wave_sum(v);
+  const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+  __syncthreads();
+  if (lane == 0) lds[wave] = v;
+  __syncthreads();
+  double s = 0.0;
+  const int waves = blockDim.x >> 6;
+  for (int w = 0; w < waves; ++w) s += lds[w];
+  return s;
+}
+
+// Body total of partial slot `slot` over the body's node tiles, summed in a
+// fixed order (identical in every block of the body).
+__device__ __forceinline__ double body_total(const double *partial, uint32_t t0, uint32_t t1,
+                                             int slot, double *lds) {
+  double s = 0.0;
+  for (uint32_t t = t0 + threadIdx.x; t < t1; t += blockDim.x) s += partial[t * NPART + slot];
+  return block_sum(s, lds);
+}
+
+template <class Real>
+__device__ __forceinline__ void load_dm(const Dev<Real> &d, uint32_t e, Real *dm) {
+#pragma unroll
+  for (int k = 0; k < 9; ++k) dm[k] = Real(d.dminv[(size_t)k * d.T + e]);
+}
+
+template <class Real>
+__device__ __forceinline__ void gather_vec(const Real *src, uint4 c, Real *out) {
+  const uint32_t n[4] = {c.x, c.y, c.z, c.w};
+#pragma unroll
+  for (int i = 0; i < 4; ++i) {
+    out[3 * i + 0] = src[3 * (size_t)n[i] + 0];
+    out[3 * i + 1] = src[3 * (size_t)n[i] + 1];
+    out[3 * i + 2] = src[3 * (size_t)n[i] + 2];
+  }
+}
+
+__device__ __forceinline__ void gather_vec64(const double *src, uint4 c, double *out) {
+  const uint32_t n[4] = {c.x, c.y, c.z, c.w};
+#pragma unroll
+  for (int i = 0; i < 4; ++i) {
+    out[3 * i + 0] = src[3 * (size_t)n[i] + 0];
+    out[3 * i + 1] = src[3 * (size_t)n[i] + 1];
+    out[3 * i + 2] = src[3 * (size_t)n[i] + 2];
+  }
+}
+
+// FP64 deformation gradient of tet e at FP64 positions `pos`.
+template <class Real>
+__device__ __forceinline__ void deformation64(const Dev<Real> &d, const double *pos, uint32_t e,
+                                              uint4 c, double *F) {
+  double x[12], X[12], u[12], dm[9];
+  gather_vec64(pos, c, x);
+  gather_vec64(d.X, c, X);
+#pragma unroll
+  for (int i = 0; i < 12; ++i) u[i] = x[i] - X[i];
+#pragma unroll
+  for (int k = 0; k < 9; ++k) dm[k] = d.dminv[(size_t)k * d.T + e];
+  displacement_gradient(u, dm, F);
+  F[0] += 1.0;
+  F[4] += 1.0;
+  F[8] += 1.0;
+}
+
+template <class Real>
+__device__ __forceinline__ void store_slots(Real *slot, uint32_t e, const Real *f) {
+  Real *o = slot + (size_t)e * 12;
+#pragma unroll
+  for (int i = 0; i < 12; ++i) o[i] = f[i];
+}
+
+// ---------------------------------------------------------------- nodes
+
+template <class Real>
+__global__ __launch_bounds__(TILE) void k_predict(Dev<Real> d) {
+  const uint32_t t = blockIdx.x, b = d.nt_body[t];
+  const StepParams &P = d.params[b];
+  if (!P.active) return;
+  const uint32_t n = d.nt_begin[t] + threadIdx.x;
+  if (n >= d.nt_end[t]) return;
+  const double af = P.af, inner = P.inner, carry = P.carry;
+#pragma unroll
+  for (int r = 0; r < 3; ++r) {
+    const size_t i = 3 * (size_t)n + r;
+    const double a = d.a[i], x = d.x[i];
+    const double shifted = x + inner * d.v[i] + carry * a;
+    const double weighted = (1.0 - af) * shifted + af * x;
+    d.carried[i] = a;
+    d.shifted[i] = shifted;
+    d.weighted[i] = weighted;
+    d.u_eval[i] = Real(weighted - d.X[i]);
+    d.x0[i] = Real(a);
+  }
+}
+
+template <class Real>
+__global__ __launch_bounds__(TILE) void k_evaluate(Dev<Real> d) {
+  const uint32_t t = blockIdx.x, b = d.nt_body[t];
+  const StepParams &P = d.params[b];
+  if (!P.active) return;
+  const uint32_t n = d.nt_begin[t] + threadIdx.x;
+  if (n >= d.nt_end[t]) return;
+#pragma unroll
+  for (int r = 0; r < 3; ++r) {
+    const size_t i = 3 * (size_t)n + r;
+    d.u_eval[i] = Real(d.weighted[i] + P.reach * d.a[i] - d.X[i]);
+  }
+}
+
+template <class Real, bool DIAG, bool TAKEN>
+__global__ __launch_bounds__(TILE) void k_balance(Dev<Real> d) {
+  const uint32_t t = blockIdx.x, b = d.nt_body[t];
+  const StepParams &P = d.params[b];
+  if (!P.active) return;
+  const uint32_t n = d.nt_begin[t] + threadIdx.x;
+  if (n >= d.nt_end[t]) return;
+  Real *Pn = d.P + 6 * (size_t)n;
+  if (d.anchored[n]) {
+#pragma unroll
+    for (int r = 0; r < 3; ++r) {
+      d.v[3 * (size_t)n + r] = 0.0;
+      d.a[3 * (size_t)n + r] = 0.0;
+      d.right[3 * (size_t)n + r] = 0.0;
+    }
+    if (DIAG) {
+      Pn[0] = Real(1); Pn[1] = Real(0); Pn[2] = Real(0);
+      Pn[3] = Real(1); Pn[4] = Real(0); Pn[5] = Real(1);
+    }
+    return;
+  }
+  Real f[3] = {Real(0), Real(0), Real(0)};
+  Real D[6] = {Real(0), Real(0), Real(0), Real(0), Real(0), Real(0)};
+  const uint32_t k1 = d.inc_start[n + 1];
+  for (uint32_t k = d.inc_start[n]; k < k1; ++k) {
+    const size_t s = d.inc_slot[k];
+    f[0] += d.slot_f[3 * s + 0];
+    f[1] += d.slot_f[3 * s + 1];
+    f[2] += d.slot_f[3 * s + 2];
+    if (DIAG) {
+#pragma unroll
+      for (int j = 0; j < 6; ++j) D[j] += d.slot_d[6 * s + j];
+    }
+  }
+  const double m = d.mass[n];
+#pragma unroll
+  for (int r = 0; r < 3; ++r) {
+    const size_t i = 3 * (size_t)n + r;
+    double rhs = double(f[r]) + m * P.g[r] - P.am * m * d.carried[i];
+    if (TAKEN) rhs -= P.inertia * m * d.a[i];
+    d.right[i] = rhs;
+  }
+  if (DIAG) {
+    const Real reach = Real(P.reach), im = Real(P.inertia * m);
+    Real A[9] = {reach * D[0] + im, reach * D[1], reach * D[2],
+                 reach * D[1], reach * D[3] + im, reach * D[4],
+                 reach * D[2], reach * D[4], reach * D[5] + im};
+    Real inv[9];
+    if (det3(A) > Real(0) && inv3(A, inv)) {
+      Pn[0] = inv[0]; Pn[1] = inv[1]; Pn[2] = inv[2];
+      Pn[3] = inv[4]; Pn[4] = inv[5]; Pn[5] = inv[8];
+    } else {
+      Pn[0] = Real(1); Pn[1] = Real(0); Pn[2] = Real(0);
+      Pn[3] = Real(1); Pn[4] = Real(0); Pn[5] = Real(1);
+    }
+  }
+}
+
+template <class Real>
+__device__ __forceinline__ void apply_P(const Real *Pn, const Real *r, Real *z) {
+  z[0] = Pn[0] * r[0] + Pn[1] * r[1] + Pn[2] * r[2];
+  z[1] = Pn[1] * r[0] + Pn[3] * r[1] + Pn[4] * r[2];
+  z[2] = Pn[2] * r[0] + Pn[4] * r[1] + Pn[5] * r[2];
+}
+
+// r = right - A x0 (A x0 from the op slots when WITH_X0), z = P r, p = z
+template <class Real, bool WITH_X0>
+__global__ __launch_bounds__(TILE) void k_pcg_init(Dev<Real> d) {
+  __shared__ double lds[TILE / 64];
+  const uint32_t t = blockIdx.x, b = d.nt_body[t];
+  const StepParams &P = d.params[b];
+  if (!P.active) return;
+  const uint32_t n = d.nt_begin[t] + threadIdx.x;
+  double rz = 0.0, rr = 0.0, bb = 0.0;
+  if (n < d.nt_end[t]) {
+    Real r[3] = {Real(0), Real(0), Real(0)}, z[3] = {Real(0), Real(0), Real(0)};
+    if (!d.anchored[n]) {
+      Real q[3] = {Real(0), Real(0), Real(0)};
+      if (WITH_X0) {
+        const uint32_t k1 = d.inc_start[n + 1];
+        for (uint32_t k = d.inc_start[n]; k < k1; ++k) {
+          const size_t s = d.inc_slot[k];
+          q[0] += d.slot_f[3 * s + 0];
+          q[1] += d.slot_f[3 * s + 1];
+          q[2] += d.slot_f[3 * s + 2];
+        }
+        const Real im = Real(P.inertia * d.mass[n]), reach = Real(P.reach);
+#pragma unroll
+        for (int c = 0; c < 3; ++c) q[c] = im * d.x0[3 * (size_t)n + c] + reach * q[c];
+      }
+#pragma unroll
+      for (int c = 0; c < 3; ++c) {
+        const double rhs = d.right[3 * (size_t)n + c];
+        r[c] = Real(rhs - double(q[c]));
+        bb += rhs * rhs;
+      }
+      apply_P(d.P + 6 * (size_t)n, r, z);
+    }
+#pragma unroll
+    for (int c = 0; c < 3; ++c) {
+      const size_t i = 3 * (size_t)n + c;
+      d.r[i] = r[c];
+      d.z[i] = z[c];
+      d.p[i] = z[c];
+      d.dx[i] = Real(0);
+      rz += double(r[c]) * double(z[c]);
+      rr += double(r[c]) * double(r[c]);
+    }
+  }
+  rz = block_sum(rz, lds);
+  rr = block_sum(rr, lds);
+  bb = block_sum(bb, lds);
+  if (threadIdx.x == 0) {
+    d.partial[t * NPART + 0] = rz;
+    d.partial[t * NPART + 1] = rr;
+    d.partial[t * NPART + 2] = bb;
+  }
+}
+
+// one block per body: starting scalars of the solve
+template <class Real>
+__global__ __launch_bounds__(TILE) void k_pcg_begin(Dev<Real> d) {
+  __shared__ double lds[TILE / 64];
+  const uint32_t b = blockIdx.x;
+  const StepParams &P = d.params[b];
+  const uint32_t t0 = d.body_nt0[b], t1 = d.body_nt1[b];
+  const double rz = body_total(d.partial, t0, t1, 0, lds);
+  const double rr = body_total(d.partial, t0, t1, 1, lds);
+  const double bb = body_total(d.partial, t0, t1, 2, lds);
+  if (threadIdx.x == 0) {
+    double reference = sqrt(bb);
+    if (!(reference > double(FLT_MIN))) reference = double(FLT_MIN);
+    const double target = P.tolerance * reference;
+    d.target2[b] = target * target;
+    d.sq[b] = rz;
+    d.done[b] = (!P.active || sqrt(rr) <= target) ? 1 : 0;
+    d.iters[b] = 0;
+    d.rr_last[b] = rr;
+  }
+}
+
+// ---------------------------------------------------------------- tets
+
+// Per-tet operator y_i = V R sigma(R^T d) g_i, sigma(D) = lambda tr(D) I +
+// mu (D + D^T); equals sum_j R K_ij R^T d_j (applyStiffness).
+template <class Real>
+__device__ __forceinline__ void op_tet(const Dev<Real> &d, uint32_t e, const Real *dir, Real *y) {
+  const uint4 c = d.conn[e];
+  Real dv[12], dm[9], g[12], Rm[9];
+  gather_vec(dir, c, dv);
+  load_dm(d, e, dm);
+  shape_gradients(dm, g);
+#pragma unroll
+  for (int k = 0; k < 9; ++k) Rm[k] = d.R[(size_t)k * d.T + e];
+  Real D[9] = {Real(0), Real(0), Real(0), Real(0), Real(0), Real(0), Real(0), Real(0), Real(0)};
+#pragma unroll
+  for (int j = 0; j < 4; ++j) {
+    Real dp[3];
+    mtv3(Rm, &dv[3 * j], dp);
+#pragma unroll
+    for (int r = 0; r < 3; ++r)
+#pragma unroll
+      for (int k = 0; k < 3; ++k) D[r * 3 + k] += dp[r] * g[3 * j + k];
+  }
+  const Real lam = d.lam[e], mu = d.mu[e], V = d.vol[e];
+  const Real tr = D[0] + D[4] + D[8];
+  Real S[9];
+#pragma unroll
+  for (int r = 0; r < 3; ++r)
+#pragma unroll
+    for (int k = 0; k < 3; ++k)
+      S[r * 3 + k] = V * (mu * (D[r * 3 + k] + D[k * 3 + r]) + (r == k ? lam * tr : Real(0)));
+#pragma unroll
+  for (int i = 0; i < 4; ++i) {
+    Real tmp[3];
+    mv3(S, &g[3 * i], tmp);
+    mv3(Rm, tmp, &y[3 * i]);
+  }
+}
+
+template <class Real>
+__global__ __launch_bounds__(TILE) void k_op_element(Dev<Real> d, const Real *dir, int parity) {
+  const uint32_t t = blockIdx.x, b = d.tt_body[t];
+  if (!d.params[b].active || (parity >= 0 && d.done[parity * d.B + b])) return;
+  const uint32_t e = d.tt_begin[t] + threadIdx.x;
+  if (e >= d.tt_end[t]) return;
+  Real y[12];
+  op_tet(d, e, dir, y);
+  store_slots(d.slot_f, e, y);
+}
+
+template <class Real>
+__global__ __launch_bounds__(TILE) void k_op_element_flat(Dev<Real> d, const Real *dir) {
+  const uint32_t e = blockIdx.x * TILE + threadIdx.x;
+  if (e >= d.T) return;
+  Real y[12];
+  op_tet(d, e, dir, y);
+  store_slots(d.slot_f, e, y);
+}
+
+// q = inertia M p + h^2 sum(slots), anchored rows zeroed; partial p.q
+template <class Real>
+__global__ __launch_bounds__(TILE) void k_op_gather(Dev<Real> d, const Real *dir, Real *out,
+                                                    int parity) {
+  __shared__ double lds[TILE / 64];
+  const uint32_t t = blockIdx.x, b = d.nt_body[t];
+  const StepParams &P = d.params[b];
+  if (!P.active || (parity >= 0 && d.done[parity * d.B + b])) return;
+  const uint32_t n = d.nt_begin[t] + threadIdx.x;
+  double pq = 0.0;
+  if (n < d.nt_end[t]) {
+    Real q[3] = {Real(0), Real(0), Real(0)};
+    if (!d.anchored[n]) {
+      const uint32_t k1 = d.inc_start[n + 1];
+      for (uint32_t k = d.inc_start[n]; k < k1; ++k) {
+        const size_t s = d.inc_slot[k];
+        q[0] += d.slot_f[3 * s + 0];
+        q[1] += d.slot_f[3 * s + 1];
+        q[2] += d.slot_f[3 * s + 2];
+      }
+      const Real im = Real(P.inertia * d.mass[n]), reach = Real(P.reach);
+#pragma unroll
+      for (int c = 0; c < 3; ++c) q[c] = im * dir[3 * (size_t)n + c] + reach * q[c];
+    }
+#pragma unroll
+    for (int c = 0; c < 3; ++c) {
+      out[3 * (size_t)n + c] = q[c];
+      pq += double(dir[3 * (size_t)n + c]) * double(q[c]);
+    }
+  }
+  pq = block_sum(pq, lds);
+  if (threadIdx.x == 0) d.partial[t * NPART + 0] = pq;
+}
+
+template <class Real>
+__global__ __launch_bounds__(TILE) void k_pcg_update(Dev<Real> d, int parity) {
+  __shared__ double lds[TILE / 64];
+  const uint32_t t = blockIdx.x, b = d.nt_body[t];
+  if (!d.params[b].active || d.done[parity * d.B + b]) return;
+  const double curvature = body_total(d.partial, d.body_nt0[b], d.body_nt1[b], 0, lds);
+  const double squared = d.sq[parity * d.B + b];
+  if (!(curvature > 0.0) || !(squared > 0.0)) return;
+  const Real stride = Real(squared / curvature);
+  const uint32_t n = d.nt_begin[t] + threadIdx.x;
+  double rz = 0.0, rr = 0.0;
+  if (n < d.nt_end[t]) {
+    Real r[3], z[3];
+#pragma unroll
+    for (int c = 0; c < 3; ++c) {
+      const size_t i = 3 * (size_t)n + c;
+      d.dx[i] += stride * d.p[i];
+      r[c] = d.r[i] - stride * d.q[i];
+      d.r[i] = r[c];
+    }
+    if (d.anchored[n]) {
+      z[0] = z[1] = z[2] = Real(0);
+    } else {
+      apply_P(d.P + 6 * (size_t)n, r, z);
+    }
+#pragma unroll
+    for (int c = 0; c < 3; ++c) {
+      d.z[3 * (size_t)n + c] = z[c];
+      rz += double(r[c]) * double(z[c]);
+      rr += double(r[c]) * double(r[c]);
+    }
+  }
+  rz = block_sum(rz, lds);
+  rr = block_sum(rr, lds);
+  if (threadIdx.x == 0) {
+    d.partial[t * NPART + 1] = rz;
+    d.partial[t * NPART + 2] = rr;
+  }
+}
+
+template <class Real>
+__global__ __launch_bounds__(TILE) void k_pcg_direction(Dev<Real> d, int parity) {
+  __shared__ double lds[TILE / 64];
+  const uint32_t t = blockIdx.x, b = d.nt_body[t];
+  if (!d.params[b].active) return;
+  const bool first = t == d.body_nt0[b];
+  const int next_parity = parity ^ 1;
+  if (d.done[parity * d.B + b]) {
+    if (first && threadIdx.x == 0) d.done[next_parity * d.B + b] = 1;
+    return;
+  }
+  const uint32_t t0 = d.body_nt0[b], t1 = d.body_nt1[b];
+  const double curvature = body_total(d.partial, t0, t1, 0, lds);
+  const double squared = d.sq[parity * d.B + b];
+  if (!(curvature > 0.0) || !(squared > 0.0)) {
+    if (first && threadIdx.x == 0) d.done[next_parity * d.B + b] = 1;
+    return;
+  }
+  const double rz = body_total(d.partial, t0, t1, 1, lds);
+  const double rr = body_total(d.partial, t0, t1, 2, lds);
+  const Real blend = Real(rz / squared);
+  const uint32_t n = d.nt_begin[t] + threadIdx.x;
+  if (n < d.nt_end[t]) {
+#pragma unroll
+    for (int c = 0; c < 3; ++c) {
+      const size_t i = 3 * (size_t)n + c;
+      d.p[i] = d.z[i] + blend * d.p[i];
+    }
+  }
+  if (first && threadIdx.x == 0) {
+    d.sq[next_parity * d.B + b] = rz;
+    d.iters[b] += 1;
+    d.rr_last[b] = rr;
+    d.done[next_parity * d.B + b] = rr <= d.target2[b] ? 1 : 0;
+  }
+}
+
+template <class Real>
+__global__ __launch_bounds__(TILE) void k_apply_increment(Dev<Real> d) {
+  const uint32_t t = blockIdx.x, b = d.nt_body[t];
+  if (!d.params[b].active) return;
+  const uint32_t n = d.nt_begin[t] + threadIdx.x;
+  if (n >= d.nt_end[t] || d.anchored[n]) return;
+#pragma unroll
+  for (int c = 0; c < 3; ++c) d.a[3 * (size_t)n + c] += double(d.dx[3 * (size_t)n + c]);
+}
+
+template <class Real>
+__global__ __launch_bounds__(TILE) void k_finish(Dev<Real> d) {
+  const uint32_t t = blockIdx.x, b = d.nt_body[t];
+  const StepParams &P = d.params[b];
+  if (!P.active) return;
+  const uint32_t n = d.nt_begin[t] + threadIdx.x;
+  if (n >= d.nt_end[t]) return;
+#pragma unroll
+  for (int c = 0; c < 3; ++c) {
+    const size_t i = 3 * (size_t)n + c;
+    if (d.anchored[n]) {
+      d.v[i] = 0.0;
+      d.a[i] = 0.0;
+      continue;
+    }
+    const double a = d.a[i];
+    d.v[i] += P.inner * ((1.0 - P.gamma) * d.carried[i] + P.gamma * a);
+    d.x[i] = d.shifted[i] + P.beta * P.inner * P.inner * a;
+  }
+}
+
+template <class Real>
+__global__ __launch_bounds__(TILE) void k_kinetic(Dev<Real> d) {
+  __shared__ double lds[TILE / 64];
+  const uint32_t t = blockIdx.x;
+  const uint32_t n = d.nt_begin[t] + threadIdx.x;
+  double k = 0.0, bad = 0.0;
+  if (n < d.nt_end[t]) {
+    double vv = 0.0;
+#pragma unroll
+    for (int c = 0; c < 3; ++c) {
+      const double v = d.v[3 * (size_t)n + c];
+      vv += v * v;
+      if (!isfinite(v) || !isfinite(d.x[3 * (size_t)n + c])) bad = 1.0;
+    }
+    k = 0.5 * d.mass[n] * vv;
+  }
+  k = block_sum(k, lds);
+  bad = block_sum(bad, lds);
+  if (threadIdx.x == 0) {
+    d.partial[t * NPART + 3] = k;
+    d.partial[t * NPART + 1] = bad;
+  }
+}
+
+// ------------------------------------------------------- element assembly
+
+template <class Real, bool DIAG, bool ENERGY>
+__global__ __launch_bounds__(TILE) void k_assemble(Dev<Real> d) {
+  __shared__ double lds[TILE / 64];
+  const uint32_t t = blockIdx.x, b = d.tt_body[t];
+  if (!d.params[b].active) return;
+  const uint32_t e = d.tt_begin[t] + threadIdx.x;
+  double psi_total = 0.0;
+  if (e < d.tt_end[t]) {
+    const uint4 c = d.conn[e];
+    Real u[12], dm[9], g[12], H[9];
+    gather_vec(d.u_eval, c, u);
+    load_dm(d, e, dm);
+    shape_gradients(dm, g);
+    displacement_gradient(u, dm, H);
+    const uint8_t flags = d.flags[e];
+    Real F[9];
+#pragma unroll
+    for (int i = 0; i < 9; ++i) F[i] = H[i] + ((i % 4 == 0) ? Real(1) : Real(0));
+    // det F = 1 + volumeChange(H); FP64 near inversion
+    Real J = Real(1) + volume_change(H);
+    bool upright = J > Real(0);
+    if (sizeof(Real) < 8 && J < Real(0.125)) {
+      double u64[12], dm64[9], H64[9];
+#pragma unroll
+      for (int i = 0; i < 12; ++i) u64[i] = double(u[i]);
+#pragma unroll
+      for (int k = 0; k < 9; ++k) dm64[k] = d.dminv[(size_t)k * d.T + e];
+      displacement_gradient(u64, dm64, H64);
+      upright = 1.0 + volume_change(H64) > 0.0;
+    }
+    Real Rm[9];
+    if (upright) {
+      polar(F, Rm);
+    } else {
+      double F64[9], R64[9];
+#pragma unroll
+      for (int i = 0; i < 9; ++i) F64[i] = double(F[i]);
+      svd_rotation(F64, R64);
+#pragma unroll
+      for (int i = 0; i < 9; ++i) Rm[i] = Real(R64[i]);
+    }
+    const Real V = d.vol[e], lam = d.lam[e], mu = d.mu[e];
+    Real f[12];
+    Real psi;
+    if (flags & FLAG_SNH) {
+      Real He[9], PF[9];
+      if (flags & FLAG_PLASTIC) {
+        Real ep[9], HE[9];
+#pragma unroll
+        for (int k = 0; k < 9; ++k) ep[k] = Real(d.plastic[(size_t)k * d.T + e]);
+        mul3(H, ep, HE);
+#pragma unroll
+        for (int k = 0; k < 9; ++k) He[k] = H[k] + ep[k] + HE[k];
+        Real Pk[9];
+        psi = snh_stress(He, lam, mu, Pk);
+        ep[0] += Real(1);
+        ep[4] += Real(1);
+        ep[8] += Real(1);
+        mul3_nt(Pk, ep, PF);
+      } else {
+        psi = snh_stress(H, lam, mu, PF);
+      }
+#pragma unroll
+      for (int i = 0; i < 4; ++i) {
+        Real tmp[3];
+        mv3(PF, &g[3 * i], tmp);
+        f[3 * i + 0] = -V * tmp[0];
+        f[3 * i + 1] = -V * tmp[1];
+        f[3 * i + 2] = -V * tmp[2];
+      }
+      psi *= V;
+    } else {
+      // corotated: D' = R^T F - I = (R^T - I) + R^T H, eps = sym(D')
+      Real RH[9], Dp[9], eps[9], S[9];
+      mul3_tn(Rm, H, RH);
+#pragma unroll
+      for (int r = 0; r < 3; ++r)
+#pragma unroll
+        for (int k = 0; k < 3; ++k)
+          Dp[r * 3 + k] = (Rm[k * 3 + r] - (r == k ? Real(1) : Real(0))) + RH[r * 3 + k];
+#pragma unroll
+      for (int r = 0; r < 3; ++r)
+#pragma unroll
+        for (int k = 0; k < 3; ++k) eps[r * 3 + k] = Real(0.5) * (Dp[r * 3 + k] + Dp[k * 3 + r]);
+      double pl[10];
+#pragma unroll
+      for (int k = 0; k < 10; ++k) pl[k] = d.plastic[(size_t)k * d.T + e];
+      if (flags & FLAG_PLASTIC) {
+        j2_return(eps, pl, lam, mu, Real(d.yield[e]), Real(d.hard[e]), S, (double *)nullptr);
+      } else {
+        const Real tr = eps[0] + eps[4] + eps[8];
+#pragma unroll
+        for (int k = 0; k < 9; ++k) S[k] = Real(2) * mu * eps[k] + ((k % 4 == 0) ? lam * tr : Real(0));
+      }
+      Real el = Real(0);
+#pragma unroll
+      for (int k = 0; k < 9; ++k) el += S[k] * (eps[k] - Real(pl[k]));
+      psi = Real(0.5) * V * el;
+#pragma unroll
+      for (int i = 0; i < 4; ++i) {
+        Real tmp[3], w[3];
+        mv3(S, &g[3 * i], tmp);
+        mv3(Rm, tmp, w);
+        f[3 * i + 0] = -V * w[0];
+        f[3 * i + 1] = -V * w[1];
+        f[3 * i + 2] = -V * w[2];
+      }
+    }
+    store_slots(d.slot_f, e, f);
+#pragma unroll
+    for (int k = 0; k < 9; ++k) d.R[(size_t)k * d.T + e] = Rm[k];
+    if (DIAG) {
+      // stiffnessDiagonal: R K_ii R^T = V((lambda+mu)(Rg)(Rg)^T + mu |g|^2 I)
+      const Real lm = V * (lam + mu), vm = V * mu;
+      Real *o = d.slot_d + (size_t)e * 24;
+#pragma unroll
+      for (int i = 0; i < 4; ++i) {
+        Real rg[3];
+        mv3(Rm, &g[3 * i], rg);
+        const Real gg = g[3 * i] * g[3 * i] + g[3 * i + 1] * g[3 * i + 1] + g[3 * i + 2] * g[3 * i + 2];
+        o[6 * i + 0] = lm * rg[0] * rg[0] + vm * gg;
+        o[6 * i + 1] = lm * rg[0] * rg[1];
+        o[6 * i + 2] = lm * rg[0] * rg[2];
+        o[6 * i + 3] = lm * rg[1] * rg[1] + vm * gg;
+        o[6 * i + 4] = lm * rg[1] * rg[2];
+        o[6 * i + 5] = lm * rg[2] * rg[2] + vm * gg;
+      }
+    }
+    if (ENERGY) {
+      d.energy[e] = psi;
+      psi_total = double(psi);
+    }
+  }
+  if (ENERGY) {
+    psi_total = block_sum(psi_total, lds);
+    if (threadIdx.x == 0) d.epartial[t] = psi_total;
+  }
+}
+
+// predictorInverts: det F <= 0 at the alpha-point predictor (FP64)
+template <class Real>
+__global__ __launch_bounds__(TILE) void k_predictor_inverts(Dev<Real> d) {
+  const uint32_t t = blockIdx.x, b = d.tt_body[t];
+  const StepParams &P = d.params[b];
+  if (!P.active) return;
+  const uint32_t e = d.tt_begin[t] + threadIdx.x;
+  if (e >= d.tt_end[t]) return;
+  const uint4 c = d.conn[e];
+  const uint32_t nn[4] = {c.x, c.y, c.z, c.w};
+  double u[12], dm[9], H[9];
+#pragma unroll
+  for (int i = 0; i < 4; ++i)
+#pragma unroll
+    for (int r = 0; r < 3; ++r) {
+      const size_t k = 3 * (size_t)nn[i] + r;
+      const double x = d.x[k];
+      const double shifted = x + P.inner * d.v[k] + P.carry * d.a[k];
+      u[3 * i + r] = (1.0 - P.af) * shifted + P.af * x - d.X[k];
+    }
+#pragma unroll
+  for (int k = 0; k < 9; ++k) dm[k] = d.dminv[(size_t)k * d.T + e];
+  displacement_gradient(u, dm, H);
+  if (!(1.0 + volume_change(H) > 0.0)) atomicOr(&d.inverts[b], 1);
+}
+
+// commitPlastic / flowLattice at final positions, FP64
+template <class Real>
+__global__ __launch_bounds__(TILE) void k_commit(Dev<Real> d) {
+  const uint32_t t = blockIdx.x, b = d.tt_body[t];
+  if (!d.params[b].active) return;
+  const uint32_t e = d.tt_begin[t] + threadIdx.x;
+  if (e >= d.tt_end[t]) return;
+  const uint8_t flags = d.flags[e];
+  if (!(flags & FLAG_PLASTIC)) return;
+  const uint4 c = d.conn[e];
+  double F[9];
+  deformation64(d, d.x, e, c, F);
+  double pl[10];
+#pragma unroll
+  for (int k = 0; k < 10; ++k) pl[k] = d.plastic[(size_t)k * d.T + e];
+  bool grew = false;
+  if (flags & FLAG_SNH) {
+    grew = hencky_flow(F, d.mu64[e], d.yield[e], d.hard[e], d.pmax[e], pl, &pl[9]);
+  } else {
+    double Rm[9];
+    if (det3(F) > 0.0)
+      polar(F, Rm);
+    else
+      svd_rotation(F, Rm);
+    double Dp[9], eps[9], S[9], next[10];
+    mul3_tn(Rm, F, Dp);
+    Dp[0] -= 1.0;
+    Dp[4] -= 1.0;
+    Dp[8] -= 1.0;
+    for (int r = 0; r < 3; ++r)
+      for (int k = 0; k < 3; ++k) eps[r * 3 + k] = 0.5 * (Dp[r * 3 + k] + Dp[k * 3 + r]);
+    j2_return(eps, pl, d.lam64[e], d.mu64[e], d.yield[e], d.hard[e], S, next);
+    grew = next[9] > pl[9];
+    for (int k = 0; k < 10; ++k) pl[k] = next[k];
+  }
+#pragma unroll
+  for (int k = 0; k < 10; ++k) d.plastic[(size_t)k * d.T + e] = pl[k];
+  if (grew) {
+    d.yflag[e] = 1;  // cleared at the start of every step
+    atomicAdd(&d.yields[b], 1u);
+  }
+}
+
+// overstretched: sqrt(lambda_max(F^T F)) > 1 + tearStrain, FP64
+template <class Real>
+__global__ __launch_bounds__(TILE) void k_tear(Dev<Real> d) {
+  const uint32_t t = blockIdx.x, b = d.tt_body[t];
+  if (!d.params[b].active) return;
+  const uint32_t e = d.tt_begin[t] + threadIdx.x;
+  if (e >= d.tt_end[t]) return;
+  uint8_t torn = 0;
+  if (d.flags[e] & FLAG_TEAR) {
+    double F[9];
+    deformation64(d, d.x, e, d.conn[e], F);
+    torn = stretch_of(F, nullptr) > d.tear[e] ? 1 : 0;
+  }
+  d.tflag[e] = torn;
+}
+
+// ------------------------------------------------ kernel-level entry points
+
+template <class Real>
+__global__ __launch_bounds__(TILE) void k_displacement(Dev<Real> d, const double *xin) {
+  const uint32_t n = blockIdx.x * TILE + threadIdx.x;
+  if (n >= d.N) return;
+#pragma unroll
+  for (int c = 0; c < 3; ++c) {
+    const size_t i = 3 * (size_t)n + c;
+    d.u_eval[i] = Real(xin[i] - d.X[i]);
+  }
+}
+
+template <class Real>
+__global__ __launch_bounds__(TILE) void k_gather_forces(Dev<Real> d, Real *out) {
+  const uint32_t n = blockIdx.x * TILE + threadIdx.x;
+  if (n >= d.N) return;
+  Real f[3] = {Real(0), Real(0), Real(0)};
+  const uint32_t k1 = d.inc_start[n + 1];
+  for (uint32_t k = d.inc_start[n]; k < k1; ++k) {
+    const size_t s = d.inc_slot[k];
+    f[0] += d.slot_f[3 * s + 0];
+    f[1] += d.slot_f[3 * s + 1];
+    f[2] += d.slot_f[3 * s + 2];
+  }
+#pragma unroll
+  for (int c = 0; c < 3; ++c) out[3 * (size_t)n + c] = f[c];
+}
+
+template <class Real>
+__global__ __launch_bounds__(TILE) void k_apply_plain(Dev<Real> d, const Real *dir, Real *out,
+                                                      double inertia, double reach) {
+  const uint32_t n = blockIdx.x * TILE + threadIdx.x;
+  if (n >= d.N) return;
+  Real q[3] = {Real(0), Real(0), Real(0)};
+  if (!d.anchored[n]) {
+    const uint32_t k1 = d.inc_start[n + 1];
+    for (uint32_t k = d.inc_start[n]; k < k1; ++k) {
+      const size_t s = d.inc_slot[k];
+      q[0] += d.slot_f[3 * s + 0];
+      q[1] += d.slot_f[3 * s + 1];
+      q[2] += d.slot_f[3 * s + 2];
+    }
+    const Real im = Real(inertia * d.mass[n]), rc = Real(reach);
+#pragma unroll
+    for (int c = 0; c < 3; ++c) q[c] = im * dir[3 * (size_t)n + c] + rc * q[c];
+  }
+#pragma unroll
+  for (int c = 0; c < 3; ++c) out[3 * (size_t)n + c] = q[c];
+}
+
+inline unsigned blocks_for(uint32_t n) { return (n + TILE - 1) / TILE; }
+
+}  // namespace
+
+// ---------------------------------------------------------------- launchers
+
+#define TMI_LAUNCH(kernel, grid, s, ...) \
+  hipLaunchKernelGGL(kernel, dim3(grid), dim3(TILE), 0, s, __VA_ARGS__)
+
+template <class Real>
+void Launch<Real>::predict(const Dev<Real> &d, hipStream_t s) {
+  TMI_LAUNCH(k_predict<Real>, d.n_ntiles, s, d);
+}
+template <class Real>
+void Launch<Real>::predictor_inverts(const Dev<Real> &d, hipStream_t s) {
+  TMI_LAUNCH(k_predictor_inverts<Real>, d.n_ttiles, s, d);
+}
+template <class Real>
+void Launch<Real>::evaluate(const Dev<Real> &d, hipStream_t s) {
+  TMI_LAUNCH(k_evaluate<Real>, d.n_ntiles, s, d);
+}
+template <class Real>
+void Launch<Real>::assemble(const Dev<Real> &d, bool diag, bool energy, hipStream_t s) {
+  if (diag && energy) TMI_LAUNCH((k_assemble<Real, true, true>), d.n_ttiles, s, d);
+  else if (diag) TMI_LAUNCH((k_assemble<Real, true, false>), d.n_ttiles, s, d);
+  else if (energy) TMI_LAUNCH((k_assemble<Real, false, true>), d.n_ttiles, s, d);
+  else TMI_LAUNCH((k_assemble<Real, false, false>), d.n_ttiles, s, d);
+}
+template <class Real>
+void Launch<Real>::balance(const Dev<Real> &d, bool diag, bool taken, hipStream_t s) {
+  if (diag && taken) TMI_LAUNCH((k_balance<Real, true, true>), d.n_ntiles, s, d);
+  else if (diag) TMI_LAUNCH((k_balance<Real, true, false>), d.n_ntiles, s, d);
+  else if (taken) TMI_LAUNCH((k_balance<Real, false, true>), d.n_ntiles, s, d);
+  else TMI_LAUNCH((k_balance<Real, false, false>), d.n_ntiles, s, d);
+}
+template <class Real>
+void Launch<Real>::pcg_init(const Dev<Real> &d, bool with_x0, hipStream_t s) {
+  if (with_x0) TMI_LAUNCH((k_pcg_init<Real, true>), d.n_ntiles, s, d);
+  else TMI_LAUNCH((k_pcg_init<Real, false>), d.n_ntiles, s, d);
+}
+template <class Real>
+void Launch<Real>::pcg_begin(const Dev<Real> &d, hipStream_t s) {
+  TMI_LAUNCH(k_pcg_begin<Real>, d.B, s, d);
+}
+template <class Real>
+void Launch<Real>::op_element(const Dev<Real> &d, const Real *dir, int parity, hipStream_t s) {
+  TMI_LAUNCH(k_op_element<Real>, d.n_ttiles, s, d, dir, parity);
+}
+template <class Real>
+void Launch<Real>::op_gather(const Dev<Real> &d, const Real *dir, Real *out, int parity,
+                             hipStream_t s) {
+  TMI_LAUNCH(k_op_gather<Real>, d.n_ntiles, s, d, dir, out, parity);
+}
+template <class Real>
+void Launch<Real>::pcg_update(const Dev<Real> &d, int parity, hipStream_t s) {
+  TMI_LAUNCH(k_pcg_update<Real>, d.n_ntiles, s, d, parity);
+}
+template <class Real>
+void Launch<Real>::pcg_direction(const Dev<Real> &d, int parity, hipStream_t s) {
+  TMI_LAUNCH(k_pcg_direction<Real>, d.n_ntiles, s, d, parity);
+}
+template <class Real>
+void Launch<Real>::apply_increment(const Dev<Real> &d, bool, hipStream_t s) {
+  TMI_LAUNCH(k_apply_increment<Real>, d.n_ntiles, s, d);
+}
+template <class Real>
+void Launch<Real>::finish(const Dev<Real> &d, hipStream_t s) {
+  TMI_LAUNCH(k_finish<Real>, d.n_ntiles, s, d);
+}
+template <class Real>
+void Launch<Real>::commit(const Dev<Real> &d, hipStream_t s) {
+  TMI_LAUNCH(k_commit<Real>, d.n_ttiles, s, d);
+}
+template <class Real>
+void Launch<Real>::tear(const Dev<Real> &d, hipStream_t s) {
+  TMI_LAUNCH(k_tear<Real>, d.n_ttiles, s, d);
+}
+template <class Real>
+void Launch<Real>::kinetic(const Dev<Real> &d, hipStream_t s) {
+  TMI_LAUNCH(k_kinetic<Real>, d.n_ntiles, s, d);
+}
+template <class Real>
+void Launch<Real>::displacement(const Dev<Real> &d, const double *xin, hipStream_t s) {
+  TMI_LAUNCH(k_displacement<Real>, blocks_for(d.N), s, d, xin);
+}
+template <class Real>
+void Launch<Real>::apply_plain(const Dev<Real> &d, const Real *dir, Real *out, double inertia,
+                               double reach, hipStream_t s) {
+  TMI_LAUNCH(k_apply_plain<Real>, blocks_for(d.N), s, d, dir, out, inertia, reach);
+}
+template <class Real>
+void Launch<Real>::op_element_plain(const Dev<Real> &d, const Real *dir, hipStream_t s) {
+  TMI_LAUNCH(k_op_element_flat<Real>, blocks_for(d.T), s, d, dir);
+}
+template <class Real>
+void Launch<Real>::gather_forces(const Dev<Real> &d, Real *out, hipStream_t s) {
+  TMI_LAUNCH(k_gather_forces<Real>, blocks_for(d.N), s, d, out);
+}
+
+template struct Launch<float>;
+template struct Launch<double>;
+
+}  // namespace tmi
