@@ -1,0 +1,100 @@
+"""CPU checks of libtalus_mi.so: it loads, exports every entry point that
+include/talus_mi.h declares, and its host mesh builder (buildMesh,
+src/talus/deformable/solver/solver.cc:2185-2418) equals the oracle's bit for
+bit on the benchmark lattices.  No device call is made."""
+import ctypes
+import os
+import re
+
+import numpy as np
+import pytest
+
+from talus_amd import api, scenes
+
+from helpers import oracle_mesh
+
+ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+
+
+def declared_entry_points():
+    text = open(os.path.join(ROOT, "include", "talus_mi.h")).read()
+    return sorted(set(re.findall(r"\b(talus_mi_[a-z_]+)\s*\(", text)))
+
+
+def test_library_exports_every_declared_entry_point():
+    lib = ctypes.CDLL(api.LIB_PATH)
+    names = declared_entry_points()
+    assert len(names) >= 20
+    missing = [n for n in names if not hasattr(lib, n)]
+    assert not missing, missing
+    assert set(names) == set(api._SIGNATURES), set(names) ^ set(api._SIGNATURES)
+
+
+def test_default_params_match_reference_settings():
+    p = api.default_params()
+    # deformable::Settings defaults (solver.h:51-65), IntegrationParameters dt
+    assert p.substeps == 1 and p.cg_iterations == 64 and p.factor_budget == 32
+    assert p.tolerance == pytest.approx(1e-3) and p.spectral_radius == pytest.approx(0.5)
+    assert p.dt == pytest.approx(1 / 60) and tuple(p.gravity) == pytest.approx((0, -9.81, 0))
+
+
+@pytest.mark.parametrize("make", [
+    scenes.cantilever,
+    lambda: scenes.cube(7, seed=3),
+    lambda: scenes.beam((12, 3, 3), seed=2),
+    lambda: scenes.slab((10, 5, 2)),
+])
+def test_mesh_builder_matches_oracle(make):
+    body = make().bodies[0]
+    mesh, _ = scenes.build_mesh(body)
+    om = oracle_mesh(body)
+    assert np.array_equal(mesh["tet_nodes"], om["tet"])
+    assert np.array_equal(mesh["anchored"], om["anchored"])
+    assert np.array_equal(mesh["rest_positions"].astype(np.float64), om["rest"])
+    assert np.array_equal(mesh["rest_inverse"], om["dminv"].astype(np.float32))
+    assert np.array_equal(mesh["volume"], om["volume"].astype(np.float32))
+    assert np.allclose(mesh["node_mass"], om["mass"], rtol=1e-6)
+    if body.per_tet:
+        assert np.array_equal(mesh["tet_material"], np.arange(mesh["tet_nodes"].shape[0]))
+
+
+def test_mesh_builder_counts_and_orientation():
+    mesh, _ = scenes.build_mesh(scenes.cube(5).bodies[0])
+    assert mesh["tet_nodes"].shape == (6 * 125, 4)
+    assert mesh["rest_positions"].shape == (216, 3)
+    X = mesh["rest_positions"].astype(np.float64)
+    T = mesh["tet_nodes"]
+    Ds = np.stack([X[T[:, k]] - X[T[:, 0]] for k in (1, 2, 3)], axis=2)
+    assert (np.linalg.det(Ds) > 0).all()
+    # 6 Kuhn tets per cell and every node used
+    assert np.unique(T).size == 216
+
+
+def test_mesh_builder_solid_mask_and_lanes():
+    res = (4, 3, 3)
+    solid = np.ones(3 * 2 * 2, dtype=np.uint8)
+    solid[0] = 0
+    lanes = np.arange(12, dtype=np.uint32) % 2
+    mats = [api.PhysicalMaterial(density=1000.0), api.PhysicalMaterial(density=3000.0)]
+    mesh = api.build_lattice(res, (0.1, 0.1, 0.1), materials=mats, solid=solid, cell_lane=lanes)
+    assert mesh["tet_nodes"].shape[0] == 6 * 11
+    assert set(np.unique(mesh["tet_material"])) == {0, 1}
+    expect = (5 * 1000.0 + 6 * 3000.0) * 1e-3
+    assert mesh["node_mass"].sum() == pytest.approx(expect, rel=1e-5)
+
+
+def test_mesh_builder_rejects_degenerate_lattice():
+    with pytest.raises(api.TalusError):
+        api.build_lattice((1, 3, 3), (0.1, 0.1, 0.1))
+
+
+def test_scenes_are_seeded():
+    a = scenes.cube(6, seed=4).bodies[0]
+    b = scenes.cube(6, seed=4).bodies[0]
+    assert np.array_equal(a.sample_positions, b.sample_positions)
+    assert np.array_equal(a.materials, b.materials)
+    c = scenes.cube(6, seed=5).bodies[0]
+    assert not np.array_equal(a.sample_positions, c.sample_positions)
+    sc = scenes.bodies(count=16, seed=1)
+    tets = [6 * (b.resolution[0] - 1) * (b.resolution[1] - 1) * (b.resolution[2] - 1) for b in sc.bodies]
+    assert min(tets) >= 100 and max(tets) <= 30000

@@ -1,0 +1,426 @@
+"""Pins the FP64 oracle by mathematics the reference's own tests use, not by
+retyping its formulas: closed forms (uniaxial strain, radial return),
+finite differences of energies, rigid-motion invariance, library routines
+(numpy SVD / eigh / solve), conservation laws and mesh invariants.
+
+Reference tests mirrored (paths relative to the Talus tree):
+  rigid motion -> zero force        tests/dynamics_strainstep_tests.cc:1115-1138,
+                                    tests/neohookean_tests.cc:115-135
+  energy gradient vs FD             dynamics_strainstep_tests.cc:1422-1534
+  SNH Hessian / stiffness vs FD     tests/dynamics_tests.cc:191-247, 2755-2800
+  SNH vs corotated at small strain  neohookean_tests.cc:137-176 (<= 2e-3)
+  inverted element finite           neohookean_tests.cc:178-199
+  rotation recovery to 1e-6         dynamics_strainstep_tests.cc:1140-1154
+  Hencky return closed forms        dynamics_strainstep_tests.cc:3090-3138
+  momentum kept to 1e-12            dynamics_strainstep_tests.cc:1156-1190
+"""
+import math
+
+import numpy as np
+import pytest
+
+import oracle as O
+from talus_amd import scenes
+
+from helpers import oracle_body, oracle_mesh
+
+
+def unit_tet(scale=1.0, seed=None):
+    X = np.array([[0, 0, 0], [1, 0, 0], [0, 1, 0], [0, 0, 1]], dtype=float) * scale
+    if seed is not None:
+        X = X + 0.1 * scale * np.random.default_rng(seed).standard_normal(X.shape)
+    Ds = (X[1:] - X[0]).T
+    assert np.linalg.det(Ds) > 0
+    return X, np.linalg.inv(Ds), np.linalg.det(Ds) / 6.0
+
+
+def rot(axis, angle):
+    axis = np.asarray(axis, float) / np.linalg.norm(axis)
+    K = np.array([[0, -axis[2], axis[1]], [axis[2], 0, -axis[0]], [-axis[1], axis[0], 0]])
+    return np.eye(3) + math.sin(angle) * K + (1 - math.cos(angle)) * K @ K
+
+
+LAM, MU = O.lame(1e6, 0.3)
+
+
+def test_lame_closed_form_and_clamp():
+    assert LAM == pytest.approx(1e6 * 0.3 / (1.3 * 0.4), rel=1e-7)
+    assert MU == pytest.approx(1e6 / 2.6, rel=1e-7)
+    lam, mu = O.lame(1e6, 0.6)  # clamped to 0.49
+    assert mu == pytest.approx(1e6 / 2.98, rel=1e-7)
+    assert lam == pytest.approx(1e6 * 0.49 / (1.49 * 0.02), rel=1e-6)
+
+
+@pytest.mark.parametrize("seed", [None, 1, 2])
+def test_snh_rigid_motion_has_no_force(seed):
+    X, Dm, V = unit_tet(seed=seed)
+    R = rot([1, 2, 3], 0.7)
+    x = X @ R.T + np.array([0.3, -1.0, 2.0])
+    f, e = O.snh_tet(x - X, Dm, V, LAM, MU)
+    assert np.abs(f).max() <= 1e-12 * MU * V
+    assert abs(e) <= 1e-12 * MU * V
+
+
+def test_snh_uniaxial_strain_closed_form():
+    X, Dm, V = unit_tet()
+    eps = 1e-3
+    F = np.diag([1 + eps, 1.0, 1.0])
+    x = X @ F.T
+    f, e = O.snh_tet(x - X, Dm, V, LAM, MU)
+    # psi = (lambda + 2 mu) eps^2 / 2; P11 = (lambda+2mu) eps, P22 = lambda eps + (lambda+mu) eps^2
+    assert e == pytest.approx(V * 0.5 * (LAM + 2 * MU) * eps ** 2, rel=1e-12)
+    P = np.diag([(LAM + 2 * MU) * eps, LAM * eps + (LAM + MU) * eps ** 2, LAM * eps + (LAM + MU) * eps ** 2])
+    g = np.vstack([-Dm.sum(axis=0), Dm])
+    expect = -V * g @ P.T
+    assert np.abs(f - expect).max() <= 1e-9 * np.abs(expect).max()
+
+
+@pytest.mark.parametrize("seed", [3, 4, 5])
+def test_snh_force_is_minus_energy_gradient(seed):
+    X, Dm, V = unit_tet(seed=seed)
+    rng = np.random.default_rng(seed)
+    x = X + 0.15 * rng.standard_normal(X.shape)
+    f, _ = O.snh_tet(x - X, Dm, V, LAM, MU)
+    h = 1e-6
+    g = np.zeros_like(x)
+    for i in range(4):
+        for k in range(3):
+            xp, xm = x.copy(), x.copy()
+            xp[i, k] += h
+            xm[i, k] -= h
+            g[i, k] = (O.snh_tet(xp - X, Dm, V, LAM, MU)[1] - O.snh_tet(xm - X, Dm, V, LAM, MU)[1]) / (2 * h)
+    assert np.abs(f + g).max() <= 1e-6 * np.abs(g).max()
+
+
+def test_snh_plastic_offset_is_chain_rule():
+    X, Dm, V = unit_tet(seed=6)
+    rng = np.random.default_rng(6)
+    x = X + 0.05 * rng.standard_normal(X.shape)
+    Ep = 0.05 * rng.standard_normal((3, 3))
+    f, e = O.snh_tet(x - X, Dm, V, LAM, MU, Ep)
+    h = 1e-6
+    for i in range(4):
+        for k in range(3):
+            xp, xm = x.copy(), x.copy()
+            xp[i, k] += h
+            xm[i, k] -= h
+            d = (O.snh_tet(xp - X, Dm, V, LAM, MU, Ep)[1] - O.snh_tet(xm - X, Dm, V, LAM, MU, Ep)[1]) / (2 * h)
+            assert abs(f[i, k] + d) <= 1e-6 * np.abs(f).max()
+    # energy equals the unplastic energy of the elastic gradient F (I + Ep)
+    F = np.eye(3) + (x[1:] - x[0]).T @ Dm - (X[1:] - X[0]).T @ Dm
+    Fe = F @ (np.eye(3) + Ep)
+    He = Fe - np.eye(3)
+    J1 = np.linalg.det(Fe) - 1
+    psi = 0.5 * MU * (He * He).sum() - MU * (J1 - np.trace(He)) + 0.5 * (LAM + MU) * J1 ** 2
+    assert e == pytest.approx(V * psi, rel=1e-9)
+
+
+def test_snh_energy_matches_textbook_stable_neo_hookean():
+    """Smith, de Goes & Kim (2018): psi = mu/2 (I_C - 3) - mu (J - 1) + (lambda+mu)/2 (J-1)^2
+    with I_C = |F|^2; equal to the H-form used by the reference."""
+    X, Dm, V = unit_tet(seed=7)
+    rng = np.random.default_rng(7)
+    x = X + 0.2 * rng.standard_normal(X.shape)
+    _, e = O.snh_tet(x - X, Dm, V, LAM, MU)
+    F = (x[1:] - x[0]).T @ Dm
+    J = np.linalg.det(F)
+    psi = 0.5 * MU * ((F * F).sum() - 3) - MU * (J - 1) + 0.5 * (LAM + MU) * (J - 1) ** 2
+    assert e == pytest.approx(V * psi, rel=1e-9, abs=1e-12 * MU * V)
+
+
+def test_snh_inverted_is_finite():
+    X, Dm, V = unit_tet()
+    x = X.copy()
+    x[3] = [0.2, 0.2, -0.5]
+    f, e = O.snh_tet(x - X, Dm, V, LAM, MU)
+    assert np.isfinite(f).all() and np.isfinite(e)
+    # pushes the apex back up through the base
+    assert f[3, 2] > 0
+
+
+def test_snh_and_corotated_agree_at_small_strain():
+    X, Dm, V = unit_tet(seed=8)
+    rng = np.random.default_rng(8)
+    R = rot([0.3, 1, -0.2], 0.5)
+    u = 1e-4 * rng.standard_normal(X.shape)
+    x = (X + u) @ R.T
+    f_snh, _ = O.snh_tet(x - X, Dm, V, LAM, MU)
+    c = O.corotated_tet(x, X, Dm, V, LAM, MU)
+    assert np.abs(f_snh - c["force"]).max() <= 2e-3 * np.abs(c["force"]).max()
+
+
+def test_corotated_rigid_motion_and_linear_limit():
+    X, Dm, V = unit_tet(seed=9)
+    R = rot([1, 0, 1], 1.1)
+    x = X @ R.T + 3.0
+    c = O.corotated_tet(x, X, Dm, V, LAM, MU)
+    assert np.abs(c["force"]).max() <= 1e-9 * MU * V
+    assert np.abs(c["R"] - R).max() <= 1e-9
+    # small displacement, no rotation: f = -K u with K the linear-elastic stiffness
+    rng = np.random.default_rng(9)
+    u = 1e-7 * rng.standard_normal(X.shape)
+    u -= u.mean(axis=0)
+    c = O.corotated_tet(X + u, X, Dm, V, LAM, MU)
+    f_lin = O.snh_tet(u, Dm, V, LAM, MU)[0]  # SNH tangent at rest = linear elasticity
+    assert np.abs(c["force"] - f_lin).max() <= 1e-5 * np.abs(f_lin).max()
+
+
+def test_polar_recovers_rotation():
+    rng = np.random.default_rng(10)
+    for _ in range(20):
+        R = rot(rng.standard_normal(3), rng.uniform(0, 3.0))
+        A = rng.standard_normal((3, 3))
+        S = np.eye(3) + 0.2 * (A + A.T)
+        F = R @ S
+        Rp = O.polar(F)
+        # converged polar factor is orthogonal and R^T F symmetric positive
+        assert np.abs(Rp.T @ Rp - np.eye(3)).max() <= 1e-9
+        assert np.abs(Rp - R).max() <= 1e-6
+
+
+def test_inverted_rotation_matches_sign_fixed_svd():
+    rng = np.random.default_rng(11)
+    for _ in range(20):
+        F = rng.standard_normal((3, 3))
+        if np.linalg.det(F) > 0:
+            F[:, 0] *= -1
+        U, s, Vt = np.linalg.svd(F)
+        if np.linalg.det(U) < 0:
+            U[:, 2] *= -1
+        if np.linalg.det(Vt) < 0:
+            Vt[2, :] *= -1
+        R = U @ Vt
+        Ro = O.element_rotation(F)
+        assert np.abs(Ro - R).max() <= 1e-9
+        assert np.linalg.det(Ro) == pytest.approx(1.0, abs=1e-12)
+
+
+def test_eigen_sym3_matches_numpy():
+    rng = np.random.default_rng(12)
+    for _ in range(20):
+        A = rng.standard_normal((3, 3))
+        A = A + A.T
+        w, V = O.eigen_sym3(A)
+        wn = np.linalg.eigvalsh(A)
+        assert np.abs(w - wn).max() <= 1e-12 * np.abs(wn).max()
+        assert np.abs(A @ V - V * w).max() <= 1e-12 * np.abs(wn).max()
+
+
+def test_stretch_is_largest_singular_value():
+    rng = np.random.default_rng(13)
+    F = np.eye(3) + 0.3 * rng.standard_normal((3, 3))
+    s, d = O.stretch(F)
+    U, sv, Vt = np.linalg.svd(F)
+    assert s == pytest.approx(sv[0], rel=1e-12)
+    assert abs(abs(d @ Vt[0]) - 1.0) <= 1e-9
+
+
+def test_j2_return_closed_forms():
+    """Radial return (Simo & Hughes 1998, ch. 3): after return the von Mises
+    stress equals the updated radius, dgamma = (q_trial - r)/(3 mu + H), the
+    pressure is unchanged, and nothing happens below yield."""
+    X, Dm, V = unit_tet()
+    yld, H = 1e3, 2e5
+    eps = 0.02
+    F = np.eye(3)
+    F[0, 1] = eps  # simple shear
+    x = X @ F.T
+    c = O.corotated_tet(x, X, Dm, V, LAM, MU, None, yld, H)
+    assert c["yielding"] == 1
+    plastic = c["plastic"]
+    gamma = plastic[9]
+    # trial von Mises stress of the corotated small strain
+    R = c["R"]
+    D = R.T @ F - np.eye(3)
+    e = 0.5 * (D + D.T)
+    dev = e - np.trace(e) / 3 * np.eye(3)
+    q_trial = math.sqrt(1.5) * 2 * MU * np.linalg.norm(dev)
+    assert gamma == pytest.approx((q_trial - yld) / (3 * MU + H), rel=1e-10)
+    ep = plastic[:9].reshape(3, 3)
+    assert abs(np.trace(ep)) <= 1e-15  # plastic flow is deviatoric
+    el = e - ep
+    s = 2 * MU * (el - np.trace(el) / 3 * np.eye(3))
+    assert math.sqrt(1.5) * np.linalg.norm(s) == pytest.approx(yld + H * gamma, rel=1e-9)
+    # below yield: elastic, state unchanged
+    c2 = O.corotated_tet(X @ (np.eye(3) + 1e-6 * (F - np.eye(3))).T, X, Dm, V, LAM, MU, None, yld, H)
+    assert c2["yielding"] == 0 and np.all(c2["plastic"] == 0)
+
+
+def _hencky_vm(Fe, mu):
+    w = np.linalg.eigvalsh(Fe @ Fe.T)
+    e = 0.5 * np.log(w)
+    return math.sqrt(1.5) * 2 * mu * np.linalg.norm(e - e.mean())
+
+
+def test_hencky_flow_closed_forms():
+    mu, yld = 1e5, 2e3
+    F = np.diag([1.08, 1 / 1.08, 1.0]) @ rot([0, 0, 1], 0.3)
+    for H in (0.0, 5e4):
+        flowed, ep, alpha = O.hencky_flow(F, mu, yld, H)
+        assert flowed
+        Fe = F @ (np.eye(3) + ep)
+        q_trial = _hencky_vm(F, mu)
+        r = H / (3 * mu)
+        y_now = (yld + r * q_trial) / (1 + r)
+        assert _hencky_vm(Fe, mu) == pytest.approx(y_now, rel=1e-9)
+        assert np.linalg.det(Fe) == pytest.approx(np.linalg.det(F), rel=1e-12)
+        assert alpha == pytest.approx((q_trial - yld) / (3 * mu + H), rel=1e-9)
+    flowed, ep, alpha = O.hencky_flow(np.diag([1.001, 1.0, 1.0]), mu, yld, 0.0)
+    assert not flowed and alpha == 0.0
+
+
+def test_hencky_plastic_max_cap():
+    mu, yld = 1e5, 1e2
+    F = np.diag([1.5, 1 / 1.5, 1.0])
+    flowed, ep, _ = O.hencky_flow(F, mu, yld, 0.0, plastic_max=0.05)
+    assert flowed
+    inv = np.linalg.inv(np.eye(3) + ep) - np.eye(3)
+    assert np.linalg.norm(inv) <= 0.05 * (1 + 1e-9) + 1e-6
+
+
+def test_cantilever_mesh_invariants():
+    body = scenes.cantilever().bodies[0]
+    m = oracle_mesh(body)
+    assert m["tet"].shape == (960, 4) and m["rest"].shape == (275, 3)
+    assert m["anchored"].sum() == 25
+    assert m["volume"].sum() == pytest.approx(0.8 * 0.2 * 0.2, rel=1e-6)
+    assert m["mass"].sum() == pytest.approx(1000 * 0.8 * 0.2 * 0.2, rel=1e-6)
+    X = m["rest"]
+    for e in range(0, 960, 37):
+        Ds = (X[m["tet"][e, 1:]] - X[m["tet"][e, 0]]).T
+        assert np.linalg.det(Ds) > 0
+        assert np.abs(np.linalg.inv(Ds) - m["dminv"][e]).max() <= 1e-9 * np.abs(m["dminv"][e]).max()
+    # first-touch numbering: cell (0,0,0) corners come first
+    assert np.allclose(X[:8], [[0, 0, 0], [0.08, 0, 0], [0, 0.05, 0], [0.08, 0.05, 0],
+                               [0, 0, 0.05], [0.08, 0, 0.05], [0, 0.05, 0.05], [0.08, 0.05, 0.05]])
+
+
+def _tiny_body(E=1e5, model=1, anchored=True):
+    sc = scenes.cantilever()
+    b = sc.bodies[0]
+    b.resolution = (4, 3, 3)
+    b.materials = scenes.material_array(1, youngs_modulus=E, poissons_ratio=0.3, density=1000.0,
+                                        elastic_model=model)
+    if not anchored:
+        b.supports = []
+    return b
+
+
+def test_operator_is_symmetric_and_matches_snh_tangent_at_rest():
+    body = _tiny_body()
+    orc, m = oracle_body(body)
+    n = m["rest"].shape[0]
+    free = m["anchored"] == 0
+    rng = np.random.default_rng(14)
+    d1, d2 = rng.standard_normal((2, n, 3))
+    d1[~free] = 0
+    d2[~free] = 0
+    step = 0.01
+    y1 = orc.apply_system(1.0, step, d1)
+    y2 = orc.apply_system(1.0, step, d2)
+    assert (d2 * y1).sum() == pytest.approx((d1 * y2).sum(), rel=1e-12)
+    # K d equals the directional derivative of the SNH forces at rest
+    h = 1e-7
+    fp = orc.eval_forces(m["rest"] + h * d1)[2]
+    fm = orc.eval_forces(m["rest"] - h * d1)[2]
+    orc.set_rotations(np.tile(np.eye(3), (m["tet"].shape[0], 1, 1)))
+    Kd = (orc.apply_system(0.0, 1.0, d1))
+    fd = -(fp - fm) / (2 * h)
+    fd[~free] = 0
+    assert np.abs(Kd - fd).max() <= 1e-6 * np.abs(fd).max()
+
+
+def test_pcg_step_solution_solves_the_linear_system():
+    """One tight-tolerance oracle step equals a dense numpy solve of the same
+    linearised system for the first solve (elastic body at rest under gravity)."""
+    body = _tiny_body(E=1e5, model=0)
+    orc, m = oracle_body(body)
+    n = m["rest"].shape[0]
+    free = np.repeat(m["anchored"] == 0, 3)
+    h = 1 / 60.0
+    rho = 0.5
+    am, af = (2 * rho - 1) / (rho + 1), rho / (rho + 1)
+    beta = 0.25 * (1 - am + af) ** 2
+    step = math.sqrt((1 - af) * beta) * h
+    A = np.zeros((3 * n, 3 * n))
+    for j in range(3 * n):
+        e = np.zeros(3 * n)
+        e[j] = 1.0
+        A[:, j] = orc.apply_system(1 - am, step, e).reshape(-1)
+    Af = A[np.ix_(free, free)]
+    assert np.abs(Af - Af.T).max() <= 1e-9 * np.abs(Af).max()
+    assert np.linalg.eigvalsh(Af).min() > 0
+    b = np.tile([0.0, -9.81, 0.0], n) * np.repeat(m["mass"], 3)
+    a_exact = np.zeros(3 * n)
+    a_exact[free] = np.linalg.solve(Af, b[free])
+    # an oracle run with only one solve's worth of physics: at rest the force is
+    # zero so the first solve's answer is the step's acceleration to first order
+    orc.step(tolerance=1e-12, cg_iterations=5000)
+    _, _, a = orc.state()
+    assert np.abs(a.reshape(-1) - a_exact).max() <= 2e-3 * np.abs(a_exact).max()
+
+
+def test_free_fall_is_exact_with_consistent_acceleration():
+    body = _tiny_body(E=0.0, anchored=False)
+    orc, m = oracle_body(body)
+    # dt and gravity cross the ABI as floats, like deltaTime and glm::vec3
+    g = np.array([0.0, -9.81, 0.0], dtype=np.float32).astype(np.float64)
+    n = m["rest"].shape[0]
+    orc.set_state(a=np.tile(g, (n, 1)))
+    dt = 1 / 60.0
+    steps = 30
+    for _ in range(steps):
+        orc.step(tolerance=1e-12, cg_iterations=100, dt=dt)
+    x, v, _ = orc.state()
+    t = steps * float(np.float32(dt))
+    assert np.abs(x - (m["rest"] + 0.5 * g * t * t)).max() <= 1e-12 * 10
+    assert np.abs(v - g * t).max() <= 1e-11
+
+
+def test_momentum_is_conserved_without_supports():
+    body = _tiny_body(E=1e5, model=1, anchored=False)
+    orc, m = oracle_body(body)
+    rng = np.random.default_rng(15)
+    n = m["rest"].shape[0]
+    v0 = 0.3 * rng.standard_normal((n, 3))
+    orc.set_state(x=m["rest"] + 0.01 * rng.standard_normal((n, 3)), v=v0)
+    p0 = (m["mass"][:, None] * v0).sum(axis=0)
+    for _ in range(10):
+        orc.step(gravity=(0, 0, 0), tolerance=1e-12, cg_iterations=5000)
+    _, v, _ = orc.state()
+    p = (m["mass"][:, None] * v).sum(axis=0)
+    assert np.abs(p - p0).max() <= 1e-12 * max(1.0, np.abs(p0).max()) * 100
+
+
+def test_trapezoidal_rule_conserves_energy_of_small_vibration():
+    """spectral radius 1 gives alpha_m = alpha_f = 1/2, beta = 1/4, gamma = 1/2:
+    energy-conserving for a linear system, so a small free vibration keeps its
+    kinetic + elastic energy."""
+    body = _tiny_body(E=1e5, model=1, anchored=True)
+    orc, m = oracle_body(body)
+    n = m["rest"].shape[0]
+    rng = np.random.default_rng(16)
+    free = m["anchored"] == 0
+    v0 = np.zeros((n, 3))
+    v0[free] = 1e-4 * rng.standard_normal((free.sum(), 3))
+    orc.set_state(v=v0)
+
+    def energy():
+        x, v, _ = orc.state()
+        _, te, _, _ = orc.eval_forces(x)
+        return 0.5 * (m["mass"][:, None] * v * v).sum() + te.sum()
+
+    e0 = energy()
+    for _ in range(40):
+        orc.step(gravity=(0, 0, 0), tolerance=1e-12, cg_iterations=5000, spectral_radius=1.0)
+    assert energy() == pytest.approx(e0, rel=2e-2)
+
+
+def test_oracle_threads_do_not_change_results():
+    body = scenes.cube(12, seed=17).bodies[0]
+    a, _ = oracle_body(body)
+    b, _ = oracle_body(body)
+    for _ in range(2):
+        a.step(threads=1)
+        b.step(threads=6)
+    assert np.array_equal(a.state()[0], b.state()[0])
