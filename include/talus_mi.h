@@ -203,6 +203,25 @@ int talus_mi_apply_system(talus_mi_ctx *ctx, uint32_t body, double inertia,
  * element pass, 2 operator node gather.  avg_ms = mean launch duration. */
 int talus_mi_time_kernel(talus_mi_ctx *ctx, uint32_t kind, uint32_t repeats,
                          double *avg_ms);
+/* Block Cholesky preconditioner alone (SystemFactor, factor.cc:549-704):
+ * factorizes inertia*M + step^2 sum_e R K R^T with the committed rotations
+ * (anchored rows removed) and returns z = (L L^T)^-1 r for one body. */
+int talus_mi_factor_solve(talus_mi_ctx *ctx, uint32_t body, double inertia, double step,
+                          const double *r, double *z);
+
+typedef struct talus_mi_factor_info {
+  uint32_t fronts;
+  uint32_t levels;
+  uint32_t ok;             /* 0 after a non-positive pivot (block Jacobi used) */
+  uint32_t reserved;
+  uint64_t entries;        /* stored factor entries (DOF) */
+  double flops;            /* dense supernodal factorization flops */
+  double factor_seconds;   /* device time of the last factorization */
+} talus_mi_factor_info;
+
+/* Statistics of the current factor (zeros before the first factorization). */
+int talus_mi_factor_info_get(talus_mi_ctx *ctx, talus_mi_factor_info *info);
+
 /* The context's HIP stream (hipStream_t) for interop. */
 void *talus_mi_stream(talus_mi_ctx *ctx);
 

@@ -116,6 +116,12 @@ class Report(C.Structure):
         return {k: getattr(self, k) for k, _ in self._fields_ if k != "reserved"}
 
 
+class FactorInfo(C.Structure):
+    _fields_ = [("fronts", C.c_uint32), ("levels", C.c_uint32), ("ok", C.c_uint32),
+                ("reserved", C.c_uint32), ("entries", C.c_uint64), ("flops", C.c_double),
+                ("factor_seconds", C.c_double)]
+
+
 class Event(C.Structure):
     _fields_ = [("kind", C.c_uint32), ("body", C.c_uint32), ("tet", C.c_uint32),
                 ("step", C.c_uint32)]
@@ -151,6 +157,9 @@ _SIGNATURES = {
     "talus_mi_apply_system": (C.c_int, [C.c_void_p, C.c_uint32, C.c_double, C.c_double,
                                         _P(C.c_double), _P(C.c_double)]),
     "talus_mi_time_kernel": (C.c_int, [C.c_void_p, C.c_uint32, C.c_uint32, _P(C.c_double)]),
+    "talus_mi_factor_solve": (C.c_int, [C.c_void_p, C.c_uint32, C.c_double, C.c_double,
+                                        _P(C.c_double), _P(C.c_double)]),
+    "talus_mi_factor_info_get": (C.c_int, [C.c_void_p, _P(FactorInfo)]),
     "talus_mi_stream": (C.c_void_p, [C.c_void_p]),
 }
 
@@ -390,6 +399,18 @@ class Context:
         ms = C.c_double()
         _check(lib().talus_mi_time_kernel(self._h, kind, repeats, C.byref(ms)))
         return float(ms.value)
+
+    def factor_solve(self, body, inertia, step, r):
+        n, _ = self.info(body)
+        r = _f64(r, 3 * n)
+        z = np.empty(3 * n)
+        _check(lib().talus_mi_factor_solve(self._h, body, inertia, step, _dp(r), _dp(z)))
+        return z.reshape(n, 3)
+
+    def factor_info(self) -> dict:
+        i = FactorInfo()
+        _check(lib().talus_mi_factor_info_get(self._h, C.byref(i)))
+        return {k: getattr(i, k) for k, _ in FactorInfo._fields_ if k != "reserved"}
 
     def stream(self) -> int:
         return int(lib().talus_mi_stream(self._h) or 0)

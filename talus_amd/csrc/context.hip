@@ -22,6 +22,7 @@
 #include <string>
 #include <vector>
 
+#include "factor.h"
 #include "host.h"
 #include "kernels.h"
 #include "tmath.h"
@@ -108,6 +109,9 @@ class EngineBase {
   virtual void apply_system(uint32_t b, double inertia, double step, const double *dv,
                             double *y) = 0;
   virtual double time_kernel(uint32_t kind, uint32_t repeats) = 0;
+  virtual void factor_solve(uint32_t b, double inertia, double step, const double *r,
+                            double *z) = 0;
+  virtual void factor_info(talus_mi_factor_info &info) = 0;
 };
 
 template <class Real>
@@ -132,6 +136,8 @@ class Engine final : public EngineBase {
   void apply_system(uint32_t b, double inertia, double step, const double *dv,
                     double *y) override;
   double time_kernel(uint32_t kind, uint32_t repeats) override;
+  void factor_solve(uint32_t b, double inertia, double step, const double *r, double *z) override;
+  void factor_info(talus_mi_factor_info &info) override;
 
  private:
   template <class T>
@@ -165,6 +171,12 @@ class Engine final : public EngineBase {
   void upload_params(const std::vector<StepParams> &params, size_t slot);
   void solve(bool first, uint32_t limit, uint32_t solve_index);
   void all_active();
+  // block Cholesky preconditioner
+  void analyze_factor();
+  void factorize(const std::vector<double> &reach, const std::vector<double> &inertia);
+  void precondition(int parity);
+  template <class T>
+  T *upload(const std::vector<T> &h) { return dev_copy(h); }
 
   hipStream_t stream_;
   std::vector<void *> owned_, pinned_;
@@ -183,11 +195,35 @@ class Engine final : public EngineBase {
   Real *hvec_ = nullptr;  // 3N + 12T staging of Real
   double *scratch64_ = nullptr;
   bool any_plastic_ = false, any_tear_ = false;
+  // factor state
+  const std::vector<Body> *bodies_ = nullptr;
+  bool sym_ready_ = false, fac_ready_ = false, fac_ok_ = false, use_factor_ = false;
+  bool refresh_ = false;
+  std::vector<double> fac_reach_, fac_inertia_;
+  FactorTables tab_;
+  FactorDev fd_{};
+  uint64_t *d_pair_target_ = nullptr;
+  uint32_t *d_pair_ld_ = nullptr, *d_pair_node_ = nullptr, *d_pair_start_ = nullptr,
+           *d_contrib_ = nullptr, *d_pair_body_ = nullptr, *d_parent_ = nullptr,
+           *d_lvl_fronts_ = nullptr, *d_potrf_ = nullptr, *d_node_body_ = nullptr;
+  uint64_t *d_ea_ = nullptr, *d_trsm_ = nullptr, *d_syrk_ = nullptr, *d_trtri_ = nullptr,
+           *d_m11r_ = nullptr, *d_l21r_ = nullptr, *d_l21c_ = nullptr, *d_m11c_ = nullptr;
+  double *d_body_reach_ = nullptr, *d_body_inertia_ = nullptr;
+  int *hfailed_ = nullptr;
+  uint32_t refactors_ = 0;
+  double factor_seconds_ = 0.0;
+
+ public:
+  const FactorTables &tables() const { return tab_; }
+  double last_factor_seconds() const { return factor_seconds_; }
 };
 
 template <class Real>
 void Engine<Real>::build(const std::vector<Body> &bodies) {
   release();
+  bodies_ = &bodies;
+  sym_ready_ = fac_ready_ = fac_ok_ = false;
+  fd_ = FactorDev{};
   const uint32_t B = uint32_t(bodies.size());
   uint64_t N = 0, T = 0;
   n0_.assign(B, 0);
@@ -355,8 +391,138 @@ void Engine<Real>::build(const std::vector<Body> &bodies) {
   hepartial_ = pinned<double>(d_.n_ttiles);
   hyields_ = pinned<uint32_t>(B);
   hvec_ = pinned<Real>(3 * N + 12 * T + 9 * T);
+  {
+    std::vector<uint32_t> nb(N);
+    for (uint32_t b = 0; b < B; ++b)
+      for (uint32_t n = 0; n < nn_[b]; ++n) nb[n0_[b] + n] = b;
+    d_node_body_ = dev_copy(nb);
+  }
+  hfailed_ = pinned<int>(B);
   HIPCHECK(hipDeviceSynchronize());
   built_ = true;
+}
+
+// Symbolic analysis of every body and upload of the factor tables
+// (analyzeSystem, factor.cc:549-565).
+template <class Real>
+void Engine<Real>::analyze_factor() {
+  const std::vector<Body> &bodies = *bodies_;
+  std::vector<Symbolic> sym(bodies.size());
+  for (size_t b = 0; b < bodies.size(); ++b) sym[b] = analyze(bodies[b], 64);
+  build_tables(sym, bodies, n0_, t0_, tab_);
+  const FactorTables &t = tab_;
+  fd_.fronts = t.fronts;
+  fd_.k = upload(t.k);
+  fd_.m = upload(t.m);
+  fd_.body = upload(t.body);
+  fd_.rows_off = upload(t.rows_off);
+  fd_.map_off = upload(t.map_off);
+  fd_.child_start = upload(t.child_start);
+  fd_.children = upload(t.children);
+  fd_.rows = upload(t.rows);
+  fd_.map = upload(t.map);
+  fd_.off = upload(t.off);
+  fd_.minv_off = upload(t.minv_off);
+  fd_.vec_off = upload(t.vec_off);
+  fd_.yoff = upload(t.yoff);
+  fd_.pool = dev<double>(t.pool);
+  fd_.minv = dev<double>(t.minv_pool);
+  fd_.vec = dev<double>(t.vec_pool);
+  fd_.y = dev<double>(t.y_pool);
+  fd_.xsol = dev<double>(3 * size_t(d_.N));
+  fd_.failed = dev<int>(d_.B);
+  d_parent_ = upload(t.parent);
+  d_pair_target_ = upload(t.pair_target);
+  d_pair_ld_ = upload(t.pair_ld);
+  d_pair_node_ = upload(t.pair_node);
+  d_pair_body_ = upload(t.pair_body);
+  d_pair_start_ = upload(t.pair_start);
+  d_contrib_ = upload(t.contrib);
+  d_lvl_fronts_ = upload(t.lvl_fronts);
+  d_ea_ = upload(t.ea_tasks);
+  d_potrf_ = upload(t.potrf_tasks);
+  d_trsm_ = upload(t.trsm_tasks);
+  d_syrk_ = upload(t.syrk_tasks);
+  d_trtri_ = upload(t.trtri_tasks);
+  d_m11r_ = upload(t.m11_rows);
+  d_l21r_ = upload(t.l21_rows);
+  d_l21c_ = upload(t.l21_cols);
+  d_m11c_ = upload(t.m11_cols);
+  d_body_reach_ = dev<double>(d_.B);
+  d_body_inertia_ = dev<double>(d_.B);
+  sym_ready_ = true;
+}
+
+// Numeric factorization (refreshFactor, factor.cc:567-613) of
+// inertia M + reach sum R K R^T with the committed rotations.
+template <class Real>
+void Engine<Real>::factorize(const std::vector<double> &reach, const std::vector<double> &inertia) {
+  if (!sym_ready_) analyze_factor();
+  hipStream_t s = stream_;
+  hipEvent_t a, b;
+  HIPCHECK(hipEventCreate(&a));
+  HIPCHECK(hipEventCreate(&b));
+  HIPCHECK(hipEventRecord(a, s));
+  const FactorTables &t = tab_;
+  HIPCHECK(hipMemcpyAsync(d_body_reach_, reach.data(), d_.B * sizeof(double), hipMemcpyHostToDevice, s));
+  HIPCHECK(hipMemcpyAsync(d_body_inertia_, inertia.data(), d_.B * sizeof(double), hipMemcpyHostToDevice, s));
+  HIPCHECK(hipMemsetAsync(fd_.pool, 0, std::max<uint64_t>(t.pool, 1) * sizeof(double), s));
+  HIPCHECK(hipMemsetAsync(fd_.minv, 0, std::max<uint64_t>(t.minv_pool, 1) * sizeof(double), s));
+  HIPCHECK(hipMemsetAsync(fd_.failed, 0, d_.B * sizeof(int), s));
+  FactorLaunch<Real>::assemble(fd_, d_pair_target_, d_pair_ld_, d_pair_node_, d_pair_start_,
+                               d_contrib_, uint32_t(t.pair_target.size()), d_.dminv, d_.vol64,
+                               d_.lam64, d_.mu64, d_.R, d_.T, d_.mass, d_body_reach_,
+                               d_body_inertia_, d_pair_body_, s);
+  size_t ea = 0, fac = 0;
+  for (uint32_t l = 0; l < t.levels; ++l) {
+    for (uint32_t r = 0; r < t.ea_rounds[l]; ++r, ++ea)
+      factor_extend_add(fd_, d_parent_, d_ea_ + t.ea_start[ea], t.ea_start[ea + 1] - t.ea_start[ea], s);
+    for (uint32_t p = 0; p < t.fac_panels[l]; ++p, ++fac) {
+      factor_potrf(fd_, d_potrf_ + t.potrf_start[fac], t.potrf_start[fac + 1] - t.potrf_start[fac], p, s);
+      factor_trsm(fd_, d_trsm_ + t.trsm_start[fac], t.trsm_start[fac + 1] - t.trsm_start[fac], p, s);
+      factor_syrk(fd_, d_syrk_ + t.syrk_start[fac], t.syrk_start[fac + 1] - t.syrk_start[fac], p, s);
+    }
+  }
+  for (size_t i = 0; i + 1 < t.trtri_start.size(); ++i)
+    factor_trtri(fd_, d_trtri_ + t.trtri_start[i], t.trtri_start[i + 1] - t.trtri_start[i], s);
+  HIPCHECK(hipMemcpyAsync(hfailed_, fd_.failed, d_.B * sizeof(int), hipMemcpyDeviceToHost, s));
+  HIPCHECK(hipEventRecord(b, s));
+  HIPCHECK(hipStreamSynchronize(s));
+  float ms = 0.0f;
+  HIPCHECK(hipEventElapsedTime(&ms, a, b));
+  HIPCHECK(hipEventDestroy(a));
+  HIPCHECK(hipEventDestroy(b));
+  factor_seconds_ = ms * 1e-3;
+  fac_ok_ = true;
+  for (uint32_t q = 0; q < d_.B; ++q) fac_ok_ = fac_ok_ && hfailed_[q] == 0;
+  fac_ready_ = true;
+  fac_reach_ = reach;
+  fac_inertia_ = inertia;
+  ++refactors_;
+}
+
+// z = (LL^T)^-1 r for the bodies still iterating (applyFactor,
+// factor.cc:648-704, without the body-frame rotation).
+template <class Real>
+void Engine<Real>::precondition(int parity) {
+  hipStream_t s = stream_;
+  const FactorTables &t = tab_;
+  const uint32_t B = d_.B;
+  for (uint32_t l = 0; l < t.levels; ++l) {
+    SolveLaunch<Real>::gather(fd_, d_lvl_fronts_ + t.lvl_fronts_start[l],
+                              t.lvl_fronts_start[l + 1] - t.lvl_fronts_start[l], d_.r, d_.done, parity, B, s);
+    SolveLaunch<Real>::m11_rows(fd_, d_m11r_ + t.m11r_start[l], t.m11r_start[l + 1] - t.m11r_start[l],
+                                d_.done, parity, B, s);
+    SolveLaunch<Real>::l21_rows(fd_, d_l21r_ + t.l21r_start[l], t.l21r_start[l + 1] - t.l21r_start[l],
+                                d_.done, parity, B, s);
+  }
+  for (uint32_t l = t.levels; l-- > 0;) {
+    SolveLaunch<Real>::l21_cols(fd_, d_l21c_ + t.l21c_start[l], t.l21c_start[l + 1] - t.l21c_start[l],
+                                d_.done, parity, B, s);
+    SolveLaunch<Real>::m11_cols(fd_, d_m11c_ + t.m11c_start[l], t.m11c_start[l + 1] - t.m11c_start[l],
+                                d_.done, parity, B, s);
+  }
+  SolveLaunch<Real>::finish(fd_, d_.z, d_.N, d_.anchored, d_node_body_, d_.done, parity, B, s);
 }
 
 template <class Real>
@@ -400,19 +566,36 @@ void Engine<Real>::all_active() {
 template <class Real>
 void Engine<Real>::solve(bool first, uint32_t limit, uint32_t solve_index) {
   hipStream_t s = stream_;
+  const bool fac = use_factor_;
   if (first) {
     Launch<Real>::op_element(d_, d_.x0, -1, s);
-    Launch<Real>::pcg_init(d_, true, s);
+    Launch<Real>::pcg_init(d_, true, fac, s);
   } else {
-    Launch<Real>::pcg_init(d_, false, s);
+    Launch<Real>::pcg_init(d_, false, fac, s);
+  }
+  if (fac) {
+    precondition(-1);
+    Launch<Real>::pcg_rz(d_, true, -1, s);
   }
   Launch<Real>::pcg_begin(d_, s);
-  constexpr uint32_t CHECK = 8;
+  const uint32_t CHECK = fac ? 1u : 8u;
   for (uint32_t k = 0; k < limit; ++k) {
     const int parity = int(k & 1u);
+    if (fac && k == 0) {
+      // convergence of the starting residual is known only on the device
+      HIPCHECK(hipMemcpyAsync(hflags_, d_.done, d_.B * sizeof(int), hipMemcpyDeviceToHost, s));
+      HIPCHECK(hipStreamSynchronize(s));
+      bool all = true;
+      for (uint32_t b = 0; b < d_.B && all; ++b) all = hflags_[b] != 0;
+      if (all) break;
+    }
     Launch<Real>::op_element(d_, d_.p, parity, s);
     Launch<Real>::op_gather(d_, d_.p, d_.q, parity, s);
-    Launch<Real>::pcg_update(d_, parity, s);
+    Launch<Real>::pcg_update(d_, parity, fac, s);
+    if (fac) {
+      precondition(parity);
+      Launch<Real>::pcg_rz(d_, false, parity, s);
+    }
     Launch<Real>::pcg_direction(d_, parity, s);
     if ((k + 1) % CHECK == 0 && k + 1 < limit) {
       HIPCHECK(hipMemcpyAsync(hflags_, d_.done + size_t((k + 1) & 1u) * d_.B, d_.B * sizeof(int),
@@ -499,10 +682,27 @@ void Engine<Real>::step(const talus_mi_step_params &sp, talus_mi_report &rep,
     hrr_ = pinned<double>(hsolve_slots_ * B);
     htarget_ = pinned<double>(hsolve_slots_ * B);
   }
-  const bool diag = !factor || true;  // block-Jacobi preconditioner
   uint32_t solve_index = 0;
+  const uint32_t refactors_before = refactors_;
   for (uint32_t pass = 0; pass < passes; ++pass) {
     for (uint32_t b = 0; b < B; ++b) params[b] = params_for(substeps[b], pass < substeps[b]);
+    if (factor) {
+      // refresh when missing, when the step or inertia keys change, or after
+      // a capped solve (solver.cc:3639-3655, 3839-3899)
+      std::vector<double> reach(B), inertia(B);
+      for (uint32_t b = 0; b < B; ++b) {
+        const StepParams q = params_for(substeps[b], true);
+        reach[b] = q.reach;
+        inertia[b] = q.inertia;
+      }
+      if (!fac_ready_ || refresh_ || reach != fac_reach_ || inertia != fac_inertia_) {
+        HIPCHECK(hipStreamSynchronize(s));
+        factorize(reach, inertia);
+        refresh_ = false;
+      }
+    }
+    use_factor_ = factor && fac_ok_;
+    const bool diag = !use_factor_;
     upload_params(params, 1 + pass);
     Launch<Real>::predict(d_, s);
     Launch<Real>::assemble(d_, diag, false, s);
@@ -550,6 +750,7 @@ void Engine<Real>::step(const talus_mi_step_params &sp, talus_mi_report &rep,
   rep.nodes = d_.N;
   rep.elements = d_.T;
   rep.substeps = passes;
+  rep.refactors = refactors_ - refactors_before;
   rep.device_seconds = ms * 1.0e-3;
   for (uint32_t si = 0; si < solve_index; ++si) {
     const uint32_t pass = si / 3;
@@ -578,6 +779,7 @@ void Engine<Real>::step(const talus_mi_step_params &sp, talus_mi_report &rep,
     rep.yield_events += hyields_[b];
   }
   rep.plastic_growth = rep.yield_events ? 1u : 0u;
+  if (use_factor_ && rep.cg_capped) refresh_ = true;
   torn.assign(d_.T, 0);
   if (tears) torn = tflag;
   for (uint32_t b = 0; b < B; ++b) {
@@ -719,6 +921,35 @@ double Engine<Real>::time_kernel(uint32_t kind, uint32_t repeats) {
   HIPCHECK(hipEventDestroy(a));
   HIPCHECK(hipEventDestroy(b));
   return double(ms) / std::max(repeats, 1u);
+}
+
+template <class Real>
+void Engine<Real>::factor_solve(uint32_t b, double inertia, double step, const double *r,
+                                double *z) {
+  hipStream_t s = stream_;
+  std::vector<double> reach(d_.B, step * step), inert(d_.B, inertia);
+  factorize(reach, inert);
+  const size_t n0 = n0_[b], nn = nn_[b];
+  std::vector<Real> in(3 * size_t(d_.N), Real(0));
+  for (size_t i = 0; i < 3 * nn; ++i) in[3 * n0 + i] = Real(r[i]);
+  HIPCHECK(hipMemcpyAsync(d_.r, in.data(), in.size() * sizeof(Real), hipMemcpyHostToDevice, s));
+  precondition(-1);
+  std::vector<Real> out(3 * nn);
+  HIPCHECK(hipMemcpyAsync(out.data(), d_.z + 3 * n0, 3 * nn * sizeof(Real), hipMemcpyDeviceToHost, s));
+  HIPCHECK(hipStreamSynchronize(s));
+  for (size_t i = 0; i < 3 * nn; ++i) z[i] = double(out[i]);
+}
+
+template <class Real>
+void Engine<Real>::factor_info(talus_mi_factor_info &info) {
+  std::memset(&info, 0, sizeof info);
+  if (!sym_ready_) return;
+  info.fronts = tab_.fronts;
+  info.levels = tab_.levels;
+  info.ok = fac_ok_ ? 1u : 0u;
+  info.entries = tab_.factor_entries;
+  info.flops = tab_.flops;
+  info.factor_seconds = factor_seconds_;
 }
 
 }  // namespace tmi
@@ -998,6 +1229,25 @@ int talus_mi_time_kernel(talus_mi_ctx *ctx, uint32_t kind, uint32_t repeats, dou
     if (ctx->bodies.empty()) return fail(TALUS_MI_ERR_STATE, "no bodies");
     ensure(ctx);
     *avg_ms = ctx->engine->time_kernel(kind, repeats);
+    return TALUS_MI_OK;
+  });
+}
+
+int talus_mi_factor_solve(talus_mi_ctx *ctx, uint32_t body, double inertia, double step,
+                          const double *r, double *z) {
+  if (!ctx || !body_ok(ctx, body) || !r || !z) return fail(TALUS_MI_ERR_ARGUMENT, "bad arguments");
+  return guarded(ctx, [&]() -> int {
+    ensure(ctx);
+    ctx->engine->factor_solve(body, inertia, step, r, z);
+    return TALUS_MI_OK;
+  });
+}
+
+int talus_mi_factor_info_get(talus_mi_ctx *ctx, talus_mi_factor_info *info) {
+  if (!info) return fail(TALUS_MI_ERR_ARGUMENT, "null info");
+  return guarded(ctx, [&]() -> int {
+    ensure(ctx);
+    ctx->engine->factor_info(*info);
     return TALUS_MI_OK;
   });
 }

@@ -1,0 +1,157 @@
+// factor.h -- supernodal block Cholesky preconditioner of libtalus_mi.
+//
+// Replaces the reference's SystemFactor (src/talus/deformable/factor/
+// factor.cc:302-314, 318-484, 549-704: Eigen SimplicialLLT with AMD ordering
+// of the explicit FP64 system, used as the PCG preconditioner).  Here:
+//   ordering   geometric nested dissection of the free nodes (split at the
+//              widest coordinate gap near the median of the longest axis,
+//              separator = the left nodes touching the right part), leaves
+//              of at most `leaf` nodes; every leaf and separator is one
+//              dense supernode ("front") in 3x3 node blocks
+//   symbolic   rows(s) = cols(s) + later nodes adjacent to cols(s) or in a
+//              child's update rows; parent = front of the first update row
+//   numeric    multifrontal, level by level from the leaves: assemble the
+//              FP64 element blocks h^2 R K R^T + inertia m I, extend-add the
+//              children's update matrices, then blocked right-looking
+//              Cholesky of each front (POTRF of 64x64 diagonal blocks, TRSM
+//              and SYRK/GEMM updates on FP64 MFMA tiles)
+//   solve      the diagonal block of each front is inverted explicitly
+//              (M11 = L11^-1) so forward and backward substitution are
+//              parallel GEMV passes per level
+// All sums run in a fixed order: the factor and every solve are
+// bit-reproducible.
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include <cstdint>
+#include <vector>
+
+#include "host.h"
+
+namespace tmi {
+
+constexpr int FNB = 64;   // panel / tile width in DOF
+constexpr int SROW = 32;  // rows per solve tile
+
+struct FrontSym {
+  uint32_t ncol = 0;   // nodes eliminated by this front
+  uint32_t nrow = 0;   // rows in nodes, the first ncol are the columns
+  uint32_t row0 = 0;   // offset into Symbolic::rows
+  int32_t parent = -1;
+  uint32_t level = 0;  // 0 = leaf
+};
+
+// Symbolic analysis of one body (body-local node indices).
+struct Symbolic {
+  uint32_t nfree = 0;
+  std::vector<int32_t> pos;      // node -> elimination index, -1 when anchored
+  std::vector<uint32_t> perm;    // elimination index -> node
+  std::vector<FrontSym> fronts;  // postorder: children before parents
+  std::vector<uint32_t> rows;    // elimination indices of each front's rows
+  std::vector<uint32_t> front_of;      // elimination index -> front
+  std::vector<uint32_t> child_start;   // CSR of children in ascending order
+  std::vector<uint32_t> children;
+  uint32_t levels = 0;
+  double flops = 0.0;            // dense supernodal factorization flops
+  uint64_t factor_entries = 0;   // sum over fronts of k*m - k(k-1)/2 (DOF)
+};
+
+Symbolic analyze(const Body &body, uint32_t leaf);
+
+// Device tables for every body of a context (fronts of all bodies are
+// concatenated; levels align at the leaves).
+struct FactorTables {
+  uint32_t fronts = 0, levels = 0, bodies = 0;
+  // per front
+  std::vector<uint32_t> k, m, body, parent, rows_off, map_off, level;
+  std::vector<uint64_t> off, minv_off, vec_off, yoff;
+  std::vector<uint32_t> rows;     // global node index of every front row
+  std::vector<uint32_t> map;      // per update row: node-row index in the parent
+  std::vector<uint32_t> child_start, children;
+  // element-pair assembly of the original matrix (lower node blocks)
+  std::vector<uint64_t> pair_target;  // FP64 offset of the 3x3 block (col-major, ld m)
+  std::vector<uint32_t> pair_ld;      // front m of the target
+  std::vector<uint32_t> pair_node;    // global node for the diagonal mass, or ~0u
+  std::vector<uint32_t> pair_body;
+  std::vector<uint32_t> pair_start;   // CSR into contrib
+  std::vector<uint32_t> contrib;      // (global tet << 4) | (a << 2) | b
+  // task lists, each a CSR over its launch index
+  std::vector<uint32_t> lvl_fronts_start, lvl_fronts;        // fronts per level
+  std::vector<uint32_t> ea_start;                            // per (level, round)
+  std::vector<uint64_t> ea_tasks;                            // child | ti<<32 | tj<<48
+  std::vector<uint32_t> ea_rounds;                           // rounds per level
+  std::vector<uint32_t> fac_start;                           // per (level, panel) launch
+  std::vector<uint32_t> fac_panels;                          // panels per level
+  std::vector<uint32_t> potrf_start, potrf_tasks;            // fronts
+  std::vector<uint32_t> trsm_start;
+  std::vector<uint64_t> trsm_tasks;                          // front | rowtile<<32
+  std::vector<uint32_t> syrk_start;
+  std::vector<uint64_t> syrk_tasks;                          // front | I<<32 | J<<48
+  std::vector<uint32_t> trtri_start;                         // per row block i
+  std::vector<uint64_t> trtri_tasks;                         // front | i<<32 | j<<48
+  std::vector<uint32_t> srow_start, scol_start;              // per level
+  std::vector<uint64_t> m11_rows, l21_rows, l21_cols, m11_cols;  // front | tile<<32
+  std::vector<uint32_t> m11r_start, l21r_start, l21c_start, m11c_start;
+  uint64_t pool = 0, minv_pool = 0, vec_pool = 0, y_pool = 0;
+  double flops = 0.0;
+  uint64_t factor_entries = 0;
+};
+
+// Builds the tables for bodies whose node/tet ranges start at n0/t0.
+void build_tables(const std::vector<Symbolic> &sym, const std::vector<Body> &bodies,
+                  const std::vector<uint32_t> &n0, const std::vector<uint32_t> &t0,
+                  FactorTables &out);
+
+// Device side.
+struct FactorDev {
+  uint32_t fronts = 0;
+  const uint32_t *k, *m, *body, *rows_off, *map_off, *child_start, *children, *rows, *map;
+  const uint64_t *off, *minv_off, *vec_off, *yoff;
+  double *pool, *minv, *vec, *y, *xsol;
+  int *failed;  // per body
+};
+
+// Launch helpers (definitions in factor.hip).  `conn/dminv/...` are the
+// engine's element arrays; R is the committed rotation in the element
+// precision.
+template <class Real>
+struct FactorLaunch {
+  static void assemble(const FactorDev &f, const uint64_t *pair_target, const uint32_t *pair_ld,
+                       const uint32_t *pair_node, const uint32_t *pair_start,
+                       const uint32_t *contrib, uint32_t pairs, const double *dminv,
+                       const double *vol, const double *lam, const double *mu, const Real *R,
+                       uint32_t T, const double *mass, const double *body_reach,
+                       const double *body_inertia, const uint32_t *pair_body, hipStream_t s);
+};
+
+void factor_extend_add(const FactorDev &f, const uint32_t *parent, const uint64_t *tasks,
+                       uint32_t count, hipStream_t s);
+void factor_potrf(const FactorDev &f, const uint32_t *tasks, uint32_t count, uint32_t panel,
+                  hipStream_t s);
+void factor_trsm(const FactorDev &f, const uint64_t *tasks, uint32_t count, uint32_t panel,
+                 hipStream_t s);
+void factor_syrk(const FactorDev &f, const uint64_t *tasks, uint32_t count, uint32_t panel,
+                 hipStream_t s);
+void factor_trtri(const FactorDev &f, const uint64_t *tasks, uint32_t count, hipStream_t s);
+
+template <class Real>
+struct SolveLaunch {
+  // z = LL^-T... applied to r for every body whose convergence flag
+  // (done[parity*B + body]) is clear; parity < 0 applies to every body.
+  static void gather(const FactorDev &f, const uint32_t *fronts, uint32_t count, const Real *r,
+                     const int *done, int parity, uint32_t B, hipStream_t s);
+  static void m11_rows(const FactorDev &f, const uint64_t *tasks, uint32_t count,
+                       const int *done, int parity, uint32_t B, hipStream_t s);
+  static void l21_rows(const FactorDev &f, const uint64_t *tasks, uint32_t count,
+                       const int *done, int parity, uint32_t B, hipStream_t s);
+  static void l21_cols(const FactorDev &f, const uint64_t *tasks, uint32_t count,
+                       const int *done, int parity, uint32_t B, hipStream_t s);
+  static void m11_cols(const FactorDev &f, const uint64_t *tasks, uint32_t count,
+                       const int *done, int parity, uint32_t B, hipStream_t s);
+  static void finish(const FactorDev &f, Real *z, uint32_t N, const uint8_t *anchored,
+                     const uint32_t *node_body, const int *done, int parity, uint32_t B,
+                     hipStream_t s);
+};
+
+}  // namespace tmi

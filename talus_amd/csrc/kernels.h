@@ -89,11 +89,12 @@ struct Launch {
   static void evaluate(const Dev<Real> &d, hipStream_t s);
   static void assemble(const Dev<Real> &d, bool diag, bool energy, hipStream_t s);
   static void balance(const Dev<Real> &d, bool diag, bool taken, hipStream_t s);
-  static void pcg_init(const Dev<Real> &d, bool with_x0, hipStream_t s);
+  static void pcg_init(const Dev<Real> &d, bool with_x0, bool factor, hipStream_t s);
+  static void pcg_rz(const Dev<Real> &d, bool start, int parity, hipStream_t s);
   static void pcg_begin(const Dev<Real> &d, hipStream_t s);
   static void op_element(const Dev<Real> &d, const Real *dir, int parity, hipStream_t s);
   static void op_gather(const Dev<Real> &d, const Real *dir, Real *out, int parity, hipStream_t s);
-  static void pcg_update(const Dev<Real> &d, int parity, hipStream_t s);
+  static void pcg_update(const Dev<Real> &d, int parity, bool factor, hipStream_t s);
   static void pcg_direction(const Dev<Real> &d, int parity, hipStream_t s);
   static void apply_increment(const Dev<Real> &d, bool from_x0, hipStream_t s);
   static void finish(const Dev<Real> &d, hipStream_t s);

@@ -210,8 +210,9 @@ __device__ __forceinline__ void apply_P(const Real *Pn, const Real *r, Real *z) 
   z[2] = Pn[2] * r[0] + Pn[4] * r[1] + Pn[5] * r[2];
 }
 
-// r = right - A x0 (A x0 from the op slots when WITH_X0), z = P r, p = z
-template <class Real, bool WITH_X0>
+// r = right - A x0 (A x0 from the op slots when WITH_X0), z = P r, p = z;
+// with FACTOR the preconditioned vector comes from the factor solve later
+template <class Real, bool WITH_X0, bool FACTOR>
 __global__ __launch_bounds__(TILE) void k_pcg_init(Dev<Real> d) {
   __shared__ double lds[TILE / 64];
   const uint32_t t = blockIdx.x, b = d.nt_body[t];
@@ -241,14 +242,16 @@ __global__ __launch_bounds__(TILE) void k_pcg_init(Dev<Real> d) {
         r[c] = Real(rhs - double(q[c]));
         bb += rhs * rhs;
       }
-      apply_P(d.P + 6 * (size_t)n, r, z);
+      if (!FACTOR) apply_P(d.P + 6 * (size_t)n, r, z);
     }
 #pragma unroll
     for (int c = 0; c < 3; ++c) {
       const size_t i = 3 * (size_t)n + c;
       d.r[i] = r[c];
-      d.z[i] = z[c];
-      d.p[i] = z[c];
+      if (!FACTOR) {
+        d.z[i] = z[c];
+        d.p[i] = z[c];
+      }
       d.dx[i] = Real(0);
       rz += double(r[c]) * double(z[c]);
       rr += double(r[c]) * double(r[c]);
@@ -379,7 +382,28 @@ __global__ __launch_bounds__(TILE) void k_op_gather(Dev<Real> d, const Real *dir
   if (threadIdx.x == 0) d.partial[t * NPART + 0] = pq;
 }
 
-template <class Real>
+// rz partial after a factor solve (slot 1, or slot 0 and p = z at the start)
+template <class Real, bool START>
+__global__ __launch_bounds__(TILE) void k_pcg_rz(Dev<Real> d, int parity) {
+  __shared__ double lds[TILE / 64];
+  const uint32_t t = blockIdx.x, b = d.nt_body[t];
+  if (!d.params[b].active || (!START && d.done[parity * d.B + b])) return;
+  const uint32_t n = d.nt_begin[t] + threadIdx.x;
+  double rz = 0.0;
+  if (n < d.nt_end[t]) {
+#pragma unroll
+    for (int c = 0; c < 3; ++c) {
+      const size_t i = 3 * (size_t)n + c;
+      const Real z = d.z[i];
+      if (START) d.p[i] = z;
+      rz += double(d.r[i]) * double(z);
+    }
+  }
+  rz = block_sum(rz, lds);
+  if (threadIdx.x == 0) d.partial[t * NPART + (START ? 0 : 1)] = rz;
+}
+
+template <class Real, bool FACTOR>
 __global__ __launch_bounds__(TILE) void k_pcg_update(Dev<Real> d, int parity) {
   __shared__ double lds[TILE / 64];
   const uint32_t t = blockIdx.x, b = d.nt_body[t];
@@ -399,22 +423,26 @@ __global__ __launch_bounds__(TILE) void k_pcg_update(Dev<Real> d, int parity) {
       r[c] = d.r[i] - stride * d.q[i];
       d.r[i] = r[c];
     }
-    if (d.anchored[n]) {
-      z[0] = z[1] = z[2] = Real(0);
-    } else {
-      apply_P(d.P + 6 * (size_t)n, r, z);
+    if (!FACTOR) {
+      if (d.anchored[n]) {
+        z[0] = z[1] = z[2] = Real(0);
+      } else {
+        apply_P(d.P + 6 * (size_t)n, r, z);
+      }
     }
 #pragma unroll
     for (int c = 0; c < 3; ++c) {
-      d.z[3 * (size_t)n + c] = z[c];
-      rz += double(r[c]) * double(z[c]);
+      if (!FACTOR) {
+        d.z[3 * (size_t)n + c] = z[c];
+        rz += double(r[c]) * double(z[c]);
+      }
       rr += double(r[c]) * double(r[c]);
     }
   }
-  rz = block_sum(rz, lds);
+  if (!FACTOR) rz = block_sum(rz, lds);
   rr = block_sum(rr, lds);
   if (threadIdx.x == 0) {
-    d.partial[t * NPART + 1] = rz;
+    if (!FACTOR) d.partial[t * NPART + 1] = rz;
     d.partial[t * NPART + 2] = rr;
   }
 }
@@ -824,9 +852,16 @@ void Launch<Real>::balance(const Dev<Real> &d, bool diag, bool taken, hipStream_
   else TMI_LAUNCH((k_balance<Real, false, false>), d.n_ntiles, s, d);
 }
 template <class Real>
-void Launch<Real>::pcg_init(const Dev<Real> &d, bool with_x0, hipStream_t s) {
-  if (with_x0) TMI_LAUNCH((k_pcg_init<Real, true>), d.n_ntiles, s, d);
-  else TMI_LAUNCH((k_pcg_init<Real, false>), d.n_ntiles, s, d);
+void Launch<Real>::pcg_init(const Dev<Real> &d, bool with_x0, bool factor, hipStream_t s) {
+  if (with_x0 && factor) TMI_LAUNCH((k_pcg_init<Real, true, true>), d.n_ntiles, s, d);
+  else if (with_x0) TMI_LAUNCH((k_pcg_init<Real, true, false>), d.n_ntiles, s, d);
+  else if (factor) TMI_LAUNCH((k_pcg_init<Real, false, true>), d.n_ntiles, s, d);
+  else TMI_LAUNCH((k_pcg_init<Real, false, false>), d.n_ntiles, s, d);
+}
+template <class Real>
+void Launch<Real>::pcg_rz(const Dev<Real> &d, bool start, int parity, hipStream_t s) {
+  if (start) TMI_LAUNCH((k_pcg_rz<Real, true>), d.n_ntiles, s, d, parity);
+  else TMI_LAUNCH((k_pcg_rz<Real, false>), d.n_ntiles, s, d, parity);
 }
 template <class Real>
 void Launch<Real>::pcg_begin(const Dev<Real> &d, hipStream_t s) {
@@ -842,8 +877,9 @@ void Launch<Real>::op_gather(const Dev<Real> &d, const Real *dir, Real *out, int
   TMI_LAUNCH(k_op_gather<Real>, d.n_ntiles, s, d, dir, out, parity);
 }
 template <class Real>
-void Launch<Real>::pcg_update(const Dev<Real> &d, int parity, hipStream_t s) {
-  TMI_LAUNCH(k_pcg_update<Real>, d.n_ntiles, s, d, parity);
+void Launch<Real>::pcg_update(const Dev<Real> &d, int parity, bool factor, hipStream_t s) {
+  if (factor) TMI_LAUNCH((k_pcg_update<Real, true>), d.n_ntiles, s, d, parity);
+  else TMI_LAUNCH((k_pcg_update<Real, false>), d.n_ntiles, s, d, parity);
 }
 template <class Real>
 void Launch<Real>::pcg_direction(const Dev<Real> &d, int parity, hipStream_t s) {

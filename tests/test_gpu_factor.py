@@ -1,0 +1,134 @@
+"""GPU block Cholesky preconditioner (SystemFactor replacement) against a
+dense numpy solve of the system the oracle assembles column by column
+(textbook Cholesky pin: residual of the solve), and FACTOR-preconditioned
+steps against the oracle's block-Jacobi steps at tight tolerance (the
+preconditioner changes the iterates, never the converged solution)."""
+import math
+
+import numpy as np
+import pytest
+
+from talus_amd import api, scenes
+
+from helpers import deformed, oracle_body, product_body
+
+pytestmark = pytest.mark.gpu
+
+
+def dense_system(orc, m, inertia, step):
+    n = m["rest"].shape[0]
+    free = np.repeat(m["anchored"] == 0, 3)
+    A = np.zeros((3 * n, 3 * n))
+    for j in np.flatnonzero(free):
+        e = np.zeros(3 * n)
+        e[j] = 1.0
+        A[:, j] = orc.apply_system(inertia, step, e).reshape(-1)
+    return A[np.ix_(free, free)], free
+
+
+def _check_solve(body, x=None, bar=1e-10, precision=api.FP64):
+    orc, m = oracle_body(body)
+    if x is not None:
+        orc.eval_forces(x)  # commits rotations
+    R = orc.rotations()
+    inertia, step = 1.0, math.sqrt(8.0 / 27.0) / 60.0
+    A, free = dense_system(orc, m, inertia, step)
+    rng = np.random.default_rng(21)
+    r = rng.standard_normal(m["rest"].shape)
+    r[m["anchored"] != 0] = 0.0
+    z_ref = np.zeros(r.size)
+    z_ref[free] = np.linalg.solve(A, r.reshape(-1)[free])
+    with api.Context(0, precision) as ctx:
+        b, _ = product_body(ctx, body)
+        ctx.set_rotations(b, R)
+        z = ctx.factor_solve(b, inertia, step, r).reshape(-1)
+        info = ctx.factor_info()
+    assert info["ok"] == 1 and info["fronts"] > 0
+    err = np.abs(z - z_ref).max() / np.abs(z_ref).max()
+    assert err <= bar, err
+    assert np.all(z[~free] == 0.0)
+    return info
+
+
+def test_factor_solve_cantilever_rest():
+    info = _check_solve(scenes.cantilever().bodies[0])
+    assert info["levels"] >= 1
+
+
+def test_factor_solve_rotated_state():
+    body = scenes.cantilever().bodies[0]
+    _, m = oracle_body(body)
+    _check_solve(body, x=deformed(m["rest"], seed=3, scale=0.002, twist=0.8))
+
+
+def test_factor_solve_jittered_cube_with_many_fronts():
+    body = scenes.cube(9, seed=6).bodies[0]
+    info = _check_solve(body)
+    assert info["fronts"] > 8 and info["levels"] >= 3
+
+
+def test_factor_solve_mixed_storage_is_a_good_preconditioner():
+    _check_solve(scenes.cube(6, seed=8).bodies[0], bar=1e-5, precision=api.MIXED)
+
+
+def test_factor_solve_batched_equals_single():
+    sc = scenes.bodies(count=3, seed=12, lo=300, hi=3000)
+    r_list, z_single = [], []
+    for body in sc.bodies:
+        with api.Context(0, api.FP64) as ctx:
+            b, mesh = product_body(ctx, body)
+            rng = np.random.default_rng(len(r_list))
+            r = rng.standard_normal(mesh["rest_positions"].shape)
+            z_single.append(ctx.factor_solve(b, 1.0, 0.005, r))
+            r_list.append(r)
+    with api.Context(0, api.FP64) as ctx:
+        ids = [product_body(ctx, body)[0] for body in sc.bodies]
+        for i, b in enumerate(ids):
+            z = ctx.factor_solve(b, 1.0, 0.005, r_list[i])
+            assert np.array_equal(z, z_single[i])
+
+
+def test_factor_preconditioned_steps_match_oracle():
+    body = scenes.cantilever().bodies[0]
+    orc, m = oracle_body(body)
+    steps = 20
+    for _ in range(steps):
+        orc.step(tolerance=1e-10, cg_iterations=4000)
+    xo, vo, _ = orc.state()
+    with api.Context(0, api.FP64) as ctx:
+        b, _ = product_body(ctx, body)
+        reps = [ctx.step(tolerance=1e-10, cg_iterations=200, factor_budget=200,
+                         preconditioner=api.FACTOR) for _ in range(steps)]
+        x, v, _ = ctx.state(b)
+    disp = np.abs(xo - m["rest"]).max()
+    assert np.abs(x - xo).max() <= 1e-8 * disp
+    assert sum(r["cg_capped"] for r in reps) == 0
+    # the factor of the exact first-step system converges almost at once
+    assert reps[0]["cg_iterations"] <= 3 * 12
+
+
+def test_factor_converges_where_block_jacobi_caps():
+    body = scenes.cube(12, seed=13).bodies[0]
+    with api.Context(0, api.MIXED) as ctx:
+        product_body(ctx, body)
+        bj = [ctx.step() for _ in range(3)]
+    with api.Context(0, api.MIXED) as ctx:
+        product_body(ctx, body)
+        fac = [ctx.step(preconditioner=api.FACTOR) for _ in range(3)]
+        info = ctx.factor_info()
+    assert sum(r["cg_capped"] for r in bj) > 0
+    assert sum(r["cg_capped"] for r in fac) == 0
+    assert all(r["cg_worst_relative"] <= 1e-3 for r in fac)
+    assert info["ok"] == 1 and info["factor_seconds"] > 0
+
+
+def test_factor_steps_are_bit_identical():
+    body = scenes.cube(8, seed=14).bodies[0]
+    outs = []
+    for _ in range(2):
+        with api.Context(0, api.MIXED) as ctx:
+            b, _ = product_body(ctx, body)
+            for _ in range(3):
+                ctx.step(preconditioner=api.FACTOR)
+            outs.append(ctx.state(b)[0])
+    assert np.array_equal(outs[0], outs[1])
