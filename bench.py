@@ -72,6 +72,7 @@ def main(argv=None):
     ap.add_argument("--skip-fp64", action="store_true")
     ap.add_argument("--skip-cpu", action="store_true")
     ap.add_argument("--cpu-seconds", type=float, default=15.0)
+    ap.add_argument("--preconditioner", choices=["factor", "jacobi"], default="factor")
     args = ap.parse_args(argv)
     if args.gpus != 1:
         print(f"bench.py measures one GPU; --gpus {args.gpus} is not supported "
@@ -85,7 +86,8 @@ def main(argv=None):
     mesh, mats = scenes.build_mesh(body)
     T = int(mesh["tet_nodes"].shape[0])
     N = int(mesh["anchored"].shape[0])
-    params = api.default_params(substeps=scene.substeps)
+    pre = api.FACTOR if args.preconditioner == "factor" else api.BLOCK_JACOBI
+    params = api.default_params(substeps=scene.substeps, preconditioner=pre)
     log(f"[bench] workload cube{args.cells}: {T} tets {N} nodes")
 
     log("[bench] mixed precision")
@@ -94,6 +96,10 @@ def main(argv=None):
     cg_m = sum(r["cg_iterations"] for r in reps_m) / args.steps
     capped_m = sum(r["cg_capped"] for r in reps_m)
     log(f"[bench] mixed {1e3 * wall_m / args.steps:.3f} ms/step, {value:.4g} tet/s, cg/step {cg_m:.1f}")
+    finfo = ctx_m.factor_info()
+    if finfo["fronts"]:
+        finfo["tflops"] = finfo["flops"] / max(finfo["factor_seconds"], 1e-12) / 1e12
+        log(f"[bench] factor {json.dumps(finfo)}")
 
     fp64 = None
     ctx_d = None
@@ -174,10 +180,13 @@ def main(argv=None):
         "config": {
             "workload": f"cube {args.cells}^3 hexes -> {T} tets, {N} nodes, jittered 2%, per-tet "
                         f"lognormal E, bottom pinned, dt 1/60, {scene.substeps} substep, tol 1e-3, "
-                        f"cg<=64, block-Jacobi PCG",
+                        f"cg<=64 (<=32 with the factor), {args.preconditioner}-preconditioned PCG",
             "tets": T, "nodes": N, "precision": "mixed",
             "cg_per_step": cg_m, "cg_capped": capped_m,
+            "preconditioner": args.preconditioner,
+            "refactors": sum(r["refactors"] for r in reps_m),
         },
+        "factor": finfo if finfo["fronts"] else None,
         "fp64": fp64,
         "mixed_over_fp64": (value / fp64["value"]) if fp64 else None,
         "rel_force_error": rel_err,

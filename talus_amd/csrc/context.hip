@@ -207,7 +207,9 @@ class Engine final : public EngineBase {
            *d_contrib_ = nullptr, *d_pair_body_ = nullptr, *d_parent_ = nullptr,
            *d_lvl_fronts_ = nullptr, *d_potrf_ = nullptr, *d_node_body_ = nullptr;
   uint64_t *d_ea_ = nullptr, *d_trsm_ = nullptr, *d_syrk_ = nullptr, *d_trtri_ = nullptr,
-           *d_m11r_ = nullptr, *d_l21r_ = nullptr, *d_l21c_ = nullptr, *d_m11c_ = nullptr;
+           *d_w21_ = nullptr, *d_pack_ = nullptr, *d_fwd_ = nullptr, *d_bwd_ = nullptr,
+           *d_gat_ = nullptr, *d_xg_ = nullptr;
+  Real *d_gr_ = nullptr, *d_gc_ = nullptr;
   double *d_body_reach_ = nullptr, *d_body_inertia_ = nullptr;
   int *hfailed_ = nullptr;
   uint32_t refactors_ = 0;
@@ -425,6 +427,9 @@ void Engine<Real>::analyze_factor() {
   fd_.minv_off = upload(t.minv_off);
   fd_.vec_off = upload(t.vec_off);
   fd_.yoff = upload(t.yoff);
+  fd_.g_off = upload(t.g_off);
+  fd_.src_start = upload(t.src_start);
+  fd_.src = upload(t.src);
   fd_.pool = dev<double>(t.pool);
   fd_.minv = dev<double>(t.minv_pool);
   fd_.vec = dev<double>(t.vec_pool);
@@ -444,10 +449,14 @@ void Engine<Real>::analyze_factor() {
   d_trsm_ = upload(t.trsm_tasks);
   d_syrk_ = upload(t.syrk_tasks);
   d_trtri_ = upload(t.trtri_tasks);
-  d_m11r_ = upload(t.m11_rows);
-  d_l21r_ = upload(t.l21_rows);
-  d_l21c_ = upload(t.l21_cols);
-  d_m11c_ = upload(t.m11_cols);
+  d_w21_ = upload(t.w21_tasks);
+  d_pack_ = upload(t.pack_tasks);
+  d_fwd_ = upload(t.fwd_tasks);
+  d_bwd_ = upload(t.bwd_tasks);
+  d_gat_ = upload(t.gat_tasks);
+  d_xg_ = upload(t.xg_tasks);
+  d_gr_ = dev<Real>(t.g_pool);
+  d_gc_ = dev<Real>(t.g_pool);
   d_body_reach_ = dev<double>(d_.B);
   d_body_inertia_ = dev<double>(d_.B);
   sym_ready_ = true;
@@ -485,6 +494,8 @@ void Engine<Real>::factorize(const std::vector<double> &reach, const std::vector
   }
   for (size_t i = 0; i + 1 < t.trtri_start.size(); ++i)
     factor_trtri(fd_, d_trtri_ + t.trtri_start[i], t.trtri_start[i + 1] - t.trtri_start[i], s);
+  SolveLaunch<Real>::pack(fd_, d_w21_, uint32_t(t.w21_tasks.size()), d_pack_,
+                          uint32_t(t.pack_tasks.size()), d_gr_, d_gc_, s);
   HIPCHECK(hipMemcpyAsync(hfailed_, fd_.failed, d_.B * sizeof(int), hipMemcpyDeviceToHost, s));
   HIPCHECK(hipEventRecord(b, s));
   HIPCHECK(hipStreamSynchronize(s));
@@ -502,24 +513,25 @@ void Engine<Real>::factorize(const std::vector<double> &reach, const std::vector
 }
 
 // z = (LL^T)^-1 r for the bodies still iterating (applyFactor,
-// factor.cc:648-704, without the body-frame rotation).
+// factor.cc:648-704, without the body-frame rotation): per level from the
+// leaves, gather the right-hand side and the children's updates, then one
+// GEMV with G; per level from the root, gather the ancestors' solution and
+// one transposed GEMV with G.
 template <class Real>
 void Engine<Real>::precondition(int parity) {
   hipStream_t s = stream_;
   const FactorTables &t = tab_;
   const uint32_t B = d_.B;
   for (uint32_t l = 0; l < t.levels; ++l) {
-    SolveLaunch<Real>::gather(fd_, d_lvl_fronts_ + t.lvl_fronts_start[l],
-                              t.lvl_fronts_start[l + 1] - t.lvl_fronts_start[l], d_.r, d_.done, parity, B, s);
-    SolveLaunch<Real>::m11_rows(fd_, d_m11r_ + t.m11r_start[l], t.m11r_start[l + 1] - t.m11r_start[l],
-                                d_.done, parity, B, s);
-    SolveLaunch<Real>::l21_rows(fd_, d_l21r_ + t.l21r_start[l], t.l21r_start[l + 1] - t.l21r_start[l],
-                                d_.done, parity, B, s);
+    SolveLaunch<Real>::gather(fd_, d_gat_ + t.gat_start[l], t.gat_start[l + 1] - t.gat_start[l], d_.r,
+                              d_.done, parity, B, s);
+    SolveLaunch<Real>::forward(fd_, d_gr_, d_fwd_ + t.fwd_start[l], t.fwd_start[l + 1] - t.fwd_start[l],
+                               d_.done, parity, B, s);
   }
   for (uint32_t l = t.levels; l-- > 0;) {
-    SolveLaunch<Real>::l21_cols(fd_, d_l21c_ + t.l21c_start[l], t.l21c_start[l + 1] - t.l21c_start[l],
-                                d_.done, parity, B, s);
-    SolveLaunch<Real>::m11_cols(fd_, d_m11c_ + t.m11c_start[l], t.m11c_start[l + 1] - t.m11c_start[l],
+    SolveLaunch<Real>::xgather(fd_, d_xg_ + t.xg_start[l], t.xg_start[l + 1] - t.xg_start[l], d_.done,
+                               parity, B, s);
+    SolveLaunch<Real>::backward(fd_, d_gc_, d_bwd_ + t.bwd_start[l], t.bwd_start[l + 1] - t.bwd_start[l],
                                 d_.done, parity, B, s);
   }
   SolveLaunch<Real>::finish(fd_, d_.z, d_.N, d_.anchored, d_node_body_, d_.done, parity, B, s);

@@ -16,8 +16,11 @@
 //              Cholesky of each front (POTRF of 64x64 diagonal blocks, TRSM
 //              and SYRK/GEMM updates on FP64 MFMA tiles)
 //   solve      the diagonal block of each front is inverted explicitly
-//              (M11 = L11^-1) so forward and backward substitution are
-//              parallel GEMV passes per level
+//              (M11 = L11^-1) and packed with W21 = L21 M11 into
+//              G = [M11; W21] (m x k, element precision, stored row-major
+//              and column-major): the forward pass of a front is one GEMV
+//              [y; w] = G v1, u = v2 - w, and the backward pass one
+//              transposed GEMV x1 = G^T [y; -x2]
 // All sums run in a fixed order: the factor and every solve are
 // bit-reproducible.
 #pragma once
@@ -32,7 +35,9 @@
 namespace tmi {
 
 constexpr int FNB = 64;   // panel / tile width in DOF
-constexpr int SROW = 32;  // rows per solve tile
+constexpr int SGRP = 4;   // solve workgroup: 4 wavefronts, 1 or 4 rows/columns each
+constexpr int SCH = 2048; // vector entries staged in LDS per chunk
+constexpr int SCHUNK = 256;  // DOF rows per gather workgroup
 
 struct FrontSym {
   uint32_t ncol = 0;   // nodes eliminated by this front
@@ -91,9 +96,17 @@ struct FactorTables {
   std::vector<uint32_t> trtri_start;                         // per row block i
   std::vector<uint64_t> trtri_tasks;                         // front | i<<32 | j<<48
   std::vector<uint32_t> srow_start, scol_start;              // per level
-  std::vector<uint64_t> m11_rows, l21_rows, l21_cols, m11_cols;  // front | tile<<32
-  std::vector<uint32_t> m11r_start, l21r_start, l21c_start, m11c_start;
-  uint64_t pool = 0, minv_pool = 0, vec_pool = 0, y_pool = 0;
+  std::vector<uint64_t> g_off;                               // per front, m*k
+  std::vector<uint64_t> w21_tasks, pack_tasks;               // front | I<<32 | J<<48
+  std::vector<uint64_t> fwd_tasks, bwd_tasks;                // front | group<<32 | size<<48
+  std::vector<uint32_t> fwd_start, bwd_start;
+  std::vector<uint64_t> gat_tasks, xg_tasks;                 // front | chunk<<32, per level
+  std::vector<uint32_t> gat_start, xg_start;
+  // pull lists: for every front node row (rows_off indexing), the vec-pool
+  // offsets of the child update entries (DOF 0) that add into it
+  std::vector<uint32_t> src_start;
+  std::vector<uint64_t> src;
+  uint64_t pool = 0, minv_pool = 0, vec_pool = 0, y_pool = 0, g_pool = 0;
   double flops = 0.0;
   uint64_t factor_entries = 0;
 };
@@ -107,7 +120,9 @@ void build_tables(const std::vector<Symbolic> &sym, const std::vector<Body> &bod
 struct FactorDev {
   uint32_t fronts = 0;
   const uint32_t *k, *m, *body, *rows_off, *map_off, *child_start, *children, *rows, *map;
-  const uint64_t *off, *minv_off, *vec_off, *yoff;
+  const uint32_t *src_start;
+  const uint64_t *src;
+  const uint64_t *off, *minv_off, *vec_off, *yoff, *g_off;
   double *pool, *minv, *vec, *y, *xsol;
   int *failed;  // per body
 };
@@ -137,17 +152,18 @@ void factor_trtri(const FactorDev &f, const uint64_t *tasks, uint32_t count, hip
 
 template <class Real>
 struct SolveLaunch {
-  // z = LL^-T... applied to r for every body whose convergence flag
+  // G = [M11; L21 M11] in row-major (gr) and column-major (gc) copies
+  static void pack(const FactorDev &f, const uint64_t *w21, uint32_t nw21, const uint64_t *m11,
+                   uint32_t nm11, Real *gr, Real *gc, hipStream_t s);
+  // z = (L L^T)^-1 r for every body whose convergence flag
   // (done[parity*B + body]) is clear; parity < 0 applies to every body.
-  static void gather(const FactorDev &f, const uint32_t *fronts, uint32_t count, const Real *r,
+  static void gather(const FactorDev &f, const uint64_t *tasks, uint32_t count, const Real *r,
                      const int *done, int parity, uint32_t B, hipStream_t s);
-  static void m11_rows(const FactorDev &f, const uint64_t *tasks, uint32_t count,
-                       const int *done, int parity, uint32_t B, hipStream_t s);
-  static void l21_rows(const FactorDev &f, const uint64_t *tasks, uint32_t count,
-                       const int *done, int parity, uint32_t B, hipStream_t s);
-  static void l21_cols(const FactorDev &f, const uint64_t *tasks, uint32_t count,
-                       const int *done, int parity, uint32_t B, hipStream_t s);
-  static void m11_cols(const FactorDev &f, const uint64_t *tasks, uint32_t count,
+  static void forward(const FactorDev &f, const Real *gr, const uint64_t *tasks, uint32_t count,
+                      const int *done, int parity, uint32_t B, hipStream_t s);
+  static void xgather(const FactorDev &f, const uint64_t *tasks, uint32_t count,
+                      const int *done, int parity, uint32_t B, hipStream_t s);
+  static void backward(const FactorDev &f, const Real *gc, const uint64_t *tasks, uint32_t count,
                        const int *done, int parity, uint32_t B, hipStream_t s);
   static void finish(const FactorDev &f, Real *z, uint32_t N, const uint8_t *anchored,
                      const uint32_t *node_body, const int *done, int parity, uint32_t B,

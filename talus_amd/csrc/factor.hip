@@ -104,9 +104,10 @@ __global__ __launch_bounds__(NT) void k_extend_add(FactorDev f, const uint32_t *
 
 // -------------------------------------------------------------------- POTRF
 
-// Cholesky of the 64x64 diagonal block of panel p in LDS; writes L_pp to
-// the front and L_pp^-1 to the (p,p) block of M11.
-__global__ __launch_bounds__(NT) void k_potrf(FactorDev f, const uint32_t *tasks, uint32_t panel) {
+// Cholesky of the 64x64 diagonal block of panel p in LDS by one wavefront
+// (lane i owns row i); writes L_pp to the front and L_pp^-1 to the (p,p)
+// block of M11.
+__global__ __launch_bounds__(64) void k_potrf(FactorDev f, const uint32_t *tasks, uint32_t panel) {
   __shared__ double a[FNB][LDP];
   __shared__ double x[FNB][LDP];
   const uint32_t fr = tasks[blockIdx.x];
@@ -114,71 +115,92 @@ __global__ __launch_bounds__(NT) void k_potrf(FactorDev f, const uint32_t *tasks
   const uint32_t P0 = panel * FNB;
   const int nb = int(min(uint32_t(FNB), k - P0));
   double *A = f.pool + f.off[fr];
-  for (int idx = threadIdx.x; idx < FNB * FNB; idx += NT) {
-    const int i = idx % FNB, j = idx / FNB;
-    a[i][j] = (i < nb && j < nb) ? A[uint64_t(P0 + j) * m + P0 + i] : 0.0;
-  }
+  const int i = threadIdx.x;
+  for (int j = 0; j < FNB; ++j) a[i][j] = (i < nb && j < nb) ? A[uint64_t(P0 + j) * m + P0 + i] : 0.0;
   __syncthreads();
+  bool bad = false;
   for (int j = 0; j < nb; ++j) {
-    if (threadIdx.x == 0) {
-      double d = a[j][j];
-      if (!(d > 0.0)) {
-        f.failed[f.body[fr]] = 1;
-        d = 1.0;
-      }
-      a[j][j] = sqrt(d);
+    double d = a[j][j];
+    if (!(d > 0.0)) {
+      bad = true;
+      d = 1.0;
     }
+    const double piv = sqrt(d);
     __syncthreads();
-    const double piv = a[j][j];
-    for (int i = j + 1 + threadIdx.x; i < nb; i += NT) a[i][j] /= piv;
+    if (i == j) a[j][j] = piv;
+    if (i > j && i < nb) a[i][j] /= piv;
     __syncthreads();
-    const int w = nb - j - 1;
-    for (int idx = threadIdx.x; idx < w * w; idx += NT) {
-      const int i = j + 1 + idx / w, l = j + 1 + idx % w;
-      if (l <= i) a[i][l] -= a[i][j] * a[l][j];
+    if (i > j && i < nb) {
+      const double aij = a[i][j];
+      for (int l = j + 1; l <= i; ++l) a[i][l] -= aij * a[l][j];
     }
     __syncthreads();
   }
-  // inverse of the lower factor, one column per thread
-  if (threadIdx.x < nb) {
-    const int c = threadIdx.x;
-    for (int i = 0; i < c; ++i) x[i][c] = 0.0;
-    x[c][c] = 1.0 / a[c][c];
-    for (int i = c + 1; i < nb; ++i) {
+  if (bad && i == 0) f.failed[f.body[fr]] = 1;
+  // inverse of the lower factor, one column per lane
+  if (i < nb) {
+    for (int r = 0; r < i; ++r) x[r][i] = 0.0;
+    x[i][i] = 1.0 / a[i][i];
+    for (int r = i + 1; r < nb; ++r) {
       double s = 0.0;
-      for (int q = c; q < i; ++q) s += a[i][q] * x[q][c];
-      x[i][c] = -s / a[i][i];
+      for (int q = i; q < r; ++q) s += a[r][q] * x[q][i];
+      x[r][i] = -s / a[r][r];
     }
   }
   __syncthreads();
   double *M = f.minv + f.minv_off[fr];
-  for (int idx = threadIdx.x; idx < nb * nb; idx += NT) {
-    const int i = idx % nb, j = idx / nb;
-    if (i >= j) {
+  for (int j = 0; j < nb; ++j)
+    if (i >= j && i < nb) {
       A[uint64_t(P0 + j) * m + P0 + i] = a[i][j];
       M[uint64_t(P0 + j) * k + P0 + i] = x[i][j];
     }
-  }
 }
 
 // --------------------------------------------------------- 64x64 tile GEMM
 
-// acc[4][4] += sum_k As[k][4tr+i] * Bs[k][4tc+j] over kk < kn.
-__device__ __forceinline__ void tile_fma(const double (*As)[LDP], const double (*Bs)[LDP], int kn,
-                                         double acc[4][4]) {
-  const int tr = threadIdx.x / 16, tc = threadIdx.x % 16;
-  for (int kk = 0; kk < kn; ++kk) {
-    double av[4], bv[4];
+// FP64 MFMA tile: C[r][c] += sum_{kk<64} As[kk][r] Bs[kk][c] for the 64x64
+// tile, LDS tiles zero-padded to 64.  Four waves, each a 32x32 quadrant of
+// 2x2 v_mfma_f64_16x16x4f64 blocks.  Operand layout of that instruction:
+// lane l holds A[l%16][l/16] and B[l/16][l%16]; result item v of lane l is
+// D[4v + l/16][l%16].
+struct MfmaAcc {
+  d4 c[2][2];
+};
+
+__device__ __forceinline__ void mfma_zero(MfmaAcc &acc) {
 #pragma unroll
-    for (int i = 0; i < 4; ++i) {
-      av[i] = As[kk][4 * tr + i];
-      bv[i] = Bs[kk][4 * tc + i];
-    }
+  for (int a = 0; a < 2; ++a)
 #pragma unroll
-    for (int i = 0; i < 4; ++i)
-#pragma unroll
-      for (int j = 0; j < 4; ++j) acc[i][j] = fma(av[i], bv[j], acc[i][j]);
+    for (int b = 0; b < 2; ++b) acc.c[a][b] = d4{0.0, 0.0, 0.0, 0.0};
+}
+
+__device__ __forceinline__ void mfma_tile(const double (*As)[LDP], const double (*Bs)[LDP],
+                                          MfmaAcc &acc) {
+  const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+  const int qr = 32 * (wave >> 1), qc = 32 * (wave & 1);
+  const int li = lane & 15, lk = lane >> 4;
+#pragma unroll 4
+  for (int k0 = 0; k0 < FNB; k0 += 4) {
+    const double a0 = As[k0 + lk][qr + li], a1 = As[k0 + lk][qr + 16 + li];
+    const double b0 = Bs[k0 + lk][qc + li], b1 = Bs[k0 + lk][qc + 16 + li];
+    acc.c[0][0] = __builtin_amdgcn_mfma_f64_16x16x4f64(a0, b0, acc.c[0][0], 0, 0, 0);
+    acc.c[0][1] = __builtin_amdgcn_mfma_f64_16x16x4f64(a0, b1, acc.c[0][1], 0, 0, 0);
+    acc.c[1][0] = __builtin_amdgcn_mfma_f64_16x16x4f64(a1, b0, acc.c[1][0], 0, 0, 0);
+    acc.c[1][1] = __builtin_amdgcn_mfma_f64_16x16x4f64(a1, b1, acc.c[1][1], 0, 0, 0);
   }
+}
+
+// writes the accumulated tile to Cs[r][c] (LDS); caller synchronizes
+__device__ __forceinline__ void mfma_store(const MfmaAcc &acc, double (*Cs)[LDP]) {
+  const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+  const int qr = 32 * (wave >> 1), qc = 32 * (wave & 1);
+#pragma unroll
+  for (int a = 0; a < 2; ++a)
+#pragma unroll
+    for (int b = 0; b < 2; ++b)
+#pragma unroll
+      for (int v = 0; v < 4; ++v)
+        Cs[qr + 16 * a + 4 * v + (lane >> 4)][qc + 16 * b + (lane & 15)] = acc.c[a][b][v];
 }
 
 // TRSM: L_{rt,p} = A_{rt,p} L_pp^-T for the rows below the panel (all m
@@ -202,16 +224,16 @@ __global__ __launch_bounds__(NT) void k_trsm(FactorDev f, const uint64_t *tasks,
     Xs[q][r] = (r < nb && q < nb && r >= q) ? X[uint64_t(P0 + q) * k + P0 + r] : 0.0;
   }
   __syncthreads();
-  double acc[4][4] = {};
-  tile_fma(As, Xs, nb, acc);
-  const int tr = threadIdx.x / 16, tc = threadIdx.x % 16;
-#pragma unroll
-  for (int i = 0; i < 4; ++i)
-#pragma unroll
-    for (int j = 0; j < 4; ++j) {
-      const int r = 4 * tr + i, c = 4 * tc + j;
-      if (r < nr && c < nb) A[uint64_t(P0 + c) * m + R0 + r] = acc[i][j];
-    }
+  MfmaAcc acc;
+  mfma_zero(acc);
+  mfma_tile(As, Xs, acc);
+  __syncthreads();
+  mfma_store(acc, As);  // As reused as C[r][c]
+  __syncthreads();
+  for (int idx = threadIdx.x; idx < FNB * FNB; idx += NT) {
+    const int r = idx % FNB, c = idx / FNB;
+    if (r < nr && c < nb) A[uint64_t(P0 + c) * m + R0 + r] = As[r][c];
+  }
 }
 
 // SYRK/GEMM trailing update A_IJ -= L_Ip L_Jp^T over the rows/columns after
@@ -234,16 +256,16 @@ __global__ __launch_bounds__(NT) void k_syrk(FactorDev f, const uint64_t *tasks,
     Js[q][r] = (inq && r < nj) ? A[uint64_t(P0 + q) * m + RJ + r] : 0.0;
   }
   __syncthreads();
-  double acc[4][4] = {};
-  tile_fma(Is, Js, nb, acc);
-  const int tr = threadIdx.x / 16, tc = threadIdx.x % 16;
-#pragma unroll
-  for (int i = 0; i < 4; ++i)
-#pragma unroll
-    for (int j = 0; j < 4; ++j) {
-      const int r = 4 * tr + i, c = 4 * tc + j;
-      if (r < ni && c < nj) A[uint64_t(RJ + c) * m + RI + r] -= acc[i][j];
-    }
+  MfmaAcc acc;
+  mfma_zero(acc);
+  mfma_tile(Is, Js, acc);
+  __syncthreads();
+  mfma_store(acc, Is);  // Is reused as C[r][c]
+  __syncthreads();
+  for (int idx = threadIdx.x; idx < FNB * FNB; idx += NT) {
+    const int r = idx % FNB, c = idx / FNB;
+    if (r < ni && c < nj) A[uint64_t(RJ + c) * m + RI + r] -= Is[r][c];
+  }
 }
 
 // M11 = L11^-1 by forward substitution on column blocks: for row block i
@@ -257,7 +279,8 @@ __global__ __launch_bounds__(NT) void k_trtri(FactorDev f, const uint64_t *tasks
   const double *L = f.pool + f.off[fr];
   double *X = f.minv + f.minv_off[fr];
   const int ni = int(min(uint32_t(FNB), k - i * FNB)), nj = int(min(uint32_t(FNB), k - j * FNB));
-  double acc[4][4] = {};
+  MfmaAcc acc;
+  mfma_zero(acc);
   for (uint32_t q = j; q < i; ++q) {
     const int nq = int(min(uint32_t(FNB), k - q * FNB));
     __syncthreads();
@@ -269,15 +292,11 @@ __global__ __launch_bounds__(NT) void k_trtri(FactorDev f, const uint64_t *tasks
       Bs[r][c] = (r < nq && c < nj && xr >= xc) ? X[uint64_t(xc) * k + xr] : 0.0;
     }
     __syncthreads();
-    tile_fma(As, Bs, FNB, acc);
+    mfma_tile(As, Bs, acc);
   }
-  // X_ij = -Linv_ii * acc
+  // X_ij = -Linv_ii * S with S the accumulated sum
   __syncthreads();
-  const int tr = threadIdx.x / 16, tc = threadIdx.x % 16;
-#pragma unroll
-  for (int a = 0; a < 4; ++a)
-#pragma unroll
-    for (int b = 0; b < 4; ++b) Bs[4 * tr + a][4 * tc + b] = acc[a][b];  // Bs[r][c] = S[r][c]
+  mfma_store(acc, Bs);  // Bs[r][c] = S[r][c]
   for (int idx = threadIdx.x; idx < FNB * FNB; idx += NT) {
     const int r = idx % FNB, c = idx / FNB;
     // As[c][r] = Linv_ii[r][c]
@@ -300,73 +319,22 @@ __device__ __forceinline__ bool skip(const int *done, int parity, uint32_t B, ui
   return parity >= 0 && done[parity * B + body];
 }
 
+// forward gather: v[i] = r at the front's columns + the children's update
+// entries that map to row i, pulled in child order
 template <class Real>
-__global__ __launch_bounds__(NT) void k_solve_gather(FactorDev f, const uint32_t *fronts,
+__global__ __launch_bounds__(NT) void k_solve_gather(FactorDev f, const uint64_t *tasks,
                                                      const Real *rin, const int *done, int parity,
                                                      uint32_t B) {
-  const uint32_t fr = fronts[blockIdx.x];
+  const uint64_t t = tasks[blockIdx.x];
+  const uint32_t fr = lo32(t), chunk = mid16(t);
   if (skip(done, parity, B, f.body[fr])) return;
   const uint32_t k = f.k[fr], m = f.m[fr];
-  double *v = f.vec + f.vec_off[fr];
-  const uint32_t *rows = f.rows + f.rows_off[fr];
-  for (uint32_t i = threadIdx.x; i < m; i += NT)
-    v[i] = i < k ? double(rin[3 * size_t(rows[i / 3]) + i % 3]) : 0.0;
-  for (uint32_t ci = f.child_start[fr]; ci < f.child_start[fr + 1]; ++ci) {
-    __syncthreads();
-    const uint32_t c = f.children[ci];
-    const uint32_t kc = f.k[c], u = f.m[c] - kc;
-    const double *vc = f.vec + f.vec_off[c] + kc;
-    const uint32_t *map = f.map + f.map_off[c];
-    for (uint32_t q = threadIdx.x; q < u; q += NT) v[3 * map[q / 3] + q % 3] += vc[q];
-  }
-}
-
-// y[rows of tile] = sum_{j<=i} M11[i][j] v[j]; 32 rows x 8 column groups
-__global__ __launch_bounds__(NT) void k_solve_m11_rows(FactorDev f, const uint64_t *tasks,
-                                                       const int *done, int parity, uint32_t B) {
-  __shared__ double part[8][SROW];
-  const uint64_t t = tasks[blockIdx.x];
-  const uint32_t fr = lo32(t), tile = mid16(t);
-  if (skip(done, parity, B, f.body[fr])) return;
-  const uint32_t k = f.k[fr];
-  const double *M = f.minv + f.minv_off[fr];
-  const double *v = f.vec + f.vec_off[fr];
-  const uint32_t ri = threadIdx.x % SROW, cg = threadIdx.x / SROW;
-  const uint32_t i = tile * SROW + ri;
-  double s = 0.0;
-  if (i < k)
-    for (uint32_t j = cg; j <= i; j += 8) s += M[uint64_t(j) * k + i] * v[j];
-  part[cg][ri] = s;
-  __syncthreads();
-  if (threadIdx.x < SROW && i < k) {
-    double y = 0.0;
-    for (int g = 0; g < 8; ++g) y += part[g][ri];
-    f.y[f.yoff[fr] + i] = y;
-  }
-}
-
-// v[k+i] -= sum_j L21[i][j] y[j]
-__global__ __launch_bounds__(NT) void k_solve_l21_rows(FactorDev f, const uint64_t *tasks,
-                                                       const int *done, int parity, uint32_t B) {
-  __shared__ double part[8][SROW];
-  const uint64_t t = tasks[blockIdx.x];
-  const uint32_t fr = lo32(t), tile = mid16(t);
-  if (skip(done, parity, B, f.body[fr])) return;
-  const uint32_t k = f.k[fr], m = f.m[fr];
-  const double *L = f.pool + f.off[fr];
-  const double *y = f.y + f.yoff[fr];
-  const uint32_t ri = threadIdx.x % SROW, cg = threadIdx.x / SROW;
-  const uint32_t i = tile * SROW + ri;
-  double s = 0.0;
-  if (k + i < m)
-    for (uint32_t j = cg; j < k; j += 8) s += L[uint64_t(j) * m + k + i] * y[j];
-  part[cg][ri] = s;
-  __syncthreads();
-  if (threadIdx.x < SROW && k + i < m) {
-    double acc = 0.0;
-    for (int g = 0; g < 8; ++g) acc += part[g][ri];
-    f.vec[f.vec_off[fr] + k + i] -= acc;
-  }
+  const uint32_t i = chunk * SCHUNK + threadIdx.x;
+  if (i >= m) return;
+  const uint32_t nrow = f.rows_off[fr] + i / 3, d = i % 3;
+  double v = i < k ? double(rin[3 * size_t(f.rows[nrow]) + d]) : 0.0;
+  for (uint32_t q = f.src_start[nrow]; q < f.src_start[nrow + 1]; ++q) v += f.vec[f.src[q] + d];
+  f.vec[f.vec_off[fr] + i] = v;
 }
 
 __device__ __forceinline__ double wave_sum64(double v) {
@@ -375,47 +343,181 @@ __device__ __forceinline__ double wave_sum64(double v) {
   return v;
 }
 
-// y[j] -= sum_i L21[i][j] x[row k+i], one wave per column
-__global__ __launch_bounds__(NT) void k_solve_l21_cols(FactorDev f, const uint64_t *tasks,
-                                                       const int *done, int parity, uint32_t B) {
+// G = [M11; W21]: W21 tile (I, J) = sum_{q >= J} L21[I][q] M11[q][J]
+template <class Real>
+__global__ __launch_bounds__(NT) void k_pack_w21(FactorDev f, const uint64_t *tasks, Real *gr,
+                                                 Real *gc) {
+  __shared__ double As[FNB][LDP];  // As[kk][r] = L21[I*64 + r][q*64 + kk]
+  __shared__ double Bs[FNB][LDP];  // Bs[kk][c] = M11[q*64 + kk][J*64 + c]
   const uint64_t t = tasks[blockIdx.x];
-  const uint32_t fr = lo32(t), tile = mid16(t);
-  if (skip(done, parity, B, f.body[fr])) return;
+  const uint32_t fr = lo32(t), I = mid16(t), J = hi16(t);
   const uint32_t k = f.k[fr], m = f.m[fr], u = m - k;
   const double *L = f.pool + f.off[fr];
-  const uint32_t *rows = f.rows + f.rows_off[fr];
-  const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
-  for (uint32_t jj = wave; jj < SROW; jj += NT / 64) {
-    const uint32_t j = tile * SROW + jj;
-    if (j >= k) break;
-    double s = 0.0;
-    for (uint32_t i = lane; i < u; i += 64) {
-      const uint32_t dof = k + i;
-      s += L[uint64_t(j) * m + dof] * f.xsol[3 * size_t(rows[dof / 3]) + dof % 3];
+  const double *M = f.minv + f.minv_off[fr];
+  const int ni = int(min(uint32_t(FNB), u - I * FNB)), nj = int(min(uint32_t(FNB), k - J * FNB));
+  MfmaAcc acc;
+  mfma_zero(acc);
+  const uint32_t kt = (k + FNB - 1) / FNB;
+  for (uint32_t q = J; q < kt; ++q) {
+    const int nq = int(min(uint32_t(FNB), k - q * FNB));
+    __syncthreads();
+    for (int idx = threadIdx.x; idx < FNB * FNB; idx += NT) {
+      const int r = idx % FNB, c = idx / FNB;
+      As[c][r] = (r < ni && c < nq) ? L[uint64_t(q * FNB + c) * m + k + I * FNB + r] : 0.0;
+      const uint32_t mr = q * FNB + r, mc = J * FNB + c;
+      Bs[r][c] = (r < nq && c < nj && mr >= mc) ? M[uint64_t(mc) * k + mr] : 0.0;
     }
-    s = wave_sum64(s);
-    if (lane == 0) f.y[f.yoff[fr] + j] -= s;
+    __syncthreads();
+    mfma_tile(As, Bs, acc);
+  }
+  __syncthreads();
+  mfma_store(acc, As);  // As[r][c] = W21 tile
+  __syncthreads();
+  Real *R = gr + f.g_off[fr];
+  Real *C = gc + f.g_off[fr];
+  for (int idx = threadIdx.x; idx < FNB * FNB; idx += NT) {
+    const int r = idx % FNB, c = idx / FNB;
+    if (r >= ni || c >= nj) continue;
+    const uint32_t gi = k + I * FNB + r, gj = J * FNB + c;
+    C[uint64_t(gj) * m + gi] = Real(As[r][c]);
+  }
+  for (int idx = threadIdx.x; idx < FNB * FNB; idx += NT) {
+    const int c = idx % FNB, r = idx / FNB;
+    if (r >= ni || c >= nj) continue;
+    const uint32_t gi = k + I * FNB + r, gj = J * FNB + c;
+    R[uint64_t(gi) * k + gj] = Real(As[r][c]);
   }
 }
 
-// x[col j] = sum_{i>=j} M11[i][j] z[i], one wave per column
-__global__ __launch_bounds__(NT) void k_solve_m11_cols(FactorDev f, const uint64_t *tasks,
-                                                       const int *done, int parity, uint32_t B) {
+// G rows 0..k-1 = M11 (lower tiles; the upper part is never read)
+template <class Real>
+__global__ __launch_bounds__(NT) void k_pack_m11(FactorDev f, const uint64_t *tasks, Real *gr,
+                                                 Real *gc) {
   const uint64_t t = tasks[blockIdx.x];
-  const uint32_t fr = lo32(t), tile = mid16(t);
-  if (skip(done, parity, B, f.body[fr])) return;
-  const uint32_t k = f.k[fr];
+  const uint32_t fr = lo32(t), I = mid16(t), J = hi16(t);
+  const uint32_t k = f.k[fr], m = f.m[fr];
   const double *M = f.minv + f.minv_off[fr];
-  const double *z = f.y + f.yoff[fr];
-  const uint32_t *rows = f.rows + f.rows_off[fr];
+  Real *R = gr + f.g_off[fr];
+  Real *C = gc + f.g_off[fr];
+  for (int idx = threadIdx.x; idx < FNB * FNB; idx += NT) {
+    const uint32_t i = I * FNB + idx % FNB, j = J * FNB + idx / FNB;
+    if (i >= k || j >= k) continue;
+    const Real v = i >= j ? Real(M[uint64_t(j) * k + i]) : Real(0);
+    R[uint64_t(i) * k + j] = v;
+    C[uint64_t(j) * m + i] = v;
+  }
+}
+
+// forward: rows of G times v[0:k], each wavefront 1 or 4 consecutive rows
+// (4 for small fronts), the vector staged in LDS chunks shared by the
+// workgroup, 4 independent loads in flight per lane; y[i] for i < k and
+// v[i] -= . for i >= k
+template <class Real>
+__global__ __launch_bounds__(NT) void k_solve_forward(FactorDev f, const Real *gr,
+                                                      const uint64_t *tasks, const int *done,
+                                                      int parity, uint32_t B) {
+  __shared__ double vs[SCH];
+  const uint64_t t = tasks[blockIdx.x];
+  const uint32_t fr = lo32(t), grp = mid16(t), gs = hi16(t), rw = gs / (NT / 64);
+  if (skip(done, parity, B, f.body[fr])) return;
+  const uint32_t k = f.k[fr], m = f.m[fr];
   const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
-  for (uint32_t jj = wave; jj < SROW; jj += NT / 64) {
-    const uint32_t j = tile * SROW + jj;
-    if (j >= k) break;
-    double s = 0.0;
-    for (uint32_t i = j + lane; i < k; i += 64) s += M[uint64_t(j) * k + i] * z[i];
-    s = wave_sum64(s);
-    if (lane == 0) f.xsol[3 * size_t(rows[j / 3]) + j % 3] = s;
+  const uint32_t i0 = grp * gs;
+  const uint32_t ilast = min(i0 + gs, m) - 1;
+  const uint32_t nmax = ilast < k ? ilast + 1 : k;  // columns any row of the group needs
+  double *v = f.vec + f.vec_off[fr];
+  double s0[4] = {}, s1[4] = {}, s2[4] = {}, s3[4] = {};
+  for (uint32_t c0 = 0; c0 < nmax; c0 += SCH) {
+    const uint32_t cn = min(uint32_t(SCH), nmax - c0);
+    __syncthreads();
+    for (uint32_t q = threadIdx.x; q < cn; q += NT) vs[q] = v[c0 + q];
+    __syncthreads();
+    for (uint32_t r = 0; r < rw; ++r) {
+      const uint32_t i = i0 + wave * rw + r;
+      const uint32_t n = i < m ? (i < k ? i + 1 : k) : 0;
+      const Real *row = gr + f.g_off[fr] + uint64_t(min(i, m - 1)) * k;
+      const uint32_t end = n > c0 ? min(n, c0 + cn) : c0;
+      uint32_t j = c0 + lane;
+      for (; j + 192 < end; j += 256) {
+        const Real a0 = row[j], a1 = row[j + 64], a2 = row[j + 128], a3 = row[j + 192];
+        s0[r] += double(a0) * vs[j - c0];
+        s1[r] += double(a1) * vs[j + 64 - c0];
+        s2[r] += double(a2) * vs[j + 128 - c0];
+        s3[r] += double(a3) * vs[j + 192 - c0];
+      }
+      for (; j < end; j += 64) s0[r] += double(row[j]) * vs[j - c0];
+    }
+  }
+  for (uint32_t r = 0; r < rw; ++r) {
+    const double tot = wave_sum64((s0[r] + s1[r]) + (s2[r] + s3[r]));
+    const uint32_t i = i0 + wave * rw + r;
+    if (lane == 0 && i < m) {
+      if (i < k)
+        f.y[f.yoff[fr] + i] = tot;
+      else
+        v[i] -= tot;
+    }
+  }
+}
+
+// backward prologue: x2 (solution at the front's update rows) into v[k:m]
+__global__ __launch_bounds__(NT) void k_solve_xgather(FactorDev f, const uint64_t *tasks,
+                                                      const int *done, int parity, uint32_t B) {
+  const uint64_t t = tasks[blockIdx.x];
+  const uint32_t fr = lo32(t), chunk = mid16(t);
+  if (skip(done, parity, B, f.body[fr])) return;
+  const uint32_t k = f.k[fr], m = f.m[fr];
+  const uint32_t i = k + chunk * SCHUNK + threadIdx.x;
+  if (i >= m) return;
+  const uint32_t *rows = f.rows + f.rows_off[fr];
+  f.vec[f.vec_off[fr] + i] = f.xsol[3 * size_t(rows[i / 3]) + i % 3];
+}
+
+// backward: x[col j] = sum_{j<=i<k} G[i][j] y[i] - sum_{i>=k} G[i][j] x2[i],
+// each wavefront 1 or 4 consecutive columns, w = [y; -x2] staged in LDS
+template <class Real>
+__global__ __launch_bounds__(NT) void k_solve_backward(FactorDev f, const Real *gc,
+                                                       const uint64_t *tasks, const int *done,
+                                                       int parity, uint32_t B) {
+  __shared__ double ws[SCH];
+  const uint64_t t = tasks[blockIdx.x];
+  const uint32_t fr = lo32(t), grp = mid16(t), gs = hi16(t), cw = gs / (NT / 64);
+  if (skip(done, parity, B, f.body[fr])) return;
+  const uint32_t k = f.k[fr], m = f.m[fr];
+  const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+  const uint32_t j0 = grp * gs;
+  const double *y = f.y + f.yoff[fr];
+  const double *v = f.vec + f.vec_off[fr];
+  double s0[4] = {}, s1[4] = {}, s2[4] = {}, s3[4] = {};
+  for (uint32_t c0 = j0; c0 < m; c0 += SCH) {
+    const uint32_t cn = min(uint32_t(SCH), m - c0);
+    __syncthreads();
+    for (uint32_t q = threadIdx.x; q < cn; q += NT) {
+      const uint32_t i = c0 + q;
+      ws[q] = i < k ? y[i] : -v[i];
+    }
+    __syncthreads();
+    const uint32_t end = c0 + cn;
+    for (uint32_t c = 0; c < cw; ++c) {
+      const uint32_t j = j0 + wave * cw + c;
+      const uint32_t start = j < k ? j : m;  // rows of this column: [j, m)
+      const Real *col = gc + f.g_off[fr] + uint64_t(min(j, k - 1)) * m;
+      uint32_t i = max(start, c0) + lane;
+      for (; i + 192 < end; i += 256) {
+        const Real a0 = col[i], a1 = col[i + 64], a2 = col[i + 128], a3 = col[i + 192];
+        s0[c] += double(a0) * ws[i - c0];
+        s1[c] += double(a1) * ws[i + 64 - c0];
+        s2[c] += double(a2) * ws[i + 128 - c0];
+        s3[c] += double(a3) * ws[i + 192 - c0];
+      }
+      for (; i < end; i += 64) s0[c] += double(col[i]) * ws[i - c0];
+    }
+  }
+  const uint32_t *rows = f.rows + f.rows_off[fr];
+  for (uint32_t c = 0; c < cw; ++c) {
+    const double tot = wave_sum64((s0[c] + s1[c]) + (s2[c] + s3[c]));
+    const uint32_t j = j0 + wave * cw + c;
+    if (lane == 0 && j < k) f.xsol[3 * size_t(rows[j / 3]) + j % 3] = tot;
   }
 }
 
@@ -456,7 +558,7 @@ void factor_extend_add(const FactorDev &f, const uint32_t *parent, const uint64_
 }
 void factor_potrf(const FactorDev &f, const uint32_t *tasks, uint32_t count, uint32_t panel,
                   hipStream_t s) {
-  if (count) hipLaunchKernelGGL(k_potrf, dim3(count), dim3(NT), 0, s, f, tasks, panel);
+  if (count) hipLaunchKernelGGL(k_potrf, dim3(count), dim3(64), 0, s, f, tasks, panel);
 }
 void factor_trsm(const FactorDev &f, const uint64_t *tasks, uint32_t count, uint32_t panel,
                  hipStream_t s) {
@@ -471,30 +573,34 @@ void factor_trtri(const FactorDev &f, const uint64_t *tasks, uint32_t count, hip
 }
 
 template <class Real>
-void SolveLaunch<Real>::gather(const FactorDev &f, const uint32_t *fronts, uint32_t count,
+void SolveLaunch<Real>::pack(const FactorDev &f, const uint64_t *w21, uint32_t nw21,
+                             const uint64_t *m11, uint32_t nm11, Real *gr, Real *gc,
+                             hipStream_t s) {
+  if (nw21) hipLaunchKernelGGL(k_pack_w21<Real>, dim3(nw21), dim3(NT), 0, s, f, w21, gr, gc);
+  if (nm11) hipLaunchKernelGGL(k_pack_m11<Real>, dim3(nm11), dim3(NT), 0, s, f, m11, gr, gc);
+}
+template <class Real>
+void SolveLaunch<Real>::gather(const FactorDev &f, const uint64_t *tasks, uint32_t count,
                                const Real *r, const int *done, int parity, uint32_t B,
                                hipStream_t s) {
-  if (count) hipLaunchKernelGGL(k_solve_gather<Real>, dim3(count), dim3(NT), 0, s, f, fronts, r, done, parity, B);
+  if (count) hipLaunchKernelGGL(k_solve_gather<Real>, dim3(count), dim3(NT), 0, s, f, tasks, r, done, parity, B);
 }
 template <class Real>
-void SolveLaunch<Real>::m11_rows(const FactorDev &f, const uint64_t *tasks, uint32_t count,
-                                 const int *done, int parity, uint32_t B, hipStream_t s) {
-  if (count) hipLaunchKernelGGL(k_solve_m11_rows, dim3(count), dim3(NT), 0, s, f, tasks, done, parity, B);
+void SolveLaunch<Real>::forward(const FactorDev &f, const Real *gr, const uint64_t *tasks,
+                                uint32_t count, const int *done, int parity, uint32_t B,
+                                hipStream_t s) {
+  if (count) hipLaunchKernelGGL(k_solve_forward<Real>, dim3(count), dim3(NT), 0, s, f, gr, tasks, done, parity, B);
 }
 template <class Real>
-void SolveLaunch<Real>::l21_rows(const FactorDev &f, const uint64_t *tasks, uint32_t count,
-                                 const int *done, int parity, uint32_t B, hipStream_t s) {
-  if (count) hipLaunchKernelGGL(k_solve_l21_rows, dim3(count), dim3(NT), 0, s, f, tasks, done, parity, B);
+void SolveLaunch<Real>::xgather(const FactorDev &f, const uint64_t *tasks, uint32_t count,
+                                const int *done, int parity, uint32_t B, hipStream_t s) {
+  if (count) hipLaunchKernelGGL(k_solve_xgather, dim3(count), dim3(NT), 0, s, f, tasks, done, parity, B);
 }
 template <class Real>
-void SolveLaunch<Real>::l21_cols(const FactorDev &f, const uint64_t *tasks, uint32_t count,
-                                 const int *done, int parity, uint32_t B, hipStream_t s) {
-  if (count) hipLaunchKernelGGL(k_solve_l21_cols, dim3(count), dim3(NT), 0, s, f, tasks, done, parity, B);
-}
-template <class Real>
-void SolveLaunch<Real>::m11_cols(const FactorDev &f, const uint64_t *tasks, uint32_t count,
-                                 const int *done, int parity, uint32_t B, hipStream_t s) {
-  if (count) hipLaunchKernelGGL(k_solve_m11_cols, dim3(count), dim3(NT), 0, s, f, tasks, done, parity, B);
+void SolveLaunch<Real>::backward(const FactorDev &f, const Real *gc, const uint64_t *tasks,
+                                 uint32_t count, const int *done, int parity, uint32_t B,
+                                 hipStream_t s) {
+  if (count) hipLaunchKernelGGL(k_solve_backward<Real>, dim3(count), dim3(NT), 0, s, f, gc, tasks, done, parity, B);
 }
 template <class Real>
 void SolveLaunch<Real>::finish(const FactorDev &f, Real *z, uint32_t N, const uint8_t *anchored,

@@ -198,8 +198,9 @@ void build_tables(const std::vector<Symbolic> &sym, const std::vector<Body> &bod
   o.minv_off.resize(F);
   o.vec_off.resize(F);
   o.yoff.resize(F);
+  o.g_off.resize(F);
   o.child_start.assign(F + 1, 0);
-  uint64_t pool = 0, minv = 0, vec = 0, yv = 0;
+  uint64_t pool = 0, minv = 0, vec = 0, yv = 0, gp = 0;
   for (size_t b = 0; b < bodies.size(); ++b) {
     const Symbolic &s = sym[b];
     for (uint32_t lf = 0; lf < s.fronts.size(); ++lf) {
@@ -218,6 +219,8 @@ void build_tables(const std::vector<Symbolic> &sym, const std::vector<Body> &bod
       vec += o.m[f];
       o.yoff[f] = yv;
       yv += o.k[f];
+      o.g_off[f] = gp;
+      gp += uint64_t(o.m[f]) * o.k[f];
       o.rows_off[f] = uint32_t(o.rows.size());
       for (uint32_t r = 0; r < fs.nrow; ++r) o.rows.push_back(n0[b] + s.perm[s.rows[fs.row0 + r]]);
       const uint32_t cs = s.child_start[lf], ce = s.child_start[lf + 1];
@@ -245,6 +248,7 @@ void build_tables(const std::vector<Symbolic> &sym, const std::vector<Body> &bod
   o.minv_pool = minv;
   o.vec_pool = vec;
   o.y_pool = yv;
+  o.g_pool = gp;
   // element pairs: lower node blocks (pos(i) >= pos(j)) of every tet
   struct Item {
     uint64_t key;
@@ -361,24 +365,52 @@ void build_tables(const std::vector<Symbolic> &sym, const std::vector<Body> &bod
     }
     o.trtri_start.push_back(uint32_t(o.trtri_tasks.size()));
   }
-  // solve tiles per level
-  o.m11r_start.assign(1, 0);
-  o.l21r_start.assign(1, 0);
-  o.l21c_start.assign(1, 0);
-  o.m11c_start.assign(1, 0);
+  // G packing: W21 tiles (rows of L21 x column blocks) and lower M11 tiles
+  for (uint32_t f = 0; f < F; ++f) {
+    const uint32_t kt = tiles(o.k[f], FNB), ut = tiles(o.m[f] - o.k[f], FNB);
+    for (uint32_t I = 0; I < ut; ++I)
+      for (uint32_t J = 0; J < kt; ++J) o.w21_tasks.push_back(uint64_t(f) | (uint64_t(I) << 32) | (uint64_t(J) << 48));
+    for (uint32_t I = 0; I < kt; ++I)
+      for (uint32_t J = 0; J <= I; ++J) o.pack_tasks.push_back(uint64_t(f) | (uint64_t(I) << 32) | (uint64_t(J) << 48));
+  }
+  // pull lists of the forward gather: child update rows in child order
+  {
+    std::vector<std::vector<uint64_t>> lists(o.rows.size());
+    for (uint32_t f = 0; f < F; ++f)
+      for (uint32_t ci = o.child_start[f]; ci < o.child_start[f + 1]; ++ci) {
+        const uint32_t c = o.children[ci];
+        const uint32_t kn = o.k[c] / 3, un = (o.m[c] - o.k[c]) / 3;
+        for (uint32_t q = 0; q < un; ++q)
+          lists[o.rows_off[f] + o.map[o.map_off[c] + q]].push_back(o.vec_off[c] + 3ull * (kn + q));
+      }
+    o.src_start.assign(1, 0);
+    for (const auto &l : lists) {
+      o.src.insert(o.src.end(), l.begin(), l.end());
+      o.src_start.push_back(uint32_t(o.src.size()));
+    }
+  }
+  // solve: per level, gather chunks, forward row groups, x2 chunks, backward
+  // column groups
+  o.fwd_start.assign(1, 0);
+  o.bwd_start.assign(1, 0);
+  o.gat_start.assign(1, 0);
+  o.xg_start.assign(1, 0);
   for (uint32_t l = 0; l < L; ++l) {
     for (uint32_t f : by_level[l]) {
-      for (uint32_t t = 0; t < tiles(o.k[f], SROW); ++t) o.m11_rows.push_back(uint64_t(f) | (uint64_t(t) << 32));
-      for (uint32_t t = 0; t < tiles(o.m[f] - o.k[f], SROW); ++t)
-        o.l21_rows.push_back(uint64_t(f) | (uint64_t(t) << 32));
-      if (o.m[f] > o.k[f])
-        for (uint32_t t = 0; t < tiles(o.k[f], SROW); ++t) o.l21_cols.push_back(uint64_t(f) | (uint64_t(t) << 32));
-      for (uint32_t t = 0; t < tiles(o.k[f], SROW); ++t) o.m11_cols.push_back(uint64_t(f) | (uint64_t(t) << 32));
+      // small fronts: 4 rows per wavefront so the staged vector is reused
+      const uint64_t gs = o.k[f] <= 192 ? 4 * SGRP : SGRP;
+      for (uint32_t g = 0; g < tiles(o.m[f], uint32_t(gs)); ++g)
+        o.fwd_tasks.push_back(uint64_t(f) | (uint64_t(g) << 32) | (gs << 48));
+      for (uint32_t g = 0; g < tiles(o.k[f], uint32_t(gs)); ++g)
+        o.bwd_tasks.push_back(uint64_t(f) | (uint64_t(g) << 32) | (gs << 48));
+      for (uint32_t g = 0; g < tiles(o.m[f], SCHUNK); ++g) o.gat_tasks.push_back(uint64_t(f) | (uint64_t(g) << 32));
+      for (uint32_t g = 0; g < tiles(o.m[f] - o.k[f], SCHUNK); ++g)
+        o.xg_tasks.push_back(uint64_t(f) | (uint64_t(g) << 32));
     }
-    o.m11r_start.push_back(uint32_t(o.m11_rows.size()));
-    o.l21r_start.push_back(uint32_t(o.l21_rows.size()));
-    o.l21c_start.push_back(uint32_t(o.l21_cols.size()));
-    o.m11c_start.push_back(uint32_t(o.m11_cols.size()));
+    o.fwd_start.push_back(uint32_t(o.fwd_tasks.size()));
+    o.bwd_start.push_back(uint32_t(o.bwd_tasks.size()));
+    o.gat_start.push_back(uint32_t(o.gat_tasks.size()));
+    o.xg_start.push_back(uint32_t(o.xg_tasks.size()));
   }
 }
 
