@@ -157,6 +157,18 @@ int talus_mi_build_lattice(const talus_mi_lattice *lattice,
                            uint32_t nsupports, talus_mi_mesh *out);
 void talus_mi_mesh_free(talus_mi_mesh *mesh);
 
+/* Tear topology update (tearCells, src/talus/dynamics/strainstep/
+ * tearing.cc:485-513, on tets) of a host mesh at positions x (3 per node):
+ * nodes around each torn tet (ascending indices) are split, new nodes are
+ * appended, masses recomputed as rho*V/4.  *out is malloc'ed (release with
+ * talus_mi_mesh_free); *origin (one entry per new node: the node it was
+ * split from) is malloc'ed, release with talus_mi_free. */
+int talus_mi_tear_mesh(const talus_mi_mesh *mesh, const talus_mi_material *materials,
+                       uint32_t lanes, const double *x, const uint32_t *torn,
+                       uint32_t ntorn, talus_mi_mesh *out, uint32_t **origin,
+                       uint32_t *new_nodes);
+void talus_mi_free(void *p);
+
 /* ---- bodies (the tet body set stepped inside World::step) ---- */
 int talus_mi_body_add(talus_mi_ctx *ctx, const talus_mi_mesh *mesh,
                       const talus_mi_material *materials, uint32_t lanes,
@@ -172,6 +184,13 @@ int talus_mi_state_get(talus_mi_ctx *ctx, uint32_t body, double *x, double *v,
  * corotated eps_p[9] + alpha, neo-Hookean (Fp^-1 - I)[9] + alpha) */
 int talus_mi_rotations_get(talus_mi_ctx *ctx, uint32_t body, double *R);
 int talus_mi_rotations_set(talus_mi_ctx *ctx, uint32_t body, const double *R);
+/* Neo-Hookean volume frames W = cof(Fe) Fp^-T (9 per tet, identity for
+ * corotated tets) of the last element evaluation: the operator's volumetric
+ * direction for neo-Hookean tets (no counterpart in the reference, whose tet
+ * path is corotated; kept with the rotations so a restored state steps
+ * identically). */
+int talus_mi_frames_get(talus_mi_ctx *ctx, uint32_t body, double *W);
+int talus_mi_frames_set(talus_mi_ctx *ctx, uint32_t body, const double *W);
 int talus_mi_plastic_get(talus_mi_ctx *ctx, uint32_t body, double *plastic);
 int talus_mi_plastic_set(talus_mi_ctx *ctx, uint32_t body,
                          const double *plastic);

@@ -113,6 +113,8 @@ def lib():
         L.orc_body_get_state.argtypes = [C.c_void_p, dp, dp, dp]
         L.orc_body_get_rotations.argtypes = [C.c_void_p, dp]
         L.orc_body_set_rotations.argtypes = [C.c_void_p, dp]
+        L.orc_body_get_frames.argtypes = [C.c_void_p, dp]
+        L.orc_body_set_frames.argtypes = [C.c_void_p, dp]
         L.orc_body_get_plastic.argtypes = [C.c_void_p, dp]
         L.orc_body_set_plastic.argtypes = [C.c_void_p, dp]
         L.orc_step.argtypes = [C.c_void_p, C.POINTER(Settings), C.POINTER(Report)]
@@ -120,6 +122,15 @@ def lib():
         L.orc_apply_system.argtypes = [C.c_void_p, C.c_double, C.c_double, dp, dp, C.c_uint32]
         L.orc_tet_stretch.argtypes = [C.c_void_p, dp]
         L.orc_yield_flags.argtypes = [C.c_void_p, C.POINTER(C.c_uint8)]
+        u32p = C.POINTER(C.c_uint32)
+        L.orc_tear_topology.argtypes = [C.c_uint32, C.c_uint32, u32p, dp, dp, C.POINTER(C.c_uint8),
+                                        C.c_uint32, u32p, C.c_uint32, u32p]
+        L.orc_tear_topology.restype = C.c_uint32
+        L.orc_body_tear.argtypes = [C.c_void_p, u32p, C.c_uint32]
+        L.orc_body_tear.restype = C.c_uint32
+        L.orc_body_nodes.argtypes = [C.c_void_p]
+        L.orc_body_nodes.restype = C.c_uint32
+        L.orc_body_topology.argtypes = [C.c_void_p, u32p, dp, dp, C.POINTER(C.c_uint8)]
         _lib = L
     return _lib
 
@@ -251,6 +262,15 @@ class Body:
         R = np.ascontiguousarray(R, dtype=np.float64).reshape(-1)
         lib().orc_body_set_rotations(self._h, _dp(R))
 
+    def frames(self):
+        W = np.empty(9 * self.tets)
+        lib().orc_body_get_frames(self._h, _dp(W))
+        return W.reshape(-1, 3, 3)
+
+    def set_frames(self, W):
+        W = np.ascontiguousarray(W, dtype=np.float64).reshape(-1)
+        lib().orc_body_set_frames(self._h, _dp(W))
+
     def plastic(self):
         p = np.empty(10 * self.tets)
         lib().orc_body_get_plastic(self._h, _dp(p))
@@ -293,6 +313,22 @@ class Body:
         s = np.empty(self.tets)
         lib().orc_tet_stretch(self._h, _dp(s))
         return s
+
+    def tear(self, torn):
+        torn = np.ascontiguousarray(np.sort(np.asarray(torn, dtype=np.uint32)))
+        added = lib().orc_body_tear(self._h, torn.ctypes.data_as(C.POINTER(C.c_uint32)), torn.size)
+        self.nodes = int(lib().orc_body_nodes(self._h))
+        return int(added)
+
+    def topology(self):
+        n = self.nodes
+        tet = np.empty(4 * self.tets, dtype=np.uint32)
+        rest = np.empty(3 * n)
+        mass = np.empty(n)
+        anchored = np.empty(n, dtype=np.uint8)
+        lib().orc_body_topology(self._h, tet.ctypes.data_as(C.POINTER(C.c_uint32)), _dp(rest), _dp(mass),
+                                anchored.ctypes.data_as(C.POINTER(C.c_uint8)))
+        return {"tet": tet.reshape(-1, 4), "rest": rest.reshape(-1, 3), "mass": mass, "anchored": anchored}
 
     def yield_flags(self):
         f = np.empty(self.tets, dtype=np.uint8)
@@ -369,3 +405,26 @@ def eigen_sym3(A):
     V = np.empty(9)
     lib().orc_eigen_sym3(_dp(A), _dp(w), _dp(V))
     return w, V.reshape(3, 3)
+
+
+def tear_topology(tet, rest, x, anchored, torn):
+    """Tear topology on plain arrays; returns (tet, rest, x, anchored, origin)."""
+    tet = np.ascontiguousarray(tet, dtype=np.uint32).reshape(-1).copy()
+    nodes = np.asarray(rest).reshape(-1, 3).shape[0]
+    torn = np.ascontiguousarray(np.sort(np.asarray(torn, dtype=np.uint32)))
+    cap = nodes + 64 * (torn.size + 1)
+    r = np.zeros(3 * cap)
+    r[: 3 * nodes] = np.asarray(rest, dtype=np.float64).reshape(-1)
+    xx = np.zeros(3 * cap)
+    xx[: 3 * nodes] = np.asarray(x, dtype=np.float64).reshape(-1)
+    an = np.zeros(cap, dtype=np.uint8)
+    an[:nodes] = np.asarray(anchored, dtype=np.uint8)
+    origin = np.zeros(cap, dtype=np.uint32)
+    u32 = C.POINTER(C.c_uint32)
+    n2 = lib().orc_tear_topology(nodes, tet.size // 4, tet.ctypes.data_as(u32), _dp(r), _dp(xx),
+                                 an.ctypes.data_as(C.POINTER(C.c_uint8)), cap,
+                                 torn.ctypes.data_as(u32), torn.size, origin.ctypes.data_as(u32))
+    if n2 == 0:
+        raise RuntimeError("tear topology overflow")
+    return (tet.reshape(-1, 4), r[: 3 * n2].reshape(-1, 3), xx[: 3 * n2].reshape(-1, 3), an[:n2],
+            origin[: n2 - nodes])

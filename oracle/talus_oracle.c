@@ -807,10 +807,11 @@ static void dot_range(void *p, size_t begin, size_t end) {
 
 struct orc_body {
   orc_mesh m;
-  double *lambda, *mu, *yield, *hard, *pmax, *tear;
+  double *lambda, *mu, *yield, *hard, *pmax, *tear, *rho;
   uint8_t *snh, *plastic_lane;
   double *x, *v, *a;
   double *R;       /* 9 per tet, committed rotation */
+  double *W;       /* 9 per tet, neo-Hookean volume frame cof(Fe) Fp^-T */
   double *plastic; /* 10 per tet */
   uint8_t *yflag;
   uint32_t *inc_start, *inc_slot;
@@ -848,6 +849,7 @@ orc_body *orc_body_create(const orc_mesh *mesh, const orc_material *materials,
   b->hard = (double *)malloc(T * sizeof(double));
   b->pmax = (double *)malloc(T * sizeof(double));
   b->tear = (double *)malloc(T * sizeof(double));
+  b->rho = (double *)malloc(T * sizeof(double) + 8);
   b->snh = (uint8_t *)malloc(T + 1);
   b->plastic_lane = (uint8_t *)malloc(T + 1);
   orc_material fb;
@@ -859,6 +861,7 @@ orc_body *orc_body_create(const orc_mesh *mesh, const orc_material *materials,
     b->hard[e] = mt->hardening_modulus > 0.0f ? (double)mt->hardening_modulus : 0.0;
     b->pmax[e] = (double)mt->plastic_max;
     b->tear[e] = 1.0 + (double)mt->tear_strain;
+    b->rho[e] = (double)mt->density;
     b->snh[e] = mt->elastic_model == 1u ? 1u : 0u;
   }
   b->x = (double *)malloc(3 * N * sizeof(double));
@@ -866,7 +869,11 @@ orc_body *orc_body_create(const orc_mesh *mesh, const orc_material *materials,
   b->v = (double *)calloc(3 * N, sizeof(double));
   b->a = (double *)calloc(3 * N, sizeof(double));
   b->R = (double *)malloc(9 * T * sizeof(double));
-  for (size_t e = 0; e < T; ++e) ident3(&b->R[9 * e]);
+  b->W = (double *)malloc(9 * T * sizeof(double));
+  for (size_t e = 0; e < T; ++e) {
+    ident3(&b->R[9 * e]);
+    ident3(&b->W[9 * e]);
+  }
   b->plastic = (double *)calloc(10 * T, sizeof(double));
   b->yflag = (uint8_t *)calloc(T + 1, 1);
   /* incidence: node -> slots (tet*4+corner) in ascending slot order
@@ -907,8 +914,8 @@ orc_body *orc_body_create(const orc_mesh *mesh, const orc_material *materials,
 void orc_body_free(orc_body *b) {
   if (!b) return;
   orc_mesh_free(&b->m);
-  void *ptrs[] = {b->lambda, b->mu, b->yield, b->hard, b->pmax, b->tear,
-                  b->snh, b->plastic_lane, b->x, b->v, b->a, b->R, b->plastic,
+  void *ptrs[] = {b->lambda, b->mu, b->yield, b->hard, b->pmax, b->tear, b->rho,
+                  b->snh, b->plastic_lane, b->x, b->v, b->a, b->R, b->W, b->plastic,
                   b->yflag, b->inc_start, b->inc_slot, b->slot_f, b->slot_d,
                   b->tet_energy, b->force, b->diag, b->precond, b->right,
                   b->weighted, b->shifted, b->carried, b->drive, b->evaluated,
@@ -936,6 +943,14 @@ void orc_body_get_state(const orc_body *b, double *x, double *v, double *a) {
 void orc_body_get_rotations(const orc_body *b, double *R) {
   memcpy(R, b->R, 9 * (size_t)b->m.tets * sizeof(double));
 }
+void orc_body_get_frames(const orc_body *b, double *W) {
+  memcpy(W, b->W, 9 * (size_t)b->m.tets * sizeof(double));
+}
+
+void orc_body_set_frames(orc_body *b, const double *W) {
+  memcpy(b->W, W, 9 * (size_t)b->m.tets * sizeof(double));
+}
+
 void orc_body_set_rotations(orc_body *b, const double *R) {
   memcpy(b->R, R, 9 * (size_t)b->m.tets * sizeof(double));
 }
@@ -964,10 +979,29 @@ static void gather4(const orc_body *b, const double *src, size_t e,
       out[3 * i + r] = src[3 * (size_t)b->m.tet[4 * e + i] + r];
 }
 
+/* Neo-Hookean volume frame W = cof(Fe) Fp^-T with Fe = F Fp^-1 and
+ * Fp^-1 = I + ep: the derivative of det(Fe) along dF is W : dF. */
+static void volume_frame(const double *F, const double *ep, double *W) {
+  if (!ep) {
+    cof3(F, W);
+    return;
+  }
+  double Fi[9], Fe[9], C[9];
+  memcpy(Fi, ep, sizeof Fi);
+  Fi[0] += 1.0;
+  Fi[4] += 1.0;
+  Fi[8] += 1.0;
+  mul3(F, Fi, Fe);
+  cof3(Fe, C);
+  mul3_nt(C, Fi, W);
+}
+
 /* One tet of assembleElastic (solver.cc:1382-1470): committed rotation,
- * forces to slots, diagonal stiffness blocks R K_ii R^T to slots
- * (stiffnessDiagonal, material.cc:290-304; closed form
- * V((lambda+mu)(Rg)(Rg)^T + mu|g|^2 I)). */
+ * forces to slots, diagonal stiffness blocks to slots.  Corotated:
+ * R K_ii R^T = V((lambda+mu)(Rg)(Rg)^T + mu|g|^2 I) (stiffnessDiagonal,
+ * material.cc:290-304).  Neo-Hookean: the same blocks with the volumetric
+ * part along the volume frame, V(mu(Rg)(Rg)^T + mu|g|^2 I +
+ * lambda (Wg)(Wg)^T) -- the operator of op_element_range. */
 static void element_one(orc_body *b, const double *at, size_t e, int diag) {
   double x[12], X[12], u[12], F[9], R[9];
   gather4(b, at, e, x);
@@ -977,14 +1011,14 @@ static void element_one(orc_body *b, const double *at, size_t e, int diag) {
   const double V = b->m.volume[e];
   double energy = 0.0;
   if (b->snh[e]) {
+    const double *ep = b->plastic_lane[e] ? &b->plastic[10 * e] : NULL;
     displacement_gradient(u, dminv, F);
     F[0] += 1.0;
     F[4] += 1.0;
     F[8] += 1.0;
     orc_element_rotation(F, R);
-    orc_snh_tet(u, dminv, V, b->lambda[e], b->mu[e],
-                b->plastic_lane[e] ? &b->plastic[10 * e] : NULL,
-                &b->slot_f[12 * e], &energy);
+    volume_frame(F, ep, &b->W[9 * e]);
+    orc_snh_tet(u, dminv, V, b->lambda[e], b->mu[e], ep, &b->slot_f[12 * e], &energy);
   } else {
     orc_corotated_tet(x, X, dminv, V, b->lambda[e], b->mu[e],
                       &b->plastic[10 * e], b->yield[e], b->hard[e], R,
@@ -995,15 +1029,18 @@ static void element_one(orc_body *b, const double *at, size_t e, int diag) {
   if (diag) {
     double g[4][3];
     shape_gradients(dminv, g);
-    const double lm = b->lambda[e] + b->mu[e];
+    const double *W = &b->W[9 * e];
+    const double mu = b->mu[e], lam = b->lambda[e];
     for (int i = 0; i < 4; ++i) {
-      double rg[3];
+      double rg[3], wg[3];
       mv3(R, g[i], rg);
+      mv3(W, g[i], wg);
       const double gg = g[i][0] * g[i][0] + g[i][1] * g[i][1] + g[i][2] * g[i][2];
       double *blk = &b->slot_d[36 * e + 9 * i];
       for (int r = 0; r < 3; ++r)
         for (int c = 0; c < 3; ++c)
-          blk[r * 3 + c] = V * (lm * rg[r] * rg[c] + (r == c ? b->mu[e] * gg : 0.0));
+          blk[r * 3 + c] = b->snh[e] ? V * (mu * rg[r] * rg[c] + lam * wg[r] * wg[c] + (r == c ? mu * gg : 0.0))
+                                     : V * ((lam + mu) * rg[r] * rg[c] + (r == c ? mu * gg : 0.0));
     }
   }
 }
@@ -1069,15 +1106,20 @@ typedef struct {
   double inertia, scale;
 } op_ctx;
 
-/* Per tet: d'_j = R^T d_j, D = sum d'_j g_j^T, sigma = lambda tr(D) I +
- * mu (D + D^T), y_i = R V sigma g_i -- equal to sum_j R K_ij R^T d_j with
+/* Per tet: d'_j = R^T d_j, D' = sum d'_j g_j^T, sigma = lambda tr(D') I +
+ * mu (D' + D'^T), y_i = R V sigma g_i -- equal to sum_j R K_ij R^T d_j with
  * K_ij = V(lambda g_i g_j^T + mu g_j g_i^T + mu (g_i.g_j) I)
- * (tetStiffness/block, material.cc:80-90, 209-234; applyStiffness 267-288). */
+ * (tetStiffness/block, material.cc:80-90, 209-234; applyStiffness 267-288).
+ * Neo-Hookean tets keep the shear part and take the volumetric part along
+ * the volume frame W: y_i = V(R mu (D' + D'^T) g_i + lambda (W:D) W g_i)
+ * with D = sum d_j g_j^T.  At rest (R = W = I) both are the linear
+ * stiffness; under stretch the second follows the stiffening of the
+ * (lambda+mu)(J-1)^2 term along cof(F), which the rest stiffness misses. */
 static void op_element_range(void *p, size_t begin, size_t end) {
   op_ctx *c = (op_ctx *)p;
   orc_body *b = c->b;
   for (size_t e = begin; e < end; ++e) {
-    double d[12], g[4][3], D[9] = {0};
+    double d[12], g[4][3], D[9] = {0}, Dw[9] = {0};
     gather4(b, c->d, e, d);
     shape_gradients(&b->m.dminv[9 * e], g);
     const double *R = &b->R[9 * e];
@@ -1085,9 +1127,14 @@ static void op_element_range(void *p, size_t begin, size_t end) {
       double dp[3];
       mtv3(R, &d[3 * j], dp);
       for (int r = 0; r < 3; ++r)
-        for (int k = 0; k < 3; ++k) D[r * 3 + k] += dp[r] * g[j][k];
+        for (int k = 0; k < 3; ++k) {
+          D[r * 3 + k] += dp[r] * g[j][k];
+          Dw[r * 3 + k] += d[3 * j + r] * g[j][k];
+        }
     }
-    const double tr = trace3(D);
+    const int snh = b->snh[e];
+    const double tr = snh ? 0.0 : trace3(D);
+    const double vol = snh ? b->lambda[e] * contract3(&b->W[9 * e], Dw) : 0.0;
     double s[9];
     for (int r = 0; r < 3; ++r)
       for (int k = 0; k < 3; ++k)
@@ -1095,10 +1142,11 @@ static void op_element_range(void *p, size_t begin, size_t end) {
                        (r == k ? b->lambda[e] * tr : 0.0);
     const double V = b->m.volume[e];
     for (int i = 0; i < 4; ++i) {
-      double t[3], y[3];
+      double t[3], y[3], w[3] = {0, 0, 0};
       mv3(s, g[i], t);
       mv3(R, t, y);
-      for (int r = 0; r < 3; ++r) b->slot_f[12 * e + 3 * i + r] = V * y[r];
+      if (snh) mv3(&b->W[9 * e], g[i], w);
+      for (int r = 0; r < 3; ++r) b->slot_f[12 * e + 3 * i + r] = V * (y[r] + vol * w[r]);
     }
   }
 }
@@ -1385,4 +1433,313 @@ void orc_tet_stretch(orc_body *b, double *stretch) {
     F[8] += 1.0;
     stretch[e] = orc_stretch(F, NULL);
   }
+}
+
+/* ------------------------------------------------------------------ */
+/* Tearing topology (tearing.cc:180-513 restated for tets)              */
+/* ------------------------------------------------------------------ */
+
+typedef struct {
+  uint32_t nodes, tets, capacity, copies;
+  uint32_t *tet;
+  double *rest, *x;
+  uint8_t *anchored;
+  uint32_t *origin;
+  int overflow;
+} topo_t;
+
+/* tets containing node, ascending; returns the count (out has room for tets) */
+static uint32_t topo_fan(const topo_t *t, uint32_t node, uint32_t *out) {
+  uint32_t n = 0;
+  for (uint32_t e = 0; e < t->tets; ++e) {
+    const uint32_t *c = &t->tet[4 * (size_t)e];
+    if (c[0] == node || c[1] == node || c[2] == node || c[3] == node) out[n++] = e;
+  }
+  return n;
+}
+
+static uint32_t topo_shared(const topo_t *t, uint32_t a, uint32_t b) {
+  uint32_t n = 0;
+  for (int i = 0; i < 4; ++i)
+    for (int j = 0; j < 4; ++j) n += t->tet[4 * (size_t)a + i] == t->tet[4 * (size_t)b + j];
+  return n;
+}
+
+static uint32_t uf_root(uint32_t *parent, uint32_t x) {
+  while (parent[x] != x) {
+    parent[x] = parent[parent[x]];
+    x = parent[x];
+  }
+  return x;
+}
+
+/* groupsOf (tearing.cc:92-129) with face sharing = 3 common nodes */
+static void topo_groups(const topo_t *t, const uint32_t *fan, uint32_t n, const uint32_t *sides,
+                        uint32_t *group) {
+  uint32_t *parent = (uint32_t *)malloc((n + 1) * sizeof(uint32_t));
+  uint32_t *id = (uint32_t *)malloc((n + 1) * sizeof(uint32_t));
+  for (uint32_t i = 0; i < n; ++i) {
+    parent[i] = i;
+    id[i] = UINT32_MAX;
+  }
+  for (uint32_t i = 0; i < n; ++i)
+    for (uint32_t j = i + 1; j < n; ++j)
+      if (sides[i] == sides[j] && topo_shared(t, fan[i], fan[j]) >= 3u) {
+        const uint32_t a = uf_root(parent, i), b = uf_root(parent, j);
+        if (a != b) parent[a > b ? a : b] = a < b ? a : b;
+      }
+  uint32_t next = 0;
+  for (uint32_t side = 0; side < 2; ++side)
+    for (uint32_t i = 0; i < n; ++i) {
+      if (sides[i] != side) continue;
+      const uint32_t r = uf_root(parent, i);
+      if (id[r] == UINT32_MAX) id[r] = next++;
+      group[i] = id[r];
+    }
+  free(parent);
+  free(id);
+}
+
+/* reachable(seeds, pivot, MIN_PIECE_CELLS = 2) >= 2 */
+static int topo_whole(const topo_t *t, const uint32_t *seeds, uint32_t nseeds, uint32_t pivot) {
+  if (nseeds >= 2) return 1;
+  const uint32_t e = seeds[0];
+  const uint32_t *c = &t->tet[4 * (size_t)e];
+  for (int f = 0; f < 4; ++f) {
+    uint32_t fc[3];
+    int w = 0;
+    for (int k = 0; k < 4; ++k)
+      if (k != f) fc[w++] = c[k];
+    if (fc[0] == pivot || fc[1] == pivot || fc[2] == pivot) continue;
+    for (uint32_t o = 0; o < t->tets; ++o) {
+      if (o == e) continue;
+      const uint32_t *d = &t->tet[4 * (size_t)o];
+      int hits = 0;
+      for (int a = 0; a < 3; ++a)
+        for (int k = 0; k < 4; ++k) hits += d[k] == fc[a];
+      if (hits == 3) return 1;
+    }
+  }
+  return 0;
+}
+
+static int topo_split(topo_t *t, uint32_t node, const uint32_t *fan, uint32_t n,
+                      const uint32_t *group) {
+  uint32_t count = 0;
+  for (uint32_t i = 0; i < n; ++i)
+    if (group[i] + 1 > count) count = group[i] + 1;
+  if (count < 2 || t->anchored[node]) return 0;
+  for (uint32_t piece = 1; piece < count; ++piece) {
+    if (t->nodes >= t->capacity) {
+      t->overflow = 1;
+      return 0;
+    }
+    const uint32_t copy = t->nodes++;
+    for (int k = 0; k < 3; ++k) {
+      t->rest[3 * (size_t)copy + k] = t->rest[3 * (size_t)node + k];
+      t->x[3 * (size_t)copy + k] = t->x[3 * (size_t)node + k];
+    }
+    t->anchored[copy] = 0;
+    t->origin[t->copies++] = node;
+    for (uint32_t i = 0; i < n; ++i) {
+      if (group[i] != piece) continue;
+      uint32_t *c = &t->tet[4 * (size_t)fan[i]];
+      for (int k = 0; k < 4; ++k)
+        if (c[k] == node) c[k] = copy;
+    }
+  }
+  return 1;
+}
+
+static void canonical_dir(double *d) {
+  int best = 0;
+  for (int k = 1; k < 3; ++k)
+    if (fabs(d[k]) > fabs(d[best])) best = k;
+  if (d[best] < 0.0)
+    for (int k = 0; k < 3; ++k) d[k] = -d[k];
+}
+
+uint32_t orc_tear_topology(uint32_t nodes, uint32_t tets, uint32_t *tet, double *rest, double *x,
+                           uint8_t *anchored, uint32_t capacity, const uint32_t *torn,
+                           uint32_t ntorn, uint32_t *origin) {
+  topo_t t = {nodes, tets, capacity, 0, tet, rest, x, anchored, origin, 0};
+  uint32_t *fan = (uint32_t *)malloc((tets + 1) * sizeof(uint32_t));
+  uint32_t *sides = (uint32_t *)malloc((tets + 1) * sizeof(uint32_t));
+  uint32_t *group = (uint32_t *)malloc((tets + 1) * sizeof(uint32_t));
+  uint32_t *seeds = (uint32_t *)malloc((tets + 1) * sizeof(uint32_t));
+  for (uint32_t q = 0; q < ntorn; ++q) {
+    const uint32_t cell = torn[q];
+    if (cell >= tets) continue;
+    uint32_t corners[4];
+    memcpy(corners, &tet[4 * (size_t)cell], sizeof corners);
+    /* crack normal: top eigenvector of F^T F, F = ds Ds^-1 */
+    double Ds[9], ds[9], Dinv[9], F[9], across[3];
+    for (int k = 0; k < 3; ++k)
+      for (int r = 0; r < 3; ++r) {
+        Ds[r * 3 + k] = rest[3 * (size_t)corners[k + 1] + r] - rest[3 * (size_t)corners[0] + r];
+        ds[r * 3 + k] = x[3 * (size_t)corners[k + 1] + r] - x[3 * (size_t)corners[0] + r];
+      }
+    inv3(Ds, Dinv);
+    mul3(ds, Dinv, F);
+    orc_stretch(F, across);
+    canonical_dir(across);
+    double centre[3] = {0, 0, 0};
+    for (int c = 0; c < 4; ++c)
+      for (int k = 0; k < 3; ++k) centre[k] += rest[3 * (size_t)corners[c] + k] / 4.0;
+    double dist[4];
+    int order[4] = {0, 1, 2, 3};
+    for (int c = 0; c < 4; ++c) {
+      double d = 0.0;
+      for (int k = 0; k < 3; ++k) d += (rest[3 * (size_t)corners[c] + k] - centre[k]) * across[k];
+      dist[c] = fabs(d);
+    }
+    for (int i = 0; i < 4; ++i)
+      for (int j = i + 1; j < 4; ++j) {
+        const int a = order[i], b = order[j];
+        if (dist[b] < dist[a] || (dist[b] == dist[a] && corners[b] < corners[a])) {
+          order[i] = b;
+          order[j] = a;
+        }
+      }
+    for (int oi = 0; oi < 4; ++oi) {
+      const uint32_t node = corners[order[oi]];
+      if (anchored[node]) continue;
+      const uint32_t n = topo_fan(&t, node, fan);
+      int low = 0, high = 0;
+      for (uint32_t i = 0; i < n; ++i) {
+        double offset = 0.0;
+        const uint32_t *c = &tet[4 * (size_t)fan[i]];
+        for (int a = 0; a < 4; ++a)
+          for (int k = 0; k < 3; ++k) offset += (rest[3 * (size_t)c[a] + k] - rest[3 * (size_t)node + k]) * across[k];
+        sides[i] = offset > 0.0 ? 1u : 0u;
+        low |= sides[i] == 0u;
+        high |= sides[i] == 1u;
+      }
+      if (!low || !high) continue;
+      topo_groups(&t, fan, n, sides, group);
+      uint32_t count = 0;
+      for (uint32_t i = 0; i < n; ++i)
+        if (group[i] + 1 > count) count = group[i] + 1;
+      int whole = 1;
+      for (uint32_t piece = 0; piece < count && whole; ++piece) {
+        uint32_t ns = 0;
+        for (uint32_t i = 0; i < n; ++i)
+          if (group[i] == piece) seeds[ns++] = fan[i];
+        whole = topo_whole(&t, seeds, ns, node);
+      }
+      if (whole && topo_split(&t, node, fan, n, group)) break;
+    }
+  }
+  if (t.copies) {
+    /* splitPinched: nodes of tets touching a split node, ascending */
+    uint8_t *near = (uint8_t *)calloc(t.nodes + 1, 1);
+    uint8_t *cand = (uint8_t *)calloc(t.nodes + 1, 1);
+    for (uint32_t i = 0; i < t.copies; ++i) {
+      near[nodes + i] = 1;
+      near[origin[i]] = 1;
+    }
+    for (uint32_t e = 0; e < tets; ++e) {
+      const uint32_t *c = &tet[4 * (size_t)e];
+      if (near[c[0]] || near[c[1]] || near[c[2]] || near[c[3]])
+        for (int k = 0; k < 4; ++k) cand[c[k]] = 1;
+    }
+    const uint32_t limit = t.nodes;
+    for (uint32_t node = 0; node < limit; ++node) {
+      if (!cand[node]) continue;
+      const uint32_t n = topo_fan(&t, node, fan);
+      if (n < 2) continue;
+      for (uint32_t i = 0; i < n; ++i) sides[i] = 0;
+      topo_groups(&t, fan, n, sides, group);
+      (void)topo_split(&t, node, fan, n, group);
+    }
+    free(near);
+    free(cand);
+  }
+  free(fan);
+  free(sides);
+  free(group);
+  free(seeds);
+  return t.overflow ? 0u : t.nodes;
+}
+
+uint32_t orc_body_nodes(const orc_body *b) { return b->m.nodes; }
+
+void orc_body_topology(const orc_body *b, uint32_t *tet, double *rest, double *mass,
+                       uint8_t *anchored) {
+  if (tet) memcpy(tet, b->m.tet, 4 * (size_t)b->m.tets * sizeof(uint32_t));
+  if (rest) memcpy(rest, b->m.rest, 3 * (size_t)b->m.nodes * sizeof(double));
+  if (mass) memcpy(mass, b->m.mass, (size_t)b->m.nodes * sizeof(double));
+  if (anchored) memcpy(anchored, b->m.anchored, b->m.nodes);
+}
+
+static void *grow(void *p, size_t bytes) {
+  void *q = realloc(p, bytes ? bytes : 1);
+  return q;
+}
+
+uint32_t orc_body_tear(orc_body *b, const uint32_t *torn, uint32_t ntorn) {
+  const uint32_t N = b->m.nodes, T = b->m.tets;
+  const uint32_t cap = N + 64u * (ntorn + 1u);
+  double *rest = (double *)malloc(3 * (size_t)cap * sizeof(double));
+  double *x = (double *)malloc(3 * (size_t)cap * sizeof(double));
+  uint8_t *anchored = (uint8_t *)malloc(cap);
+  uint32_t *origin = (uint32_t *)malloc((size_t)cap * sizeof(uint32_t));
+  memcpy(rest, b->m.rest, 3 * (size_t)N * sizeof(double));
+  memcpy(x, b->x, 3 * (size_t)N * sizeof(double));
+  memcpy(anchored, b->m.anchored, N);
+  const uint32_t n2 = orc_tear_topology(N, T, b->m.tet, rest, x, anchored, cap, torn, ntorn, origin);
+  const uint32_t added = n2 > N ? n2 - N : 0;
+  if (added) {
+    free(b->m.rest);
+    b->m.rest = rest;
+    free(b->m.anchored);
+    b->m.anchored = anchored;
+    free(b->x);
+    b->x = x;
+    b->v = (double *)grow(b->v, 3 * (size_t)n2 * sizeof(double));
+    b->a = (double *)grow(b->a, 3 * (size_t)n2 * sizeof(double));
+    for (uint32_t i = 0; i < added; ++i)
+      for (int k = 0; k < 3; ++k) {
+        b->v[3 * (size_t)(N + i) + k] = b->v[3 * (size_t)origin[i] + k];
+        b->a[3 * (size_t)(N + i) + k] = b->a[3 * (size_t)origin[i] + k];
+      }
+    b->m.nodes = n2;
+    /* recomputeMasses: rho V / 4 per corner */
+    b->m.mass = (double *)grow(b->m.mass, (size_t)n2 * sizeof(double));
+    memset(b->m.mass, 0, (size_t)n2 * sizeof(double));
+    for (size_t e = 0; e < T; ++e)
+      for (int c = 0; c < 4; ++c)
+        b->m.mass[b->m.tet[4 * e + c]] += b->rho[e] * b->m.volume[e] * 0.25;
+    /* incidence and node scratch for the new node count */
+    free(b->inc_start);
+    free(b->inc_slot);
+    b->inc_start = (uint32_t *)calloc((size_t)n2 + 1, sizeof(uint32_t));
+    for (size_t q = 0; q < 4 * (size_t)T; ++q) ++b->inc_start[b->m.tet[q] + 1];
+    for (size_t n = 0; n < n2; ++n) b->inc_start[n + 1] += b->inc_start[n];
+    b->inc_slot = (uint32_t *)malloc(4 * (size_t)T * sizeof(uint32_t) + 4);
+    uint32_t *cursor = (uint32_t *)malloc((size_t)n2 * sizeof(uint32_t) + 4);
+    memcpy(cursor, b->inc_start, (size_t)n2 * sizeof(uint32_t));
+    for (size_t q = 0; q < 4 * (size_t)T; ++q) b->inc_slot[cursor[b->m.tet[q]]++] = (uint32_t)q;
+    free(cursor);
+    double **vecs[] = {&b->force, &b->right, &b->weighted, &b->shifted, &b->carried, &b->drive,
+                       &b->evaluated, &b->residual, &b->direction, &b->applied,
+                       &b->conditioned, &b->increment, &b->unconstrained};
+    for (size_t i = 0; i < sizeof vecs / sizeof vecs[0]; ++i) {
+      free(*vecs[i]);
+      *vecs[i] = (double *)calloc(3 * (size_t)n2, sizeof(double));
+    }
+    free(b->diag);
+    free(b->precond);
+    b->diag = (double *)calloc(9 * (size_t)n2, sizeof(double));
+    b->precond = (double *)calloc(9 * (size_t)n2, sizeof(double));
+    free(b->partial);
+    b->nblocks = ((size_t)n2 + DOT_BLOCK - 1) / DOT_BLOCK;
+    b->partial = (double *)calloc(b->nblocks + 1, sizeof(double));
+  } else {
+    free(rest);
+    free(x);
+    free(anchored);
+  }
+  free(origin);
+  return added;
 }

@@ -122,6 +122,10 @@ void orc_body_get_rotations(const orc_body *body, double *R);
 void orc_body_get_plastic(const orc_body *body, double *plastic);
 void orc_body_set_plastic(orc_body *body, const double *plastic);
 void orc_body_set_rotations(orc_body *body, const double *R);
+/* Neo-Hookean volume frames cof(Fe) Fp^-T of the last evaluation (9 per tet,
+ * identity for corotated tets); the operator's volumetric direction. */
+void orc_body_get_frames(const orc_body *body, double *W);
+void orc_body_set_frames(orc_body *body, const double *W);
 int orc_step(orc_body *body, const orc_settings *settings, orc_report *report);
 
 /* Per-tet forces (12 per tet) and energies at positions x, and the node
@@ -137,6 +141,21 @@ void orc_apply_system(orc_body *body, double inertia, double step,
 void orc_tet_stretch(orc_body *body, double *stretch);
 /* Yield flags of the last step (1 per tet). */
 void orc_yield_flags(const orc_body *body, uint8_t *flags);
+
+/* Tear topology (tearCells, tearing.cc:485-513, restated for tets) on plain
+ * arrays with room for `capacity` nodes: tet (4*tets, in/out), rest and x
+ * (3*capacity, in/out), anchored (capacity, in/out); torn tets ascending.
+ * Appends split nodes (origin[i] = node that node nodes+i came from) and
+ * returns the new node count. */
+uint32_t orc_tear_topology(uint32_t nodes, uint32_t tets, uint32_t *tet, double *rest,
+                           double *x, uint8_t *anchored, uint32_t capacity,
+                           const uint32_t *torn, uint32_t ntorn, uint32_t *origin);
+/* Tears a body at its current positions; returns the number of new nodes
+ * (state of new nodes copied from their origin, masses recomputed). */
+uint32_t orc_body_tear(orc_body *body, const uint32_t *torn, uint32_t ntorn);
+uint32_t orc_body_nodes(const orc_body *body);
+void orc_body_topology(const orc_body *body, uint32_t *tet, double *rest, double *mass,
+                       uint8_t *anchored);
 
 #ifdef __cplusplus
 }

@@ -137,6 +137,10 @@ _SIGNATURES = {
     "talus_mi_build_lattice": (C.c_int, [_P(_Lattice), _P(PhysicalMaterial), C.c_uint32,
                                          _P(Support), C.c_uint32, _P(_Mesh)]),
     "talus_mi_mesh_free": (None, [_P(_Mesh)]),
+    "talus_mi_free": (None, [C.c_void_p]),
+    "talus_mi_tear_mesh": (C.c_int, [_P(_Mesh), _P(PhysicalMaterial), C.c_uint32, _P(C.c_double),
+                                     _P(C.c_uint32), C.c_uint32, _P(_Mesh), _P(_P(C.c_uint32)),
+                                     _P(C.c_uint32)]),
     "talus_mi_body_add": (C.c_int, [C.c_void_p, _P(_Mesh), _P(PhysicalMaterial), C.c_uint32,
                                     _P(C.c_uint32)]),
     "talus_mi_body_info": (C.c_int, [C.c_void_p, C.c_uint32, _P(C.c_uint32), _P(C.c_uint32)]),
@@ -146,6 +150,8 @@ _SIGNATURES = {
                                      _P(C.c_double)]),
     "talus_mi_rotations_get": (C.c_int, [C.c_void_p, C.c_uint32, _P(C.c_double)]),
     "talus_mi_rotations_set": (C.c_int, [C.c_void_p, C.c_uint32, _P(C.c_double)]),
+    "talus_mi_frames_set": (C.c_int, [C.c_void_p, C.c_uint32, _P(C.c_double)]),
+    "talus_mi_frames_get": (C.c_int, [C.c_void_p, C.c_uint32, _P(C.c_double)]),
     "talus_mi_plastic_get": (C.c_int, [C.c_void_p, C.c_uint32, _P(C.c_double)]),
     "talus_mi_plastic_set": (C.c_int, [C.c_void_p, C.c_uint32, _P(C.c_double)]),
     "talus_mi_topology_get": (C.c_int, [C.c_void_p, C.c_uint32, _P(C.c_uint32), _P(C.c_double),
@@ -261,6 +267,59 @@ def build_lattice(resolution, voxel_size, origin=(0.0, 0.0, 0.0), materials=(), 
     return mesh
 
 
+def _mesh_in(mesh, keep):
+    rest = np.ascontiguousarray(mesh["rest_positions"], dtype=np.float32).reshape(-1)
+    anchored = np.ascontiguousarray(mesh["anchored"], dtype=np.uint8)
+    tets = np.ascontiguousarray(mesh["tet_nodes"], dtype=np.uint32).reshape(-1)
+    lanes = np.ascontiguousarray(mesh["tet_material"], dtype=np.uint32)
+    keep += [rest, anchored, tets, lanes]
+    m = _Mesh()
+    m.nodes = anchored.size
+    m.tets = lanes.size
+    m.rest_positions = rest.ctypes.data_as(C.POINTER(C.c_float))
+    m.anchored = anchored.ctypes.data_as(C.POINTER(C.c_uint8))
+    m.tet_nodes = tets.ctypes.data_as(C.POINTER(C.c_uint32))
+    m.tet_material = lanes.ctypes.data_as(C.POINTER(C.c_uint32))
+    return m
+
+
+def _mesh_out(out):
+    n, t = out.nodes, out.tets
+    return {
+        "rest_positions": np.ctypeslib.as_array(out.rest_positions, (3 * n,)).reshape(n, 3).copy(),
+        "node_mass": np.ctypeslib.as_array(out.node_mass, (n,)).copy(),
+        "anchored": np.ctypeslib.as_array(out.anchored, (n,)).copy(),
+        "tet_nodes": np.ctypeslib.as_array(out.tet_nodes, (4 * t,)).reshape(t, 4).copy(),
+        "rest_inverse": np.ctypeslib.as_array(out.rest_inverse, (9 * t,)).reshape(t, 3, 3).copy(),
+        "volume": np.ctypeslib.as_array(out.volume, (t,)).copy(),
+        "tet_material": np.ctypeslib.as_array(out.tet_material, (t,)).copy(),
+    }
+
+
+def tear_mesh(mesh, materials, x, torn):
+    """Tear topology of a host mesh at positions x (talus_mi_tear_mesh);
+    returns (new mesh dict, origin of every new node)."""
+    L = lib()
+    keep = []
+    m = _mesh_in(mesh, keep)
+    mats = _materials(materials if isinstance(materials, np.ndarray) else list(materials))
+    x = _f64(x)
+    torn = np.ascontiguousarray(torn, dtype=np.uint32)
+    out = _Mesh()
+    origin = C.POINTER(C.c_uint32)()
+    added = C.c_uint32()
+    _check(L.talus_mi_tear_mesh(C.byref(m), mats, len(materials), _dp(x),
+                                torn.ctypes.data_as(C.POINTER(C.c_uint32)), torn.size, C.byref(out),
+                                C.byref(origin), C.byref(added)))
+    try:
+        new = _mesh_out(out)
+        org = np.ctypeslib.as_array(origin, (max(added.value, 1),))[: added.value].copy()
+    finally:
+        L.talus_mi_mesh_free(C.byref(out))
+        L.talus_mi_free(origin)
+    return new, org
+
+
 def default_params(**kw) -> StepParams:
     p = StepParams()
     lib().talus_mi_default_params(C.byref(p))
@@ -345,6 +404,17 @@ class Context:
         _, t = self.info(body)
         R = _f64(R, 9 * t)
         _check(lib().talus_mi_rotations_set(self._h, body, _dp(R)))
+
+    def frames(self, body):
+        _, t = self.info(body)
+        W = np.empty(9 * t)
+        _check(lib().talus_mi_frames_get(self._h, body, _dp(W)))
+        return W.reshape(t, 3, 3)
+
+    def set_frames(self, body, W):
+        _, t = self.info(body)
+        W = _f64(W, 9 * t)
+        _check(lib().talus_mi_frames_set(self._h, body, _dp(W)))
 
     def plastic(self, body):
         _, t = self.info(body)

@@ -17,7 +17,7 @@ LIB = os.path.join(HERE, "libtalus_mi.so")
 
 HIPCC = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")
 ARCH = "gfx950"
-SOURCES = ["kernels.hip", "factor.hip", "context.hip", "mesh.cpp", "factor_symbolic.cpp"]
+SOURCES = ["kernels.hip", "factor.hip", "context.hip", "mesh.cpp", "factor_symbolic.cpp", "tear.cpp"]
 COMMON = [
     "-O3", "-fPIC", "-std=c++20", "-Wall", "-Wno-unused-function",
     f"-I{os.path.join(ROOT, 'include')}",
@@ -34,7 +34,9 @@ def _compile(src: str) -> str:
     if src.endswith(".hip"):
         cmd = [HIPCC, "-x", "hip", *DEVICE, *COMMON, "-c", path, "-o", obj]
     else:
-        cmd = [HIPCC, *COMMON, "-c", path, "-o", obj]
+        # host topology and symbolic code: no contraction, so results match
+        # the oracle's -ffp-contract=off build operation for operation
+        cmd = [HIPCC, *COMMON, "-ffp-contract=off", "-c", path, "-o", obj]
     r = subprocess.run(cmd, capture_output=True, text=True)
     if r.returncode != 0:
         raise RuntimeError(f"compile failed: {' '.join(cmd)}\n{r.stdout}\n{r.stderr}")

@@ -102,6 +102,8 @@ class EngineBase {
   virtual void state_get(uint32_t b, double *x, double *v, double *a) = 0;
   virtual void rotations_get(uint32_t b, double *R) = 0;
   virtual void rotations_set(uint32_t b, const double *R) = 0;
+  virtual void frames_get(uint32_t b, double *W) = 0;
+  virtual void frames_set(uint32_t b, const double *W) = 0;
   virtual void plastic_get(uint32_t b, double *p) = 0;
   virtual void plastic_set(uint32_t b, const double *p) = 0;
   virtual void eval_forces(uint32_t b, const double *x, double *tf, double *te, double *nf,
@@ -127,8 +129,12 @@ class Engine final : public EngineBase {
             std::vector<uint8_t> &torn) override;
   void state_set(uint32_t b, const double *x, const double *v, const double *a) override;
   void state_get(uint32_t b, double *x, double *v, double *a) override;
-  void rotations_get(uint32_t b, double *R) override;
-  void rotations_set(uint32_t b, const double *R) override;
+  void rotations_get(uint32_t b, double *R) override { planes_get(d_.R, b, R); }
+  void rotations_set(uint32_t b, const double *R) override { planes_set(d_.R, b, R); }
+  void frames_get(uint32_t b, double *W) override { planes_get(d_.W, b, W); }
+  void frames_set(uint32_t b, const double *W) override { planes_set(d_.W, b, W); }
+  void planes_get(const Real *base, uint32_t b, double *out);
+  void planes_set(Real *base, uint32_t b, const double *in);
   void plastic_get(uint32_t b, double *p) override;
   void plastic_set(uint32_t b, const double *p) override;
   void eval_forces(uint32_t b, const double *x, double *tf, double *te, double *nf,
@@ -284,7 +290,7 @@ void Engine<Real>::build(const std::vector<Body> &bodies) {
   std::vector<uint4> conn(T);
   std::vector<double> dminv(9 * T), vol64(T), lam64(T), mu64(T), yield(T), hard(T), pmax(T),
       tear(T), plastic(10 * T);
-  std::vector<Real> vol(T), lam(T), mu(T), R(9 * T);
+  std::vector<Real> vol(T), lam(T), mu(T), R(9 * T), W(9 * T);
   std::vector<uint8_t> flags(T);
   any_plastic_ = any_tear_ = false;
   for (uint32_t b = 0; b < B; ++b) {
@@ -305,6 +311,7 @@ void Engine<Real>::build(const std::vector<Body> &bodies) {
       for (int k = 0; k < 9; ++k) {
         dminv[size_t(k) * T + g] = bd.dminv[9 * size_t(e) + k];
         R[size_t(k) * T + g] = Real(bd.R[9 * size_t(e) + k]);
+        W[size_t(k) * T + g] = Real(bd.W[9 * size_t(e) + k]);
       }
       for (int k = 0; k < 10; ++k) plastic[size_t(k) * T + g] = bd.plastic[10 * size_t(e) + k];
       vol64[g] = bd.volume[e];
@@ -362,6 +369,7 @@ void Engine<Real>::build(const std::vector<Body> &bodies) {
   d_.pmax = dev_copy(pmax);
   d_.tear = dev_copy(tear);
   d_.R = dev_copy(R);
+  d_.W = dev_copy(W);
   d_.plastic = dev_copy(plastic);
   d_.slot_f = dev<Real>(12 * T);
   d_.slot_d = dev<Real>(24 * T);
@@ -480,7 +488,8 @@ void Engine<Real>::factorize(const std::vector<double> &reach, const std::vector
   HIPCHECK(hipMemsetAsync(fd_.failed, 0, d_.B * sizeof(int), s));
   FactorLaunch<Real>::assemble(fd_, d_pair_target_, d_pair_ld_, d_pair_node_, d_pair_start_,
                                d_contrib_, uint32_t(t.pair_target.size()), d_.dminv, d_.vol64,
-                               d_.lam64, d_.mu64, d_.R, d_.T, d_.mass, d_body_reach_,
+                               d_.lam64, d_.mu64, d_.R, d_.W, d_.flags, d_.T, d_.mass,
+                               d_body_reach_,
                                d_body_inertia_, d_pair_body_, s);
   size_t ea = 0, fac = 0;
   for (uint32_t l = 0; l < t.levels; ++l) {
@@ -549,6 +558,8 @@ void Engine<Real>::download(std::vector<Body> &bodies) {
     state_get(b, bd.x.data(), bd.v.data(), bd.a.data());
     bd.R.resize(9 * size_t(bd.tets));
     rotations_get(b, bd.R.data());
+    bd.W.resize(9 * size_t(bd.tets));
+    frames_get(b, bd.W.data());
     bd.plastic.resize(10 * size_t(bd.tets));
     plastic_get(b, bd.plastic.data());
   }
@@ -825,23 +836,23 @@ void Engine<Real>::state_get(uint32_t b, double *x, double *v, double *a) {
 }
 
 template <class Real>
-void Engine<Real>::rotations_get(uint32_t b, double *R) {
+void Engine<Real>::planes_get(const Real *base, uint32_t b, double *out) {
   HIPCHECK(hipStreamSynchronize(stream_));
   const size_t T = d_.T, t0 = t0_[b], n = tt_[b];
   std::vector<Real> plane(n);
   for (int k = 0; k < 9; ++k) {
-    HIPCHECK(hipMemcpy(plane.data(), d_.R + k * T + t0, n * sizeof(Real), hipMemcpyDeviceToHost));
-    for (size_t e = 0; e < n; ++e) R[9 * e + k] = double(plane[e]);
+    HIPCHECK(hipMemcpy(plane.data(), base + k * T + t0, n * sizeof(Real), hipMemcpyDeviceToHost));
+    for (size_t e = 0; e < n; ++e) out[9 * e + k] = double(plane[e]);
   }
 }
 
 template <class Real>
-void Engine<Real>::rotations_set(uint32_t b, const double *R) {
+void Engine<Real>::planes_set(Real *base, uint32_t b, const double *in) {
   const size_t T = d_.T, t0 = t0_[b], n = tt_[b];
   std::vector<Real> plane(n);
   for (int k = 0; k < 9; ++k) {
-    for (size_t e = 0; e < n; ++e) plane[e] = Real(R[9 * e + k]);
-    HIPCHECK(hipMemcpy(d_.R + k * T + t0, plane.data(), n * sizeof(Real), hipMemcpyHostToDevice));
+    for (size_t e = 0; e < n; ++e) plane[e] = Real(in[9 * e + k]);
+    HIPCHECK(hipMemcpy(base + k * T + t0, plane.data(), n * sizeof(Real), hipMemcpyHostToDevice));
   }
 }
 
@@ -1077,6 +1088,71 @@ int talus_mi_build_lattice(const talus_mi_lattice *lattice, const talus_mi_mater
 
 void talus_mi_mesh_free(talus_mi_mesh *mesh) { tmi::mesh_free(mesh); }
 
+void talus_mi_free(void *p) { std::free(p); }
+
+int talus_mi_tear_mesh(const talus_mi_mesh *mesh, const talus_mi_material *materials,
+                       uint32_t lanes, const double *x, const uint32_t *torn, uint32_t ntorn,
+                       talus_mi_mesh *out, uint32_t **origin, uint32_t *new_nodes) {
+  if (!mesh || !mesh->rest_positions || !mesh->tet_nodes || !mesh->anchored || !x || !out ||
+      !origin || !new_nodes || (ntorn && !torn) || (lanes && !materials))
+    return fail(TALUS_MI_ERR_ARGUMENT, "bad arguments");
+  try {
+    std::memset(out, 0, sizeof *out);
+    *origin = nullptr;
+    *new_nodes = 0;
+    tmi::Body b;
+    b.nodes = mesh->nodes;
+    b.tets = mesh->tets;
+    b.rest.resize(3 * size_t(b.nodes));
+    for (size_t i = 0; i < b.rest.size(); ++i) b.rest[i] = double(mesh->rest_positions[i]);
+    b.anchored.assign(mesh->anchored, mesh->anchored + b.nodes);
+    b.tet.assign(mesh->tet_nodes, mesh->tet_nodes + 4 * size_t(b.tets));
+    b.lane = mesh->tet_material ? std::vector<uint32_t>(mesh->tet_material, mesh->tet_material + b.tets)
+                                : std::vector<uint32_t>(b.tets, 0u);
+    b.materials.assign(materials, materials + lanes);
+    int rc = tmi::prepare_body(b);
+    if (rc != TALUS_MI_OK) return rc;
+    b.x.assign(x, x + 3 * size_t(b.nodes));
+    b.v.assign(3 * size_t(b.nodes), 0.0);
+    b.a.assign(3 * size_t(b.nodes), 0.0);
+    std::vector<uint32_t> list(torn, torn + ntorn), org;
+    std::sort(list.begin(), list.end());
+    list.erase(std::unique(list.begin(), list.end()), list.end());
+    const uint32_t split = tmi::tear_topology(b, list, org);
+    rc = tmi::prepare_body(b);
+    if (rc != TALUS_MI_OK) return rc;
+    out->nodes = b.nodes;
+    out->tets = b.tets;
+    out->rest_positions = (float *)std::malloc(3 * size_t(b.nodes) * sizeof(float));
+    out->node_mass = (float *)std::malloc(size_t(b.nodes) * sizeof(float));
+    out->anchored = (uint8_t *)std::malloc(b.nodes);
+    out->tet_nodes = (uint32_t *)std::malloc(4 * size_t(b.tets) * sizeof(uint32_t));
+    out->rest_inverse = (float *)std::malloc(9 * size_t(b.tets) * sizeof(float));
+    out->volume = (float *)std::malloc(size_t(b.tets) * sizeof(float));
+    out->tet_material = (uint32_t *)std::malloc(size_t(b.tets) * sizeof(uint32_t));
+    *origin = (uint32_t *)std::malloc(std::max<size_t>(org.size(), 1) * sizeof(uint32_t));
+    if (!out->rest_positions || !out->node_mass || !out->anchored || !out->tet_nodes ||
+        !out->rest_inverse || !out->volume || !out->tet_material || !*origin) {
+      tmi::mesh_free(out);
+      std::free(*origin);
+      *origin = nullptr;
+      return fail(TALUS_MI_ERR_MEMORY, "out of host memory");
+    }
+    for (size_t i = 0; i < b.rest.size(); ++i) out->rest_positions[i] = float(b.rest[i]);
+    for (uint32_t n = 0; n < b.nodes; ++n) out->node_mass[n] = float(b.mass[n]);
+    std::memcpy(out->anchored, b.anchored.data(), b.nodes);
+    std::memcpy(out->tet_nodes, b.tet.data(), b.tet.size() * sizeof(uint32_t));
+    for (size_t i = 0; i < b.dminv.size(); ++i) out->rest_inverse[i] = float(b.dminv[i]);
+    for (uint32_t e = 0; e < b.tets; ++e) out->volume[e] = float(b.volume[e]);
+    std::memcpy(out->tet_material, b.lane.data(), b.lane.size() * sizeof(uint32_t));
+    std::copy(org.begin(), org.end(), *origin);
+    *new_nodes = split;
+    return TALUS_MI_OK;
+  } catch (const std::bad_alloc &) {
+    return fail(TALUS_MI_ERR_MEMORY, "out of host memory");
+  }
+}
+
 int talus_mi_body_add(talus_mi_ctx *ctx, const talus_mi_mesh *mesh,
                       const talus_mi_material *materials, uint32_t lanes, uint32_t *body) {
   if (!mesh || !mesh->rest_positions || !mesh->tet_nodes || !mesh->anchored)
@@ -1108,6 +1184,7 @@ int talus_mi_body_add(talus_mi_ctx *ctx, const talus_mi_mesh *mesh,
     b.a.assign(3 * size_t(b.nodes), 0.0);
     b.R.assign(9 * size_t(b.tets), 0.0);
     for (uint32_t e = 0; e < b.tets; ++e) b.R[9 * size_t(e)] = b.R[9 * size_t(e) + 4] = b.R[9 * size_t(e) + 8] = 1.0;
+    b.W = b.R;
     b.plastic.assign(10 * size_t(b.tets), 0.0);
     ctx->engine->download(ctx->bodies);
     ctx->bodies.push_back(std::move(b));
@@ -1161,6 +1238,24 @@ int talus_mi_rotations_set(talus_mi_ctx *ctx, uint32_t body, const double *R) {
   });
 }
 
+int talus_mi_frames_get(talus_mi_ctx *ctx, uint32_t body, double *W) {
+  if (!ctx || !body_ok(ctx, body) || !W) return fail(TALUS_MI_ERR_ARGUMENT, "bad arguments");
+  return guarded(ctx, [&]() -> int {
+    ensure(ctx);
+    ctx->engine->frames_get(body, W);
+    return TALUS_MI_OK;
+  });
+}
+
+int talus_mi_frames_set(talus_mi_ctx *ctx, uint32_t body, const double *W) {
+  if (!ctx || !body_ok(ctx, body) || !W) return fail(TALUS_MI_ERR_ARGUMENT, "bad arguments");
+  return guarded(ctx, [&]() -> int {
+    ensure(ctx);
+    ctx->engine->frames_set(body, W);
+    return TALUS_MI_OK;
+  });
+}
+
 int talus_mi_plastic_get(talus_mi_ctx *ctx, uint32_t body, double *p) {
   if (!ctx || !body_ok(ctx, body) || !p) return fail(TALUS_MI_ERR_ARGUMENT, "bad arguments");
   return guarded(ctx, [&]() -> int {
@@ -1201,6 +1296,27 @@ int talus_mi_step(talus_mi_ctx *ctx, const talus_mi_step_params *params, talus_m
     std::vector<uint8_t> torn;
     ctx->engine->step(*params, *report, ctx->events, ctx->steps, torn);
     ++ctx->steps;
+    if ((params->flags & TALUS_MI_DETECT_TEARS) &&
+        std::any_of(torn.begin(), torn.end(), [](uint8_t t) { return t != 0; })) {
+      // host topology update after the step (overstretched + tearCells)
+      ctx->engine->download(ctx->bodies);
+      size_t t0 = 0;
+      for (tmi::Body &b : ctx->bodies) {
+        std::vector<uint32_t> list, origin;
+        for (uint32_t e = 0; e < b.tets; ++e)
+          if (torn[t0 + e]) list.push_back(e);
+        t0 += b.tets;
+        if (list.empty()) continue;
+        const uint32_t split = tmi::tear_topology(b, list, origin);
+        if (!split) continue;
+        b.density_mass = true;
+        const int rc = tmi::prepare_body(b);
+        if (rc != TALUS_MI_OK) return rc;
+        report->split_nodes += split;
+        report->nodes += split;
+        ctx->dirty = true;
+      }
+    }
     return TALUS_MI_OK;
   });
 }

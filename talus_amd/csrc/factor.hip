@@ -29,13 +29,16 @@ __device__ __forceinline__ uint32_t hi16(uint64_t t) { return uint32_t(t >> 48);
 // ------------------------------------------------------------- assembly
 
 // One lower node block of A = inertia M + h^2 sum_e R K_e R^T per thread,
-// summed over its tets in ascending order (factor.cc:32-88 precise blocks).
+// summed over its tets in ascending order (factor.cc:32-88 precise blocks);
+// neo-Hookean tets contribute the operator's blocks (shear part rotated,
+// volumetric part along the volume frame W).
 template <class Real>
 __global__ __launch_bounds__(NT) void k_assemble_pairs(
     double *pool, const uint64_t *pair_target, const uint32_t *pair_ld, const uint32_t *pair_node,
     const uint32_t *pair_start, const uint32_t *contrib, uint32_t pairs, const double *dminv,
-    const double *vol, const double *lam, const double *mu, const Real *R, uint32_t T,
-    const double *mass, const double *reach_of, const double *inertia_of, const uint32_t *pair_body) {
+    const double *vol, const double *lam, const double *mu, const Real *R, const Real *W,
+    const uint8_t *flags, uint32_t T, const double *mass, const double *reach_of,
+    const double *inertia_of, const uint32_t *pair_body) {
   const uint32_t q = blockIdx.x * NT + threadIdx.x;
   if (q >= pairs) return;
   double B[9] = {0, 0, 0, 0, 0, 0, 0, 0, 0};
@@ -50,7 +53,8 @@ __global__ __launch_bounds__(NT) void k_assemble_pairs(
     }
     shape_gradients(dm, g);
     const double *ga = &g[3 * a], *gb = &g[3 * b];
-    const double V = vol[e], l = lam[e], m = mu[e];
+    const bool snh = (flags[e] & FLAG_SNH) != 0;
+    const double V = vol[e], l = snh ? 0.0 : lam[e], m = mu[e];
     const double gg = ga[0] * gb[0] + ga[1] * gb[1] + ga[2] * gb[2];
     double K[9];
 #pragma unroll
@@ -63,6 +67,19 @@ __global__ __launch_bounds__(NT) void k_assemble_pairs(
     mul3_nt(RK, Rm, RKR);
 #pragma unroll
     for (int k = 0; k < 9; ++k) B[k] += RKR[k];
+    if (snh) {
+      // volumetric part along the volume frame: V lambda (W g_a)(W g_b)^T
+      double Wm[9], wa[3], wb[3];
+#pragma unroll
+      for (int k = 0; k < 9; ++k) Wm[k] = double(W[size_t(k) * T + e]);
+      mv3(Wm, ga, wa);
+      mv3(Wm, gb, wb);
+      const double vl = V * lam[e];
+#pragma unroll
+      for (int r = 0; r < 3; ++r)
+#pragma unroll
+        for (int s = 0; s < 3; ++s) B[r * 3 + s] += vl * wa[r] * wb[s];
+    }
   }
   const uint32_t body = pair_body[q];
   const double reach = reach_of[body];
@@ -542,14 +559,15 @@ void FactorLaunch<Real>::assemble(const FactorDev &f, const uint64_t *pair_targe
                                   const uint32_t *pair_ld, const uint32_t *pair_node,
                                   const uint32_t *pair_start, const uint32_t *contrib,
                                   uint32_t pairs, const double *dminv, const double *vol,
-                                  const double *lam, const double *mu, const Real *R, uint32_t T,
+                                  const double *lam, const double *mu, const Real *R,
+                                  const Real *W, const uint8_t *flags, uint32_t T,
                                   const double *mass, const double *body_reach,
                                   const double *body_inertia, const uint32_t *pair_body,
                                   hipStream_t s) {
   if (!pairs) return;
   hipLaunchKernelGGL(k_assemble_pairs<Real>, dim3((pairs + NT - 1) / NT), dim3(NT), 0, s, f.pool,
                      pair_target, pair_ld, pair_node, pair_start, contrib, pairs, dminv, vol, lam,
-                     mu, R, T, mass, body_reach, body_inertia, pair_body);
+                     mu, R, W, flags, T, mass, body_reach, body_inertia, pair_body);
 }
 
 void factor_extend_add(const FactorDev &f, const uint32_t *parent, const uint64_t *tasks,

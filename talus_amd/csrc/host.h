@@ -40,11 +40,18 @@ struct Body {
   // state mirror (valid when the device copy is not)
   std::vector<double> x, v, a;
   std::vector<double> R;        // 9*tets
+  std::vector<double> W;        // 9*tets, neo-Hookean volume frames
   std::vector<double> plastic;  // 10*tets
   bool density_mass = true;     // masses follow rho V / 4 (recomputed on tears)
 };
 
 // Fills per-tet material data and FP64 rest frames; returns a status.
 int prepare_body(Body &body);
+
+// Tear topology update (tear.cpp): splits nodes around the torn tets
+// (ascending body-local indices) at the body's positions b.x; new nodes are
+// appended and origin[i] is the node that node (old count + i) came from.
+// Returns the number of new nodes.  Masses are not recomputed here.
+uint32_t tear_topology(Body &body, const std::vector<uint32_t> &torn, std::vector<uint32_t> &origin);
 
 }  // namespace tmi

@@ -64,6 +64,7 @@ struct Dev {
   const uint8_t *flags;  // bit0 neo-Hookean, bit1 plastic lane, bit2 tearable
   const double *yield, *hard, *pmax, *tear;
   Real *R;
+  Real *W;          // neo-Hookean volume frame cof(Fe) Fp^-T, 9 SoA planes
   double *plastic;  // 10 SoA planes
   Real *slot_f, *slot_d, *energy;
   uint8_t *yflag, *tflag;

@@ -291,8 +291,13 @@ __global__ __launch_bounds__(TILE) void k_pcg_begin(Dev<Real> d) {
 
 // ---------------------------------------------------------------- tets
 
-// Per-tet operator y_i = V R sigma(R^T d) g_i, sigma(D) = lambda tr(D) I +
-// mu (D + D^T); equals sum_j R K_ij R^T d_j (applyStiffness).
+// Per-tet operator y_i = V R sigma(R^T d) g_i, sigma(D') = lambda tr(D') I +
+// mu (D' + D'^T); equals sum_j R K_ij R^T d_j (applyStiffness).  Neo-Hookean
+// tets keep the shear part and take the volumetric part along the volume
+// frame W = cof(Fe) Fp^-T: y_i = V(R mu (D' + D'^T) g_i + lambda (W:D) W g_i)
+// with D = sum_j d_j g_j^T, so W:D = sum_j d_j . (W g_j).  At rest both are
+// the linear stiffness; under stretch the volumetric part follows the
+// (lambda+mu)(J-1)^2 term's stiffening along cof(F).
 template <class Real>
 __device__ __forceinline__ void op_tet(const Dev<Real> &d, uint32_t e, const Real *dir, Real *y) {
   const uint4 c = d.conn[e];
@@ -312,8 +317,9 @@ __device__ __forceinline__ void op_tet(const Dev<Real> &d, uint32_t e, const Rea
 #pragma unroll
       for (int k = 0; k < 3; ++k) D[r * 3 + k] += dp[r] * g[3 * j + k];
   }
+  const bool snh = (d.flags[e] & FLAG_SNH) != 0;
   const Real lam = d.lam[e], mu = d.mu[e], V = d.vol[e];
-  const Real tr = D[0] + D[4] + D[8];
+  const Real tr = snh ? Real(0) : D[0] + D[4] + D[8];
   Real S[9];
 #pragma unroll
   for (int r = 0; r < 3; ++r)
@@ -325,6 +331,26 @@ __device__ __forceinline__ void op_tet(const Dev<Real> &d, uint32_t e, const Rea
     Real tmp[3];
     mv3(S, &g[3 * i], tmp);
     mv3(Rm, tmp, &y[3 * i]);
+  }
+  if (snh) {
+    Real Wm[9];
+#pragma unroll
+    for (int k = 0; k < 9; ++k) Wm[k] = d.W[(size_t)k * d.T + e];
+    Real vol = Real(0);
+#pragma unroll
+    for (int j = 0; j < 4; ++j) {
+      Real wg[3];
+      mv3(Wm, &g[3 * j], wg);
+      vol += dv[3 * j] * wg[0] + dv[3 * j + 1] * wg[1] + dv[3 * j + 2] * wg[2];
+    }
+    vol *= V * lam;
+#pragma unroll
+    for (int i = 0; i < 4; ++i) {
+      Real wg[3];
+      mv3(Wm, &g[3 * i], wg);
+#pragma unroll
+      for (int r = 0; r < 3; ++r) y[3 * i + r] += vol * wg[r];
+    }
   }
 }
 
@@ -585,6 +611,7 @@ __global__ __launch_bounds__(TILE) void k_assemble(Dev<Real> d) {
     const Real V = d.vol[e], lam = d.lam[e], mu = d.mu[e];
     Real f[12];
     Real psi;
+    Real Wm[9];
     if (flags & FLAG_SNH) {
       Real He[9], PF[9];
       if (flags & FLAG_PLASTIC) {
@@ -600,9 +627,18 @@ __global__ __launch_bounds__(TILE) void k_assemble(Dev<Real> d) {
         ep[4] += Real(1);
         ep[8] += Real(1);
         mul3_nt(Pk, ep, PF);
+        // volume frame cof(Fe) Fp^-T, Fe = I + He, Fp^-1 = I + ep
+        Real Fe[9], C[9];
+#pragma unroll
+        for (int k = 0; k < 9; ++k) Fe[k] = He[k] + ((k % 4 == 0) ? Real(1) : Real(0));
+        cof3(Fe, C);
+        mul3_nt(C, ep, Wm);
       } else {
         psi = snh_stress(H, lam, mu, PF);
+        cof3(F, Wm);
       }
+#pragma unroll
+      for (int k = 0; k < 9; ++k) d.W[(size_t)k * d.T + e] = Wm[k];
 #pragma unroll
       for (int i = 0; i < 4; ++i) {
         Real tmp[3];
@@ -653,20 +689,27 @@ __global__ __launch_bounds__(TILE) void k_assemble(Dev<Real> d) {
 #pragma unroll
     for (int k = 0; k < 9; ++k) d.R[(size_t)k * d.T + e] = Rm[k];
     if (DIAG) {
-      // stiffnessDiagonal: R K_ii R^T = V((lambda+mu)(Rg)(Rg)^T + mu |g|^2 I)
-      const Real lm = V * (lam + mu), vm = V * mu;
+      // stiffnessDiagonal: R K_ii R^T = V((lambda+mu)(Rg)(Rg)^T + mu |g|^2 I);
+      // neo-Hookean: V(mu (Rg)(Rg)^T + lambda (Wg)(Wg)^T + mu |g|^2 I)
+      const bool snh = (flags & FLAG_SNH) != 0;
+      const Real lr = V * (snh ? mu : lam + mu), lw = snh ? V * lam : Real(0), vm = V * mu;
+      if (!snh) {
+#pragma unroll
+        for (int k = 0; k < 9; ++k) Wm[k] = Real(0);
+      }
       Real *o = d.slot_d + (size_t)e * 24;
 #pragma unroll
       for (int i = 0; i < 4; ++i) {
-        Real rg[3];
+        Real rg[3], wg[3];
         mv3(Rm, &g[3 * i], rg);
+        mv3(Wm, &g[3 * i], wg);
         const Real gg = g[3 * i] * g[3 * i] + g[3 * i + 1] * g[3 * i + 1] + g[3 * i + 2] * g[3 * i + 2];
-        o[6 * i + 0] = lm * rg[0] * rg[0] + vm * gg;
-        o[6 * i + 1] = lm * rg[0] * rg[1];
-        o[6 * i + 2] = lm * rg[0] * rg[2];
-        o[6 * i + 3] = lm * rg[1] * rg[1] + vm * gg;
-        o[6 * i + 4] = lm * rg[1] * rg[2];
-        o[6 * i + 5] = lm * rg[2] * rg[2] + vm * gg;
+        o[6 * i + 0] = lr * rg[0] * rg[0] + lw * wg[0] * wg[0] + vm * gg;
+        o[6 * i + 1] = lr * rg[0] * rg[1] + lw * wg[0] * wg[1];
+        o[6 * i + 2] = lr * rg[0] * rg[2] + lw * wg[0] * wg[2];
+        o[6 * i + 3] = lr * rg[1] * rg[1] + lw * wg[1] * wg[1] + vm * gg;
+        o[6 * i + 4] = lr * rg[1] * rg[2] + lw * wg[1] * wg[2];
+        o[6 * i + 5] = lr * rg[2] * rg[2] + lw * wg[2] * wg[2] + vm * gg;
       }
     }
     if (ENERGY) {

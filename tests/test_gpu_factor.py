@@ -29,8 +29,8 @@ def dense_system(orc, m, inertia, step):
 def _check_solve(body, x=None, bar=1e-10, precision=api.FP64):
     orc, m = oracle_body(body)
     if x is not None:
-        orc.eval_forces(x)  # commits rotations
-    R = orc.rotations()
+        orc.eval_forces(x)  # commits rotations and volume frames
+    R, W = orc.rotations(), orc.frames()
     inertia, step = 1.0, math.sqrt(8.0 / 27.0) / 60.0
     A, free = dense_system(orc, m, inertia, step)
     rng = np.random.default_rng(21)
@@ -41,6 +41,7 @@ def _check_solve(body, x=None, bar=1e-10, precision=api.FP64):
     with api.Context(0, precision) as ctx:
         b, _ = product_body(ctx, body)
         ctx.set_rotations(b, R)
+        ctx.set_frames(b, W)
         z = ctx.factor_solve(b, inertia, step, r).reshape(-1)
         info = ctx.factor_info()
     assert info["ok"] == 1 and info["fronts"] > 0

@@ -76,14 +76,15 @@ def test_operator_matches_oracle():
     body = _scene_body()
     orc, om = oracle_body(body)
     x = deformed(om["rest"], seed=4, scale=0.002)
-    orc.eval_forces(x)  # commits rotations
-    R = orc.rotations()
+    orc.eval_forces(x)  # commits rotations and volume frames
+    R, W = orc.rotations(), orc.frames()
     d = np.random.default_rng(5).standard_normal(om["rest"].shape)
     inertia, step = 1.0, np.sqrt(8.0 / 27.0) / 60.0
     y_o = orc.apply_system(inertia, step, d)
     with api.Context(0, api.FP64) as ctx:
         b, _ = product_body(ctx, body)
         ctx.set_rotations(b, R)
+        ctx.set_frames(b, W)
         y = ctx.apply_system(b, inertia, step, d)
     assert np.abs(y - y_o).max() <= 1e-12 * np.abs(y_o).max()
 

@@ -1,0 +1,126 @@
+"""GPU tearing: overstretched detection inside the step (tearing.cc:469-483)
+against the oracle's stretch, the host topology update the step runs after
+it against the oracle's restatement, and the strip pulled apart
+(tests/dynamics_strainstep_tests.cc:3720-3755: pieces >= 2, no cell removed,
+mass kept to 1e-6, more nodes than before, no orphan node, finite state)."""
+import numpy as np
+import pytest
+
+import oracle as O
+from talus_amd import api, scenes
+
+from helpers import oracle_body, product_body
+from test_tear_cpu import pieces
+
+pytestmark = pytest.mark.gpu
+
+TEAR_STRAIN = 0.4
+
+
+def strip_body(model, cells=(12, 2, 2), h=0.2):
+    """The reference's tearable jelly strip (E 5 kPa, rho 20, nu 0.3, tear
+    strain 0.4; dynamics_strainstep_tests.cc:60-69, 905-911) 2.4 m long,
+    held at both ends."""
+    mats = scenes.material_array(1, youngs_modulus=5e3, poissons_ratio=0.3, density=20.0,
+                                 elastic_model=model, tear_strain=TEAR_STRAIN)
+    L = cells[0] * h
+    ends = [((0.0, 0.0, 0.0), (0.01, 10.0, 10.0)), ((L, 0.0, 0.0), (0.01, 10.0, 10.0))]
+    return scenes.Body(resolution=tuple(c + 1 for c in cells), voxel=(h, h, h), supports=ends,
+                       materials=mats)
+
+
+MODELS = [api.COROTATED, api.NEO_HOOKEAN]
+
+
+@pytest.mark.parametrize("model", MODELS)
+def test_tears_detected_and_split_like_the_oracle(model):
+    body = strip_body(model)
+    orc, om = oracle_body(body)
+    rest = om["rest"]
+    x0 = rest.copy()
+    x0[rest[:, 0] > 1.3, 0] += 0.3  # one hex column stretched 2.5x along x
+    zero = np.zeros_like(x0)
+    orc.set_state(x0, zero, zero)
+    orc.step(tolerance=1e-10, cg_iterations=2000)
+    stretch = orc.stretch()
+    torn_o = np.flatnonzero(stretch > 1.0 + np.float32(TEAR_STRAIN))
+    assert torn_o.size > 0
+    # no stretch within 1e-9 of the threshold: the detection is not a tie
+    assert np.abs(stretch - (1.0 + np.float32(TEAR_STRAIN))).min() > 1e-9
+    n0 = rest.shape[0]
+    with api.Context(0, api.FP64) as ctx:
+        b, mesh = product_body(ctx, body)
+        ctx.set_state(b, x0, zero, zero)
+        rep = ctx.step(tolerance=1e-10, cg_iterations=2000,
+                       flags=api.DETECT_TEARS | api.RECORD_EVENTS)
+        ev = ctx.events()
+        topo = ctx.topology(b)
+        x, v, a = ctx.state(b)
+        torn = np.array(sorted(e[2] for e in ev if e[0] == api.EVENT_TEAR))
+        np.testing.assert_array_equal(torn, torn_o)
+        assert rep["tear_events"] == torn.size
+        assert rep["split_nodes"] == x.shape[0] - n0 > 0
+        # the step's host update equals the restatement at the step's positions
+        ot, orest, _, oan, oorg = O.tear_topology(mesh["tet_nodes"], rest, x[:n0], mesh["anchored"],
+                                                  torn)
+        np.testing.assert_array_equal(topo["tet_nodes"], ot)
+        np.testing.assert_array_equal(topo["anchored"], oan)
+        np.testing.assert_array_equal(topo["rest"], orest)
+        back = np.concatenate([np.arange(n0), oorg])
+        for i in range(n0, back.size):
+            back[i] = back[back[i]]
+        np.testing.assert_array_equal(x[n0:], x[back[n0:]])
+        np.testing.assert_array_equal(v[n0:], v[back[n0:]])
+        # masses as the oracle recomputes them after the same tear
+        xs, vs, as_ = orc.state()
+        orc.set_state(x[:n0], v[:n0], a[:n0])
+        assert orc.tear(torn) == x.shape[0] - n0
+        ot2 = orc.topology()
+        np.testing.assert_allclose(topo["mass"], ot2["mass"], rtol=1e-12)
+        assert abs(topo["mass"].sum() - om["mass"].sum()) <= 1e-9 * om["mass"].sum()
+        # both step on from the torn state and agree
+        orc.set_state(x, v, a)
+        orc.step(tolerance=1e-10, cg_iterations=2000)
+        rep2 = ctx.step(tolerance=1e-10, cg_iterations=2000)
+        x2, _, _ = ctx.state(b)
+        xo2, _, _ = orc.state()
+        assert not rep2["non_finite"]
+        assert np.abs(x2 - xo2).max() <= 1e-8 * np.abs(x - x0[back]).max()
+
+
+@pytest.mark.parametrize("model", MODELS)
+@pytest.mark.parametrize("precision", [api.FP64, api.MIXED])
+def test_strip_pulled_apart_tears_into_pieces(precision, model):
+    body = strip_body(model)
+    with api.Context(0, precision) as ctx:
+        b, mesh = product_body(ctx, body)
+        n0, t0 = ctx.info(b)
+        mass0 = ctx.topology(b)["mass"].sum()
+        rest = mesh["rest_positions"]
+        right = np.flatnonzero(rest[:, 0] > 2.39)
+        x, _, _ = ctx.state(b)
+        start = x[right, 0].copy()
+        tears = 0
+        for frame in range(240):
+            x, v, a = ctx.state(b)
+            x[right, 0] = start + 2.4 * min(frame / 180.0, 1.0)
+            ctx.set_state(b, x, v, a)
+            rep = ctx.step(flags=api.DETECT_TEARS)
+            assert not rep["non_finite"], frame
+            tears += rep["tear_events"]
+        topo = ctx.topology(b)
+        n, t = ctx.info(b)
+        x, _, _ = ctx.state(b)
+    assert tears > 0
+    assert t == t0
+    assert n > n0
+    assert abs(topo["mass"].sum() - mass0) <= 1e-6 * mass0
+    assert np.unique(topo["tet_nodes"]).size == n  # no orphans
+    assert pieces(topo["tet_nodes"], n) >= 2
+    # longest edge stretch below 1.8 (dynamics_strainstep_tests.cc:3757)
+    T = topo["tet_nodes"]
+    for i in range(4):
+        for j in range(i + 1, 4):
+            now = np.linalg.norm(x[T[:, i]] - x[T[:, j]], axis=1)
+            was = np.linalg.norm(topo["rest"][T[:, i]] - topo["rest"][T[:, j]], axis=1)
+            assert (now / was).max() < 1.8

@@ -324,10 +324,59 @@ def test_operator_is_symmetric_and_matches_snh_tangent_at_rest():
     fp = orc.eval_forces(m["rest"] + h * d1)[2]
     fm = orc.eval_forces(m["rest"] - h * d1)[2]
     orc.set_rotations(np.tile(np.eye(3), (m["tet"].shape[0], 1, 1)))
+    orc.set_frames(np.tile(np.eye(3), (m["tet"].shape[0], 1, 1)))
     Kd = (orc.apply_system(0.0, 1.0, d1))
     fd = -(fp - fm) / (2 * h)
     fd[~free] = 0
     assert np.abs(Kd - fd).max() <= 1e-6 * np.abs(fd).max()
+
+
+@pytest.mark.parametrize("plastic", [False, True])
+def test_snh_operator_is_rotated_shear_plus_volume_gradient_under_stretch(plastic):
+    """Under a large stretch and turn, d^T K d of a neo-Hookean body equals
+    sum_e V (2 mu |sym(R^T D)|^2 + lambda (dJ_e/dt)^2), with R the polar
+    factor of F (numpy SVD), D = sum_j d_j g_j^T and dJ_e/dt the derivative
+    of det(F_e) = det(F Fp^-1) along D (central differences of numpy det).
+    A transposed cofactor, a dropped Fp^-1 or a wrong coefficient fails it."""
+    body = _tiny_body(anchored=False)
+    if plastic:
+        body.materials["yield_stress"] = 1e12  # a plastic lane that does not flow here
+    orc, m = oracle_body(body)
+    T = m["tet"].shape[0]
+    rng = np.random.default_rng(21)
+    Fp_inv = np.tile(np.eye(3), (T, 1, 1))
+    if plastic:
+        pl = np.zeros((T, 10))
+        ep = 0.05 * rng.standard_normal((T, 3, 3))
+        pl[:, :9] = ep.reshape(T, 9)
+        orc.set_plastic(pl)
+        Fp_inv = Fp_inv + ep
+    X = m["rest"]
+    A = rot([1, 2, 3], 0.7) @ np.diag([1.6, 0.8, 0.9])
+    x = X @ A.T + 0.002 * rng.standard_normal(X.shape)
+    orc.eval_forces(x)  # evaluates rotations and volume frames at x
+    lam, mu = O.lame(1e5, 0.3)
+    tets, V = m["tet"], m["volume"]
+    for trial in range(3):
+        d = rng.standard_normal(X.shape)
+        Kd = orc.apply_system(0.0, 1.0, d)
+        quad = (d * Kd).sum()
+        expect = 0.0
+        for e in range(T):
+            t = tets[e]
+            Ds = (X[t[1:]] - X[t[0]]).T
+            ds = (x[t[1:]] - x[t[0]]).T
+            F = ds @ np.linalg.inv(Ds)
+            assert np.linalg.det(F) > 0
+            U, _, Vt = np.linalg.svd(F)
+            R = U @ Vt
+            D = (d[t[1:]] - d[t[0]]).T @ np.linalg.inv(Ds)
+            Dp = R.T @ D
+            sym = 0.5 * (Dp + Dp.T)
+            h = 1e-6
+            dJ = (np.linalg.det((F + h * D) @ Fp_inv[e]) - np.linalg.det((F - h * D) @ Fp_inv[e])) / (2 * h)
+            expect += V[e] * (2 * mu * (sym * sym).sum() + lam * dJ * dJ)
+        assert quad == pytest.approx(expect, rel=1e-6)
 
 
 def test_pcg_step_solution_solves_the_linear_system():

@@ -1,0 +1,168 @@
+"""Tear topology (host side, no GPU): the library's talus_mi_tear_mesh against
+the oracle's restatement, and the invariants the reference's strip tests hold
+(tests/dynamics_strainstep_tests.cc:3720-3796: no cell removed, mass kept to
+1e-6, at least two pieces, no orphan node, split nodes start where their
+origin is)."""
+from __future__ import annotations
+
+import numpy as np
+import pytest
+
+import oracle as O
+from talus_amd import api, scenes
+
+from helpers import oracle_body
+
+
+def tearable(**kw):
+    return scenes.material_array(1, youngs_modulus=1e6, tear_strain=0.4,
+                                 elastic_model=api.NEO_HOOKEAN, **kw)
+
+
+def strip(cells=(6, 1, 1), voxel=0.2, supports=()):
+    body = scenes.Body(resolution=tuple(c + 1 for c in cells), voxel=(voxel,) * 3,
+                       materials=tearable(), supports=list(supports))
+    mesh, mats = scenes.build_mesh(body)
+    return body, mesh, mats
+
+
+def pieces(tet, nodes):
+    """Node-connected components over the tets."""
+    parent = np.arange(nodes)
+
+    def root(a):
+        while parent[a] != a:
+            parent[a] = parent[parent[a]]
+            a = parent[a]
+        return a
+
+    for t in tet:
+        for c in t[1:]:
+            a, b = root(int(t[0])), root(int(c))
+            if a != b:
+                parent[max(a, b)] = min(a, b)
+    return len({root(int(n)) for n in np.unique(tet)})
+
+
+def check_tear(mesh, mats, x, torn):
+    """Tears with the library and the oracle; asserts identical topology and
+    the tear invariants; returns the library's mesh and origins."""
+    new, origin = api.tear_mesh(mesh, scenes.materials_list(mats), x, torn)
+    rest = mesh["rest_positions"].astype(np.float64)
+    ot, orest, ox, oan, oorg = O.tear_topology(mesh["tet_nodes"], rest, x, mesh["anchored"], torn)
+    n0 = mesh["anchored"].size
+    # identical connectivity, origins, anchoring and rest positions
+    np.testing.assert_array_equal(new["tet_nodes"], ot)
+    np.testing.assert_array_equal(origin, oorg)
+    np.testing.assert_array_equal(new["anchored"], oan)
+    np.testing.assert_array_equal(new["rest_positions"], orest.astype(np.float32))
+    # no tet removed, every tet keeps its corners up to the split copies
+    assert new["tet_nodes"].shape == mesh["tet_nodes"].shape
+    back = np.concatenate([np.arange(n0), origin]).astype(np.uint32)
+    while True:  # copies of copies map back in several hops
+        nxt = back[back]
+        if np.array_equal(nxt, back):
+            break
+        back = nxt
+    np.testing.assert_array_equal(back[new["tet_nodes"]], mesh["tet_nodes"])
+    # new nodes copy the rest position of their origin; none is anchored
+    np.testing.assert_array_equal(new["rest_positions"][n0:], mesh["rest_positions"][back[n0:]])
+    assert not new["anchored"][n0:].any()
+    # no orphan node; mass kept
+    assert np.unique(new["tet_nodes"]).size == new["anchored"].size
+    m0, m1 = float(mesh["node_mass"].astype(np.float64).sum()), float(new["node_mass"].astype(np.float64).sum())
+    assert abs(m1 - m0) <= 1e-6 * m0
+    # masses are rho V / 4 per incident tet on the torn mesh
+    rho = float(mats["density"][0]) if mats.shape[0] == 1 else None
+    if rho is not None:
+        expect = np.zeros(new["anchored"].size)
+        np.add.at(expect, new["tet_nodes"].reshape(-1),
+                  np.repeat(rho * new["volume"].astype(np.float64) / 4.0, 4))
+        np.testing.assert_allclose(new["node_mass"], expect, rtol=1e-5)
+    return new, origin
+
+
+def test_strip_cut_through_one_hex_column_falls_apart():
+    _, mesh, mats = strip()
+    rest = mesh["rest_positions"].astype(np.float64)
+    x = rest.copy()
+    x[:, 0] *= 1.01
+    cent = rest[mesh["tet_nodes"]].mean(axis=1)
+    torn = np.nonzero((cent[:, 0] > 0.4) & (cent[:, 0] < 0.6))[0]
+    new, origin = check_tear(mesh, mats, x, torn)
+    assert origin.size > 0
+    assert pieces(new["tet_nodes"], new["anchored"].size) >= 2
+
+
+def test_pinned_nodes_are_never_split():
+    sups = [((0.6, 0.0, 0.0), (1.0, 1.0, 1.0))]  # every node anchored
+    _, mesh, mats = strip(supports=sups)
+    assert mesh["anchored"].all()
+    x = mesh["rest_positions"].astype(np.float64) * 1.05
+    new, origin = check_tear(mesh, mats, x, np.arange(mesh["tet_nodes"].shape[0]))
+    assert origin.size == 0
+    np.testing.assert_array_equal(new["tet_nodes"], mesh["tet_nodes"])
+
+
+def test_empty_and_out_of_range_torn_lists_change_nothing():
+    _, mesh, mats = strip()
+    x = mesh["rest_positions"].astype(np.float64)
+    for torn in (np.zeros(0, np.uint32), np.array([10 ** 6], np.uint32)):
+        new, origin = check_tear(mesh, mats, x, torn)
+        assert origin.size == 0
+        np.testing.assert_array_equal(new["tet_nodes"], mesh["tet_nodes"])
+
+
+@pytest.mark.parametrize("seed", [0, 1, 2, 3])
+def test_random_tears_of_a_jittered_block_match_the_oracle(seed):
+    rng = np.random.default_rng(seed)
+    cells = (5, 4, 3)
+    h = 0.1
+    res = tuple(c + 1 for c in cells)
+    iz, iy, ix = np.meshgrid(*(np.arange(r) for r in res[::-1]), indexing="ij")
+    pos = (np.stack([ix, iy, iz], -1).reshape(-1, 3) * h
+           + rng.uniform(-0.05, 0.05, (ix.size, 3)) * h).astype(np.float32)
+    body = scenes.Body(resolution=res, voxel=(h, h, h), sample_positions=pos, materials=tearable(),
+                       supports=[((0.0, 0.0, 0.0), (0.01, 1.0, 1.0))])
+    mesh, mats = scenes.build_mesh(body)
+    rest = mesh["rest_positions"].astype(np.float64)
+    # anisotropic stretch plus noise so every tet has a distinct crack normal
+    A = np.diag([1.0, 1.0, 1.0]) + 0.3 * rng.standard_normal((3, 3))
+    x = rest @ A.T + 0.01 * h * rng.standard_normal(rest.shape)
+    T = mesh["tet_nodes"].shape[0]
+    torn = np.sort(rng.choice(T, size=T // 6, replace=False)).astype(np.uint32)
+    new, origin = check_tear(mesh, mats, x, torn)
+    assert origin.size > 0
+    # tear again on the torn mesh: repeated tears compose
+    x2 = list(x)
+    for o in origin:  # a copy may come from an earlier copy
+        x2.append(x2[o])
+    x2 = np.asarray(x2)
+    torn2 = np.sort(rng.choice(T, size=T // 8, replace=False)).astype(np.uint32)
+    check_tear(new, mats, x2, torn2)
+
+
+def test_oracle_body_tear_matches_the_library_masses_and_keeps_state():
+    body, mesh, mats = strip(cells=(8, 2, 2), voxel=0.1)
+    orc, _ = oracle_body(body)
+    x, v, a = orc.state()
+    rng = np.random.default_rng(4)
+    x = x * np.array([1.3, 1.0, 1.0]) + 1e-3 * rng.standard_normal(x.shape)
+    v = rng.standard_normal(x.shape)
+    orc.set_state(x, v, a)
+    cent = mesh["rest_positions"][mesh["tet_nodes"]].mean(axis=1)
+    torn = np.nonzero(np.abs(cent[:, 0] - 0.4) < 0.05)[0].astype(np.uint32)
+    new, origin = check_tear(mesh, mats, x, torn)
+    added = orc.tear(torn)
+    assert added == origin.size
+    topo = orc.topology()
+    np.testing.assert_array_equal(topo["tet"], new["tet_nodes"])
+    np.testing.assert_allclose(topo["mass"], new["node_mass"], rtol=1e-6)
+    x2, v2, _ = orc.state()
+    n0 = x.shape[0]
+    np.testing.assert_array_equal(x2[:n0], x)
+    back = np.concatenate([np.arange(n0), origin])
+    for i in range(n0, back.size):
+        back[i] = back[back[i]]
+    np.testing.assert_array_equal(x2[n0:], x[back[n0:]])
+    np.testing.assert_array_equal(v2[n0:], v[back[n0:]])
