@@ -1,24 +1,31 @@
 #!/usr/bin/env python3
 """Benchmark: tet updates/s of the deformable step on one MI355X.
 
-Workload (default): the 1 m cube of 40x40x40 hexes split into 384,000 Kuhn
-tets (68,921 nodes), nodes jittered +-2% of the edge, per-tet neo-Hookean
-material with E = 5e6 exp(0.3 N(0,1)) Pa and nu ~ U[0.25, 0.4], bottom face
-pinned, gravity, dt = 1/60 s, 1 substep, PCG tolerance 1e-3 with at most 64
-iterations (the reference Settings defaults).  A step is one complete
-deformable step: element kernels, assembly, three implicit solves, velocity
-and position updates and the plastic commit.
+Workload (default, "bodies"): 512 independent Kuhn-cuboid bodies with tet
+counts log-uniform in [200, 20000] (2,228,550 tets in total), per-body
+neo-Hookean material E log-uniform [1e5, 1e8] Pa, nu ~ U[0.2, 0.45], yield on
+for half of them (yield stress 1e-3 E, hardening 1e-2 E), x=0 face pinned,
+gravity, dt = 1/60 s, 4 substeps, PCG tolerance 1e-3 preconditioned with the
+supernodal block Cholesky (at most 32 iterations), every body in one batched
+launch sequence per pass.  "cube" is the 40^3-hex cube (384,000 tets, per-tet
+lognormal E, 1 substep).
 
-value  = tets x substeps x steps / wall time of the timed steps, mixed
-         precision (FP32 element kernels and PCG vectors, FP64 rest inverse,
-         node state, dot products and pivots).
-fp64   = the same for the plain FP64 path.
+A step is one complete deformable step: element kernels, assembly, three
+implicit solves per substep, velocity and position updates, plastic commit.
+
+value   = sum(tets) x substeps x steps / wall time of the timed steps for the
+          mixed-precision batched path (FP32 element kernels and vectors, FP64
+          rest inverse, node state, dot products and Cholesky pivots), with
+          the solve plan chosen by the autotuner.
+fp64    = the same batched path in FP64; fp64_plain = FP64 with every body in
+          its own context, stepped one after another (no batching).
 Inputs are resident in HBM before the timed region; every step returns after
 its stream has synchronized, so the wall clock brackets finished work.
 """
 from __future__ import annotations
 
 import argparse
+import concurrent.futures as cf
 import json
 import os
 import sys
@@ -31,137 +38,200 @@ sys.path.insert(0, ROOT)
 
 METRIC = "tet updates/sec: mixed-precision batched vs plain FP64, 1 GPU, ≤1e-5 rel force err"
 HBM_PEAK_GBS = 8000.0
+KINDS = {"assemble": 0, "op_element": 1, "op_gather": 2, "solve_forward_lanes": 3,
+         "solve_backward_lanes": 4, "solve_forward_rows": 5, "solve_backward_rows": 6}
+KERNEL_NAMES = {"assemble": "k_assemble", "op_element": "k_op_element", "op_gather": "k_op_gather",
+                "solve_forward_lanes": "k_solve_forward_lanes",
+                "solve_backward_lanes": "k_solve_backward_lanes",
+                "solve_forward_rows": "k_solve_forward", "solve_backward_rows": "k_solve_backward"}
 
 
 def log(msg):
     print(msg, file=sys.stderr, flush=True)
 
 
-def run_gpu(api, mesh, mats, precision, steps, warmup, params):
+def workload(name):
+    from talus_amd import scenes
+
+    if name == "cube":
+        sc = scenes.cube(40)
+        desc = ("cube 40^3 hexes -> 384,000 tets, 68,921 nodes, jittered 2%, per-tet lognormal E "
+                "(median 5e6 Pa, sigma 0.3), nu U[0.25,0.4], neo-Hookean, bottom pinned, dt 1/60, "
+                "1 substep, PCG tol 1e-3 with the block Cholesky preconditioner")
+    else:
+        sc = scenes.bodies()
+        desc = ("512 bodies, tets log-uniform [200, 20000] (Kuhn cuboids, 0.05 m cells), E log-uniform "
+                "[1e5, 1e8] Pa, nu U[0.2,0.45], yield on p=0.5 (1e-3 E, hardening 1e-2 E), neo-Hookean, "
+                "x=0 pinned, dt 1/60, 4 substeps, PCG tol 1e-3 with the block Cholesky preconditioner, "
+                "all bodies batched")
+    meshes = [scenes.build_mesh(b) for b in sc.bodies]
+    return sc, meshes, desc
+
+
+def batched(api, meshes, precision, params, steps, warmup, tune):
     ctx = api.Context(0, precision)
-    b = ctx.add_body(mesh, mats)
-    for _ in range(warmup):
+    ids = [ctx.add_body(m, mats) for m, mats in meshes]
+    ctx.step(params)
+    tuned = ctx.tune(3) if tune else None
+    for _ in range(max(warmup - 1, 0)):
         ctx.step(params)
-    reps = []
     t0 = time.perf_counter()
-    for _ in range(steps):
-        reps.append(ctx.step(params))
+    reps = [ctx.step(params) for _ in range(steps)]
     wall = time.perf_counter() - t0
-    return ctx, b, wall, reps
+    return ctx, ids, wall, reps, tuned
 
 
-def bytes_per_tet(kind, s, nt):
-    """Algorithmic HBM bytes per tet for one launch of a kernel, s = bytes of
-    the element precision, nt = nodes per tet.  Dm^-1 is FP64 (72 B) in both
-    paths; node vectors are xyz of s bytes; conn is 16 B."""
-    if kind == "assemble":  # conn, Dm^-1, V/lambda/mu, R write, 12 slot writes, u read
-        return 16 + 72 + 3 * s + 9 * s + 12 * s + nt * 3 * s
-    if kind == "op_element":  # conn, Dm^-1, V/lambda/mu, R read, slot writes, d read
-        return 16 + 72 + 3 * s + 9 * s + 12 * s + nt * 3 * s
-    if kind == "op_gather":  # slot reads, incidence codes, p read, q write, mass, anchored
-        return 12 * s + 16 + nt * (3 * s + 3 * s + 8 + 1 + 4)
-    raise ValueError(kind)
+def plain(api, meshes, params, steps):
+    """Every body in its own FP64 context, one after another."""
+    wall = 0.0
+    for m, mats in meshes:
+        with api.Context(0, api.FP64) as ctx:
+            ctx.add_body(m, mats)
+            ctx.step(params)
+            t0 = time.perf_counter()
+            for _ in range(steps):
+                ctx.step(params)
+            wall += time.perf_counter() - t0
+    return wall
+
+
+def force_error(ctx_m, ids_m, ctx_d, ids_d):
+    """Worst body of max_e |f_e^mixed - f_e^FP64|_2 / max_e |f_e^FP64|_2 on the
+    FP64 state."""
+    worst = 0.0
+    for bm, bd in zip(ids_m, ids_d):
+        x, _, _ = ctx_d.state(bd)
+        fd, *_ = ctx_d.eval_forces(bd, x)
+        fm, *_ = ctx_m.eval_forces(bm, x)
+        T = fd.shape[0]
+        num = np.linalg.norm((fm - fd).reshape(T, -1), axis=1).max()
+        den = np.linalg.norm(fd.reshape(T, -1), axis=1).max()
+        worst = max(worst, float(num / den))
+    return worst
+
+
+def traffic_for(kernel, work):
+    """HBM bytes per launch measured by rocprofv3 PMC counters for this kernel
+    and workload (profiles/traffic_<kernel>_<workload>.json), else None."""
+    path = os.path.join(ROOT, "profiles", f"traffic_{kernel}_{work}.json")
+    if not os.path.exists(path):
+        return None
+    with open(path) as f:
+        return json.load(f).get("bytes_per_launch")
+
+
+def cpu_baseline(sc, budget_s, threads):
+    """The oracle's FP64 step (block-Jacobi PCG, its own CPU restatement) on a
+    bounded sample of the same bodies, one body per worker thread; only the
+    steps are timed."""
+    sys.path.insert(0, os.path.join(ROOT, "tests"))
+    from helpers import oracle_body
+
+    order = np.random.default_rng(99).permutation(len(sc.bodies))
+    done = updates = tets = 0
+    wall = 0.0
+    with cf.ThreadPoolExecutor(threads) as pool:
+        while done < len(order) and wall < budget_s:
+            chunk = [oracle_body(sc.bodies[i])[0] for i in order[done:done + threads]]
+            t0 = time.perf_counter()
+            list(pool.map(lambda b: b.step(substeps=sc.substeps, threads=1), chunk))
+            wall += time.perf_counter() - t0
+            tets += sum(b.tets for b in chunk)
+            updates += sum(b.tets for b in chunk) * sc.substeps
+            done += len(chunk)
+    return {"value": updates / wall, "unit": "tet updates/s", "cores": threads, "kind": "port",
+            "sample": f"one FP64 step ({sc.substeps} substeps) of {done} of the {len(order)} bodies "
+                      f"(seeded order, {tets} tets), {threads} worker threads, one body per thread, "
+                      f"with the oracle's C restatement (block-Jacobi PCG)"}
 
 
 def main(argv=None):
     ap = argparse.ArgumentParser(description=__doc__)
     ap.add_argument("--gpus", type=int, default=1)
-    ap.add_argument("--steps", type=int, default=20)
+    ap.add_argument("--steps", type=int, default=10)
     ap.add_argument("--warmup", type=int, default=5)
-    ap.add_argument("--cells", type=int, default=40)
+    ap.add_argument("--workload", choices=["bodies", "cube"], default="bodies")
     ap.add_argument("--skip-fp64", action="store_true")
+    ap.add_argument("--skip-plain", action="store_true")
     ap.add_argument("--skip-cpu", action="store_true")
+    ap.add_argument("--no-tune", action="store_true")
     ap.add_argument("--cpu-seconds", type=float, default=15.0)
-    ap.add_argument("--preconditioner", choices=["factor", "jacobi"], default="factor")
+    ap.add_argument("--cpu-threads", type=int, default=16)
+    ap.add_argument("--plain-steps", type=int, default=2)
     args = ap.parse_args(argv)
     if args.gpus != 1:
         print(f"bench.py measures one GPU; --gpus {args.gpus} is not supported "
               "(the multi-GPU sharding is described, not built)", file=sys.stderr)
         return 2
 
-    from talus_amd import api, scenes
+    from talus_amd import api
 
-    scene = scenes.cube(args.cells)
-    body = scene.bodies[0]
-    mesh, mats = scenes.build_mesh(body)
-    T = int(mesh["tet_nodes"].shape[0])
-    N = int(mesh["anchored"].shape[0])
-    pre = api.FACTOR if args.preconditioner == "factor" else api.BLOCK_JACOBI
-    params = api.default_params(substeps=scene.substeps, preconditioner=pre)
-    log(f"[bench] workload cube{args.cells}: {T} tets {N} nodes")
+    sc, meshes, desc = workload(args.workload)
+    T = sum(int(m["tet_nodes"].shape[0]) for m, _ in meshes)
+    N = sum(int(m["anchored"].shape[0]) for m, _ in meshes)
+    sub = sc.substeps
+    params = api.default_params(substeps=sub, preconditioner=api.FACTOR)
+    log(f"[bench] workload {args.workload}: {len(meshes)} bodies, {T} tets, {N} nodes")
 
-    log("[bench] mixed precision")
-    ctx_m, bm, wall_m, reps_m = run_gpu(api, mesh, mats, api.MIXED, args.steps, args.warmup, params)
-    value = T * scene.substeps * args.steps / wall_m
-    cg_m = sum(r["cg_iterations"] for r in reps_m) / args.steps
-    capped_m = sum(r["cg_capped"] for r in reps_m)
-    log(f"[bench] mixed {1e3 * wall_m / args.steps:.3f} ms/step, {value:.4g} tet/s, cg/step {cg_m:.1f}")
+    log("[bench] mixed batched")
+    ctx_m, ids_m, wall_m, reps_m, tuned = batched(api, meshes, api.MIXED, params, args.steps,
+                                                 args.warmup, not args.no_tune)
+    value = T * sub * args.steps / wall_m
+    apps = sum(r["preconditioner_applications"] for r in reps_m) / args.steps
+    solves = sum(r["solves"] for r in reps_m) / args.steps
+    log(f"[bench] mixed {1e3 * wall_m / args.steps:.3f} ms/step, {value:.4g} tet/s, "
+        f"{apps:.1f} preconditioner applications/step, tuned {tuned}")
     finfo = ctx_m.factor_info()
-    if finfo["fronts"]:
-        finfo["tflops"] = finfo["flops"] / max(finfo["factor_seconds"], 1e-12) / 1e12
-        log(f"[bench] factor {json.dumps(finfo)}")
 
     fp64 = None
-    ctx_d = None
+    ctx_d = ids_d = None
     if not args.skip_fp64:
-        log("[bench] fp64")
-        ctx_d, bd, wall_d, reps_d = run_gpu(api, mesh, mats, api.FP64, args.steps, args.warmup, params)
-        v64 = T * scene.substeps * args.steps / wall_d
-        fp64 = {"value": v64, "ms_per_step": 1e3 * wall_d / args.steps,
-                "cg_per_step": sum(r["cg_iterations"] for r in reps_d) / args.steps,
+        log("[bench] fp64 batched")
+        ctx_d, ids_d, wall_d, reps_d, tuned_d = batched(api, meshes, api.FP64, params, args.steps,
+                                                        args.warmup, not args.no_tune)
+        fp64 = {"value": T * sub * args.steps / wall_d, "ms_per_step": 1e3 * wall_d / args.steps,
+                "cg_iterations_per_step": sum(r["cg_iterations"] for r in reps_d) / args.steps,
                 "cg_capped": sum(r["cg_capped"] for r in reps_d)}
-        log(f"[bench] fp64 {fp64['ms_per_step']:.3f} ms/step, {v64:.4g} tet/s")
+        log(f"[bench] fp64 batched {fp64['ms_per_step']:.3f} ms/step, {fp64['value']:.4g} tet/s")
 
-    # relative force error of the mixed element kernel on an FP64 state
-    rel_err = None
-    if ctx_d is not None:
-        x, _, _ = ctx_d.state(bd)
-        tf_d, *_ = ctx_d.eval_forces(bd, x)
-        tf_m, *_ = ctx_m.eval_forces(bm, x)
-        num = np.linalg.norm((tf_m - tf_d).reshape(T, -1), axis=1).max()
-        den = np.linalg.norm(tf_d.reshape(T, -1), axis=1).max()
-        rel_err = float(num / den)
-        log(f"[bench] rel force error mixed vs fp64: {rel_err:.3e}")
+    fp64_plain = None
+    if not args.skip_plain:
+        log("[bench] fp64 plain (one context per body, sequential)")
+        wall_p = plain(api, meshes, params, args.plain_steps)
+        fp64_plain = {"value": T * sub * args.plain_steps / wall_p,
+                      "ms_per_step": 1e3 * wall_p / args.plain_steps, "steps": args.plain_steps}
+        log(f"[bench] fp64 plain {fp64_plain['ms_per_step']:.1f} ms/step, {fp64_plain['value']:.4g} tet/s")
 
-    # per-kernel launch durations (HIP events on the context stream)
-    nt = N / T
-    kinds = {"assemble": 0, "op_element": 1, "op_gather": 2}
-    per_step_launches = {"assemble": 3 * scene.substeps,
-                         "op_element": cg_m + 1 * scene.substeps,
-                         "op_gather": cg_m}
+    rel_err = force_error(ctx_m, ids_m, ctx_d, ids_d) if ctx_d is not None else None
+    if rel_err is not None:
+        log(f"[bench] rel force error mixed vs fp64 (worst body): {rel_err:.3e}")
+
+    # per-kernel launch durations (HIP events on the context stream) and the
+    # launches of each kernel per step
+    passes = sub
+    iters = max(apps - solves, 0.0)
     kernels = {}
-    for name, kind in kinds.items():
-        ms = ctx_m.time_kernel(kind, 50)
-        B = bytes_per_tet(name, 4, nt) * T
-        kernels[name] = {"avg_ms": ms, "bytes": B, "gbs": B / (ms * 1e-3) / 1e9,
-                         "ms_per_step": ms * per_step_launches[name]}
+    for name, kind in KINDS.items():
+        t = ctx_m.time_kernel(kind, 20)
+        if not t["launches"]:
+            continue
+        per_step = {"assemble": 3 * passes, "op_element": iters + passes, "op_gather": iters + passes}
+        launches = per_step.get(name, apps * t["launches"])
+        kernels[name] = {"avg_ms": t["avg_ms"], "bytes": t["bytes"],
+                         "gbs": t["bytes"] / (t["avg_ms"] * 1e-3) / 1e9,
+                         "launches_per_step": launches, "ms_per_step": t["avg_ms"] * launches}
     dominant = max(kernels, key=lambda k: kernels[k]["ms_per_step"])
     kd = kernels[dominant]
-    roofline = {"bound": "hbm", "kernel": dominant, "achieved": kd["gbs"], "peak": HBM_PEAK_GBS,
-                "unit": "GB/s", "frac": kd["gbs"] / HBM_PEAK_GBS, "traffic": None}
+    roofline = {"bound": "hbm", "kernel": KERNEL_NAMES[dominant], "achieved": kd["gbs"],
+                "peak": HBM_PEAK_GBS, "unit": "GB/s", "frac": kd["gbs"] / HBM_PEAK_GBS,
+                "traffic": traffic_for(KERNEL_NAMES[dominant], args.workload),
+                "algorithmic_bytes": kd["bytes"], "avg_ms": kd["avg_ms"]}
     log(f"[bench] kernels {json.dumps(kernels)}")
 
     cpu = None
     if not args.skip_cpu:
-        import oracle
-
-        sys.path.insert(0, os.path.join(ROOT, "tests"))
-        from helpers import oracle_body
-
-        threads = min(os.cpu_count() or 1, 64)
-        orc, _ = oracle_body(body)
-        log(f"[bench] cpu baseline, {threads} threads")
-        t0 = time.perf_counter()
-        orc.step(threads=threads)
-        one = time.perf_counter() - t0
-        n = max(1, min(20, int(args.cpu_seconds / max(one, 1e-3))))
-        t0 = time.perf_counter()
-        for _ in range(n):
-            orc.step(threads=threads)
-        wall = time.perf_counter() - t0
-        cpu = {"value": T * n / wall, "unit": "tet updates/s", "cores": threads, "kind": "port",
-               "sample": f"{n} FP64 steps of the same {T}-tet workload (after 1 warm-up) with the "
-                         f"oracle's threaded C restatement"}
+        log(f"[bench] cpu baseline, {args.cpu_threads} threads, ~{args.cpu_seconds:.0f} s")
+        cpu = cpu_baseline(sc, args.cpu_seconds, args.cpu_threads)
         log(f"[bench] cpu {cpu['value']:.4g} tet/s")
 
     line = {
@@ -175,24 +245,27 @@ def main(argv=None):
         "higher_is_better": True,
         "scaling": "weak",
         "vs_baseline": None,
-        "dtype": "f32 elements + f64 state",
+        "dtype": "f32 elements and vectors + f64 rest inverse, node state, dots, pivots",
         "data": "synthetic",
         "config": {
-            "workload": f"cube {args.cells}^3 hexes -> {T} tets, {N} nodes, jittered 2%, per-tet "
-                        f"lognormal E, bottom pinned, dt 1/60, {scene.substeps} substep, tol 1e-3, "
-                        f"cg<=64 (<=32 with the factor), {args.preconditioner}-preconditioned PCG",
-            "tets": T, "nodes": N, "precision": "mixed",
-            "cg_per_step": cg_m, "cg_capped": capped_m,
-            "preconditioner": args.preconditioner,
+            "workload": desc,
+            "bodies": len(meshes), "tets": T, "nodes": N, "substeps": sub, "precision": "mixed",
+            "cg_iterations_per_step": sum(r["cg_iterations"] for r in reps_m) / args.steps,
+            "cg_capped": sum(r["cg_capped"] for r in reps_m),
+            "preconditioner_applications_per_step": apps,
             "refactors": sum(r["refactors"] for r in reps_m),
+            "solve_plan": "".join(map(str, ctx_m.plan())), "tune": tuned,
         },
-        "factor": finfo if finfo["fronts"] else None,
+        "factor": finfo,
         "fp64": fp64,
+        "fp64_plain": fp64_plain,
         "mixed_over_fp64": (value / fp64["value"]) if fp64 else None,
+        "mixed_over_fp64_plain": (value / fp64_plain["value"]) if fp64_plain else None,
         "rel_force_error": rel_err,
         "kernels": kernels,
         "roofline": roofline,
         "cpu_baseline": cpu,
+        "gpu_over_cpu": (value / cpu["value"]) if cpu else None,
     }
     print(json.dumps(line), flush=True)
     return 0

@@ -126,7 +126,7 @@ typedef struct talus_mi_report {
   uint32_t tear_events;
   uint32_t split_nodes;
   uint32_t non_finite;
-  uint32_t reserved;
+  uint32_t preconditioner_applications; /* block Cholesky solves this step */
   double device_seconds;    /* hipEvent time of the whole step */
 } talus_mi_report;
 
@@ -217,11 +217,24 @@ int talus_mi_eval_forces(talus_mi_ctx *ctx, uint32_t body, const double *x,
  * solver.cc:904-1048) with the committed rotations. */
 int talus_mi_apply_system(talus_mi_ctx *ctx, uint32_t body, double inertia,
                           double step, const double *d, double *y);
-/* Times `repeats` launches of one kernel over every body at the current state
- * with HIP events on the context stream: kind 0 element assembly, 1 operator
- * element pass, 2 operator node gather.  avg_ms = mean launch duration. */
+/* Times one kernel at the current state with HIP events on the context
+ * stream (`repeats` rounds after one warm-up) and states its algorithmic
+ * HBM bytes (what the algorithm must move: element arrays once per tet, node
+ * arrays once per node, factor entries once).  Kinds: 0 element assembly,
+ * 1 operator element pass, 2 operator node gather (one launch over every
+ * body each); 3 / 4 forward / backward solve, one-lane-per-row shape, and
+ * 5 / 6 the rows-per-wavefront shape (one launch per elimination level that
+ * uses the shape under the current plan).  avg_ms and bytes are per launch;
+ * launches = launches of the kernel per timed unit (per preconditioner
+ * application for kinds 3-6, else 1). */
+typedef struct talus_mi_kernel_timing {
+  double avg_ms;
+  double bytes;
+  uint32_t launches;
+  uint32_t reserved;
+} talus_mi_kernel_timing;
 int talus_mi_time_kernel(talus_mi_ctx *ctx, uint32_t kind, uint32_t repeats,
-                         double *avg_ms);
+                         talus_mi_kernel_timing *out);
 /* Block Cholesky preconditioner alone (SystemFactor, factor.cc:549-704):
  * factorizes inertia*M + step^2 sum_e R K R^T with the committed rotations
  * (anchored rows removed) and returns z = (L L^T)^-1 r for one body. */
@@ -240,6 +253,31 @@ typedef struct talus_mi_factor_info {
 
 /* Statistics of the current factor (zeros before the first factorization). */
 int talus_mi_factor_info_get(talus_mi_ctx *ctx, talus_mi_factor_info *info);
+
+/* Autotuner of the preconditioner solve (no counterpart in the reference,
+ * whose factor solve is Eigen's).  The solve plan holds, per elimination
+ * level, the kernel shape of the forward and of the backward pass (2 bytes
+ * per level: 0 = rows per wavefront, 1 = one row/column per lane for the
+ * fronts with few pivots).  The factor's analysis sets a default plan from
+ * the front shapes; talus_mi_tune times both shapes of every level on the
+ * device and keeps the faster.  The plan changes the summation order of the
+ * solve: results are bit-identical for a fixed plan, so a caller that needs
+ * run-to-run identity tunes once and re-applies the plan with
+ * talus_mi_plan_set.  A new analysis (bodies added, tears) restores the
+ * default plan. */
+typedef struct talus_mi_tune_report {
+  uint32_t levels;
+  uint32_t lanes_forward;  /* levels whose forward pass uses the lane shape */
+  uint32_t lanes_backward;
+  uint32_t reserved;
+  double before_seconds;   /* one preconditioner application, previous plan */
+  double after_seconds;    /* ... tuned plan */
+} talus_mi_tune_report;
+
+int talus_mi_tune(talus_mi_ctx *ctx, uint32_t repeats, talus_mi_tune_report *report);
+/* Copies the plan (2*levels bytes) when capacity allows; *count = its size. */
+int talus_mi_plan_get(talus_mi_ctx *ctx, uint8_t *plan, uint32_t capacity, uint32_t *count);
+int talus_mi_plan_set(talus_mi_ctx *ctx, const uint8_t *plan, uint32_t count);
 
 /* The context's HIP stream (hipStream_t) for interop. */
 void *talus_mi_stream(talus_mi_ctx *ctx);

@@ -109,17 +109,28 @@ class Report(C.Structure):
         ("elastic_energy", C.c_double), ("kinetic_energy", C.c_double),
         ("plastic_growth", C.c_uint32), ("yield_events", C.c_uint32),
         ("tear_events", C.c_uint32), ("split_nodes", C.c_uint32), ("non_finite", C.c_uint32),
-        ("reserved", C.c_uint32), ("device_seconds", C.c_double),
+        ("preconditioner_applications", C.c_uint32), ("device_seconds", C.c_double),
     ]
 
     def as_dict(self):
-        return {k: getattr(self, k) for k, _ in self._fields_ if k != "reserved"}
+        return {k: getattr(self, k) for k, _ in self._fields_}
 
 
 class FactorInfo(C.Structure):
     _fields_ = [("fronts", C.c_uint32), ("levels", C.c_uint32), ("ok", C.c_uint32),
                 ("reserved", C.c_uint32), ("entries", C.c_uint64), ("flops", C.c_double),
                 ("factor_seconds", C.c_double)]
+
+
+class TuneReport(C.Structure):
+    _fields_ = [("levels", C.c_uint32), ("lanes_forward", C.c_uint32),
+                ("lanes_backward", C.c_uint32), ("reserved", C.c_uint32),
+                ("before_seconds", C.c_double), ("after_seconds", C.c_double)]
+
+
+class KernelTiming(C.Structure):
+    _fields_ = [("avg_ms", C.c_double), ("bytes", C.c_double), ("launches", C.c_uint32),
+                ("reserved", C.c_uint32)]
 
 
 class Event(C.Structure):
@@ -162,10 +173,13 @@ _SIGNATURES = {
                                        _P(C.c_double), _P(C.c_double), _P(C.c_double)]),
     "talus_mi_apply_system": (C.c_int, [C.c_void_p, C.c_uint32, C.c_double, C.c_double,
                                         _P(C.c_double), _P(C.c_double)]),
-    "talus_mi_time_kernel": (C.c_int, [C.c_void_p, C.c_uint32, C.c_uint32, _P(C.c_double)]),
+    "talus_mi_time_kernel": (C.c_int, [C.c_void_p, C.c_uint32, C.c_uint32, _P(KernelTiming)]),
     "talus_mi_factor_solve": (C.c_int, [C.c_void_p, C.c_uint32, C.c_double, C.c_double,
                                         _P(C.c_double), _P(C.c_double)]),
     "talus_mi_factor_info_get": (C.c_int, [C.c_void_p, _P(FactorInfo)]),
+    "talus_mi_tune": (C.c_int, [C.c_void_p, C.c_uint32, _P(TuneReport)]),
+    "talus_mi_plan_get": (C.c_int, [C.c_void_p, _P(C.c_uint8), C.c_uint32, _P(C.c_uint32)]),
+    "talus_mi_plan_set": (C.c_int, [C.c_void_p, _P(C.c_uint8), C.c_uint32]),
     "talus_mi_stream": (C.c_void_p, [C.c_void_p]),
 }
 
@@ -465,10 +479,13 @@ class Context:
         _check(lib().talus_mi_apply_system(self._h, body, inertia, step, _dp(d), _dp(y)))
         return y.reshape(n, 3)
 
-    def time_kernel(self, kind, repeats=20) -> float:
-        ms = C.c_double()
-        _check(lib().talus_mi_time_kernel(self._h, kind, repeats, C.byref(ms)))
-        return float(ms.value)
+    def time_kernel(self, kind, repeats=20) -> dict:
+        """Per-launch HIP-event time and algorithmic bytes of one kernel
+        (kinds: 0 assembly, 1 operator element, 2 operator gather, 3/4 solve
+        forward/backward lane shape, 5/6 rows shape)."""
+        t = KernelTiming()
+        _check(lib().talus_mi_time_kernel(self._h, kind, repeats, C.byref(t)))
+        return {"avg_ms": t.avg_ms, "bytes": t.bytes, "launches": t.launches}
 
     def factor_solve(self, body, inertia, step, r):
         n, _ = self.info(body)
@@ -481,6 +498,24 @@ class Context:
         i = FactorInfo()
         _check(lib().talus_mi_factor_info_get(self._h, C.byref(i)))
         return {k: getattr(i, k) for k, _ in FactorInfo._fields_ if k != "reserved"}
+
+    def tune(self, repeats=3) -> dict:
+        """Autotunes the preconditioner solve's kernel shapes per level."""
+        r = TuneReport()
+        _check(lib().talus_mi_tune(self._h, repeats, C.byref(r)))
+        return {k: getattr(r, k) for k, _ in TuneReport._fields_ if k != "reserved"}
+
+    def plan(self) -> np.ndarray:
+        n = C.c_uint32()
+        _check(lib().talus_mi_plan_get(self._h, None, 0, C.byref(n)))
+        p = np.zeros(n.value, dtype=np.uint8)
+        if n.value:
+            _check(lib().talus_mi_plan_get(self._h, p.ctypes.data_as(_P(C.c_uint8)), n.value, C.byref(n)))
+        return p
+
+    def set_plan(self, plan):
+        p = np.ascontiguousarray(plan, dtype=np.uint8)
+        _check(lib().talus_mi_plan_set(self._h, p.ctypes.data_as(_P(C.c_uint8)), p.size))
 
     def stream(self) -> int:
         return int(lib().talus_mi_stream(self._h) or 0)

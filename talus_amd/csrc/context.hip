@@ -110,10 +110,13 @@ class EngineBase {
                            double *R) = 0;
   virtual void apply_system(uint32_t b, double inertia, double step, const double *dv,
                             double *y) = 0;
-  virtual double time_kernel(uint32_t kind, uint32_t repeats) = 0;
+  virtual void time_kernel(uint32_t kind, uint32_t repeats, talus_mi_kernel_timing &out) = 0;
   virtual void factor_solve(uint32_t b, double inertia, double step, const double *r,
                             double *z) = 0;
   virtual void factor_info(talus_mi_factor_info &info) = 0;
+  virtual void tune(uint32_t repeats, talus_mi_tune_report &report) = 0;
+  virtual std::vector<uint8_t> plan() = 0;
+  virtual bool set_plan(const std::vector<uint8_t> &plan) = 0;
 };
 
 template <class Real>
@@ -141,9 +144,22 @@ class Engine final : public EngineBase {
                    double *R) override;
   void apply_system(uint32_t b, double inertia, double step, const double *dv,
                     double *y) override;
-  double time_kernel(uint32_t kind, uint32_t repeats) override;
+  void time_kernel(uint32_t kind, uint32_t repeats, talus_mi_kernel_timing &out) override;
   void factor_solve(uint32_t b, double inertia, double step, const double *r, double *z) override;
   void factor_info(talus_mi_factor_info &info) override;
+  void tune(uint32_t repeats, talus_mi_tune_report &report) override;
+  std::vector<uint8_t> plan() override {
+    if (!sym_ready_) analyze_factor();
+    return plan_;
+  }
+  bool set_plan(const std::vector<uint8_t> &plan) override {
+    if (!sym_ready_) analyze_factor();
+    if (plan.size() != plan_.size()) return false;
+    for (uint8_t p : plan)
+      if (p > 1) return false;
+    plan_ = plan;
+    return true;
+  }
 
  private:
   template <class T>
@@ -181,6 +197,8 @@ class Engine final : public EngineBase {
   void analyze_factor();
   void factorize(const std::vector<double> &reach, const std::vector<double> &inertia);
   void precondition(int parity);
+  void forward_level(uint32_t l, int mode, int parity);
+  void backward_level(uint32_t l, int mode, int parity);
   template <class T>
   T *upload(const std::vector<T> &h) { return dev_copy(h); }
 
@@ -201,12 +219,15 @@ class Engine final : public EngineBase {
   Real *hvec_ = nullptr;  // 3N + 12T staging of Real
   double *scratch64_ = nullptr;
   bool any_plastic_ = false, any_tear_ = false;
+  uint64_t n_snh_ = 0, n_plastic_ = 0;
+  uint32_t applications_ = 0;  // preconditioner applications this step
   // factor state
   const std::vector<Body> *bodies_ = nullptr;
   bool sym_ready_ = false, fac_ready_ = false, fac_ok_ = false, use_factor_ = false;
   bool refresh_ = false;
   std::vector<double> fac_reach_, fac_inertia_;
   FactorTables tab_;
+  std::vector<uint8_t> plan_;  // per level: forward, backward kernel shape
   FactorDev fd_{};
   uint64_t *d_pair_target_ = nullptr;
   uint32_t *d_pair_ld_ = nullptr, *d_pair_node_ = nullptr, *d_pair_start_ = nullptr,
@@ -214,6 +235,7 @@ class Engine final : public EngineBase {
            *d_lvl_fronts_ = nullptr, *d_potrf_ = nullptr, *d_node_body_ = nullptr;
   uint64_t *d_ea_ = nullptr, *d_trsm_ = nullptr, *d_syrk_ = nullptr, *d_trtri_ = nullptr,
            *d_w21_ = nullptr, *d_pack_ = nullptr, *d_fwd_ = nullptr, *d_bwd_ = nullptr,
+           *d_fwl_ = nullptr, *d_bwl_ = nullptr, *d_fwr_ = nullptr, *d_bwr_ = nullptr,
            *d_gat_ = nullptr, *d_xg_ = nullptr;
   Real *d_gr_ = nullptr, *d_gc_ = nullptr;
   double *d_body_reach_ = nullptr, *d_body_inertia_ = nullptr;
@@ -293,6 +315,7 @@ void Engine<Real>::build(const std::vector<Body> &bodies) {
   std::vector<Real> vol(T), lam(T), mu(T), R(9 * T), W(9 * T);
   std::vector<uint8_t> flags(T);
   any_plastic_ = any_tear_ = false;
+  n_snh_ = n_plastic_ = 0;
   for (uint32_t b = 0; b < B; ++b) {
     const Body &bd = bodies[b];
     const size_t n0 = n0_[b], t0 = t0_[b];
@@ -326,6 +349,8 @@ void Engine<Real>::build(const std::vector<Body> &bodies) {
       tear[g] = bd.tear[e];
       flags[g] = bd.flags[e];
       any_plastic_ = any_plastic_ || (bd.flags[e] & FLAG_PLASTIC);
+      n_snh_ += (bd.flags[e] & FLAG_SNH) ? 1u : 0u;
+      n_plastic_ += (bd.flags[e] & FLAG_PLASTIC) ? 1u : 0u;
       any_tear_ = any_tear_ || (bd.flags[e] & FLAG_TEAR);
     }
   }
@@ -347,7 +372,7 @@ void Engine<Real>::build(const std::vector<Body> &bodies) {
   d_.shifted = dev<double>(3 * N);
   d_.weighted = dev<double>(3 * N);
   d_.right = dev<double>(3 * N);
-  d_.u_eval = dev<Real>(3 * N);
+  d_.u_eval = dev<double>(3 * N);
   d_.P = dev<Real>(6 * N);
   d_.x0 = dev<Real>(3 * N);
   d_.dx = dev<Real>(3 * N);
@@ -461,6 +486,11 @@ void Engine<Real>::analyze_factor() {
   d_pack_ = upload(t.pack_tasks);
   d_fwd_ = upload(t.fwd_tasks);
   d_bwd_ = upload(t.bwd_tasks);
+  d_fwl_ = upload(t.fwl_tasks);
+  d_bwl_ = upload(t.bwl_tasks);
+  d_fwr_ = upload(t.fwr_tasks);
+  d_bwr_ = upload(t.bwr_tasks);
+  plan_ = t.plan;
   d_gat_ = upload(t.gat_tasks);
   d_xg_ = upload(t.xg_tasks);
   d_gr_ = dev<Real>(t.g_pool);
@@ -527,6 +557,40 @@ void Engine<Real>::factorize(const std::vector<double> &reach, const std::vector
 // GEMV with G; per level from the root, gather the ancestors' solution and
 // one transposed GEMV with G.
 template <class Real>
+void Engine<Real>::forward_level(uint32_t l, int mode, int parity) {
+  const FactorTables &t = tab_;
+  if (mode) {
+    SolveLaunch<Real>::forward_lanes(fd_, d_gc_, d_fwl_ + t.fwl_start[l], t.fwl_start[l + 1] - t.fwl_start[l],
+                                     d_.done, parity, d_.B, stream_);
+    SolveLaunch<Real>::forward(fd_, d_gr_, d_fwr_ + t.fwr_start[l], t.fwr_start[l + 1] - t.fwr_start[l],
+                               d_.done, parity, d_.B, stream_);
+  } else {
+    SolveLaunch<Real>::forward(fd_, d_gr_, d_fwd_ + t.fwd_start[l], t.fwd_start[l + 1] - t.fwd_start[l],
+                               d_.done, parity, d_.B, stream_);
+  }
+}
+
+template <class Real>
+void Engine<Real>::backward_level(uint32_t l, int mode, int parity) {
+  const FactorTables &t = tab_;
+  if (mode) {
+    SolveLaunch<Real>::backward_lanes(fd_, d_gr_, d_bwl_ + t.bwl_start[l], t.bwl_start[l + 1] - t.bwl_start[l],
+                                      d_.done, parity, d_.B, stream_);
+    SolveLaunch<Real>::backward(fd_, d_gc_, d_bwr_ + t.bwr_start[l], t.bwr_start[l + 1] - t.bwr_start[l],
+                                d_.done, parity, d_.B, stream_);
+  } else {
+    SolveLaunch<Real>::backward(fd_, d_gc_, d_bwd_ + t.bwd_start[l], t.bwd_start[l + 1] - t.bwd_start[l],
+                                d_.done, parity, d_.B, stream_);
+  }
+}
+
+// z = (LL^T)^-1 r for the bodies still iterating (applyFactor,
+// factor.cc:648-704, without the body-frame rotation): per level from the
+// leaves, gather the right-hand side and the children's updates, then one
+// GEMV with G; per level from the root, gather the ancestors' solution and
+// one transposed GEMV with G.  The plan picks the kernel shape per level and
+// direction; a fixed plan gives bit-identical results.
+template <class Real>
 void Engine<Real>::precondition(int parity) {
   hipStream_t s = stream_;
   const FactorTables &t = tab_;
@@ -534,16 +598,71 @@ void Engine<Real>::precondition(int parity) {
   for (uint32_t l = 0; l < t.levels; ++l) {
     SolveLaunch<Real>::gather(fd_, d_gat_ + t.gat_start[l], t.gat_start[l + 1] - t.gat_start[l], d_.r,
                               d_.done, parity, B, s);
-    SolveLaunch<Real>::forward(fd_, d_gr_, d_fwd_ + t.fwd_start[l], t.fwd_start[l + 1] - t.fwd_start[l],
-                               d_.done, parity, B, s);
+    forward_level(l, plan_[2 * l], parity);
   }
   for (uint32_t l = t.levels; l-- > 0;) {
     SolveLaunch<Real>::xgather(fd_, d_xg_ + t.xg_start[l], t.xg_start[l + 1] - t.xg_start[l], d_.done,
                                parity, B, s);
-    SolveLaunch<Real>::backward(fd_, d_gc_, d_bwd_ + t.bwd_start[l], t.bwd_start[l + 1] - t.bwd_start[l],
-                                d_.done, parity, B, s);
+    backward_level(l, plan_[2 * l + 1], parity);
   }
   SolveLaunch<Real>::finish(fd_, d_.z, d_.N, d_.anchored, d_node_body_, d_.done, parity, B, s);
+}
+
+// Autotuner: times both kernel shapes of every level's forward and backward
+// pass on the device (HIP events, best of `repeats` after a warm-up) and
+// keeps the faster; reports one preconditioner application before and after.
+template <class Real>
+void Engine<Real>::tune(uint32_t repeats, talus_mi_tune_report &rep) {
+  std::memset(&rep, 0, sizeof rep);
+  if (!built_ || d_.B == 0) return;
+  if (!fac_ready_) {
+    std::vector<double> reach(d_.B, 1.0), inert(d_.B, 1.0);
+    factorize(reach, inert);
+    fac_reach_.clear();  // the next step refactors with its own keys
+  }
+  const uint32_t reps = std::max(repeats, 1u);
+  hipStream_t s = stream_;
+  hipEvent_t a, b;
+  HIPCHECK(hipEventCreate(&a));
+  HIPCHECK(hipEventCreate(&b));
+  auto time = [&](auto &&launch) {
+    launch();
+    float best = 1e30f;
+    for (uint32_t r = 0; r < reps; ++r) {
+      HIPCHECK(hipEventRecord(a, s));
+      launch();
+      HIPCHECK(hipEventRecord(b, s));
+      HIPCHECK(hipEventSynchronize(b));
+      float ms = 0.0f;
+      HIPCHECK(hipEventElapsedTime(&ms, a, b));
+      best = std::min(best, ms);
+    }
+    return double(best) * 1.0e-3;
+  };
+  const FactorTables &t = tab_;
+  rep.levels = t.levels;
+  rep.before_seconds = time([&] { precondition(-1); });
+  for (uint32_t l = 0; l < t.levels; ++l) {
+    if (t.fwl_start[l + 1] > t.fwl_start[l]) {
+      const double rows = time([&] { forward_level(l, 0, -1); });
+      const double lanes = time([&] { forward_level(l, 1, -1); });
+      plan_[2 * l] = lanes < rows ? 1 : 0;
+    } else {
+      plan_[2 * l] = 0;
+    }
+    if (t.bwl_start[l + 1] > t.bwl_start[l]) {
+      const double rows = time([&] { backward_level(l, 0, -1); });
+      const double lanes = time([&] { backward_level(l, 1, -1); });
+      plan_[2 * l + 1] = lanes < rows ? 1 : 0;
+    } else {
+      plan_[2 * l + 1] = 0;
+    }
+    rep.lanes_forward += plan_[2 * l];
+    rep.lanes_backward += plan_[2 * l + 1];
+  }
+  rep.after_seconds = time([&] { precondition(-1); });
+  HIPCHECK(hipEventDestroy(a));
+  HIPCHECK(hipEventDestroy(b));
 }
 
 template <class Real>
@@ -598,6 +717,7 @@ void Engine<Real>::solve(bool first, uint32_t limit, uint32_t solve_index) {
   }
   if (fac) {
     precondition(-1);
+    ++applications_;
     Launch<Real>::pcg_rz(d_, true, -1, s);
   }
   Launch<Real>::pcg_begin(d_, s);
@@ -617,6 +737,7 @@ void Engine<Real>::solve(bool first, uint32_t limit, uint32_t solve_index) {
     Launch<Real>::pcg_update(d_, parity, fac, s);
     if (fac) {
       precondition(parity);
+      ++applications_;
       Launch<Real>::pcg_rz(d_, false, parity, s);
     }
     Launch<Real>::pcg_direction(d_, parity, s);
@@ -707,6 +828,7 @@ void Engine<Real>::step(const talus_mi_step_params &sp, talus_mi_report &rep,
   }
   uint32_t solve_index = 0;
   const uint32_t refactors_before = refactors_;
+  applications_ = 0;
   for (uint32_t pass = 0; pass < passes; ++pass) {
     for (uint32_t b = 0; b < B; ++b) params[b] = params_for(substeps[b], pass < substeps[b]);
     if (factor) {
@@ -774,6 +896,7 @@ void Engine<Real>::step(const talus_mi_step_params &sp, talus_mi_report &rep,
   rep.elements = d_.T;
   rep.substeps = passes;
   rep.refactors = refactors_ - refactors_before;
+  rep.preconditioner_applications = applications_;
   rep.device_seconds = ms * 1.0e-3;
   for (uint32_t si = 0; si < solve_index; ++si) {
     const uint32_t pass = si / 3;
@@ -920,30 +1043,120 @@ void Engine<Real>::apply_system(uint32_t b, double inertia, double step, const d
 }
 
 template <class Real>
-double Engine<Real>::time_kernel(uint32_t kind, uint32_t repeats) {
+void Engine<Real>::time_kernel(uint32_t kind, uint32_t repeats, talus_mi_kernel_timing &out) {
+  std::memset(&out, 0, sizeof out);
   hipStream_t s = stream_;
-  all_active();
-  // inputs at the current state
-  Launch<Real>::displacement(d_, d_.x, s);
-  HIPCHECK(hipMemsetAsync(d_.p, 0, 3 * size_t(d_.N) * sizeof(Real), s));
-  auto launch = [&]() {
-    if (kind == 0) Launch<Real>::assemble(d_, false, false, s);
-    else if (kind == 1) Launch<Real>::op_element(d_, d_.p, -1, s);
-    else Launch<Real>::op_gather(d_, d_.p, d_.q, -1, s);
-  };
-  launch();
+  const uint32_t reps = std::max(repeats, 1u);
+  const double sz = double(sizeof(Real)), T = double(d_.T), N = double(d_.N);
   hipEvent_t a, b;
   HIPCHECK(hipEventCreate(&a));
   HIPCHECK(hipEventCreate(&b));
-  HIPCHECK(hipEventRecord(a, s));
-  for (uint32_t i = 0; i < repeats; ++i) launch();
-  HIPCHECK(hipEventRecord(b, s));
-  HIPCHECK(hipEventSynchronize(b));
-  float ms = 0.0f;
-  HIPCHECK(hipEventElapsedTime(&ms, a, b));
+  if (kind <= 2) {
+    all_active();
+    // inputs at the current state
+    Launch<Real>::displacement(d_, d_.x, s);
+    HIPCHECK(hipMemsetAsync(d_.p, 0, 3 * size_t(d_.N) * sizeof(Real), s));
+    auto launch = [&]() {
+      if (kind == 0) Launch<Real>::assemble(d_, false, false, s);
+      else if (kind == 1) Launch<Real>::op_element(d_, d_.p, -1, s);
+      else Launch<Real>::op_gather(d_, d_.p, d_.q, -1, s);
+    };
+    launch();
+    HIPCHECK(hipEventRecord(a, s));
+    for (uint32_t i = 0; i < reps; ++i) launch();
+    HIPCHECK(hipEventRecord(b, s));
+    HIPCHECK(hipEventSynchronize(b));
+    float ms = 0.0f;
+    HIPCHECK(hipEventElapsedTime(&ms, a, b));
+    out.avg_ms = double(ms) / reps;
+    out.launches = 1;
+    // element data once per tet (connectivity 16, FP64 rest inverse 72,
+    // V/lambda/mu, flag byte), node vectors once per node
+    const double tet = 16.0 + 72.0 + 3.0 * sz + 1.0;
+    const double snh = double(n_snh_), plastic = double(n_plastic_);
+    if (kind == 0)
+      out.bytes = T * (tet + 9.0 * sz + 12.0 * sz) + snh * 9.0 * sz + plastic * (80.0 + 16.0) + N * 3.0 * sz;
+    else if (kind == 1)
+      out.bytes = T * (tet + 9.0 * sz + 12.0 * sz) + snh * 9.0 * sz + N * 3.0 * sz;
+    else
+      out.bytes = T * (12.0 * sz + 16.0) + (N + 1.0) * 4.0 + N * (6.0 * sz + 8.0 + 1.0);
+  } else if (kind <= 6) {
+    if (!fac_ready_) {
+      std::vector<double> reach(d_.B, 1.0), inert(d_.B, 1.0);
+      factorize(reach, inert);
+      fac_reach_.clear();  // the next step refactors with its own keys
+    }
+    const FactorTables &t = tab_;
+    const bool forward = kind == 3 || kind == 5, lanes = kind <= 4;
+    // the levels (and task lists) this kind runs on under the current plan
+    std::vector<uint32_t> levels;
+    for (uint32_t l = 0; l < t.levels; ++l) {
+      const int mode = plan_[2 * l + (forward ? 0 : 1)];
+      const auto &st = forward ? (lanes ? t.fwl_start : (mode ? t.fwr_start : t.fwd_start))
+                               : (lanes ? t.bwl_start : (mode ? t.bwr_start : t.bwd_start));
+      if ((lanes && !mode) || st[l + 1] == st[l]) continue;
+      levels.push_back(l);
+    }
+    // algorithmic bytes: G entries on or below the diagonal once, the
+    // front vector, the pivot outputs
+    std::vector<uint8_t> in_kind(t.fronts, 0);
+    for (uint32_t l : levels) {
+      const int mode = plan_[2 * l + (forward ? 0 : 1)];
+      const auto &tasks = forward ? (lanes ? t.fwl_tasks : (mode ? t.fwr_tasks : t.fwd_tasks))
+                                  : (lanes ? t.bwl_tasks : (mode ? t.bwr_tasks : t.bwd_tasks));
+      const auto &st = forward ? (lanes ? t.fwl_start : (mode ? t.fwr_start : t.fwd_start))
+                               : (lanes ? t.bwl_start : (mode ? t.bwr_start : t.bwd_start));
+      for (uint32_t q = st[l]; q < st[l + 1]; ++q) in_kind[uint32_t(tasks[q] & 0xffffffffu)] = 1;
+    }
+    double bytes = 0.0;
+    for (uint32_t f = 0; f < t.fronts; ++f) {
+      if (!in_kind[f]) continue;
+      const double k = t.k[f], m = t.m[f];
+      const double g = m * k - 0.5 * k * (k - 1.0);
+      bytes += forward ? g * sz + 16.0 * m : g * sz + 8.0 * m + 12.0 * k;
+    }
+    all_active();
+    std::vector<hipEvent_t> ev(2 * levels.size() * reps);
+    for (auto &e : ev) HIPCHECK(hipEventCreate(&e));
+    auto run = [&](uint32_t l) {
+      const int mode = plan_[2 * l + (forward ? 0 : 1)];
+      if (forward && lanes)
+        SolveLaunch<Real>::forward_lanes(fd_, d_gc_, d_fwl_ + t.fwl_start[l], t.fwl_start[l + 1] - t.fwl_start[l], d_.done, -1, d_.B, s);
+      else if (forward)
+        SolveLaunch<Real>::forward(fd_, d_gr_, (mode ? d_fwr_ : d_fwd_) + (mode ? t.fwr_start[l] : t.fwd_start[l]),
+                                   mode ? t.fwr_start[l + 1] - t.fwr_start[l] : t.fwd_start[l + 1] - t.fwd_start[l],
+                                   d_.done, -1, d_.B, s);
+      else if (lanes)
+        SolveLaunch<Real>::backward_lanes(fd_, d_gr_, d_bwl_ + t.bwl_start[l], t.bwl_start[l + 1] - t.bwl_start[l], d_.done, -1, d_.B, s);
+      else
+        SolveLaunch<Real>::backward(fd_, d_gc_, (mode ? d_bwr_ : d_bwd_) + (mode ? t.bwr_start[l] : t.bwd_start[l]),
+                                    mode ? t.bwr_start[l + 1] - t.bwr_start[l] : t.bwd_start[l + 1] - t.bwd_start[l],
+                                    d_.done, -1, d_.B, s);
+    };
+    for (uint32_t l : levels) run(l);  // warm-up
+    size_t e = 0;
+    for (uint32_t r = 0; r < reps; ++r)
+      for (uint32_t l : levels) {
+        HIPCHECK(hipEventRecord(ev[e++], s));
+        run(l);
+        HIPCHECK(hipEventRecord(ev[e++], s));
+      }
+    HIPCHECK(hipStreamSynchronize(s));
+    double total = 0.0;
+    for (size_t q = 0; q < ev.size(); q += 2) {
+      float ms = 0.0f;
+      HIPCHECK(hipEventElapsedTime(&ms, ev[q], ev[q + 1]));
+      total += ms;
+    }
+    for (auto &x : ev) HIPCHECK(hipEventDestroy(x));
+    out.launches = uint32_t(levels.size());
+    if (!levels.empty()) {
+      out.avg_ms = total / double(reps * levels.size());
+      out.bytes = bytes / double(levels.size());
+    }
+  }
   HIPCHECK(hipEventDestroy(a));
   HIPCHECK(hipEventDestroy(b));
-  return double(ms) / std::max(repeats, 1u);
 }
 
 template <class Real>
@@ -1351,12 +1564,13 @@ int talus_mi_apply_system(talus_mi_ctx *ctx, uint32_t body, double inertia, doub
   });
 }
 
-int talus_mi_time_kernel(talus_mi_ctx *ctx, uint32_t kind, uint32_t repeats, double *avg_ms) {
-  if (!avg_ms || kind > 2) return fail(TALUS_MI_ERR_ARGUMENT, "bad arguments");
+int talus_mi_time_kernel(talus_mi_ctx *ctx, uint32_t kind, uint32_t repeats,
+                         talus_mi_kernel_timing *out) {
+  if (!ctx || !out || kind > 6) return fail(TALUS_MI_ERR_ARGUMENT, "bad arguments");
   return guarded(ctx, [&]() -> int {
     if (ctx->bodies.empty()) return fail(TALUS_MI_ERR_STATE, "no bodies");
     ensure(ctx);
-    *avg_ms = ctx->engine->time_kernel(kind, repeats);
+    ctx->engine->time_kernel(kind, repeats, *out);
     return TALUS_MI_OK;
   });
 }
@@ -1376,6 +1590,43 @@ int talus_mi_factor_info_get(talus_mi_ctx *ctx, talus_mi_factor_info *info) {
   return guarded(ctx, [&]() -> int {
     ensure(ctx);
     ctx->engine->factor_info(*info);
+    return TALUS_MI_OK;
+  });
+}
+
+int talus_mi_tune(talus_mi_ctx *ctx, uint32_t repeats, talus_mi_tune_report *report) {
+  if (!ctx || !report) return fail(TALUS_MI_ERR_ARGUMENT, "null context or report");
+  return guarded(ctx, [&]() -> int {
+    std::memset(report, 0, sizeof *report);
+    if (ctx->bodies.empty()) return TALUS_MI_OK;
+    ensure(ctx);
+    ctx->engine->tune(repeats, *report);
+    return TALUS_MI_OK;
+  });
+}
+
+int talus_mi_plan_get(talus_mi_ctx *ctx, uint8_t *plan, uint32_t capacity, uint32_t *count) {
+  if (!ctx || !count) return fail(TALUS_MI_ERR_ARGUMENT, "null context or count");
+  return guarded(ctx, [&]() -> int {
+    *count = 0;
+    if (ctx->bodies.empty()) return TALUS_MI_OK;
+    ensure(ctx);
+    const std::vector<uint8_t> p = ctx->engine->plan();
+    *count = uint32_t(p.size());
+    if (capacity < p.size()) return plan ? fail(TALUS_MI_ERR_ARGUMENT, "plan capacity too small") : TALUS_MI_OK;
+    if (!plan) return fail(TALUS_MI_ERR_ARGUMENT, "null plan");
+    std::copy(p.begin(), p.end(), plan);
+    return TALUS_MI_OK;
+  });
+}
+
+int talus_mi_plan_set(talus_mi_ctx *ctx, const uint8_t *plan, uint32_t count) {
+  if (!ctx || (count && !plan)) return fail(TALUS_MI_ERR_ARGUMENT, "bad arguments");
+  return guarded(ctx, [&]() -> int {
+    if (ctx->bodies.empty()) return count ? fail(TALUS_MI_ERR_ARGUMENT, "no bodies") : TALUS_MI_OK;
+    ensure(ctx);
+    if (!ctx->engine->set_plan(std::vector<uint8_t>(plan, plan + count)))
+      return fail(TALUS_MI_ERR_ARGUMENT, "plan does not match the factor's levels");
     return TALUS_MI_OK;
   });
 }

@@ -38,6 +38,9 @@ constexpr int FNB = 64;   // panel / tile width in DOF
 constexpr int SGRP = 4;   // solve workgroup: 4 wavefronts, 1 or 4 rows/columns each
 constexpr int SCH = 2048; // vector entries staged in LDS per chunk
 constexpr int SCHUNK = 256;  // DOF rows per gather workgroup
+constexpr int KLANES = 512;   // fronts with k <= KLANES solve one row/column per lane
+constexpr int MLANES = 2048;  // ... backward also needs m <= MLANES (LDS)
+constexpr uint32_t LANES_MIN_GROUPS = 2048;  // default plan: lanes when a level has this many
 
 struct FrontSym {
   uint32_t ncol = 0;   // nodes eliminated by this front
@@ -100,6 +103,13 @@ struct FactorTables {
   std::vector<uint64_t> w21_tasks, pack_tasks;               // front | I<<32 | J<<48
   std::vector<uint64_t> fwd_tasks, bwd_tasks;                // front | group<<32 | size<<48
   std::vector<uint32_t> fwd_start, bwd_start;
+  std::vector<uint64_t> fwl_tasks, bwl_tasks;                // small fronts: front | chunk<<32
+  std::vector<uint32_t> fwl_start, bwl_start;
+  std::vector<uint64_t> fwr_tasks, bwr_tasks;                // the other fronts, rows per wave
+  std::vector<uint32_t> fwr_start, bwr_start;
+  // default solve plan, 2 per level (forward, backward): 1 = lanes kernel for
+  // the small fronts of the level, 0 = rows-per-wavefront kernel throughout
+  std::vector<uint8_t> plan;
   std::vector<uint64_t> gat_tasks, xg_tasks;                 // front | chunk<<32, per level
   std::vector<uint32_t> gat_start, xg_start;
   // pull lists: for every front node row (rows_off indexing), the vec-pool
@@ -162,10 +172,16 @@ struct SolveLaunch {
                      const int *done, int parity, uint32_t B, hipStream_t s);
   static void forward(const FactorDev &f, const Real *gr, const uint64_t *tasks, uint32_t count,
                       const int *done, int parity, uint32_t B, hipStream_t s);
+  static void forward_lanes(const FactorDev &f, const Real *gc, const uint64_t *tasks,
+                            uint32_t count, const int *done, int parity, uint32_t B,
+                            hipStream_t s);
   static void xgather(const FactorDev &f, const uint64_t *tasks, uint32_t count,
                       const int *done, int parity, uint32_t B, hipStream_t s);
   static void backward(const FactorDev &f, const Real *gc, const uint64_t *tasks, uint32_t count,
                        const int *done, int parity, uint32_t B, hipStream_t s);
+  static void backward_lanes(const FactorDev &f, const Real *gr, const uint64_t *tasks,
+                             uint32_t count, const int *done, int parity, uint32_t B,
+                             hipStream_t s);
   static void finish(const FactorDev &f, Real *z, uint32_t N, const uint8_t *anchored,
                      const uint32_t *node_body, const int *done, int parity, uint32_t B,
                      hipStream_t s);

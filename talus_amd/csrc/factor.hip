@@ -477,6 +477,89 @@ __global__ __launch_bounds__(NT) void k_solve_forward(FactorDev f, const Real *g
   }
 }
 
+// forward for fronts with k <= KLANES: one lane per row of G, read from the
+// column-major copy so a wavefront loads 64 consecutive entries of a column
+// at a time; v[0:k] staged in LDS; no cross-lane reduction.  A wavefront
+// whose rows all lie below j_end = min(k, first row + 64) skips the zero
+// upper triangle of M11 beyond it.
+template <class Real>
+__global__ __launch_bounds__(NT) void k_solve_forward_lanes(FactorDev f, const Real *gc,
+                                                            const uint64_t *tasks, const int *done,
+                                                            int parity, uint32_t B) {
+  __shared__ double vs[KLANES];
+  const uint64_t t = tasks[blockIdx.x];
+  const uint32_t fr = lo32(t), chunk = mid16(t);
+  if (skip(done, parity, B, f.body[fr])) return;
+  const uint32_t k = f.k[fr], m = f.m[fr];
+  double *v = f.vec + f.vec_off[fr];
+  for (uint32_t q = threadIdx.x; q < k; q += NT) vs[q] = v[q];
+  __syncthreads();
+  const uint32_t first = chunk * NT + (threadIdx.x & ~63u);
+  if (first >= m) return;
+  const uint32_t i = chunk * NT + threadIdx.x;
+  const uint32_t jend = min(k, first + 64u);
+  const Real *g = gc + f.g_off[fr] + min(i, m - 1);
+  const uint64_t ld = m;
+  double s0 = 0.0, s1 = 0.0, s2 = 0.0, s3 = 0.0;
+  uint32_t j = 0;
+  for (; j + 8 <= jend; j += 8) {
+    Real a[8];
+#pragma unroll
+    for (int q = 0; q < 8; ++q) a[q] = g[uint64_t(j + q) * ld];
+    s0 += double(a[0]) * vs[j] + double(a[4]) * vs[j + 4];
+    s1 += double(a[1]) * vs[j + 1] + double(a[5]) * vs[j + 5];
+    s2 += double(a[2]) * vs[j + 2] + double(a[6]) * vs[j + 6];
+    s3 += double(a[3]) * vs[j + 3] + double(a[7]) * vs[j + 7];
+  }
+  for (; j < jend; ++j) s0 += double(g[uint64_t(j) * ld]) * vs[j];
+  if (i >= m) return;
+  const double tot = (s0 + s1) + (s2 + s3);
+  if (i < k)
+    f.y[f.yoff[fr] + i] = tot;
+  else
+    v[i] -= tot;
+}
+
+// backward for fronts with k <= KLANES and m <= MLANES: one lane per column
+// of G, read from the row-major copy (64 consecutive entries of a row per
+// wavefront load); w = [y; -x2] from the wavefront's first column on staged
+// in LDS.  Rows above a column's diagonal hold stored zeros.
+template <class Real>
+__global__ __launch_bounds__(NT) void k_solve_backward_lanes(FactorDev f, const Real *gr,
+                                                             const uint64_t *tasks, const int *done,
+                                                             int parity, uint32_t B) {
+  __shared__ double ws[MLANES];
+  const uint64_t t = tasks[blockIdx.x];
+  const uint32_t fr = lo32(t), chunk = mid16(t);
+  if (skip(done, parity, B, f.body[fr])) return;
+  const uint32_t k = f.k[fr], m = f.m[fr];
+  const double *y = f.y + f.yoff[fr];
+  const double *v = f.vec + f.vec_off[fr];
+  const uint32_t c0 = chunk * NT;
+  for (uint32_t q = c0 + threadIdx.x; q < m; q += NT) ws[q] = q < k ? y[q] : -v[q];
+  __syncthreads();
+  const uint32_t first = c0 + (threadIdx.x & ~63u);
+  if (first >= k) return;
+  const uint32_t j = c0 + threadIdx.x;
+  const Real *g = gr + f.g_off[fr] + min(j, k - 1);
+  const uint64_t ld = k;
+  double s0 = 0.0, s1 = 0.0, s2 = 0.0, s3 = 0.0;
+  uint32_t i = first;
+  for (; i + 8 <= m; i += 8) {
+    Real a[8];
+#pragma unroll
+    for (int q = 0; q < 8; ++q) a[q] = g[uint64_t(i + q) * ld];
+    s0 += double(a[0]) * ws[i] + double(a[4]) * ws[i + 4];
+    s1 += double(a[1]) * ws[i + 1] + double(a[5]) * ws[i + 5];
+    s2 += double(a[2]) * ws[i + 2] + double(a[6]) * ws[i + 6];
+    s3 += double(a[3]) * ws[i + 3] + double(a[7]) * ws[i + 7];
+  }
+  for (; i < m; ++i) s0 += double(g[uint64_t(i) * ld]) * ws[i];
+  if (j >= k) return;
+  const uint32_t *rows = f.rows + f.rows_off[fr];
+  f.xsol[3 * size_t(rows[j / 3]) + j % 3] = (s0 + s1) + (s2 + s3);
+}
+
 // backward prologue: x2 (solution at the front's update rows) into v[k:m]
 __global__ __launch_bounds__(NT) void k_solve_xgather(FactorDev f, const uint64_t *tasks,
                                                       const int *done, int parity, uint32_t B) {
@@ -608,6 +691,18 @@ void SolveLaunch<Real>::forward(const FactorDev &f, const Real *gr, const uint64
                                 uint32_t count, const int *done, int parity, uint32_t B,
                                 hipStream_t s) {
   if (count) hipLaunchKernelGGL(k_solve_forward<Real>, dim3(count), dim3(NT), 0, s, f, gr, tasks, done, parity, B);
+}
+template <class Real>
+void SolveLaunch<Real>::forward_lanes(const FactorDev &f, const Real *gc, const uint64_t *tasks,
+                                      uint32_t count, const int *done, int parity, uint32_t B,
+                                      hipStream_t s) {
+  if (count) hipLaunchKernelGGL(k_solve_forward_lanes<Real>, dim3(count), dim3(NT), 0, s, f, gc, tasks, done, parity, B);
+}
+template <class Real>
+void SolveLaunch<Real>::backward_lanes(const FactorDev &f, const Real *gr, const uint64_t *tasks,
+                                       uint32_t count, const int *done, int parity, uint32_t B,
+                                       hipStream_t s) {
+  if (count) hipLaunchKernelGGL(k_solve_backward_lanes<Real>, dim3(count), dim3(NT), 0, s, f, gr, tasks, done, parity, B);
 }
 template <class Real>
 void SolveLaunch<Real>::xgather(const FactorDev &f, const uint64_t *tasks, uint32_t count,

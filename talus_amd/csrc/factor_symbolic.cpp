@@ -393,24 +393,48 @@ void build_tables(const std::vector<Symbolic> &sym, const std::vector<Body> &bod
   // column groups
   o.fwd_start.assign(1, 0);
   o.bwd_start.assign(1, 0);
+  o.fwl_start.assign(1, 0);
+  o.bwl_start.assign(1, 0);
+  o.fwr_start.assign(1, 0);
+  o.bwr_start.assign(1, 0);
   o.gat_start.assign(1, 0);
   o.xg_start.assign(1, 0);
+  o.plan.assign(2 * size_t(L), 0);
   for (uint32_t l = 0; l < L; ++l) {
     for (uint32_t f : by_level[l]) {
-      // small fronts: 4 rows per wavefront so the staged vector is reused
+      // rows kernel: 1 or 4 rows (columns) per wavefront, every front; lanes
+      // kernel: one row (column) per lane, 256 per workgroup, for fronts with
+      // few pivots, the rest of the level again with the rows kernel
+      const bool fl = o.k[f] <= uint32_t(KLANES), bl = fl && o.m[f] <= uint32_t(MLANES);
       const uint64_t gs = o.k[f] <= 192 ? 4 * SGRP : SGRP;
-      for (uint32_t g = 0; g < tiles(o.m[f], uint32_t(gs)); ++g)
-        o.fwd_tasks.push_back(uint64_t(f) | (uint64_t(g) << 32) | (gs << 48));
-      for (uint32_t g = 0; g < tiles(o.k[f], uint32_t(gs)); ++g)
-        o.bwd_tasks.push_back(uint64_t(f) | (uint64_t(g) << 32) | (gs << 48));
+      for (uint32_t g = 0; g < tiles(o.m[f], uint32_t(gs)); ++g) {
+        const uint64_t task = uint64_t(f) | (uint64_t(g) << 32) | (gs << 48);
+        o.fwd_tasks.push_back(task);
+        if (!fl) o.fwr_tasks.push_back(task);
+      }
+      for (uint32_t g = 0; g < tiles(o.k[f], uint32_t(gs)); ++g) {
+        const uint64_t task = uint64_t(f) | (uint64_t(g) << 32) | (gs << 48);
+        o.bwd_tasks.push_back(task);
+        if (!bl) o.bwr_tasks.push_back(task);
+      }
+      if (fl)
+        for (uint32_t g = 0; g < tiles(o.m[f], 256u); ++g) o.fwl_tasks.push_back(uint64_t(f) | (uint64_t(g) << 32));
+      if (bl)
+        for (uint32_t g = 0; g < tiles(o.k[f], 256u); ++g) o.bwl_tasks.push_back(uint64_t(f) | (uint64_t(g) << 32));
       for (uint32_t g = 0; g < tiles(o.m[f], SCHUNK); ++g) o.gat_tasks.push_back(uint64_t(f) | (uint64_t(g) << 32));
       for (uint32_t g = 0; g < tiles(o.m[f] - o.k[f], SCHUNK); ++g)
         o.xg_tasks.push_back(uint64_t(f) | (uint64_t(g) << 32));
     }
     o.fwd_start.push_back(uint32_t(o.fwd_tasks.size()));
     o.bwd_start.push_back(uint32_t(o.bwd_tasks.size()));
+    o.fwl_start.push_back(uint32_t(o.fwl_tasks.size()));
+    o.bwl_start.push_back(uint32_t(o.bwl_tasks.size()));
+    o.fwr_start.push_back(uint32_t(o.fwr_tasks.size()));
+    o.bwr_start.push_back(uint32_t(o.bwr_tasks.size()));
     o.gat_start.push_back(uint32_t(o.gat_tasks.size()));
     o.xg_start.push_back(uint32_t(o.xg_tasks.size()));
+    o.plan[2 * l] = o.fwl_start[l + 1] - o.fwl_start[l] >= LANES_MIN_GROUPS ? 1 : 0;
+    o.plan[2 * l + 1] = o.bwl_start[l + 1] - o.bwl_start[l] >= LANES_MIN_GROUPS ? 1 : 0;
   }
 }
 

@@ -54,7 +54,8 @@ struct Dev {
   const uint8_t *anchored;
   const uint32_t *inc_start, *inc_slot;
   double *x, *v, *a, *carried, *shifted, *weighted, *right;
-  Real *u_eval, *P, *x0;
+  double *u_eval;  // displacement at the evaluation point (FP64 in both paths)
+  Real *P, *x0;
   Real *dx, *r, *z, *p, *q;
   // tets
   const uint4 *conn;

@@ -126,7 +126,7 @@ __global__ __launch_bounds__(TILE) void k_predict(Dev<Real> d) {
     d.carried[i] = a;
     d.shifted[i] = shifted;
     d.weighted[i] = weighted;
-    d.u_eval[i] = Real(weighted - d.X[i]);
+    d.u_eval[i] = weighted - d.X[i];
     d.x0[i] = Real(a);
   }
 }
@@ -141,7 +141,7 @@ __global__ __launch_bounds__(TILE) void k_evaluate(Dev<Real> d) {
 #pragma unroll
   for (int r = 0; r < 3; ++r) {
     const size_t i = 3 * (size_t)n + r;
-    d.u_eval[i] = Real(d.weighted[i] + P.reach * d.a[i] - d.X[i]);
+    d.u_eval[i] = d.weighted[i] + P.reach * d.a[i] - d.X[i];
   }
 }
 
@@ -569,6 +569,7 @@ __global__ __launch_bounds__(TILE) void k_kinetic(Dev<Real> d) {
 
 template <class Real, bool DIAG, bool ENERGY>
 __global__ __launch_bounds__(TILE) void k_assemble(Dev<Real> d) {
+  constexpr bool MIXED = sizeof(Real) < 8;
   __shared__ double lds[TILE / 64];
   const uint32_t t = blockIdx.x, b = d.tt_body[t];
   if (!d.params[b].active) return;
@@ -576,34 +577,40 @@ __global__ __launch_bounds__(TILE) void k_assemble(Dev<Real> d) {
   double psi_total = 0.0;
   if (e < d.tt_end[t]) {
     const uint4 c = d.conn[e];
-    Real u[12], dm[9], g[12], H[9];
-    gather_vec(d.u_eval, c, u);
-    load_dm(d, e, dm);
-    shape_gradients(dm, g);
-    displacement_gradient(u, dm, H);
-    const uint8_t flags = d.flags[e];
-    Real F[9];
+    // F in FP64 from the FP64 displacement and rest inverse; det F from the
+    // displacement form, so inversion is decided in FP64 in both paths
+    double u64[12], dm64[9], H64[9];
+    gather_vec64(d.u_eval, c, u64);
 #pragma unroll
-    for (int i = 0; i < 9; ++i) F[i] = H[i] + ((i % 4 == 0) ? Real(1) : Real(0));
-    // det F = 1 + volumeChange(H); FP64 near inversion
-    Real J = Real(1) + volume_change(H);
-    bool upright = J > Real(0);
-    if (sizeof(Real) < 8 && J < Real(0.125)) {
-      double u64[12], dm64[9], H64[9];
+    for (int k = 0; k < 9; ++k) dm64[k] = d.dminv[(size_t)k * d.T + e];
+    displacement_gradient(u64, dm64, H64);
+    const bool upright = 1.0 + volume_change(H64) > 0.0;
+    Real dm[9], g[12], H[9], F[9];
 #pragma unroll
-      for (int i = 0; i < 12; ++i) u64[i] = double(u[i]);
-#pragma unroll
-      for (int k = 0; k < 9; ++k) dm64[k] = d.dminv[(size_t)k * d.T + e];
-      displacement_gradient(u64, dm64, H64);
-      upright = 1.0 + volume_change(H64) > 0.0;
+    for (int k = 0; k < 9; ++k) {
+      dm[k] = Real(dm64[k]);
+      H[k] = Real(H64[k]);
+      F[k] = H[k] + ((k % 4 == 0) ? Real(1) : Real(0));
     }
+    shape_gradients(dm, g);
+    const uint8_t flags = d.flags[e];
+    // committed rotation: polar in the element precision, sign-fixed SVD in
+    // FP64 when inverted; the mixed path re-orthogonalises it in FP64
     Real Rm[9];
+    double R64[9];
     if (upright) {
       polar(F, Rm);
-    } else {
-      double F64[9], R64[9];
 #pragma unroll
-      for (int i = 0; i < 9; ++i) F64[i] = double(F[i]);
+      for (int i = 0; i < 9; ++i) R64[i] = double(Rm[i]);
+      if (MIXED) {
+        orthonormalize(R64);
+#pragma unroll
+        for (int i = 0; i < 9; ++i) Rm[i] = Real(R64[i]);
+      }
+    } else {
+      double F64[9];
+#pragma unroll
+      for (int i = 0; i < 9; ++i) F64[i] = H64[i] + ((i % 4 == 0) ? 1.0 : 0.0);
       svd_rotation(F64, R64);
 #pragma unroll
       for (int i = 0; i < 9; ++i) Rm[i] = Real(R64[i]);
@@ -613,29 +620,65 @@ __global__ __launch_bounds__(TILE) void k_assemble(Dev<Real> d) {
     Real psi;
     Real Wm[9];
     if (flags & FLAG_SNH) {
-      Real He[9], PF[9];
-      if (flags & FLAG_PLASTIC) {
-        Real ep[9], HE[9];
+      const bool plastic = (flags & FLAG_PLASTIC) != 0;
+      Real ep[9], He[9], Fe[9];
 #pragma unroll
-        for (int k = 0; k < 9; ++k) ep[k] = Real(d.plastic[(size_t)k * d.T + e]);
+      for (int k = 0; k < 9; ++k) ep[k] = plastic ? Real(d.plastic[(size_t)k * d.T + e]) : Real(0);
+      if (MIXED) {
+        // frame indifference, P(Fe) = R P(R^T Fe): the rotation-free
+        // R^T Fe - I is formed in FP64 and the law runs in FP32 on it, so a
+        // large rotation costs no FP32 digits of the strain
+        double Fe64[9], G64[9];
+#pragma unroll
+        for (int k = 0; k < 9; ++k) Fe64[k] = H64[k] + ((k % 4 == 0) ? 1.0 : 0.0);
+        if (plastic) {
+          double F64[9], Fi[9];
+#pragma unroll
+          for (int k = 0; k < 9; ++k) {
+            F64[k] = Fe64[k];
+            Fi[k] = d.plastic[(size_t)k * d.T + e] + ((k % 4 == 0) ? 1.0 : 0.0);
+          }
+          mul3(F64, Fi, Fe64);
+        }
+        mul3_tn(R64, Fe64, G64);
+#pragma unroll
+        for (int k = 0; k < 9; ++k) {
+          He[k] = Real(G64[k] - ((k % 4 == 0) ? 1.0 : 0.0));
+          Fe[k] = Real(Fe64[k]);
+        }
+      } else {
+        // H-form at Fe = F Fp^-1: He = H + ep + H ep (Fp^-1 = I + ep)
+        Real HE[9];
         mul3(H, ep, HE);
 #pragma unroll
-        for (int k = 0; k < 9; ++k) He[k] = H[k] + ep[k] + HE[k];
-        Real Pk[9];
-        psi = snh_stress(He, lam, mu, Pk);
+        for (int k = 0; k < 9; ++k) {
+          He[k] = H[k] + ep[k] + HE[k];
+          Fe[k] = He[k] + ((k % 4 == 0) ? Real(1) : Real(0));
+        }
+      }
+      Real Pk[9], PF[9];
+      psi = snh_stress(He, lam, mu, Pk);
+      if (MIXED) {
+        Real RP[9];
+        mul3(Rm, Pk, RP);
+#pragma unroll
+        for (int k = 0; k < 9; ++k) Pk[k] = RP[k];
+      }
+      // P = P(Fe) Fp^-T and the volume frame cof(Fe) Fp^-T
+      Real C[9];
+      cof3(Fe, C);
+      if (plastic) {
         ep[0] += Real(1);
         ep[4] += Real(1);
         ep[8] += Real(1);
         mul3_nt(Pk, ep, PF);
-        // volume frame cof(Fe) Fp^-T, Fe = I + He, Fp^-1 = I + ep
-        Real Fe[9], C[9];
-#pragma unroll
-        for (int k = 0; k < 9; ++k) Fe[k] = He[k] + ((k % 4 == 0) ? Real(1) : Real(0));
-        cof3(Fe, C);
         mul3_nt(C, ep, Wm);
       } else {
-        psi = snh_stress(H, lam, mu, PF);
-        cof3(F, Wm);
+#pragma unroll
+        for (int k = 0; k < 9; ++k) {
+          PF[k] = Pk[k];
+          Wm[k] = C[k];
+        }
       }
 #pragma unroll
       for (int k = 0; k < 9; ++k) d.W[(size_t)k * d.T + e] = Wm[k];
@@ -649,14 +692,25 @@ __global__ __launch_bounds__(TILE) void k_assemble(Dev<Real> d) {
       }
       psi *= V;
     } else {
-      // corotated: D' = R^T F - I = (R^T - I) + R^T H, eps = sym(D')
-      Real RH[9], Dp[9], eps[9], S[9];
-      mul3_tn(Rm, H, RH);
+      // corotated: D' = R^T F - I, eps = sym(D'); FP64 in the mixed path
+      // (orthogonal R, rotation-free result), else (R^T - I) + R^T H
+      Real Dp[9], eps[9], S[9];
+      if (MIXED) {
+        double F64[9], G64[9];
 #pragma unroll
-      for (int r = 0; r < 3; ++r)
+        for (int k = 0; k < 9; ++k) F64[k] = H64[k] + ((k % 4 == 0) ? 1.0 : 0.0);
+        mul3_tn(R64, F64, G64);
 #pragma unroll
-        for (int k = 0; k < 3; ++k)
-          Dp[r * 3 + k] = (Rm[k * 3 + r] - (r == k ? Real(1) : Real(0))) + RH[r * 3 + k];
+        for (int k = 0; k < 9; ++k) Dp[k] = Real(G64[k] - ((k % 4 == 0) ? 1.0 : 0.0));
+      } else {
+        Real RH[9];
+        mul3_tn(Rm, H, RH);
+#pragma unroll
+        for (int r = 0; r < 3; ++r)
+#pragma unroll
+          for (int k = 0; k < 3; ++k)
+            Dp[r * 3 + k] = (Rm[k * 3 + r] - (r == k ? Real(1) : Real(0))) + RH[r * 3 + k];
+      }
 #pragma unroll
       for (int r = 0; r < 3; ++r)
 #pragma unroll
@@ -817,7 +871,7 @@ __global__ __launch_bounds__(TILE) void k_displacement(Dev<Real> d, const double
 #pragma unroll
   for (int c = 0; c < 3; ++c) {
     const size_t i = 3 * (size_t)n + c;
-    d.u_eval[i] = Real(xin[i] - d.X[i]);
+    d.u_eval[i] = xin[i] - d.X[i];
   }
 }
 

@@ -200,6 +200,19 @@ TM_HD void polar(const R *F, R *out) {
   for (int i = 0; i < 9; ++i) out[i] = r[i];
 }
 
+// Two Newton steps of R <- (R + R^-T)/2 in FP64 (R^-T = cof(R)/det R): an
+// FP32 polar factor (orthogonal to ~1e-7) becomes orthogonal to rounding.
+TM_HD void orthonormalize(double *R) {
+#pragma unroll
+  for (int pass = 0; pass < 2; ++pass) {
+    double c[9];
+    cof3(R, c);
+    const double inv = 1.0 / (R[0] * c[0] + R[1] * c[1] + R[2] * c[2]);
+#pragma unroll
+    for (int i = 0; i < 9; ++i) R[i] = 0.5 * (R[i] + c[i] * inv);
+  }
+}
+
 // Symmetric 3x3 eigen decomposition by cyclic Jacobi in FP64; values
 // ascending, eigenvector i in column i of V (row-major).
 TM_HD void eigen_sym3(const double *A, double *w, double *V) {

@@ -26,7 +26,7 @@ def dense_system(orc, m, inertia, step):
     return A[np.ix_(free, free)], free
 
 
-def _check_solve(body, x=None, bar=1e-10, precision=api.FP64):
+def _check_solve(body, x=None, bar=1e-10, precision=api.FP64, plan=None):
     orc, m = oracle_body(body)
     if x is not None:
         orc.eval_forces(x)  # commits rotations and volume frames
@@ -42,6 +42,8 @@ def _check_solve(body, x=None, bar=1e-10, precision=api.FP64):
         b, _ = product_body(ctx, body)
         ctx.set_rotations(b, R)
         ctx.set_frames(b, W)
+        if plan is not None:
+            ctx.set_plan(np.full(ctx.plan().size, plan, dtype=np.uint8))
         z = ctx.factor_solve(b, inertia, step, r).reshape(-1)
         info = ctx.factor_info()
     assert info["ok"] == 1 and info["fronts"] > 0
@@ -66,6 +68,38 @@ def test_factor_solve_jittered_cube_with_many_fronts():
     body = scenes.cube(9, seed=6).bodies[0]
     info = _check_solve(body)
     assert info["fronts"] > 8 and info["levels"] >= 3
+
+
+@pytest.mark.parametrize("plan", [0, 1])
+def test_factor_solve_either_kernel_shape(plan):
+    """Both solve kernel shapes (rows per wavefront, one row/column per lane)
+    give the dense solution on every level."""
+    body = scenes.cube(9, seed=6).bodies[0]
+    _check_solve(body, plan=plan)
+
+
+def test_tuned_plan_is_valid_and_reproducible():
+    sc = scenes.bodies(count=6, seed=13, lo=300, hi=3000)
+    meshes = [scenes.build_mesh(b) for b in sc.bodies]
+    runs = []
+    plan = None
+    for _ in range(2):
+        with api.Context(0, api.MIXED) as ctx:
+            ids = [ctx.add_body(m, mats) for m, mats in meshes]
+            if plan is None:
+                rep = ctx.tune(2)
+                assert rep["levels"] > 0 and rep["after_seconds"] > 0.0
+                plan = ctx.plan()
+                assert plan.size == 2 * rep["levels"] and set(np.unique(plan)) <= {0, 1}
+            else:
+                ctx.set_plan(plan)
+                assert np.array_equal(ctx.plan(), plan)
+            for _ in range(3):
+                r = ctx.step(substeps=2)
+                assert not r["non_finite"]
+            runs.append([ctx.state(b)[0] for b in ids])
+    for a, b in zip(*runs):
+        assert np.array_equal(a, b)
 
 
 def test_factor_solve_mixed_storage_is_a_good_preconditioner():

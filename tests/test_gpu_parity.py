@@ -34,13 +34,42 @@ def test_element_forces_match_oracle(model, precision, bar):
         b, _ = product_body(ctx, body)
         tf, te, nf, R = ctx.eval_forces(b, x)
     err = rel_force_error(tf, tf_o)
-    if model == api.COROTATED and precision == api.MIXED:
-        bar = 1e-4  # corotated FP32 strain cancels (R^T F - I); stated in the design notes
     assert err <= bar, err
     assert np.abs(R - R_o).max() <= (1e-9 if precision == api.FP64 else 1e-5)
     scale = np.abs(nf_o).max()
     assert np.abs(nf - nf_o).max() / scale <= 10 * bar
     assert abs(te.sum() - te_o.sum()) <= 10 * bar * abs(te_o).sum()
+
+
+@pytest.mark.parametrize("model", [api.NEO_HOOKEAN, api.COROTATED])
+@pytest.mark.parametrize("plastic", [False, True])
+def test_mixed_forces_under_large_rotation_and_small_strain(model, plastic):
+    """A 1 rad turn with 1e-4 strain: FP32 F - I would lose the strain to
+    rounding; the mixed kernel forms R^T F - I in FP64 and stays within the
+    1e-5 bar."""
+    kw = dict(yield_stress=1e12, hardening_modulus=0.0) if plastic else {}
+    body = _scene_body(model, **kw)
+    orc, om = oracle_body(body)
+    rest = om["rest"]
+    rng = np.random.default_rng(31)
+    c = rest.mean(axis=0)
+    a = 1.0
+    Rz = np.array([[np.cos(a), -np.sin(a), 0.0], [np.sin(a), np.cos(a), 0.0], [0.0, 0.0, 1.0]])
+    strain = 1e-4 * rng.standard_normal((3, 3))
+    x = (rest - c) @ (Rz @ (np.eye(3) + strain)).T + c + 1e-6 * rng.standard_normal(rest.shape)
+    T = om["tet"].shape[0]
+    pl = None
+    if plastic:
+        pl = np.zeros((T, 10))
+        pl[:, :9] = (2e-4 * rng.standard_normal((T, 3, 3))).reshape(T, 9)
+        orc.set_plastic(pl)
+    tf_o, *_ = orc.eval_forces(x)
+    with api.Context(0, api.MIXED) as ctx:
+        b, _ = product_body(ctx, body)
+        if plastic:
+            ctx.set_plastic(b, pl)
+        tf, *_ = ctx.eval_forces(b, x)
+    assert rel_force_error(tf, tf_o) <= 1e-5
 
 
 def test_element_forces_jittered_per_tet_materials():
