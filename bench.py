@@ -68,11 +68,15 @@ def workload(name):
     return sc, meshes, desc
 
 
-def batched(api, meshes, precision, params, steps, warmup, tune):
+def batched(api, meshes, precision, params, steps, warmup, tune, plan=None):
     ctx = api.Context(0, precision)
     ids = [ctx.add_body(m, mats) for m, mats in meshes]
     ctx.step(params)
-    tuned = ctx.tune(3) if tune else None
+    tuned = None
+    if plan:
+        ctx.set_plan([int(c) for c in plan])
+    elif tune:
+        tuned = ctx.tune(3)
     for _ in range(max(warmup - 1, 0)):
         ctx.step(params)
     t0 = time.perf_counter()
@@ -110,14 +114,16 @@ def force_error(ctx_m, ids_m, ctx_d, ids_d):
     return worst
 
 
-def traffic_for(kernel, work):
-    """HBM bytes per launch measured by rocprofv3 PMC counters for this kernel
-    and workload (profiles/traffic_<kernel>_<workload>.json), else None."""
+def traffic_for(kernel, work, plan):
+    """HBM bytes per launch measured by rocprofv3 PMC counters for this kernel,
+    workload and solve plan (profiles/traffic_<kernel>_<workload>.json), else
+    None: a different plan launches the kernel on other levels."""
     path = os.path.join(ROOT, "profiles", f"traffic_{kernel}_{work}.json")
     if not os.path.exists(path):
         return None
     with open(path) as f:
-        return json.load(f).get("bytes_per_launch")
+        rec = json.load(f)
+    return rec.get("bytes_per_launch") if rec.get("plan") == plan else None
 
 
 def cpu_baseline(sc, budget_s, threads):
@@ -145,6 +151,27 @@ def cpu_baseline(sc, budget_s, threads):
                       f"with the oracle's C restatement (block-Jacobi PCG)"}
 
 
+def copy_probe(nbytes):
+    """One device-to-device hipMemcpy of nbytes through the HIP runtime (the
+    reference transfer for scaling PMC byte counters)."""
+    import ctypes as C
+
+    hip = C.CDLL("libamdhip64.so")
+    src, dst = C.c_void_p(), C.c_void_p()
+    for p in (src, dst):
+        if hip.hipMalloc(C.byref(p), C.c_size_t(nbytes)) != 0:
+            raise RuntimeError("hipMalloc failed")
+    try:
+        if hip.hipMemset(src, 1, C.c_size_t(nbytes)) != 0 or hip.hipDeviceSynchronize() != 0:
+            raise RuntimeError("hipMemset failed")
+        if hip.hipMemcpy(dst, src, C.c_size_t(nbytes), 3) != 0 or hip.hipDeviceSynchronize() != 0:
+            raise RuntimeError("hipMemcpy failed")  # 3 = hipMemcpyDeviceToDevice
+    finally:
+        hip.hipFree(src)
+        hip.hipFree(dst)
+    log(f"[bench] copy probe: {nbytes} bytes read, {nbytes} bytes written")
+
+
 def main(argv=None):
     ap = argparse.ArgumentParser(description=__doc__)
     ap.add_argument("--gpus", type=int, default=1)
@@ -155,9 +182,13 @@ def main(argv=None):
     ap.add_argument("--skip-plain", action="store_true")
     ap.add_argument("--skip-cpu", action="store_true")
     ap.add_argument("--no-tune", action="store_true")
+    ap.add_argument("--plan", default=None,
+                    help="fixed solve plan (digits, 2 per level) instead of autotuning the mixed context")
     ap.add_argument("--cpu-seconds", type=float, default=15.0)
     ap.add_argument("--cpu-threads", type=int, default=16)
     ap.add_argument("--plain-steps", type=int, default=2)
+    ap.add_argument("--copy-probe", action="store_true",
+                    help="end with one 256 MiB device-to-device copy (calibrates PMC byte counters)")
     args = ap.parse_args(argv)
     if args.gpus != 1:
         print(f"bench.py measures one GPU; --gpus {args.gpus} is not supported "
@@ -175,7 +206,7 @@ def main(argv=None):
 
     log("[bench] mixed batched")
     ctx_m, ids_m, wall_m, reps_m, tuned = batched(api, meshes, api.MIXED, params, args.steps,
-                                                 args.warmup, not args.no_tune)
+                                                 args.warmup, not args.no_tune, args.plan)
     value = T * sub * args.steps / wall_m
     apps = sum(r["preconditioner_applications"] for r in reps_m) / args.steps
     solves = sum(r["solves"] for r in reps_m) / args.steps
@@ -224,7 +255,8 @@ def main(argv=None):
     kd = kernels[dominant]
     roofline = {"bound": "hbm", "kernel": KERNEL_NAMES[dominant], "achieved": kd["gbs"],
                 "peak": HBM_PEAK_GBS, "unit": "GB/s", "frac": kd["gbs"] / HBM_PEAK_GBS,
-                "traffic": traffic_for(KERNEL_NAMES[dominant], args.workload),
+                "traffic": traffic_for(KERNEL_NAMES[dominant], args.workload,
+                                       "".join(map(str, ctx_m.plan()))),
                 "algorithmic_bytes": kd["bytes"], "avg_ms": kd["avg_ms"]}
     log(f"[bench] kernels {json.dumps(kernels)}")
 
@@ -268,6 +300,8 @@ def main(argv=None):
         "gpu_over_cpu": (value / cpu["value"]) if cpu else None,
     }
     print(json.dumps(line), flush=True)
+    if args.copy_probe:
+        copy_probe(256 << 20)
     return 0
 
 

@@ -1116,8 +1116,6 @@ void Engine<Real>::time_kernel(uint32_t kind, uint32_t repeats, talus_mi_kernel_
       bytes += forward ? g * sz + 16.0 * m : g * sz + 8.0 * m + 12.0 * k;
     }
     all_active();
-    std::vector<hipEvent_t> ev(2 * levels.size() * reps);
-    for (auto &e : ev) HIPCHECK(hipEventCreate(&e));
     auto run = [&](uint32_t l) {
       const int mode = plan_[2 * l + (forward ? 0 : 1)];
       if (forward && lanes)
@@ -1134,18 +1132,19 @@ void Engine<Real>::time_kernel(uint32_t kind, uint32_t repeats, talus_mi_kernel_
                                     d_.done, -1, d_.B, s);
     };
     for (uint32_t l : levels) run(l);  // warm-up
-    size_t e = 0;
-    for (uint32_t r = 0; r < reps; ++r)
-      for (uint32_t l : levels) {
-        HIPCHECK(hipEventRecord(ev[e++], s));
-        run(l);
-        HIPCHECK(hipEventRecord(ev[e++], s));
-      }
+    // one event pair per round of this kind's launches over its levels
+    std::vector<hipEvent_t> ev(2 * reps);
+    for (auto &x : ev) HIPCHECK(hipEventCreate(&x));
+    for (uint32_t r = 0; r < reps; ++r) {
+      HIPCHECK(hipEventRecord(ev[2 * r], s));
+      for (uint32_t l : levels) run(l);
+      HIPCHECK(hipEventRecord(ev[2 * r + 1], s));
+    }
     HIPCHECK(hipStreamSynchronize(s));
     double total = 0.0;
-    for (size_t q = 0; q < ev.size(); q += 2) {
+    for (uint32_t r = 0; r < reps; ++r) {
       float ms = 0.0f;
-      HIPCHECK(hipEventElapsedTime(&ms, ev[q], ev[q + 1]));
+      HIPCHECK(hipEventElapsedTime(&ms, ev[2 * r], ev[2 * r + 1]));
       total += ms;
     }
     for (auto &x : ev) HIPCHECK(hipEventDestroy(x));
