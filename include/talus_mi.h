@@ -38,6 +38,10 @@ extern "C" {
 /* Step flags. */
 #define TALUS_MI_DETECT_TEARS 1u /* tear events and host topology update */
 #define TALUS_MI_RECORD_EVENTS 2u
+/* corrections repeated until the increment is below 1e-12 of the
+ * acceleration in every body (at most 63): each substep lands on the root of
+ * its residual, as the differentiable rollout requires */
+#define TALUS_MI_CONVERGE 4u
 
 typedef struct talus_mi_ctx talus_mi_ctx;
 
@@ -253,6 +257,33 @@ typedef struct talus_mi_factor_info {
 
 /* Statistics of the current factor (zeros before the first factorization). */
 int talus_mi_factor_info_get(talus_mi_ctx *ctx, talus_mi_factor_info *info);
+
+/* ---- differentiation (no counterpart in the reference: the adjoint of
+ * the implicit rollout, for gradient-based identification and design) ---- */
+
+/* Per-tet Lame parameters in FP64, overriding the lanes' E and nu for this
+ * body until the next talus_mi_lame_set (tears keep them).  Gradients of
+ * E and nu follow from these by the chain rule of the Lame map. */
+int talus_mi_lame_set(talus_mi_ctx *ctx, uint32_t body, const double *lambda, const double *mu);
+int talus_mi_lame_get(talus_mi_ctx *ctx, uint32_t body, double *lambda, double *mu);
+/* FP64 rest positions (3 per node): rest inverse, volumes and (for density
+ * masses) the lumped masses are recomputed; the state is kept. */
+int talus_mi_rest_set(talus_mi_ctx *ctx, uint32_t body, const double *rest);
+/* `steps` converged steps (TALUS_MI_CONVERGE, tears off) from the current
+ * state, keeping x, v, a of every body every `every` steps on the host. */
+int talus_mi_rollout(talus_mi_ctx *ctx, const talus_mi_step_params *params, uint32_t steps,
+                     uint32_t every, talus_mi_report *report);
+/* Reverse pass of the last rollout for one body, for a loss with gradients
+ * x_bar, v_bar (3 per node; v_bar may be NULL) at its final state: per-tet
+ * dL/dlambda, dL/dmu, per-node dL/drest and dL/dx0, dL/dv0, dL/da0 of the
+ * rollout's initial state (NULL outputs skipped).  Segments are recomputed
+ * from the checkpoints; every substep's adjoint system (inertia M + h_eff^2
+ * K(x_w)) is solved by PCG to `tolerance` with the step's preconditioner.
+ * Elastic neo-Hookean bodies only; the state is left at the rollout's end. */
+int talus_mi_rollout_backward(talus_mi_ctx *ctx, uint32_t body, const double *x_bar,
+                              const double *v_bar, double tolerance, double *lambda_bar,
+                              double *mu_bar, double *rest_bar, double *x0_bar, double *v0_bar,
+                              double *a0_bar);
 
 /* Autotuner of the preconditioner solve (no counterpart in the reference,
  * whose factor solve is Eigen's).  The solve plan holds, per elimination

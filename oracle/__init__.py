@@ -65,7 +65,7 @@ class Settings(C.Structure):
         ("dt", C.c_float), ("gravity", C.c_float * 3),
         ("substeps", C.c_uint32), ("cg_iterations", C.c_uint32),
         ("tolerance", C.c_float), ("spectral_radius", C.c_float),
-        ("threads", C.c_uint32),
+        ("threads", C.c_uint32), ("converge", C.c_uint32),
     ]
 
 
@@ -131,6 +131,15 @@ def lib():
         L.orc_body_nodes.argtypes = [C.c_void_p]
         L.orc_body_nodes.restype = C.c_uint32
         L.orc_body_topology.argtypes = [C.c_void_p, u32p, dp, dp, C.POINTER(C.c_uint8)]
+        L.orc_body_set_lame.argtypes = [C.c_void_p, dp, dp]
+        L.orc_body_get_lame.argtypes = [C.c_void_p, dp, dp]
+        L.orc_body_set_rest.argtypes = [C.c_void_p, dp]
+        L.orc_body_set_rest.restype = C.c_int
+        L.orc_rollout.argtypes = [C.c_void_p, C.POINTER(Settings), C.c_uint32, C.c_uint32]
+        L.orc_rollout.restype = C.c_int
+        L.orc_rollout_backward.argtypes = [C.c_void_p, dp, dp, dp, dp, dp, dp, dp, dp, C.c_double]
+        L.orc_rollout_backward.restype = C.c_int
+        L.orc_hessian_apply.argtypes = [C.c_void_p, dp, dp, dp]
         _lib = L
     return _lib
 
@@ -281,8 +290,9 @@ class Body:
         lib().orc_body_set_plastic(self._h, _dp(p))
 
     def step(self, dt=1.0 / 60.0, gravity=(0.0, -9.81, 0.0), substeps=1, cg_iterations=64,
-             tolerance=1e-3, spectral_radius=0.5, threads=1):
+             tolerance=1e-3, spectral_radius=0.5, threads=1, converge=False):
         s = Settings()
+        s.converge = 1 if converge else 0
         s.dt = dt
         s.gravity[:] = list(gravity)
         s.substeps = substeps
@@ -329,6 +339,57 @@ class Body:
         lib().orc_body_topology(self._h, tet.ctypes.data_as(C.POINTER(C.c_uint32)), _dp(rest), _dp(mass),
                                 anchored.ctypes.data_as(C.POINTER(C.c_uint8)))
         return {"tet": tet.reshape(-1, 4), "rest": rest.reshape(-1, 3), "mass": mass, "anchored": anchored}
+
+    def set_lame(self, lam, mu):
+        lam = np.ascontiguousarray(lam, dtype=np.float64)
+        mu = np.ascontiguousarray(mu, dtype=np.float64)
+        lib().orc_body_set_lame(self._h, _dp(lam), _dp(mu))
+
+    def lame(self):
+        lam, mu = np.empty(self.tets), np.empty(self.tets)
+        lib().orc_body_get_lame(self._h, _dp(lam), _dp(mu))
+        return lam, mu
+
+    def set_rest(self, X):
+        X = np.ascontiguousarray(X, dtype=np.float64).reshape(-1)
+        if lib().orc_body_set_rest(self._h, _dp(X)) != 0:
+            raise ValueError("a tet degenerates or inverts at these rest positions")
+
+    def hessian_apply(self, pos, d):
+        pos = np.ascontiguousarray(pos, dtype=np.float64).reshape(-1)
+        d = np.ascontiguousarray(d, dtype=np.float64).reshape(-1)
+        y = np.empty_like(d)
+        lib().orc_hessian_apply(self._h, _dp(pos), _dp(d), _dp(y))
+        return y.reshape(-1, 3)
+
+    def rollout(self, steps, every=1, dt=1.0 / 60.0, gravity=(0.0, -9.81, 0.0), substeps=1,
+                cg_iterations=20000, tolerance=1e-13, spectral_radius=0.5, threads=1):
+        s = Settings()
+        s.dt = dt
+        s.gravity[:] = list(gravity)
+        s.substeps = substeps
+        s.cg_iterations = cg_iterations
+        s.tolerance = tolerance
+        s.spectral_radius = spectral_radius
+        s.threads = threads
+        s.converge = 1
+        if lib().orc_rollout(self._h, C.byref(s), steps, every) != 0:
+            raise RuntimeError("rollout became non-finite")
+
+    def rollout_backward(self, x_bar, v_bar=None, tolerance=1e-13):
+        n, t = self.nodes, self.tets
+        xb = np.ascontiguousarray(x_bar, dtype=np.float64).reshape(-1)
+        vb = np.zeros(3 * n) if v_bar is None else np.ascontiguousarray(v_bar, dtype=np.float64).reshape(-1)
+        out = {k: np.empty(t) for k in ("lambda", "mu")}
+        out.update({k: np.empty(3 * n) for k in ("X", "x0", "v0", "a0")})
+        rc = lib().orc_rollout_backward(self._h, _dp(xb), _dp(vb), _dp(out["lambda"]), _dp(out["mu"]),
+                                        _dp(out["X"]), _dp(out["x0"]), _dp(out["v0"]), _dp(out["a0"]),
+                                        tolerance)
+        if rc != 0:
+            raise RuntimeError(f"rollout backward failed ({rc})")
+        for k in ("X", "x0", "v0", "a0"):
+            out[k] = out[k].reshape(-1, 3)
+        return out
 
     def yield_flags(self):
         f = np.empty(self.tets, dtype=np.uint8)

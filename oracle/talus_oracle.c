@@ -822,6 +822,10 @@ struct orc_body {
       *increment, *unconstrained;
   double *partial;
   size_t nblocks;
+  /* rollout checkpoints (x, v, a every `every` steps) and final state */
+  double *ckpt, *final_state;
+  uint32_t ckpt_steps, ckpt_every;
+  orc_settings ckpt_settings;
 };
 
 orc_body *orc_body_create(const orc_mesh *mesh, const orc_material *materials,
@@ -920,7 +924,7 @@ void orc_body_free(orc_body *b) {
                   b->tet_energy, b->force, b->diag, b->precond, b->right,
                   b->weighted, b->shifted, b->carried, b->drive, b->evaluated,
                   b->residual, b->direction, b->applied, b->conditioned,
-                  b->increment, b->unconstrained, b->partial};
+                  b->increment, b->unconstrained, b->partial, b->ckpt, b->final_state};
   for (size_t i = 0; i < sizeof ptrs / sizeof ptrs[0]; ++i) free(ptrs[i]);
   free(b);
 }
@@ -1338,6 +1342,75 @@ static uint32_t commit_plastic(orc_body *b) {
   return grew;
 }
 
+/* One substep (solver.cc:3691-3950, 4351-4368, 5829-5849): predictor,
+ * first solve, corrections ("anchored reads"), velocity and position
+ * updates, plastic commit.  converge != 0 repeats the correction until the
+ * increment is below 1e-12 of the acceleration (at most 64 passes): the root
+ * of the substep residual, which the differentiable rollout needs. */
+static double substep(orc_body *b, const orc_settings *s, const rule_t *rule, double inner,
+                      orc_report *rep) {
+  const size_t N = b->m.nodes;
+  const uint32_t threads = s->threads;
+  const double inertia = 1.0 - rule->am;
+  const double step = sqrt((1.0 - rule->af) * rule->beta) * inner;
+  const double carry = (0.5 - rule->beta) * inner * inner;
+  const double reach = step * step;
+  for (size_t n = 0; n < N; ++n)
+    for (int r = 0; r < 3; ++r) {
+      const size_t i = 3 * n + r;
+      b->carried[i] = b->a[i];
+      b->shifted[i] = b->x[i] + inner * b->v[i] + carry * b->a[i];
+      b->weighted[i] = (1.0 - rule->af) * b->shifted[i] + rule->af * b->x[i];
+      b->drive[i] = b->m.mass[n] * (double)s->gravity[r];
+    }
+  double elastic = assemble(b, b->weighted, step, threads);
+  balance(b, rule, inertia, NULL);
+  int capped = 0;
+  double rel = 0.0;
+  rep->cg_iterations += pcg(b, inertia, step, s->cg_iterations, s->tolerance, b->a, threads,
+                            &capped, &rel);
+  rep->cg_capped += (uint32_t)capped;
+  if (rel > rep->cg_worst_relative) rep->cg_worst_relative = rel;
+  ++rep->solves;
+  const uint32_t reads = s->converge ? 64u : 3u;
+  for (uint32_t read = 1; read < reads; ++read) {
+    for (size_t i = 0; i < 3 * N; ++i) b->evaluated[i] = b->weighted[i] + reach * b->a[i];
+    elastic = assemble(b, b->evaluated, step, threads);
+    balance(b, rule, inertia, b->a);
+    memset(b->increment, 0, 3 * N * sizeof(double));
+    rep->cg_iterations += pcg(b, inertia, step, s->cg_iterations, s->tolerance, b->increment,
+                              threads, &capped, &rel);
+    rep->cg_capped += (uint32_t)capped;
+    if (rel > rep->cg_worst_relative) rep->cg_worst_relative = rel;
+    ++rep->solves;
+    double change = 0.0, size = 0.0;
+    for (size_t n = 0; n < N; ++n)
+      if (!b->m.anchored[n])
+        for (int r = 0; r < 3; ++r) {
+          b->a[3 * n + r] += b->increment[3 * n + r];
+          change = fmax(change, fabs(b->increment[3 * n + r]));
+          size = fmax(size, fabs(b->a[3 * n + r]));
+        }
+    if (s->converge && change <= 1.0e-12 * size) break;
+  }
+  /* velocity (4351-4368) and position (5829-5849) */
+  for (size_t n = 0; n < N; ++n) {
+    if (b->m.anchored[n]) {
+      for (int r = 0; r < 3; ++r) b->v[3 * n + r] = b->a[3 * n + r] = 0.0;
+      continue;
+    }
+    for (int r = 0; r < 3; ++r) {
+      const size_t i = 3 * n + r;
+      b->v[i] += inner * ((1.0 - rule->gamma) * b->carried[i] + rule->gamma * b->a[i]);
+      b->x[i] = b->shifted[i] + rule->beta * inner * inner * b->a[i];
+    }
+  }
+  const uint32_t grew = commit_plastic(b);
+  rep->yield_events += grew;
+  rep->plastic_growth |= grew ? 1u : 0u;
+  return elastic;
+}
+
 /* One step of deformable::step for a non-floating tet body
  * (solver.cc:3472-5901 without contacts, damping, floating frame; the
  * elastic path 3895-3950 is used for every lane, see the header note). */
@@ -1346,66 +1419,13 @@ int orc_step(orc_body *b, const orc_settings *s, orc_report *rep) {
   const size_t N = b->m.nodes;
   const rule_t rule = rule_of(s->spectral_radius);
   const uint32_t configured = s->substeps > 1 ? s->substeps : 1;
-  const uint32_t threads = s->threads;
   uint32_t substeps = configured;
   if (predictor_inverts(b, &rule, (double)s->dt / configured))
     substeps = configured > 4 ? configured : 4;
   const double inner = (double)s->dt / substeps;
-  const double inertia = 1.0 - rule.am;
-  const double step = sqrt((1.0 - rule.af) * rule.beta) * inner;
-  const double carry = (0.5 - rule.beta) * inner * inner;
-  const double reach = step * step;
   rep->substeps = substeps;
   double elastic = 0.0;
-  for (uint32_t pass = 0; pass < substeps; ++pass) {
-    for (size_t n = 0; n < N; ++n)
-      for (int r = 0; r < 3; ++r) {
-        const size_t i = 3 * n + r;
-        b->carried[i] = b->a[i];
-        b->shifted[i] = b->x[i] + inner * b->v[i] + carry * b->a[i];
-        b->weighted[i] = (1.0 - rule.af) * b->shifted[i] + rule.af * b->x[i];
-        b->drive[i] = b->m.mass[n] * (double)s->gravity[r];
-      }
-    elastic = assemble(b, b->weighted, step, threads);
-    balance(b, &rule, inertia, NULL);
-    int capped = 0;
-    double rel = 0.0;
-    rep->cg_iterations += pcg(b, inertia, step, s->cg_iterations, s->tolerance,
-                              b->a, threads, &capped, &rel);
-    rep->cg_capped += (uint32_t)capped;
-    if (rel > rep->cg_worst_relative) rep->cg_worst_relative = rel;
-    ++rep->solves;
-    for (uint32_t read = 1; read < 3; ++read) {
-      for (size_t i = 0; i < 3 * N; ++i) b->evaluated[i] = b->weighted[i] + reach * b->a[i];
-      elastic = assemble(b, b->evaluated, step, threads);
-      balance(b, &rule, inertia, b->a);
-      memset(b->increment, 0, 3 * N * sizeof(double));
-      rep->cg_iterations += pcg(b, inertia, step, s->cg_iterations,
-                                s->tolerance, b->increment, threads, &capped,
-                                &rel);
-      rep->cg_capped += (uint32_t)capped;
-      if (rel > rep->cg_worst_relative) rep->cg_worst_relative = rel;
-      ++rep->solves;
-      for (size_t n = 0; n < N; ++n)
-        if (!b->m.anchored[n])
-          for (int r = 0; r < 3; ++r) b->a[3 * n + r] += b->increment[3 * n + r];
-    }
-    /* velocity (4351-4368) and position (5829-5849) */
-    for (size_t n = 0; n < N; ++n) {
-      if (b->m.anchored[n]) {
-        for (int r = 0; r < 3; ++r) b->v[3 * n + r] = b->a[3 * n + r] = 0.0;
-        continue;
-      }
-      for (int r = 0; r < 3; ++r) {
-        const size_t i = 3 * n + r;
-        b->v[i] += inner * ((1.0 - rule.gamma) * b->carried[i] + rule.gamma * b->a[i]);
-        b->x[i] = b->shifted[i] + rule.beta * inner * inner * b->a[i];
-      }
-    }
-    const uint32_t grew = commit_plastic(b);
-    rep->yield_events += grew;
-    rep->plastic_growth |= grew ? 1u : 0u;
-  }
+  for (uint32_t pass = 0; pass < substeps; ++pass) elastic = substep(b, s, &rule, inner, rep);
   double kinetic = 0.0;
   for (size_t n = 0; n < N; ++n)
     kinetic += 0.5 * b->m.mass[n] *
@@ -1742,4 +1762,396 @@ uint32_t orc_body_tear(orc_body *b, const uint32_t *torn, uint32_t ntorn) {
   }
   free(origin);
   return added;
+}
+
+/* ------------------------------------------------------------------ */
+/* Per-element Lame parameters and rest positions in FP64               */
+/* ------------------------------------------------------------------ */
+
+void orc_body_set_lame(orc_body *b, const double *lambda, const double *mu) {
+  memcpy(b->lambda, lambda, b->m.tets * sizeof(double));
+  memcpy(b->mu, mu, b->m.tets * sizeof(double));
+}
+
+void orc_body_get_lame(const orc_body *b, double *lambda, double *mu) {
+  memcpy(lambda, b->lambda, b->m.tets * sizeof(double));
+  memcpy(mu, b->mu, b->m.tets * sizeof(double));
+}
+
+/* New rest positions: rest inverse, volume det(Dm)/6 and lumped masses
+ * rho V / 4 recomputed in FP64 (buildMesh, solver.cc:2352-2364). */
+int orc_body_set_rest(orc_body *b, const double *X) {
+  const size_t N = b->m.nodes, T = b->m.tets;
+  memcpy(b->m.rest, X, 3 * N * sizeof(double));
+  memset(b->m.mass, 0, N * sizeof(double));
+  for (size_t e = 0; e < T; ++e) {
+    const uint32_t *t = &b->m.tet[4 * e];
+    double Ds[9];
+    for (int k = 0; k < 3; ++k)
+      for (int r = 0; r < 3; ++r) Ds[r * 3 + k] = X[3 * t[k + 1] + r] - X[3 * t[0] + r];
+    const double det = orc_det3(Ds);
+    if (!(det > 0.0) || !inv3(Ds, &b->m.dminv[9 * e])) return -1;
+    b->m.volume[e] = det / 6.0;
+    for (int c = 0; c < 4; ++c) b->m.mass[t[c]] += b->rho[e] * b->m.volume[e] * 0.25;
+  }
+  return 0;
+}
+
+/* ------------------------------------------------------------------ */
+/* Adjoint of the converged rollout (no counterpart in the reference:   */
+/* the differentiation pattern of this build), neo-Hookean, elastic     */
+/* ------------------------------------------------------------------ */
+
+/* P(F) = mu (F - cof F) + (lambda + mu)(J - 1) cof F, the SNH first Piola
+ * stress in F form (equal to the H form of orc_snh_tet). */
+static void snh_P(const double *F, double lam, double mu, double *P) {
+  double C[9];
+  cof3(F, C);
+  const double J = orc_det3(F);
+  for (int i = 0; i < 9; ++i) P[i] = mu * (F[i] - C[i]) + (lam + mu) * (J - 1.0) * C[i];
+}
+
+/* d cof(F)[D]: the columns of cof F are f1 x f2, f2 x f0, f0 x f1 (0-based
+ * columns f of F), so each column's derivative is two cross products. */
+static void cof_dir(const double *F, const double *D, double *dC) {
+  for (int c = 0; c < 3; ++c) {
+    const int a = (c + 1) % 3, q = (c + 2) % 3;
+    const double fa[3] = {F[a], F[3 + a], F[6 + a]}, fb[3] = {F[q], F[3 + q], F[6 + q]};
+    const double da[3] = {D[a], D[3 + a], D[6 + a]}, db[3] = {D[q], D[3 + q], D[6 + q]};
+    const double x1[3] = {da[1] * fb[2] - da[2] * fb[1], da[2] * fb[0] - da[0] * fb[2],
+                          da[0] * fb[1] - da[1] * fb[0]};
+    const double x2[3] = {fa[1] * db[2] - fa[2] * db[1], fa[2] * db[0] - fa[0] * db[2],
+                          fa[0] * db[1] - fa[1] * db[0]};
+    for (int r = 0; r < 3; ++r) dC[r * 3 + c] = x1[r] + x2[r];
+  }
+}
+
+/* dP = dP/dF [D] = mu (D - dcof) + (lambda + mu)((cof F : D) cof F + (J - 1) dcof) */
+static void snh_dP(const double *F, const double *D, double lam, double mu, double *dP) {
+  double C[9], dC[9];
+  cof3(F, C);
+  cof_dir(F, D, dC);
+  const double J = orc_det3(F), cd = contract3(C, D);
+  for (int i = 0; i < 9; ++i)
+    dP[i] = mu * (D[i] - dC[i]) + (lam + mu) * (cd * C[i] + (J - 1.0) * dC[i]);
+}
+
+static void tet_F(const orc_body *b, const double *pos, size_t e, double *F) {
+  double x[12], X[12], u[12];
+  gather4(b, pos, e, x);
+  gather4(b, b->m.rest, e, X);
+  for (int i = 0; i < 12; ++i) u[i] = x[i] - X[i];
+  displacement_gradient(u, &b->m.dminv[9 * e], F);
+  F[0] += 1.0;
+  F[4] += 1.0;
+  F[8] += 1.0;
+}
+
+/* y = K(pos) d, K the SNH Hessian of the elastic energy (all rows). */
+void orc_hessian_apply(orc_body *b, const double *pos, const double *d, double *y) {
+  const size_t N = b->m.nodes, T = b->m.tets;
+  memset(y, 0, 3 * N * sizeof(double));
+  for (size_t e = 0; e < T; ++e) {
+    double F[9], de[12], D[9], dP[9], g[4][3];
+    tet_F(b, pos, e, F);
+    gather4(b, d, e, de);
+    displacement_gradient(de, &b->m.dminv[9 * e], D);
+    snh_dP(F, D, b->lambda[e], b->mu[e], dP);
+    shape_gradients(&b->m.dminv[9 * e], g);
+    const double V = b->m.volume[e];
+    for (int i = 0; i < 4; ++i) {
+      double t[3];
+      mv3(dP, g[i], t);
+      const uint32_t n = b->m.tet[4 * e + i];
+      for (int r = 0; r < 3; ++r) y[3 * n + r] += V * t[r];
+    }
+  }
+}
+
+/* Solves (inertia M + reach K(pos)) z = rhs on the free rows (anchored rows
+ * and entries zero) by block-Jacobi PCG to `tolerance` relative. */
+static uint32_t adjoint_solve(orc_body *b, const double *pos, double inertia, double reach,
+                              const double *rhs, double *z, double tolerance) {
+  const size_t N = b->m.nodes, T = b->m.tets;
+  double *blk = (double *)calloc(9 * N, sizeof(double));
+  for (size_t e = 0; e < T; ++e) {
+    double F[9], g[4][3];
+    tet_F(b, pos, e, F);
+    shape_gradients(&b->m.dminv[9 * e], g);
+    const double V = b->m.volume[e];
+    for (int i = 0; i < 4; ++i) {
+      const uint32_t n = b->m.tet[4 * e + i];
+      for (int c = 0; c < 3; ++c) {
+        double D[9] = {0}, dP[9], t[3];
+        for (int k = 0; k < 3; ++k) D[c * 3 + k] = g[i][k];
+        snh_dP(F, D, b->lambda[e], b->mu[e], dP);
+        mv3(dP, g[i], t);
+        for (int r = 0; r < 3; ++r) blk[9 * n + r * 3 + c] += reach * V * t[r];
+      }
+    }
+  }
+  double *inv = (double *)calloc(9 * N, sizeof(double));
+  for (size_t n = 0; n < N; ++n) {
+    double *B = &blk[9 * n];
+    B[0] += inertia * b->m.mass[n];
+    B[4] += inertia * b->m.mass[n];
+    B[8] += inertia * b->m.mass[n];
+    if (b->m.anchored[n] || !(orc_det3(B) > 0.0) || !inv3(B, &inv[9 * n])) ident3(&inv[9 * n]);
+  }
+  double *r = (double *)calloc(3 * N, sizeof(double)), *p = (double *)calloc(3 * N, sizeof(double));
+  double *q = (double *)calloc(3 * N, sizeof(double)), *w = (double *)calloc(3 * N, sizeof(double));
+  double *Kp = (double *)calloc(3 * N, sizeof(double));
+  memset(z, 0, 3 * N * sizeof(double));
+  double ref = 0.0;
+  for (size_t n = 0; n < N; ++n)
+    for (int c = 0; c < 3; ++c) {
+      r[3 * n + c] = b->m.anchored[n] ? 0.0 : rhs[3 * n + c];
+      ref += r[3 * n + c] * r[3 * n + c];
+    }
+  ref = sqrt(ref);
+  for (size_t n = 0; n < N; ++n) mv3(&inv[9 * n], &r[3 * n], &w[3 * n]);
+  memcpy(p, w, 3 * N * sizeof(double));
+  double rz = 0.0;
+  for (size_t i = 0; i < 3 * N; ++i) rz += r[i] * w[i];
+  uint32_t it = 0;
+  for (; it < 100000u && ref > 0.0; ++it) {
+    double rr = 0.0;
+    for (size_t i = 0; i < 3 * N; ++i) rr += r[i] * r[i];
+    if (sqrt(rr) <= tolerance * ref) break;
+    orc_hessian_apply(b, pos, p, Kp);
+    double pq = 0.0;
+    for (size_t n = 0; n < N; ++n)
+      for (int c = 0; c < 3; ++c) {
+        const size_t i = 3 * n + c;
+        q[i] = b->m.anchored[n] ? 0.0 : inertia * b->m.mass[n] * p[i] + reach * Kp[i];
+        pq += p[i] * q[i];
+      }
+    if (!(pq > 0.0)) break;
+    const double alpha = rz / pq;
+    for (size_t i = 0; i < 3 * N; ++i) {
+      z[i] += alpha * p[i];
+      r[i] -= alpha * q[i];
+    }
+    for (size_t n = 0; n < N; ++n) mv3(&inv[9 * n], &r[3 * n], &w[3 * n]);
+    double next = 0.0;
+    for (size_t i = 0; i < 3 * N; ++i) next += r[i] * w[i];
+    const double beta = next / rz;
+    rz = next;
+    for (size_t i = 0; i < 3 * N; ++i) p[i] = w[i] + beta * p[i];
+  }
+  free(blk);
+  free(inv);
+  free(r);
+  free(p);
+  free(q);
+  free(w);
+  free(Kp);
+  return it;
+}
+
+int orc_rollout(orc_body *b, const orc_settings *s, uint32_t steps, uint32_t every) {
+  const size_t N = b->m.nodes;
+  if (every == 0) every = 1;
+  free(b->ckpt);
+  free(b->final_state);
+  const uint32_t count = (steps + every - 1) / every + 1;
+  b->ckpt = (double *)malloc((size_t)count * 9 * N * sizeof(double));
+  b->final_state = (double *)malloc(9 * N * sizeof(double));
+  b->ckpt_steps = steps;
+  b->ckpt_every = every;
+  b->ckpt_settings = *s;
+  b->ckpt_settings.converge = 1;
+  for (uint32_t t = 0; t < steps; ++t) {
+    if (t % every == 0) {
+      double *c = &b->ckpt[(size_t)(t / every) * 9 * N];
+      memcpy(c, b->x, 3 * N * sizeof(double));
+      memcpy(c + 3 * N, b->v, 3 * N * sizeof(double));
+      memcpy(c + 6 * N, b->a, 3 * N * sizeof(double));
+    }
+    orc_report rep;
+    orc_step(b, &b->ckpt_settings, &rep);
+    if (rep.non_finite) return -1;
+  }
+  memcpy(b->final_state, b->x, 3 * N * sizeof(double));
+  memcpy(b->final_state + 3 * N, b->v, 3 * N * sizeof(double));
+  memcpy(b->final_state + 6 * N, b->a, 3 * N * sizeof(double));
+  return 0;
+}
+
+/* Reverse pass of one converged substep.  Forward (free node i):
+ *   x_w = x_n + (1-af)(h v_n + c a_n) + reach a',  c = (1/2-beta) h^2,
+ *   r(a') = inertia M a' + am M a_n - f(x_w) - M g = 0,
+ *   x' = x_n + h v_n + c a_n + beta h^2 a',  v' = v_n + h((1-gamma) a_n + gamma a');
+ * anchored nodes keep x and have v = a = 0.  With lambda = J^-1 abar_total,
+ * J = inertia M + reach K(x_w) (implicit function theorem):
+ *   xbar_n = xbar' - K lambda,  vbar_n = h xbar' + vbar' - (1-af) h K lambda,
+ *   abar_n = c xbar' + h(1-gamma) vbar' - am M lambda - (1-af) c K lambda,
+ * and the parameters get -d/dtheta [sum_e V_e P_e : D_e(lambda)] minus the
+ * mass terms (inertia a' + am a_n - g) . lambda dm/dtheta. */
+static void adjoint_substep(orc_body *b, const rule_t *rule, double inner, const float *gravity,
+                            const double *xn, const double *vn, const double *an,
+                            const double *anext, double *xb, double *vb, double *ab,
+                            double *lam_bar, double *mu_bar, double *X_bar, double tolerance) {
+  const size_t N = b->m.nodes, T = b->m.tets;
+  const double inertia = 1.0 - rule->am;
+  const double carry = (0.5 - rule->beta) * inner * inner;
+  const double reach = (1.0 - rule->af) * rule->beta * inner * inner;
+  double *xw = (double *)malloc(3 * N * sizeof(double));
+  double *seed = (double *)calloc(3 * N, sizeof(double));
+  double *lam = (double *)calloc(3 * N, sizeof(double));
+  double *Kl = (double *)calloc(3 * N, sizeof(double));
+  double *w = (double *)calloc(N, sizeof(double));
+  for (size_t i = 0; i < 3 * N; ++i)
+    xw[i] = xn[i] + (1.0 - rule->af) * (inner * vn[i] + carry * an[i]) + reach * anext[i];
+  for (size_t n = 0; n < N; ++n)
+    if (!b->m.anchored[n])
+      for (int r = 0; r < 3; ++r) {
+        const size_t i = 3 * n + r;
+        seed[i] = ab[i] + rule->beta * inner * inner * xb[i] + rule->gamma * inner * vb[i];
+      }
+  adjoint_solve(b, xw, inertia, reach, seed, lam, tolerance);
+  orc_hessian_apply(b, xw, lam, Kl);
+  for (size_t n = 0; n < N; ++n) {
+    if (b->m.anchored[n]) continue;
+    for (int r = 0; r < 3; ++r) {
+      const size_t i = 3 * n + r;
+      w[n] += (inertia * anext[i] + rule->am * an[i] - (double)gravity[r]) * lam[i];
+    }
+  }
+  /* parameters, at x_w with D = sum_j lambda_j g_j^T */
+  for (size_t e = 0; e < T; ++e) {
+    double F[9], C[9], le[12], D[9], P[9], dP[9];
+    tet_F(b, xw, e, F);
+    cof3(F, C);
+    const double J = orc_det3(F);
+    gather4(b, lam, e, le);
+    const double *dm = &b->m.dminv[9 * e];
+    displacement_gradient(le, dm, D);
+    const double V = b->m.volume[e];
+    double pl = 0.0, pm = 0.0;
+    for (int i = 0; i < 9; ++i) {
+      pl += (J - 1.0) * C[i] * D[i];
+      pm += ((F[i] - C[i]) + (J - 1.0) * C[i]) * D[i];
+    }
+    lam_bar[e] -= V * pl;
+    mu_bar[e] -= V * pm;
+    if (X_bar) {
+      /* dPhi/dDm = V[(P:D) I - F^T dP(D) - D^T P] Dm^-T, then the mass term
+       * (rho/4) sum_{i in e} w_i V Dm^-T */
+      snh_P(F, b->lambda[e], b->mu[e], P);
+      snh_dP(F, D, b->lambda[e], b->mu[e], dP);
+      const double PD = contract3(P, D);
+      const uint32_t *t = &b->m.tet[4 * e];
+      const double we = w[t[0]] + w[t[1]] + w[t[2]] + w[t[3]];
+      double A[9], G[9];
+      for (int r = 0; r < 3; ++r)
+        for (int c = 0; c < 3; ++c) {
+          double fq = 0.0, dp = 0.0;
+          for (int k = 0; k < 3; ++k) {
+            fq += F[k * 3 + r] * dP[k * 3 + c];
+            dp += D[k * 3 + r] * P[k * 3 + c];
+          }
+          A[r * 3 + c] = V * ((r == c ? PD : 0.0) - fq - dp) + (r == c ? 0.25 * b->rho[e] * we * V : 0.0);
+        }
+      mul3_nt(A, dm, G); /* G = A Dm^-T */
+      for (int k = 0; k < 3; ++k)
+        for (int r = 0; r < 3; ++r) {
+          X_bar[3 * t[k + 1] + r] -= G[r * 3 + k];
+          X_bar[3 * t[0] + r] += G[r * 3 + k];
+        }
+    }
+  }
+  for (size_t n = 0; n < N; ++n)
+    for (int r = 0; r < 3; ++r) {
+      const size_t i = 3 * n + r;
+      const double x1 = xb[i], v1 = vb[i];
+      if (b->m.anchored[n]) {
+        xb[i] = x1 - Kl[i];
+        vb[i] = -(1.0 - rule->af) * inner * Kl[i];
+        ab[i] = -(1.0 - rule->af) * carry * Kl[i];
+      } else {
+        xb[i] = x1 - Kl[i];
+        vb[i] = inner * x1 + v1 - (1.0 - rule->af) * inner * Kl[i];
+        ab[i] = carry * x1 + inner * (1.0 - rule->gamma) * v1 - rule->am * b->m.mass[n] * lam[i] -
+                (1.0 - rule->af) * carry * Kl[i];
+      }
+    }
+  free(xw);
+  free(seed);
+  free(lam);
+  free(Kl);
+  free(w);
+}
+
+int orc_rollout_backward(orc_body *b, const double *x_bar, const double *v_bar, double *lam_bar,
+                         double *mu_bar, double *X_bar, double *x0_bar, double *v0_bar,
+                         double *a0_bar, double tolerance) {
+  const size_t N = b->m.nodes, T = b->m.tets;
+  if (!b->ckpt) return -1;
+  for (size_t e = 0; e < T; ++e)
+    if (!b->snh[e] || b->plastic_lane[e]) return -2;
+  const orc_settings *s = &b->ckpt_settings;
+  const rule_t rule = rule_of(s->spectral_radius);
+  const uint32_t configured = s->substeps > 1 ? s->substeps : 1;
+  double *xb = (double *)malloc(3 * N * sizeof(double));
+  double *vb = (double *)malloc(3 * N * sizeof(double));
+  double *ab = (double *)calloc(3 * N, sizeof(double));
+  memcpy(xb, x_bar, 3 * N * sizeof(double));
+  if (v_bar) memcpy(vb, v_bar, 3 * N * sizeof(double));
+  else memset(vb, 0, 3 * N * sizeof(double));
+  memset(lam_bar, 0, T * sizeof(double));
+  memset(mu_bar, 0, T * sizeof(double));
+  if (X_bar) memset(X_bar, 0, 3 * N * sizeof(double));
+  const uint32_t every = b->ckpt_every, steps = b->ckpt_steps;
+  const uint32_t segments = (steps + every - 1) / every;
+  /* per substep of a segment: x_n, v_n, a_n, a', inner */
+  const size_t per = 12 * N;
+  for (uint32_t seg = segments; seg-- > 0;) {
+    const double *c = &b->ckpt[(size_t)seg * 9 * N];
+    memcpy(b->x, c, 3 * N * sizeof(double));
+    memcpy(b->v, c + 3 * N, 3 * N * sizeof(double));
+    memcpy(b->a, c + 6 * N, 3 * N * sizeof(double));
+    const uint32_t t0 = seg * every, t1 = t0 + every < steps ? t0 + every : steps;
+    size_t cap = 0, used = 0;
+    double *rec = NULL, *inner_of = NULL;
+    for (uint32_t t = t0; t < t1; ++t) {
+      uint32_t substeps = configured;
+      if (predictor_inverts(b, &rule, (double)s->dt / configured))
+        substeps = configured > 4 ? configured : 4;
+      const double inner = (double)s->dt / substeps;
+      for (uint32_t k = 0; k < substeps; ++k) {
+        if (used == cap) {
+          cap = cap ? 2 * cap : 16;
+          rec = (double *)realloc(rec, cap * per * sizeof(double));
+          inner_of = (double *)realloc(inner_of, cap * sizeof(double));
+        }
+        double *r = &rec[used * per];
+        memcpy(r, b->x, 3 * N * sizeof(double));
+        memcpy(r + 3 * N, b->v, 3 * N * sizeof(double));
+        memcpy(r + 6 * N, b->a, 3 * N * sizeof(double));
+        orc_report rep;
+        memset(&rep, 0, sizeof rep);
+        substep(b, s, &rule, inner, &rep);
+        memcpy(r + 9 * N, b->a, 3 * N * sizeof(double));
+        inner_of[used++] = inner;
+      }
+    }
+    for (size_t k = used; k-- > 0;) {
+      const double *r = &rec[k * per];
+      adjoint_substep(b, &rule, inner_of[k], s->gravity, r, r + 3 * N, r + 6 * N, r + 9 * N, xb,
+                      vb, ab, lam_bar, mu_bar, X_bar, tolerance);
+    }
+    free(rec);
+    free(inner_of);
+  }
+  if (x0_bar) memcpy(x0_bar, xb, 3 * N * sizeof(double));
+  if (v0_bar) memcpy(v0_bar, vb, 3 * N * sizeof(double));
+  if (a0_bar) memcpy(a0_bar, ab, 3 * N * sizeof(double));
+  memcpy(b->x, b->final_state, 3 * N * sizeof(double));
+  memcpy(b->v, b->final_state + 3 * N, 3 * N * sizeof(double));
+  memcpy(b->a, b->final_state + 6 * N, 3 * N * sizeof(double));
+  free(xb);
+  free(vb);
+  free(ab);
+  return 0;
 }

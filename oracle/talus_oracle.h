@@ -67,6 +67,7 @@ typedef struct orc_settings {
   float tolerance;
   float spectral_radius;
   uint32_t threads;         /* 0/1 = serial */
+  uint32_t converge;        /* corrections until the substep residual's root */
 } orc_settings;
 
 typedef struct orc_report {
@@ -156,6 +157,27 @@ uint32_t orc_body_tear(orc_body *body, const uint32_t *torn, uint32_t ntorn);
 uint32_t orc_body_nodes(const orc_body *body);
 void orc_body_topology(const orc_body *body, uint32_t *tet, double *rest, double *mass,
                        uint8_t *anchored);
+
+/* Per-element Lame parameters (FP64, overriding the material lanes) and
+ * FP64 rest positions (rest inverse, volumes and lumped masses recomputed;
+ * -1 if a tet degenerates or inverts). */
+void orc_body_set_lame(orc_body *body, const double *lambda, const double *mu);
+void orc_body_get_lame(const orc_body *body, double *lambda, double *mu);
+int orc_body_set_rest(orc_body *body, const double *X);
+
+/* Differentiable rollout (no counterpart in the reference): `steps`
+ * converged steps (corrections to the root of every substep's residual),
+ * checkpointing x, v, a every `every` steps; then the adjoint of a loss with
+ * gradients x_bar, v_bar (3 per node) at the last state: per-tet dL/dlambda,
+ * dL/dmu, per-node dL/dX (rest), and the initial-state gradients.  Elastic
+ * neo-Hookean bodies only (-2 otherwise).  The body's state is left at the
+ * end of the rollout. */
+int orc_rollout(orc_body *body, const orc_settings *settings, uint32_t steps, uint32_t every);
+int orc_rollout_backward(orc_body *body, const double *x_bar, const double *v_bar,
+                         double *lam_bar, double *mu_bar, double *X_bar, double *x0_bar,
+                         double *v0_bar, double *a0_bar, double tolerance);
+/* y = K(pos) d with K the neo-Hookean Hessian of the elastic energy. */
+void orc_hessian_apply(orc_body *body, const double *pos, const double *d, double *y);
 
 #ifdef __cplusplus
 }

@@ -25,6 +25,7 @@ BLOCK_JACOBI = 0
 FACTOR = 1
 DETECT_TEARS = 1
 RECORD_EVENTS = 2
+CONVERGE = 4
 EVENT_YIELD = 0
 EVENT_TEAR = 1
 COROTATED = 0
@@ -178,6 +179,13 @@ _SIGNATURES = {
                                         _P(C.c_double), _P(C.c_double)]),
     "talus_mi_factor_info_get": (C.c_int, [C.c_void_p, _P(FactorInfo)]),
     "talus_mi_tune": (C.c_int, [C.c_void_p, C.c_uint32, _P(TuneReport)]),
+    "talus_mi_lame_set": (C.c_int, [C.c_void_p, C.c_uint32, _P(C.c_double), _P(C.c_double)]),
+    "talus_mi_lame_get": (C.c_int, [C.c_void_p, C.c_uint32, _P(C.c_double), _P(C.c_double)]),
+    "talus_mi_rest_set": (C.c_int, [C.c_void_p, C.c_uint32, _P(C.c_double)]),
+    "talus_mi_rollout": (C.c_int, [C.c_void_p, _P(StepParams), C.c_uint32, C.c_uint32, _P(Report)]),
+    "talus_mi_rollout_backward": (C.c_int, [C.c_void_p, C.c_uint32, _P(C.c_double), _P(C.c_double),
+                                            C.c_double, _P(C.c_double), _P(C.c_double), _P(C.c_double),
+                                            _P(C.c_double), _P(C.c_double), _P(C.c_double)]),
     "talus_mi_plan_get": (C.c_int, [C.c_void_p, _P(C.c_uint8), C.c_uint32, _P(C.c_uint32)]),
     "talus_mi_plan_set": (C.c_int, [C.c_void_p, _P(C.c_uint8), C.c_uint32]),
     "talus_mi_stream": (C.c_void_p, [C.c_void_p]),
@@ -498,6 +506,47 @@ class Context:
         i = FactorInfo()
         _check(lib().talus_mi_factor_info_get(self._h, C.byref(i)))
         return {k: getattr(i, k) for k, _ in FactorInfo._fields_ if k != "reserved"}
+
+    def set_lame(self, body, lam, mu):
+        """Per-tet Lame parameters in FP64 (override the lanes' E and nu)."""
+        _, t = self.info(body)
+        lam, mu = _f64(lam, t), _f64(mu, t)
+        _check(lib().talus_mi_lame_set(self._h, body, _dp(lam), _dp(mu)))
+
+    def lame(self, body):
+        _, t = self.info(body)
+        lam, mu = np.empty(t), np.empty(t)
+        _check(lib().talus_mi_lame_get(self._h, body, _dp(lam), _dp(mu)))
+        return lam, mu
+
+    def set_rest(self, body, rest):
+        """FP64 rest positions; rest inverse, volumes and masses follow."""
+        n, _ = self.info(body)
+        rest = _f64(rest, 3 * n)
+        _check(lib().talus_mi_rest_set(self._h, body, _dp(rest)))
+
+    def rollout(self, steps, every=1, params: StepParams | None = None, **kw) -> dict:
+        """`steps` converged steps, checkpointing every `every` steps."""
+        p = params if params is not None else default_params(**kw)
+        r = Report()
+        _check(lib().talus_mi_rollout(self._h, C.byref(p), steps, every, C.byref(r)))
+        return r.as_dict()
+
+    def rollout_backward(self, body, x_bar, v_bar=None, tolerance=1e-12) -> dict:
+        """Gradients of the last rollout for one body: dL/dlambda, dL/dmu
+        (per tet), dL/drest, dL/dx0, dL/dv0, dL/da0 (per node)."""
+        n, t = self.info(body)
+        xb = _f64(x_bar, 3 * n)
+        vb = _f64(v_bar, 3 * n) if v_bar is not None else None
+        out = {"lambda": np.empty(t), "mu": np.empty(t)}
+        out.update({k: np.empty(3 * n) for k in ("rest", "x0", "v0", "a0")})
+        _check(lib().talus_mi_rollout_backward(self._h, body, _dp(xb), _dp(vb) if vb is not None else None,
+                                               tolerance, _dp(out["lambda"]), _dp(out["mu"]),
+                                               _dp(out["rest"]), _dp(out["x0"]), _dp(out["v0"]),
+                                               _dp(out["a0"])))
+        for k in ("rest", "x0", "v0", "a0"):
+            out[k] = out[k].reshape(n, 3)
+        return out
 
     def tune(self, repeats=3) -> dict:
         """Autotunes the preconditioner solve's kernel shapes per level."""

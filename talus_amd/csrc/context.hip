@@ -15,6 +15,7 @@
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
+#include <functional>
 #include <cmath>
 #include <cstring>
 #include <memory>
@@ -51,8 +52,10 @@ struct DeviceError : std::runtime_error {
 
 int prepare_body(Body &b) {
   const uint32_t lanes = uint32_t(b.materials.size());
+  if (b.lame_set && (b.lambda.size() != b.tets || b.mu.size() != b.tets)) b.lame_set = false;
   b.lambda.resize(b.tets);
   b.mu.resize(b.tets);
+  b.rho.assign(b.tets, 0.0);
   b.yield.resize(b.tets);
   b.hardening.resize(b.tets);
   b.plastic_max.resize(b.tets);
@@ -66,7 +69,8 @@ int prepare_body(Body &b) {
     if (!rest_frame(b.rest.data(), &b.tet[4 * size_t(e)], &b.dminv[9 * size_t(e)], &b.volume[e]))
       return fail(TALUS_MI_ERR_ARGUMENT, "tet " + std::to_string(e) + " is not positively oriented");
     const talus_mi_material &m = lane_material(b.materials.data(), lanes, b.lane[e]);
-    precise_lame(m.youngs_modulus, m.poissons_ratio, &b.lambda[e], &b.mu[e]);
+    if (!b.lame_set) precise_lame(m.youngs_modulus, m.poissons_ratio, &b.lambda[e], &b.mu[e]);
+    if (b.density_mass) b.rho[e] = double(m.density);
     const bool plastic = std::isfinite(m.yield_stress) && m.yield_stress > 0.0f;
     b.yield[e] = plastic ? double(m.yield_stress) : INFINITY;
     b.hardening[e] = m.hardening_modulus > 0.0f ? double(m.hardening_modulus) : 0.0;
@@ -115,6 +119,15 @@ class EngineBase {
                             double *z) = 0;
   virtual void factor_info(talus_mi_factor_info &info) = 0;
   virtual void tune(uint32_t repeats, talus_mi_tune_report &report) = 0;
+  virtual void state_all_get(std::vector<double> &xva) = 0;
+  virtual void state_all_set(const std::vector<double> &xva) = 0;
+  virtual void rollout_backward(const talus_mi_step_params &sp,
+                                const std::vector<std::vector<double>> &ckpt,
+                                const std::vector<double> &final_state, uint32_t steps,
+                                uint32_t every, uint32_t body, const double *x_bar,
+                                const double *v_bar, double tolerance, double *lambda_bar,
+                                double *mu_bar, double *rest_bar, double *x0_bar, double *v0_bar,
+                                double *a0_bar) = 0;
   virtual std::vector<uint8_t> plan() = 0;
   virtual bool set_plan(const std::vector<uint8_t> &plan) = 0;
 };
@@ -148,6 +161,26 @@ class Engine final : public EngineBase {
   void factor_solve(uint32_t b, double inertia, double step, const double *r, double *z) override;
   void factor_info(talus_mi_factor_info &info) override;
   void tune(uint32_t repeats, talus_mi_tune_report &report) override;
+  void state_all_get(std::vector<double> &xva) override {
+    const size_t n3 = 3 * size_t(d_.N);
+    xva.resize(3 * n3);
+    HIPCHECK(hipStreamSynchronize(stream_));
+    HIPCHECK(hipMemcpy(xva.data(), d_.x, n3 * sizeof(double), hipMemcpyDeviceToHost));
+    HIPCHECK(hipMemcpy(xva.data() + n3, d_.v, n3 * sizeof(double), hipMemcpyDeviceToHost));
+    HIPCHECK(hipMemcpy(xva.data() + 2 * n3, d_.a, n3 * sizeof(double), hipMemcpyDeviceToHost));
+  }
+  void state_all_set(const std::vector<double> &xva) override {
+    const size_t n3 = 3 * size_t(d_.N);
+    HIPCHECK(hipStreamSynchronize(stream_));
+    HIPCHECK(hipMemcpy(d_.x, xva.data(), n3 * sizeof(double), hipMemcpyHostToDevice));
+    HIPCHECK(hipMemcpy(d_.v, xva.data() + n3, n3 * sizeof(double), hipMemcpyHostToDevice));
+    HIPCHECK(hipMemcpy(d_.a, xva.data() + 2 * n3, n3 * sizeof(double), hipMemcpyHostToDevice));
+  }
+  void rollout_backward(const talus_mi_step_params &sp, const std::vector<std::vector<double>> &ckpt,
+                        const std::vector<double> &final_state, uint32_t steps, uint32_t every,
+                        uint32_t body, const double *x_bar, const double *v_bar, double tolerance,
+                        double *lambda_bar, double *mu_bar, double *rest_bar, double *x0_bar,
+                        double *v0_bar, double *a0_bar) override;
   std::vector<uint8_t> plan() override {
     if (!sym_ready_) analyze_factor();
     return plan_;
@@ -191,7 +224,7 @@ class Engine final : public EngineBase {
     built_ = false;
   }
   void upload_params(const std::vector<StepParams> &params, size_t slot);
-  void solve(bool first, uint32_t limit, uint32_t solve_index);
+  void solve(bool first, uint32_t limit, uint32_t solve_index, bool tangent = false);
   void all_active();
   // block Cholesky preconditioner
   void analyze_factor();
@@ -221,6 +254,12 @@ class Engine final : public EngineBase {
   bool any_plastic_ = false, any_tear_ = false;
   uint64_t n_snh_ = 0, n_plastic_ = 0;
   uint32_t applications_ = 0;  // preconditioner applications this step
+  double *hnorms_ = nullptr, *d_norms_ = nullptr;  // corrections' convergence
+  struct Recorder {
+    std::vector<double *> buffers;  // per pass: x_n, v_n, a_n, a' (3N each)
+    std::vector<std::vector<StepParams>> params;
+  };
+  Recorder *recorder_ = nullptr;
   // factor state
   const std::vector<Body> *bodies_ = nullptr;
   bool sym_ready_ = false, fac_ready_ = false, fac_ok_ = false, use_factor_ = false;
@@ -253,6 +292,7 @@ void Engine<Real>::build(const std::vector<Body> &bodies) {
   release();
   bodies_ = &bodies;
   sym_ready_ = fac_ready_ = fac_ok_ = false;
+  hnorms_ = d_norms_ = nullptr;
   fd_ = FactorDev{};
   const uint32_t B = uint32_t(bodies.size());
   uint64_t N = 0, T = 0;
@@ -310,7 +350,7 @@ void Engine<Real>::build(const std::vector<Body> &bodies) {
   std::vector<uint8_t> anchored(N);
   std::vector<uint32_t> inc_start(N + 1, 0), inc_slot(4 * T);
   std::vector<uint4> conn(T);
-  std::vector<double> dminv(9 * T), vol64(T), lam64(T), mu64(T), yield(T), hard(T), pmax(T),
+  std::vector<double> dminv(9 * T), vol64(T), lam64(T), mu64(T), rho(T), yield(T), hard(T), pmax(T),
       tear(T), plastic(10 * T);
   std::vector<Real> vol(T), lam(T), mu(T), R(9 * T), W(9 * T);
   std::vector<uint8_t> flags(T);
@@ -338,6 +378,7 @@ void Engine<Real>::build(const std::vector<Body> &bodies) {
       }
       for (int k = 0; k < 10; ++k) plastic[size_t(k) * T + g] = bd.plastic[10 * size_t(e) + k];
       vol64[g] = bd.volume[e];
+      rho[g] = bd.rho[e];
       lam64[g] = bd.lambda[e];
       mu64[g] = bd.mu[e];
       vol[g] = Real(bd.volume[e]);
@@ -386,6 +427,7 @@ void Engine<Real>::build(const std::vector<Body> &bodies) {
   d_.lam = dev_copy(lam);
   d_.mu = dev_copy(mu);
   d_.vol64 = dev_copy(vol64);
+  d_.rho = dev_copy(rho);
   d_.lam64 = dev_copy(lam64);
   d_.mu64 = dev_copy(mu64);
   d_.flags = dev_copy(flags);
@@ -706,11 +748,17 @@ void Engine<Real>::all_active() {
 // solve) or zero (corrections) is the starting iterate; the iterate change
 // accumulates in dx and is added to a by apply_increment.
 template <class Real>
-void Engine<Real>::solve(bool first, uint32_t limit, uint32_t solve_index) {
+void Engine<Real>::solve(bool first, uint32_t limit, uint32_t solve_index, bool tangent) {
   hipStream_t s = stream_;
   const bool fac = use_factor_;
-  if (first) {
-    Launch<Real>::op_element(d_, d_.x0, -1, s);
+  // tangent: the neo-Hookean Hessian operator of the adjoint solve, zero
+  // initial guess, solution left in dx
+  auto op = [&](const Real *dir, int parity) {
+    if (tangent) Launch<Real>::hvp(d_, dir, parity, s);
+    else Launch<Real>::op_element(d_, dir, parity, s);
+  };
+  if (first && !tangent) {
+    op(d_.x0, -1);
     Launch<Real>::pcg_init(d_, true, fac, s);
   } else {
     Launch<Real>::pcg_init(d_, false, fac, s);
@@ -732,7 +780,7 @@ void Engine<Real>::solve(bool first, uint32_t limit, uint32_t solve_index) {
       for (uint32_t b = 0; b < d_.B && all; ++b) all = hflags_[b] != 0;
       if (all) break;
     }
-    Launch<Real>::op_element(d_, d_.p, parity, s);
+    op(d_.p, parity);
     Launch<Real>::op_gather(d_, d_.p, d_.q, parity, s);
     Launch<Real>::pcg_update(d_, parity, fac, s);
     if (fac) {
@@ -750,6 +798,7 @@ void Engine<Real>::solve(bool first, uint32_t limit, uint32_t solve_index) {
       if (all) break;
     }
   }
+  if (tangent) return;
   Launch<Real>::apply_increment(d_, first, s);
   const size_t slot = (solve_index % hsolve_slots_) * d_.B;
   HIPCHECK(hipMemcpyAsync(hiters_ + slot, d_.iters, d_.B * sizeof(uint32_t), hipMemcpyDeviceToHost, s));
@@ -819,16 +868,22 @@ void Engine<Real>::step(const talus_mi_step_params &sp, talus_mi_report &rep,
     hparams_slots_ = passes + 2;
     hparams_ = pinned<StepParams>(hparams_slots_ * B);
   }
-  if (3 * passes > hsolve_slots_) {
+  const bool converge = (sp.flags & TALUS_MI_CONVERGE) != 0;
+  const uint32_t reads = converge ? 64u : 3u;  // solves per pass at most
+  if (reads * passes > hsolve_slots_) {
     HIPCHECK(hipStreamSynchronize(s));
-    hsolve_slots_ = 3 * passes;
+    hsolve_slots_ = reads * passes;
     hiters_ = pinned<uint32_t>(hsolve_slots_ * B);
     hrr_ = pinned<double>(hsolve_slots_ * B);
     htarget_ = pinned<double>(hsolve_slots_ * B);
   }
+  if (converge && !hnorms_) hnorms_ = pinned<double>(2 * size_t(d_.n_ntiles));
+  if (converge && !d_norms_) d_norms_ = dev<double>(2 * size_t(d_.n_ntiles));
+  std::vector<uint32_t> solve_pass;
   uint32_t solve_index = 0;
   const uint32_t refactors_before = refactors_;
   applications_ = 0;
+  const size_t n3 = 3 * size_t(d_.N);
   for (uint32_t pass = 0; pass < passes; ++pass) {
     for (uint32_t b = 0; b < B; ++b) params[b] = params_for(substeps[b], pass < substeps[b]);
     if (factor) {
@@ -849,18 +904,50 @@ void Engine<Real>::step(const talus_mi_step_params &sp, talus_mi_report &rep,
     use_factor_ = factor && fac_ok_;
     const bool diag = !use_factor_;
     upload_params(params, 1 + pass);
+    double *record = nullptr;
+    if (recorder_) {
+      // x_n, v_n, a_n now, a' after the pass (adjoint replay)
+      HIPCHECK(hipMalloc(&record, 4 * n3 * sizeof(double)));
+      recorder_->buffers.push_back(record);
+      recorder_->params.push_back(params);
+      HIPCHECK(hipMemcpyAsync(record, d_.x, n3 * sizeof(double), hipMemcpyDeviceToDevice, s));
+      HIPCHECK(hipMemcpyAsync(record + n3, d_.v, n3 * sizeof(double), hipMemcpyDeviceToDevice, s));
+      HIPCHECK(hipMemcpyAsync(record + 2 * n3, d_.a, n3 * sizeof(double), hipMemcpyDeviceToDevice, s));
+    }
     Launch<Real>::predict(d_, s);
     Launch<Real>::assemble(d_, diag, false, s);
     Launch<Real>::balance(d_, diag, false, s);
+    solve_pass.push_back(pass);
     solve(true, limit, solve_index++);
-    for (uint32_t read = 1; read < 3; ++read) {
+    for (uint32_t read = 1; read < reads; ++read) {
       Launch<Real>::evaluate(d_, s);
-      Launch<Real>::assemble(d_, diag, read == 2, s);
+      Launch<Real>::assemble(d_, diag, converge || read == 2, s);
       Launch<Real>::balance(d_, diag, true, s);
+      solve_pass.push_back(pass);
       solve(false, limit, solve_index++);
+      if (converge) {
+        // corrections until max |increment| <= 1e-12 max |a| in every body
+        Launch<Real>::increment_norms(d_, d_norms_, s);
+        HIPCHECK(hipMemcpyAsync(hnorms_, d_norms_, 2 * size_t(d_.n_ntiles) * sizeof(double),
+                                hipMemcpyDeviceToHost, s));
+        HIPCHECK(hipStreamSynchronize(s));
+        bool all = true;
+        for (uint32_t b = 0; b < B && all; ++b) {
+          if (pass >= substeps[b]) continue;
+          double dx = 0.0, a = 0.0;
+          for (uint32_t t = nt0_[b]; t < nt1_[b]; ++t) {
+            dx = std::max(dx, hnorms_[2 * t]);
+            a = std::max(a, hnorms_[2 * t + 1]);
+          }
+          all = dx <= 1.0e-12 * a;
+        }
+        if (all) break;
+      }
     }
     Launch<Real>::finish(d_, s);
     if (any_plastic_) Launch<Real>::commit(d_, s);
+    if (record)
+      HIPCHECK(hipMemcpyAsync(record + 3 * n3, d_.a, n3 * sizeof(double), hipMemcpyDeviceToDevice, s));
   }
   // report reductions over every body
   for (uint32_t b = 0; b < B; ++b) params[b] = params_for(substeps[b], true);
@@ -899,7 +986,7 @@ void Engine<Real>::step(const talus_mi_step_params &sp, talus_mi_report &rep,
   rep.preconditioner_applications = applications_;
   rep.device_seconds = ms * 1.0e-3;
   for (uint32_t si = 0; si < solve_index; ++si) {
-    const uint32_t pass = si / 3;
+    const uint32_t pass = solve_pass[si];
     const size_t slot = (si % hsolve_slots_) * B;
     for (uint32_t b = 0; b < B; ++b) {
       if (pass >= substeps[b]) continue;
@@ -1175,6 +1262,105 @@ void Engine<Real>::factor_solve(uint32_t b, double inertia, double step, const d
   for (size_t i = 0; i < 3 * nn; ++i) z[i] = double(out[i]);
 }
 
+// Reverse pass of a converged rollout (implicit function theorem per
+// substep; the derivation is in oracle/talus_oracle.c, adjoint_substep):
+// per checkpoint segment from the last, replay the segment recording every
+// substep, then walk the substeps backwards: x_w and the adjoint right-hand
+// side, PCG on inertia M + h_eff^2 K(x_w) with the step's preconditioner,
+// K lambda over every row, the adjoint state one substep back, and the
+// parameter / rest-position gradients.
+template <class Real>
+void Engine<Real>::rollout_backward(const talus_mi_step_params &sp,
+                                    const std::vector<std::vector<double>> &ckpt,
+                                    const std::vector<double> &final_state, uint32_t steps,
+                                    uint32_t every, uint32_t body, const double *x_bar,
+                                    const double *v_bar, double tolerance, double *lambda_bar,
+                                    double *mu_bar, double *rest_bar, double *x0_bar,
+                                    double *v0_bar, double *a0_bar) {
+  hipStream_t s = stream_;
+  const size_t N = d_.N, T = d_.T, n3 = 3 * N;
+  const size_t b0 = n0_[body], bn = nn_[body], e0 = t0_[body], et = tt_[body];
+  std::vector<void *> mine;
+  auto alloc = [&](size_t count) {
+    double *p = nullptr;
+    HIPCHECK(hipMalloc(&p, std::max<size_t>(count, 1) * sizeof(double)));
+    mine.push_back(p);
+    HIPCHECK(hipMemsetAsync(p, 0, std::max<size_t>(count, 1) * sizeof(double), s));
+    return p;
+  };
+  Recorder rec;
+  try {
+    AdjDev a{};
+    a.xb = alloc(n3);
+    a.vb = alloc(n3);
+    a.ab = alloc(n3);
+    a.kl = alloc(n3);
+    a.w = alloc(N);
+    a.lam_bar = alloc(T);
+    a.mu_bar = alloc(T);
+    a.xslot = alloc(12 * T);
+    a.X_bar = alloc(n3);
+    HIPCHECK(hipMemcpyAsync(a.xb + 3 * b0, x_bar, 3 * bn * sizeof(double), hipMemcpyHostToDevice, s));
+    if (v_bar)
+      HIPCHECK(hipMemcpyAsync(a.vb + 3 * b0, v_bar, 3 * bn * sizeof(double), hipMemcpyHostToDevice, s));
+    talus_mi_step_params conv = sp;
+    conv.flags = (sp.flags | TALUS_MI_CONVERGE) & ~(TALUS_MI_DETECT_TEARS | TALUS_MI_RECORD_EVENTS);
+    const uint32_t limit = std::max(sp.cg_iterations, 2000u);
+    std::vector<talus_mi_event> events;
+    std::vector<uint8_t> torn;
+    talus_mi_report rep;
+    const uint32_t segments = (steps + every - 1) / every;
+    for (uint32_t seg = segments; seg-- > 0;) {
+      state_all_set(ckpt[seg]);
+      rec = Recorder{};
+      recorder_ = &rec;
+      const uint32_t t1 = std::min(steps, (seg + 1) * every);
+      for (uint32_t t = seg * every; t < t1; ++t) step(conv, rep, events, 0, torn);
+      recorder_ = nullptr;
+      for (size_t k = rec.buffers.size(); k-- > 0;) {
+        std::vector<StepParams> params = rec.params[k];
+        for (auto &p : params) {
+          p.tolerance = tolerance;
+          p.limit = limit;
+        }
+        HIPCHECK(hipStreamSynchronize(s));
+        upload_params(params, 0);
+        const double *r = rec.buffers[k];
+        a.xn = r;
+        a.vn = r + n3;
+        a.an = r + 2 * n3;
+        a.anext = r + 3 * n3;
+        Launch<Real>::adj_prepare(d_, a, s);
+        solve(false, limit, 0, true);
+        Launch<Real>::hvp(d_, d_.dx, -1, s);
+        Launch<Real>::adj_update(d_, a, s);
+        Launch<Real>::adj_params(d_, a, s);
+      }
+      HIPCHECK(hipStreamSynchronize(s));
+      for (double *p : rec.buffers) (void)hipFree(p);
+      rec.buffers.clear();
+    }
+    HIPCHECK(hipStreamSynchronize(s));
+    auto out = [&](double *dst, const double *src, size_t count) {
+      if (dst) HIPCHECK(hipMemcpy(dst, src, count * sizeof(double), hipMemcpyDeviceToHost));
+    };
+    out(lambda_bar, a.lam_bar + e0, et);
+    out(mu_bar, a.mu_bar + e0, et);
+    out(rest_bar, a.X_bar + 3 * b0, 3 * bn);
+    out(x0_bar, a.xb + 3 * b0, 3 * bn);
+    out(v0_bar, a.vb + 3 * b0, 3 * bn);
+    out(a0_bar, a.ab + 3 * b0, 3 * bn);
+    state_all_set(final_state);
+  } catch (...) {
+    recorder_ = nullptr;
+    (void)hipStreamSynchronize(s);
+    for (double *p : rec.buffers) (void)hipFree(p);
+    for (void *p : mine) (void)hipFree(p);
+    throw;
+  }
+  for (void *p : mine) (void)hipFree(p);
+}
+
 template <class Real>
 void Engine<Real>::factor_info(talus_mi_factor_info &info) {
   std::memset(&info, 0, sizeof info);
@@ -1198,6 +1384,13 @@ struct talus_mi_ctx {
   std::vector<tmi::Body> bodies;
   std::unique_ptr<tmi::EngineBase> engine;
   bool dirty = true;
+  uint64_t version = 0;  // counts edits that rebuild the engine
+  // differentiable rollout: checkpoints (x, v, a of every body) and settings
+  std::vector<std::vector<double>> ckpt;
+  std::vector<double> ckpt_final;
+  uint32_t ckpt_steps = 0, ckpt_every = 0;
+  uint64_t ckpt_version = ~0ull;
+  talus_mi_step_params ckpt_params{};
   uint32_t steps = 0;
   std::vector<talus_mi_event> events;
 };
@@ -1401,6 +1594,7 @@ int talus_mi_body_add(talus_mi_ctx *ctx, const talus_mi_mesh *mesh,
     ctx->engine->download(ctx->bodies);
     ctx->bodies.push_back(std::move(b));
     ctx->dirty = true;
+    ++ctx->version;
     if (body) *body = uint32_t(ctx->bodies.size() - 1);
     return TALUS_MI_OK;
   });
@@ -1527,6 +1721,7 @@ int talus_mi_step(talus_mi_ctx *ctx, const talus_mi_step_params *params, talus_m
         report->split_nodes += split;
         report->nodes += split;
         ctx->dirty = true;
+        ++ctx->version;
       }
     }
     return TALUS_MI_OK;
@@ -1633,3 +1828,135 @@ int talus_mi_plan_set(talus_mi_ctx *ctx, const uint8_t *plan, uint32_t count) {
 void *talus_mi_stream(talus_mi_ctx *ctx) { return ctx ? (void *)ctx->stream : nullptr; }
 
 }  // extern "C"
+
+// ------------------------------------------------------- differentiation
+
+namespace {
+
+int edit_body(talus_mi_ctx *ctx, uint32_t body, const std::function<int(tmi::Body &)> &edit) {
+  if (!ctx || !body_ok(ctx, body)) return fail(TALUS_MI_ERR_ARGUMENT, "unknown body");
+  return guarded(ctx, [&]() -> int {
+    ensure(ctx);
+    ctx->engine->download(ctx->bodies);
+    tmi::Body &b = ctx->bodies[body];
+    tmi::Body saved = b;
+    int rc = edit(b);
+    if (rc == TALUS_MI_OK) rc = tmi::prepare_body(b);
+    if (rc != TALUS_MI_OK) {
+      b = saved;
+      return rc;
+    }
+    ctx->dirty = true;
+    ++ctx->version;
+    return TALUS_MI_OK;
+  });
+}
+
+}  // namespace
+
+int talus_mi_lame_set(talus_mi_ctx *ctx, uint32_t body, const double *lambda, const double *mu) {
+  if (!lambda || !mu) return fail(TALUS_MI_ERR_ARGUMENT, "null parameters");
+  return edit_body(ctx, body, [&](tmi::Body &b) -> int {
+    for (uint32_t e = 0; e < b.tets; ++e)
+      if (!(std::isfinite(lambda[e]) && std::isfinite(mu[e]) && mu[e] > 0.0 && lambda[e] >= 0.0))
+        return fail(TALUS_MI_ERR_ARGUMENT, "Lame parameters must be finite, mu > 0, lambda >= 0");
+    b.lambda.assign(lambda, lambda + b.tets);
+    b.mu.assign(mu, mu + b.tets);
+    b.lame_set = true;
+    return TALUS_MI_OK;
+  });
+}
+
+int talus_mi_lame_get(talus_mi_ctx *ctx, uint32_t body, double *lambda, double *mu) {
+  if (!ctx || !body_ok(ctx, body) || !lambda || !mu) return fail(TALUS_MI_ERR_ARGUMENT, "bad arguments");
+  const tmi::Body &b = ctx->bodies[body];
+  std::copy(b.lambda.begin(), b.lambda.end(), lambda);
+  std::copy(b.mu.begin(), b.mu.end(), mu);
+  return TALUS_MI_OK;
+}
+
+int talus_mi_rest_set(talus_mi_ctx *ctx, uint32_t body, const double *rest) {
+  if (!rest) return fail(TALUS_MI_ERR_ARGUMENT, "null rest positions");
+  return edit_body(ctx, body, [&](tmi::Body &b) -> int {
+    b.rest.assign(rest, rest + 3 * size_t(b.nodes));
+    return TALUS_MI_OK;
+  });
+}
+
+int talus_mi_rollout(talus_mi_ctx *ctx, const talus_mi_step_params *params, uint32_t steps,
+                     uint32_t every, talus_mi_report *report) {
+  if (!params || !report || steps == 0) return fail(TALUS_MI_ERR_ARGUMENT, "bad arguments");
+  if (!(params->dt > 0.0f)) return fail(TALUS_MI_ERR_ARGUMENT, "dt must be positive");
+  if (params->preconditioner > TALUS_MI_FACTOR) return fail(TALUS_MI_ERR_ARGUMENT, "unknown preconditioner");
+  return guarded(ctx, [&]() -> int {
+    std::memset(report, 0, sizeof *report);
+    if (ctx->bodies.empty()) return fail(TALUS_MI_ERR_STATE, "no bodies");
+    ensure(ctx);
+    talus_mi_step_params p = *params;
+    p.flags = (p.flags | TALUS_MI_CONVERGE) & ~TALUS_MI_DETECT_TEARS;
+    if (every == 0) every = 1;
+    ctx->ckpt.clear();
+    ctx->ckpt_version = ~0ull;
+    std::vector<uint8_t> torn;
+    for (uint32_t t = 0; t < steps; ++t) {
+      if (t % every == 0) {
+        ctx->ckpt.emplace_back();
+        ctx->engine->state_all_get(ctx->ckpt.back());
+      }
+      talus_mi_report r;
+      ctx->engine->step(p, r, ctx->events, ctx->steps, torn);
+      ++ctx->steps;
+      report->bodies = r.bodies;
+      report->nodes = r.nodes;
+      report->elements = r.elements;
+      report->substeps += r.substeps;
+      report->cg_iterations += r.cg_iterations;
+      report->cg_capped += r.cg_capped;
+      report->solves += r.solves;
+      report->refactors += r.refactors;
+      report->cg_worst_relative = std::max(report->cg_worst_relative, r.cg_worst_relative);
+      report->elastic_energy = r.elastic_energy;
+      report->kinetic_energy = r.kinetic_energy;
+      report->yield_events += r.yield_events;
+      report->preconditioner_applications += r.preconditioner_applications;
+      report->device_seconds += r.device_seconds;
+      if (r.non_finite) {
+        report->non_finite = 1;
+        return fail(TALUS_MI_ERR_STATE, "rollout became non-finite at step " + std::to_string(t));
+      }
+    }
+    ctx->engine->state_all_get(ctx->ckpt_final);
+    ctx->ckpt_steps = steps;
+    ctx->ckpt_every = every;
+    ctx->ckpt_params = p;
+    ctx->ckpt_version = ctx->version;
+    return TALUS_MI_OK;
+  });
+}
+
+int talus_mi_rollout_backward(talus_mi_ctx *ctx, uint32_t body, const double *x_bar,
+                              const double *v_bar, double tolerance, double *lambda_bar,
+                              double *mu_bar, double *rest_bar, double *x0_bar, double *v0_bar,
+                              double *a0_bar) {
+  if (!ctx || !body_ok(ctx, body) || !x_bar) return fail(TALUS_MI_ERR_ARGUMENT, "bad arguments");
+  if (!(tolerance > 0.0)) return fail(TALUS_MI_ERR_ARGUMENT, "tolerance must be positive");
+  return guarded(ctx, [&]() -> int {
+    if (ctx->dirty || ctx->ckpt.empty() || ctx->ckpt_version != ctx->version)
+      return fail(TALUS_MI_ERR_STATE, "no rollout of the current bodies to differentiate");
+    const tmi::Body &b = ctx->bodies[body];
+    for (uint32_t e = 0; e < b.tets; ++e)
+      if ((b.flags[e] & tmi::FLAG_PLASTIC) || !(b.flags[e] & tmi::FLAG_SNH))
+        return fail(TALUS_MI_ERR_ARGUMENT, "the adjoint supports elastic neo-Hookean bodies only");
+    // the replays overwrite the per-tet operator state: keep it
+    ctx->engine->download(ctx->bodies);
+    ctx->engine->rollout_backward(ctx->ckpt_params, ctx->ckpt, ctx->ckpt_final, ctx->ckpt_steps,
+                                  ctx->ckpt_every, body, x_bar, v_bar, tolerance, lambda_bar,
+                                  mu_bar, rest_bar, x0_bar, v0_bar, a0_bar);
+    for (uint32_t q = 0; q < ctx->bodies.size(); ++q) {
+      ctx->engine->rotations_set(q, ctx->bodies[q].R.data());
+      ctx->engine->frames_set(q, ctx->bodies[q].W.data());
+      ctx->engine->plastic_set(q, ctx->bodies[q].plastic.data());
+    }
+    return TALUS_MI_OK;
+  });
+}

@@ -34,8 +34,11 @@ struct Body {
   std::vector<double> volume;     // tets
   std::vector<uint32_t> lane;     // tets
   std::vector<talus_mi_material> materials;
-  // per-tet material data
+  // per-tet material data; lame_set keeps lambda and mu as given
+  // (talus_mi_lame_set) instead of deriving them from the lanes
   std::vector<double> lambda, mu, yield, hardening, plastic_max, tear;
+  std::vector<double> rho;  // density when masses follow rho V / 4, else 0
+  bool lame_set = false;
   std::vector<uint8_t> flags;
   // state mirror (valid when the device copy is not)
   std::vector<double> x, v, a;

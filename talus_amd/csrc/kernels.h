@@ -62,6 +62,7 @@ struct Dev {
   const double *dminv;
   const Real *vol, *lam, *mu;
   const double *vol64, *lam64, *mu64;
+  const double *rho;  // density when masses follow rho V / 4, else 0
   const uint8_t *flags;  // bit0 neo-Hookean, bit1 plastic lane, bit2 tearable
   const double *yield, *hard, *pmax, *tear;
   Real *R;
@@ -83,6 +84,18 @@ struct Dev {
 };
 
 enum : uint8_t { FLAG_SNH = 1, FLAG_PLASTIC = 2, FLAG_TEAR = 4 };
+
+// Adjoint of one converged substep (differentiable rollout): adjoint state,
+// accumulated gradients and the recorded substep (x_n, v_n, a_n, a').
+struct AdjDev {
+  double *xb, *vb, *ab;       // 3N adjoint of x, v, a
+  double *w;                  // N, mass-term weight (inertia a' + am a_n - g) . lambda
+  double *kl;                 // 3N, K(x_w) lambda (all rows)
+  double *lam_bar, *mu_bar;   // T
+  double *xslot;              // 12 T, per-corner rest-position gradient
+  double *X_bar;              // 3N
+  const double *xn, *vn, *an, *anext;
+};
 
 template <class Real>
 struct Launch {
@@ -109,6 +122,13 @@ struct Launch {
                           double reach, hipStream_t s);
   static void op_element_plain(const Dev<Real> &d, const Real *dir, hipStream_t s);
   static void gather_forces(const Dev<Real> &d, Real *out, hipStream_t s);
+  // convergence of the corrections: per node tile max |dx| and max |a|
+  static void increment_norms(const Dev<Real> &d, double *out, hipStream_t s);
+  // adjoint of a converged substep (neo-Hookean Hessian)
+  static void hvp(const Dev<Real> &d, const Real *dir, int parity, hipStream_t s);
+  static void adj_prepare(const Dev<Real> &d, const AdjDev &a, hipStream_t s);
+  static void adj_update(const Dev<Real> &d, const AdjDev &a, hipStream_t s);
+  static void adj_params(const Dev<Real> &d, const AdjDev &a, hipStream_t s);
 };
 
 }  // namespace tmi

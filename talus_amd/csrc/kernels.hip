@@ -915,6 +915,255 @@ __global__ __launch_bounds__(TILE) void k_apply_plain(Dev<Real> d, const Real *d
 
 inline unsigned blocks_for(uint32_t n) { return (n + TILE - 1) / TILE; }
 
+// ------------------------------------------------- corrections converge
+
+// per node tile: max |dx| and max |a| over the tile's free nodes
+template <class Real>
+__global__ __launch_bounds__(TILE) void k_increment_norms(Dev<Real> d, double *out) {
+  __shared__ double lds[2][TILE / 64];
+  const uint32_t t = blockIdx.x, b = d.nt_body[t];
+  const uint32_t n = d.nt_begin[t] + threadIdx.x;
+  double dx = 0.0, a = 0.0;
+  if (d.params[b].active && n < d.nt_end[t] && !d.anchored[n]) {
+#pragma unroll
+    for (int c = 0; c < 3; ++c) {
+      dx = fmax(dx, fabs(double(d.dx[3 * (size_t)n + c])));
+      a = fmax(a, fabs(d.a[3 * (size_t)n + c]));
+    }
+  }
+#pragma unroll
+  for (int off = 32; off > 0; off >>= 1) {
+    dx = fmax(dx, __shfl_down(dx, off, 64));
+    a = fmax(a, __shfl_down(a, off, 64));
+  }
+  const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+  if (lane == 0) {
+    lds[0][wave] = dx;
+    lds[1][wave] = a;
+  }
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    double m0 = 0.0, m1 = 0.0;
+    for (int w = 0; w < TILE / 64; ++w) {
+      m0 = fmax(m0, lds[0][w]);
+      m1 = fmax(m1, lds[1][w]);
+    }
+    out[2 * t] = m0;
+    out[2 * t + 1] = m1;
+  }
+}
+
+// ------------------------------------------------------------- adjoint
+
+// d cof(F)[D]: the columns of cof F are f1 x f2, f2 x f0, f0 x f1 (0-based
+// columns of F), so each column's derivative is two cross products.
+__device__ __forceinline__ void cof_dir(const double *F, const double *D, double *dC) {
+#pragma unroll
+  for (int c = 0; c < 3; ++c) {
+    const int a = (c + 1) % 3, q = (c + 2) % 3;
+    const double fa[3] = {F[a], F[3 + a], F[6 + a]}, fb[3] = {F[q], F[3 + q], F[6 + q]};
+    const double da[3] = {D[a], D[3 + a], D[6 + a]}, db[3] = {D[q], D[3 + q], D[6 + q]};
+    dC[0 * 3 + c] = (da[1] * fb[2] - da[2] * fb[1]) + (fa[1] * db[2] - fa[2] * db[1]);
+    dC[1 * 3 + c] = (da[2] * fb[0] - da[0] * fb[2]) + (fa[2] * db[0] - fa[0] * db[2]);
+    dC[2 * 3 + c] = (da[0] * fb[1] - da[1] * fb[0]) + (fa[0] * db[1] - fa[1] * db[0]);
+  }
+}
+
+// SNH stress derivative dP/dF[D] = mu (D - dcof) + (lambda+mu)((C:D) C + (J-1) dcof)
+__device__ __forceinline__ void snh_dP(const double *F, const double *D, double lam, double mu,
+                                       double *dP) {
+  double C[9], dC[9];
+  cof3(F, C);
+  cof_dir(F, D, dC);
+  const double J = det3(F);
+  double cd = 0.0;
+#pragma unroll
+  for (int i = 0; i < 9; ++i) cd += C[i] * D[i];
+#pragma unroll
+  for (int i = 0; i < 9; ++i) dP[i] = mu * (D[i] - dC[i]) + (lam + mu) * (cd * C[i] + (J - 1.0) * dC[i]);
+}
+
+// F of tet e at the evaluation point (u_eval) and its FP64 shape gradients
+template <class Real>
+__device__ __forceinline__ void eval_frame(const Dev<Real> &d, uint32_t e, uint4 c, double *F,
+                                           double *dm, double *g) {
+  double u[12];
+  gather_vec64(d.u_eval, c, u);
+#pragma unroll
+  for (int k = 0; k < 9; ++k) dm[k] = d.dminv[(size_t)k * d.T + e];
+  displacement_gradient(u, dm, F);
+  F[0] += 1.0;
+  F[4] += 1.0;
+  F[8] += 1.0;
+  shape_gradients(dm, g);
+}
+
+// y_i = V dP(D) g_i, D = sum_j dir_j g_j^T: the neo-Hookean Hessian at the
+// evaluation point applied to dir, per tet into the force slots
+template <class Real>
+__global__ __launch_bounds__(TILE) void k_hvp(Dev<Real> d, const Real *dir, int parity) {
+  const uint32_t t = blockIdx.x, b = d.tt_body[t];
+  if (!d.params[b].active || (parity >= 0 && d.done[parity * d.B + b])) return;
+  const uint32_t e = d.tt_begin[t] + threadIdx.x;
+  if (e >= d.tt_end[t]) return;
+  const uint4 c = d.conn[e];
+  double F[9], dm[9], g[12], dv[12], D[9], dP[9];
+  eval_frame(d, e, c, F, dm, g);
+  const uint32_t n[4] = {c.x, c.y, c.z, c.w};
+#pragma unroll
+  for (int i = 0; i < 4; ++i)
+#pragma unroll
+    for (int r = 0; r < 3; ++r) dv[3 * i + r] = double(dir[3 * (size_t)n[i] + r]);
+  displacement_gradient(dv, dm, D);
+  snh_dP(F, D, d.lam64[e], d.mu64[e], dP);
+  const double V = d.vol64[e];
+  Real y[12];
+#pragma unroll
+  for (int i = 0; i < 4; ++i) {
+    double tmp[3];
+    mv3(dP, &g[3 * i], tmp);
+#pragma unroll
+    for (int r = 0; r < 3; ++r) y[3 * i + r] = Real(V * tmp[r]);
+  }
+  store_slots(d.slot_f, e, y);
+}
+
+// x_w = x_n + (1-af)(h v_n + c a_n) + reach a' (as u_eval = x_w - X), the
+// adjoint right-hand side abar + beta h^2 xbar + gamma h vbar on free rows,
+// zero initial guess
+template <class Real>
+__global__ __launch_bounds__(TILE) void k_adj_prepare(Dev<Real> d, AdjDev a) {
+  const uint32_t t = blockIdx.x, b = d.nt_body[t];
+  const StepParams &P = d.params[b];
+  if (!P.active) return;
+  const uint32_t n = d.nt_begin[t] + threadIdx.x;
+  if (n >= d.nt_end[t]) return;
+#pragma unroll
+  for (int c = 0; c < 3; ++c) {
+    const size_t i = 3 * (size_t)n + c;
+    const double xw = a.xn[i] + (1.0 - P.af) * (P.inner * a.vn[i] + P.carry * a.an[i]) + P.reach * a.anext[i];
+    d.u_eval[i] = xw - d.X[i];
+    d.right[i] = d.anchored[n] ? 0.0
+                               : a.ab[i] + P.beta * P.inner * P.inner * a.xb[i] + P.gamma * P.inner * a.vb[i];
+    d.x0[i] = Real(0);
+  }
+}
+
+// K lambda over every row (slots of k_hvp on lambda = dx, fixed CSR order),
+// the mass-term weight, and the adjoint state one substep back
+template <class Real>
+__global__ __launch_bounds__(TILE) void k_adj_update(Dev<Real> d, AdjDev a) {
+  const uint32_t t = blockIdx.x, b = d.nt_body[t];
+  const StepParams &P = d.params[b];
+  if (!P.active) return;
+  const uint32_t n = d.nt_begin[t] + threadIdx.x;
+  if (n >= d.nt_end[t]) return;
+  double kl[3] = {0.0, 0.0, 0.0};
+  const uint32_t k1 = d.inc_start[n + 1];
+  for (uint32_t k = d.inc_start[n]; k < k1; ++k) {
+    const size_t s = d.inc_slot[k];
+#pragma unroll
+    for (int c = 0; c < 3; ++c) kl[c] += double(d.slot_f[3 * s + c]);
+  }
+  const bool fixed = d.anchored[n] != 0;
+  const double m = d.mass[n];
+  double w = 0.0;
+#pragma unroll
+  for (int c = 0; c < 3; ++c) {
+    const size_t i = 3 * (size_t)n + c;
+    const double lam = fixed ? 0.0 : double(d.dx[i]);
+    const double x1 = a.xb[i], v1 = a.vb[i];
+    a.kl[i] = kl[c];
+    w += (P.inertia * a.anext[i] + P.am * a.an[i] - P.g[c]) * lam;
+    a.xb[i] = x1 - kl[c];
+    if (fixed) {
+      a.vb[i] = -(1.0 - P.af) * P.inner * kl[c];
+      a.ab[i] = -(1.0 - P.af) * P.carry * kl[c];
+    } else {
+      a.vb[i] = P.inner * x1 + v1 - (1.0 - P.af) * P.inner * kl[c];
+      a.ab[i] = P.carry * x1 + P.inner * (1.0 - P.gamma) * v1 - P.am * m * lam -
+                (1.0 - P.af) * P.carry * kl[c];
+    }
+  }
+  a.w[n] = fixed ? 0.0 : w;
+}
+
+// per tet at x_w with D = sum_j lambda_j g_j^T:
+//   lambda_bar -= V (J-1) C:D,  mu_bar -= V ((F - C) + (J-1) C):D,
+//   rest slots: -dPhi/dX with dPhi/dDm = V[(P:D) I - F^T dP(D) - D^T P] Dm^-T
+//   and the mass term (rho/4) sum_{i in e} w_i V Dm^-T
+template <class Real>
+__global__ __launch_bounds__(TILE) void k_adj_params(Dev<Real> d, AdjDev a) {
+  const uint32_t t = blockIdx.x, b = d.tt_body[t];
+  if (!d.params[b].active) return;
+  const uint32_t e = d.tt_begin[t] + threadIdx.x;
+  if (e >= d.tt_end[t]) return;
+  const uint4 c = d.conn[e];
+  const uint32_t n[4] = {c.x, c.y, c.z, c.w};
+  double F[9], dm[9], g[12], le[12], D[9], C[9], P[9], dP[9];
+  eval_frame(d, e, c, F, dm, g);
+#pragma unroll
+  for (int i = 0; i < 4; ++i)
+#pragma unroll
+    for (int r = 0; r < 3; ++r) le[3 * i + r] = d.anchored[n[i]] ? 0.0 : double(d.dx[3 * (size_t)n[i] + r]);
+  displacement_gradient(le, dm, D);
+  cof3(F, C);
+  const double J = det3(F), lam = d.lam64[e], mu = d.mu64[e], V = d.vol64[e];
+  double pl = 0.0, pm = 0.0;
+#pragma unroll
+  for (int i = 0; i < 9; ++i) {
+    pl += (J - 1.0) * C[i] * D[i];
+    pm += ((F[i] - C[i]) + (J - 1.0) * C[i]) * D[i];
+    P[i] = mu * (F[i] - C[i]) + (lam + mu) * (J - 1.0) * C[i];
+  }
+  a.lam_bar[e] -= V * pl;
+  a.mu_bar[e] -= V * pm;
+  snh_dP(F, D, lam, mu, dP);
+  double PD = 0.0;
+#pragma unroll
+  for (int i = 0; i < 9; ++i) PD += P[i] * D[i];
+  const double we = 0.25 * d.rho[e] * (a.w[n[0]] + a.w[n[1]] + a.w[n[2]] + a.w[n[3]]);
+  double A[9], G[9];
+#pragma unroll
+  for (int r = 0; r < 3; ++r)
+#pragma unroll
+    for (int q = 0; q < 3; ++q) {
+      double fq = 0.0, dp = 0.0;
+#pragma unroll
+      for (int k = 0; k < 3; ++k) {
+        fq += F[k * 3 + r] * dP[k * 3 + q];
+        dp += D[k * 3 + r] * P[k * 3 + q];
+      }
+      A[r * 3 + q] = V * ((r == q ? PD + we : 0.0) - fq - dp);
+    }
+  mul3_nt(A, dm, G);
+  double *o = a.xslot + (size_t)e * 12;
+#pragma unroll
+  for (int r = 0; r < 3; ++r) {
+    o[r] = G[r * 3 + 0] + G[r * 3 + 1] + G[r * 3 + 2];
+#pragma unroll
+    for (int k = 0; k < 3; ++k) o[3 * (k + 1) + r] = -G[r * 3 + k];
+  }
+}
+
+// X_bar += per-node sums of the rest slots in CSR order
+template <class Real>
+__global__ __launch_bounds__(TILE) void k_adj_gather_rest(Dev<Real> d, AdjDev a) {
+  const uint32_t t = blockIdx.x, b = d.nt_body[t];
+  if (!d.params[b].active) return;
+  const uint32_t n = d.nt_begin[t] + threadIdx.x;
+  if (n >= d.nt_end[t]) return;
+  double s3[3] = {0.0, 0.0, 0.0};
+  const uint32_t k1 = d.inc_start[n + 1];
+  for (uint32_t k = d.inc_start[n]; k < k1; ++k) {
+    const size_t s = d.inc_slot[k];
+#pragma unroll
+    for (int c = 0; c < 3; ++c) s3[c] += a.xslot[3 * s + c];
+  }
+#pragma unroll
+  for (int c = 0; c < 3; ++c) a.X_bar[3 * (size_t)n + c] += s3[c];
+}
+
 }  // namespace
 
 // ---------------------------------------------------------------- launchers
@@ -997,6 +1246,27 @@ void Launch<Real>::commit(const Dev<Real> &d, hipStream_t s) {
 template <class Real>
 void Launch<Real>::tear(const Dev<Real> &d, hipStream_t s) {
   TMI_LAUNCH(k_tear<Real>, d.n_ttiles, s, d);
+}
+template <class Real>
+void Launch<Real>::increment_norms(const Dev<Real> &d, double *out, hipStream_t s) {
+  TMI_LAUNCH(k_increment_norms<Real>, d.n_ntiles, s, d, out);
+}
+template <class Real>
+void Launch<Real>::hvp(const Dev<Real> &d, const Real *dir, int parity, hipStream_t s) {
+  TMI_LAUNCH(k_hvp<Real>, d.n_ttiles, s, d, dir, parity);
+}
+template <class Real>
+void Launch<Real>::adj_prepare(const Dev<Real> &d, const AdjDev &a, hipStream_t s) {
+  TMI_LAUNCH(k_adj_prepare<Real>, d.n_ntiles, s, d, a);
+}
+template <class Real>
+void Launch<Real>::adj_update(const Dev<Real> &d, const AdjDev &a, hipStream_t s) {
+  TMI_LAUNCH(k_adj_update<Real>, d.n_ntiles, s, d, a);
+}
+template <class Real>
+void Launch<Real>::adj_params(const Dev<Real> &d, const AdjDev &a, hipStream_t s) {
+  TMI_LAUNCH(k_adj_params<Real>, d.n_ttiles, s, d, a);
+  TMI_LAUNCH(k_adj_gather_rest<Real>, d.n_ntiles, s, d, a);
 }
 template <class Real>
 void Launch<Real>::kinetic(const Dev<Real> &d, hipStream_t s) {
