@@ -39,11 +39,14 @@ sys.path.insert(0, ROOT)
 METRIC = "tet updates/sec: mixed-precision batched vs plain FP64, 1 GPU, ≤1e-5 rel force err"
 HBM_PEAK_GBS = 8000.0
 KINDS = {"assemble": 0, "op_element": 1, "op_gather": 2, "solve_forward_lanes": 3,
-         "solve_backward_lanes": 4, "solve_forward_rows": 5, "solve_backward_rows": 6}
+         "solve_backward_lanes": 4, "solve_forward_rows": 5, "solve_backward_rows": 6,
+         "solve_forward_split": 7, "solve_backward_split": 8}
 KERNEL_NAMES = {"assemble": "k_assemble", "op_element": "k_op_element", "op_gather": "k_op_gather",
                 "solve_forward_lanes": "k_solve_forward_lanes",
                 "solve_backward_lanes": "k_solve_backward_lanes",
-                "solve_forward_rows": "k_solve_forward", "solve_backward_rows": "k_solve_backward"}
+                "solve_forward_rows": "k_solve_forward", "solve_backward_rows": "k_solve_backward",
+                "solve_forward_split": "k_solve_forward_split",
+                "solve_backward_split": "k_solve_backward_split"}
 
 
 def log(msg):
@@ -172,6 +175,80 @@ def copy_probe(nbytes):
     log(f"[bench] copy probe: {nbytes} bytes read, {nbytes} bytes written")
 
 
+def adjoint_check(steps, every, elements, seed=5):
+    """Differentiation workload: the 1 x 0.1 x 0.1 m beam (103x9x9 hexes,
+    50,058 tets, per-tet lognormal E, nu U[0.25,0.4]), a converged 60-step
+    rollout with checkpoints every 10 steps, loss |x_tip - target|^2; the
+    autograd gradients in E and nu of `elements` random tets and in 8 rest
+    coordinates against central finite differences (h = 1e-6 relative for
+    E, nu; 1e-7 m for rest positions) of the same FP64 rollout."""
+    import torch
+
+    from talus_amd import adjoint, api, scenes
+
+    sc = scenes.beam()
+    body = sc.bodies[0]
+    mesh, mats = scenes.build_mesh(body)
+    E0 = mats["youngs_modulus"].astype(np.float64)
+    nu0 = mats["poissons_ratio"].astype(np.float64)
+    params = api.default_params(preconditioner=api.FACTOR, tolerance=1e-8, cg_iterations=500)
+    out = {"workload": f"beam 103x9x9 hexes, {E0.size} tets, {steps} converged steps, checkpoints "
+                       f"every {every}, loss |x_tip - target|^2", "tets": int(E0.size)}
+    with api.Context(0, api.FP64) as ctx:
+        b = ctx.add_body(mesh, mats)
+        rest0 = torch.tensor(mesh["rest_positions"].astype(np.float64))
+        pos = mesh["rest_positions"]
+        tip = int(np.argmax(pos[:, 0] * 1e3 + pos[:, 1] + 1e-3 * pos[:, 2]))
+        target = torch.tensor(sc.extra["target"], dtype=torch.float64)
+
+        def loss(E, nu, rest):
+            x, _ = adjoint.rollout(ctx, b, E, nu, rest, steps=steps, every=every, params=params)
+            return ((x[tip] - target) ** 2).sum()
+
+        E = torch.tensor(E0, requires_grad=True)
+        nu = torch.tensor(nu0, requires_grad=True)
+        rest = rest0.clone().requires_grad_(True)
+        t0 = time.perf_counter()
+        L = loss(E, nu, rest)
+        t1 = time.perf_counter()
+        L.backward()
+        t2 = time.perf_counter()
+        out.update(loss=float(L.detach()), forward_s=t1 - t0, backward_s=t2 - t1)
+        log(f"[bench] adjoint: forward {t1 - t0:.2f} s, backward {t2 - t1:.2f} s")
+        grads = {"E": E.grad.numpy(), "nu": nu.grad.numpy(), "rest": rest.grad.numpy().reshape(-1)}
+        rng = np.random.default_rng(seed)
+        picks = rng.choice(E0.size, elements, replace=False)
+        free = np.flatnonzero(np.repeat(mesh["anchored"] == 0, 3))
+        rest_picks = rng.choice(free, 8, replace=False)
+        for kind, idxs in (("E", picks), ("nu", picks), ("rest", rest_picks)):
+            errs, rels = [], []
+            for idx in idxs:
+                vals = []
+                with torch.no_grad():
+                    for sgn in (1.0, -1.0):
+                        Ep, nup, rp = torch.tensor(E0), torch.tensor(nu0), rest0.clone().reshape(-1)
+                        if kind == "E":
+                            h = 1e-6 * E0[idx]
+                            Ep[idx] += sgn * h
+                        elif kind == "nu":
+                            h = 1e-6 * nu0[idx]
+                            nup[idx] += sgn * h
+                        else:
+                            h = 1e-7
+                            rp[idx] += sgn * h
+                        vals.append(float(loss(Ep, nup, rp.reshape(-1, 3))))
+                fd = (vals[0] - vals[1]) / (2 * h)
+                g = float(grads[kind][idx])
+                log(f"[bench] adjoint {kind}[{int(idx)}]: gradient {g:.6e}, central difference {fd:.6e}")
+                errs.append(abs(g - fd))
+                rels.append(abs(g - fd) / max(abs(fd), 1e-300))
+            scale = float(np.abs(grads[kind]).max())
+            out[kind] = {"checked": len(idxs), "max_abs_err_over_max_grad": max(errs) / scale,
+                         "median_rel_err": float(np.median(rels)), "max_grad": scale}
+            log(f"[bench] adjoint {kind}: {json.dumps(out[kind])}")
+    return out
+
+
 def main(argv=None):
     ap = argparse.ArgumentParser(description=__doc__)
     ap.add_argument("--gpus", type=int, default=1)
@@ -187,6 +264,10 @@ def main(argv=None):
     ap.add_argument("--cpu-seconds", type=float, default=15.0)
     ap.add_argument("--cpu-threads", type=int, default=16)
     ap.add_argument("--plain-steps", type=int, default=2)
+    ap.add_argument("--adjoint", action="store_true",
+                    help="run the differentiation check (50k-tet beam) instead of the throughput bench")
+    ap.add_argument("--adjoint-steps", type=int, default=60)
+    ap.add_argument("--adjoint-elements", type=int, default=32)
     ap.add_argument("--copy-probe", action="store_true",
                     help="end with one 256 MiB device-to-device copy (calibrates PMC byte counters)")
     args = ap.parse_args(argv)
@@ -194,6 +275,14 @@ def main(argv=None):
         print(f"bench.py measures one GPU; --gpus {args.gpus} is not supported "
               "(the multi-GPU sharding is described, not built)", file=sys.stderr)
         return 2
+
+    if args.adjoint:
+        res = adjoint_check(args.adjoint_steps, 10, args.adjoint_elements)
+        print(json.dumps({"metric": "adjoint gradient vs central finite differences (max |g - fd| / max |g|)",
+                          "value": max(res[k]["max_abs_err_over_max_grad"] for k in ("E", "nu", "rest")),
+                          "unit": "relative", "n_gpus": 1, "higher_is_better": False, "dtype": "f64",
+                          "data": "synthetic", "config": res}), flush=True)
+        return 0
 
     from talus_amd import api
 

@@ -42,6 +42,13 @@ extern "C" {
  * acceleration in every body (at most 63): each substep lands on the root of
  * its residual, as the differentiable rollout requires */
 #define TALUS_MI_CONVERGE 4u
+/* launch every PCG kernel directly instead of replaying the captured HIP
+ * graph of one iteration (for comparison; results are identical) */
+#define TALUS_MI_DIRECT_LAUNCH 8u
+
+/* elastic_model (ElasticModel, src/talus/material.h:8-11) */
+#define TALUS_MI_COROTATED 0u
+#define TALUS_MI_NEO_HOOKEAN 1u
 
 typedef struct talus_mi_ctx talus_mi_ctx;
 
@@ -226,9 +233,9 @@ int talus_mi_apply_system(talus_mi_ctx *ctx, uint32_t body, double inertia,
  * HBM bytes (what the algorithm must move: element arrays once per tet, node
  * arrays once per node, factor entries once).  Kinds: 0 element assembly,
  * 1 operator element pass, 2 operator node gather (one launch over every
- * body each); 3 / 4 forward / backward solve, one-lane-per-row shape, and
- * 5 / 6 the rows-per-wavefront shape (one launch per elimination level that
- * uses the shape under the current plan).  avg_ms and bytes are per launch;
+ * body each); 3 / 4 forward / backward solve, one-lane-per-row shape,
+ * 5 / 6 the rows-per-wavefront shape, 7 / 8 the split shape (one launch per
+ * elimination level that uses the shape under the current plan).  avg_ms and bytes are per launch;
  * launches = launches of the kernel per timed unit (per preconditioner
  * application for kinds 3-6, else 1). */
 typedef struct talus_mi_kernel_timing {
@@ -288,8 +295,9 @@ int talus_mi_rollout_backward(talus_mi_ctx *ctx, uint32_t body, const double *x_
 /* Autotuner of the preconditioner solve (no counterpart in the reference,
  * whose factor solve is Eigen's).  The solve plan holds, per elimination
  * level, the kernel shape of the forward and of the backward pass (2 bytes
- * per level: 0 = rows per wavefront, 1 = one row/column per lane for the
- * fronts with few pivots).  The factor's analysis sets a default plan from
+ * per level: 0 = rows per wavefront; for the fronts with few pivots 1 = one
+ * row/column per lane, 2 = 64 rows/columns per workgroup with the other
+ * dimension split over its wavefronts).  The factor's analysis sets a default plan from
  * the front shapes; talus_mi_tune times both shapes of every level on the
  * device and keeps the faster.  The plan changes the summation order of the
  * solve: results are bit-identical for a fixed plan, so a caller that needs

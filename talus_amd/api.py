@@ -26,6 +26,7 @@ FACTOR = 1
 DETECT_TEARS = 1
 RECORD_EVENTS = 2
 CONVERGE = 4
+DIRECT_LAUNCH = 8
 EVENT_YIELD = 0
 EVENT_TEAR = 1
 COROTATED = 0
@@ -490,7 +491,7 @@ class Context:
     def time_kernel(self, kind, repeats=20) -> dict:
         """Per-launch HIP-event time and algorithmic bytes of one kernel
         (kinds: 0 assembly, 1 operator element, 2 operator gather, 3/4 solve
-        forward/backward lane shape, 5/6 rows shape)."""
+        forward/backward lane shape, 5/6 rows shape, 7/8 split shape)."""
         t = KernelTiming()
         _check(lib().talus_mi_time_kernel(self._h, kind, repeats, C.byref(t)))
         return {"avg_ms": t.avg_ms, "bytes": t.bytes, "launches": t.launches}

@@ -189,8 +189,9 @@ class Engine final : public EngineBase {
     if (!sym_ready_) analyze_factor();
     if (plan.size() != plan_.size()) return false;
     for (uint8_t p : plan)
-      if (p > 1) return false;
+      if (p > 2) return false;
     plan_ = plan;
+    ++graph_epoch_;
     return true;
   }
 
@@ -217,6 +218,13 @@ class Engine final : public EngineBase {
   }
   void release() {
     if (stream_) (void)hipStreamSynchronize(stream_);
+    for (auto &a : graph_)
+      for (auto &b : a)
+        for (Graph &g : b) {
+          if (g.exec) (void)hipGraphExecDestroy(g.exec);
+          g = Graph{};
+        }
+    ++graph_epoch_;
     for (void *p : owned_) (void)hipFree(p);
     for (void *p : pinned_) (void)hipHostFree(p);
     owned_.clear();
@@ -228,7 +236,8 @@ class Engine final : public EngineBase {
   void all_active();
   // block Cholesky preconditioner
   void analyze_factor();
-  void factorize(const std::vector<double> &reach, const std::vector<double> &inertia);
+  void factorize(const std::vector<double> &reach, const std::vector<double> &inertia,
+                 const std::vector<int> *mask = nullptr);
   void precondition(int parity);
   void forward_level(uint32_t l, int mode, int parity);
   void backward_level(uint32_t l, int mode, int parity);
@@ -260,10 +269,23 @@ class Engine final : public EngineBase {
     std::vector<std::vector<StepParams>> params;
   };
   Recorder *recorder_ = nullptr;
+  // captured PCG iterations, [parity][factor][tangent]; rebuilt when the
+  // engine, the factor's analysis or the solve plan change (graph_epoch_)
+  struct Graph {
+    hipGraphExec_t exec = nullptr;
+    uint64_t epoch = ~0ull;
+  };
+  Graph graph_[2][2][2];
+  uint64_t graph_epoch_ = 0;
+  bool graphs_ = true;
+
+ public:
+  void use_graphs(bool on) { graphs_ = on; }
+
+ private:
   // factor state
   const std::vector<Body> *bodies_ = nullptr;
   bool sym_ready_ = false, fac_ready_ = false, fac_ok_ = false, use_factor_ = false;
-  bool refresh_ = false;
   std::vector<double> fac_reach_, fac_inertia_;
   FactorTables tab_;
   std::vector<uint8_t> plan_;  // per level: forward, backward kernel shape
@@ -275,10 +297,19 @@ class Engine final : public EngineBase {
   uint64_t *d_ea_ = nullptr, *d_trsm_ = nullptr, *d_syrk_ = nullptr, *d_trtri_ = nullptr,
            *d_w21_ = nullptr, *d_pack_ = nullptr, *d_fwd_ = nullptr, *d_bwd_ = nullptr,
            *d_fwl_ = nullptr, *d_bwl_ = nullptr, *d_fwr_ = nullptr, *d_bwr_ = nullptr,
+           *d_fws_ = nullptr, *d_bws_ = nullptr, *d_zero_ = nullptr,
            *d_gat_ = nullptr, *d_xg_ = nullptr;
   Real *d_gr_ = nullptr, *d_gc_ = nullptr;
   double *d_body_reach_ = nullptr, *d_body_inertia_ = nullptr;
   int *hfailed_ = nullptr;
+  int *d_refresh_ = nullptr;
+  // early refresh of single bodies' factors: a body whose solves needed at
+  // least REFRESH_ITERS iterations in a step is refactored at the next step
+  // (at most every REFRESH_COOLDOWN steps); a capped solve refreshes its body
+  static constexpr uint32_t REFRESH_ITERS = 5, REFRESH_COOLDOWN = 4;
+  std::vector<int> want_refresh_;
+  std::vector<uint64_t> next_refresh_;
+  uint64_t steps_taken_ = 0;
   uint32_t refactors_ = 0;
   double factor_seconds_ = 0.0;
 
@@ -293,6 +324,8 @@ void Engine<Real>::build(const std::vector<Body> &bodies) {
   bodies_ = &bodies;
   sym_ready_ = fac_ready_ = fac_ok_ = false;
   hnorms_ = d_norms_ = nullptr;
+  want_refresh_.assign(bodies.size(), 0);
+  next_refresh_.assign(bodies.size(), 0);
   fd_ = FactorDev{};
   const uint32_t B = uint32_t(bodies.size());
   uint64_t N = 0, T = 0;
@@ -511,6 +544,9 @@ void Engine<Real>::analyze_factor() {
   fd_.y = dev<double>(t.y_pool);
   fd_.xsol = dev<double>(3 * size_t(d_.N));
   fd_.failed = dev<int>(d_.B);
+  d_refresh_ = dev<int>(d_.B);
+  fd_.refresh = d_refresh_;
+  d_zero_ = upload(t.zero_tasks);
   d_parent_ = upload(t.parent);
   d_pair_target_ = upload(t.pair_target);
   d_pair_ld_ = upload(t.pair_ld);
@@ -532,7 +568,10 @@ void Engine<Real>::analyze_factor() {
   d_bwl_ = upload(t.bwl_tasks);
   d_fwr_ = upload(t.fwr_tasks);
   d_bwr_ = upload(t.bwr_tasks);
+  d_fws_ = upload(t.fws_tasks);
+  d_bws_ = upload(t.bws_tasks);
   plan_ = t.plan;
+  ++graph_epoch_;
   d_gat_ = upload(t.gat_tasks);
   d_xg_ = upload(t.xg_tasks);
   d_gr_ = dev<Real>(t.g_pool);
@@ -545,8 +584,11 @@ void Engine<Real>::analyze_factor() {
 // Numeric factorization (refreshFactor, factor.cc:567-613) of
 // inertia M + reach sum R K R^T with the committed rotations.
 template <class Real>
-void Engine<Real>::factorize(const std::vector<double> &reach, const std::vector<double> &inertia) {
+void Engine<Real>::factorize(const std::vector<double> &reach, const std::vector<double> &inertia,
+                             const std::vector<int> *mask) {
   if (!sym_ready_) analyze_factor();
+  const std::vector<int> every(d_.B, 1);
+  const std::vector<int> &fresh = mask ? *mask : every;
   hipStream_t s = stream_;
   hipEvent_t a, b;
   HIPCHECK(hipEventCreate(&a));
@@ -555,8 +597,14 @@ void Engine<Real>::factorize(const std::vector<double> &reach, const std::vector
   const FactorTables &t = tab_;
   HIPCHECK(hipMemcpyAsync(d_body_reach_, reach.data(), d_.B * sizeof(double), hipMemcpyHostToDevice, s));
   HIPCHECK(hipMemcpyAsync(d_body_inertia_, inertia.data(), d_.B * sizeof(double), hipMemcpyHostToDevice, s));
-  HIPCHECK(hipMemsetAsync(fd_.pool, 0, std::max<uint64_t>(t.pool, 1) * sizeof(double), s));
-  HIPCHECK(hipMemsetAsync(fd_.minv, 0, std::max<uint64_t>(t.minv_pool, 1) * sizeof(double), s));
+  HIPCHECK(hipMemcpyAsync(d_refresh_, fresh.data(), d_.B * sizeof(int), hipMemcpyHostToDevice, s));
+  if (mask) {
+    // partial refresh: only the flagged bodies' fronts are rebuilt
+    factor_zero(fd_, d_zero_, uint32_t(t.zero_tasks.size()), s);
+  } else {
+    HIPCHECK(hipMemsetAsync(fd_.pool, 0, std::max<uint64_t>(t.pool, 1) * sizeof(double), s));
+    HIPCHECK(hipMemsetAsync(fd_.minv, 0, std::max<uint64_t>(t.minv_pool, 1) * sizeof(double), s));
+  }
   HIPCHECK(hipMemsetAsync(fd_.failed, 0, d_.B * sizeof(int), s));
   FactorLaunch<Real>::assemble(fd_, d_pair_target_, d_pair_ld_, d_pair_node_, d_pair_start_,
                                d_contrib_, uint32_t(t.pair_target.size()), d_.dminv, d_.vol64,
@@ -593,45 +641,48 @@ void Engine<Real>::factorize(const std::vector<double> &reach, const std::vector
   ++refactors_;
 }
 
-// z = (LL^T)^-1 r for the bodies still iterating (applyFactor,
-// factor.cc:648-704, without the body-frame rotation): per level from the
-// leaves, gather the right-hand side and the children's updates, then one
-// GEMV with G; per level from the root, gather the ancestors' solution and
-// one transposed GEMV with G.
+// One level's GEMVs with G in the plan's shape (0 rows per wavefront, 1 one
+// row/column per lane, 2 64 rows/columns per workgroup split over its
+// wavefronts; the fronts too large for 1 and 2 always use 0).
 template <class Real>
 void Engine<Real>::forward_level(uint32_t l, int mode, int parity) {
   const FactorTables &t = tab_;
-  if (mode) {
+  if (mode == 1)
     SolveLaunch<Real>::forward_lanes(fd_, d_gc_, d_fwl_ + t.fwl_start[l], t.fwl_start[l + 1] - t.fwl_start[l],
                                      d_.done, parity, d_.B, stream_);
+  else if (mode == 2)
+    SolveLaunch<Real>::forward_split(fd_, d_gc_, d_fws_ + t.fws_start[l], t.fws_start[l + 1] - t.fws_start[l],
+                                     d_.done, parity, d_.B, stream_);
+  if (mode)
     SolveLaunch<Real>::forward(fd_, d_gr_, d_fwr_ + t.fwr_start[l], t.fwr_start[l + 1] - t.fwr_start[l],
                                d_.done, parity, d_.B, stream_);
-  } else {
+  else
     SolveLaunch<Real>::forward(fd_, d_gr_, d_fwd_ + t.fwd_start[l], t.fwd_start[l + 1] - t.fwd_start[l],
                                d_.done, parity, d_.B, stream_);
-  }
 }
 
 template <class Real>
 void Engine<Real>::backward_level(uint32_t l, int mode, int parity) {
   const FactorTables &t = tab_;
-  if (mode) {
+  if (mode == 1)
     SolveLaunch<Real>::backward_lanes(fd_, d_gr_, d_bwl_ + t.bwl_start[l], t.bwl_start[l + 1] - t.bwl_start[l],
                                       d_.done, parity, d_.B, stream_);
+  else if (mode == 2)
+    SolveLaunch<Real>::backward_split(fd_, d_gr_, d_bws_ + t.bws_start[l], t.bws_start[l + 1] - t.bws_start[l],
+                                      d_.done, parity, d_.B, stream_);
+  if (mode)
     SolveLaunch<Real>::backward(fd_, d_gc_, d_bwr_ + t.bwr_start[l], t.bwr_start[l + 1] - t.bwr_start[l],
                                 d_.done, parity, d_.B, stream_);
-  } else {
+  else
     SolveLaunch<Real>::backward(fd_, d_gc_, d_bwd_ + t.bwd_start[l], t.bwd_start[l + 1] - t.bwd_start[l],
                                 d_.done, parity, d_.B, stream_);
-  }
 }
 
 // z = (LL^T)^-1 r for the bodies still iterating (applyFactor,
 // factor.cc:648-704, without the body-frame rotation): per level from the
 // leaves, gather the right-hand side and the children's updates, then one
 // GEMV with G; per level from the root, gather the ancestors' solution and
-// one transposed GEMV with G.  The plan picks the kernel shape per level and
-// direction; a fixed plan gives bit-identical results.
+// one transposed GEMV with G.  A fixed plan gives bit-identical results.
 template <class Real>
 void Engine<Real>::precondition(int parity) {
   hipStream_t s = stream_;
@@ -685,23 +736,28 @@ void Engine<Real>::tune(uint32_t repeats, talus_mi_tune_report &rep) {
   rep.levels = t.levels;
   rep.before_seconds = time([&] { precondition(-1); });
   for (uint32_t l = 0; l < t.levels; ++l) {
-    if (t.fwl_start[l + 1] > t.fwl_start[l]) {
-      const double rows = time([&] { forward_level(l, 0, -1); });
-      const double lanes = time([&] { forward_level(l, 1, -1); });
-      plan_[2 * l] = lanes < rows ? 1 : 0;
-    } else {
-      plan_[2 * l] = 0;
+    for (int dir = 0; dir < 2; ++dir) {
+      const bool small = dir == 0 ? t.fwl_start[l + 1] > t.fwl_start[l] : t.bwl_start[l + 1] > t.bwl_start[l];
+      int best = 0;
+      if (small) {
+        double fastest = 0.0;
+        for (int mode = 0; mode < 3; ++mode) {
+          const double sec = time([&] {
+            if (dir == 0) forward_level(l, mode, -1);
+            else backward_level(l, mode, -1);
+          });
+          if (mode == 0 || sec < fastest) {
+            fastest = sec;
+            best = mode;
+          }
+        }
+      }
+      plan_[2 * l + dir] = uint8_t(best);
     }
-    if (t.bwl_start[l + 1] > t.bwl_start[l]) {
-      const double rows = time([&] { backward_level(l, 0, -1); });
-      const double lanes = time([&] { backward_level(l, 1, -1); });
-      plan_[2 * l + 1] = lanes < rows ? 1 : 0;
-    } else {
-      plan_[2 * l + 1] = 0;
-    }
-    rep.lanes_forward += plan_[2 * l];
-    rep.lanes_backward += plan_[2 * l + 1];
+    rep.lanes_forward += plan_[2 * l] ? 1u : 0u;
+    rep.lanes_backward += plan_[2 * l + 1] ? 1u : 0u;
   }
+  ++graph_epoch_;
   rep.after_seconds = time([&] { precondition(-1); });
   HIPCHECK(hipEventDestroy(a));
   HIPCHECK(hipEventDestroy(b));
@@ -769,6 +825,44 @@ void Engine<Real>::solve(bool first, uint32_t limit, uint32_t solve_index, bool 
     Launch<Real>::pcg_rz(d_, true, -1, s);
   }
   Launch<Real>::pcg_begin(d_, s);
+  // one PCG iteration: operator, gather, update, preconditioner, direction;
+  // replayed from a captured HIP graph per (parity, preconditioner, operator)
+  auto body = [&](int parity) {
+    op(d_.p, parity);
+    Launch<Real>::op_gather(d_, d_.p, d_.q, parity, s);
+    Launch<Real>::pcg_update(d_, parity, fac, s);
+    if (fac) {
+      precondition(parity);
+      Launch<Real>::pcg_rz(d_, false, parity, s);
+    }
+    Launch<Real>::pcg_direction(d_, parity, s);
+  };
+  auto iterate = [&](int parity) {
+    if (!graphs_) {
+      body(parity);
+      return;
+    }
+    Graph &g = graph_[parity][fac ? 1 : 0][tangent ? 1 : 0];
+    if (!g.exec || g.epoch != graph_epoch_) {
+      if (g.exec) (void)hipGraphExecDestroy(g.exec);
+      g.exec = nullptr;
+      hipGraph_t graph = nullptr;
+      HIPCHECK(hipStreamBeginCapture(s, hipStreamCaptureModeThreadLocal));
+      try {
+        body(parity);
+      } catch (...) {
+        (void)hipStreamEndCapture(s, &graph);
+        if (graph) (void)hipGraphDestroy(graph);
+        throw;
+      }
+      HIPCHECK(hipStreamEndCapture(s, &graph));
+      const hipError_t made = hipGraphInstantiate(&g.exec, graph, nullptr, nullptr, 0);
+      (void)hipGraphDestroy(graph);
+      HIPCHECK(made);
+      g.epoch = graph_epoch_;
+    }
+    HIPCHECK(hipGraphLaunch(g.exec, s));
+  };
   const uint32_t CHECK = fac ? 1u : 8u;
   for (uint32_t k = 0; k < limit; ++k) {
     const int parity = int(k & 1u);
@@ -780,15 +874,8 @@ void Engine<Real>::solve(bool first, uint32_t limit, uint32_t solve_index, bool 
       for (uint32_t b = 0; b < d_.B && all; ++b) all = hflags_[b] != 0;
       if (all) break;
     }
-    op(d_.p, parity);
-    Launch<Real>::op_gather(d_, d_.p, d_.q, parity, s);
-    Launch<Real>::pcg_update(d_, parity, fac, s);
-    if (fac) {
-      precondition(parity);
-      ++applications_;
-      Launch<Real>::pcg_rz(d_, false, parity, s);
-    }
-    Launch<Real>::pcg_direction(d_, parity, s);
+    iterate(parity);
+    if (fac) ++applications_;
     if ((k + 1) % CHECK == 0 && k + 1 < limit) {
       HIPCHECK(hipMemcpyAsync(hflags_, d_.done + size_t((k + 1) & 1u) * d_.B, d_.B * sizeof(int),
                               hipMemcpyDeviceToHost, s));
@@ -869,6 +956,7 @@ void Engine<Real>::step(const talus_mi_step_params &sp, talus_mi_report &rep,
     hparams_ = pinned<StepParams>(hparams_slots_ * B);
   }
   const bool converge = (sp.flags & TALUS_MI_CONVERGE) != 0;
+  graphs_ = (sp.flags & TALUS_MI_DIRECT_LAUNCH) == 0;
   const uint32_t reads = converge ? 64u : 3u;  // solves per pass at most
   if (reads * passes > hsolve_slots_) {
     HIPCHECK(hipStreamSynchronize(s));
@@ -895,10 +983,18 @@ void Engine<Real>::step(const talus_mi_step_params &sp, talus_mi_report &rep,
         reach[b] = q.reach;
         inertia[b] = q.inertia;
       }
-      if (!fac_ready_ || refresh_ || reach != fac_reach_ || inertia != fac_inertia_) {
+      if (!fac_ready_ || reach != fac_reach_ || inertia != fac_inertia_) {
         HIPCHECK(hipStreamSynchronize(s));
         factorize(reach, inertia);
-        refresh_ = false;
+        std::fill(want_refresh_.begin(), want_refresh_.end(), 0);
+      } else if (pass == 0 && std::any_of(want_refresh_.begin(), want_refresh_.end(), [](int w) { return w != 0; })) {
+        // bodies whose solves were slow or capped last step: rebuild their
+        // fronts only
+        HIPCHECK(hipStreamSynchronize(s));
+        factorize(reach, inertia, &want_refresh_);
+        for (uint32_t b = 0; b < B; ++b)
+          if (want_refresh_[b]) next_refresh_[b] = steps_taken_ + REFRESH_COOLDOWN;
+        std::fill(want_refresh_.begin(), want_refresh_.end(), 0);
       }
     }
     use_factor_ = factor && fac_ok_;
@@ -985,13 +1081,17 @@ void Engine<Real>::step(const talus_mi_step_params &sp, talus_mi_report &rep,
   rep.refactors = refactors_ - refactors_before;
   rep.preconditioner_applications = applications_;
   rep.device_seconds = ms * 1.0e-3;
+  std::vector<uint32_t> worst(B, 0);
+  std::vector<int> capped_body(B, 0);
   for (uint32_t si = 0; si < solve_index; ++si) {
     const uint32_t pass = solve_pass[si];
     const size_t slot = (si % hsolve_slots_) * B;
     for (uint32_t b = 0; b < B; ++b) {
       if (pass >= substeps[b]) continue;
       rep.cg_iterations += hiters_[slot + b];
+      worst[b] = std::max(worst[b], hiters_[slot + b]);
       const bool capped = hiters_[slot + b] >= limit && !(hrr_[slot + b] <= htarget_[slot + b]);
+      capped_body[b] |= capped ? 1 : 0;
       rep.cg_capped += capped ? 1u : 0u;
       const double reference2 = htarget_[slot + b] / (sp.tolerance * double(sp.tolerance));
       const double rel = reference2 > 0.0 ? std::sqrt(hrr_[slot + b] / reference2) : 0.0;
@@ -1012,7 +1112,12 @@ void Engine<Real>::step(const talus_mi_step_params &sp, talus_mi_report &rep,
     rep.yield_events += hyields_[b];
   }
   rep.plastic_growth = rep.yield_events ? 1u : 0u;
-  if (use_factor_ && rep.cg_capped) refresh_ = true;
+  if (use_factor_) {
+    for (uint32_t b = 0; b < B; ++b)
+      if (capped_body[b] || (worst[b] >= REFRESH_ITERS && steps_taken_ >= next_refresh_[b]))
+        want_refresh_[b] = 1;
+  }
+  ++steps_taken_;
   torn.assign(d_.T, 0);
   if (tears) torn = tflag;
   for (uint32_t b = 0; b < B; ++b) {
@@ -1167,33 +1272,38 @@ void Engine<Real>::time_kernel(uint32_t kind, uint32_t repeats, talus_mi_kernel_
       out.bytes = T * (tet + 9.0 * sz + 12.0 * sz) + snh * 9.0 * sz + N * 3.0 * sz;
     else
       out.bytes = T * (12.0 * sz + 16.0) + (N + 1.0) * 4.0 + N * (6.0 * sz + 8.0 + 1.0);
-  } else if (kind <= 6) {
+  } else if (kind <= 8) {
     if (!fac_ready_) {
       std::vector<double> reach(d_.B, 1.0), inert(d_.B, 1.0);
       factorize(reach, inert);
       fac_reach_.clear();  // the next step refactors with its own keys
     }
     const FactorTables &t = tab_;
-    const bool forward = kind == 3 || kind == 5, lanes = kind <= 4;
-    // the levels (and task lists) this kind runs on under the current plan
+    // kinds 3/4 lanes, 5/6 rows, 7/8 split; odd forward
+    const bool forward = (kind % 2) == 1;
+    const int shape = kind <= 4 ? 1 : (kind <= 6 ? 0 : 2);
+    auto list = [&](uint32_t l) -> std::pair<const std::vector<uint64_t> *, const std::vector<uint32_t> *> {
+      const int mode = plan_[2 * l + (forward ? 0 : 1)];
+      if (shape == 0) {
+        if (mode) return forward ? std::make_pair(&t.fwr_tasks, &t.fwr_start) : std::make_pair(&t.bwr_tasks, &t.bwr_start);
+        return forward ? std::make_pair(&t.fwd_tasks, &t.fwd_start) : std::make_pair(&t.bwd_tasks, &t.bwd_start);
+      }
+      if (mode != shape) return {nullptr, nullptr};
+      if (shape == 1) return forward ? std::make_pair(&t.fwl_tasks, &t.fwl_start) : std::make_pair(&t.bwl_tasks, &t.bwl_start);
+      return forward ? std::make_pair(&t.fws_tasks, &t.fws_start) : std::make_pair(&t.bws_tasks, &t.bws_start);
+    };
+    // the levels this kind runs on under the current plan
     std::vector<uint32_t> levels;
     for (uint32_t l = 0; l < t.levels; ++l) {
-      const int mode = plan_[2 * l + (forward ? 0 : 1)];
-      const auto &st = forward ? (lanes ? t.fwl_start : (mode ? t.fwr_start : t.fwd_start))
-                               : (lanes ? t.bwl_start : (mode ? t.bwr_start : t.bwd_start));
-      if ((lanes && !mode) || st[l + 1] == st[l]) continue;
-      levels.push_back(l);
+      const auto [tasks, st] = list(l);
+      if (tasks && (*st)[l + 1] > (*st)[l]) levels.push_back(l);
     }
     // algorithmic bytes: G entries on or below the diagonal once, the
     // front vector, the pivot outputs
     std::vector<uint8_t> in_kind(t.fronts, 0);
     for (uint32_t l : levels) {
-      const int mode = plan_[2 * l + (forward ? 0 : 1)];
-      const auto &tasks = forward ? (lanes ? t.fwl_tasks : (mode ? t.fwr_tasks : t.fwd_tasks))
-                                  : (lanes ? t.bwl_tasks : (mode ? t.bwr_tasks : t.bwd_tasks));
-      const auto &st = forward ? (lanes ? t.fwl_start : (mode ? t.fwr_start : t.fwd_start))
-                               : (lanes ? t.bwl_start : (mode ? t.bwr_start : t.bwd_start));
-      for (uint32_t q = st[l]; q < st[l + 1]; ++q) in_kind[uint32_t(tasks[q] & 0xffffffffu)] = 1;
+      const auto [tasks, st] = list(l);
+      for (uint32_t q = (*st)[l]; q < (*st)[l + 1]; ++q) in_kind[uint32_t((*tasks)[q] & 0xffffffffu)] = 1;
     }
     double bytes = 0.0;
     for (uint32_t f = 0; f < t.fronts; ++f) {
@@ -1204,19 +1314,24 @@ void Engine<Real>::time_kernel(uint32_t kind, uint32_t repeats, talus_mi_kernel_
     }
     all_active();
     auto run = [&](uint32_t l) {
-      const int mode = plan_[2 * l + (forward ? 0 : 1)];
-      if (forward && lanes)
-        SolveLaunch<Real>::forward_lanes(fd_, d_gc_, d_fwl_ + t.fwl_start[l], t.fwl_start[l + 1] - t.fwl_start[l], d_.done, -1, d_.B, s);
-      else if (forward)
-        SolveLaunch<Real>::forward(fd_, d_gr_, (mode ? d_fwr_ : d_fwd_) + (mode ? t.fwr_start[l] : t.fwd_start[l]),
-                                   mode ? t.fwr_start[l + 1] - t.fwr_start[l] : t.fwd_start[l + 1] - t.fwd_start[l],
-                                   d_.done, -1, d_.B, s);
-      else if (lanes)
-        SolveLaunch<Real>::backward_lanes(fd_, d_gr_, d_bwl_ + t.bwl_start[l], t.bwl_start[l + 1] - t.bwl_start[l], d_.done, -1, d_.B, s);
-      else
-        SolveLaunch<Real>::backward(fd_, d_gc_, (mode ? d_bwr_ : d_bwd_) + (mode ? t.bwr_start[l] : t.bwd_start[l]),
-                                    mode ? t.bwr_start[l + 1] - t.bwr_start[l] : t.bwd_start[l + 1] - t.bwd_start[l],
-                                    d_.done, -1, d_.B, s);
+      const auto [tasks, st] = list(l);
+      const uint64_t *dev = nullptr;
+      if (tasks == &t.fwl_tasks) dev = d_fwl_;
+      else if (tasks == &t.bwl_tasks) dev = d_bwl_;
+      else if (tasks == &t.fws_tasks) dev = d_fws_;
+      else if (tasks == &t.bws_tasks) dev = d_bws_;
+      else if (tasks == &t.fwr_tasks) dev = d_fwr_;
+      else if (tasks == &t.bwr_tasks) dev = d_bwr_;
+      else if (tasks == &t.fwd_tasks) dev = d_fwd_;
+      else dev = d_bwd_;
+      const uint32_t count = (*st)[l + 1] - (*st)[l];
+      dev += (*st)[l];
+      if (forward && shape == 1) SolveLaunch<Real>::forward_lanes(fd_, d_gc_, dev, count, d_.done, -1, d_.B, s);
+      else if (forward && shape == 2) SolveLaunch<Real>::forward_split(fd_, d_gc_, dev, count, d_.done, -1, d_.B, s);
+      else if (forward) SolveLaunch<Real>::forward(fd_, d_gr_, dev, count, d_.done, -1, d_.B, s);
+      else if (shape == 1) SolveLaunch<Real>::backward_lanes(fd_, d_gr_, dev, count, d_.done, -1, d_.B, s);
+      else if (shape == 2) SolveLaunch<Real>::backward_split(fd_, d_gr_, dev, count, d_.done, -1, d_.B, s);
+      else SolveLaunch<Real>::backward(fd_, d_gc_, dev, count, d_.done, -1, d_.B, s);
     };
     for (uint32_t l : levels) run(l);  // warm-up
     // one event pair per round of this kind's launches over its levels
@@ -1760,7 +1875,7 @@ int talus_mi_apply_system(talus_mi_ctx *ctx, uint32_t body, double inertia, doub
 
 int talus_mi_time_kernel(talus_mi_ctx *ctx, uint32_t kind, uint32_t repeats,
                          talus_mi_kernel_timing *out) {
-  if (!ctx || !out || kind > 6) return fail(TALUS_MI_ERR_ARGUMENT, "bad arguments");
+  if (!ctx || !out || kind > 8) return fail(TALUS_MI_ERR_ARGUMENT, "bad arguments");
   return guarded(ctx, [&]() -> int {
     if (ctx->bodies.empty()) return fail(TALUS_MI_ERR_STATE, "no bodies");
     ensure(ctx);

@@ -38,6 +38,7 @@ constexpr int FNB = 64;   // panel / tile width in DOF
 constexpr int SGRP = 4;   // solve workgroup: 4 wavefronts, 1 or 4 rows/columns each
 constexpr int SCH = 2048; // vector entries staged in LDS per chunk
 constexpr int SCHUNK = 256;  // DOF rows per gather workgroup
+constexpr uint32_t ZCHUNK = 65536;  // entries zeroed per workgroup before a factorization
 constexpr int KLANES = 512;   // fronts with k <= KLANES solve one row/column per lane
 constexpr int MLANES = 2048;  // ... backward also needs m <= MLANES (LDS)
 constexpr uint32_t LANES_MIN_GROUPS = 2048;  // default plan: lanes when a level has this many
@@ -107,8 +108,13 @@ struct FactorTables {
   std::vector<uint32_t> fwl_start, bwl_start;
   std::vector<uint64_t> fwr_tasks, bwr_tasks;                // the other fronts, rows per wave
   std::vector<uint32_t> fwr_start, bwr_start;
-  // default solve plan, 2 per level (forward, backward): 1 = lanes kernel for
-  // the small fronts of the level, 0 = rows-per-wavefront kernel throughout
+  std::vector<uint64_t> zero_tasks;  // front | chunk<<32 | array<<48, ZCHUNK entries per chunk
+  std::vector<uint64_t> fws_tasks, bws_tasks;                // small fronts, 64 rows/columns per workgroup
+  std::vector<uint32_t> fws_start, bws_start;
+  // default solve plan, 2 per level (forward, backward): 0 = rows-per-
+  // wavefront kernel throughout; for the small fronts of the level 1 = one
+  // row/column per lane, 256 per workgroup; 2 = 64 rows/columns per
+  // workgroup with the other dimension split over its 4 wavefronts
   std::vector<uint8_t> plan;
   std::vector<uint64_t> gat_tasks, xg_tasks;                 // front | chunk<<32, per level
   std::vector<uint32_t> gat_start, xg_start;
@@ -134,7 +140,8 @@ struct FactorDev {
   const uint64_t *src;
   const uint64_t *off, *minv_off, *vec_off, *yoff, *g_off;
   double *pool, *minv, *vec, *y, *xsol;
-  int *failed;  // per body
+  int *failed;         // per body
+  const int *refresh;  // per body: nonzero = this factorization rebuilds its fronts
 };
 
 // Launch helpers (definitions in factor.hip).  `conn/dminv/...` are the
@@ -153,6 +160,9 @@ struct FactorLaunch {
 
 void factor_extend_add(const FactorDev &f, const uint32_t *parent, const uint64_t *tasks,
                        uint32_t count, hipStream_t s);
+// zeroes the front storage (pool) and M11 blocks (minv) of the refreshed
+// bodies' fronts; tasks = front | chunk<<32 | array<<48 (0 pool, 1 minv)
+void factor_zero(const FactorDev &f, const uint64_t *tasks, uint32_t count, hipStream_t s);
 void factor_potrf(const FactorDev &f, const uint32_t *tasks, uint32_t count, uint32_t panel,
                   hipStream_t s);
 void factor_trsm(const FactorDev &f, const uint64_t *tasks, uint32_t count, uint32_t panel,
@@ -180,6 +190,12 @@ struct SolveLaunch {
   static void backward(const FactorDev &f, const Real *gc, const uint64_t *tasks, uint32_t count,
                        const int *done, int parity, uint32_t B, hipStream_t s);
   static void backward_lanes(const FactorDev &f, const Real *gr, const uint64_t *tasks,
+                             uint32_t count, const int *done, int parity, uint32_t B,
+                             hipStream_t s);
+  static void forward_split(const FactorDev &f, const Real *gc, const uint64_t *tasks,
+                            uint32_t count, const int *done, int parity, uint32_t B,
+                            hipStream_t s);
+  static void backward_split(const FactorDev &f, const Real *gr, const uint64_t *tasks,
                              uint32_t count, const int *done, int parity, uint32_t B,
                              hipStream_t s);
   static void finish(const FactorDev &f, Real *z, uint32_t N, const uint8_t *anchored,

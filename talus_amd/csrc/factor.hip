@@ -38,9 +38,9 @@ __global__ __launch_bounds__(NT) void k_assemble_pairs(
     const uint32_t *pair_start, const uint32_t *contrib, uint32_t pairs, const double *dminv,
     const double *vol, const double *lam, const double *mu, const Real *R, const Real *W,
     const uint8_t *flags, uint32_t T, const double *mass, const double *reach_of,
-    const double *inertia_of, const uint32_t *pair_body) {
+    const double *inertia_of, const uint32_t *pair_body, const int *refresh) {
   const uint32_t q = blockIdx.x * NT + threadIdx.x;
-  if (q >= pairs) return;
+  if (q >= pairs || !refresh[pair_body[q]]) return;
   double B[9] = {0, 0, 0, 0, 0, 0, 0, 0, 0};
   for (uint32_t c = pair_start[q]; c < pair_start[q + 1]; ++c) {
     const uint32_t code = contrib[c];
@@ -99,12 +99,29 @@ __global__ __launch_bounds__(NT) void k_assemble_pairs(
     for (int s = 0; s < 3; ++s) o[uint64_t(s) * ld + r] = B[r * 3 + s];
 }
 
+// true when this factorization rebuilds the front's body
+__device__ __forceinline__ bool stale(const FactorDev &f, uint32_t fr) {
+  return f.refresh[f.body[fr]] != 0;
+}
+
+__global__ __launch_bounds__(NT) void k_zero(FactorDev f, const uint64_t *tasks) {
+  const uint64_t t = tasks[blockIdx.x];
+  const uint32_t fr = lo32(t), chunk = mid16(t);
+  if (!stale(f, fr)) return;
+  const bool m11 = hi16(t) != 0;
+  const uint64_t n = m11 ? uint64_t(f.k[fr]) * f.k[fr] : uint64_t(f.m[fr]) * f.m[fr];
+  double *base = m11 ? f.minv + f.minv_off[fr] : f.pool + f.off[fr];
+  const uint64_t b0 = uint64_t(chunk) * ZCHUNK, b1 = min(n, b0 + ZCHUNK);
+  for (uint64_t i = b0 + threadIdx.x; i < b1; i += NT) base[i] = 0.0;
+}
+
 // --------------------------------------------------------------- extend-add
 
 __global__ __launch_bounds__(NT) void k_extend_add(FactorDev f, const uint32_t *parent,
                                                    const uint64_t *tasks) {
   const uint64_t t = tasks[blockIdx.x];
   const uint32_t c = lo32(t), ti = mid16(t), tj = hi16(t);
+  if (!stale(f, c)) return;
   const uint32_t p = parent[c];
   const uint32_t kc = f.k[c], mc = f.m[c], mp = f.m[p];
   const uint32_t u = mc - kc;
@@ -128,6 +145,7 @@ __global__ __launch_bounds__(64) void k_potrf(FactorDev f, const uint32_t *tasks
   __shared__ double a[FNB][LDP];
   __shared__ double x[FNB][LDP];
   const uint32_t fr = tasks[blockIdx.x];
+  if (!stale(f, fr)) return;
   const uint32_t k = f.k[fr], m = f.m[fr];
   const uint32_t P0 = panel * FNB;
   const int nb = int(min(uint32_t(FNB), k - P0));
@@ -227,6 +245,7 @@ __global__ __launch_bounds__(NT) void k_trsm(FactorDev f, const uint64_t *tasks,
   __shared__ double Xs[FNB][LDP];  // Xs[q][c] = X[c][q] = (L_pp^-1)[c][q]
   const uint64_t t = tasks[blockIdx.x];
   const uint32_t fr = lo32(t), rt = mid16(t);
+  if (!stale(f, fr)) return;
   const uint32_t k = f.k[fr], m = f.m[fr];
   const uint32_t P0 = panel * FNB;
   const int nb = int(min(uint32_t(FNB), k - P0));
@@ -260,6 +279,7 @@ __global__ __launch_bounds__(NT) void k_syrk(FactorDev f, const uint64_t *tasks,
   __shared__ double Js[FNB][LDP];
   const uint64_t t = tasks[blockIdx.x];
   const uint32_t fr = lo32(t), I = mid16(t), J = hi16(t);
+  if (!stale(f, fr)) return;
   const uint32_t k = f.k[fr], m = f.m[fr];
   const uint32_t P0 = panel * FNB;
   const int nb = int(min(uint32_t(FNB), k - P0));
@@ -292,6 +312,7 @@ __global__ __launch_bounds__(NT) void k_trtri(FactorDev f, const uint64_t *tasks
   __shared__ double Bs[FNB][LDP];  // Bs[q][c] = X[q-block row][j-block c]
   const uint64_t t = tasks[blockIdx.x];
   const uint32_t fr = lo32(t), i = mid16(t), j = hi16(t);
+  if (!stale(f, fr)) return;
   const uint32_t k = f.k[fr], m = f.m[fr];
   const double *L = f.pool + f.off[fr];
   double *X = f.minv + f.minv_off[fr];
@@ -368,6 +389,7 @@ __global__ __launch_bounds__(NT) void k_pack_w21(FactorDev f, const uint64_t *ta
   __shared__ double Bs[FNB][LDP];  // Bs[kk][c] = M11[q*64 + kk][J*64 + c]
   const uint64_t t = tasks[blockIdx.x];
   const uint32_t fr = lo32(t), I = mid16(t), J = hi16(t);
+  if (!stale(f, fr)) return;
   const uint32_t k = f.k[fr], m = f.m[fr], u = m - k;
   const double *L = f.pool + f.off[fr];
   const double *M = f.minv + f.minv_off[fr];
@@ -412,6 +434,7 @@ __global__ __launch_bounds__(NT) void k_pack_m11(FactorDev f, const uint64_t *ta
                                                  Real *gc) {
   const uint64_t t = tasks[blockIdx.x];
   const uint32_t fr = lo32(t), I = mid16(t), J = hi16(t);
+  if (!stale(f, fr)) return;
   const uint32_t k = f.k[fr], m = f.m[fr];
   const double *M = f.minv + f.minv_off[fr];
   Real *R = gr + f.g_off[fr];
@@ -560,6 +583,98 @@ __global__ __launch_bounds__(NT) void k_solve_backward_lanes(FactorDev f, const 
   f.xsol[3 * size_t(rows[j / 3]) + j % 3] = (s0 + s1) + (s2 + s3);
 }
 
+// forward, split shape (fronts with k <= KLANES): 64 rows of G per
+// workgroup, one per lane, the columns [0, min(k, first row + 64)) split in
+// four contiguous quarters over the workgroup's wavefronts (column-major copy,
+// 64 consecutive entries of a column per wavefront load), partial sums
+// added in LDS in wavefront order
+template <class Real>
+__global__ __launch_bounds__(NT) void k_solve_forward_split(FactorDev f, const Real *gc,
+                                                            const uint64_t *tasks, const int *done,
+                                                            int parity, uint32_t B) {
+  __shared__ double vs[KLANES];
+  __shared__ double red[NT / 64][64];
+  const uint64_t t = tasks[blockIdx.x];
+  const uint32_t fr = lo32(t), chunk = mid16(t);
+  if (skip(done, parity, B, f.body[fr])) return;
+  const uint32_t k = f.k[fr], m = f.m[fr];
+  double *v = f.vec + f.vec_off[fr];
+  const uint32_t first = chunk * 64u;
+  const uint32_t jend = min(k, first + 64u);
+  for (uint32_t q = threadIdx.x; q < jend; q += NT) vs[q] = v[q];
+  __syncthreads();
+  const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+  const uint32_t i = first + lane;
+  const uint32_t quarter = (jend + 3u) / 4u;
+  const uint32_t j0 = min(jend, wave * quarter), j1 = min(jend, j0 + quarter);
+  const Real *g = gc + f.g_off[fr] + min(i, m - 1);
+  const uint64_t ld = m;
+  double s0 = 0.0, s1 = 0.0, s2 = 0.0, s3 = 0.0;
+  uint32_t j = j0;
+  for (; j + 8 <= j1; j += 8) {
+    Real a[8];
+#pragma unroll
+    for (int q = 0; q < 8; ++q) a[q] = g[uint64_t(j + q) * ld];
+    s0 += double(a[0]) * vs[j] + double(a[4]) * vs[j + 4];
+    s1 += double(a[1]) * vs[j + 1] + double(a[5]) * vs[j + 5];
+    s2 += double(a[2]) * vs[j + 2] + double(a[6]) * vs[j + 6];
+    s3 += double(a[3]) * vs[j + 3] + double(a[7]) * vs[j + 7];
+  }
+  for (; j < j1; ++j) s0 += double(g[uint64_t(j) * ld]) * vs[j];
+  red[wave][lane] = (s0 + s1) + (s2 + s3);
+  __syncthreads();
+  if (wave != 0 || i >= m) return;
+  const double tot = ((red[0][lane] + red[1][lane]) + red[2][lane]) + red[3][lane];
+  if (i < k)
+    f.y[f.yoff[fr] + i] = tot;
+  else
+    v[i] -= tot;
+}
+
+// backward, split shape (k <= KLANES, m <= MLANES): 64 columns of G per
+// workgroup, one per lane, the rows [first column, m) split in four
+// contiguous quarters over the wavefronts (row-major copy), w = [y; -x2]
+// staged in LDS, partial sums added in LDS in wavefront order
+template <class Real>
+__global__ __launch_bounds__(NT) void k_solve_backward_split(FactorDev f, const Real *gr,
+                                                             const uint64_t *tasks, const int *done,
+                                                             int parity, uint32_t B) {
+  __shared__ double ws[MLANES];
+  __shared__ double red[NT / 64][64];
+  const uint64_t t = tasks[blockIdx.x];
+  const uint32_t fr = lo32(t), chunk = mid16(t);
+  if (skip(done, parity, B, f.body[fr])) return;
+  const uint32_t k = f.k[fr], m = f.m[fr];
+  const double *y = f.y + f.yoff[fr];
+  const double *v = f.vec + f.vec_off[fr];
+  const uint32_t first = chunk * 64u;
+  for (uint32_t q = first + threadIdx.x; q < m; q += NT) ws[q] = q < k ? y[q] : -v[q];
+  __syncthreads();
+  const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+  const uint32_t j = first + lane;
+  const uint32_t rows = m - first, quarter = (rows + 3u) / 4u;
+  const uint32_t i0 = first + min(rows, wave * quarter), i1 = first + min(rows, wave * quarter + quarter);
+  const Real *g = gr + f.g_off[fr] + min(j, k - 1);
+  const uint64_t ld = k;
+  double s0 = 0.0, s1 = 0.0, s2 = 0.0, s3 = 0.0;
+  uint32_t i = i0;
+  for (; i + 8 <= i1; i += 8) {
+    Real a[8];
+#pragma unroll
+    for (int q = 0; q < 8; ++q) a[q] = g[uint64_t(i + q) * ld];
+    s0 += double(a[0]) * ws[i] + double(a[4]) * ws[i + 4];
+    s1 += double(a[1]) * ws[i + 1] + double(a[5]) * ws[i + 5];
+    s2 += double(a[2]) * ws[i + 2] + double(a[6]) * ws[i + 6];
+    s3 += double(a[3]) * ws[i + 3] + double(a[7]) * ws[i + 7];
+  }
+  for (; i < i1; ++i) s0 += double(g[uint64_t(i) * ld]) * ws[i];
+  red[wave][lane] = (s0 + s1) + (s2 + s3);
+  __syncthreads();
+  if (wave != 0 || j >= k) return;
+  const uint32_t *rowsf = f.rows + f.rows_off[fr];
+  f.xsol[3 * size_t(rowsf[j / 3]) + j % 3] = ((red[0][lane] + red[1][lane]) + red[2][lane]) + red[3][lane];
+}
+
 // backward prologue: x2 (solution at the front's update rows) into v[k:m]
 __global__ __launch_bounds__(NT) void k_solve_xgather(FactorDev f, const uint64_t *tasks,
                                                       const int *done, int parity, uint32_t B) {
@@ -650,9 +765,12 @@ void FactorLaunch<Real>::assemble(const FactorDev &f, const uint64_t *pair_targe
   if (!pairs) return;
   hipLaunchKernelGGL(k_assemble_pairs<Real>, dim3((pairs + NT - 1) / NT), dim3(NT), 0, s, f.pool,
                      pair_target, pair_ld, pair_node, pair_start, contrib, pairs, dminv, vol, lam,
-                     mu, R, W, flags, T, mass, body_reach, body_inertia, pair_body);
+                     mu, R, W, flags, T, mass, body_reach, body_inertia, pair_body, f.refresh);
 }
 
+void factor_zero(const FactorDev &f, const uint64_t *tasks, uint32_t count, hipStream_t s) {
+  if (count) hipLaunchKernelGGL(k_zero, dim3(count), dim3(NT), 0, s, f, tasks);
+}
 void factor_extend_add(const FactorDev &f, const uint32_t *parent, const uint64_t *tasks,
                        uint32_t count, hipStream_t s) {
   if (count) hipLaunchKernelGGL(k_extend_add, dim3(count), dim3(NT), 0, s, f, parent, tasks);
@@ -703,6 +821,18 @@ void SolveLaunch<Real>::backward_lanes(const FactorDev &f, const Real *gr, const
                                        uint32_t count, const int *done, int parity, uint32_t B,
                                        hipStream_t s) {
   if (count) hipLaunchKernelGGL(k_solve_backward_lanes<Real>, dim3(count), dim3(NT), 0, s, f, gr, tasks, done, parity, B);
+}
+template <class Real>
+void SolveLaunch<Real>::forward_split(const FactorDev &f, const Real *gc, const uint64_t *tasks,
+                                      uint32_t count, const int *done, int parity, uint32_t B,
+                                      hipStream_t s) {
+  if (count) hipLaunchKernelGGL(k_solve_forward_split<Real>, dim3(count), dim3(NT), 0, s, f, gc, tasks, done, parity, B);
+}
+template <class Real>
+void SolveLaunch<Real>::backward_split(const FactorDev &f, const Real *gr, const uint64_t *tasks,
+                                       uint32_t count, const int *done, int parity, uint32_t B,
+                                       hipStream_t s) {
+  if (count) hipLaunchKernelGGL(k_solve_backward_split<Real>, dim3(count), dim3(NT), 0, s, f, gr, tasks, done, parity, B);
 }
 template <class Real>
 void SolveLaunch<Real>::xgather(const FactorDev &f, const uint64_t *tasks, uint32_t count,

@@ -389,6 +389,12 @@ void build_tables(const std::vector<Symbolic> &sym, const std::vector<Body> &bod
       o.src_start.push_back(uint32_t(o.src.size()));
     }
   }
+  // zeroing before a (partial) factorization: front storage m*m, M11 k*k
+  for (uint32_t f = 0; f < o.fronts; ++f) {
+    const uint64_t mm = uint64_t(o.m[f]) * o.m[f], kk = uint64_t(o.k[f]) * o.k[f];
+    for (uint64_t c = 0; c * ZCHUNK < mm; ++c) o.zero_tasks.push_back(uint64_t(f) | (c << 32));
+    for (uint64_t c = 0; c * ZCHUNK < kk; ++c) o.zero_tasks.push_back(uint64_t(f) | (c << 32) | (1ull << 48));
+  }
   // solve: per level, gather chunks, forward row groups, x2 chunks, backward
   // column groups
   o.fwd_start.assign(1, 0);
@@ -397,6 +403,8 @@ void build_tables(const std::vector<Symbolic> &sym, const std::vector<Body> &bod
   o.bwl_start.assign(1, 0);
   o.fwr_start.assign(1, 0);
   o.bwr_start.assign(1, 0);
+  o.fws_start.assign(1, 0);
+  o.bws_start.assign(1, 0);
   o.gat_start.assign(1, 0);
   o.xg_start.assign(1, 0);
   o.plan.assign(2 * size_t(L), 0);
@@ -417,10 +425,14 @@ void build_tables(const std::vector<Symbolic> &sym, const std::vector<Body> &bod
         o.bwd_tasks.push_back(task);
         if (!bl) o.bwr_tasks.push_back(task);
       }
-      if (fl)
+      if (fl) {
         for (uint32_t g = 0; g < tiles(o.m[f], 256u); ++g) o.fwl_tasks.push_back(uint64_t(f) | (uint64_t(g) << 32));
-      if (bl)
+        for (uint32_t g = 0; g < tiles(o.m[f], 64u); ++g) o.fws_tasks.push_back(uint64_t(f) | (uint64_t(g) << 32));
+      }
+      if (bl) {
         for (uint32_t g = 0; g < tiles(o.k[f], 256u); ++g) o.bwl_tasks.push_back(uint64_t(f) | (uint64_t(g) << 32));
+        for (uint32_t g = 0; g < tiles(o.k[f], 64u); ++g) o.bws_tasks.push_back(uint64_t(f) | (uint64_t(g) << 32));
+      }
       for (uint32_t g = 0; g < tiles(o.m[f], SCHUNK); ++g) o.gat_tasks.push_back(uint64_t(f) | (uint64_t(g) << 32));
       for (uint32_t g = 0; g < tiles(o.m[f] - o.k[f], SCHUNK); ++g)
         o.xg_tasks.push_back(uint64_t(f) | (uint64_t(g) << 32));
@@ -431,6 +443,8 @@ void build_tables(const std::vector<Symbolic> &sym, const std::vector<Body> &bod
     o.bwl_start.push_back(uint32_t(o.bwl_tasks.size()));
     o.fwr_start.push_back(uint32_t(o.fwr_tasks.size()));
     o.bwr_start.push_back(uint32_t(o.bwr_tasks.size()));
+    o.fws_start.push_back(uint32_t(o.fws_tasks.size()));
+    o.bws_start.push_back(uint32_t(o.bws_tasks.size()));
     o.gat_start.push_back(uint32_t(o.gat_tasks.size()));
     o.xg_start.push_back(uint32_t(o.xg_tasks.size()));
     o.plan[2 * l] = o.fwl_start[l + 1] - o.fwl_start[l] >= LANES_MIN_GROUPS ? 1 : 0;

@@ -70,10 +70,11 @@ def test_factor_solve_jittered_cube_with_many_fronts():
     assert info["fronts"] > 8 and info["levels"] >= 3
 
 
-@pytest.mark.parametrize("plan", [0, 1])
+@pytest.mark.parametrize("plan", [0, 1, 2])
 def test_factor_solve_either_kernel_shape(plan):
-    """Both solve kernel shapes (rows per wavefront, one row/column per lane)
-    give the dense solution on every level."""
+    """Every solve kernel shape (rows per wavefront, one row/column per lane,
+    64 rows/columns per workgroup split over its wavefronts) gives the dense
+    solution on every level."""
     body = scenes.cube(9, seed=6).bodies[0]
     _check_solve(body, plan=plan)
 
@@ -90,7 +91,7 @@ def test_tuned_plan_is_valid_and_reproducible():
                 rep = ctx.tune(2)
                 assert rep["levels"] > 0 and rep["after_seconds"] > 0.0
                 plan = ctx.plan()
-                assert plan.size == 2 * rep["levels"] and set(np.unique(plan)) <= {0, 1}
+                assert plan.size == 2 * rep["levels"] and set(np.unique(plan)) <= {0, 1, 2}
             else:
                 ctx.set_plan(plan)
                 assert np.array_equal(ctx.plan(), plan)
@@ -167,3 +168,46 @@ def test_factor_steps_are_bit_identical():
                 ctx.step(preconditioner=api.FACTOR)
             outs.append(ctx.state(b)[0])
     assert np.array_equal(outs[0], outs[1])
+
+
+@pytest.mark.parametrize("precision", [api.FP64, api.MIXED])
+def test_graph_replay_equals_direct_launch(precision):
+    """The captured HIP graph of a PCG iteration runs the same kernels as the
+    direct launches: trajectories are bit-identical, block Jacobi and factor,
+    across several graph replays with host work in between."""
+    body = scenes.cube(6, seed=12).bodies[0]
+    for pre in (api.BLOCK_JACOBI, api.FACTOR):
+        runs = []
+        for flags in (0, api.DIRECT_LAUNCH):
+            with api.Context(0, precision) as ctx:
+                b, _ = product_body(ctx, body)
+                for _ in range(4):
+                    r = ctx.step(preconditioner=pre, flags=flags)
+                    assert not r["non_finite"]
+                    ctx.state(b)  # host synchronisation between replays
+                runs.append(ctx.state(b))
+        for a, c in zip(*runs):
+            assert np.array_equal(a, c)
+
+
+def test_partial_refresh_keeps_batched_equal_to_single_bodies():
+    """Bodies whose solves slow down get their own fronts refactored (the
+    other bodies' fronts are skipped); a body's trajectory is the same
+    whether it steps alone or in the batch."""
+    sc = scenes.bodies(count=5, seed=21, lo=300, hi=3000)
+    for b in sc.bodies:
+        b.materials["youngs_modulus"] = 3e4  # soft: large rotations, drifting factors
+    p = api.default_params(substeps=2, preconditioner=api.FACTOR)
+    singles = []
+    for body in sc.bodies:
+        with api.Context(0, api.MIXED) as ctx:
+            i, _ = product_body(ctx, body)
+            for _ in range(12):
+                ctx.step(p)
+            singles.append(ctx.state(i)[0])
+    with api.Context(0, api.MIXED) as ctx:
+        ids = [product_body(ctx, body)[0] for body in sc.bodies]
+        refactors = sum(ctx.step(p)["refactors"] for _ in range(12))
+        for i, x in zip(ids, singles):
+            assert np.array_equal(ctx.state(i)[0], x)
+    assert refactors > 1  # the first factorization and at least one partial refresh
