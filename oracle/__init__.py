@@ -121,6 +121,7 @@ def lib():
         L.orc_eval_forces.argtypes = [C.c_void_p, dp, C.c_uint32, dp, dp, dp, dp]
         L.orc_apply_system.argtypes = [C.c_void_p, C.c_double, C.c_double, dp, dp, C.c_uint32]
         L.orc_tet_stretch.argtypes = [C.c_void_p, dp]
+        L.orc_stretch_tets.argtypes = [C.POINTER(C.c_uint32), C.c_size_t, dp, dp, dp]
         L.orc_yield_flags.argtypes = [C.c_void_p, C.POINTER(C.c_uint8)]
         u32p = C.POINTER(C.c_uint32)
         L.orc_tear_topology.argtypes = [C.c_uint32, C.c_uint32, u32p, dp, dp, C.POINTER(C.c_uint8),
@@ -458,6 +459,20 @@ def stretch(F):
     d = np.empty(3)
     s = lib().orc_stretch(_dp(F), _dp(d))
     return s, d
+
+
+def stretch_tets(tet, rest, x):
+    """Per-tet stretch (overstretched's measure) of connectivity `tet` at
+    positions `x`, with the rest inverse rebuilt from `rest`."""
+    tet = np.ascontiguousarray(tet, dtype=np.uint32).reshape(-1, 4)
+    rest = np.ascontiguousarray(rest, dtype=np.float64).reshape(-1, 3)
+    x = np.ascontiguousarray(x, dtype=np.float64).reshape(-1, 3)
+    if tet.size and int(tet.max()) >= min(rest.shape[0], x.shape[0]):
+        raise ValueError("node index out of range")
+    out = np.empty(tet.shape[0])
+    lib().orc_stretch_tets(tet.ctypes.data_as(C.POINTER(C.c_uint32)), tet.shape[0], _dp(rest), _dp(x),
+                           _dp(out))
+    return out
 
 
 def eigen_sym3(A):

@@ -1455,6 +1455,29 @@ void orc_tet_stretch(orc_body *b, double *stretch) {
   }
 }
 
+/* overstretched (tearing.cc:469-483) on plain arrays: the stretch of every
+ * tet of `tet` (4 node indices each, positive orientation at rest) at the
+ * positions `x`, with the rest inverse rebuilt from `rest` as the lattice
+ * build makes it */
+void orc_stretch_tets(const uint32_t *tet, size_t tets, const double *rest, const double *x,
+                      double *stretch) {
+  for (size_t e = 0; e < tets; ++e) {
+    const uint32_t *tn = &tet[4 * e];
+    double shape[9], dminv[9], u[12], F[9];
+    for (int c = 0; c < 3; ++c)
+      for (int r = 0; r < 3; ++r)
+        shape[r * 3 + c] = rest[3 * tn[c + 1] + r] - rest[3 * tn[0] + r];
+    inv3(shape, dminv);
+    for (int k = 0; k < 4; ++k)
+      for (int r = 0; r < 3; ++r) u[3 * k + r] = x[3 * tn[k] + r] - rest[3 * tn[k] + r];
+    displacement_gradient(u, dminv, F);
+    F[0] += 1.0;
+    F[4] += 1.0;
+    F[8] += 1.0;
+    stretch[e] = orc_stretch(F, NULL);
+  }
+}
+
 /* ------------------------------------------------------------------ */
 /* Tearing topology (tearing.cc:180-513 restated for tets)              */
 /* ------------------------------------------------------------------ */

@@ -13,6 +13,7 @@
 #ifndef TALUS_ORACLE_H
 #define TALUS_ORACLE_H
 
+#include <stddef.h>
 #include <stdint.h>
 
 #ifdef __cplusplus
@@ -140,6 +141,8 @@ void orc_apply_system(orc_body *body, double inertia, double step,
                       const double *d, double *y, uint32_t threads);
 /* Per-tet stretch sqrt(lambda_max(F^T F)) at the body's positions. */
 void orc_tet_stretch(orc_body *body, double *stretch);
+void orc_stretch_tets(const uint32_t *tet, size_t tets, const double *rest, const double *x,
+                      double *stretch);
 /* Yield flags of the last step (1 per tet). */
 void orc_yield_flags(const orc_body *body, uint8_t *flags);
 

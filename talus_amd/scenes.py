@@ -125,19 +125,41 @@ def cube(cells: int = 40, seed: int = 2024) -> Scene:
     return Scene(f"cube{cells}", [body], steps=240)
 
 
-def slab(cells=(82, 41, 5)) -> Scene:
+def slab(cells=(82, 41, 5), youngs_modulus=1e6, held_footprint=True) -> Scene:
     size = (2.0, 1.0, 0.1)
     voxel = tuple(size[k] / cells[k] for k in range(3))
-    mats = material_array(1, youngs_modulus=1e6, poissons_ratio=0.3, density=1000.0,
+    mats = material_array(1, youngs_modulus=youngs_modulus, poissons_ratio=0.3, density=1000.0,
                           elastic_model=api.NEO_HOOKEAN, yield_stress=2e5,
                           hardening_modulus=1e5, tear_strain=0.35)
+    # the box footprint on the top face is held kinematically: its nodes are
+    # anchored and driven down at 1 m/s (slab_drive)
+    driver = dict(center=(1.0, 0.5), half=(0.2, 0.2), top=size[2], speed=1.0)
     ends = [_box_support((0.0, 0.5, 0.05), (0.2 * voxel[0], 1.0, 1.0)),
             _box_support((2.0, 0.5, 0.05), (0.2 * voxel[0], 1.0, 1.0))]
+    if held_footprint:
+        ends.append(_box_support((driver["center"][0], driver["center"][1], size[2]),
+                                 (driver["half"][0] + 0.25 * voxel[0], driver["half"][1] + 0.25 * voxel[1],
+                                  0.2 * voxel[2])))
     body = Body(resolution=tuple(c + 1 for c in cells), voxel=voxel, supports=ends,
                 materials=mats)
-    # the box footprint on the top face, driven down at 1 m/s
-    driver = dict(center=(1.0, 0.5), half=(0.2, 0.2), top=size[2], speed=1.0)
     return Scene("slab", [body], steps=180, gravity=(0.0, 0.0, -9.81), extra={"driver": driver})
+
+
+def slab_driven_nodes(scene: Scene, rest: np.ndarray, anchored: np.ndarray) -> np.ndarray:
+    """Indices of the anchored top-face nodes under the crushing box."""
+    d = scene.extra["driver"]
+    v = scene.bodies[0].voxel
+    inside = ((np.abs(rest[:, 0] - d["center"][0]) <= d["half"][0] + 0.5 * v[0])
+              & (np.abs(rest[:, 1] - d["center"][1]) <= d["half"][1] + 0.5 * v[1])
+              & (np.abs(rest[:, 2] - d["top"]) <= 0.2 * v[2]))
+    held = np.asarray(anchored) != 0
+    return np.flatnonzero(inside & held) if held[inside].any() else np.flatnonzero(inside)
+
+
+def slab_drive(scene: Scene, rest_z: np.ndarray, step: int) -> np.ndarray:
+    """Heights of the driven nodes at the start of `step`: the box moves down
+    at the driver speed (m/s) from the top face."""
+    return rest_z - scene.extra["driver"]["speed"] * scene.dt * step
 
 
 def bodies(count: int = 512, seed: int = 7, lo: int = 200, hi: int = 20000, cell: float = 0.05) -> Scene:

@@ -166,3 +166,23 @@ def test_oracle_body_tear_matches_the_library_masses_and_keeps_state():
         back[i] = back[back[i]]
     np.testing.assert_array_equal(x2[n0:], x[back[n0:]])
     np.testing.assert_array_equal(v2[n0:], v[back[n0:]])
+
+
+def test_stretch_on_plain_arrays_equals_the_body_stretch_and_numpy():
+    """O.stretch_tets (the tear test on a mesh given as arrays) equals the
+    oracle body's own stretch test bit for bit, and the per-F stretch of a tet
+    whose F is known."""
+    sc = scenes.slab(cells=(6, 4, 2))
+    orc, m = oracle_body(sc.bodies[0])
+    rng = np.random.default_rng(3)
+    x = m["rest"] + 0.004 * rng.standard_normal(m["rest"].shape)
+    z = np.zeros_like(x)
+    orc.set_state(x, z, z)
+    t = orc.topology()
+    np.testing.assert_array_equal(O.stretch_tets(t["tet"], t["rest"], x), orc.stretch())
+    rest = np.array([[0.0, 0, 0], [0.1, 0, 0], [0, 0.1, 0], [0, 0, 0.1]])
+    F = np.array([[1.3, 0.2, 0.0], [0.0, 0.9, 0.1], [0.05, 0.0, 1.1]])
+    s = O.stretch_tets([[0, 1, 2, 3]], rest, rest @ F.T)
+    assert abs(s[0] - O.stretch(F)[0]) <= 1e-12
+    with pytest.raises(ValueError):
+        O.stretch_tets([[0, 1, 2, 9]], rest, rest)
