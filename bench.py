@@ -298,7 +298,6 @@ def main(argv=None):
                                                  args.warmup, not args.no_tune, args.plan)
     value = T * sub * args.steps / wall_m
     apps = sum(r["preconditioner_applications"] for r in reps_m) / args.steps
-    solves = sum(r["solves"] for r in reps_m) / args.steps
     log(f"[bench] mixed {1e3 * wall_m / args.steps:.3f} ms/step, {value:.4g} tet/s, "
         f"{apps:.1f} preconditioner applications/step, tuned {tuned}")
     finfo = ctx_m.factor_info()
@@ -328,8 +327,10 @@ def main(argv=None):
 
     # per-kernel launch durations (HIP events on the context stream) and the
     # launches of each kernel per step
+    # with the factor every solve applies the preconditioner once before its
+    # first iteration and once per iteration; a step makes 3 solves per pass
     passes = sub
-    iters = max(apps - solves, 0.0)
+    iters = max(apps - 3.0 * passes, 0.0)
     kernels = {}
     for name, kind in KINDS.items():
         t = ctx_m.time_kernel(kind, 20)

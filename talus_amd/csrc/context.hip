@@ -263,6 +263,9 @@ class Engine final : public EngineBase {
   bool any_plastic_ = false, any_tear_ = false;
   uint64_t n_snh_ = 0, n_plastic_ = 0;
   uint32_t applications_ = 0;  // preconditioner applications this step
+  // host copies of the incidence CSR: node ranges and the CSR position of
+  // every (tet, corner) slot code 4e+c
+  std::vector<uint32_t> inc_start_, slot_pos_;
   double *hnorms_ = nullptr, *d_norms_ = nullptr;  // corrections' convergence
   struct Recorder {
     std::vector<double *> buffers;  // per pass: x_n, v_n, a_n, a' (3N each)
@@ -381,7 +384,8 @@ void Engine<Real>::build(const std::vector<Body> &bodies) {
   // nodes
   std::vector<double> X(3 * N), mass(N), x(3 * N), v(3 * N), a(3 * N);
   std::vector<uint8_t> anchored(N);
-  std::vector<uint32_t> inc_start(N + 1, 0), inc_slot(4 * T);
+  std::vector<uint32_t> inc_start(N + 1, 0);
+  slot_pos_.assign(4 * T, 0);
   std::vector<uint4> conn(T);
   std::vector<double> dminv(9 * T), vol64(T), lam64(T), mu64(T), rho(T), yield(T), hard(T), pmax(T),
       tear(T), plastic(10 * T);
@@ -430,15 +434,17 @@ void Engine<Real>::build(const std::vector<Body> &bodies) {
   }
   for (size_t n = 0; n < N; ++n) inc_start[n + 1] += inc_start[n];
   {
+    // slot code 4e+c -> its CSR position; codes ascend within each node
     std::vector<uint32_t> cursor(inc_start.begin(), inc_start.end() - 1);
     const uint32_t *cn = reinterpret_cast<const uint32_t *>(conn.data());
-    for (size_t s = 0; s < 4 * T; ++s) inc_slot[cursor[cn[s]]++] = uint32_t(s);
+    for (size_t s = 0; s < 4 * T; ++s) slot_pos_[s] = cursor[cn[s]]++;
   }
   d_.X = dev_copy(X);
   d_.mass = dev_copy(mass);
   d_.anchored = dev_copy(anchored);
   d_.inc_start = dev_copy(inc_start);
-  d_.inc_slot = dev_copy(inc_slot);
+  d_.slot_pos = reinterpret_cast<const uint4 *>(dev_copy(slot_pos_));
+  inc_start_.assign(inc_start.begin(), inc_start.end());
   d_.x = dev_copy(x);
   d_.v = dev_copy(v);
   d_.a = dev_copy(a);
@@ -1202,12 +1208,18 @@ void Engine<Real>::eval_forces(uint32_t b, const double *x, double *tf, double *
   Launch<Real>::displacement(d_, scratch64_, s);
   Launch<Real>::assemble(d_, false, true, s);
   Launch<Real>::gather_forces(d_, d_.q, s);
+  // the body's slots are the CSR run of its nodes, in node order
   Real *h = hvec_;
-  HIPCHECK(hipMemcpyAsync(h, d_.slot_f + 12 * t0, 12 * nt * sizeof(Real), hipMemcpyDeviceToHost, s));
+  const size_t k0 = inc_start_[n0];
+  HIPCHECK(hipMemcpyAsync(h, d_.slot_f + 3 * k0, 12 * nt * sizeof(Real), hipMemcpyDeviceToHost, s));
   HIPCHECK(hipMemcpyAsync(h + 12 * nt, d_.q + 3 * n0, 3 * nn * sizeof(Real), hipMemcpyDeviceToHost, s));
   HIPCHECK(hipStreamSynchronize(s));
   if (tf)
-    for (size_t i = 0; i < 12 * nt; ++i) tf[i] = double(h[i]);
+    for (size_t e = 0; e < nt; ++e)
+      for (int c = 0; c < 4; ++c) {
+        const size_t k = slot_pos_[4 * (t0 + e) + c] - k0;
+        for (int r = 0; r < 3; ++r) tf[12 * e + 3 * c + r] = double(h[3 * k + r]);
+      }
   if (nf)
     for (size_t i = 0; i < 3 * nn; ++i) nf[i] = double(h[12 * nt + i]);
   if (te) {
@@ -1262,16 +1274,16 @@ void Engine<Real>::time_kernel(uint32_t kind, uint32_t repeats, talus_mi_kernel_
     HIPCHECK(hipEventElapsedTime(&ms, a, b));
     out.avg_ms = double(ms) / reps;
     out.launches = 1;
-    // element data once per tet (connectivity 16, FP64 rest inverse 72,
-    // V/lambda/mu, flag byte), node vectors once per node
-    const double tet = 16.0 + 72.0 + 3.0 * sz + 1.0;
+    // element data once per tet (connectivity 16, slot positions 16, FP64
+    // rest inverse 72, V/lambda/mu, flag byte), node vectors once per node
+    const double tet = 16.0 + 16.0 + 72.0 + 3.0 * sz + 1.0;
     const double snh = double(n_snh_), plastic = double(n_plastic_);
     if (kind == 0)
       out.bytes = T * (tet + 9.0 * sz + 12.0 * sz) + snh * 9.0 * sz + plastic * (80.0 + 16.0) + N * 3.0 * sz;
     else if (kind == 1)
       out.bytes = T * (tet + 9.0 * sz + 12.0 * sz) + snh * 9.0 * sz + N * 3.0 * sz;
     else
-      out.bytes = T * (12.0 * sz + 16.0) + (N + 1.0) * 4.0 + N * (6.0 * sz + 8.0 + 1.0);
+      out.bytes = T * 12.0 * sz + (N + 1.0) * 4.0 + N * (6.0 * sz + 8.0 + 1.0);
   } else if (kind <= 8) {
     if (!fac_ready_) {
       std::vector<double> reach(d_.B, 1.0), inert(d_.B, 1.0);

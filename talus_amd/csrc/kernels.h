@@ -8,10 +8,13 @@
 //   Dm^-1             FP64, 9 SoA planes dminv[k*T + e] (coalesced)
 //   V, lambda, mu     Real per tet
 //   rotations R       Real, 9 SoA planes
-//   force slots       Real, AoS per (tet, corner): slot_f[(4e+c)*3 + r], so a
-//                     node's gather reads 12 contiguous bytes per incidence
-//   incidence CSR     node -> slot codes 4e+c in ascending order, summed in
-//                     that fixed order (deterministic assembly, no atomics)
+//   force slots       Real, 3 per (tet, corner), stored in node order: the
+//                     corner c of tet e writes slot_f[3*pos + r] with pos =
+//                     slot_pos[e].c, its position in the incidence CSR, so a
+//                     node's gather is one contiguous run
+//   incidence CSR     node n owns positions inc_start[n]..inc_start[n+1]-1,
+//                     one per incident (tet, corner) in ascending 4e+c order,
+//                     summed in that fixed order (deterministic, no atomics)
 // Work is cut into tiles of TILE items of one body (node tiles and tet
 // tiles); block b processes tile b, and per-body scalars (PCG alpha/beta,
 // convergence) are reduced from per-tile partial sums in a fixed order.
@@ -52,7 +55,8 @@ struct Dev {
   // nodes
   const double *X, *mass;
   const uint8_t *anchored;
-  const uint32_t *inc_start, *inc_slot;
+  const uint32_t *inc_start;
+  const uint4 *slot_pos;  // per tet: CSR positions of its 4 corner slots
   double *x, *v, *a, *carried, *shifted, *weighted, *right;
   double *u_eval;  // displacement at the evaluation point (FP64 in both paths)
   Real *P, *x0;

@@ -100,11 +100,18 @@ __device__ __forceinline__ void deformation64(const Dev<Real> &d, const double *
   F[8] += 1.0;
 }
 
+// corner c of a tet writes its 3 values at the CSR position pos.c of its
+// slot, so each node's incident slots are one contiguous run
 template <class Real>
-__device__ __forceinline__ void store_slots(Real *slot, uint32_t e, const Real *f) {
-  Real *o = slot + (size_t)e * 12;
+__device__ __forceinline__ void store_slots(Real *slot, uint4 pos, const Real *f) {
+  const uint32_t at[4] = {pos.x, pos.y, pos.z, pos.w};
 #pragma unroll
-  for (int i = 0; i < 12; ++i) o[i] = f[i];
+  for (int c = 0; c < 4; ++c) {
+    Real *o = slot + 3 * (size_t)at[c];
+    o[0] = f[3 * c + 0];
+    o[1] = f[3 * c + 1];
+    o[2] = f[3 * c + 2];
+  }
 }
 
 // ---------------------------------------------------------------- nodes
@@ -170,13 +177,12 @@ __global__ __launch_bounds__(TILE) void k_balance(Dev<Real> d) {
   Real D[6] = {Real(0), Real(0), Real(0), Real(0), Real(0), Real(0)};
   const uint32_t k1 = d.inc_start[n + 1];
   for (uint32_t k = d.inc_start[n]; k < k1; ++k) {
-    const size_t s = d.inc_slot[k];
-    f[0] += d.slot_f[3 * s + 0];
-    f[1] += d.slot_f[3 * s + 1];
-    f[2] += d.slot_f[3 * s + 2];
+    f[0] += d.slot_f[3 * (size_t)k + 0];
+    f[1] += d.slot_f[3 * (size_t)k + 1];
+    f[2] += d.slot_f[3 * (size_t)k + 2];
     if (DIAG) {
 #pragma unroll
-      for (int j = 0; j < 6; ++j) D[j] += d.slot_d[6 * s + j];
+      for (int j = 0; j < 6; ++j) D[j] += d.slot_d[6 * (size_t)k + j];
     }
   }
   const double m = d.mass[n];
@@ -227,10 +233,9 @@ __global__ __launch_bounds__(TILE) void k_pcg_init(Dev<Real> d) {
       if (WITH_X0) {
         const uint32_t k1 = d.inc_start[n + 1];
         for (uint32_t k = d.inc_start[n]; k < k1; ++k) {
-          const size_t s = d.inc_slot[k];
-          q[0] += d.slot_f[3 * s + 0];
-          q[1] += d.slot_f[3 * s + 1];
-          q[2] += d.slot_f[3 * s + 2];
+          q[0] += d.slot_f[3 * (size_t)k + 0];
+          q[1] += d.slot_f[3 * (size_t)k + 1];
+          q[2] += d.slot_f[3 * (size_t)k + 2];
         }
         const Real im = Real(P.inertia * d.mass[n]), reach = Real(P.reach);
 #pragma unroll
@@ -362,7 +367,7 @@ __global__ __launch_bounds__(TILE) void k_op_element(Dev<Real> d, const Real *di
   if (e >= d.tt_end[t]) return;
   Real y[12];
   op_tet(d, e, dir, y);
-  store_slots(d.slot_f, e, y);
+  store_slots(d.slot_f, d.slot_pos[e], y);
 }
 
 template <class Real>
@@ -371,7 +376,7 @@ __global__ __launch_bounds__(TILE) void k_op_element_flat(Dev<Real> d, const Rea
   if (e >= d.T) return;
   Real y[12];
   op_tet(d, e, dir, y);
-  store_slots(d.slot_f, e, y);
+  store_slots(d.slot_f, d.slot_pos[e], y);
 }
 
 // q = inertia M p + h^2 sum(slots), anchored rows zeroed; partial p.q
@@ -389,10 +394,9 @@ __global__ __launch_bounds__(TILE) void k_op_gather(Dev<Real> d, const Real *dir
     if (!d.anchored[n]) {
       const uint32_t k1 = d.inc_start[n + 1];
       for (uint32_t k = d.inc_start[n]; k < k1; ++k) {
-        const size_t s = d.inc_slot[k];
-        q[0] += d.slot_f[3 * s + 0];
-        q[1] += d.slot_f[3 * s + 1];
-        q[2] += d.slot_f[3 * s + 2];
+        q[0] += d.slot_f[3 * (size_t)k + 0];
+        q[1] += d.slot_f[3 * (size_t)k + 1];
+        q[2] += d.slot_f[3 * (size_t)k + 2];
       }
       const Real im = Real(P.inertia * d.mass[n]), reach = Real(P.reach);
 #pragma unroll
@@ -739,7 +743,7 @@ __global__ __launch_bounds__(TILE) void k_assemble(Dev<Real> d) {
         f[3 * i + 2] = -V * w[2];
       }
     }
-    store_slots(d.slot_f, e, f);
+    store_slots(d.slot_f, d.slot_pos[e], f);
 #pragma unroll
     for (int k = 0; k < 9; ++k) d.R[(size_t)k * d.T + e] = Rm[k];
     if (DIAG) {
@@ -751,19 +755,21 @@ __global__ __launch_bounds__(TILE) void k_assemble(Dev<Real> d) {
 #pragma unroll
         for (int k = 0; k < 9; ++k) Wm[k] = Real(0);
       }
-      Real *o = d.slot_d + (size_t)e * 24;
+      const uint4 pos = d.slot_pos[e];
+      const uint32_t at[4] = {pos.x, pos.y, pos.z, pos.w};
 #pragma unroll
       for (int i = 0; i < 4; ++i) {
+        Real *o = d.slot_d + 6 * (size_t)at[i];
         Real rg[3], wg[3];
         mv3(Rm, &g[3 * i], rg);
         mv3(Wm, &g[3 * i], wg);
         const Real gg = g[3 * i] * g[3 * i] + g[3 * i + 1] * g[3 * i + 1] + g[3 * i + 2] * g[3 * i + 2];
-        o[6 * i + 0] = lr * rg[0] * rg[0] + lw * wg[0] * wg[0] + vm * gg;
-        o[6 * i + 1] = lr * rg[0] * rg[1] + lw * wg[0] * wg[1];
-        o[6 * i + 2] = lr * rg[0] * rg[2] + lw * wg[0] * wg[2];
-        o[6 * i + 3] = lr * rg[1] * rg[1] + lw * wg[1] * wg[1] + vm * gg;
-        o[6 * i + 4] = lr * rg[1] * rg[2] + lw * wg[1] * wg[2];
-        o[6 * i + 5] = lr * rg[2] * rg[2] + lw * wg[2] * wg[2] + vm * gg;
+        o[0] = lr * rg[0] * rg[0] + lw * wg[0] * wg[0] + vm * gg;
+        o[1] = lr * rg[0] * rg[1] + lw * wg[0] * wg[1];
+        o[2] = lr * rg[0] * rg[2] + lw * wg[0] * wg[2];
+        o[3] = lr * rg[1] * rg[1] + lw * wg[1] * wg[1] + vm * gg;
+        o[4] = lr * rg[1] * rg[2] + lw * wg[1] * wg[2];
+        o[5] = lr * rg[2] * rg[2] + lw * wg[2] * wg[2] + vm * gg;
       }
     }
     if (ENERGY) {
@@ -882,10 +888,9 @@ __global__ __launch_bounds__(TILE) void k_gather_forces(Dev<Real> d, Real *out) 
   Real f[3] = {Real(0), Real(0), Real(0)};
   const uint32_t k1 = d.inc_start[n + 1];
   for (uint32_t k = d.inc_start[n]; k < k1; ++k) {
-    const size_t s = d.inc_slot[k];
-    f[0] += d.slot_f[3 * s + 0];
-    f[1] += d.slot_f[3 * s + 1];
-    f[2] += d.slot_f[3 * s + 2];
+    f[0] += d.slot_f[3 * (size_t)k + 0];
+    f[1] += d.slot_f[3 * (size_t)k + 1];
+    f[2] += d.slot_f[3 * (size_t)k + 2];
   }
 #pragma unroll
   for (int c = 0; c < 3; ++c) out[3 * (size_t)n + c] = f[c];
@@ -900,10 +905,9 @@ __global__ __launch_bounds__(TILE) void k_apply_plain(Dev<Real> d, const Real *d
   if (!d.anchored[n]) {
     const uint32_t k1 = d.inc_start[n + 1];
     for (uint32_t k = d.inc_start[n]; k < k1; ++k) {
-      const size_t s = d.inc_slot[k];
-      q[0] += d.slot_f[3 * s + 0];
-      q[1] += d.slot_f[3 * s + 1];
-      q[2] += d.slot_f[3 * s + 2];
+      q[0] += d.slot_f[3 * (size_t)k + 0];
+      q[1] += d.slot_f[3 * (size_t)k + 1];
+      q[2] += d.slot_f[3 * (size_t)k + 2];
     }
     const Real im = Real(inertia * d.mass[n]), rc = Real(reach);
 #pragma unroll
@@ -1025,7 +1029,7 @@ __global__ __launch_bounds__(TILE) void k_hvp(Dev<Real> d, const Real *dir, int 
 #pragma unroll
     for (int r = 0; r < 3; ++r) y[3 * i + r] = Real(V * tmp[r]);
   }
-  store_slots(d.slot_f, e, y);
+  store_slots(d.slot_f, d.slot_pos[e], y);
 }
 
 // x_w = x_n + (1-af)(h v_n + c a_n) + reach a' (as u_eval = x_w - X), the
@@ -1061,9 +1065,8 @@ __global__ __launch_bounds__(TILE) void k_adj_update(Dev<Real> d, AdjDev a) {
   double kl[3] = {0.0, 0.0, 0.0};
   const uint32_t k1 = d.inc_start[n + 1];
   for (uint32_t k = d.inc_start[n]; k < k1; ++k) {
-    const size_t s = d.inc_slot[k];
 #pragma unroll
-    for (int c = 0; c < 3; ++c) kl[c] += double(d.slot_f[3 * s + c]);
+    for (int c = 0; c < 3; ++c) kl[c] += double(d.slot_f[3 * (size_t)k + c]);
   }
   const bool fixed = d.anchored[n] != 0;
   const double m = d.mass[n];
@@ -1137,12 +1140,15 @@ __global__ __launch_bounds__(TILE) void k_adj_params(Dev<Real> d, AdjDev a) {
       A[r * 3 + q] = V * ((r == q ? PD + we : 0.0) - fq - dp);
     }
   mul3_nt(A, dm, G);
-  double *o = a.xslot + (size_t)e * 12;
+  const uint4 pos = d.slot_pos[e];
+  double *o0 = a.xslot + 3 * (size_t)pos.x, *o1 = a.xslot + 3 * (size_t)pos.y,
+         *o2 = a.xslot + 3 * (size_t)pos.z, *o3 = a.xslot + 3 * (size_t)pos.w;
 #pragma unroll
   for (int r = 0; r < 3; ++r) {
-    o[r] = G[r * 3 + 0] + G[r * 3 + 1] + G[r * 3 + 2];
-#pragma unroll
-    for (int k = 0; k < 3; ++k) o[3 * (k + 1) + r] = -G[r * 3 + k];
+    o0[r] = G[r * 3 + 0] + G[r * 3 + 1] + G[r * 3 + 2];
+    o1[r] = -G[r * 3 + 0];
+    o2[r] = -G[r * 3 + 1];
+    o3[r] = -G[r * 3 + 2];
   }
 }
 
@@ -1156,9 +1162,8 @@ __global__ __launch_bounds__(TILE) void k_adj_gather_rest(Dev<Real> d, AdjDev a)
   double s3[3] = {0.0, 0.0, 0.0};
   const uint32_t k1 = d.inc_start[n + 1];
   for (uint32_t k = d.inc_start[n]; k < k1; ++k) {
-    const size_t s = d.inc_slot[k];
 #pragma unroll
-    for (int c = 0; c < 3; ++c) s3[c] += a.xslot[3 * s + c];
+    for (int c = 0; c < 3; ++c) s3[c] += a.xslot[3 * (size_t)k + c];
   }
 #pragma unroll
   for (int c = 0; c < 3; ++c) a.X_bar[3 * (size_t)n + c] += s3[c];
