@@ -302,7 +302,8 @@ class Engine final : public EngineBase {
            *d_fwl_ = nullptr, *d_bwl_ = nullptr, *d_fwr_ = nullptr, *d_bwr_ = nullptr,
            *d_fws_ = nullptr, *d_bws_ = nullptr, *d_zero_ = nullptr,
            *d_gat_ = nullptr, *d_xg_ = nullptr;
-  Real *d_gr_ = nullptr, *d_gc_ = nullptr;
+  using GT = typename GStore<Real>::type;  // storage of the solve matrix G
+  GT *d_gr_ = nullptr, *d_gc_ = nullptr;
   double *d_body_reach_ = nullptr, *d_body_inertia_ = nullptr;
   int *hfailed_ = nullptr;
   int *d_refresh_ = nullptr;
@@ -548,6 +549,7 @@ void Engine<Real>::analyze_factor() {
   fd_.minv = dev<double>(t.minv_pool);
   fd_.vec = dev<double>(t.vec_pool);
   fd_.y = dev<double>(t.y_pool);
+  fd_.gscale = dev<double>(t.y_pool);
   fd_.xsol = dev<double>(3 * size_t(d_.N));
   fd_.failed = dev<int>(d_.B);
   d_refresh_ = dev<int>(d_.B);
@@ -580,8 +582,8 @@ void Engine<Real>::analyze_factor() {
   ++graph_epoch_;
   d_gat_ = upload(t.gat_tasks);
   d_xg_ = upload(t.xg_tasks);
-  d_gr_ = dev<Real>(t.g_pool);
-  d_gc_ = dev<Real>(t.g_pool);
+  d_gr_ = dev<GT>(t.g_pool);
+  d_gc_ = dev<GT>(t.g_pool);
   d_body_reach_ = dev<double>(d_.B);
   d_body_inertia_ = dev<double>(d_.B);
   sym_ready_ = true;
@@ -1322,7 +1324,9 @@ void Engine<Real>::time_kernel(uint32_t kind, uint32_t repeats, talus_mi_kernel_
       if (!in_kind[f]) continue;
       const double k = t.k[f], m = t.m[f];
       const double g = m * k - 0.5 * k * (k - 1.0);
-      bytes += forward ? g * sz + 16.0 * m : g * sz + 8.0 * m + 12.0 * k;
+      // G in its storage type, the front vector, the column scales
+      const double gsz = double(sizeof(GT));
+      bytes += forward ? g * gsz + 16.0 * m + 8.0 * k : g * gsz + 8.0 * m + 20.0 * k;
     }
     all_active();
     auto run = [&](uint32_t l) {

@@ -17,14 +17,19 @@
 //              and SYRK/GEMM updates on FP64 MFMA tiles)
 //   solve      the diagonal block of each front is inverted explicitly
 //              (M11 = L11^-1) and packed with W21 = L21 M11 into
-//              G = [M11; W21] (m x k, element precision, stored row-major
-//              and column-major): the forward pass of a front is one GEMV
-//              [y; w] = G v1, u = v2 - w, and the backward pass one
-//              transposed GEMV x1 = G^T [y; -x2]
+//              G = [M11; W21] (m x k, stored row-major and column-major):
+//              the forward pass of a front is one GEMV [y; w] = G v1,
+//              u = v2 - w, and the backward pass one transposed GEMV
+//              x1 = G^T [y; -x2].  Column j is stored divided by s_j, the
+//              power of two nearest M11[j][j] (exact scaling), as FP16 in
+//              the mixed path and FP64 in the FP64 path; the solves scale
+//              v1 by s on the way in (forward) and x1 by s on the way out
+//              (backward), accumulating in FP64
 // All sums run in a fixed order: the factor and every solve are
 // bit-reproducible.
 #pragma once
 
+#include <hip/hip_fp16.h>
 #include <hip/hip_runtime.h>
 
 #include <cstdint>
@@ -33,6 +38,18 @@
 #include "host.h"
 
 namespace tmi {
+
+// storage type of the solve matrix G per element precision: the mixed path
+// halves the bytes the preconditioner streams (it only shapes the PCG
+// iterates; the solve still converges to the requested tolerance)
+template <class Real>
+struct GStore {
+  using type = Real;
+};
+template <>
+struct GStore<float> {
+  using type = __half;
+};
 
 constexpr int FNB = 64;   // panel / tile width in DOF
 constexpr int SGRP = 4;   // solve workgroup: 4 wavefronts, 1 or 4 rows/columns each
@@ -140,6 +157,7 @@ struct FactorDev {
   const uint64_t *src;
   const uint64_t *off, *minv_off, *vec_off, *yoff, *g_off;
   double *pool, *minv, *vec, *y, *xsol;
+  double *gscale;      // per front column (yoff indexing): s_j of G's column j
   int *failed;         // per body
   const int *refresh;  // per body: nonzero = this factorization rebuilds its fronts
 };
@@ -173,29 +191,30 @@ void factor_trtri(const FactorDev &f, const uint64_t *tasks, uint32_t count, hip
 
 template <class Real>
 struct SolveLaunch {
+  using G = typename GStore<Real>::type;
   // G = [M11; L21 M11] in row-major (gr) and column-major (gc) copies
   static void pack(const FactorDev &f, const uint64_t *w21, uint32_t nw21, const uint64_t *m11,
-                   uint32_t nm11, Real *gr, Real *gc, hipStream_t s);
+                   uint32_t nm11, G *gr, G *gc, hipStream_t s);
   // z = (L L^T)^-1 r for every body whose convergence flag
   // (done[parity*B + body]) is clear; parity < 0 applies to every body.
   static void gather(const FactorDev &f, const uint64_t *tasks, uint32_t count, const Real *r,
                      const int *done, int parity, uint32_t B, hipStream_t s);
-  static void forward(const FactorDev &f, const Real *gr, const uint64_t *tasks, uint32_t count,
+  static void forward(const FactorDev &f, const G *gr, const uint64_t *tasks, uint32_t count,
                       const int *done, int parity, uint32_t B, hipStream_t s);
-  static void forward_lanes(const FactorDev &f, const Real *gc, const uint64_t *tasks,
+  static void forward_lanes(const FactorDev &f, const G *gc, const uint64_t *tasks,
                             uint32_t count, const int *done, int parity, uint32_t B,
                             hipStream_t s);
   static void xgather(const FactorDev &f, const uint64_t *tasks, uint32_t count,
                       const int *done, int parity, uint32_t B, hipStream_t s);
-  static void backward(const FactorDev &f, const Real *gc, const uint64_t *tasks, uint32_t count,
+  static void backward(const FactorDev &f, const G *gc, const uint64_t *tasks, uint32_t count,
                        const int *done, int parity, uint32_t B, hipStream_t s);
-  static void backward_lanes(const FactorDev &f, const Real *gr, const uint64_t *tasks,
+  static void backward_lanes(const FactorDev &f, const G *gr, const uint64_t *tasks,
                              uint32_t count, const int *done, int parity, uint32_t B,
                              hipStream_t s);
-  static void forward_split(const FactorDev &f, const Real *gc, const uint64_t *tasks,
+  static void forward_split(const FactorDev &f, const G *gc, const uint64_t *tasks,
                             uint32_t count, const int *done, int parity, uint32_t B,
                             hipStream_t s);
-  static void backward_split(const FactorDev &f, const Real *gr, const uint64_t *tasks,
+  static void backward_split(const FactorDev &f, const G *gr, const uint64_t *tasks,
                              uint32_t count, const int *done, int parity, uint32_t B,
                              hipStream_t s);
   static void finish(const FactorDev &f, Real *z, uint32_t N, const uint8_t *anchored,

@@ -381,12 +381,58 @@ __device__ __forceinline__ double wave_sum64(double v) {
   return v;
 }
 
+// stored entries of G and their FP64 value
+__device__ __forceinline__ double gval(double x) { return x; }
+__device__ __forceinline__ double gval(float x) { return double(x); }
+__device__ __forceinline__ double gval(__half x) { return double(__half2float(x)); }
+template <class G>
+__device__ __forceinline__ G gput(double x);
+template <>
+__device__ __forceinline__ double gput<double>(double x) { return x; }
+template <>
+__device__ __forceinline__ float gput<float>(double x) { return float(x); }
+template <>
+__device__ __forceinline__ __half gput<__half>(double x) {
+  // saturate instead of overflowing to inf (FP16 max 65504)
+  return __float2half(fminf(fmaxf(float(x), -65504.0f), 65504.0f));
+}
+// s_j of column j: the power of two nearest the pivot inverse M11[j][j] > 0,
+// so G[i][j] / s_j is exact and the stored column is O(1)
+__device__ __forceinline__ double col_scale(double diag) {
+  return diag > 0.0 && diag < DBL_MAX ? ldexp(1.0, int(rint(log2(diag)))) : 1.0;
+}
+
+// One unrolled step of a strided dot product: U entries g[q*ld] (q < U)
+// against w[q], U = 8 for 4- and 8-byte storage and 16 for FP16 (the same
+// bytes in flight per lane), the q-th product into accumulator q % 4.
+template <class G>
+struct GemvStep {
+  static constexpr uint32_t U = sizeof(G) == 2 ? 16u : 8u;
+  static __device__ __forceinline__ void run(const G *g, uint64_t ld, const double *w, double *acc) {
+    G a[U];
+#pragma unroll
+    for (uint32_t q = 0; q < U; ++q) a[q] = g[uint64_t(q) * ld];
+    if constexpr (U == 8) {
+#pragma unroll
+      for (int r = 0; r < 4; ++r) acc[r] += gval(a[r]) * w[r] + gval(a[r + 4]) * w[r + 4];
+    } else {
+#pragma unroll
+      for (int r = 0; r < 4; ++r)
+        acc[r] += (gval(a[r]) * w[r] + gval(a[r + 4]) * w[r + 4]) +
+                  (gval(a[r + 8]) * w[r + 8] + gval(a[r + 12]) * w[r + 12]);
+    }
+  }
+};
+
 // G = [M11; W21]: W21 tile (I, J) = sum_{q >= J} L21[I][q] M11[q][J]
 template <class Real>
-__global__ __launch_bounds__(NT) void k_pack_w21(FactorDev f, const uint64_t *tasks, Real *gr,
-                                                 Real *gc) {
+__global__ __launch_bounds__(NT) void k_pack_w21(FactorDev f, const uint64_t *tasks,
+                                                 typename GStore<Real>::type *gr,
+                                                 typename GStore<Real>::type *gc) {
+  using G = typename GStore<Real>::type;
   __shared__ double As[FNB][LDP];  // As[kk][r] = L21[I*64 + r][q*64 + kk]
   __shared__ double Bs[FNB][LDP];  // Bs[kk][c] = M11[q*64 + kk][J*64 + c]
+  __shared__ double inv_s[FNB];    // 1 / s_j of the tile's columns
   const uint64_t t = tasks[blockIdx.x];
   const uint32_t fr = lo32(t), I = mid16(t), J = hi16(t);
   if (!stale(f, fr)) return;
@@ -411,38 +457,46 @@ __global__ __launch_bounds__(NT) void k_pack_w21(FactorDev f, const uint64_t *ta
   }
   __syncthreads();
   mfma_store(acc, As);  // As[r][c] = W21 tile
+  if (threadIdx.x < FNB && int(threadIdx.x) < nj) {
+    const uint32_t gj = J * FNB + threadIdx.x;
+    inv_s[threadIdx.x] = 1.0 / col_scale(M[uint64_t(gj) * k + gj]);
+  }
   __syncthreads();
-  Real *R = gr + f.g_off[fr];
-  Real *C = gc + f.g_off[fr];
+  G *R = gr + f.g_off[fr];
+  G *C = gc + f.g_off[fr];
   for (int idx = threadIdx.x; idx < FNB * FNB; idx += NT) {
     const int r = idx % FNB, c = idx / FNB;
     if (r >= ni || c >= nj) continue;
     const uint32_t gi = k + I * FNB + r, gj = J * FNB + c;
-    C[uint64_t(gj) * m + gi] = Real(As[r][c]);
+    C[uint64_t(gj) * m + gi] = gput<G>(As[r][c] * inv_s[c]);
   }
   for (int idx = threadIdx.x; idx < FNB * FNB; idx += NT) {
     const int c = idx % FNB, r = idx / FNB;
     if (r >= ni || c >= nj) continue;
     const uint32_t gi = k + I * FNB + r, gj = J * FNB + c;
-    R[uint64_t(gi) * k + gj] = Real(As[r][c]);
+    R[uint64_t(gi) * k + gj] = gput<G>(As[r][c] * inv_s[c]);
   }
 }
 
 // G rows 0..k-1 = M11 (lower tiles; the upper part is never read)
 template <class Real>
-__global__ __launch_bounds__(NT) void k_pack_m11(FactorDev f, const uint64_t *tasks, Real *gr,
-                                                 Real *gc) {
+__global__ __launch_bounds__(NT) void k_pack_m11(FactorDev f, const uint64_t *tasks,
+                                                 typename GStore<Real>::type *gr,
+                                                 typename GStore<Real>::type *gc) {
+  using G = typename GStore<Real>::type;
   const uint64_t t = tasks[blockIdx.x];
   const uint32_t fr = lo32(t), I = mid16(t), J = hi16(t);
   if (!stale(f, fr)) return;
   const uint32_t k = f.k[fr], m = f.m[fr];
   const double *M = f.minv + f.minv_off[fr];
-  Real *R = gr + f.g_off[fr];
-  Real *C = gc + f.g_off[fr];
+  G *R = gr + f.g_off[fr];
+  G *C = gc + f.g_off[fr];
   for (int idx = threadIdx.x; idx < FNB * FNB; idx += NT) {
     const uint32_t i = I * FNB + idx % FNB, j = J * FNB + idx / FNB;
     if (i >= k || j >= k) continue;
-    const Real v = i >= j ? Real(M[uint64_t(j) * k + i]) : Real(0);
+    const double sj = col_scale(M[uint64_t(j) * k + j]);
+    if (i == j) f.gscale[f.yoff[fr] + j] = sj;
+    const G v = i >= j ? gput<G>(M[uint64_t(j) * k + i] / sj) : gput<G>(0.0);
     R[uint64_t(i) * k + j] = v;
     C[uint64_t(j) * m + i] = v;
   }
@@ -453,9 +507,10 @@ __global__ __launch_bounds__(NT) void k_pack_m11(FactorDev f, const uint64_t *ta
 // workgroup, 4 independent loads in flight per lane; y[i] for i < k and
 // v[i] -= . for i >= k
 template <class Real>
-__global__ __launch_bounds__(NT) void k_solve_forward(FactorDev f, const Real *gr,
+__global__ __launch_bounds__(NT) void k_solve_forward(FactorDev f, const typename GStore<Real>::type *gr,
                                                       const uint64_t *tasks, const int *done,
                                                       int parity, uint32_t B) {
+  using G = typename GStore<Real>::type;
   __shared__ double vs[SCH];
   const uint64_t t = tasks[blockIdx.x];
   const uint32_t fr = lo32(t), grp = mid16(t), gs = hi16(t), rw = gs / (NT / 64);
@@ -466,26 +521,27 @@ __global__ __launch_bounds__(NT) void k_solve_forward(FactorDev f, const Real *g
   const uint32_t ilast = min(i0 + gs, m) - 1;
   const uint32_t nmax = ilast < k ? ilast + 1 : k;  // columns any row of the group needs
   double *v = f.vec + f.vec_off[fr];
+  const double *sc = f.gscale + f.yoff[fr];
   double s0[4] = {}, s1[4] = {}, s2[4] = {}, s3[4] = {};
   for (uint32_t c0 = 0; c0 < nmax; c0 += SCH) {
     const uint32_t cn = min(uint32_t(SCH), nmax - c0);
     __syncthreads();
-    for (uint32_t q = threadIdx.x; q < cn; q += NT) vs[q] = v[c0 + q];
+    for (uint32_t q = threadIdx.x; q < cn; q += NT) vs[q] = v[c0 + q] * sc[c0 + q];
     __syncthreads();
     for (uint32_t r = 0; r < rw; ++r) {
       const uint32_t i = i0 + wave * rw + r;
       const uint32_t n = i < m ? (i < k ? i + 1 : k) : 0;
-      const Real *row = gr + f.g_off[fr] + uint64_t(min(i, m - 1)) * k;
+      const G *row = gr + f.g_off[fr] + uint64_t(min(i, m - 1)) * k;
       const uint32_t end = n > c0 ? min(n, c0 + cn) : c0;
       uint32_t j = c0 + lane;
       for (; j + 192 < end; j += 256) {
-        const Real a0 = row[j], a1 = row[j + 64], a2 = row[j + 128], a3 = row[j + 192];
-        s0[r] += double(a0) * vs[j - c0];
-        s1[r] += double(a1) * vs[j + 64 - c0];
-        s2[r] += double(a2) * vs[j + 128 - c0];
-        s3[r] += double(a3) * vs[j + 192 - c0];
+        const G a0 = row[j], a1 = row[j + 64], a2 = row[j + 128], a3 = row[j + 192];
+        s0[r] += gval(a0) * vs[j - c0];
+        s1[r] += gval(a1) * vs[j + 64 - c0];
+        s2[r] += gval(a2) * vs[j + 128 - c0];
+        s3[r] += gval(a3) * vs[j + 192 - c0];
       }
-      for (; j < end; j += 64) s0[r] += double(row[j]) * vs[j - c0];
+      for (; j < end; j += 64) s0[r] += gval(row[j]) * vs[j - c0];
     }
   }
   for (uint32_t r = 0; r < rw; ++r) {
@@ -506,37 +562,32 @@ __global__ __launch_bounds__(NT) void k_solve_forward(FactorDev f, const Real *g
 // whose rows all lie below j_end = min(k, first row + 64) skips the zero
 // upper triangle of M11 beyond it.
 template <class Real>
-__global__ __launch_bounds__(NT) void k_solve_forward_lanes(FactorDev f, const Real *gc,
+__global__ __launch_bounds__(NT) void k_solve_forward_lanes(FactorDev f, const typename GStore<Real>::type *gc,
                                                             const uint64_t *tasks, const int *done,
                                                             int parity, uint32_t B) {
+  using G = typename GStore<Real>::type;
   __shared__ double vs[KLANES];
   const uint64_t t = tasks[blockIdx.x];
   const uint32_t fr = lo32(t), chunk = mid16(t);
   if (skip(done, parity, B, f.body[fr])) return;
   const uint32_t k = f.k[fr], m = f.m[fr];
   double *v = f.vec + f.vec_off[fr];
-  for (uint32_t q = threadIdx.x; q < k; q += NT) vs[q] = v[q];
+  const double *sc = f.gscale + f.yoff[fr];
+  for (uint32_t q = threadIdx.x; q < k; q += NT) vs[q] = v[q] * sc[q];
   __syncthreads();
   const uint32_t first = chunk * NT + (threadIdx.x & ~63u);
   if (first >= m) return;
   const uint32_t i = chunk * NT + threadIdx.x;
   const uint32_t jend = min(k, first + 64u);
-  const Real *g = gc + f.g_off[fr] + min(i, m - 1);
+  const G *g = gc + f.g_off[fr] + min(i, m - 1);
   const uint64_t ld = m;
-  double s0 = 0.0, s1 = 0.0, s2 = 0.0, s3 = 0.0;
+  double acc[4] = {0.0, 0.0, 0.0, 0.0};
   uint32_t j = 0;
-  for (; j + 8 <= jend; j += 8) {
-    Real a[8];
-#pragma unroll
-    for (int q = 0; q < 8; ++q) a[q] = g[uint64_t(j + q) * ld];
-    s0 += double(a[0]) * vs[j] + double(a[4]) * vs[j + 4];
-    s1 += double(a[1]) * vs[j + 1] + double(a[5]) * vs[j + 5];
-    s2 += double(a[2]) * vs[j + 2] + double(a[6]) * vs[j + 6];
-    s3 += double(a[3]) * vs[j + 3] + double(a[7]) * vs[j + 7];
-  }
-  for (; j < jend; ++j) s0 += double(g[uint64_t(j) * ld]) * vs[j];
+  for (; j + GemvStep<G>::U <= jend; j += GemvStep<G>::U)
+    GemvStep<G>::run(g + uint64_t(j) * ld, ld, vs + j, acc);
+  for (; j < jend; ++j) acc[0] += gval(g[uint64_t(j) * ld]) * vs[j];
   if (i >= m) return;
-  const double tot = (s0 + s1) + (s2 + s3);
+  const double tot = (acc[0] + acc[1]) + (acc[2] + acc[3]);
   if (i < k)
     f.y[f.yoff[fr] + i] = tot;
   else
@@ -548,9 +599,10 @@ __global__ __launch_bounds__(NT) void k_solve_forward_lanes(FactorDev f, const R
 // wavefront load); w = [y; -x2] from the wavefront's first column on staged
 // in LDS.  Rows above a column's diagonal hold stored zeros.
 template <class Real>
-__global__ __launch_bounds__(NT) void k_solve_backward_lanes(FactorDev f, const Real *gr,
+__global__ __launch_bounds__(NT) void k_solve_backward_lanes(FactorDev f, const typename GStore<Real>::type *gr,
                                                              const uint64_t *tasks, const int *done,
                                                              int parity, uint32_t B) {
+  using G = typename GStore<Real>::type;
   __shared__ double ws[MLANES];
   const uint64_t t = tasks[blockIdx.x];
   const uint32_t fr = lo32(t), chunk = mid16(t);
@@ -564,23 +616,16 @@ __global__ __launch_bounds__(NT) void k_solve_backward_lanes(FactorDev f, const 
   const uint32_t first = c0 + (threadIdx.x & ~63u);
   if (first >= k) return;
   const uint32_t j = c0 + threadIdx.x;
-  const Real *g = gr + f.g_off[fr] + min(j, k - 1);
+  const G *g = gr + f.g_off[fr] + min(j, k - 1);
   const uint64_t ld = k;
-  double s0 = 0.0, s1 = 0.0, s2 = 0.0, s3 = 0.0;
+  double acc[4] = {0.0, 0.0, 0.0, 0.0};
   uint32_t i = first;
-  for (; i + 8 <= m; i += 8) {
-    Real a[8];
-#pragma unroll
-    for (int q = 0; q < 8; ++q) a[q] = g[uint64_t(i + q) * ld];
-    s0 += double(a[0]) * ws[i] + double(a[4]) * ws[i + 4];
-    s1 += double(a[1]) * ws[i + 1] + double(a[5]) * ws[i + 5];
-    s2 += double(a[2]) * ws[i + 2] + double(a[6]) * ws[i + 6];
-    s3 += double(a[3]) * ws[i + 3] + double(a[7]) * ws[i + 7];
-  }
-  for (; i < m; ++i) s0 += double(g[uint64_t(i) * ld]) * ws[i];
+  for (; i + GemvStep<G>::U <= m; i += GemvStep<G>::U)
+    GemvStep<G>::run(g + uint64_t(i) * ld, ld, ws + i, acc);
+  for (; i < m; ++i) acc[0] += gval(g[uint64_t(i) * ld]) * ws[i];
   if (j >= k) return;
   const uint32_t *rows = f.rows + f.rows_off[fr];
-  f.xsol[3 * size_t(rows[j / 3]) + j % 3] = (s0 + s1) + (s2 + s3);
+  f.xsol[3 * size_t(rows[j / 3]) + j % 3] = ((acc[0] + acc[1]) + (acc[2] + acc[3])) * f.gscale[f.yoff[fr] + j];
 }
 
 // forward, split shape (fronts with k <= KLANES): 64 rows of G per
@@ -589,9 +634,10 @@ __global__ __launch_bounds__(NT) void k_solve_backward_lanes(FactorDev f, const 
 // 64 consecutive entries of a column per wavefront load), partial sums
 // added in LDS in wavefront order
 template <class Real>
-__global__ __launch_bounds__(NT) void k_solve_forward_split(FactorDev f, const Real *gc,
+__global__ __launch_bounds__(NT) void k_solve_forward_split(FactorDev f, const typename GStore<Real>::type *gc,
                                                             const uint64_t *tasks, const int *done,
                                                             int parity, uint32_t B) {
+  using G = typename GStore<Real>::type;
   __shared__ double vs[KLANES];
   __shared__ double red[NT / 64][64];
   const uint64_t t = tasks[blockIdx.x];
@@ -601,27 +647,21 @@ __global__ __launch_bounds__(NT) void k_solve_forward_split(FactorDev f, const R
   double *v = f.vec + f.vec_off[fr];
   const uint32_t first = chunk * 64u;
   const uint32_t jend = min(k, first + 64u);
-  for (uint32_t q = threadIdx.x; q < jend; q += NT) vs[q] = v[q];
+  const double *sc = f.gscale + f.yoff[fr];
+  for (uint32_t q = threadIdx.x; q < jend; q += NT) vs[q] = v[q] * sc[q];
   __syncthreads();
   const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
   const uint32_t i = first + lane;
   const uint32_t quarter = (jend + 3u) / 4u;
   const uint32_t j0 = min(jend, wave * quarter), j1 = min(jend, j0 + quarter);
-  const Real *g = gc + f.g_off[fr] + min(i, m - 1);
+  const G *g = gc + f.g_off[fr] + min(i, m - 1);
   const uint64_t ld = m;
-  double s0 = 0.0, s1 = 0.0, s2 = 0.0, s3 = 0.0;
+  double acc[4] = {0.0, 0.0, 0.0, 0.0};
   uint32_t j = j0;
-  for (; j + 8 <= j1; j += 8) {
-    Real a[8];
-#pragma unroll
-    for (int q = 0; q < 8; ++q) a[q] = g[uint64_t(j + q) * ld];
-    s0 += double(a[0]) * vs[j] + double(a[4]) * vs[j + 4];
-    s1 += double(a[1]) * vs[j + 1] + double(a[5]) * vs[j + 5];
-    s2 += double(a[2]) * vs[j + 2] + double(a[6]) * vs[j + 6];
-    s3 += double(a[3]) * vs[j + 3] + double(a[7]) * vs[j + 7];
-  }
-  for (; j < j1; ++j) s0 += double(g[uint64_t(j) * ld]) * vs[j];
-  red[wave][lane] = (s0 + s1) + (s2 + s3);
+  for (; j + GemvStep<G>::U <= j1; j += GemvStep<G>::U)
+    GemvStep<G>::run(g + uint64_t(j) * ld, ld, vs + j, acc);
+  for (; j < j1; ++j) acc[0] += gval(g[uint64_t(j) * ld]) * vs[j];
+  red[wave][lane] = (acc[0] + acc[1]) + (acc[2] + acc[3]);
   __syncthreads();
   if (wave != 0 || i >= m) return;
   const double tot = ((red[0][lane] + red[1][lane]) + red[2][lane]) + red[3][lane];
@@ -636,9 +676,10 @@ __global__ __launch_bounds__(NT) void k_solve_forward_split(FactorDev f, const R
 // contiguous quarters over the wavefronts (row-major copy), w = [y; -x2]
 // staged in LDS, partial sums added in LDS in wavefront order
 template <class Real>
-__global__ __launch_bounds__(NT) void k_solve_backward_split(FactorDev f, const Real *gr,
+__global__ __launch_bounds__(NT) void k_solve_backward_split(FactorDev f, const typename GStore<Real>::type *gr,
                                                              const uint64_t *tasks, const int *done,
                                                              int parity, uint32_t B) {
+  using G = typename GStore<Real>::type;
   __shared__ double ws[MLANES];
   __shared__ double red[NT / 64][64];
   const uint64_t t = tasks[blockIdx.x];
@@ -654,25 +695,19 @@ __global__ __launch_bounds__(NT) void k_solve_backward_split(FactorDev f, const 
   const uint32_t j = first + lane;
   const uint32_t rows = m - first, quarter = (rows + 3u) / 4u;
   const uint32_t i0 = first + min(rows, wave * quarter), i1 = first + min(rows, wave * quarter + quarter);
-  const Real *g = gr + f.g_off[fr] + min(j, k - 1);
+  const G *g = gr + f.g_off[fr] + min(j, k - 1);
   const uint64_t ld = k;
-  double s0 = 0.0, s1 = 0.0, s2 = 0.0, s3 = 0.0;
+  double acc[4] = {0.0, 0.0, 0.0, 0.0};
   uint32_t i = i0;
-  for (; i + 8 <= i1; i += 8) {
-    Real a[8];
-#pragma unroll
-    for (int q = 0; q < 8; ++q) a[q] = g[uint64_t(i + q) * ld];
-    s0 += double(a[0]) * ws[i] + double(a[4]) * ws[i + 4];
-    s1 += double(a[1]) * ws[i + 1] + double(a[5]) * ws[i + 5];
-    s2 += double(a[2]) * ws[i + 2] + double(a[6]) * ws[i + 6];
-    s3 += double(a[3]) * ws[i + 3] + double(a[7]) * ws[i + 7];
-  }
-  for (; i < i1; ++i) s0 += double(g[uint64_t(i) * ld]) * ws[i];
-  red[wave][lane] = (s0 + s1) + (s2 + s3);
+  for (; i + GemvStep<G>::U <= i1; i += GemvStep<G>::U)
+    GemvStep<G>::run(g + uint64_t(i) * ld, ld, ws + i, acc);
+  for (; i < i1; ++i) acc[0] += gval(g[uint64_t(i) * ld]) * ws[i];
+  red[wave][lane] = (acc[0] + acc[1]) + (acc[2] + acc[3]);
   __syncthreads();
   if (wave != 0 || j >= k) return;
   const uint32_t *rowsf = f.rows + f.rows_off[fr];
-  f.xsol[3 * size_t(rowsf[j / 3]) + j % 3] = ((red[0][lane] + red[1][lane]) + red[2][lane]) + red[3][lane];
+  f.xsol[3 * size_t(rowsf[j / 3]) + j % 3] =
+      (((red[0][lane] + red[1][lane]) + red[2][lane]) + red[3][lane]) * f.gscale[f.yoff[fr] + j];
 }
 
 // backward prologue: x2 (solution at the front's update rows) into v[k:m]
@@ -691,9 +726,10 @@ __global__ __launch_bounds__(NT) void k_solve_xgather(FactorDev f, const uint64_
 // backward: x[col j] = sum_{j<=i<k} G[i][j] y[i] - sum_{i>=k} G[i][j] x2[i],
 // each wavefront 1 or 4 consecutive columns, w = [y; -x2] staged in LDS
 template <class Real>
-__global__ __launch_bounds__(NT) void k_solve_backward(FactorDev f, const Real *gc,
+__global__ __launch_bounds__(NT) void k_solve_backward(FactorDev f, const typename GStore<Real>::type *gc,
                                                        const uint64_t *tasks, const int *done,
                                                        int parity, uint32_t B) {
+  using G = typename GStore<Real>::type;
   __shared__ double ws[SCH];
   const uint64_t t = tasks[blockIdx.x];
   const uint32_t fr = lo32(t), grp = mid16(t), gs = hi16(t), cw = gs / (NT / 64);
@@ -716,23 +752,23 @@ __global__ __launch_bounds__(NT) void k_solve_backward(FactorDev f, const Real *
     for (uint32_t c = 0; c < cw; ++c) {
       const uint32_t j = j0 + wave * cw + c;
       const uint32_t start = j < k ? j : m;  // rows of this column: [j, m)
-      const Real *col = gc + f.g_off[fr] + uint64_t(min(j, k - 1)) * m;
+      const G *col = gc + f.g_off[fr] + uint64_t(min(j, k - 1)) * m;
       uint32_t i = max(start, c0) + lane;
       for (; i + 192 < end; i += 256) {
-        const Real a0 = col[i], a1 = col[i + 64], a2 = col[i + 128], a3 = col[i + 192];
-        s0[c] += double(a0) * ws[i - c0];
-        s1[c] += double(a1) * ws[i + 64 - c0];
-        s2[c] += double(a2) * ws[i + 128 - c0];
-        s3[c] += double(a3) * ws[i + 192 - c0];
+        const G a0 = col[i], a1 = col[i + 64], a2 = col[i + 128], a3 = col[i + 192];
+        s0[c] += gval(a0) * ws[i - c0];
+        s1[c] += gval(a1) * ws[i + 64 - c0];
+        s2[c] += gval(a2) * ws[i + 128 - c0];
+        s3[c] += gval(a3) * ws[i + 192 - c0];
       }
-      for (; i < end; i += 64) s0[c] += double(col[i]) * ws[i - c0];
+      for (; i < end; i += 64) s0[c] += gval(col[i]) * ws[i - c0];
     }
   }
   const uint32_t *rows = f.rows + f.rows_off[fr];
   for (uint32_t c = 0; c < cw; ++c) {
     const double tot = wave_sum64((s0[c] + s1[c]) + (s2[c] + s3[c]));
     const uint32_t j = j0 + wave * cw + c;
-    if (lane == 0 && j < k) f.xsol[3 * size_t(rows[j / 3]) + j % 3] = tot;
+    if (lane == 0 && j < k) f.xsol[3 * size_t(rows[j / 3]) + j % 3] = tot * f.gscale[f.yoff[fr] + j];
   }
 }
 
@@ -793,7 +829,7 @@ void factor_trtri(const FactorDev &f, const uint64_t *tasks, uint32_t count, hip
 
 template <class Real>
 void SolveLaunch<Real>::pack(const FactorDev &f, const uint64_t *w21, uint32_t nw21,
-                             const uint64_t *m11, uint32_t nm11, Real *gr, Real *gc,
+                             const uint64_t *m11, uint32_t nm11, G *gr, G *gc,
                              hipStream_t s) {
   if (nw21) hipLaunchKernelGGL(k_pack_w21<Real>, dim3(nw21), dim3(NT), 0, s, f, w21, gr, gc);
   if (nm11) hipLaunchKernelGGL(k_pack_m11<Real>, dim3(nm11), dim3(NT), 0, s, f, m11, gr, gc);
@@ -805,31 +841,31 @@ void SolveLaunch<Real>::gather(const FactorDev &f, const uint64_t *tasks, uint32
   if (count) hipLaunchKernelGGL(k_solve_gather<Real>, dim3(count), dim3(NT), 0, s, f, tasks, r, done, parity, B);
 }
 template <class Real>
-void SolveLaunch<Real>::forward(const FactorDev &f, const Real *gr, const uint64_t *tasks,
+void SolveLaunch<Real>::forward(const FactorDev &f, const G *gr, const uint64_t *tasks,
                                 uint32_t count, const int *done, int parity, uint32_t B,
                                 hipStream_t s) {
   if (count) hipLaunchKernelGGL(k_solve_forward<Real>, dim3(count), dim3(NT), 0, s, f, gr, tasks, done, parity, B);
 }
 template <class Real>
-void SolveLaunch<Real>::forward_lanes(const FactorDev &f, const Real *gc, const uint64_t *tasks,
+void SolveLaunch<Real>::forward_lanes(const FactorDev &f, const G *gc, const uint64_t *tasks,
                                       uint32_t count, const int *done, int parity, uint32_t B,
                                       hipStream_t s) {
   if (count) hipLaunchKernelGGL(k_solve_forward_lanes<Real>, dim3(count), dim3(NT), 0, s, f, gc, tasks, done, parity, B);
 }
 template <class Real>
-void SolveLaunch<Real>::backward_lanes(const FactorDev &f, const Real *gr, const uint64_t *tasks,
+void SolveLaunch<Real>::backward_lanes(const FactorDev &f, const G *gr, const uint64_t *tasks,
                                        uint32_t count, const int *done, int parity, uint32_t B,
                                        hipStream_t s) {
   if (count) hipLaunchKernelGGL(k_solve_backward_lanes<Real>, dim3(count), dim3(NT), 0, s, f, gr, tasks, done, parity, B);
 }
 template <class Real>
-void SolveLaunch<Real>::forward_split(const FactorDev &f, const Real *gc, const uint64_t *tasks,
+void SolveLaunch<Real>::forward_split(const FactorDev &f, const G *gc, const uint64_t *tasks,
                                       uint32_t count, const int *done, int parity, uint32_t B,
                                       hipStream_t s) {
   if (count) hipLaunchKernelGGL(k_solve_forward_split<Real>, dim3(count), dim3(NT), 0, s, f, gc, tasks, done, parity, B);
 }
 template <class Real>
-void SolveLaunch<Real>::backward_split(const FactorDev &f, const Real *gr, const uint64_t *tasks,
+void SolveLaunch<Real>::backward_split(const FactorDev &f, const G *gr, const uint64_t *tasks,
                                        uint32_t count, const int *done, int parity, uint32_t B,
                                        hipStream_t s) {
   if (count) hipLaunchKernelGGL(k_solve_backward_split<Real>, dim3(count), dim3(NT), 0, s, f, gr, tasks, done, parity, B);
@@ -840,7 +876,7 @@ void SolveLaunch<Real>::xgather(const FactorDev &f, const uint64_t *tasks, uint3
   if (count) hipLaunchKernelGGL(k_solve_xgather, dim3(count), dim3(NT), 0, s, f, tasks, done, parity, B);
 }
 template <class Real>
-void SolveLaunch<Real>::backward(const FactorDev &f, const Real *gc, const uint64_t *tasks,
+void SolveLaunch<Real>::backward(const FactorDev &f, const G *gc, const uint64_t *tasks,
                                  uint32_t count, const int *done, int parity, uint32_t B,
                                  hipStream_t s) {
   if (count) hipLaunchKernelGGL(k_solve_backward<Real>, dim3(count), dim3(NT), 0, s, f, gc, tasks, done, parity, B);

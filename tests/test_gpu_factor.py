@@ -104,7 +104,11 @@ def test_tuned_plan_is_valid_and_reproducible():
 
 
 def test_factor_solve_mixed_storage_is_a_good_preconditioner():
-    _check_solve(scenes.cube(6, seed=8).bodies[0], bar=1e-5, precision=api.MIXED)
+    # the mixed path stores G in FP16 (unit roundoff 2^-11 = 4.9e-4) with
+    # exact power-of-two column scales and accumulates in FP64: the solve is
+    # within a few roundoffs of the dense solve
+    _check_solve(scenes.cube(6, seed=8).bodies[0], bar=1e-3, precision=api.MIXED)
+    _check_solve(scenes.cantilever().bodies[0], bar=1e-3, precision=api.MIXED)
 
 
 def test_factor_solve_batched_equals_single():
