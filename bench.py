@@ -265,6 +265,9 @@ def main(argv=None):
     ap.add_argument("--skip-plain", action="store_true")
     ap.add_argument("--skip-cpu", action="store_true")
     ap.add_argument("--no-tune", action="store_true")
+    ap.add_argument("--direct-launch", action="store_true",
+                    help="launch the PCG iteration kernels directly instead of replaying a "
+                         "captured HIP graph (for tracers that do not follow graph launches)")
     ap.add_argument("--plan", default=None,
                     help="fixed solve plan (digits, 2 per level) instead of autotuning the mixed context")
     ap.add_argument("--cpu-seconds", type=float, default=15.0)
@@ -296,7 +299,8 @@ def main(argv=None):
     T = sum(int(m["tet_nodes"].shape[0]) for m, _ in meshes)
     N = sum(int(m["anchored"].shape[0]) for m, _ in meshes)
     sub = sc.substeps
-    params = api.default_params(substeps=sub, preconditioner=api.FACTOR)
+    params = api.default_params(substeps=sub, preconditioner=api.FACTOR,
+                                flags=api.DIRECT_LAUNCH if args.direct_launch else 0)
     log(f"[bench] workload {args.workload}: {len(meshes)} bodies, {T} tets, {N} nodes")
 
     log("[bench] mixed batched")

@@ -300,8 +300,7 @@ class Engine final : public EngineBase {
   uint64_t *d_ea_ = nullptr, *d_trsm_ = nullptr, *d_syrk_ = nullptr, *d_trtri_ = nullptr,
            *d_w21_ = nullptr, *d_pack_ = nullptr, *d_fwd_ = nullptr, *d_bwd_ = nullptr,
            *d_fwl_ = nullptr, *d_bwl_ = nullptr, *d_fwr_ = nullptr, *d_bwr_ = nullptr,
-           *d_fws_ = nullptr, *d_bws_ = nullptr, *d_zero_ = nullptr,
-           *d_gat_ = nullptr, *d_xg_ = nullptr;
+           *d_fws_ = nullptr, *d_bws_ = nullptr, *d_zero_ = nullptr;
   using GT = typename GStore<Real>::type;  // storage of the solve matrix G
   GT *d_gr_ = nullptr, *d_gc_ = nullptr;
   double *d_body_reach_ = nullptr, *d_body_inertia_ = nullptr;
@@ -580,8 +579,6 @@ void Engine<Real>::analyze_factor() {
   d_bws_ = upload(t.bws_tasks);
   plan_ = t.plan;
   ++graph_epoch_;
-  d_gat_ = upload(t.gat_tasks);
-  d_xg_ = upload(t.xg_tasks);
   d_gr_ = dev<GT>(t.g_pool);
   d_gc_ = dev<GT>(t.g_pool);
   d_body_reach_ = dev<double>(d_.B);
@@ -656,16 +653,16 @@ template <class Real>
 void Engine<Real>::forward_level(uint32_t l, int mode, int parity) {
   const FactorTables &t = tab_;
   if (mode == 1)
-    SolveLaunch<Real>::forward_lanes(fd_, d_gc_, d_fwl_ + t.fwl_start[l], t.fwl_start[l + 1] - t.fwl_start[l],
+    SolveLaunch<Real>::forward_lanes(fd_, d_gc_, d_fwl_ + t.fwl_start[l], t.fwl_start[l + 1] - t.fwl_start[l], d_.r,
                                      d_.done, parity, d_.B, stream_);
   else if (mode == 2)
-    SolveLaunch<Real>::forward_split(fd_, d_gc_, d_fws_ + t.fws_start[l], t.fws_start[l + 1] - t.fws_start[l],
+    SolveLaunch<Real>::forward_split(fd_, d_gc_, d_fws_ + t.fws_start[l], t.fws_start[l + 1] - t.fws_start[l], d_.r,
                                      d_.done, parity, d_.B, stream_);
   if (mode)
-    SolveLaunch<Real>::forward(fd_, d_gr_, d_fwr_ + t.fwr_start[l], t.fwr_start[l + 1] - t.fwr_start[l],
+    SolveLaunch<Real>::forward(fd_, d_gr_, d_fwr_ + t.fwr_start[l], t.fwr_start[l + 1] - t.fwr_start[l], d_.r,
                                d_.done, parity, d_.B, stream_);
   else
-    SolveLaunch<Real>::forward(fd_, d_gr_, d_fwd_ + t.fwd_start[l], t.fwd_start[l + 1] - t.fwd_start[l],
+    SolveLaunch<Real>::forward(fd_, d_gr_, d_fwd_ + t.fwd_start[l], t.fwd_start[l + 1] - t.fwd_start[l], d_.r,
                                d_.done, parity, d_.B, stream_);
 }
 
@@ -689,23 +686,16 @@ void Engine<Real>::backward_level(uint32_t l, int mode, int parity) {
 // z = (LL^T)^-1 r for the bodies still iterating (applyFactor,
 // factor.cc:648-704, without the body-frame rotation): per level from the
 // leaves, gather the right-hand side and the children's updates, then one
-// GEMV with G; per level from the root, gather the ancestors' solution and
-// one transposed GEMV with G.  A fixed plan gives bit-identical results.
+// GEMV with G; per level from the root, one transposed GEMV with G against
+// [y; -x2], x2 read from the ancestors' solution at the front's update
+// rows.  A fixed plan gives bit-identical results.
 template <class Real>
 void Engine<Real>::precondition(int parity) {
   hipStream_t s = stream_;
   const FactorTables &t = tab_;
   const uint32_t B = d_.B;
-  for (uint32_t l = 0; l < t.levels; ++l) {
-    SolveLaunch<Real>::gather(fd_, d_gat_ + t.gat_start[l], t.gat_start[l + 1] - t.gat_start[l], d_.r,
-                              d_.done, parity, B, s);
-    forward_level(l, plan_[2 * l], parity);
-  }
-  for (uint32_t l = t.levels; l-- > 0;) {
-    SolveLaunch<Real>::xgather(fd_, d_xg_ + t.xg_start[l], t.xg_start[l + 1] - t.xg_start[l], d_.done,
-                               parity, B, s);
-    backward_level(l, plan_[2 * l + 1], parity);
-  }
+  for (uint32_t l = 0; l < t.levels; ++l) forward_level(l, plan_[2 * l], parity);
+  for (uint32_t l = t.levels; l-- > 0;) backward_level(l, plan_[2 * l + 1], parity);
   SolveLaunch<Real>::finish(fd_, d_.z, d_.N, d_.anchored, d_node_body_, d_.done, parity, B, s);
 }
 
@@ -1342,9 +1332,9 @@ void Engine<Real>::time_kernel(uint32_t kind, uint32_t repeats, talus_mi_kernel_
       else dev = d_bwd_;
       const uint32_t count = (*st)[l + 1] - (*st)[l];
       dev += (*st)[l];
-      if (forward && shape == 1) SolveLaunch<Real>::forward_lanes(fd_, d_gc_, dev, count, d_.done, -1, d_.B, s);
-      else if (forward && shape == 2) SolveLaunch<Real>::forward_split(fd_, d_gc_, dev, count, d_.done, -1, d_.B, s);
-      else if (forward) SolveLaunch<Real>::forward(fd_, d_gr_, dev, count, d_.done, -1, d_.B, s);
+      if (forward && shape == 1) SolveLaunch<Real>::forward_lanes(fd_, d_gc_, dev, count, d_.r, d_.done, -1, d_.B, s);
+      else if (forward && shape == 2) SolveLaunch<Real>::forward_split(fd_, d_gc_, dev, count, d_.r, d_.done, -1, d_.B, s);
+      else if (forward) SolveLaunch<Real>::forward(fd_, d_gr_, dev, count, d_.r, d_.done, -1, d_.B, s);
       else if (shape == 1) SolveLaunch<Real>::backward_lanes(fd_, d_gr_, dev, count, d_.done, -1, d_.B, s);
       else if (shape == 2) SolveLaunch<Real>::backward_split(fd_, d_gr_, dev, count, d_.done, -1, d_.B, s);
       else SolveLaunch<Real>::backward(fd_, d_gc_, dev, count, d_.done, -1, d_.B, s);

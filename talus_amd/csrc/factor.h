@@ -54,7 +54,6 @@ struct GStore<float> {
 constexpr int FNB = 64;   // panel / tile width in DOF
 constexpr int SGRP = 4;   // solve workgroup: 4 wavefronts, 1 or 4 rows/columns each
 constexpr int SCH = 2048; // vector entries staged in LDS per chunk
-constexpr int SCHUNK = 256;  // DOF rows per gather workgroup
 constexpr uint32_t ZCHUNK = 65536;  // entries zeroed per workgroup before a factorization
 constexpr int KLANES = 512;   // fronts with k <= KLANES solve one row/column per lane
 constexpr int MLANES = 2048;  // ... backward also needs m <= MLANES (LDS)
@@ -133,8 +132,6 @@ struct FactorTables {
   // row/column per lane, 256 per workgroup; 2 = 64 rows/columns per
   // workgroup with the other dimension split over its 4 wavefronts
   std::vector<uint8_t> plan;
-  std::vector<uint64_t> gat_tasks, xg_tasks;                 // front | chunk<<32, per level
-  std::vector<uint32_t> gat_start, xg_start;
   // pull lists: for every front node row (rows_off indexing), the vec-pool
   // offsets of the child update entries (DOF 0) that add into it
   std::vector<uint32_t> src_start;
@@ -195,24 +192,22 @@ struct SolveLaunch {
   // G = [M11; L21 M11] in row-major (gr) and column-major (gc) copies
   static void pack(const FactorDev &f, const uint64_t *w21, uint32_t nw21, const uint64_t *m11,
                    uint32_t nm11, G *gr, G *gc, hipStream_t s);
-  // z = (L L^T)^-1 r for every body whose convergence flag
-  // (done[parity*B + body]) is clear; parity < 0 applies to every body.
-  static void gather(const FactorDev &f, const uint64_t *tasks, uint32_t count, const Real *r,
-                     const int *done, int parity, uint32_t B, hipStream_t s);
+  // forward and backward passes of z = (L L^T)^-1 r, one level per call, for
+  // every body whose convergence flag (done[parity*B + body]) is clear;
+  // parity < 0 applies to every body.  The forward kernels gather their
+  // fronts' right-hand side (r and the children's updates) themselves.
   static void forward(const FactorDev &f, const G *gr, const uint64_t *tasks, uint32_t count,
-                      const int *done, int parity, uint32_t B, hipStream_t s);
+                      const Real *rin, const int *done, int parity, uint32_t B, hipStream_t s);
   static void forward_lanes(const FactorDev &f, const G *gc, const uint64_t *tasks,
-                            uint32_t count, const int *done, int parity, uint32_t B,
+                            uint32_t count, const Real *rin, const int *done, int parity, uint32_t B,
                             hipStream_t s);
-  static void xgather(const FactorDev &f, const uint64_t *tasks, uint32_t count,
-                      const int *done, int parity, uint32_t B, hipStream_t s);
   static void backward(const FactorDev &f, const G *gc, const uint64_t *tasks, uint32_t count,
                        const int *done, int parity, uint32_t B, hipStream_t s);
   static void backward_lanes(const FactorDev &f, const G *gr, const uint64_t *tasks,
                              uint32_t count, const int *done, int parity, uint32_t B,
                              hipStream_t s);
   static void forward_split(const FactorDev &f, const G *gc, const uint64_t *tasks,
-                            uint32_t count, const int *done, int parity, uint32_t B,
+                            uint32_t count, const Real *rin, const int *done, int parity, uint32_t B,
                             hipStream_t s);
   static void backward_split(const FactorDev &f, const G *gr, const uint64_t *tasks,
                              uint32_t count, const int *done, int parity, uint32_t B,

@@ -357,22 +357,15 @@ __device__ __forceinline__ bool skip(const int *done, int parity, uint32_t B, ui
   return parity >= 0 && done[parity * B + body];
 }
 
-// forward gather: v[i] = r at the front's columns + the children's update
-// entries that map to row i, pulled in child order
+// forward gather of row i of a front: r at the front's columns (i < k) plus
+// the children's update entries that map to row i, pulled in child order
 template <class Real>
-__global__ __launch_bounds__(NT) void k_solve_gather(FactorDev f, const uint64_t *tasks,
-                                                     const Real *rin, const int *done, int parity,
-                                                     uint32_t B) {
-  const uint64_t t = tasks[blockIdx.x];
-  const uint32_t fr = lo32(t), chunk = mid16(t);
-  if (skip(done, parity, B, f.body[fr])) return;
-  const uint32_t k = f.k[fr], m = f.m[fr];
-  const uint32_t i = chunk * SCHUNK + threadIdx.x;
-  if (i >= m) return;
+__device__ __forceinline__ double gather_row(const FactorDev &f, uint32_t fr, uint32_t i, uint32_t k,
+                                             const Real *rin) {
   const uint32_t nrow = f.rows_off[fr] + i / 3, d = i % 3;
   double v = i < k ? double(rin[3 * size_t(f.rows[nrow]) + d]) : 0.0;
   for (uint32_t q = f.src_start[nrow]; q < f.src_start[nrow + 1]; ++q) v += f.vec[f.src[q] + d];
-  f.vec[f.vec_off[fr] + i] = v;
+  return v;
 }
 
 __device__ __forceinline__ double wave_sum64(double v) {
@@ -508,8 +501,8 @@ __global__ __launch_bounds__(NT) void k_pack_m11(FactorDev f, const uint64_t *ta
 // v[i] -= . for i >= k
 template <class Real>
 __global__ __launch_bounds__(NT) void k_solve_forward(FactorDev f, const typename GStore<Real>::type *gr,
-                                                      const uint64_t *tasks, const int *done,
-                                                      int parity, uint32_t B) {
+                                                      const uint64_t *tasks, const Real *rin,
+                                                      const int *done, int parity, uint32_t B) {
   using G = typename GStore<Real>::type;
   __shared__ double vs[SCH];
   const uint64_t t = tasks[blockIdx.x];
@@ -526,7 +519,7 @@ __global__ __launch_bounds__(NT) void k_solve_forward(FactorDev f, const typenam
   for (uint32_t c0 = 0; c0 < nmax; c0 += SCH) {
     const uint32_t cn = min(uint32_t(SCH), nmax - c0);
     __syncthreads();
-    for (uint32_t q = threadIdx.x; q < cn; q += NT) vs[q] = v[c0 + q] * sc[c0 + q];
+    for (uint32_t q = threadIdx.x; q < cn; q += NT) vs[q] = gather_row(f, fr, c0 + q, k, rin) * sc[c0 + q];
     __syncthreads();
     for (uint32_t r = 0; r < rw; ++r) {
       const uint32_t i = i0 + wave * rw + r;
@@ -551,7 +544,7 @@ __global__ __launch_bounds__(NT) void k_solve_forward(FactorDev f, const typenam
       if (i < k)
         f.y[f.yoff[fr] + i] = tot;
       else
-        v[i] -= tot;
+        v[i] = gather_row(f, fr, i, k, rin) - tot;
     }
   }
 }
@@ -563,8 +556,8 @@ __global__ __launch_bounds__(NT) void k_solve_forward(FactorDev f, const typenam
 // upper triangle of M11 beyond it.
 template <class Real>
 __global__ __launch_bounds__(NT) void k_solve_forward_lanes(FactorDev f, const typename GStore<Real>::type *gc,
-                                                            const uint64_t *tasks, const int *done,
-                                                            int parity, uint32_t B) {
+                                                            const uint64_t *tasks, const Real *rin,
+                                                            const int *done, int parity, uint32_t B) {
   using G = typename GStore<Real>::type;
   __shared__ double vs[KLANES];
   const uint64_t t = tasks[blockIdx.x];
@@ -573,7 +566,7 @@ __global__ __launch_bounds__(NT) void k_solve_forward_lanes(FactorDev f, const t
   const uint32_t k = f.k[fr], m = f.m[fr];
   double *v = f.vec + f.vec_off[fr];
   const double *sc = f.gscale + f.yoff[fr];
-  for (uint32_t q = threadIdx.x; q < k; q += NT) vs[q] = v[q] * sc[q];
+  for (uint32_t q = threadIdx.x; q < k; q += NT) vs[q] = gather_row(f, fr, q, k, rin) * sc[q];
   __syncthreads();
   const uint32_t first = chunk * NT + (threadIdx.x & ~63u);
   if (first >= m) return;
@@ -591,13 +584,14 @@ __global__ __launch_bounds__(NT) void k_solve_forward_lanes(FactorDev f, const t
   if (i < k)
     f.y[f.yoff[fr] + i] = tot;
   else
-    v[i] -= tot;
+    v[i] = gather_row(f, fr, i, k, rin) - tot;
 }
 
 // backward for fronts with k <= KLANES and m <= MLANES: one lane per column
 // of G, read from the row-major copy (64 consecutive entries of a row per
 // wavefront load); w = [y; -x2] from the wavefront's first column on staged
-// in LDS.  Rows above a column's diagonal hold stored zeros.
+// in LDS, x2 read from the solution of the ancestors' levels (xsol at the
+// front's update rows).  Rows above a column's diagonal hold stored zeros.
 template <class Real>
 __global__ __launch_bounds__(NT) void k_solve_backward_lanes(FactorDev f, const typename GStore<Real>::type *gr,
                                                              const uint64_t *tasks, const int *done,
@@ -609,9 +603,10 @@ __global__ __launch_bounds__(NT) void k_solve_backward_lanes(FactorDev f, const 
   if (skip(done, parity, B, f.body[fr])) return;
   const uint32_t k = f.k[fr], m = f.m[fr];
   const double *y = f.y + f.yoff[fr];
-  const double *v = f.vec + f.vec_off[fr];
+  const uint32_t *rows = f.rows + f.rows_off[fr];
   const uint32_t c0 = chunk * NT;
-  for (uint32_t q = c0 + threadIdx.x; q < m; q += NT) ws[q] = q < k ? y[q] : -v[q];
+  for (uint32_t q = c0 + threadIdx.x; q < m; q += NT)
+    ws[q] = q < k ? y[q] : -f.xsol[3 * size_t(rows[q / 3]) + q % 3];
   __syncthreads();
   const uint32_t first = c0 + (threadIdx.x & ~63u);
   if (first >= k) return;
@@ -624,7 +619,6 @@ __global__ __launch_bounds__(NT) void k_solve_backward_lanes(FactorDev f, const 
     GemvStep<G>::run(g + uint64_t(i) * ld, ld, ws + i, acc);
   for (; i < m; ++i) acc[0] += gval(g[uint64_t(i) * ld]) * ws[i];
   if (j >= k) return;
-  const uint32_t *rows = f.rows + f.rows_off[fr];
   f.xsol[3 * size_t(rows[j / 3]) + j % 3] = ((acc[0] + acc[1]) + (acc[2] + acc[3])) * f.gscale[f.yoff[fr] + j];
 }
 
@@ -635,8 +629,8 @@ __global__ __launch_bounds__(NT) void k_solve_backward_lanes(FactorDev f, const 
 // added in LDS in wavefront order
 template <class Real>
 __global__ __launch_bounds__(NT) void k_solve_forward_split(FactorDev f, const typename GStore<Real>::type *gc,
-                                                            const uint64_t *tasks, const int *done,
-                                                            int parity, uint32_t B) {
+                                                            const uint64_t *tasks, const Real *rin,
+                                                            const int *done, int parity, uint32_t B) {
   using G = typename GStore<Real>::type;
   __shared__ double vs[KLANES];
   __shared__ double red[NT / 64][64];
@@ -648,7 +642,7 @@ __global__ __launch_bounds__(NT) void k_solve_forward_split(FactorDev f, const t
   const uint32_t first = chunk * 64u;
   const uint32_t jend = min(k, first + 64u);
   const double *sc = f.gscale + f.yoff[fr];
-  for (uint32_t q = threadIdx.x; q < jend; q += NT) vs[q] = v[q] * sc[q];
+  for (uint32_t q = threadIdx.x; q < jend; q += NT) vs[q] = gather_row(f, fr, q, k, rin) * sc[q];
   __syncthreads();
   const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
   const uint32_t i = first + lane;
@@ -668,7 +662,7 @@ __global__ __launch_bounds__(NT) void k_solve_forward_split(FactorDev f, const t
   if (i < k)
     f.y[f.yoff[fr] + i] = tot;
   else
-    v[i] -= tot;
+    v[i] = gather_row(f, fr, i, k, rin) - tot;
 }
 
 // backward, split shape (k <= KLANES, m <= MLANES): 64 columns of G per
@@ -687,9 +681,10 @@ __global__ __launch_bounds__(NT) void k_solve_backward_split(FactorDev f, const 
   if (skip(done, parity, B, f.body[fr])) return;
   const uint32_t k = f.k[fr], m = f.m[fr];
   const double *y = f.y + f.yoff[fr];
-  const double *v = f.vec + f.vec_off[fr];
+  const uint32_t *rowsf = f.rows + f.rows_off[fr];
   const uint32_t first = chunk * 64u;
-  for (uint32_t q = first + threadIdx.x; q < m; q += NT) ws[q] = q < k ? y[q] : -v[q];
+  for (uint32_t q = first + threadIdx.x; q < m; q += NT)
+    ws[q] = q < k ? y[q] : -f.xsol[3 * size_t(rowsf[q / 3]) + q % 3];
   __syncthreads();
   const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
   const uint32_t j = first + lane;
@@ -705,26 +700,13 @@ __global__ __launch_bounds__(NT) void k_solve_backward_split(FactorDev f, const 
   red[wave][lane] = (acc[0] + acc[1]) + (acc[2] + acc[3]);
   __syncthreads();
   if (wave != 0 || j >= k) return;
-  const uint32_t *rowsf = f.rows + f.rows_off[fr];
   f.xsol[3 * size_t(rowsf[j / 3]) + j % 3] =
       (((red[0][lane] + red[1][lane]) + red[2][lane]) + red[3][lane]) * f.gscale[f.yoff[fr] + j];
 }
 
-// backward prologue: x2 (solution at the front's update rows) into v[k:m]
-__global__ __launch_bounds__(NT) void k_solve_xgather(FactorDev f, const uint64_t *tasks,
-                                                      const int *done, int parity, uint32_t B) {
-  const uint64_t t = tasks[blockIdx.x];
-  const uint32_t fr = lo32(t), chunk = mid16(t);
-  if (skip(done, parity, B, f.body[fr])) return;
-  const uint32_t k = f.k[fr], m = f.m[fr];
-  const uint32_t i = k + chunk * SCHUNK + threadIdx.x;
-  if (i >= m) return;
-  const uint32_t *rows = f.rows + f.rows_off[fr];
-  f.vec[f.vec_off[fr] + i] = f.xsol[3 * size_t(rows[i / 3]) + i % 3];
-}
-
 // backward: x[col j] = sum_{j<=i<k} G[i][j] y[i] - sum_{i>=k} G[i][j] x2[i],
-// each wavefront 1 or 4 consecutive columns, w = [y; -x2] staged in LDS
+// each wavefront 1 or 4 consecutive columns, w = [y; -x2] staged in LDS (x2
+// from the ancestors' solution at the front's update rows)
 template <class Real>
 __global__ __launch_bounds__(NT) void k_solve_backward(FactorDev f, const typename GStore<Real>::type *gc,
                                                        const uint64_t *tasks, const int *done,
@@ -738,14 +720,14 @@ __global__ __launch_bounds__(NT) void k_solve_backward(FactorDev f, const typena
   const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
   const uint32_t j0 = grp * gs;
   const double *y = f.y + f.yoff[fr];
-  const double *v = f.vec + f.vec_off[fr];
+  const uint32_t *rows = f.rows + f.rows_off[fr];
   double s0[4] = {}, s1[4] = {}, s2[4] = {}, s3[4] = {};
   for (uint32_t c0 = j0; c0 < m; c0 += SCH) {
     const uint32_t cn = min(uint32_t(SCH), m - c0);
     __syncthreads();
     for (uint32_t q = threadIdx.x; q < cn; q += NT) {
       const uint32_t i = c0 + q;
-      ws[q] = i < k ? y[i] : -v[i];
+      ws[q] = i < k ? y[i] : -f.xsol[3 * size_t(rows[i / 3]) + i % 3];
     }
     __syncthreads();
     const uint32_t end = c0 + cn;
@@ -764,7 +746,6 @@ __global__ __launch_bounds__(NT) void k_solve_backward(FactorDev f, const typena
       for (; i < end; i += 64) s0[c] += gval(col[i]) * ws[i - c0];
     }
   }
-  const uint32_t *rows = f.rows + f.rows_off[fr];
   for (uint32_t c = 0; c < cw; ++c) {
     const double tot = wave_sum64((s0[c] + s1[c]) + (s2[c] + s3[c]));
     const uint32_t j = j0 + wave * cw + c;
@@ -835,22 +816,16 @@ void SolveLaunch<Real>::pack(const FactorDev &f, const uint64_t *w21, uint32_t n
   if (nm11) hipLaunchKernelGGL(k_pack_m11<Real>, dim3(nm11), dim3(NT), 0, s, f, m11, gr, gc);
 }
 template <class Real>
-void SolveLaunch<Real>::gather(const FactorDev &f, const uint64_t *tasks, uint32_t count,
-                               const Real *r, const int *done, int parity, uint32_t B,
-                               hipStream_t s) {
-  if (count) hipLaunchKernelGGL(k_solve_gather<Real>, dim3(count), dim3(NT), 0, s, f, tasks, r, done, parity, B);
-}
-template <class Real>
 void SolveLaunch<Real>::forward(const FactorDev &f, const G *gr, const uint64_t *tasks,
-                                uint32_t count, const int *done, int parity, uint32_t B,
+                                uint32_t count, const Real *rin, const int *done, int parity, uint32_t B,
                                 hipStream_t s) {
-  if (count) hipLaunchKernelGGL(k_solve_forward<Real>, dim3(count), dim3(NT), 0, s, f, gr, tasks, done, parity, B);
+  if (count) hipLaunchKernelGGL(k_solve_forward<Real>, dim3(count), dim3(NT), 0, s, f, gr, tasks, rin, done, parity, B);
 }
 template <class Real>
 void SolveLaunch<Real>::forward_lanes(const FactorDev &f, const G *gc, const uint64_t *tasks,
-                                      uint32_t count, const int *done, int parity, uint32_t B,
+                                      uint32_t count, const Real *rin, const int *done, int parity, uint32_t B,
                                       hipStream_t s) {
-  if (count) hipLaunchKernelGGL(k_solve_forward_lanes<Real>, dim3(count), dim3(NT), 0, s, f, gc, tasks, done, parity, B);
+  if (count) hipLaunchKernelGGL(k_solve_forward_lanes<Real>, dim3(count), dim3(NT), 0, s, f, gc, tasks, rin, done, parity, B);
 }
 template <class Real>
 void SolveLaunch<Real>::backward_lanes(const FactorDev &f, const G *gr, const uint64_t *tasks,
@@ -860,20 +835,15 @@ void SolveLaunch<Real>::backward_lanes(const FactorDev &f, const G *gr, const ui
 }
 template <class Real>
 void SolveLaunch<Real>::forward_split(const FactorDev &f, const G *gc, const uint64_t *tasks,
-                                      uint32_t count, const int *done, int parity, uint32_t B,
+                                      uint32_t count, const Real *rin, const int *done, int parity, uint32_t B,
                                       hipStream_t s) {
-  if (count) hipLaunchKernelGGL(k_solve_forward_split<Real>, dim3(count), dim3(NT), 0, s, f, gc, tasks, done, parity, B);
+  if (count) hipLaunchKernelGGL(k_solve_forward_split<Real>, dim3(count), dim3(NT), 0, s, f, gc, tasks, rin, done, parity, B);
 }
 template <class Real>
 void SolveLaunch<Real>::backward_split(const FactorDev &f, const G *gr, const uint64_t *tasks,
                                        uint32_t count, const int *done, int parity, uint32_t B,
                                        hipStream_t s) {
   if (count) hipLaunchKernelGGL(k_solve_backward_split<Real>, dim3(count), dim3(NT), 0, s, f, gr, tasks, done, parity, B);
-}
-template <class Real>
-void SolveLaunch<Real>::xgather(const FactorDev &f, const uint64_t *tasks, uint32_t count,
-                                const int *done, int parity, uint32_t B, hipStream_t s) {
-  if (count) hipLaunchKernelGGL(k_solve_xgather, dim3(count), dim3(NT), 0, s, f, tasks, done, parity, B);
 }
 template <class Real>
 void SolveLaunch<Real>::backward(const FactorDev &f, const G *gc, const uint64_t *tasks,

@@ -405,8 +405,6 @@ void build_tables(const std::vector<Symbolic> &sym, const std::vector<Body> &bod
   o.bwr_start.assign(1, 0);
   o.fws_start.assign(1, 0);
   o.bws_start.assign(1, 0);
-  o.gat_start.assign(1, 0);
-  o.xg_start.assign(1, 0);
   o.plan.assign(2 * size_t(L), 0);
   for (uint32_t l = 0; l < L; ++l) {
     for (uint32_t f : by_level[l]) {
@@ -433,9 +431,6 @@ void build_tables(const std::vector<Symbolic> &sym, const std::vector<Body> &bod
         for (uint32_t g = 0; g < tiles(o.k[f], 256u); ++g) o.bwl_tasks.push_back(uint64_t(f) | (uint64_t(g) << 32));
         for (uint32_t g = 0; g < tiles(o.k[f], 64u); ++g) o.bws_tasks.push_back(uint64_t(f) | (uint64_t(g) << 32));
       }
-      for (uint32_t g = 0; g < tiles(o.m[f], SCHUNK); ++g) o.gat_tasks.push_back(uint64_t(f) | (uint64_t(g) << 32));
-      for (uint32_t g = 0; g < tiles(o.m[f] - o.k[f], SCHUNK); ++g)
-        o.xg_tasks.push_back(uint64_t(f) | (uint64_t(g) << 32));
     }
     o.fwd_start.push_back(uint32_t(o.fwd_tasks.size()));
     o.bwd_start.push_back(uint32_t(o.bwd_tasks.size()));
@@ -445,8 +440,6 @@ void build_tables(const std::vector<Symbolic> &sym, const std::vector<Body> &bod
     o.bwr_start.push_back(uint32_t(o.bwr_tasks.size()));
     o.fws_start.push_back(uint32_t(o.fws_tasks.size()));
     o.bws_start.push_back(uint32_t(o.bws_tasks.size()));
-    o.gat_start.push_back(uint32_t(o.gat_tasks.size()));
-    o.xg_start.push_back(uint32_t(o.xg_tasks.size()));
     o.plan[2 * l] = o.fwl_start[l + 1] - o.fwl_start[l] >= LANES_MIN_GROUPS ? 1 : 0;
     o.plan[2 * l + 1] = o.bwl_start[l + 1] - o.bwl_start[l] >= LANES_MIN_GROUPS ? 1 : 0;
   }
