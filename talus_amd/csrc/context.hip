@@ -581,6 +581,9 @@ void Engine<Real>::analyze_factor() {
   ++graph_epoch_;
   d_gr_ = dev<GT>(t.g_pool);
   d_gc_ = dev<GT>(t.g_pool);
+  // the tiles' padding is read by the solves and never written: clear once
+  HIPCHECK(hipMemset(d_gr_, 0, std::max<uint64_t>(t.g_pool, 1) * sizeof(GT)));
+  HIPCHECK(hipMemset(d_gc_, 0, std::max<uint64_t>(t.g_pool, 1) * sizeof(GT)));
   d_body_reach_ = dev<double>(d_.B);
   d_body_inertia_ = dev<double>(d_.B);
   sym_ready_ = true;

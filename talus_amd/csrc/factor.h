@@ -52,12 +52,15 @@ struct GStore<float> {
 };
 
 constexpr int FNB = 64;   // panel / tile width in DOF
-constexpr int SGRP = 4;   // solve workgroup: 4 wavefronts, 1 or 4 rows/columns each
+constexpr uint32_t ROWS_GROUP = 32;  // rows-shape solve workgroup: 4 wavefronts x 8 rows/columns
 constexpr int SCH = 2048; // vector entries staged in LDS per chunk
 constexpr uint32_t ZCHUNK = 65536;  // entries zeroed per workgroup before a factorization
 constexpr int KLANES = 512;   // fronts with k <= KLANES solve one row/column per lane
 constexpr int MLANES = 2048;  // ... backward also needs m <= MLANES (LDS)
 constexpr uint32_t LANES_MIN_GROUPS = 2048;  // default plan: lanes when a level has this many
+// G's storage per front is padded to multiples of 8 rows and columns (the
+// solve kernels read 16-byte tiles of up to 8 entries)
+inline uint32_t pad8(uint32_t n) { return (n + 7u) & ~7u; }
 
 struct FrontSym {
   uint32_t ncol = 0;   // nodes eliminated by this front

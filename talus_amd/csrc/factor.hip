@@ -395,27 +395,58 @@ __device__ __forceinline__ double col_scale(double diag) {
   return diag > 0.0 && diag < DBL_MAX ? ldexp(1.0, int(rint(log2(diag)))) : 1.0;
 }
 
-// One unrolled step of a strided dot product: U entries g[q*ld] (q < U)
-// against w[q], U = 8 for 4- and 8-byte storage and 16 for FP16 (the same
-// bytes in flight per lane), the q-th product into accumulator q % 4.
+// G's two copies are tiled by W = 16 / sizeof(G) entries so that a lane
+// loads 16 bytes at a time and a wavefront's 64 lanes read one contiguous
+// kilobyte:
+//   column-tiled (gc): entry (i, j) at (j / W * m + i) * W + j % W -- the W
+//     consecutive columns of a row are contiguous, consecutive rows adjacent;
+//   row-tiled (gr): entry (i, j) at (i / W * k + j) * W + i % W -- the W
+//     consecutive rows of a column are contiguous, consecutive columns
+//     adjacent.
+// Column tiles past k and row tiles past m are padded with zeros (the pool
+// is cleared when allocated and the padding is never written).
 template <class G>
-struct GemvStep {
-  static constexpr uint32_t U = sizeof(G) == 2 ? 16u : 8u;
-  static __device__ __forceinline__ void run(const G *g, uint64_t ld, const double *w, double *acc) {
-    G a[U];
-#pragma unroll
-    for (uint32_t q = 0; q < U; ++q) a[q] = g[uint64_t(q) * ld];
-    if constexpr (U == 8) {
-#pragma unroll
-      for (int r = 0; r < 4; ++r) acc[r] += gval(a[r]) * w[r] + gval(a[r + 4]) * w[r + 4];
-    } else {
-#pragma unroll
-      for (int r = 0; r < 4; ++r)
-        acc[r] += (gval(a[r]) * w[r] + gval(a[r + 4]) * w[r + 4]) +
-                  (gval(a[r + 8]) * w[r + 8] + gval(a[r + 12]) * w[r + 12]);
-    }
-  }
+struct Tile {
+  static constexpr uint32_t W = 16u / uint32_t(sizeof(G));
 };
+template <class G>
+struct alignas(16) Chunk {
+  G v[16 / sizeof(G)];
+};
+template <class G>
+__device__ __forceinline__ Chunk<G> load_chunk(const G *p) {
+  return *reinterpret_cast<const Chunk<G> *>(p);
+}
+__device__ __forceinline__ uint64_t gc_at(uint32_t i, uint32_t j, uint32_t m, uint32_t W) {
+  return (uint64_t(j / W) * m + i) * W + j % W;
+}
+__device__ __forceinline__ uint64_t gr_at(uint32_t i, uint32_t j, uint32_t k, uint32_t W) {
+  return (uint64_t(i / W) * k + j) * W + i % W;
+}
+
+// acc[q % 4] += g[b * ts + q] * w[b * W + q] over tiles b in [b0, b1) and
+// q < W: one lane's dot product along its row (gc) or column (gr), four
+// 16-byte tiles in flight
+template <class G>
+__device__ __forceinline__ void tile_dot(const G *g, uint64_t ts, const double *w, uint32_t b0,
+                                         uint32_t b1, double *acc) {
+  constexpr uint32_t W = Tile<G>::W, UB = 4;
+  uint32_t b = b0;
+  for (; b + UB <= b1; b += UB) {
+    Chunk<G> c[UB];
+#pragma unroll
+    for (uint32_t u = 0; u < UB; ++u) c[u] = load_chunk(g + uint64_t(b + u) * ts);
+#pragma unroll
+    for (uint32_t u = 0; u < UB; ++u)
+#pragma unroll
+      for (uint32_t q = 0; q < W; ++q) acc[q % 4] += gval(c[u].v[q]) * w[(b + u) * W + q];
+  }
+  for (; b < b1; ++b) {
+    const Chunk<G> c = load_chunk(g + uint64_t(b) * ts);
+#pragma unroll
+    for (uint32_t q = 0; q < W; ++q) acc[q % 4] += gval(c.v[q]) * w[b * W + q];
+  }
+}
 
 // G = [M11; W21]: W21 tile (I, J) = sum_{q >= J} L21[I][q] M11[q][J]
 template <class Real>
@@ -423,6 +454,7 @@ __global__ __launch_bounds__(NT) void k_pack_w21(FactorDev f, const uint64_t *ta
                                                  typename GStore<Real>::type *gr,
                                                  typename GStore<Real>::type *gc) {
   using G = typename GStore<Real>::type;
+  constexpr uint32_t W = Tile<G>::W;
   __shared__ double As[FNB][LDP];  // As[kk][r] = L21[I*64 + r][q*64 + kk]
   __shared__ double Bs[FNB][LDP];  // Bs[kk][c] = M11[q*64 + kk][J*64 + c]
   __shared__ double inv_s[FNB];    // 1 / s_j of the tile's columns
@@ -461,22 +493,19 @@ __global__ __launch_bounds__(NT) void k_pack_w21(FactorDev f, const uint64_t *ta
     const int r = idx % FNB, c = idx / FNB;
     if (r >= ni || c >= nj) continue;
     const uint32_t gi = k + I * FNB + r, gj = J * FNB + c;
-    C[uint64_t(gj) * m + gi] = gput<G>(As[r][c] * inv_s[c]);
-  }
-  for (int idx = threadIdx.x; idx < FNB * FNB; idx += NT) {
-    const int c = idx % FNB, r = idx / FNB;
-    if (r >= ni || c >= nj) continue;
-    const uint32_t gi = k + I * FNB + r, gj = J * FNB + c;
-    R[uint64_t(gi) * k + gj] = gput<G>(As[r][c] * inv_s[c]);
+    const G v = gput<G>(As[r][c] * inv_s[c]);
+    C[gc_at(gi, gj, m, W)] = v;
+    R[gr_at(gi, gj, k, W)] = v;
   }
 }
 
-// G rows 0..k-1 = M11 (lower tiles; the upper part is never read)
+// G rows 0..k-1 = M11 (lower tiles; the upper part is stored as zeros)
 template <class Real>
 __global__ __launch_bounds__(NT) void k_pack_m11(FactorDev f, const uint64_t *tasks,
                                                  typename GStore<Real>::type *gr,
                                                  typename GStore<Real>::type *gc) {
   using G = typename GStore<Real>::type;
+  constexpr uint32_t W = Tile<G>::W;
   const uint64_t t = tasks[blockIdx.x];
   const uint32_t fr = lo32(t), I = mid16(t), J = hi16(t);
   if (!stale(f, fr)) return;
@@ -490,56 +519,81 @@ __global__ __launch_bounds__(NT) void k_pack_m11(FactorDev f, const uint64_t *ta
     const double sj = col_scale(M[uint64_t(j) * k + j]);
     if (i == j) f.gscale[f.yoff[fr] + j] = sj;
     const G v = i >= j ? gput<G>(M[uint64_t(j) * k + i] / sj) : gput<G>(0.0);
-    R[uint64_t(i) * k + j] = v;
-    C[uint64_t(j) * m + i] = v;
+    R[gr_at(i, j, k, W)] = v;
+    C[gc_at(i, j, m, W)] = v;
   }
 }
 
-// forward: rows of G times v[0:k], each wavefront 1 or 4 consecutive rows
-// (4 for small fronts), the vector staged in LDS chunks shared by the
-// workgroup, 4 independent loads in flight per lane; y[i] for i < k and
-// v[i] -= . for i >= k
+// Rows kernels (fronts too large for the lane shapes, and the plan's
+// per-row choice): a workgroup takes 32 consecutive rows (forward) or
+// columns (backward), 8 per wavefront, and the wavefront's lanes run along
+// the other dimension, each loading the 16-byte tile that holds its column's
+// (row's) entries for all of the wavefront's rows (columns); 8 FP64
+// wavefront reductions at the end.
+constexpr uint32_t ROWS_PER_WAVE = 8;
+static_assert(ROWS_GROUP == (NT / 64) * ROWS_PER_WAVE, "rows tasks cover 4 wavefronts of 8");
+
+// forward, rows shape: y[i] (i < k) or v[i] - . (i >= k) for the 32 rows,
+// row-tiled copy, the vector staged in LDS chunks shared by the workgroup
 template <class Real>
 __global__ __launch_bounds__(NT) void k_solve_forward(FactorDev f, const typename GStore<Real>::type *gr,
                                                       const uint64_t *tasks, const Real *rin,
                                                       const int *done, int parity, uint32_t B) {
   using G = typename GStore<Real>::type;
+  constexpr uint32_t W = Tile<G>::W, RW = ROWS_PER_WAVE;
   __shared__ double vs[SCH];
   const uint64_t t = tasks[blockIdx.x];
-  const uint32_t fr = lo32(t), grp = mid16(t), gs = hi16(t), rw = gs / (NT / 64);
+  const uint32_t fr = lo32(t), grp = mid16(t), gs = hi16(t);
   if (skip(done, parity, B, f.body[fr])) return;
   const uint32_t k = f.k[fr], m = f.m[fr];
   const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
-  const uint32_t i0 = grp * gs;
-  const uint32_t ilast = min(i0 + gs, m) - 1;
-  const uint32_t nmax = ilast < k ? ilast + 1 : k;  // columns any row of the group needs
+  const uint32_t g0 = grp * gs;
+  const uint32_t glast = min(g0 + gs, m) - 1;
+  const uint32_t nmax = glast < k ? glast + 1 : k;  // columns any row of the group needs
+  const uint32_t i0 = g0 + wave * RW;               // first row of this wavefront (tile aligned)
+  const uint32_t wlast = min(i0 + RW, m) - 1;
+  const uint32_t nw = i0 < m ? (wlast < k ? wlast + 1 : k) : 0;
   double *v = f.vec + f.vec_off[fr];
   const double *sc = f.gscale + f.yoff[fr];
-  double s0[4] = {}, s1[4] = {}, s2[4] = {}, s3[4] = {};
+  const G *g = gr + f.g_off[fr] + uint64_t(i0 / W) * k * W;  // the wavefront's first row tile
+  const uint64_t ts = uint64_t(k) * W;                        // between row tiles
+  double acc[RW];
+#pragma unroll
+  for (uint32_t r = 0; r < RW; ++r) acc[r] = 0.0;
   for (uint32_t c0 = 0; c0 < nmax; c0 += SCH) {
     const uint32_t cn = min(uint32_t(SCH), nmax - c0);
     __syncthreads();
     for (uint32_t q = threadIdx.x; q < cn; q += NT) vs[q] = gather_row(f, fr, c0 + q, k, rin) * sc[c0 + q];
     __syncthreads();
-    for (uint32_t r = 0; r < rw; ++r) {
-      const uint32_t i = i0 + wave * rw + r;
-      const uint32_t n = i < m ? (i < k ? i + 1 : k) : 0;
-      const G *row = gr + f.g_off[fr] + uint64_t(min(i, m - 1)) * k;
-      const uint32_t end = n > c0 ? min(n, c0 + cn) : c0;
-      uint32_t j = c0 + lane;
-      for (; j + 192 < end; j += 256) {
-        const G a0 = row[j], a1 = row[j + 64], a2 = row[j + 128], a3 = row[j + 192];
-        s0[r] += gval(a0) * vs[j - c0];
-        s1[r] += gval(a1) * vs[j + 64 - c0];
-        s2[r] += gval(a2) * vs[j + 128 - c0];
-        s3[r] += gval(a3) * vs[j + 192 - c0];
+    const uint32_t end = min(nw, c0 + cn);
+    uint32_t j = c0 + lane;
+    for (; j + 64 < end; j += 128) {
+      Chunk<G> c[2][RW / W];
+#pragma unroll
+      for (uint32_t a = 0; a < RW / W; ++a) {
+        c[0][a] = load_chunk(g + a * ts + uint64_t(j) * W);
+        c[1][a] = load_chunk(g + a * ts + uint64_t(j + 64) * W);
       }
-      for (; j < end; j += 64) s0[r] += gval(row[j]) * vs[j - c0];
+      const double w0 = vs[j - c0], w1 = vs[j + 64 - c0];
+#pragma unroll
+      for (uint32_t a = 0; a < RW / W; ++a)
+#pragma unroll
+        for (uint32_t q = 0; q < W; ++q) acc[a * W + q] += gval(c[0][a].v[q]) * w0 + gval(c[1][a].v[q]) * w1;
+    }
+    for (; j < end; j += 64) {
+      const double w0 = vs[j - c0];
+#pragma unroll
+      for (uint32_t a = 0; a < RW / W; ++a) {
+        const Chunk<G> c = load_chunk(g + a * ts + uint64_t(j) * W);
+#pragma unroll
+        for (uint32_t q = 0; q < W; ++q) acc[a * W + q] += gval(c.v[q]) * w0;
+      }
     }
   }
-  for (uint32_t r = 0; r < rw; ++r) {
-    const double tot = wave_sum64((s0[r] + s1[r]) + (s2[r] + s3[r]));
-    const uint32_t i = i0 + wave * rw + r;
+#pragma unroll
+  for (uint32_t r = 0; r < RW; ++r) {
+    const double tot = wave_sum64(acc[r]);
+    const uint32_t i = i0 + r;
     if (lane == 0 && i < m) {
       if (i < k)
         f.y[f.yoff[fr] + i] = tot;
@@ -549,36 +603,34 @@ __global__ __launch_bounds__(NT) void k_solve_forward(FactorDev f, const typenam
   }
 }
 
-// forward for fronts with k <= KLANES: one lane per row of G, read from the
-// column-major copy so a wavefront loads 64 consecutive entries of a column
-// at a time; v[0:k] staged in LDS; no cross-lane reduction.  A wavefront
-// whose rows all lie below j_end = min(k, first row + 64) skips the zero
-// upper triangle of M11 beyond it.
+// forward for fronts with k <= KLANES: one lane per row of G, along the
+// row's column tiles of the column-tiled copy; v[0:k] staged in LDS; no
+// cross-lane reduction.  A wavefront whose rows all lie below
+// j_end = min(k, first row + 64) skips the zero upper triangle of M11
+// beyond it.
 template <class Real>
 __global__ __launch_bounds__(NT) void k_solve_forward_lanes(FactorDev f, const typename GStore<Real>::type *gc,
                                                             const uint64_t *tasks, const Real *rin,
                                                             const int *done, int parity, uint32_t B) {
   using G = typename GStore<Real>::type;
+  constexpr uint32_t W = Tile<G>::W;
   __shared__ double vs[KLANES];
   const uint64_t t = tasks[blockIdx.x];
   const uint32_t fr = lo32(t), chunk = mid16(t);
   if (skip(done, parity, B, f.body[fr])) return;
   const uint32_t k = f.k[fr], m = f.m[fr];
+  const uint32_t kp = (k + W - 1) / W * W;
   double *v = f.vec + f.vec_off[fr];
   const double *sc = f.gscale + f.yoff[fr];
-  for (uint32_t q = threadIdx.x; q < k; q += NT) vs[q] = gather_row(f, fr, q, k, rin) * sc[q];
+  for (uint32_t q = threadIdx.x; q < kp; q += NT) vs[q] = q < k ? gather_row(f, fr, q, k, rin) * sc[q] : 0.0;
   __syncthreads();
   const uint32_t first = chunk * NT + (threadIdx.x & ~63u);
   if (first >= m) return;
   const uint32_t i = chunk * NT + threadIdx.x;
-  const uint32_t jend = min(k, first + 64u);
-  const G *g = gc + f.g_off[fr] + min(i, m - 1);
-  const uint64_t ld = m;
+  const uint32_t nb = (min(k, first + 64u) + W - 1) / W;  // column tiles the wavefront needs
+  const G *g = gc + f.g_off[fr] + uint64_t(min(i, m - 1)) * W;
   double acc[4] = {0.0, 0.0, 0.0, 0.0};
-  uint32_t j = 0;
-  for (; j + GemvStep<G>::U <= jend; j += GemvStep<G>::U)
-    GemvStep<G>::run(g + uint64_t(j) * ld, ld, vs + j, acc);
-  for (; j < jend; ++j) acc[0] += gval(g[uint64_t(j) * ld]) * vs[j];
+  tile_dot(g, uint64_t(m) * W, vs, 0, nb, acc);
   if (i >= m) return;
   const double tot = (acc[0] + acc[1]) + (acc[2] + acc[3]);
   if (i < k)
@@ -588,50 +640,47 @@ __global__ __launch_bounds__(NT) void k_solve_forward_lanes(FactorDev f, const t
 }
 
 // backward for fronts with k <= KLANES and m <= MLANES: one lane per column
-// of G, read from the row-major copy (64 consecutive entries of a row per
-// wavefront load); w = [y; -x2] from the wavefront's first column on staged
-// in LDS, x2 read from the solution of the ancestors' levels (xsol at the
-// front's update rows).  Rows above a column's diagonal hold stored zeros.
+// of G, along the column's row tiles of the row-tiled copy from the
+// wavefront's first column on; w = [y; -x2] staged in LDS, x2 read from the
+// solution of the ancestors' levels (xsol at the front's update rows).
 template <class Real>
 __global__ __launch_bounds__(NT) void k_solve_backward_lanes(FactorDev f, const typename GStore<Real>::type *gr,
                                                              const uint64_t *tasks, const int *done,
                                                              int parity, uint32_t B) {
   using G = typename GStore<Real>::type;
+  constexpr uint32_t W = Tile<G>::W;
   __shared__ double ws[MLANES];
   const uint64_t t = tasks[blockIdx.x];
   const uint32_t fr = lo32(t), chunk = mid16(t);
   if (skip(done, parity, B, f.body[fr])) return;
   const uint32_t k = f.k[fr], m = f.m[fr];
+  const uint32_t mp = (m + W - 1) / W * W;
   const double *y = f.y + f.yoff[fr];
   const uint32_t *rows = f.rows + f.rows_off[fr];
   const uint32_t c0 = chunk * NT;
-  for (uint32_t q = c0 + threadIdx.x; q < m; q += NT)
-    ws[q] = q < k ? y[q] : -f.xsol[3 * size_t(rows[q / 3]) + q % 3];
+  for (uint32_t q = c0 + threadIdx.x; q < mp; q += NT)
+    ws[q] = q < k ? y[q] : (q < m ? -f.xsol[3 * size_t(rows[q / 3]) + q % 3] : 0.0);
   __syncthreads();
   const uint32_t first = c0 + (threadIdx.x & ~63u);
   if (first >= k) return;
   const uint32_t j = c0 + threadIdx.x;
-  const G *g = gr + f.g_off[fr] + min(j, k - 1);
-  const uint64_t ld = k;
+  const G *g = gr + f.g_off[fr] + uint64_t(min(j, k - 1)) * W;
   double acc[4] = {0.0, 0.0, 0.0, 0.0};
-  uint32_t i = first;
-  for (; i + GemvStep<G>::U <= m; i += GemvStep<G>::U)
-    GemvStep<G>::run(g + uint64_t(i) * ld, ld, ws + i, acc);
-  for (; i < m; ++i) acc[0] += gval(g[uint64_t(i) * ld]) * ws[i];
+  tile_dot(g, uint64_t(k) * W, ws, first / W, mp / W, acc);
   if (j >= k) return;
   f.xsol[3 * size_t(rows[j / 3]) + j % 3] = ((acc[0] + acc[1]) + (acc[2] + acc[3])) * f.gscale[f.yoff[fr] + j];
 }
 
 // forward, split shape (fronts with k <= KLANES): 64 rows of G per
-// workgroup, one per lane, the columns [0, min(k, first row + 64)) split in
-// four contiguous quarters over the workgroup's wavefronts (column-major copy,
-// 64 consecutive entries of a column per wavefront load), partial sums
-// added in LDS in wavefront order
+// workgroup, one per lane, the column tiles [0, min(k, first row + 64))
+// split in four contiguous quarters over the workgroup's wavefronts
+// (column-tiled copy), partial sums added in LDS in wavefront order
 template <class Real>
 __global__ __launch_bounds__(NT) void k_solve_forward_split(FactorDev f, const typename GStore<Real>::type *gc,
                                                             const uint64_t *tasks, const Real *rin,
                                                             const int *done, int parity, uint32_t B) {
   using G = typename GStore<Real>::type;
+  constexpr uint32_t W = Tile<G>::W;
   __shared__ double vs[KLANES];
   __shared__ double red[NT / 64][64];
   const uint64_t t = tasks[blockIdx.x];
@@ -640,21 +689,17 @@ __global__ __launch_bounds__(NT) void k_solve_forward_split(FactorDev f, const t
   const uint32_t k = f.k[fr], m = f.m[fr];
   double *v = f.vec + f.vec_off[fr];
   const uint32_t first = chunk * 64u;
-  const uint32_t jend = min(k, first + 64u);
+  const uint32_t nb = (min(k, first + 64u) + W - 1) / W;
   const double *sc = f.gscale + f.yoff[fr];
-  for (uint32_t q = threadIdx.x; q < jend; q += NT) vs[q] = gather_row(f, fr, q, k, rin) * sc[q];
+  for (uint32_t q = threadIdx.x; q < nb * W; q += NT) vs[q] = q < k ? gather_row(f, fr, q, k, rin) * sc[q] : 0.0;
   __syncthreads();
   const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
   const uint32_t i = first + lane;
-  const uint32_t quarter = (jend + 3u) / 4u;
-  const uint32_t j0 = min(jend, wave * quarter), j1 = min(jend, j0 + quarter);
-  const G *g = gc + f.g_off[fr] + min(i, m - 1);
-  const uint64_t ld = m;
+  const uint32_t quarter = (nb + 3u) / 4u;
+  const uint32_t b0 = min(nb, wave * quarter), b1 = min(nb, b0 + quarter);
+  const G *g = gc + f.g_off[fr] + uint64_t(min(i, m - 1)) * W;
   double acc[4] = {0.0, 0.0, 0.0, 0.0};
-  uint32_t j = j0;
-  for (; j + GemvStep<G>::U <= j1; j += GemvStep<G>::U)
-    GemvStep<G>::run(g + uint64_t(j) * ld, ld, vs + j, acc);
-  for (; j < j1; ++j) acc[0] += gval(g[uint64_t(j) * ld]) * vs[j];
+  tile_dot(g, uint64_t(m) * W, vs, b0, b1, acc);
   red[wave][lane] = (acc[0] + acc[1]) + (acc[2] + acc[3]);
   __syncthreads();
   if (wave != 0 || i >= m) return;
@@ -666,37 +711,35 @@ __global__ __launch_bounds__(NT) void k_solve_forward_split(FactorDev f, const t
 }
 
 // backward, split shape (k <= KLANES, m <= MLANES): 64 columns of G per
-// workgroup, one per lane, the rows [first column, m) split in four
-// contiguous quarters over the wavefronts (row-major copy), w = [y; -x2]
+// workgroup, one per lane, the row tiles [first column, m) split in four
+// contiguous quarters over the wavefronts (row-tiled copy), w = [y; -x2]
 // staged in LDS, partial sums added in LDS in wavefront order
 template <class Real>
 __global__ __launch_bounds__(NT) void k_solve_backward_split(FactorDev f, const typename GStore<Real>::type *gr,
                                                              const uint64_t *tasks, const int *done,
                                                              int parity, uint32_t B) {
   using G = typename GStore<Real>::type;
+  constexpr uint32_t W = Tile<G>::W;
   __shared__ double ws[MLANES];
   __shared__ double red[NT / 64][64];
   const uint64_t t = tasks[blockIdx.x];
   const uint32_t fr = lo32(t), chunk = mid16(t);
   if (skip(done, parity, B, f.body[fr])) return;
   const uint32_t k = f.k[fr], m = f.m[fr];
+  const uint32_t mp = (m + W - 1) / W * W;
   const double *y = f.y + f.yoff[fr];
   const uint32_t *rowsf = f.rows + f.rows_off[fr];
   const uint32_t first = chunk * 64u;
-  for (uint32_t q = first + threadIdx.x; q < m; q += NT)
-    ws[q] = q < k ? y[q] : -f.xsol[3 * size_t(rowsf[q / 3]) + q % 3];
+  for (uint32_t q = first + threadIdx.x; q < mp; q += NT)
+    ws[q] = q < k ? y[q] : (q < m ? -f.xsol[3 * size_t(rowsf[q / 3]) + q % 3] : 0.0);
   __syncthreads();
   const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
   const uint32_t j = first + lane;
-  const uint32_t rows = m - first, quarter = (rows + 3u) / 4u;
-  const uint32_t i0 = first + min(rows, wave * quarter), i1 = first + min(rows, wave * quarter + quarter);
-  const G *g = gr + f.g_off[fr] + min(j, k - 1);
-  const uint64_t ld = k;
+  const uint32_t t0 = first / W, tiles = mp / W - t0, quarter = (tiles + 3u) / 4u;
+  const uint32_t b0 = t0 + min(tiles, wave * quarter), b1 = t0 + min(tiles, wave * quarter + quarter);
+  const G *g = gr + f.g_off[fr] + uint64_t(min(j, k - 1)) * W;
   double acc[4] = {0.0, 0.0, 0.0, 0.0};
-  uint32_t i = i0;
-  for (; i + GemvStep<G>::U <= i1; i += GemvStep<G>::U)
-    GemvStep<G>::run(g + uint64_t(i) * ld, ld, ws + i, acc);
-  for (; i < i1; ++i) acc[0] += gval(g[uint64_t(i) * ld]) * ws[i];
+  tile_dot(g, uint64_t(k) * W, ws, b0, b1, acc);
   red[wave][lane] = (acc[0] + acc[1]) + (acc[2] + acc[3]);
   __syncthreads();
   if (wave != 0 || j >= k) return;
@@ -704,25 +747,32 @@ __global__ __launch_bounds__(NT) void k_solve_backward_split(FactorDev f, const 
       (((red[0][lane] + red[1][lane]) + red[2][lane]) + red[3][lane]) * f.gscale[f.yoff[fr] + j];
 }
 
-// backward: x[col j] = sum_{j<=i<k} G[i][j] y[i] - sum_{i>=k} G[i][j] x2[i],
-// each wavefront 1 or 4 consecutive columns, w = [y; -x2] staged in LDS (x2
-// from the ancestors' solution at the front's update rows)
+// backward, rows shape: x[col j] = sum_{j<=i<k} G[i][j] y[i]
+// - sum_{i>=k} G[i][j] x2[i] for the 32 columns, column-tiled copy, lanes
+// along the rows, w = [y; -x2] staged in LDS chunks (x2 from the ancestors'
+// solution at the front's update rows)
 template <class Real>
 __global__ __launch_bounds__(NT) void k_solve_backward(FactorDev f, const typename GStore<Real>::type *gc,
                                                        const uint64_t *tasks, const int *done,
                                                        int parity, uint32_t B) {
   using G = typename GStore<Real>::type;
+  constexpr uint32_t W = Tile<G>::W, CW = ROWS_PER_WAVE;
   __shared__ double ws[SCH];
   const uint64_t t = tasks[blockIdx.x];
-  const uint32_t fr = lo32(t), grp = mid16(t), gs = hi16(t), cw = gs / (NT / 64);
+  const uint32_t fr = lo32(t), grp = mid16(t), gs = hi16(t);
   if (skip(done, parity, B, f.body[fr])) return;
   const uint32_t k = f.k[fr], m = f.m[fr];
   const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
-  const uint32_t j0 = grp * gs;
+  const uint32_t g0 = grp * gs;                // first column of the group (tile aligned)
+  const uint32_t j0 = g0 + wave * CW;          // first column of this wavefront
   const double *y = f.y + f.yoff[fr];
   const uint32_t *rows = f.rows + f.rows_off[fr];
-  double s0[4] = {}, s1[4] = {}, s2[4] = {}, s3[4] = {};
-  for (uint32_t c0 = j0; c0 < m; c0 += SCH) {
+  const G *g = gc + f.g_off[fr] + uint64_t(j0 / W) * m * W;  // the wavefront's first column tile
+  const uint64_t ts = uint64_t(m) * W;                        // between column tiles
+  double acc[CW];
+#pragma unroll
+  for (uint32_t r = 0; r < CW; ++r) acc[r] = 0.0;
+  for (uint32_t c0 = g0; c0 < m; c0 += SCH) {
     const uint32_t cn = min(uint32_t(SCH), m - c0);
     __syncthreads();
     for (uint32_t q = threadIdx.x; q < cn; q += NT) {
@@ -730,25 +780,36 @@ __global__ __launch_bounds__(NT) void k_solve_backward(FactorDev f, const typena
       ws[q] = i < k ? y[i] : -f.xsol[3 * size_t(rows[i / 3]) + i % 3];
     }
     __syncthreads();
+    if (j0 >= k) continue;
     const uint32_t end = c0 + cn;
-    for (uint32_t c = 0; c < cw; ++c) {
-      const uint32_t j = j0 + wave * cw + c;
-      const uint32_t start = j < k ? j : m;  // rows of this column: [j, m)
-      const G *col = gc + f.g_off[fr] + uint64_t(min(j, k - 1)) * m;
-      uint32_t i = max(start, c0) + lane;
-      for (; i + 192 < end; i += 256) {
-        const G a0 = col[i], a1 = col[i + 64], a2 = col[i + 128], a3 = col[i + 192];
-        s0[c] += gval(a0) * ws[i - c0];
-        s1[c] += gval(a1) * ws[i + 64 - c0];
-        s2[c] += gval(a2) * ws[i + 128 - c0];
-        s3[c] += gval(a3) * ws[i + 192 - c0];
+    uint32_t i = max(j0, c0) + lane;
+    for (; i + 64 < end; i += 128) {
+      Chunk<G> c[2][CW / W];
+#pragma unroll
+      for (uint32_t a = 0; a < CW / W; ++a) {
+        c[0][a] = load_chunk(g + a * ts + uint64_t(i) * W);
+        c[1][a] = load_chunk(g + a * ts + uint64_t(i + 64) * W);
       }
-      for (; i < end; i += 64) s0[c] += gval(col[i]) * ws[i - c0];
+      const double w0 = ws[i - c0], w1 = ws[i + 64 - c0];
+#pragma unroll
+      for (uint32_t a = 0; a < CW / W; ++a)
+#pragma unroll
+        for (uint32_t q = 0; q < W; ++q) acc[a * W + q] += gval(c[0][a].v[q]) * w0 + gval(c[1][a].v[q]) * w1;
+    }
+    for (; i < end; i += 64) {
+      const double w0 = ws[i - c0];
+#pragma unroll
+      for (uint32_t a = 0; a < CW / W; ++a) {
+        const Chunk<G> c = load_chunk(g + a * ts + uint64_t(i) * W);
+#pragma unroll
+        for (uint32_t q = 0; q < W; ++q) acc[a * W + q] += gval(c.v[q]) * w0;
+      }
     }
   }
-  for (uint32_t c = 0; c < cw; ++c) {
-    const double tot = wave_sum64((s0[c] + s1[c]) + (s2[c] + s3[c]));
-    const uint32_t j = j0 + wave * cw + c;
+#pragma unroll
+  for (uint32_t r = 0; r < CW; ++r) {
+    const double tot = wave_sum64(acc[r]);
+    const uint32_t j = j0 + r;
     if (lane == 0 && j < k) f.xsol[3 * size_t(rows[j / 3]) + j % 3] = tot * f.gscale[f.yoff[fr] + j];
   }
 }

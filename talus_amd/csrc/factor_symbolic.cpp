@@ -220,7 +220,7 @@ void build_tables(const std::vector<Symbolic> &sym, const std::vector<Body> &bod
       o.yoff[f] = yv;
       yv += o.k[f];
       o.g_off[f] = gp;
-      gp += uint64_t(o.m[f]) * o.k[f];
+      gp += uint64_t(pad8(o.m[f])) * pad8(o.k[f]);  // both tiled copies fit, padding included
       o.rows_off[f] = uint32_t(o.rows.size());
       for (uint32_t r = 0; r < fs.nrow; ++r) o.rows.push_back(n0[b] + s.perm[s.rows[fs.row0 + r]]);
       const uint32_t cs = s.child_start[lf], ce = s.child_start[lf + 1];
@@ -408,11 +408,11 @@ void build_tables(const std::vector<Symbolic> &sym, const std::vector<Body> &bod
   o.plan.assign(2 * size_t(L), 0);
   for (uint32_t l = 0; l < L; ++l) {
     for (uint32_t f : by_level[l]) {
-      // rows kernel: 1 or 4 rows (columns) per wavefront, every front; lanes
+      // rows kernel: 32 rows (columns) per workgroup, every front; lanes
       // kernel: one row (column) per lane, 256 per workgroup, for fronts with
       // few pivots, the rest of the level again with the rows kernel
       const bool fl = o.k[f] <= uint32_t(KLANES), bl = fl && o.m[f] <= uint32_t(MLANES);
-      const uint64_t gs = o.k[f] <= 192 ? 4 * SGRP : SGRP;
+      const uint64_t gs = ROWS_GROUP;
       for (uint32_t g = 0; g < tiles(o.m[f], uint32_t(gs)); ++g) {
         const uint64_t task = uint64_t(f) | (uint64_t(g) << 32) | (gs << 48);
         o.fwd_tasks.push_back(task);
