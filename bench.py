@@ -76,10 +76,10 @@ def batched(api, meshes, precision, params, steps, warmup, tune, plan=None):
     ids = [ctx.add_body(m, mats) for m, mats in meshes]
     ctx.step(params)
     tuned = None
-    if plan:
+    if plan and len(plan) == len(ctx.plan()):
         ctx.set_plan([int(c) for c in plan])
-    elif tune:
-        tuned = ctx.tune(3)
+    elif tune or plan:
+        tuned = ctx.tune(3)  # no plan, or one made for another level structure
     for _ in range(max(warmup - 1, 0)):
         ctx.step(params)
     t0 = time.perf_counter()
@@ -270,6 +270,9 @@ def main(argv=None):
                          "captured HIP graph (for tracers that do not follow graph launches)")
     ap.add_argument("--plan", default=None,
                     help="fixed solve plan (digits, 2 per level) instead of autotuning the mixed context")
+    ap.add_argument("--retune", action="store_true",
+                    help="autotune the mixed context even where profiles/solve_plan_<workload>.txt "
+                         "holds a plan tuned earlier")
     ap.add_argument("--cpu-seconds", type=float, default=15.0)
     ap.add_argument("--cpu-threads", type=int, default=16)
     ap.add_argument("--plain-steps", type=int, default=2)
@@ -294,6 +297,15 @@ def main(argv=None):
         return 0
 
     from talus_amd import api
+
+    # a plan tuned once (talus_mi_tune) and re-applied, as a host would do,
+    # so runs are reproducible and match the committed profiles of that plan
+    plan_source = "fixed" if args.plan else "tuned"
+    saved = os.path.join(ROOT, "profiles", f"solve_plan_{args.workload}.txt")
+    if args.plan is None and not args.retune and not args.no_tune and os.path.exists(saved):
+        with open(saved) as f:
+            args.plan = f.read().split()[0]
+        plan_source = f"profiles/solve_plan_{args.workload}.txt (tuned earlier, re-applied)"
 
     sc, meshes, desc = workload(args.workload)
     T = sum(int(m["tet_nodes"].shape[0]) for m, _ in meshes)
@@ -386,7 +398,8 @@ def main(argv=None):
             "cg_capped": sum(r["cg_capped"] for r in reps_m),
             "preconditioner_applications_per_step": apps,
             "refactors": sum(r["refactors"] for r in reps_m),
-            "solve_plan": "".join(map(str, ctx_m.plan())), "tune": tuned,
+            "solve_plan": "".join(map(str, ctx_m.plan())), "solve_plan_source": plan_source,
+            "tune": tuned,
         },
         "factor": finfo,
         "fp64": fp64,
