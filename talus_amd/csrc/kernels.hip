@@ -152,13 +152,43 @@ __global__ __launch_bounds__(TILE) void k_evaluate(Dev<Real> d) {
   }
 }
 
+// Per-node sums over a node tile's slots.  The tile's nodes own one
+// contiguous run of CSR positions [k0, k1); the workgroup streams it through
+// LDS in coalesced chunks and each thread adds its node's part [s, e) of
+// every chunk in ascending position order, the order of a plain per-node
+// loop.  Every thread of the block calls it with the same k0, k1.
+constexpr uint32_t RUN_CHUNK = 1024;  // slots per LDS chunk
+template <class Real, int C>
+__device__ __forceinline__ void tile_run_sum(const Real *slots, uint32_t k0, uint32_t k1, uint32_t s,
+                                             uint32_t e, Real *acc, Real *buf) {
+  for (uint32_t c0 = k0; c0 < k1; c0 += RUN_CHUNK) {
+    const uint32_t c1 = min(k1, c0 + RUN_CHUNK);
+    __syncthreads();
+    const Real *src = slots + (size_t)C * c0;
+    for (uint32_t idx = threadIdx.x; idx < uint32_t(C) * (c1 - c0); idx += TILE) buf[idx] = src[idx];
+    __syncthreads();
+    const uint32_t lo = max(s, c0), hi = min(e, c1);
+    for (uint32_t kk = lo; kk < hi; ++kk)
+#pragma unroll
+      for (int r = 0; r < C; ++r) acc[r] += buf[C * (kk - c0) + r];
+  }
+}
+
 template <class Real, bool DIAG, bool TAKEN>
 __global__ __launch_bounds__(TILE) void k_balance(Dev<Real> d) {
   const uint32_t t = blockIdx.x, b = d.nt_body[t];
   const StepParams &P = d.params[b];
   if (!P.active) return;
-  const uint32_t n = d.nt_begin[t] + threadIdx.x;
-  if (n >= d.nt_end[t]) return;
+  __shared__ Real buf[6 * RUN_CHUNK];
+  const uint32_t n0 = d.nt_begin[t], n1 = d.nt_end[t];
+  const uint32_t n = n0 + threadIdx.x;
+  const bool own = n < n1 && !d.anchored[n];
+  const uint32_t s = own ? d.inc_start[n] : 0u, e = own ? d.inc_start[n + 1] : 0u;
+  Real f[3] = {Real(0), Real(0), Real(0)};
+  Real D[6] = {Real(0), Real(0), Real(0), Real(0), Real(0), Real(0)};
+  tile_run_sum<Real, 3>(d.slot_f, d.inc_start[n0], d.inc_start[n1], s, e, f, buf);
+  if (DIAG) tile_run_sum<Real, 6>(d.slot_d, d.inc_start[n0], d.inc_start[n1], s, e, D, buf);
+  if (n >= n1) return;
   Real *Pn = d.P + 6 * (size_t)n;
   if (d.anchored[n]) {
 #pragma unroll
@@ -172,18 +202,6 @@ __global__ __launch_bounds__(TILE) void k_balance(Dev<Real> d) {
       Pn[3] = Real(1); Pn[4] = Real(0); Pn[5] = Real(1);
     }
     return;
-  }
-  Real f[3] = {Real(0), Real(0), Real(0)};
-  Real D[6] = {Real(0), Real(0), Real(0), Real(0), Real(0), Real(0)};
-  const uint32_t k1 = d.inc_start[n + 1];
-  for (uint32_t k = d.inc_start[n]; k < k1; ++k) {
-    f[0] += d.slot_f[3 * (size_t)k + 0];
-    f[1] += d.slot_f[3 * (size_t)k + 1];
-    f[2] += d.slot_f[3 * (size_t)k + 2];
-    if (DIAG) {
-#pragma unroll
-      for (int j = 0; j < 6; ++j) D[j] += d.slot_d[6 * (size_t)k + j];
-    }
   }
   const double m = d.mass[n];
 #pragma unroll
@@ -387,17 +405,16 @@ __global__ __launch_bounds__(TILE) void k_op_gather(Dev<Real> d, const Real *dir
   const uint32_t t = blockIdx.x, b = d.nt_body[t];
   const StepParams &P = d.params[b];
   if (!P.active || (parity >= 0 && d.done[parity * d.B + b])) return;
-  const uint32_t n = d.nt_begin[t] + threadIdx.x;
+  __shared__ Real buf[3 * RUN_CHUNK];
+  const uint32_t n0 = d.nt_begin[t], n1 = d.nt_end[t];
+  const uint32_t n = n0 + threadIdx.x;
+  const bool own = n < n1 && !d.anchored[n];
+  Real q[3] = {Real(0), Real(0), Real(0)};
+  tile_run_sum<Real, 3>(d.slot_f, d.inc_start[n0], d.inc_start[n1], own ? d.inc_start[n] : 0u,
+                        own ? d.inc_start[n + 1] : 0u, q, buf);
   double pq = 0.0;
-  if (n < d.nt_end[t]) {
-    Real q[3] = {Real(0), Real(0), Real(0)};
-    if (!d.anchored[n]) {
-      const uint32_t k1 = d.inc_start[n + 1];
-      for (uint32_t k = d.inc_start[n]; k < k1; ++k) {
-        q[0] += d.slot_f[3 * (size_t)k + 0];
-        q[1] += d.slot_f[3 * (size_t)k + 1];
-        q[2] += d.slot_f[3 * (size_t)k + 2];
-      }
+  if (n < n1) {
+    if (own) {
       const Real im = Real(P.inertia * d.mass[n]), reach = Real(P.reach);
 #pragma unroll
       for (int c = 0; c < 3; ++c) q[c] = im * dir[3 * (size_t)n + c] + reach * q[c];
