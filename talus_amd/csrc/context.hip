@@ -462,6 +462,12 @@ void Engine<Real>::build(const std::vector<Body> &bodies) {
   d_.q = dev<Real>(3 * N);
   d_.conn = dev_copy(conn);
   d_.dminv = dev_copy(dminv);
+  if constexpr (sizeof(Real) == sizeof(double)) {
+    d_.dminv_r = reinterpret_cast<const Real *>(d_.dminv);
+  } else {
+    std::vector<Real> dminv_r(dminv.begin(), dminv.end());
+    d_.dminv_r = dev_copy(dminv_r);
+  }
   d_.vol = dev_copy(vol);
   d_.lam = dev_copy(lam);
   d_.mu = dev_copy(mu);
@@ -1275,8 +1281,8 @@ void Engine<Real>::time_kernel(uint32_t kind, uint32_t repeats, talus_mi_kernel_
     const double snh = double(n_snh_), plastic = double(n_plastic_);
     if (kind == 0)
       out.bytes = T * (tet + 9.0 * sz + 12.0 * sz) + snh * 9.0 * sz + plastic * (80.0 + 16.0) + N * 3.0 * sz;
-    else if (kind == 1)
-      out.bytes = T * (tet + 9.0 * sz + 12.0 * sz) + snh * 9.0 * sz + N * 3.0 * sz;
+    else if (kind == 1)  // the operator reads Dm^-1 in the element precision
+      out.bytes = T * (tet - 72.0 + 9.0 * sz + 9.0 * sz + 12.0 * sz) + snh * 9.0 * sz + N * 3.0 * sz;
     else
       out.bytes = T * 12.0 * sz + (N + 1.0) * 4.0 + N * (6.0 * sz + 8.0 + 1.0);
   } else if (kind <= 8) {

@@ -64,6 +64,7 @@ struct Dev {
   // tets
   const uint4 *conn;
   const double *dminv;
+  const Real *dminv_r;  // Dm^-1 in the element precision (the operator's copy)
   const Real *vol, *lam, *mu;
   const double *vol64, *lam64, *mu64;
   const double *rho;  // density when masses follow rho V / 4, else 0

@@ -59,7 +59,7 @@ __device__ __forceinline__ double body_total(const double *partial, uint32_t t0,
 template <class Real>
 __device__ __forceinline__ void load_dm(const Dev<Real> &d, uint32_t e, Real *dm) {
 #pragma unroll
-  for (int k = 0; k < 9; ++k) dm[k] = Real(d.dminv[(size_t)k * d.T + e]);
+  for (int k = 0; k < 9; ++k) dm[k] = d.dminv_r[(size_t)k * d.T + e];
 }
 
 template <class Real>
