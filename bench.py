@@ -71,9 +71,10 @@ def workload(name):
     return sc, meshes, desc
 
 
-def batched(api, meshes, precision, params, steps, warmup, tune, plan=None):
+def batched(api, meshes, precision, params, steps, warmup, tune, plan=None, streams=1):
     ctx = api.Context(0, precision)
     ids = [ctx.add_body(m, mats) for m, mats in meshes]
+    ctx.set_streams(streams)
     ctx.step(params)
     tuned = None
     if plan and len(plan) == len(ctx.plan()):
@@ -265,6 +266,8 @@ def main(argv=None):
     ap.add_argument("--skip-plain", action="store_true")
     ap.add_argument("--skip-cpu", action="store_true")
     ap.add_argument("--no-tune", action="store_true")
+    ap.add_argument("--streams", type=int, default=3,
+                    help="engines the batch is split over, stepped concurrently (talus_mi_set_streams)")
     ap.add_argument("--direct-launch", action="store_true",
                     help="launch the PCG iteration kernels directly instead of replaying a "
                          "captured HIP graph (for tracers that do not follow graph launches)")
@@ -301,11 +304,12 @@ def main(argv=None):
     # a plan tuned once (talus_mi_tune) and re-applied, as a host would do,
     # so runs are reproducible and match the committed profiles of that plan
     plan_source = "fixed" if args.plan else "tuned"
-    saved = os.path.join(ROOT, "profiles", f"solve_plan_{args.workload}.txt")
+    suffix = f"_s{args.streams}" if args.streams > 1 else ""
+    saved = os.path.join(ROOT, "profiles", f"solve_plan_{args.workload}{suffix}.txt")
     if args.plan is None and not args.retune and not args.no_tune and os.path.exists(saved):
         with open(saved) as f:
             args.plan = f.read().split()[0]
-        plan_source = f"profiles/solve_plan_{args.workload}.txt (tuned earlier, re-applied)"
+        plan_source = f"profiles/solve_plan_{args.workload}{suffix}.txt (tuned earlier, re-applied)"
 
     sc, meshes, desc = workload(args.workload)
     T = sum(int(m["tet_nodes"].shape[0]) for m, _ in meshes)
@@ -317,8 +321,9 @@ def main(argv=None):
 
     log("[bench] mixed batched")
     ctx_m, ids_m, wall_m, reps_m, tuned = batched(api, meshes, api.MIXED, params, args.steps,
-                                                 args.warmup, not args.no_tune, args.plan)
+                                                 args.warmup, not args.no_tune, args.plan, args.streams)
     value = T * sub * args.steps / wall_m
+    plan_steps = "".join(map(str, ctx_m.plan()))
     apps = sum(r["preconditioner_applications"] for r in reps_m) / args.steps
     log(f"[bench] mixed {1e3 * wall_m / args.steps:.3f} ms/step, {value:.4g} tet/s, "
         f"{apps:.1f} preconditioner applications/step, tuned {tuned}")
@@ -329,7 +334,7 @@ def main(argv=None):
     if not args.skip_fp64:
         log("[bench] fp64 batched")
         ctx_d, ids_d, wall_d, reps_d, tuned_d = batched(api, meshes, api.FP64, params, args.steps,
-                                                        args.warmup, not args.no_tune)
+                                                        args.warmup, not args.no_tune, None, args.streams)
         fp64 = {"value": T * sub * args.steps / wall_d, "ms_per_step": 1e3 * wall_d / args.steps,
                 "cg_iterations_per_step": sum(r["cg_iterations"] for r in reps_d) / args.steps,
                 "cg_capped": sum(r["cg_capped"] for r in reps_d)}
@@ -348,18 +353,31 @@ def main(argv=None):
         log(f"[bench] rel force error mixed vs fp64 (worst body): {rel_err:.3e}")
 
     # per-kernel launch durations (HIP events on the context stream) and the
-    # launches of each kernel per step
+    # launches of each kernel per step, on the whole batch in one engine (the
+    # kernels' own efficiency; the timed steps overlap S engines' kernels),
+    # with the one-engine plan the committed profiles were taken with
+    if args.streams > 1:
+        ctx_m.set_streams(1)
+        one = os.path.join(ROOT, "profiles", f"solve_plan_{args.workload}.txt")
+        plan1 = open(one).read().split()[0] if os.path.exists(one) and not args.retune else None
+        if plan1 and len(plan1) == len(ctx_m.plan()):
+            ctx_m.set_plan([int(c) for c in plan1])
+        else:
+            ctx_m.tune(3)
     # with the factor every solve applies the preconditioner once before its
     # first iteration and once per iteration; a step makes 3 solves per pass
+    # (the kernel table below is an estimate of one engine's step)
+    S = args.streams
+    apps1 = apps / S  # preconditioner applications per step of one engine (estimate for S > 1)
     passes = sub
-    iters = max(apps - 3.0 * passes, 0.0)
+    iters = max(apps1 - 3.0 * passes, 0.0)
     kernels = {}
     for name, kind in KINDS.items():
         t = ctx_m.time_kernel(kind, 20)
         if not t["launches"]:
             continue
         per_step = {"assemble": 3 * passes, "op_element": iters + passes, "op_gather": iters + passes}
-        launches = per_step.get(name, apps * t["launches"])
+        launches = per_step.get(name, apps1 * t["launches"])
         kernels[name] = {"avg_ms": t["avg_ms"], "bytes": t["bytes"],
                          "gbs": t["bytes"] / (t["avg_ms"] * 1e-3) / 1e9,
                          "launches_per_step": launches, "ms_per_step": t["avg_ms"] * launches}
@@ -394,11 +412,13 @@ def main(argv=None):
         "config": {
             "workload": desc,
             "bodies": len(meshes), "tets": T, "nodes": N, "substeps": sub, "precision": "mixed",
+            "streams": args.streams,
             "cg_iterations_per_step": sum(r["cg_iterations"] for r in reps_m) / args.steps,
             "cg_capped": sum(r["cg_capped"] for r in reps_m),
             "preconditioner_applications_per_step": apps,
             "refactors": sum(r["refactors"] for r in reps_m),
-            "solve_plan": "".join(map(str, ctx_m.plan())), "solve_plan_source": plan_source,
+            "solve_plan": plan_steps, "solve_plan_source": plan_source,
+            "kernel_table_plan": "".join(map(str, ctx_m.plan())),
             "tune": tuned,
         },
         "factor": finfo,

@@ -321,6 +321,19 @@ int talus_mi_plan_set(talus_mi_ctx *ctx, const uint8_t *plan, uint32_t count);
 /* The context's HIP stream (hipStream_t) for interop. */
 void *talus_mi_stream(talus_mi_ctx *ctx);
 
+/* Splits the context's bodies over `streams` engines (1..TALUS_MI_MAX_STREAMS),
+ * each with its own HIP stream, stepped concurrently by one host thread each
+ * inside talus_mi_step; the state carries over.  No counterpart in the
+ * reference: a batch's preconditioner runs level by level, and with two or
+ * three groups one group's small top levels overlap another group's work.
+ * Bodies go to the group with the fewest tets, largest first.  Reports,
+ * events and torn lists are merged in body order; plans are the groups'
+ * plans one after another; kernel timings average over every group's
+ * launches.  Differentiation (talus_mi_rollout*) needs one stream.
+ * Default: 1. */
+#define TALUS_MI_MAX_STREAMS 4u
+int talus_mi_set_streams(talus_mi_ctx *ctx, uint32_t streams);
+
 #ifdef __cplusplus
 }
 #endif

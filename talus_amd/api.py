@@ -189,6 +189,7 @@ _SIGNATURES = {
                                             _P(C.c_double), _P(C.c_double), _P(C.c_double)]),
     "talus_mi_plan_get": (C.c_int, [C.c_void_p, _P(C.c_uint8), C.c_uint32, _P(C.c_uint32)]),
     "talus_mi_plan_set": (C.c_int, [C.c_void_p, _P(C.c_uint8), C.c_uint32]),
+    "talus_mi_set_streams": (C.c_int, [C.c_void_p, C.c_uint32]),
     "talus_mi_stream": (C.c_void_p, [C.c_void_p]),
 }
 
@@ -461,6 +462,10 @@ class Context:
                                            anchored.ctypes.data_as(C.POINTER(C.c_uint8))))
         return {"tet_nodes": tets.reshape(t, 4), "rest": rest.reshape(n, 3), "mass": mass,
                 "anchored": anchored}
+
+    def set_streams(self, streams: int):
+        """Split the bodies over `streams` concurrently stepped engines."""
+        _check(lib().talus_mi_set_streams(self._h, streams))
 
     def step(self, params: StepParams | None = None, **kw) -> dict:
         p = params if params is not None else default_params(**kw)

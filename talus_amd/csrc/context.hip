@@ -15,12 +15,14 @@
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
+#include <exception>
 #include <functional>
 #include <cmath>
 #include <cstring>
 #include <memory>
 #include <stdexcept>
 #include <string>
+#include <thread>
 #include <vector>
 
 #include "factor.h"
@@ -1503,6 +1505,263 @@ void Engine<Real>::factor_info(talus_mi_factor_info &info) {
   info.factor_seconds = factor_seconds_;
 }
 
+
+// Bodies split over several engines, each with its own HIP stream, stepped
+// concurrently from one host thread per engine.  A batch's PCG runs in
+// lockstep through the preconditioner's elimination levels, and the levels
+// near the roots launch few workgroups; with two or three groups one group's
+// small levels run beside another group's large ones.  Bodies go to the
+// group with the fewest tets so far, largest body first (ties to the lower
+// group), and keep their order inside a group.  Every per-body result is
+// deterministic; results can differ from a one-group context in the last
+// bits because each group's solve plan follows its own level structure.
+class GroupedEngine final : public EngineBase {
+ public:
+  using Factory = std::function<std::unique_ptr<EngineBase>(hipStream_t)>;
+  GroupedEngine(int device, uint32_t groups, hipStream_t first, const Factory &make) : device_(device) {
+    streams_.push_back(first);
+    for (uint32_t g = 1; g < groups; ++g) {
+      hipStream_t s = nullptr;
+      HIPCHECK(hipStreamCreateWithFlags(&s, hipStreamNonBlocking));
+      streams_.push_back(s);
+      owned_streams_.push_back(s);
+    }
+    for (uint32_t g = 0; g < groups; ++g) eng_.push_back(make(streams_[g]));
+    sub_.resize(groups);
+  }
+  ~GroupedEngine() override {
+    eng_.clear();
+    for (hipStream_t s : owned_streams_) (void)hipStreamDestroy(s);
+  }
+
+  void build(const std::vector<Body> &bodies) override {
+    const uint32_t G = uint32_t(eng_.size()), B = uint32_t(bodies.size());
+    std::vector<uint32_t> order(B);
+    for (uint32_t b = 0; b < B; ++b) order[b] = b;
+    std::stable_sort(order.begin(), order.end(),
+                     [&](uint32_t a, uint32_t c) { return bodies[a].tets > bodies[c].tets; });
+    std::vector<uint64_t> load(G, 0);
+    group_.assign(B, 0);
+    for (uint32_t b : order) {
+      const uint32_t g = uint32_t(std::min_element(load.begin(), load.end()) - load.begin());
+      group_[b] = g;
+      load[g] += bodies[b].tets;
+    }
+    members_.assign(G, {});
+    local_.assign(B, 0);
+    tet0_.assign(B, 0);
+    for (uint32_t b = 0; b < B; ++b) {
+      local_[b] = uint32_t(members_[group_[b]].size());
+      members_[group_[b]].push_back(b);
+      tet0_[b] = b ? tet0_[b - 1] + bodies[b - 1].tets : 0;
+    }
+    tets_ = B ? tet0_[B - 1] + bodies[B - 1].tets : 0;
+    for (uint32_t g = 0; g < G; ++g) {
+      sub_[g].clear();
+      for (uint32_t b : members_[g]) sub_[g].push_back(bodies[b]);
+      eng_[g]->build(sub_[g]);
+    }
+  }
+  void download(std::vector<Body> &bodies) override {
+    for (uint32_t g = 0; g < eng_.size(); ++g) {
+      if (sub_[g].empty()) continue;
+      eng_[g]->download(sub_[g]);
+      for (uint32_t l = 0; l < members_[g].size(); ++l) {
+        const uint32_t b = members_[g][l];
+        if (b >= bodies.size()) continue;
+        Body &dst = bodies[b];
+        const Body &src = sub_[g][l];
+        if (dst.nodes != src.nodes || dst.tets != src.tets) continue;
+        dst.x = src.x;
+        dst.v = src.v;
+        dst.a = src.a;
+        dst.R = src.R;
+        dst.W = src.W;
+        dst.plastic = src.plastic;
+      }
+    }
+  }
+  void step(const talus_mi_step_params &sp, talus_mi_report &rep, std::vector<talus_mi_event> &events,
+            uint32_t step_index, std::vector<uint8_t> &torn) override {
+    const uint32_t G = uint32_t(eng_.size());
+    std::vector<talus_mi_report> reps(G);
+    std::vector<std::vector<talus_mi_event>> evs(G);
+    std::vector<std::vector<uint8_t>> torns(G);
+    std::vector<std::exception_ptr> errs(G);
+    // graph capture is stream-ordered state of the runtime; with several
+    // threads capturing at once it failed, so the groups launch directly
+    talus_mi_step_params direct = sp;
+    direct.flags |= TALUS_MI_DIRECT_LAUNCH;
+    auto work = [&](uint32_t g) {
+      try {
+        std::memset(&reps[g], 0, sizeof reps[g]);
+        if (sub_[g].empty()) return;
+        HIPCHECK(hipSetDevice(device_));
+        eng_[g]->step(direct, reps[g], evs[g], step_index, torns[g]);
+      } catch (...) {
+        errs[g] = std::current_exception();
+      }
+    };
+    std::vector<std::thread> threads;
+    for (uint32_t g = 1; g < G; ++g) threads.emplace_back(work, g);
+    work(0);
+    for (std::thread &t : threads) t.join();
+    for (const std::exception_ptr &e : errs)
+      if (e) std::rethrow_exception(e);
+    std::memset(&rep, 0, sizeof rep);
+    for (const talus_mi_report &r : reps) {
+      rep.bodies += r.bodies;
+      rep.nodes += r.nodes;
+      rep.elements += r.elements;
+      rep.substeps = std::max(rep.substeps, r.substeps);
+      rep.cg_iterations += r.cg_iterations;
+      rep.cg_capped += r.cg_capped;
+      rep.solves += r.solves;
+      rep.refactors += r.refactors;
+      rep.cg_worst_relative = std::max(rep.cg_worst_relative, r.cg_worst_relative);
+      rep.elastic_energy += r.elastic_energy;
+      rep.kinetic_energy += r.kinetic_energy;
+      rep.plastic_growth |= r.plastic_growth;
+      rep.yield_events += r.yield_events;
+      rep.tear_events += r.tear_events;
+      rep.non_finite |= r.non_finite;
+      rep.preconditioner_applications += r.preconditioner_applications;
+      rep.device_seconds = std::max(rep.device_seconds, r.device_seconds);
+    }
+    // events in the one-engine order: by body, then tet, yield before tear
+    const size_t first = events.size();
+    for (uint32_t g = 0; g < G; ++g)
+      for (talus_mi_event e : evs[g]) {
+        e.body = members_[g][e.body];
+        events.push_back(e);
+      }
+    std::stable_sort(events.begin() + first, events.end(),
+                     [](const talus_mi_event &a, const talus_mi_event &c) { return a.body < c.body; });
+    torn.assign(tets_, 0);
+    for (uint32_t g = 0; g < G; ++g) {
+      size_t off = 0;
+      for (uint32_t b : members_[g]) {
+        const uint32_t n = sub_[g][local_[b]].tets;
+        if (torns[g].size() >= off + n)
+          std::copy(torns[g].begin() + off, torns[g].begin() + off + n, torn.begin() + tet0_[b]);
+        off += n;
+      }
+    }
+  }
+  void state_set(uint32_t b, const double *x, const double *v, const double *a) override {
+    eng_[group_[b]]->state_set(local_[b], x, v, a);
+  }
+  void state_get(uint32_t b, double *x, double *v, double *a) override {
+    eng_[group_[b]]->state_get(local_[b], x, v, a);
+  }
+  void rotations_get(uint32_t b, double *R) override { eng_[group_[b]]->rotations_get(local_[b], R); }
+  void rotations_set(uint32_t b, const double *R) override { eng_[group_[b]]->rotations_set(local_[b], R); }
+  void frames_get(uint32_t b, double *W) override { eng_[group_[b]]->frames_get(local_[b], W); }
+  void frames_set(uint32_t b, const double *W) override { eng_[group_[b]]->frames_set(local_[b], W); }
+  void plastic_get(uint32_t b, double *p) override { eng_[group_[b]]->plastic_get(local_[b], p); }
+  void plastic_set(uint32_t b, const double *p) override { eng_[group_[b]]->plastic_set(local_[b], p); }
+  void eval_forces(uint32_t b, const double *x, double *tf, double *te, double *nf, double *R) override {
+    eng_[group_[b]]->eval_forces(local_[b], x, tf, te, nf, R);
+  }
+  void apply_system(uint32_t b, double inertia, double step, const double *dv, double *y) override {
+    eng_[group_[b]]->apply_system(local_[b], inertia, step, dv, y);
+  }
+  void factor_solve(uint32_t b, double inertia, double step, const double *r, double *z) override {
+    eng_[group_[b]]->factor_solve(local_[b], inertia, step, r, z);
+  }
+  // per launch over every group's launches of the kind
+  void time_kernel(uint32_t kind, uint32_t repeats, talus_mi_kernel_timing &out) override {
+    std::memset(&out, 0, sizeof out);
+    double time = 0.0, bytes = 0.0;
+    for (uint32_t g = 0; g < eng_.size(); ++g) {
+      if (sub_[g].empty()) continue;
+      talus_mi_kernel_timing t;
+      eng_[g]->time_kernel(kind, repeats, t);
+      time += t.avg_ms * t.launches;
+      bytes += t.bytes * t.launches;
+      out.launches += t.launches;
+    }
+    if (out.launches) {
+      out.avg_ms = time / out.launches;
+      out.bytes = bytes / out.launches;
+    }
+  }
+  void factor_info(talus_mi_factor_info &info) override {
+    std::memset(&info, 0, sizeof info);
+    info.ok = 1;
+    for (uint32_t g = 0; g < eng_.size(); ++g) {
+      if (sub_[g].empty()) continue;
+      talus_mi_factor_info i;
+      eng_[g]->factor_info(i);
+      info.fronts += i.fronts;
+      info.levels = std::max(info.levels, i.levels);
+      info.ok &= i.ok;
+      info.entries += i.entries;
+      info.flops += i.flops;
+      info.factor_seconds += i.factor_seconds;
+    }
+  }
+  void tune(uint32_t repeats, talus_mi_tune_report &report) override {
+    std::memset(&report, 0, sizeof report);
+    for (uint32_t g = 0; g < eng_.size(); ++g) {
+      if (sub_[g].empty()) continue;
+      talus_mi_tune_report r;
+      eng_[g]->tune(repeats, r);
+      report.levels += r.levels;
+      report.lanes_forward += r.lanes_forward;
+      report.lanes_backward += r.lanes_backward;
+      report.before_seconds += r.before_seconds;
+      report.after_seconds += r.after_seconds;
+    }
+  }
+  void state_all_get(std::vector<double> &) override { unsupported(); }
+  void state_all_set(const std::vector<double> &) override { unsupported(); }
+  void rollout_backward(const talus_mi_step_params &, const std::vector<std::vector<double>> &,
+                        const std::vector<double> &, uint32_t, uint32_t, uint32_t, const double *,
+                        const double *, double, double *, double *, double *, double *, double *,
+                        double *) override {
+    unsupported();
+  }
+  // the groups' plans one after another
+  std::vector<uint8_t> plan() override {
+    std::vector<uint8_t> out;
+    for (uint32_t g = 0; g < eng_.size(); ++g) {
+      if (sub_[g].empty()) continue;
+      const std::vector<uint8_t> p = eng_[g]->plan();
+      out.insert(out.end(), p.begin(), p.end());
+    }
+    return out;
+  }
+  bool set_plan(const std::vector<uint8_t> &plan) override {
+    std::vector<std::vector<uint8_t>> parts(eng_.size());
+    size_t at = 0;
+    for (uint32_t g = 0; g < eng_.size(); ++g) {
+      if (sub_[g].empty()) continue;
+      const size_t n = eng_[g]->plan().size();
+      if (at + n > plan.size()) return false;
+      parts[g].assign(plan.begin() + at, plan.begin() + at + n);
+      at += n;
+    }
+    if (at != plan.size()) return false;
+    for (size_t g = 0; g < eng_.size(); ++g)
+      if (!sub_[g].empty() && !eng_[g]->set_plan(parts[g])) return false;
+    return true;
+  }
+
+ private:
+  [[noreturn]] static void unsupported() {
+    throw std::runtime_error("differentiation needs a context with one stream (talus_mi_set_streams)");
+  }
+  int device_;
+  std::vector<hipStream_t> streams_, owned_streams_;
+  std::vector<std::unique_ptr<EngineBase>> eng_;
+  std::vector<std::vector<Body>> sub_;
+  std::vector<std::vector<uint32_t>> members_;
+  std::vector<uint32_t> group_, local_;
+  std::vector<size_t> tet0_;
+  size_t tets_ = 0;
+};
+
 }  // namespace tmi
 
 // ------------------------------------------------------------------ C ABI
@@ -1523,11 +1782,22 @@ struct talus_mi_ctx {
   talus_mi_step_params ckpt_params{};
   uint32_t steps = 0;
   std::vector<talus_mi_event> events;
+  uint32_t streams = 1;  // engines stepped concurrently (talus_mi_set_streams)
 };
 
 namespace {
 
 using tmi::fail;
+
+// one engine on the context's stream, or bodies split over `streams` engines
+std::unique_ptr<tmi::EngineBase> make_engine(talus_mi_ctx *ctx) {
+  auto one = [precision = ctx->precision](hipStream_t s) -> std::unique_ptr<tmi::EngineBase> {
+    if (precision == TALUS_MI_MIXED) return std::make_unique<tmi::Engine<float>>(s);
+    return std::make_unique<tmi::Engine<double>>(s);
+  };
+  if (ctx->streams <= 1) return one(ctx->stream);
+  return std::make_unique<tmi::GroupedEngine>(ctx->device, ctx->streams, ctx->stream, one);
+}
 
 void ensure(talus_mi_ctx *ctx) {
   if (!ctx->dirty) return;
@@ -1589,10 +1859,7 @@ int talus_mi_create(talus_mi_ctx **out, int device, uint32_t precision) {
   try {
     HIPCHECK(hipSetDevice(device));
     HIPCHECK(hipStreamCreateWithFlags(&ctx->stream, hipStreamNonBlocking));
-    if (precision == TALUS_MI_MIXED)
-      ctx->engine = std::make_unique<tmi::Engine<float>>(ctx->stream);
-    else
-      ctx->engine = std::make_unique<tmi::Engine<double>>(ctx->stream);
+    ctx->engine = make_engine(ctx.get());
   } catch (const std::exception &e) {
     return fail(TALUS_MI_ERR_DEVICE, e.what());
   }
@@ -1956,6 +2223,21 @@ int talus_mi_plan_set(talus_mi_ctx *ctx, const uint8_t *plan, uint32_t count) {
 }
 
 void *talus_mi_stream(talus_mi_ctx *ctx) { return ctx ? (void *)ctx->stream : nullptr; }
+
+int talus_mi_set_streams(talus_mi_ctx *ctx, uint32_t streams) {
+  if (streams < 1 || streams > TALUS_MI_MAX_STREAMS)
+    return fail(TALUS_MI_ERR_ARGUMENT, "streams must be 1.." + std::to_string(TALUS_MI_MAX_STREAMS));
+  return guarded(ctx, [&]() -> int {
+    if (streams == ctx->streams) return TALUS_MI_OK;
+    if (!ctx->dirty) ctx->engine->download(ctx->bodies);  // keep the state of the current engine
+    ctx->engine.reset();
+    ctx->streams = streams;
+    ctx->engine = make_engine(ctx);
+    ctx->dirty = true;
+    ++ctx->version;
+    return TALUS_MI_OK;
+  });
+}
 
 }  // extern "C"
 

@@ -215,3 +215,28 @@ def test_partial_refresh_keeps_batched_equal_to_single_bodies():
         for i, x in zip(ids, singles):
             assert np.array_equal(ctx.state(i)[0], x)
     assert refactors > 1  # the first factorization and at least one partial refresh
+
+
+def test_streams_split_the_batch_without_changing_any_body():
+    """Bodies split over three concurrently stepped engines give every body
+    the same state, events and report sums as one engine, with the same
+    solve-kernel shape on every level (lanes)."""
+    sc = scenes.bodies(count=7, seed=21, lo=300, hi=4000)
+    params = api.default_params(substeps=2, preconditioner=api.FACTOR, flags=api.RECORD_EVENTS)
+    runs = []
+    for streams in (1, 3):
+        with api.Context(0, api.MIXED) as ctx:
+            ids = [product_body(ctx, body)[0] for body in sc.bodies]
+            ctx.set_streams(streams)
+            ctx.set_plan(np.ones(ctx.plan().size, dtype=np.uint8))
+            reps = [ctx.step(params) for _ in range(3)]
+            runs.append(([ctx.state(b) for b in ids], reps, ctx.events(), ctx.factor_info()))
+    (s1, r1, e1, f1), (s3, r3, e3, f3) = runs
+    for a, b in zip(s1, s3):
+        for u, w in zip(a, b):
+            assert np.array_equal(u, w)
+    for a, b in zip(r1, r3):
+        for key in ("cg_iterations", "cg_capped", "solves", "yield_events", "bodies", "nodes", "elements"):
+            assert a[key] == b[key], key
+    assert e1 == e3
+    assert f1["fronts"] == f3["fronts"] and f1["entries"] == f3["entries"]
