@@ -301,6 +301,8 @@ def main(argv=None):
 
     from talus_amd import api
 
+    sc, meshes, desc = workload(args.workload)
+    args.streams = max(1, min(args.streams, len(meshes)))
     # a plan tuned once (talus_mi_tune) and re-applied, as a host would do,
     # so runs are reproducible and match the committed profiles of that plan
     plan_source = "fixed" if args.plan else "tuned"
@@ -311,7 +313,6 @@ def main(argv=None):
             args.plan = f.read().split()[0]
         plan_source = f"profiles/solve_plan_{args.workload}{suffix}.txt (tuned earlier, re-applied)"
 
-    sc, meshes, desc = workload(args.workload)
     T = sum(int(m["tet_nodes"].shape[0]) for m, _ in meshes)
     N = sum(int(m["anchored"].shape[0]) for m, _ in meshes)
     sub = sc.substeps

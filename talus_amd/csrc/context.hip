@@ -302,7 +302,8 @@ class Engine final : public EngineBase {
   uint64_t *d_ea_ = nullptr, *d_trsm_ = nullptr, *d_syrk_ = nullptr, *d_trtri_ = nullptr,
            *d_w21_ = nullptr, *d_pack_ = nullptr, *d_fwd_ = nullptr, *d_bwd_ = nullptr,
            *d_fwl_ = nullptr, *d_bwl_ = nullptr, *d_fwr_ = nullptr, *d_bwr_ = nullptr,
-           *d_fws_ = nullptr, *d_bws_ = nullptr, *d_zero_ = nullptr;
+           *d_fws_ = nullptr, *d_bws_ = nullptr, *d_zero_ = nullptr, *d_gat_all_ = nullptr,
+           *d_gat_big_ = nullptr, *d_xg_all_ = nullptr, *d_xg_big_ = nullptr;
   using GT = typename GStore<Real>::type;  // storage of the solve matrix G
   GT *d_gr_ = nullptr, *d_gc_ = nullptr;
   double *d_body_reach_ = nullptr, *d_body_inertia_ = nullptr;
@@ -585,6 +586,10 @@ void Engine<Real>::analyze_factor() {
   d_bwr_ = upload(t.bwr_tasks);
   d_fws_ = upload(t.fws_tasks);
   d_bws_ = upload(t.bws_tasks);
+  d_gat_all_ = upload(t.gat_all);
+  d_gat_big_ = upload(t.gat_big);
+  d_xg_all_ = upload(t.xg_all);
+  d_xg_big_ = upload(t.xg_big);
   plan_ = t.plan;
   ++graph_epoch_;
   d_gr_ = dev<GT>(t.g_pool);
@@ -669,12 +674,17 @@ void Engine<Real>::forward_level(uint32_t l, int mode, int parity) {
   else if (mode == 2)
     SolveLaunch<Real>::forward_split(fd_, d_gc_, d_fws_ + t.fws_start[l], t.fws_start[l + 1] - t.fws_start[l], d_.r,
                                      d_.done, parity, d_.B, stream_);
-  if (mode)
+  if (mode) {
+    SolveLaunch<Real>::gather(fd_, d_gat_big_ + t.gat_big_start[l], t.gat_big_start[l + 1] - t.gat_big_start[l],
+                              d_.r, d_.done, parity, d_.B, stream_);
     SolveLaunch<Real>::forward(fd_, d_gr_, d_fwr_ + t.fwr_start[l], t.fwr_start[l + 1] - t.fwr_start[l], d_.r,
                                d_.done, parity, d_.B, stream_);
-  else
+  } else {
+    SolveLaunch<Real>::gather(fd_, d_gat_all_ + t.gat_all_start[l], t.gat_all_start[l + 1] - t.gat_all_start[l],
+                              d_.r, d_.done, parity, d_.B, stream_);
     SolveLaunch<Real>::forward(fd_, d_gr_, d_fwd_ + t.fwd_start[l], t.fwd_start[l + 1] - t.fwd_start[l], d_.r,
                                d_.done, parity, d_.B, stream_);
+  }
 }
 
 template <class Real>
@@ -686,12 +696,17 @@ void Engine<Real>::backward_level(uint32_t l, int mode, int parity) {
   else if (mode == 2)
     SolveLaunch<Real>::backward_split(fd_, d_gr_, d_bws_ + t.bws_start[l], t.bws_start[l + 1] - t.bws_start[l],
                                       d_.done, parity, d_.B, stream_);
-  if (mode)
+  if (mode) {
+    SolveLaunch<Real>::xgather(fd_, d_xg_big_ + t.xg_big_start[l], t.xg_big_start[l + 1] - t.xg_big_start[l],
+                               d_.done, parity, d_.B, stream_);
     SolveLaunch<Real>::backward(fd_, d_gc_, d_bwr_ + t.bwr_start[l], t.bwr_start[l + 1] - t.bwr_start[l],
                                 d_.done, parity, d_.B, stream_);
-  else
+  } else {
+    SolveLaunch<Real>::xgather(fd_, d_xg_all_ + t.xg_all_start[l], t.xg_all_start[l + 1] - t.xg_all_start[l],
+                               d_.done, parity, d_.B, stream_);
     SolveLaunch<Real>::backward(fd_, d_gc_, d_bwd_ + t.bwd_start[l], t.bwd_start[l + 1] - t.bwd_start[l],
                                 d_.done, parity, d_.B, stream_);
+  }
 }
 
 // z = (LL^T)^-1 r for the bodies still iterating (applyFactor,

@@ -52,8 +52,8 @@ struct GStore<float> {
 };
 
 constexpr int FNB = 64;   // panel / tile width in DOF
-constexpr uint32_t ROWS_GROUP = 32;  // rows-shape solve workgroup: 4 wavefronts x 8 rows/columns
-constexpr int SCH = 2048; // vector entries staged in LDS per chunk
+constexpr uint32_t ROWS_GROUP = 8;  // rows-shape solve workgroup: 8 rows/columns, 4 wavefronts along the other dimension
+constexpr uint32_t GCHUNK = 256;     // front rows per gather workgroup
 constexpr uint32_t ZCHUNK = 65536;  // entries zeroed per workgroup before a factorization
 constexpr int KLANES = 512;   // fronts with k <= KLANES solve one row/column per lane
 constexpr int MLANES = 2048;  // ... backward also needs m <= MLANES (LDS)
@@ -130,6 +130,12 @@ struct FactorTables {
   std::vector<uint64_t> zero_tasks;  // front | chunk<<32 | array<<48, ZCHUNK entries per chunk
   std::vector<uint64_t> fws_tasks, bws_tasks;                // small fronts, 64 rows/columns per workgroup
   std::vector<uint32_t> fws_start, bws_start;
+  // the rows-shape kernels read a front vector gathered once per front
+  // (forward: r and the children's updates; backward: x2 at the update
+  // rows): front | chunk<<32, GCHUNK rows per chunk, for every front of a
+  // level (plan 0) or the fronts too large for the lane shapes (fwr/bwr)
+  std::vector<uint64_t> gat_all, gat_big, xg_all, xg_big;
+  std::vector<uint32_t> gat_all_start, gat_big_start, xg_all_start, xg_big_start;
   // default solve plan, 2 per level (forward, backward): 0 = rows-per-
   // wavefront kernel throughout; for the small fronts of the level 1 = one
   // row/column per lane, 256 per workgroup; 2 = 64 rows/columns per
@@ -199,6 +205,12 @@ struct SolveLaunch {
   // every body whose convergence flag (done[parity*B + body]) is clear;
   // parity < 0 applies to every body.  The forward kernels gather their
   // fronts' right-hand side (r and the children's updates) themselves.
+  // front vectors of the rows shape: gather (r and the children's updates)
+  // before the forward pass, xgather (x2) before the backward pass
+  static void gather(const FactorDev &f, const uint64_t *tasks, uint32_t count, const Real *rin,
+                     const int *done, int parity, uint32_t B, hipStream_t s);
+  static void xgather(const FactorDev &f, const uint64_t *tasks, uint32_t count, const int *done,
+                      int parity, uint32_t B, hipStream_t s);
   static void forward(const FactorDev &f, const G *gr, const uint64_t *tasks, uint32_t count,
                       const Real *rin, const int *done, int parity, uint32_t B, hipStream_t s);
   static void forward_lanes(const FactorDev &f, const G *gc, const uint64_t *tasks,

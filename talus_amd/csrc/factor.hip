@@ -20,6 +20,7 @@ namespace {
 typedef double d4 __attribute__((ext_vector_type(4)));
 
 constexpr int NT = 256;  // threads per workgroup
+static_assert(GCHUNK == NT, "one gather thread per front row of a chunk");
 constexpr int LDP = FNB + 1;
 
 __device__ __forceinline__ uint32_t lo32(uint64_t t) { return uint32_t(t & 0xffffffffu); }
@@ -368,6 +369,32 @@ __device__ __forceinline__ double gather_row(const FactorDev &f, uint32_t fr, ui
   return v;
 }
 
+// the rows shape's front vectors, gathered once per front: v[i] for the
+// chunk's rows (forward), v[i] = x2 at the update rows (backward)
+template <class Real>
+__global__ __launch_bounds__(NT) void k_solve_gather(FactorDev f, const uint64_t *tasks, const Real *rin,
+                                                     const int *done, int parity, uint32_t B) {
+  const uint64_t t = tasks[blockIdx.x];
+  const uint32_t fr = lo32(t), chunk = mid16(t);
+  if (skip(done, parity, B, f.body[fr])) return;
+  const uint32_t k = f.k[fr], m = f.m[fr];
+  const uint32_t i = chunk * GCHUNK + threadIdx.x;
+  if (i >= m) return;
+  f.vec[f.vec_off[fr] + i] = gather_row(f, fr, i, k, rin);
+}
+
+__global__ __launch_bounds__(NT) void k_solve_xgather(FactorDev f, const uint64_t *tasks, const int *done,
+                                                      int parity, uint32_t B) {
+  const uint64_t t = tasks[blockIdx.x];
+  const uint32_t fr = lo32(t), chunk = mid16(t);
+  if (skip(done, parity, B, f.body[fr])) return;
+  const uint32_t k = f.k[fr], m = f.m[fr];
+  const uint32_t i = k + chunk * GCHUNK + threadIdx.x;
+  if (i >= m) return;
+  const uint32_t *rows = f.rows + f.rows_off[fr];
+  f.vec[f.vec_off[fr] + i] = f.xsol[3 * size_t(rows[i / 3]) + i % 3];
+}
+
 __device__ __forceinline__ double wave_sum64(double v) {
 #pragma unroll
   for (int off = 32; off > 0; off >>= 1) v += __shfl_down(v, off, 64);
@@ -525,82 +552,82 @@ __global__ __launch_bounds__(NT) void k_pack_m11(FactorDev f, const uint64_t *ta
 }
 
 // Rows kernels (fronts too large for the lane shapes, and the plan's
-// per-row choice): a workgroup takes 32 consecutive rows (forward) or
-// columns (backward), 8 per wavefront, and the wavefront's lanes run along
-// the other dimension, each loading the 16-byte tile that holds its column's
-// (row's) entries for all of the wavefront's rows (columns); 8 FP64
-// wavefront reductions at the end.
-constexpr uint32_t ROWS_PER_WAVE = 8;
-static_assert(ROWS_GROUP == (NT / 64) * ROWS_PER_WAVE, "rows tasks cover 4 wavefronts of 8");
+// rows-everywhere choice): a workgroup takes RW = 8 rows (forward) or
+// columns (backward) of G, i.e. 8 / W tiles, its four wavefronts split the
+// other dimension in contiguous quarters, and each lane runs along its
+// quarter loading the 16-byte tiles that hold its column's (row's) entries
+// for the 8 rows (columns); 8 FP64 wavefront reductions, then the four
+// wavefronts' sums added in LDS in wavefront order.  The front vector comes
+// from the gather kernels (k_solve_gather / k_solve_xgather), read where it
+// lies.
+constexpr uint32_t RW = ROWS_GROUP;
+static_assert(RW % 8 == 0, "whole tiles of every storage type");
 
-// forward, rows shape: y[i] (i < k) or v[i] - . (i >= k) for the 32 rows,
-// row-tiled copy, the vector staged in LDS chunks shared by the workgroup
+// forward, rows shape: y[i] (i < k) or v[i] - . (i >= k) for the group's
+// rows, row-tiled copy
 template <class Real>
 __global__ __launch_bounds__(NT) void k_solve_forward(FactorDev f, const typename GStore<Real>::type *gr,
                                                       const uint64_t *tasks, const Real *rin,
                                                       const int *done, int parity, uint32_t B) {
   using G = typename GStore<Real>::type;
-  constexpr uint32_t W = Tile<G>::W, RW = ROWS_PER_WAVE;
-  __shared__ double vs[SCH];
+  constexpr uint32_t W = Tile<G>::W, NTL = RW / W;
+  (void)rin;  // the front vector was gathered by k_solve_gather
+  __shared__ double red[NT / 64][RW];
   const uint64_t t = tasks[blockIdx.x];
-  const uint32_t fr = lo32(t), grp = mid16(t), gs = hi16(t);
+  const uint32_t fr = lo32(t), grp = mid16(t);
   if (skip(done, parity, B, f.body[fr])) return;
   const uint32_t k = f.k[fr], m = f.m[fr];
   const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
-  const uint32_t g0 = grp * gs;
-  const uint32_t glast = min(g0 + gs, m) - 1;
-  const uint32_t nmax = glast < k ? glast + 1 : k;  // columns any row of the group needs
-  const uint32_t i0 = g0 + wave * RW;               // first row of this wavefront (tile aligned)
-  const uint32_t wlast = min(i0 + RW, m) - 1;
-  const uint32_t nw = i0 < m ? (wlast < k ? wlast + 1 : k) : 0;
+  const uint32_t i0 = grp * RW;  // the group's first row (tile aligned)
+  const uint32_t ilast = min(i0 + RW, m) - 1;
+  const uint32_t n = ilast < k ? ilast + 1 : k;  // columns any row of the group needs
+  const uint32_t quarter = (n + 3u) / 4u;
+  const uint32_t j0 = min(n, wave * quarter), j1 = min(n, j0 + quarter);
   double *v = f.vec + f.vec_off[fr];
   const double *sc = f.gscale + f.yoff[fr];
-  const G *g = gr + f.g_off[fr] + uint64_t(i0 / W) * k * W;  // the wavefront's first row tile
-  const uint64_t ts = uint64_t(k) * W;                        // between row tiles
+  const G *g = gr + f.g_off[fr] + uint64_t(i0 / W) * k * W;
+  const uint64_t ts = uint64_t(k) * W;  // between row tiles
   double acc[RW];
 #pragma unroll
-  for (uint32_t r = 0; r < RW; ++r) acc[r] = 0.0;
-  for (uint32_t c0 = 0; c0 < nmax; c0 += SCH) {
-    const uint32_t cn = min(uint32_t(SCH), nmax - c0);
-    __syncthreads();
-    for (uint32_t q = threadIdx.x; q < cn; q += NT) vs[q] = gather_row(f, fr, c0 + q, k, rin) * sc[c0 + q];
-    __syncthreads();
-    const uint32_t end = min(nw, c0 + cn);
-    uint32_t j = c0 + lane;
-    for (; j + 64 < end; j += 128) {
-      Chunk<G> c[2][RW / W];
+  for (uint32_t q = 0; q < RW; ++q) acc[q] = 0.0;
+  uint32_t j = j0 + lane;
+  for (; j + 64 < j1; j += 128) {  // 2 x 8/W tiles per lane in flight
+    Chunk<G> c[2][NTL];
+    double w[2];
 #pragma unroll
-      for (uint32_t a = 0; a < RW / W; ++a) {
-        c[0][a] = load_chunk(g + a * ts + uint64_t(j) * W);
-        c[1][a] = load_chunk(g + a * ts + uint64_t(j + 64) * W);
-      }
-      const double w0 = vs[j - c0], w1 = vs[j + 64 - c0];
+    for (uint32_t u = 0; u < 2; ++u) {
 #pragma unroll
-      for (uint32_t a = 0; a < RW / W; ++a)
-#pragma unroll
-        for (uint32_t q = 0; q < W; ++q) acc[a * W + q] += gval(c[0][a].v[q]) * w0 + gval(c[1][a].v[q]) * w1;
+      for (uint32_t a = 0; a < NTL; ++a) c[u][a] = load_chunk(g + a * ts + uint64_t(j + 64 * u) * W);
+      w[u] = v[j + 64 * u] * sc[j + 64 * u];
     }
-    for (; j < end; j += 64) {
-      const double w0 = vs[j - c0];
 #pragma unroll
-      for (uint32_t a = 0; a < RW / W; ++a) {
-        const Chunk<G> c = load_chunk(g + a * ts + uint64_t(j) * W);
+    for (uint32_t a = 0; a < NTL; ++a)
 #pragma unroll
-        for (uint32_t q = 0; q < W; ++q) acc[a * W + q] += gval(c.v[q]) * w0;
-      }
+      for (uint32_t q = 0; q < W; ++q) acc[a * W + q] += gval(c[0][a].v[q]) * w[0] + gval(c[1][a].v[q]) * w[1];
+  }
+  for (; j < j1; j += 64) {
+    const double w = v[j] * sc[j];
+#pragma unroll
+    for (uint32_t a = 0; a < NTL; ++a) {
+      const Chunk<G> c = load_chunk(g + a * ts + uint64_t(j) * W);
+#pragma unroll
+      for (uint32_t q = 0; q < W; ++q) acc[a * W + q] += gval(c.v[q]) * w;
     }
   }
 #pragma unroll
-  for (uint32_t r = 0; r < RW; ++r) {
-    const double tot = wave_sum64(acc[r]);
-    const uint32_t i = i0 + r;
-    if (lane == 0 && i < m) {
-      if (i < k)
-        f.y[f.yoff[fr] + i] = tot;
-      else
-        v[i] = gather_row(f, fr, i, k, rin) - tot;
-    }
+  for (uint32_t q = 0; q < RW; ++q) {
+    const double s = wave_sum64(acc[q]);
+    if (lane == 0) red[wave][q] = s;
   }
+  __syncthreads();
+  if (threadIdx.x >= RW) return;
+  const uint32_t i = i0 + threadIdx.x;
+  if (i >= m) return;
+  const double tot = ((red[0][threadIdx.x] + red[1][threadIdx.x]) + red[2][threadIdx.x]) + red[3][threadIdx.x];
+  if (i < k)
+    f.y[f.yoff[fr] + i] = tot;
+  else
+    v[i] -= tot;
 }
 
 // forward for fronts with k <= KLANES: one lane per row of G, along the
@@ -748,70 +775,68 @@ __global__ __launch_bounds__(NT) void k_solve_backward_split(FactorDev f, const 
 }
 
 // backward, rows shape: x[col j] = sum_{j<=i<k} G[i][j] y[i]
-// - sum_{i>=k} G[i][j] x2[i] for the 32 columns, column-tiled copy, lanes
-// along the rows, w = [y; -x2] staged in LDS chunks (x2 from the ancestors'
-// solution at the front's update rows)
+// - sum_{i>=k} G[i][j] x2[i] for the group's columns, column-tiled copy,
+// lanes along the rows from the group's first column on (x2 gathered by
+// k_solve_xgather)
 template <class Real>
 __global__ __launch_bounds__(NT) void k_solve_backward(FactorDev f, const typename GStore<Real>::type *gc,
                                                        const uint64_t *tasks, const int *done,
                                                        int parity, uint32_t B) {
   using G = typename GStore<Real>::type;
-  constexpr uint32_t W = Tile<G>::W, CW = ROWS_PER_WAVE;
-  __shared__ double ws[SCH];
+  constexpr uint32_t W = Tile<G>::W, NTL = RW / W;
+  __shared__ double red[NT / 64][RW];
   const uint64_t t = tasks[blockIdx.x];
-  const uint32_t fr = lo32(t), grp = mid16(t), gs = hi16(t);
+  const uint32_t fr = lo32(t), grp = mid16(t);
   if (skip(done, parity, B, f.body[fr])) return;
   const uint32_t k = f.k[fr], m = f.m[fr];
   const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
-  const uint32_t g0 = grp * gs;                // first column of the group (tile aligned)
-  const uint32_t j0 = g0 + wave * CW;          // first column of this wavefront
+  const uint32_t j0 = grp * RW;  // the group's first column (tile aligned)
+  const uint32_t rows = m - j0, quarter = (rows + 3u) / 4u;
+  const uint32_t r0 = j0 + min(rows, wave * quarter), r1 = j0 + min(rows, wave * quarter + quarter);
   const double *y = f.y + f.yoff[fr];
-  const uint32_t *rows = f.rows + f.rows_off[fr];
-  const G *g = gc + f.g_off[fr] + uint64_t(j0 / W) * m * W;  // the wavefront's first column tile
-  const uint64_t ts = uint64_t(m) * W;                        // between column tiles
-  double acc[CW];
+  const double *v = f.vec + f.vec_off[fr];
+  const G *g = gc + f.g_off[fr] + uint64_t(j0 / W) * m * W;
+  const uint64_t ts = uint64_t(m) * W;  // between column tiles
+  double acc[RW];
 #pragma unroll
-  for (uint32_t r = 0; r < CW; ++r) acc[r] = 0.0;
-  for (uint32_t c0 = g0; c0 < m; c0 += SCH) {
-    const uint32_t cn = min(uint32_t(SCH), m - c0);
-    __syncthreads();
-    for (uint32_t q = threadIdx.x; q < cn; q += NT) {
-      const uint32_t i = c0 + q;
-      ws[q] = i < k ? y[i] : -f.xsol[3 * size_t(rows[i / 3]) + i % 3];
+  for (uint32_t q = 0; q < RW; ++q) acc[q] = 0.0;
+  uint32_t i = r0 + lane;
+  for (; i + 64 < r1; i += 128) {  // 2 x 8/W tiles per lane in flight
+    Chunk<G> c[2][NTL];
+    double w[2];
+#pragma unroll
+    for (uint32_t u = 0; u < 2; ++u) {
+      const uint32_t r = i + 64 * u;
+#pragma unroll
+      for (uint32_t a = 0; a < NTL; ++a) c[u][a] = load_chunk(g + a * ts + uint64_t(r) * W);
+      w[u] = r < k ? y[r] : -v[r];
     }
-    __syncthreads();
-    if (j0 >= k) continue;
-    const uint32_t end = c0 + cn;
-    uint32_t i = max(j0, c0) + lane;
-    for (; i + 64 < end; i += 128) {
-      Chunk<G> c[2][CW / W];
 #pragma unroll
-      for (uint32_t a = 0; a < CW / W; ++a) {
-        c[0][a] = load_chunk(g + a * ts + uint64_t(i) * W);
-        c[1][a] = load_chunk(g + a * ts + uint64_t(i + 64) * W);
-      }
-      const double w0 = ws[i - c0], w1 = ws[i + 64 - c0];
+    for (uint32_t a = 0; a < NTL; ++a)
 #pragma unroll
-      for (uint32_t a = 0; a < CW / W; ++a)
+      for (uint32_t q = 0; q < W; ++q) acc[a * W + q] += gval(c[0][a].v[q]) * w[0] + gval(c[1][a].v[q]) * w[1];
+  }
+  for (; i < r1; i += 64) {
+    const double w = i < k ? y[i] : -v[i];
 #pragma unroll
-        for (uint32_t q = 0; q < W; ++q) acc[a * W + q] += gval(c[0][a].v[q]) * w0 + gval(c[1][a].v[q]) * w1;
-    }
-    for (; i < end; i += 64) {
-      const double w0 = ws[i - c0];
+    for (uint32_t a = 0; a < NTL; ++a) {
+      const Chunk<G> c = load_chunk(g + a * ts + uint64_t(i) * W);
 #pragma unroll
-      for (uint32_t a = 0; a < CW / W; ++a) {
-        const Chunk<G> c = load_chunk(g + a * ts + uint64_t(i) * W);
-#pragma unroll
-        for (uint32_t q = 0; q < W; ++q) acc[a * W + q] += gval(c.v[q]) * w0;
-      }
+      for (uint32_t q = 0; q < W; ++q) acc[a * W + q] += gval(c.v[q]) * w;
     }
   }
 #pragma unroll
-  for (uint32_t r = 0; r < CW; ++r) {
-    const double tot = wave_sum64(acc[r]);
-    const uint32_t j = j0 + r;
-    if (lane == 0 && j < k) f.xsol[3 * size_t(rows[j / 3]) + j % 3] = tot * f.gscale[f.yoff[fr] + j];
+  for (uint32_t q = 0; q < RW; ++q) {
+    const double s = wave_sum64(acc[q]);
+    if (lane == 0) red[wave][q] = s;
   }
+  __syncthreads();
+  if (threadIdx.x >= RW) return;
+  const uint32_t j = j0 + threadIdx.x;
+  if (j >= k) return;
+  const double tot = ((red[0][threadIdx.x] + red[1][threadIdx.x]) + red[2][threadIdx.x]) + red[3][threadIdx.x];
+  const uint32_t *rowsf = f.rows + f.rows_off[fr];
+  f.xsol[3 * size_t(rowsf[j / 3]) + j % 3] = tot * f.gscale[f.yoff[fr] + j];
 }
 
 template <class Real>
@@ -875,6 +900,16 @@ void SolveLaunch<Real>::pack(const FactorDev &f, const uint64_t *w21, uint32_t n
                              hipStream_t s) {
   if (nw21) hipLaunchKernelGGL(k_pack_w21<Real>, dim3(nw21), dim3(NT), 0, s, f, w21, gr, gc);
   if (nm11) hipLaunchKernelGGL(k_pack_m11<Real>, dim3(nm11), dim3(NT), 0, s, f, m11, gr, gc);
+}
+template <class Real>
+void SolveLaunch<Real>::gather(const FactorDev &f, const uint64_t *tasks, uint32_t count, const Real *rin,
+                               const int *done, int parity, uint32_t B, hipStream_t s) {
+  if (count) hipLaunchKernelGGL(k_solve_gather<Real>, dim3(count), dim3(NT), 0, s, f, tasks, rin, done, parity, B);
+}
+template <class Real>
+void SolveLaunch<Real>::xgather(const FactorDev &f, const uint64_t *tasks, uint32_t count, const int *done,
+                                int parity, uint32_t B, hipStream_t s) {
+  if (count) hipLaunchKernelGGL(k_solve_xgather, dim3(count), dim3(NT), 0, s, f, tasks, done, parity, B);
 }
 template <class Real>
 void SolveLaunch<Real>::forward(const FactorDev &f, const G *gr, const uint64_t *tasks,

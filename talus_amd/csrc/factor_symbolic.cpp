@@ -405,6 +405,10 @@ void build_tables(const std::vector<Symbolic> &sym, const std::vector<Body> &bod
   o.bwr_start.assign(1, 0);
   o.fws_start.assign(1, 0);
   o.bws_start.assign(1, 0);
+  o.gat_all_start.assign(1, 0);
+  o.gat_big_start.assign(1, 0);
+  o.xg_all_start.assign(1, 0);
+  o.xg_big_start.assign(1, 0);
   o.plan.assign(2 * size_t(L), 0);
   for (uint32_t l = 0; l < L; ++l) {
     for (uint32_t f : by_level[l]) {
@@ -423,6 +427,14 @@ void build_tables(const std::vector<Symbolic> &sym, const std::vector<Body> &bod
         o.bwd_tasks.push_back(task);
         if (!bl) o.bwr_tasks.push_back(task);
       }
+      for (uint32_t g = 0; g < tiles(o.m[f], GCHUNK); ++g) {
+        o.gat_all.push_back(uint64_t(f) | (uint64_t(g) << 32));
+        if (!fl) o.gat_big.push_back(uint64_t(f) | (uint64_t(g) << 32));
+      }
+      for (uint32_t g = 0; g < tiles(o.m[f] - o.k[f], GCHUNK); ++g) {
+        o.xg_all.push_back(uint64_t(f) | (uint64_t(g) << 32));
+        if (!bl) o.xg_big.push_back(uint64_t(f) | (uint64_t(g) << 32));
+      }
       if (fl) {
         for (uint32_t g = 0; g < tiles(o.m[f], 256u); ++g) o.fwl_tasks.push_back(uint64_t(f) | (uint64_t(g) << 32));
         for (uint32_t g = 0; g < tiles(o.m[f], 64u); ++g) o.fws_tasks.push_back(uint64_t(f) | (uint64_t(g) << 32));
@@ -440,6 +452,10 @@ void build_tables(const std::vector<Symbolic> &sym, const std::vector<Body> &bod
     o.bwr_start.push_back(uint32_t(o.bwr_tasks.size()));
     o.fws_start.push_back(uint32_t(o.fws_tasks.size()));
     o.bws_start.push_back(uint32_t(o.bws_tasks.size()));
+    o.gat_all_start.push_back(uint32_t(o.gat_all.size()));
+    o.gat_big_start.push_back(uint32_t(o.gat_big.size()));
+    o.xg_all_start.push_back(uint32_t(o.xg_all.size()));
+    o.xg_big_start.push_back(uint32_t(o.xg_big.size()));
     o.plan[2 * l] = o.fwl_start[l + 1] - o.fwl_start[l] >= LANES_MIN_GROUPS ? 1 : 0;
     o.plan[2 * l + 1] = o.bwl_start[l + 1] - o.bwl_start[l] >= LANES_MIN_GROUPS ? 1 : 0;
   }

@@ -240,3 +240,25 @@ def test_streams_split_the_batch_without_changing_any_body():
             assert a[key] == b[key], key
     assert e1 == e3
     assert f1["fronts"] == f3["fronts"] and f1["entries"] == f3["entries"]
+
+
+def test_large_fronts_solve_the_system_with_every_plan():
+    """A 14^3 cube has separators with more than 512 pivots, so the lane and
+    split plans also run the rows shape (with its pre-gathered front vectors)
+    on part of a level: every plan solves A z = r (A applied matrix-free by
+    the operator) to 1e-9 in FP64."""
+    body = scenes.cube(14, seed=5).bodies[0]
+    inertia, step = 1.0, math.sqrt(8.0 / 27.0) / 60.0
+    with api.Context(0, api.FP64) as ctx:
+        b, mesh = product_body(ctx, body)
+        free = np.repeat(mesh["anchored"] == 0, 3)
+        r = np.random.default_rng(3).standard_normal(mesh["rest_positions"].shape)
+        r[mesh["anchored"] != 0] = 0.0
+        levels = ctx.plan().size
+        for plan in (0, 1, 2):
+            ctx.set_plan(np.full(levels, plan, dtype=np.uint8))
+            z = ctx.factor_solve(b, inertia, step, r).reshape(-1)
+            Az = ctx.apply_system(b, inertia, step, z).reshape(-1)
+            err = np.abs(Az[free] - r.reshape(-1)[free]).max() / np.abs(r).max()
+            assert err <= 1e-9, (plan, err)
+        assert ctx.factor_info()["ok"] == 1
