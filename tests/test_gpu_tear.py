@@ -124,3 +124,40 @@ def test_strip_pulled_apart_tears_into_pieces(precision, model):
             now = np.linalg.norm(x[T[:, i]] - x[T[:, j]], axis=1)
             was = np.linalg.norm(topo["rest"][T[:, i]] - topo["rest"][T[:, j]], axis=1)
             assert (now / was).max() < 1.8
+
+
+def test_torn_strips_on_two_streams_equal_one_stream():
+    """Two tearable strips pulled apart in one context: stepped on two
+    concurrent engines (one strip each) they tear exactly as on one engine --
+    same events in the same order, same connectivity and states (the merged
+    torn list and the host topology update run per body)."""
+    bodies = [strip_body(api.NEO_HOOKEAN), strip_body(api.NEO_HOOKEAN, cells=(10, 2, 2))]
+    runs = []
+    for streams in (1, 2):
+        with api.Context(0, api.FP64) as ctx:
+            ids, rights, starts = [], [], []
+            for body in bodies:
+                b, mesh = product_body(ctx, body)
+                rest = mesh["rest_positions"]
+                right = np.flatnonzero(rest[:, 0] > rest[:, 0].max() - 0.01)
+                ids.append(b), rights.append(right), starts.append(rest[right, 0].copy())
+            ctx.set_streams(streams)
+            tears = 0
+            for frame in range(200):
+                for b, right, start in zip(ids, rights, starts):
+                    x, v, a = ctx.state(b)
+                    x[right, 0] = start + 2.4 * min(frame / 180.0, 1.0)
+                    ctx.set_state(b, x, v, a)
+                rep = ctx.step(flags=api.DETECT_TEARS | api.RECORD_EVENTS)
+                assert not rep["non_finite"], frame
+                tears += rep["tear_events"]
+            runs.append((tears, ctx.events(), [ctx.topology(b) for b in ids], [ctx.state(b) for b in ids]))
+    (t1, e1, top1, s1), (t2, e2, top2, s2) = runs
+    assert t1 > 0 and t1 == t2
+    assert e1 == e2
+    for a, b in zip(top1, top2):
+        for key in ("tet_nodes", "rest", "anchored"):
+            np.testing.assert_array_equal(a[key], b[key])
+    for a, b in zip(s1, s2):
+        for u, w in zip(a, b):
+            np.testing.assert_array_equal(u, w)
