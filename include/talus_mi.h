@@ -168,6 +168,29 @@ int talus_mi_build_lattice(const talus_mi_lattice *lattice,
                            uint32_t nsupports, talus_mi_mesh *out);
 void talus_mi_mesh_free(talus_mi_mesh *mesh);
 
+/* Boundary surface of a tet mesh and its crack faces (the tet faces of
+ * collectSurface, src/talus/deformable/solver/solver.cc:220-364, and
+ * markCracks, src/talus/dynamics/strainstep/tearing.cc:406-461, for tets).
+ * A face (corner triple {0,1,2}, {0,1,3}, {0,2,3} or {1,2,3} of a tet) is on
+ * the surface when exactly one tet has it; its triangle keeps the tet's
+ * corner order, flipped to face away from the tet.  Triangles ascend by
+ * their sorted node triple; edges are unique (low, high) pairs in ascending
+ * order; surface nodes ascend.  crack_faces[e] has bit f set when face f of
+ * tet e is a crack face: a surface face whose three rest positions are
+ * those of another surface face (the two sides of a tear).  Arrays are
+ * malloc'ed; release with talus_mi_surface_free. */
+typedef struct talus_mi_surface {
+  uint32_t triangles, edges, nodes, tets;
+  uint32_t *triangle_nodes; /* 3*triangles, outward */
+  uint32_t *triangle_tet;   /* triangles */
+  uint32_t *triangle_lane;  /* triangles: the tet's material lane */
+  uint32_t *edge_nodes;     /* 2*edges */
+  uint32_t *surface_nodes;  /* nodes */
+  uint8_t *crack_faces;     /* tets */
+} talus_mi_surface;
+int talus_mi_mesh_surface(const talus_mi_mesh *mesh, talus_mi_surface *out);
+void talus_mi_surface_free(talus_mi_surface *surface);
+
 /* Tear topology update (tearCells, src/talus/dynamics/strainstep/
  * tearing.cc:485-513, on tets) of a host mesh at positions x (3 per node):
  * nodes around each torn tet (ascending indices) are split, new nodes are
@@ -317,6 +340,10 @@ int talus_mi_tune(talus_mi_ctx *ctx, uint32_t repeats, talus_mi_tune_report *rep
 /* Copies the plan (2*levels bytes) when capacity allows; *count = its size. */
 int talus_mi_plan_get(talus_mi_ctx *ctx, uint8_t *plan, uint32_t capacity, uint32_t *count);
 int talus_mi_plan_set(talus_mi_ctx *ctx, const uint8_t *plan, uint32_t count);
+
+/* Surface of a context body's current mesh (after its tears), as
+ * talus_mi_mesh_surface with the FP64 rest positions. */
+int talus_mi_surface_get(talus_mi_ctx *ctx, uint32_t body, talus_mi_surface *out);
 
 /* The context's HIP stream (hipStream_t) for interop. */
 void *talus_mi_stream(talus_mi_ctx *ctx);

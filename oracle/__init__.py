@@ -124,6 +124,9 @@ def lib():
         L.orc_stretch_tets.argtypes = [C.POINTER(C.c_uint32), C.c_size_t, dp, dp, dp]
         L.orc_yield_flags.argtypes = [C.c_void_p, C.POINTER(C.c_uint8)]
         u32p = C.POINTER(C.c_uint32)
+        L.orc_surface.argtypes = [C.c_uint32, C.c_uint32, u32p, u32p, dp, u32p, u32p, u32p, u32p, u32p,
+                                  u32p, u32p, C.POINTER(C.c_uint8)]
+        L.orc_surface.restype = C.c_uint32
         L.orc_tear_topology.argtypes = [C.c_uint32, C.c_uint32, u32p, dp, dp, C.POINTER(C.c_uint8),
                                         C.c_uint32, u32p, C.c_uint32, u32p]
         L.orc_tear_topology.restype = C.c_uint32
@@ -473,6 +476,31 @@ def stretch_tets(tet, rest, x):
     lib().orc_stretch_tets(tet.ctypes.data_as(C.POINTER(C.c_uint32)), tet.shape[0], _dp(rest), _dp(x),
                            _dp(out))
     return out
+
+
+def surface(tet, rest, lane=None):
+    """Boundary surface and crack faces of a tet mesh given as arrays: the
+    same dict as the library's mesh_surface."""
+    tet = np.ascontiguousarray(tet, dtype=np.uint32).reshape(-1, 4)
+    rest = np.ascontiguousarray(rest, dtype=np.float64).reshape(-1, 3)
+    T, N = tet.shape[0], rest.shape[0]
+    if T and int(tet.max()) >= N:
+        raise ValueError("node index out of range")
+    lane = None if lane is None else np.ascontiguousarray(lane, dtype=np.uint32)
+    u32 = lambda a: a.ctypes.data_as(C.POINTER(C.c_uint32))
+    tri = np.zeros(12 * T + 1, dtype=np.uint32)
+    tri_tet = np.zeros(4 * T + 1, dtype=np.uint32)
+    tri_lane = np.zeros(4 * T + 1, dtype=np.uint32)
+    edges = np.zeros(24 * T + 2, dtype=np.uint32)
+    surf = np.zeros(N + 1, dtype=np.uint32)
+    crack = np.zeros(T + 1, dtype=np.uint8)
+    ne, ns = C.c_uint32(), C.c_uint32()
+    nt = lib().orc_surface(N, T, u32(tet), None if lane is None else u32(lane), _dp(rest), u32(tri),
+                           u32(tri_tet), u32(tri_lane), u32(edges), C.byref(ne), u32(surf), C.byref(ns),
+                           crack.ctypes.data_as(C.POINTER(C.c_uint8)))
+    return {"triangles": tri[: 3 * nt].reshape(-1, 3).copy(), "triangle_tet": tri_tet[:nt].copy(),
+            "triangle_lane": tri_lane[:nt].copy(), "edges": edges[: 2 * ne.value].reshape(-1, 2).copy(),
+            "nodes": surf[: ns.value].copy(), "crack_faces": crack[:T].copy()}
 
 
 def eigen_sym3(A):

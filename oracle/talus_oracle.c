@@ -2178,3 +2178,163 @@ int orc_rollout_backward(orc_body *b, const double *x_bar, const double *v_bar, 
   free(ab);
   return 0;
 }
+
+/* ------------------------------------------------------------------ */
+/* Boundary surface of a tet mesh (collectSurface's tet faces,          */
+/* solver.cc:220-364) and crack faces (markCracks, tearing.cc:406-461,  */
+/* restated for tets).                                                  */
+/* ------------------------------------------------------------------ */
+
+typedef struct {
+  uint32_t s[3]; /* sorted nodes */
+  uint32_t tet;
+  uint32_t face;
+} orc_face_t;
+
+static int face_cmp(const void *pa, const void *pb) {
+  const orc_face_t *a = (const orc_face_t *)pa, *b = (const orc_face_t *)pb;
+  for (int k = 0; k < 3; ++k)
+    if (a->s[k] != b->s[k]) return a->s[k] < b->s[k] ? -1 : 1;
+  if (a->tet != b->tet) return a->tet < b->tet ? -1 : 1;
+  return a->face < b->face ? -1 : (a->face > b->face);
+}
+
+typedef struct {
+  double p[9];
+  uint32_t idx;
+} orc_spot_t;
+
+static int spot_cmp(const void *pa, const void *pb) {
+  const orc_spot_t *a = (const orc_spot_t *)pa, *b = (const orc_spot_t *)pb;
+  for (int k = 0; k < 9; ++k)
+    if (a->p[k] != b->p[k]) return a->p[k] < b->p[k] ? -1 : 1;
+  return a->idx < b->idx ? -1 : (a->idx > b->idx);
+}
+
+static int spot_cmp_points(const orc_spot_t *a, const orc_spot_t *b) {
+  for (int k = 0; k < 9; ++k)
+    if (a->p[k] != b->p[k]) return a->p[k] < b->p[k] ? -1 : 1;
+  return 0;
+}
+
+static int point_cmp(const void *pa, const void *pb) {
+  const double *a = (const double *)pa, *b = (const double *)pb;
+  for (int k = 0; k < 3; ++k)
+    if (a[k] != b[k]) return a[k] < b[k] ? -1 : 1;
+  return 0;
+}
+
+static int edge_cmp(const void *pa, const void *pb) {
+  const uint32_t *a = (const uint32_t *)pa, *b = (const uint32_t *)pb;
+  if (a[0] != b[0]) return a[0] < b[0] ? -1 : 1;
+  return a[1] < b[1] ? -1 : (a[1] > b[1]);
+}
+
+/* Outputs: tri (3 per surface face, capacity 3*4*tets), tri_tet, tri_lane
+ * (4*tets), edges (2 per edge, capacity 2*3*4*tets), surface nodes
+ * (capacity nodes), crack (tets).  Returns the triangle count; *edges_out and
+ * *nodes_out receive the edge and node counts. */
+uint32_t orc_surface(uint32_t nodes, uint32_t tets, const uint32_t *tet, const uint32_t *lane,
+                     const double *rest, uint32_t *tri, uint32_t *tri_tet, uint32_t *tri_lane,
+                     uint32_t *edges, uint32_t *edges_out, uint32_t *surf, uint32_t *nodes_out,
+                     uint8_t *crack) {
+  static const int corner[4][3] = {{0, 1, 2}, {0, 1, 3}, {0, 2, 3}, {1, 2, 3}};
+  const size_t nf = 4 * (size_t)tets;
+  orc_face_t *faces = (orc_face_t *)malloc((nf ? nf : 1) * sizeof *faces);
+  for (size_t e = 0; e < tets; ++e)
+    for (int f = 0; f < 4; ++f) {
+      orc_face_t *q = &faces[4 * e + f];
+      uint32_t a = tet[4 * e + corner[f][0]], b = tet[4 * e + corner[f][1]], c = tet[4 * e + corner[f][2]], t;
+      if (a > b) { t = a; a = b; b = t; }
+      if (b > c) { t = b; b = c; c = t; }
+      if (a > b) { t = a; a = b; b = t; }
+      q->s[0] = a; q->s[1] = b; q->s[2] = c;
+      q->tet = (uint32_t)e;
+      q->face = (uint32_t)f;
+    }
+  qsort(faces, nf, sizeof *faces, face_cmp);
+  uint32_t nt = 0, ne = 0;
+  uint8_t *on = (uint8_t *)calloc(nodes ? nodes : 1, 1);
+  memset(crack, 0, tets);
+  orc_spot_t *spots = (orc_spot_t *)malloc((nf ? nf : 1) * sizeof *spots);
+  for (size_t i = 0; i < nf;) {
+    size_t j = i + 1;
+    while (j < nf && faces[j].s[0] == faces[i].s[0] && faces[j].s[1] == faces[i].s[1] &&
+           faces[j].s[2] == faces[i].s[2])
+      ++j;
+    if (j - i == 1) {
+      const orc_face_t *q = &faces[i];
+      const uint32_t *tn = &tet[4 * (size_t)q->tet];
+      const int *c3 = corner[q->face];
+      uint32_t a = tn[c3[0]], b = tn[c3[1]], c = tn[c3[2]];
+      const uint32_t d = tn[6 - c3[0] - c3[1] - c3[2]];
+      /* orientation: det[b-a, c-a, d-a] > 0 means the normal (b-a)x(c-a)
+         points toward d, into the tet */
+      double m[9];
+      for (int r = 0; r < 3; ++r) {
+        m[r * 3 + 0] = rest[3 * (size_t)b + r] - rest[3 * (size_t)a + r];
+        m[r * 3 + 1] = rest[3 * (size_t)c + r] - rest[3 * (size_t)a + r];
+        m[r * 3 + 2] = rest[3 * (size_t)d + r] - rest[3 * (size_t)a + r];
+      }
+      if (orc_det3(m) > 0.0) {
+        const uint32_t t = b;
+        b = c;
+        c = t;
+      }
+      tri[3 * nt] = a; tri[3 * nt + 1] = b; tri[3 * nt + 2] = c;
+      tri_tet[nt] = q->tet;
+      tri_lane[nt] = lane ? lane[q->tet] : 0u;
+      on[a] = on[b] = on[c] = 1;
+      const uint32_t pairs[3][2] = {{q->s[0], q->s[1]}, {q->s[0], q->s[2]}, {q->s[1], q->s[2]}};
+      for (int k = 0; k < 3; ++k) {
+        edges[2 * ne] = pairs[k][0];
+        edges[2 * ne + 1] = pairs[k][1];
+        ++ne;
+      }
+      orc_spot_t *sp = &spots[nt];
+      for (int k = 0; k < 3; ++k)
+        for (int r = 0; r < 3; ++r) sp->p[3 * k + r] = rest[3 * (size_t)q->s[k] + r];
+      qsort(sp->p, 3, 3 * sizeof(double), point_cmp);
+      sp->idx = nt;
+      ++nt;
+    }
+    i = j;
+  }
+  qsort(edges, ne, 2 * sizeof(uint32_t), edge_cmp);
+  uint32_t unique = 0;
+  for (uint32_t k = 0; k < ne; ++k)
+    if (!unique || edges[2 * k] != edges[2 * (unique - 1)] || edges[2 * k + 1] != edges[2 * (unique - 1) + 1]) {
+      edges[2 * unique] = edges[2 * k];
+      edges[2 * unique + 1] = edges[2 * k + 1];
+      ++unique;
+    }
+  *edges_out = unique;
+  uint32_t ns = 0;
+  for (uint32_t v = 0; v < nodes; ++v)
+    if (on[v]) surf[ns++] = v;
+  *nodes_out = ns;
+  qsort(spots, nt, sizeof *spots, spot_cmp);
+  for (uint32_t i = 0; i < nt;) {
+    uint32_t j = i + 1;
+    while (j < nt && spot_cmp_points(&spots[j], &spots[i]) == 0) ++j;
+    if (j - i > 1)
+      for (uint32_t k = i; k < j; ++k) {
+        const uint32_t tr = spots[k].idx;
+        /* the face index of triangle tr: find it among its tet's faces */
+        const uint32_t e = tri_tet[tr];
+        for (int f = 0; f < 4; ++f) {
+          uint32_t s3[3] = {tet[4 * (size_t)e + corner[f][0]], tet[4 * (size_t)e + corner[f][1]],
+                            tet[4 * (size_t)e + corner[f][2]]};
+          int same = 1;
+          for (int r = 0; r < 3 && same; ++r)
+            same = s3[r] == tri[3 * tr + 0] || s3[r] == tri[3 * tr + 1] || s3[r] == tri[3 * tr + 2];
+          if (same) crack[e] = (uint8_t)(crack[e] | (1u << f));
+        }
+      }
+    i = j;
+  }
+  free(spots);
+  free(on);
+  free(faces);
+  return nt;
+}

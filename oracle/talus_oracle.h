@@ -182,6 +182,12 @@ int orc_rollout_backward(orc_body *body, const double *x_bar, const double *v_ba
 /* y = K(pos) d with K the neo-Hookean Hessian of the elastic energy. */
 void orc_hessian_apply(orc_body *body, const double *pos, const double *d, double *y);
 
+/* Boundary surface and crack faces of a tet mesh (see talus_oracle.c). */
+uint32_t orc_surface(uint32_t nodes, uint32_t tets, const uint32_t *tet, const uint32_t *lane,
+                     const double *rest, uint32_t *tri, uint32_t *tri_tet, uint32_t *tri_lane,
+                     uint32_t *edges, uint32_t *edges_out, uint32_t *surf, uint32_t *nodes_out,
+                     uint8_t *crack);
+
 #ifdef __cplusplus
 }
 #endif

@@ -190,6 +190,9 @@ _SIGNATURES = {
     "talus_mi_plan_get": (C.c_int, [C.c_void_p, _P(C.c_uint8), C.c_uint32, _P(C.c_uint32)]),
     "talus_mi_plan_set": (C.c_int, [C.c_void_p, _P(C.c_uint8), C.c_uint32]),
     "talus_mi_set_streams": (C.c_int, [C.c_void_p, C.c_uint32]),
+    "talus_mi_mesh_surface": (C.c_int, [C.c_void_p, C.c_void_p]),
+    "talus_mi_surface_get": (C.c_int, [C.c_void_p, C.c_uint32, C.c_void_p]),
+    "talus_mi_surface_free": (None, [C.c_void_p]),
     "talus_mi_stream": (C.c_void_p, [C.c_void_p]),
 }
 
@@ -344,6 +347,39 @@ def tear_mesh(mesh, materials, x, torn):
     return new, org
 
 
+class _Surface(C.Structure):
+    _fields_ = [("triangles", C.c_uint32), ("edges", C.c_uint32), ("nodes", C.c_uint32),
+                ("tets", C.c_uint32), ("triangle_nodes", C.POINTER(C.c_uint32)),
+                ("triangle_tet", C.POINTER(C.c_uint32)), ("triangle_lane", C.POINTER(C.c_uint32)),
+                ("edge_nodes", C.POINTER(C.c_uint32)), ("surface_nodes", C.POINTER(C.c_uint32)),
+                ("crack_faces", C.POINTER(C.c_uint8))]
+
+
+def _surface_out(s: _Surface) -> dict:
+    def arr(ptr, n):
+        return np.ctypeslib.as_array(ptr, (max(n, 1),))[:n].copy()
+
+    return {"triangles": arr(s.triangle_nodes, 3 * s.triangles).reshape(-1, 3),
+            "triangle_tet": arr(s.triangle_tet, s.triangles),
+            "triangle_lane": arr(s.triangle_lane, s.triangles),
+            "edges": arr(s.edge_nodes, 2 * s.edges).reshape(-1, 2),
+            "nodes": arr(s.surface_nodes, s.nodes),
+            "crack_faces": arr(s.crack_faces, s.tets)}
+
+
+def mesh_surface(mesh) -> dict:
+    """Boundary surface and crack faces of a host mesh (talus_mi_mesh_surface)."""
+    L = lib()
+    keep = []
+    m = _mesh_in(mesh, keep)
+    out = _Surface()
+    _check(L.talus_mi_mesh_surface(C.byref(m), C.byref(out)))
+    try:
+        return _surface_out(out)
+    finally:
+        L.talus_mi_surface_free(C.byref(out))
+
+
 def default_params(**kw) -> StepParams:
     p = StepParams()
     lib().talus_mi_default_params(C.byref(p))
@@ -462,6 +498,15 @@ class Context:
                                            anchored.ctypes.data_as(C.POINTER(C.c_uint8))))
         return {"tet_nodes": tets.reshape(t, 4), "rest": rest.reshape(n, 3), "mass": mass,
                 "anchored": anchored}
+
+    def surface(self, body) -> dict:
+        """Boundary surface and crack faces of a body's current mesh."""
+        out = _Surface()
+        _check(lib().talus_mi_surface_get(self._h, body, C.byref(out)))
+        try:
+            return _surface_out(out)
+        finally:
+            lib().talus_mi_surface_free(C.byref(out))
 
     def set_streams(self, streams: int):
         """Split the bodies over `streams` concurrently stepped engines."""

@@ -1907,6 +1907,72 @@ void talus_mi_mesh_free(talus_mi_mesh *mesh) { tmi::mesh_free(mesh); }
 
 void talus_mi_free(void *p) { std::free(p); }
 
+namespace {
+
+int export_surface(const tmi::SurfaceMesh &sm, uint32_t tets, talus_mi_surface *out) {
+  std::memset(out, 0, sizeof *out);
+  out->triangles = uint32_t(sm.triangle_tet.size());
+  out->edges = uint32_t(sm.edges.size() / 2);
+  out->nodes = uint32_t(sm.nodes.size());
+  out->tets = tets;
+  auto copy = [](const auto &v) {
+    using T = typename std::decay_t<decltype(v)>::value_type;
+    T *p = (T *)std::malloc(std::max<size_t>(v.size(), 1) * sizeof(T));
+    if (p && !v.empty()) std::memcpy(p, v.data(), v.size() * sizeof(T));
+    return p;
+  };
+  out->triangle_nodes = copy(sm.triangles);
+  out->triangle_tet = copy(sm.triangle_tet);
+  out->triangle_lane = copy(sm.triangle_lane);
+  out->edge_nodes = copy(sm.edges);
+  out->surface_nodes = copy(sm.nodes);
+  out->crack_faces = copy(sm.crack);
+  if (!out->triangle_nodes || !out->triangle_tet || !out->triangle_lane || !out->edge_nodes ||
+      !out->surface_nodes || !out->crack_faces) {
+    talus_mi_surface_free(out);
+    return fail(TALUS_MI_ERR_MEMORY, "out of host memory");
+  }
+  return TALUS_MI_OK;
+}
+
+}  // namespace
+
+int talus_mi_mesh_surface(const talus_mi_mesh *mesh, talus_mi_surface *out) {
+  if (!mesh || !out || !mesh->rest_positions || (mesh->tets && !mesh->tet_nodes))
+    return fail(TALUS_MI_ERR_ARGUMENT, "bad arguments");
+  try {
+    for (size_t i = 0; i < 4 * size_t(mesh->tets); ++i)
+      if (mesh->tet_nodes[i] >= mesh->nodes) return fail(TALUS_MI_ERR_ARGUMENT, "tet node out of range");
+    std::vector<double> rest(mesh->rest_positions, mesh->rest_positions + 3 * size_t(mesh->nodes));
+    tmi::SurfaceMesh sm;
+    tmi::mesh_surface(mesh->nodes, mesh->tets, mesh->tet_nodes, mesh->tet_material, rest.data(), sm);
+    return export_surface(sm, mesh->tets, out);
+  } catch (const std::bad_alloc &) {
+    return fail(TALUS_MI_ERR_MEMORY, "out of host memory");
+  }
+}
+
+int talus_mi_surface_get(talus_mi_ctx *ctx, uint32_t body, talus_mi_surface *out) {
+  if (!ctx || !body_ok(ctx, body) || !out) return fail(TALUS_MI_ERR_ARGUMENT, "bad arguments");
+  return guarded(ctx, [&]() -> int {
+    const tmi::Body &b = ctx->bodies[body];
+    tmi::SurfaceMesh sm;
+    tmi::mesh_surface(b.nodes, b.tets, b.tet.data(), b.lane.data(), b.rest.data(), sm);
+    return export_surface(sm, b.tets, out);
+  });
+}
+
+void talus_mi_surface_free(talus_mi_surface *s) {
+  if (!s) return;
+  std::free(s->triangle_nodes);
+  std::free(s->triangle_tet);
+  std::free(s->triangle_lane);
+  std::free(s->edge_nodes);
+  std::free(s->surface_nodes);
+  std::free(s->crack_faces);
+  std::memset(s, 0, sizeof *s);
+}
+
 int talus_mi_tear_mesh(const talus_mi_mesh *mesh, const talus_mi_material *materials,
                        uint32_t lanes, const double *x, const uint32_t *torn, uint32_t ntorn,
                        talus_mi_mesh *out, uint32_t **origin, uint32_t *new_nodes) {

@@ -51,6 +51,21 @@ struct Body {
 // Fills per-tet material data and FP64 rest frames; returns a status.
 int prepare_body(Body &body);
 
+// Boundary surface of a tet mesh (surface.cpp): outward triangles in
+// ascending order of their sorted node triple with their tet and lane,
+// unique sorted edges, surface nodes, and per tet the faces that are crack
+// faces (bit f for the corner triple f of {0,1,2}, {0,1,3}, {0,2,3},
+// {1,2,3}).
+struct SurfaceMesh {
+  std::vector<uint32_t> triangles;  // 3 per triangle
+  std::vector<uint32_t> triangle_tet, triangle_lane;
+  std::vector<uint32_t> edges;  // 2 per edge
+  std::vector<uint32_t> nodes;
+  std::vector<uint8_t> crack;   // per tet
+};
+void mesh_surface(uint32_t nodes, uint32_t tets, const uint32_t *tet, const uint32_t *lane,
+                  const double *rest, SurfaceMesh &out);
+
 // Tear topology update (tear.cpp): splits nodes around the torn tets
 // (ascending body-local indices) at the body's positions b.x; new nodes are
 // appended and origin[i] is the node that node (old count + i) came from.

@@ -14,7 +14,8 @@ tests/dynamics_strainstep_tests.cc:3740-3744).
   the GPU's positions (no stretch within 1e-9 of the threshold) and the
   connectivity after the tears is bit-exact with the oracle's restatement
   applied to the GPU's pre-tear mesh; tets are never removed, mass is kept, no
-  node is orphaned and the state is finite.
+  node is orphaned and the state is finite; the torn body's surface and crack
+  faces equal the oracle's and still enclose the slab's volume.
 """
 import numpy as np
 import pytest
@@ -133,3 +134,15 @@ def test_full_slab_tear_onset_connectivity_and_invariants():
             assert abs(after["mass"].sum() - mass0) <= 1e-6 * mass0
             total += torn.size
         assert total > 1000
+        # the torn body's surface: the library's equals the oracle's on the
+        # same mesh, and it still encloses the body's volume
+        srf = ctx.surface(b)
+        topo = ctx.topology(b)
+        ref = O.surface(topo["tet_nodes"], topo["rest"], np.zeros(t0, dtype=np.uint32))
+        for key in ("triangles", "triangle_tet", "edges", "nodes", "crack_faces"):
+            np.testing.assert_array_equal(srf[key], ref[key], err_msg=key)
+        tri = srf["triangles"]
+        r = topo["rest"]
+        enclosed = np.einsum("ij,ij->i", r[tri[:, 0]], np.cross(r[tri[:, 1]], r[tri[:, 2]])).sum() / 6.0
+        assert abs(enclosed - 2.0 * 1.0 * 0.1) <= 1e-6
+        assert srf["crack_faces"].sum() > 0
