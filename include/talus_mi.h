@@ -361,6 +361,122 @@ void *talus_mi_stream(talus_mi_ctx *ctx);
 #define TALUS_MI_MAX_STREAMS 4u
 int talus_mi_set_streams(talus_mi_ctx *ctx, uint32_t streams);
 
+/* ---- strainstep hex lattices -------------------------------------------
+ * The deformable path World::step runs today: a LatticeBody of hexes solved
+ * by solveStrainstep (src/talus/dynamics/strainstep/strainstep.cc:569-842)
+ * with one 18-row soft-constraint element per hex
+ * (strainstep/element.cc:244-523).  FP64 throughout, like the reference. */
+
+/* Hex PreparedMesh (src/talus/mesh.h:18-44): corner c of a hex sits at
+ * offsets (c&1, c>>1&1, c>>2&1) of its cell.  Arrays are caller-owned and
+ * copied by talus_mi_lattice_add; the builder allocates them (free with
+ * talus_mi_hex_mesh_free). */
+typedef struct talus_mi_hex_mesh {
+  uint32_t nodes;
+  uint32_t hexes;
+  float *rest_positions; /* 3*nodes */
+  float *node_mass;      /* nodes */
+  uint8_t *anchored;     /* nodes */
+  uint32_t *hex_nodes;   /* 8*hexes */
+  float *volume;         /* hexes */
+  uint32_t *hex_material;/* hexes, material lane */
+  float voxel_size[3];   /* PreparedMesh::voxelSize: cell size and colours */
+} talus_mi_hex_mesh;
+
+/* buildMesh with VolumeElementType::Hex (src/talus/deformable/solver/
+ * solver.cc:2185-2316) on a structured lattice: first-touch nodes in a
+ * z, y, x cell scan, volume = voxel product, each corner gets rho V / 8 of
+ * mass (summed in float), nodes in a support volume anchored. */
+int talus_mi_build_hex_lattice(const talus_mi_lattice *lattice,
+                               const talus_mi_material *materials, uint32_t lanes,
+                               const talus_mi_support *supports, uint32_t nsupports,
+                               talus_mi_hex_mesh *out);
+void talus_mi_hex_mesh_free(talus_mi_hex_mesh *mesh);
+
+/* makeElement's outputs (StrainElement, strainstep/element.h:40-73) for one
+ * hex: host code, no device needed. */
+typedef struct talus_mi_hex_element {
+  double gradient[24];      /* per corner: rest shape^-T * corner signs / 8 */
+  double inverse_mass[8];
+  double rest_hourglass[12];
+  double bend;
+  double size;
+  double stiffness[18];
+  double rate[18];
+  double softness[18];
+  double cap[18];
+  double inverse[324];      /* 18x18 row-major */
+  double volume_stiffness;
+  double volume_rate;
+  double volume_softness;
+  double volume_cap;
+  uint32_t volumetric;
+  uint32_t reserved;
+} talus_mi_hex_element;
+
+/* cornerRest + elementMaterial + makeElement (element.cc:185-190, 199-214,
+ * 244-342) for an undamped material, from the eight rest corners (3 each)
+ * and the corners' inverse masses; TALUS_MI_ERR_ARGUMENT when the reference
+ * builds no element (degenerate rest shape, no shear modulus, no volume or
+ * no free corner). */
+int talus_mi_hex_element_make(const double *corners, const double *inverse_mass, double volume,
+                              float youngs_modulus, float poissons_ratio, double h,
+                              talus_mi_hex_element *out);
+
+/* Step parameters: IntegrationParameters (integrationparameters.h:9-33)
+ * subset the strainstep path reads. */
+typedef struct talus_mi_lattice_params {
+  double dt;                  /* default 1/60 */
+  double gravity[3];          /* default (0, -9.81, 0) */
+  uint32_t softstep_substeps; /* default 4; islandSubsteps' floor */
+  uint32_t flags;             /* TALUS_MI_DIRECT_LAUNCH */
+} talus_mi_lattice_params;
+
+typedef struct talus_mi_lattice_report {
+  uint32_t lattices;        /* bodies stepped */
+  uint32_t substeps;        /* largest over bodies */
+  uint32_t elements;        /* hex elements solved */
+  uint32_t rebuilds;        /* bodies whose elements were built this step */
+  uint64_t element_solves;  /* solveElement calls */
+  double device_ms;         /* GPU time of the step's launches */
+} talus_mi_lattice_report;
+
+void talus_mi_lattice_default_params(talus_mi_lattice_params *params);
+
+/* addLattice / makeLattice (src/talus/dynamics/bodyset/bodyset.cc:13-86,
+ * 120-136): positions rest + offset, velocity given (zero when anchored),
+ * cell size = the smallest voxel side.  The strainstep path needs material
+ * lanes (strainstep.cc:86-94: lanes past the array read the last one).
+ * Damping ratios, yield stresses and tear strains must be infinite (the
+ * lowest-mode damping, lattice plasticity and cell tearing are not in this
+ * library): TALUS_MI_ERR_ARGUMENT otherwise. */
+int talus_mi_lattice_add(talus_mi_ctx *ctx, const talus_mi_hex_mesh *mesh,
+                         const talus_mi_material *materials, uint32_t lanes,
+                         const double *offset, const double *velocity, uint32_t *lattice);
+/* LatticeBody::substeps: the body asks for at least this many substeps. */
+int talus_mi_lattice_set_substeps(talus_mi_ctx *ctx, uint32_t lattice, uint32_t substeps);
+int talus_mi_lattice_state_get(talus_mi_ctx *ctx, uint32_t lattice, double *x, double *v);
+int talus_mi_lattice_state_set(talus_mi_ctx *ctx, uint32_t lattice, const double *x,
+                               const double *v);
+/* The body's elements (hexes that built one, in cell order): cells and the
+ * accumulated impulses, 18 rows then the volume row (19 per element).  Either
+ * output may be NULL; *count = elements (0 before the first step). */
+int talus_mi_lattice_impulses_get(talus_mi_ctx *ctx, uint32_t lattice, uint32_t *cells,
+                                  double *impulses, uint32_t capacity, uint32_t *count);
+/* Node and hex colours (makeLattice, bodyset.cc:30-37): the parity of a
+ * node's lattice index per axis; a hex takes its corner 0's colour.  Hexes of
+ * one colour share no node, so each colour's elements are solved together. */
+int talus_mi_lattice_colours_get(talus_mi_ctx *ctx, uint32_t lattice, uint8_t *node_colour,
+                                 uint8_t *hex_colour);
+/* One solveStrainstep over every lattice of the context, each its own
+ * island (no contacts, joints or rigid partners): per body islandSubsteps
+ * (strainstep.cc:551-567) substeps of integrate, element update and
+ * warm-start, 4 Gauss-Seidel passes of solveElement, advance.  The passes
+ * sweep the hexes colour by colour (all bodies' hexes of one colour in one
+ * launch), where the reference sweeps them in cell order. */
+int talus_mi_lattice_step(talus_mi_ctx *ctx, const talus_mi_lattice_params *params,
+                          talus_mi_lattice_report *report);
+
 #ifdef __cplusplus
 }
 #endif

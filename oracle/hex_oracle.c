@@ -1,0 +1,885 @@
+/*
+ * hex_oracle.c -- FP64 CPU restatement of the strainstep hex-lattice solve.
+ *
+ * TEST INFRASTRUCTURE ONLY (see hex_oracle.h).
+ *
+ * Reference (paths under src/talus/):
+ *   dynamics/strainstep/element.cc:22-27    springOf
+ *   dynamics/strainstep/element.cc:42-99    symmetricInverse (Gauss-Jordan,
+ *                                           no pivoting, inert rows zeroed)
+ *   dynamics/strainstep/element.cc:113-158  addRows / rowForce
+ *   dynamics/strainstep/element.cc:162-242  corner signs, hourglass signs,
+ *                                           elementMaterial, cellFrequency,
+ *                                           cornerRest, rowJacobian
+ *   dynamics/strainstep/element.cc:244-342  makeElement
+ *   dynamics/strainstep/element.cc:344-372  rotationOf
+ *   dynamics/strainstep/element.cc:374-459  updateElement
+ *   dynamics/strainstep/element.cc:461-523  applyElement / solveElement
+ *   dynamics/strainstep/strainstep.cc:56-126, 138-149, 551-567  pinned nodes,
+ *                                           integrate/advance, substep rule,
+ *                                           inverse masses
+ *   dynamics/strainstep/strainstep.cc:569-842  the substep loop (elements
+ *                                           only)
+ *   dynamics/bodyset/bodyset.cc:13-86       makeLattice (positions, velocity,
+ *                                           node colours, cell size)
+ *   dynamics/strainstep/plasticflow.cc:62-72  authoredRest
+ *   deformable/solver/solver.cc:2185-2316   buildMesh, hex elements
+ *
+ * The lattice turn (plasticflow.cc:342-389) seeds each element's rotation;
+ * the reference takes U V^T of an Eigen JacobiSVD of the mass-weighted
+ * moment.  Here the same proper rotation is found with the rotationOf
+ * iteration run from the identity (it maximises tr(R^T M), as U V^T does when
+ * det M > 0); it only seeds rotationOf, which then iterates to 1e-9.
+ */
+#include "hex_oracle.h"
+
+#include <math.h>
+#include <stdlib.h>
+#include <string.h>
+
+#define ROWS ORC_HEX_ROWS
+
+static const double MAX_CORRECTIVE_SPEED = 3.0; /* element.cc:11 */
+static const uint32_t ROTATION_ITERATIONS = 16; /* element.cc:12 */
+static const double ROTATION_TOLERANCE = 1.0e-9;
+static const double RESOLVED = 1.5;             /* element.h:14 */
+static const uint32_t MOST_SUBSTEPS = 64;       /* element.h:15 */
+static const double MAX_SPEED = 400.0;          /* strainstep.cc:56 */
+static const uint32_t PASSES = 4;               /* strainstep.cc:58 */
+
+static void signs_of(uint32_t c, double *s) {
+  s[0] = (c & 1u) ? 1.0 : -1.0;
+  s[1] = (c & 2u) ? 1.0 : -1.0;
+  s[2] = (c & 4u) ? 1.0 : -1.0;
+}
+
+double orc_hourglass_sign(uint32_t mode, uint32_t corner) {
+  double s[3];
+  signs_of(corner, s);
+  switch (mode) {
+    case 0: return s[0] * s[1];
+    case 1: return s[1] * s[2];
+    case 2: return s[2] * s[0];
+    default: return s[0] * s[1] * s[2];
+  }
+}
+
+static double dot3(const double *a, const double *b) { return a[0] * b[0] + a[1] * b[1] + a[2] * b[2]; }
+
+static void cross3(const double *a, const double *b, double *c) {
+  c[0] = a[1] * b[2] - b[1] * a[2];
+  c[1] = a[2] * b[0] - b[2] * a[0];
+  c[2] = a[0] * b[1] - b[0] * a[1];
+}
+
+/* y = M x, M row-major */
+static void mv3(const double *M, const double *x, double *y) {
+  for (int r = 0; r < 3; ++r) y[r] = M[3 * r] * x[0] + M[3 * r + 1] * x[1] + M[3 * r + 2] * x[2];
+}
+
+/* y = M^T x */
+static void mtv3(const double *M, const double *x, double *y) {
+  for (int r = 0; r < 3; ++r) y[r] = M[r] * x[0] + M[3 + r] * x[1] + M[6 + r] * x[2];
+}
+
+/* adjugate inverse of a row-major 3x3 (entries of adj(A) times 1/det) */
+static void inverse3(const double *a, double *inv) {
+  const double det = orc_det3(a);
+  const double s = 1.0 / det;
+  inv[0] = (a[4] * a[8] - a[5] * a[7]) * s;
+  inv[1] = (a[2] * a[7] - a[1] * a[8]) * s;
+  inv[2] = (a[1] * a[5] - a[2] * a[4]) * s;
+  inv[3] = (a[5] * a[6] - a[3] * a[8]) * s;
+  inv[4] = (a[0] * a[8] - a[2] * a[6]) * s;
+  inv[5] = (a[2] * a[3] - a[0] * a[5]) * s;
+  inv[6] = (a[3] * a[7] - a[4] * a[6]) * s;
+  inv[7] = (a[1] * a[6] - a[0] * a[7]) * s;
+  inv[8] = (a[0] * a[4] - a[1] * a[3]) * s;
+}
+
+void orc_hex_corner_rest(const double *corners, double *shape, double *hourglass) {
+  memset(shape, 0, 9 * sizeof(double));
+  memset(hourglass, 0, 12 * sizeof(double));
+  for (uint32_t c = 0; c < 8; ++c) {
+    double s[3];
+    signs_of(c, s);
+    for (int r = 0; r < 3; ++r)
+      for (int k = 0; k < 3; ++k) shape[3 * r + k] += corners[3 * c + r] * s[k] / 8.0;
+    for (uint32_t m = 0; m < 4; ++m) {
+      const double g = orc_hourglass_sign(m, c);
+      for (int r = 0; r < 3; ++r) hourglass[3 * m + r] += g * corners[3 * c + r];
+    }
+  }
+}
+
+void orc_hex_material(float youngs, float poisson, double *shear, double *first) {
+  double nu = (double)poisson;
+  nu = nu < 0.0 ? 0.0 : (nu > 0.499 ? 0.499 : nu);
+  const double e = (double)youngs > 0.0 ? (double)youngs : 0.0;
+  *shear = e / (2.0 * (1.0 + nu));
+  *first = e * nu / ((1.0 + nu) * (1.0 - 2.0 * nu));
+}
+
+double orc_hex_cell_frequency(float youngs, float poisson, float density, double cell) {
+  double nu = (double)poisson;
+  nu = nu < 0.0 ? 0.0 : (nu > 0.499 ? 0.499 : nu);
+  const double e = (double)youngs > 0.0 ? (double)youngs : 0.0;
+  return sqrt(e / ((1.0 + nu) * (double)density)) / cell;
+}
+
+/* Jacobian of row `row` with respect to a corner's local velocity */
+static void row_jacobian(const orc_hex *e, uint32_t corner, uint32_t row, double *j) {
+  const double *c = e->gradient + 3 * corner;
+  j[0] = j[1] = j[2] = 0.0;
+  switch (row) {
+    case 0: j[0] = c[0]; return;
+    case 1: j[1] = c[1]; return;
+    case 2: j[2] = c[2]; return;
+    case 3: j[0] = 0.5 * c[1]; j[1] = 0.5 * c[0]; return;
+    case 4: j[1] = 0.5 * c[2]; j[2] = 0.5 * c[1]; return;
+    case 5: j[0] = 0.5 * c[2]; j[2] = 0.5 * c[0]; return;
+    default: j[(row - 6) % 3] = e->bend * orc_hourglass_sign((row - 6) / 3, corner); return;
+  }
+}
+
+/* rows += J_corner * local for all 18 rows */
+static void add_rows(double *rows, const orc_hex *e, uint32_t corner, const double *l) {
+  const double *c = e->gradient + 3 * corner;
+  rows[0] += l[0] * c[0];
+  rows[1] += l[1] * c[1];
+  rows[2] += l[2] * c[2];
+  rows[3] += 0.5 * (l[0] * c[1] + l[1] * c[0]);
+  rows[4] += 0.5 * (l[1] * c[2] + l[2] * c[1]);
+  rows[5] += 0.5 * (l[2] * c[0] + l[0] * c[2]);
+  for (uint32_t m = 0; m < 4; ++m) {
+    const double sign = orc_hourglass_sign(m, corner) * e->bend;
+    rows[6 + 3 * m] += sign * l[0];
+    rows[7 + 3 * m] += sign * l[1];
+    rows[8 + 3 * m] += sign * l[2];
+  }
+}
+
+/* J_corner^T * impulse, the corner's force in the element frame */
+static void row_force(const orc_hex *e, uint32_t corner, const double *imp, double *out) {
+  const double *c = e->gradient + 3 * corner;
+  out[0] = imp[0] * c[0] + 0.5 * (imp[3] * c[1] + imp[5] * c[2]);
+  out[1] = imp[1] * c[1] + 0.5 * (imp[3] * c[0] + imp[4] * c[2]);
+  out[2] = imp[2] * c[2] + 0.5 * (imp[4] * c[1] + imp[5] * c[0]);
+  for (uint32_t m = 0; m < 4; ++m) {
+    const double sign = orc_hourglass_sign(m, corner) * e->bend;
+    out[0] += sign * imp[6 + 3 * m];
+    out[1] += sign * imp[7 + 3 * m];
+    out[2] += sign * imp[8 + 3 * m];
+  }
+}
+
+/* critically damped soft spring of the row (theta = 1, zeta = 1) */
+static void spring_of(double omega, double zeta, double h, double theta, double *rate,
+                      double *softness) {
+  const double b = h * omega;
+  const double a = theta * b * b + 2.0 * zeta * b;
+  if (a > 0.0) {
+    *rate = h * omega * omega / a;
+    *softness = 1.0 / a;
+  } else {
+    *rate = 0.0;
+    *softness = 0.0;
+  }
+}
+
+static void symmetric_inverse(const double *block, const int *inert, double *out) {
+  double m[ROWS * ROWS];
+  memcpy(m, block, sizeof m);
+  memset(out, 0, ROWS * ROWS * sizeof(double));
+  for (uint32_t r = 0; r < ROWS; ++r) {
+    if (inert[r]) {
+      for (uint32_t k = 0; k < ROWS; ++k) {
+        m[ROWS * r + k] = 0.0;
+        m[ROWS * k + r] = 0.0;
+      }
+      m[(ROWS + 1) * r] = 1.0;
+    }
+    out[(ROWS + 1) * r] = 1.0;
+  }
+  for (uint32_t col = 0; col < ROWS; ++col) {
+    const double pivot = m[(ROWS + 1) * col];
+    if (!(pivot > 0.0)) {
+      memset(out, 0, ROWS * ROWS * sizeof(double));
+      return;
+    }
+    for (uint32_t k = 0; k < ROWS; ++k) {
+      m[ROWS * col + k] /= pivot;
+      out[ROWS * col + k] /= pivot;
+    }
+    for (uint32_t r = 0; r < ROWS; ++r) {
+      const double f = m[ROWS * r + col];
+      if (r == col || f == 0.0) continue;
+      for (uint32_t k = 0; k < ROWS; ++k) {
+        m[ROWS * r + k] -= f * m[ROWS * col + k];
+        out[ROWS * r + k] -= f * out[ROWS * col + k];
+      }
+    }
+  }
+  for (uint32_t r = 0; r < ROWS; ++r)
+    if (inert[r])
+      for (uint32_t k = 0; k < ROWS; ++k) {
+        out[ROWS * r + k] = 0.0;
+        out[ROWS * k + r] = 0.0;
+      }
+}
+
+int orc_hex_make(orc_hex *e, uint32_t cell, const uint32_t *nodes, const double *inv_mass,
+                 const double *shape, const double *hourglass, double volume, double shear,
+                 double first, double h) {
+  const double det = orc_det3(shape);
+  if (!(fabs(det) > 0.0) || !(shear > 0.0) || !(volume > 0.0)) return 0;
+  memset(e, 0, sizeof *e);
+  double inv[9];
+  inverse3(shape, inv);
+  e->cell = cell;
+  e->size = cbrt(8.0 * fabs(det));
+  e->bend = 1.0 / (4.0 * e->size);
+  memcpy(e->rest_hg, hourglass, sizeof e->rest_hg);
+  for (uint32_t c = 0; c < 8; ++c) {
+    double s[3], g[3];
+    signs_of(c, s);
+    e->nodes[c] = nodes[c];
+    e->inv_mass[c] = inv_mass[c];
+    mtv3(inv, s, g); /* shape^-T s */
+    for (int r = 0; r < 3; ++r) e->gradient[3 * c + r] = g[r] / 8.0;
+  }
+  double block[ROWS * ROWS];
+  memset(block, 0, sizeof block);
+  double vweight = 0.0;
+  for (uint32_t c = 0; c < 8; ++c) {
+    const double w = e->inv_mass[c];
+    if (w == 0.0) continue;
+    vweight += w * dot3(e->gradient + 3 * c, e->gradient + 3 * c);
+    for (uint32_t r = 0; r < ROWS; ++r) {
+      double jl[3];
+      row_jacobian(e, c, r, jl);
+      for (uint32_t k = 0; k < ROWS; ++k) {
+        double jr[3];
+        row_jacobian(e, c, k, jr);
+        block[ROWS * r + k] += w * dot3(jl, jr);
+      }
+    }
+  }
+  if (!(vweight > 0.0)) return 0;
+  const double hg = shear * (3.0 * first + 2.0 * shear) / (3.0 * (first + shear));
+  int inert[ROWS];
+  for (uint32_t r = 0; r < ROWS; ++r) {
+    const double modulus = r < 3 ? 2.0 * shear : (r < 6 ? 4.0 * shear : hg);
+    const double weight = block[(ROWS + 1) * r];
+    const double omega = sqrt(modulus * volume * weight);
+    e->stiffness[r] = modulus * volume;
+    inert[r] = 0;
+    if (omega > 0.0) {
+      double rate, soft;
+      spring_of(omega, 1.0, h, 1.0, &rate, &soft);
+      e->rate[r] = rate;
+      e->softness[r] = weight * soft;
+      block[(ROWS + 1) * r] += e->softness[r];
+      e->cap[r] = MAX_CORRECTIVE_SPEED / (rate * e->size);
+    } else {
+      inert[r] = 1;
+      e->cap[r] = INFINITY;
+    }
+  }
+  symmetric_inverse(block, inert, e->inverse);
+  e->vol_stiffness = first * volume;
+  e->vol_cap = INFINITY;
+  const double vomega = sqrt(e->vol_stiffness * vweight);
+  if (vomega > 0.0) {
+    double rate, soft;
+    spring_of(vomega, 1.0, h, 1.0, &rate, &soft);
+    e->vol_rate = rate;
+    e->vol_softness = soft;
+    e->vol_cap = rate > 0.0 ? MAX_CORRECTIVE_SPEED / (rate * e->size) : INFINITY;
+    e->volumetric = 1;
+  }
+  e->quat[0] = 1.0;
+  e->turn[0] = e->turn[4] = e->turn[8] = 1.0;
+  return 1;
+}
+
+/* ---- quaternions (w, x, y, z) ------------------------------------------ */
+
+static void quat_normalize(double *q) {
+  const double len = sqrt(q[0] * q[0] + q[1] * q[1] + q[2] * q[2] + q[3] * q[3]);
+  if (len <= 0.0) {
+    q[0] = 1.0;
+    q[1] = q[2] = q[3] = 0.0;
+    return;
+  }
+  const double s = 1.0 / len;
+  for (int k = 0; k < 4; ++k) q[k] *= s;
+}
+
+/* Hamilton product p q */
+static void quat_mul(const double *p, const double *q, double *out) {
+  out[0] = p[0] * q[0] - p[1] * q[1] - p[2] * q[2] - p[3] * q[3];
+  out[1] = p[0] * q[1] + p[1] * q[0] + p[2] * q[3] - p[3] * q[2];
+  out[2] = p[0] * q[2] + p[2] * q[0] + p[3] * q[1] - p[1] * q[3];
+  out[3] = p[0] * q[3] + p[3] * q[0] + p[1] * q[2] - p[2] * q[1];
+}
+
+void orc_quat_matrix(const double *q, double *R) {
+  const double w = q[0], x = q[1], y = q[2], z = q[3];
+  const double xx = x * x, yy = y * y, zz = z * z, xy = x * y, xz = x * z, yz = y * z;
+  const double wx = w * x, wy = w * y, wz = w * z;
+  R[0] = 1.0 - 2.0 * (yy + zz);
+  R[1] = 2.0 * (xy - wz);
+  R[2] = 2.0 * (xz + wy);
+  R[3] = 2.0 * (xy + wz);
+  R[4] = 1.0 - 2.0 * (xx + zz);
+  R[5] = 2.0 * (yz - wx);
+  R[6] = 2.0 * (xz - wy);
+  R[7] = 2.0 * (yz + wx);
+  R[8] = 1.0 - 2.0 * (xx + yy);
+}
+
+void orc_rotation_of(const double *F, const double *guess, uint32_t iterations, double *out) {
+  double q[4] = {guess[0], guess[1], guess[2], guess[3]};
+  quat_normalize(q);
+  for (uint32_t it = 0; it < iterations; ++it) {
+    double T[9];
+    orc_quat_matrix(q, T);
+    /* torque = sum_k T_col_k x F_col_k, scale = |sum_k T_col_k . F_col_k| */
+    double torque[3] = {0.0, 0.0, 0.0}, tr = 0.0;
+    for (int k = 0; k < 3; ++k) {
+      const double tc[3] = {T[k], T[3 + k], T[6 + k]};
+      const double fc[3] = {F[k], F[3 + k], F[6 + k]};
+      double cr[3];
+      cross3(tc, fc, cr);
+      if (k == 0) {
+        torque[0] = cr[0];
+        torque[1] = cr[1];
+        torque[2] = cr[2];
+        tr = dot3(tc, fc);
+      } else {
+        torque[0] += cr[0];
+        torque[1] += cr[1];
+        torque[2] += cr[2];
+        tr += dot3(tc, fc);
+      }
+    }
+    const double scale = fabs(tr) + 1.0e-9;
+    const double om[3] = {torque[0] / scale, torque[1] / scale, torque[2] / scale};
+    const double angle = sqrt(dot3(om, om));
+    if (!(angle > 1.0e-12)) break;
+    const double half = angle * 0.5, s = sin(half);
+    const double turn[4] = {cos(half), om[0] / angle * s, om[1] / angle * s, om[2] / angle * s};
+    double next[4];
+    quat_mul(turn, q, next);
+    memcpy(q, next, sizeof q);
+    quat_normalize(q);
+    if (angle <= ROTATION_TOLERANCE) break;
+  }
+  memcpy(out, q, sizeof q);
+}
+
+/* ---- element state ------------------------------------------------------ */
+
+static double clampd(double v, double lo, double hi) { return v < lo ? lo : (hi < v ? hi : v); }
+
+void orc_hex_update(orc_hex *e, const double *pos) {
+  double F[9] = {0}, hgv[12] = {0};
+  for (uint32_t c = 0; c < 8; ++c) {
+    const double *x = pos + 3 * (size_t)e->nodes[c];
+    const double *g = e->gradient + 3 * c;
+    for (int r = 0; r < 3; ++r)
+      for (int k = 0; k < 3; ++k) F[3 * r + k] += x[r] * g[k];
+    for (uint32_t m = 0; m < 4; ++m) {
+      const double sg = orc_hourglass_sign(m, c);
+      for (int r = 0; r < 3; ++r) hgv[3 * m + r] += sg * x[r];
+    }
+  }
+  orc_rotation_of(F, e->quat, ROTATION_ITERATIONS, e->quat);
+  orc_quat_matrix(e->quat, e->turn);
+  const double *T = e->turn;
+  double L[9]; /* T^T F */
+  for (int r = 0; r < 3; ++r)
+    for (int k = 0; k < 3; ++k) L[3 * r + k] = T[r] * F[k] + T[3 + r] * F[3 + k] + T[6 + r] * F[6 + k];
+  const double det = orc_det3(F);
+  const int inverted = det < 0.0;
+  double raw[ROWS];
+  raw[0] = L[0] - 1.0;
+  raw[1] = L[4] - 1.0;
+  raw[2] = L[8] - 1.0;
+  raw[3] = 0.5 * (L[1] + L[3]);
+  raw[4] = 0.5 * (L[5] + L[7]);
+  raw[5] = 0.5 * (L[6] + L[2]);
+  for (uint32_t m = 0; m < 4; ++m) {
+    double b[3];
+    mtv3(T, hgv + 3 * m, b);
+    for (int k = 0; k < 3; ++k) raw[6 + 3 * m + k] = (b[k] - e->rest_hg[3 * m + k]) * e->bend;
+  }
+  for (uint32_t r = 0; r < ROWS; ++r) {
+    const double cap = inverted ? e->cap[r] : (e->cap[r] < 1.0 ? 1.0 : e->cap[r]);
+    e->strain[r] = clampd(raw[r], -cap, cap);
+  }
+  /* cofactor columns: F1 x F2, F2 x F0, F0 x F1 */
+  double cof[9];
+  {
+    const double f0[3] = {F[0], F[3], F[6]}, f1[3] = {F[1], F[4], F[7]}, f2[3] = {F[2], F[5], F[8]};
+    double c0[3], c1[3], c2[3];
+    cross3(f1, f2, c0);
+    cross3(f2, f0, c1);
+    cross3(f0, f1, c2);
+    for (int r = 0; r < 3; ++r) {
+      cof[3 * r] = c0[r];
+      cof[3 * r + 1] = c1[r];
+      cof[3 * r + 2] = c2[r];
+    }
+  }
+  double vweight = 0.0;
+  memset(e->coupling, 0, sizeof e->coupling);
+  for (uint32_t c = 0; c < 8; ++c) {
+    double vg[3], b[3], l[3];
+    mv3(cof, e->gradient + 3 * c, vg);
+    const double w = e->inv_mass[c];
+    memcpy(e->vol_gradient + 3 * c, vg, sizeof vg);
+    vweight += w * dot3(vg, vg);
+    mtv3(T, vg, b);
+    for (int k = 0; k < 3; ++k) l[k] = b[k] * w;
+    add_rows(e->coupling, e, c, l);
+  }
+  const double vcap = inverted ? e->vol_cap : (e->vol_cap < 1.0 ? 1.0 : e->vol_cap);
+  const double volume = clampd(det - 1.0, -vcap, vcap);
+  double coupled = 0.0;
+  for (uint32_t r = 0; r < ROWS; ++r) {
+    double t = 0.0;
+    for (uint32_t k = 0; k < ROWS; ++k) t += e->inverse[ROWS * r + k] * e->coupling[k];
+    e->image[r] = t;
+    coupled += e->coupling[r] * t;
+  }
+  e->gain = vweight * e->vol_softness;
+  const double schur = vweight + e->gain - coupled;
+  e->schur = e->volumetric && schur > 0.0 ? 1.0 / schur : 0.0;
+  e->rhs = volume * e->vol_rate;
+}
+
+void orc_hex_apply(const orc_hex *e, double *vel, const double *impulse, double vol_impulse) {
+  for (uint32_t c = 0; c < 8; ++c) {
+    const double w = e->inv_mass[c];
+    if (w == 0.0) continue;
+    double lf[3], f[3];
+    row_force(e, c, impulse, lf);
+    mv3(e->turn, lf, f);
+    double *v = vel + 3 * (size_t)e->nodes[c];
+    for (int k = 0; k < 3; ++k) v[k] -= (f[k] + e->vol_gradient[3 * c + k] * vol_impulse) * w;
+  }
+}
+
+void orc_hex_solve(orc_hex *e, double *vel) {
+  double res[ROWS];
+  memset(res, 0, sizeof res);
+  double vspeed = 0.0;
+  for (uint32_t c = 0; c < 8; ++c) {
+    const double *v = vel + 3 * (size_t)e->nodes[c];
+    double b[3];
+    vspeed += dot3(e->vol_gradient + 3 * c, v);
+    mtv3(e->turn, v, b);
+    add_rows(res, e, c, b);
+  }
+  for (uint32_t r = 0; r < ROWS; ++r)
+    res[r] += e->rate[r] * e->strain[r] - e->softness[r] * e->impulse[r];
+  double sol[ROWS];
+  double coupled = 0.0;
+  for (uint32_t r = 0; r < ROWS; ++r) {
+    double t = 0.0;
+    for (uint32_t k = 0; k < ROWS; ++k) t += e->inverse[ROWS * r + k] * res[k];
+    sol[r] = t;
+    coupled += e->coupling[r] * t;
+  }
+  const double volume = e->rhs - e->gain * e->vol_impulse + vspeed;
+  const double change = e->schur * (volume - coupled);
+  for (uint32_t r = 0; r < ROWS; ++r) {
+    sol[r] -= e->image[r] * change;
+    e->impulse[r] += sol[r];
+  }
+  e->vol_impulse += change;
+  orc_hex_apply(e, vel, sol, change);
+}
+
+/* ---- hex lattice mesh --------------------------------------------------- */
+
+/* laneOf (solver.cc:30-37): a lane past the table reads the default material */
+static const orc_material *mat_of(const orc_material *m, uint32_t lanes, uint32_t lane,
+                                  orc_material *fallback) {
+  if (lane < lanes) return &m[lane];
+  memset(fallback, 0, sizeof *fallback);
+  fallback->friction = 0.5f;
+  fallback->density = 1000.0f;
+  fallback->strength = INFINITY;
+  fallback->poissons_ratio = 0.3f;
+  fallback->yield_stress = INFINITY;
+  fallback->damping_ratio = INFINITY;
+  fallback->tear_strain = INFINITY;
+  fallback->plastic_max = INFINITY;
+  fallback->crush_stress = INFINITY;
+  fallback->crush_limit = 1.0f;
+  return fallback;
+}
+
+int orc_build_hex_lattice(const orc_lattice *L, const orc_material *materials, uint32_t lanes,
+                          const orc_support *supports, uint32_t nsupports, orc_hex_mesh *out) {
+  memset(out, 0, sizeof *out);
+  if (L->res[0] < 2 || L->res[1] < 2 || L->res[2] < 2) return -1;
+  const uint32_t cx = L->res[0] - 1, cy = L->res[1] - 1, cz = L->res[2] - 1;
+  const size_t samples = (size_t)L->res[0] * L->res[1] * L->res[2];
+  uint32_t *node_of = (uint32_t *)malloc(samples * sizeof(uint32_t));
+  for (size_t s = 0; s < samples; ++s) node_of[s] = UINT32_MAX;
+  uint32_t nodes = 0, hexes = 0;
+  for (uint32_t z = 0; z < cz; ++z)
+    for (uint32_t y = 0; y < cy; ++y)
+      for (uint32_t x = 0; x < cx; ++x) {
+        const size_t cell = x + (size_t)cx * (y + (size_t)cy * z);
+        if (L->solid && !L->solid[cell]) continue;
+        ++hexes;
+        for (uint32_t c = 0; c < 8; ++c) {
+          const size_t s = (x + (c & 1u)) + (size_t)L->res[0] * ((y + ((c >> 1) & 1u)) +
+                                                                 (size_t)L->res[1] * (z + ((c >> 2) & 1u)));
+          if (node_of[s] == UINT32_MAX) node_of[s] = nodes++;
+        }
+      }
+  if (nodes == 0) {
+    free(node_of);
+    return -1;
+  }
+  out->nodes = nodes;
+  out->hexes = hexes;
+  for (int k = 0; k < 3; ++k) out->voxel[k] = L->voxel[k];
+  out->rest = (double *)calloc(3 * (size_t)nodes, sizeof(double));
+  out->mass = (float *)calloc(nodes, sizeof(float));
+  out->anchored = (uint8_t *)calloc(nodes, 1);
+  out->hex = (uint32_t *)malloc(8 * (size_t)hexes * sizeof(uint32_t));
+  out->volume = (float *)malloc((size_t)hexes * sizeof(float));
+  out->lane = (uint32_t *)malloc((size_t)hexes * sizeof(uint32_t));
+  float *restf = (float *)malloc(3 * (size_t)nodes * sizeof(float));
+  for (uint32_t az = 0; az < L->res[2]; ++az)
+    for (uint32_t ay = 0; ay < L->res[1]; ++ay)
+      for (uint32_t ax = 0; ax < L->res[0]; ++ax) {
+        const size_t s = ax + (size_t)L->res[0] * (ay + (size_t)L->res[1] * az);
+        const uint32_t n = node_of[s];
+        if (n == UINT32_MAX) continue;
+        float p[3];
+        if (L->sample_pos) {
+          p[0] = L->sample_pos[3 * s + 0];
+          p[1] = L->sample_pos[3 * s + 1];
+          p[2] = L->sample_pos[3 * s + 2];
+        } else {
+          p[0] = L->origin[0] + L->voxel[0] * (float)ax;
+          p[1] = L->origin[1] + L->voxel[1] * (float)ay;
+          p[2] = L->origin[2] + L->voxel[2] * (float)az;
+        }
+        for (int k = 0; k < 3; ++k) {
+          restf[3 * n + k] = p[k];
+          out->rest[3 * n + k] = (double)p[k];
+        }
+      }
+  for (uint32_t n = 0; n < nodes; ++n)
+    for (uint32_t v = 0; v < nsupports; ++v) {
+      const float *M = supports[v].world_to_support;
+      int inside = 1;
+      for (int r = 0; r < 3 && inside; ++r) {
+        const float l = M[r * 4 + 0] * restf[3 * n + 0] + M[r * 4 + 1] * restf[3 * n + 1] +
+                        M[r * 4 + 2] * restf[3 * n + 2] + M[r * 4 + 3];
+        if (!(fabsf(l) <= supports[v].half[r])) inside = 0;
+      }
+      if (inside) {
+        out->anchored[n] = 1;
+        break;
+      }
+    }
+  uint32_t h = 0;
+  orc_material fb;
+  for (uint32_t z = 0; z < cz; ++z)
+    for (uint32_t y = 0; y < cy; ++y)
+      for (uint32_t x = 0; x < cx; ++x) {
+        const size_t cell = x + (size_t)cx * (y + (size_t)cy * z);
+        if (L->solid && !L->solid[cell]) continue;
+        const uint32_t lane = L->cell_lane ? L->cell_lane[cell] : 0u;
+        const float density = mat_of(materials, lanes, lane, &fb)->density;
+        const float vol = 1.0f * L->voxel[0] * L->voxel[1] * L->voxel[2];
+        const float share = density * vol * 0.125f;
+        for (uint32_t c = 0; c < 8; ++c) {
+          const size_t s = (x + (c & 1u)) + (size_t)L->res[0] * ((y + ((c >> 1) & 1u)) +
+                                                                 (size_t)L->res[1] * (z + ((c >> 2) & 1u)));
+          out->hex[8 * (size_t)h + c] = node_of[s];
+          out->mass[node_of[s]] += share;
+        }
+        out->volume[h] = vol;
+        out->lane[h] = lane;
+        ++h;
+      }
+  free(restf);
+  free(node_of);
+  return 0;
+}
+
+void orc_hex_mesh_free(orc_hex_mesh *m) {
+  if (!m) return;
+  free(m->rest);
+  free(m->mass);
+  free(m->anchored);
+  free(m->hex);
+  free(m->volume);
+  free(m->lane);
+  memset(m, 0, sizeof *m);
+}
+
+/* ---- lattice body ------------------------------------------------------- */
+
+struct orc_lattice_body {
+  uint32_t nodes, hexes, lanes;
+  double *rest;
+  float *mass;
+  uint8_t *anchored;
+  uint32_t *hex;
+  float *volume;
+  uint32_t *lane;
+  orc_material *materials;
+  double cell; /* min voxel */
+  double *x, *v;
+  uint8_t *node_colour, *hex_colour;
+  uint32_t substeps; /* LatticeBody::substeps */
+  /* cached elements (StrainLattice) */
+  double h;          /* 0 = not built */
+  orc_hex *elements;
+  uint32_t count;
+};
+
+orc_lattice_body *orc_lattice_create(const orc_hex_mesh *mesh, const orc_material *materials,
+                                     uint32_t lanes, const double *offset,
+                                     const double *velocity) {
+  if (lanes == 0) return NULL; /* strainstep.cc:87-89 throws without lanes */
+  orc_lattice_body *b = (orc_lattice_body *)calloc(1, sizeof *b);
+  b->nodes = mesh->nodes;
+  b->hexes = mesh->hexes;
+  b->lanes = lanes;
+  const size_t n = mesh->nodes;
+  b->rest = (double *)malloc(3 * n * sizeof(double));
+  memcpy(b->rest, mesh->rest, 3 * n * sizeof(double));
+  b->mass = (float *)malloc(n * sizeof(float));
+  memcpy(b->mass, mesh->mass, n * sizeof(float));
+  b->anchored = (uint8_t *)malloc(n);
+  memcpy(b->anchored, mesh->anchored, n);
+  b->hex = (uint32_t *)malloc(8 * (size_t)mesh->hexes * sizeof(uint32_t));
+  memcpy(b->hex, mesh->hex, 8 * (size_t)mesh->hexes * sizeof(uint32_t));
+  b->volume = (float *)malloc((size_t)mesh->hexes * sizeof(float));
+  memcpy(b->volume, mesh->volume, (size_t)mesh->hexes * sizeof(float));
+  b->lane = (uint32_t *)malloc((size_t)mesh->hexes * sizeof(uint32_t));
+  memcpy(b->lane, mesh->lane, (size_t)mesh->hexes * sizeof(uint32_t));
+  b->materials = (orc_material *)malloc(lanes * sizeof(orc_material));
+  memcpy(b->materials, materials, lanes * sizeof(orc_material));
+  double vox[3] = {mesh->voxel[0], mesh->voxel[1], mesh->voxel[2]};
+  b->cell = vox[0] < vox[1] ? vox[0] : vox[1];
+  if (vox[2] < b->cell) b->cell = vox[2];
+  b->x = (double *)malloc(3 * n * sizeof(double));
+  b->v = (double *)malloc(3 * n * sizeof(double));
+  b->node_colour = (uint8_t *)malloc(n);
+  b->hex_colour = (uint8_t *)malloc(mesh->hexes ? mesh->hexes : 1);
+  double lo[3] = {INFINITY, INFINITY, INFINITY};
+  for (size_t i = 0; i < n; ++i)
+    for (int k = 0; k < 3; ++k)
+      if ((double)(float)b->rest[3 * i + k] < lo[k]) lo[k] = (double)(float)b->rest[3 * i + k];
+  for (size_t i = 0; i < n; ++i) {
+    uint32_t colour = 0;
+    for (int k = 0; k < 3; ++k) {
+      b->x[3 * i + k] = b->rest[3 * i + k] + offset[k];
+      b->v[3 * i + k] = b->anchored[i] ? 0.0 : velocity[k];
+      const double index = round((b->rest[3 * i + k] - lo[k]) / vox[k]);
+      colour |= ((uint32_t)index & 1u) << k;
+    }
+    b->node_colour[i] = (uint8_t)colour;
+  }
+  for (uint32_t c = 0; c < mesh->hexes; ++c) b->hex_colour[c] = b->node_colour[b->hex[8 * (size_t)c]];
+  return b;
+}
+
+void orc_lattice_free(orc_lattice_body *b) {
+  if (!b) return;
+  free(b->rest);
+  free(b->mass);
+  free(b->anchored);
+  free(b->hex);
+  free(b->volume);
+  free(b->lane);
+  free(b->materials);
+  free(b->x);
+  free(b->v);
+  free(b->node_colour);
+  free(b->hex_colour);
+  free(b->elements);
+  free(b);
+}
+
+void orc_lattice_set_substeps(orc_lattice_body *b, uint32_t substeps) { b->substeps = substeps; }
+
+void orc_lattice_get_state(const orc_lattice_body *b, double *x, double *v) {
+  if (x) memcpy(x, b->x, 3 * (size_t)b->nodes * sizeof(double));
+  if (v) memcpy(v, b->v, 3 * (size_t)b->nodes * sizeof(double));
+}
+
+void orc_lattice_set_state(orc_lattice_body *b, const double *x, const double *v) {
+  if (x) memcpy(b->x, x, 3 * (size_t)b->nodes * sizeof(double));
+  if (v) memcpy(b->v, v, 3 * (size_t)b->nodes * sizeof(double));
+}
+
+void orc_lattice_colours(const orc_lattice_body *b, uint8_t *node_colour, uint8_t *hex_colour) {
+  if (node_colour) memcpy(node_colour, b->node_colour, b->nodes);
+  if (hex_colour) memcpy(hex_colour, b->hex_colour, b->hexes);
+}
+
+uint32_t orc_lattice_elements(const orc_lattice_body *b) { return b->count; }
+
+void orc_lattice_impulses(const orc_lattice_body *b, uint32_t *cells, double *impulses) {
+  for (uint32_t i = 0; i < b->count; ++i) {
+    if (cells) cells[i] = b->elements[i].cell;
+    if (impulses) {
+      memcpy(impulses + 19 * (size_t)i, b->elements[i].impulse, ROWS * sizeof(double));
+      impulses[19 * (size_t)i + ROWS] = b->elements[i].vol_impulse;
+    }
+  }
+}
+
+static const orc_material *lane_mat(const orc_lattice_body *b, uint32_t cell) {
+  const uint32_t lane = b->lane[cell];
+  return &b->materials[lane < b->lanes ? lane : b->lanes - 1];
+}
+
+static int pinned(const orc_lattice_body *b, size_t node) {
+  return !(b->mass[node] > 0.0f) || b->anchored[node] != 0;
+}
+
+static double stiffest(const orc_lattice_body *b) {
+  double f = 0.0;
+  for (uint32_t c = 0; c < b->hexes; ++c) {
+    const orc_material *m = lane_mat(b, c);
+    const double g = orc_hex_cell_frequency(m->youngs_modulus, m->poissons_ratio, m->density, b->cell);
+    f = f < g ? g : f; /* std::max(f, g) */
+  }
+  return f;
+}
+
+uint32_t orc_lattice_substeps(const orc_lattice_body *b, double dt, uint32_t softstep) {
+  const double f = stiffest(b);
+  uint32_t needed = 1;
+  if (f > 0.0 && isfinite(f)) {
+    needed = (uint32_t)clampd(ceil(dt * f / RESOLVED), 1.0, (double)MOST_SUBSTEPS);
+    while (needed < MOST_SUBSTEPS && dt / (double)needed * f > RESOLVED) ++needed;
+  }
+  uint32_t s = softstep > 1 ? softstep : 1;
+  if (b->substeps > s) s = b->substeps;
+  if (needed > s) s = needed;
+  return s;
+}
+
+/* the proper rotation of the mass-weighted moment of positions about the
+ * centre of mass against rest positions (the lattice turn) */
+static void lattice_turn(const orc_lattice_body *b, double *q) {
+  double c[3] = {0, 0, 0}, rc[3] = {0, 0, 0}, total = 0.0;
+  for (size_t i = 0; i < b->nodes; ++i) {
+    const double m = b->mass[i];
+    for (int k = 0; k < 3; ++k) {
+      c[k] += m * b->x[3 * i + k];
+      rc[k] += m * b->rest[3 * i + k];
+    }
+    total += m;
+  }
+  q[0] = 1.0;
+  q[1] = q[2] = q[3] = 0.0;
+  if (!(total > 0.0)) return;
+  for (int k = 0; k < 3; ++k) {
+    c[k] /= total;
+    rc[k] /= total;
+  }
+  double M[9] = {0};
+  for (size_t i = 0; i < b->nodes; ++i) {
+    const double m = b->mass[i];
+    for (int r = 0; r < 3; ++r)
+      for (int k = 0; k < 3; ++k)
+        M[3 * r + k] += m * (b->x[3 * i + r] - c[r]) * (b->rest[3 * i + k] - rc[k]);
+  }
+  const double id[4] = {1.0, 0.0, 0.0, 0.0};
+  orc_rotation_of(M, id, 64, q);
+}
+
+static void build_elements(orc_lattice_body *b, double h) {
+  double *inv = (double *)calloc(b->nodes, sizeof(double));
+  for (size_t i = 0; i < b->nodes; ++i)
+    if (!pinned(b, i) && b->mass[i] > 0.0f) inv[i] = 1.0 / (double)b->mass[i];
+  double turn[4];
+  lattice_turn(b, turn);
+  orc_hex *fresh = (orc_hex *)malloc((b->hexes ? b->hexes : 1) * sizeof(orc_hex));
+  uint32_t count = 0;
+  for (uint32_t cell = 0; cell < b->hexes; ++cell) {
+    const uint32_t *nd = b->hex + 8 * (size_t)cell;
+    double corners[24], w[8], shape[9], hg[12], shear, first;
+    for (uint32_t c = 0; c < 8; ++c) {
+      w[c] = inv[nd[c]];
+      for (int k = 0; k < 3; ++k) corners[3 * c + k] = b->rest[3 * (size_t)nd[c] + k];
+    }
+    orc_hex_corner_rest(corners, shape, hg);
+    const orc_material *m = lane_mat(b, cell);
+    orc_hex_material(m->youngs_modulus, m->poissons_ratio, &shear, &first);
+    if (orc_hex_make(&fresh[count], cell, nd, w, shape, hg, (double)b->volume[cell], shear, first, h)) {
+      memcpy(fresh[count].quat, turn, sizeof turn);
+      orc_quat_matrix(turn, fresh[count].turn);
+      ++count;
+    }
+  }
+  free(inv);
+  free(b->elements);
+  b->elements = fresh;
+  b->count = count;
+  b->h = h;
+}
+
+uint32_t orc_lattice_step(orc_lattice_body *b, double dt, const double *g, uint32_t softstep,
+                          uint32_t colored) {
+  const uint32_t substeps = orc_lattice_substeps(b, dt, softstep);
+  const double h = dt / (double)substeps;
+  if (b->h != h) build_elements(b, h); /* fresh elements: impulses start at zero */
+  const int exact = !(h * stiffest(b) > RESOLVED);
+  /* sweep order: cell order, or colour by colour */
+  uint32_t *order = (uint32_t *)malloc((b->count ? b->count : 1) * sizeof(uint32_t));
+  uint32_t n = 0;
+  if (colored) {
+    for (uint32_t col = 0; col < 8; ++col)
+      for (uint32_t i = 0; i < b->count; ++i)
+        if (b->hex_colour[b->elements[i].cell] == col) order[n++] = i;
+  } else {
+    for (uint32_t i = 0; i < b->count; ++i) order[n++] = i;
+  }
+  for (uint32_t s = 0; s < substeps; ++s) {
+    for (size_t i = 0; i < b->nodes; ++i)
+      for (int k = 0; k < 3; ++k) b->v[3 * i + k] = pinned(b, i) ? 0.0 : b->v[3 * i + k] + h * g[k];
+    if (exact)
+      for (uint32_t i = 0; i < n; ++i) {
+        orc_hex *e = &b->elements[order[i]];
+        orc_hex_update(e, b->x);
+        orc_hex_apply(e, b->v, e->impulse, e->vol_impulse);
+      }
+    for (uint32_t pass = 0; pass < PASSES; ++pass)
+      for (uint32_t i = 0; i < n; ++i) {
+        orc_hex *e = &b->elements[order[i]];
+        if (pass == 0 && !exact) {
+          orc_hex_update(e, b->x);
+          orc_hex_apply(e, b->v, e->impulse, e->vol_impulse);
+        }
+        orc_hex_solve(e, b->v);
+      }
+    for (size_t i = 0; i < b->nodes; ++i) {
+      double *v = b->v + 3 * i;
+      const double speed = sqrt(dot3(v, v));
+      if (speed > MAX_SPEED)
+        for (int k = 0; k < 3; ++k) v[k] *= MAX_SPEED / speed;
+      for (int k = 0; k < 3; ++k) b->x[3 * i + k] += h * v[k];
+    }
+  }
+  free(order);
+  return substeps;
+}

@@ -140,6 +140,46 @@ class Event(C.Structure):
                 ("step", C.c_uint32)]
 
 
+class _HexMesh(C.Structure):
+    _fields_ = [
+        ("nodes", C.c_uint32), ("hexes", C.c_uint32),
+        ("rest_positions", C.POINTER(C.c_float)), ("node_mass", C.POINTER(C.c_float)),
+        ("anchored", C.POINTER(C.c_uint8)), ("hex_nodes", C.POINTER(C.c_uint32)),
+        ("volume", C.POINTER(C.c_float)), ("hex_material", C.POINTER(C.c_uint32)),
+        ("voxel_size", C.c_float * 3),
+    ]
+
+
+class HexElement(C.Structure):
+    """makeElement's outputs (StrainElement, strainstep/element.h:40-73)."""
+
+    _fields_ = [
+        ("gradient", C.c_double * 24), ("inverse_mass", C.c_double * 8),
+        ("rest_hourglass", C.c_double * 12), ("bend", C.c_double), ("size", C.c_double),
+        ("stiffness", C.c_double * 18), ("rate", C.c_double * 18),
+        ("softness", C.c_double * 18), ("cap", C.c_double * 18),
+        ("inverse", C.c_double * 324), ("volume_stiffness", C.c_double),
+        ("volume_rate", C.c_double), ("volume_softness", C.c_double),
+        ("volume_cap", C.c_double), ("volumetric", C.c_uint32), ("reserved", C.c_uint32),
+    ]
+
+    def arr(self, name):
+        return np.ctypeslib.as_array(getattr(self, name)).copy()
+
+
+class LatticeParams(C.Structure):
+    _fields_ = [("dt", C.c_double), ("gravity", C.c_double * 3),
+                ("softstep_substeps", C.c_uint32), ("flags", C.c_uint32)]
+
+
+class LatticeReport(C.Structure):
+    _fields_ = [("lattices", C.c_uint32), ("substeps", C.c_uint32), ("elements", C.c_uint32),
+                ("rebuilds", C.c_uint32), ("element_solves", C.c_uint64), ("device_ms", C.c_double)]
+
+    def as_dict(self):
+        return {k: getattr(self, k) for k, _ in self._fields_}
+
+
 # every entry point of include/talus_mi.h with its ctypes signature
 _P = C.POINTER
 _SIGNATURES = {
@@ -194,6 +234,21 @@ _SIGNATURES = {
     "talus_mi_surface_get": (C.c_int, [C.c_void_p, C.c_uint32, C.c_void_p]),
     "talus_mi_surface_free": (None, [C.c_void_p]),
     "talus_mi_stream": (C.c_void_p, [C.c_void_p]),
+    "talus_mi_build_hex_lattice": (C.c_int, [_P(_Lattice), _P(PhysicalMaterial), C.c_uint32,
+                                             _P(Support), C.c_uint32, _P(_HexMesh)]),
+    "talus_mi_hex_mesh_free": (None, [_P(_HexMesh)]),
+    "talus_mi_hex_element_make": (C.c_int, [_P(C.c_double), _P(C.c_double), C.c_double, C.c_float,
+                                            C.c_float, C.c_double, _P(HexElement)]),
+    "talus_mi_lattice_default_params": (None, [_P(LatticeParams)]),
+    "talus_mi_lattice_add": (C.c_int, [C.c_void_p, _P(_HexMesh), _P(PhysicalMaterial), C.c_uint32,
+                                       _P(C.c_double), _P(C.c_double), _P(C.c_uint32)]),
+    "talus_mi_lattice_set_substeps": (C.c_int, [C.c_void_p, C.c_uint32, C.c_uint32]),
+    "talus_mi_lattice_state_get": (C.c_int, [C.c_void_p, C.c_uint32, _P(C.c_double), _P(C.c_double)]),
+    "talus_mi_lattice_state_set": (C.c_int, [C.c_void_p, C.c_uint32, _P(C.c_double), _P(C.c_double)]),
+    "talus_mi_lattice_impulses_get": (C.c_int, [C.c_void_p, C.c_uint32, _P(C.c_uint32), _P(C.c_double),
+                                                C.c_uint32, _P(C.c_uint32)]),
+    "talus_mi_lattice_colours_get": (C.c_int, [C.c_void_p, C.c_uint32, _P(C.c_uint8), _P(C.c_uint8)]),
+    "talus_mi_lattice_step": (C.c_int, [C.c_void_p, _P(LatticeParams), _P(LatticeReport)]),
 }
 
 _lib = None
@@ -391,6 +446,69 @@ def default_params(**kw) -> StepParams:
     return p
 
 
+def build_hex_lattice(resolution, voxel_size, origin=(0.0, 0.0, 0.0), materials=(), supports=(),
+                      sample_positions=None, cell_lane=None, solid=None):
+    """Hex PreparedMesh of a structured lattice (buildMesh with hex elements);
+    returns a dict of numpy arrays (rest_positions, node_mass, anchored,
+    hex_nodes, volume, hex_material, voxel_size)."""
+    L = lib()
+    lat = _Lattice()
+    lat.resolution[:] = [int(r) for r in resolution]
+    lat.voxel_size[:] = [float(v) for v in voxel_size]
+    lat.origin[:] = [float(v) for v in origin]
+    keep = []
+    for name, arr, ct in (("sample_positions", sample_positions, (np.float32, C.c_float)),
+                          ("cell_lane", cell_lane, (np.uint32, C.c_uint32)),
+                          ("solid", solid, (np.uint8, C.c_uint8))):
+        if arr is not None:
+            a = np.ascontiguousarray(arr, dtype=ct[0]).reshape(-1)
+            keep.append(a)
+            setattr(lat, name, a.ctypes.data_as(C.POINTER(ct[1])))
+    mats = _materials(materials if isinstance(materials, np.ndarray) else list(materials))
+    sups = (Support * max(len(supports), 1))()
+    for i, s in enumerate(supports):
+        sups[i] = s
+    out = _HexMesh()
+    _check(L.talus_mi_build_hex_lattice(C.byref(lat), mats, len(materials), sups, len(supports),
+                                        C.byref(out)))
+    n, h = out.nodes, out.hexes
+    try:
+        mesh = {
+            "rest_positions": np.ctypeslib.as_array(out.rest_positions, (3 * n,)).reshape(n, 3).copy(),
+            "node_mass": np.ctypeslib.as_array(out.node_mass, (n,)).copy(),
+            "anchored": np.ctypeslib.as_array(out.anchored, (n,)).copy(),
+            "hex_nodes": np.ctypeslib.as_array(out.hex_nodes, (8 * h,)).reshape(h, 8).copy(),
+            "volume": np.ctypeslib.as_array(out.volume, (h,)).copy(),
+            "hex_material": np.ctypeslib.as_array(out.hex_material, (h,)).copy(),
+            "voxel_size": np.array(list(out.voxel_size), dtype=np.float32),
+        }
+    finally:
+        L.talus_mi_hex_mesh_free(C.byref(out))
+    return mesh
+
+
+def hex_element_make(corners, inverse_mass, volume, youngs_modulus, poissons_ratio, h) -> HexElement:
+    """makeElement of one hex from its 8 rest corners (cornerRest,
+    elementMaterial, makeElement); raises when the hex builds no element."""
+    c = _f64(corners, 24)
+    w = _f64(inverse_mass, 8)
+    e = HexElement()
+    _check(lib().talus_mi_hex_element_make(_dp(c), _dp(w), float(volume), float(youngs_modulus),
+                                           float(poissons_ratio), float(h), C.byref(e)))
+    return e
+
+
+def lattice_params(**kw) -> LatticeParams:
+    p = LatticeParams()
+    lib().talus_mi_lattice_default_params(C.byref(p))
+    for k, v in kw.items():
+        if k == "gravity":
+            p.gravity[:] = [float(g) for g in v]
+        else:
+            setattr(p, k, v)
+    return p
+
+
 class Context:
     """One device context: bodies, device buffers and a HIP stream."""
 
@@ -400,6 +518,7 @@ class Context:
         self._h = h
         self.precision = precision
         self.bodies = []  # (nodes, tets)
+        self.lattices = []  # (nodes, hexes)
 
     def close(self):
         if getattr(self, "_h", None):
@@ -619,3 +738,73 @@ class Context:
 
     def stream(self) -> int:
         return int(lib().talus_mi_stream(self._h) or 0)
+
+    # ---- strainstep hex lattices ------------------------------------------
+
+    def add_lattice(self, mesh, materials, offset=(0.0, 0.0, 0.0), velocity=(0.0, 0.0, 0.0)) -> int:
+        """addLattice: a hex lattice body at rest + offset moving at velocity."""
+        keep = {
+            "rest": np.ascontiguousarray(mesh["rest_positions"], dtype=np.float32).reshape(-1),
+            "mass": np.ascontiguousarray(mesh["node_mass"], dtype=np.float32),
+            "anchored": np.ascontiguousarray(mesh["anchored"], dtype=np.uint8),
+            "hex": np.ascontiguousarray(mesh["hex_nodes"], dtype=np.uint32).reshape(-1),
+            "volume": np.ascontiguousarray(mesh["volume"], dtype=np.float32),
+            "lane": np.ascontiguousarray(mesh["hex_material"], dtype=np.uint32),
+        }
+        m = _HexMesh()
+        m.nodes = keep["mass"].size
+        m.hexes = keep["volume"].size
+        m.rest_positions = keep["rest"].ctypes.data_as(C.POINTER(C.c_float))
+        m.node_mass = keep["mass"].ctypes.data_as(C.POINTER(C.c_float))
+        m.anchored = keep["anchored"].ctypes.data_as(C.POINTER(C.c_uint8))
+        m.hex_nodes = keep["hex"].ctypes.data_as(C.POINTER(C.c_uint32))
+        m.volume = keep["volume"].ctypes.data_as(C.POINTER(C.c_float))
+        m.hex_material = keep["lane"].ctypes.data_as(C.POINTER(C.c_uint32))
+        m.voxel_size[:] = [float(v) for v in mesh["voxel_size"]]
+        mats = _materials(materials if isinstance(materials, np.ndarray) else list(materials))
+        off, vel = _f64(offset, 3), _f64(velocity, 3)
+        out = C.c_uint32()
+        _check(lib().talus_mi_lattice_add(self._h, C.byref(m), mats, len(materials), _dp(off), _dp(vel),
+                                          C.byref(out)))
+        self.lattices.append((int(m.nodes), int(m.hexes)))
+        return int(out.value)
+
+    def set_lattice_substeps(self, lattice, substeps):
+        _check(lib().talus_mi_lattice_set_substeps(self._h, lattice, int(substeps)))
+
+    def lattice_state(self, lattice):
+        n = self.lattices[lattice][0]
+        x, v = np.empty(3 * n), np.empty(3 * n)
+        _check(lib().talus_mi_lattice_state_get(self._h, lattice, _dp(x), _dp(v)))
+        return x.reshape(n, 3), v.reshape(n, 3)
+
+    def set_lattice_state(self, lattice, x=None, v=None):
+        n = self.lattices[lattice][0]
+        x, v = _f64(x, 3 * n), _f64(v, 3 * n)
+        _check(lib().talus_mi_lattice_state_set(self._h, lattice, _dp(x), _dp(v)))
+
+    def lattice_impulses(self, lattice):
+        """(cells, impulses[elements, 19]) of the lattice's elements."""
+        count = C.c_uint32()
+        _check(lib().talus_mi_lattice_impulses_get(self._h, lattice, None, None, 0, C.byref(count)))
+        k = count.value
+        cells = np.zeros(max(k, 1), dtype=np.uint32)
+        imp = np.zeros(19 * max(k, 1))
+        _check(lib().talus_mi_lattice_impulses_get(self._h, lattice, cells.ctypes.data_as(C.POINTER(C.c_uint32)),
+                                                   _dp(imp), k, C.byref(count)))
+        return cells[:k], imp[: 19 * k].reshape(-1, 19)
+
+    def lattice_colours(self, lattice):
+        n, h = self.lattices[lattice]
+        nc = np.zeros(n, dtype=np.uint8)
+        hc = np.zeros(h, dtype=np.uint8)
+        _check(lib().talus_mi_lattice_colours_get(self._h, lattice, nc.ctypes.data_as(C.POINTER(C.c_uint8)),
+                                                  hc.ctypes.data_as(C.POINTER(C.c_uint8))))
+        return nc, hc
+
+    def lattice_step(self, params: LatticeParams | None = None, **kw) -> dict:
+        """solveStrainstep over every lattice of the context."""
+        p = params if params is not None else lattice_params(**kw)
+        r = LatticeReport()
+        _check(lib().talus_mi_lattice_step(self._h, C.byref(p), C.byref(r)))
+        return r.as_dict()

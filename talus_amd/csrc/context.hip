@@ -27,6 +27,8 @@
 
 #include "factor.h"
 #include "host.h"
+#include "device.h"
+#include "hexlattice.h"
 #include "kernels.h"
 #include "tmath.h"
 
@@ -40,17 +42,6 @@ int fail(int code, const std::string &message) {
   g_error = message;
   return code;
 }
-
-struct DeviceError : std::runtime_error {
-  using std::runtime_error::runtime_error;
-};
-
-#define HIPCHECK(expr)                                                                       \
-  do {                                                                                       \
-    const hipError_t status_ = (expr);                                                       \
-    if (status_ != hipSuccess)                                                               \
-      throw ::tmi::DeviceError(std::string(#expr) + " failed: " + hipGetErrorString(status_));      \
-  } while (0)
 
 int prepare_body(Body &b) {
   const uint32_t lanes = uint32_t(b.materials.size());
@@ -1798,6 +1789,7 @@ struct talus_mi_ctx {
   uint32_t steps = 0;
   std::vector<talus_mi_event> events;
   uint32_t streams = 1;  // engines stepped concurrently (talus_mi_set_streams)
+  std::unique_ptr<tmi::LatticeSet> lattices;  // strainstep hex lattices
 };
 
 namespace {
@@ -1886,6 +1878,7 @@ int talus_mi_destroy(talus_mi_ctx *ctx) {
   if (!ctx) return TALUS_MI_OK;
   (void)hipSetDevice(ctx->device);
   ctx->engine.reset();
+  ctx->lattices.reset();
   if (ctx->stream) (void)hipStreamDestroy(ctx->stream);
   delete ctx;
   return TALUS_MI_OK;
@@ -2450,6 +2443,105 @@ int talus_mi_rollout_backward(talus_mi_ctx *ctx, uint32_t body, const double *x_
       ctx->engine->frames_set(q, ctx->bodies[q].W.data());
       ctx->engine->plastic_set(q, ctx->bodies[q].plastic.data());
     }
+    return TALUS_MI_OK;
+  });
+}
+
+// ---- strainstep hex lattices ----------------------------------------------
+
+void talus_mi_lattice_default_params(talus_mi_lattice_params *p) {
+  if (!p) return;
+  std::memset(p, 0, sizeof *p);
+  p->dt = 1.0 / 60.0;
+  p->gravity[1] = -9.81;
+  p->softstep_substeps = 4;
+}
+
+namespace {
+
+bool lattice_ok(talus_mi_ctx *ctx, uint32_t lattice) {
+  return ctx->lattices && lattice < ctx->lattices->count();
+}
+
+}  // namespace
+
+int talus_mi_lattice_add(talus_mi_ctx *ctx, const talus_mi_hex_mesh *mesh,
+                         const talus_mi_material *materials, uint32_t lanes, const double *offset,
+                         const double *velocity, uint32_t *lattice) {
+  if (!mesh || !lattice) return fail(TALUS_MI_ERR_ARGUMENT, "null argument");
+  return guarded(ctx, [&]() -> int {
+    tmi::LatticeBody b;
+    const int rc = tmi::make_lattice(b, mesh, materials, lanes, offset, velocity);
+    if (rc != TALUS_MI_OK) return rc;
+    if (!ctx->lattices) ctx->lattices = std::make_unique<tmi::LatticeSet>(ctx->stream);
+    *lattice = ctx->lattices->add(std::move(b));
+    return TALUS_MI_OK;
+  });
+}
+
+int talus_mi_lattice_set_substeps(talus_mi_ctx *ctx, uint32_t lattice, uint32_t substeps) {
+  return guarded(ctx, [&]() -> int {
+    if (!lattice_ok(ctx, lattice)) return fail(TALUS_MI_ERR_ARGUMENT, "no such lattice");
+    ctx->lattices->set_substeps(lattice, substeps);
+    return TALUS_MI_OK;
+  });
+}
+
+int talus_mi_lattice_state_get(talus_mi_ctx *ctx, uint32_t lattice, double *x, double *v) {
+  return guarded(ctx, [&]() -> int {
+    if (!lattice_ok(ctx, lattice)) return fail(TALUS_MI_ERR_ARGUMENT, "no such lattice");
+    ctx->lattices->get_state(lattice, x, v);
+    return TALUS_MI_OK;
+  });
+}
+
+int talus_mi_lattice_state_set(talus_mi_ctx *ctx, uint32_t lattice, const double *x, const double *v) {
+  return guarded(ctx, [&]() -> int {
+    if (!lattice_ok(ctx, lattice)) return fail(TALUS_MI_ERR_ARGUMENT, "no such lattice");
+    ctx->lattices->set_state(lattice, x, v);
+    return TALUS_MI_OK;
+  });
+}
+
+int talus_mi_lattice_impulses_get(talus_mi_ctx *ctx, uint32_t lattice, uint32_t *cells, double *impulses,
+                                  uint32_t capacity, uint32_t *count) {
+  if (!count) return fail(TALUS_MI_ERR_ARGUMENT, "null count");
+  return guarded(ctx, [&]() -> int {
+    if (!lattice_ok(ctx, lattice)) return fail(TALUS_MI_ERR_ARGUMENT, "no such lattice");
+    const tmi::LatticeBody &b = ctx->lattices->body(lattice);
+    const uint32_t n = uint32_t(b.elements.size());
+    *count = n;
+    if ((cells || impulses) && capacity < n) return fail(TALUS_MI_ERR_ARGUMENT, "capacity too small");
+    for (uint32_t i = 0; i < n; ++i) {
+      if (cells) cells[i] = b.elements[i].cell;
+      if (impulses) {
+        std::memcpy(impulses + 19 * size_t(i), b.elements[i].impulse, tmi::HEX_ROWS * sizeof(double));
+        impulses[19 * size_t(i) + tmi::HEX_ROWS] = b.elements[i].vol_impulse;
+      }
+    }
+    return TALUS_MI_OK;
+  });
+}
+
+int talus_mi_lattice_colours_get(talus_mi_ctx *ctx, uint32_t lattice, uint8_t *node_colour,
+                                 uint8_t *hex_colour) {
+  return guarded(ctx, [&]() -> int {
+    if (!lattice_ok(ctx, lattice)) return fail(TALUS_MI_ERR_ARGUMENT, "no such lattice");
+    const tmi::LatticeBody &b = ctx->lattices->body(lattice);
+    if (node_colour) std::memcpy(node_colour, b.node_colour.data(), b.nodes);
+    if (hex_colour) std::memcpy(hex_colour, b.hex_colour.data(), b.hexes);
+    return TALUS_MI_OK;
+  });
+}
+
+int talus_mi_lattice_step(talus_mi_ctx *ctx, const talus_mi_lattice_params *params,
+                          talus_mi_lattice_report *report) {
+  if (!params) return fail(TALUS_MI_ERR_ARGUMENT, "null parameters");
+  if (!(params->dt > 0.0) || !std::isfinite(params->dt)) return fail(TALUS_MI_ERR_ARGUMENT, "dt must be positive");
+  return guarded(ctx, [&]() -> int {
+    talus_mi_lattice_report r{};
+    if (ctx->lattices) ctx->lattices->step(*params, r);
+    if (report) *report = r;
     return TALUS_MI_OK;
   });
 }

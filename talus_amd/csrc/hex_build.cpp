@@ -1,0 +1,628 @@
+// hex_build.cpp -- host side of the strainstep hex lattices: the hex mesh
+// builder, the lattice body set-up and the 18-row element build.
+//
+// Reference (under src/talus/):
+//   deformable/solver/solver.cc:2185-2316   buildMesh, VolumeElementType::Hex
+//   dynamics/bodyset/bodyset.cc:13-86       makeLattice
+//   dynamics/strainstep/element.cc:22-99    springOf, symmetricInverse
+//   dynamics/strainstep/element.cc:162-342  signs, elementMaterial,
+//                                           cellFrequency, cornerRest,
+//                                           rowJacobian, makeElement
+//   dynamics/strainstep/strainstep.cc:61-149, 551-567  pinned nodes, substep
+//                                           rule, inverse masses
+//   dynamics/strainstep/plasticflow.cc:62-72, 342-389  authoredRest, the
+//                                           lattice turn
+// Elements are set-up work (built once per body and substep length, as the
+// reference caches them in StrainLattice); the per-substep update and solves
+// run on the GPU (lattice.hip).  Compiled without floating-point
+// contraction, so a build equals the oracle's operation for operation.
+#include <algorithm>
+#include <cmath>
+#include <cstdlib>
+#include <cstring>
+#include <limits>
+#include <thread>
+
+#include "hexlattice.h"
+#include "host.h"
+
+namespace tmi {
+
+namespace {
+
+constexpr double MAX_CORRECTIVE_SPEED = 3.0;
+constexpr double RESOLVED = 1.5;
+constexpr uint32_t MOST_SUBSTEPS = 64;
+constexpr double INF = std::numeric_limits<double>::infinity();
+
+void signs_of(uint32_t c, double *s) {
+  s[0] = (c & 1u) ? 1.0 : -1.0;
+  s[1] = (c & 2u) ? 1.0 : -1.0;
+  s[2] = (c & 4u) ? 1.0 : -1.0;
+}
+
+double hg_sign(uint32_t mode, uint32_t corner) {
+  double s[3];
+  signs_of(corner, s);
+  switch (mode) {
+    case 0: return s[0] * s[1];
+    case 1: return s[1] * s[2];
+    case 2: return s[2] * s[0];
+    default: return s[0] * s[1] * s[2];
+  }
+}
+
+double dot3(const double *a, const double *b) { return a[0] * b[0] + a[1] * b[1] + a[2] * b[2]; }
+
+double det3(const double *a) {
+  return a[0] * (a[4] * a[8] - a[5] * a[7]) - a[1] * (a[3] * a[8] - a[5] * a[6]) +
+         a[2] * (a[3] * a[7] - a[4] * a[6]);
+}
+
+void inverse3(const double *a, double *inv) {
+  const double s = 1.0 / det3(a);
+  inv[0] = (a[4] * a[8] - a[5] * a[7]) * s;
+  inv[1] = (a[2] * a[7] - a[1] * a[8]) * s;
+  inv[2] = (a[1] * a[5] - a[2] * a[4]) * s;
+  inv[3] = (a[5] * a[6] - a[3] * a[8]) * s;
+  inv[4] = (a[0] * a[8] - a[2] * a[6]) * s;
+  inv[5] = (a[2] * a[3] - a[0] * a[5]) * s;
+  inv[6] = (a[3] * a[7] - a[4] * a[6]) * s;
+  inv[7] = (a[1] * a[6] - a[0] * a[7]) * s;
+  inv[8] = (a[0] * a[4] - a[1] * a[3]) * s;
+}
+
+// Jacobian of one row with respect to one corner's velocity in the element frame
+void row_jacobian(const HexElement &e, uint32_t corner, uint32_t row, double *j) {
+  const double *c = e.gradient + 3 * corner;
+  j[0] = j[1] = j[2] = 0.0;
+  switch (row) {
+    case 0: j[0] = c[0]; return;
+    case 1: j[1] = c[1]; return;
+    case 2: j[2] = c[2]; return;
+    case 3: j[0] = 0.5 * c[1]; j[1] = 0.5 * c[0]; return;
+    case 4: j[1] = 0.5 * c[2]; j[2] = 0.5 * c[1]; return;
+    case 5: j[0] = 0.5 * c[2]; j[2] = 0.5 * c[0]; return;
+    default: j[(row - 6) % 3] = e.bend * hg_sign((row - 6) / 3, corner); return;
+  }
+}
+
+void spring_of(double omega, double zeta, double h, double theta, double *rate, double *softness) {
+  const double b = h * omega;
+  const double a = theta * b * b + 2.0 * zeta * b;
+  if (a > 0.0) {
+    *rate = h * omega * omega / a;
+    *softness = 1.0 / a;
+  } else {
+    *rate = 0.0;
+    *softness = 0.0;
+  }
+}
+
+// Gauss-Jordan inverse of the SPD row block without pivoting; inert rows are
+// decoupled before and zeroed after; a non-positive pivot gives zero.
+void symmetric_inverse(const double *block, const bool *inert, double *out) {
+  constexpr uint32_t N = HEX_ROWS;
+  double m[N * N];
+  std::memcpy(m, block, sizeof m);
+  std::memset(out, 0, N * N * sizeof(double));
+  for (uint32_t r = 0; r < N; ++r) {
+    if (inert[r]) {
+      for (uint32_t k = 0; k < N; ++k) m[N * r + k] = m[N * k + r] = 0.0;
+      m[(N + 1) * r] = 1.0;
+    }
+    out[(N + 1) * r] = 1.0;
+  }
+  for (uint32_t col = 0; col < N; ++col) {
+    const double pivot = m[(N + 1) * col];
+    if (!(pivot > 0.0)) {
+      std::memset(out, 0, N * N * sizeof(double));
+      return;
+    }
+    for (uint32_t k = 0; k < N; ++k) {
+      m[N * col + k] /= pivot;
+      out[N * col + k] /= pivot;
+    }
+    for (uint32_t r = 0; r < N; ++r) {
+      const double f = m[N * r + col];
+      if (r == col || f == 0.0) continue;
+      for (uint32_t k = 0; k < N; ++k) {
+        m[N * r + k] -= f * m[N * col + k];
+        out[N * r + k] -= f * out[N * col + k];
+      }
+    }
+  }
+  for (uint32_t r = 0; r < N; ++r)
+    if (inert[r])
+      for (uint32_t k = 0; k < N; ++k) out[N * r + k] = out[N * k + r] = 0.0;
+}
+
+void quat_normalize(double *q) {
+  const double len = std::sqrt(q[0] * q[0] + q[1] * q[1] + q[2] * q[2] + q[3] * q[3]);
+  if (len <= 0.0) {
+    q[0] = 1.0;
+    q[1] = q[2] = q[3] = 0.0;
+    return;
+  }
+  const double s = 1.0 / len;
+  for (int k = 0; k < 4; ++k) q[k] *= s;
+}
+
+void quat_matrix(const double *q, double *R) {
+  const double w = q[0], x = q[1], y = q[2], z = q[3];
+  const double xx = x * x, yy = y * y, zz = z * z, xy = x * y, xz = x * z, yz = y * z;
+  const double wx = w * x, wy = w * y, wz = w * z;
+  R[0] = 1.0 - 2.0 * (yy + zz);
+  R[1] = 2.0 * (xy - wz);
+  R[2] = 2.0 * (xz + wy);
+  R[3] = 2.0 * (xy + wz);
+  R[4] = 1.0 - 2.0 * (xx + zz);
+  R[5] = 2.0 * (yz - wx);
+  R[6] = 2.0 * (xz - wy);
+  R[7] = 2.0 * (yz + wx);
+  R[8] = 1.0 - 2.0 * (xx + yy);
+}
+
+double clampd(double v, double lo, double hi) { return v < lo ? lo : (hi < v ? hi : v); }
+
+}  // namespace
+
+void hex_corner_rest(const double *corners, double *shape, double *hourglass) {
+  std::memset(shape, 0, 9 * sizeof(double));
+  std::memset(hourglass, 0, 12 * sizeof(double));
+  for (uint32_t c = 0; c < 8; ++c) {
+    double s[3];
+    signs_of(c, s);
+    for (int r = 0; r < 3; ++r)
+      for (int k = 0; k < 3; ++k) shape[3 * r + k] += corners[3 * c + r] * s[k] / 8.0;
+    for (uint32_t m = 0; m < 4; ++m) {
+      const double g = hg_sign(m, c);
+      for (int r = 0; r < 3; ++r) hourglass[3 * m + r] += g * corners[3 * c + r];
+    }
+  }
+}
+
+void hex_material(float youngs, float poisson, double *shear, double *first) {
+  const double nu = clampd(double(poisson), 0.0, 0.499);
+  const double e = std::max(0.0, double(youngs));
+  *shear = e / (2.0 * (1.0 + nu));
+  *first = e * nu / ((1.0 + nu) * (1.0 - 2.0 * nu));
+}
+
+double hex_cell_frequency(float youngs, float poisson, float density, double cell) {
+  const double nu = clampd(double(poisson), 0.0, 0.499);
+  const double e = std::max(0.0, double(youngs));
+  return std::sqrt(e / ((1.0 + nu) * double(density))) / cell;
+}
+
+bool hex_make(HexElement &e, uint32_t cell, const uint32_t *nodes, const double *inv_mass,
+              const double *shape, const double *hourglass, double volume, double shear,
+              double first, double h) {
+  const double det = det3(shape);
+  if (!(std::fabs(det) > 0.0) || !(shear > 0.0) || !(volume > 0.0)) return false;
+  e = HexElement{};
+  inverse3(shape, e.shape_inverse);
+  e.cell = cell;
+  e.size = std::cbrt(8.0 * std::fabs(det));
+  e.bend = 1.0 / (4.0 * e.size);
+  std::memcpy(e.rest_hg, hourglass, sizeof e.rest_hg);
+  const double *inv = e.shape_inverse;
+  for (uint32_t c = 0; c < 8; ++c) {
+    double s[3];
+    signs_of(c, s);
+    e.nodes[c] = nodes[c];
+    e.inv_mass[c] = inv_mass[c];
+    for (int r = 0; r < 3; ++r)
+      e.gradient[3 * c + r] = (inv[r] * s[0] + inv[3 + r] * s[1] + inv[6 + r] * s[2]) / 8.0;
+  }
+  double block[HEX_INVERSE] = {};
+  double vweight = 0.0;
+  for (uint32_t c = 0; c < 8; ++c) {
+    const double w = e.inv_mass[c];
+    if (w == 0.0) continue;
+    vweight += w * dot3(e.gradient + 3 * c, e.gradient + 3 * c);
+    for (uint32_t r = 0; r < HEX_ROWS; ++r) {
+      double jl[3];
+      row_jacobian(e, c, r, jl);
+      for (uint32_t k = 0; k < HEX_ROWS; ++k) {
+        double jr[3];
+        row_jacobian(e, c, k, jr);
+        block[HEX_ROWS * r + k] += w * dot3(jl, jr);
+      }
+    }
+  }
+  if (!(vweight > 0.0)) return false;
+  const double hg = shear * (3.0 * first + 2.0 * shear) / (3.0 * (first + shear));
+  bool inert[HEX_ROWS] = {};
+  for (uint32_t r = 0; r < HEX_ROWS; ++r) {
+    const double modulus = r < 3 ? 2.0 * shear : (r < 6 ? 4.0 * shear : hg);
+    const double weight = block[(HEX_ROWS + 1) * r];
+    const double omega = std::sqrt(modulus * volume * weight);
+    e.stiffness[r] = modulus * volume;
+    if (omega > 0.0) {
+      double rate, soft;
+      spring_of(omega, 1.0, h, 1.0, &rate, &soft);
+      e.rate[r] = rate;
+      e.softness[r] = weight * soft;
+      block[(HEX_ROWS + 1) * r] += e.softness[r];
+      e.cap[r] = MAX_CORRECTIVE_SPEED / (rate * e.size);
+    } else {
+      inert[r] = true;
+      e.cap[r] = INF;
+    }
+  }
+  symmetric_inverse(block, inert, e.inverse.data());
+  e.vol_stiffness = first * volume;
+  e.vol_cap = INF;
+  const double vomega = std::sqrt(e.vol_stiffness * vweight);
+  if (vomega > 0.0) {
+    double rate, soft;
+    spring_of(vomega, 1.0, h, 1.0, &rate, &soft);
+    e.vol_rate = rate;
+    e.vol_softness = soft;
+    e.vol_cap = rate > 0.0 ? MAX_CORRECTIVE_SPEED / (rate * e.size) : INF;
+    e.volumetric = true;
+  }
+  return true;
+}
+
+void hex_rotation_of(const double *F, const double *guess, uint32_t iterations, double *out) {
+  double q[4] = {guess[0], guess[1], guess[2], guess[3]};
+  quat_normalize(q);
+  for (uint32_t it = 0; it < iterations; ++it) {
+    double T[9];
+    quat_matrix(q, T);
+    double torque[3] = {0.0, 0.0, 0.0}, tr = 0.0;
+    for (int k = 0; k < 3; ++k) {
+      const double tc[3] = {T[k], T[3 + k], T[6 + k]};
+      const double fc[3] = {F[k], F[3 + k], F[6 + k]};
+      const double cr[3] = {tc[1] * fc[2] - fc[1] * tc[2], tc[2] * fc[0] - fc[2] * tc[0],
+                            tc[0] * fc[1] - fc[0] * tc[1]};
+      if (k == 0) {
+        torque[0] = cr[0];
+        torque[1] = cr[1];
+        torque[2] = cr[2];
+        tr = dot3(tc, fc);
+      } else {
+        torque[0] += cr[0];
+        torque[1] += cr[1];
+        torque[2] += cr[2];
+        tr += dot3(tc, fc);
+      }
+    }
+    const double scale = std::fabs(tr) + 1.0e-9;
+    const double om[3] = {torque[0] / scale, torque[1] / scale, torque[2] / scale};
+    const double angle = std::sqrt(dot3(om, om));
+    if (!(angle > 1.0e-12)) break;
+    const double half = angle * 0.5, s = std::sin(half);
+    const double p[4] = {std::cos(half), om[0] / angle * s, om[1] / angle * s, om[2] / angle * s};
+    const double n[4] = {p[0] * q[0] - p[1] * q[1] - p[2] * q[2] - p[3] * q[3],
+                         p[0] * q[1] + p[1] * q[0] + p[2] * q[3] - p[3] * q[2],
+                         p[0] * q[2] + p[2] * q[0] + p[3] * q[1] - p[1] * q[3],
+                         p[0] * q[3] + p[3] * q[0] + p[1] * q[2] - p[2] * q[1]};
+    std::memcpy(q, n, sizeof q);
+    quat_normalize(q);
+    if (angle <= 1.0e-9) break;
+  }
+  std::memcpy(out, q, sizeof q);
+}
+
+// ---- hex lattice mesh (buildMesh, hex elements) ---------------------------
+
+int build_hex_lattice(const talus_mi_lattice *L, const talus_mi_material *materials, uint32_t lanes,
+                      const talus_mi_support *supports, uint32_t nsupports, talus_mi_hex_mesh *out) {
+  std::memset(out, 0, sizeof *out);
+  for (int k = 0; k < 3; ++k)
+    if (L->resolution[k] < 2) return fail(TALUS_MI_ERR_ARGUMENT, "lattice resolution below 2");
+  const uint32_t cx = L->resolution[0] - 1, cy = L->resolution[1] - 1, cz = L->resolution[2] - 1;
+  const size_t rx = L->resolution[0], ry = L->resolution[1];
+  const size_t samples = rx * ry * L->resolution[2];
+  auto sample_index = [&](uint32_t x, uint32_t y, uint32_t z) { return x + rx * (y + ry * size_t(z)); };
+  std::vector<uint32_t> node_of(samples, UINT32_MAX), sample_of;
+  size_t hexes = 0;
+  for (uint32_t z = 0; z < cz; ++z)
+    for (uint32_t y = 0; y < cy; ++y)
+      for (uint32_t x = 0; x < cx; ++x) {
+        if (L->solid && !L->solid[x + size_t(cx) * (y + size_t(cy) * z)]) continue;
+        ++hexes;
+        for (uint32_t c = 0; c < 8; ++c) {
+          const size_t s = sample_index(x + (c & 1u), y + ((c >> 1) & 1u), z + ((c >> 2) & 1u));
+          if (node_of[s] == UINT32_MAX) {
+            node_of[s] = uint32_t(sample_of.size());
+            sample_of.push_back(uint32_t(s));
+          }
+        }
+      }
+  const uint32_t nodes = uint32_t(sample_of.size());
+  if (nodes == 0) return fail(TALUS_MI_ERR_ARGUMENT, "lattice has no solid cell");
+  std::vector<float> restf(3 * size_t(nodes));
+  for (uint32_t n = 0; n < nodes; ++n) {
+    const size_t s = sample_of[n];
+    for (int k = 0; k < 3; ++k) {
+      if (L->sample_positions) {
+        restf[3 * size_t(n) + k] = L->sample_positions[3 * s + k];
+      } else {
+        const size_t idx = k == 0 ? s % rx : (k == 1 ? (s / rx) % ry : s / (rx * ry));
+        restf[3 * size_t(n) + k] = L->origin[k] + L->voxel_size[k] * float(idx);
+      }
+    }
+  }
+  std::vector<uint8_t> anchored(nodes, 0);
+  for (uint32_t n = 0; n < nodes; ++n)
+    for (uint32_t v = 0; v < nsupports; ++v) {
+      const float *M = supports[v].world_to_support;
+      bool inside = true;
+      for (int r = 0; r < 3 && inside; ++r) {
+        const float l = M[r * 4 + 0] * restf[3 * size_t(n)] + M[r * 4 + 1] * restf[3 * size_t(n) + 1] +
+                        M[r * 4 + 2] * restf[3 * size_t(n) + 2] + M[r * 4 + 3];
+        if (!(std::fabs(l) <= supports[v].half_extents[r])) inside = false;
+      }
+      if (inside) {
+        anchored[n] = 1;
+        break;
+      }
+    }
+  out->rest_positions = (float *)std::malloc(restf.size() * sizeof(float));
+  out->node_mass = (float *)std::calloc(nodes, sizeof(float));
+  out->anchored = (uint8_t *)std::malloc(nodes);
+  out->hex_nodes = (uint32_t *)std::malloc(8 * hexes * sizeof(uint32_t));
+  out->volume = (float *)std::malloc(hexes * sizeof(float));
+  out->hex_material = (uint32_t *)std::malloc(hexes * sizeof(uint32_t));
+  if (!out->rest_positions || !out->node_mass || !out->anchored || !out->hex_nodes ||
+      !out->volume || !out->hex_material) {
+    hex_mesh_free(out);
+    return fail(TALUS_MI_ERR_MEMORY, "out of host memory building a hex mesh");
+  }
+  out->nodes = nodes;
+  out->hexes = uint32_t(hexes);
+  for (int k = 0; k < 3; ++k) out->voxel_size[k] = L->voxel_size[k];
+  std::memcpy(out->rest_positions, restf.data(), restf.size() * sizeof(float));
+  std::memcpy(out->anchored, anchored.data(), nodes);
+  size_t h = 0;
+  for (uint32_t z = 0; z < cz; ++z)
+    for (uint32_t y = 0; y < cy; ++y)
+      for (uint32_t x = 0; x < cx; ++x) {
+        const size_t cell = x + size_t(cx) * (y + size_t(cy) * z);
+        if (L->solid && !L->solid[cell]) continue;
+        const uint32_t lane = L->cell_lane ? L->cell_lane[cell] : 0u;
+        const float density = lane_material(materials, lanes, lane).density;
+        const float vol = 1.0f * L->voxel_size[0] * L->voxel_size[1] * L->voxel_size[2];
+        const float share = density * vol * 0.125f;
+        for (uint32_t c = 0; c < 8; ++c) {
+          const uint32_t n = node_of[sample_index(x + (c & 1u), y + ((c >> 1) & 1u), z + ((c >> 2) & 1u))];
+          out->hex_nodes[8 * h + c] = n;
+          out->node_mass[n] += share;
+        }
+        out->volume[h] = vol;
+        out->hex_material[h] = lane;
+        ++h;
+      }
+  return TALUS_MI_OK;
+}
+
+void hex_mesh_free(talus_mi_hex_mesh *m) {
+  if (!m) return;
+  std::free(m->rest_positions);
+  std::free(m->node_mass);
+  std::free(m->anchored);
+  std::free(m->hex_nodes);
+  std::free(m->volume);
+  std::free(m->hex_material);
+  std::memset(m, 0, sizeof *m);
+}
+
+// ---- lattice bodies --------------------------------------------------------
+
+const talus_mi_material &LatticeBody::material_of(uint32_t cell_index) const {
+  const uint32_t l = lane[cell_index];
+  return materials[std::min<size_t>(l, materials.size() - 1)];
+}
+
+bool LatticeBody::pinned(size_t node) const { return !(mass[node] > 0.0f) || anchored[node] != 0; }
+
+double LatticeBody::stiffest() const {
+  double f = 0.0;
+  for (uint32_t c = 0; c < hexes; ++c) {
+    const talus_mi_material &m = material_of(c);
+    f = std::max(f, hex_cell_frequency(m.youngs_modulus, m.poissons_ratio, m.density, cell));
+  }
+  return f;
+}
+
+uint32_t LatticeBody::island_substeps(double dt, uint32_t softstep) const {
+  const double f = stiffest();
+  uint32_t needed = 1;
+  if (f > 0.0 && std::isfinite(f)) {
+    needed = uint32_t(clampd(std::ceil(dt * f / RESOLVED), 1.0, double(MOST_SUBSTEPS)));
+    while (needed < MOST_SUBSTEPS && dt / double(needed) * f > RESOLVED) ++needed;
+  }
+  return std::max({std::max(softstep, 1u), substeps, needed});
+}
+
+void LatticeBody::build_elements(double step) {
+  std::vector<double> inv(nodes, 0.0);
+  for (size_t i = 0; i < nodes; ++i)
+    if (!pinned(i) && mass[i] > 0.0f) inv[i] = 1.0 / double(mass[i]);
+  // lattice turn: the proper rotation of the mass-weighted moment of the
+  // positions about their centre against the rest positions
+  double q[4] = {1.0, 0.0, 0.0, 0.0};
+  {
+    double c[3] = {0, 0, 0}, rc[3] = {0, 0, 0}, total = 0.0;
+    for (size_t i = 0; i < nodes; ++i) {
+      const double m = mass[i];
+      for (int k = 0; k < 3; ++k) {
+        c[k] += m * x[3 * i + k];
+        rc[k] += m * rest[3 * i + k];
+      }
+      total += m;
+    }
+    if (total > 0.0) {
+      for (int k = 0; k < 3; ++k) {
+        c[k] /= total;
+        rc[k] /= total;
+      }
+      double M[9] = {};
+      for (size_t i = 0; i < nodes; ++i) {
+        const double m = mass[i];
+        for (int r = 0; r < 3; ++r)
+          for (int k = 0; k < 3; ++k) M[3 * r + k] += m * (x[3 * i + r] - c[r]) * (rest[3 * i + k] - rc[k]);
+      }
+      const double id[4] = {1.0, 0.0, 0.0, 0.0};
+      hex_rotation_of(M, id, 64, q);
+    }
+  }
+  std::vector<HexElement> built(hexes);
+  std::vector<uint8_t> ok(hexes, 0);
+  auto work = [&](uint32_t first, uint32_t last) {
+    for (uint32_t cell_index = first; cell_index < last; ++cell_index) {
+      const uint32_t *nd = &hex[8 * size_t(cell_index)];
+      double corners[24], w[8], shape[9], hgv[12], shear, firstl;
+      for (uint32_t c = 0; c < 8; ++c) {
+        w[c] = inv[nd[c]];
+        for (int k = 0; k < 3; ++k) corners[3 * c + k] = rest[3 * size_t(nd[c]) + k];
+      }
+      hex_corner_rest(corners, shape, hgv);
+      const talus_mi_material &m = material_of(cell_index);
+      hex_material(m.youngs_modulus, m.poissons_ratio, &shear, &firstl);
+      HexElement &e = built[cell_index];
+      if (hex_make(e, cell_index, nd, w, shape, hgv, double(volume[cell_index]), shear, firstl, step)) {
+        std::memcpy(e.quat, q, sizeof q);
+        ok[cell_index] = 1;
+      }
+    }
+  };
+  const uint32_t threads = hexes >= 4096 ? std::min(16u, std::max(1u, std::thread::hardware_concurrency())) : 1u;
+  if (threads <= 1) {
+    work(0, hexes);
+  } else {
+    std::vector<std::thread> pool;
+    const uint32_t chunk = (hexes + threads - 1) / threads;
+    for (uint32_t t = 0; t < threads; ++t) {
+      const uint32_t a = std::min(hexes, t * chunk), b = std::min(hexes, a + chunk);
+      if (a < b) pool.emplace_back(work, a, b);
+    }
+    for (auto &t : pool) t.join();
+  }
+  elements.clear();
+  for (uint32_t c = 0; c < hexes; ++c)
+    if (ok[c]) elements.push_back(built[c]);
+  h = step;
+}
+
+int make_lattice(LatticeBody &b, const talus_mi_hex_mesh *mesh, const talus_mi_material *materials,
+                 uint32_t lanes, const double *offset, const double *velocity) {
+  if (!mesh || !mesh->rest_positions || !mesh->node_mass || !mesh->anchored || !mesh->hex_nodes ||
+      !mesh->volume || !mesh->hex_material)
+    return fail(TALUS_MI_ERR_ARGUMENT, "incomplete hex mesh");
+  if (mesh->nodes == 0 || mesh->hexes == 0) return fail(TALUS_MI_ERR_ARGUMENT, "empty hex mesh");
+  if (!materials || lanes == 0)
+    return fail(TALUS_MI_ERR_ARGUMENT, "a strainstep lattice needs its material lanes");
+  for (uint32_t l = 0; l < lanes; ++l) {
+    const talus_mi_material &m = materials[l];
+    if (std::isfinite(m.damping_ratio))
+      return fail(TALUS_MI_ERR_ARGUMENT, "lattice damping ratios (lowest-mode damping) are not supported");
+    if (std::isfinite(m.yield_stress) && m.yield_stress > 0.0f)
+      return fail(TALUS_MI_ERR_ARGUMENT, "lattice plasticity (flowLattice) is not supported");
+    if (std::isfinite(m.tear_strain))
+      return fail(TALUS_MI_ERR_ARGUMENT, "lattice cell tearing is not supported");
+  }
+  for (uint32_t c = 0; c < 8 * mesh->hexes; ++c)
+    if (mesh->hex_nodes[c] >= mesh->nodes) return fail(TALUS_MI_ERR_ARGUMENT, "hex node out of range");
+  for (int k = 0; k < 3; ++k)
+    if (!(mesh->voxel_size[k] > 0.0f)) return fail(TALUS_MI_ERR_ARGUMENT, "voxel size must be positive");
+  b = LatticeBody{};
+  b.nodes = mesh->nodes;
+  b.hexes = mesh->hexes;
+  const size_t n = b.nodes;
+  b.rest.resize(3 * n);
+  for (size_t i = 0; i < 3 * n; ++i) b.rest[i] = double(mesh->rest_positions[i]);
+  b.mass.assign(mesh->node_mass, mesh->node_mass + n);
+  b.anchored.assign(mesh->anchored, mesh->anchored + n);
+  b.hex.assign(mesh->hex_nodes, mesh->hex_nodes + 8 * size_t(b.hexes));
+  b.volume.assign(mesh->volume, mesh->volume + b.hexes);
+  b.lane.assign(mesh->hex_material, mesh->hex_material + b.hexes);
+  b.materials.assign(materials, materials + lanes);
+  const double vox[3] = {mesh->voxel_size[0], mesh->voxel_size[1], mesh->voxel_size[2]};
+  b.cell = std::min({vox[0], vox[1], vox[2]});
+  double lo[3] = {INF, INF, INF};
+  for (size_t i = 0; i < n; ++i)
+    for (int k = 0; k < 3; ++k) lo[k] = std::min(lo[k], b.rest[3 * i + k]);
+  b.x.resize(3 * n);
+  b.v.resize(3 * n);
+  b.node_colour.resize(n);
+  for (size_t i = 0; i < n; ++i) {
+    uint32_t colour = 0;
+    for (int k = 0; k < 3; ++k) {
+      b.x[3 * i + k] = b.rest[3 * i + k] + (offset ? offset[k] : 0.0);
+      b.v[3 * i + k] = b.anchored[i] ? 0.0 : (velocity ? velocity[k] : 0.0);
+      const double index = std::round((b.rest[3 * i + k] - lo[k]) / vox[k]);
+      colour |= (uint32_t(int64_t(index)) & 1u) << k;
+    }
+    b.node_colour[i] = uint8_t(colour);
+  }
+  b.hex_colour.resize(b.hexes);
+  for (uint32_t c = 0; c < b.hexes; ++c) b.hex_colour[c] = b.node_colour[b.hex[8 * size_t(c)]];
+  // a colour's hexes are solved at once: no two of them may share a node
+  {
+    std::vector<uint32_t> seen(size_t(n) * 8, UINT32_MAX);
+    for (uint32_t c = 0; c < b.hexes; ++c)
+      for (uint32_t k = 0; k < 8; ++k) {
+        uint32_t &slot = seen[size_t(b.hex[8 * size_t(c) + k]) * 8 + b.hex_colour[c]];
+        if (slot != UINT32_MAX && slot != c)
+          return fail(TALUS_MI_ERR_ARGUMENT,
+                      "two hexes of one colour share a node: the mesh is not a lattice of its voxel size");
+        slot = c;
+      }
+  }
+  return TALUS_MI_OK;
+}
+
+}  // namespace tmi
+
+extern "C" {
+
+int talus_mi_build_hex_lattice(const talus_mi_lattice *lattice, const talus_mi_material *materials,
+                               uint32_t lanes, const talus_mi_support *supports, uint32_t nsupports,
+                               talus_mi_hex_mesh *out) {
+  if (!lattice || !out || (nsupports && !supports))
+    return tmi::fail(TALUS_MI_ERR_ARGUMENT, "null argument");
+  try {
+    return tmi::build_hex_lattice(lattice, materials, lanes, supports, nsupports, out);
+  } catch (const std::bad_alloc &) {
+    return tmi::fail(TALUS_MI_ERR_MEMORY, "out of host memory");
+  }
+}
+
+void talus_mi_hex_mesh_free(talus_mi_hex_mesh *mesh) { tmi::hex_mesh_free(mesh); }
+
+int talus_mi_hex_element_make(const double *corners, const double *inverse_mass, double volume,
+                              float youngs_modulus, float poissons_ratio, double h,
+                              talus_mi_hex_element *out) {
+  if (!corners || !inverse_mass || !out) return tmi::fail(TALUS_MI_ERR_ARGUMENT, "null argument");
+  double shape[9], hg[12], shear, first;
+  tmi::hex_corner_rest(corners, shape, hg);
+  tmi::hex_material(youngs_modulus, poissons_ratio, &shear, &first);
+  const uint32_t nodes[8] = {0, 1, 2, 3, 4, 5, 6, 7};
+  tmi::HexElement e;
+  if (!tmi::hex_make(e, 0, nodes, inverse_mass, shape, hg, volume, shear, first, h))
+    return tmi::fail(TALUS_MI_ERR_ARGUMENT, "the hex builds no element");
+  std::memset(out, 0, sizeof *out);
+  std::memcpy(out->gradient, e.gradient, sizeof out->gradient);
+  std::memcpy(out->inverse_mass, e.inv_mass, sizeof out->inverse_mass);
+  std::memcpy(out->rest_hourglass, e.rest_hg, sizeof out->rest_hourglass);
+  out->bend = e.bend;
+  out->size = e.size;
+  std::memcpy(out->stiffness, e.stiffness, sizeof out->stiffness);
+  std::memcpy(out->rate, e.rate, sizeof out->rate);
+  std::memcpy(out->softness, e.softness, sizeof out->softness);
+  std::memcpy(out->cap, e.cap, sizeof out->cap);
+  std::memcpy(out->inverse, e.inverse.data(), sizeof out->inverse);
+  out->volume_stiffness = e.vol_stiffness;
+  out->volume_rate = e.vol_rate;
+  out->volume_softness = e.vol_softness;
+  out->volume_cap = e.vol_cap;
+  out->volumetric = e.volumetric ? 1u : 0u;
+  return TALUS_MI_OK;
+}
+
+}  // extern "C"

@@ -1,0 +1,114 @@
+// hexlattice.h -- strainstep hex lattices in libtalus_mi (not part of the
+// C ABI): the host element build (hex_build.cpp) and the device-side set of
+// lattice bodies stepped by the HIP kernels (lattice.hip).
+//
+// Reference: src/talus/dynamics/strainstep/element.cc (the 18-row element),
+// strainstep.cc:56-126, 551-842 (the substep loop), bodyset.cc:13-86
+// (LatticeBody).
+#pragma once
+
+#include <hip/hip_runtime_api.h>
+
+#include <array>
+#include <cstdint>
+#include <memory>
+#include <vector>
+
+#include "../../include/talus_mi.h"
+
+namespace tmi {
+
+constexpr uint32_t HEX_ROWS = 18;
+constexpr uint32_t HEX_INVERSE = HEX_ROWS * HEX_ROWS;
+
+// makeElement's outputs plus the state that carries over between steps.
+struct HexElement {
+  uint32_t cell = 0;
+  uint32_t nodes[8] = {};
+  double shape_inverse[9] = {};  // rest shape^-1, row-major (gradients: inv^T s / 8)
+  double gradient[24] = {};
+  double inv_mass[8] = {};
+  double rest_hg[12] = {};
+  double bend = 0, size = 0;
+  double stiffness[HEX_ROWS] = {}, rate[HEX_ROWS] = {}, softness[HEX_ROWS] = {},
+         cap[HEX_ROWS] = {};
+  std::array<double, HEX_INVERSE> inverse{};
+  double vol_stiffness = 0, vol_rate = 0, vol_softness = 0, vol_cap = 0;
+  bool volumetric = false;
+  // state
+  double impulse[HEX_ROWS] = {};
+  double vol_impulse = 0;
+  double quat[4] = {1, 0, 0, 0};  // w, x, y, z
+};
+
+void hex_corner_rest(const double *corners, double *shape, double *hourglass);
+void hex_material(float youngs, float poisson, double *shear, double *first);
+double hex_cell_frequency(float youngs, float poisson, float density, double cell);
+bool hex_make(HexElement &e, uint32_t cell, const uint32_t *nodes, const double *inv_mass,
+              const double *shape, const double *hourglass, double volume, double shear,
+              double first, double h);
+// rotationOf on the host: the quaternion iteration maximising tr(R^T F)
+void hex_rotation_of(const double *F, const double *guess, uint32_t iterations, double *out);
+
+int build_hex_lattice(const talus_mi_lattice *lattice, const talus_mi_material *materials,
+                      uint32_t lanes, const talus_mi_support *supports, uint32_t nsupports,
+                      talus_mi_hex_mesh *out);
+void hex_mesh_free(talus_mi_hex_mesh *mesh);
+
+// One lattice body: mesh, materials, host state mirror and built elements.
+struct LatticeBody {
+  uint32_t nodes = 0, hexes = 0;
+  std::vector<double> rest;       // 3*nodes
+  std::vector<float> mass;        // nodes
+  std::vector<uint8_t> anchored;  // nodes
+  std::vector<uint32_t> hex;      // 8*hexes
+  std::vector<float> volume;      // hexes
+  std::vector<uint32_t> lane;     // hexes
+  std::vector<talus_mi_material> materials;
+  double cell = 0;                // smallest voxel side
+  std::vector<uint8_t> node_colour, hex_colour;
+  uint32_t substeps = 0;          // LatticeBody::substeps
+  std::vector<double> x, v;       // host mirror of the state
+  double h = 0;                   // substep the elements were built for (0 = none)
+  std::vector<HexElement> elements;
+
+  const talus_mi_material &material_of(uint32_t cell_index) const;
+  bool pinned(size_t node) const;
+  double stiffest() const;
+  uint32_t island_substeps(double dt, uint32_t softstep) const;
+  // builds the elements for substep h, seeded with the lattice turn of x
+  void build_elements(double h);
+};
+
+// Fills a body from a hex mesh (makeLattice); returns a status.
+int make_lattice(LatticeBody &body, const talus_mi_hex_mesh *mesh,
+                 const talus_mi_material *materials, uint32_t lanes, const double *offset,
+                 const double *velocity);
+
+// Device arrays of every lattice body, stepped together (lattice.hip).
+class LatticeSet {
+ public:
+  explicit LatticeSet(hipStream_t stream);
+  ~LatticeSet();
+  LatticeSet(const LatticeSet &) = delete;
+  LatticeSet &operator=(const LatticeSet &) = delete;
+
+  uint32_t add(LatticeBody &&body);
+  uint32_t count() const { return uint32_t(bodies_.size()); }
+  LatticeBody &body(uint32_t i);  // host mirror, current after download
+  void set_state(uint32_t i, const double *x, const double *v);
+  void get_state(uint32_t i, double *x, double *v);
+  void set_substeps(uint32_t i, uint32_t substeps);
+  void step(const talus_mi_lattice_params &params, talus_mi_lattice_report &report);
+
+ private:
+  struct Device;
+  void download();  // device state -> host mirrors
+  void upload();    // host mirrors -> device arrays (layout rebuilt)
+  hipStream_t stream_;
+  std::vector<LatticeBody> bodies_;
+  std::unique_ptr<Device> dev_;
+  bool device_valid_ = false;  // device arrays hold the current state
+};
+
+}  // namespace tmi

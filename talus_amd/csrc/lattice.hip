@@ -1,0 +1,789 @@
+// lattice.hip -- gfx950 kernels of the strainstep hex-lattice solve and the
+// device set of lattice bodies they step.
+//
+// Reference (under src/talus/dynamics/strainstep/):
+//   element.cc:113-158  addRows / rowForce
+//   element.cc:344-372  rotationOf
+//   element.cc:374-459  updateElement
+//   element.cc:461-523  applyElement / solveElement
+//   strainstep.cc:61-84 integrateNodes / advanceNodes
+//   strainstep.cc:708-801 the substep loop: integrate, update and warm-start
+//                       every element, PASSES = 4 Gauss-Seidel passes of
+//                       solveElement, advance
+//
+// Layout in HBM.  All bodies are concatenated.  Node state x, v is FP64
+// xyz-interleaved.  Elements are stored in slot order: colour 0..7, then
+// body, then cell, so one colour is one contiguous slot range and one
+// launch.  Per-element data are SoA planes plane[k * E + slot] (coalesced
+// over consecutive slots).  The 18x18 row-block inverses are interned: equal
+// inverses (same rest shape, corner masses, material and substep) share one
+// 2592-byte table entry, indexed per element.  Gradients are not stored:
+// corner c's gradient is (shape^-T s_c) / 8 with s_c = +-1, recomputed
+// exactly from the 9 entries of shape^-1.
+//
+// Order.  The reference sweeps a body's elements in cell order, one after
+// another (Gauss-Seidel).  Here the passes sweep colour by colour: the hex
+// colour is the parity of its corner-0 lattice index (makeLattice's node
+// colour, bodyset.cc:30-37), two hexes of one colour share no node, so every
+// element of a colour is solved at once with no write conflict, and the
+// result is that of the sequential sweep in (colour, cell) order.
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <cmath>
+#include <cstring>
+#include <string>
+#include <unordered_map>
+#include <vector>
+
+#include "device.h"
+#include "hexlattice.h"
+#include "host.h"
+
+namespace tmi {
+
+namespace {
+
+constexpr int LAT_BLOCK = 128;
+constexpr int NODE_BLOCK = 256;
+constexpr uint32_t PASSES = 4;
+constexpr uint32_t ROTATION_ITERATIONS = 16;
+constexpr double MAX_SPEED = 400.0;
+constexpr uint32_t R = HEX_ROWS;
+
+struct LatDev {
+  uint32_t N, E;
+  double *x, *v;
+  const uint32_t *node_body;
+  const uint8_t *node_pinned;
+  const double *body_h;
+  const uint32_t *body_substeps;
+  const uint8_t *body_exact;
+  // elements: planes [k * E + slot]
+  const uint32_t *e_body;      // 1 plane
+  const uint32_t *e_nodes;     // 8
+  const uint32_t *e_inv_id;    // 1
+  const double *e_shape_inv;   // 9
+  const double *e_inv_mass;    // 8
+  const double *e_bend;        // 1
+  const double *e_rest_hg;     // 12
+  const double *e_rate;        // 18
+  const double *e_soft;        // 18
+  const double *e_cap;         // 18
+  const double *e_vol;         // 3: volume rate, softness, cap
+  const uint8_t *e_volumetric; // 1
+  const double *inv_table;     // [id][324]
+  double *e_impulse;           // 18
+  double *e_vimp;              // 1
+  double *e_quat;              // 4
+  double *e_turn;              // 9
+  double *e_bias;              // 18: rate * strain
+  double *e_vgrad;             // 24
+  double *e_coupling;          // 18
+  double *e_image;             // 18
+  double *e_scal;              // 3: gain, rhs, schur
+};
+
+__device__ __forceinline__ double corner_sign(uint32_t c, int axis) {
+  return ((c >> axis) & 1u) ? 1.0 : -1.0;
+}
+
+__device__ __forceinline__ double hg_sign(uint32_t m, uint32_t c) {
+  const double sx = corner_sign(c, 0), sy = corner_sign(c, 1), sz = corner_sign(c, 2);
+  return m == 0 ? sx * sy : (m == 1 ? sy * sz : (m == 2 ? sz * sx : sx * sy * sz));
+}
+
+// corner gradient (shape^-T s_c) / 8; s_c = +-1 makes every product exact
+__device__ __forceinline__ void grad_of(const double *inv, uint32_t c, double *g) {
+  const double s0 = corner_sign(c, 0), s1 = corner_sign(c, 1), s2 = corner_sign(c, 2);
+#pragma unroll
+  for (int r = 0; r < 3; ++r) g[r] = (inv[r] * s0 + inv[3 + r] * s1 + inv[6 + r] * s2) / 8.0;
+}
+
+__device__ __forceinline__ void add_rows(double *rows, const double *g, double bend, uint32_t c,
+                                         const double *l) {
+  rows[0] += l[0] * g[0];
+  rows[1] += l[1] * g[1];
+  rows[2] += l[2] * g[2];
+  rows[3] += 0.5 * (l[0] * g[1] + l[1] * g[0]);
+  rows[4] += 0.5 * (l[1] * g[2] + l[2] * g[1]);
+  rows[5] += 0.5 * (l[2] * g[0] + l[0] * g[2]);
+#pragma unroll
+  for (uint32_t m = 0; m < 4; ++m) {
+    const double s = hg_sign(m, c) * bend;
+    rows[6 + 3 * m] += s * l[0];
+    rows[7 + 3 * m] += s * l[1];
+    rows[8 + 3 * m] += s * l[2];
+  }
+}
+
+__device__ __forceinline__ void row_force(const double *imp, const double *g, double bend, uint32_t c,
+                                          double *out) {
+  out[0] = imp[0] * g[0] + 0.5 * (imp[3] * g[1] + imp[5] * g[2]);
+  out[1] = imp[1] * g[1] + 0.5 * (imp[3] * g[0] + imp[4] * g[2]);
+  out[2] = imp[2] * g[2] + 0.5 * (imp[4] * g[1] + imp[5] * g[0]);
+#pragma unroll
+  for (uint32_t m = 0; m < 4; ++m) {
+    const double s = hg_sign(m, c) * bend;
+    out[0] += s * imp[6 + 3 * m];
+    out[1] += s * imp[7 + 3 * m];
+    out[2] += s * imp[8 + 3 * m];
+  }
+}
+
+__device__ __forceinline__ void cross3(const double *a, const double *b, double *c) {
+  c[0] = a[1] * b[2] - b[1] * a[2];
+  c[1] = a[2] * b[0] - b[2] * a[0];
+  c[2] = a[0] * b[1] - b[0] * a[1];
+}
+
+__device__ __forceinline__ void quat_matrix(const double *q, double *T) {
+  const double w = q[0], x = q[1], y = q[2], z = q[3];
+  const double xx = x * x, yy = y * y, zz = z * z, xy = x * y, xz = x * z, yz = y * z;
+  const double wx = w * x, wy = w * y, wz = w * z;
+  T[0] = 1.0 - 2.0 * (yy + zz);
+  T[1] = 2.0 * (xy - wz);
+  T[2] = 2.0 * (xz + wy);
+  T[3] = 2.0 * (xy + wz);
+  T[4] = 1.0 - 2.0 * (xx + zz);
+  T[5] = 2.0 * (yz - wx);
+  T[6] = 2.0 * (xz - wy);
+  T[7] = 2.0 * (yz + wx);
+  T[8] = 1.0 - 2.0 * (xx + yy);
+}
+
+__device__ __forceinline__ void quat_normalize(double *q) {
+  const double len = sqrt(q[0] * q[0] + q[1] * q[1] + q[2] * q[2] + q[3] * q[3]);
+  if (len <= 0.0) {
+    q[0] = 1.0;
+    q[1] = q[2] = q[3] = 0.0;
+    return;
+  }
+  const double s = 1.0 / len;
+#pragma unroll
+  for (int k = 0; k < 4; ++k) q[k] *= s;
+}
+
+// the rotation maximising tr(T^T F), iterated from the guess q
+__device__ void rotation_of(const double *F, double *q) {
+  quat_normalize(q);
+  for (uint32_t it = 0; it < ROTATION_ITERATIONS; ++it) {
+    double T[9];
+    quat_matrix(q, T);
+    double torque[3] = {0.0, 0.0, 0.0}, tr = 0.0;
+#pragma unroll
+    for (int k = 0; k < 3; ++k) {
+      const double t0 = T[k], t1 = T[3 + k], t2 = T[6 + k];
+      const double f0 = F[k], f1 = F[3 + k], f2 = F[6 + k];
+      torque[0] += t1 * f2 - f1 * t2;
+      torque[1] += t2 * f0 - f2 * t0;
+      torque[2] += t0 * f1 - f0 * t1;
+      tr += t0 * f0 + t1 * f1 + t2 * f2;
+    }
+    const double scale = fabs(tr) + 1.0e-9;
+    const double om0 = torque[0] / scale, om1 = torque[1] / scale, om2 = torque[2] / scale;
+    const double angle = sqrt(om0 * om0 + om1 * om1 + om2 * om2);
+    if (!(angle > 1.0e-12)) break;
+    double s, c;
+    sincos(angle * 0.5, &s, &c);
+    const double p0 = c, p1 = om0 / angle * s, p2 = om1 / angle * s, p3 = om2 / angle * s;
+    const double n0 = p0 * q[0] - p1 * q[1] - p2 * q[2] - p3 * q[3];
+    const double n1 = p0 * q[1] + p1 * q[0] + p2 * q[3] - p3 * q[2];
+    const double n2 = p0 * q[2] + p2 * q[0] + p3 * q[1] - p1 * q[3];
+    const double n3 = p0 * q[3] + p3 * q[0] + p1 * q[2] - p2 * q[1];
+    q[0] = n0;
+    q[1] = n1;
+    q[2] = n2;
+    q[3] = n3;
+    quat_normalize(q);
+    if (angle <= 1.0e-9) break;
+  }
+}
+
+__device__ __forceinline__ double clampd(double v, double lo, double hi) {
+  return v < lo ? lo : (hi < v ? hi : v);
+}
+
+// updateElement at the current positions, then applyElement with the
+// element's accumulated impulses (the warm start)
+__device__ void lat_update_apply(const LatDev &d, uint32_t e) {
+  const size_t E = d.E;
+  double inv[9];
+#pragma unroll
+  for (int k = 0; k < 9; ++k) inv[k] = d.e_shape_inv[k * E + e];
+  uint32_t nd[8];
+#pragma unroll
+  for (int c = 0; c < 8; ++c) nd[c] = d.e_nodes[c * E + e];
+  double F[9] = {}, hg[12] = {};
+#pragma unroll
+  for (uint32_t c = 0; c < 8; ++c) {
+    const double *xp = d.x + 3 * size_t(nd[c]);
+    const double xc[3] = {xp[0], xp[1], xp[2]};
+    double g[3];
+    grad_of(inv, c, g);
+#pragma unroll
+    for (int r = 0; r < 3; ++r)
+#pragma unroll
+      for (int k = 0; k < 3; ++k) F[3 * r + k] += xc[r] * g[k];
+#pragma unroll
+    for (uint32_t m = 0; m < 4; ++m) {
+      const double s = hg_sign(m, c);
+#pragma unroll
+      for (int r = 0; r < 3; ++r) hg[3 * m + r] += s * xc[r];
+    }
+  }
+  double q[4];
+#pragma unroll
+  for (int k = 0; k < 4; ++k) q[k] = d.e_quat[k * E + e];
+  rotation_of(F, q);
+#pragma unroll
+  for (int k = 0; k < 4; ++k) d.e_quat[k * E + e] = q[k];
+  double T[9];
+  quat_matrix(q, T);
+#pragma unroll
+  for (int k = 0; k < 9; ++k) d.e_turn[k * E + e] = T[k];
+  // local = T^T F
+  double L[9];
+#pragma unroll
+  for (int r = 0; r < 3; ++r)
+#pragma unroll
+    for (int k = 0; k < 3; ++k) L[3 * r + k] = T[r] * F[k] + T[3 + r] * F[3 + k] + T[6 + r] * F[6 + k];
+  const double det = F[0] * (F[4] * F[8] - F[5] * F[7]) - F[1] * (F[3] * F[8] - F[5] * F[6]) +
+                     F[2] * (F[3] * F[7] - F[4] * F[6]);
+  const bool inverted = det < 0.0;
+  const double bend = d.e_bend[e];
+  double raw[R];
+  raw[0] = L[0] - 1.0;
+  raw[1] = L[4] - 1.0;
+  raw[2] = L[8] - 1.0;
+  raw[3] = 0.5 * (L[1] + L[3]);
+  raw[4] = 0.5 * (L[5] + L[7]);
+  raw[5] = 0.5 * (L[6] + L[2]);
+#pragma unroll
+  for (uint32_t m = 0; m < 4; ++m)
+#pragma unroll
+    for (int k = 0; k < 3; ++k) {
+      const double b = T[k] * hg[3 * m] + T[3 + k] * hg[3 * m + 1] + T[6 + k] * hg[3 * m + 2];
+      raw[6 + 3 * m + k] = (b - d.e_rest_hg[(3 * m + k) * E + e]) * bend;
+    }
+#pragma unroll
+  for (uint32_t r = 0; r < R; ++r) {
+    const double cap0 = d.e_cap[r * E + e];
+    const double cap = inverted ? cap0 : (cap0 < 1.0 ? 1.0 : cap0);
+    d.e_bias[r * E + e] = d.e_rate[r * E + e] * clampd(raw[r], -cap, cap);
+  }
+  // cofactor columns F1 x F2, F2 x F0, F0 x F1 (row-major cof[3r + col])
+  double cof[9];
+  {
+    const double f0[3] = {F[0], F[3], F[6]}, f1[3] = {F[1], F[4], F[7]}, f2[3] = {F[2], F[5], F[8]};
+    double c0[3], c1[3], c2[3];
+    cross3(f1, f2, c0);
+    cross3(f2, f0, c1);
+    cross3(f0, f1, c2);
+#pragma unroll
+    for (int r = 0; r < 3; ++r) {
+      cof[3 * r] = c0[r];
+      cof[3 * r + 1] = c1[r];
+      cof[3 * r + 2] = c2[r];
+    }
+  }
+  double coupling[R] = {};
+  double vweight = 0.0;
+#pragma unroll
+  for (uint32_t c = 0; c < 8; ++c) {
+    double g[3], vg[3];
+    grad_of(inv, c, g);
+#pragma unroll
+    for (int r = 0; r < 3; ++r) vg[r] = cof[3 * r] * g[0] + cof[3 * r + 1] * g[1] + cof[3 * r + 2] * g[2];
+#pragma unroll
+    for (int r = 0; r < 3; ++r) d.e_vgrad[(3 * c + r) * E + e] = vg[r];
+    const double w = d.e_inv_mass[c * E + e];
+    vweight += w * (vg[0] * vg[0] + vg[1] * vg[1] + vg[2] * vg[2]);
+    double l[3];
+#pragma unroll
+    for (int k = 0; k < 3; ++k) l[k] = (T[k] * vg[0] + T[3 + k] * vg[1] + T[6 + k] * vg[2]) * w;
+    add_rows(coupling, g, bend, c, l);
+  }
+#pragma unroll
+  for (uint32_t r = 0; r < R; ++r) d.e_coupling[r * E + e] = coupling[r];
+  const double vrate = d.e_vol[e], vsoft = d.e_vol[E + e], vcap0 = d.e_vol[2 * E + e];
+  const double vcap = inverted ? vcap0 : (vcap0 < 1.0 ? 1.0 : vcap0);
+  const double volume = clampd(det - 1.0, -vcap, vcap);
+  const double *Inv = d.inv_table + size_t(d.e_inv_id[e]) * HEX_INVERSE;
+  double coupled = 0.0;
+  for (uint32_t r = 0; r < R; ++r) {
+    double t = 0.0;
+#pragma unroll
+    for (uint32_t k = 0; k < R; ++k) t += Inv[R * r + k] * coupling[k];
+    d.e_image[r * E + e] = t;
+    coupled += coupling[r] * t;
+  }
+  const double gain = vweight * vsoft;
+  const double schur = vweight + gain - coupled;
+  d.e_scal[e] = gain;
+  d.e_scal[E + e] = volume * vrate;
+  d.e_scal[2 * E + e] = d.e_volumetric[e] && schur > 0.0 ? 1.0 / schur : 0.0;
+  // warm start: applyElement with the accumulated impulses
+  double imp[R];
+#pragma unroll
+  for (uint32_t r = 0; r < R; ++r) imp[r] = d.e_impulse[r * E + e];
+  const double vimp = d.e_vimp[e];
+#pragma unroll
+  for (uint32_t c = 0; c < 8; ++c) {
+    const double w = d.e_inv_mass[c * E + e];
+    if (w == 0.0) continue;
+    double g[3], lf[3];
+    grad_of(inv, c, g);
+    row_force(imp, g, bend, c, lf);
+    double *vp = d.v + 3 * size_t(nd[c]);
+#pragma unroll
+    for (int r = 0; r < 3; ++r) {
+      const double f = T[3 * r] * lf[0] + T[3 * r + 1] * lf[1] + T[3 * r + 2] * lf[2];
+      vp[r] -= (f + d.e_vgrad[(3 * c + r) * E + e] * vimp) * w;
+    }
+  }
+}
+
+// solveElement: one block solve of the 18 rows with the volume row folded in
+// through its Schur complement, applied to the corner velocities
+__device__ void lat_solve(const LatDev &d, uint32_t e) {
+  const size_t E = d.E;
+  double inv[9], T[9];
+#pragma unroll
+  for (int k = 0; k < 9; ++k) {
+    inv[k] = d.e_shape_inv[k * E + e];
+    T[k] = d.e_turn[k * E + e];
+  }
+  uint32_t nd[8];
+#pragma unroll
+  for (int c = 0; c < 8; ++c) nd[c] = d.e_nodes[c * E + e];
+  const double bend = d.e_bend[e];
+  double res[R] = {};
+  double vspeed = 0.0;
+#pragma unroll
+  for (uint32_t c = 0; c < 8; ++c) {
+    const double *vp = d.v + 3 * size_t(nd[c]);
+    const double vc[3] = {vp[0], vp[1], vp[2]};
+    vspeed += d.e_vgrad[(3 * c) * E + e] * vc[0] + d.e_vgrad[(3 * c + 1) * E + e] * vc[1] +
+              d.e_vgrad[(3 * c + 2) * E + e] * vc[2];
+    double g[3], b[3];
+    grad_of(inv, c, g);
+#pragma unroll
+    for (int k = 0; k < 3; ++k) b[k] = T[k] * vc[0] + T[3 + k] * vc[1] + T[6 + k] * vc[2];
+    add_rows(res, g, bend, c, b);
+  }
+#pragma unroll
+  for (uint32_t r = 0; r < R; ++r)
+    res[r] += d.e_bias[r * E + e] - d.e_soft[r * E + e] * d.e_impulse[r * E + e];
+  const double *Inv = d.inv_table + size_t(d.e_inv_id[e]) * HEX_INVERSE;
+  double sol[R];
+  double coupled = 0.0;
+  for (uint32_t r = 0; r < R; ++r) {
+    double t = 0.0;
+#pragma unroll
+    for (uint32_t k = 0; k < R; ++k) t += Inv[R * r + k] * res[k];
+    sol[r] = t;
+    coupled += d.e_coupling[r * E + e] * t;
+  }
+  const double gain = d.e_scal[e], rhs = d.e_scal[E + e], schur = d.e_scal[2 * E + e];
+  const double vimp = d.e_vimp[e];
+  const double change = schur * ((rhs - gain * vimp + vspeed) - coupled);
+#pragma unroll
+  for (uint32_t r = 0; r < R; ++r) {
+    sol[r] -= d.e_image[r * E + e] * change;
+    d.e_impulse[r * E + e] += sol[r];
+  }
+  d.e_vimp[e] = vimp + change;
+#pragma unroll
+  for (uint32_t c = 0; c < 8; ++c) {
+    const double w = d.e_inv_mass[c * E + e];
+    if (w == 0.0) continue;
+    double g[3], lf[3];
+    grad_of(inv, c, g);
+    row_force(sol, g, bend, c, lf);
+    double *vp = d.v + 3 * size_t(nd[c]);
+#pragma unroll
+    for (int r = 0; r < 3; ++r) {
+      const double f = T[3 * r] * lf[0] + T[3 * r + 1] * lf[1] + T[3 * r + 2] * lf[2];
+      vp[r] -= (f + d.e_vgrad[(3 * c + r) * E + e] * change) * w;
+    }
+  }
+}
+
+__global__ void __launch_bounds__(NODE_BLOCK) k_lat_integrate(LatDev d, uint32_t substep, double gx,
+                                                             double gy, double gz) {
+  const uint32_t i = blockIdx.x * NODE_BLOCK + threadIdx.x;
+  if (i >= d.N) return;
+  const uint32_t b = d.node_body[i];
+  if (substep >= d.body_substeps[b]) return;
+  const double h = d.body_h[b];
+  double *vp = d.v + 3 * size_t(i);
+  if (d.node_pinned[i]) {
+    vp[0] = vp[1] = vp[2] = 0.0;
+  } else {
+    vp[0] = vp[0] + h * gx;
+    vp[1] = vp[1] + h * gy;
+    vp[2] = vp[2] + h * gz;
+  }
+}
+
+__global__ void __launch_bounds__(NODE_BLOCK) k_lat_advance(LatDev d, uint32_t substep) {
+  const uint32_t i = blockIdx.x * NODE_BLOCK + threadIdx.x;
+  if (i >= d.N) return;
+  const uint32_t b = d.node_body[i];
+  if (substep >= d.body_substeps[b]) return;
+  const double h = d.body_h[b];
+  double *vp = d.v + 3 * size_t(i);
+  double *xp = d.x + 3 * size_t(i);
+  double v0 = vp[0], v1 = vp[1], v2 = vp[2];
+  const double speed = sqrt(v0 * v0 + v1 * v1 + v2 * v2);
+  if (speed > MAX_SPEED) {
+    const double s = MAX_SPEED / speed;
+    v0 *= s;
+    v1 *= s;
+    v2 *= s;
+    vp[0] = v0;
+    vp[1] = v1;
+    vp[2] = v2;
+  }
+  xp[0] += h * v0;
+  xp[1] += h * v1;
+  xp[2] += h * v2;
+}
+
+// update + warm start of one colour's elements of the bodies whose exact
+// flag equals `exact`: exact bodies update every element before the passes,
+// the others (too stiff for their substep) update each colour right before
+// its first solve, as the reference updates such an element right before
+// solving it in pass 0
+__global__ void __launch_bounds__(LAT_BLOCK) k_lat_update(LatDev d, uint32_t first, uint32_t count,
+                                                          uint32_t substep, uint32_t exact) {
+  const uint32_t t = blockIdx.x * LAT_BLOCK + threadIdx.x;
+  if (t >= count) return;
+  const uint32_t e = first + t;
+  const uint32_t b = d.e_body[e];
+  if (substep >= d.body_substeps[b] || uint32_t(d.body_exact[b]) != exact) return;
+  lat_update_apply(d, e);
+}
+
+// one Gauss-Seidel pass over one colour's elements
+__global__ void __launch_bounds__(LAT_BLOCK) k_lat_solve(LatDev d, uint32_t first, uint32_t count,
+                                                         uint32_t substep) {
+  const uint32_t t = blockIdx.x * LAT_BLOCK + threadIdx.x;
+  if (t >= count) return;
+  const uint32_t e = first + t;
+  const uint32_t b = d.e_body[e];
+  if (substep >= d.body_substeps[b]) return;
+  lat_solve(d, e);
+}
+
+}  // namespace
+
+struct LatticeSet::Device {
+  std::vector<void *> owned;
+  LatDev d{};
+  uint32_t colour_start[9] = {};
+  std::vector<uint32_t> node0;               // first node of each body
+  std::vector<std::vector<uint32_t>> slot_of;  // per body, per element: slot
+  std::vector<uint32_t> body_substeps;
+  uint32_t inverses = 0;
+  hipEvent_t start = nullptr, stop = nullptr;
+
+  template <class T>
+  T *alloc(size_t n) {
+    T *p = nullptr;
+    HIPCHECK(hipMalloc(&p, std::max<size_t>(n, 1) * sizeof(T)));
+    owned.push_back(p);
+    return p;
+  }
+  template <class T>
+  T *copy(const std::vector<T> &h) {
+    T *p = alloc<T>(h.size());
+    if (!h.empty()) HIPCHECK(hipMemcpy(p, h.data(), h.size() * sizeof(T), hipMemcpyHostToDevice));
+    return p;
+  }
+  ~Device() {
+    for (void *p : owned) (void)hipFree(p);
+    if (start) (void)hipEventDestroy(start);
+    if (stop) (void)hipEventDestroy(stop);
+  }
+};
+
+LatticeSet::LatticeSet(hipStream_t stream) : stream_(stream) {}
+
+LatticeSet::~LatticeSet() {
+  if (stream_) (void)hipStreamSynchronize(stream_);
+}
+
+uint32_t LatticeSet::add(LatticeBody &&body) {
+  download();
+  bodies_.push_back(std::move(body));
+  device_valid_ = false;
+  return uint32_t(bodies_.size() - 1);
+}
+
+LatticeBody &LatticeSet::body(uint32_t i) {
+  download();
+  return bodies_[i];
+}
+
+void LatticeSet::set_substeps(uint32_t i, uint32_t substeps) { bodies_[i].substeps = substeps; }
+
+void LatticeSet::get_state(uint32_t i, double *x, double *v) {
+  LatticeBody &b = bodies_[i];
+  const size_t n = 3 * size_t(b.nodes);
+  if (device_valid_) {
+    const size_t off = 3 * size_t(dev_->node0[i]);
+    if (x) HIPCHECK(hipMemcpyAsync(b.x.data(), dev_->d.x + off, n * sizeof(double), hipMemcpyDeviceToHost, stream_));
+    if (v) HIPCHECK(hipMemcpyAsync(b.v.data(), dev_->d.v + off, n * sizeof(double), hipMemcpyDeviceToHost, stream_));
+    HIPCHECK(hipStreamSynchronize(stream_));
+  }
+  if (x) std::memcpy(x, b.x.data(), n * sizeof(double));
+  if (v) std::memcpy(v, b.v.data(), n * sizeof(double));
+}
+
+void LatticeSet::set_state(uint32_t i, const double *x, const double *v) {
+  LatticeBody &b = bodies_[i];
+  const size_t n = 3 * size_t(b.nodes);
+  if (x) std::memcpy(b.x.data(), x, n * sizeof(double));
+  if (v) std::memcpy(b.v.data(), v, n * sizeof(double));
+  if (device_valid_) {
+    const size_t off = 3 * size_t(dev_->node0[i]);
+    if (x) HIPCHECK(hipMemcpyAsync(dev_->d.x + off, b.x.data(), n * sizeof(double), hipMemcpyHostToDevice, stream_));
+    if (v) HIPCHECK(hipMemcpyAsync(dev_->d.v + off, b.v.data(), n * sizeof(double), hipMemcpyHostToDevice, stream_));
+    HIPCHECK(hipStreamSynchronize(stream_));
+  }
+}
+
+void LatticeSet::download() {
+  if (!device_valid_ || !dev_) return;
+  const LatDev &d = dev_->d;
+  const size_t E = d.E;
+  std::vector<double> x(3 * size_t(d.N)), v(3 * size_t(d.N)), imp(size_t(R) * E), vimp(E), quat(4 * E);
+  HIPCHECK(hipStreamSynchronize(stream_));
+  HIPCHECK(hipMemcpy(x.data(), d.x, x.size() * sizeof(double), hipMemcpyDeviceToHost));
+  HIPCHECK(hipMemcpy(v.data(), d.v, v.size() * sizeof(double), hipMemcpyDeviceToHost));
+  if (E) {
+    HIPCHECK(hipMemcpy(imp.data(), d.e_impulse, imp.size() * sizeof(double), hipMemcpyDeviceToHost));
+    HIPCHECK(hipMemcpy(vimp.data(), d.e_vimp, vimp.size() * sizeof(double), hipMemcpyDeviceToHost));
+    HIPCHECK(hipMemcpy(quat.data(), d.e_quat, quat.size() * sizeof(double), hipMemcpyDeviceToHost));
+  }
+  for (size_t b = 0; b < bodies_.size(); ++b) {
+    LatticeBody &body = bodies_[b];
+    const size_t off = 3 * size_t(dev_->node0[b]);
+    std::copy(x.begin() + off, x.begin() + off + 3 * size_t(body.nodes), body.x.begin());
+    std::copy(v.begin() + off, v.begin() + off + 3 * size_t(body.nodes), body.v.begin());
+    for (size_t i = 0; i < body.elements.size(); ++i) {
+      const size_t s = dev_->slot_of[b][i];
+      HexElement &el = body.elements[i];
+      for (uint32_t r = 0; r < R; ++r) el.impulse[r] = imp[r * E + s];
+      el.vol_impulse = vimp[s];
+      for (int k = 0; k < 4; ++k) el.quat[k] = quat[k * E + s];
+    }
+  }
+}
+
+void LatticeSet::upload() {
+  dev_.reset();
+  auto dv = std::make_unique<Device>();
+  const uint32_t B = uint32_t(bodies_.size());
+  uint32_t N = 0, E = 0;
+  dv->node0.resize(B);
+  dv->slot_of.resize(B);
+  for (uint32_t b = 0; b < B; ++b) {
+    dv->node0[b] = N;
+    N += bodies_[b].nodes;
+    E += uint32_t(bodies_[b].elements.size());
+    dv->slot_of[b].assign(bodies_[b].elements.size(), 0);
+  }
+  // slots: colour, body, cell
+  std::vector<std::pair<uint32_t, uint32_t>> order;  // (body, element)
+  order.reserve(E);
+  for (uint32_t col = 0; col < 8; ++col) {
+    dv->colour_start[col] = uint32_t(order.size());
+    for (uint32_t b = 0; b < B; ++b) {
+      const LatticeBody &body = bodies_[b];
+      for (uint32_t i = 0; i < body.elements.size(); ++i)
+        if (body.hex_colour[body.elements[i].cell] == col) {
+          dv->slot_of[b][i] = uint32_t(order.size());
+          order.emplace_back(b, i);
+        }
+    }
+  }
+  dv->colour_start[8] = uint32_t(order.size());
+  // nodes
+  std::vector<double> x(3 * size_t(N)), v(3 * size_t(N));
+  std::vector<uint32_t> node_body(N);
+  std::vector<uint8_t> node_pinned(N);
+  std::vector<double> body_h(B);
+  std::vector<uint32_t> body_substeps(B);
+  std::vector<uint8_t> body_exact(B);
+  for (uint32_t b = 0; b < B; ++b) {
+    const LatticeBody &body = bodies_[b];
+    const size_t n0 = dv->node0[b];
+    std::copy(body.x.begin(), body.x.end(), x.begin() + 3 * n0);
+    std::copy(body.v.begin(), body.v.end(), v.begin() + 3 * n0);
+    for (uint32_t i = 0; i < body.nodes; ++i) {
+      node_body[n0 + i] = b;
+      node_pinned[n0 + i] = body.pinned(i) ? 1 : 0;
+    }
+    body_h[b] = body.h;
+  }
+  // elements
+  const size_t Es = E;
+  std::vector<uint32_t> e_body(Es), e_nodes(8 * Es), e_inv_id(Es);
+  std::vector<double> e_shape_inv(9 * Es), e_inv_mass(8 * Es), e_bend(Es), e_rest_hg(12 * Es),
+      e_rate(R * Es), e_soft(R * Es), e_cap(R * Es), e_vol(3 * Es), e_impulse(R * Es), e_vimp(Es),
+      e_quat(4 * Es);
+  std::vector<uint8_t> e_volumetric(Es);
+  std::unordered_map<std::string, uint32_t> interned;  // inverse bytes -> table entry
+  std::vector<double> table;
+  for (size_t s = 0; s < Es; ++s) {
+    const uint32_t b = order[s].first;
+    const HexElement &el = bodies_[b].elements[order[s].second];
+    const uint32_t n0 = dv->node0[b];
+    e_body[s] = b;
+    for (int c = 0; c < 8; ++c) {
+      e_nodes[c * Es + s] = n0 + el.nodes[c];
+      e_inv_mass[c * Es + s] = el.inv_mass[c];
+    }
+    for (int k = 0; k < 9; ++k) e_shape_inv[k * Es + s] = el.shape_inverse[k];
+    e_bend[s] = el.bend;
+    for (int k = 0; k < 12; ++k) e_rest_hg[k * Es + s] = el.rest_hg[k];
+    for (uint32_t r = 0; r < R; ++r) {
+      e_rate[r * Es + s] = el.rate[r];
+      e_soft[r * Es + s] = el.softness[r];
+      e_cap[r * Es + s] = el.cap[r];
+      e_impulse[r * Es + s] = el.impulse[r];
+    }
+    e_vol[s] = el.vol_rate;
+    e_vol[Es + s] = el.vol_softness;
+    e_vol[2 * Es + s] = el.vol_cap;
+    e_volumetric[s] = el.volumetric ? 1 : 0;
+    e_vimp[s] = el.vol_impulse;
+    for (int k = 0; k < 4; ++k) e_quat[k * Es + s] = el.quat[k];
+    std::string key(reinterpret_cast<const char *>(el.inverse.data()), HEX_INVERSE * sizeof(double));
+    auto it = interned.find(key);
+    if (it == interned.end()) {
+      it = interned.emplace(std::move(key), uint32_t(table.size() / HEX_INVERSE)).first;
+      table.insert(table.end(), el.inverse.begin(), el.inverse.end());
+    }
+    e_inv_id[s] = it->second;
+  }
+  dv->inverses = uint32_t(table.size() / HEX_INVERSE);
+  LatDev &d = dv->d;
+  d.N = N;
+  d.E = E;
+  d.x = dv->copy(x);
+  d.v = dv->copy(v);
+  d.node_body = dv->copy(node_body);
+  d.node_pinned = dv->copy(node_pinned);
+  d.body_h = dv->copy(body_h);
+  d.body_substeps = dv->alloc<uint32_t>(B);
+  d.body_exact = dv->alloc<uint8_t>(B);
+  d.e_body = dv->copy(e_body);
+  d.e_nodes = dv->copy(e_nodes);
+  d.e_inv_id = dv->copy(e_inv_id);
+  d.e_shape_inv = dv->copy(e_shape_inv);
+  d.e_inv_mass = dv->copy(e_inv_mass);
+  d.e_bend = dv->copy(e_bend);
+  d.e_rest_hg = dv->copy(e_rest_hg);
+  d.e_rate = dv->copy(e_rate);
+  d.e_soft = dv->copy(e_soft);
+  d.e_cap = dv->copy(e_cap);
+  d.e_vol = dv->copy(e_vol);
+  d.e_volumetric = dv->copy(e_volumetric);
+  d.inv_table = dv->copy(table);
+  d.e_impulse = dv->copy(e_impulse);
+  d.e_vimp = dv->copy(e_vimp);
+  d.e_quat = dv->copy(e_quat);
+  d.e_turn = dv->alloc<double>(9 * Es);
+  d.e_bias = dv->alloc<double>(R * Es);
+  d.e_vgrad = dv->alloc<double>(24 * Es);
+  d.e_coupling = dv->alloc<double>(R * Es);
+  d.e_image = dv->alloc<double>(R * Es);
+  d.e_scal = dv->alloc<double>(3 * Es);
+  HIPCHECK(hipEventCreate(&dv->start));
+  HIPCHECK(hipEventCreate(&dv->stop));
+  dev_ = std::move(dv);
+  device_valid_ = true;
+}
+
+void LatticeSet::step(const talus_mi_lattice_params &p, talus_mi_lattice_report &rep) {
+  std::memset(&rep, 0, sizeof rep);
+  const uint32_t B = uint32_t(bodies_.size());
+  rep.lattices = B;
+  if (B == 0) return;
+  if (!(p.dt > 0.0) || !std::isfinite(p.dt)) throw std::invalid_argument("dt must be positive");
+  std::vector<uint32_t> substeps(B);
+  std::vector<double> h(B);
+  std::vector<uint8_t> exact(B);
+  bool rebuild = false;
+  for (uint32_t b = 0; b < B; ++b) {
+    substeps[b] = bodies_[b].island_substeps(p.dt, p.softstep_substeps);
+    h[b] = p.dt / double(substeps[b]);
+    exact[b] = !(h[b] * bodies_[b].stiffest() > 1.5) ? 1 : 0;
+    if (bodies_[b].h != h[b]) rebuild = true;
+  }
+  if (rebuild) {
+    download();
+    for (uint32_t b = 0; b < B; ++b)
+      if (bodies_[b].h != h[b]) {
+        bodies_[b].build_elements(h[b]);
+        ++rep.rebuilds;
+      }
+    device_valid_ = false;
+  }
+  if (!device_valid_) upload();
+  Device &dv = *dev_;
+  LatDev &d = dv.d;
+  if (dv.body_substeps != substeps) {
+    HIPCHECK(hipMemcpyAsync(const_cast<uint32_t *>(d.body_substeps), substeps.data(), B * sizeof(uint32_t),
+                            hipMemcpyHostToDevice, stream_));
+    HIPCHECK(hipMemcpyAsync(const_cast<uint8_t *>(d.body_exact), exact.data(), B, hipMemcpyHostToDevice,
+                            stream_));
+    HIPCHECK(hipStreamSynchronize(stream_));
+    dv.body_substeps = substeps;
+  }
+  uint32_t S = 0;
+  bool any_exact = false, any_inexact = false;
+  for (uint32_t b = 0; b < B; ++b) {
+    any_exact = any_exact || exact[b];
+    any_inexact = any_inexact || !exact[b];
+    S = std::max(S, substeps[b]);
+    rep.element_solves += uint64_t(substeps[b]) * PASSES * bodies_[b].elements.size();
+  }
+  rep.substeps = S;
+  rep.elements = d.E;
+  const uint32_t node_grid = (d.N + NODE_BLOCK - 1) / NODE_BLOCK;
+  HIPCHECK(hipEventRecord(dv.start, stream_));
+  for (uint32_t s = 0; s < S; ++s) {
+    hipLaunchKernelGGL(k_lat_integrate, dim3(node_grid), dim3(NODE_BLOCK), 0, stream_, d, s, p.gravity[0],
+                       p.gravity[1], p.gravity[2]);
+    if (any_exact)
+      for (uint32_t col = 0; col < 8; ++col) {
+        const uint32_t first = dv.colour_start[col], count = dv.colour_start[col + 1] - first;
+        if (count)
+          hipLaunchKernelGGL(k_lat_update, dim3((count + LAT_BLOCK - 1) / LAT_BLOCK), dim3(LAT_BLOCK), 0,
+                             stream_, d, first, count, s, 1u);
+      }
+    for (uint32_t pass = 0; pass < PASSES; ++pass)
+      for (uint32_t col = 0; col < 8; ++col) {
+        const uint32_t first = dv.colour_start[col], count = dv.colour_start[col + 1] - first;
+        if (!count) continue;
+        const dim3 grid((count + LAT_BLOCK - 1) / LAT_BLOCK);
+        if (pass == 0 && any_inexact)
+          hipLaunchKernelGGL(k_lat_update, grid, dim3(LAT_BLOCK), 0, stream_, d, first, count, s, 0u);
+        hipLaunchKernelGGL(k_lat_solve, grid, dim3(LAT_BLOCK), 0, stream_, d, first, count, s);
+      }
+    hipLaunchKernelGGL(k_lat_advance, dim3(node_grid), dim3(NODE_BLOCK), 0, stream_, d, s);
+  }
+  HIPCHECK(hipGetLastError());
+  HIPCHECK(hipEventRecord(dv.stop, stream_));
+  HIPCHECK(hipEventSynchronize(dv.stop));
+  float ms = 0.0f;
+  HIPCHECK(hipEventElapsedTime(&ms, dv.start, dv.stop));
+  rep.device_ms = ms;
+}
+
+}  // namespace tmi

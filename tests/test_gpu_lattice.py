@@ -1,0 +1,198 @@
+"""Strainstep hex lattices on the GPU (talus_mi_lattice_*), against the hex
+oracle stepping the same bodies in the same (colour, cell) order, and against
+the closed forms the reference's strainstep tests hold
+(tests/dynamics_strainstep_tests.cc): free fall as semi-implicit Euler
+(156-183, 1071-1077) and cantilevers within 10 % of beam theory (1806-1820).
+Bodies batched in one launch sequence equal each body stepped alone, bit for
+bit (the reference's parallel = serial identity, 4998-5050).  FP64."""
+from __future__ import annotations
+
+import math
+
+import numpy as np
+import pytest
+
+import oracle
+from oracle import hex as H
+from talus_amd import api
+
+pytestmark = pytest.mark.gpu
+
+
+def material(E, density=1000.0, poisson=0.4, friction=0.7):
+    return api.PhysicalMaterial(youngs_modulus=E, density=density, poissons_ratio=poisson,
+                                friction=friction)
+
+
+def orc_material(m):
+    o = oracle.Material()
+    for name, _ in oracle.Material._fields_:
+        setattr(o, name, getattr(m, name))
+    return o
+
+
+def box(half, cell, mat, supports=(), lanes=2):
+    cells = [int(round(2 * h / cell)) for h in half]
+    return api.build_hex_lattice([c + 1 for c in cells], [cell] * 3, [-h for h in half], [mat] * lanes,
+                                 list(supports))
+
+
+def orc_mesh(mesh):
+    return {"rest": mesh["rest_positions"].astype(np.float64), "mass": mesh["node_mass"],
+            "anchored": mesh["anchored"], "hex": mesh["hex_nodes"], "volume": mesh["volume"],
+            "lane": mesh["hex_material"], "voxel": mesh["voxel_size"]}
+
+
+def rel(a, b, scale):
+    return float(np.abs(a - b).max() / scale)
+
+
+@pytest.mark.parametrize("requested,expected", [(0, 4), (8, 8)])
+def test_free_fall_is_semi_implicit_euler(requested, expected):
+    mat = material(8.0e3)
+    mesh = box((0.3, 0.3, 0.3), 0.15, mat)
+    with api.Context() as ctx:
+        b = ctx.add_lattice(mesh, [mat, mat], offset=(0.0, 50.0, 0.0))
+        ctx.set_lattice_substeps(b, requested)
+        x0, _ = ctx.lattice_state(b)
+        for _ in range(30):
+            assert ctx.lattice_step()["substeps"] == expected
+        x, v = ctx.lattice_state(b)
+    h = (1.0 / 60.0) / expected
+    n = expected * 30
+    assert np.abs(v[:, 1] + 9.81 * h * n).max() <= 1e-9
+    assert np.abs(x[:, 1] - x0[:, 1] + 9.81 * h * h * n * (n + 1) / 2).max() <= 1e-9
+    assert np.abs(x[:, [0, 2]] - x0[:, [0, 2]]).max() <= 1e-12
+
+
+def sagging_block(E=2.0e5, poisson=0.3, cells=(8, 2, 2), cell=0.1):
+    mat = material(E, poisson=poisson)
+    half = [c * cell / 2 for c in cells]
+    clamp = [api.Support.box((-half[0], 0.0, 0.0), (0.01, 1.0, 1.0))]
+    return box(half, cell, mat, clamp, lanes=1), mat
+
+
+@pytest.mark.parametrize("steps", [1, 40])
+def test_gpu_matches_the_oracle_colour_sweep(steps):
+    mesh, mat = sagging_block()
+    orc = H.LatticeBody(orc_mesh(mesh), [orc_material(mat)])
+    with api.Context() as ctx:
+        b = ctx.add_lattice(mesh, [mat])
+        nc, hc = ctx.lattice_colours(b)
+        onc, ohc = orc.colours()
+        np.testing.assert_array_equal(nc, onc)
+        np.testing.assert_array_equal(hc, ohc)
+        for _ in range(steps):
+            r = ctx.lattice_step()
+            assert orc.step(colored=True) == r["substeps"]
+        x, v = ctx.lattice_state(b)
+        cells, imp = ctx.lattice_impulses(b)
+    ox, ov = orc.state()
+    ocells, oimp = orc.impulses()
+    np.testing.assert_array_equal(cells, ocells)
+    motion = np.abs(ox - mesh["rest_positions"]).max()
+    assert motion > 1e-5
+    assert rel(x, ox, motion) <= 1e-9
+    assert rel(v, ov, np.abs(ov).max()) <= 1e-9
+    assert rel(imp, oimp, np.abs(oimp).max()) <= 1e-9
+
+
+def test_gpu_matches_the_oracle_on_a_turned_moving_jelly():
+    """A block thrown with a spin: large rotations through rotationOf."""
+    mat = material(5.0e4, poisson=0.45)
+    mesh = box((0.3, 0.2, 0.2), 0.1, mat)
+    rest = mesh["rest_positions"].astype(np.float64)
+    w = np.array([0.0, 0.0, 6.0])
+    v0 = np.cross(w, rest) + [1.0, 3.0, 0.0]
+    orc = H.LatticeBody(orc_mesh(mesh), [orc_material(mat)] * 2)
+    orc.set_state(rest, v0)
+    with api.Context() as ctx:
+        b = ctx.add_lattice(mesh, [mat, mat])
+        ctx.set_lattice_state(b, rest, v0)
+        for _ in range(30):
+            ctx.lattice_step()
+            orc.step(colored=True)
+        x, v = ctx.lattice_state(b)
+    ox, ov = orc.state()
+    assert rel(x, ox, np.abs(ox - rest).max()) <= 1e-9
+    assert rel(v, ov, np.abs(ov).max()) <= 1e-9
+
+
+def cantilever(ctx, thickness, cell, poisson, substeps, steps):
+    length = 4.0
+    mat = material(5.0e5, density=5.0, poisson=poisson, friction=0.5)
+    clamp = [api.Support.box((-length / 2, 0.0, 0.0), (0.01, 1.0, 1.0))]
+    mesh = box((length / 2, thickness / 2, thickness / 2), cell, mat, clamp)
+    b = ctx.add_lattice(mesh, [mat, mat])
+    ctx.set_lattice_substeps(b, substeps)
+    rest = mesh["rest_positions"]
+    tip = int(np.argmin(np.linalg.norm(rest - [length / 2, thickness / 2, thickness / 2], axis=1)))
+    return b, mesh, tip, 5.0 * 9.81 * thickness ** 2 * length ** 4 / (8 * 5.0e5 * thickness ** 4 / 12.0)
+
+
+@pytest.mark.parametrize("thickness,cell,poisson,substeps,cells",
+                         [(0.4, 0.4, 0.0, 48, 10), (0.4, 0.4, 0.45, 48, 10), (0.4, 0.2, 0.3, 16, 80)])
+def test_cantilever_bends_as_beam_theory(thickness, cell, poisson, substeps, cells):
+    """dynamics_strainstep_tests.cc:1806-1820; sag averaged over the last
+    300 of 600 frames."""
+    steps = 600
+    with api.Context() as ctx:
+        b, mesh, tip, theory = cantilever(ctx, thickness, cell, poisson, substeps, steps)
+        assert mesh["hex_nodes"].shape[0] == cells
+        sag = 0.0
+        for frame in range(1, steps + 1):
+            ctx.lattice_step()
+            if frame > steps // 2:
+                x, _ = ctx.lattice_state(b)
+                sag += (mesh["rest_positions"][tip, 1] - x[tip, 1]) / (steps // 2)
+    assert abs(sag - theory) / theory < 0.1
+
+
+def test_batched_bodies_equal_each_body_alone_bitwise():
+    """Uneven bodies with different substep counts in one launch sequence."""
+    specs = [((0.3, 0.2, 0.2), 0.1, 2.0e5, 0.3, 0), ((0.2, 0.2, 0.2), 0.1, 8.0e3, 0.4, 8),
+             ((0.5, 0.1, 0.1), 0.1, 5.0e6, 0.45, 0)]
+    bodies = []
+    for half, cell, E, nu, sub in specs:
+        mat = material(E, poisson=nu)
+        clamp = [api.Support.box((-half[0], 0.0, 0.0), (0.01, 1.0, 1.0))]
+        bodies.append((box(half, cell, mat, clamp, lanes=1), mat, sub))
+
+    def run(which):
+        with api.Context() as ctx:
+            ids = []
+            for k in which:
+                mesh, mat, sub = bodies[k]
+                i = ctx.add_lattice(mesh, [mat], offset=(0.0, 1.0 * k, 0.0))
+                ctx.set_lattice_substeps(i, sub)
+                ids.append(i)
+            subs = [ctx.lattice_step()["substeps"] for _ in range(12)]
+            return [ctx.lattice_state(i) for i in ids], subs
+
+    together, subs = run([0, 1, 2])
+    assert max(subs) >= 8
+    for k in range(3):
+        alone, _ = run([k])
+        np.testing.assert_array_equal(together[k][0], alone[0][0])
+        np.testing.assert_array_equal(together[k][1], alone[0][1])
+
+
+def test_state_round_trips_and_unsupported_materials_are_refused():
+    mesh, mat = sagging_block()
+    with api.Context() as ctx:
+        b = ctx.add_lattice(mesh, [mat])
+        x, v = ctx.lattice_state(b)
+        np.testing.assert_array_equal(x, mesh["rest_positions"].astype(np.float64))
+        ctx.lattice_step()
+        x1, v1 = ctx.lattice_state(b)
+        ctx.set_lattice_state(b, x1 + 0.5, v1)
+        x2, _ = ctx.lattice_state(b)
+        np.testing.assert_array_equal(x2, x1 + 0.5)
+        for bad in (dict(damping_ratio=0.5), dict(yield_stress=1e4), dict(tear_strain=0.3)):
+            m = material(2.0e5)
+            for k, val in bad.items():
+                setattr(m, k, val)
+            with pytest.raises(api.TalusError):
+                ctx.add_lattice(mesh, [m])
+        with pytest.raises(api.TalusError):
+            ctx.add_lattice(mesh, [])

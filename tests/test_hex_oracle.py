@@ -1,0 +1,352 @@
+"""The hex-lattice oracle (oracle/hex_oracle.c) against the closed forms the
+reference's own strainstep tests hold (tests/dynamics_strainstep_tests.cc):
+rigid motion gives no strain and no impulse, rotationOf finds a turn, the
+solve keeps momentum and is exact in one block solve, row stiffness, strain
+rows, bending energy of one beam cell, the volume gradient against finite
+differences, the rows' soft springs against the soft-step formula, free fall
+as semi-implicit Euler and a cantilever against beam theory.  No GPU."""
+from __future__ import annotations
+
+import math
+
+import numpy as np
+import pytest
+
+import oracle
+from oracle import hex as H
+
+
+def unit_cell(half, mass):
+    """testCell (dynamics_strainstep_tests.cc:187-201): corners at half *
+    signs, inverse masses 1/(m (1 + 0.1 c))."""
+    rest = np.array([half * H.corner_signs(c) for c in range(8)])
+    inv = np.array([1.0 / (mass * (1.0 + 0.1 * c)) for c in range(8)])
+    return rest, inv
+
+
+def unit_element(rest, inv, youngs, poisson, h):
+    side = rest[7, 0] - rest[0, 0]
+    e = H.make(rest, inv, side ** 3, np.float32(youngs), np.float32(poisson), h)
+    assert e is not None
+    return e
+
+
+def rot(axis, angle):
+    return H.quat_from_axis_angle(axis, angle)
+
+
+def qmat(q):
+    return H.quat_matrix(q)
+
+
+def test_turned_and_moved_hex_has_no_strain_and_no_impulse():
+    """dynamics_strainstep_tests.cc:1115-1138"""
+    rest, inv = unit_cell(0.1, 1.0)
+    e = unit_element(rest, inv, 1.0e5, 0.3, 1.0 / 240.0)
+    q = rot([1.0, 2.0, 3.0], 2.0)
+    pos = np.array([0.5, -1.0, 2.0]) + rest @ qmat(q).T
+    vel = np.zeros((8, 3))
+    H.set_rotation(e, q)
+    H.update(e, pos)
+    assert np.abs(e.arr("strain")).max() <= 1e-12
+    H.solve(e, vel)
+    assert np.linalg.norm(vel, axis=1).max() <= 1e-12
+
+
+def test_rotation_of_recovers_a_turn_from_half_a_radian_away():
+    """dynamics_strainstep_tests.cc:1140-1154"""
+    q = rot([1.0, -1.0, 2.0], 0.5)
+    stretch = np.array([[1.1, 0.02, 0.0], [0.02, 0.95, 0.01], [0.0, 0.01, 1.05]]).T
+    found = H.rotation_of(qmat(q) @ stretch)
+    diff = qmat(found) - qmat(q)
+    assert np.linalg.norm(diff, axis=0).max() <= 1e-6
+
+
+def test_solves_keep_a_deformed_hex_momentum():
+    """dynamics_strainstep_tests.cc:1156-1190"""
+    rest, inv = unit_cell(0.1, 2.0)
+    e = unit_element(rest, inv, 1.0e5, 0.45, 1.0 / 240.0)
+    # glm column-major dmat3(c0, c1, c2): columns given
+    shape = np.array([[1.08, 0.03, -0.02], [-0.05, 0.93, 0.04], [0.02, -0.01, 1.11]]).T
+    pos = np.zeros((8, 3))
+    vel = np.zeros((8, 3))
+    before = np.zeros(3)
+    scale = 0.0
+    for c in range(8):
+        t = float(c)
+        pos[c] = shape @ rest[c] + 1e-3 * np.array([math.sin(1 + t), math.cos(2 * t), math.sin(3 * t)])
+        vel[c] = [math.cos(t), 0.5 * math.sin(2 * t), -0.3 * t]
+        before += vel[c] / inv[c]
+        scale += np.linalg.norm(vel[c]) / inv[c]
+    H.update(e, pos)
+    H.solve(e, vel)
+    H.solve(e, vel)
+    after = (vel / inv[:, None]).sum(axis=0)
+    assert np.linalg.norm(after - before) <= 1e-12 * scale
+
+
+def test_one_block_solve_is_exact():
+    """dynamics_strainstep_tests.cc:1192-1222: a second solve changes no row."""
+    rest, inv = unit_cell(0.1, 1.0)
+    e = unit_element(rest, inv, 1.0e5, 0.3, 1.0 / 240.0)
+    q = rot([0.0, 1.0, 1.0], 0.3)
+    shape = np.array([[1.05, 0.0, 0.0], [0.02, 0.97, 0.0], [0.0, 0.01, 1.02]]).T
+    pos = (rest @ shape.T) @ qmat(q).T
+    vel = np.zeros((8, 3))
+    H.set_rotation(e, q)
+    H.update(e, pos)
+    H.solve(e, vel)
+    first, fv = e.arr("impulse"), e.vol_impulse
+    H.solve(e, vel)
+    assert np.all(np.abs(e.arr("impulse") - first) <= 1e-9 * (1 + np.abs(first)))
+    assert abs(e.vol_impulse - fv) <= 1e-9 * (1 + abs(fv))
+    assert abs(first[0]) > 0.0
+
+
+def test_bent_cell_energy_equals_beam_theory():
+    """dynamics_strainstep_tests.cc:1224-1248"""
+    half, side, curv, E = 0.1, 0.2, 0.05, 1.0e5
+    rest, inv = unit_cell(half, 1.0)
+    e = unit_element(rest, inv, E, 0.3, 1.0 / 240.0)
+    pos = rest + np.stack([-curv * rest[:, 0] * rest[:, 1], np.zeros(8), np.zeros(8)], axis=1)
+    H.update(e, pos)
+    strain = e.arr("strain")
+    assert np.abs(np.delete(strain, 6)).max() <= 1e-12
+    energy = 0.5 * e.stiffness[6] * strain[6] ** 2
+    theory = E * curv ** 2 * side ** 5 / 24.0
+    assert abs(energy - theory) <= 1e-9 * theory
+
+
+def expected_stiffness(row, E, nu, V):
+    mu = E / (2 * (1 + nu))
+    return 2 * mu * V if row < 3 else (4 * mu * V if row < 6 else E * V / 3.0)
+
+
+def test_row_stiffness_from_youngs_modulus_and_poisson():
+    """dynamics_strainstep_tests.cc:1250-1268"""
+    half, E, nu = 0.1, 1.0e5, 0.375
+    V = (2 * half) ** 3
+    rest, inv = unit_cell(half, 1.0)
+    e = unit_element(rest, inv, E, nu, 1.0 / 240.0)
+    for r in range(18):
+        x = expected_stiffness(r, E, nu, V)
+        assert abs(e.stiffness[r] - x) <= 1e-9 * x
+    xv = E * nu / ((1 + nu) * (1 - 2 * nu)) * V
+    assert abs(e.vol_stiffness - xv) <= 1e-9 * xv
+
+
+def test_six_strain_rows_of_a_stretched_and_sheared_hex():
+    """dynamics_strainstep_tests.cc:1270-1290"""
+    rest, inv = unit_cell(0.1, 1.0)
+    e = unit_element(rest, inv, 1.0e5, 0.3, 1.0 / 240.0)
+    S = np.array([[1.02, 0.03, 0.01], [0.03, 0.99, 0.02], [0.01, 0.02, 1.04]])
+    H.update(e, rest @ S.T)
+    expected = np.array([0.02, -0.01, 0.04, 0.03, 0.02, 0.01] + [0.0] * 12)
+    assert np.abs(e.arr("strain") - expected).max() <= 1e-12
+
+
+def test_sheared_or_warped_rest_cells_that_only_turn_have_no_strain():
+    """dynamics_strainstep_tests.cc:1292-1331"""
+    half = 0.1
+    shear = np.array([[1.0, 0.3, 0.0], [0.0, 1.0, 0.0], [0.0, 0.0, 1.0]])  # columns (1,0,0),(0.3,1,0),(0,0,1)
+    q = rot([-1.0, 0.5, 2.0], 1.1)
+    sheared = np.array([half * (shear @ H.corner_signs(c)) for c in range(8)])
+    warped = sheared.copy()
+    warped[7] += [0.02, 0.01, -0.015]
+    warped[3] += [-0.01, 0.0, 0.02]
+    s_shape, _ = H.corner_rest(sheared)
+    w_shape, w_hg = H.corner_rest(warped)
+    assert abs(s_shape[0, 1] - s_shape[1, 0]) > 1e-3
+    assert abs(w_shape[0, 1] - w_shape[1, 0]) > 1e-3
+    assert np.linalg.norm(w_hg[0]) > 1e-3
+    _, inv = unit_cell(half, 1.0)
+    for cell in (sheared, warped):
+        side = cell[7, 0] - cell[0, 0]
+        e = H.make(cell, inv, side ** 3, np.float32(1e5), np.float32(0.3), 1.0 / 240.0)
+        pos = np.array([1.0, 2.0, 3.0]) + cell @ qmat(q).T
+        H.set_rotation(e, q)
+        H.update(e, pos)
+        assert np.abs(e.arr("strain")).max() <= 1e-12
+        assert abs(e.rhs / e.vol_rate) <= 1e-12
+
+
+def test_volume_gradient_is_the_derivative_of_the_volume():
+    """dynamics_strainstep_tests.cc:1333-1374"""
+    delta = 1e-6
+    rest, inv = unit_cell(0.1, 1.0)
+    e = unit_element(rest, inv, 1.0e5, 0.3, 1.0 / 240.0)
+    q = rot([1.0, 2.0, 3.0], 0.3)
+    S = np.array([[1.02, 0.03, 0.01], [0.03, 0.99, 0.02], [0.01, 0.02, 1.04]])
+    pos = (rest @ S.T) @ qmat(q).T
+    H.update(e, pos)
+    grad = e.arr("vol_gradient").reshape(8, 3)
+
+    def volume_at(moved):
+        c = H.Hex.from_buffer_copy(e)
+        H.update(c, moved)
+        return c.rhs / c.vol_rate + 1.0
+
+    assert abs(volume_at(pos) - np.linalg.det(S)) <= 1e-12
+    for c in range(8):
+        for k in range(3):
+            f, b = pos.copy(), pos.copy()
+            f[c, k] += delta
+            b[c, k] -= delta
+            slope = (volume_at(f) - volume_at(b)) / (2 * delta)
+            assert abs(slope - grad[c, k]) <= 1e-7
+
+
+def make_soft(hertz, zeta, h):
+    """Soft step of a spring of `hertz` and damping ratio `zeta` over h
+    (E. Catto, "Solver2D", 2024: omega = 2 pi f, a1 = 2 zeta + h omega,
+    a2 = h omega a1, a3 = 1 / (1 + a2); bias rate omega / a1, mass scale
+    a2 a3, impulse scale a3)."""
+    omega = 2 * math.pi * hertz
+    a1 = 2 * zeta + h * omega
+    a2 = h * omega * a1
+    a3 = 1 / (1 + a2)
+    return omega / a1, a2 * a3, a3
+
+
+def test_row_springs_are_critically_damped_soft_steps():
+    """dynamics_strainstep_tests.cc:1376-1420"""
+    half, E, nu, h = 0.1, 1.0e5, 0.375, 1.0 / 240.0
+    V = (2 * half) ** 3
+    grad, bend = 1.0 / (8 * half), 1.0 / (4 * 2 * half)
+    rest, inv = unit_cell(half, 1.0)
+    e = unit_element(rest, inv, E, nu, h)
+    sum_inv = inv.sum()
+
+    def expect(stiffness, weight, rate, softness):
+        hertz = math.sqrt(stiffness * weight) / (2 * math.pi)
+        bias, mass_scale, impulse_scale = make_soft(hertz, 1.0, h)
+        assert abs(rate - bias) <= 1e-9 * bias
+        assert abs(softness - impulse_scale / mass_scale) <= 1e-9 * softness
+
+    for r in range(18):
+        jsq = grad * grad if r < 3 else (0.5 * grad * grad if r < 6 else bend * bend)
+        w = jsq * sum_inv
+        expect(expected_stiffness(r, E, nu, V), w, e.rate[r], e.softness[r] / w)
+    lam = E * nu / ((1 + nu) * (1 - 2 * nu)) * V
+    expect(lam, 3 * grad * grad * sum_inv, e.vol_rate, e.vol_softness)
+
+
+# ---- lattice bodies --------------------------------------------------------
+
+def jelly(E, density=1000.0, poisson=0.4):
+    m = oracle.Material()
+    m.friction, m.density, m.strength = 0.7, density, math.inf
+    m.youngs_modulus, m.poissons_ratio = E, poisson
+    m.yield_stress, m.damping_ratio, m.tear_strain = math.inf, math.inf, math.inf
+    m.plastic_max, m.crush_stress, m.crush_limit = math.inf, math.inf, 1.0
+    return m
+
+
+def hold_at(centre, half):
+    s = oracle.Support()
+    M = [1, 0, 0, -centre[0], 0, 1, 0, -centre[1], 0, 0, 1, -centre[2]]
+    s.world_to_support[:] = [float(v) for v in M]
+    s.half[:] = [float(v) for v in half]
+    return s
+
+
+def box(half, cell, material, supports=(), lanes=2):
+    """A solid box lattice of hexes centred on the origin (softBox,
+    dynamics_strainstep_tests.cc:103-123, for boxes the voxel divides)."""
+    cells = [int(round(2 * h / cell)) for h in half]
+    mesh = H.build_hex_lattice([c + 1 for c in cells], [cell] * 3, [-h for h in half],
+                               [material] * lanes, list(supports))
+    return mesh
+
+
+@pytest.mark.parametrize("requested,expected", [(0, 4), (8, 8)])
+def test_free_fall_is_semi_implicit_euler(requested, expected):
+    """checkFreeFall (dynamics_strainstep_tests.cc:156-183, 1071-1077)"""
+    mat = jelly(8.0e3)
+    mesh = box((0.3, 0.3, 0.3), 0.15, mat)
+    body = H.LatticeBody(mesh, [mat, mat], offset=(0.0, 50.0, 0.0))
+    body.set_substeps(requested)
+    x0, _ = body.state()
+    steps = 30
+    for _ in range(steps):
+        assert body.step() == expected
+    x, v = body.state()
+    h = (1.0 / 60.0) / expected
+    n = expected * steps
+    assert np.abs(v[:, 1] + 9.81 * h * n).max() <= 1e-9
+    assert np.abs(x[:, 1] - x0[:, 1] + 9.81 * h * h * n * (n + 1) / 2).max() <= 1e-9
+    assert np.abs(x[:, [0, 2]] - x0[:, [0, 2]]).max() <= 1e-12
+
+
+def test_hex_mesh_counts_masses_and_colours():
+    mat = jelly(8.0e3)
+    mesh = box((0.3, 0.3, 0.3), 0.15, mat)
+    assert mesh["hex"].shape == (64, 8) and mesh["rest"].shape == (125, 3)
+    # rho V / 8 per corner, summed in float: total mass rho * box volume
+    assert abs(mesh["mass"].astype(np.float64).sum() - 1000.0 * 0.6 ** 3) <= 1e-4
+    body = H.LatticeBody(mesh, [mat])
+    ncol, hcol = body.colours()
+    assert set(hcol.tolist()) == set(range(8))
+    # no two hexes of one colour share a node
+    for c in range(8):
+        nodes = mesh["hex"][hcol == c].reshape(-1)
+        assert np.unique(nodes).size == nodes.size
+    # an interior node gathers 8 corner shares, a box corner node one
+    assert mesh["mass"].max() == 8 * mesh["mass"].min()
+
+
+def cantilever(thickness, cell, poisson, substeps, steps, colored=False):
+    """cantilever (dynamics_strainstep_tests.cc:305-360): a 4 m beam held at
+    x = -2, tip sag averaged over the last half of the frames."""
+    length = 4.0
+    mat = jelly(5.0e5, density=5.0, poisson=poisson)
+    mat.friction = 0.5
+    clamp = [hold_at((-length / 2, 0.0, 0.0), (0.01, 1.0, 1.0))]
+    mesh = box((length / 2, thickness / 2, thickness / 2), cell, mat, clamp)
+    body = H.LatticeBody(mesh, [mat, mat])
+    body.set_substeps(substeps)
+    tip = int(np.argmin(np.linalg.norm(mesh["rest"] - [length / 2, thickness / 2, thickness / 2], axis=1)))
+    window = steps // 2
+    sag = 0.0
+    for frame in range(1, steps + 1):
+        body.step(colored=colored)
+        if frame > steps - window:
+            x, _ = body.state()
+            sag += (mesh["rest"][tip, 1] - x[tip, 1]) / window
+    inertia = thickness ** 4 / 12.0
+    theory = 5.0 * 9.81 * thickness ** 2 * length ** 4 / (8 * 5.0e5 * inertia)
+    return sag, theory, mesh["hex"].shape[0]
+
+
+@pytest.mark.parametrize("colored", [False, True])
+@pytest.mark.parametrize("poisson", [0.0, 0.45])
+def test_one_cell_thick_cantilever_bends_as_beam_theory(poisson, colored):
+    """dynamics_strainstep_tests.cc:1806-1813"""
+    sag, theory, cells = cantilever(0.4, 0.4, poisson, 48, 600, colored)
+    assert cells == 10
+    assert abs(sag - theory) / theory < 0.1
+
+
+@pytest.mark.parametrize("colored", [False, True])
+def test_two_cell_thick_cantilever_bends_as_beam_theory(colored):
+    """dynamics_strainstep_tests.cc:1815-1820"""
+    sag, theory, cells = cantilever(0.4, 0.2, 0.3, 16, 600, colored)
+    assert cells == 80
+    assert abs(sag - theory) / theory < 0.1
+
+
+def test_colour_order_and_cell_order_land_together():
+    """The colour sweep is another Gauss-Seidel order of the same element
+    solves: on a sagging block both orders settle to the same shape."""
+    mat = jelly(2.0e5, poisson=0.3)
+    clamp = [hold_at((-0.4, 0.0, 0.0), (0.01, 1.0, 1.0))]
+    mesh = box((0.4, 0.1, 0.1), 0.1, mat, clamp)
+    a, b = H.LatticeBody(mesh, [mat]), H.LatticeBody(mesh, [mat])
+    for _ in range(120):
+        a.step(colored=False)
+        b.step(colored=True)
+    xa, _ = a.state()
+    xb, _ = b.state()
+    sag = np.abs(xa - mesh["rest"]).max()
+    assert sag > 1e-3
+    assert np.abs(xa - xb).max() <= 0.02 * sag
