@@ -226,6 +226,103 @@ def slab_run(steps):
                        "tets": T, "nodes_after": nodes, "per_step": per_step}}
 
 
+# Algorithmic bytes of one element solve and one element update of the
+# strainstep hex lattice (FP64, SoA planes, lattice.hip): everything the
+# kernel must read and write once for that element.  The 18x18 inverse (2592
+# bytes) counts in full when every element has its own, and in proportion to
+# the distinct inverses when equal ones are shared.
+LAT_SOLVE_BYTES = (72 + 72 + 32 + 8 + 4 + 4 + 64     # shape^-1, turn, nodes, bend, body, inverse id, 1/m
+                   + 8 * 3 * 8 * 2                   # corner velocities read and written
+                   + 192 + 144 + 144 + 144 * 2       # volume gradient, bias, softness, impulse r/w
+                   + 144 + 144 + 24 + 16)            # coupling, image, gain/rhs/schur, volume impulse r/w
+LAT_UPDATE_BYTES = (8 * 3 * 8 + 72 + 32 + 4 * 8 * 2  # corner positions, shape^-1, nodes, rotation r/w
+                    + 96 + 144 + 144 + 64 + 24 + 1 + 4 + 4   # rest hourglass, rate, cap, 1/m, volume, flags
+                    + 144 + 8 + 8 * 3 * 8 * 2        # impulses, corner velocities r/w (warm start)
+                    + 72 + 144 + 192 + 144 + 144 + 24)  # turn, bias, volume gradient, coupling, image, scalars
+LAT_INVERSE_BYTES = 18 * 18 * 8
+
+
+def lattice_run(steps, warmup, cpu_seconds, direct=False):
+    """Strainstep hex lattice on the GPU: the 40^3 cube (64,000 hexes, every
+    hex its own material), FP64, islandSubsteps substeps of integrate,
+    update + warm start, 4 colour-swept Gauss-Seidel passes of the 18-row
+    element solve, advance."""
+    from talus_amd import api, scenes
+
+    kw = scenes.hex_cube(40)
+    mats = kw.pop("materials")
+    mesh = api.build_hex_lattice(materials=mats, **kw)
+    H = int(mesh["hex_nodes"].shape[0])
+    params = api.lattice_params(flags=api.DIRECT_LAUNCH if direct else 0)
+    with api.Context() as ctx:
+        b = ctx.add_lattice(mesh, mats)
+        for _ in range(max(warmup, 1)):
+            r = ctx.lattice_step(params)
+        S = r["substeps"]
+        t0 = time.perf_counter()
+        reps = [ctx.lattice_step(params) for _ in range(steps)]
+        wall = time.perf_counter() - t0
+        # kernel durations: one more step with an event pair around every
+        # update and solve launch (the same stream, direct launches)
+        timed = ctx.lattice_step(api.lattice_params(flags=api.KERNEL_TIMING))
+        x, _ = ctx.lattice_state(b)
+    solves = reps[-1]["element_solves"]
+    E = reps[-1]["elements"]
+    share = min(1.0, timed["inverses"] / max(E, 1))
+    solve_bytes = (LAT_SOLVE_BYTES + share * LAT_INVERSE_BYTES) * solves / timed["solve_launches"]
+    update_bytes = (LAT_UPDATE_BYTES + share * LAT_INVERSE_BYTES) * (E * S) / max(timed["update_launches"], 1)
+    solve_ms = timed["solve_ms"] / timed["solve_launches"]
+    update_ms = timed["update_ms"] / max(timed["update_launches"], 1)
+    kernels = {
+        "k_lat_solve": {"avg_ms": solve_ms, "bytes": solve_bytes, "gbs": solve_bytes / (solve_ms * 1e-3) / 1e9,
+                        "launches_per_step": timed["solve_launches"], "ms_per_step": timed["solve_ms"]},
+        "k_lat_update": {"avg_ms": update_ms, "bytes": update_bytes,
+                         "gbs": update_bytes / (update_ms * 1e-3) / 1e9,
+                         "launches_per_step": timed["update_launches"], "ms_per_step": timed["update_ms"]},
+    }
+    dom = max(kernels, key=lambda k: kernels[k]["ms_per_step"])
+    value = solves * steps / wall
+    line = {
+        "metric": "hex element solves/sec, strainstep lattice (FP64), 1 GPU",
+        "value": value, "unit": "element solves/s", "n_gpus": 1, "steps": steps, "warmup": warmup,
+        "ms_per_step": 1e3 * wall / steps, "higher_is_better": True, "scaling": "weak", "vs_baseline": None,
+        "dtype": "f64", "data": "synthetic",
+        "config": {"workload": "40^3 hex cube, unit edge, nodes jittered 2 %, per-hex E = 5e6 exp(0.3 N) Pa, "
+                               "nu U[0.25, 0.4], rho 1000, bottom face held, dt 1/60, islandSubsteps, 4 passes",
+                   "hexes": H, "elements": E, "substeps": S, "element_solves_per_step": solves,
+                   "hex_updates_per_s": E * S * steps / wall, "inverses": timed["inverses"],
+                   "device_ms_per_step": sum(r["device_ms"] for r in reps) / steps,
+                   "finite": bool(np.isfinite(x).all())},
+        "kernels": kernels,
+        "roofline": {"bound": "hbm", "kernel": dom, "achieved": kernels[dom]["gbs"], "peak": HBM_PEAK_GBS,
+                     "unit": "GB/s", "frac": kernels[dom]["gbs"] / HBM_PEAK_GBS, "traffic": None,
+                     "algorithmic_bytes": kernels[dom]["bytes"], "avg_ms": kernels[dom]["avg_ms"]},
+    }
+    if cpu_seconds > 0:
+        from oracle import hex as OH
+
+        om = {"rest": mesh["rest_positions"].astype(np.float64), "mass": mesh["node_mass"],
+              "anchored": mesh["anchored"], "hex": mesh["hex_nodes"], "volume": mesh["volume"],
+              "lane": mesh["hex_material"], "voxel": mesh["voxel_size"]}
+        body = OH.LatticeBody(om, mats)
+        body.step()  # builds the elements
+        t0 = time.perf_counter()
+        n = 0
+        while True:
+            body.step()
+            n += 1
+            if time.perf_counter() - t0 > cpu_seconds:
+                break
+        cw = time.perf_counter() - t0
+        line["cpu_baseline"] = {"value": solves * n / cw, "unit": "element solves/s", "cores": 1,
+                                "kind": "port",
+                                "sample": f"{n} step(s) of the whole 40^3 cube with the oracle's C restatement "
+                                          "(serial cell-order sweep, as the reference solves one body on one "
+                                          "thread)"}
+        line["gpu_over_cpu"] = value / line["cpu_baseline"]["value"]
+    return line
+
+
 def adjoint_check(steps, every, elements, seed=5):
     """Differentiation workload: the 1 x 0.1 x 0.1 m beam (103x9x9 hexes,
     50,058 tets, per-tet lognormal E, nu U[0.25,0.4]), a converged 60-step
@@ -329,6 +426,8 @@ def main(argv=None):
                     help="run the differentiation check (50k-tet beam) instead of the throughput bench")
     ap.add_argument("--adjoint-steps", type=int, default=60)
     ap.add_argument("--adjoint-elements", type=int, default=32)
+    ap.add_argument("--lattice", action="store_true",
+                    help="strainstep hex lattice (40^3 cube, FP64): element solves/s")
     ap.add_argument("--copy-probe", action="store_true",
                     help="end with one 256 MiB device-to-device copy (calibrates PMC byte counters)")
     args = ap.parse_args(argv)
@@ -339,6 +438,13 @@ def main(argv=None):
 
     if args.slab:
         print(json.dumps(slab_run(args.slab)), flush=True)
+        return 0
+
+    if args.lattice:
+        print(json.dumps(lattice_run(args.steps, args.warmup, 0.0 if args.skip_cpu else args.cpu_seconds,
+                                     args.direct_launch)), flush=True)
+        if args.copy_probe:
+            copy_probe(256 << 20)
         return 0
 
     if args.adjoint:

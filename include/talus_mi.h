@@ -425,11 +425,15 @@ int talus_mi_hex_element_make(const double *corners, const double *inverse_mass,
 
 /* Step parameters: IntegrationParameters (integrationparameters.h:9-33)
  * subset the strainstep path reads. */
+/* lattice step flag: time every update and solve launch with HIP events
+ * (the report's solve_ms / update_ms); otherwise those stay 0 */
+#define TALUS_MI_KERNEL_TIMING 16u
+
 typedef struct talus_mi_lattice_params {
   double dt;                  /* default 1/60 */
   double gravity[3];          /* default (0, -9.81, 0) */
   uint32_t softstep_substeps; /* default 4; islandSubsteps' floor */
-  uint32_t flags;             /* TALUS_MI_DIRECT_LAUNCH */
+  uint32_t flags;             /* TALUS_MI_DIRECT_LAUNCH | TALUS_MI_KERNEL_TIMING */
 } talus_mi_lattice_params;
 
 typedef struct talus_mi_lattice_report {
@@ -439,6 +443,12 @@ typedef struct talus_mi_lattice_report {
   uint32_t rebuilds;        /* bodies whose elements were built this step */
   uint64_t element_solves;  /* solveElement calls */
   double device_ms;         /* GPU time of the step's launches */
+  uint32_t inverses;        /* distinct 18x18 row-block inverses on the device */
+  uint32_t solve_launches;  /* launches of the solve kernel this step */
+  uint32_t update_launches; /* launches of the update kernel this step */
+  uint32_t reserved;
+  double solve_ms;          /* their summed HIP-event time (KERNEL_TIMING) */
+  double update_ms;
 } talus_mi_lattice_report;
 
 void talus_mi_lattice_default_params(talus_mi_lattice_params *params);

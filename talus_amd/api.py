@@ -27,6 +27,7 @@ DETECT_TEARS = 1
 RECORD_EVENTS = 2
 CONVERGE = 4
 DIRECT_LAUNCH = 8
+KERNEL_TIMING = 16
 EVENT_YIELD = 0
 EVENT_TEAR = 1
 COROTATED = 0
@@ -174,7 +175,10 @@ class LatticeParams(C.Structure):
 
 class LatticeReport(C.Structure):
     _fields_ = [("lattices", C.c_uint32), ("substeps", C.c_uint32), ("elements", C.c_uint32),
-                ("rebuilds", C.c_uint32), ("element_solves", C.c_uint64), ("device_ms", C.c_double)]
+                ("rebuilds", C.c_uint32), ("element_solves", C.c_uint64), ("device_ms", C.c_double),
+                ("inverses", C.c_uint32), ("solve_launches", C.c_uint32),
+                ("update_launches", C.c_uint32), ("reserved", C.c_uint32),
+                ("solve_ms", C.c_double), ("update_ms", C.c_double)]
 
     def as_dict(self):
         return {k: getattr(self, k) for k, _ in self._fields_}

@@ -14,10 +14,14 @@
 // Layout in HBM.  All bodies are concatenated.  Node state x, v is FP64
 // xyz-interleaved.  Elements are stored in slot order: colour 0..7, then
 // body, then cell, so one colour is one contiguous slot range and one
-// launch.  Per-element data are SoA planes plane[k * E + slot] (coalesced
-// over consecutive slots).  The 18x18 row-block inverses are interned: equal
-// inverses (same rest shape, corner masses, material and substep) share one
-// 2592-byte table entry, indexed per element.  Gradients are not stored:
+// launch.  Per-element data are element-major, field[slot * width + k], and
+// G = 16 lanes work on one element (lanes 0..7 its corners, lane l its rows l
+// and l + 16), so the lanes of an element touch consecutive words.  The 18x18
+// row-block inverses are interned: equal inverses (same rest shape, corner
+// masses, material and substep) share one 2592-byte table entry, indexed per
+// element and stored transposed, entry[18 k + r] = Inv[r][k], so the lanes
+// computing rows r read row-block column k as one contiguous run.  Gradients
+// are not stored:
 // corner c's gradient is (shape^-T s_c) / 8 with s_c = +-1, recomputed
 // exactly from the 9 entries of shape^-1.
 //
@@ -44,8 +48,10 @@ namespace tmi {
 
 namespace {
 
-constexpr int LAT_BLOCK = 128;
 constexpr int NODE_BLOCK = 256;
+constexpr uint32_t G = 16;              // lanes per element
+constexpr uint32_t EPB = 16;            // elements per block
+constexpr uint32_t EBLOCK = G * EPB;    // threads per element block
 constexpr uint32_t PASSES = 4;
 constexpr uint32_t ROTATION_ITERATIONS = 16;
 constexpr double MAX_SPEED = 400.0;
@@ -59,7 +65,7 @@ struct LatDev {
   const double *body_h;
   const uint32_t *body_substeps;
   const uint8_t *body_exact;
-  // elements: planes [k * E + slot]
+  // elements, element-major [slot * width + k]; widths below
   const uint32_t *e_body;      // 1 plane
   const uint32_t *e_nodes;     // 8
   const uint32_t *e_inv_id;    // 1
@@ -98,23 +104,6 @@ __device__ __forceinline__ void grad_of(const double *inv, uint32_t c, double *g
   const double s0 = corner_sign(c, 0), s1 = corner_sign(c, 1), s2 = corner_sign(c, 2);
 #pragma unroll
   for (int r = 0; r < 3; ++r) g[r] = (inv[r] * s0 + inv[3 + r] * s1 + inv[6 + r] * s2) / 8.0;
-}
-
-__device__ __forceinline__ void add_rows(double *rows, const double *g, double bend, uint32_t c,
-                                         const double *l) {
-  rows[0] += l[0] * g[0];
-  rows[1] += l[1] * g[1];
-  rows[2] += l[2] * g[2];
-  rows[3] += 0.5 * (l[0] * g[1] + l[1] * g[0]);
-  rows[4] += 0.5 * (l[1] * g[2] + l[2] * g[1]);
-  rows[5] += 0.5 * (l[2] * g[0] + l[0] * g[2]);
-#pragma unroll
-  for (uint32_t m = 0; m < 4; ++m) {
-    const double s = hg_sign(m, c) * bend;
-    rows[6 + 3 * m] += s * l[0];
-    rows[7 + 3 * m] += s * l[1];
-    rows[8 + 3 * m] += s * l[2];
-  }
 }
 
 __device__ __forceinline__ void row_force(const double *imp, const double *g, double bend, uint32_t c,
@@ -204,212 +193,6 @@ __device__ __forceinline__ double clampd(double v, double lo, double hi) {
   return v < lo ? lo : (hi < v ? hi : v);
 }
 
-// updateElement at the current positions, then applyElement with the
-// element's accumulated impulses (the warm start)
-__device__ void lat_update_apply(const LatDev &d, uint32_t e) {
-  const size_t E = d.E;
-  double inv[9];
-#pragma unroll
-  for (int k = 0; k < 9; ++k) inv[k] = d.e_shape_inv[k * E + e];
-  uint32_t nd[8];
-#pragma unroll
-  for (int c = 0; c < 8; ++c) nd[c] = d.e_nodes[c * E + e];
-  double F[9] = {}, hg[12] = {};
-#pragma unroll
-  for (uint32_t c = 0; c < 8; ++c) {
-    const double *xp = d.x + 3 * size_t(nd[c]);
-    const double xc[3] = {xp[0], xp[1], xp[2]};
-    double g[3];
-    grad_of(inv, c, g);
-#pragma unroll
-    for (int r = 0; r < 3; ++r)
-#pragma unroll
-      for (int k = 0; k < 3; ++k) F[3 * r + k] += xc[r] * g[k];
-#pragma unroll
-    for (uint32_t m = 0; m < 4; ++m) {
-      const double s = hg_sign(m, c);
-#pragma unroll
-      for (int r = 0; r < 3; ++r) hg[3 * m + r] += s * xc[r];
-    }
-  }
-  double q[4];
-#pragma unroll
-  for (int k = 0; k < 4; ++k) q[k] = d.e_quat[k * E + e];
-  rotation_of(F, q);
-#pragma unroll
-  for (int k = 0; k < 4; ++k) d.e_quat[k * E + e] = q[k];
-  double T[9];
-  quat_matrix(q, T);
-#pragma unroll
-  for (int k = 0; k < 9; ++k) d.e_turn[k * E + e] = T[k];
-  // local = T^T F
-  double L[9];
-#pragma unroll
-  for (int r = 0; r < 3; ++r)
-#pragma unroll
-    for (int k = 0; k < 3; ++k) L[3 * r + k] = T[r] * F[k] + T[3 + r] * F[3 + k] + T[6 + r] * F[6 + k];
-  const double det = F[0] * (F[4] * F[8] - F[5] * F[7]) - F[1] * (F[3] * F[8] - F[5] * F[6]) +
-                     F[2] * (F[3] * F[7] - F[4] * F[6]);
-  const bool inverted = det < 0.0;
-  const double bend = d.e_bend[e];
-  double raw[R];
-  raw[0] = L[0] - 1.0;
-  raw[1] = L[4] - 1.0;
-  raw[2] = L[8] - 1.0;
-  raw[3] = 0.5 * (L[1] + L[3]);
-  raw[4] = 0.5 * (L[5] + L[7]);
-  raw[5] = 0.5 * (L[6] + L[2]);
-#pragma unroll
-  for (uint32_t m = 0; m < 4; ++m)
-#pragma unroll
-    for (int k = 0; k < 3; ++k) {
-      const double b = T[k] * hg[3 * m] + T[3 + k] * hg[3 * m + 1] + T[6 + k] * hg[3 * m + 2];
-      raw[6 + 3 * m + k] = (b - d.e_rest_hg[(3 * m + k) * E + e]) * bend;
-    }
-#pragma unroll
-  for (uint32_t r = 0; r < R; ++r) {
-    const double cap0 = d.e_cap[r * E + e];
-    const double cap = inverted ? cap0 : (cap0 < 1.0 ? 1.0 : cap0);
-    d.e_bias[r * E + e] = d.e_rate[r * E + e] * clampd(raw[r], -cap, cap);
-  }
-  // cofactor columns F1 x F2, F2 x F0, F0 x F1 (row-major cof[3r + col])
-  double cof[9];
-  {
-    const double f0[3] = {F[0], F[3], F[6]}, f1[3] = {F[1], F[4], F[7]}, f2[3] = {F[2], F[5], F[8]};
-    double c0[3], c1[3], c2[3];
-    cross3(f1, f2, c0);
-    cross3(f2, f0, c1);
-    cross3(f0, f1, c2);
-#pragma unroll
-    for (int r = 0; r < 3; ++r) {
-      cof[3 * r] = c0[r];
-      cof[3 * r + 1] = c1[r];
-      cof[3 * r + 2] = c2[r];
-    }
-  }
-  double coupling[R] = {};
-  double vweight = 0.0;
-#pragma unroll
-  for (uint32_t c = 0; c < 8; ++c) {
-    double g[3], vg[3];
-    grad_of(inv, c, g);
-#pragma unroll
-    for (int r = 0; r < 3; ++r) vg[r] = cof[3 * r] * g[0] + cof[3 * r + 1] * g[1] + cof[3 * r + 2] * g[2];
-#pragma unroll
-    for (int r = 0; r < 3; ++r) d.e_vgrad[(3 * c + r) * E + e] = vg[r];
-    const double w = d.e_inv_mass[c * E + e];
-    vweight += w * (vg[0] * vg[0] + vg[1] * vg[1] + vg[2] * vg[2]);
-    double l[3];
-#pragma unroll
-    for (int k = 0; k < 3; ++k) l[k] = (T[k] * vg[0] + T[3 + k] * vg[1] + T[6 + k] * vg[2]) * w;
-    add_rows(coupling, g, bend, c, l);
-  }
-#pragma unroll
-  for (uint32_t r = 0; r < R; ++r) d.e_coupling[r * E + e] = coupling[r];
-  const double vrate = d.e_vol[e], vsoft = d.e_vol[E + e], vcap0 = d.e_vol[2 * E + e];
-  const double vcap = inverted ? vcap0 : (vcap0 < 1.0 ? 1.0 : vcap0);
-  const double volume = clampd(det - 1.0, -vcap, vcap);
-  const double *Inv = d.inv_table + size_t(d.e_inv_id[e]) * HEX_INVERSE;
-  double coupled = 0.0;
-  for (uint32_t r = 0; r < R; ++r) {
-    double t = 0.0;
-#pragma unroll
-    for (uint32_t k = 0; k < R; ++k) t += Inv[R * r + k] * coupling[k];
-    d.e_image[r * E + e] = t;
-    coupled += coupling[r] * t;
-  }
-  const double gain = vweight * vsoft;
-  const double schur = vweight + gain - coupled;
-  d.e_scal[e] = gain;
-  d.e_scal[E + e] = volume * vrate;
-  d.e_scal[2 * E + e] = d.e_volumetric[e] && schur > 0.0 ? 1.0 / schur : 0.0;
-  // warm start: applyElement with the accumulated impulses
-  double imp[R];
-#pragma unroll
-  for (uint32_t r = 0; r < R; ++r) imp[r] = d.e_impulse[r * E + e];
-  const double vimp = d.e_vimp[e];
-#pragma unroll
-  for (uint32_t c = 0; c < 8; ++c) {
-    const double w = d.e_inv_mass[c * E + e];
-    if (w == 0.0) continue;
-    double g[3], lf[3];
-    grad_of(inv, c, g);
-    row_force(imp, g, bend, c, lf);
-    double *vp = d.v + 3 * size_t(nd[c]);
-#pragma unroll
-    for (int r = 0; r < 3; ++r) {
-      const double f = T[3 * r] * lf[0] + T[3 * r + 1] * lf[1] + T[3 * r + 2] * lf[2];
-      vp[r] -= (f + d.e_vgrad[(3 * c + r) * E + e] * vimp) * w;
-    }
-  }
-}
-
-// solveElement: one block solve of the 18 rows with the volume row folded in
-// through its Schur complement, applied to the corner velocities
-__device__ void lat_solve(const LatDev &d, uint32_t e) {
-  const size_t E = d.E;
-  double inv[9], T[9];
-#pragma unroll
-  for (int k = 0; k < 9; ++k) {
-    inv[k] = d.e_shape_inv[k * E + e];
-    T[k] = d.e_turn[k * E + e];
-  }
-  uint32_t nd[8];
-#pragma unroll
-  for (int c = 0; c < 8; ++c) nd[c] = d.e_nodes[c * E + e];
-  const double bend = d.e_bend[e];
-  double res[R] = {};
-  double vspeed = 0.0;
-#pragma unroll
-  for (uint32_t c = 0; c < 8; ++c) {
-    const double *vp = d.v + 3 * size_t(nd[c]);
-    const double vc[3] = {vp[0], vp[1], vp[2]};
-    vspeed += d.e_vgrad[(3 * c) * E + e] * vc[0] + d.e_vgrad[(3 * c + 1) * E + e] * vc[1] +
-              d.e_vgrad[(3 * c + 2) * E + e] * vc[2];
-    double g[3], b[3];
-    grad_of(inv, c, g);
-#pragma unroll
-    for (int k = 0; k < 3; ++k) b[k] = T[k] * vc[0] + T[3 + k] * vc[1] + T[6 + k] * vc[2];
-    add_rows(res, g, bend, c, b);
-  }
-#pragma unroll
-  for (uint32_t r = 0; r < R; ++r)
-    res[r] += d.e_bias[r * E + e] - d.e_soft[r * E + e] * d.e_impulse[r * E + e];
-  const double *Inv = d.inv_table + size_t(d.e_inv_id[e]) * HEX_INVERSE;
-  double sol[R];
-  double coupled = 0.0;
-  for (uint32_t r = 0; r < R; ++r) {
-    double t = 0.0;
-#pragma unroll
-    for (uint32_t k = 0; k < R; ++k) t += Inv[R * r + k] * res[k];
-    sol[r] = t;
-    coupled += d.e_coupling[r * E + e] * t;
-  }
-  const double gain = d.e_scal[e], rhs = d.e_scal[E + e], schur = d.e_scal[2 * E + e];
-  const double vimp = d.e_vimp[e];
-  const double change = schur * ((rhs - gain * vimp + vspeed) - coupled);
-#pragma unroll
-  for (uint32_t r = 0; r < R; ++r) {
-    sol[r] -= d.e_image[r * E + e] * change;
-    d.e_impulse[r * E + e] += sol[r];
-  }
-  d.e_vimp[e] = vimp + change;
-#pragma unroll
-  for (uint32_t c = 0; c < 8; ++c) {
-    const double w = d.e_inv_mass[c * E + e];
-    if (w == 0.0) continue;
-    double g[3], lf[3];
-    grad_of(inv, c, g);
-    row_force(sol, g, bend, c, lf);
-    double *vp = d.v + 3 * size_t(nd[c]);
-#pragma unroll
-    for (int r = 0; r < 3; ++r) {
-      const double f = T[3 * r] * lf[0] + T[3 * r + 1] * lf[1] + T[3 * r + 2] * lf[2];
-      vp[r] -= (f + d.e_vgrad[(3 * c + r) * E + e] * change) * w;
-    }
-  }
-}
-
 __global__ void __launch_bounds__(NODE_BLOCK) k_lat_integrate(LatDev d, uint32_t substep, double gx,
                                                              double gy, double gz) {
   const uint32_t i = blockIdx.x * NODE_BLOCK + threadIdx.x;
@@ -451,30 +234,310 @@ __global__ void __launch_bounds__(NODE_BLOCK) k_lat_advance(LatDev d, uint32_t s
   xp[2] += h * v2;
 }
 
-// update + warm start of one colour's elements of the bodies whose exact
-// flag equals `exact`: exact bodies update every element before the passes,
-// the others (too stiff for their substep) update each colour right before
-// its first solve, as the reference updates such an element right before
-// solving it in pass 0
-__global__ void __launch_bounds__(LAT_BLOCK) k_lat_update(LatDev d, uint32_t first, uint32_t count,
-                                                          uint32_t substep, uint32_t exact) {
-  const uint32_t t = blockIdx.x * LAT_BLOCK + threadIdx.x;
-  if (t >= count) return;
-  const uint32_t e = first + t;
-  const uint32_t b = d.e_body[e];
-  if (substep >= d.body_substeps[b] || uint32_t(d.body_exact[b]) != exact) return;
-  lat_update_apply(d, e);
+// Element-major planes: row-indexed data of element e sit at [e * width + k],
+// so the lanes of one element read and write consecutive words.
+__device__ __forceinline__ size_t at(uint32_t e, uint32_t width, uint32_t k) {
+  return size_t(e) * width + k;
 }
 
-// one Gauss-Seidel pass over one colour's elements
-__global__ void __launch_bounds__(LAT_BLOCK) k_lat_solve(LatDev d, uint32_t first, uint32_t count,
-                                                         uint32_t substep) {
-  const uint32_t t = blockIdx.x * LAT_BLOCK + threadIdx.x;
-  if (t >= count) return;
+// contribution of corner c's frame vector b (3) to row r of J b
+__device__ __forceinline__ double row_term(uint32_t r, uint32_t c, const double *g, double bend,
+                                           const double *b) {
+  switch (r) {
+    case 0: return b[0] * g[0];
+    case 1: return b[1] * g[1];
+    case 2: return b[2] * g[2];
+    case 3: return 0.5 * (b[0] * g[1] + b[1] * g[0]);
+    case 4: return 0.5 * (b[1] * g[2] + b[2] * g[1]);
+    case 5: return 0.5 * (b[2] * g[0] + b[0] * g[2]);
+    default: return hg_sign((r - 6) / 3, c) * bend * b[(r - 6) % 3];
+  }
+}
+
+// updateElement at the current positions, then applyElement with the
+// element's accumulated impulses (the warm start) for one colour's elements
+// of the bodies whose exact flag equals `exact`.  G lanes per element:
+// lanes 0..7 own the corners, lane l the rows l and l + G; sums over corners
+// and rows run in the reference's order.
+__global__ void __launch_bounds__(EBLOCK) k_lat_update(LatDev d, uint32_t first, uint32_t count,
+                                                       uint32_t substep, uint32_t exact) {
+  __shared__ double s_x[EPB][24];   // corner positions
+  __shared__ double s_l[EPB][24];   // per corner (T^T vg_c) w_c
+  __shared__ double s_w[EPB][8];    // per corner w_c |vg_c|^2
+  __shared__ double s_cp[EPB][R];   // coupling rows
+  __shared__ double s_img[EPB][R];  // image rows
+  __shared__ double s_imp[EPB][R];  // accumulated impulses
+  const uint32_t lane = threadIdx.x % G, slot = threadIdx.x / G;
+  const uint32_t t = blockIdx.x * EPB + slot;
   const uint32_t e = first + t;
-  const uint32_t b = d.e_body[e];
-  if (substep >= d.body_substeps[b]) return;
-  lat_solve(d, e);
+  bool active = t < count;
+  if (active) {
+    const uint32_t b = d.e_body[e];
+    active = substep < d.body_substeps[b] && uint32_t(d.body_exact[b]) == exact;
+  }
+  double inv[9];
+  double g[8][3];
+  uint32_t node = 0;
+  if (active) {
+#pragma unroll
+    for (int k = 0; k < 9; ++k) inv[k] = d.e_shape_inv[at(e, 9, k)];
+#pragma unroll
+    for (uint32_t c = 0; c < 8; ++c) grad_of(inv, c, g[c]);
+    if (lane < 8) {
+      node = d.e_nodes[at(e, 8, lane)];
+      const double *xp = d.x + 3 * size_t(node);
+      s_x[slot][3 * lane] = xp[0];
+      s_x[slot][3 * lane + 1] = xp[1];
+      s_x[slot][3 * lane + 2] = xp[2];
+    }
+    for (uint32_t r = lane; r < R; r += G) s_imp[slot][r] = d.e_impulse[at(e, R, r)];
+  }
+  __syncthreads();
+  double T[9], cof[9], det = 0.0, bend = 0.0;
+  bool inverted = false;
+  if (active) {
+    double F[9] = {}, hg[12] = {};
+#pragma unroll
+    for (uint32_t c = 0; c < 8; ++c) {
+      const double xc[3] = {s_x[slot][3 * c], s_x[slot][3 * c + 1], s_x[slot][3 * c + 2]};
+#pragma unroll
+      for (int r = 0; r < 3; ++r)
+#pragma unroll
+        for (int k = 0; k < 3; ++k) F[3 * r + k] += xc[r] * g[c][k];
+#pragma unroll
+      for (uint32_t m = 0; m < 4; ++m) {
+        const double s = hg_sign(m, c);
+#pragma unroll
+        for (int r = 0; r < 3; ++r) hg[3 * m + r] += s * xc[r];
+      }
+    }
+    double q[4];
+#pragma unroll
+    for (int k = 0; k < 4; ++k) q[k] = d.e_quat[at(e, 4, k)];
+    rotation_of(F, q);  // every lane of the element runs the same iteration
+    quat_matrix(q, T);
+    if (lane == 0) {
+#pragma unroll
+      for (int k = 0; k < 4; ++k) d.e_quat[at(e, 4, k)] = q[k];
+#pragma unroll
+      for (int k = 0; k < 9; ++k) d.e_turn[at(e, 9, k)] = T[k];
+    }
+    det = F[0] * (F[4] * F[8] - F[5] * F[7]) - F[1] * (F[3] * F[8] - F[5] * F[6]) +
+          F[2] * (F[3] * F[7] - F[4] * F[6]);
+    inverted = det < 0.0;
+    bend = d.e_bend[e];
+    // strain rows of T^T F and the hourglass offsets, capped, times the rate
+    double L[9], off[12];
+#pragma unroll
+    for (int i = 0; i < 3; ++i)
+#pragma unroll
+      for (int j = 0; j < 3; ++j) L[3 * i + j] = T[i] * F[j] + T[3 + i] * F[3 + j] + T[6 + i] * F[6 + j];
+#pragma unroll
+    for (uint32_t m = 0; m < 4; ++m)
+#pragma unroll
+      for (int k = 0; k < 3; ++k)
+        off[3 * m + k] = T[k] * hg[3 * m] + T[3 + k] * hg[3 * m + 1] + T[6 + k] * hg[3 * m + 2];
+    for (uint32_t r = lane; r < R; r += G) {
+      double raw = 0.0;
+      switch (r) {
+        case 0: raw = L[0] - 1.0; break;
+        case 1: raw = L[4] - 1.0; break;
+        case 2: raw = L[8] - 1.0; break;
+        case 3: raw = 0.5 * (L[1] + L[3]); break;
+        case 4: raw = 0.5 * (L[5] + L[7]); break;
+        case 5: raw = 0.5 * (L[6] + L[2]); break;
+        default: {
+          double o = 0.0;
+#pragma unroll
+          for (uint32_t q2 = 0; q2 < 12; ++q2)
+            if (q2 + 6 == r) o = off[q2];
+          raw = (o - d.e_rest_hg[at(e, 12, r - 6)]) * bend;
+        }
+      }
+      const double cap0 = d.e_cap[at(e, R, r)];
+      const double cap = inverted ? cap0 : (cap0 < 1.0 ? 1.0 : cap0);
+      d.e_bias[at(e, R, r)] = d.e_rate[at(e, R, r)] * clampd(raw, -cap, cap);
+    }
+    // cofactor columns F1 x F2, F2 x F0, F0 x F1
+    {
+      const double f0[3] = {F[0], F[3], F[6]}, f1[3] = {F[1], F[4], F[7]}, f2[3] = {F[2], F[5], F[8]};
+      double c0[3], c1[3], c2[3];
+      cross3(f1, f2, c0);
+      cross3(f2, f0, c1);
+      cross3(f0, f1, c2);
+#pragma unroll
+      for (int r = 0; r < 3; ++r) {
+        cof[3 * r] = c0[r];
+        cof[3 * r + 1] = c1[r];
+        cof[3 * r + 2] = c2[r];
+      }
+    }
+    if (lane < 8) {
+      const uint32_t c = lane;
+      double gc[3], vg[3];
+      grad_of(inv, c, gc);
+#pragma unroll
+      for (int r = 0; r < 3; ++r) vg[r] = cof[3 * r] * gc[0] + cof[3 * r + 1] * gc[1] + cof[3 * r + 2] * gc[2];
+#pragma unroll
+      for (int r = 0; r < 3; ++r) d.e_vgrad[at(e, 24, 3 * c + r)] = vg[r];
+      const double w = d.e_inv_mass[at(e, 8, c)];
+      s_w[slot][c] = w * (vg[0] * vg[0] + vg[1] * vg[1] + vg[2] * vg[2]);
+#pragma unroll
+      for (int k = 0; k < 3; ++k) s_l[slot][3 * c + k] = (T[k] * vg[0] + T[3 + k] * vg[1] + T[6 + k] * vg[2]) * w;
+    }
+  }
+  __syncthreads();
+  if (active) {
+    for (uint32_t r = lane; r < R; r += G) {
+      double acc = 0.0;
+#pragma unroll
+      for (uint32_t c = 0; c < 8; ++c) acc += row_term(r, c, g[c], bend, &s_l[slot][3 * c]);
+      s_cp[slot][r] = acc;
+      d.e_coupling[at(e, R, r)] = acc;
+    }
+  }
+  __syncthreads();
+  if (active) {
+    const double *Inv = d.inv_table + size_t(d.e_inv_id[e]) * HEX_INVERSE;  // transposed
+    for (uint32_t r = lane; r < R; r += G) {
+      double t2 = 0.0;
+#pragma unroll
+      for (uint32_t k = 0; k < R; ++k) t2 += Inv[R * k + r] * s_cp[slot][k];
+      s_img[slot][r] = t2;
+      d.e_image[at(e, R, r)] = t2;
+    }
+  }
+  __syncthreads();
+  if (active) {
+    double coupled = 0.0, vweight = 0.0;
+#pragma unroll
+    for (uint32_t r = 0; r < R; ++r) coupled += s_cp[slot][r] * s_img[slot][r];
+#pragma unroll
+    for (uint32_t c = 0; c < 8; ++c) vweight += s_w[slot][c];
+    const double vrate = d.e_vol[at(e, 3, 0)], vsoft = d.e_vol[at(e, 3, 1)], vcap0 = d.e_vol[at(e, 3, 2)];
+    const double vcap = inverted ? vcap0 : (vcap0 < 1.0 ? 1.0 : vcap0);
+    const double volume = clampd(det - 1.0, -vcap, vcap);
+    const double gain = vweight * vsoft;
+    const double schur = vweight + gain - coupled;
+    if (lane == 0) {
+      d.e_scal[at(e, 3, 0)] = gain;
+      d.e_scal[at(e, 3, 1)] = volume * vrate;
+      d.e_scal[at(e, 3, 2)] = d.e_volumetric[e] && schur > 0.0 ? 1.0 / schur : 0.0;
+    }
+    // warm start: applyElement with the accumulated impulses
+    if (lane < 8) {
+      const uint32_t c = lane;
+      const double w = d.e_inv_mass[at(e, 8, c)];
+      if (w != 0.0) {
+        const double vimp = d.e_vimp[e];
+        double gc[3], lf[3], vg[3];
+        grad_of(inv, c, gc);
+        row_force(s_imp[slot], gc, bend, c, lf);
+#pragma unroll
+        for (int r = 0; r < 3; ++r) vg[r] = cof[3 * r] * gc[0] + cof[3 * r + 1] * gc[1] + cof[3 * r + 2] * gc[2];
+        double *vp = d.v + 3 * size_t(node);
+#pragma unroll
+        for (int r = 0; r < 3; ++r) {
+          const double f = T[3 * r] * lf[0] + T[3 * r + 1] * lf[1] + T[3 * r + 2] * lf[2];
+          vp[r] -= (f + vg[r] * vimp) * w;
+        }
+      }
+    }
+  }
+}
+
+// solveElement for one colour's elements: one block solve of the 18 rows with
+// the volume row folded in through its Schur complement, applied to the
+// corner velocities.  G lanes per element as in k_lat_update.
+__global__ void __launch_bounds__(EBLOCK) k_lat_solve(LatDev d, uint32_t first, uint32_t count,
+                                                      uint32_t substep) {
+  __shared__ double s_b[EPB][24];   // per corner T^T v_c
+  __shared__ double s_vs[EPB][8];   // per corner vg_c . v_c
+  __shared__ double s_res[EPB][R];  // residual rows, then the final solution
+  __shared__ double s_sol[EPB][R];  // Inv * residual
+  const uint32_t lane = threadIdx.x % G, slot = threadIdx.x / G;
+  const uint32_t t = blockIdx.x * EPB + slot;
+  const uint32_t e = first + t;
+  bool active = t < count;
+  if (active) active = substep < d.body_substeps[d.e_body[e]];
+  double inv[9], T[9], bend = 0.0;
+  double g[8][3];
+  double vg[3] = {0.0, 0.0, 0.0};
+  uint32_t node = 0;
+  if (active) {
+#pragma unroll
+    for (int k = 0; k < 9; ++k) {
+      inv[k] = d.e_shape_inv[at(e, 9, k)];
+      T[k] = d.e_turn[at(e, 9, k)];
+    }
+#pragma unroll
+    for (uint32_t c = 0; c < 8; ++c) grad_of(inv, c, g[c]);
+    bend = d.e_bend[e];
+    if (lane < 8) {
+      const uint32_t c = lane;
+      node = d.e_nodes[at(e, 8, c)];
+      const double *vp = d.v + 3 * size_t(node);
+      const double vc[3] = {vp[0], vp[1], vp[2]};
+#pragma unroll
+      for (int k = 0; k < 3; ++k) vg[k] = d.e_vgrad[at(e, 24, 3 * c + k)];
+      s_vs[slot][c] = vg[0] * vc[0] + vg[1] * vc[1] + vg[2] * vc[2];
+#pragma unroll
+      for (int k = 0; k < 3; ++k) s_b[slot][3 * c + k] = T[k] * vc[0] + T[3 + k] * vc[1] + T[6 + k] * vc[2];
+    }
+  }
+  __syncthreads();
+  if (active) {
+    for (uint32_t r = lane; r < R; r += G) {
+      double acc = 0.0;
+#pragma unroll
+      for (uint32_t c = 0; c < 8; ++c) acc += row_term(r, c, g[c], bend, &s_b[slot][3 * c]);
+      s_res[slot][r] = acc + (d.e_bias[at(e, R, r)] - d.e_soft[at(e, R, r)] * d.e_impulse[at(e, R, r)]);
+    }
+  }
+  __syncthreads();
+  if (active) {
+    const double *Inv = d.inv_table + size_t(d.e_inv_id[e]) * HEX_INVERSE;  // transposed
+    for (uint32_t r = lane; r < R; r += G) {
+      double t2 = 0.0;
+#pragma unroll
+      for (uint32_t k = 0; k < R; ++k) t2 += Inv[R * k + r] * s_res[slot][k];
+      s_sol[slot][r] = t2;
+    }
+  }
+  __syncthreads();
+  double change = 0.0;
+  if (active) {
+    double coupled = 0.0, vspeed = 0.0;
+#pragma unroll
+    for (uint32_t r = 0; r < R; ++r) coupled += d.e_coupling[at(e, R, r)] * s_sol[slot][r];
+#pragma unroll
+    for (uint32_t c = 0; c < 8; ++c) vspeed += s_vs[slot][c];
+    const double gain = d.e_scal[at(e, 3, 0)], rhs = d.e_scal[at(e, 3, 1)], schur = d.e_scal[at(e, 3, 2)];
+    const double vimp = d.e_vimp[e];
+    change = schur * ((rhs - gain * vimp + vspeed) - coupled);
+    for (uint32_t r = lane; r < R; r += G) {
+      const double sol = s_sol[slot][r] - d.e_image[at(e, R, r)] * change;
+      d.e_impulse[at(e, R, r)] += sol;
+      s_res[slot][r] = sol;
+    }
+    if (lane == 0) d.e_vimp[e] = vimp + change;
+  }
+  __syncthreads();
+  if (active && lane < 8) {
+    const uint32_t c = lane;
+    const double w = d.e_inv_mass[at(e, 8, c)];
+    if (w != 0.0) {
+      double gc[3], lf[3];
+      grad_of(inv, c, gc);
+      row_force(s_res[slot], gc, bend, c, lf);
+      double *vp = d.v + 3 * size_t(node);
+#pragma unroll
+      for (int r = 0; r < 3; ++r) {
+        const double f = T[3 * r] * lf[0] + T[3 * r + 1] * lf[1] + T[3 * r + 2] * lf[2];
+        vp[r] -= (f + vg[r] * change) * w;
+      }
+    }
+  }
 }
 
 }  // namespace
@@ -488,6 +551,12 @@ struct LatticeSet::Device {
   std::vector<uint32_t> body_substeps;
   uint32_t inverses = 0;
   hipEvent_t start = nullptr, stop = nullptr;
+  std::vector<hipEvent_t> marks;  // per-launch timing events (KERNEL_TIMING)
+  // the captured launch sequence of one step, valid for these arrays and
+  // this (substeps, exact flags) combination
+  hipGraphExec_t graph = nullptr;
+  uint32_t graph_substeps = 0;
+  bool graph_exact = false, graph_inexact = false;
 
   template <class T>
   T *alloc(size_t n) {
@@ -503,9 +572,11 @@ struct LatticeSet::Device {
     return p;
   }
   ~Device() {
+    if (graph) (void)hipGraphExecDestroy(graph);
     for (void *p : owned) (void)hipFree(p);
     if (start) (void)hipEventDestroy(start);
     if (stop) (void)hipEventDestroy(stop);
+    for (hipEvent_t e : marks) (void)hipEventDestroy(e);
   }
 };
 
@@ -576,9 +647,9 @@ void LatticeSet::download() {
     for (size_t i = 0; i < body.elements.size(); ++i) {
       const size_t s = dev_->slot_of[b][i];
       HexElement &el = body.elements[i];
-      for (uint32_t r = 0; r < R; ++r) el.impulse[r] = imp[r * E + s];
+      for (uint32_t r = 0; r < R; ++r) el.impulse[r] = imp[s * R + r];
       el.vol_impulse = vimp[s];
-      for (int k = 0; k < 4; ++k) el.quat[k] = quat[k * E + s];
+      for (int k = 0; k < 4; ++k) el.quat[k] = quat[s * 4 + k];
     }
   }
 }
@@ -644,29 +715,30 @@ void LatticeSet::upload() {
     const uint32_t n0 = dv->node0[b];
     e_body[s] = b;
     for (int c = 0; c < 8; ++c) {
-      e_nodes[c * Es + s] = n0 + el.nodes[c];
-      e_inv_mass[c * Es + s] = el.inv_mass[c];
+      e_nodes[s * 8 + c] = n0 + el.nodes[c];
+      e_inv_mass[s * 8 + c] = el.inv_mass[c];
     }
-    for (int k = 0; k < 9; ++k) e_shape_inv[k * Es + s] = el.shape_inverse[k];
+    for (int k = 0; k < 9; ++k) e_shape_inv[s * 9 + k] = el.shape_inverse[k];
     e_bend[s] = el.bend;
-    for (int k = 0; k < 12; ++k) e_rest_hg[k * Es + s] = el.rest_hg[k];
+    for (int k = 0; k < 12; ++k) e_rest_hg[s * 12 + k] = el.rest_hg[k];
     for (uint32_t r = 0; r < R; ++r) {
-      e_rate[r * Es + s] = el.rate[r];
-      e_soft[r * Es + s] = el.softness[r];
-      e_cap[r * Es + s] = el.cap[r];
-      e_impulse[r * Es + s] = el.impulse[r];
+      e_rate[s * R + r] = el.rate[r];
+      e_soft[s * R + r] = el.softness[r];
+      e_cap[s * R + r] = el.cap[r];
+      e_impulse[s * R + r] = el.impulse[r];
     }
-    e_vol[s] = el.vol_rate;
-    e_vol[Es + s] = el.vol_softness;
-    e_vol[2 * Es + s] = el.vol_cap;
+    e_vol[s * 3] = el.vol_rate;
+    e_vol[s * 3 + 1] = el.vol_softness;
+    e_vol[s * 3 + 2] = el.vol_cap;
     e_volumetric[s] = el.volumetric ? 1 : 0;
     e_vimp[s] = el.vol_impulse;
-    for (int k = 0; k < 4; ++k) e_quat[k * Es + s] = el.quat[k];
+    for (int k = 0; k < 4; ++k) e_quat[s * 4 + k] = el.quat[k];
     std::string key(reinterpret_cast<const char *>(el.inverse.data()), HEX_INVERSE * sizeof(double));
     auto it = interned.find(key);
     if (it == interned.end()) {
       it = interned.emplace(std::move(key), uint32_t(table.size() / HEX_INVERSE)).first;
-      table.insert(table.end(), el.inverse.begin(), el.inverse.end());
+      for (uint32_t k = 0; k < R; ++k)  // transposed: entry[R k + r] = Inv[r][k]
+        for (uint32_t r = 0; r < R; ++r) table.push_back(el.inverse[R * r + k]);
     }
     e_inv_id[s] = it->second;
   }
@@ -756,34 +828,95 @@ void LatticeSet::step(const talus_mi_lattice_params &p, talus_mi_lattice_report 
   rep.substeps = S;
   rep.elements = d.E;
   const uint32_t node_grid = (d.N + NODE_BLOCK - 1) / NODE_BLOCK;
+  const bool timing = (p.flags & TALUS_MI_KERNEL_TIMING) != 0;
+  const bool direct = timing || (p.flags & TALUS_MI_DIRECT_LAUNCH) != 0;
+  // kernel timing: an event before and after every update and solve launch
+  std::vector<std::pair<uint32_t, int>> timed;  // (first event, kind 0 update / 1 solve)
+  auto mark = [&](int kind, auto &&launch) {
+    if (!timing) {
+      launch();
+      return;
+    }
+    const uint32_t k = uint32_t(2 * timed.size());
+    while (dv.marks.size() < k + 2) {
+      hipEvent_t ev;
+      HIPCHECK(hipEventCreate(&ev));
+      dv.marks.push_back(ev);
+    }
+    HIPCHECK(hipEventRecord(dv.marks[k], stream_));
+    launch();
+    HIPCHECK(hipEventRecord(dv.marks[k + 1], stream_));
+    timed.emplace_back(k, kind);
+  };
+  auto enqueue = [&]() {
+    for (uint32_t s = 0; s < S; ++s) {
+      hipLaunchKernelGGL(k_lat_integrate, dim3(node_grid), dim3(NODE_BLOCK), 0, stream_, d, s, p.gravity[0],
+                         p.gravity[1], p.gravity[2]);
+      if (any_exact)
+        for (uint32_t col = 0; col < 8; ++col) {
+          const uint32_t first = dv.colour_start[col], count = dv.colour_start[col + 1] - first;
+          if (count)
+            mark(0, [&] {
+              hipLaunchKernelGGL(k_lat_update, dim3((count + EPB - 1) / EPB), dim3(EBLOCK), 0, stream_, d,
+                                 first, count, s, 1u);
+            });
+        }
+      for (uint32_t pass = 0; pass < PASSES; ++pass)
+        for (uint32_t col = 0; col < 8; ++col) {
+          const uint32_t first = dv.colour_start[col], count = dv.colour_start[col + 1] - first;
+          if (!count) continue;
+          const dim3 grid((count + EPB - 1) / EPB);
+          if (pass == 0 && any_inexact)
+            mark(0, [&] {
+              hipLaunchKernelGGL(k_lat_update, grid, dim3(EBLOCK), 0, stream_, d, first, count, s, 0u);
+            });
+          mark(1, [&] { hipLaunchKernelGGL(k_lat_solve, grid, dim3(EBLOCK), 0, stream_, d, first, count, s); });
+        }
+      hipLaunchKernelGGL(k_lat_advance, dim3(node_grid), dim3(NODE_BLOCK), 0, stream_, d, s);
+    }
+  };
+  for (uint32_t col = 0; col < 8; ++col)
+    if (dv.colour_start[col + 1] > dv.colour_start[col]) {
+      rep.solve_launches += S * PASSES;
+      rep.update_launches += S * ((any_exact ? 1 : 0) + (any_inexact ? 1 : 0));
+    }
+  rep.inverses = dv.inverses;
   HIPCHECK(hipEventRecord(dv.start, stream_));
-  for (uint32_t s = 0; s < S; ++s) {
-    hipLaunchKernelGGL(k_lat_integrate, dim3(node_grid), dim3(NODE_BLOCK), 0, stream_, d, s, p.gravity[0],
-                       p.gravity[1], p.gravity[2]);
-    if (any_exact)
-      for (uint32_t col = 0; col < 8; ++col) {
-        const uint32_t first = dv.colour_start[col], count = dv.colour_start[col + 1] - first;
-        if (count)
-          hipLaunchKernelGGL(k_lat_update, dim3((count + LAT_BLOCK - 1) / LAT_BLOCK), dim3(LAT_BLOCK), 0,
-                             stream_, d, first, count, s, 1u);
+  if (direct) {
+    enqueue();
+    HIPCHECK(hipGetLastError());
+  } else {
+    if (!dv.graph || dv.graph_substeps != S || dv.graph_exact != any_exact || dv.graph_inexact != any_inexact) {
+      if (dv.graph) {
+        HIPCHECK(hipGraphExecDestroy(dv.graph));
+        dv.graph = nullptr;
       }
-    for (uint32_t pass = 0; pass < PASSES; ++pass)
-      for (uint32_t col = 0; col < 8; ++col) {
-        const uint32_t first = dv.colour_start[col], count = dv.colour_start[col + 1] - first;
-        if (!count) continue;
-        const dim3 grid((count + LAT_BLOCK - 1) / LAT_BLOCK);
-        if (pass == 0 && any_inexact)
-          hipLaunchKernelGGL(k_lat_update, grid, dim3(LAT_BLOCK), 0, stream_, d, first, count, s, 0u);
-        hipLaunchKernelGGL(k_lat_solve, grid, dim3(LAT_BLOCK), 0, stream_, d, first, count, s);
-      }
-    hipLaunchKernelGGL(k_lat_advance, dim3(node_grid), dim3(NODE_BLOCK), 0, stream_, d, s);
+      hipGraph_t g = nullptr;
+      HIPCHECK(hipStreamBeginCapture(stream_, hipStreamCaptureModeThreadLocal));
+      enqueue();
+      const hipError_t launched = hipGetLastError();
+      const hipError_t ended = hipStreamEndCapture(stream_, &g);
+      HIPCHECK(launched);
+      HIPCHECK(ended);
+      const hipError_t made = hipGraphInstantiate(&dv.graph, g, nullptr, nullptr, 0);
+      (void)hipGraphDestroy(g);
+      HIPCHECK(made);
+      dv.graph_substeps = S;
+      dv.graph_exact = any_exact;
+      dv.graph_inexact = any_inexact;
+    }
+    HIPCHECK(hipGraphLaunch(dv.graph, stream_));
   }
-  HIPCHECK(hipGetLastError());
   HIPCHECK(hipEventRecord(dv.stop, stream_));
   HIPCHECK(hipEventSynchronize(dv.stop));
   float ms = 0.0f;
   HIPCHECK(hipEventElapsedTime(&ms, dv.start, dv.stop));
   rep.device_ms = ms;
+  for (const auto &t : timed) {
+    float k = 0.0f;
+    HIPCHECK(hipEventElapsedTime(&k, dv.marks[t.first], dv.marks[t.first + 1]));
+    (t.second ? rep.solve_ms : rep.update_ms) += k;
+  }
 }
 
 }  // namespace tmi
