@@ -227,10 +227,10 @@ def slab_run(steps):
 
 
 # Algorithmic bytes of one element solve and one element update of the
-# strainstep hex lattice (FP64, SoA planes, lattice.hip): everything the
-# kernel must read and write once for that element.  The 18x18 inverse (2592
-# bytes) counts in full when every element has its own, and in proportion to
-# the distinct inverses when equal ones are shared.
+# strainstep hex lattice (FP64, element-major rows, lattice.hip): everything
+# the kernel must read and write once for that element.  The 18x18 inverse
+# (its upper triangle, 171 doubles) counts in full when every element has its
+# own, and in proportion to the distinct inverses when equal ones are shared.
 LAT_SOLVE_BYTES = (72 + 72 + 32 + 8 + 4 + 4 + 64     # shape^-1, turn, nodes, bend, body, inverse id, 1/m
                    + 8 * 3 * 8 * 2                   # corner velocities read and written
                    + 192 + 144 + 144 + 144 * 2       # volume gradient, bias, softness, impulse r/w
@@ -239,7 +239,7 @@ LAT_UPDATE_BYTES = (8 * 3 * 8 + 72 + 32 + 4 * 8 * 2  # corner positions, shape^-
                     + 96 + 144 + 144 + 64 + 24 + 1 + 4 + 4   # rest hourglass, rate, cap, 1/m, volume, flags
                     + 144 + 8 + 8 * 3 * 8 * 2        # impulses, corner velocities r/w (warm start)
                     + 72 + 144 + 192 + 144 + 144 + 24)  # turn, bias, volume gradient, coupling, image, scalars
-LAT_INVERSE_BYTES = 18 * 18 * 8
+LAT_INVERSE_BYTES = 18 * 19 // 2 * 8
 
 
 def lattice_run(steps, warmup, cpu_seconds, direct=False):

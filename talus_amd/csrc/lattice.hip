@@ -15,13 +15,15 @@
 // xyz-interleaved.  Elements are stored in slot order: colour 0..7, then
 // body, then cell, so one colour is one contiguous slot range and one
 // launch.  Per-element data are element-major, field[slot * width + k], and
-// G = 16 lanes work on one element (lanes 0..7 its corners, lane l its rows l
-// and l + 16), so the lanes of an element touch consecutive words.  The 18x18
+// 18 lanes work on one element (lane r its row r, lanes 0..7 also its
+// corners), so the lanes of an element touch consecutive words.  The 18x18
 // row-block inverses are interned: equal inverses (same rest shape, corner
-// masses, material and substep) share one 2592-byte table entry, indexed per
-// element and stored transposed, entry[18 k + r] = Inv[r][k], so the lanes
-// computing rows r read row-block column k as one contiguous run.  Gradients
-// are not stored:
+// masses, material and substep) share one table entry, indexed per element.
+// An entry is the upper triangle of the inverse, row by row (171 doubles,
+// 1368 bytes instead of 2592): the row block is symmetric, so entry (k, r)
+// with k < r stands for (r, k), which the Gauss-Jordan elimination gives
+// equal up to rounding.  An element's entry is staged in LDS by its 18 lanes
+// with coalesced loads before the rows use it.  Gradients are not stored:
 // corner c's gradient is (shape^-T s_c) / 8 with s_c = +-1, recomputed
 // exactly from the 9 entries of shape^-1.
 //
@@ -49,13 +51,44 @@ namespace tmi {
 namespace {
 
 constexpr int NODE_BLOCK = 256;
-constexpr uint32_t G = 16;              // lanes per element
-constexpr uint32_t EPB = 16;            // elements per block
-constexpr uint32_t EBLOCK = G * EPB;    // threads per element block
+constexpr uint32_t EBLOCK = 64;         // one wave per element block
+constexpr uint32_t EPW = 3;             // elements per wave: 18 lanes each
 constexpr uint32_t PASSES = 4;
 constexpr uint32_t ROTATION_ITERATIONS = 16;
 constexpr double MAX_SPEED = 400.0;
 constexpr uint32_t R = HEX_ROWS;
+constexpr uint32_t PACKED = R * (R + 1) / 2;  // upper triangle of an 18x18 inverse
+
+// position of entry (i, j), i <= j, in the row-major upper triangle
+__device__ __forceinline__ uint32_t packed_at(uint32_t i, uint32_t j) {
+  return i * R - (i * (i - 1)) / 2 + (j - i);
+}
+
+// an element's packed inverse goes to LDS in two steps, so its loads are
+// issued with the element's other loads: lane r loads entries r, r + 18, ...
+constexpr uint32_t STAGE = (PACKED + R - 1) / R;
+__device__ __forceinline__ void load_inverse(const double *packed, uint32_t r, double *reg) {
+#pragma unroll
+  for (uint32_t k = 0; k < STAGE; ++k) {
+    const uint32_t i = r + k * R;
+    reg[k] = i < PACKED ? packed[i] : 0.0;
+  }
+}
+__device__ __forceinline__ void store_inverse(const double *reg, uint32_t r, double *dst) {
+#pragma unroll
+  for (uint32_t k = 0; k < STAGE; ++k) {
+    const uint32_t i = r + k * R;
+    if (i < PACKED) dst[i] = reg[k];
+  }
+}
+
+// row r of Inv * x with Inv symmetric, from its upper triangle in LDS
+__device__ __forceinline__ double inverse_row(const double *tri, uint32_t r, const double *x) {
+  double t = 0.0;
+#pragma unroll
+  for (uint32_t k = 0; k < R; ++k) t += tri[k < r ? packed_at(k, r) : packed_at(r, k)] * x[k];
+  return t;
+}
 
 struct LatDev {
   uint32_t N, E;
@@ -67,6 +100,7 @@ struct LatDev {
   const uint8_t *body_exact;
   // elements, element-major [slot * width + k]; widths below
   const uint32_t *e_body;      // 1 plane
+  const uint32_t *e_sub;       // the body's substeps, bit 31 = the body is exact
   const uint32_t *e_nodes;     // 8
   const uint32_t *e_inv_id;    // 1
   const double *e_shape_inv;   // 9
@@ -87,7 +121,8 @@ struct LatDev {
   double *e_vgrad;             // 24
   double *e_coupling;          // 18
   double *e_image;             // 18
-  double *e_scal;              // 3: gain, rhs, schur
+  double *e_scal;              // 3: gain, rhs, 1 / schur (vweight + gain before the first pass)
+  double *e_dv;                // 24: warm-start corner velocity changes
 };
 
 __device__ __forceinline__ double corner_sign(uint32_t c, int axis) {
@@ -210,11 +245,14 @@ __global__ void __launch_bounds__(NODE_BLOCK) k_lat_integrate(LatDev d, uint32_t
   }
 }
 
-__global__ void __launch_bounds__(NODE_BLOCK) k_lat_advance(LatDev d, uint32_t substep) {
+// advanceNodes of substep `substep`, then integrateNodes of the next one
+__global__ void __launch_bounds__(NODE_BLOCK) k_lat_advance(LatDev d, uint32_t substep, double gx, double gy,
+                                                           double gz) {
   const uint32_t i = blockIdx.x * NODE_BLOCK + threadIdx.x;
   if (i >= d.N) return;
   const uint32_t b = d.node_body[i];
-  if (substep >= d.body_substeps[b]) return;
+  const uint32_t subs = d.body_substeps[b];
+  if (substep >= subs) return;
   const double h = d.body_h[b];
   double *vp = d.v + 3 * size_t(i);
   double *xp = d.x + 3 * size_t(i);
@@ -225,13 +263,22 @@ __global__ void __launch_bounds__(NODE_BLOCK) k_lat_advance(LatDev d, uint32_t s
     v0 *= s;
     v1 *= s;
     v2 *= s;
-    vp[0] = v0;
-    vp[1] = v1;
-    vp[2] = v2;
   }
   xp[0] += h * v0;
   xp[1] += h * v1;
   xp[2] += h * v2;
+  if (substep + 1 < subs) {
+    if (d.node_pinned[i]) {
+      v0 = v1 = v2 = 0.0;
+    } else {
+      v0 = v0 + h * gx;
+      v1 = v1 + h * gy;
+      v2 = v2 + h * gz;
+    }
+  }
+  vp[0] = v0;
+  vp[1] = v1;
+  vp[2] = v2;
 }
 
 // Element-major planes: row-indexed data of element e sit at [e * width + k],
@@ -254,232 +301,222 @@ __device__ __forceinline__ double row_term(uint32_t r, uint32_t c, const double 
   }
 }
 
-// updateElement at the current positions, then applyElement with the
-// element's accumulated impulses (the warm start) for one colour's elements
-// of the bodies whose exact flag equals `exact`.  G lanes per element:
-// lanes 0..7 own the corners, lane l the rows l and l + G; sums over corners
-// and rows run in the reference's order.
-__global__ void __launch_bounds__(EBLOCK) k_lat_update(LatDev d, uint32_t first, uint32_t count,
-                                                       uint32_t substep, uint32_t exact) {
-  __shared__ double s_x[EPB][24];   // corner positions
-  __shared__ double s_l[EPB][24];   // per corner (T^T vg_c) w_c
-  __shared__ double s_w[EPB][8];    // per corner w_c |vg_c|^2
-  __shared__ double s_cp[EPB][R];   // coupling rows
-  __shared__ double s_img[EPB][R];  // image rows
-  __shared__ double s_imp[EPB][R];  // accumulated impulses
-  const uint32_t lane = threadIdx.x % G, slot = threadIdx.x / G;
-  const uint32_t t = blockIdx.x * EPB + slot;
+// updateElement at the current positions, one lane per element, for the
+// elements [first, first + count) of the bodies whose exact flag equals
+// `exact` (an exact body updates every element before the passes, a too-stiff
+// body each colour right before its first solve).  Writes the rotation, the
+// rate-scaled capped strain rows, the volume gradients, the coupling rows,
+// gain, rhs and vweight + gain (the Schur complement's coupling term is
+// folded in by the first pass's solve, which holds the row-block inverse),
+// and the warm-start corner velocity changes applyElement makes with the
+// accumulated impulses (k_lat_warm applies them colour by colour).
+__global__ void __launch_bounds__(EBLOCK) k_lat_frame(LatDev d, uint32_t first, uint32_t count,
+                                                      uint32_t substep, uint32_t exact) {
+  const uint32_t t = blockIdx.x * EBLOCK + threadIdx.x;
+  if (t >= count) return;
   const uint32_t e = first + t;
-  bool active = t < count;
-  if (active) {
-    const uint32_t b = d.e_body[e];
-    active = substep < d.body_substeps[b] && uint32_t(d.body_exact[b]) == exact;
-  }
+  const uint32_t sub = d.e_sub[e];
+  if (substep >= (sub & 0x7fffffffu) || (sub >> 31) != exact) return;
   double inv[9];
-  double g[8][3];
-  uint32_t node = 0;
-  if (active) {
 #pragma unroll
-    for (int k = 0; k < 9; ++k) inv[k] = d.e_shape_inv[at(e, 9, k)];
+  for (int k = 0; k < 9; ++k) inv[k] = d.e_shape_inv[at(e, 9, k)];
+  const double bend = d.e_bend[e];
+  double F[9] = {}, hg[12] = {};
 #pragma unroll
-    for (uint32_t c = 0; c < 8; ++c) grad_of(inv, c, g[c]);
-    if (lane < 8) {
-      node = d.e_nodes[at(e, 8, lane)];
-      const double *xp = d.x + 3 * size_t(node);
-      s_x[slot][3 * lane] = xp[0];
-      s_x[slot][3 * lane + 1] = xp[1];
-      s_x[slot][3 * lane + 2] = xp[2];
+  for (uint32_t c = 0; c < 8; ++c) {
+    const uint32_t node = d.e_nodes[at(e, 8, c)];
+    const double *xp = d.x + 3 * size_t(node);
+    const double xc[3] = {xp[0], xp[1], xp[2]};
+    double g[3];
+    grad_of(inv, c, g);
+#pragma unroll
+    for (int i = 0; i < 3; ++i)
+#pragma unroll
+      for (int j = 0; j < 3; ++j) F[3 * i + j] += xc[i] * g[j];
+#pragma unroll
+    for (uint32_t m = 0; m < 4; ++m) {
+      const double s = hg_sign(m, c);
+#pragma unroll
+      for (int i = 0; i < 3; ++i) hg[3 * m + i] += s * xc[i];
     }
-    for (uint32_t r = lane; r < R; r += G) s_imp[slot][r] = d.e_impulse[at(e, R, r)];
   }
-  __syncthreads();
-  double T[9], cof[9], det = 0.0, bend = 0.0;
-  bool inverted = false;
-  if (active) {
-    double F[9] = {}, hg[12] = {};
+  double q[4], T[9];
 #pragma unroll
-    for (uint32_t c = 0; c < 8; ++c) {
-      const double xc[3] = {s_x[slot][3 * c], s_x[slot][3 * c + 1], s_x[slot][3 * c + 2]};
+  for (int k = 0; k < 4; ++k) q[k] = d.e_quat[at(e, 4, k)];
+  rotation_of(F, q);
+  quat_matrix(q, T);
 #pragma unroll
-      for (int r = 0; r < 3; ++r)
+  for (int k = 0; k < 4; ++k) d.e_quat[at(e, 4, k)] = q[k];
 #pragma unroll
-        for (int k = 0; k < 3; ++k) F[3 * r + k] += xc[r] * g[c][k];
-#pragma unroll
-      for (uint32_t m = 0; m < 4; ++m) {
-        const double s = hg_sign(m, c);
-#pragma unroll
-        for (int r = 0; r < 3; ++r) hg[3 * m + r] += s * xc[r];
-      }
-    }
-    double q[4];
-#pragma unroll
-    for (int k = 0; k < 4; ++k) q[k] = d.e_quat[at(e, 4, k)];
-    rotation_of(F, q);  // every lane of the element runs the same iteration
-    quat_matrix(q, T);
-    if (lane == 0) {
-#pragma unroll
-      for (int k = 0; k < 4; ++k) d.e_quat[at(e, 4, k)] = q[k];
-#pragma unroll
-      for (int k = 0; k < 9; ++k) d.e_turn[at(e, 9, k)] = T[k];
-    }
-    det = F[0] * (F[4] * F[8] - F[5] * F[7]) - F[1] * (F[3] * F[8] - F[5] * F[6]) +
-          F[2] * (F[3] * F[7] - F[4] * F[6]);
-    inverted = det < 0.0;
-    bend = d.e_bend[e];
-    // strain rows of T^T F and the hourglass offsets, capped, times the rate
-    double L[9], off[12];
+  for (int k = 0; k < 9; ++k) d.e_turn[at(e, 9, k)] = T[k];
+  const double det = F[0] * (F[4] * F[8] - F[5] * F[7]) - F[1] * (F[3] * F[8] - F[5] * F[6]) +
+                     F[2] * (F[3] * F[7] - F[4] * F[6]);
+  const bool inverted = det < 0.0;
+  {
+    double L[9];
 #pragma unroll
     for (int i = 0; i < 3; ++i)
 #pragma unroll
       for (int j = 0; j < 3; ++j) L[3 * i + j] = T[i] * F[j] + T[3 + i] * F[3 + j] + T[6 + i] * F[6 + j];
+    double raw[R];
+    raw[0] = L[0] - 1.0;
+    raw[1] = L[4] - 1.0;
+    raw[2] = L[8] - 1.0;
+    raw[3] = 0.5 * (L[1] + L[3]);
+    raw[4] = 0.5 * (L[5] + L[7]);
+    raw[5] = 0.5 * (L[6] + L[2]);
 #pragma unroll
     for (uint32_t m = 0; m < 4; ++m)
 #pragma unroll
-      for (int k = 0; k < 3; ++k)
-        off[3 * m + k] = T[k] * hg[3 * m] + T[3 + k] * hg[3 * m + 1] + T[6 + k] * hg[3 * m + 2];
-    for (uint32_t r = lane; r < R; r += G) {
-      double raw = 0.0;
-      switch (r) {
-        case 0: raw = L[0] - 1.0; break;
-        case 1: raw = L[4] - 1.0; break;
-        case 2: raw = L[8] - 1.0; break;
-        case 3: raw = 0.5 * (L[1] + L[3]); break;
-        case 4: raw = 0.5 * (L[5] + L[7]); break;
-        case 5: raw = 0.5 * (L[6] + L[2]); break;
-        default: {
-          double o = 0.0;
-#pragma unroll
-          for (uint32_t q2 = 0; q2 < 12; ++q2)
-            if (q2 + 6 == r) o = off[q2];
-          raw = (o - d.e_rest_hg[at(e, 12, r - 6)]) * bend;
-        }
+      for (int k = 0; k < 3; ++k) {
+        const double b = T[k] * hg[3 * m] + T[3 + k] * hg[3 * m + 1] + T[6 + k] * hg[3 * m + 2];
+        raw[6 + 3 * m + k] = (b - d.e_rest_hg[at(e, 12, 3 * m + k)]) * bend;
       }
+#pragma unroll
+    for (uint32_t r = 0; r < R; ++r) {
       const double cap0 = d.e_cap[at(e, R, r)];
       const double cap = inverted ? cap0 : (cap0 < 1.0 ? 1.0 : cap0);
-      d.e_bias[at(e, R, r)] = d.e_rate[at(e, R, r)] * clampd(raw, -cap, cap);
+      d.e_bias[at(e, R, r)] = d.e_rate[at(e, R, r)] * clampd(raw[r], -cap, cap);
     }
-    // cofactor columns F1 x F2, F2 x F0, F0 x F1
-    {
-      const double f0[3] = {F[0], F[3], F[6]}, f1[3] = {F[1], F[4], F[7]}, f2[3] = {F[2], F[5], F[8]};
-      double c0[3], c1[3], c2[3];
-      cross3(f1, f2, c0);
-      cross3(f2, f0, c1);
-      cross3(f0, f1, c2);
+  }
+  // cofactor columns F1 x F2, F2 x F0, F0 x F1
+  double cof[9];
+  {
+    const double f0[3] = {F[0], F[3], F[6]}, f1[3] = {F[1], F[4], F[7]}, f2[3] = {F[2], F[5], F[8]};
+    double c0[3], c1[3], c2[3];
+    cross3(f1, f2, c0);
+    cross3(f2, f0, c1);
+    cross3(f0, f1, c2);
 #pragma unroll
-      for (int r = 0; r < 3; ++r) {
-        cof[3 * r] = c0[r];
-        cof[3 * r + 1] = c1[r];
-        cof[3 * r + 2] = c2[r];
+    for (int i = 0; i < 3; ++i) {
+      cof[3 * i] = c0[i];
+      cof[3 * i + 1] = c1[i];
+      cof[3 * i + 2] = c2[i];
+    }
+  }
+  double imp[R];
+#pragma unroll
+  for (uint32_t r = 0; r < R; ++r) imp[r] = d.e_impulse[at(e, R, r)];
+  const double vimp = d.e_vimp[e];
+  double coupling[R] = {};
+  double vweight = 0.0;
+#pragma unroll
+  for (uint32_t c = 0; c < 8; ++c) {
+    double g[3], vg[3];
+    grad_of(inv, c, g);
+#pragma unroll
+    for (int i = 0; i < 3; ++i) vg[i] = cof[3 * i] * g[0] + cof[3 * i + 1] * g[1] + cof[3 * i + 2] * g[2];
+#pragma unroll
+    for (int i = 0; i < 3; ++i) d.e_vgrad[at(e, 24, 3 * c + i)] = vg[i];
+    const double w = d.e_inv_mass[at(e, 8, c)];
+    vweight += w * (vg[0] * vg[0] + vg[1] * vg[1] + vg[2] * vg[2]);
+    double l[3];
+#pragma unroll
+    for (int k = 0; k < 3; ++k) l[k] = (T[k] * vg[0] + T[3 + k] * vg[1] + T[6 + k] * vg[2]) * w;
+#pragma unroll
+    for (uint32_t r = 0; r < R; ++r) coupling[r] += row_term(r, c, g, bend, l);
+    // warm start: this corner's velocity change from the accumulated impulses
+    double dv[3] = {0.0, 0.0, 0.0};
+    if (w != 0.0) {
+      double lf[3];
+      row_force(imp, g, bend, c, lf);
+#pragma unroll
+      for (int i = 0; i < 3; ++i) {
+        const double f = T[3 * i] * lf[0] + T[3 * i + 1] * lf[1] + T[3 * i + 2] * lf[2];
+        dv[i] = (f + vg[i] * vimp) * w;
       }
     }
-    if (lane < 8) {
-      const uint32_t c = lane;
-      double gc[3], vg[3];
-      grad_of(inv, c, gc);
 #pragma unroll
-      for (int r = 0; r < 3; ++r) vg[r] = cof[3 * r] * gc[0] + cof[3 * r + 1] * gc[1] + cof[3 * r + 2] * gc[2];
-#pragma unroll
-      for (int r = 0; r < 3; ++r) d.e_vgrad[at(e, 24, 3 * c + r)] = vg[r];
-      const double w = d.e_inv_mass[at(e, 8, c)];
-      s_w[slot][c] = w * (vg[0] * vg[0] + vg[1] * vg[1] + vg[2] * vg[2]);
-#pragma unroll
-      for (int k = 0; k < 3; ++k) s_l[slot][3 * c + k] = (T[k] * vg[0] + T[3 + k] * vg[1] + T[6 + k] * vg[2]) * w;
-    }
+    for (int i = 0; i < 3; ++i) d.e_dv[at(e, 24, 3 * c + i)] = dv[i];
   }
-  __syncthreads();
-  if (active) {
-    for (uint32_t r = lane; r < R; r += G) {
-      double acc = 0.0;
 #pragma unroll
-      for (uint32_t c = 0; c < 8; ++c) acc += row_term(r, c, g[c], bend, &s_l[slot][3 * c]);
-      s_cp[slot][r] = acc;
-      d.e_coupling[at(e, R, r)] = acc;
-    }
-  }
-  __syncthreads();
-  if (active) {
-    const double *Inv = d.inv_table + size_t(d.e_inv_id[e]) * HEX_INVERSE;  // transposed
-    for (uint32_t r = lane; r < R; r += G) {
-      double t2 = 0.0;
+  for (uint32_t r = 0; r < R; ++r) d.e_coupling[at(e, R, r)] = coupling[r];
+  const double vrate = d.e_vol[at(e, 3, 0)], vsoft = d.e_vol[at(e, 3, 1)], vcap0 = d.e_vol[at(e, 3, 2)];
+  const double vcap = inverted ? vcap0 : (vcap0 < 1.0 ? 1.0 : vcap0);
+  const double volume = clampd(det - 1.0, -vcap, vcap);
+  const double gain = vweight * vsoft;
+  d.e_scal[at(e, 3, 0)] = gain;
+  d.e_scal[at(e, 3, 1)] = volume * vrate;
+  d.e_scal[at(e, 3, 2)] = vweight + gain;  // the first pass turns it into 1 / schur
+}
+
+// applyElement's warm start for one colour's elements of the bodies whose
+// exact flag equals `exact`: the corner velocity changes k_lat_frame made,
+// one lane per corner, eight elements per wave.
+__global__ void __launch_bounds__(EBLOCK) k_lat_warm(LatDev d, uint32_t first, uint32_t count, uint32_t substep,
+                                                     uint32_t exact) {
+  const uint32_t slot = threadIdx.x / 8, c = threadIdx.x % 8;
+  const uint32_t t = blockIdx.x * (EBLOCK / 8) + slot;
+  if (t >= count) return;
+  const uint32_t e = first + t;
+  const uint32_t sub = d.e_sub[e];
+  if (substep >= (sub & 0x7fffffffu) || (sub >> 31) != exact) return;
+  if (d.e_inv_mass[at(e, 8, c)] == 0.0) return;
+  double *vp = d.v + 3 * size_t(d.e_nodes[at(e, 8, c)]);
 #pragma unroll
-      for (uint32_t k = 0; k < R; ++k) t2 += Inv[R * k + r] * s_cp[slot][k];
-      s_img[slot][r] = t2;
-      d.e_image[at(e, R, r)] = t2;
-    }
-  }
-  __syncthreads();
-  if (active) {
-    double coupled = 0.0, vweight = 0.0;
-#pragma unroll
-    for (uint32_t r = 0; r < R; ++r) coupled += s_cp[slot][r] * s_img[slot][r];
-#pragma unroll
-    for (uint32_t c = 0; c < 8; ++c) vweight += s_w[slot][c];
-    const double vrate = d.e_vol[at(e, 3, 0)], vsoft = d.e_vol[at(e, 3, 1)], vcap0 = d.e_vol[at(e, 3, 2)];
-    const double vcap = inverted ? vcap0 : (vcap0 < 1.0 ? 1.0 : vcap0);
-    const double volume = clampd(det - 1.0, -vcap, vcap);
-    const double gain = vweight * vsoft;
-    const double schur = vweight + gain - coupled;
-    if (lane == 0) {
-      d.e_scal[at(e, 3, 0)] = gain;
-      d.e_scal[at(e, 3, 1)] = volume * vrate;
-      d.e_scal[at(e, 3, 2)] = d.e_volumetric[e] && schur > 0.0 ? 1.0 / schur : 0.0;
-    }
-    // warm start: applyElement with the accumulated impulses
-    if (lane < 8) {
-      const uint32_t c = lane;
-      const double w = d.e_inv_mass[at(e, 8, c)];
-      if (w != 0.0) {
-        const double vimp = d.e_vimp[e];
-        double gc[3], lf[3], vg[3];
-        grad_of(inv, c, gc);
-        row_force(s_imp[slot], gc, bend, c, lf);
-#pragma unroll
-        for (int r = 0; r < 3; ++r) vg[r] = cof[3 * r] * gc[0] + cof[3 * r + 1] * gc[1] + cof[3 * r + 2] * gc[2];
-        double *vp = d.v + 3 * size_t(node);
-#pragma unroll
-        for (int r = 0; r < 3; ++r) {
-          const double f = T[3 * r] * lf[0] + T[3 * r + 1] * lf[1] + T[3 * r + 2] * lf[2];
-          vp[r] -= (f + vg[r] * vimp) * w;
-        }
-      }
-    }
-  }
+  for (int i = 0; i < 3; ++i) vp[i] -= d.e_dv[at(e, 24, 3 * c + i)];
 }
 
 // solveElement for one colour's elements: one block solve of the 18 rows with
 // the volume row folded in through its Schur complement, applied to the
-// corner velocities.  G lanes per element as in k_lat_update.
+// corner velocities.  One lane per row, 18 lanes per element, three
+// elements per wave; lanes 0..7 of an element also own its corners.  The
+// first pass also finishes updateElement: image = Inv coupling and 1 / schur.
 __global__ void __launch_bounds__(EBLOCK) k_lat_solve(LatDev d, uint32_t first, uint32_t count,
-                                                      uint32_t substep) {
-  __shared__ double s_b[EPB][24];   // per corner T^T v_c
-  __shared__ double s_vs[EPB][8];   // per corner vg_c . v_c
-  __shared__ double s_res[EPB][R];  // residual rows, then the final solution
-  __shared__ double s_sol[EPB][R];  // Inv * residual
-  const uint32_t lane = threadIdx.x % G, slot = threadIdx.x / G;
-  const uint32_t t = blockIdx.x * EPB + slot;
+                                                      uint32_t substep, uint32_t first_pass) {
+  __shared__ double s_b[EPW + 1][24];    // per corner T^T v_c
+  __shared__ double s_vs[EPW + 1][8];    // per corner vg_c . v_c
+  __shared__ double s_res[EPW + 1][R];   // residual rows, then the final solution
+  __shared__ double s_prod[EPW + 1][R];  // coupling_r * (Inv residual)_r
+  __shared__ double s_inv[EPW + 1][PACKED];  // the element's packed inverse
+  __shared__ double s_cp[EPW + 1][R];    // coupling rows (first pass)
+  __shared__ double s_pi[EPW + 1][R];    // coupling_r * image_r (first pass)
+  const uint32_t slot = threadIdx.x / R, r = threadIdx.x - slot * R;
+  const uint32_t t = blockIdx.x * EPW + slot;
   const uint32_t e = first + t;
-  bool active = t < count;
-  if (active) active = substep < d.body_substeps[d.e_body[e]];
-  double inv[9], T[9], bend = 0.0;
-  double g[8][3];
-  double vg[3] = {0.0, 0.0, 0.0};
+  // loads are issued for every element in range; the work and the stores
+  // only for elements whose body takes this substep
+  const bool inrange = slot < EPW && t < count;
+  const bool active = inrange && substep < (d.e_sub[e] & 0x7fffffffu);
+  const bool corner = r < 8;
+  const uint32_t c = r;
+  double inv[9], T[9];
+  double bend = 0.0, bias = 0.0, soft = 0.0, imp = 0.0, coupling = 0.0, image = 0.0;
+  double gain = 0.0, rhs = 0.0, schur = 0.0, vimp = 0.0, w = 0.0;
+  double vg[3] = {0.0, 0.0, 0.0}, vc[3] = {0.0, 0.0, 0.0};
   uint32_t node = 0;
-  if (active) {
+  if (inrange) {
+    // every global load first (the corner gather after its node index), then
+    // the LDS writes, so the element waits for memory once
+    if (corner) node = d.e_nodes[at(e, 8, c)];
+    double tri[STAGE];
+    load_inverse(d.inv_table + size_t(d.e_inv_id[e]) * PACKED, r, tri);
 #pragma unroll
-    for (int k = 0; k < 9; ++k) {
-      inv[k] = d.e_shape_inv[at(e, 9, k)];
-      T[k] = d.e_turn[at(e, 9, k)];
-    }
-#pragma unroll
-    for (uint32_t c = 0; c < 8; ++c) grad_of(inv, c, g[c]);
+    for (int k = 0; k < 9; ++k) inv[k] = d.e_shape_inv[at(e, 9, k)];
     bend = d.e_bend[e];
-    if (lane < 8) {
-      const uint32_t c = lane;
-      node = d.e_nodes[at(e, 8, c)];
-      const double *vp = d.v + 3 * size_t(node);
-      const double vc[3] = {vp[0], vp[1], vp[2]};
+    gain = d.e_scal[at(e, 3, 0)];
+    rhs = d.e_scal[at(e, 3, 1)];
+    schur = d.e_scal[at(e, 3, 2)];
+    vimp = d.e_vimp[e];
+    bias = d.e_bias[at(e, R, r)];
+    soft = d.e_soft[at(e, R, r)];
+    imp = d.e_impulse[at(e, R, r)];
+    coupling = d.e_coupling[at(e, R, r)];
+    if (!first_pass) image = d.e_image[at(e, R, r)];
+    if (corner) {
+#pragma unroll
+      for (int k = 0; k < 9; ++k) T[k] = d.e_turn[at(e, 9, k)];
+      w = d.e_inv_mass[at(e, 8, c)];
 #pragma unroll
       for (int k = 0; k < 3; ++k) vg[k] = d.e_vgrad[at(e, 24, 3 * c + k)];
+      const double *vp = d.v + 3 * size_t(node);
+      vc[0] = vp[0];
+      vc[1] = vp[1];
+      vc[2] = vp[2];
+    }
+    store_inverse(tri, r, s_inv[slot]);
+    s_cp[slot][r] = coupling;
+    if (corner) {
       s_vs[slot][c] = vg[0] * vc[0] + vg[1] * vc[1] + vg[2] * vc[2];
 #pragma unroll
       for (int k = 0; k < 3; ++k) s_b[slot][3 * c + k] = T[k] * vc[0] + T[3 + k] * vc[1] + T[6 + k] * vc[2];
@@ -487,21 +524,34 @@ __global__ void __launch_bounds__(EBLOCK) k_lat_solve(LatDev d, uint32_t first, 
   }
   __syncthreads();
   if (active) {
-    for (uint32_t r = lane; r < R; r += G) {
-      double acc = 0.0;
+    double acc = 0.0;
 #pragma unroll
-      for (uint32_t c = 0; c < 8; ++c) acc += row_term(r, c, g[c], bend, &s_b[slot][3 * c]);
-      s_res[slot][r] = acc + (d.e_bias[at(e, R, r)] - d.e_soft[at(e, R, r)] * d.e_impulse[at(e, R, r)]);
+    for (uint32_t k = 0; k < 8; ++k) {
+      double gk[3];
+      grad_of(inv, k, gk);
+      acc += row_term(r, k, gk, bend, &s_b[slot][3 * k]);
+    }
+    s_res[slot][r] = acc + (bias - soft * imp);
+    if (first_pass) {
+      // the rest of updateElement: image = Inv coupling and the Schur
+      // complement of the volume row
+      image = inverse_row(s_inv[slot], r, s_cp[slot]);
+      s_pi[slot][r] = coupling * image;
     }
   }
   __syncthreads();
+  double sol = 0.0;
   if (active) {
-    const double *Inv = d.inv_table + size_t(d.e_inv_id[e]) * HEX_INVERSE;  // transposed
-    for (uint32_t r = lane; r < R; r += G) {
-      double t2 = 0.0;
+    sol = inverse_row(s_inv[slot], r, s_res[slot]);
+    s_prod[slot][r] = coupling * sol;
+    if (first_pass) {
+      double coupled = 0.0;
 #pragma unroll
-      for (uint32_t k = 0; k < R; ++k) t2 += Inv[R * k + r] * s_res[slot][k];
-      s_sol[slot][r] = t2;
+      for (uint32_t k = 0; k < R; ++k) coupled += s_pi[slot][k];
+      const double left = schur - coupled;  // schur holds vweight + gain here
+      schur = d.e_volumetric[e] && left > 0.0 ? 1.0 / left : 0.0;
+      d.e_image[at(e, R, r)] = image;
+      if (r == 0) d.e_scal[at(e, 3, 2)] = schur;
     }
   }
   __syncthreads();
@@ -509,33 +559,25 @@ __global__ void __launch_bounds__(EBLOCK) k_lat_solve(LatDev d, uint32_t first, 
   if (active) {
     double coupled = 0.0, vspeed = 0.0;
 #pragma unroll
-    for (uint32_t r = 0; r < R; ++r) coupled += d.e_coupling[at(e, R, r)] * s_sol[slot][r];
+    for (uint32_t k = 0; k < R; ++k) coupled += s_prod[slot][k];
 #pragma unroll
-    for (uint32_t c = 0; c < 8; ++c) vspeed += s_vs[slot][c];
-    const double gain = d.e_scal[at(e, 3, 0)], rhs = d.e_scal[at(e, 3, 1)], schur = d.e_scal[at(e, 3, 2)];
-    const double vimp = d.e_vimp[e];
+    for (uint32_t k = 0; k < 8; ++k) vspeed += s_vs[slot][k];
     change = schur * ((rhs - gain * vimp + vspeed) - coupled);
-    for (uint32_t r = lane; r < R; r += G) {
-      const double sol = s_sol[slot][r] - d.e_image[at(e, R, r)] * change;
-      d.e_impulse[at(e, R, r)] += sol;
-      s_res[slot][r] = sol;
-    }
-    if (lane == 0) d.e_vimp[e] = vimp + change;
+    sol -= image * change;
+    d.e_impulse[at(e, R, r)] = imp + sol;
+    s_res[slot][r] = sol;
+    if (r == 0) d.e_vimp[e] = vimp + change;
   }
   __syncthreads();
-  if (active && lane < 8) {
-    const uint32_t c = lane;
-    const double w = d.e_inv_mass[at(e, 8, c)];
-    if (w != 0.0) {
-      double gc[3], lf[3];
-      grad_of(inv, c, gc);
-      row_force(s_res[slot], gc, bend, c, lf);
-      double *vp = d.v + 3 * size_t(node);
+  if (active && corner && w != 0.0) {
+    double gc[3], lf[3];
+    grad_of(inv, c, gc);
+    row_force(s_res[slot], gc, bend, c, lf);
+    double *vp = d.v + 3 * size_t(node);  // only this element of its colour touches the node
 #pragma unroll
-      for (int r = 0; r < 3; ++r) {
-        const double f = T[3 * r] * lf[0] + T[3 * r + 1] * lf[1] + T[3 * r + 2] * lf[2];
-        vp[r] -= (f + vg[r] * change) * w;
-      }
+    for (int i = 0; i < 3; ++i) {
+      const double f = T[3 * i] * lf[0] + T[3 * i + 1] * lf[1] + T[3 * i + 2] * lf[2];
+      vp[i] = vc[i] - (f + vg[i] * change) * w;
     }
   }
 }
@@ -549,6 +591,7 @@ struct LatticeSet::Device {
   std::vector<uint32_t> node0;               // first node of each body
   std::vector<std::vector<uint32_t>> slot_of;  // per body, per element: slot
   std::vector<uint32_t> body_substeps;
+  std::vector<uint32_t> slot_body;             // per slot: its body
   uint32_t inverses = 0;
   hipEvent_t start = nullptr, stop = nullptr;
   std::vector<hipEvent_t> marks;  // per-launch timing events (KERNEL_TIMING)
@@ -736,13 +779,13 @@ void LatticeSet::upload() {
     std::string key(reinterpret_cast<const char *>(el.inverse.data()), HEX_INVERSE * sizeof(double));
     auto it = interned.find(key);
     if (it == interned.end()) {
-      it = interned.emplace(std::move(key), uint32_t(table.size() / HEX_INVERSE)).first;
-      for (uint32_t k = 0; k < R; ++k)  // transposed: entry[R k + r] = Inv[r][k]
-        for (uint32_t r = 0; r < R; ++r) table.push_back(el.inverse[R * r + k]);
+      it = interned.emplace(std::move(key), uint32_t(interned.size())).first;
+      for (uint32_t i = 0; i < R; ++i)  // upper triangle, row by row
+        for (uint32_t j = i; j < R; ++j) table.push_back(el.inverse[R * i + j]);
     }
     e_inv_id[s] = it->second;
   }
-  dv->inverses = uint32_t(table.size() / HEX_INVERSE);
+  dv->inverses = uint32_t(interned.size());
   LatDev &d = dv->d;
   d.N = N;
   d.E = E;
@@ -754,6 +797,8 @@ void LatticeSet::upload() {
   d.body_substeps = dv->alloc<uint32_t>(B);
   d.body_exact = dv->alloc<uint8_t>(B);
   d.e_body = dv->copy(e_body);
+  d.e_sub = dv->alloc<uint32_t>(Es);
+  dv->slot_body = e_body;
   d.e_nodes = dv->copy(e_nodes);
   d.e_inv_id = dv->copy(e_inv_id);
   d.e_shape_inv = dv->copy(e_shape_inv);
@@ -775,6 +820,7 @@ void LatticeSet::upload() {
   d.e_coupling = dv->alloc<double>(R * Es);
   d.e_image = dv->alloc<double>(R * Es);
   d.e_scal = dv->alloc<double>(3 * Es);
+  d.e_dv = dv->alloc<double>(24 * Es);
   HIPCHECK(hipEventCreate(&dv->start));
   HIPCHECK(hipEventCreate(&dv->stop));
   dev_ = std::move(dv);
@@ -814,6 +860,14 @@ void LatticeSet::step(const talus_mi_lattice_params &p, talus_mi_lattice_report 
                             hipMemcpyHostToDevice, stream_));
     HIPCHECK(hipMemcpyAsync(const_cast<uint8_t *>(d.body_exact), exact.data(), B, hipMemcpyHostToDevice,
                             stream_));
+    std::vector<uint32_t> e_sub(dv.slot_body.size());
+    for (size_t s = 0; s < e_sub.size(); ++s) {
+      const uint32_t b = dv.slot_body[s];
+      e_sub[s] = substeps[b] | (exact[b] ? 0x80000000u : 0u);
+    }
+    if (!e_sub.empty())
+      HIPCHECK(hipMemcpyAsync(const_cast<uint32_t *>(d.e_sub), e_sub.data(), e_sub.size() * sizeof(uint32_t),
+                              hipMemcpyHostToDevice, stream_));
     HIPCHECK(hipStreamSynchronize(stream_));
     dv.body_substeps = substeps;
   }
@@ -849,37 +903,55 @@ void LatticeSet::step(const talus_mi_lattice_params &p, talus_mi_lattice_report 
     timed.emplace_back(k, kind);
   };
   auto enqueue = [&]() {
+    // integrate of substep 0; each advance integrates the next substep
+    hipLaunchKernelGGL(k_lat_integrate, dim3(node_grid), dim3(NODE_BLOCK), 0, stream_, d, 0u, p.gravity[0],
+                       p.gravity[1], p.gravity[2]);
     for (uint32_t s = 0; s < S; ++s) {
-      hipLaunchKernelGGL(k_lat_integrate, dim3(node_grid), dim3(NODE_BLOCK), 0, stream_, d, s, p.gravity[0],
-                         p.gravity[1], p.gravity[2]);
-      if (any_exact)
+      if (any_exact) {
+        // every exact body's elements are updated before the passes: the
+        // update reads positions only, so one launch covers every colour;
+        // the warm starts then go colour by colour
+        mark(0, [&] {
+          hipLaunchKernelGGL(k_lat_frame, dim3((d.E + EBLOCK - 1) / EBLOCK), dim3(EBLOCK), 0, stream_, d, 0u, d.E,
+                             s, 1u);
+        });
         for (uint32_t col = 0; col < 8; ++col) {
           const uint32_t first = dv.colour_start[col], count = dv.colour_start[col + 1] - first;
           if (count)
             mark(0, [&] {
-              hipLaunchKernelGGL(k_lat_update, dim3((count + EPB - 1) / EPB), dim3(EBLOCK), 0, stream_, d,
-                                 first, count, s, 1u);
+              hipLaunchKernelGGL(k_lat_warm, dim3((count + 7) / 8), dim3(EBLOCK), 0, stream_, d, first, count, s,
+                                 1u);
             });
         }
+      }
       for (uint32_t pass = 0; pass < PASSES; ++pass)
         for (uint32_t col = 0; col < 8; ++col) {
           const uint32_t first = dv.colour_start[col], count = dv.colour_start[col + 1] - first;
           if (!count) continue;
-          const dim3 grid((count + EPB - 1) / EPB);
-          if (pass == 0 && any_inexact)
+          if (pass == 0 && any_inexact) {
             mark(0, [&] {
-              hipLaunchKernelGGL(k_lat_update, grid, dim3(EBLOCK), 0, stream_, d, first, count, s, 0u);
+              hipLaunchKernelGGL(k_lat_frame, dim3((count + EBLOCK - 1) / EBLOCK), dim3(EBLOCK), 0, stream_, d,
+                                 first, count, s, 0u);
             });
-          mark(1, [&] { hipLaunchKernelGGL(k_lat_solve, grid, dim3(EBLOCK), 0, stream_, d, first, count, s); });
+            mark(0, [&] {
+              hipLaunchKernelGGL(k_lat_warm, dim3((count + 7) / 8), dim3(EBLOCK), 0, stream_, d, first, count, s, 0u);
+            });
+          }
+          mark(1, [&] {
+            hipLaunchKernelGGL(k_lat_solve, dim3((count + EPW - 1) / EPW), dim3(EBLOCK), 0, stream_, d, first, count,
+                               s, pass == 0 ? 1u : 0u);
+          });
         }
-      hipLaunchKernelGGL(k_lat_advance, dim3(node_grid), dim3(NODE_BLOCK), 0, stream_, d, s);
+      hipLaunchKernelGGL(k_lat_advance, dim3(node_grid), dim3(NODE_BLOCK), 0, stream_, d, s, p.gravity[0],
+                         p.gravity[1], p.gravity[2]);
     }
   };
   for (uint32_t col = 0; col < 8; ++col)
     if (dv.colour_start[col + 1] > dv.colour_start[col]) {
       rep.solve_launches += S * PASSES;
-      rep.update_launches += S * ((any_exact ? 1 : 0) + (any_inexact ? 1 : 0));
+      rep.update_launches += S * ((any_exact ? 1 : 0) + (any_inexact ? 2 : 0));  // warm starts (+ geometry)
     }
+  if (any_exact && d.E) rep.update_launches += S;  // the geometry launch over every element
   rep.inverses = dv.inverses;
   HIPCHECK(hipEventRecord(dv.start, stream_));
   if (direct) {
