@@ -104,7 +104,9 @@ class LatticeSet {
  private:
   struct Device;
   void download();  // device state -> host mirrors
-  void upload();    // host mirrors -> device arrays (layout rebuilt)
+  // host mirrors -> device arrays, laid out for these per-body substep
+  // counts and exact flags
+  void upload(const std::vector<uint32_t> &substeps, const std::vector<uint8_t> &exact);
   hipStream_t stream_;
   std::vector<LatticeBody> bodies_;
   std::unique_ptr<Device> dev_;

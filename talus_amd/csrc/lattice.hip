@@ -123,6 +123,9 @@ struct LatDev {
   double *e_image;             // 18
   double *e_scal;              // 3: gain, rhs, 1 / schur (vweight + gain before the first pass)
   double *e_dv;                // 24: warm-start corner velocity changes
+  // per node, its (slot * 8 + corner) entries with a free corner, in slot
+  // (so colour) order: the warm start gathers them node by node
+  const uint32_t *warm_start, *warm_entry;
 };
 
 __device__ __forceinline__ double corner_sign(uint32_t c, int axis) {
@@ -457,6 +460,30 @@ __global__ void __launch_bounds__(EBLOCK) k_lat_warm(LatDev d, uint32_t first, u
   for (int i = 0; i < 3; ++i) vp[i] -= d.e_dv[at(e, 24, 3 * c + i)];
 }
 
+// applyElement's warm start of every exact body, node by node: each node
+// subtracts its elements' corner changes in slot order, which is colour order,
+// so the node sees the same subtractions in the same order as the colour by
+// colour application.
+__global__ void __launch_bounds__(NODE_BLOCK) k_lat_warm_gather(LatDev d, uint32_t substep) {
+  const uint32_t i = blockIdx.x * NODE_BLOCK + threadIdx.x;
+  if (i >= d.N) return;
+  const uint32_t b = d.node_body[i];
+  if (substep >= d.body_substeps[b] || !d.body_exact[b]) return;
+  const uint32_t k0 = d.warm_start[i], k1 = d.warm_start[i + 1];
+  if (k0 == k1) return;
+  double *vp = d.v + 3 * size_t(i);
+  double v0 = vp[0], v1 = vp[1], v2 = vp[2];
+  for (uint32_t k = k0; k < k1; ++k) {
+    const double *dv = d.e_dv + 3 * size_t(d.warm_entry[k]);  // entry = slot * 8 + corner
+    v0 -= dv[0];
+    v1 -= dv[1];
+    v2 -= dv[2];
+  }
+  vp[0] = v0;
+  vp[1] = v1;
+  vp[2] = v2;
+}
+
 // solveElement for one colour's elements: one block solve of the 18 rows with
 // the volume row folded in through its Schur complement, applied to the
 // corner velocities.  One lane per row, 18 lanes per element, three
@@ -590,7 +617,9 @@ struct LatticeSet::Device {
   uint32_t colour_start[9] = {};
   std::vector<uint32_t> node0;               // first node of each body
   std::vector<std::vector<uint32_t>> slot_of;  // per body, per element: slot
-  std::vector<uint32_t> body_substeps;
+  std::vector<uint32_t> body_substeps;         // per body, as laid out
+  std::vector<uint8_t> body_exact;
+  std::vector<uint32_t> active;                // [substep][colour]: elements taking that substep
   std::vector<uint32_t> slot_body;             // per slot: its body
   uint32_t inverses = 0;
   hipEvent_t start = nullptr, stop = nullptr;
@@ -697,7 +726,7 @@ void LatticeSet::download() {
   }
 }
 
-void LatticeSet::upload() {
+void LatticeSet::upload(const std::vector<uint32_t> &substeps, const std::vector<uint8_t> &exact) {
   dev_.reset();
   auto dv = std::make_unique<Device>();
   const uint32_t B = uint32_t(bodies_.size());
@@ -710,17 +739,26 @@ void LatticeSet::upload() {
     E += uint32_t(bodies_[b].elements.size());
     dv->slot_of[b].assign(bodies_[b].elements.size(), 0);
   }
-  // slots: colour, body, cell
+  // slots: colour, then the bodies by descending substep count (ties in
+  // body order), then cell, so the elements that take substep s form a
+  // prefix of every colour's range
+  std::vector<uint32_t> by_substeps(B);
+  for (uint32_t b = 0; b < B; ++b) by_substeps[b] = b;
+  std::stable_sort(by_substeps.begin(), by_substeps.end(),
+                   [&](uint32_t a, uint32_t b) { return substeps[a] > substeps[b]; });
+  const uint32_t smax = B ? substeps[by_substeps[0]] : 0u;
   std::vector<std::pair<uint32_t, uint32_t>> order;  // (body, element)
   order.reserve(E);
+  dv->active.assign(8 * size_t(smax), 0u);
   for (uint32_t col = 0; col < 8; ++col) {
     dv->colour_start[col] = uint32_t(order.size());
-    for (uint32_t b = 0; b < B; ++b) {
+    for (uint32_t b : by_substeps) {
       const LatticeBody &body = bodies_[b];
       for (uint32_t i = 0; i < body.elements.size(); ++i)
         if (body.hex_colour[body.elements[i].cell] == col) {
           dv->slot_of[b][i] = uint32_t(order.size());
           order.emplace_back(b, i);
+          for (uint32_t s = 0; s < substeps[b]; ++s) ++dv->active[8 * size_t(s) + col];
         }
     }
   }
@@ -786,6 +824,19 @@ void LatticeSet::upload() {
     e_inv_id[s] = it->second;
   }
   dv->inverses = uint32_t(interned.size());
+  // warm-start entries per node, in slot order
+  std::vector<uint32_t> warm_start(size_t(N) + 1, 0), warm_entry;
+  for (size_t s = 0; s < Es; ++s)
+    for (uint32_t c = 0; c < 8; ++c)
+      if (e_inv_mass[s * 8 + c] != 0.0) ++warm_start[size_t(e_nodes[s * 8 + c]) + 1];
+  for (size_t n = 0; n < N; ++n) warm_start[n + 1] += warm_start[n];
+  warm_entry.resize(warm_start[N]);
+  {
+    std::vector<uint32_t> fill(warm_start.begin(), warm_start.end() - 1);
+    for (size_t s = 0; s < Es; ++s)
+      for (uint32_t c = 0; c < 8; ++c)
+        if (e_inv_mass[s * 8 + c] != 0.0) warm_entry[fill[e_nodes[s * 8 + c]]++] = uint32_t(s * 8 + c);
+  }
   LatDev &d = dv->d;
   d.N = N;
   d.E = E;
@@ -821,8 +872,25 @@ void LatticeSet::upload() {
   d.e_image = dv->alloc<double>(R * Es);
   d.e_scal = dv->alloc<double>(3 * Es);
   d.e_dv = dv->alloc<double>(24 * Es);
+  d.warm_start = dv->copy(warm_start);
+  d.warm_entry = dv->copy(warm_entry);
   HIPCHECK(hipEventCreate(&dv->start));
   HIPCHECK(hipEventCreate(&dv->stop));
+  // the bodies' substep counts and exact flags, per body and per slot
+  HIPCHECK(hipMemcpy(const_cast<uint32_t *>(d.body_substeps), substeps.data(), B * sizeof(uint32_t),
+                     hipMemcpyHostToDevice));
+  HIPCHECK(hipMemcpy(const_cast<uint8_t *>(d.body_exact), exact.data(), B, hipMemcpyHostToDevice));
+  {
+    std::vector<uint32_t> e_sub(Es);
+    for (size_t s = 0; s < Es; ++s) {
+      const uint32_t b = e_body[s];
+      e_sub[s] = substeps[b] | (exact[b] ? 0x80000000u : 0u);
+    }
+    if (Es) HIPCHECK(hipMemcpy(const_cast<uint32_t *>(d.e_sub), e_sub.data(), Es * sizeof(uint32_t),
+                               hipMemcpyHostToDevice));
+  }
+  dv->body_substeps = substeps;
+  dv->body_exact = exact;
   dev_ = std::move(dv);
   device_valid_ = true;
 }
@@ -852,25 +920,15 @@ void LatticeSet::step(const talus_mi_lattice_params &p, talus_mi_lattice_report 
       }
     device_valid_ = false;
   }
-  if (!device_valid_) upload();
+  // the slot order and the per-substep launch sizes follow the substep
+  // counts: a change of them lays the device arrays out again
+  if (device_valid_ && (dev_->body_substeps != substeps || dev_->body_exact != exact)) {
+    download();
+    device_valid_ = false;
+  }
+  if (!device_valid_) upload(substeps, exact);
   Device &dv = *dev_;
   LatDev &d = dv.d;
-  if (dv.body_substeps != substeps) {
-    HIPCHECK(hipMemcpyAsync(const_cast<uint32_t *>(d.body_substeps), substeps.data(), B * sizeof(uint32_t),
-                            hipMemcpyHostToDevice, stream_));
-    HIPCHECK(hipMemcpyAsync(const_cast<uint8_t *>(d.body_exact), exact.data(), B, hipMemcpyHostToDevice,
-                            stream_));
-    std::vector<uint32_t> e_sub(dv.slot_body.size());
-    for (size_t s = 0; s < e_sub.size(); ++s) {
-      const uint32_t b = dv.slot_body[s];
-      e_sub[s] = substeps[b] | (exact[b] ? 0x80000000u : 0u);
-    }
-    if (!e_sub.empty())
-      HIPCHECK(hipMemcpyAsync(const_cast<uint32_t *>(d.e_sub), e_sub.data(), e_sub.size() * sizeof(uint32_t),
-                              hipMemcpyHostToDevice, stream_));
-    HIPCHECK(hipStreamSynchronize(stream_));
-    dv.body_substeps = substeps;
-  }
   uint32_t S = 0;
   bool any_exact = false, any_inexact = false;
   for (uint32_t b = 0; b < B; ++b) {
@@ -910,19 +968,14 @@ void LatticeSet::step(const talus_mi_lattice_params &p, talus_mi_lattice_report 
       if (any_exact) {
         // every exact body's elements are updated before the passes: the
         // update reads positions only, so one launch covers every colour;
-        // the warm starts then go colour by colour
+        // the warm starts are then gathered node by node
         mark(0, [&] {
           hipLaunchKernelGGL(k_lat_frame, dim3((d.E + EBLOCK - 1) / EBLOCK), dim3(EBLOCK), 0, stream_, d, 0u, d.E,
                              s, 1u);
         });
-        for (uint32_t col = 0; col < 8; ++col) {
-          const uint32_t first = dv.colour_start[col], count = dv.colour_start[col + 1] - first;
-          if (count)
-            mark(0, [&] {
-              hipLaunchKernelGGL(k_lat_warm, dim3((count + 7) / 8), dim3(EBLOCK), 0, stream_, d, first, count, s,
-                                 1u);
-            });
-        }
+        mark(0, [&] {
+          hipLaunchKernelGGL(k_lat_warm_gather, dim3(node_grid), dim3(NODE_BLOCK), 0, stream_, d, s);
+        });
       }
       for (uint32_t pass = 0; pass < PASSES; ++pass)
         for (uint32_t col = 0; col < 8; ++col) {
@@ -937,10 +990,12 @@ void LatticeSet::step(const talus_mi_lattice_params &p, talus_mi_lattice_report 
               hipLaunchKernelGGL(k_lat_warm, dim3((count + 7) / 8), dim3(EBLOCK), 0, stream_, d, first, count, s, 0u);
             });
           }
-          mark(1, [&] {
-            hipLaunchKernelGGL(k_lat_solve, dim3((count + EPW - 1) / EPW), dim3(EBLOCK), 0, stream_, d, first, count,
-                               s, pass == 0 ? 1u : 0u);
-          });
+          const uint32_t live = dv.active[8 * size_t(s) + col];  // a prefix of the colour's range
+          if (live)
+            mark(1, [&] {
+              hipLaunchKernelGGL(k_lat_solve, dim3((live + EPW - 1) / EPW), dim3(EBLOCK), 0, stream_, d, first,
+                                 live, s, pass == 0 ? 1u : 0u);
+            });
         }
       hipLaunchKernelGGL(k_lat_advance, dim3(node_grid), dim3(NODE_BLOCK), 0, stream_, d, s, p.gravity[0],
                          p.gravity[1], p.gravity[2]);
@@ -948,10 +1003,10 @@ void LatticeSet::step(const talus_mi_lattice_params &p, talus_mi_lattice_report 
   };
   for (uint32_t col = 0; col < 8; ++col)
     if (dv.colour_start[col + 1] > dv.colour_start[col]) {
-      rep.solve_launches += S * PASSES;
-      rep.update_launches += S * ((any_exact ? 1 : 0) + (any_inexact ? 2 : 0));  // warm starts (+ geometry)
+      for (uint32_t s = 0; s < S; ++s) rep.solve_launches += dv.active[8 * size_t(s) + col] ? PASSES : 0;
+      rep.update_launches += S * (any_inexact ? 2 : 0);  // a too-stiff body's frame + warm start per colour
     }
-  if (any_exact && d.E) rep.update_launches += S;  // the geometry launch over every element
+  if (any_exact && d.E) rep.update_launches += 2 * S;  // frame over every element, warm-start gather
   rep.inverses = dv.inverses;
   HIPCHECK(hipEventRecord(dv.start, stream_));
   if (direct) {

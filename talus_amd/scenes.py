@@ -203,6 +203,27 @@ def bodies(count: int = 512, seed: int = 7, lo: int = 200, hi: int = 20000, cell
     return Scene("bodies", out, steps=120, substeps=4)
 
 
+def hex_bodies(count: int = 512, seed: int = 7, lo: int = 200, hi: int = 20000, cell: float = 0.05) -> list:
+    """The batch of uneven cuboid bodies (the same sizes, E and nu as
+    bodies(): a body of n tets is a cuboid of about n / 6 cells) as strainstep
+    hex lattices, elastic, each held on its x = 0 face.  A list of keyword
+    arguments of api.build_hex_lattice plus "materials"."""
+    rng = np.random.default_rng(seed)
+    out = []
+    targets = np.exp(rng.uniform(math.log(lo), math.log(hi), count))
+    E = np.exp(rng.uniform(math.log(1e5), math.log(1e8), count))
+    nu = rng.uniform(0.2, 0.45, count)
+    rng.uniform(0.0, 1.0, count)  # the tet batch's plasticity draw, kept so the sizes match
+    for b in range(count):
+        hexes = max(targets[b] / 6.0, 2.0)
+        c = max(1, int(round((hexes / 2.0) ** (1.0 / 3.0))))
+        a = max(2, int(round(hexes / (c * c))))
+        out.append(dict(resolution=(a + 1, c + 1, c + 1), voxel_size=(cell, cell, cell), origin=(0.0, 0.0, 0.0),
+                        materials=material_array(1, youngs_modulus=E[b], poissons_ratio=nu[b], density=1000.0),
+                        supports=[api.Support.box((0.0, 0.0, 0.0), (0.2 * cell, 1e3, 1e3))]))
+    return out
+
+
 def beam(cells=(103, 9, 9), seed: int = 5) -> Scene:
     rng = np.random.default_rng(seed)
     size = (1.0, 0.1, 0.1)
