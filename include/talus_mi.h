@@ -425,8 +425,9 @@ int talus_mi_hex_element_make(const double *corners, const double *inverse_mass,
 
 /* Step parameters: IntegrationParameters (integrationparameters.h:9-33)
  * subset the strainstep path reads. */
-/* lattice step flag: time every update and solve launch with HIP events
- * (the report's solve_ms / update_ms); otherwise those stay 0 */
+/* lattice step flag: time the update and the solve launches with HIP events
+ * around each back-to-back run of them (the report's solve_ms / update_ms);
+ * otherwise those stay 0 */
 #define TALUS_MI_KERNEL_TIMING 16u
 
 typedef struct talus_mi_lattice_params {

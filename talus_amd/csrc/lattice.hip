@@ -305,7 +305,8 @@ __device__ __forceinline__ double row_term(uint32_t r, uint32_t c, const double 
 }
 
 // updateElement at the current positions, one lane per element, for the
-// elements [first, first + count) of the bodies whose exact flag equals
+// elements [first, first + count) (of `order` when given) of the bodies whose
+// exact flag equals
 // `exact` (an exact body updates every element before the passes, a too-stiff
 // body each colour right before its first solve).  Writes the rotation, the
 // rate-scaled capped strain rows, the volume gradients, the coupling rows,
@@ -313,11 +314,11 @@ __device__ __forceinline__ double row_term(uint32_t r, uint32_t c, const double 
 // folded in by the first pass's solve, which holds the row-block inverse),
 // and the warm-start corner velocity changes applyElement makes with the
 // accumulated impulses (k_lat_warm applies them colour by colour).
-__global__ void __launch_bounds__(EBLOCK) k_lat_frame(LatDev d, uint32_t first, uint32_t count,
-                                                      uint32_t substep, uint32_t exact) {
+__global__ void __launch_bounds__(EBLOCK) k_lat_frame(LatDev d, const uint32_t *order, uint32_t first,
+                                                      uint32_t count, uint32_t substep, uint32_t exact) {
   const uint32_t t = blockIdx.x * EBLOCK + threadIdx.x;
   if (t >= count) return;
-  const uint32_t e = first + t;
+  const uint32_t e = order ? order[first + t] : first + t;  // order: slots by descending substeps
   const uint32_t sub = d.e_sub[e];
   if (substep >= (sub & 0x7fffffffu) || (sub >> 31) != exact) return;
   double inv[9];
@@ -620,6 +621,8 @@ struct LatticeSet::Device {
   std::vector<uint32_t> body_substeps;         // per body, as laid out
   std::vector<uint8_t> body_exact;
   std::vector<uint32_t> active;                // [substep][colour]: elements taking that substep
+  std::vector<uint32_t> active_total;          // [substep]: elements taking it, all colours
+  uint32_t *frame_order = nullptr;             // slots by descending substeps (device)
   std::vector<uint32_t> slot_body;             // per slot: its body
   uint32_t inverses = 0;
   hipEvent_t start = nullptr, stop = nullptr;
@@ -763,6 +766,14 @@ void LatticeSet::upload(const std::vector<uint32_t> &substeps, const std::vector
     }
   }
   dv->colour_start[8] = uint32_t(order.size());
+  dv->active_total.assign(smax, 0u);
+  for (uint32_t s = 0; s < smax; ++s)
+    for (uint32_t col = 0; col < 8; ++col) dv->active_total[s] += dv->active[8 * size_t(s) + col];
+  std::vector<uint32_t> frame_order(order.size());
+  for (size_t s = 0; s < frame_order.size(); ++s) frame_order[s] = uint32_t(s);
+  std::stable_sort(frame_order.begin(), frame_order.end(), [&](uint32_t a, uint32_t b) {
+    return substeps[order[a].first] > substeps[order[b].first];
+  });
   // nodes
   std::vector<double> x(3 * size_t(N)), v(3 * size_t(N));
   std::vector<uint32_t> node_body(N);
@@ -872,6 +883,7 @@ void LatticeSet::upload(const std::vector<uint32_t> &substeps, const std::vector
   d.e_image = dv->alloc<double>(R * Es);
   d.e_scal = dv->alloc<double>(3 * Es);
   d.e_dv = dv->alloc<double>(24 * Es);
+  dv->frame_order = dv->copy(frame_order);
   d.warm_start = dv->copy(warm_start);
   d.warm_entry = dv->copy(warm_entry);
   HIPCHECK(hipEventCreate(&dv->start));
@@ -942,13 +954,11 @@ void LatticeSet::step(const talus_mi_lattice_params &p, talus_mi_lattice_report 
   const uint32_t node_grid = (d.N + NODE_BLOCK - 1) / NODE_BLOCK;
   const bool timing = (p.flags & TALUS_MI_KERNEL_TIMING) != 0;
   const bool direct = timing || (p.flags & TALUS_MI_DIRECT_LAUNCH) != 0;
-  // kernel timing: an event before and after every update and solve launch
+  // kernel timing: an event pair around each back-to-back run of one kind
+  // (the frame and warm-start gather of a substep; its solve passes), so the
+  // per-launch average is the kernels' own time and not the events'
   std::vector<std::pair<uint32_t, int>> timed;  // (first event, kind 0 update / 1 solve)
-  auto mark = [&](int kind, auto &&launch) {
-    if (!timing) {
-      launch();
-      return;
-    }
+  auto open_span = [&]() -> uint32_t {
     const uint32_t k = uint32_t(2 * timed.size());
     while (dv.marks.size() < k + 2) {
       hipEvent_t ev;
@@ -956,7 +966,9 @@ void LatticeSet::step(const talus_mi_lattice_params &p, talus_mi_lattice_report 
       dv.marks.push_back(ev);
     }
     HIPCHECK(hipEventRecord(dv.marks[k], stream_));
-    launch();
+    return k;
+  };
+  auto close_span = [&](uint32_t k, int kind) {
     HIPCHECK(hipEventRecord(dv.marks[k + 1], stream_));
     timed.emplace_back(k, kind);
   };
@@ -965,38 +977,37 @@ void LatticeSet::step(const talus_mi_lattice_params &p, talus_mi_lattice_report 
     hipLaunchKernelGGL(k_lat_integrate, dim3(node_grid), dim3(NODE_BLOCK), 0, stream_, d, 0u, p.gravity[0],
                        p.gravity[1], p.gravity[2]);
     for (uint32_t s = 0; s < S; ++s) {
+      uint32_t span = 0;
+      if (timing && any_exact) span = open_span();
       if (any_exact) {
         // every exact body's elements are updated before the passes: the
         // update reads positions only, so one launch covers every colour;
         // the warm starts are then gathered node by node
-        mark(0, [&] {
-          hipLaunchKernelGGL(k_lat_frame, dim3((d.E + EBLOCK - 1) / EBLOCK), dim3(EBLOCK), 0, stream_, d, 0u, d.E,
-                             s, 1u);
-        });
-        mark(0, [&] {
-          hipLaunchKernelGGL(k_lat_warm_gather, dim3(node_grid), dim3(NODE_BLOCK), 0, stream_, d, s);
-        });
+        const uint32_t live = dv.active_total[s];  // a prefix of the substep order
+        if (live)
+          hipLaunchKernelGGL(k_lat_frame, dim3((live + EBLOCK - 1) / EBLOCK), dim3(EBLOCK), 0, stream_, d,
+                             dv.frame_order, 0u, live, s, 1u);
+        hipLaunchKernelGGL(k_lat_warm_gather, dim3(node_grid), dim3(NODE_BLOCK), 0, stream_, d, s);
       }
+      if (timing && any_exact) close_span(span, 0);
+      // (a too-stiff body's per-colour update runs inside the solve span)
+      if (timing) span = open_span();
       for (uint32_t pass = 0; pass < PASSES; ++pass)
         for (uint32_t col = 0; col < 8; ++col) {
           const uint32_t first = dv.colour_start[col], count = dv.colour_start[col + 1] - first;
           if (!count) continue;
           if (pass == 0 && any_inexact) {
-            mark(0, [&] {
-              hipLaunchKernelGGL(k_lat_frame, dim3((count + EBLOCK - 1) / EBLOCK), dim3(EBLOCK), 0, stream_, d,
-                                 first, count, s, 0u);
-            });
-            mark(0, [&] {
-              hipLaunchKernelGGL(k_lat_warm, dim3((count + 7) / 8), dim3(EBLOCK), 0, stream_, d, first, count, s, 0u);
-            });
+            hipLaunchKernelGGL(k_lat_frame, dim3((count + EBLOCK - 1) / EBLOCK), dim3(EBLOCK), 0, stream_, d,
+                               static_cast<const uint32_t *>(nullptr), first, count, s, 0u);
+            hipLaunchKernelGGL(k_lat_warm, dim3((count + 7) / 8), dim3(EBLOCK), 0, stream_, d, first, count, s,
+                               0u);
           }
           const uint32_t live = dv.active[8 * size_t(s) + col];  // a prefix of the colour's range
           if (live)
-            mark(1, [&] {
-              hipLaunchKernelGGL(k_lat_solve, dim3((live + EPW - 1) / EPW), dim3(EBLOCK), 0, stream_, d, first,
-                                 live, s, pass == 0 ? 1u : 0u);
-            });
+            hipLaunchKernelGGL(k_lat_solve, dim3((live + EPW - 1) / EPW), dim3(EBLOCK), 0, stream_, d, first,
+                               live, s, pass == 0 ? 1u : 0u);
         }
+      if (timing) close_span(span, 1);
       hipLaunchKernelGGL(k_lat_advance, dim3(node_grid), dim3(NODE_BLOCK), 0, stream_, d, s, p.gravity[0],
                          p.gravity[1], p.gravity[2]);
     }
