@@ -447,7 +447,7 @@ typedef struct talus_mi_lattice_report {
   uint32_t inverses;        /* distinct 18x18 row-block inverses on the device */
   uint32_t solve_launches;  /* launches of the solve kernel this step */
   uint32_t update_launches; /* launches of the update kernel this step */
-  uint32_t reserved;
+  uint32_t refreshed;       /* elements rebuilt for plastic flow (refreshFlowed) */
   double solve_ms;          /* their summed HIP-event time (KERNEL_TIMING) */
   double update_ms;
 } talus_mi_lattice_report;
@@ -458,8 +458,9 @@ void talus_mi_lattice_default_params(talus_mi_lattice_params *params);
  * 120-136): positions rest + offset, velocity given (zero when anchored),
  * cell size = the smallest voxel side.  The strainstep path needs material
  * lanes (strainstep.cc:86-94: lanes past the array read the last one).
- * Damping ratios, yield stresses and tear strains must be infinite (the
- * lowest-mode damping, lattice plasticity and cell tearing are not in this
+ * Lanes with a finite yield stress flow plastically (flowLattice after each
+ * step, refreshFlowed before the next).  Damping ratios and tear strains must
+ * be infinite (the lowest-mode damping and cell tearing are not in this
  * library): TALUS_MI_ERR_ARGUMENT otherwise. */
 int talus_mi_lattice_add(talus_mi_ctx *ctx, const talus_mi_hex_mesh *mesh,
                          const talus_mi_material *materials, uint32_t lanes,
@@ -474,15 +475,26 @@ int talus_mi_lattice_state_set(talus_mi_ctx *ctx, uint32_t lattice, const double
  * output may be NULL; *count = elements (0 before the first step). */
 int talus_mi_lattice_impulses_get(talus_mi_ctx *ctx, uint32_t lattice, uint32_t *cells,
                                   double *impulses, uint32_t capacity, uint32_t *count);
+/* LatticeBody::plastic / hardening (plasticflow.cc): per hex and Gauss point
+ * (point p at the corner-p signs / sqrt(3)) the inverse plastic deformation P
+ * (elastic = F P, 9 per point, row-major) and the accumulated plastic strain
+ * (1 per point).  *present = 0 (outputs untouched) until the body first
+ * flows.  Setting P makes the next step rebuild the elements of the cells
+ * whose P moved (refreshFlowed). */
+int talus_mi_lattice_plastic_get(talus_mi_ctx *ctx, uint32_t lattice, double *plastic,
+                                 double *hardening, uint32_t *present);
+int talus_mi_lattice_plastic_set(talus_mi_ctx *ctx, uint32_t lattice, const double *plastic,
+                                 const double *hardening);
 /* Node and hex colours (makeLattice, bodyset.cc:30-37): the parity of a
  * node's lattice index per axis; a hex takes its corner 0's colour.  Hexes of
  * one colour share no node, so each colour's elements are solved together. */
 int talus_mi_lattice_colours_get(talus_mi_ctx *ctx, uint32_t lattice, uint8_t *node_colour,
                                  uint8_t *hex_colour);
 /* One solveStrainstep over every lattice of the context, each its own
- * island (no contacts, joints or rigid partners): per body islandSubsteps
- * (strainstep.cc:551-567) substeps of integrate, element update and
- * warm-start, 4 Gauss-Seidel passes of solveElement, advance.  The passes
+ * island (no contacts, joints or rigid partners): refreshFlowed of the cells
+ * the last step's flow moved, per body islandSubsteps (strainstep.cc:551-567)
+ * substeps of integrate, element update and warm-start, 4 Gauss-Seidel passes
+ * of solveElement, advance, then flowLattice.  The passes
  * sweep the hexes colour by colour (all bodies' hexes of one colour in one
  * launch), where the reference sweeps them in cell order. */
 int talus_mi_lattice_step(talus_mi_ctx *ctx, const talus_mi_lattice_params *params,

@@ -89,6 +89,20 @@ def lib():
         L.orc_lattice_substeps.restype = C.c_uint32
         L.orc_lattice_step.argtypes = [C.c_void_p, C.c_double, dp, C.c_uint32, C.c_uint32]
         L.orc_lattice_step.restype = C.c_uint32
+        L.orc_hex_shape_gradients.argtypes = [dp, dp, dp]
+        L.orc_hex_return_map.argtypes = [dp, C.c_double, C.c_double, dp, dp, dp]
+        L.orc_hex_return_map.restype = C.c_int
+        L.orc_hex_fitted_jacobian.argtypes = [dp, dp, C.c_uint32, dp]
+        L.orc_hex_fit_holds.argtypes = [dp, dp]
+        L.orc_hex_fit_holds.restype = C.c_int
+        L.orc_lattice_has_plastic.argtypes = [C.c_void_p]
+        L.orc_lattice_has_plastic.restype = C.c_uint32
+        L.orc_lattice_get_plastic.argtypes = [C.c_void_p, dp, dp]
+        L.orc_lattice_set_plastic.argtypes = [C.c_void_p, dp, dp]
+        L.orc_lattice_flow.argtypes = [C.c_void_p]
+        L.orc_lattice_flow.restype = C.c_double
+        L.orc_lattice_plastic_rest.argtypes = [C.c_void_p, C.c_uint32, dp, dp, dp]
+        L.orc_lattice_write_fitted.argtypes = [C.c_void_p, C.c_uint32, dp, dp]
         _ready = True
     return L
 
@@ -185,6 +199,35 @@ def apply(e, velocities, impulse, vol_impulse):
     assert velocities.dtype == np.float64 and velocities.flags.c_contiguous
     imp = _f64(impulse, ROWS)
     lib().orc_hex_apply(C.byref(e), _dp(velocities.reshape(-1)), _dp(imp), float(vol_impulse))
+
+
+def shape_gradients(at, half):
+    """shapeGradients: trilinear shape gradients (8, 3) at reference point `at`."""
+    a, hh = _f64(at, 3), _f64(half, 3)
+    g = np.zeros(24)
+    lib().orc_hex_shape_gradients(_dp(a), _dp(hh), _dp(g))
+    return g.reshape(8, 3)
+
+
+def return_map(elastic, shear, yield_stress):
+    """returnMap: (mapped elastic F, trial stress, plastic strain) or None."""
+    E = _f64(elastic, 9)
+    out = np.zeros(9)
+    t, st = C.c_double(), C.c_double()
+    ok = lib().orc_hex_return_map(_dp(E), shear, yield_stress, _dp(out), C.byref(t), C.byref(st))
+    return (out.reshape(3, 3), t.value, st.value) if ok else None
+
+
+def fitted_jacobian(shape, hourglass, point):
+    sh, hg = _f64(shape, 9), _f64(hourglass, 12)
+    J = np.zeros(9)
+    lib().orc_hex_fitted_jacobian(_dp(sh), _dp(hg), point, _dp(J))
+    return J.reshape(3, 3)
+
+
+def fit_holds(shape, hourglass):
+    sh, hg = _f64(shape, 9), _f64(hourglass, 12)
+    return bool(lib().orc_hex_fit_holds(_dp(sh), _dp(hg)))
 
 
 def build_hex_lattice(res, voxel, origin=(0.0, 0.0, 0.0), materials=(), supports=(),
@@ -301,3 +344,34 @@ class LatticeBody:
     def step(self, dt=1.0 / 60.0, gravity=(0.0, -9.81, 0.0), softstep_substeps=4, colored=False):
         g = _f64(gravity, 3)
         return int(lib().orc_lattice_step(self._h, dt, _dp(g), softstep_substeps, 1 if colored else 0))
+
+    # ---- plasticity -------------------------------------------------------
+
+    def has_plastic(self):
+        return bool(lib().orc_lattice_has_plastic(self._h))
+
+    def plastic(self):
+        """(P[8 hexes, 3, 3], plastic strain[8 hexes]) per Gauss point, or None."""
+        if not self.has_plastic():
+            return None
+        P = np.zeros(72 * self.hexes)
+        a = np.zeros(8 * self.hexes)
+        lib().orc_lattice_get_plastic(self._h, _dp(P), _dp(a))
+        return P.reshape(-1, 3, 3), a
+
+    def set_plastic(self, P=None, hardening=None):
+        P = None if P is None else _f64(P, 72 * self.hexes)
+        a = None if hardening is None else _f64(hardening, 8 * self.hexes)
+        lib().orc_lattice_set_plastic(self._h, _dp(P), _dp(a))
+
+    def flow(self):
+        return float(lib().orc_lattice_flow(self._h))
+
+    def plastic_rest(self, cell):
+        shape, hg, vol = np.zeros(9), np.zeros(12), C.c_double()
+        lib().orc_lattice_plastic_rest(self._h, cell, _dp(shape), _dp(hg), C.byref(vol))
+        return shape.reshape(3, 3), hg.reshape(4, 3), vol.value
+
+    def write_fitted(self, cell, shape, hourglass):
+        sh, hg = _f64(shape, 9), _f64(hourglass, 12)
+        lib().orc_lattice_write_fitted(self._h, cell, _dp(sh), _dp(hg))

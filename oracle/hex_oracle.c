@@ -641,6 +641,7 @@ struct orc_lattice_body {
   float *volume;
   uint32_t *lane;
   orc_material *materials;
+  float voxel[3];
   double cell; /* min voxel */
   double *x, *v;
   uint8_t *node_colour, *hex_colour;
@@ -649,6 +650,10 @@ struct orc_lattice_body {
   double h;          /* 0 = not built */
   orc_hex *elements;
   uint32_t count;
+  /* plasticity: per Gauss point P (9) and plastic strain, the P the elements
+   * were last built from, and per cell the rest the element was built on */
+  double *plastic, *hardening, *cached;
+  double *rest_shape, *rest_hg, *rest_vol; /* 9, 12, 1 per cell; volume 0 = none */
 };
 
 orc_lattice_body *orc_lattice_create(const orc_hex_mesh *mesh, const orc_material *materials,
@@ -675,6 +680,7 @@ orc_lattice_body *orc_lattice_create(const orc_hex_mesh *mesh, const orc_materia
   b->materials = (orc_material *)malloc(lanes * sizeof(orc_material));
   memcpy(b->materials, materials, lanes * sizeof(orc_material));
   double vox[3] = {mesh->voxel[0], mesh->voxel[1], mesh->voxel[2]};
+  for (int k = 0; k < 3; ++k) b->voxel[k] = mesh->voxel[k];
   b->cell = vox[0] < vox[1] ? vox[0] : vox[1];
   if (vox[2] < b->cell) b->cell = vox[2];
   b->x = (double *)malloc(3 * n * sizeof(double));
@@ -696,6 +702,10 @@ orc_lattice_body *orc_lattice_create(const orc_hex_mesh *mesh, const orc_materia
     b->node_colour[i] = (uint8_t)colour;
   }
   for (uint32_t c = 0; c < mesh->hexes; ++c) b->hex_colour[c] = b->node_colour[b->hex[8 * (size_t)c]];
+  const size_t H = mesh->hexes ? mesh->hexes : 1;
+  b->rest_shape = (double *)calloc(9 * H, sizeof(double));
+  b->rest_hg = (double *)calloc(12 * H, sizeof(double));
+  b->rest_vol = (double *)calloc(H, sizeof(double));
   return b;
 }
 
@@ -713,6 +723,12 @@ void orc_lattice_free(orc_lattice_body *b) {
   free(b->node_colour);
   free(b->hex_colour);
   free(b->elements);
+  free(b->plastic);
+  free(b->hardening);
+  free(b->cached);
+  free(b->rest_shape);
+  free(b->rest_hg);
+  free(b->rest_vol);
   free(b);
 }
 
@@ -807,30 +823,347 @@ static void lattice_turn(const orc_lattice_body *b, double *q) {
   orc_rotation_of(M, id, 64, q);
 }
 
+/* ---- plasticity ---------------------------------------------------------- */
+
+static const double GAUSS_OFFSET = 0.5773502691896258; /* plasticflow.cc:14 */
+
+static void mm3(const double *A, const double *B, double *C) {
+  for (int i = 0; i < 3; ++i)
+    for (int j = 0; j < 3; ++j) C[3 * i + j] = A[3 * i] * B[j] + A[3 * i + 1] * B[3 + j] + A[3 * i + 2] * B[6 + j];
+}
+
+void orc_hex_shape_gradients(const double *at, const double *half, double *grad) {
+  for (uint32_t c = 0; c < 8; ++c) {
+    double s[3];
+    signs_of(c, s);
+    grad[3 * c + 0] = 0.125 * s[0] * (1.0 + s[1] * at[1]) * (1.0 + s[2] * at[2]) / half[0];
+    grad[3 * c + 1] = 0.125 * s[1] * (1.0 + s[0] * at[0]) * (1.0 + s[2] * at[2]) / half[1];
+    grad[3 * c + 2] = 0.125 * s[2] * (1.0 + s[0] * at[0]) * (1.0 + s[1] * at[1]) / half[2];
+  }
+}
+
+int orc_hex_return_map(const double *E, double shear, double yield, double *mapped, double *trial,
+                       double *strain) {
+  double left[9], w[3], V[9];
+  for (int i = 0; i < 3; ++i)
+    for (int j = 0; j < 3; ++j) left[3 * i + j] = E[3 * i] * E[3 * j] + E[3 * i + 1] * E[3 * j + 1] + E[3 * i + 2] * E[3 * j + 2];
+  orc_eigen_sym3(left, w, V);
+  const double lo = w[0] < w[1] ? (w[0] < w[2] ? w[0] : w[2]) : (w[1] < w[2] ? w[1] : w[2]);
+  if (!(lo > 0.0)) return 0;
+  double e[3], dev[3];
+  for (int k = 0; k < 3; ++k) e[k] = 0.5 * log(w[k]);
+  const double mean = (e[0] + e[1] + e[2]) / 3.0;
+  for (int k = 0; k < 3; ++k) dev[k] = e[k] - mean;
+  const double size = sqrt(dev[0] * dev[0] + dev[1] * dev[1] + dev[2] * dev[2]);
+  const double stress = sqrt(1.5) * 2.0 * shear * size;
+  if (!(stress > yield) || size <= 1.0e-15) return 0;
+  double scale[3];
+  for (int k = 0; k < 3; ++k) scale[k] = exp(dev[k] * (yield / stress - 1.0));
+  double corr[9];
+  for (int i = 0; i < 3; ++i)
+    for (int j = 0; j < 3; ++j) {
+      double t = 0.0;
+      for (int k = 0; k < 3; ++k) t += V[3 * i + k] * scale[k] * V[3 * j + k];
+      corr[3 * i + j] = t;
+    }
+  mm3(corr, E, mapped);
+  *trial = stress;
+  *strain = (stress - yield) / (3.0 * shear);
+  return 1;
+}
+
+void orc_hex_fitted_jacobian(const double *shape, const double *hourglass, uint32_t point, double *J) {
+  double s[3];
+  signs_of(point, s);
+  const double at[3] = {GAUSS_OFFSET * s[0], GAUSS_OFFSET * s[1], GAUSS_OFFSET * s[2]};
+  const double slopes[12] = {at[1], at[0], 0.0, 0.0, at[2], at[1], at[2], 0.0, at[0],
+                             at[1] * at[2], at[0] * at[2], at[0] * at[1]};
+  memcpy(J, shape, 9 * sizeof(double));
+  for (uint32_t m = 0; m < 4; ++m)
+    for (int i = 0; i < 3; ++i)
+      for (int j = 0; j < 3; ++j) J[3 * i + j] += hourglass[3 * m + i] * slopes[3 * m + j] / 8.0;
+}
+
+int orc_hex_fit_holds(const double *shape, const double *hourglass) {
+  for (uint32_t p = 0; p < 8; ++p) {
+    double J[9];
+    orc_hex_fitted_jacobian(shape, hourglass, p, J);
+    if (!(orc_det3(J) > 0.0)) return 0;
+  }
+  return 1;
+}
+
+static void authored_rest(const orc_lattice_body *b, uint32_t cell, double *shape, double *hg, double *vol) {
+  double corners[24];
+  for (uint32_t c = 0; c < 8; ++c)
+    for (int k = 0; k < 3; ++k) corners[3 * c + k] = b->rest[3 * (size_t)b->hex[8 * (size_t)cell + c] + k];
+  orc_hex_corner_rest(corners, shape, hg);
+  *vol = (double)b->volume[cell];
+}
+
+void orc_lattice_plastic_rest(const orc_lattice_body *b, uint32_t cell, double *shape, double *hg,
+                              double *volume) {
+  double ashape[9], ahg[12], avol;
+  authored_rest(b, cell, ashape, ahg, &avol);
+  if (!b->plastic) {
+    memcpy(shape, ashape, sizeof ashape);
+    memcpy(hg, ahg, sizeof ahg);
+    *volume = avol;
+    return;
+  }
+  double stretch[8][9], mean[9] = {0}, vol = 0.0;
+  for (uint32_t p = 0; p < 8; ++p) {
+    inverse3(b->plastic + 9 * (8 * (size_t)cell + p), stretch[p]);
+    for (int k = 0; k < 9; ++k) mean[k] += stretch[p][k] / 8.0;
+    vol += orc_det3(stretch[p]) / 8.0;
+  }
+  const double factor = sqrt(3.0) / 2.0;
+  mm3(mean, ashape, shape);
+  *volume = avol * vol;
+  memset(hg, 0, 12 * sizeof(double));
+  for (uint32_t p = 0; p < 8; ++p) {
+    double target[9], s[3];
+    mm3(stretch[p], ashape, target);
+    signs_of(p, s);
+    const double d0[3] = {s[1], s[0], 0.0}, d1[3] = {0.0, s[2], s[1]}, d2[3] = {s[2], 0.0, s[0]};
+    const double d3[3] = {s[1] * s[2], s[0] * s[2], s[0] * s[1]};
+    double t[3];
+    mv3(target, d0, t);
+    for (int k = 0; k < 3; ++k) hg[k] += factor * t[k];
+    mv3(target, d1, t);
+    for (int k = 0; k < 3; ++k) hg[3 + k] += factor * t[k];
+    mv3(target, d2, t);
+    for (int k = 0; k < 3; ++k) hg[6 + k] += factor * t[k];
+    mv3(target, d3, t);
+    for (int k = 0; k < 3; ++k) hg[9 + k] += t[k];
+  }
+  double fitted = 0.0;
+  for (uint32_t p = 0; p < 8; ++p) {
+    double J[9];
+    orc_hex_fitted_jacobian(shape, hg, p, J);
+    fitted += orc_det3(J) / 8.0;
+  }
+  const double ratio = vol * orc_det3(ashape) / fitted;
+  const double size = isfinite(ratio) && ratio > 0.0 ? cbrt(ratio) : 1.0;
+  for (int k = 0; k < 9; ++k) shape[k] *= size;
+  for (int k = 0; k < 12; ++k) hg[k] *= size;
+}
+
+void orc_lattice_write_fitted(orc_lattice_body *b, uint32_t cell, const double *shape, const double *hg) {
+  if (!b->plastic) return;
+  double ashape[9], ahg[12], avol;
+  authored_rest(b, cell, ashape, ahg, &avol);
+  for (uint32_t p = 0; p < 8; ++p) {
+    double J[9], Ji[9];
+    orc_hex_fitted_jacobian(shape, hg, p, J);
+    inverse3(J, Ji);
+    mm3(ashape, Ji, b->plastic + 9 * (8 * (size_t)cell + p));
+  }
+}
+
+uint32_t orc_lattice_has_plastic(const orc_lattice_body *b) { return b->plastic != NULL; }
+
+void orc_lattice_get_plastic(const orc_lattice_body *b, double *plastic, double *hardening) {
+  if (!b->plastic) return;
+  if (plastic) memcpy(plastic, b->plastic, 72 * (size_t)b->hexes * sizeof(double));
+  if (hardening) memcpy(hardening, b->hardening, 8 * (size_t)b->hexes * sizeof(double));
+}
+
+void orc_lattice_set_plastic(orc_lattice_body *b, const double *plastic, const double *hardening) {
+  if (!b->plastic) {
+    b->plastic = (double *)malloc(72 * (size_t)b->hexes * sizeof(double));
+    b->hardening = (double *)calloc(8 * (size_t)b->hexes, sizeof(double));
+    b->cached = (double *)malloc(72 * (size_t)b->hexes * sizeof(double));
+    for (size_t i = 0; i < 8 * (size_t)b->hexes; ++i) {
+      memset(b->plastic + 9 * i, 0, 9 * sizeof(double));
+      b->plastic[9 * i] = b->plastic[9 * i + 4] = b->plastic[9 * i + 8] = 1.0;
+    }
+    memcpy(b->cached, b->plastic, 72 * (size_t)b->hexes * sizeof(double));
+  }
+  if (plastic) memcpy(b->plastic, plastic, 72 * (size_t)b->hexes * sizeof(double));
+  if (hardening) memcpy(b->hardening, hardening, 8 * (size_t)b->hexes * sizeof(double));
+}
+
+static double frobenius3(const double *m) {
+  double t = 0.0;
+  for (int k = 0; k < 9; ++k) t += m[k] * m[k];
+  return sqrt(t);
+}
+
+double orc_lattice_flow(orc_lattice_body *b) {
+  int yields = 0;
+  for (uint32_t c = 0; c < b->hexes; ++c) yields = yields || isfinite(lane_mat(b, c)->yield_stress);
+  if (!yields) return 0.0;
+  if (!b->plastic) orc_lattice_set_plastic(b, NULL, NULL);
+  double largest = 0.0;
+  const double half[3] = {(double)(0.5f * (float)b->voxel[0]), (double)(0.5f * (float)b->voxel[1]),
+                          (double)(0.5f * (float)b->voxel[2])};
+  for (uint32_t cell = 0; cell < b->hexes; ++cell) {
+    const orc_material *m = lane_mat(b, cell);
+    if (!isfinite(m->yield_stress)) continue;
+    double shear, first;
+    orc_hex_material(m->youngs_modulus, m->poissons_ratio, &shear, &first);
+    const double bulk = first + 2.0 * shear / 3.0;
+    const double ratio = (double)m->hardening_modulus / (3.0 * shear);
+    const int crushable = isfinite(m->crush_stress) && m->crush_limit < 1.0f;
+    const double crush = (double)m->crush_stress / bulk;
+    for (uint32_t p = 0; p < 8; ++p) {
+      const size_t at = 8 * (size_t)cell + p;
+      double s[3], grad[24];
+      signs_of(p, s);
+      const double xi[3] = {GAUSS_OFFSET * s[0], GAUSS_OFFSET * s[1], GAUSS_OFFSET * s[2]};
+      orc_hex_shape_gradients(xi, half, grad);
+      double F[9] = {0};
+      for (uint32_t c = 0; c < 8; ++c) {
+        const double *x = b->x + 3 * (size_t)b->hex[8 * (size_t)cell + c];
+        for (int i = 0; i < 3; ++i)
+          for (int j = 0; j < 3; ++j) F[3 * i + j] += x[i] * grad[3 * c + j];
+      }
+      if (!(orc_det3(F) > 0.0)) continue;
+      double P[9], E[9];
+      memcpy(P, b->plastic + 9 * at, sizeof P);
+      mm3(F, P, E);
+      const double start = (double)m->yield_stress + (double)m->hardening_modulus * b->hardening[at];
+      const double room = crushable ? -log(orc_det3(P) * (double)m->crush_limit) : 0.0;
+      const int squeezed = crushable && room > 0.0 && orc_det3(E) < exp(-crush);
+      double tE[9], tt, ts;
+      const int trial = orc_hex_return_map(E, shear, start, tE, &tt, &ts);
+      const double now = trial ? (start + ratio * tt) / (1.0 + ratio) : start;
+      double mE[9], mt = 0.0, ms = 0.0;
+      const int mapped = trial ? orc_hex_return_map(E, shear, now, mE, &mt, &ms) : 0;
+      const double *after = mapped ? mE : E;
+      const double volume = orc_det3(after);
+      double compaction = 0.0;
+      if (squeezed && volume > 0.0) {
+        compaction = -crush - log(volume);
+        if (room < compaction) compaction = room;
+      }
+      if (!mapped && !(compaction > 0.0)) continue;
+      const double grow = compaction > 0.0 ? exp(compaction / 3.0) : 1.0;
+      double Fi[9], ga[9], next[9];
+      inverse3(F, Fi);
+      for (int k = 0; k < 9; ++k) ga[k] = grow * after[k];
+      mm3(Fi, ga, next);
+      const double kept = orc_det3(next);
+      const double cap = (double)m->plastic_max;
+      if (isfinite(cap) && kept > 0.0) {
+        double st[9];
+        inverse3(next, st);
+        st[0] -= 1.0;
+        st[4] -= 1.0;
+        st[8] -= 1.0;
+        const double size = frobenius3(st);
+        if (size > cap) {
+          double capped[9];
+          for (int k = 0; k < 9; ++k) capped[k] = (cap / size) * st[k];
+          capped[0] += 1.0;
+          capped[4] += 1.0;
+          capped[8] += 1.0;
+          const double cv = orc_det3(capped);
+          if (!(cv > 1.0e-6)) continue;
+          double ci[9];
+          inverse3(capped, ci);
+          const double sc = cbrt(kept * cv);
+          for (int k = 0; k < 9; ++k) next[k] = sc * ci[k];
+        }
+      }
+      double change = 0.0;
+      for (int k = 0; k < 9; ++k) change += fabs(next[k] - P[k]);
+      if (!(change > 1.0e-12)) continue;
+      if (largest < change) largest = change;
+      memcpy(b->plastic + 9 * at, next, sizeof next);
+      b->hardening[at] += mapped ? ms : 0.0;
+    }
+  }
+  return largest;
+}
+
+static double remaining(const double *before, const double *after, int n) {
+  double was = 0.0, now = 0.0;
+  for (int k = 0; k < n; ++k) {
+    was += before[k] * before[k];
+    now += after[k] * after[k];
+  }
+  if (!(was > 0.0)) return 1.0;
+  const double r = sqrt(now / was);
+  return r < 1.0 ? r : 1.0;
+}
+
+static void inverse_masses(const orc_lattice_body *b, double *inv) {
+  for (size_t i = 0; i < b->nodes; ++i) inv[i] = (!pinned(b, i) && b->mass[i] > 0.0f) ? 1.0 / (double)b->mass[i] : 0.0;
+}
+
+static int make_cell(const orc_lattice_body *b, const double *inv, uint32_t cell, const double *shape,
+                     const double *hg, double volume, double h, orc_hex *out) {
+  const uint32_t *nd = b->hex + 8 * (size_t)cell;
+  double w[8], shear, first;
+  for (uint32_t c = 0; c < 8; ++c) w[c] = inv[nd[c]];
+  const orc_material *m = lane_mat(b, cell);
+  orc_hex_material(m->youngs_modulus, m->poissons_ratio, &shear, &first);
+  return orc_hex_make(out, cell, nd, w, shape, hg, volume, shear, first, h);
+}
+
+/* refreshFlowed (strainstep.cc:371-451): rebuilds the elements of the cells
+ * whose plastic state moved since their element was built */
+static void refresh_flowed(orc_lattice_body *b) {
+  if (!b->plastic) return;
+  double *inv = (double *)malloc((b->nodes ? b->nodes : 1) * sizeof(double));
+  inverse_masses(b, inv);
+  for (uint32_t i = 0; i < b->count; ++i) {
+    orc_hex *old = &b->elements[i];
+    const uint32_t cell = old->cell;
+    if (!memcmp(b->plastic + 72 * (size_t)cell, b->cached + 72 * (size_t)cell, 72 * sizeof(double))) continue;
+    double shape[9], hg[12], vol;
+    orc_lattice_plastic_rest(b, cell, shape, hg, &vol);
+    if (!orc_hex_fit_holds(shape, hg)) continue;
+    orc_hex fresh;
+    if (!make_cell(b, inv, cell, shape, hg, vol, b->h, &fresh)) continue;
+    memcpy(fresh.quat, old->quat, sizeof fresh.quat);
+    memcpy(fresh.impulse, old->impulse, sizeof fresh.impulse);
+    fresh.vol_impulse = old->vol_impulse;
+    orc_hex_update(old, b->x);
+    orc_hex_update(&fresh, b->x);
+    const double strain = remaining(old->strain, fresh.strain, 6);
+    const double bend = remaining(old->strain + 6, fresh.strain + 6, ROWS - 6);
+    for (uint32_t r = 0; r < ROWS; ++r) fresh.impulse[r] *= r < 6 ? strain : bend;
+    orc_lattice_write_fitted(b, cell, shape, hg);
+    memcpy(b->cached + 72 * (size_t)cell, b->plastic + 72 * (size_t)cell, 72 * sizeof(double));
+    memcpy(b->rest_shape + 9 * (size_t)cell, shape, sizeof shape);
+    memcpy(b->rest_hg + 12 * (size_t)cell, hg, sizeof hg);
+    b->rest_vol[cell] = vol;
+    *old = fresh;
+  }
+  free(inv);
+}
+
 static void build_elements(orc_lattice_body *b, double h) {
-  double *inv = (double *)calloc(b->nodes, sizeof(double));
-  for (size_t i = 0; i < b->nodes; ++i)
-    if (!pinned(b, i) && b->mass[i] > 0.0f) inv[i] = 1.0 / (double)b->mass[i];
+  double *inv = (double *)calloc(b->nodes ? b->nodes : 1, sizeof(double));
+  inverse_masses(b, inv);
   double turn[4];
   lattice_turn(b, turn);
   orc_hex *fresh = (orc_hex *)malloc((b->hexes ? b->hexes : 1) * sizeof(orc_hex));
   uint32_t count = 0;
   for (uint32_t cell = 0; cell < b->hexes; ++cell) {
-    const uint32_t *nd = b->hex + 8 * (size_t)cell;
-    double corners[24], w[8], shape[9], hg[12], shear, first;
-    for (uint32_t c = 0; c < 8; ++c) {
-      w[c] = inv[nd[c]];
-      for (int k = 0; k < 3; ++k) corners[3 * c + k] = b->rest[3 * (size_t)nd[c] + k];
+    /* restOf (strainstep.cc:128-136): the plastic rest, or the rest kept from
+     * the last build when the fit does not hold */
+    double shape[9], hg[12], vol;
+    orc_lattice_plastic_rest(b, cell, shape, hg, &vol);
+    if (!orc_hex_fit_holds(shape, hg) && b->rest_vol[cell] > 0.0) {
+      memcpy(shape, b->rest_shape + 9 * (size_t)cell, sizeof shape);
+      memcpy(hg, b->rest_hg + 12 * (size_t)cell, sizeof hg);
+      vol = b->rest_vol[cell];
     }
-    orc_hex_corner_rest(corners, shape, hg);
-    const orc_material *m = lane_mat(b, cell);
-    orc_hex_material(m->youngs_modulus, m->poissons_ratio, &shear, &first);
-    if (orc_hex_make(&fresh[count], cell, nd, w, shape, hg, (double)b->volume[cell], shear, first, h)) {
+    if (make_cell(b, inv, cell, shape, hg, vol, h, &fresh[count])) {
       memcpy(fresh[count].quat, turn, sizeof turn);
       orc_quat_matrix(turn, fresh[count].turn);
       ++count;
     }
+    const int holds = orc_hex_fit_holds(shape, hg);
+    memcpy(b->rest_shape + 9 * (size_t)cell, shape, sizeof shape);
+    memcpy(b->rest_hg + 12 * (size_t)cell, hg, sizeof hg);
+    b->rest_vol[cell] = holds ? vol : 0.0;
   }
+  if (b->plastic) memcpy(b->cached, b->plastic, 72 * (size_t)b->hexes * sizeof(double));
   free(inv);
   free(b->elements);
   b->elements = fresh;
@@ -842,7 +1175,10 @@ uint32_t orc_lattice_step(orc_lattice_body *b, double dt, const double *g, uint3
                           uint32_t colored) {
   const uint32_t substeps = orc_lattice_substeps(b, dt, softstep);
   const double h = dt / (double)substeps;
-  if (b->h != h) build_elements(b, h); /* fresh elements: impulses start at zero */
+  if (b->h != h)
+    build_elements(b, h); /* fresh elements: impulses start at zero */
+  else
+    refresh_flowed(b);
   const int exact = !(h * stiffest(b) > RESOLVED);
   /* sweep order: cell order, or colour by colour */
   uint32_t *order = (uint32_t *)malloc((b->count ? b->count : 1) * sizeof(uint32_t));
@@ -881,5 +1217,6 @@ uint32_t orc_lattice_step(orc_lattice_body *b, double dt, const double *g, uint3
     }
   }
   free(order);
+  (void)orc_lattice_flow(b);
   return substeps;
 }

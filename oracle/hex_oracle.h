@@ -110,11 +110,40 @@ void orc_lattice_impulses(const orc_lattice_body *body, uint32_t *cells, double 
 /* The substeps one step takes (islandSubsteps, strainstep.cc:551-567). */
 uint32_t orc_lattice_substeps(const orc_lattice_body *body, double dt,
                               uint32_t softstep_substeps);
+/* ---- lattice plasticity (plasticflow.cc, strainstep.cc:371-451) ---- */
+/* shapeGradients (neohookean.cc:92-112): trilinear shape gradients at the
+ * reference point `at` of a hex with half extents `half`, 3 per corner. */
+void orc_hex_shape_gradients(const double *at, const double *half, double *grad);
+/* returnMap (plasticflow.cc:110-155): radial return of the Hencky strain of
+ * the elastic F to `yield`; returns 0 when the point stays elastic. */
+int orc_hex_return_map(const double *elastic, double shear, double yield, double *mapped,
+                       double *trial, double *strain);
+/* fittedJacobian (plasticflow.cc:83-92) at Gauss point `point`. */
+void orc_hex_fitted_jacobian(const double *shape, const double *hourglass, uint32_t point,
+                             double *J);
+/* Per Gauss point (8 per hex, point p at GAUSS_OFFSET * signs(p)) the
+ * inverse plastic deformation P (elastic = F P, row-major 3x3) and the
+ * accumulated plastic strain; empty until the first flow of a yielding body. */
+uint32_t orc_lattice_has_plastic(const orc_lattice_body *body);
+void orc_lattice_get_plastic(const orc_lattice_body *body, double *plastic, double *hardening);
+void orc_lattice_set_plastic(orc_lattice_body *body, const double *plastic, const double *hardening);
+/* flowLattice (plasticflow.cc:161-260) at the body's positions; returns the
+ * largest change (one-norm of P) of any point. */
+double orc_lattice_flow(orc_lattice_body *body);
+/* plasticRest (plasticflow.cc:268-318), fitHolds (320-328), writeFitted
+ * (330-340) of one cell. */
+void orc_lattice_plastic_rest(const orc_lattice_body *body, uint32_t cell, double *shape,
+                              double *hourglass, double *volume);
+int orc_hex_fit_holds(const double *shape, const double *hourglass);
+void orc_lattice_write_fitted(orc_lattice_body *body, uint32_t cell, const double *shape,
+                              const double *hourglass);
+
 /* One solveStrainstep of this body alone (strainstep.cc:569-842, no
- * contacts, joints, manifolds, plasticity or tearing).  colored = 0 sweeps the
- * elements in cell order as the reference does; colored = 1 sweeps them
- * colour by colour (hex colour 0..7), cells in order within a colour.
- * Returns the substeps taken. */
+ * contacts, joints, manifolds or tearing): refreshFlowed of the cells the
+ * last flow moved, the substeps, then flowLattice when a lane yields.
+ * colored = 0 sweeps the elements in cell order as the reference does;
+ * colored = 1 sweeps them colour by colour (hex colour 0..7), cells in order
+ * within a colour.  Returns the substeps taken. */
 uint32_t orc_lattice_step(orc_lattice_body *body, double dt, const double *gravity,
                           uint32_t softstep_substeps, uint32_t colored);
 

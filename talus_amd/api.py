@@ -177,7 +177,7 @@ class LatticeReport(C.Structure):
     _fields_ = [("lattices", C.c_uint32), ("substeps", C.c_uint32), ("elements", C.c_uint32),
                 ("rebuilds", C.c_uint32), ("element_solves", C.c_uint64), ("device_ms", C.c_double),
                 ("inverses", C.c_uint32), ("solve_launches", C.c_uint32),
-                ("update_launches", C.c_uint32), ("reserved", C.c_uint32),
+                ("update_launches", C.c_uint32), ("refreshed", C.c_uint32),
                 ("solve_ms", C.c_double), ("update_ms", C.c_double)]
 
     def as_dict(self):
@@ -253,6 +253,9 @@ _SIGNATURES = {
                                                 C.c_uint32, _P(C.c_uint32)]),
     "talus_mi_lattice_colours_get": (C.c_int, [C.c_void_p, C.c_uint32, _P(C.c_uint8), _P(C.c_uint8)]),
     "talus_mi_lattice_step": (C.c_int, [C.c_void_p, _P(LatticeParams), _P(LatticeReport)]),
+    "talus_mi_lattice_plastic_get": (C.c_int, [C.c_void_p, C.c_uint32, _P(C.c_double), _P(C.c_double),
+                                               _P(C.c_uint32)]),
+    "talus_mi_lattice_plastic_set": (C.c_int, [C.c_void_p, C.c_uint32, _P(C.c_double), _P(C.c_double)]),
 }
 
 _lib = None
@@ -805,6 +808,19 @@ class Context:
         _check(lib().talus_mi_lattice_colours_get(self._h, lattice, nc.ctypes.data_as(C.POINTER(C.c_uint8)),
                                                   hc.ctypes.data_as(C.POINTER(C.c_uint8))))
         return nc, hc
+
+    def lattice_plastic(self, lattice):
+        """(P[8 hexes, 3, 3], plastic strain[8 hexes]) per Gauss point, or None
+        before the body first flows."""
+        h = self.lattices[lattice][1]
+        P, a, present = np.zeros(72 * h), np.zeros(8 * h), C.c_uint32()
+        _check(lib().talus_mi_lattice_plastic_get(self._h, lattice, _dp(P), _dp(a), C.byref(present)))
+        return (P.reshape(-1, 3, 3), a) if present.value else None
+
+    def set_lattice_plastic(self, lattice, P=None, hardening=None):
+        h = self.lattices[lattice][1]
+        P, a = _f64(P, 72 * h), _f64(hardening, 8 * h)
+        _check(lib().talus_mi_lattice_plastic_set(self._h, lattice, _dp(P), _dp(a)))
 
     def lattice_step(self, params: LatticeParams | None = None, **kw) -> dict:
         """solveStrainstep over every lattice of the context."""

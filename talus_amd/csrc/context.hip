@@ -2523,6 +2523,24 @@ int talus_mi_lattice_impulses_get(talus_mi_ctx *ctx, uint32_t lattice, uint32_t 
   });
 }
 
+int talus_mi_lattice_plastic_get(talus_mi_ctx *ctx, uint32_t lattice, double *plastic, double *hardening,
+                                 uint32_t *present) {
+  return guarded(ctx, [&]() -> int {
+    if (!lattice_ok(ctx, lattice)) return fail(TALUS_MI_ERR_ARGUMENT, "no such lattice");
+    ctx->lattices->get_plastic(lattice, plastic, hardening, present);
+    return TALUS_MI_OK;
+  });
+}
+
+int talus_mi_lattice_plastic_set(talus_mi_ctx *ctx, uint32_t lattice, const double *plastic,
+                                 const double *hardening) {
+  return guarded(ctx, [&]() -> int {
+    if (!lattice_ok(ctx, lattice)) return fail(TALUS_MI_ERR_ARGUMENT, "no such lattice");
+    ctx->lattices->set_plastic(lattice, plastic, hardening);
+    return TALUS_MI_OK;
+  });
+}
+
 int talus_mi_lattice_colours_get(talus_mi_ctx *ctx, uint32_t lattice, uint8_t *node_colour,
                                  uint8_t *hex_colour) {
   return guarded(ctx, [&]() -> int {

@@ -165,7 +165,87 @@ void quat_matrix(const double *q, double *R) {
 
 double clampd(double v, double lo, double hi) { return v < lo ? lo : (hi < v ? hi : v); }
 
+constexpr double GAUSS_OFFSET = 0.5773502691896258;  // 1 / sqrt(3)
+
+void mm3(const double *A, const double *B, double *C) {
+  for (int i = 0; i < 3; ++i)
+    for (int j = 0; j < 3; ++j) C[3 * i + j] = A[3 * i] * B[j] + A[3 * i + 1] * B[3 + j] + A[3 * i + 2] * B[6 + j];
+}
+
+void mv3(const double *M, const double *x, double *y) {
+  for (int r = 0; r < 3; ++r) y[r] = M[3 * r] * x[0] + M[3 * r + 1] * x[1] + M[3 * r + 2] * x[2];
+}
+
+// fittedJacobian (plasticflow.cc:83-92): the rest Jacobian of a fitted cell
+// at Gauss point `point`
+void fitted_jacobian(const double *shape, const double *hg, uint32_t point, double *J) {
+  double s[3];
+  signs_of(point, s);
+  const double at[3] = {GAUSS_OFFSET * s[0], GAUSS_OFFSET * s[1], GAUSS_OFFSET * s[2]};
+  const double slopes[12] = {at[1], at[0], 0.0, 0.0, at[2], at[1], at[2], 0.0, at[0],
+                             at[1] * at[2], at[0] * at[2], at[0] * at[1]};
+  std::memcpy(J, shape, 9 * sizeof(double));
+  for (uint32_t m = 0; m < 4; ++m)
+    for (int i = 0; i < 3; ++i)
+      for (int j = 0; j < 3; ++j) J[3 * i + j] += hg[3 * m + i] * slopes[3 * m + j] / 8.0;
+}
+
+double remaining(const double *before, const double *after, int n) {
+  double was = 0.0, now = 0.0;
+  for (int k = 0; k < n; ++k) {
+    was += before[k] * before[k];
+    now += after[k] * after[k];
+  }
+  return was > 0.0 ? std::min(1.0, std::sqrt(now / was)) : 1.0;
+}
+
 }  // namespace
+
+bool hex_fit_holds(const double *shape, const double *hg) {
+  for (uint32_t p = 0; p < 8; ++p) {
+    double J[9];
+    fitted_jacobian(shape, hg, p, J);
+    if (!(det3(J) > 0.0)) return false;
+  }
+  return true;
+}
+
+void hex_strain(HexElement &e, const double *x, double *strain) {
+  double F[9] = {}, hgv[12] = {};
+  for (uint32_t c = 0; c < 8; ++c) {
+    const double *xc = x + 3 * size_t(e.nodes[c]);
+    const double *g = e.gradient + 3 * c;
+    for (int r = 0; r < 3; ++r)
+      for (int k = 0; k < 3; ++k) F[3 * r + k] += xc[r] * g[k];
+    for (uint32_t m = 0; m < 4; ++m) {
+      const double sg = hg_sign(m, c);
+      for (int r = 0; r < 3; ++r) hgv[3 * m + r] += sg * xc[r];
+    }
+  }
+  hex_rotation_of(F, e.quat, 16, e.quat);
+  double T[9];
+  quat_matrix(e.quat, T);
+  double L[9];
+  for (int r = 0; r < 3; ++r)
+    for (int k = 0; k < 3; ++k) L[3 * r + k] = T[r] * F[k] + T[3 + r] * F[3 + k] + T[6 + r] * F[6 + k];
+  const bool inverted = det3(F) < 0.0;
+  double raw[HEX_ROWS];
+  raw[0] = L[0] - 1.0;
+  raw[1] = L[4] - 1.0;
+  raw[2] = L[8] - 1.0;
+  raw[3] = 0.5 * (L[1] + L[3]);
+  raw[4] = 0.5 * (L[5] + L[7]);
+  raw[5] = 0.5 * (L[6] + L[2]);
+  for (uint32_t m = 0; m < 4; ++m)
+    for (int k = 0; k < 3; ++k) {
+      const double b = T[k] * hgv[3 * m] + T[3 + k] * hgv[3 * m + 1] + T[6 + k] * hgv[3 * m + 2];
+      raw[6 + 3 * m + k] = (b - e.rest_hg[3 * m + k]) * e.bend;
+    }
+  for (uint32_t r = 0; r < HEX_ROWS; ++r) {
+    const double cap = inverted ? e.cap[r] : std::max(e.cap[r], 1.0);
+    strain[r] = clampd(raw[r], -cap, cap);
+  }
+}
 
 void hex_corner_rest(const double *corners, double *shape, double *hourglass) {
   std::memset(shape, 0, 9 * sizeof(double));
@@ -473,19 +553,27 @@ void LatticeBody::build_elements(double step) {
   }
   std::vector<HexElement> built(hexes);
   std::vector<uint8_t> ok(hexes, 0);
+  const bool kept = rest_vol.size() == hexes;
+  std::vector<double> shapes(9 * size_t(hexes)), hgs(12 * size_t(hexes)), vols(hexes);
   auto work = [&](uint32_t first, uint32_t last) {
     for (uint32_t cell_index = first; cell_index < last; ++cell_index) {
       const uint32_t *nd = &hex[8 * size_t(cell_index)];
-      double corners[24], w[8], shape[9], hgv[12], shear, firstl;
-      for (uint32_t c = 0; c < 8; ++c) {
-        w[c] = inv[nd[c]];
-        for (int k = 0; k < 3; ++k) corners[3 * c + k] = rest[3 * size_t(nd[c]) + k];
+      double w[8], shear, firstl;
+      double *shape = &shapes[9 * size_t(cell_index)], *hgv = &hgs[12 * size_t(cell_index)];
+      double &vol = vols[cell_index];
+      for (uint32_t c = 0; c < 8; ++c) w[c] = inv[nd[c]];
+      // restOf (strainstep.cc:128-136): the plastic rest, or the rest kept
+      // from the last build when the plastic rest's fit does not hold
+      plastic_rest(cell_index, shape, hgv, &vol);
+      if (!hex_fit_holds(shape, hgv) && kept && rest_vol[cell_index] > 0.0) {
+        std::memcpy(shape, &rest_shape[9 * size_t(cell_index)], 9 * sizeof(double));
+        std::memcpy(hgv, &rest_hg[12 * size_t(cell_index)], 12 * sizeof(double));
+        vol = rest_vol[cell_index];
       }
-      hex_corner_rest(corners, shape, hgv);
       const talus_mi_material &m = material_of(cell_index);
       hex_material(m.youngs_modulus, m.poissons_ratio, &shear, &firstl);
       HexElement &e = built[cell_index];
-      if (hex_make(e, cell_index, nd, w, shape, hgv, double(volume[cell_index]), shear, firstl, step)) {
+      if (hex_make(e, cell_index, nd, w, shape, hgv, vol, shear, firstl, step)) {
         std::memcpy(e.quat, q, sizeof q);
         ok[cell_index] = 1;
       }
@@ -507,6 +595,122 @@ void LatticeBody::build_elements(double step) {
   for (uint32_t c = 0; c < hexes; ++c)
     if (ok[c]) elements.push_back(built[c]);
   h = step;
+  // kept rests: those whose fit holds
+  rest_shape.swap(shapes);
+  rest_hg.swap(hgs);
+  rest_vol.swap(vols);
+  for (uint32_t c = 0; c < hexes; ++c)
+    if (!hex_fit_holds(&rest_shape[9 * size_t(c)], &rest_hg[12 * size_t(c)])) rest_vol[c] = 0.0;
+  if (!plastic.empty()) cached = plastic;
+}
+
+void LatticeBody::ensure_plastic() {
+  if (!plastic.empty()) return;
+  plastic.assign(72 * size_t(hexes), 0.0);
+  for (size_t i = 0; i < 8 * size_t(hexes); ++i) plastic[9 * i] = plastic[9 * i + 4] = plastic[9 * i + 8] = 1.0;
+  hardening.assign(8 * size_t(hexes), 0.0);
+  cached = plastic;
+}
+
+void LatticeBody::plastic_rest(uint32_t cell_index, double *shape, double *hg, double *vol) const {
+  double corners[24], ashape[9], ahg[12];
+  const uint32_t *nd = &hex[8 * size_t(cell_index)];
+  for (uint32_t c = 0; c < 8; ++c)
+    for (int k = 0; k < 3; ++k) corners[3 * c + k] = rest[3 * size_t(nd[c]) + k];
+  hex_corner_rest(corners, ashape, ahg);
+  const double avol = double(volume[cell_index]);
+  if (plastic.empty()) {
+    std::memcpy(shape, ashape, sizeof ashape);
+    std::memcpy(hg, ahg, sizeof ahg);
+    *vol = avol;
+    return;
+  }
+  double stretch[8][9], mean[9] = {}, v = 0.0;
+  for (uint32_t p = 0; p < 8; ++p) {
+    inverse3(&plastic[9 * (8 * size_t(cell_index) + p)], stretch[p]);
+    for (int k = 0; k < 9; ++k) mean[k] += stretch[p][k] / 8.0;
+    v += det3(stretch[p]) / 8.0;
+  }
+  const double factor = std::sqrt(3.0) / 2.0;
+  mm3(mean, ashape, shape);
+  *vol = avol * v;
+  std::memset(hg, 0, 12 * sizeof(double));
+  for (uint32_t p = 0; p < 8; ++p) {
+    double target[9], s[3], t[3];
+    mm3(stretch[p], ashape, target);
+    signs_of(p, s);
+    const double d[4][3] = {{s[1], s[0], 0.0}, {0.0, s[2], s[1]}, {s[2], 0.0, s[0]},
+                            {s[1] * s[2], s[0] * s[2], s[0] * s[1]}};
+    for (uint32_t m = 0; m < 4; ++m) {
+      mv3(target, d[m], t);
+      for (int k = 0; k < 3; ++k) hg[3 * m + k] += m < 3 ? factor * t[k] : t[k];
+    }
+  }
+  double fitted = 0.0;
+  for (uint32_t p = 0; p < 8; ++p) {
+    double J[9];
+    fitted_jacobian(shape, hg, p, J);
+    fitted += det3(J) / 8.0;
+  }
+  const double ratio = v * det3(ashape) / fitted;
+  const double size = std::isfinite(ratio) && ratio > 0.0 ? std::cbrt(ratio) : 1.0;
+  for (int k = 0; k < 9; ++k) shape[k] *= size;
+  for (int k = 0; k < 12; ++k) hg[k] *= size;
+}
+
+void LatticeBody::write_fitted(uint32_t cell_index, const double *shape, const double *hg) {
+  if (plastic.empty()) return;
+  double corners[24], ashape[9], ahg[12];
+  const uint32_t *nd = &hex[8 * size_t(cell_index)];
+  for (uint32_t c = 0; c < 8; ++c)
+    for (int k = 0; k < 3; ++k) corners[3 * c + k] = rest[3 * size_t(nd[c]) + k];
+  hex_corner_rest(corners, ashape, ahg);
+  for (uint32_t p = 0; p < 8; ++p) {
+    double J[9], Ji[9];
+    fitted_jacobian(shape, hg, p, J);
+    inverse3(J, Ji);
+    mm3(ashape, Ji, &plastic[9 * (8 * size_t(cell_index) + p)]);
+  }
+}
+
+uint32_t LatticeBody::refresh_flowed() {
+  if (plastic.empty() || h == 0.0) return 0;
+  std::vector<double> inv(nodes, 0.0);
+  for (size_t i = 0; i < nodes; ++i)
+    if (!pinned(i) && mass[i] > 0.0f) inv[i] = 1.0 / double(mass[i]);
+  uint32_t rebuilt = 0;
+  for (HexElement &old : elements) {
+    const uint32_t cell_index = old.cell;
+    const size_t at = 72 * size_t(cell_index);
+    if (!std::memcmp(&plastic[at], &cached[at], 72 * sizeof(double))) continue;
+    double shape[9], hgv[12], vol;
+    plastic_rest(cell_index, shape, hgv, &vol);
+    if (!hex_fit_holds(shape, hgv)) continue;
+    const uint32_t *nd = &hex[8 * size_t(cell_index)];
+    double w[8], shear, firstl;
+    for (uint32_t c = 0; c < 8; ++c) w[c] = inv[nd[c]];
+    const talus_mi_material &m = material_of(cell_index);
+    hex_material(m.youngs_modulus, m.poissons_ratio, &shear, &firstl);
+    HexElement fresh;
+    if (!hex_make(fresh, cell_index, nd, w, shape, hgv, vol, shear, firstl, h)) continue;
+    std::memcpy(fresh.quat, old.quat, sizeof fresh.quat);
+    std::memcpy(fresh.impulse, old.impulse, sizeof fresh.impulse);
+    fresh.vol_impulse = old.vol_impulse;
+    double before[HEX_ROWS], after[HEX_ROWS];
+    hex_strain(old, x.data(), before);
+    hex_strain(fresh, x.data(), after);
+    const double strain = remaining(before, after, 6);
+    const double bend = remaining(before + 6, after + 6, HEX_ROWS - 6);
+    for (uint32_t r = 0; r < HEX_ROWS; ++r) fresh.impulse[r] *= r < 6 ? strain : bend;
+    write_fitted(cell_index, shape, hgv);
+    std::memcpy(&cached[at], &plastic[at], 72 * sizeof(double));
+    std::memcpy(&rest_shape[9 * size_t(cell_index)], shape, sizeof shape);
+    std::memcpy(&rest_hg[12 * size_t(cell_index)], hgv, sizeof hgv);
+    rest_vol[cell_index] = vol;
+    old = fresh;
+    ++rebuilt;
+  }
+  return rebuilt;
 }
 
 int make_lattice(LatticeBody &b, const talus_mi_hex_mesh *mesh, const talus_mi_material *materials,
@@ -521,8 +725,6 @@ int make_lattice(LatticeBody &b, const talus_mi_hex_mesh *mesh, const talus_mi_m
     const talus_mi_material &m = materials[l];
     if (std::isfinite(m.damping_ratio))
       return fail(TALUS_MI_ERR_ARGUMENT, "lattice damping ratios (lowest-mode damping) are not supported");
-    if (std::isfinite(m.yield_stress) && m.yield_stress > 0.0f)
-      return fail(TALUS_MI_ERR_ARGUMENT, "lattice plasticity (flowLattice) is not supported");
     if (std::isfinite(m.tear_strain))
       return fail(TALUS_MI_ERR_ARGUMENT, "lattice cell tearing is not supported");
   }
@@ -543,6 +745,7 @@ int make_lattice(LatticeBody &b, const talus_mi_hex_mesh *mesh, const talus_mi_m
   b.lane.assign(mesh->hex_material, mesh->hex_material + b.hexes);
   b.materials.assign(materials, materials + lanes);
   const double vox[3] = {mesh->voxel_size[0], mesh->voxel_size[1], mesh->voxel_size[2]};
+  for (int k = 0; k < 3; ++k) b.voxel[k] = mesh->voxel_size[k];
   b.cell = std::min({vox[0], vox[1], vox[2]});
   double lo[3] = {INF, INF, INF};
   for (size_t i = 0; i < n; ++i)
@@ -560,6 +763,7 @@ int make_lattice(LatticeBody &b, const talus_mi_hex_mesh *mesh, const talus_mi_m
     }
     b.node_colour[i] = uint8_t(colour);
   }
+  for (uint32_t c = 0; c < b.hexes; ++c) b.yields = b.yields || std::isfinite(b.material_of(c).yield_stress);
   b.hex_colour.resize(b.hexes);
   for (uint32_t c = 0; c < b.hexes; ++c) b.hex_colour[c] = b.node_colour[b.hex[8 * size_t(c)]];
   // a colour's hexes are solved at once: no two of them may share a node

@@ -49,6 +49,10 @@ bool hex_make(HexElement &e, uint32_t cell, const uint32_t *nodes, const double 
               double first, double h);
 // rotationOf on the host: the quaternion iteration maximising tr(R^T F)
 void hex_rotation_of(const double *F, const double *guess, uint32_t iterations, double *out);
+// updateElement's rotation and capped strain rows at positions x (the
+// element's quaternion advances, as updateElement's does)
+void hex_strain(HexElement &e, const double *x, double *strain);
+bool hex_fit_holds(const double *shape, const double *hourglass);
 
 int build_hex_lattice(const talus_mi_lattice *lattice, const talus_mi_material *materials,
                       uint32_t lanes, const talus_mi_support *supports, uint32_t nsupports,
@@ -65,19 +69,37 @@ struct LatticeBody {
   std::vector<float> volume;      // hexes
   std::vector<uint32_t> lane;     // hexes
   std::vector<talus_mi_material> materials;
+  float voxel[3] = {};            // PreparedMesh::voxelSize (hex half extents are half of it)
   double cell = 0;                // smallest voxel side
   std::vector<uint8_t> node_colour, hex_colour;
   uint32_t substeps = 0;          // LatticeBody::substeps
   std::vector<double> x, v;       // host mirror of the state
   double h = 0;                   // substep the elements were built for (0 = none)
   std::vector<HexElement> elements;
+  // plasticity (plasticflow.cc): per Gauss point the inverse plastic
+  // deformation P (elastic = F P, row-major) and the plastic strain, empty
+  // until the first flow; the P each cell's element was built from; and per
+  // cell the rest its element was built on (volume 0 = none kept)
+  bool yields = false;            // a lane has a finite yield stress
+  std::vector<double> plastic;    // 72*hexes
+  std::vector<double> hardening;  // 8*hexes
+  std::vector<double> cached;     // 72*hexes
+  std::vector<double> rest_shape, rest_hg, rest_vol;  // 9, 12, 1 per cell
 
   const talus_mi_material &material_of(uint32_t cell_index) const;
   bool pinned(size_t node) const;
   double stiffest() const;
   uint32_t island_substeps(double dt, uint32_t softstep) const;
   // builds the elements for substep h, seeded with the lattice turn of x
+  // (restOf: the plastic rest, or the kept rest when its fit fails)
   void build_elements(double h);
+  void ensure_plastic();  // identity P, zero strain
+  // plasticRest, writeFitted (plasticflow.cc:268-340) of one cell
+  void plastic_rest(uint32_t cell, double *shape, double *hourglass, double *volume) const;
+  void write_fitted(uint32_t cell, const double *shape, const double *hourglass);
+  // refreshFlowed (strainstep.cc:371-451): rebuilds the elements of the cells
+  // whose P moved since their element was built; returns how many
+  uint32_t refresh_flowed();
 };
 
 // Fills a body from a hex mesh (makeLattice); returns a status.
@@ -100,6 +122,9 @@ class LatticeSet {
   void get_state(uint32_t i, double *x, double *v);
   void set_substeps(uint32_t i, uint32_t substeps);
   void step(const talus_mi_lattice_params &params, talus_mi_lattice_report &report);
+  // per Gauss point P and plastic strain (empty when the body never flowed)
+  void get_plastic(uint32_t i, double *plastic, double *hardening, uint32_t *present);
+  void set_plastic(uint32_t i, const double *plastic, const double *hardening);
 
  private:
   struct Device;
@@ -111,6 +136,8 @@ class LatticeSet {
   std::vector<LatticeBody> bodies_;
   std::unique_ptr<Device> dev_;
   bool device_valid_ = false;  // device arrays hold the current state
+  bool flowed_ = false;        // the last step flowed on the device: refresh before the next
+  bool refresh_pending_ = false;  // P was set from the host
 };
 
 }  // namespace tmi

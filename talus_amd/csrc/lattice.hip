@@ -45,6 +45,7 @@
 #include "device.h"
 #include "hexlattice.h"
 #include "host.h"
+#include "tmath.h"
 
 namespace tmi {
 
@@ -126,6 +127,18 @@ struct LatDev {
   // per node, its (slot * 8 + corner) entries with a free corner, in slot
   // (so colour) order: the warm start gathers them node by node
   const uint32_t *warm_start, *warm_entry;
+  // plasticity, per cell of every body (bodies one after another, cells in
+  // order, with or without an element): corner nodes, half extents, the
+  // material (shear, bulk, H / 3 shear, yield stress, H, plastic cap, crush
+  // strain, crush limit; yield infinite when the lane does not yield), and
+  // per Gauss point P (9) and the plastic strain, and a moved flag
+  uint32_t C;
+  const uint32_t *c_nodes;  // 8
+  const double *c_half;     // 3
+  const double *c_pmat;     // 8
+  double *c_plastic;        // 72
+  double *c_hard;           // 8
+  uint32_t *c_moved;        // 1
 };
 
 __device__ __forceinline__ double corner_sign(uint32_t c, int axis) {
@@ -485,6 +498,134 @@ __global__ void __launch_bounds__(NODE_BLOCK) k_lat_warm_gather(LatDev d, uint32
   vp[2] = v2;
 }
 
+// returnMap (plasticflow.cc:110-155): radial return of the Hencky strain of
+// the elastic deformation E to `yield`
+__device__ bool return_map(const double *E, double shear, double yield, double *mapped, double *trial,
+                           double *strain) {
+  double left[9], w[3], V[9];
+#pragma unroll
+  for (int i = 0; i < 3; ++i)
+#pragma unroll
+    for (int j = 0; j < 3; ++j) left[3 * i + j] = E[3 * i] * E[3 * j] + E[3 * i + 1] * E[3 * j + 1] + E[3 * i + 2] * E[3 * j + 2];
+  eigen_sym3(left, w, V);
+  if (!(w[0] > 0.0)) return false;  // ascending: w[0] is the smallest
+  double e[3], dev[3];
+#pragma unroll
+  for (int k = 0; k < 3; ++k) e[k] = 0.5 * log(w[k]);
+  const double mean = (e[0] + e[1] + e[2]) / 3.0;
+#pragma unroll
+  for (int k = 0; k < 3; ++k) dev[k] = e[k] - mean;
+  const double size = sqrt(dev[0] * dev[0] + dev[1] * dev[1] + dev[2] * dev[2]);
+  const double stress = sqrt(1.5) * 2.0 * shear * size;
+  if (!(stress > yield) || size <= 1.0e-15) return false;
+  double scale[3], corr[9];
+#pragma unroll
+  for (int k = 0; k < 3; ++k) scale[k] = exp(dev[k] * (yield / stress - 1.0));
+#pragma unroll
+  for (int i = 0; i < 3; ++i)
+#pragma unroll
+    for (int j = 0; j < 3; ++j)
+      corr[3 * i + j] = V[3 * i] * scale[0] * V[3 * j] + V[3 * i + 1] * scale[1] * V[3 * j + 1] +
+                        V[3 * i + 2] * scale[2] * V[3 * j + 2];
+  mul3(corr, E, mapped);
+  *trial = stress;
+  *strain = (stress - yield) / (3.0 * shear);
+  return true;
+}
+
+// flowLattice (plasticflow.cc:161-260) after a step: one lane per Gauss point
+// of every yielding cell (point p at (signs of corner p) / sqrt(3)); a point
+// whose P changes by more than 1e-12 (one-norm) flags its cell as moved
+__global__ void __launch_bounds__(NODE_BLOCK) k_lat_flow(LatDev d) {
+  const uint32_t t = blockIdx.x * NODE_BLOCK + threadIdx.x;
+  if (t >= 8u * d.C) return;
+  const uint32_t c = t >> 3, p = t & 7u;
+  const double *pm = d.c_pmat + size_t(c) * 8;
+  const double yield = pm[3];
+  if (!isfinite(yield)) return;
+  const double shear = pm[0], ratio = pm[2], hmod = pm[4], cap = pm[5], crush = pm[6], limit = pm[7];
+  const bool crushable = isfinite(crush) && limit < 1.0;
+  const double at[3] = {0.5773502691896258 * corner_sign(p, 0), 0.5773502691896258 * corner_sign(p, 1),
+                        0.5773502691896258 * corner_sign(p, 2)};
+  const double h0 = d.c_half[3 * size_t(c)], h1 = d.c_half[3 * size_t(c) + 1], h2 = d.c_half[3 * size_t(c) + 2];
+  double F[9] = {};
+#pragma unroll
+  for (uint32_t k = 0; k < 8; ++k) {
+    const double s0 = corner_sign(k, 0), s1 = corner_sign(k, 1), s2 = corner_sign(k, 2);
+    const double g[3] = {0.125 * s0 * (1.0 + s1 * at[1]) * (1.0 + s2 * at[2]) / h0,
+                         0.125 * s1 * (1.0 + s0 * at[0]) * (1.0 + s2 * at[2]) / h1,
+                         0.125 * s2 * (1.0 + s0 * at[0]) * (1.0 + s1 * at[1]) / h2};
+    const double *xp = d.x + 3 * size_t(d.c_nodes[8 * size_t(c) + k]);
+#pragma unroll
+    for (int i = 0; i < 3; ++i)
+#pragma unroll
+      for (int j = 0; j < 3; ++j) F[3 * i + j] += xp[i] * g[j];
+  }
+  if (!(det3(F) > 0.0)) return;
+  double *Pp = d.c_plastic + 72 * size_t(c) + 9 * p;
+  double P[9], E[9];
+#pragma unroll
+  for (int k = 0; k < 9; ++k) P[k] = Pp[k];
+  mul3(F, P, E);
+  const double hard = d.c_hard[8 * size_t(c) + p];
+  const double start = yield + hmod * hard;
+  const double room = crushable ? -log(det3(P) * limit) : 0.0;
+  const bool squeezed = crushable && room > 0.0 && det3(E) < exp(-crush);
+  double tE[9], tt = 0.0, ts = 0.0;
+  const bool trial = return_map(E, shear, start, tE, &tt, &ts);
+  const double now = trial ? (start + ratio * tt) / (1.0 + ratio) : start;
+  double mE[9], mt = 0.0, ms = 0.0;
+  const bool mapped = trial && return_map(E, shear, now, mE, &mt, &ms);
+  double after[9];
+#pragma unroll
+  for (int k = 0; k < 9; ++k) after[k] = mapped ? mE[k] : E[k];
+  const double volume = det3(after);
+  double compaction = 0.0;
+  if (squeezed && volume > 0.0) compaction = fmin(-crush - log(volume), room);
+  if (!mapped && !(compaction > 0.0)) return;
+  const double grow = compaction > 0.0 ? exp(compaction / 3.0) : 1.0;
+  double Fi[9], ga[9], next[9];
+  inv3(F, Fi);
+#pragma unroll
+  for (int k = 0; k < 9; ++k) ga[k] = grow * after[k];
+  mul3(Fi, ga, next);
+  const double kept = det3(next);
+  if (isfinite(cap) && kept > 0.0) {
+    double st[9];
+    inv3(next, st);
+    st[0] -= 1.0;
+    st[4] -= 1.0;
+    st[8] -= 1.0;
+    double size = 0.0;
+#pragma unroll
+    for (int k = 0; k < 9; ++k) size += st[k] * st[k];
+    size = sqrt(size);
+    if (size > cap) {
+      double capped[9];
+#pragma unroll
+      for (int k = 0; k < 9; ++k) capped[k] = (cap / size) * st[k];
+      capped[0] += 1.0;
+      capped[4] += 1.0;
+      capped[8] += 1.0;
+      const double cv = det3(capped);
+      if (!(cv > 1.0e-6)) return;
+      double ci[9];
+      inv3(capped, ci);
+      const double sc = cbrt(kept * cv);
+#pragma unroll
+      for (int k = 0; k < 9; ++k) next[k] = sc * ci[k];
+    }
+  }
+  double change = 0.0;
+#pragma unroll
+  for (int k = 0; k < 9; ++k) change += fabs(next[k] - P[k]);
+  if (!(change > 1.0e-12)) return;
+#pragma unroll
+  for (int k = 0; k < 9; ++k) Pp[k] = next[k];
+  d.c_hard[8 * size_t(c) + p] = hard + (mapped ? ms : 0.0);
+  d.c_moved[c] = 1u;  // every writer stores the same value
+}
+
 // solveElement for one colour's elements: one block solve of the 18 rows with
 // the volume row folded in through its Schur complement, applied to the
 // corner velocities.  One lane per row, 18 lanes per element, three
@@ -623,6 +764,8 @@ struct LatticeSet::Device {
   std::vector<uint32_t> active;                // [substep][colour]: elements taking that substep
   std::vector<uint32_t> active_total;          // [substep]: elements taking it, all colours
   uint32_t *frame_order = nullptr;             // slots by descending substeps (device)
+  std::vector<uint32_t> cell0;                 // first plasticity cell of each body
+  bool yields = false;                         // some body has a yielding lane
   std::vector<uint32_t> slot_body;             // per slot: its body
   uint32_t inverses = 0;
   hipEvent_t start = nullptr, stop = nullptr;
@@ -675,6 +818,26 @@ LatticeBody &LatticeSet::body(uint32_t i) {
 
 void LatticeSet::set_substeps(uint32_t i, uint32_t substeps) { bodies_[i].substeps = substeps; }
 
+void LatticeSet::get_plastic(uint32_t i, double *plastic, double *hardening, uint32_t *present) {
+  download();
+  const LatticeBody &b = bodies_[i];
+  if (present) *present = b.plastic.empty() ? 0u : 1u;
+  if (b.plastic.empty()) return;
+  if (plastic) std::memcpy(plastic, b.plastic.data(), b.plastic.size() * sizeof(double));
+  if (hardening) std::memcpy(hardening, b.hardening.data(), b.hardening.size() * sizeof(double));
+}
+
+void LatticeSet::set_plastic(uint32_t i, const double *plastic, const double *hardening) {
+  download();
+  LatticeBody &b = bodies_[i];
+  b.ensure_plastic();
+  if (plastic) std::memcpy(b.plastic.data(), plastic, b.plastic.size() * sizeof(double));
+  if (hardening) std::memcpy(b.hardening.data(), hardening, b.hardening.size() * sizeof(double));
+  device_valid_ = false;
+  flowed_ = false;
+  refresh_pending_ = true;  // the next step rebuilds the elements of the cells whose P moved
+}
+
 void LatticeSet::get_state(uint32_t i, double *x, double *v) {
   LatticeBody &b = bodies_[i];
   const size_t n = 3 * size_t(b.nodes);
@@ -725,6 +888,18 @@ void LatticeSet::download() {
       for (uint32_t r = 0; r < R; ++r) el.impulse[r] = imp[s * R + r];
       el.vol_impulse = vimp[s];
       for (int k = 0; k < 4; ++k) el.quat[k] = quat[s * 4 + k];
+    }
+  }
+  if (dev_->yields) {
+    std::vector<double> P(72 * size_t(d.C)), a(8 * size_t(d.C));
+    HIPCHECK(hipMemcpy(P.data(), d.c_plastic, P.size() * sizeof(double), hipMemcpyDeviceToHost));
+    HIPCHECK(hipMemcpy(a.data(), d.c_hard, a.size() * sizeof(double), hipMemcpyDeviceToHost));
+    for (size_t b = 0; b < bodies_.size(); ++b) {
+      LatticeBody &body = bodies_[b];
+      if (body.plastic.empty()) continue;
+      const size_t c0 = dev_->cell0[b];
+      std::copy(P.begin() + 72 * c0, P.begin() + 72 * (c0 + body.hexes), body.plastic.begin());
+      std::copy(a.begin() + 8 * c0, a.begin() + 8 * (c0 + body.hexes), body.hardening.begin());
     }
   }
 }
@@ -886,6 +1061,64 @@ void LatticeSet::upload(const std::vector<uint32_t> &substeps, const std::vector
   dv->frame_order = dv->copy(frame_order);
   d.warm_start = dv->copy(warm_start);
   d.warm_entry = dv->copy(warm_entry);
+  // plasticity cells: every cell of every body, bodies in order
+  {
+    uint32_t C = 0;
+    dv->cell0.resize(B);
+    for (uint32_t b = 0; b < B; ++b) {
+      dv->cell0[b] = C;
+      C += bodies_[b].hexes;
+      if (bodies_[b].yields) {
+        bodies_[b].ensure_plastic();
+        dv->yields = true;
+      }
+    }
+    d.C = dv->yields ? C : 0u;
+    if (dv->yields) {
+      std::vector<uint32_t> c_nodes(8 * size_t(C));
+      std::vector<double> c_half(3 * size_t(C)), c_pmat(8 * size_t(C)), c_plastic(72 * size_t(C)),
+          c_hard(8 * size_t(C), 0.0);
+      const double inf = std::numeric_limits<double>::infinity();
+      for (uint32_t b = 0; b < B; ++b) {
+        const LatticeBody &body = bodies_[b];
+        const size_t c0 = dv->cell0[b];
+        for (uint32_t c = 0; c < body.hexes; ++c) {
+          const size_t g = c0 + c;
+          for (int k = 0; k < 8; ++k) c_nodes[8 * g + k] = dv->node0[b] + body.hex[8 * size_t(c) + k];
+          for (int k = 0; k < 3; ++k) c_half[3 * g + k] = double(0.5f * body.voxel[k]);
+          const talus_mi_material &m = body.material_of(c);
+          double shear, first;
+          hex_material(m.youngs_modulus, m.poissons_ratio, &shear, &first);
+          const double bulk = first + 2.0 * shear / 3.0;
+          double *pm = &c_pmat[8 * g];
+          pm[0] = shear;
+          pm[1] = bulk;
+          pm[2] = double(m.hardening_modulus) / (3.0 * shear);
+          pm[3] = body.yields && std::isfinite(m.yield_stress) ? double(m.yield_stress) : inf;
+          pm[4] = double(m.hardening_modulus);
+          pm[5] = double(m.plastic_max);
+          const bool crushable = std::isfinite(m.crush_stress) && m.crush_limit < 1.0f;
+          pm[6] = crushable ? double(m.crush_stress) / bulk : inf;
+          pm[7] = double(m.crush_limit);
+          if (!body.plastic.empty()) {
+            std::copy(body.plastic.begin() + 72 * size_t(c), body.plastic.begin() + 72 * size_t(c + 1),
+                      c_plastic.begin() + 72 * g);
+            std::copy(body.hardening.begin() + 8 * size_t(c), body.hardening.begin() + 8 * size_t(c + 1),
+                      c_hard.begin() + 8 * g);
+          } else {
+            for (int q = 0; q < 8; ++q)
+              c_plastic[72 * g + 9 * q] = c_plastic[72 * g + 9 * q + 4] = c_plastic[72 * g + 9 * q + 8] = 1.0;
+          }
+        }
+      }
+      d.c_nodes = dv->copy(c_nodes);
+      d.c_half = dv->copy(c_half);
+      d.c_pmat = dv->copy(c_pmat);
+      d.c_plastic = dv->copy(c_plastic);
+      d.c_hard = dv->copy(c_hard);
+      d.c_moved = dv->copy(std::vector<uint32_t>(C, 0u));
+    }
+  }
   HIPCHECK(hipEventCreate(&dv->start));
   HIPCHECK(hipEventCreate(&dv->stop));
   // the bodies' substep counts and exact flags, per body and per slot
@@ -923,6 +1156,24 @@ void LatticeSet::step(const talus_mi_lattice_params &p, talus_mi_lattice_report 
     exact[b] = !(h[b] * bodies_[b].stiffest() > 1.5) ? 1 : 0;
     if (bodies_[b].h != h[b]) rebuild = true;
   }
+  // refreshFlowed: the cells the last step's flow moved get their elements
+  // rebuilt on the host (at the positions the step starts from)
+  bool refresh = refresh_pending_;
+  if (flowed_ && device_valid_ && dev_->d.C) {
+    std::vector<uint32_t> moved(dev_->d.C);
+    HIPCHECK(hipStreamSynchronize(stream_));
+    HIPCHECK(hipMemcpy(moved.data(), dev_->d.c_moved, moved.size() * sizeof(uint32_t), hipMemcpyDeviceToHost));
+    for (uint32_t m : moved) refresh = refresh || m != 0u;
+    if (refresh) download();
+  }
+  if (refresh) {
+    for (uint32_t b = 0; b < B; ++b)  // a body whose substep changes is rebuilt whole below
+      if (bodies_[b].h == h[b]) rep.refreshed += bodies_[b].refresh_flowed();
+    device_valid_ = false;  // laid out again with the new elements and P
+  }
+  flowed_ = false;
+  refresh_pending_ = false;
+
   if (rebuild) {
     download();
     for (uint32_t b = 0; b < B; ++b)
@@ -1044,6 +1295,13 @@ void LatticeSet::step(const talus_mi_lattice_params &p, talus_mi_lattice_report 
       dv.graph_inexact = any_inexact;
     }
     HIPCHECK(hipGraphLaunch(dv.graph, stream_));
+  }
+  // flowLattice once per step, after the substeps
+  if (d.C) {
+    hipLaunchKernelGGL(k_lat_flow, dim3((8 * d.C + NODE_BLOCK - 1) / NODE_BLOCK), dim3(NODE_BLOCK), 0, stream_,
+                       d);
+    HIPCHECK(hipGetLastError());
+    flowed_ = true;
   }
   HIPCHECK(hipEventRecord(dv.stop, stream_));
   HIPCHECK(hipEventSynchronize(dv.stop));

@@ -188,7 +188,7 @@ def test_state_round_trips_and_unsupported_materials_are_refused():
         ctx.set_lattice_state(b, x1 + 0.5, v1)
         x2, _ = ctx.lattice_state(b)
         np.testing.assert_array_equal(x2, x1 + 0.5)
-        for bad in (dict(damping_ratio=0.5), dict(yield_stress=1e4), dict(tear_strain=0.3)):
+        for bad in (dict(damping_ratio=0.5), dict(tear_strain=0.3)):
             m = material(2.0e5)
             for k, val in bad.items():
                 setattr(m, k, val)
@@ -196,3 +196,93 @@ def test_state_round_trips_and_unsupported_materials_are_refused():
                 ctx.add_lattice(mesh, [m])
         with pytest.raises(api.TalusError):
             ctx.add_lattice(mesh, [])
+
+
+# ---- plasticity (flowLattice, refreshFlowed) --------------------------------
+
+def clay_block(E=2.0e5, yield_stress=1.5e3, cells=(8, 2, 2), cell=0.1, hardening=0.0, cap=1.0):
+    mat = material(E, poisson=0.35)
+    mat.yield_stress, mat.hardening_modulus, mat.plastic_max = yield_stress, hardening, cap
+    half = [c * cell / 2 for c in cells]
+    clamp = [api.Support.box((-half[0], 0.0, 0.0), (0.01, 1.0, 1.0))]
+    return box(half, cell, mat, clamp, lanes=1), mat
+
+
+@pytest.mark.parametrize("hardening", [0.0, 5.0e4])
+def test_plastic_sag_matches_the_oracle(hardening):
+    """A clay beam held at one end sags past its yield: positions, velocities,
+    impulses, per-point P and plastic strain follow the oracle's colour sweep,
+    through the flow after each step and the element rebuilds before the next."""
+    mesh, mat = clay_block(hardening=hardening)
+    orc = H.LatticeBody(orc_mesh(mesh), [orc_material(mat)])
+    steps = 40
+    with api.Context() as ctx:
+        b = ctx.add_lattice(mesh, [mat])
+        refreshed = 0
+        for _ in range(steps):
+            r = ctx.lattice_step()
+            refreshed += r["refreshed"]
+            orc.step(colored=True)
+        x, v = ctx.lattice_state(b)
+        cells, imp = ctx.lattice_impulses(b)
+        P, a = ctx.lattice_plastic(b)
+    ox, ov = orc.state()
+    oP, oa = orc.plastic()
+    _, oimp = orc.impulses()
+    assert a.sum() > 0.0 and refreshed > 0  # it flowed and elements were rebuilt
+    motion = np.abs(ox - mesh["rest_positions"]).max()
+    assert rel(x, ox, motion) <= 1e-8
+    assert rel(v, ov, np.abs(ov).max()) <= 1e-8
+    assert rel(imp, oimp, np.abs(oimp).max()) <= 1e-8
+    assert np.abs(P - oP).max() <= 1e-9
+    assert rel(a, oa, oa.max()) <= 1e-8
+
+
+def test_plastic_state_round_trips_and_drives_a_refresh():
+    mesh, mat = clay_block()
+    rng = np.random.default_rng(3)
+    with api.Context() as ctx:
+        b = ctx.add_lattice(mesh, [mat])
+        assert ctx.lattice_plastic(b) is None
+        ctx.lattice_step()
+        P, a = ctx.lattice_plastic(b)
+        P2 = P + 0.01 * rng.standard_normal(P.shape)
+        ctx.set_lattice_plastic(b, P2, a + 0.5)
+        Q, a2 = ctx.lattice_plastic(b)
+        np.testing.assert_array_equal(Q, P2)
+        np.testing.assert_array_equal(a2, a + 0.5)
+        assert ctx.lattice_step()["refreshed"] > 0
+
+
+def test_strip_held_bent_past_yield_stops_flowing_on_the_gpu():
+    """dynamics_strainstep_tests.cc:3329-3359 (the strip with the default
+    damping ratio), GPU against the oracle's plastic strain"""
+    mat = material(1.0e5, density=10.0, poisson=0.3, friction=0.5)
+    mat.yield_stress, mat.plastic_max = 1.0e5 * 0.002, 1.0
+    ends = [api.Support.box((-1.2, 0.0, 0.0), (0.01, 1.0, 1.0)), api.Support.box((1.2, 0.0, 0.0), (0.01, 1.0, 1.0))]
+    mesh = box((1.2, 0.1, 0.1), 0.2, mat, ends, lanes=1)
+    tip = np.nonzero(mesh["rest_positions"][:, 0] > 1.19)[0]
+    orc = H.LatticeBody(orc_mesh(mesh), [orc_material(mat)], offset=(0.0, 2.0, 0.0))
+    with api.Context() as ctx:
+        b = ctx.add_lattice(mesh, [mat], offset=(0.0, 2.0, 0.0))
+        y0 = ctx.lattice_state(b)[0][tip, 1].copy()
+        held = None
+        for frame in range(1, 121):
+            x, v = ctx.lattice_state(b)
+            x[tip, 1] = y0 - 0.5 * min(frame / 60.0, 1.0)
+            ctx.set_lattice_state(b, x, v)
+            ox, ov = orc.state()
+            ox[tip, 1] = y0 - 0.5 * min(frame / 60.0, 1.0)
+            orc.set_state(ox, ov)
+            ctx.lattice_step(gravity=(0.0, 0.0, 0.0))
+            orc.step(gravity=(0.0, 0.0, 0.0), colored=True)
+            if frame == 90:
+                held = ctx.lattice_plastic(b)[1].sum() / 8.0
+        final = ctx.lattice_plastic(b)[1].sum() / 8.0
+        x, _ = ctx.lattice_state(b)
+    assert held > 0.0
+    assert final - held < 0.01 * held
+    oa = orc.plastic()[1].sum() / 8.0
+    assert abs(final - oa) <= 1e-6 * oa
+    ox, _ = orc.state()
+    assert np.abs(x - ox).max() <= 1e-6

@@ -350,3 +350,175 @@ def test_colour_order_and_cell_order_land_together():
     sag = np.abs(xa - mesh["rest"]).max()
     assert sag > 1e-3
     assert np.abs(xa - xb).max() <= 0.02 * sag
+
+
+# ---- lattice plasticity (plasticflow.cc) -----------------------------------
+
+def von_mises(elastic, shear):
+    """vonMises (plasticflow.cc:96-108) from numpy's eigenvalues of F F^T."""
+    w = np.linalg.eigvalsh(elastic @ elastic.T)
+    e = 0.5 * np.log(w)
+    return math.sqrt(1.5) * 2.0 * shear * np.linalg.norm(e - e.mean())
+
+
+def clay(E, yield_stress, **kw):
+    m = jelly(E, density=1000.0, poisson=0.35)
+    m.yield_stress, m.hardening_modulus, m.plastic_max = yield_stress, 0.0, 1.0
+    for k, v in kw.items():
+        setattr(m, k, v)
+    return m
+
+
+def one_cell(mat, half=0.1, voxel=0.2):
+    mesh = box((half, half, half), voxel, mat, lanes=1)
+    assert mesh["hex"].shape[0] == 1
+    return mesh, H.LatticeBody(mesh, [mat])
+
+
+def sheared_cell(mat):
+    mesh, body = one_cell(mat)
+    rest = mesh["rest"]
+    body.set_state(rest + np.stack([0.2 * rest[:, 1], np.zeros(len(rest)), np.zeros(len(rest))], axis=1))
+    return mesh, body
+
+
+def test_shape_gradients_sum_to_zero_and_reproduce_linear_fields():
+    """trilinear gradients: a linear field x -> A x has gradient A at every point"""
+    half = np.array([0.1, 0.2, 0.15])
+    corners = np.array([half * H.corner_signs(c) for c in range(8)])
+    A = np.array([[1.0, 0.2, -0.3], [0.4, 0.9, 0.1], [0.0, 0.5, 1.2]])
+    for p in range(8):
+        g = H.shape_gradients(0.5773502691896258 * H.corner_signs(p), half)
+        assert np.abs(g.sum(axis=0)).max() <= 1e-14
+        F = (corners @ A.T).T @ g
+        np.testing.assert_allclose(F, A, atol=1e-13)
+
+
+def test_point_inside_its_yield_surface_stays_elastic():
+    """dynamics_strainstep_tests.cc:3090-3094"""
+    E = np.array([[1.0, 0.001, 0.0], [0.0, 1.0, 0.0], [0.0, 0.0, 1.0]])  # columns (1,0,0), (0.001,1,0), (0,0,1)
+    assert H.return_map(E, 1.0e5, 1.0e3) is None
+
+
+def test_point_past_yield_returns_to_its_surface_and_keeps_volume():
+    """dynamics_strainstep_tests.cc:3096-3104"""
+    E = np.array([[1.05, 0.02, 0.0], [0.2, 0.98, 0.01], [0.0, 0.03, 1.01]]).T
+    mapped, trial, strain = H.return_map(E, 1.0e5, 2.0e3)
+    assert abs(von_mises(mapped, 1.0e5) - 2.0e3) <= 1e-9 * 2.0e3
+    assert abs(np.linalg.det(mapped) - np.linalg.det(E)) <= 1e-12
+    assert abs(strain - (trial - 2.0e3) / 3.0e5) <= 1e-12
+
+
+def test_hardening_follows_the_hardening_modulus():
+    """dynamics_strainstep_tests.cc:3106-3138"""
+    mat = clay(1.0e5, 2.0e3, hardening_modulus=5.0e4)
+    mesh, body = sheared_cell(mat)
+    body.flow()
+    P, a = body.plastic()
+    shear, _ = H.material(np.float32(1e5), np.float32(0.35))
+    half = 0.5 * mesh["voxel"].astype(np.float64)
+    g = H.shape_gradients(0.5773502691896258 * H.corner_signs(0), half)
+    x, _ = body.state()
+    F = x[mesh["hex"][0]].T @ g
+    y = 2.0e3 + 5.0e4 * a[0]
+    assert a[0] > 0.0
+    assert abs(von_mises(F @ P[0], shear) - y) <= 1e-6 * y
+
+
+def test_plastic_stretch_stays_within_the_cap():
+    """dynamics_strainstep_tests.cc:3140-3167"""
+    mat = clay(5.0e3, 50.0)
+    mesh, body = one_cell(mat, half=0.25, voxel=0.5)
+    rest = mesh["rest"]
+    for pull in range(1, 7):
+        s = 1.0 + pull
+        body.set_state(np.stack([rest[:, 0] * s, rest[:, 1] / s, rest[:, 2]], axis=1))
+        body.flow()
+    P, _ = body.plastic()
+    largest = max(np.linalg.norm(np.linalg.inv(p) - np.eye(3)) for p in P)
+    assert largest > 0.5
+    assert largest <= 1.0 + 1e-9
+
+
+def test_crush_stops_at_the_crush_limit():
+    """dynamics_strainstep_tests.cc:3169-3196"""
+    mat = clay(1.0e5, 1.0e9, crush_stress=1.0e3, crush_limit=0.7)
+    mesh, body = one_cell(mat)
+    for _ in range(10):
+        body.set_state(0.6 * mesh["rest"])
+        body.flow()
+    P, _ = body.plastic()
+    smallest = min(np.linalg.det(np.linalg.inv(p)) for p in P)
+    assert smallest < 0.99
+    assert smallest >= float(np.float32(0.7)) - 1e-9
+
+
+def test_rest_fit_of_a_uniform_stretch_writes_the_same_stretch_back():
+    """dynamics_strainstep_tests.cc:3198-3223"""
+    mesh, body = one_cell(clay(1.0e5, 2.0e3))
+    inv = np.linalg.inv(np.array([[1.1, 0.05, 0.0], [0.0, 0.9, 0.02], [0.01, 0.0, 1.0]]).T)
+    body.set_plastic(np.tile(inv, (8, 1, 1)), np.zeros(8))
+    shape, hg, _ = body.plastic_rest(0)
+    body.write_fitted(0, shape, hg)
+    P, _ = body.plastic()
+    assert max(np.linalg.norm(p - inv) for p in P) <= 1e-12
+    assert np.linalg.norm(hg, axis=1).max() <= 1e-12
+
+
+def test_rest_fit_keeps_the_plastic_volume_and_is_idempotent():
+    """dynamics_strainstep_tests.cc:3225-3263"""
+    mesh, body = one_cell(clay(1.0e5, 2.0e3))
+    P = []
+    for p in range(8):
+        side = H.corner_signs(p)[2]
+        P.append(np.linalg.inv(np.diag([math.exp(0.2 * side), math.exp(-0.2 * side), 1.0])))
+    body.set_plastic(np.array(P), np.zeros(8))
+    body.write_fitted(0, *body.plastic_rest(0)[:2])
+    W, _ = body.plastic()
+    assert abs(np.mean([np.linalg.det(np.linalg.inv(p)) for p in W]) - 1.0) <= 1e-12
+    for _ in range(5):
+        body.write_fitted(0, *body.plastic_rest(0)[:2])
+    W2, _ = body.plastic()
+    assert max(np.linalg.norm(a - b) for a, b in zip(W, W2)) <= 1e-12
+
+
+def test_resting_plastic_state_is_untouched_when_nothing_moves():
+    """dynamics_strainstep_tests.cc:3265-3285"""
+    _, body = sheared_cell(clay(1.0e5, 2.0e3))
+    assert body.flow() > 0.0
+    P0, a0 = body.plastic()
+    assert body.flow() <= 1e-12
+    P1, a1 = body.plastic()
+    np.testing.assert_array_equal(P0, P1)
+    np.testing.assert_array_equal(a0, a1)
+
+
+def bent_strip(release_after=None, colored=False, frames=120):
+    """A strip one cell thick, both ends held, its tip end driven down 0.5 m
+    over a second (stripShape / 3329-3397, with the default damping ratio:
+    the reference's strip uses damping ratio 1, whose lowest-mode damping is
+    not restated)."""
+    mat = jelly(1.0e5, density=10.0, poisson=0.3)
+    mat.yield_stress, mat.plastic_max = 1.0e5 * 0.002, 1.0
+    ends = [hold_at((-1.2, 0.0, 0.0), (0.01, 1.0, 1.0)), hold_at((1.2, 0.0, 0.0), (0.01, 1.0, 1.0))]
+    mesh = box((1.2, 0.1, 0.1), 0.2, mat, ends, lanes=1)
+    body = H.LatticeBody(mesh, [mat], offset=(0.0, 2.0, 0.0))
+    tip = np.nonzero(mesh["rest"][:, 0] > 1.19)[0]
+    y0 = body.state()[0][tip, 1].copy()
+    held = None
+    for frame in range(1, frames + 1):
+        x, v = body.state()
+        x[tip, 1] = y0 - 0.5 * min(frame / 60.0, 1.0)
+        body.set_state(x, v)
+        body.step(gravity=(0.0, 0.0, 0.0), colored=colored)
+        if frame == 90:
+            held = body.plastic()[1].sum() / 8.0
+    return body, mesh, tip, y0, held
+
+
+def test_strip_held_bent_past_yield_stops_flowing():
+    """dynamics_strainstep_tests.cc:3329-3359"""
+    body, _, _, _, held = bent_strip()
+    final = body.plastic()[1].sum() / 8.0
+    assert held > 0.0
+    assert final - held < 0.01 * held
