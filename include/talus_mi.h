@@ -450,6 +450,8 @@ typedef struct talus_mi_lattice_report {
   uint32_t refreshed;       /* elements rebuilt for plastic flow (refreshFlowed) */
   double solve_ms;          /* their summed HIP-event time (KERNEL_TIMING) */
   double update_ms;
+  uint32_t torn;            /* cells found overstretched at the step's end */
+  uint32_t node_copies;     /* nodes split off by the tears they caused */
 } talus_mi_lattice_report;
 
 void talus_mi_lattice_default_params(talus_mi_lattice_params *params);
@@ -459,9 +461,10 @@ void talus_mi_lattice_default_params(talus_mi_lattice_params *params);
  * cell size = the smallest voxel side.  The strainstep path needs material
  * lanes (strainstep.cc:86-94: lanes past the array read the last one).
  * Lanes with a finite yield stress flow plastically (flowLattice after each
- * step, refreshFlowed before the next).  Damping ratios and tear strains must
- * be infinite (the lowest-mode damping and cell tearing are not in this
- * library): TALUS_MI_ERR_ARGUMENT otherwise. */
+ * step, refreshFlowed before the next).  Lanes with a finite tear strain
+ * tear (overstretched / tearCells after each step).  Damping ratios must be
+ * infinite (the lowest-mode damping is not in this library):
+ * TALUS_MI_ERR_ARGUMENT otherwise. */
 int talus_mi_lattice_add(talus_mi_ctx *ctx, const talus_mi_hex_mesh *mesh,
                          const talus_mi_material *materials, uint32_t lanes,
                          const double *offset, const double *velocity, uint32_t *lattice);
@@ -490,11 +493,34 @@ int talus_mi_lattice_plastic_set(talus_mi_ctx *ctx, uint32_t lattice, const doub
  * one colour share no node, so each colour's elements are solved together. */
 int talus_mi_lattice_colours_get(talus_mi_ctx *ctx, uint32_t lattice, uint8_t *node_colour,
                                  uint8_t *hex_colour);
+/* The body's current topology: *nodes = its node count (tearing adds node
+ * copies), and, each when not NULL, the hex corner nodes (8 per hex), the node
+ * masses and per hex its crack-face bits (bit 2 axis + side: the face whose
+ * grid neighbour's facing face no longer holds the same nodes; zero before
+ * the first tear).  PreparedMesh::hexes / nodeMass / crackFaces after
+ * tearCells (src/talus/dynamics/strainstep/tearing.cc:426-484). */
+int talus_mi_lattice_topology_get(talus_mi_ctx *ctx, uint32_t lattice, uint32_t *nodes,
+                                  uint32_t *hex_nodes, float *node_mass, uint8_t *crack_faces);
+/* tearCells (tearing.cc:505-536) at the body's current positions: a crack
+ * through each listed cell, the pinched nodes split, node masses recomputed,
+ * crack faces marked; *copies = the node copies made.  A torn body's
+ * elements are built again at its next step, keeping each cell's impulses. */
+int talus_mi_lattice_tear(talus_mi_ctx *ctx, uint32_t lattice, const uint32_t *cells,
+                          uint32_t count, uint32_t *copies);
+/* cellStretch (tearing.cc:279-281): the largest principal stretch of the
+ * cell's eight corner frames and its direction (sign: the largest component
+ * positive). */
+int talus_mi_lattice_cell_stretch(talus_mi_ctx *ctx, uint32_t lattice, uint32_t cell,
+                                  double *stretch, double *direction);
+/* latticePieces (resolution.cc:198-228): pieces of cells joined by nodes. */
+int talus_mi_lattice_pieces(talus_mi_ctx *ctx, uint32_t lattice, uint32_t *pieces);
 /* One solveStrainstep over every lattice of the context, each its own
  * island (no contacts, joints or rigid partners): refreshFlowed of the cells
  * the last step's flow moved, per body islandSubsteps (strainstep.cc:551-567)
  * substeps of integrate, element update and warm-start, 4 Gauss-Seidel passes
- * of solveElement, advance, then flowLattice.  The passes
+ * of solveElement, advance, then flowLattice, and overstretched / tearCells
+ * for lattices with a tearing lane (a torn body's elements are rebuilt at its
+ * next step with each cell's impulses kept).  The passes
  * sweep the hexes colour by colour (all bodies' hexes of one colour in one
  * launch), where the reference sweeps them in cell order. */
 int talus_mi_lattice_step(talus_mi_ctx *ctx, const talus_mi_lattice_params *params,

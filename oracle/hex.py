@@ -103,6 +103,17 @@ def lib():
         L.orc_lattice_flow.restype = C.c_double
         L.orc_lattice_plastic_rest.argtypes = [C.c_void_p, C.c_uint32, dp, dp, dp]
         L.orc_lattice_write_fitted.argtypes = [C.c_void_p, C.c_uint32, dp, dp]
+        u32p, u8p = C.POINTER(C.c_uint32), C.POINTER(C.c_uint8)
+        L.orc_lattice_cell_stretch.argtypes = [C.c_void_p, C.c_uint32, dp, dp]
+        L.orc_lattice_overstretched.argtypes = [C.c_void_p, u32p]
+        L.orc_lattice_overstretched.restype = C.c_uint32
+        L.orc_lattice_tear.argtypes = [C.c_void_p, u32p, C.c_uint32]
+        L.orc_lattice_tear.restype = C.c_uint32
+        L.orc_lattice_pieces.argtypes = [C.c_void_p]
+        L.orc_lattice_pieces.restype = C.c_uint32
+        L.orc_lattice_topology.argtypes = [C.c_void_p, u32p, C.POINTER(C.c_float), u8p]
+        L.orc_lattice_topology.restype = C.c_uint32
+        L.orc_sym_eigen3.argtypes = [dp, dp, dp]
         _ready = True
     return L
 
@@ -302,8 +313,12 @@ class LatticeBody:
         self._h = L.orc_lattice_create(C.byref(m), mats, len(materials), _dp(off), _dp(vel))
         if not self._h:
             raise ValueError("a lattice needs its material lanes")
-        self.nodes = int(m.nodes)
         self.hexes = int(m.hexes)
+
+    @property
+    def nodes(self):
+        """the current node count (tearing adds node copies)"""
+        return int(lib().orc_lattice_topology(self._h, None, None, None))
 
     def __del__(self):
         if getattr(self, "_h", None):
@@ -375,3 +390,39 @@ class LatticeBody:
     def write_fitted(self, cell, shape, hourglass):
         sh, hg = _f64(shape, 9), _f64(hourglass, 12)
         lib().orc_lattice_write_fitted(self._h, cell, _dp(sh), _dp(hg))
+
+    def cell_stretch(self, cell):
+        """(stretch, direction) of cellStretch"""
+        s, d = C.c_double(), np.zeros(3)
+        lib().orc_lattice_cell_stretch(self._h, int(cell), C.byref(s), _dp(d))
+        return s.value, d
+
+    def overstretched(self):
+        out = np.zeros(max(self.hexes, 1), dtype=np.uint32)
+        n = lib().orc_lattice_overstretched(self._h, out.ctypes.data_as(C.POINTER(C.c_uint32)))
+        return out[:n]
+
+    def tear(self, cells):
+        """tearCells: the node copies made"""
+        c = np.ascontiguousarray(cells, dtype=np.uint32)
+        return int(lib().orc_lattice_tear(self._h, c.ctypes.data_as(C.POINTER(C.c_uint32)), c.size))
+
+    def pieces(self):
+        return int(lib().orc_lattice_pieces(self._h))
+
+    def topology(self):
+        """(hex nodes [hexes, 8], node masses, crack-face bits per cell)"""
+        hx = np.zeros(8 * self.hexes, dtype=np.uint32)
+        mass = np.zeros(self.nodes, dtype=np.float32)
+        crack = np.zeros(self.hexes, dtype=np.uint8)
+        lib().orc_lattice_topology(self._h, hx.ctypes.data_as(C.POINTER(C.c_uint32)),
+                                   mass.ctypes.data_as(C.POINTER(C.c_float)),
+                                   crack.ctypes.data_as(C.POINTER(C.c_uint8)))
+        return hx.reshape(-1, 8), mass, crack
+
+
+def sym_eigen3(A):
+    """(eigenvalues, eigenvector columns) by the oracle's cyclic Jacobi"""
+    w, V = np.zeros(3), np.zeros(9)
+    lib().orc_sym_eigen3(_dp(_f64(A, 9)), _dp(w), _dp(V))
+    return w, V.reshape(3, 3)

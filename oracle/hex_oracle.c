@@ -654,6 +654,10 @@ struct orc_lattice_body {
    * were last built from, and per cell the rest the element was built on */
   double *plastic, *hardening, *cached;
   double *rest_shape, *rest_hg, *rest_vol; /* 9, 12, 1 per cell; volume 0 = none */
+  /* tearing: per-cell crack-face bits (NULL before the first tear), and the
+   * mesh changed since the elements were built */
+  uint8_t *crack;
+  int retopo;
 };
 
 orc_lattice_body *orc_lattice_create(const orc_hex_mesh *mesh, const orc_material *materials,
@@ -729,6 +733,7 @@ void orc_lattice_free(orc_lattice_body *b) {
   free(b->rest_shape);
   free(b->rest_hg);
   free(b->rest_vol);
+  free(b->crack);
   free(b);
 }
 
@@ -1171,14 +1176,504 @@ static void build_elements(orc_lattice_body *b, double h) {
   b->h = h;
 }
 
+/* ---- cell tearing (strainstep/tearing.cc) -------------------------------- */
+
+#define MIN_PIECE_CELLS 2u /* tearing.h:13 */
+
+/* FACES (tearing.cc:27-34): the corners of face 2 axis + side */
+static const uint32_t FACES[6][4] = {{0, 2, 4, 6}, {1, 3, 5, 7}, {0, 1, 4, 5},
+                                     {2, 3, 6, 7}, {0, 1, 2, 3}, {4, 5, 6, 7}};
+
+void orc_sym_eigen3(const double *A, double *w, double *V) {
+  double a[3][3];
+  for (int r = 0; r < 3; ++r)
+    for (int c = 0; c < 3; ++c) {
+      a[r][c] = A[3 * r + c];
+      V[3 * r + c] = r == c ? 1.0 : 0.0;
+    }
+  static const int P[3] = {0, 0, 1}, Q[3] = {1, 2, 2};
+  for (int sweep = 0; sweep < 32; ++sweep) {
+    const double off = a[0][1] * a[0][1] + a[0][2] * a[0][2] + a[1][2] * a[1][2];
+    const double diag = a[0][0] * a[0][0] + a[1][1] * a[1][1] + a[2][2] * a[2][2];
+    if (!(off > 1.0e-40 * diag)) break;
+    for (int k = 0; k < 3; ++k) {
+      const int p = P[k], q = Q[k];
+      const double apq = a[p][q];
+      if (apq == 0.0) continue;
+      const double theta = (a[q][q] - a[p][p]) / (2.0 * apq);
+      const double t = fabs(theta) > 1.0e150 ? 0.5 / theta
+                                             : (theta < 0.0 ? -1.0 : 1.0) / (fabs(theta) + sqrt(theta * theta + 1.0));
+      const double c = 1.0 / sqrt(t * t + 1.0), s = t * c;
+      const int r = 3 - p - q;
+      const double arp = a[r][p], arq = a[r][q];
+      a[p][p] -= t * apq;
+      a[q][q] += t * apq;
+      a[p][q] = a[q][p] = 0.0;
+      a[r][p] = a[p][r] = c * arp - s * arq;
+      a[r][q] = a[q][r] = s * arp + c * arq;
+      for (int i = 0; i < 3; ++i) {
+        const double vp = V[3 * i + p], vq = V[3 * i + q];
+        V[3 * i + p] = c * vp - s * vq;
+        V[3 * i + q] = s * vp + c * vq;
+      }
+    }
+  }
+  for (int k = 0; k < 3; ++k) w[k] = a[k][k];
+}
+
+static void rest_of(const orc_lattice_body *b, uint32_t node, double *r) {
+  for (int k = 0; k < 3; ++k) r[k] = b->rest[3 * (size_t)node + k];
+}
+
+/* stretchOf (tearing.cc:258-300): per corner the frame of its three edges,
+ * F = now rest^-1, and the largest eigenvalue of F^T F */
+void orc_lattice_cell_stretch(const orc_lattice_body *b, uint32_t cell, double *stretch, double *direction) {
+  const uint32_t *nd = b->hex + 8 * (size_t)cell;
+  double best = 0.0, dir[3] = {1.0, 0.0, 0.0};
+  for (uint32_t corner = 0; corner < 8; ++corner) {
+    const uint32_t base = nd[corner];
+    double R[9], N[9]; /* row-major, edge e in column e */
+    for (uint32_t e = 0; e < 3; ++e) {
+      const uint32_t other = nd[corner ^ (1u << e)];
+      for (int k = 0; k < 3; ++k) {
+        R[3 * k + e] = b->rest[3 * (size_t)other + k] - b->rest[3 * (size_t)base + k];
+        N[3 * k + e] = b->x[3 * (size_t)other + k] - b->x[3 * (size_t)base + k];
+      }
+    }
+    const double det = R[0] * (R[4] * R[8] - R[5] * R[7]) - R[1] * (R[3] * R[8] - R[5] * R[6]) +
+                       R[2] * (R[3] * R[7] - R[4] * R[6]);
+    if (!(fabs(det) > 0.0)) continue;
+    double Ri[9], F[9], C[9], w[3], V[9];
+    inverse3(R, Ri);
+    mm3(N, Ri, F);
+    for (int r = 0; r < 3; ++r)
+      for (int c = 0; c < 3; ++c) C[3 * r + c] = F[r] * F[c] + F[3 + r] * F[3 + c] + F[6 + r] * F[6 + c];
+    orc_sym_eigen3(C, w, V);
+    int top = 0;
+    for (int k = 1; k < 3; ++k)
+      if (w[k] > w[top]) top = k;
+    const double s = sqrt(w[top] > 0.0 ? w[top] : 0.0);
+    if (s > best) {
+      double v[3] = {V[top], V[3 + top], V[6 + top]};
+      const double len = sqrt(dot3(v, v));
+      int big = 0;
+      for (int k = 1; k < 3; ++k)
+        if (fabs(v[k]) > fabs(v[big])) big = k;
+      const double sign = v[big] < 0.0 ? -1.0 : 1.0;
+      for (int k = 0; k < 3; ++k) dir[k] = sign * v[k] / len;
+      best = s;
+    }
+  }
+  *stretch = best;
+  memcpy(direction, dir, sizeof dir);
+}
+
+uint32_t orc_lattice_overstretched(const orc_lattice_body *b, uint32_t *cells) {
+  uint32_t n = 0;
+  for (uint32_t c = 0; c < b->hexes; ++c) {
+    const double limit = 1.0 + (double)lane_mat(b, c)->tear_strain;
+    double s, d[3];
+    if (!isfinite(limit)) continue;
+    orc_lattice_cell_stretch(b, c, &s, d);
+    if (s > limit) cells[n++] = c;
+  }
+  return n;
+}
+
+static void face_of(const orc_lattice_body *b, uint32_t cell, uint32_t face, uint32_t *out) {
+  for (int k = 0; k < 4; ++k) out[k] = b->hex[8 * (size_t)cell + FACES[face][k]];
+  for (int i = 1; i < 4; ++i) /* sorted */
+    for (int j = i; j > 0 && out[j] < out[j - 1]; --j) {
+      const uint32_t t = out[j];
+      out[j] = out[j - 1];
+      out[j - 1] = t;
+    }
+}
+
+static int face_holds(const uint32_t *f, uint32_t node) {
+  return f[0] == node || f[1] == node || f[2] == node || f[3] == node;
+}
+
+static int hex_has(const orc_lattice_body *b, uint32_t cell, uint32_t node) {
+  for (int k = 0; k < 8; ++k)
+    if (b->hex[8 * (size_t)cell + k] == node) return 1;
+  return 0;
+}
+
+/* cellsAt (tearing.cc:58-70): the cells holding node, ascending */
+static uint32_t cells_at(const orc_lattice_body *b, uint32_t node, uint32_t *out) {
+  uint32_t n = 0;
+  for (uint32_t c = 0; c < b->hexes; ++c)
+    if (hex_has(b, c, node)) out[n++] = c;
+  return n;
+}
+
+static uint32_t shared_nodes(const orc_lattice_body *b, uint32_t x, uint32_t y) {
+  uint32_t n = 0;
+  for (int k = 0; k < 8; ++k) n += hex_has(b, y, b->hex[8 * (size_t)x + k]) ? 1u : 0u;
+  return n;
+}
+
+static uint32_t root_of(uint32_t *parent, uint32_t at) {
+  while (parent[at] != at) {
+    parent[at] = parent[parent[at]];
+    at = parent[at];
+  }
+  return at;
+}
+
+/* groupsOf (tearing.cc:92-137): cells of the fan on one side joined through
+ * shared faces (four or more shared nodes); groups numbered side 0 first,
+ * each side in fan order */
+static uint32_t groups_of(const orc_lattice_body *b, const uint32_t *fan, uint32_t n, const uint8_t *sides,
+                          uint32_t *group) {
+  uint32_t *parent = (uint32_t *)malloc((n ? n : 1) * sizeof(uint32_t));
+  uint32_t *number = (uint32_t *)malloc((n ? n : 1) * sizeof(uint32_t));
+  for (uint32_t i = 0; i < n; ++i) {
+    parent[i] = i;
+    number[i] = UINT32_MAX;
+  }
+  for (uint32_t i = 0; i < n; ++i)
+    for (uint32_t j = i + 1; j < n; ++j)
+      if (sides[i] == sides[j] && shared_nodes(b, fan[i], fan[j]) >= 4) {
+        const uint32_t x = root_of(parent, i), y = root_of(parent, j);
+        if (x != y) parent[x > y ? x : y] = x < y ? x : y;
+      }
+  uint32_t count = 0;
+  for (uint8_t side = 0; side < 2; ++side)
+    for (uint32_t i = 0; i < n; ++i) {
+      if (sides[i] != side) continue;
+      const uint32_t r = root_of(parent, i);
+      if (number[r] == UINT32_MAX) number[r] = count++;
+      group[i] = number[r];
+    }
+  free(parent);
+  free(number);
+  return count;
+}
+
+/* reachable (tearing.cc:139-180): how many cells the seeds reach through
+ * faces that do not hold the pivot, stopping at limit */
+static size_t reachable(const orc_lattice_body *b, const uint32_t *seeds, uint32_t nseeds, uint32_t pivot,
+                        size_t limit) {
+  uint8_t *seen = (uint8_t *)calloc(b->hexes ? b->hexes : 1, 1);
+  uint32_t *stack = (uint32_t *)malloc((b->hexes ? b->hexes : 1) * sizeof(uint32_t));
+  size_t count = 0, top = 0;
+  for (uint32_t i = 0; i < nseeds; ++i)
+    if (!seen[seeds[i]]) {
+      seen[seeds[i]] = 1;
+      ++count;
+      stack[top++] = seeds[i];
+    }
+  while (top > 0 && count < limit) {
+    const uint32_t cell = stack[--top];
+    for (uint32_t face = 0; face < 6; ++face) {
+      uint32_t f[4];
+      face_of(b, cell, face, f);
+      if (face_holds(f, pivot)) continue;
+      for (uint32_t other = 0; other < b->hexes; ++other) {
+        if (seen[other]) continue;
+        for (uint32_t g = 0; g < 6; ++g) {
+          uint32_t o[4];
+          face_of(b, other, g, o);
+          if (!memcmp(o, f, sizeof o)) {
+            seen[other] = 1;
+            ++count;
+            stack[top++] = other;
+            break;
+          }
+        }
+      }
+    }
+  }
+  free(seen);
+  free(stack);
+  return count;
+}
+
+typedef struct {
+  uint32_t *copy, *original;
+  uint32_t n, cap;
+} copies_t;
+
+/* splitNode (tearing.cc:182-212): a copy of node with its rest, state and
+ * colour, no mass and not anchored */
+static uint32_t split_node(orc_lattice_body *b, uint32_t node) {
+  const uint32_t copy = b->nodes;
+  const size_t n = (size_t)b->nodes + 1;
+  b->rest = (double *)realloc(b->rest, 3 * n * sizeof(double));
+  b->x = (double *)realloc(b->x, 3 * n * sizeof(double));
+  b->v = (double *)realloc(b->v, 3 * n * sizeof(double));
+  b->mass = (float *)realloc(b->mass, n * sizeof(float));
+  b->anchored = (uint8_t *)realloc(b->anchored, n);
+  b->node_colour = (uint8_t *)realloc(b->node_colour, n);
+  for (int k = 0; k < 3; ++k) {
+    b->rest[3 * (size_t)copy + k] = b->rest[3 * (size_t)node + k];
+    b->x[3 * (size_t)copy + k] = b->x[3 * (size_t)node + k];
+    b->v[3 * (size_t)copy + k] = b->v[3 * (size_t)node + k];
+  }
+  b->mass[copy] = 0.0f;
+  b->anchored[copy] = 0;
+  b->node_colour[copy] = b->node_colour[node];
+  b->nodes = (uint32_t)n;
+  return copy;
+}
+
+/* split (tearing.cc:214-249): every group past the first gets a copy of the
+ * node; anchored nodes never split */
+static int split(orc_lattice_body *b, uint32_t node, const uint32_t *fan, uint32_t n, const uint32_t *group,
+                 uint32_t count, copies_t *copies) {
+  if (count < 2 || b->anchored[node]) return 0;
+  for (uint32_t piece = 1; piece < count; ++piece) {
+    const uint32_t copy = split_node(b, node);
+    if (copies->n == copies->cap) {
+      copies->cap = copies->cap ? 2 * copies->cap : 16;
+      copies->copy = (uint32_t *)realloc(copies->copy, copies->cap * sizeof(uint32_t));
+      copies->original = (uint32_t *)realloc(copies->original, copies->cap * sizeof(uint32_t));
+    }
+    copies->copy[copies->n] = copy;
+    copies->original[copies->n++] = node;
+    for (uint32_t at = 0; at < n; ++at) {
+      if (group[at] != piece) continue;
+      uint32_t *nd = b->hex + 8 * (size_t)fan[at];
+      for (int k = 0; k < 8; ++k)
+        if (nd[k] == node) nd[k] = copy;
+    }
+  }
+  return 1;
+}
+
+typedef struct {
+  double d;
+  uint32_t node;
+} by_distance;
+
+static int by_distance_cmp(const void *x, const void *y) {
+  const by_distance *a = (const by_distance *)x, *c = (const by_distance *)y;
+  if (a->d < c->d) return -1;
+  if (c->d < a->d) return 1;
+  return a->node < c->node ? -1 : (a->node > c->node ? 1 : 0);
+}
+
+/* crackThroughCell (tearing.cc:302-376): the corner nearest the plane
+ * through the cell's rest centre across the stretch whose cells lie on both
+ * sides, each side-group of them reaching a whole piece, is split */
+static int crack_through_cell(orc_lattice_body *b, uint32_t cell, copies_t *copies) {
+  double s, across[3], centre[3] = {0.0, 0.0, 0.0};
+  orc_lattice_cell_stretch(b, cell, &s, across);
+  uint32_t corners[8];
+  memcpy(corners, b->hex + 8 * (size_t)cell, sizeof corners);
+  for (int k = 0; k < 8; ++k) {
+    double r[3];
+    rest_of(b, corners[k], r);
+    for (int a = 0; a < 3; ++a) centre[a] += r[a] / 8.0;
+  }
+  by_distance order[8];
+  for (int k = 0; k < 8; ++k) {
+    double r[3];
+    rest_of(b, corners[k], r);
+    for (int a = 0; a < 3; ++a) r[a] -= centre[a];
+    order[k].d = fabs(dot3(r, across));
+    order[k].node = corners[k];
+  }
+  qsort(order, 8, sizeof order[0], by_distance_cmp);
+  uint32_t *fan = (uint32_t *)malloc(b->hexes * sizeof(uint32_t));
+  uint32_t *group = (uint32_t *)malloc(b->hexes * sizeof(uint32_t));
+  uint32_t *seeds = (uint32_t *)malloc(b->hexes * sizeof(uint32_t));
+  uint8_t *sides = (uint8_t *)malloc(b->hexes);
+  int done = 0;
+  for (int k = 0; k < 8 && !done; ++k) {
+    const uint32_t node = order[k].node;
+    if (b->anchored[node]) continue;
+    const uint32_t n = cells_at(b, node, fan);
+    int low = 0, high = 0;
+    double rn[3];
+    rest_of(b, node, rn);
+    for (uint32_t at = 0; at < n; ++at) {
+      double offset = 0.0;
+      for (int c = 0; c < 8; ++c) {
+        double r[3];
+        rest_of(b, b->hex[8 * (size_t)fan[at] + c], r);
+        for (int a = 0; a < 3; ++a) r[a] -= rn[a];
+        offset += dot3(r, across);
+      }
+      sides[at] = offset > 0.0 ? 1 : 0;
+      low = low || sides[at] == 0;
+      high = high || sides[at] == 1;
+    }
+    if (!low || !high) continue;
+    const uint32_t count = groups_of(b, fan, n, sides, group);
+    int whole = 1;
+    for (uint32_t piece = 0; piece < count && whole; ++piece) {
+      uint32_t ns = 0;
+      for (uint32_t at = 0; at < n; ++at)
+        if (group[at] == piece) seeds[ns++] = fan[at];
+      whole = reachable(b, seeds, ns, node, MIN_PIECE_CELLS) >= MIN_PIECE_CELLS;
+    }
+    if (whole && split(b, node, fan, n, group, count, copies)) done = 1;
+  }
+  free(fan);
+  free(group);
+  free(seeds);
+  free(sides);
+  return done;
+}
+
+/* splitPinched (tearing.cc:378-408): every node of a cell touching a split
+ * node, in ascending order, is split into its face-connected groups */
+static void split_pinched(orc_lattice_body *b, copies_t *copies) {
+  uint8_t *near = (uint8_t *)calloc(b->nodes, 1);
+  for (uint32_t i = 0; i < copies->n; ++i) near[copies->copy[i]] = near[copies->original[i]] = 1;
+  const uint32_t count = b->nodes;
+  uint8_t *candidate = (uint8_t *)calloc(count, 1);
+  for (uint32_t c = 0; c < b->hexes; ++c) {
+    const uint32_t *nd = b->hex + 8 * (size_t)c;
+    int touches = 0;
+    for (int k = 0; k < 8; ++k) touches = touches || near[nd[k]];
+    if (touches)
+      for (int k = 0; k < 8; ++k) candidate[nd[k]] = 1;
+  }
+  free(near);
+  uint32_t *fan = (uint32_t *)malloc(b->hexes * sizeof(uint32_t));
+  uint32_t *group = (uint32_t *)malloc(b->hexes * sizeof(uint32_t));
+  uint8_t *sides = (uint8_t *)calloc(b->hexes, 1);
+  for (uint32_t node = 0; node < count; ++node) {
+    if (!candidate[node]) continue;
+    const uint32_t n = cells_at(b, node, fan);
+    if (n < 2) continue;
+    const uint32_t groups = groups_of(b, fan, n, sides, group);
+    (void)split(b, node, fan, n, group, groups, copies);
+  }
+  free(candidate);
+  free(fan);
+  free(group);
+  free(sides);
+}
+
+/* recomputeMasses (tearing.cc:410-424), in single precision */
+static void recompute_masses(orc_lattice_body *b) {
+  for (uint32_t i = 0; i < b->nodes; ++i) b->mass[i] = 0.0f;
+  for (uint32_t c = 0; c < b->hexes; ++c) {
+    const float density = b->lane[c] < b->lanes ? b->materials[b->lane[c]].density : 1000.0f;
+    const float share = density * b->volume[c] * 0.125f;
+    for (int k = 0; k < 8; ++k) b->mass[b->hex[8 * (size_t)c + k]] += share;
+  }
+}
+
+/* markCracks (tearing.cc:426-484): the faces of each cell whose grid
+ * neighbour's facing face no longer holds the same nodes (surface nodes are
+ * not kept here) */
+static void mark_cracks(orc_lattice_body *b) {
+  float lo[3] = {INFINITY, INFINITY, INFINITY};
+  for (uint32_t i = 0; i < b->nodes; ++i)
+    for (int k = 0; k < 3; ++k) {
+      const float r = (float)b->rest[3 * (size_t)i + k];
+      lo[k] = r < lo[k] ? r : lo[k];
+    }
+  int32_t *index = (int32_t *)malloc(3 * (size_t)b->hexes * sizeof(int32_t));
+  for (uint32_t c = 0; c < b->hexes; ++c)
+    for (int k = 0; k < 3; ++k)
+      index[3 * (size_t)c + k] =
+          (int32_t)roundf(((float)b->rest[3 * (size_t)b->hex[8 * (size_t)c] + k] - lo[k]) / b->voxel[k]);
+  if (!b->crack) b->crack = (uint8_t *)calloc(b->hexes ? b->hexes : 1, 1);
+  for (uint32_t c = 0; c < b->hexes; ++c) {
+    b->crack[c] = 0;
+    for (uint32_t axis = 0; axis < 3; ++axis)
+      for (uint32_t side = 0; side < 2; ++side) {
+        int32_t next[3] = {index[3 * (size_t)c], index[3 * (size_t)c + 1], index[3 * (size_t)c + 2]};
+        next[axis] += side == 0 ? -1 : 1;
+        uint32_t found = UINT32_MAX; /* the first cell at that index (unordered_map::emplace keeps it) */
+        for (uint32_t o = 0; o < b->hexes && found == UINT32_MAX; ++o)
+          if (index[3 * (size_t)o] == next[0] && index[3 * (size_t)o + 1] == next[1] &&
+              index[3 * (size_t)o + 2] == next[2])
+            found = o;
+        if (found == UINT32_MAX) continue;
+        uint32_t mine[4], theirs[4];
+        face_of(b, c, 2 * axis + side, mine);
+        face_of(b, found, 2 * axis + (1 - side), theirs);
+        if (memcmp(mine, theirs, sizeof mine)) b->crack[c] |= (uint8_t)(1u << (2 * axis + side));
+      }
+  }
+  free(index);
+}
+
+uint32_t orc_lattice_tear(orc_lattice_body *b, const uint32_t *cells, uint32_t count) {
+  if (count == 0) return 0;
+  copies_t copies = {NULL, NULL, 0, 0};
+  for (uint32_t i = 0; i < count; ++i) (void)crack_through_cell(b, cells[i], &copies);
+  const uint32_t made0 = copies.n;
+  if (made0 == 0) return 0;
+  split_pinched(b, &copies);
+  recompute_masses(b);
+  mark_cracks(b);
+  b->retopo = 1;
+  const uint32_t made = copies.n;
+  free(copies.copy);
+  free(copies.original);
+  return made;
+}
+
+uint32_t orc_lattice_pieces(const orc_lattice_body *b) {
+  uint32_t *parent = (uint32_t *)malloc((b->hexes ? b->hexes : 1) * sizeof(uint32_t));
+  uint32_t *owner = (uint32_t *)malloc((b->nodes ? b->nodes : 1) * sizeof(uint32_t));
+  for (uint32_t c = 0; c < b->hexes; ++c) parent[c] = c;
+  for (uint32_t i = 0; i < b->nodes; ++i) owner[i] = UINT32_MAX;
+  for (uint32_t c = 0; c < b->hexes; ++c)
+    for (int k = 0; k < 8; ++k) {
+      const uint32_t node = b->hex[8 * (size_t)c + k];
+      if (owner[node] == UINT32_MAX) {
+        owner[node] = c;
+        continue;
+      }
+      const uint32_t x = root_of(parent, owner[node]), y = root_of(parent, c);
+      if (x != y) parent[x > y ? x : y] = x < y ? x : y;
+    }
+  uint32_t pieces = 0;
+  for (uint32_t c = 0; c < b->hexes; ++c) pieces += root_of(parent, c) == c ? 1u : 0u;
+  free(parent);
+  free(owner);
+  return pieces;
+}
+
+uint32_t orc_lattice_topology(const orc_lattice_body *b, uint32_t *hex_nodes, float *node_mass,
+                              uint8_t *crack_faces) {
+  if (hex_nodes) memcpy(hex_nodes, b->hex, 8 * (size_t)b->hexes * sizeof(uint32_t));
+  if (node_mass) memcpy(node_mass, b->mass, (size_t)b->nodes * sizeof(float));
+  if (crack_faces) {
+    if (b->crack)
+      memcpy(crack_faces, b->crack, b->hexes);
+    else
+      memset(crack_faces, 0, b->hexes);
+  }
+  return b->nodes;
+}
+
 uint32_t orc_lattice_step(orc_lattice_body *b, double dt, const double *g, uint32_t softstep,
                           uint32_t colored) {
   const uint32_t substeps = orc_lattice_substeps(b, dt, softstep);
   const double h = dt / (double)substeps;
-  if (b->h != h)
+  if (b->h != h) {
     build_elements(b, h); /* fresh elements: impulses start at zero */
-  else
+  } else if (b->retopo) {
+    /* the mesh tore: every element is built again and keeps its cell's
+     * impulses and rotation (strainstep.cc:610-623) */
+    orc_hex *old = b->elements;
+    const uint32_t was = b->count;
+    b->elements = NULL;
+    build_elements(b, h);
+    if (was == b->count)
+      for (uint32_t i = 0; i < was; ++i)
+        if (b->elements[i].cell == old[i].cell) {
+          memcpy(b->elements[i].impulse, old[i].impulse, sizeof old[i].impulse);
+          b->elements[i].vol_impulse = old[i].vol_impulse;
+          memcpy(b->elements[i].quat, old[i].quat, sizeof old[i].quat);
+          orc_quat_matrix(old[i].quat, b->elements[i].turn);
+        }
+    free(old);
+  } else {
     refresh_flowed(b);
+  }
+  b->retopo = 0;
   const int exact = !(h * stiffest(b) > RESOLVED);
   /* sweep order: cell order, or colour by colour */
   uint32_t *order = (uint32_t *)malloc((b->count ? b->count : 1) * sizeof(uint32_t));
@@ -1218,5 +1713,10 @@ uint32_t orc_lattice_step(orc_lattice_body *b, double dt, const double *g, uint3
   }
   free(order);
   (void)orc_lattice_flow(b);
+  /* overstretched cells tear at the step's end (strainstep.cc:826-830) */
+  uint32_t *torn = (uint32_t *)malloc((b->hexes ? b->hexes : 1) * sizeof(uint32_t));
+  const uint32_t ntorn = orc_lattice_overstretched(b, torn);
+  if (ntorn) (void)orc_lattice_tear(b, torn, ntorn);
+  free(torn);
   return substeps;
 }

@@ -138,9 +138,34 @@ int orc_hex_fit_holds(const double *shape, const double *hourglass);
 void orc_lattice_write_fitted(orc_lattice_body *body, uint32_t cell, const double *shape,
                               const double *hourglass);
 
+/* Cell tearing (strainstep/tearing.cc).  cellStretch (tearing.cc:279-281
+ * over stretchOf 258-300): the largest principal stretch of the cell's eight
+ * corner frames and its direction (the eigenvector's sign: its largest
+ * component positive).  overstretched (tearing.cc:488-503): the cells whose
+ * stretch passes 1 + their lane's tear strain, written to cells, returns the
+ * count.  tearCells (tearing.cc:505-536): cracks through each listed cell,
+ * splits the nodes pinched by the cracks, recomputes the node masses and
+ * marks the crack faces; returns the node copies made.  latticePieces
+ * (resolution.cc:198-228): face-or-node-connected pieces of cells. */
+void orc_lattice_cell_stretch(const orc_lattice_body *body, uint32_t cell, double *stretch,
+                              double *direction);
+uint32_t orc_lattice_overstretched(const orc_lattice_body *body, uint32_t *cells);
+uint32_t orc_lattice_tear(orc_lattice_body *body, const uint32_t *cells, uint32_t count);
+uint32_t orc_lattice_pieces(const orc_lattice_body *body);
+/* the current node count, and (each when not null) the hex corner nodes
+ * (8 per cell), node masses and per-cell crack-face bits (bit 2 axis + side,
+ * zero before the first tear) */
+uint32_t orc_lattice_topology(const orc_lattice_body *body, uint32_t *hex_nodes, float *node_mass,
+                              uint8_t *crack_faces);
+/* symmetric 3x3 eigen decomposition by cyclic Jacobi rotations: eigenvalues
+ * w and eigenvectors as the columns of V (row-major) */
+void orc_sym_eigen3(const double *A, double *w, double *V);
+
 /* One solveStrainstep of this body alone (strainstep.cc:569-842, no
- * contacts, joints, manifolds or tearing): refreshFlowed of the cells the
- * last flow moved, the substeps, then flowLattice when a lane yields.
+ * contacts, joints or manifolds): the element rebuild after a tear (impulses
+ * and rotations carried cell by cell) or refreshFlowed of the cells the last
+ * flow moved, the substeps, then flowLattice when a lane yields and
+ * overstretched / tearCells when a lane tears.
  * colored = 0 sweeps the elements in cell order as the reference does;
  * colored = 1 sweeps them colour by colour (hex colour 0..7), cells in order
  * within a colour.  Returns the substeps taken. */

@@ -178,7 +178,8 @@ class LatticeReport(C.Structure):
                 ("rebuilds", C.c_uint32), ("element_solves", C.c_uint64), ("device_ms", C.c_double),
                 ("inverses", C.c_uint32), ("solve_launches", C.c_uint32),
                 ("update_launches", C.c_uint32), ("refreshed", C.c_uint32),
-                ("solve_ms", C.c_double), ("update_ms", C.c_double)]
+                ("solve_ms", C.c_double), ("update_ms", C.c_double),
+                ("torn", C.c_uint32), ("node_copies", C.c_uint32)]
 
     def as_dict(self):
         return {k: getattr(self, k) for k, _ in self._fields_}
@@ -253,6 +254,12 @@ _SIGNATURES = {
                                                 C.c_uint32, _P(C.c_uint32)]),
     "talus_mi_lattice_colours_get": (C.c_int, [C.c_void_p, C.c_uint32, _P(C.c_uint8), _P(C.c_uint8)]),
     "talus_mi_lattice_step": (C.c_int, [C.c_void_p, _P(LatticeParams), _P(LatticeReport)]),
+    "talus_mi_lattice_topology_get": (C.c_int, [C.c_void_p, C.c_uint32, _P(C.c_uint32), _P(C.c_uint32),
+                                                _P(C.c_float), _P(C.c_uint8)]),
+    "talus_mi_lattice_tear": (C.c_int, [C.c_void_p, C.c_uint32, _P(C.c_uint32), C.c_uint32, _P(C.c_uint32)]),
+    "talus_mi_lattice_cell_stretch": (C.c_int, [C.c_void_p, C.c_uint32, C.c_uint32, _P(C.c_double),
+                                                _P(C.c_double)]),
+    "talus_mi_lattice_pieces": (C.c_int, [C.c_void_p, C.c_uint32, _P(C.c_uint32)]),
     "talus_mi_lattice_plastic_get": (C.c_int, [C.c_void_p, C.c_uint32, _P(C.c_double), _P(C.c_double),
                                                _P(C.c_uint32)]),
     "talus_mi_lattice_plastic_set": (C.c_int, [C.c_void_p, C.c_uint32, _P(C.c_double), _P(C.c_double)]),
@@ -779,14 +786,50 @@ class Context:
     def set_lattice_substeps(self, lattice, substeps):
         _check(lib().talus_mi_lattice_set_substeps(self._h, lattice, int(substeps)))
 
+    def lattice_nodes(self, lattice) -> int:
+        """the lattice's current node count (tearing adds node copies)"""
+        n = C.c_uint32()
+        _check(lib().talus_mi_lattice_topology_get(self._h, lattice, C.byref(n), None, None, None))
+        return int(n.value)
+
+    def lattice_topology(self, lattice):
+        """(hex nodes [hexes, 8], node masses, crack-face bits per hex)"""
+        h = self.lattices[lattice][1]
+        n = self.lattice_nodes(lattice)
+        hx = np.zeros(8 * h, dtype=np.uint32)
+        mass = np.zeros(n, dtype=np.float32)
+        crack = np.zeros(h, dtype=np.uint8)
+        _check(lib().talus_mi_lattice_topology_get(self._h, lattice, None, hx.ctypes.data_as(C.POINTER(C.c_uint32)),
+                                                   mass.ctypes.data_as(C.POINTER(C.c_float)),
+                                                   crack.ctypes.data_as(C.POINTER(C.c_uint8))))
+        return hx.reshape(-1, 8), mass, crack
+
+    def lattice_tear(self, lattice, cells) -> int:
+        """tearCells at the lattice's current positions: the node copies made"""
+        c = np.ascontiguousarray(cells, dtype=np.uint32)
+        out = C.c_uint32()
+        _check(lib().talus_mi_lattice_tear(self._h, lattice, c.ctypes.data_as(C.POINTER(C.c_uint32)), c.size,
+                                           C.byref(out)))
+        return int(out.value)
+
+    def lattice_cell_stretch(self, lattice, cell):
+        s, d = C.c_double(), np.zeros(3)
+        _check(lib().talus_mi_lattice_cell_stretch(self._h, lattice, int(cell), C.byref(s), _dp(d)))
+        return s.value, d
+
+    def lattice_pieces(self, lattice) -> int:
+        out = C.c_uint32()
+        _check(lib().talus_mi_lattice_pieces(self._h, lattice, C.byref(out)))
+        return int(out.value)
+
     def lattice_state(self, lattice):
-        n = self.lattices[lattice][0]
+        n = self.lattice_nodes(lattice)
         x, v = np.empty(3 * n), np.empty(3 * n)
         _check(lib().talus_mi_lattice_state_get(self._h, lattice, _dp(x), _dp(v)))
         return x.reshape(n, 3), v.reshape(n, 3)
 
     def set_lattice_state(self, lattice, x=None, v=None):
-        n = self.lattices[lattice][0]
+        n = self.lattice_nodes(lattice)
         x, v = _f64(x, 3 * n), _f64(v, 3 * n)
         _check(lib().talus_mi_lattice_state_set(self._h, lattice, _dp(x), _dp(v)))
 
@@ -802,7 +845,7 @@ class Context:
         return cells[:k], imp[: 19 * k].reshape(-1, 19)
 
     def lattice_colours(self, lattice):
-        n, h = self.lattices[lattice]
+        n, h = self.lattice_nodes(lattice), self.lattices[lattice][1]
         nc = np.zeros(n, dtype=np.uint8)
         hc = np.zeros(h, dtype=np.uint8)
         _check(lib().talus_mi_lattice_colours_get(self._h, lattice, nc.ctypes.data_as(C.POINTER(C.c_uint8)),

@@ -2552,6 +2552,58 @@ int talus_mi_lattice_colours_get(talus_mi_ctx *ctx, uint32_t lattice, uint8_t *n
   });
 }
 
+int talus_mi_lattice_topology_get(talus_mi_ctx *ctx, uint32_t lattice, uint32_t *nodes, uint32_t *hex_nodes,
+                                  float *node_mass, uint8_t *crack_faces) {
+  return guarded(ctx, [&]() -> int {
+    if (!lattice_ok(ctx, lattice)) return fail(TALUS_MI_ERR_ARGUMENT, "no such lattice");
+    const tmi::LatticeBody &b = ctx->lattices->body(lattice);
+    if (nodes) *nodes = b.nodes;
+    if (hex_nodes) std::copy(b.hex.begin(), b.hex.end(), hex_nodes);
+    if (node_mass) std::copy(b.mass.begin(), b.mass.end(), node_mass);
+    if (crack_faces) {
+      if (b.crack.empty())
+        std::fill(crack_faces, crack_faces + b.hexes, uint8_t(0));
+      else
+        std::copy(b.crack.begin(), b.crack.end(), crack_faces);
+    }
+    return TALUS_MI_OK;
+  });
+}
+
+int talus_mi_lattice_tear(talus_mi_ctx *ctx, uint32_t lattice, const uint32_t *cells, uint32_t count,
+                          uint32_t *copies) {
+  return guarded(ctx, [&]() -> int {
+    if (!lattice_ok(ctx, lattice)) return fail(TALUS_MI_ERR_ARGUMENT, "no such lattice");
+    if (count && !cells) return fail(TALUS_MI_ERR_ARGUMENT, "null cells");
+    const uint32_t hexes = ctx->lattices->body(lattice).hexes;
+    for (uint32_t i = 0; i < count; ++i)
+      if (cells[i] >= hexes) return fail(TALUS_MI_ERR_ARGUMENT, "cell out of range");
+    const uint32_t made = ctx->lattices->tear(lattice, cells, count);
+    if (copies) *copies = made;
+    return TALUS_MI_OK;
+  });
+}
+
+int talus_mi_lattice_cell_stretch(talus_mi_ctx *ctx, uint32_t lattice, uint32_t cell, double *stretch,
+                                  double *direction) {
+  return guarded(ctx, [&]() -> int {
+    if (!lattice_ok(ctx, lattice)) return fail(TALUS_MI_ERR_ARGUMENT, "no such lattice");
+    if (!stretch || !direction) return fail(TALUS_MI_ERR_ARGUMENT, "null output");
+    if (cell >= ctx->lattices->body(lattice).hexes) return fail(TALUS_MI_ERR_ARGUMENT, "cell out of range");
+    ctx->lattices->cell_stretch(lattice, cell, stretch, direction);
+    return TALUS_MI_OK;
+  });
+}
+
+int talus_mi_lattice_pieces(talus_mi_ctx *ctx, uint32_t lattice, uint32_t *pieces) {
+  return guarded(ctx, [&]() -> int {
+    if (!lattice_ok(ctx, lattice)) return fail(TALUS_MI_ERR_ARGUMENT, "no such lattice");
+    if (!pieces) return fail(TALUS_MI_ERR_ARGUMENT, "null output");
+    *pieces = ctx->lattices->body(lattice).pieces();
+    return TALUS_MI_OK;
+  });
+}
+
 int talus_mi_lattice_step(talus_mi_ctx *ctx, const talus_mi_lattice_params *params,
                           talus_mi_lattice_report *report) {
   if (!params) return fail(TALUS_MI_ERR_ARGUMENT, "null parameters");

@@ -22,6 +22,7 @@
 #include <cstring>
 #include <limits>
 #include <thread>
+#include <unordered_map>
 
 #include "hexlattice.h"
 #include "host.h"
@@ -713,6 +714,380 @@ uint32_t LatticeBody::refresh_flowed() {
   return rebuilt;
 }
 
+// ---- cell tearing (strainstep/tearing.cc) ---------------------------------
+
+namespace {
+
+constexpr uint32_t MIN_PIECE_CELLS = 2;  // tearing.h:13
+
+// FACES (tearing.cc:27-34): the corners of face 2 axis + side
+constexpr uint32_t FACES[6][4] = {{0, 2, 4, 6}, {1, 3, 5, 7}, {0, 1, 4, 5},
+                                  {2, 3, 6, 7}, {0, 1, 2, 3}, {4, 5, 6, 7}};
+
+using Face = std::array<uint32_t, 4>;
+
+uint32_t root_of(std::vector<uint32_t> &parent, uint32_t at) {
+  while (parent[at] != at) {
+    parent[at] = parent[parent[at]];
+    at = parent[at];
+  }
+  return at;
+}
+
+// the mesh being torn: the body's corner lists with, per node, the cells
+// holding it in ascending order (cellsAt, kept current as nodes split)
+struct Tearing {
+  LatticeBody &b;
+  std::vector<std::vector<uint32_t>> cells_of;
+  std::vector<std::pair<uint32_t, uint32_t>> copies;  // (copy, original)
+
+  explicit Tearing(LatticeBody &body) : b(body), cells_of(body.nodes) {
+    for (uint32_t c = 0; c < b.hexes; ++c)
+      for (int k = 0; k < 8; ++k) {
+        auto &list = cells_of[b.hex[8 * size_t(c) + k]];
+        if (list.empty() || list.back() != c) list.push_back(c);
+      }
+  }
+  const uint32_t *nodes_of(uint32_t cell) const { return &b.hex[8 * size_t(cell)]; }
+  bool has(uint32_t cell, uint32_t node) const {
+    const uint32_t *nd = nodes_of(cell);
+    for (int k = 0; k < 8; ++k)
+      if (nd[k] == node) return true;
+    return false;
+  }
+  Face face_of(uint32_t cell, uint32_t face) const {
+    Face f;
+    for (int k = 0; k < 4; ++k) f[k] = nodes_of(cell)[FACES[face][k]];
+    std::sort(f.begin(), f.end());
+    return f;
+  }
+  uint32_t shared_nodes(uint32_t x, uint32_t y) const {
+    uint32_t n = 0;
+    for (int k = 0; k < 8; ++k) n += has(y, nodes_of(x)[k]) ? 1u : 0u;
+    return n;
+  }
+  // groupsOf (tearing.cc:92-137)
+  uint32_t groups_of(const std::vector<uint32_t> &fan, const std::vector<uint8_t> &sides,
+                     std::vector<uint32_t> &group) const {
+    const uint32_t n = uint32_t(fan.size());
+    std::vector<uint32_t> parent(n), number(n, UINT32_MAX);
+    for (uint32_t i = 0; i < n; ++i) parent[i] = i;
+    for (uint32_t i = 0; i < n; ++i)
+      for (uint32_t j = i + 1; j < n; ++j)
+        if (sides[i] == sides[j] && shared_nodes(fan[i], fan[j]) >= 4) {
+          const uint32_t x = root_of(parent, i), y = root_of(parent, j);
+          if (x != y) parent[std::max(x, y)] = std::min(x, y);
+        }
+    group.assign(n, 0);
+    uint32_t count = 0;
+    for (uint8_t side = 0; side < 2; ++side)
+      for (uint32_t i = 0; i < n; ++i) {
+        if (sides[i] != side) continue;
+        const uint32_t r = root_of(parent, i);
+        if (number[r] == UINT32_MAX) number[r] = count++;
+        group[i] = number[r];
+      }
+    return count;
+  }
+  // reachable (tearing.cc:139-180): cells the seeds reach through faces that
+  // do not hold the pivot, stopping once limit are seen
+  size_t reachable(const std::vector<uint32_t> &seeds, uint32_t pivot, size_t limit) const {
+    std::vector<uint32_t> seen(seeds.begin(), seeds.end()), stack(seeds.begin(), seeds.end());
+    std::sort(seen.begin(), seen.end());
+    seen.erase(std::unique(seen.begin(), seen.end()), seen.end());
+    auto saw = [&](uint32_t c) { return std::binary_search(seen.begin(), seen.end(), c); };
+    while (!stack.empty() && seen.size() < limit) {
+      const uint32_t cell = stack.back();
+      stack.pop_back();
+      for (uint32_t face = 0; face < 6; ++face) {
+        const Face f = face_of(cell, face);
+        if (f[0] == pivot || f[1] == pivot || f[2] == pivot || f[3] == pivot) continue;
+        for (const uint32_t other : cells_of[f[0]]) {
+          if (saw(other)) continue;
+          bool match = false;
+          for (uint32_t g = 0; g < 6 && !match; ++g) match = face_of(other, g) == f;
+          if (match) {
+            seen.insert(std::upper_bound(seen.begin(), seen.end(), other), other);
+            stack.push_back(other);
+          }
+        }
+      }
+    }
+    return seen.size();
+  }
+  // splitNode (tearing.cc:182-212)
+  uint32_t split_node(uint32_t node) {
+    const uint32_t copy = b.nodes;
+    for (int k = 0; k < 3; ++k) {
+      b.rest.push_back(b.rest[3 * size_t(node) + k]);
+      b.x.push_back(b.x[3 * size_t(node) + k]);
+      b.v.push_back(b.v[3 * size_t(node) + k]);
+    }
+    b.mass.push_back(0.0f);
+    b.anchored.push_back(0);
+    b.node_colour.push_back(b.node_colour[node]);
+    cells_of.emplace_back();
+    ++b.nodes;
+    return copy;
+  }
+  // split (tearing.cc:214-249)
+  bool split(uint32_t node, const std::vector<uint32_t> &fan, const std::vector<uint32_t> &group, uint32_t count) {
+    if (count < 2 || b.anchored[node]) return false;
+    for (uint32_t piece = 1; piece < count; ++piece) {
+      const uint32_t copy = split_node(node);
+      copies.emplace_back(copy, node);
+      for (size_t at = 0; at < fan.size(); ++at) {
+        if (group[at] != piece) continue;
+        uint32_t *nd = &b.hex[8 * size_t(fan[at])];
+        for (int k = 0; k < 8; ++k)
+          if (nd[k] == node) nd[k] = copy;
+        auto &list = cells_of[node];
+        list.erase(std::find(list.begin(), list.end(), fan[at]));
+        cells_of[copy].push_back(fan[at]);  // fan is ascending
+      }
+    }
+    return true;
+  }
+  // crackThroughCell (tearing.cc:302-376)
+  bool crack_through_cell(uint32_t cell) {
+    double s, across[3], centre[3] = {0.0, 0.0, 0.0};
+    b.cell_stretch(cell, &s, across);
+    uint32_t corners[8];
+    std::copy(nodes_of(cell), nodes_of(cell) + 8, corners);
+    for (int k = 0; k < 8; ++k)
+      for (int a = 0; a < 3; ++a) centre[a] += b.rest[3 * size_t(corners[k]) + a] / 8.0;
+    std::array<std::pair<double, uint32_t>, 8> order;
+    for (int k = 0; k < 8; ++k) {
+      double r[3];
+      for (int a = 0; a < 3; ++a) r[a] = b.rest[3 * size_t(corners[k]) + a] - centre[a];
+      order[k] = {std::fabs(dot3(r, across)), corners[k]};
+    }
+    std::sort(order.begin(), order.end());
+    std::vector<uint32_t> group, seeds;
+    std::vector<uint8_t> sides;
+    for (const auto &entry : order) {
+      const uint32_t node = entry.second;
+      if (b.anchored[node]) continue;
+      const std::vector<uint32_t> fan = cells_of[node];
+      sides.assign(fan.size(), 0);
+      bool low = false, high = false;
+      const double *rn = &b.rest[3 * size_t(node)];
+      for (size_t at = 0; at < fan.size(); ++at) {
+        double offset = 0.0;
+        for (int c = 0; c < 8; ++c) {
+          const double *ro = &b.rest[3 * size_t(nodes_of(fan[at])[c])];
+          const double r[3] = {ro[0] - rn[0], ro[1] - rn[1], ro[2] - rn[2]};
+          offset += dot3(r, across);
+        }
+        sides[at] = offset > 0.0 ? 1 : 0;
+        low = low || sides[at] == 0;
+        high = high || sides[at] == 1;
+      }
+      if (!low || !high) continue;
+      const uint32_t count = groups_of(fan, sides, group);
+      bool whole = true;
+      for (uint32_t piece = 0; piece < count && whole; ++piece) {
+        seeds.clear();
+        for (size_t at = 0; at < fan.size(); ++at)
+          if (group[at] == piece) seeds.push_back(fan[at]);
+        whole = reachable(seeds, node, MIN_PIECE_CELLS) >= MIN_PIECE_CELLS;
+      }
+      if (whole && split(node, fan, group, count)) return true;
+    }
+    return false;
+  }
+  // splitPinched (tearing.cc:378-408)
+  void split_pinched() {
+    std::vector<uint8_t> near(b.nodes, 0), candidate(b.nodes, 0);
+    for (const auto &cp : copies) near[cp.first] = near[cp.second] = 1;
+    for (uint32_t c = 0; c < b.hexes; ++c) {
+      const uint32_t *nd = nodes_of(c);
+      bool touches = false;
+      for (int k = 0; k < 8; ++k) touches = touches || near[nd[k]];
+      if (touches)
+        for (int k = 0; k < 8; ++k) candidate[nd[k]] = 1;
+    }
+    const uint32_t count = b.nodes;
+    std::vector<uint32_t> group;
+    for (uint32_t node = 0; node < count; ++node) {
+      if (!candidate[node] || cells_of[node].size() < 2) continue;
+      const std::vector<uint32_t> fan = cells_of[node];
+      const std::vector<uint8_t> sides(fan.size(), 0);
+      const uint32_t groups = groups_of(fan, sides, group);
+      (void)split(node, fan, group, groups);
+    }
+  }
+  // recomputeMasses (tearing.cc:410-424), in single precision
+  void recompute_masses() {
+    std::fill(b.mass.begin(), b.mass.end(), 0.0f);
+    for (uint32_t c = 0; c < b.hexes; ++c) {
+      const float density = b.lane[c] < b.materials.size() ? b.materials[b.lane[c]].density : 1000.0f;
+      const float share = density * b.volume[c] * 0.125f;
+      for (int k = 0; k < 8; ++k) b.mass[nodes_of(c)[k]] += share;
+    }
+  }
+  // markCracks (tearing.cc:426-484) without the surface node list
+  void mark_cracks() {
+    float lo[3] = {std::numeric_limits<float>::max(), std::numeric_limits<float>::max(),
+                   std::numeric_limits<float>::max()};
+    for (uint32_t i = 0; i < b.nodes; ++i)
+      for (int k = 0; k < 3; ++k) lo[k] = std::min(lo[k], float(b.rest[3 * size_t(i) + k]));
+    constexpr int64_t OFFSET = 1 << 20;
+    auto key_of = [](const int32_t *at) {
+      return uint64_t(at[0] + OFFSET) | uint64_t(at[1] + OFFSET) << 21 | uint64_t(at[2] + OFFSET) << 42;
+    };
+    std::vector<int32_t> index(3 * size_t(b.hexes));
+    std::unordered_map<uint64_t, uint32_t> cell_at;
+    for (uint32_t c = 0; c < b.hexes; ++c) {
+      for (int k = 0; k < 3; ++k)
+        index[3 * size_t(c) + k] =
+            int32_t(std::round((float(b.rest[3 * size_t(nodes_of(c)[0]) + k]) - lo[k]) / b.voxel[k]));
+      cell_at.emplace(key_of(&index[3 * size_t(c)]), c);
+    }
+    b.crack.assign(b.hexes, 0);
+    for (uint32_t c = 0; c < b.hexes; ++c)
+      for (uint32_t axis = 0; axis < 3; ++axis)
+        for (uint32_t side = 0; side < 2; ++side) {
+          int32_t next[3] = {index[3 * size_t(c)], index[3 * size_t(c) + 1], index[3 * size_t(c) + 2]};
+          next[axis] += side == 0 ? -1 : 1;
+          const auto found = cell_at.find(key_of(next));
+          if (found == cell_at.end()) continue;
+          if (face_of(c, 2 * axis + side) != face_of(found->second, 2 * axis + (1 - side)))
+            b.crack[c] = uint8_t(b.crack[c] | (1u << (2 * axis + side)));
+        }
+  }
+};
+
+}  // namespace
+
+void sym_eigen3(const double *A, double *w, double *V) {
+  double a[3][3];
+  for (int r = 0; r < 3; ++r)
+    for (int c = 0; c < 3; ++c) {
+      a[r][c] = A[3 * r + c];
+      V[3 * r + c] = r == c ? 1.0 : 0.0;
+    }
+  constexpr int P[3] = {0, 0, 1}, Q[3] = {1, 2, 2};
+  for (int sweep = 0; sweep < 32; ++sweep) {
+    const double off = a[0][1] * a[0][1] + a[0][2] * a[0][2] + a[1][2] * a[1][2];
+    const double diag = a[0][0] * a[0][0] + a[1][1] * a[1][1] + a[2][2] * a[2][2];
+    if (!(off > 1.0e-40 * diag)) break;
+    for (int k = 0; k < 3; ++k) {
+      const int p = P[k], q = Q[k];
+      const double apq = a[p][q];
+      if (apq == 0.0) continue;
+      const double theta = (a[q][q] - a[p][p]) / (2.0 * apq);
+      const double t = std::fabs(theta) > 1.0e150
+                           ? 0.5 / theta
+                           : (theta < 0.0 ? -1.0 : 1.0) / (std::fabs(theta) + std::sqrt(theta * theta + 1.0));
+      const double c = 1.0 / std::sqrt(t * t + 1.0), s = t * c;
+      const int r = 3 - p - q;
+      const double arp = a[r][p], arq = a[r][q];
+      a[p][p] -= t * apq;
+      a[q][q] += t * apq;
+      a[p][q] = a[q][p] = 0.0;
+      a[r][p] = a[p][r] = c * arp - s * arq;
+      a[r][q] = a[q][r] = s * arp + c * arq;
+      for (int i = 0; i < 3; ++i) {
+        const double vp = V[3 * i + p], vq = V[3 * i + q];
+        V[3 * i + p] = c * vp - s * vq;
+        V[3 * i + q] = s * vp + c * vq;
+      }
+    }
+  }
+  for (int k = 0; k < 3; ++k) w[k] = a[k][k];
+}
+
+double LatticeBody::tear_limit(uint32_t cell_index) const {
+  return 1.0 + double(material_of(cell_index).tear_strain);
+}
+
+void LatticeBody::cell_stretch(uint32_t cell_index, double *stretch, double *direction) const {
+  const uint32_t *nd = &hex[8 * size_t(cell_index)];
+  double best = 0.0, dir[3] = {1.0, 0.0, 0.0};
+  for (uint32_t corner = 0; corner < 8; ++corner) {
+    const uint32_t base = nd[corner];
+    double Rm[9], N[9];
+    for (uint32_t e = 0; e < 3; ++e) {
+      const uint32_t other = nd[corner ^ (1u << e)];
+      for (int k = 0; k < 3; ++k) {
+        Rm[3 * k + e] = rest[3 * size_t(other) + k] - rest[3 * size_t(base) + k];
+        N[3 * k + e] = x[3 * size_t(other) + k] - x[3 * size_t(base) + k];
+      }
+    }
+    const double det = Rm[0] * (Rm[4] * Rm[8] - Rm[5] * Rm[7]) - Rm[1] * (Rm[3] * Rm[8] - Rm[5] * Rm[6]) +
+                       Rm[2] * (Rm[3] * Rm[7] - Rm[4] * Rm[6]);
+    if (!(std::fabs(det) > 0.0)) continue;
+    double Ri[9], F[9], C[9], w[3], V[9];
+    inverse3(Rm, Ri);
+    mm3(N, Ri, F);
+    for (int r = 0; r < 3; ++r)
+      for (int c = 0; c < 3; ++c) C[3 * r + c] = F[r] * F[c] + F[3 + r] * F[3 + c] + F[6 + r] * F[6 + c];
+    sym_eigen3(C, w, V);
+    int top = 0;
+    for (int k = 1; k < 3; ++k)
+      if (w[k] > w[top]) top = k;
+    const double s = std::sqrt(w[top] > 0.0 ? w[top] : 0.0);
+    if (s > best) {
+      const double v[3] = {V[top], V[3 + top], V[6 + top]};
+      const double len = std::sqrt(dot3(v, v));
+      int big = 0;
+      for (int k = 1; k < 3; ++k)
+        if (std::fabs(v[k]) > std::fabs(v[big])) big = k;
+      const double sign = v[big] < 0.0 ? -1.0 : 1.0;
+      for (int k = 0; k < 3; ++k) dir[k] = sign * v[k] / len;
+      best = s;
+    }
+  }
+  *stretch = best;
+  std::copy(dir, dir + 3, direction);
+}
+
+uint32_t LatticeBody::tear_cells(const uint32_t *cells, uint32_t count) {
+  if (count == 0) return 0;
+  Tearing t(*this);
+  for (uint32_t i = 0; i < count; ++i)
+    if (cells[i] < hexes) (void)t.crack_through_cell(cells[i]);
+  if (t.copies.empty()) return 0;
+  t.split_pinched();
+  t.recompute_masses();
+  t.mark_cracks();
+  retopo = true;
+  return uint32_t(t.copies.size());
+}
+
+uint32_t LatticeBody::pieces() const {
+  std::vector<uint32_t> parent(hexes), owner(nodes, UINT32_MAX);
+  for (uint32_t c = 0; c < hexes; ++c) parent[c] = c;
+  for (uint32_t c = 0; c < hexes; ++c)
+    for (int k = 0; k < 8; ++k) {
+      const uint32_t node = hex[8 * size_t(c) + k];
+      if (owner[node] == UINT32_MAX) {
+        owner[node] = c;
+        continue;
+      }
+      const uint32_t x0 = root_of(parent, owner[node]), y0 = root_of(parent, c);
+      if (x0 != y0) parent[std::max(x0, y0)] = std::min(x0, y0);
+    }
+  uint32_t n = 0;
+  for (uint32_t c = 0; c < hexes; ++c) n += root_of(parent, c) == c ? 1u : 0u;
+  return n;
+}
+
+void LatticeBody::rebuild_torn() {
+  std::vector<HexElement> old;
+  old.swap(elements);
+  build_elements(h);
+  if (old.size() == elements.size())
+    for (size_t i = 0; i < old.size(); ++i)
+      if (elements[i].cell == old[i].cell) {
+        std::copy(old[i].impulse, old[i].impulse + HEX_ROWS, elements[i].impulse);
+        elements[i].vol_impulse = old[i].vol_impulse;
+        std::copy(old[i].quat, old[i].quat + 4, elements[i].quat);
+      }
+  retopo = false;
+}
+
 int make_lattice(LatticeBody &b, const talus_mi_hex_mesh *mesh, const talus_mi_material *materials,
                  uint32_t lanes, const double *offset, const double *velocity) {
   if (!mesh || !mesh->rest_positions || !mesh->node_mass || !mesh->anchored || !mesh->hex_nodes ||
@@ -725,8 +1100,6 @@ int make_lattice(LatticeBody &b, const talus_mi_hex_mesh *mesh, const talus_mi_m
     const talus_mi_material &m = materials[l];
     if (std::isfinite(m.damping_ratio))
       return fail(TALUS_MI_ERR_ARGUMENT, "lattice damping ratios (lowest-mode damping) are not supported");
-    if (std::isfinite(m.tear_strain))
-      return fail(TALUS_MI_ERR_ARGUMENT, "lattice cell tearing is not supported");
   }
   for (uint32_t c = 0; c < 8 * mesh->hexes; ++c)
     if (mesh->hex_nodes[c] >= mesh->nodes) return fail(TALUS_MI_ERR_ARGUMENT, "hex node out of range");
@@ -764,6 +1137,7 @@ int make_lattice(LatticeBody &b, const talus_mi_hex_mesh *mesh, const talus_mi_m
     b.node_colour[i] = uint8_t(colour);
   }
   for (uint32_t c = 0; c < b.hexes; ++c) b.yields = b.yields || std::isfinite(b.material_of(c).yield_stress);
+  for (uint32_t c = 0; c < b.hexes; ++c) b.tears = b.tears || std::isfinite(b.tear_limit(c));
   b.hex_colour.resize(b.hexes);
   for (uint32_t c = 0; c < b.hexes; ++c) b.hex_colour[c] = b.node_colour[b.hex[8 * size_t(c)]];
   // a colour's hexes are solved at once: no two of them may share a node

@@ -100,7 +100,33 @@ struct LatticeBody {
   // refreshFlowed (strainstep.cc:371-451): rebuilds the elements of the cells
   // whose P moved since their element was built; returns how many
   uint32_t refresh_flowed();
+
+  // tearing (strainstep/tearing.cc): per-cell crack-face bits (bit
+  // 2 axis + side; empty before the first tear) and whether the mesh changed
+  // since the elements were built
+  bool tears = false;             // a lane has a finite tear strain
+  std::vector<uint8_t> crack;     // hexes
+  bool retopo = false;
+  // 1 + the cell's lane tear strain (infinite when the lane never tears)
+  double tear_limit(uint32_t cell) const;
+  // stretchOf (tearing.cc:258-300) at the host positions: the largest
+  // principal stretch over the eight corner frames and its direction (the
+  // eigenvector's largest component positive)
+  void cell_stretch(uint32_t cell, double *stretch, double *direction) const;
+  // tearCells (tearing.cc:505-536) at the host positions; returns the node
+  // copies made
+  uint32_t tear_cells(const uint32_t *cells, uint32_t count);
+  // latticePieces (resolution.cc:198-228)
+  uint32_t pieces() const;
+  // the element rebuild after a tear: every element is built again at the
+  // same substep and keeps its cell's impulses and rotation
+  // (strainstep.cc:610-623)
+  void rebuild_torn();
 };
+
+// symmetric 3x3 eigen decomposition by cyclic Jacobi rotations (w: values,
+// V: vectors as columns, row-major)
+void sym_eigen3(const double *A, double *w, double *V);
 
 // Fills a body from a hex mesh (makeLattice); returns a status.
 int make_lattice(LatticeBody &body, const talus_mi_hex_mesh *mesh,
@@ -125,6 +151,9 @@ class LatticeSet {
   // per Gauss point P and plastic strain (empty when the body never flowed)
   void get_plastic(uint32_t i, double *plastic, double *hardening, uint32_t *present);
   void set_plastic(uint32_t i, const double *plastic, const double *hardening);
+  // tearCells / cellStretch at the body's current positions
+  uint32_t tear(uint32_t i, const uint32_t *cells, uint32_t count);
+  void cell_stretch(uint32_t i, uint32_t cell, double *stretch, double *direction);
 
  private:
   struct Device;

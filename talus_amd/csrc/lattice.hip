@@ -139,6 +139,14 @@ struct LatDev {
   double *c_plastic;        // 72
   double *c_hard;           // 8
   uint32_t *c_moved;        // 1
+  // tearing, per cell of every body (the same order as the plasticity cells)
+  // when a lane tears: corner nodes, corner rest positions, 1 + the lane's
+  // tear strain (infinite when it never tears) and the overstretched flag
+  uint32_t T;
+  const uint32_t *t_nodes;  // 8
+  const double *t_rest;     // 24
+  const double *t_limit;    // 1
+  uint32_t *t_torn;         // 1
 };
 
 __device__ __forceinline__ double corner_sign(uint32_t c, int axis) {
@@ -533,6 +541,91 @@ __device__ bool return_map(const double *E, double shear, double yield, double *
   return true;
 }
 
+
+// one Jacobi rotation of a symmetric 3x3 in the (p, q) plane; r is the third
+// index (the rotation the host's sym_eigen3 applies, values only)
+__device__ __forceinline__ void jacobi_rotate(double &app, double &aqq, double &apq, double &arp, double &arq) {
+  if (apq == 0.0) return;
+  const double theta = (aqq - app) / (2.0 * apq);
+  const double t = fabs(theta) > 1.0e150 ? 0.5 / theta
+                                         : (theta < 0.0 ? -1.0 : 1.0) / (fabs(theta) + sqrt(theta * theta + 1.0));
+  const double c = 1.0 / sqrt(t * t + 1.0), s = t * c;
+  app -= t * apq;
+  aqq += t * apq;
+  apq = 0.0;
+  const double rp = arp, rq = arq;
+  arp = c * rp - s * rq;
+  arq = s * rp + c * rq;
+}
+
+__device__ double largest_eigenvalue(double a00, double a11, double a22, double a01, double a02, double a12) {
+  for (int sweep = 0; sweep < 32; ++sweep) {
+    const double off = a01 * a01 + a02 * a02 + a12 * a12;
+    const double diag = a00 * a00 + a11 * a11 + a22 * a22;
+    if (!(off > 1.0e-40 * diag)) break;
+    jacobi_rotate(a00, a11, a01, a02, a12);
+    jacobi_rotate(a00, a22, a02, a01, a12);
+    jacobi_rotate(a11, a22, a12, a01, a02);
+  }
+  return fmax(a00, fmax(a11, a22));
+}
+
+// overstretched (tearing.cc:488-503) after a step: one lane per cell; over
+// its eight corner frames (the corner's three edges, now against rest) the
+// largest principal stretch sqrt(lambda_max(F^T F)), flagged past the limit
+__global__ void __launch_bounds__(NODE_BLOCK) k_lat_stretch(LatDev d) {
+  const uint32_t c = blockIdx.x * NODE_BLOCK + threadIdx.x;
+  if (c >= d.T) return;
+  const double limit = d.t_limit[c];
+  uint32_t torn = 0u;
+  if (isfinite(limit)) {
+    double px[8][3], pr[8][3];
+#pragma unroll
+    for (int k = 0; k < 8; ++k) {
+      const double *xp = d.x + 3 * size_t(d.t_nodes[8 * size_t(c) + k]);
+      const double *rp = d.t_rest + 24 * size_t(c) + 3 * k;
+#pragma unroll
+      for (int i = 0; i < 3; ++i) {
+        px[k][i] = xp[i];
+        pr[k][i] = rp[i];
+      }
+    }
+    double best = 0.0;
+#pragma unroll
+    for (int corner = 0; corner < 8; ++corner) {
+      double R[9], N[9];
+#pragma unroll
+      for (int e = 0; e < 3; ++e)
+#pragma unroll
+        for (int i = 0; i < 3; ++i) {
+          R[3 * i + e] = pr[corner ^ (1 << e)][i] - pr[corner][i];
+          N[3 * i + e] = px[corner ^ (1 << e)][i] - px[corner][i];
+        }
+      const double c0 = R[4] * R[8] - R[5] * R[7], c1 = R[5] * R[6] - R[3] * R[8], c2 = R[3] * R[7] - R[4] * R[6];
+      const double det = R[0] * c0 + R[1] * c1 + R[2] * c2;
+      if (!(fabs(det) > 0.0)) continue;
+      const double id = 1.0 / det;
+      const double Ri[9] = {c0 * id, (R[2] * R[7] - R[1] * R[8]) * id, (R[1] * R[5] - R[2] * R[4]) * id,
+                            c1 * id, (R[0] * R[8] - R[2] * R[6]) * id, (R[2] * R[3] - R[0] * R[5]) * id,
+                            c2 * id, (R[1] * R[6] - R[0] * R[7]) * id, (R[0] * R[4] - R[1] * R[3]) * id};
+      double F[9];
+#pragma unroll
+      for (int i = 0; i < 3; ++i)
+#pragma unroll
+        for (int j = 0; j < 3; ++j) F[3 * i + j] = N[3 * i] * Ri[j] + N[3 * i + 1] * Ri[3 + j] + N[3 * i + 2] * Ri[6 + j];
+      double C[9];
+#pragma unroll
+      for (int i = 0; i < 3; ++i)
+#pragma unroll
+        for (int j = 0; j < 3; ++j) C[3 * i + j] = F[i] * F[j] + F[3 + i] * F[3 + j] + F[6 + i] * F[6 + j];
+      const double lam = largest_eigenvalue(C[0], C[4], C[8], C[1], C[2], C[5]);
+      best = fmax(best, sqrt(fmax(lam, 0.0)));
+    }
+    torn = best > limit ? 1u : 0u;
+  }
+  d.t_torn[c] = torn;
+}
+
 // flowLattice (plasticflow.cc:161-260) after a step: one lane per Gauss point
 // of every yielding cell (point p at (signs of corner p) / sqrt(3)); a point
 // whose P changes by more than 1e-12 (one-norm) flags its cell as moved
@@ -766,6 +859,7 @@ struct LatticeSet::Device {
   uint32_t *frame_order = nullptr;             // slots by descending substeps (device)
   std::vector<uint32_t> cell0;                 // first plasticity cell of each body
   bool yields = false;                         // some body has a yielding lane
+  bool tears = false;                          // some body has a tearing lane
   std::vector<uint32_t> slot_body;             // per slot: its body
   uint32_t inverses = 0;
   hipEvent_t start = nullptr, stop = nullptr;
@@ -1118,6 +1212,29 @@ void LatticeSet::upload(const std::vector<uint32_t> &substeps, const std::vector
       d.c_hard = dv->copy(c_hard);
       d.c_moved = dv->copy(std::vector<uint32_t>(C, 0u));
     }
+    // tearing cells: the same cells, when some lane tears
+    for (uint32_t b = 0; b < B; ++b) dv->tears = dv->tears || bodies_[b].tears;
+    d.T = dv->tears ? C : 0u;
+    if (dv->tears) {
+      std::vector<uint32_t> t_nodes(8 * size_t(C));
+      std::vector<double> t_rest(24 * size_t(C)), t_limit(C);
+      for (uint32_t b = 0; b < B; ++b) {
+        const LatticeBody &body = bodies_[b];
+        for (uint32_t c = 0; c < body.hexes; ++c) {
+          const size_t g = dv->cell0[b] + c;
+          for (int k = 0; k < 8; ++k) {
+            const uint32_t n = body.hex[8 * size_t(c) + k];
+            t_nodes[8 * g + k] = dv->node0[b] + n;
+            for (int i = 0; i < 3; ++i) t_rest[24 * g + 3 * k + i] = body.rest[3 * size_t(n) + i];
+          }
+          t_limit[g] = body.tear_limit(c);
+        }
+      }
+      d.t_nodes = dv->copy(t_nodes);
+      d.t_rest = dv->copy(t_rest);
+      d.t_limit = dv->copy(t_limit);
+      d.t_torn = dv->alloc<uint32_t>(C);
+    }
   }
   HIPCHECK(hipEventCreate(&dv->start));
   HIPCHECK(hipEventCreate(&dv->stop));
@@ -1154,7 +1271,7 @@ void LatticeSet::step(const talus_mi_lattice_params &p, talus_mi_lattice_report 
     substeps[b] = bodies_[b].island_substeps(p.dt, p.softstep_substeps);
     h[b] = p.dt / double(substeps[b]);
     exact[b] = !(h[b] * bodies_[b].stiffest() > 1.5) ? 1 : 0;
-    if (bodies_[b].h != h[b]) rebuild = true;
+    if (bodies_[b].h != h[b] || bodies_[b].retopo) rebuild = true;
   }
   // refreshFlowed: the cells the last step's flow moved get their elements
   // rebuilt on the host (at the positions the step starts from)
@@ -1167,8 +1284,8 @@ void LatticeSet::step(const talus_mi_lattice_params &p, talus_mi_lattice_report 
     if (refresh) download();
   }
   if (refresh) {
-    for (uint32_t b = 0; b < B; ++b)  // a body whose substep changes is rebuilt whole below
-      if (bodies_[b].h == h[b]) rep.refreshed += bodies_[b].refresh_flowed();
+    for (uint32_t b = 0; b < B; ++b)  // a body whose substep changes or that tore is rebuilt whole below
+      if (bodies_[b].h == h[b] && !bodies_[b].retopo) rep.refreshed += bodies_[b].refresh_flowed();
     device_valid_ = false;  // laid out again with the new elements and P
   }
   flowed_ = false;
@@ -1178,7 +1295,11 @@ void LatticeSet::step(const talus_mi_lattice_params &p, talus_mi_lattice_report 
     download();
     for (uint32_t b = 0; b < B; ++b)
       if (bodies_[b].h != h[b]) {
-        bodies_[b].build_elements(h[b]);
+        bodies_[b].build_elements(h[b]);  // fresh elements: impulses start at zero
+        bodies_[b].retopo = false;
+        ++rep.rebuilds;
+      } else if (bodies_[b].retopo) {
+        bodies_[b].rebuild_torn();  // the same substep: impulses carry over cell by cell
         ++rep.rebuilds;
       }
     device_valid_ = false;
@@ -1303,6 +1424,11 @@ void LatticeSet::step(const talus_mi_lattice_params &p, talus_mi_lattice_report 
     HIPCHECK(hipGetLastError());
     flowed_ = true;
   }
+  // overstretched cells, flagged on the device at the step's end
+  if (d.T) {
+    hipLaunchKernelGGL(k_lat_stretch, dim3((d.T + NODE_BLOCK - 1) / NODE_BLOCK), dim3(NODE_BLOCK), 0, stream_, d);
+    HIPCHECK(hipGetLastError());
+  }
   HIPCHECK(hipEventRecord(dv.stop, stream_));
   HIPCHECK(hipEventSynchronize(dv.stop));
   float ms = 0.0f;
@@ -1313,6 +1439,40 @@ void LatticeSet::step(const talus_mi_lattice_params &p, talus_mi_lattice_report 
     HIPCHECK(hipEventElapsedTime(&k, dv.marks[t.first], dv.marks[t.first + 1]));
     (t.second ? rep.solve_ms : rep.update_ms) += k;
   }
+  // tearCells on the host for the flagged cells (strainstep.cc:826-830);
+  // the torn bodies are laid out again, and their elements rebuilt, at the
+  // next step
+  if (d.T) {
+    std::vector<uint32_t> flags(d.T);
+    HIPCHECK(hipMemcpy(flags.data(), d.t_torn, flags.size() * sizeof(uint32_t), hipMemcpyDeviceToHost));
+    if (std::find(flags.begin(), flags.end(), 1u) != flags.end()) {
+      download();
+      std::vector<uint32_t> cells;
+      for (uint32_t b = 0; b < B; ++b) {
+        LatticeBody &body = bodies_[b];
+        cells.clear();
+        for (uint32_t c = 0; c < body.hexes; ++c)
+          if (flags[dv.cell0[b] + c]) cells.push_back(c);
+        if (cells.empty()) continue;
+        rep.torn += uint32_t(cells.size());
+        const uint32_t made = body.tear_cells(cells.data(), uint32_t(cells.size()));
+        rep.node_copies += made;
+        if (made) device_valid_ = false;
+      }
+    }
+  }
+}
+
+uint32_t LatticeSet::tear(uint32_t i, const uint32_t *cells, uint32_t count) {
+  download();
+  const uint32_t made = bodies_.at(i).tear_cells(cells, count);
+  if (made) device_valid_ = false;
+  return made;
+}
+
+void LatticeSet::cell_stretch(uint32_t i, uint32_t cell, double *stretch, double *direction) {
+  download();
+  bodies_.at(i).cell_stretch(cell, stretch, direction);
 }
 
 }  // namespace tmi

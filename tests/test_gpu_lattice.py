@@ -188,7 +188,7 @@ def test_state_round_trips_and_unsupported_materials_are_refused():
         ctx.set_lattice_state(b, x1 + 0.5, v1)
         x2, _ = ctx.lattice_state(b)
         np.testing.assert_array_equal(x2, x1 + 0.5)
-        for bad in (dict(damping_ratio=0.5), dict(tear_strain=0.3)):
+        for bad in (dict(damping_ratio=0.5),):
             m = material(2.0e5)
             for k, val in bad.items():
                 setattr(m, k, val)

@@ -522,3 +522,184 @@ def test_strip_held_bent_past_yield_stops_flowing():
     final = body.plastic()[1].sum() / 8.0
     assert held > 0.0
     assert final - held < 0.01 * held
+
+
+# ---- cell tearing ----------------------------------------------------------
+
+def tearable():
+    """tearable (dynamics_strainstep_tests.cc:905-911)"""
+    m = jelly(5.0e3, density=20.0, poisson=0.3)
+    m.tear_strain = 0.4
+    return m
+
+
+def orphans(body):
+    hx, _, _ = body.topology()
+    return body.nodes - np.unique(hx).size
+
+
+def test_sym_eigen3_matches_numpy():
+    rng = np.random.default_rng(3)
+    for _ in range(50):
+        A = rng.standard_normal((3, 3))
+        A = A @ A.T if rng.random() < 0.5 else A + A.T
+        w, V = H.sym_eigen3(A)
+        ref = np.linalg.eigvalsh(A)
+        np.testing.assert_allclose(np.sort(w), ref, rtol=0, atol=1e-12 * np.abs(ref).max())
+        np.testing.assert_allclose(A @ V, V * w, atol=1e-12 * np.abs(ref).max())
+        np.testing.assert_allclose(V.T @ V, np.eye(3), atol=1e-13)
+    w, V = H.sym_eigen3(np.diag([2.0, 5.0, 1.0]))  # diagonal: no rotation
+    assert w.tolist() == [2.0, 5.0, 1.0] and V.tolist() == np.eye(3).tolist()
+
+
+def test_cell_stretch_is_the_largest_singular_value_of_a_uniform_deformation():
+    """stretchOf (tearing.cc:258-300): a uniform x -> F x gives every corner
+    frame F, so the stretch is F's largest singular value and the direction
+    its right singular vector; a rotation stretches nothing"""
+    mat = tearable()
+    mesh = box((0.4, 0.2, 0.2), 0.2, mat)
+    rng = np.random.default_rng(8)
+    for trial in range(6):
+        F = np.eye(3) + 0.3 * rng.standard_normal((3, 3))
+        if np.linalg.det(F) < 0:
+            F[:, 0] *= -1
+        body = H.LatticeBody(mesh, [mat, mat])
+        x, v = body.state()
+        body.set_state(x @ F.T + 0.3, v)
+        _, s, Vt = np.linalg.svd(F)
+        for cell in (0, 3, 7):
+            stretch, d = body.cell_stretch(cell)
+            assert abs(stretch - s[0]) <= 1e-12 * s[0]
+            assert abs(abs(d @ Vt[0]) - 1.0) <= 1e-9
+            assert abs(d).max() == d.max()  # sign: largest component positive
+    c, s_ = math.cos(0.7), math.sin(0.7)
+    body = H.LatticeBody(mesh, [mat, mat])
+    x, v = body.state()
+    body.set_state(x @ np.array([[c, -s_, 0], [s_, c, 0], [0, 0, 1]]).T, v)
+    assert abs(body.cell_stretch(2)[0] - 1.0) <= 1e-12
+    assert body.overstretched().size == 0
+
+
+def test_overstretched_uses_one_plus_the_lane_tear_strain():
+    """overstretched (tearing.cc:488-503): strictly past 1 + tearStrain, per
+    lane; lanes without a finite tear strain never tear"""
+    mat = tearable()
+    mesh = box((0.4, 0.2, 0.2), 0.2, mat)
+    for scale, torn in ((1.39, 0), (1.41, 16)):
+        body = H.LatticeBody(mesh, [mat, mat])
+        x, v = body.state()
+        x[:, 0] *= scale
+        body.set_state(x, v)
+        assert body.overstretched().size == torn
+    plain = jelly(5.0e3)
+    body = H.LatticeBody(mesh, [plain, plain])
+    x, v = body.state()
+    x[:, 0] *= 3.0
+    body.set_state(x, v)
+    assert body.overstretched().size == 0
+
+
+def test_tear_splits_the_nodes_two_halves_share():
+    """dynamics_strainstep_tests.cc:3759-3796: a crack through the second
+    cell of a strip one cell thick splits the four nodes the halves share"""
+    mat = tearable()
+    mesh = box((1.0, 0.25, 0.25), 0.5, mat)
+    body = H.LatticeBody(mesh, [mat, mat], offset=(0.0, 1.0, 0.0))
+    nodes = body.nodes
+    mass = mesh["mass"].astype(np.float64).sum()
+    assert mesh["hex"].shape[0] == 4 and body.pieces() == 1
+    x, v = body.state()
+    x[:, 0] *= 1.01
+    body.set_state(x, v)
+    assert body.tear([1]) == 4
+    assert body.nodes == nodes + 4
+    assert body.pieces() == 2
+    hx, m, crack = body.topology()
+    assert hx.shape == (4, 8)
+    assert abs(m.astype(np.float64).sum() - mass) <= 1e-6 * mass
+    # the crack runs between cells 1 and 2: cell 1's +x face, cell 2's -x face
+    assert crack.tolist() == [0, 2, 1, 0]
+    # the copies sit where their originals are
+    xs, _ = body.state()
+    assert xs.shape == (nodes + 4, 3)
+    for _ in range(120):
+        body.step()
+    x, v = body.state()
+    assert np.isfinite(x).all() and np.isfinite(v).all()
+    assert orphans(body) == 0
+
+
+def test_tear_never_chips_a_single_cell():
+    """MIN_PIECE_CELLS (tearing.h:13): a crack through an end cell would cut
+    off one cell alone, so nothing splits; a listed cell whose corners are
+    all anchored cannot split either"""
+    mat = tearable()
+    mesh = box((1.0, 0.25, 0.25), 0.5, mat)
+    body = H.LatticeBody(mesh, [mat, mat])
+    x, v = body.state()
+    x[:, 0] *= 1.01
+    body.set_state(x, v)
+    assert body.tear([0]) == 0 and body.tear([3]) == 0 and body.tear([]) == 0
+    assert body.nodes == 20 and body.pieces() == 1
+    crack = body.topology()[2]
+    assert not crack.any()
+
+
+def test_pieces_count_separate_blocks():
+    """latticePieces (resolution.cc:198-228) of a lattice with a gap"""
+    mat = tearable()
+    solid = np.ones(4 * 1 * 1, dtype=np.uint8)
+    solid[1] = 0
+    mesh = H.build_hex_lattice([5, 2, 2], [0.5] * 3, [0.0] * 3, [mat], [], solid=solid)
+    body = H.LatticeBody(mesh, [mat])
+    assert body.pieces() == 2
+
+
+def pulled_strip(frames=240):
+    """dynamics_strainstep_tests.cc:3719-3757: a strip held at both ends, the
+    right end drawn out 2.4 m over three seconds"""
+    mat = tearable()
+    ends = [hold_at((-1.2, 0.0, 0.0), (0.01, 1.0, 1.0)), hold_at((1.2, 0.0, 0.0), (0.01, 1.0, 1.0))]
+    mesh = box((1.2, 0.2, 0.2), 0.2, mat, ends)
+    body = H.LatticeBody(mesh, [mat, mat], offset=(0.0, 2.0, 0.0))
+    right = np.nonzero(mesh["rest"][:, 0] > 1.19)[0]
+    start = body.state()[0][right, 0].copy()
+    first = None
+    for frame in range(frames):
+        x, v = body.state()
+        x[right, 0] = start + 2.4 * min(frame / 180.0, 1.0)
+        body.set_state(x, v)
+        body.step()
+        if first is None and body.nodes > mesh["rest"].shape[0]:
+            first = frame
+    return body, mesh, first
+
+
+def longest_edge(body, cell):
+    hx, _, _ = body.topology()
+    x, _ = body.state()
+    longest = 0.0
+    for nd in hx:
+        for c in range(8):
+            for a in range(3):
+                o = c ^ (1 << a)
+                if o > c:
+                    longest = max(longest, np.linalg.norm(x[nd[c]] - x[nd[o]]) / cell)
+    return longest
+
+
+def test_strip_pulled_apart_tears_into_pieces():
+    """dynamics_strainstep_tests.cc:3719-3757"""
+    body, mesh, first = pulled_strip()
+    mass = mesh["mass"].astype(np.float64).sum()
+    hx, m, crack = body.topology()
+    assert first is not None
+    assert body.pieces() >= 2
+    assert hx.shape[0] == mesh["hex"].shape[0]
+    assert abs(m.astype(np.float64).sum() - mass) <= 1e-6 * mass
+    assert body.nodes > mesh["rest"].shape[0]
+    assert orphans(body) == 0
+    assert longest_edge(body, 0.2) < 1.8
+    assert crack.any()
+    x, v = body.state()
+    assert np.isfinite(x).all() and np.isfinite(v).all()
