@@ -462,9 +462,11 @@ void talus_mi_lattice_default_params(talus_mi_lattice_params *params);
  * lanes (strainstep.cc:86-94: lanes past the array read the last one).
  * Lanes with a finite yield stress flow plastically (flowLattice after each
  * step, refreshFlowed before the next).  Lanes with a finite tear strain
- * tear (overstretched / tearCells after each step).  Damping ratios must be
- * infinite (the lowest-mode damping is not in this library):
- * TALUS_MI_ERR_ARGUMENT otherwise. */
+ * tear (overstretched / tearCells after each step).  Lanes with a finite
+ * damping ratio damp each row toward it at the lattice's lowest angular
+ * frequency (cellDamping, strainstep.cc:151-160), found once at the first
+ * build; a damped lattice whose banded lowest-mode solve would exceed 4e10
+ * operations or 1e8 band entries is refused with TALUS_MI_ERR_ARGUMENT. */
 int talus_mi_lattice_add(talus_mi_ctx *ctx, const talus_mi_hex_mesh *mesh,
                          const talus_mi_material *materials, uint32_t lanes,
                          const double *offset, const double *velocity, uint32_t *lattice);
@@ -512,6 +514,11 @@ int talus_mi_lattice_tear(talus_mi_ctx *ctx, uint32_t lattice, const uint32_t *c
  * positive). */
 int talus_mi_lattice_cell_stretch(talus_mi_ctx *ctx, uint32_t lattice, uint32_t cell,
                                   double *stretch, double *direction);
+/* StrainLattice::lowest (strainstep.cc:300-331, lowestAngular in
+ * lowestmode.cc:31-203): the lowest angular frequency (rad/s) the body's
+ * elements are damped for; 0 before a damped body's first step and for an
+ * undamped body. */
+int talus_mi_lattice_lowest_get(talus_mi_ctx *ctx, uint32_t lattice, double *lowest);
 /* latticePieces (resolution.cc:198-228): pieces of cells joined by nodes. */
 int talus_mi_lattice_pieces(talus_mi_ctx *ctx, uint32_t lattice, uint32_t *pieces);
 /* One solveStrainstep over every lattice of the context, each its own

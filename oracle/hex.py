@@ -114,6 +114,10 @@ def lib():
         L.orc_lattice_topology.argtypes = [C.c_void_p, u32p, C.POINTER(C.c_float), u8p]
         L.orc_lattice_topology.restype = C.c_uint32
         L.orc_sym_eigen3.argtypes = [dp, dp, dp]
+        L.orc_lattice_lowest_angular.argtypes = [C.c_void_p]
+        L.orc_lattice_lowest_angular.restype = C.c_double
+        L.orc_lattice_lowest.argtypes = [C.c_void_p]
+        L.orc_lattice_lowest.restype = C.c_double
         _ready = True
     return L
 
@@ -409,6 +413,14 @@ class LatticeBody:
 
     def pieces(self):
         return int(lib().orc_lattice_pieces(self._h))
+
+    def lowest_angular(self):
+        """lowestAngular of the undamped elements on the current rests"""
+        return float(lib().orc_lattice_lowest_angular(self._h))
+
+    def lowest(self):
+        """the lowest frequency the elements were last damped for"""
+        return float(lib().orc_lattice_lowest(self._h))
 
     def topology(self):
         """(hex nodes [hexes, 8], node masses, crack-face bits per cell)"""

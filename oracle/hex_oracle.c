@@ -228,9 +228,35 @@ static void symmetric_inverse(const double *block, const int *inert, double *out
       }
 }
 
+/* ElementDamping (element.h:33-38) and rowDamping (element.cc:29-40): the
+ * row's damping ratio from the material's target at the lattice's lowest
+ * angular frequency, 1 when either is missing */
+static double clampd(double v, double lo, double hi);
+
+typedef struct {
+  double target, lowest, theta, own;
+} damping_t;
+static const damping_t UNDAMPED = {0.0, 0.0, 1.0, 0.0};
+
+static double row_damping(const damping_t *d, double omega) {
+  if (!isfinite(d->target) || !isfinite(d->lowest) || !(d->lowest > 0.0)) return 1.0;
+  const double z = d->target * omega / d->lowest - d->own * omega / 2.0;
+  return clampd(z, 0.0, 1.0);
+}
+
+static int hex_make_damped(orc_hex *e, uint32_t cell, const uint32_t *nodes, const double *inv_mass,
+                           const double *shape, const double *hourglass, double volume, double shear,
+                           double first, double h, const damping_t *damping);
+
 int orc_hex_make(orc_hex *e, uint32_t cell, const uint32_t *nodes, const double *inv_mass,
                  const double *shape, const double *hourglass, double volume, double shear,
                  double first, double h) {
+  return hex_make_damped(e, cell, nodes, inv_mass, shape, hourglass, volume, shear, first, h, &UNDAMPED);
+}
+
+static int hex_make_damped(orc_hex *e, uint32_t cell, const uint32_t *nodes, const double *inv_mass,
+                           const double *shape, const double *hourglass, double volume, double shear,
+                           double first, double h, const damping_t *damping) {
   const double det = orc_det3(shape);
   if (!(fabs(det) > 0.0) || !(shear > 0.0) || !(volume > 0.0)) return 0;
   memset(e, 0, sizeof *e);
@@ -276,7 +302,7 @@ int orc_hex_make(orc_hex *e, uint32_t cell, const uint32_t *nodes, const double 
     inert[r] = 0;
     if (omega > 0.0) {
       double rate, soft;
-      spring_of(omega, 1.0, h, 1.0, &rate, &soft);
+      spring_of(omega, row_damping(damping, omega), h, damping->theta, &rate, &soft);
       e->rate[r] = rate;
       e->softness[r] = weight * soft;
       block[(ROWS + 1) * r] += e->softness[r];
@@ -292,7 +318,7 @@ int orc_hex_make(orc_hex *e, uint32_t cell, const uint32_t *nodes, const double 
   const double vomega = sqrt(e->vol_stiffness * vweight);
   if (vomega > 0.0) {
     double rate, soft;
-    spring_of(vomega, 1.0, h, 1.0, &rate, &soft);
+    spring_of(vomega, row_damping(damping, vomega), h, damping->theta, &rate, &soft);
     e->vol_rate = rate;
     e->vol_softness = soft;
     e->vol_cap = rate > 0.0 ? MAX_CORRECTIVE_SPEED / (rate * e->size) : INFINITY;
@@ -658,6 +684,9 @@ struct orc_lattice_body {
    * mesh changed since the elements were built */
   uint8_t *crack;
   int retopo;
+  /* the lowest angular frequency the elements were damped for
+   * (StrainLattice::lowest; 0 = not found yet) */
+  double lowest;
 };
 
 orc_lattice_body *orc_lattice_create(const orc_hex_mesh *mesh, const orc_material *materials,
@@ -1083,6 +1112,206 @@ double orc_lattice_flow(orc_lattice_body *b) {
   return largest;
 }
 
+/* ---- lowest mode (strainstep/lowestmode.cc) ----------------------------- */
+
+#define LOWEST_ITERATIONS 300u /* lowestmode.cc:16 */
+#define LOWEST_TOLERANCE 1.0e-12
+
+static double mass_dot(const double *a, const double *b, const double *m, size_t n) {
+  double s = 0.0;
+  for (size_t i = 0; i < n; ++i) s += a[i] * m[i] * b[i];
+  return s;
+}
+
+static void deflate(double *x, const double *rigid, uint32_t modes, const double *m, size_t n) {
+  for (uint32_t k = 0; k < modes; ++k) {
+    const double *mode = rigid + k * n;
+    const double c = mass_dot(mode, x, m, n);
+    for (size_t i = 0; i < n; ++i) x[i] -= c * mode[i];
+  }
+}
+
+/* lowestAngular (lowestmode.cc:31-203): inverse iteration on K v = w^2 M v
+ * over the free coordinates (pinned and massless nodes left out) with the
+ * rows' stiffnesses along their local Jacobians, rigid modes deflated and a
+ * small mass shift when nothing is held.  The shifted stiffness is factored
+ * as a band L D L^T in coordinate order (the reference factors it sparse);
+ * INFINITY without free coordinates, 0 when a pivot vanishes. */
+static double lowest_angular(const orc_hex *el, uint32_t count, const double *mass, const uint8_t *pinned_node,
+                             const double *rest, size_t nodes) {
+  int64_t *index = (int64_t *)malloc((nodes ? 3 * nodes : 1) * sizeof(int64_t));
+  double *w = (double *)malloc((nodes ? 3 * nodes : 1) * sizeof(double));
+  size_t *owner = (size_t *)malloc((nodes ? 3 * nodes : 1) * sizeof(size_t));
+  size_t n = 0;
+  int held = 0;
+  for (size_t node = 0; node < nodes; ++node) {
+    for (int a = 0; a < 3; ++a) index[3 * node + a] = -1;
+    if (pinned_node[node]) {
+      held = 1;
+      continue;
+    }
+    if (!(mass[node] > 0.0)) continue;
+    for (int a = 0; a < 3; ++a) {
+      index[3 * node + a] = (int64_t)n;
+      w[n] = mass[node];
+      owner[n++] = 3 * node + (size_t)a;
+    }
+  }
+  if (n == 0) {
+    free(index);
+    free(w);
+    free(owner);
+    return INFINITY;
+  }
+  size_t bw = 0;
+  for (uint32_t e = 0; e < count; ++e)
+    for (int i = 0; i < 24; ++i) {
+      const int64_t p = index[3 * (size_t)el[e].nodes[i / 3] + i % 3];
+      if (p < 0) continue;
+      for (int j = 0; j < 24; ++j) {
+        const int64_t q = index[3 * (size_t)el[e].nodes[j / 3] + j % 3];
+        if (q >= 0 && (size_t)(p > q ? p - q : q - p) > bw) bw = (size_t)(p > q ? p - q : q - p);
+      }
+    }
+  const size_t W = bw + 1; /* band row p holds (p, p - k), k = 0..bw */
+  double *K = (double *)calloc(n * W, sizeof(double));
+  for (uint32_t e = 0; e < count; ++e)
+    for (uint32_t row = 0; row < ROWS + 1; ++row) {
+      const double st = row < ROWS ? el[e].stiffness[row] : el[e].vol_stiffness;
+      if (!(st > 0.0)) continue;
+      double line[24];
+      for (uint32_t c = 0; c < 8; ++c) {
+        if (row < ROWS)
+          row_jacobian(&el[e], c, row, line + 3 * c);
+        else
+          memcpy(line + 3 * c, el[e].gradient + 3 * c, 3 * sizeof(double));
+      }
+      for (int i = 0; i < 24; ++i) {
+        const int64_t p = index[3 * (size_t)el[e].nodes[i / 3] + i % 3];
+        if (p < 0 || line[i] == 0.0) continue;
+        for (int j = 0; j < 24; ++j) {
+          const int64_t q = index[3 * (size_t)el[e].nodes[j / 3] + j % 3];
+          if (q >= 0 && q <= p && line[j] != 0.0) K[(size_t)p * W + (size_t)(p - q)] += st * line[i] * line[j];
+        }
+      }
+    }
+  double *rigid = (double *)malloc(6 * n * sizeof(double));
+  uint32_t modes = 0;
+  if (!held) {
+    double centre[3] = {0.0, 0.0, 0.0}, total = 0.0;
+    for (size_t i = 0; i < n; i += 3) {
+      const size_t node = owner[i] / 3;
+      for (int a = 0; a < 3; ++a) centre[a] += w[i] * rest[3 * node + a];
+      total += w[i];
+    }
+    for (int a = 0; a < 3; ++a) centre[a] /= total;
+    for (int mode = 0; mode < 6; ++mode) {
+      double *shape = rigid + modes * n;
+      for (size_t i = 0; i < n; ++i) {
+        const size_t node = owner[i] / 3;
+        const int axis = (int)(owner[i] % 3);
+        double dir[3] = {0.0, 0.0, 0.0};
+        if (mode < 3) {
+          dir[mode] = 1.0;
+        } else {
+          double spin[3] = {0.0, 0.0, 0.0}, r[3];
+          spin[mode - 3] = 1.0;
+          for (int a = 0; a < 3; ++a) r[a] = rest[3 * node + a] - centre[a];
+          cross3(spin, r, dir);
+        }
+        shape[i] = dir[axis];
+      }
+      deflate(shape, rigid, modes, w, n);
+      const double norm = sqrt(mass_dot(shape, shape, w, n));
+      if (norm > 0.0) {
+        for (size_t i = 0; i < n; ++i) shape[i] /= norm;
+        ++modes;
+      }
+    }
+  }
+  double traceK = 0.0, sumw = 0.0;
+  for (size_t i = 0; i < n; ++i) {
+    traceK += K[i * W];
+    sumw += w[i];
+  }
+  const double shift = held ? 0.0 : 1.0e-6 * traceK / sumw;
+  double *L = (double *)malloc(n * W * sizeof(double));
+  memcpy(L, K, n * W * sizeof(double));
+  for (size_t i = 0; i < n; ++i) L[i * W] += shift * w[i];
+  double *D = (double *)malloc(n * sizeof(double));
+  int ok = 1;
+  for (size_t j = 0; j < n && ok; ++j) { /* band L D L^T, in place */
+    const size_t lo = j > bw ? j - bw : 0;
+    double d = L[j * W];
+    for (size_t k = lo; k < j; ++k) d -= L[j * W + (j - k)] * L[j * W + (j - k)] * D[k];
+    if (d == 0.0) {
+      ok = 0;
+      break;
+    }
+    D[j] = d;
+    const size_t hi = j + bw < n - 1 ? j + bw : n - 1;
+    for (size_t i = j + 1; i <= hi; ++i) {
+      double v = L[i * W + (i - j)];
+      for (size_t k = (i > bw ? i - bw : 0) > lo ? (i > bw ? i - bw : 0) : lo; k < j; ++k)
+        v -= L[i * W + (i - k)] * L[j * W + (j - k)] * D[k];
+      L[i * W + (i - j)] = v / d;
+    }
+  }
+  double value = INFINITY;
+  if (ok) {
+    double *x = (double *)malloc(n * sizeof(double));
+    double *z = (double *)malloc(n * sizeof(double));
+    double *kz = (double *)malloc(n * sizeof(double));
+    for (size_t i = 0; i < n; ++i) x[i] = sin(1.7 * (double)i + 0.3) + 0.5;
+    double previous = INFINITY;
+    for (uint32_t it = 0; it < LOWEST_ITERATIONS; ++it) {
+      deflate(x, rigid, modes, w, n);
+      const double norm = sqrt(mass_dot(x, x, w, n));
+      for (size_t i = 0; i < n; ++i) x[i] /= norm;
+      for (size_t i = 0; i < n; ++i) { /* z = (K + shift M)^-1 M x */
+        double v = x[i] * w[i];
+        for (size_t k = i > bw ? i - bw : 0; k < i; ++k) v -= L[i * W + (i - k)] * z[k];
+        z[i] = v;
+      }
+      for (size_t i = 0; i < n; ++i) z[i] /= D[i];
+      for (size_t i = n; i-- > 0;) {
+        double v = z[i];
+        const size_t hi = i + bw < n - 1 ? i + bw : n - 1;
+        for (size_t k = i + 1; k <= hi; ++k) v -= L[k * W + (k - i)] * z[k];
+        z[i] = v;
+      }
+      deflate(z, rigid, modes, w, n);
+      for (size_t i = 0; i < n; ++i) kz[i] = 0.0;
+      for (size_t p = 0; p < n; ++p) { /* K z from the lower band */
+        kz[p] += K[p * W] * z[p];
+        for (size_t k = 1; k <= bw && k <= p; ++k) {
+          const double a = K[p * W + k];
+          kz[p] += a * z[p - k];
+          kz[p - k] += a * z[p];
+        }
+      }
+      double zkz = 0.0;
+      for (size_t i = 0; i < n; ++i) zkz += z[i] * kz[i];
+      value = zkz / mass_dot(z, z, w, n);
+      memcpy(x, z, n * sizeof(double));
+      if (fabs(value - previous) <= LOWEST_TOLERANCE * fabs(value)) break;
+      previous = value;
+    }
+    free(x);
+    free(z);
+    free(kz);
+  }
+  free(index);
+  free(w);
+  free(owner);
+  free(K);
+  free(L);
+  free(D);
+  free(rigid);
+  if (!ok) return 0.0;
+  return sqrt(value > 0.0 ? value : 0.0);
+}
+
 static double remaining(const double *before, const double *after, int n) {
   double was = 0.0, now = 0.0;
   for (int k = 0; k < n; ++k) {
@@ -1098,15 +1327,72 @@ static void inverse_masses(const orc_lattice_body *b, double *inv) {
   for (size_t i = 0; i < b->nodes; ++i) inv[i] = (!pinned(b, i) && b->mass[i] > 0.0f) ? 1.0 / (double)b->mass[i] : 0.0;
 }
 
+/* cellDamping (strainstep.cc:151-160): the lane's target ratio at the
+ * lattice's lowest frequency, with the implicit weight theta lowered toward
+ * the midpoint rule for a body solved exactly */
+static damping_t cell_damping(const orc_lattice_body *b, uint32_t cell, double h, double lowest) {
+  const double target = lane_mat(b, cell)->damping_ratio;
+  damping_t d = {target, isfinite(lowest) ? lowest : 0.0, 1.0, 0.0};
+  const int exact = !(h * stiffest(b) > RESOLVED);
+  if (exact && lowest > 0.0 && isfinite(lowest) && isfinite(target)) {
+    d.theta = clampd(2.0 * target / (h * lowest), 0.5, 1.0); /* MIDPOINT, strainstep.cc:58 */
+    d.own = d.theta * h;
+  }
+  return d;
+}
+
 static int make_cell(const orc_lattice_body *b, const double *inv, uint32_t cell, const double *shape,
-                     const double *hg, double volume, double h, orc_hex *out) {
+                     const double *hg, double volume, double h, const damping_t *damping, orc_hex *out) {
   const uint32_t *nd = b->hex + 8 * (size_t)cell;
   double w[8], shear, first;
   for (uint32_t c = 0; c < 8; ++c) w[c] = inv[nd[c]];
   const orc_material *m = lane_mat(b, cell);
   orc_hex_material(m->youngs_modulus, m->poissons_ratio, &shear, &first);
-  return orc_hex_make(out, cell, nd, w, shape, hg, volume, shear, first, h);
+  return hex_make_damped(out, cell, nd, w, shape, hg, volume, shear, first, h, damping);
 }
+
+/* the rest each cell's element is built on (restOf, strainstep.cc:128-136) */
+static void rest_for(const orc_lattice_body *b, uint32_t cell, double *shape, double *hg, double *vol) {
+  orc_lattice_plastic_rest(b, cell, shape, hg, vol);
+  if (!orc_hex_fit_holds(shape, hg) && b->rest_vol[cell] > 0.0) {
+    memcpy(shape, b->rest_shape + 9 * (size_t)cell, 9 * sizeof(double));
+    memcpy(hg, b->rest_hg + 12 * (size_t)cell, 12 * sizeof(double));
+    *vol = b->rest_vol[cell];
+  }
+}
+
+/* lowestAngular of the body's undamped elements on their rests
+ * (strainstep.cc:303-331) */
+static double plain_lowest(const orc_lattice_body *b, const double *inv, double h) {
+  orc_hex *plain = (orc_hex *)malloc((b->hexes ? b->hexes : 1) * sizeof(orc_hex));
+  uint32_t count = 0;
+  for (uint32_t cell = 0; cell < b->hexes; ++cell) {
+    double shape[9], hg[12], vol;
+    rest_for(b, cell, shape, hg, &vol);
+    if (make_cell(b, inv, cell, shape, hg, vol, h, &UNDAMPED, &plain[count])) ++count;
+  }
+  double *mass = (double *)malloc((b->nodes ? b->nodes : 1) * sizeof(double));
+  uint8_t *held = (uint8_t *)malloc(b->nodes ? b->nodes : 1);
+  for (size_t i = 0; i < b->nodes; ++i) {
+    mass[i] = b->mass[i];
+    held[i] = (uint8_t)pinned(b, i);
+  }
+  const double omega = lowest_angular(plain, count, mass, held, b->rest, b->nodes);
+  free(plain);
+  free(mass);
+  free(held);
+  return omega;
+}
+
+double orc_lattice_lowest_angular(const orc_lattice_body *b) {
+  double *inv = (double *)calloc(b->nodes ? b->nodes : 1, sizeof(double));
+  inverse_masses(b, inv);
+  const double omega = plain_lowest(b, inv, 1.0 / 240.0);
+  free(inv);
+  return omega;
+}
+
+double orc_lattice_lowest(const orc_lattice_body *b) { return b->lowest; }
 
 /* refreshFlowed (strainstep.cc:371-451): rebuilds the elements of the cells
  * whose plastic state moved since their element was built */
@@ -1122,7 +1408,8 @@ static void refresh_flowed(orc_lattice_body *b) {
     orc_lattice_plastic_rest(b, cell, shape, hg, &vol);
     if (!orc_hex_fit_holds(shape, hg)) continue;
     orc_hex fresh;
-    if (!make_cell(b, inv, cell, shape, hg, vol, b->h, &fresh)) continue;
+    const damping_t damping = cell_damping(b, cell, b->h, b->lowest);
+    if (!make_cell(b, inv, cell, shape, hg, vol, b->h, &damping, &fresh)) continue;
     memcpy(fresh.quat, old->quat, sizeof fresh.quat);
     memcpy(fresh.impulse, old->impulse, sizeof fresh.impulse);
     fresh.vol_impulse = old->vol_impulse;
@@ -1146,19 +1433,20 @@ static void build_elements(orc_lattice_body *b, double h) {
   inverse_masses(b, inv);
   double turn[4];
   lattice_turn(b, turn);
+  /* a damped lattice keeps the lowest frequency it was first built with
+   * (strainstep.cc:300-331) */
+  int damped = 0;
+  for (uint32_t cell = 0; cell < b->hexes; ++cell) damped = damped || isfinite(lane_mat(b, cell)->damping_ratio);
+  if (damped && !(b->lowest > 0.0 && isfinite(b->lowest))) b->lowest = plain_lowest(b, inv, h);
   orc_hex *fresh = (orc_hex *)malloc((b->hexes ? b->hexes : 1) * sizeof(orc_hex));
   uint32_t count = 0;
   for (uint32_t cell = 0; cell < b->hexes; ++cell) {
     /* restOf (strainstep.cc:128-136): the plastic rest, or the rest kept from
      * the last build when the fit does not hold */
     double shape[9], hg[12], vol;
-    orc_lattice_plastic_rest(b, cell, shape, hg, &vol);
-    if (!orc_hex_fit_holds(shape, hg) && b->rest_vol[cell] > 0.0) {
-      memcpy(shape, b->rest_shape + 9 * (size_t)cell, sizeof shape);
-      memcpy(hg, b->rest_hg + 12 * (size_t)cell, sizeof hg);
-      vol = b->rest_vol[cell];
-    }
-    if (make_cell(b, inv, cell, shape, hg, vol, h, &fresh[count])) {
+    rest_for(b, cell, shape, hg, &vol);
+    const damping_t damping = cell_damping(b, cell, h, b->lowest);
+    if (make_cell(b, inv, cell, shape, hg, vol, h, &damping, &fresh[count])) {
       memcpy(fresh[count].quat, turn, sizeof turn);
       orc_quat_matrix(turn, fresh[count].turn);
       ++count;

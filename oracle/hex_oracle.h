@@ -157,6 +157,14 @@ uint32_t orc_lattice_pieces(const orc_lattice_body *body);
  * zero before the first tear) */
 uint32_t orc_lattice_topology(const orc_lattice_body *body, uint32_t *hex_nodes, float *node_mass,
                               uint8_t *crack_faces);
+/* lowestAngular (lowestmode.cc:31-203) of the body's undamped elements on
+ * their current rests (inverse iteration, band L D L^T of the shifted
+ * stiffness): INFINITY when every node is pinned, 0 when the factor fails;
+ * and the lowest frequency the body's elements were last damped for (0
+ * before a damped body's first build).  Lanes with a finite damping ratio
+ * damp each row toward it at that frequency (cellDamping, rowDamping). */
+double orc_lattice_lowest_angular(const orc_lattice_body *body);
+double orc_lattice_lowest(const orc_lattice_body *body);
 /* symmetric 3x3 eigen decomposition by cyclic Jacobi rotations: eigenvalues
  * w and eigenvectors as the columns of V (row-major) */
 void orc_sym_eigen3(const double *A, double *w, double *V);

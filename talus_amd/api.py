@@ -260,6 +260,7 @@ _SIGNATURES = {
     "talus_mi_lattice_cell_stretch": (C.c_int, [C.c_void_p, C.c_uint32, C.c_uint32, _P(C.c_double),
                                                 _P(C.c_double)]),
     "talus_mi_lattice_pieces": (C.c_int, [C.c_void_p, C.c_uint32, _P(C.c_uint32)]),
+    "talus_mi_lattice_lowest_get": (C.c_int, [C.c_void_p, C.c_uint32, _P(C.c_double)]),
     "talus_mi_lattice_plastic_get": (C.c_int, [C.c_void_p, C.c_uint32, _P(C.c_double), _P(C.c_double),
                                                _P(C.c_uint32)]),
     "talus_mi_lattice_plastic_set": (C.c_int, [C.c_void_p, C.c_uint32, _P(C.c_double), _P(C.c_double)]),
@@ -816,6 +817,12 @@ class Context:
         s, d = C.c_double(), np.zeros(3)
         _check(lib().talus_mi_lattice_cell_stretch(self._h, lattice, int(cell), C.byref(s), _dp(d)))
         return s.value, d
+
+    def lattice_lowest(self, lattice) -> float:
+        """the lowest angular frequency the lattice's elements are damped for"""
+        out = C.c_double()
+        _check(lib().talus_mi_lattice_lowest_get(self._h, lattice, C.byref(out)))
+        return float(out.value)
 
     def lattice_pieces(self, lattice) -> int:
         out = C.c_uint32()

@@ -2595,6 +2595,15 @@ int talus_mi_lattice_cell_stretch(talus_mi_ctx *ctx, uint32_t lattice, uint32_t 
   });
 }
 
+int talus_mi_lattice_lowest_get(talus_mi_ctx *ctx, uint32_t lattice, double *lowest) {
+  return guarded(ctx, [&]() -> int {
+    if (!lattice_ok(ctx, lattice)) return fail(TALUS_MI_ERR_ARGUMENT, "no such lattice");
+    if (!lowest) return fail(TALUS_MI_ERR_ARGUMENT, "null output");
+    *lowest = ctx->lattices->body(lattice).lowest;
+    return TALUS_MI_OK;
+  });
+}
+
 int talus_mi_lattice_pieces(talus_mi_ctx *ctx, uint32_t lattice, uint32_t *pieces) {
   return guarded(ctx, [&]() -> int {
     if (!lattice_ok(ctx, lattice)) return fail(TALUS_MI_ERR_ARGUMENT, "no such lattice");

@@ -276,9 +276,17 @@ double hex_cell_frequency(float youngs, float poisson, float density, double cel
   return std::sqrt(e / ((1.0 + nu) * double(density))) / cell;
 }
 
+// rowDamping (element.cc:29-40)
+double row_damping(const HexDamping &d, double omega) {
+  if (!std::isfinite(d.target) || !std::isfinite(d.lowest) || !(d.lowest > 0.0)) return 1.0;
+  return clampd(d.target * omega / d.lowest - d.own * omega / 2.0, 0.0, 1.0);
+}
+
 bool hex_make(HexElement &e, uint32_t cell, const uint32_t *nodes, const double *inv_mass,
               const double *shape, const double *hourglass, double volume, double shear,
-              double first, double h) {
+              double first, double h, const HexDamping *damping) {
+  const HexDamping undamped{};
+  const HexDamping &dmp = damping ? *damping : undamped;
   const double det = det3(shape);
   if (!(std::fabs(det) > 0.0) || !(shear > 0.0) || !(volume > 0.0)) return false;
   e = HexElement{};
@@ -322,7 +330,7 @@ bool hex_make(HexElement &e, uint32_t cell, const uint32_t *nodes, const double 
     e.stiffness[r] = modulus * volume;
     if (omega > 0.0) {
       double rate, soft;
-      spring_of(omega, 1.0, h, 1.0, &rate, &soft);
+      spring_of(omega, row_damping(dmp, omega), h, dmp.theta, &rate, &soft);
       e.rate[r] = rate;
       e.softness[r] = weight * soft;
       block[(HEX_ROWS + 1) * r] += e.softness[r];
@@ -338,7 +346,7 @@ bool hex_make(HexElement &e, uint32_t cell, const uint32_t *nodes, const double 
   const double vomega = std::sqrt(e.vol_stiffness * vweight);
   if (vomega > 0.0) {
     double rate, soft;
-    spring_of(vomega, 1.0, h, 1.0, &rate, &soft);
+    spring_of(vomega, row_damping(dmp, vomega), h, dmp.theta, &rate, &soft);
     e.vol_rate = rate;
     e.vol_softness = soft;
     e.vol_cap = rate > 0.0 ? MAX_CORRECTIVE_SPEED / (rate * e.size) : INF;
@@ -552,6 +560,9 @@ void LatticeBody::build_elements(double step) {
       hex_rotation_of(M, id, 64, q);
     }
   }
+  // a damped lattice keeps the lowest frequency it was first built with
+  // (strainstep.cc:300-331)
+  if (damped && !(lowest > 0.0 && std::isfinite(lowest))) lowest = lowest_angular(step);
   std::vector<HexElement> built(hexes);
   std::vector<uint8_t> ok(hexes, 0);
   const bool kept = rest_vol.size() == hexes;
@@ -574,7 +585,8 @@ void LatticeBody::build_elements(double step) {
       const talus_mi_material &m = material_of(cell_index);
       hex_material(m.youngs_modulus, m.poissons_ratio, &shear, &firstl);
       HexElement &e = built[cell_index];
-      if (hex_make(e, cell_index, nd, w, shape, hgv, vol, shear, firstl, step)) {
+      const HexDamping damping = cell_damping(cell_index, step);
+      if (hex_make(e, cell_index, nd, w, shape, hgv, vol, shear, firstl, step, &damping)) {
         std::memcpy(e.quat, q, sizeof q);
         ok[cell_index] = 1;
       }
@@ -693,7 +705,8 @@ uint32_t LatticeBody::refresh_flowed() {
     const talus_mi_material &m = material_of(cell_index);
     hex_material(m.youngs_modulus, m.poissons_ratio, &shear, &firstl);
     HexElement fresh;
-    if (!hex_make(fresh, cell_index, nd, w, shape, hgv, vol, shear, firstl, h)) continue;
+    const HexDamping damping = cell_damping(cell_index, h);
+    if (!hex_make(fresh, cell_index, nd, w, shape, hgv, vol, shear, firstl, h, &damping)) continue;
     std::memcpy(fresh.quat, old.quat, sizeof fresh.quat);
     std::memcpy(fresh.impulse, old.impulse, sizeof fresh.impulse);
     fresh.vol_impulse = old.vol_impulse;
@@ -712,6 +725,214 @@ uint32_t LatticeBody::refresh_flowed() {
     ++rebuilt;
   }
   return rebuilt;
+}
+
+
+// ---- lowest mode (strainstep/lowestmode.cc) --------------------------------
+
+namespace {
+
+double mass_dot(const std::vector<double> &a, const std::vector<double> &b, const std::vector<double> &m) {
+  double s = 0.0;
+  for (size_t i = 0; i < a.size(); ++i) s += a[i] * m[i] * b[i];
+  return s;
+}
+
+void deflate(std::vector<double> &x, const std::vector<std::vector<double>> &rigid, const std::vector<double> &m) {
+  for (const auto &mode : rigid) {
+    const double c = mass_dot(mode, x, m);
+    for (size_t i = 0; i < x.size(); ++i) x[i] -= c * mode[i];
+  }
+}
+
+}  // namespace
+
+double hex_lowest_angular(const std::vector<HexElement> &el, const std::vector<double> &mass,
+                          const std::vector<uint8_t> &pinned_node, const std::vector<double> &rest) {
+  const size_t nodes = mass.size();
+  std::vector<int64_t> index(3 * nodes, -1);
+  std::vector<double> w;
+  std::vector<size_t> owner;
+  bool held = false;
+  for (size_t node = 0; node < nodes; ++node) {
+    if (pinned_node[node]) {
+      held = true;
+      continue;
+    }
+    if (!(mass[node] > 0.0)) continue;
+    for (int a = 0; a < 3; ++a) {
+      index[3 * node + a] = int64_t(w.size());
+      w.push_back(mass[node]);
+      owner.push_back(3 * node + size_t(a));
+    }
+  }
+  const size_t n = w.size();
+  if (n == 0) return INF;
+  size_t bw = 0;
+  for (const HexElement &e : el)
+    for (int i = 0; i < 24; ++i) {
+      const int64_t p = index[3 * size_t(e.nodes[i / 3]) + i % 3];
+      if (p < 0) continue;
+      for (int j = 0; j < 24; ++j) {
+        const int64_t q = index[3 * size_t(e.nodes[j / 3]) + j % 3];
+        if (q >= 0) bw = std::max(bw, size_t(p > q ? p - q : q - p));
+      }
+    }
+  const size_t W = bw + 1;
+  std::vector<double> K(n * W, 0.0);
+  for (const HexElement &e : el)
+    for (uint32_t row = 0; row < HEX_ROWS + 1; ++row) {
+      const double st = row < HEX_ROWS ? e.stiffness[row] : e.vol_stiffness;
+      if (!(st > 0.0)) continue;
+      double line[24];
+      for (uint32_t c = 0; c < 8; ++c) {
+        if (row < HEX_ROWS)
+          row_jacobian(e, c, row, line + 3 * c);
+        else
+          std::memcpy(line + 3 * c, e.gradient + 3 * c, 3 * sizeof(double));
+      }
+      for (int i = 0; i < 24; ++i) {
+        const int64_t p = index[3 * size_t(e.nodes[i / 3]) + i % 3];
+        if (p < 0 || line[i] == 0.0) continue;
+        for (int j = 0; j < 24; ++j) {
+          const int64_t q = index[3 * size_t(e.nodes[j / 3]) + j % 3];
+          if (q >= 0 && q <= p && line[j] != 0.0) K[size_t(p) * W + size_t(p - q)] += st * line[i] * line[j];
+        }
+      }
+    }
+  std::vector<std::vector<double>> rigid;
+  if (!held) {
+    double centre[3] = {0.0, 0.0, 0.0}, total = 0.0;
+    for (size_t i = 0; i < n; i += 3) {
+      const size_t node = owner[i] / 3;
+      for (int a = 0; a < 3; ++a) centre[a] += w[i] * rest[3 * node + a];
+      total += w[i];
+    }
+    for (int a = 0; a < 3; ++a) centre[a] /= total;
+    for (int mode = 0; mode < 6; ++mode) {
+      std::vector<double> shape(n);
+      for (size_t i = 0; i < n; ++i) {
+        const size_t node = owner[i] / 3;
+        const int axis = int(owner[i] % 3);
+        double dir[3] = {0.0, 0.0, 0.0};
+        if (mode < 3) {
+          dir[mode] = 1.0;
+        } else {
+          double spin[3] = {0.0, 0.0, 0.0}, r[3];
+          spin[mode - 3] = 1.0;
+          for (int a = 0; a < 3; ++a) r[a] = rest[3 * node + a] - centre[a];
+          dir[0] = spin[1] * r[2] - spin[2] * r[1];
+          dir[1] = spin[2] * r[0] - spin[0] * r[2];
+          dir[2] = spin[0] * r[1] - spin[1] * r[0];
+        }
+        shape[i] = dir[axis];
+      }
+      deflate(shape, rigid, w);
+      const double norm = std::sqrt(mass_dot(shape, shape, w));
+      if (norm > 0.0) {
+        for (double &v : shape) v /= norm;
+        rigid.push_back(std::move(shape));
+      }
+    }
+  }
+  double traceK = 0.0, sumw = 0.0;
+  for (size_t i = 0; i < n; ++i) {
+    traceK += K[i * W];
+    sumw += w[i];
+  }
+  const double shift = held ? 0.0 : 1.0e-6 * traceK / sumw;
+  std::vector<double> L(K), D(n);
+  for (size_t i = 0; i < n; ++i) L[i * W] += shift * w[i];
+  for (size_t j = 0; j < n; ++j) {  // band L D L^T, in place
+    const size_t lo = j > bw ? j - bw : 0;
+    double d = L[j * W];
+    for (size_t k = lo; k < j; ++k) d -= L[j * W + (j - k)] * L[j * W + (j - k)] * D[k];
+    if (d == 0.0) return 0.0;
+    D[j] = d;
+    const size_t hi = std::min(j + bw, n - 1);
+    for (size_t i = j + 1; i <= hi; ++i) {
+      double v = L[i * W + (i - j)];
+      for (size_t k = std::max(i > bw ? i - bw : 0, lo); k < j; ++k) v -= L[i * W + (i - k)] * L[j * W + (j - k)] * D[k];
+      L[i * W + (i - j)] = v / d;
+    }
+  }
+  std::vector<double> x(n), z(n), kz(n);
+  for (size_t i = 0; i < n; ++i) x[i] = std::sin(1.7 * double(i) + 0.3) + 0.5;
+  double previous = INF, value = INF;
+  for (uint32_t it = 0; it < 300; ++it) {  // ITERATIONS, TOLERANCE (lowestmode.cc:16-17)
+    deflate(x, rigid, w);
+    const double norm = std::sqrt(mass_dot(x, x, w));
+    for (double &v : x) v /= norm;
+    for (size_t i = 0; i < n; ++i) {
+      double v = x[i] * w[i];
+      for (size_t k = i > bw ? i - bw : 0; k < i; ++k) v -= L[i * W + (i - k)] * z[k];
+      z[i] = v;
+    }
+    for (size_t i = 0; i < n; ++i) z[i] /= D[i];
+    for (size_t i = n; i-- > 0;) {
+      double v = z[i];
+      const size_t hi = std::min(i + bw, n - 1);
+      for (size_t k = i + 1; k <= hi; ++k) v -= L[k * W + (k - i)] * z[k];
+      z[i] = v;
+    }
+    deflate(z, rigid, w);
+    std::fill(kz.begin(), kz.end(), 0.0);
+    for (size_t p = 0; p < n; ++p) {
+      kz[p] += K[p * W] * z[p];
+      for (size_t k = 1; k <= bw && k <= p; ++k) {
+        const double a = K[p * W + k];
+        kz[p] += a * z[p - k];
+        kz[p - k] += a * z[p];
+      }
+    }
+    double zkz = 0.0;
+    for (size_t i = 0; i < n; ++i) zkz += z[i] * kz[i];
+    value = zkz / mass_dot(z, z, w);
+    x = z;
+    if (std::fabs(value - previous) <= 1.0e-12 * std::fabs(value)) break;
+    previous = value;
+  }
+  return std::sqrt(value > 0.0 ? value : 0.0);
+}
+
+HexDamping LatticeBody::cell_damping(uint32_t cell_index, double step) const {
+  const double target = material_of(cell_index).damping_ratio;
+  HexDamping d{target, std::isfinite(lowest) ? lowest : 0.0, 1.0, 0.0};
+  const bool exact = !(step * stiffest() > 1.5);
+  if (exact && lowest > 0.0 && std::isfinite(lowest) && std::isfinite(target)) {
+    d.theta = clampd(2.0 * target / (step * lowest), 0.5, 1.0);  // MIDPOINT (strainstep.cc:58)
+    d.own = d.theta * step;
+  }
+  return d;
+}
+
+double LatticeBody::lowest_angular(double step) const {
+  std::vector<double> inv(nodes, 0.0), m(nodes);
+  std::vector<uint8_t> held(nodes);
+  for (size_t i = 0; i < nodes; ++i) {
+    if (!pinned(i) && mass[i] > 0.0f) inv[i] = 1.0 / double(mass[i]);
+    m[i] = mass[i];
+    held[i] = pinned(i) ? 1 : 0;
+  }
+  const bool kept = rest_vol.size() == hexes;
+  std::vector<HexElement> plain;
+  plain.reserve(hexes);
+  for (uint32_t c = 0; c < hexes; ++c) {
+    double shape[9], hgv[12], vol, w[8], shear, firstl;
+    plastic_rest(c, shape, hgv, &vol);
+    if (!hex_fit_holds(shape, hgv) && kept && rest_vol[c] > 0.0) {
+      std::memcpy(shape, &rest_shape[9 * size_t(c)], 9 * sizeof(double));
+      std::memcpy(hgv, &rest_hg[12 * size_t(c)], 12 * sizeof(double));
+      vol = rest_vol[c];
+    }
+    const uint32_t *nd = &hex[8 * size_t(c)];
+    for (uint32_t k = 0; k < 8; ++k) w[k] = inv[nd[k]];
+    const talus_mi_material &mat = material_of(c);
+    hex_material(mat.youngs_modulus, mat.poissons_ratio, &shear, &firstl);
+    HexElement e;
+    if (hex_make(e, c, nd, w, shape, hgv, vol, shear, firstl, step)) plain.push_back(e);
+  }
+  return hex_lowest_angular(plain, m, held, rest);
 }
 
 // ---- cell tearing (strainstep/tearing.cc) ---------------------------------
@@ -1096,11 +1317,6 @@ int make_lattice(LatticeBody &b, const talus_mi_hex_mesh *mesh, const talus_mi_m
   if (mesh->nodes == 0 || mesh->hexes == 0) return fail(TALUS_MI_ERR_ARGUMENT, "empty hex mesh");
   if (!materials || lanes == 0)
     return fail(TALUS_MI_ERR_ARGUMENT, "a strainstep lattice needs its material lanes");
-  for (uint32_t l = 0; l < lanes; ++l) {
-    const talus_mi_material &m = materials[l];
-    if (std::isfinite(m.damping_ratio))
-      return fail(TALUS_MI_ERR_ARGUMENT, "lattice damping ratios (lowest-mode damping) are not supported");
-  }
   for (uint32_t c = 0; c < 8 * mesh->hexes; ++c)
     if (mesh->hex_nodes[c] >= mesh->nodes) return fail(TALUS_MI_ERR_ARGUMENT, "hex node out of range");
   for (int k = 0; k < 3; ++k)
@@ -1138,6 +1354,26 @@ int make_lattice(LatticeBody &b, const talus_mi_hex_mesh *mesh, const talus_mi_m
   }
   for (uint32_t c = 0; c < b.hexes; ++c) b.yields = b.yields || std::isfinite(b.material_of(c).yield_stress);
   for (uint32_t c = 0; c < b.hexes; ++c) b.tears = b.tears || std::isfinite(b.tear_limit(c));
+  for (uint32_t c = 0; c < b.hexes; ++c) b.damped = b.damped || std::isfinite(b.material_of(c).damping_ratio);
+  if (b.damped) {
+    // the lowest-mode solve factors a band of the lattice's stiffness: its
+    // cost (free coordinates x band width squared) and size are bounded
+    std::vector<int64_t> index(n, -1);
+    int64_t free_nodes = 0;
+    for (size_t i = 0; i < n; ++i)
+      if (!b.pinned(i) && b.mass[i] > 0.0f) index[i] = free_nodes++;
+    int64_t band = 0;
+    for (uint32_t c = 0; c < b.hexes; ++c)
+      for (int i = 0; i < 8; ++i)
+        for (int j = 0; j < 8; ++j) {
+          const int64_t p = index[b.hex[8 * size_t(c) + i]], q = index[b.hex[8 * size_t(c) + j]];
+          if (p >= 0 && q >= 0) band = std::max(band, 3 * std::abs(p - q) + 2);
+        }
+    const double coords = 3.0 * double(free_nodes), width = double(band + 1);
+    if (coords * width * width > 4.0e10 || coords * width > 1.0e8)
+      return fail(TALUS_MI_ERR_ARGUMENT,
+                  "a damped lattice this large exceeds the banded lowest-mode solve (damping ratio must be infinite)");
+  }
   b.hex_colour.resize(b.hexes);
   for (uint32_t c = 0; c < b.hexes; ++c) b.hex_colour[c] = b.node_colour[b.hex[8 * size_t(c)]];
   // a colour's hexes are solved at once: no two of them may share a node

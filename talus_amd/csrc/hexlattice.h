@@ -41,12 +41,23 @@ struct HexElement {
   double quat[4] = {1, 0, 0, 0};  // w, x, y, z
 };
 
+// ElementDamping (element.h:33-38): the lane's target ratio, the lattice's
+// lowest angular frequency, the implicit weight theta and theta h
+struct HexDamping {
+  double target = 0.0, lowest = 0.0, theta = 1.0, own = 0.0;
+};
+
 void hex_corner_rest(const double *corners, double *shape, double *hourglass);
 void hex_material(float youngs, float poisson, double *shear, double *first);
 double hex_cell_frequency(float youngs, float poisson, float density, double cell);
 bool hex_make(HexElement &e, uint32_t cell, const uint32_t *nodes, const double *inv_mass,
               const double *shape, const double *hourglass, double volume, double shear,
-              double first, double h);
+              double first, double h, const HexDamping *damping = nullptr);
+// lowestAngular (lowestmode.cc:31-203) of elements over nodes (mass, pinned
+// flag, rest position): inverse iteration with a band L D L^T of the shifted
+// stiffness; infinity without a free coordinate, 0 when a pivot vanishes
+double hex_lowest_angular(const std::vector<HexElement> &elements, const std::vector<double> &mass,
+                          const std::vector<uint8_t> &pinned, const std::vector<double> &rest);
 // rotationOf on the host: the quaternion iteration maximising tr(R^T F)
 void hex_rotation_of(const double *F, const double *guess, uint32_t iterations, double *out);
 // updateElement's rotation and capped strain rows at positions x (the
@@ -105,6 +116,13 @@ struct LatticeBody {
   // 2 axis + side; empty before the first tear) and whether the mesh changed
   // since the elements were built
   bool tears = false;             // a lane has a finite tear strain
+  // damping (cellDamping, strainstep.cc:151-160): the lowest angular
+  // frequency the elements were damped for (0 = not found yet)
+  bool damped = false;            // a lane has a finite damping ratio
+  double lowest = 0.0;
+  HexDamping cell_damping(uint32_t cell, double h) const;
+  // lowestAngular of the undamped elements on their rests
+  double lowest_angular(double h) const;
   std::vector<uint8_t> crack;     // hexes
   bool retopo = false;
   // 1 + the cell's lane tear strain (infinite when the lane never tears)

@@ -295,12 +295,14 @@ def test_hex_mesh_counts_masses_and_colours():
     assert mesh["mass"].max() == 8 * mesh["mass"].min()
 
 
-def cantilever(thickness, cell, poisson, substeps, steps, colored=False):
+def cantilever(thickness, cell, poisson, substeps, steps, colored=False, damping=math.inf, change=None):
     """cantilever (dynamics_strainstep_tests.cc:305-360): a 4 m beam held at
-    x = -2, tip sag averaged over the last half of the frames."""
+    x = -2, tip sag averaged over the last half of the frames; change =
+    (frame, substeps) asks for other substeps from that frame on."""
     length = 4.0
     mat = jelly(5.0e5, density=5.0, poisson=poisson)
     mat.friction = 0.5
+    mat.damping_ratio = damping
     clamp = [hold_at((-length / 2, 0.0, 0.0), (0.01, 1.0, 1.0))]
     mesh = box((length / 2, thickness / 2, thickness / 2), cell, mat, clamp)
     body = H.LatticeBody(mesh, [mat, mat])
@@ -309,6 +311,8 @@ def cantilever(thickness, cell, poisson, substeps, steps, colored=False):
     window = steps // 2
     sag = 0.0
     for frame in range(1, steps + 1):
+        if change is not None and frame == change[0]:
+            body.set_substeps(change[1])
         body.step(colored=colored)
         if frame > steps - window:
             x, _ = body.state()
@@ -703,3 +707,75 @@ def test_strip_pulled_apart_tears_into_pieces():
     assert crack.any()
     x, v = body.state()
     assert np.isfinite(x).all() and np.isfinite(v).all()
+
+
+# ---- lowest mode and damping ----------------------------------------------
+
+def beam_lowest(poisson=0.0, held=True):
+    mat = jelly(1.0e5, density=5.0, poisson=poisson)
+    clamp = [hold_at((-2.0, 0.0, 0.0), (0.01, 1.0, 1.0))] if held else []
+    mesh = box((2.0, 0.2, 0.2), 0.2, mat, clamp)
+    body = H.LatticeBody(mesh, [mat, mat])
+    inertia, area = 0.4 ** 4 / 12.0, 0.4 * 0.4
+    root = 1.8751 if held else 4.7300
+    theory = root * root * math.sqrt(1.0e5 * inertia / (5.0 * area * 4.0 ** 4))
+    return body.lowest_angular(), theory, mesh["hex"].shape[0]
+
+
+@pytest.mark.parametrize("poisson,held", [(0.0, True), (0.0, False), (0.4, True)])
+def test_lowest_mode_of_a_beam_is_beam_theory(poisson, held):
+    """dynamics_strainstep_tests.cc:1536-1563 (clamped), 1565-1592 (free),
+    1614-1650 (clamped, volume row on): within 10 % of Euler-Bernoulli"""
+    omega, theory, cells = beam_lowest(poisson, held)
+    assert cells == 80
+    assert abs(omega - theory) / theory < 0.1
+
+
+def test_lowest_mode_of_a_free_cell_is_its_hourglass_mode():
+    """dynamics_strainstep_tests.cc:1594-1612: 2 / side sqrt(E / 3 rho)"""
+    mat = jelly(1.0e5, density=5.0, poisson=0.0)
+    mesh = box((0.1, 0.1, 0.1), 0.2, mat)
+    body = H.LatticeBody(mesh, [mat])
+    expected = 2.0 / 0.2 * math.sqrt(1.0e5 / (3.0 * 5.0))
+    assert abs(body.lowest_angular() - expected) <= 1e-6 * expected
+
+
+def test_lowest_mode_of_a_pinned_lattice_is_infinite():
+    """dynamics_strainstep_tests.cc:1652-1666"""
+    mat = jelly(1.0e5, density=5.0)
+    mesh = box((0.1, 0.1, 0.1), 0.2, mat, [hold_at((0.0, 0.0, 0.0), (1.0, 1.0, 1.0))])
+    assert mesh["anchored"].all()
+    assert H.LatticeBody(mesh, [mat]).lowest_angular() == math.inf
+
+
+@pytest.mark.parametrize("change", [None, (100, 32)])
+def test_damped_cantilever_bends_as_beam_theory(change):
+    """dynamics_strainstep_tests.cc:1852-1870: damping ratio 0.2, 16
+    substeps, and 32 from frame 100 (the lowest frequency found at the first
+    build is kept through the rebuild)"""
+    sag, theory, cells = cantilever(0.4, 0.2, 0.3, 16, 600, damping=0.2, change=change)
+    assert cells == 80
+    assert abs(sag - theory) / theory < 0.1
+
+
+def test_a_damped_lattice_finds_its_lowest_frequency_once():
+    """buildElements (strainstep.cc:300-331): the first build of a damped
+    lattice finds the lowest frequency of its undamped elements; rebuilds for
+    other substeps keep it; an undamped lattice never looks for it"""
+    mat = jelly(5.0e5, density=5.0, poisson=0.3)
+    mat.damping_ratio = 0.2
+    clamp = [hold_at((-2.0, 0.0, 0.0), (0.01, 1.0, 1.0))]
+    mesh = box((2.0, 0.2, 0.2), 0.2, mat, clamp)
+    body = H.LatticeBody(mesh, [mat, mat])
+    assert body.lowest() == 0.0
+    first = body.lowest_angular()
+    body.step()
+    assert body.lowest() == first > 0.0
+    for _ in range(5):
+        body.step()
+    body.set_substeps(32)
+    body.step()
+    assert body.lowest() == first
+    plain = H.LatticeBody(mesh, [jelly(5.0e5, density=5.0, poisson=0.3)] * 2)
+    plain.step()
+    assert plain.lowest() == 0.0
