@@ -1459,6 +1459,9 @@ void LatticeSet::step(const talus_mi_lattice_params &p, talus_mi_lattice_report 
         rep.node_copies += made;
         if (made) device_valid_ = false;
       }
+      // the flow flags are not read once the arrays are laid out again: the
+      // bodies that did not tear compare P with their built P instead
+      if (!device_valid_ && flowed_) refresh_pending_ = true;
     }
   }
 }
@@ -1466,7 +1469,10 @@ void LatticeSet::step(const talus_mi_lattice_params &p, talus_mi_lattice_report 
 uint32_t LatticeSet::tear(uint32_t i, const uint32_t *cells, uint32_t count) {
   download();
   const uint32_t made = bodies_.at(i).tear_cells(cells, count);
-  if (made) device_valid_ = false;
+  if (made) {
+    device_valid_ = false;
+    if (flowed_) refresh_pending_ = true;  // as after a tear inside a step
+  }
   return made;
 }
 

@@ -186,3 +186,37 @@ def test_strip_pulled_apart_tears_like_the_oracle():
         assert np.isfinite(x).all() and np.isfinite(v).all()
         ox, _ = orc.state()
         assert float(np.abs(x - ox).max()) <= 1e-6
+
+
+def test_a_tear_keeps_the_plastic_refresh_of_the_other_bodies():
+    """A tear lays the device arrays out again; a plastic body stepped in the
+    same context whose cells flowed in that step still has their elements
+    rebuilt (refreshFlowed) at the next step, bit for bit as when it is
+    stepped alone"""
+    from test_gpu_lattice import clay_block
+    clay_mesh, clay = clay_block()
+    mat = tearable()
+    strip = box((1.0, 0.25, 0.25), 0.5, mat)
+    states = []
+    for with_strip in (False, True):
+        with api.Context() as ctx:
+            s = ctx.add_lattice(strip, [mat, mat], offset=(0.0, 5.0, 0.0)) if with_strip else None
+            b = ctx.add_lattice(clay_mesh, [clay])
+            refreshed = 0
+            for frame in range(30):
+                r = ctx.lattice_step()
+                refreshed += r["refreshed"]
+                if with_strip and frame == 10:
+                    assert ctx.lattice_plastic(b) is not None
+                    x, v = ctx.lattice_state(s)
+                    x[:, 0] *= 1.01  # a stretch along the strip sets the crack across it
+                    ctx.set_lattice_state(s, x, v)
+                    assert ctx.lattice_tear(s, [1]) == 4
+            states.append((ctx.lattice_state(b), ctx.lattice_plastic(b), refreshed))
+    (x0, v0), (P0, a0), n0 = states[0]
+    (x1, v1), (P1, a1), n1 = states[1]
+    assert n0 > 0 and n0 == n1
+    np.testing.assert_array_equal(x1, x0)
+    np.testing.assert_array_equal(v1, v0)
+    np.testing.assert_array_equal(P1, P0)
+    np.testing.assert_array_equal(a1, a0)
