@@ -519,6 +519,13 @@ int talus_mi_lattice_cell_stretch(talus_mi_ctx *ctx, uint32_t lattice, uint32_t 
  * elements are damped for; 0 before a damped body's first step and for an
  * undamped body. */
 int talus_mi_lattice_lowest_get(talus_mi_ctx *ctx, uint32_t lattice, double *lowest);
+/* PreparedMesh::surfaceNodes, ascending: buildMesh's surface nodes (the
+ * corners of every face no other hex shares, solver.cc:2369-2413) with
+ * markCracks's additions after tears (crack-face nodes and the copies of
+ * surface nodes, tearing.cc:426-484).  *count = the number; the nodes are
+ * written when nodes is not NULL and capacity holds them. */
+int talus_mi_lattice_surface_get(talus_mi_ctx *ctx, uint32_t lattice, uint32_t *nodes,
+                                 uint32_t capacity, uint32_t *count);
 /* latticePieces (resolution.cc:198-228): pieces of cells joined by nodes. */
 int talus_mi_lattice_pieces(talus_mi_ctx *ctx, uint32_t lattice, uint32_t *pieces);
 /* One solveStrainstep over every lattice of the context, each its own

@@ -109,6 +109,8 @@ def lib():
         L.orc_lattice_overstretched.restype = C.c_uint32
         L.orc_lattice_tear.argtypes = [C.c_void_p, u32p, C.c_uint32]
         L.orc_lattice_tear.restype = C.c_uint32
+        L.orc_lattice_surface.argtypes = [C.c_void_p, u32p]
+        L.orc_lattice_surface.restype = C.c_uint32
         L.orc_lattice_pieces.argtypes = [C.c_void_p]
         L.orc_lattice_pieces.restype = C.c_uint32
         L.orc_lattice_topology.argtypes = [C.c_void_p, u32p, C.POINTER(C.c_float), u8p]
@@ -413,6 +415,13 @@ class LatticeBody:
 
     def pieces(self):
         return int(lib().orc_lattice_pieces(self._h))
+
+    def surface(self):
+        """surfaceNodes, ascending"""
+        n = int(lib().orc_lattice_surface(self._h, None))
+        out = np.zeros(max(n, 1), dtype=np.uint32)
+        lib().orc_lattice_surface(self._h, out.ctypes.data_as(C.POINTER(C.c_uint32)))
+        return out[:n]
 
     def lowest_angular(self):
         """lowestAngular of the undamped elements on the current rests"""

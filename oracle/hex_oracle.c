@@ -684,6 +684,10 @@ struct orc_lattice_body {
    * mesh changed since the elements were built */
   uint8_t *crack;
   int retopo;
+  /* PreparedMesh::surfaceNodes as a per-node flag (NULL until first asked
+   * for or torn) */
+  uint8_t *surface;
+  uint32_t surface_nodes; /* nodes the flags cover */
   /* the lowest angular frequency the elements were damped for
    * (StrainLattice::lowest; 0 = not found yet) */
   double lowest;
@@ -763,6 +767,7 @@ void orc_lattice_free(orc_lattice_body *b) {
   free(b->rest_hg);
   free(b->rest_vol);
   free(b->crack);
+  free(b->surface);
   free(b);
 }
 
@@ -1848,10 +1853,37 @@ static void recompute_masses(orc_lattice_body *b) {
   }
 }
 
+/* the surface nodes buildMesh marks (solver.cc:2369-2413): the corners of
+ * every cell face no other cell shares (for a lattice the grid builds, the
+ * faces whose neighbour cell is not solid or off the grid) */
+static void ensure_surface(orc_lattice_body *b) {
+  if (b->surface) return;
+  b->surface = (uint8_t *)calloc(b->nodes ? b->nodes : 1, 1);
+  for (uint32_t c = 0; c < b->hexes; ++c)
+    for (uint32_t face = 0; face < 6; ++face) {
+      uint32_t f[4];
+      face_of(b, c, face, f);
+      int shared = 0;
+      for (uint32_t o = 0; o < b->hexes && !shared; ++o) {
+        if (o == c) continue;
+        for (uint32_t g = 0; g < 6 && !shared; ++g) {
+          uint32_t h[4];
+          face_of(b, o, g, h);
+          shared = !memcmp(f, h, sizeof f);
+        }
+      }
+      if (!shared)
+        for (int k = 0; k < 4; ++k) b->surface[f[k]] = 1;
+    }
+}
+
 /* markCracks (tearing.cc:426-484): the faces of each cell whose grid
- * neighbour's facing face no longer holds the same nodes (surface nodes are
- * not kept here) */
-static void mark_cracks(orc_lattice_body *b) {
+ * neighbour's facing face no longer holds the same nodes, their nodes put on
+ * the surface, then each copy of a surface node */
+static void mark_cracks(orc_lattice_body *b, const copies_t *copies) {
+  b->surface = (uint8_t *)realloc(b->surface, b->nodes);
+  for (uint32_t i = b->surface_nodes; i < b->nodes; ++i) b->surface[i] = 0;
+  b->surface_nodes = b->nodes;
   float lo[3] = {INFINITY, INFINITY, INFINITY};
   for (uint32_t i = 0; i < b->nodes; ++i)
     for (int k = 0; k < 3; ++k) {
@@ -1879,26 +1911,47 @@ static void mark_cracks(orc_lattice_body *b) {
         uint32_t mine[4], theirs[4];
         face_of(b, c, 2 * axis + side, mine);
         face_of(b, found, 2 * axis + (1 - side), theirs);
-        if (memcmp(mine, theirs, sizeof mine)) b->crack[c] |= (uint8_t)(1u << (2 * axis + side));
+        if (memcmp(mine, theirs, sizeof mine)) {
+          b->crack[c] |= (uint8_t)(1u << (2 * axis + side));
+          for (int k = 0; k < 4; ++k) b->surface[mine[k]] = 1;
+        }
       }
   }
   free(index);
+  for (uint32_t i = 0; i < copies->n; ++i)
+    if (b->surface[copies->original[i]]) b->surface[copies->copy[i]] = 1;
 }
 
 uint32_t orc_lattice_tear(orc_lattice_body *b, const uint32_t *cells, uint32_t count) {
   if (count == 0) return 0;
+  ensure_surface(b);
+  b->surface_nodes = b->nodes;
   copies_t copies = {NULL, NULL, 0, 0};
   for (uint32_t i = 0; i < count; ++i) (void)crack_through_cell(b, cells[i], &copies);
   const uint32_t made0 = copies.n;
   if (made0 == 0) return 0;
   split_pinched(b, &copies);
   recompute_masses(b);
-  mark_cracks(b);
+  mark_cracks(b, &copies);
   b->retopo = 1;
   const uint32_t made = copies.n;
   free(copies.copy);
   free(copies.original);
   return made;
+}
+
+uint32_t orc_lattice_surface(orc_lattice_body *b, uint32_t *nodes) {
+  if (!b->surface) {
+    ensure_surface(b);
+    b->surface_nodes = b->nodes;
+  }
+  uint32_t n = 0;
+  for (uint32_t i = 0; i < b->nodes; ++i)
+    if (b->surface[i]) {
+      if (nodes) nodes[n] = i;
+      ++n;
+    }
+  return n;
 }
 
 uint32_t orc_lattice_pieces(const orc_lattice_body *b) {

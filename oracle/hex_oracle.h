@@ -152,6 +152,10 @@ void orc_lattice_cell_stretch(const orc_lattice_body *body, uint32_t cell, doubl
 uint32_t orc_lattice_overstretched(const orc_lattice_body *body, uint32_t *cells);
 uint32_t orc_lattice_tear(orc_lattice_body *body, const uint32_t *cells, uint32_t count);
 uint32_t orc_lattice_pieces(const orc_lattice_body *body);
+/* PreparedMesh::surfaceNodes, ascending: buildMesh's (the corners of faces
+ * no other cell shares), then markCracks's additions; returns the count and
+ * writes the nodes when not NULL */
+uint32_t orc_lattice_surface(orc_lattice_body *body, uint32_t *nodes);
 /* the current node count, and (each when not null) the hex corner nodes
  * (8 per cell), node masses and per-cell crack-face bits (bit 2 axis + side,
  * zero before the first tear) */

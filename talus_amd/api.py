@@ -261,6 +261,8 @@ _SIGNATURES = {
                                                 _P(C.c_double)]),
     "talus_mi_lattice_pieces": (C.c_int, [C.c_void_p, C.c_uint32, _P(C.c_uint32)]),
     "talus_mi_lattice_lowest_get": (C.c_int, [C.c_void_p, C.c_uint32, _P(C.c_double)]),
+    "talus_mi_lattice_surface_get": (C.c_int, [C.c_void_p, C.c_uint32, _P(C.c_uint32), C.c_uint32,
+                                               _P(C.c_uint32)]),
     "talus_mi_lattice_plastic_get": (C.c_int, [C.c_void_p, C.c_uint32, _P(C.c_double), _P(C.c_double),
                                                _P(C.c_uint32)]),
     "talus_mi_lattice_plastic_set": (C.c_int, [C.c_void_p, C.c_uint32, _P(C.c_double), _P(C.c_double)]),
@@ -823,6 +825,15 @@ class Context:
         out = C.c_double()
         _check(lib().talus_mi_lattice_lowest_get(self._h, lattice, C.byref(out)))
         return float(out.value)
+
+    def lattice_surface(self, lattice):
+        """surfaceNodes of the lattice, ascending"""
+        n = C.c_uint32()
+        _check(lib().talus_mi_lattice_surface_get(self._h, lattice, None, 0, C.byref(n)))
+        out = np.zeros(max(n.value, 1), dtype=np.uint32)
+        _check(lib().talus_mi_lattice_surface_get(self._h, lattice, out.ctypes.data_as(C.POINTER(C.c_uint32)),
+                                                  n.value, C.byref(n)))
+        return out[: n.value]
 
     def lattice_pieces(self, lattice) -> int:
         out = C.c_uint32()

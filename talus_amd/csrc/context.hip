@@ -2604,6 +2604,26 @@ int talus_mi_lattice_lowest_get(talus_mi_ctx *ctx, uint32_t lattice, double *low
   });
 }
 
+int talus_mi_lattice_surface_get(talus_mi_ctx *ctx, uint32_t lattice, uint32_t *nodes, uint32_t capacity,
+                                 uint32_t *count) {
+  return guarded(ctx, [&]() -> int {
+    if (!lattice_ok(ctx, lattice)) return fail(TALUS_MI_ERR_ARGUMENT, "no such lattice");
+    if (!count) return fail(TALUS_MI_ERR_ARGUMENT, "null count");
+    tmi::LatticeBody &b = ctx->lattices->body(lattice);
+    b.ensure_surface();
+    uint32_t n = 0;
+    for (uint32_t i = 0; i < b.nodes; ++i) n += b.surface[i] ? 1u : 0u;
+    *count = n;
+    if (nodes) {
+      if (capacity < n) return fail(TALUS_MI_ERR_ARGUMENT, "surface capacity too small");
+      uint32_t k = 0;
+      for (uint32_t i = 0; i < b.nodes; ++i)
+        if (b.surface[i]) nodes[k++] = i;
+    }
+    return TALUS_MI_OK;
+  });
+}
+
 int talus_mi_lattice_pieces(talus_mi_ctx *ctx, uint32_t lattice, uint32_t *pieces) {
   return guarded(ctx, [&]() -> int {
     if (!lattice_ok(ctx, lattice)) return fail(TALUS_MI_ERR_ARGUMENT, "no such lattice");

@@ -21,6 +21,7 @@
 #include <cstdlib>
 #include <cstring>
 #include <limits>
+#include <map>
 #include <thread>
 #include <unordered_map>
 
@@ -1147,8 +1148,10 @@ struct Tearing {
       for (int k = 0; k < 8; ++k) b.mass[nodes_of(c)[k]] += share;
     }
   }
-  // markCracks (tearing.cc:426-484) without the surface node list
+  // markCracks (tearing.cc:426-484): crack faces, their nodes put on the
+  // surface, then each copy of a surface node
   void mark_cracks() {
+    b.surface.resize(b.nodes, 0);
     float lo[3] = {std::numeric_limits<float>::max(), std::numeric_limits<float>::max(),
                    std::numeric_limits<float>::max()};
     for (uint32_t i = 0; i < b.nodes; ++i)
@@ -1173,9 +1176,14 @@ struct Tearing {
           next[axis] += side == 0 ? -1 : 1;
           const auto found = cell_at.find(key_of(next));
           if (found == cell_at.end()) continue;
-          if (face_of(c, 2 * axis + side) != face_of(found->second, 2 * axis + (1 - side)))
+          const Face mine = face_of(c, 2 * axis + side);
+          if (mine != face_of(found->second, 2 * axis + (1 - side))) {
             b.crack[c] = uint8_t(b.crack[c] | (1u << (2 * axis + side)));
+            for (const uint32_t n : mine) b.surface[n] = 1;
+          }
         }
+    for (const auto &cp : copies)
+      if (b.surface[cp.second]) b.surface[cp.first] = 1;
   }
 };
 
@@ -1264,8 +1272,25 @@ void LatticeBody::cell_stretch(uint32_t cell_index, double *stretch, double *dir
   std::copy(dir, dir + 3, direction);
 }
 
+void LatticeBody::ensure_surface() {
+  if (!surface.empty()) return;
+  std::map<Face, uint32_t> faces;
+  for (uint32_t c = 0; c < hexes; ++c)
+    for (uint32_t face = 0; face < 6; ++face) {
+      Face f;
+      for (int k = 0; k < 4; ++k) f[k] = hex[8 * size_t(c) + FACES[face][k]];
+      std::sort(f.begin(), f.end());
+      ++faces[f];
+    }
+  surface.assign(nodes, 0);
+  for (const auto &f : faces)
+    if (f.second == 1)
+      for (const uint32_t n : f.first) surface[n] = 1;
+}
+
 uint32_t LatticeBody::tear_cells(const uint32_t *cells, uint32_t count) {
   if (count == 0) return 0;
+  ensure_surface();
   Tearing t(*this);
   for (uint32_t i = 0; i < count; ++i)
     if (cells[i] < hexes) (void)t.crack_through_cell(cells[i]);

@@ -125,6 +125,11 @@ struct LatticeBody {
   double lowest_angular(double h) const;
   std::vector<uint8_t> crack;     // hexes
   bool retopo = false;
+  // PreparedMesh::surfaceNodes as a per-node flag: buildMesh's (corners of
+  // faces no other cell shares), found at the first tear or request, then
+  // markCracks's additions; empty until then
+  std::vector<uint8_t> surface;
+  void ensure_surface();
   // 1 + the cell's lane tear strain (infinite when the lane never tears)
   double tear_limit(uint32_t cell) const;
   // stretchOf (tearing.cc:258-300) at the host positions: the largest

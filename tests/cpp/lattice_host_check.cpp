@@ -2,7 +2,7 @@
 // oracle, in one process on the CPU: tearCells (LatticeBody::tear_cells, with
 // per-node cell lists and a hashed crack lookup) equals the oracle's
 // restatement (linear scans) node for node on random deformed lattices with
-// holes and anchored faces, and lowestAngular (band L D L^T inverse
+// holes and anchored faces (surface nodes included), and lowestAngular (band L D L^T inverse
 // iteration) equals the oracle's bit for bit on held and free beams.
 // Built by tests/test_lattice_host_cpu.py with g++ against the built
 // libraries; the oracle is the checker only.
@@ -82,8 +82,13 @@ int check_tears() {
     std::vector<double> ox(3 * on), ov(3 * on);
     orc_lattice_get_state(ob, ox.data(), ov.data());
     std::vector<uint8_t> lcr = lb.crack.empty() ? std::vector<uint8_t>(H, 0) : lb.crack;
+    std::vector<uint32_t> osurf(on), lsurf;
+    osurf.resize(orc_lattice_surface(ob, osurf.data()));
+    lb.ensure_surface();
+    for (uint32_t i = 0; i < lb.nodes; ++i)
+      if (lb.surface[i]) lsurf.push_back(i);
     const bool same = a == b && on == lb.nodes && ohx == lb.hex && om2 == lb.mass && ocr == lcr && ox == lb.x &&
-                      ov == lb.v && orc_lattice_pieces(ob) == lb.pieces();
+                      ov == lb.v && orc_lattice_pieces(ob) == lb.pieces() && osurf == lsurf;
     printf("seed %2d cells %3u copies %3u/%3u nodes %4u pieces %u/%u %s\n", seed, H, a, b, lb.nodes, lb.pieces(),
            orc_lattice_pieces(ob), same ? "same" : "DIFFERENT");
     bad += !same;

@@ -50,6 +50,7 @@ def same_topology(ctx, b, orc):
     np.testing.assert_array_equal(hx, ohx)
     np.testing.assert_array_equal(m, om)
     np.testing.assert_array_equal(crack, ocrack)
+    np.testing.assert_array_equal(ctx.lattice_surface(b), orc.surface())
     assert ctx.lattice_pieces(b) == orc.pieces()
 
 

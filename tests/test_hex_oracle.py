@@ -779,3 +779,68 @@ def test_a_damped_lattice_finds_its_lowest_frequency_once():
     plain = H.LatticeBody(mesh, [jelly(5.0e5, density=5.0, poisson=0.3)] * 2)
     plain.step()
     assert plain.lowest() == 0.0
+
+
+FACE_CORNERS = [(0, 2, 4, 6), (1, 3, 5, 7), (0, 1, 4, 5), (2, 3, 6, 7), (0, 1, 2, 3), (4, 5, 6, 7)]
+
+
+def unshared_face_nodes(hx):
+    """nodes of the cell faces no other cell shares, from the corner lists"""
+    count = {}
+    for nd in hx:
+        for f in FACE_CORNERS:
+            key = tuple(sorted(int(nd[c]) for c in f))
+            count[key] = count.get(key, 0) + 1
+    return sorted({n for key, k in count.items() if k == 1 for n in key})
+
+
+def test_surface_nodes_of_a_lattice_with_holes():
+    """buildMesh's surface nodes (solver.cc:2369-2413): the corners of every
+    face whose neighbour cell is missing or off the grid"""
+    mat = tearable()
+    res = (6, 5, 4)
+    cells = [r - 1 for r in res]
+    rng = np.random.default_rng(2)
+    solid = (rng.random(int(np.prod(cells))) > 0.25).astype(np.uint8)
+    mesh = H.build_hex_lattice(list(res), [0.1] * 3, [0.0] * 3, [mat], [], solid=solid)
+    body = H.LatticeBody(mesh, [mat])
+    # the grid rule, from the solid mask: node ids by position
+    pos = {tuple(np.round(p / 0.1).astype(int)): i for i, p in enumerate(mesh["rest"])}
+    expect = set()
+    for z in range(cells[2]):
+        for y in range(cells[1]):
+            for x in range(cells[0]):
+                if not solid[x + cells[0] * (y + cells[1] * z)]:
+                    continue
+                for axis in range(3):
+                    for side in (0, 1):
+                        at = [x, y, z]
+                        at[axis] += 1 if side else -1
+                        off = not (0 <= at[axis] < cells[axis])
+                        if off or not solid[at[0] + cells[0] * (at[1] + cells[1] * at[2])]:
+                            for c in range(8):
+                                if ((c >> axis) & 1) == side:
+                                    corner = (x + (c & 1), y + ((c >> 1) & 1), z + ((c >> 2) & 1))
+                                    expect.add(pos[corner])
+    assert body.surface().tolist() == sorted(expect)
+    full = H.LatticeBody(box((0.4, 0.3, 0.2), 0.2, mat), [mat])
+    assert full.surface().size == 5 * 4 * 3 - 3 * 2 * 1  # every node but the interior ones
+
+
+@pytest.mark.parametrize("cells", [[1], [5, 6, 9]])
+def test_tear_puts_crack_faces_and_copies_on_the_surface(cells):
+    """markCracks (tearing.cc:426-484; dynamics_strainstep_tests.cc:3786-3789:
+    every copy of the split strip is a surface node): after a tear the surface
+    is exactly the nodes of the faces no cell shares any more"""
+    mat = tearable()
+    mesh = box((1.0, 0.25, 0.25), 0.5, mat) if cells == [1] else box((0.6, 0.3, 0.2), 0.2, mat)
+    body = H.LatticeBody(mesh, [mat, mat])
+    nodes = body.nodes
+    x, v = body.state()
+    x[:, 0] *= 1.3
+    body.set_state(x, v)
+    made = body.tear(cells)
+    assert made > 0
+    surface = body.surface().tolist()
+    assert all(c in surface for c in range(nodes, nodes + made))
+    assert surface == unshared_face_nodes(body.topology()[0])
