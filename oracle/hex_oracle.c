@@ -1136,11 +1136,51 @@ static void deflate(double *x, const double *rigid, uint32_t modes, const double
   }
 }
 
+/* the order the free coordinates are numbered in: nodes by rest position
+ * along the lattice's longest extent, then the other axes, then id, so the
+ * band spans one cross-section of the body */
+typedef struct {
+  double key[3];
+  uint32_t node;
+} node_key;
+
+static int node_key_cmp(const void *x, const void *y) {
+  const node_key *a = (const node_key *)x, *b = (const node_key *)y;
+  for (int k = 0; k < 3; ++k) {
+    if (a->key[k] < b->key[k]) return -1;
+    if (b->key[k] < a->key[k]) return 1;
+  }
+  return a->node < b->node ? -1 : (a->node > b->node ? 1 : 0);
+}
+
+static void band_order(const double *rest, size_t nodes, uint32_t *order) {
+  double lo[3] = {INFINITY, INFINITY, INFINITY}, hi[3] = {-INFINITY, -INFINITY, -INFINITY};
+  for (size_t i = 0; i < nodes; ++i)
+    for (int k = 0; k < 3; ++k) {
+      lo[k] = rest[3 * i + k] < lo[k] ? rest[3 * i + k] : lo[k];
+      hi[k] = rest[3 * i + k] > hi[k] ? rest[3 * i + k] : hi[k];
+    }
+  int axes[3] = {0, 1, 2};
+  for (int k = 1; k < 3; ++k) /* longest extent first (first of equals) */
+    if (hi[k] - lo[k] > hi[axes[0]] - lo[axes[0]]) axes[0] = k;
+  for (int k = 0, at = 1; k < 3; ++k)
+    if (k != axes[0]) axes[at++] = k;
+  node_key *keys = (node_key *)malloc((nodes ? nodes : 1) * sizeof(node_key));
+  for (size_t i = 0; i < nodes; ++i) {
+    for (int k = 0; k < 3; ++k) keys[i].key[k] = rest[3 * i + axes[k]];
+    keys[i].node = (uint32_t)i;
+  }
+  qsort(keys, nodes, sizeof(node_key), node_key_cmp);
+  for (size_t i = 0; i < nodes; ++i) order[i] = keys[i].node;
+  free(keys);
+}
+
 /* lowestAngular (lowestmode.cc:31-203): inverse iteration on K v = w^2 M v
  * over the free coordinates (pinned and massless nodes left out) with the
  * rows' stiffnesses along their local Jacobians, rigid modes deflated and a
  * small mass shift when nothing is held.  The shifted stiffness is factored
- * as a band L D L^T in coordinate order (the reference factors it sparse);
+ * as a band L D L^T with the coordinates numbered along the longest extent
+ * (the reference factors it sparse);
  * INFINITY without free coordinates, 0 when a pivot vanishes. */
 static double lowest_angular(const orc_hex *el, uint32_t count, const double *mass, const uint8_t *pinned_node,
                              const double *rest, size_t nodes) {
@@ -1149,8 +1189,11 @@ static double lowest_angular(const orc_hex *el, uint32_t count, const double *ma
   size_t *owner = (size_t *)malloc((nodes ? 3 * nodes : 1) * sizeof(size_t));
   size_t n = 0;
   int held = 0;
-  for (size_t node = 0; node < nodes; ++node) {
-    for (int a = 0; a < 3; ++a) index[3 * node + a] = -1;
+  uint32_t *order = (uint32_t *)malloc((nodes ? nodes : 1) * sizeof(uint32_t));
+  band_order(rest, nodes, order);
+  for (size_t i = 0; i < 3 * nodes; ++i) index[i] = -1;
+  for (size_t at = 0; at < nodes; ++at) {
+    const size_t node = order[at];
     if (pinned_node[node]) {
       held = 1;
       continue;
@@ -1162,6 +1205,7 @@ static double lowest_angular(const orc_hex *el, uint32_t count, const double *ma
       owner[n++] = 3 * node + (size_t)a;
     }
   }
+  free(order);
   if (n == 0) {
     free(index);
     free(w);

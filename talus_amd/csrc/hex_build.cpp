@@ -748,6 +748,34 @@ void deflate(std::vector<double> &x, const std::vector<std::vector<double>> &rig
 
 }  // namespace
 
+// the order the free coordinates are numbered in: nodes by rest position
+// along the longest extent, then the other axes, then id, so the band of
+// the lowest-mode factor spans one cross-section of the body
+std::vector<uint32_t> band_order(const std::vector<double> &rest, size_t nodes) {
+  double lo[3] = {INF, INF, INF}, hi[3] = {-INF, -INF, -INF};
+  for (size_t i = 0; i < nodes; ++i)
+    for (int k = 0; k < 3; ++k) {
+      lo[k] = rest[3 * i + k] < lo[k] ? rest[3 * i + k] : lo[k];
+      hi[k] = rest[3 * i + k] > hi[k] ? rest[3 * i + k] : hi[k];
+    }
+  int axes[3] = {0, 1, 2};
+  for (int k = 1; k < 3; ++k)
+    if (hi[k] - lo[k] > hi[axes[0]] - lo[axes[0]]) axes[0] = k;
+  for (int k = 0, at = 1; k < 3; ++k)
+    if (k != axes[0]) axes[at++] = k;
+  std::vector<uint32_t> order(nodes);
+  for (size_t i = 0; i < nodes; ++i) order[i] = uint32_t(i);
+  std::sort(order.begin(), order.end(), [&](uint32_t a, uint32_t b) {
+    for (int k = 0; k < 3; ++k) {
+      const double x = rest[3 * size_t(a) + axes[k]], y = rest[3 * size_t(b) + axes[k]];
+      if (x < y) return true;
+      if (y < x) return false;
+    }
+    return a < b;
+  });
+  return order;
+}
+
 double hex_lowest_angular(const std::vector<HexElement> &el, const std::vector<double> &mass,
                           const std::vector<uint8_t> &pinned_node, const std::vector<double> &rest) {
   const size_t nodes = mass.size();
@@ -755,7 +783,7 @@ double hex_lowest_angular(const std::vector<HexElement> &el, const std::vector<d
   std::vector<double> w;
   std::vector<size_t> owner;
   bool held = false;
-  for (size_t node = 0; node < nodes; ++node) {
+  for (const uint32_t node : band_order(rest, nodes)) {
     if (pinned_node[node]) {
       held = true;
       continue;
@@ -1385,7 +1413,7 @@ int make_lattice(LatticeBody &b, const talus_mi_hex_mesh *mesh, const talus_mi_m
     // cost (free coordinates x band width squared) and size are bounded
     std::vector<int64_t> index(n, -1);
     int64_t free_nodes = 0;
-    for (size_t i = 0; i < n; ++i)
+    for (const uint32_t i : band_order(b.rest, n))
       if (!b.pinned(i) && b.mass[i] > 0.0f) index[i] = free_nodes++;
     int64_t band = 0;
     for (uint32_t c = 0; c < b.hexes; ++c)

@@ -58,6 +58,9 @@ bool hex_make(HexElement &e, uint32_t cell, const uint32_t *nodes, const double 
 // stiffness; infinity without a free coordinate, 0 when a pivot vanishes
 double hex_lowest_angular(const std::vector<HexElement> &elements, const std::vector<double> &mass,
                           const std::vector<uint8_t> &pinned, const std::vector<double> &rest);
+// nodes by rest position along the longest extent, then the other axes,
+// then id: the coordinate order of the lowest-mode band factor
+std::vector<uint32_t> band_order(const std::vector<double> &rest, size_t nodes);
 // rotationOf on the host: the quaternion iteration maximising tr(R^T F)
 void hex_rotation_of(const double *F, const double *guess, uint32_t iterations, double *out);
 // updateElement's rotation and capped strain rows at positions x (the
