@@ -221,3 +221,43 @@ def test_a_tear_keeps_the_plastic_refresh_of_the_other_bodies():
     np.testing.assert_array_equal(v1, v0)
     np.testing.assert_array_equal(P1, P0)
     np.testing.assert_array_equal(a1, a0)
+
+
+def test_damped_plastic_strip_tears_like_the_oracle():
+    """Damping, plastic flow and tearing together: a damped (ratio 0.3)
+    yielding tearable strip drawn out as in dynamics_strainstep_tests.cc:
+    3719-3757; the GPU yields, rebuilds and tears as the oracle's colour
+    sweep does, and keeps the lowest frequency of its first build"""
+    mat = tearable()
+    mat.damping_ratio, mat.yield_stress, mat.plastic_max = 0.3, 200.0, 1.0
+    ends = [api.Support.box((-1.2, 0.0, 0.0), (0.01, 1.0, 1.0)), api.Support.box((1.2, 0.0, 0.0), (0.01, 1.0, 1.0))]
+    mesh = box((1.2, 0.2, 0.2), 0.2, mat, ends)
+    orc = H.LatticeBody(orc_mesh(mesh), [orc_material(mat)] * 2, offset=(0.0, 2.0, 0.0))
+    right = np.nonzero(mesh["rest_positions"][:, 0] > 1.19)[0]
+    tears = refreshed = 0
+    with api.Context() as ctx:
+        b = ctx.add_lattice(mesh, [mat, mat], offset=(0.0, 2.0, 0.0))
+        start = ctx.lattice_state(b)[0][right, 0].copy()
+        for frame in range(120):
+            x, v = ctx.lattice_state(b)
+            x[right, 0] = start + 2.4 * min(frame / 90.0, 1.0)
+            ctx.set_lattice_state(b, x, v)
+            ox, ov = orc.state()
+            ox[right, 0] = x[right, 0]
+            orc.set_state(ox, ov)
+            r = ctx.lattice_step()
+            before = orc.nodes
+            orc.step(colored=True)
+            refreshed += r["refreshed"]
+            assert (r["node_copies"] > 0) == (orc.nodes > before), frame
+            if r["node_copies"]:
+                tears += 1
+                same_topology(ctx, b, orc)
+        assert ctx.lattice_lowest(b) == orc.lowest() > 0.0
+        assert tears > 0 and refreshed > 0 and ctx.lattice_pieces(b) >= 2
+        x, _ = ctx.lattice_state(b)
+        ox, _ = orc.state()
+        P, a = ctx.lattice_plastic(b)
+        oP, oa = orc.plastic()
+    assert float(np.abs(x - ox).max()) <= 1e-6
+    assert float(np.abs(a - oa).max()) <= 1e-6 * max(1.0, float(np.abs(oa).max()))
