@@ -1000,6 +1000,20 @@ static void volume_frame(const double *F, const double *ep, double *W) {
   mul3_nt(C, Fi, W);
 }
 
+/* Shape gradients of the plastic reference, Fp^-T g = (I + ep)^T g, for a
+ * plastic neo-Hookean tet; the rest gradients otherwise. */
+static void plastic_gradients(const orc_body *b, size_t e, double g[4][3], double gs[4][3]) {
+  const int plastic = b->snh[e] && b->plastic_lane[e];
+  const double *ep = &b->plastic[10 * e];
+  for (int i = 0; i < 4; ++i)
+    for (int r = 0; r < 3; ++r) {
+      double s = g[i][r];
+      if (plastic)
+        for (int k = 0; k < 3; ++k) s += ep[k * 3 + r] * g[i][k];
+      gs[i][r] = s;
+    }
+}
+
 /* One tet of assembleElastic (solver.cc:1382-1470): committed rotation,
  * forces to slots, diagonal stiffness blocks to slots.  Corotated:
  * R K_ii R^T = V((lambda+mu)(Rg)(Rg)^T + mu|g|^2 I) (stiffnessDiagonal,
@@ -1020,7 +1034,17 @@ static void element_one(orc_body *b, const double *at, size_t e, int diag) {
     F[0] += 1.0;
     F[4] += 1.0;
     F[8] += 1.0;
-    orc_element_rotation(F, R);
+    if (ep) {
+      double Fi[9], Fe[9];
+      memcpy(Fi, ep, sizeof Fi);
+      Fi[0] += 1.0;
+      Fi[4] += 1.0;
+      Fi[8] += 1.0;
+      mul3(F, Fi, Fe);
+      orc_element_rotation(Fe, R);
+    } else {
+      orc_element_rotation(F, R);
+    }
     volume_frame(F, ep, &b->W[9 * e]);
     orc_snh_tet(u, dminv, V, b->lambda[e], b->mu[e], ep, &b->slot_f[12 * e], &energy);
   } else {
@@ -1035,11 +1059,13 @@ static void element_one(orc_body *b, const double *at, size_t e, int diag) {
     shape_gradients(dminv, g);
     const double *W = &b->W[9 * e];
     const double mu = b->mu[e], lam = b->lambda[e];
+    double gs[4][3];
+    plastic_gradients(b, e, g, gs);
     for (int i = 0; i < 4; ++i) {
       double rg[3], wg[3];
-      mv3(R, g[i], rg);
+      mv3(R, gs[i], rg);
       mv3(W, g[i], wg);
-      const double gg = g[i][0] * g[i][0] + g[i][1] * g[i][1] + g[i][2] * g[i][2];
+      const double gg = gs[i][0] * gs[i][0] + gs[i][1] * gs[i][1] + gs[i][2] * gs[i][2];
       double *blk = &b->slot_d[36 * e + 9 * i];
       for (int r = 0; r < 3; ++r)
         for (int c = 0; c < 3; ++c)
@@ -1123,16 +1149,17 @@ static void op_element_range(void *p, size_t begin, size_t end) {
   op_ctx *c = (op_ctx *)p;
   orc_body *b = c->b;
   for (size_t e = begin; e < end; ++e) {
-    double d[12], g[4][3], D[9] = {0}, Dw[9] = {0};
+    double d[12], g[4][3], gs[4][3], D[9] = {0}, Dw[9] = {0};
     gather4(b, c->d, e, d);
     shape_gradients(&b->m.dminv[9 * e], g);
+    plastic_gradients(b, e, g, gs);
     const double *R = &b->R[9 * e];
     for (int j = 0; j < 4; ++j) {
       double dp[3];
       mtv3(R, &d[3 * j], dp);
       for (int r = 0; r < 3; ++r)
         for (int k = 0; k < 3; ++k) {
-          D[r * 3 + k] += dp[r] * g[j][k];
+          D[r * 3 + k] += dp[r] * gs[j][k];
           Dw[r * 3 + k] += d[3 * j + r] * g[j][k];
         }
     }
@@ -1147,7 +1174,7 @@ static void op_element_range(void *p, size_t begin, size_t end) {
     const double V = b->m.volume[e];
     for (int i = 0; i < 4; ++i) {
       double t[3], y[3], w[3] = {0, 0, 0};
-      mv3(s, g[i], t);
+      mv3(s, gs[i], t);
       mv3(R, t, y);
       if (snh) mv3(&b->W[9 * e], g[i], w);
       for (int r = 0; r < 3; ++r) b->slot_f[12 * e + 3 * i + r] = V * (y[r] + vol * w[r]);

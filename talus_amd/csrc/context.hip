@@ -620,7 +620,7 @@ void Engine<Real>::factorize(const std::vector<double> &reach, const std::vector
   HIPCHECK(hipMemsetAsync(fd_.failed, 0, d_.B * sizeof(int), s));
   FactorLaunch<Real>::assemble(fd_, d_pair_target_, d_pair_ld_, d_pair_node_, d_pair_start_,
                                d_contrib_, uint32_t(t.pair_target.size()), d_.dminv, d_.vol64,
-                               d_.lam64, d_.mu64, d_.R, d_.W, d_.flags, d_.T, d_.mass,
+                               d_.lam64, d_.mu64, d_.R, d_.W, d_.plastic, d_.flags, d_.T, d_.mass,
                                d_body_reach_,
                                d_body_inertia_, d_pair_body_, s);
   size_t ea = 0, fac = 0;

@@ -170,15 +170,16 @@ struct FactorDev {
 
 // Launch helpers (definitions in factor.hip).  `conn/dminv/...` are the
 // engine's element arrays; R is the committed rotation and W the
-// neo-Hookean volume frame, both in the element precision.
+// neo-Hookean volume frame, both in the element precision; plastic the FP64
+// plastic planes (Fp^-1 - I for plastic neo-Hookean tets).
 template <class Real>
 struct FactorLaunch {
   static void assemble(const FactorDev &f, const uint64_t *pair_target, const uint32_t *pair_ld,
                        const uint32_t *pair_node, const uint32_t *pair_start,
                        const uint32_t *contrib, uint32_t pairs, const double *dminv,
                        const double *vol, const double *lam, const double *mu, const Real *R,
-                       const Real *W, const uint8_t *flags, uint32_t T, const double *mass,
-                       const double *body_reach,
+                       const Real *W, const double *plastic, const uint8_t *flags, uint32_t T,
+                       const double *mass, const double *body_reach,
                        const double *body_inertia, const uint32_t *pair_body, hipStream_t s);
 };
 

@@ -38,7 +38,7 @@ __global__ __launch_bounds__(NT) void k_assemble_pairs(
     double *pool, const uint64_t *pair_target, const uint32_t *pair_ld, const uint32_t *pair_node,
     const uint32_t *pair_start, const uint32_t *contrib, uint32_t pairs, const double *dminv,
     const double *vol, const double *lam, const double *mu, const Real *R, const Real *W,
-    const uint8_t *flags, uint32_t T, const double *mass, const double *reach_of,
+    const double *plastic, const uint8_t *flags, uint32_t T, const double *mass, const double *reach_of,
     const double *inertia_of, const uint32_t *pair_body, const int *refresh) {
   const uint32_t q = blockIdx.x * NT + threadIdx.x;
   if (q >= pairs || !refresh[pair_body[q]]) return;
@@ -53,8 +53,23 @@ __global__ __launch_bounds__(NT) void k_assemble_pairs(
       Rm[k] = double(R[size_t(k) * T + e]);
     }
     shape_gradients(dm, g);
-    const double *ga = &g[3 * a], *gb = &g[3 * b];
     const bool snh = (flags[e] & FLAG_SNH) != 0;
+    // the shear part of a plastic neo-Hookean tet is taken along its plastic
+    // reference's gradients Fp^-T g = (I + ep)^T g, as in the operator
+    double gs[12];
+#pragma unroll
+    for (int i = 0; i < 12; ++i) gs[i] = g[i];
+    if (snh && (flags[e] & FLAG_PLASTIC)) {
+      double ep[9];
+#pragma unroll
+      for (int k = 0; k < 9; ++k) ep[k] = plastic[size_t(k) * T + e];
+#pragma unroll
+      for (int i = 0; i < 4; ++i)
+#pragma unroll
+        for (int r = 0; r < 3; ++r)
+          gs[3 * i + r] += ep[r] * g[3 * i] + ep[3 + r] * g[3 * i + 1] + ep[6 + r] * g[3 * i + 2];
+    }
+    const double *ga = &gs[3 * a], *gb = &gs[3 * b];
     const double V = vol[e], l = snh ? 0.0 : lam[e], m = mu[e];
     const double gg = ga[0] * gb[0] + ga[1] * gb[1] + ga[2] * gb[2];
     double K[9];
@@ -73,8 +88,8 @@ __global__ __launch_bounds__(NT) void k_assemble_pairs(
       double Wm[9], wa[3], wb[3];
 #pragma unroll
       for (int k = 0; k < 9; ++k) Wm[k] = double(W[size_t(k) * T + e]);
-      mv3(Wm, ga, wa);
-      mv3(Wm, gb, wb);
+      mv3(Wm, &g[3 * a], wa);
+      mv3(Wm, &g[3 * b], wb);
       const double vl = V * lam[e];
 #pragma unroll
       for (int r = 0; r < 3; ++r)
@@ -861,14 +876,15 @@ void FactorLaunch<Real>::assemble(const FactorDev &f, const uint64_t *pair_targe
                                   const uint32_t *pair_start, const uint32_t *contrib,
                                   uint32_t pairs, const double *dminv, const double *vol,
                                   const double *lam, const double *mu, const Real *R,
-                                  const Real *W, const uint8_t *flags, uint32_t T,
-                                  const double *mass, const double *body_reach,
+                                  const Real *W, const double *plastic, const uint8_t *flags,
+                                  uint32_t T, const double *mass, const double *body_reach,
                                   const double *body_inertia, const uint32_t *pair_body,
                                   hipStream_t s) {
   if (!pairs) return;
   hipLaunchKernelGGL(k_assemble_pairs<Real>, dim3((pairs + NT - 1) / NT), dim3(NT), 0, s, f.pool,
                      pair_target, pair_ld, pair_node, pair_start, contrib, pairs, dminv, vol, lam,
-                     mu, R, W, flags, T, mass, body_reach, body_inertia, pair_body, f.refresh);
+                     mu, R, W, plastic, flags, T, mass, body_reach, body_inertia, pair_body,
+                     f.refresh);
 }
 
 void factor_zero(const FactorDev &f, const uint64_t *tasks, uint32_t count, hipStream_t s) {

@@ -62,6 +62,26 @@ __device__ __forceinline__ void load_dm(const Dev<Real> &d, uint32_t e, Real *dm
   for (int k = 0; k < 9; ++k) dm[k] = d.dminv_r[(size_t)k * d.T + e];
 }
 
+// Shape gradients of a plastic neo-Hookean tet's plastic reference,
+// Fp^-T g_i = (I + ep)^T g_i; the rest gradients for every other tet.  The
+// system operator's shear part is taken along them, so that it follows the
+// elastic response P(F Fp^-1) Fp^-T after the tet has flowed.
+template <class Real>
+__device__ __forceinline__ void plastic_gradients(const Dev<Real> &d, uint32_t e, bool flowing,
+                                                  const Real *g, Real *gs) {
+#pragma unroll
+  for (int i = 0; i < 12; ++i) gs[i] = g[i];
+  if (!flowing) return;
+  Real ep[9];
+#pragma unroll
+  for (int k = 0; k < 9; ++k) ep[k] = Real(d.plastic[(size_t)k * d.T + e]);
+#pragma unroll
+  for (int i = 0; i < 4; ++i)
+#pragma unroll
+    for (int r = 0; r < 3; ++r)
+      gs[3 * i + r] += ep[r] * g[3 * i] + ep[3 + r] * g[3 * i + 1] + ep[6 + r] * g[3 * i + 2];
+}
+
 template <class Real>
 __device__ __forceinline__ void gather_vec(const Real *src, uint4 c, Real *out) {
   const uint32_t n[4] = {c.x, c.y, c.z, c.w};
@@ -324,10 +344,12 @@ __global__ __launch_bounds__(TILE) void k_pcg_begin(Dev<Real> d) {
 template <class Real>
 __device__ __forceinline__ void op_tet(const Dev<Real> &d, uint32_t e, const Real *dir, Real *y) {
   const uint4 c = d.conn[e];
-  Real dv[12], dm[9], g[12], Rm[9];
+  Real dv[12], dm[9], g[12], gs[12], Rm[9];
   gather_vec(dir, c, dv);
   load_dm(d, e, dm);
   shape_gradients(dm, g);
+  const uint8_t flags = d.flags[e];
+  plastic_gradients(d, e, (flags & FLAG_SNH) && (flags & FLAG_PLASTIC), g, gs);
 #pragma unroll
   for (int k = 0; k < 9; ++k) Rm[k] = d.R[(size_t)k * d.T + e];
   Real D[9] = {Real(0), Real(0), Real(0), Real(0), Real(0), Real(0), Real(0), Real(0), Real(0)};
@@ -338,9 +360,9 @@ __device__ __forceinline__ void op_tet(const Dev<Real> &d, uint32_t e, const Rea
 #pragma unroll
     for (int r = 0; r < 3; ++r)
 #pragma unroll
-      for (int k = 0; k < 3; ++k) D[r * 3 + k] += dp[r] * g[3 * j + k];
+      for (int k = 0; k < 3; ++k) D[r * 3 + k] += dp[r] * gs[3 * j + k];
   }
-  const bool snh = (d.flags[e] & FLAG_SNH) != 0;
+  const bool snh = (flags & FLAG_SNH) != 0;
   const Real lam = d.lam[e], mu = d.mu[e], V = d.vol[e];
   const Real tr = snh ? Real(0) : D[0] + D[4] + D[8];
   Real S[9];
@@ -352,7 +374,7 @@ __device__ __forceinline__ void op_tet(const Dev<Real> &d, uint32_t e, const Rea
 #pragma unroll
   for (int i = 0; i < 4; ++i) {
     Real tmp[3];
-    mv3(S, &g[3 * i], tmp);
+    mv3(S, &gs[3 * i], tmp);
     mv3(Rm, tmp, &y[3 * i]);
   }
   if (snh) {
@@ -607,14 +629,30 @@ __global__ __launch_bounds__(TILE) void k_assemble(Dev<Real> d) {
     displacement_gradient(u64, dm64, H64);
     const bool upright = 1.0 + volume_change(H64) > 0.0;
     Real dm[9], g[12], H[9], F[9];
+    const uint8_t flags = d.flags[e];
+    // a plastic neo-Hookean tet turns with its elastic part Fe = F Fp^-1
+    // (Fp^-1 = I + ep, FP64): the operator's shear frame
+    const bool flowing = (flags & FLAG_SNH) && (flags & FLAG_PLASTIC);
+    double Frot64[9];
+    if (flowing) {
+      double F64[9], Fi[9];
+#pragma unroll
+      for (int k = 0; k < 9; ++k) {
+        F64[k] = H64[k] + ((k % 4 == 0) ? 1.0 : 0.0);
+        Fi[k] = d.plastic[(size_t)k * d.T + e] + ((k % 4 == 0) ? 1.0 : 0.0);
+      }
+      mul3(F64, Fi, Frot64);
+    } else {
+#pragma unroll
+      for (int k = 0; k < 9; ++k) Frot64[k] = H64[k] + ((k % 4 == 0) ? 1.0 : 0.0);
+    }
 #pragma unroll
     for (int k = 0; k < 9; ++k) {
       dm[k] = Real(dm64[k]);
       H[k] = Real(H64[k]);
-      F[k] = H[k] + ((k % 4 == 0) ? Real(1) : Real(0));
+      F[k] = Real(Frot64[k]);
     }
     shape_gradients(dm, g);
-    const uint8_t flags = d.flags[e];
     // committed rotation: polar in the element precision, sign-fixed SVD in
     // FP64 when inverted; the mixed path re-orthogonalises it in FP64
     Real Rm[9];
@@ -629,10 +667,7 @@ __global__ __launch_bounds__(TILE) void k_assemble(Dev<Real> d) {
         for (int i = 0; i < 9; ++i) Rm[i] = Real(R64[i]);
       }
     } else {
-      double F64[9];
-#pragma unroll
-      for (int i = 0; i < 9; ++i) F64[i] = H64[i] + ((i % 4 == 0) ? 1.0 : 0.0);
-      svd_rotation(F64, R64);
+      svd_rotation(Frot64, R64);
 #pragma unroll
       for (int i = 0; i < 9; ++i) Rm[i] = Real(R64[i]);
     }
@@ -649,23 +684,12 @@ __global__ __launch_bounds__(TILE) void k_assemble(Dev<Real> d) {
         // frame indifference, P(Fe) = R P(R^T Fe): the rotation-free
         // R^T Fe - I is formed in FP64 and the law runs in FP32 on it, so a
         // large rotation costs no FP32 digits of the strain
-        double Fe64[9], G64[9];
-#pragma unroll
-        for (int k = 0; k < 9; ++k) Fe64[k] = H64[k] + ((k % 4 == 0) ? 1.0 : 0.0);
-        if (plastic) {
-          double F64[9], Fi[9];
-#pragma unroll
-          for (int k = 0; k < 9; ++k) {
-            F64[k] = Fe64[k];
-            Fi[k] = d.plastic[(size_t)k * d.T + e] + ((k % 4 == 0) ? 1.0 : 0.0);
-          }
-          mul3(F64, Fi, Fe64);
-        }
-        mul3_tn(R64, Fe64, G64);
+        double G64[9];
+        mul3_tn(R64, Frot64, G64);
 #pragma unroll
         for (int k = 0; k < 9; ++k) {
           He[k] = Real(G64[k] - ((k % 4 == 0) ? 1.0 : 0.0));
-          Fe[k] = Real(Fe64[k]);
+          Fe[k] = Real(Frot64[k]);
         }
       } else {
         // H-form at Fe = F Fp^-1: He = H + ep + H ep (Fp^-1 = I + ep)
@@ -772,15 +796,18 @@ __global__ __launch_bounds__(TILE) void k_assemble(Dev<Real> d) {
 #pragma unroll
         for (int k = 0; k < 9; ++k) Wm[k] = Real(0);
       }
+      // the shear part along the plastic reference's gradients Fp^-T g
+      Real gs[12];
+      plastic_gradients(d, e, flowing, g, gs);
       const uint4 pos = d.slot_pos[e];
       const uint32_t at[4] = {pos.x, pos.y, pos.z, pos.w};
 #pragma unroll
       for (int i = 0; i < 4; ++i) {
         Real *o = d.slot_d + 6 * (size_t)at[i];
         Real rg[3], wg[3];
-        mv3(Rm, &g[3 * i], rg);
+        mv3(Rm, &gs[3 * i], rg);
         mv3(Wm, &g[3 * i], wg);
-        const Real gg = g[3 * i] * g[3 * i] + g[3 * i + 1] * g[3 * i + 1] + g[3 * i + 2] * g[3 * i + 2];
+        const Real gg = gs[3 * i] * gs[3 * i] + gs[3 * i + 1] * gs[3 * i + 1] + gs[3 * i + 2] * gs[3 * i + 2];
         o[0] = lr * rg[0] * rg[0] + lw * wg[0] * wg[0] + vm * gg;
         o[1] = lr * rg[0] * rg[1] + lw * wg[0] * wg[1];
         o[2] = lr * rg[0] * rg[2] + lw * wg[0] * wg[2];

@@ -334,10 +334,12 @@ def test_operator_is_symmetric_and_matches_snh_tangent_at_rest():
 @pytest.mark.parametrize("plastic", [False, True])
 def test_snh_operator_is_rotated_shear_plus_volume_gradient_under_stretch(plastic):
     """Under a large stretch and turn, d^T K d of a neo-Hookean body equals
-    sum_e V (2 mu |sym(R^T D)|^2 + lambda (dJ_e/dt)^2), with R the polar
-    factor of F (numpy SVD), D = sum_j d_j g_j^T and dJ_e/dt the derivative
-    of det(F_e) = det(F Fp^-1) along D (central differences of numpy det).
-    A transposed cofactor, a dropped Fp^-1 or a wrong coefficient fails it."""
+    sum_e V (2 mu |sym(R^T D Fp^-1)|^2 + lambda (dJ_e/dt)^2), with R the polar
+    factor of the elastic part F_e = F Fp^-1 (numpy SVD), D = sum_j d_j g_j^T
+    and dJ_e/dt the derivative of det(F_e) along D (central differences of
+    numpy det): the elastic response P(F_e) Fp^-T linearised in F_e.  A
+    transposed cofactor, a dropped or transposed Fp^-1 (in the shear part as
+    in the volume part) or a wrong coefficient fails it."""
     body = _tiny_body(anchored=False)
     if plastic:
         body.materials["yield_stress"] = 1e12  # a plastic lane that does not flow here
@@ -368,10 +370,10 @@ def test_snh_operator_is_rotated_shear_plus_volume_gradient_under_stretch(plasti
             ds = (x[t[1:]] - x[t[0]]).T
             F = ds @ np.linalg.inv(Ds)
             assert np.linalg.det(F) > 0
-            U, _, Vt = np.linalg.svd(F)
+            U, _, Vt = np.linalg.svd(F @ Fp_inv[e])
             R = U @ Vt
             D = (d[t[1:]] - d[t[0]]).T @ np.linalg.inv(Ds)
-            Dp = R.T @ D
+            Dp = R.T @ D @ Fp_inv[e]
             sym = 0.5 * (Dp + Dp.T)
             h = 1e-6
             dJ = (np.linalg.det((F + h * D) @ Fp_inv[e]) - np.linalg.det((F - h * D) @ Fp_inv[e])) / (2 * h)
