@@ -13,14 +13,25 @@ lognormal E, 1 substep).
 A step is one complete deformable step: element kernels, assembly, three
 implicit solves per substep, velocity and position updates, plastic commit.
 
+Each batched run takes the workload's whole step count (120 for the bodies,
+240 for the cube): `--warmup` untimed steps, the `--steps` timed steps (the
+window), then the rest untimed, so that divergence anywhere in the run is
+seen.
+
 value   = sum(tets) x substeps x steps / wall time of the timed steps for the
           mixed-precision batched path (FP32 element kernels and vectors, FP64
           rest inverse, node state, dot products and Cholesky pivots), with
           the solve plan chosen by the autotuner.
 fp64    = the same batched path in FP64; fp64_plain = FP64 with every body in
-          its own context, stepped one after another (no batching).
+          its own context, stepped one after another (no batching), over the
+          first --plain-steps steps of the same window from the same state.
 Inputs are resident in HBM before the timed region; every step returns after
 its stream has synchronized, so the wall clock brackets finished work.
+
+The line is valid only when both batched runs stay finite with no capped
+solve over every step and the mixed element forces are within 1e-5 of FP64
+(at the window end and the run end); otherwise "value" is null, "failed"
+says why and the exit status is 1.
 """
 from __future__ import annotations
 
@@ -38,6 +49,7 @@ sys.path.insert(0, ROOT)
 
 METRIC = "tet updates/sec: mixed-precision batched vs plain FP64, 1 GPU, ≤1e-5 rel force err"
 HBM_PEAK_GBS = 8000.0
+FORCE_BAR = 1.0e-5  # the metric's relative force error bar, mixed against FP64
 KINDS = {"assemble": 0, "op_element": 1, "op_gather": 2, "solve_forward_lanes": 3,
          "solve_backward_lanes": 4, "solve_forward_rows": 5, "solve_backward_rows": 6,
          "solve_forward_split": 7, "solve_backward_split": 8}
@@ -71,36 +83,72 @@ def workload(name):
     return sc, meshes, desc
 
 
-def batched(api, meshes, precision, params, steps, warmup, tune, plan=None, streams=1):
+def run_batch(api, meshes, precision, params, steps, warmup, tune, plan=None, streams=1,
+              snapshot=False):
+    """Every body in one context, stepped `warmup` untimed steps, then `steps`
+    timed steps (the window).  Returns the context, body ids, the window's
+    wall time, the report of every step so far and, if asked, the state of
+    every body at the start of the window."""
     ctx = api.Context(0, precision)
     ids = [ctx.add_body(m, mats) for m, mats in meshes]
     ctx.set_streams(streams)
-    ctx.step(params)
+    reps = [ctx.step(params)]
     tuned = None
     if plan and len(plan) == len(ctx.plan()):
         ctx.set_plan([int(c) for c in plan])
     elif tune or plan:
         tuned = ctx.tune(3)  # no plan, or one made for another level structure
     for _ in range(max(warmup - 1, 0)):
-        ctx.step(params)
+        reps.append(ctx.step(params))
+    snap = body_snapshot(ctx, ids) if snapshot else None
     t0 = time.perf_counter()
-    reps = [ctx.step(params) for _ in range(steps)]
+    timed = [ctx.step(params) for _ in range(steps)]
     wall = time.perf_counter() - t0
-    return ctx, ids, wall, reps, tuned
+    return ctx, ids, wall, reps + timed, tuned, snap
 
 
-def plain(api, meshes, params, steps):
-    """Every body in its own FP64 context, one after another."""
+def body_snapshot(ctx, ids):
+    """x, v, a, plastic points, rotations and volume frames of every body."""
+    out = []
+    for b in ids:
+        x, v, a = ctx.state(b)
+        out.append({"x": x, "v": v, "a": a, "plastic": ctx.plastic(b), "R": ctx.rotations(b),
+                    "W": ctx.frames(b)})
+    return out
+
+
+def restore(ctx, b, snap):
+    ctx.set_state(b, snap["x"], snap["v"], snap["a"])
+    ctx.set_plastic(b, snap["plastic"])
+    ctx.set_rotations(b, snap["R"])
+    ctx.set_frames(b, snap["W"])
+
+
+def health(reps):
+    """Non-finite steps and capped solves over the given step reports."""
+    return {"steps": len(reps), "non_finite_steps": sum(1 for r in reps if r["non_finite"]),
+            "cg_capped": sum(r["cg_capped"] for r in reps)}
+
+
+def plain(api, meshes, params, steps, snaps):
+    """Every body in its own FP64 context, one after another, over the same
+    window as the batch: each body starts from the batch's FP64 state at the
+    window start, takes one untimed step (its factorization), is put back to
+    that state and then steps `steps` timed steps."""
     wall = 0.0
-    for m, mats in meshes:
+    bad = capped = 0
+    for (m, mats), snap in zip(meshes, snaps):
         with api.Context(0, api.FP64) as ctx:
-            ctx.add_body(m, mats)
+            b = ctx.add_body(m, mats)
+            restore(ctx, b, snap)
             ctx.step(params)
+            restore(ctx, b, snap)
             t0 = time.perf_counter()
-            for _ in range(steps):
-                ctx.step(params)
+            reps = [ctx.step(params) for _ in range(steps)]
             wall += time.perf_counter() - t0
-    return wall
+            bad += sum(1 for r in reps if r["non_finite"])
+            capped += sum(r["cg_capped"] for r in reps)
+    return wall, bad, capped
 
 
 def force_error(ctx_m, ids_m, ctx_d, ids_d):
@@ -108,20 +156,48 @@ def force_error(ctx_m, ids_m, ctx_d, ids_d):
     FP64 state: positions, committed plastic points, rotations and volume
     frames of the FP64 run are given to the mixed kernel, so the difference
     is the element kernel's precision alone (the two runs' trajectories
-    differ within the PCG tolerance)."""
-    worst = 0.0
+    differ within the PCG tolerance).  A body whose forces are not finite
+    counts as an infinite error.  Returns (error, body)."""
+    worst, at = 0.0, None
     for bm, bd in zip(ids_m, ids_d):
         x, _, _ = ctx_d.state(bd)
+        own = (ctx_m.plastic(bm), ctx_m.rotations(bm), ctx_m.frames(bm))
         ctx_m.set_plastic(bm, ctx_d.plastic(bd))
         ctx_m.set_rotations(bm, ctx_d.rotations(bd))
         ctx_m.set_frames(bm, ctx_d.frames(bd))
         fd, *_ = ctx_d.eval_forces(bd, x)
         fm, *_ = ctx_m.eval_forces(bm, x)
+        # the mixed run continues from its own plastic state
+        ctx_m.set_plastic(bm, own[0])
+        ctx_m.set_rotations(bm, own[1])
+        ctx_m.set_frames(bm, own[2])
         T = fd.shape[0]
-        num = np.linalg.norm((fm - fd).reshape(T, -1), axis=1).max()
-        den = np.linalg.norm(fd.reshape(T, -1), axis=1).max()
-        worst = max(worst, float(num / den))
-    return worst
+        with np.errstate(over="ignore", invalid="ignore"):
+            num = float(np.linalg.norm((fm - fd).reshape(T, -1), axis=1).max())
+            den = float(np.linalg.norm(fd.reshape(T, -1), axis=1).max())
+        if not (np.isfinite(num) and np.isfinite(den)):
+            err = float("inf")
+        else:
+            err = num / den if den > 0.0 else (0.0 if num == 0.0 else float("inf"))
+        if not err <= worst:
+            worst, at = err, bd
+    return worst, at
+
+
+def trajectory_drift(ctx_m, ids_m, ctx_d, ids_d, meshes):
+    """Worst body of max |x^mixed - x^FP64| / max |x^FP64 - X| after both runs
+    took the same steps: the mixed path's error in positions, FP64 as the
+    reference (the PCG tolerance bounds it, not the element precision)."""
+    worst, at = 0.0, None
+    for (m, _), bm, bd in zip(meshes, ids_m, ids_d):
+        xm, _, _ = ctx_m.state(bm)
+        xd, _, _ = ctx_d.state(bd)
+        X = m["rest_positions"].astype(np.float64)
+        with np.errstate(over="ignore", invalid="ignore"):
+            err = float(np.abs(xm - xd).max() / max(np.abs(xd - X).max(), 1e-300))
+        if not err <= worst:
+            worst, at = err, bd
+    return worst, at
 
 
 def traffic_for(kernel, work, plan):
@@ -136,29 +212,59 @@ def traffic_for(kernel, work, plan):
     return rec.get("bytes_per_launch") if rec.get("plan") == plan else None
 
 
-def cpu_baseline(sc, budget_s, threads):
-    """The oracle's FP64 step (block-Jacobi PCG, its own CPU restatement) on a
-    bounded sample of the same bodies, one body per worker thread; only the
-    steps are timed."""
+def host_cpu():
+    """CPU model, logical CPUs and the CPUs this process may run on."""
+    model = "unknown"
+    try:
+        with open("/proc/cpuinfo") as f:
+            for line in f:
+                if line.startswith("model name"):
+                    model = line.split(":", 1)[1].strip()
+                    break
+    except OSError:
+        pass
+    return {"model": model, "nproc": os.cpu_count(), "affinity": len(os.sched_getaffinity(0))}
+
+
+def cpu_baseline(sc, snaps, budget_s, threads, tolerance):
+    """The oracle's FP64 step (its C restatement, block-Jacobi PCG to the same
+    relative tolerance as the GPU, no iteration cap) on a bounded sample of
+    the same bodies, each from the GPU FP64 batch's state at the window start,
+    one body per worker thread; only the steps are timed."""
     sys.path.insert(0, os.path.join(ROOT, "tests"))
     from helpers import oracle_body
 
     order = np.random.default_rng(99).permutation(len(sc.bodies))
-    done = updates = tets = 0
+
+    def prepared(i):
+        b = oracle_body(sc.bodies[i])[0]
+        s = snaps[i]
+        b.set_state(s["x"], s["v"], s["a"])
+        b.set_plastic(s["plastic"])
+        b.set_rotations(s["R"])
+        b.set_frames(s["W"])
+        return b
+
+    done = updates = tets = iters = 0
     wall = 0.0
     with cf.ThreadPoolExecutor(threads) as pool:
         while done < len(order) and wall < budget_s:
-            chunk = [oracle_body(sc.bodies[i])[0] for i in order[done:done + threads]]
+            chunk = [prepared(i) for i in order[done:done + threads]]
             t0 = time.perf_counter()
-            list(pool.map(lambda b: b.step(substeps=sc.substeps, threads=1), chunk))
+            reps = list(pool.map(lambda b: b.step(substeps=sc.substeps, threads=1, cg_iterations=1 << 20,
+                                                  tolerance=tolerance), chunk))
             wall += time.perf_counter() - t0
             tets += sum(b.tets for b in chunk)
-            updates += sum(b.tets for b in chunk) * sc.substeps
+            updates += sum(b.tets * r["substeps"] for b, r in zip(chunk, reps))
+            iters += sum(r["cg_iterations"] for r in reps)
             done += len(chunk)
     return {"value": updates / wall, "unit": "tet updates/s", "cores": threads, "kind": "port",
             "sample": f"one FP64 step ({sc.substeps} substeps) of {done} of the {len(order)} bodies "
-                      f"(seeded order, {tets} tets), {threads} worker threads, one body per thread, "
-                      f"with the oracle's C restatement (block-Jacobi PCG)"}
+                      f"(seeded order, {tets} tets) from the GPU FP64 batch's state at the window start, "
+                      f"{threads} worker thread(s), one body per thread, with the oracle's C restatement "
+                      f"(block-Jacobi PCG to relative tolerance {tolerance:g}, uncapped: "
+                      f"{iters} CG iterations)",
+            "host": host_cpu()}
 
 
 def copy_probe(nbytes):
@@ -415,7 +521,7 @@ def adjoint_check(steps, every, elements, seed=5):
 def main(argv=None):
     ap = argparse.ArgumentParser(description=__doc__)
     ap.add_argument("--gpus", type=int, default=1)
-    ap.add_argument("--steps", type=int, default=10)
+    ap.add_argument("--steps", type=int, default=20)
     ap.add_argument("--warmup", type=int, default=5)
     ap.add_argument("--workload", choices=["bodies", "cube"], default="bodies")
     ap.add_argument("--skip-fp64", action="store_true")
@@ -433,8 +539,10 @@ def main(argv=None):
                     help="autotune the mixed context even where profiles/solve_plan_<workload>.txt "
                          "holds a plan tuned earlier")
     ap.add_argument("--cpu-seconds", type=float, default=15.0)
+    ap.add_argument("--cpu-seconds-single", type=float, default=10.0)
     ap.add_argument("--cpu-threads", type=int, default=16)
-    ap.add_argument("--plain-steps", type=int, default=2)
+    ap.add_argument("--plain-steps", type=int, default=10,
+                    help="timed steps of the unbatched FP64 run, from the start of the timed window")
     ap.add_argument("--slab", type=int, default=0, metavar="STEPS",
                     help="instead of the batch: the crushed slab's first STEPS steps with tearing")
     ap.add_argument("--adjoint", action="store_true",
@@ -492,38 +600,73 @@ def main(argv=None):
                                 flags=api.DIRECT_LAUNCH if args.direct_launch else 0)
     log(f"[bench] workload {args.workload}: {len(meshes)} bodies, {T} tets, {N} nodes")
 
+    w0 = max(args.warmup, 1)  # the first step (factorization, plan) is never timed
+    total = max(sc.steps, w0 + args.steps)
+    window = [w0, w0 + args.steps]
+    log(f"[bench] {total} steps per run, timed window [{window[0]}, {window[1]})")
+
     log("[bench] mixed batched")
-    ctx_m, ids_m, wall_m, reps_m, tuned = batched(api, meshes, api.MIXED, params, args.steps,
-                                                 args.warmup, not args.no_tune, args.plan, args.streams)
+    ctx_m, ids_m, wall_m, reps_m, tuned, _ = run_batch(api, meshes, api.MIXED, params, args.steps,
+                                                      args.warmup, not args.no_tune, args.plan,
+                                                      args.streams)
     value = T * sub * args.steps / wall_m
+    timed_m = reps_m[-args.steps:]
     plan_steps = "".join(map(str, ctx_m.plan()))
-    apps = sum(r["preconditioner_applications"] for r in reps_m) / args.steps
+    apps = sum(r["preconditioner_applications"] for r in timed_m) / args.steps
     log(f"[bench] mixed {1e3 * wall_m / args.steps:.3f} ms/step, {value:.4g} tet/s, "
         f"{apps:.1f} preconditioner applications/step, tuned {tuned}")
     finfo = ctx_m.factor_info()
 
     fp64 = None
-    ctx_d = ids_d = None
+    ctx_d = ids_d = snaps = None
+    reps_d = []
     if not args.skip_fp64:
         log("[bench] fp64 batched")
-        ctx_d, ids_d, wall_d, reps_d, tuned_d = batched(api, meshes, api.FP64, params, args.steps,
-                                                        args.warmup, not args.no_tune, None, args.streams)
+        ctx_d, ids_d, wall_d, reps_d, tuned_d, snaps = run_batch(api, meshes, api.FP64, params, args.steps,
+                                                                 args.warmup, not args.no_tune, None,
+                                                                 args.streams, snapshot=True)
+        timed_d = reps_d[-args.steps:]
         fp64 = {"value": T * sub * args.steps / wall_d, "ms_per_step": 1e3 * wall_d / args.steps,
-                "cg_iterations_per_step": sum(r["cg_iterations"] for r in reps_d) / args.steps,
-                "cg_capped": sum(r["cg_capped"] for r in reps_d)}
+                "cg_iterations_per_step": sum(r["cg_iterations"] for r in timed_d) / args.steps,
+                "cg_capped": sum(r["cg_capped"] for r in timed_d)}
         log(f"[bench] fp64 batched {fp64['ms_per_step']:.3f} ms/step, {fp64['value']:.4g} tet/s")
 
-    fp64_plain = None
-    if not args.skip_plain:
-        log("[bench] fp64 plain (one context per body, sequential)")
-        wall_p = plain(api, meshes, params, args.plain_steps)
-        fp64_plain = {"value": T * sub * args.plain_steps / wall_p,
-                      "ms_per_step": 1e3 * wall_p / args.plain_steps, "steps": args.plain_steps}
-        log(f"[bench] fp64 plain {fp64_plain['ms_per_step']:.1f} ms/step, {fp64_plain['value']:.4g} tet/s")
+    # precision, at the end of the timed window and after the whole run
+    precision = {}
+    if ctx_d is not None:
+        err, body = force_error(ctx_m, ids_m, ctx_d, ids_d)
+        drift, dbody = trajectory_drift(ctx_m, ids_m, ctx_d, ids_d, meshes)
+        precision[f"step_{window[1]}"] = {"rel_force_error": err, "worst_body": body,
+                                          "trajectory_drift": drift, "drift_body": dbody}
+        log(f"[bench] after step {window[1]}: rel force error {err:.3e} (body {body}), "
+            f"trajectory drift {drift:.3e} (body {dbody})")
+    while len(reps_m) < total:
+        reps_m.append(ctx_m.step(params))
+    if ctx_d is not None:
+        while len(reps_d) < total:
+            reps_d.append(ctx_d.step(params))
+        err, body = force_error(ctx_m, ids_m, ctx_d, ids_d)
+        drift, dbody = trajectory_drift(ctx_m, ids_m, ctx_d, ids_d, meshes)
+        precision[f"step_{total}"] = {"rel_force_error": err, "worst_body": body,
+                                      "trajectory_drift": drift, "drift_body": dbody}
+        log(f"[bench] after step {total}: rel force error {err:.3e} (body {body}), "
+            f"trajectory drift {drift:.3e} (body {dbody})")
+    rel_err = max((p["rel_force_error"] for p in precision.values()), default=None,
+                  key=lambda e: e if e == e else float("inf"))
+    runs = {"mixed": health(reps_m)}
+    if reps_d:
+        runs["fp64"] = health(reps_d)
+    log(f"[bench] run health {json.dumps(runs)}")
 
-    rel_err = force_error(ctx_m, ids_m, ctx_d, ids_d) if ctx_d is not None else None
-    if rel_err is not None:
-        log(f"[bench] rel force error mixed vs fp64 (worst body): {rel_err:.3e}")
+    fp64_plain = None
+    if not args.skip_plain and snaps is not None:
+        k = max(1, min(args.plain_steps, args.steps))
+        log(f"[bench] fp64 plain (one context per body, sequential), steps [{window[0]}, {window[0] + k})")
+        wall_p, bad_p, capped_p = plain(api, meshes, params, k, snaps)
+        fp64_plain = {"value": T * sub * k / wall_p, "ms_per_step": 1e3 * wall_p / k,
+                      "steps": k, "window": [window[0], window[0] + k],
+                      "non_finite_steps": bad_p, "cg_capped": capped_p}
+        log(f"[bench] fp64 plain {fp64_plain['ms_per_step']:.1f} ms/step, {fp64_plain['value']:.4g} tet/s")
 
     # per-kernel launch durations (HIP events on the context stream) and the
     # launches of each kernel per step, on the whole batch in one engine (the
@@ -564,10 +707,15 @@ def main(argv=None):
     log(f"[bench] kernels {json.dumps(kernels)}")
 
     cpu = None
-    if not args.skip_cpu:
-        log(f"[bench] cpu baseline, {args.cpu_threads} threads, ~{args.cpu_seconds:.0f} s")
-        cpu = cpu_baseline(sc, args.cpu_seconds, args.cpu_threads)
+    if not args.skip_cpu and snaps is not None:
+        threads = max(1, min(args.cpu_threads, len(os.sched_getaffinity(0))))
+        log(f"[bench] cpu baseline, {threads} threads, ~{args.cpu_seconds:.0f} s")
+        cpu = cpu_baseline(sc, snaps, args.cpu_seconds, threads, params.tolerance)
         log(f"[bench] cpu {cpu['value']:.4g} tet/s")
+        log(f"[bench] cpu baseline, 1 thread, ~{args.cpu_seconds_single:.0f} s")
+        one = cpu_baseline(sc, snaps, args.cpu_seconds_single, 1, params.tolerance)
+        cpu["single_thread"] = {"value": one["value"], "sample": one["sample"]}
+        log(f"[bench] cpu 1 thread {one['value']:.4g} tet/s")
 
     line = {
         "metric": METRIC,
@@ -586,10 +734,11 @@ def main(argv=None):
             "workload": desc,
             "bodies": len(meshes), "tets": T, "nodes": N, "substeps": sub, "precision": "mixed",
             "streams": args.streams,
-            "cg_iterations_per_step": sum(r["cg_iterations"] for r in reps_m) / args.steps,
-            "cg_capped": sum(r["cg_capped"] for r in reps_m),
+            "steps_run": total, "timed_window": window,
+            "cg_iterations_per_step": sum(r["cg_iterations"] for r in timed_m) / args.steps,
+            "cg_capped": sum(r["cg_capped"] for r in timed_m),
             "preconditioner_applications_per_step": apps,
-            "refactors": sum(r["refactors"] for r in reps_m),
+            "refactors": sum(r["refactors"] for r in timed_m),
             "solve_plan": plan_steps, "solve_plan_source": plan_source,
             "kernel_table_plan": "".join(map(str, ctx_m.plan())),
             "tune": tuned,
@@ -600,14 +749,37 @@ def main(argv=None):
         "mixed_over_fp64": (value / fp64["value"]) if fp64 else None,
         "mixed_over_fp64_plain": (value / fp64_plain["value"]) if fp64_plain else None,
         "rel_force_error": rel_err,
+        "precision": precision,
+        "runs": runs,
         "kernels": kernels,
         "roofline": roofline,
         "cpu_baseline": cpu,
         "gpu_over_cpu": (value / cpu["value"]) if cpu else None,
     }
+    # the metric holds only for a finite run with converged solves and the
+    # mixed kernel within 1e-5 of FP64: otherwise the line carries no value
+    failed = []
+    for name, h in runs.items():
+        if h["non_finite_steps"]:
+            failed.append(f"{name}: {h['non_finite_steps']} non-finite step(s)")
+        if h["cg_capped"]:
+            failed.append(f"{name}: {h['cg_capped']} capped solve(s)")
+    if fp64_plain and (fp64_plain["non_finite_steps"] or fp64_plain["cg_capped"]):
+        failed.append("fp64 plain: non-finite steps or capped solves")
+    if rel_err is None:
+        failed.append("no FP64 run to check the force error against")
+    elif not rel_err <= FORCE_BAR:
+        failed.append(f"relative force error {rel_err} above {FORCE_BAR}")
+    if failed:
+        line["failed"] = failed
+        line["value_if_valid"] = line["value"]
+        line["value"] = None
     print(json.dumps(line), flush=True)
     if args.copy_probe:
         copy_probe(256 << 20)
+    if failed:
+        log("[bench] INVALID: " + "; ".join(failed))
+        return 1
     return 0
 
 
