@@ -251,6 +251,24 @@ int talus_mi_eval_forces(talus_mi_ctx *ctx, uint32_t body, const double *x,
  * solver.cc:904-1048) with the committed rotations. */
 int talus_mi_apply_system(talus_mi_ctx *ctx, uint32_t body, double inertia,
                           double step, const double *d, double *y);
+/* Size of the allocation-free path's per-GPU scratch array (MiB). */
+#define TALUS_MI_NOALLOC_SCRATCH_MIB 48
+/* talus_mi_apply_system without any allocation.  No counterpart in the
+ * reference: it is for callers that cannot allocate device memory (code
+ * under HIP graph capture, tight serving loops, harnesses that forbid
+ * hipMalloc).  d and y are DEVICE pointers to 3*nodes doubles of the body;
+ * the operator is enqueued on talus_mi_stream(ctx) and y holds the same
+ * values as talus_mi_apply_system's once that stream has synchronized.  The
+ * body's per-tet slots (12*tets values in the context's precision) live in
+ * one static device array of TALUS_MI_NOALLOC_SCRATCH_MIB MiB per GPU,
+ * resolved by talus_mi_create: bodies beyond it (more than 524,288 tets in
+ * FP64, 1,048,576 in mixed precision) return TALUS_MI_ERR_ARGUMENT.  The
+ * context must be built since its last body edit (talus_mi_step,
+ * talus_mi_apply_system, ...), else TALUS_MI_ERR_STATE.  Every call on a GPU
+ * shares that GPU's array and keeps no state in it between calls: calls that
+ * overlap on one GPU (other streams or contexts) race. */
+int talus_mi_apply_system_noalloc(talus_mi_ctx *ctx, uint32_t body, double inertia,
+                                  double step, const double *d, double *y);
 /* Times one kernel at the current state with HIP events on the context
  * stream (`repeats` rounds after one warm-up) and states its algorithmic
  * HBM bytes (what the algorithm must move: element arrays once per tet, node

@@ -220,6 +220,8 @@ _SIGNATURES = {
                                        _P(C.c_double), _P(C.c_double), _P(C.c_double)]),
     "talus_mi_apply_system": (C.c_int, [C.c_void_p, C.c_uint32, C.c_double, C.c_double,
                                         _P(C.c_double), _P(C.c_double)]),
+    "talus_mi_apply_system_noalloc": (C.c_int, [C.c_void_p, C.c_uint32, C.c_double, C.c_double,
+                                                C.c_void_p, C.c_void_p]),
     "talus_mi_time_kernel": (C.c_int, [C.c_void_p, C.c_uint32, C.c_uint32, _P(KernelTiming)]),
     "talus_mi_factor_solve": (C.c_int, [C.c_void_p, C.c_uint32, C.c_double, C.c_double,
                                         _P(C.c_double), _P(C.c_double)]),
@@ -673,6 +675,13 @@ class Context:
         y = np.empty(3 * n)
         _check(lib().talus_mi_apply_system(self._h, body, inertia, step, _dp(d), _dp(y)))
         return y.reshape(n, 3)
+
+    def apply_system_noalloc(self, body, inertia, step, d_ptr: int, y_ptr: int):
+        """The operator on device vectors (addresses of 3*nodes doubles),
+        enqueued on the context's stream with no allocation; y is ready once
+        that stream has synchronized (talus_mi_apply_system_noalloc)."""
+        _check(lib().talus_mi_apply_system_noalloc(self._h, body, inertia, step, C.c_void_p(d_ptr),
+                                                   C.c_void_p(y_ptr)))
 
     def time_kernel(self, kind, repeats=20) -> dict:
         """Per-launch HIP-event time and algorithmic bytes of one kernel

@@ -20,6 +20,7 @@
 #include <cmath>
 #include <cstring>
 #include <memory>
+#include <mutex>
 #include <stdexcept>
 #include <string>
 #include <thread>
@@ -31,6 +32,13 @@
 #include "hexlattice.h"
 #include "kernels.h"
 #include "tmath.h"
+
+// Slot buffer of the allocation-free operator entry point: a file-scope
+// device array, one copy per GPU for the life of the process.  Host code
+// uses only the per-device address from hipGetSymbolAddress (the array's
+// name is a host-side stand-in, not a device address).
+constexpr size_t NOALLOC_SCRATCH_BYTES = size_t(TALUS_MI_NOALLOC_SCRATCH_MIB) << 20;
+static __device__ __align__(256) double scratch[NOALLOC_SCRATCH_BYTES / sizeof(double)];
 
 namespace tmi {
 
@@ -107,6 +115,10 @@ class EngineBase {
                            double *R) = 0;
   virtual void apply_system(uint32_t b, double inertia, double step, const double *dv,
                             double *y) = 0;
+  // the operator on device vectors with a caller-given slot buffer; launches
+  // only (no allocation, no host copy, no wait) on stream s
+  virtual void apply_system_device(uint32_t b, double inertia, double step, const double *dv,
+                                   double *y, void *slots, hipStream_t s) = 0;
   virtual void time_kernel(uint32_t kind, uint32_t repeats, talus_mi_kernel_timing &out) = 0;
   virtual void factor_solve(uint32_t b, double inertia, double step, const double *r,
                             double *z) = 0;
@@ -150,6 +162,11 @@ class Engine final : public EngineBase {
                    double *R) override;
   void apply_system(uint32_t b, double inertia, double step, const double *dv,
                     double *y) override;
+  void apply_system_device(uint32_t b, double inertia, double step, const double *dv, double *y,
+                           void *slots, hipStream_t s) override {
+    Launch<Real>::apply_body(d_, t0_[b], t0_[b] + tt_[b], n0_[b], n0_[b] + nn_[b], inc_start_[n0_[b]], dv, y,
+                             inertia, step * step, static_cast<Real *>(slots), s);
+  }
   void time_kernel(uint32_t kind, uint32_t repeats, talus_mi_kernel_timing &out) override;
   void factor_solve(uint32_t b, double inertia, double step, const double *r, double *z) override;
   void factor_info(talus_mi_factor_info &info) override;
@@ -1672,6 +1689,10 @@ class GroupedEngine final : public EngineBase {
   void apply_system(uint32_t b, double inertia, double step, const double *dv, double *y) override {
     eng_[group_[b]]->apply_system(local_[b], inertia, step, dv, y);
   }
+  void apply_system_device(uint32_t b, double inertia, double step, const double *dv, double *y, void *slots,
+                           hipStream_t s) override {
+    eng_[group_[b]]->apply_system_device(local_[b], inertia, step, dv, y, slots, s);
+  }
   void factor_solve(uint32_t b, double inertia, double step, const double *r, double *z) override {
     eng_[group_[b]]->factor_solve(local_[b], inertia, step, r, z);
   }
@@ -1796,6 +1817,28 @@ namespace {
 
 using tmi::fail;
 
+constexpr int MAX_DEVICES = 64;
+std::mutex g_scratch_mutex;
+void *g_scratch_ptr[MAX_DEVICES] = {};
+
+// resolves the array's address on `device`, which must be current; once per
+// device (talus_mi_create)
+void resolve_scratch(int device) {
+  std::lock_guard<std::mutex> lock(g_scratch_mutex);
+  if (device < 0 || device >= MAX_DEVICES) throw std::runtime_error("device index beyond the scratch table");
+  if (g_scratch_ptr[device]) return;
+  void *ptr = nullptr;
+  const hipError_t rc = hipGetSymbolAddress(&ptr, HIP_SYMBOL(scratch));
+  if (rc != hipSuccess || !ptr)
+    throw std::runtime_error(std::string("hipGetSymbolAddress(scratch): ") + hipGetErrorString(rc));
+  g_scratch_ptr[device] = ptr;
+}
+
+void *scratch_of(int device) {
+  std::lock_guard<std::mutex> lock(g_scratch_mutex);
+  return device >= 0 && device < MAX_DEVICES ? g_scratch_ptr[device] : nullptr;
+}
+
 // one engine on the context's stream, or bodies split over `streams` engines
 std::unique_ptr<tmi::EngineBase> make_engine(talus_mi_ctx *ctx) {
   auto one = [precision = ctx->precision](hipStream_t s) -> std::unique_ptr<tmi::EngineBase> {
@@ -1865,6 +1908,7 @@ int talus_mi_create(talus_mi_ctx **out, int device, uint32_t precision) {
   ctx->precision = precision;
   try {
     HIPCHECK(hipSetDevice(device));
+    resolve_scratch(device);
     HIPCHECK(hipStreamCreateWithFlags(&ctx->stream, hipStreamNonBlocking));
     ctx->engine = make_engine(ctx.get());
   } catch (const std::exception &e) {
@@ -2225,6 +2269,27 @@ int talus_mi_apply_system(talus_mi_ctx *ctx, uint32_t body, double inertia, doub
   return guarded(ctx, [&]() -> int {
     ensure(ctx);
     ctx->engine->apply_system(body, inertia, step, d, y);
+    return TALUS_MI_OK;
+  });
+}
+
+int talus_mi_apply_system_noalloc(talus_mi_ctx *ctx, uint32_t body, double inertia, double step,
+                                  const double *d, double *y) {
+  if (!ctx || !body_ok(ctx, body) || !d || !y) return fail(TALUS_MI_ERR_ARGUMENT, "bad arguments");
+  const size_t real = ctx->precision == TALUS_MI_MIXED ? sizeof(float) : sizeof(double);
+  const size_t need = 12 * size_t(ctx->bodies[body].tets) * real;
+  if (need > NOALLOC_SCRATCH_BYTES)
+    return fail(TALUS_MI_ERR_ARGUMENT, "body of " + std::to_string(ctx->bodies[body].tets) +
+                                           " tets needs " + std::to_string(need) + " scratch bytes, above the " +
+                                           std::to_string(NOALLOC_SCRATCH_BYTES) + " of the allocation-free path");
+  if (ctx->dirty)
+    return fail(TALUS_MI_ERR_STATE, "the context's device arrays are not built since the last body edit "
+                                    "(call talus_mi_step or talus_mi_apply_system first)");
+  return guarded(ctx, [&]() -> int {
+    void *slots = scratch_of(ctx->device);
+    if (!slots) return fail(TALUS_MI_ERR_STATE, "scratch address not resolved on this device");
+    ctx->engine->apply_system_device(body, inertia, step, d, y, slots, ctx->stream);
+    HIPCHECK(hipGetLastError());
     return TALUS_MI_OK;
   });
 }

@@ -126,6 +126,11 @@ struct Launch {
   static void apply_plain(const Dev<Real> &d, const Real *dir, Real *out, double inertia,
                           double reach, hipStream_t s);
   static void op_element_plain(const Dev<Real> &d, const Real *dir, hipStream_t s);
+  // one body's operator on body-local FP64 device vectors, its 4 T_b corner
+  // slots in `slots` (3 T_b * 4 values) instead of the engine's arrays
+  static void apply_body(const Dev<Real> &d, uint32_t t0, uint32_t t1, uint32_t n0, uint32_t n1,
+                         uint32_t k0, const double *din, double *yout, double inertia, double reach,
+                         Real *slots, hipStream_t s);
   static void gather_forces(const Dev<Real> &d, Real *out, hipStream_t s);
   // convergence of the corrections: per node tile max |dx| and max |a|
   static void increment_norms(const Dev<Real> &d, double *out, hipStream_t s);

@@ -342,10 +342,8 @@ __global__ __launch_bounds__(TILE) void k_pcg_begin(Dev<Real> d) {
 // the linear stiffness; under stretch the volumetric part follows the
 // (lambda+mu)(J-1)^2 term's stiffening along cof(F).
 template <class Real>
-__device__ __forceinline__ void op_tet(const Dev<Real> &d, uint32_t e, const Real *dir, Real *y) {
-  const uint4 c = d.conn[e];
-  Real dv[12], dm[9], g[12], gs[12], Rm[9];
-  gather_vec(dir, c, dv);
+__device__ __forceinline__ void op_tet_values(const Dev<Real> &d, uint32_t e, const Real *dv, Real *y) {
+  Real dm[9], g[12], gs[12], Rm[9];
   load_dm(d, e, dm);
   shape_gradients(dm, g);
   const uint8_t flags = d.flags[e];
@@ -397,6 +395,13 @@ __device__ __forceinline__ void op_tet(const Dev<Real> &d, uint32_t e, const Rea
       for (int r = 0; r < 3; ++r) y[3 * i + r] += vol * wg[r];
     }
   }
+}
+
+template <class Real>
+__device__ __forceinline__ void op_tet(const Dev<Real> &d, uint32_t e, const Real *dir, Real *y) {
+  Real dv[12];
+  gather_vec(dir, d.conn[e], dv);
+  op_tet_values(d, e, dv, y);
 }
 
 template <class Real>
@@ -961,6 +966,53 @@ __global__ __launch_bounds__(TILE) void k_apply_plain(Dev<Real> d, const Real *d
   for (int c = 0; c < 3; ++c) out[3 * (size_t)n + c] = q[c];
 }
 
+// The operator of one body with its own input, output and slot buffer (the
+// allocation-free entry point): body-local FP64 vectors d and y (3 per node
+// from the body's first node n0), the body's 4 T_b corner slots at their CSR
+// positions less the body's first position k0.  The same operations in the
+// same order as k_op_element_flat + k_apply_plain.
+template <class Real>
+__global__ __launch_bounds__(TILE) void k_apply_body_element(Dev<Real> d, uint32_t t0, uint32_t t1,
+                                                             uint32_t n0, uint32_t k0,
+                                                             const double *din, Real *slots) {
+  const uint32_t e = t0 + blockIdx.x * TILE + threadIdx.x;
+  if (e >= t1) return;
+  const uint4 c = d.conn[e];
+  const uint32_t nn[4] = {c.x - n0, c.y - n0, c.z - n0, c.w - n0};
+  Real dv[12], y[12];
+#pragma unroll
+  for (int i = 0; i < 4; ++i)
+#pragma unroll
+    for (int r = 0; r < 3; ++r) dv[3 * i + r] = Real(din[3 * (size_t)nn[i] + r]);
+  op_tet_values(d, e, dv, y);
+  const uint4 pos = d.slot_pos[e];
+  store_slots(slots, make_uint4(pos.x - k0, pos.y - k0, pos.z - k0, pos.w - k0), y);
+}
+
+template <class Real>
+__global__ __launch_bounds__(TILE) void k_apply_body_gather(Dev<Real> d, uint32_t n0, uint32_t n1,
+                                                            uint32_t k0, const Real *slots,
+                                                            const double *din, double *yout,
+                                                            double inertia, double reach) {
+  const uint32_t n = n0 + blockIdx.x * TILE + threadIdx.x;
+  if (n >= n1) return;
+  const size_t l = n - n0;
+  Real q[3] = {Real(0), Real(0), Real(0)};
+  if (!d.anchored[n]) {
+    const uint32_t k1 = d.inc_start[n + 1] - k0;
+    for (uint32_t k = d.inc_start[n] - k0; k < k1; ++k) {
+      q[0] += slots[3 * (size_t)k + 0];
+      q[1] += slots[3 * (size_t)k + 1];
+      q[2] += slots[3 * (size_t)k + 2];
+    }
+    const Real im = Real(inertia * d.mass[n]), rc = Real(reach);
+#pragma unroll
+    for (int c = 0; c < 3; ++c) q[c] = im * Real(din[3 * l + c]) + rc * q[c];
+  }
+#pragma unroll
+  for (int c = 0; c < 3; ++c) yout[3 * l + c] = double(q[c]);
+}
+
 inline unsigned blocks_for(uint32_t n) { return (n + TILE - 1) / TILE; }
 
 // ------------------------------------------------- corrections converge
@@ -1330,6 +1382,16 @@ void Launch<Real>::apply_plain(const Dev<Real> &d, const Real *dir, Real *out, d
                                double reach, hipStream_t s) {
   TMI_LAUNCH(k_apply_plain<Real>, blocks_for(d.N), s, d, dir, out, inertia, reach);
 }
+template <class Real>
+void Launch<Real>::apply_body(const Dev<Real> &d, uint32_t t0, uint32_t t1, uint32_t n0, uint32_t n1,
+                              uint32_t k0, const double *din, double *yout, double inertia,
+                              double reach, Real *slots, hipStream_t s) {
+  if (t1 > t0) TMI_LAUNCH(k_apply_body_element<Real>, blocks_for(t1 - t0), s, d, t0, t1, n0, k0, din, slots);
+  if (n1 > n0)
+    TMI_LAUNCH(k_apply_body_gather<Real>, blocks_for(n1 - n0), s, d, n0, n1, k0, slots, din, yout, inertia,
+               reach);
+}
+
 template <class Real>
 void Launch<Real>::op_element_plain(const Dev<Real> &d, const Real *dir, hipStream_t s) {
   TMI_LAUNCH(k_op_element_flat<Real>, blocks_for(d.T), s, d, dir);
