@@ -261,3 +261,5 @@ def test_damped_plastic_strip_tears_like_the_oracle():
         oP, oa = orc.plastic()
     assert float(np.abs(x - ox).max()) <= 1e-6
     assert float(np.abs(a - oa).max()) <= 1e-6 * max(1.0, float(np.abs(oa).max()))
+    assert P.shape == oP.shape
+    assert float(np.abs(P - oP).max()) <= 1e-6 * max(1.0, float(np.abs(oP).max()))
