@@ -1337,7 +1337,6 @@ static int predictor_inverts(orc_body *b, const rule_t *rule, double inner) {
 static uint32_t commit_plastic(orc_body *b) {
   uint32_t grew = 0;
   for (size_t e = 0; e < b->m.tets; ++e) {
-    b->yflag[e] = 0;
     if (!b->plastic_lane[e]) continue;
     double x[12], X[12], u[12], F[9];
     gather4(b, b->x, e, x);
@@ -1443,6 +1442,8 @@ static double substep(orc_body *b, const orc_settings *s, const rule_t *rule, do
  * elastic path 3895-3950 is used for every lane, see the header note). */
 int orc_step(orc_body *b, const orc_settings *s, orc_report *rep) {
   memset(rep, 0, sizeof *rep);
+  /* yield events of this step: tets whose plastic state grew in any substep */
+  memset(b->yflag, 0, b->m.tets);
   const size_t N = b->m.nodes;
   const rule_t rule = rule_of(s->spectral_radius);
   const uint32_t configured = s->substeps > 1 ? s->substeps : 1;

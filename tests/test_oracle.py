@@ -475,3 +475,26 @@ def test_oracle_threads_do_not_change_results():
         a.step(threads=1)
         b.step(threads=6)
     assert np.array_equal(a.state()[0], b.state()[0])
+
+
+def test_inverting_predictor_raises_the_step_to_four_substeps():
+    """predictorInverts (solver.cc:568-623, 3574-3583): a velocity that moves
+    the tip face past the middle layer in the alpha-point predictor inverts
+    tets, and the step takes 4 substeps; the body at rest keeps 1; a
+    configured 6 stays 6 (max(configured, 4))."""
+    mats = scenes.material_array(1, youngs_modulus=1e5, poissons_ratio=0.3, density=1000.0,
+                                 elastic_model=1)
+    body = scenes.Body(resolution=(3, 2, 2), voxel=(0.05, 0.05, 0.05),
+                       supports=[((0.0, 0.0, 0.0), (0.01, 1e3, 1e3))], materials=mats)
+    orc, om = oracle_body(body)
+    rest = om["rest"]
+    assert orc.step(tolerance=1e-10, cg_iterations=2000)["substeps"] == 1
+    v = np.zeros_like(rest)
+    v[rest[:, 0] > 0.09, 0] = -20.0
+    for configured, expect in ((1, 4), (6, 6)):
+        orc.set_state(rest, v, np.zeros_like(rest))
+        r = orc.step(substeps=configured, tolerance=1e-10, cg_iterations=2000)
+        assert r["substeps"] == expect
+    # a tenth of that velocity compresses the tip cells by 44 %: no inversion
+    orc.set_state(rest, 0.1 * v, np.zeros_like(rest))
+    assert orc.step(tolerance=1e-10, cg_iterations=2000)["substeps"] == 1
