@@ -66,6 +66,7 @@ class Settings(C.Structure):
         ("substeps", C.c_uint32), ("cg_iterations", C.c_uint32),
         ("tolerance", C.c_float), ("spectral_radius", C.c_float),
         ("threads", C.c_uint32), ("converge", C.c_uint32),
+        ("plastic_passes", C.c_uint32),
     ]
 
 
@@ -76,6 +77,10 @@ class Report(C.Structure):
         ("cg_worst_relative", C.c_double), ("elastic_energy", C.c_double),
         ("kinetic_energy", C.c_double), ("plastic_growth", C.c_uint32),
         ("yield_events", C.c_uint32), ("non_finite", C.c_uint32),
+        ("plastic_evaluations", C.c_uint32), ("plastic_solves", C.c_uint32),
+        ("plastic_backtracks", C.c_uint32), ("plastic_linear_reads", C.c_uint32),
+        ("plastic_handoffs", C.c_uint32), ("plastic_fallbacks", C.c_uint32),
+        ("plastic_rejections", C.c_uint32), ("plastic_capped", C.c_uint32),
     ]
 
 
@@ -118,6 +123,11 @@ def lib():
         L.orc_body_get_plastic.argtypes = [C.c_void_p, dp]
         L.orc_body_set_plastic.argtypes = [C.c_void_p, dp]
         L.orc_step.argtypes = [C.c_void_p, C.POINTER(Settings), C.POINTER(Report)]
+        L.orc_j2_point.argtypes = [dp, dp, C.c_double, C.c_double, C.c_double, C.c_double, dp, dp]
+        L.orc_j2_point.restype = C.c_int
+        L.orc_j2_tangent.argtypes = [dp, C.c_double, C.c_double, dp, dp]
+        L.orc_tangent_apply.argtypes = [C.c_void_p, dp, dp, dp]
+        L.orc_tangent_apply.restype = C.c_uint32
         L.orc_eval_forces.argtypes = [C.c_void_p, dp, C.c_uint32, dp, dp, dp, dp]
         L.orc_apply_system.argtypes = [C.c_void_p, C.c_double, C.c_double, dp, dp, C.c_uint32]
         L.orc_tet_stretch.argtypes = [C.c_void_p, dp]
@@ -294,9 +304,10 @@ class Body:
         lib().orc_body_set_plastic(self._h, _dp(p))
 
     def step(self, dt=1.0 / 60.0, gravity=(0.0, -9.81, 0.0), substeps=1, cg_iterations=64,
-             tolerance=1e-3, spectral_radius=0.5, threads=1, converge=False):
+             tolerance=1e-3, spectral_radius=0.5, threads=1, converge=False, plastic_passes=40):
         s = Settings()
         s.converge = 1 if converge else 0
+        s.plastic_passes = plastic_passes
         s.dt = dt
         s.gravity[:] = list(gravity)
         s.substeps = substeps
@@ -395,6 +406,15 @@ class Body:
             out[k] = out[k].reshape(-1, 3)
         return out
 
+    def tangent_apply(self, x, d):
+        """(K_tangent d, loading tets) at positions x, as a Newton attempt
+        assembles it (loading margin 1e-3, consistent tangent bound)."""
+        x = np.ascontiguousarray(x, dtype=np.float64).reshape(-1)
+        d = np.ascontiguousarray(d, dtype=np.float64).reshape(-1)
+        y = np.empty_like(d)
+        n = lib().orc_tangent_apply(self._h, _dp(x), _dp(d), _dp(y))
+        return y.reshape(-1, 3), int(n)
+
     def yield_flags(self):
         f = np.empty(self.tets, dtype=np.uint8)
         lib().orc_yield_flags(self._h, f.ctypes.data_as(C.POINTER(C.c_uint8)))
@@ -433,6 +453,24 @@ def corotated_tet(x, X, dminv, volume, lam, mu, plastic=None, yield_stress=np.in
                             _dp(R), _dp(f), C.byref(e), _dp(nxt), C.byref(y))
     return {"R": R.reshape(3, 3), "force": f.reshape(4, 3), "energy": e.value,
             "plastic": nxt, "yielding": y.value}
+
+
+def j2_point(strain, lam, mu, yield_stress, hardening, plastic=None):
+    """(stress, [theta, thetaBar, normal(9)], returned) of the J2 return map."""
+    e = np.ascontiguousarray(strain, dtype=np.float64).reshape(9)
+    p = np.zeros(10) if plastic is None else np.ascontiguousarray(plastic, dtype=np.float64)
+    out, lin = np.empty(9), np.empty(11)
+    r = lib().orc_j2_point(_dp(e), _dp(p), lam, mu, yield_stress, hardening, _dp(out), _dp(lin))
+    return out.reshape(3, 3), lin, bool(r)
+
+
+def j2_tangent(lin, lam, mu, e):
+    """tangentOf: the consistent tangent's stress for the strain e."""
+    lin = np.ascontiguousarray(lin, dtype=np.float64)
+    e = np.ascontiguousarray(e, dtype=np.float64).reshape(9)
+    out = np.empty(9)
+    lib().orc_j2_tangent(_dp(lin), lam, mu, _dp(e), _dp(out))
+    return out.reshape(3, 3)
 
 
 def polar(F):

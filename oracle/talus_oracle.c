@@ -543,11 +543,21 @@ void orc_snh_tet(const double *u, const double *dminv, double volume,
 /* Corotated tet with optional J2 return map                            */
 /* ------------------------------------------------------------------ */
 
+/* Linearisation of one return map (PointResponse, material.cc:581-592):
+ * theta, thetaBar and the flow normal of a loading point; the default
+ * (theta 1, thetaBar 0) is the elastic law. */
+typedef struct {
+  double theta, theta_bar, normal[9], excess;
+  int plastic;
+} j2_lin_t;
+
 /* returnMap (material.cc:594-630) on the small strain in the corotated frame,
- * plastic = {eps_p (9, row-major), alpha}.  Returns the stress. */
-static int j2_return(const double *strain, const double *plastic,
-                     double lambda, double mu, double yield, double hardening,
-                     double *stress, double *next) {
+ * plastic = {eps_p (9, row-major), alpha}.  Returns 1 when the point returns
+ * to the yield surface; return_mapping = 0 is the elastic trial
+ * (returnMapping false).  lin, if given, receives the linearisation. */
+static int j2_return_map(const double *strain, const double *plastic, double lambda, double mu,
+                         double yield, double hardening, int return_mapping, double *stress,
+                         double *next, j2_lin_t *lin) {
   const double bulk = lambda + 2.0 * mu / 3.0;
   double el[9];
   for (int i = 0; i < 9; ++i) el[i] = strain[i] - (plastic ? plastic[i] : 0.0);
@@ -562,7 +572,11 @@ static int j2_return(const double *strain, const double *plastic,
   const double radius = yield + hardening * alpha;
   if (next && plastic) memcpy(next, plastic, 10 * sizeof(double));
   if (next && !plastic) memset(next, 0, 10 * sizeof(double));
-  if (!(equivalent > radius) || !(norm > 0.0)) {
+  if (lin) {
+    memset(lin, 0, sizeof *lin);
+    lin->theta = 1.0;
+  }
+  if (!return_mapping || !(equivalent > radius) || !(norm > 0.0)) {
     for (int i = 0; i < 9; ++i) stress[i] = trial[i];
     for (int d = 0; d < 3; ++d) stress[d * 4] += pressure;
     return 0;
@@ -575,19 +589,40 @@ static int j2_return(const double *strain, const double *plastic,
     for (int i = 0; i < 9; ++i) next[i] += sqrt(1.5) * gamma * trial[i] / norm;
     next[9] += gamma;
   }
+  if (lin) {
+    lin->theta = theta;
+    lin->theta_bar = 3.0 * mu / (3.0 * mu + hardening) - (1.0 - theta);
+    for (int i = 0; i < 9; ++i) lin->normal[i] = trial[i] / norm;
+    lin->excess = (equivalent - radius) / radius;
+    lin->plastic = 1;
+  }
   return 1;
+}
+
+/* tangentOf (material.cc:636-647): the consistent tangent's stress for a
+ * strain e, K tr(e) I + 2 mu theta dev(e) - 2 mu thetaBar (n:e) n. */
+static void j2_tangent_stress(const j2_lin_t *lin, double lambda, double mu, const double *e,
+                              double *out) {
+  const double bulk = lambda + 2.0 * mu / 3.0;
+  const double tr = trace3(e);
+  const double along = contract3(lin->normal, e);
+  for (int r = 0; r < 3; ++r)
+    for (int c = 0; c < 3; ++c)
+      out[r * 3 + c] = 2.0 * mu * lin->theta * (e[r * 3 + c] - (r == c ? tr / 3.0 : 0.0)) +
+                       (r == c ? bulk * tr : 0.0) -
+                       2.0 * mu * lin->theta_bar * along * lin->normal[r * 3 + c];
 }
 
 /* tetStrain + trial / tetPointTrial (material.cc:136-164, 236-265,
  * 978-1031): R = elementRotation(F); centred offsets
  * o_i = R^T (x_i - xbar) - (X_i - Xbar); eps = sym(sum o_i g_i^T);
- * sigma from the return map (elastic when the lane does not yield);
- * f_i = -R V sigma g_i; energy V/2 sigma:(eps - eps_p_next). */
-void orc_corotated_tet(const double *x, const double *X, const double *dminv,
-                       double volume, double lambda, double mu,
-                       const double *plastic, double yield, double hardening,
-                       double *R, double *force, double *energy,
-                       double *plastic_next, uint32_t *yielding) {
+ * sigma from the return map (elastic when the lane does not yield or
+ * return_mapping is 0); f_i = -R V sigma g_i; energy V/2 sigma:(eps -
+ * eps_p_next).  lin receives the return map's linearisation. */
+static void corotated_eval(const double *x, const double *X, const double *dminv, double volume,
+                           double lambda, double mu, const double *plastic, double yield,
+                           double hardening, int return_mapping, double *R, double *force,
+                           double *energy, double *plastic_next, uint32_t *yielding, j2_lin_t *lin) {
   double g[4][3], F[9], u[12];
   shape_gradients(dminv, g);
   for (int i = 0; i < 12; ++i) u[i] = x[i] - X[i];
@@ -615,12 +650,11 @@ void orc_corotated_tet(const double *x, const double *X, const double *dminv,
   for (int r = 0; r < 3; ++r)
     for (int c = 0; c < 3; ++c) eps[r * 3 + c] = 0.5 * (D[r * 3 + c] + D[c * 3 + r]);
   double sigma[9], next[10];
+  const int yields_now = isfinite(yield) && yield > 0.0;
   const int plastic_now =
-      isfinite(yield) && yield > 0.0
-          ? j2_return(eps, plastic, lambda, mu, yield, hardening, sigma, next)
-          : j2_return(eps, NULL, lambda, mu, INFINITY, 0.0, sigma, next);
-  if (!(isfinite(yield) && yield > 0.0) && plastic)
-    memcpy(next, plastic, 10 * sizeof(double));
+      yields_now ? j2_return_map(eps, plastic, lambda, mu, yield, hardening, return_mapping, sigma, next, lin)
+                 : j2_return_map(eps, NULL, lambda, mu, INFINITY, 0.0, 1, sigma, next, lin);
+  if (!yields_now && plastic) memcpy(next, plastic, 10 * sizeof(double));
   for (int i = 0; i < 4; ++i) {
     double t[3], f[3];
     mv3(sigma, g[i], t);
@@ -632,6 +666,15 @@ void orc_corotated_tet(const double *x, const double *X, const double *dminv,
   if (energy) *energy = volume * 0.5 * contract3(sigma, el);
   if (plastic_next) memcpy(plastic_next, next, 10 * sizeof(double));
   if (yielding) *yielding = (uint32_t)plastic_now;
+}
+
+void orc_corotated_tet(const double *x, const double *X, const double *dminv,
+                       double volume, double lambda, double mu,
+                       const double *plastic, double yield, double hardening,
+                       double *R, double *force, double *energy,
+                       double *plastic_next, uint32_t *yielding) {
+  corotated_eval(x, X, dminv, volume, lambda, mu, plastic, yield, hardening, 1, R, force, energy,
+                 plastic_next, yielding, NULL);
 }
 
 /* ------------------------------------------------------------------ */
@@ -814,6 +857,15 @@ struct orc_body {
   double *W;       /* 9 per tet, neo-Hookean volume frame cof(Fe) Fp^-T */
   double *plastic; /* 10 per tet */
   uint8_t *yflag;
+  /* plastic Newton of yielding corotated tets: the assembly's return-map
+   * switch, loading margin and stiffness binding (bindStiffness: the
+   * consistent tangent when set), the per-tet linearisation (theta,
+   * thetaBar, normal[9]) and loading flags */
+  int newton, tangent, return_mapping;
+  double margin;
+  double *lin;
+  uint8_t *loading;
+  double *total, *lowest_it;
   uint32_t *inc_start, *inc_slot;
   /* scratch */
   double *slot_f, *slot_d, *tet_energy;
@@ -880,6 +932,15 @@ orc_body *orc_body_create(const orc_mesh *mesh, const orc_material *materials,
   }
   b->plastic = (double *)calloc(10 * T, sizeof(double));
   b->yflag = (uint8_t *)calloc(T + 1, 1);
+  b->lin = (double *)calloc(11 * T + 1, sizeof(double));
+  b->loading = (uint8_t *)calloc(T + 1, 1);
+  b->total = (double *)calloc(3 * N + 1, sizeof(double));
+  b->lowest_it = (double *)calloc(3 * N + 1, sizeof(double));
+  b->return_mapping = 1;
+  for (size_t e = 0; e < T; ++e) {
+    b->lin[11 * e] = 1.0;
+    if (b->plastic_lane[e] && !b->snh[e]) b->newton = 1;
+  }
   /* incidence: node -> slots (tet*4+corner) in ascending slot order
    * (buildIncidence, solver.cc:1065-1101) */
   b->inc_start = (uint32_t *)calloc(N + 1, sizeof(uint32_t));
@@ -924,7 +985,8 @@ void orc_body_free(orc_body *b) {
                   b->tet_energy, b->force, b->diag, b->precond, b->right,
                   b->weighted, b->shifted, b->carried, b->drive, b->evaluated,
                   b->residual, b->direction, b->applied, b->conditioned,
-                  b->increment, b->unconstrained, b->partial, b->ckpt, b->final_state};
+                  b->increment, b->unconstrained, b->partial, b->ckpt, b->final_state,
+                  b->lin, b->loading, b->total, b->lowest_it};
   for (size_t i = 0; i < sizeof ptrs / sizeof ptrs[0]; ++i) free(ptrs[i]);
   free(b);
 }
@@ -1048,9 +1110,24 @@ static void element_one(orc_body *b, const double *at, size_t e, int diag) {
     volume_frame(F, ep, &b->W[9 * e]);
     orc_snh_tet(u, dminv, V, b->lambda[e], b->mu[e], ep, &b->slot_f[12 * e], &energy);
   } else {
-    orc_corotated_tet(x, X, dminv, V, b->lambda[e], b->mu[e],
-                      &b->plastic[10 * e], b->yield[e], b->hard[e], R,
-                      &b->slot_f[12 * e], &energy, NULL, NULL);
+    j2_lin_t lin;
+    corotated_eval(x, X, dminv, V, b->lambda[e], b->mu[e], &b->plastic[10 * e], b->yield[e],
+                   b->hard[e], b->return_mapping, R, &b->slot_f[12 * e], &energy, NULL, NULL, &lin);
+    if (b->plastic_lane[e]) {
+      /* tetPointTrial (material.cc:1010-1022): loading past the margin;
+       * the linearisation is the return map's when loading or elastic,
+       * else the elastic law */
+      const int loading = lin.plastic && lin.excess > b->margin;
+      b->loading[e] = (uint8_t)loading;
+      double *l = &b->lin[11 * e];
+      memset(l, 0, 11 * sizeof(double));
+      l[0] = 1.0;
+      if (loading) {
+        l[0] = lin.theta;
+        l[1] = lin.theta_bar;
+        memcpy(l + 2, lin.normal, sizeof lin.normal);
+      }
+    }
   }
   memcpy(&b->R[9 * e], R, sizeof R);
   b->tet_energy[e] = energy;
@@ -1059,6 +1136,25 @@ static void element_one(orc_body *b, const double *at, size_t e, int diag) {
     shape_gradients(dminv, g);
     const double *W = &b->W[9 * e];
     const double mu = b->mu[e], lam = b->lambda[e];
+    if (!b->snh[e] && b->plastic_lane[e] && b->tangent) {
+      /* stiffnessDiagonal of the consistent tangent blocks (material.cc:
+       * 1033-1054, 290-304): R V((K + mu theta/3) g g^T + mu theta |g|^2 I
+       * - 2 mu thetaBar (n g)(n g)^T) R^T, K = lambda + 2 mu / 3 */
+      const double *l = &b->lin[11 * e];
+      const double bulk = lam + 2.0 * mu / 3.0;
+      for (int i = 0; i < 4; ++i) {
+        double ng[3], B[9], RB[9];
+        mv3(l + 2, g[i], ng);
+        const double gg = g[i][0] * g[i][0] + g[i][1] * g[i][1] + g[i][2] * g[i][2];
+        for (int r = 0; r < 3; ++r)
+          for (int c = 0; c < 3; ++c)
+            B[r * 3 + c] = V * ((bulk + mu * l[0] / 3.0) * g[i][r] * g[i][c] +
+                                (r == c ? mu * l[0] * gg : 0.0) - 2.0 * mu * l[1] * ng[r] * ng[c]);
+        mul3(R, B, RB);
+        mul3_nt(RB, R, &b->slot_d[36 * e + 9 * i]);
+      }
+      return;
+    }
     double gs[4][3];
     plastic_gradients(b, e, g, gs);
     for (int i = 0; i < 4; ++i) {
@@ -1167,10 +1263,24 @@ static void op_element_range(void *p, size_t begin, size_t end) {
     const double tr = snh ? 0.0 : trace3(D);
     const double vol = snh ? b->lambda[e] * contract3(&b->W[9 * e], Dw) : 0.0;
     double s[9];
-    for (int r = 0; r < 3; ++r)
-      for (int k = 0; k < 3; ++k)
-        s[r * 3 + k] = b->mu[e] * (D[r * 3 + k] + D[k * 3 + r]) +
-                       (r == k ? b->lambda[e] * tr : 0.0);
+    if (!snh && b->plastic_lane[e] && b->tangent) {
+      /* the consistent tangent (tangentOf, material.cc:636-647) on the
+       * corotated strain sym(D') */
+      double eps[9];
+      for (int r = 0; r < 3; ++r)
+        for (int k = 0; k < 3; ++k) eps[r * 3 + k] = 0.5 * (D[r * 3 + k] + D[k * 3 + r]);
+      j2_lin_t lin;
+      const double *l = &b->lin[11 * e];
+      lin.theta = l[0];
+      lin.theta_bar = l[1];
+      memcpy(lin.normal, l + 2, sizeof lin.normal);
+      j2_tangent_stress(&lin, b->lambda[e], b->mu[e], eps, s);
+    } else {
+      for (int r = 0; r < 3; ++r)
+        for (int k = 0; k < 3; ++k)
+          s[r * 3 + k] = b->mu[e] * (D[r * 3 + k] + D[k * 3 + r]) +
+                         (r == k ? b->lambda[e] * tr : 0.0);
+    }
     const double V = b->m.volume[e];
     for (int i = 0; i < 4; ++i) {
       double t[3], y[3], w[3] = {0, 0, 0};
@@ -1437,6 +1547,262 @@ static double substep(orc_body *b, const orc_settings *s, const rule_t *rule, do
   return elastic;
 }
 
+/* LOADING_MARGIN, PLASTIC_REDUCTION, PLASTIC_RESOLUTION (solver.cc:372-380) */
+#define LOADING_MARGIN 1.0e-3
+#define PLASTIC_REDUCTION 1.0e-2
+#define PLASTIC_RESOLUTION 8.0
+#define FLOAT_EPSILON 1.1920928955078125e-7
+
+/* The J2 point law alone: stress and linearisation {theta, thetaBar,
+ * normal[9]} of the return map at `strain` from the committed {eps_p,
+ * alpha}; returns 1 when the point returns to the yield surface. */
+int orc_j2_point(const double *strain, const double *plastic, double lambda, double mu,
+                 double yield, double hardening, double *stress, double *lin11) {
+  j2_lin_t lin;
+  const int r = j2_return_map(strain, plastic, lambda, mu, yield, hardening, 1, stress, NULL, &lin);
+  lin11[0] = lin.theta;
+  lin11[1] = lin.theta_bar;
+  memcpy(lin11 + 2, lin.normal, sizeof lin.normal);
+  return r;
+}
+
+void orc_j2_tangent(const double *lin11, double lambda, double mu, const double *e, double *out) {
+  j2_lin_t lin;
+  lin.theta = lin11[0];
+  lin.theta_bar = lin11[1];
+  memcpy(lin.normal, lin11 + 2, sizeof lin.normal);
+  j2_tangent_stress(&lin, lambda, mu, e, out);
+}
+
+static double assemble_plastic(orc_body *b, const double *at, double step, uint32_t threads,
+                               double margin, int return_mapping, uint32_t *loading) {
+  b->margin = margin;
+  b->return_mapping = return_mapping;
+  memset(b->loading, 0, b->m.tets);
+  const double energy = assemble(b, at, step, threads);
+  b->margin = 0.0;
+  b->return_mapping = 1;
+  uint32_t n = 0;
+  for (size_t e = 0; e < b->m.tets; ++e) n += b->loading[e];
+  *loading = n;
+  return energy;
+}
+
+/* K_tangent d at positions x: assembled with the loading margin and the
+ * consistent tangent bound (a Newton attempt's operator), y = K d (no
+ * inertia, h = 1); returns the number of loading tets. */
+uint32_t orc_tangent_apply(orc_body *b, const double *x, const double *d, double *y) {
+  b->tangent = 1;
+  uint32_t loading = 0;
+  assemble_plastic(b, x, 1.0, 1, LOADING_MARGIN, 1, &loading);
+  orc_apply_system(b, 0.0, 1.0, d, y, 1);
+  b->tangent = 0;
+  return loading;
+}
+
+/* assembleElastic with the loading margin and return-map switch; returns
+ * the elastic energy and the number of loading tets (Assembly). */
+static double right_squared(const orc_body *b) {
+  double s = 0.0;
+  for (size_t i = 0; i < 3 * (size_t)b->m.nodes; ++i) s += b->right[i] * b->right[i];
+  return s;
+}
+
+/* One substep of a body with yielding corotated lanes: the damped plastic
+ * Newton of deformable::step (solver.cc:3951-4250).  The unknown is the
+ * acceleration `total`, evaluated at x_w + h_eff^2 total.  Attempt 0 solves
+ * with the elastic stiffness from the previous acceleration; later attempts
+ * solve K_tangent increment = residual with the consistent tangent and
+ * halve the last increment while the residual norm grows (backtracking).
+ * Converged when two successive evaluations load nothing or the position
+ * change is below float resolution.  A predictor that loads nothing and a
+ * residual that grows after the first solve take the elastic
+ * three-solve scheme instead ("linear reads"), handing back to Newton if a
+ * re-evaluation loads; a Newton that does not converge within `cap`
+ * attempts falls back to the elastic scheme when that loads nothing and
+ * reduces the residual 100-fold, else keeps the lowest-residual iterate. */
+static double newton_substep(orc_body *b, const orc_settings *s, const rule_t *rule,
+                             double inner, uint32_t cap, orc_report *rep) {
+  const size_t N = b->m.nodes;
+  const uint32_t threads = s->threads;
+  const double inertia = 1.0 - rule->am;
+  const double step = sqrt((1.0 - rule->af) * rule->beta) * inner;
+  const double carry = (0.5 - rule->beta) * inner * inner;
+  const double reach = step * step;
+  const double travel = rule->beta * inner * inner;
+  for (size_t n = 0; n < N; ++n)
+    for (int r = 0; r < 3; ++r) {
+      const size_t i = 3 * n + r;
+      b->carried[i] = b->a[i];
+      b->shifted[i] = b->x[i] + inner * b->v[i] + carry * b->a[i];
+      b->weighted[i] = (1.0 - rule->af) * b->shifted[i] + rule->af * b->x[i];
+      b->drive[i] = b->m.mass[n] * (double)s->gravity[r];
+    }
+  double *total = b->total, *inc = b->increment;
+  double elastic = 0.0;
+  int capped = 0;
+  double rel = 0.0;
+#define EVALUATE()                                                           for (size_t i = 0; i < 3 * N; ++i) b->evaluated[i] = b->weighted[i] + reach * total[i]
+#define SOLVE()                                                                            do {                                                                                       rep->cg_iterations += pcg(b, inertia, step, s->cg_iterations, s->tolerance, inc,                                    threads, &capped, &rel);                                       rep->cg_capped += (uint32_t)capped;                                                      if (rel > rep->cg_worst_relative) rep->cg_worst_relative = rel;                          ++rep->solves;                                                                         } while (0)
+#define ADD_INCREMENT()                                                      for (size_t n = 0; n < N; ++n)                                               if (!b->m.anchored[n])                                                       for (int r = 0; r < 3; ++r) total[3 * n + r] += inc[3 * n + r]
+  int linear_allowed = 1;
+  for (;;) {
+    memset(total, 0, 3 * N * sizeof(double));
+    double accepted = INFINITY, opening = 0.0, lowest = INFINITY;
+    uint32_t loading_before = 0, solves = 0, loading = 0;
+    int converged = 0, predictor_unloaded = 0, handed_off = 0;
+    for (uint32_t attempt = 0; attempt < cap; ++attempt) {
+      if (attempt < 2) b->tangent = attempt != 0;
+      EVALUATE();
+      elastic = assemble_plastic(b, b->evaluated, step, threads, LOADING_MARGIN, 1, &loading);
+      ++rep->plastic_evaluations;
+      balance(b, rule, inertia, total);
+      const double squared = right_squared(b);
+      if (attempt != 0) {
+        const double ranked = isfinite(squared) ? squared : INFINITY;
+        if (attempt == 1 || ranked < lowest) {
+          lowest = ranked;
+          memcpy(b->lowest_it, total, 3 * N * sizeof(double));
+        }
+      }
+      if (attempt == 0) predictor_unloaded = loading == 0;
+      if (attempt == 1 && linear_allowed && predictor_unloaded && isfinite(squared) &&
+          squared > accepted) {
+        ++rep->plastic_linear_reads;
+        b->tangent = 0;
+        for (uint32_t read = 1; read < 3; ++read) {
+          EVALUATE();
+          uint32_t reread = 0;
+          elastic = assemble_plastic(b, b->evaluated, step, threads, LOADING_MARGIN, 1, &reread);
+          ++rep->plastic_evaluations;
+          if (reread != 0) {
+            handed_off = 1;
+            break;
+          }
+          balance(b, rule, inertia, total);
+          memset(inc, 0, 3 * N * sizeof(double));
+          SOLVE();
+          ++solves;
+          ADD_INCREMENT();
+        }
+        converged = !handed_off;
+        break;
+      }
+      if (attempt != 0 && !(squared <= accepted)) {
+        for (size_t i = 0; i < 3 * N; ++i) {
+          inc[i] *= 0.5;
+          total[i] -= inc[i];
+        }
+        ++rep->plastic_backtracks;
+        continue;
+      }
+      if (attempt != 0 && loading_before == 0 && loading == 0) {
+        converged = 1;
+        break;
+      }
+      if (attempt == 0) opening = squared;
+      accepted = squared;
+      loading_before = loading;
+      if (attempt == 0)
+        memcpy(inc, b->a, 3 * N * sizeof(double));
+      else
+        memset(inc, 0, 3 * N * sizeof(double));
+      SOLVE();
+      ++solves;
+      double moved = 0.0, extent = 0.0;
+      for (size_t n = 0; n < N; ++n) {
+        double *d = &inc[3 * n];
+        if (b->m.anchored[n]) {
+          d[0] = d[1] = d[2] = 0.0;
+          continue;
+        }
+        for (int r = 0; r < 3; ++r) total[3 * n + r] += d[r];
+        const double *p = &b->evaluated[3 * n];
+        moved = fmax(moved, sqrt(d[0] * d[0] + d[1] * d[1] + d[2] * d[2]));
+        extent = fmax(extent, sqrt(p[0] * p[0] + p[1] * p[1] + p[2] * p[2]));
+      }
+      if (solves > 1 && travel * moved <= PLASTIC_RESOLUTION * FLOAT_EPSILON * extent) {
+        converged = 1;
+        break;
+      }
+    }
+    rep->plastic_solves += solves;
+    if (handed_off) {
+      ++rep->plastic_handoffs;
+      linear_allowed = 0;
+      continue;
+    }
+    if (!converged) {
+      b->tangent = 0;
+      memset(total, 0, 3 * N * sizeof(double));
+      int fell_back = 1;
+      for (uint32_t read = 0; read < 3; ++read) {
+        EVALUATE();
+        uint32_t reread = 0;
+        elastic = assemble_plastic(b, b->evaluated, step, threads, LOADING_MARGIN, read != 0, &reread);
+        ++rep->plastic_evaluations;
+        if (reread != 0) {
+          fell_back = 0;
+          break;
+        }
+        balance(b, rule, inertia, total);
+        if (read == 0)
+          memcpy(inc, b->a, 3 * N * sizeof(double));
+        else
+          memset(inc, 0, 3 * N * sizeof(double));
+        SOLVE();
+        ++rep->plastic_solves;
+        ADD_INCREMENT();
+      }
+      if (fell_back) {
+        EVALUATE();
+        uint32_t reread = 0;
+        elastic = assemble_plastic(b, b->evaluated, step, threads, LOADING_MARGIN, 1, &reread);
+        ++rep->plastic_evaluations;
+        balance(b, rule, inertia, total);
+        const double squared = right_squared(b);
+        fell_back = reread == 0 && squared <= PLASTIC_REDUCTION * PLASTIC_REDUCTION * opening;
+      }
+      if (fell_back) {
+        ++rep->plastic_fallbacks;
+        converged = 1;
+      } else {
+        ++rep->plastic_rejections;
+        b->tangent = 1;
+        memcpy(total, b->lowest_it, 3 * N * sizeof(double));
+        EVALUATE();
+        uint32_t reread = 0;
+        elastic = assemble_plastic(b, b->evaluated, step, threads, LOADING_MARGIN, 1, &reread);
+        ++rep->plastic_evaluations;
+        balance(b, rule, inertia, total);
+      }
+    }
+    rep->plastic_capped += converged ? 0u : 1u;
+    memcpy(b->a, total, 3 * N * sizeof(double));
+    break;
+  }
+#undef EVALUATE
+#undef SOLVE
+#undef ADD_INCREMENT
+  b->tangent = 0;
+  /* velocity (4351-4368) and position (5829-5849) */
+  for (size_t n = 0; n < N; ++n) {
+    if (b->m.anchored[n]) {
+      for (int r = 0; r < 3; ++r) b->v[3 * n + r] = b->a[3 * n + r] = 0.0;
+      continue;
+    }
+    for (int r = 0; r < 3; ++r) {
+      const size_t i = 3 * n + r;
+      b->v[i] += inner * ((1.0 - rule->gamma) * b->carried[i] + rule->gamma * b->a[i]);
+      b->x[i] = b->shifted[i] + rule->beta * inner * inner * b->a[i];
+    }
+  }
+  const uint32_t grew = commit_plastic(b);
+  rep->yield_events += grew;
+  rep->plastic_growth |= grew ? 1u : 0u;
+  return elastic;
+}
+
 /* One step of deformable::step for a non-floating tet body
  * (solver.cc:3472-5901 without contacts, damping, floating frame; the
  * elastic path 3895-3950 is used for every lane, see the header note). */
@@ -1453,7 +1819,12 @@ int orc_step(orc_body *b, const orc_settings *s, orc_report *rep) {
   const double inner = (double)s->dt / substeps;
   rep->substeps = substeps;
   double elastic = 0.0;
-  for (uint32_t pass = 0; pass < substeps; ++pass) elastic = substep(b, s, &rule, inner, rep);
+  /* newton = plasticPasses > 1 && anyYields (solver.cc:3517): here, bodies
+   * with a yielding corotated lane (the return map runs in the trial) */
+  const uint32_t passes = s->plastic_passes ? s->plastic_passes : 40u;
+  const int newton = b->newton && passes > 1 && !s->converge;
+  for (uint32_t pass = 0; pass < substeps; ++pass)
+    elastic = newton ? newton_substep(b, s, &rule, inner, passes, rep) : substep(b, s, &rule, inner, rep);
   double kinetic = 0.0;
   for (size_t n = 0; n < N; ++n)
     kinetic += 0.5 * b->m.mass[n] *

@@ -69,6 +69,7 @@ typedef struct orc_settings {
   float spectral_radius;
   uint32_t threads;         /* 0/1 = serial */
   uint32_t converge;        /* corrections until the substep residual's root */
+  uint32_t plastic_passes;  /* Settings::plasticPasses (solver.h:56); 0 = 40 */
 } orc_settings;
 
 typedef struct orc_report {
@@ -82,6 +83,11 @@ typedef struct orc_report {
   uint32_t plastic_growth;  /* any yield this step */
   uint32_t yield_events;    /* tets whose plastic state changed */
   uint32_t non_finite;
+  /* the plastic Newton of bodies with yielding corotated lanes
+   * (StepReport::plastic*, solver.cc:3951-4250) */
+  uint32_t plastic_evaluations, plastic_solves, plastic_backtracks,
+      plastic_linear_reads, plastic_handoffs, plastic_fallbacks,
+      plastic_rejections, plastic_capped;
 } orc_report;
 
 typedef struct orc_body orc_body;
@@ -143,6 +149,15 @@ void orc_apply_system(orc_body *body, double inertia, double step,
 void orc_tet_stretch(orc_body *body, double *stretch);
 void orc_stretch_tets(const uint32_t *tet, size_t tets, const double *rest, const double *x,
                       double *stretch);
+/* J2 point law (returnMap, material.cc:594-630): stress and linearisation
+ * lin11 = {theta, thetaBar, normal[9]}; tangentOf (636-647) applied to a
+ * strain e. */
+int orc_j2_point(const double *strain, const double *plastic, double lambda, double mu,
+                 double yield, double hardening, double *stress, double *lin11);
+void orc_j2_tangent(const double *lin11, double lambda, double mu, const double *e, double *out);
+/* The consistent-tangent operator K d of a Newton attempt at positions x
+ * (h = 1, no inertia); returns the loading tets. */
+uint32_t orc_tangent_apply(orc_body *body, const double *x, const double *d, double *y);
 /* Yield flags of the last step (1 per tet). */
 void orc_yield_flags(const orc_body *body, uint8_t *flags);
 
