@@ -117,6 +117,10 @@ typedef struct talus_mi_step_params {
   float spectral_radius;    /* default 0.5 */
   uint32_t preconditioner;  /* TALUS_MI_BLOCK_JACOBI / TALUS_MI_FACTOR */
   uint32_t flags;           /* TALUS_MI_DETECT_TEARS | TALUS_MI_RECORD_EVENTS */
+  uint32_t plastic_passes;  /* Settings::plasticPasses (solver.h:56), default
+                               40: Newton attempts per substep of a body with
+                               yielding corotated lanes; 1 = the elastic
+                               three-solve scheme */
 } talus_mi_step_params;
 
 /* deformable::StepReport subset (solver.h:211-311); filled per call. */
@@ -139,6 +143,18 @@ typedef struct talus_mi_report {
   uint32_t non_finite;
   uint32_t preconditioner_applications; /* block Cholesky solves this step */
   double device_seconds;    /* hipEvent time of the whole step */
+  /* the plastic Newton (StepReport::plastic*, solver.h:211-311), summed over
+   * bodies and substeps: assemblies, solves, halved increments, elastic
+   * reads taken instead, hand-backs to Newton, accepted elastic fallbacks,
+   * rejected fallbacks (lowest-residual iterate kept), unconverged substeps */
+  uint32_t plastic_evaluations;
+  uint32_t plastic_solves;
+  uint32_t plastic_backtracks;
+  uint32_t plastic_linear_reads;
+  uint32_t plastic_handoffs;
+  uint32_t plastic_fallbacks;
+  uint32_t plastic_rejections;
+  uint32_t plastic_capped;
 } talus_mi_report;
 
 #define TALUS_MI_EVENT_YIELD 0u

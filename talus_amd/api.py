@@ -101,6 +101,7 @@ class StepParams(C.Structure):
         ("cg_iterations", C.c_uint32), ("factor_budget", C.c_uint32),
         ("tolerance", C.c_float), ("spectral_radius", C.c_float),
         ("preconditioner", C.c_uint32), ("flags", C.c_uint32),
+        ("plastic_passes", C.c_uint32),
     ]
 
 
@@ -113,6 +114,10 @@ class Report(C.Structure):
         ("plastic_growth", C.c_uint32), ("yield_events", C.c_uint32),
         ("tear_events", C.c_uint32), ("split_nodes", C.c_uint32), ("non_finite", C.c_uint32),
         ("preconditioner_applications", C.c_uint32), ("device_seconds", C.c_double),
+        ("plastic_evaluations", C.c_uint32), ("plastic_solves", C.c_uint32),
+        ("plastic_backtracks", C.c_uint32), ("plastic_linear_reads", C.c_uint32),
+        ("plastic_handoffs", C.c_uint32), ("plastic_fallbacks", C.c_uint32),
+        ("plastic_rejections", C.c_uint32), ("plastic_capped", C.c_uint32),
     ]
 
     def as_dict(self):

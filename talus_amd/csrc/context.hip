@@ -19,6 +19,8 @@
 #include <functional>
 #include <cmath>
 #include <cstring>
+#include <limits>
+#include <map>
 #include <memory>
 #include <mutex>
 #include <stdexcept>
@@ -92,6 +94,193 @@ int prepare_body(Body &b) {
   }
   return TALUS_MI_OK;
 }
+
+// ------------------------------------------------------------ plastic Newton
+
+// One device operation of the plastic Newton: an evaluation (x_eval, the
+// assembly with the return maps and the loading count, balance, |right|^2),
+// a PCG solve, or an iterate update (NewtonVec).
+struct NewtonOp {
+  enum Kind : uint8_t { DONE = 0, EVAL = 1, SOLVE = 2, VEC = 3 };
+  uint8_t kind = DONE;
+  uint8_t first = 0;    // EVAL at the predictor (total 0) / SOLVE from the previous acceleration
+  uint8_t tangent = 0;  // the consistent tangent bound (bindStiffness(consistent))
+  uint8_t elastic = 0;  // EVAL without the return map
+  int vec = 0;          // VEC: NewtonVec mode
+  int key() const { return kind | first << 2 | tangent << 3 | elastic << 4 | vec << 5; }
+};
+
+// The plastic branch of deformable::step for one body and substep
+// (solver.cc:3951-4250) as a resumable routine: next() follows the
+// reference's control flow until it needs the device, returns that
+// operation, and continues from there when called again with the
+// operation's results filled in (squared, loading after an EVAL; moved,
+// extent after a SOLVE).  The unknown, total, is the engine's acceleration
+// after the first solve.
+struct NewtonBody {
+  static constexpr double MARGIN = 1.0e-3;     // LOADING_MARGIN (solver.cc:372)
+  static constexpr double REDUCTION = 1.0e-2;  // PLASTIC_REDUCTION (374)
+  static constexpr double RESOLUTION = 8.0;    // PLASTIC_RESOLUTION (380)
+  static constexpr double FLOAT_EPSILON = 1.1920928955078125e-7;
+  uint32_t cap = 40;    // plasticPasses
+  double travel = 0.0;  // beta h^2: acceleration to position
+  double squared = 0.0, moved = 0.0, extent = 0.0;
+  uint32_t loading = 0;
+  // StepReport counters
+  uint32_t evaluations = 0, solves_all = 0, backtracks = 0, linear_reads = 0, handoffs = 0,
+           fallbacks = 0, rejections = 0, capped = 0;
+
+  NewtonOp next();
+
+ private:
+  static NewtonOp eval(bool first, bool tangent, bool elastic) {
+    NewtonOp o;
+    o.kind = NewtonOp::EVAL;
+    o.first = first;
+    o.tangent = tangent;
+    o.elastic = elastic;
+    return o;
+  }
+  static NewtonOp solve(bool first, bool tangent) {
+    NewtonOp o;
+    o.kind = NewtonOp::SOLVE;
+    o.first = first;
+    o.tangent = tangent;
+    return o;
+  }
+  static NewtonOp vec(int mode) {
+    NewtonOp o;
+    o.kind = NewtonOp::VEC;
+    o.vec = mode;
+    return o;
+  }
+  int pc_ = 0;
+  bool linear_allowed_ = true, tangent_ = false, predictor_unloaded_ = false, converged_ = false,
+       handed_off_ = false, fell_back_ = false;
+  double accepted_ = 0.0, opening_ = 0.0, lowest_ = 0.0, ranked_ = 0.0;
+  uint32_t attempt_ = 0, solves_ = 0, loading_before_ = 0, read_ = 0;
+};
+
+// returns `op` and resumes after this point on the next call
+#define TMI_AWAIT(op)    \
+  do {                   \
+    pc_ = __LINE__;      \
+    return (op);         \
+    case __LINE__:;      \
+  } while (0)
+
+NewtonOp NewtonBody::next() {
+  constexpr double INF = std::numeric_limits<double>::infinity();
+  switch (pc_) {
+    case 0:
+      for (;;) {
+        accepted_ = INF;
+        opening_ = 0.0;
+        lowest_ = INF;
+        loading_before_ = 0;
+        solves_ = 0;
+        predictor_unloaded_ = converged_ = handed_off_ = false;
+        for (attempt_ = 0; attempt_ < cap; ++attempt_) {
+          if (attempt_ < 2) tangent_ = attempt_ != 0;
+          TMI_AWAIT(eval(attempt_ == 0, tangent_, false));
+          ++evaluations;
+          if (attempt_ != 0) {
+            ranked_ = std::isfinite(squared) ? squared : INF;
+            if (attempt_ == 1 || ranked_ < lowest_) {
+              lowest_ = ranked_;
+              TMI_AWAIT(vec(NV_SAVE));
+            }
+          }
+          if (attempt_ == 0) predictor_unloaded_ = loading == 0;
+          if (attempt_ == 1 && linear_allowed_ && predictor_unloaded_ && std::isfinite(squared) &&
+              squared > accepted_) {
+            // the elastic reads from the first solve's total
+            ++linear_reads;
+            tangent_ = false;
+            for (read_ = 1; read_ < 3; ++read_) {
+              TMI_AWAIT(eval(false, false, false));
+              ++evaluations;
+              if (loading != 0) {
+                handed_off_ = true;
+                break;
+              }
+              TMI_AWAIT(solve(false, false));
+              ++solves_;
+            }
+            converged_ = !handed_off_;
+            break;
+          }
+          if (attempt_ != 0 && !(squared <= accepted_)) {
+            ++backtracks;
+            TMI_AWAIT(vec(NV_BACKTRACK));
+            continue;
+          }
+          if (attempt_ != 0 && loading_before_ == 0 && loading == 0) {
+            converged_ = true;
+            break;
+          }
+          if (attempt_ == 0) opening_ = squared;
+          accepted_ = squared;
+          loading_before_ = loading;
+          TMI_AWAIT(solve(attempt_ == 0, tangent_));
+          ++solves_;
+          if (attempt_ == 0) TMI_AWAIT(vec(NV_DX_FROM_A));
+          if (solves_ > 1 && travel * moved <= RESOLUTION * FLOAT_EPSILON * extent) {
+            converged_ = true;
+            break;
+          }
+        }
+        solves_all += solves_;
+        if (handed_off_) {
+          ++handoffs;
+          linear_allowed_ = false;
+          TMI_AWAIT(vec(NV_RESET));
+          continue;
+        }
+        if (!converged_) {
+          // the elastic scheme from total 0, accepted when it loads nothing
+          // and reduces the residual 100-fold
+          tangent_ = false;
+          TMI_AWAIT(vec(NV_RESET));
+          fell_back_ = true;
+          for (read_ = 0; read_ < 3; ++read_) {
+            TMI_AWAIT(eval(read_ == 0, false, read_ == 0));
+            ++evaluations;
+            if (loading != 0) {
+              fell_back_ = false;
+              break;
+            }
+            TMI_AWAIT(solve(read_ == 0, false));
+            ++solves_all;
+          }
+          if (fell_back_) {
+            TMI_AWAIT(eval(false, false, false));
+            ++evaluations;
+            fell_back_ = loading == 0 && squared <= REDUCTION * REDUCTION * opening_;
+          }
+          if (fell_back_) {
+            ++fallbacks;
+            converged_ = true;
+          } else {
+            // keep the iterate with the lowest residual
+            ++rejections;
+            tangent_ = true;
+            TMI_AWAIT(vec(NV_RESTORE));
+            TMI_AWAIT(eval(false, true, false));
+            ++evaluations;
+          }
+        }
+        capped += converged_ ? 0u : 1u;
+        break;
+      }
+      pc_ = -1;
+      return NewtonOp{};
+    default:
+      return NewtonOp{};
+  }
+}
+
+#undef TMI_AWAIT
 
 // ------------------------------------------------------------------ engine
 
@@ -277,6 +466,20 @@ class Engine final : public EngineBase {
   // every (tet, corner) slot code 4e+c
   std::vector<uint32_t> inc_start_, slot_pos_;
   double *hnorms_ = nullptr, *d_norms_ = nullptr;  // corrections' convergence
+  // plastic Newton: bodies with a yielding corotated tet, pinned staging of
+  // its per-tile norms, loading counts and per-operation parameters
+  std::vector<uint8_t> newton_body_;
+  double *hnnorm_ = nullptr;
+  uint32_t *hloading_ = nullptr;
+  static constexpr size_t NEWTON_SLOTS = 32;
+  StepParams *hnparams_ = nullptr;
+  size_t nslot_ = 0;
+  void upload_newton(const std::vector<StepParams> &params);
+  void ensure_solve_slot(uint32_t solve_index);
+  void newton_pass(const std::vector<uint32_t> &bodies, const std::vector<StepParams> &params,
+                   uint32_t cap, uint32_t limit, bool diag, uint32_t pass,
+                   std::vector<uint32_t> &solve_pass, std::vector<std::vector<uint8_t>> &solve_active,
+                   uint32_t &solve_index, talus_mi_report &rep);
   struct Recorder {
     std::vector<double *> buffers;  // per pass: x_n, v_n, a_n, a' (3N each)
     std::vector<std::vector<StepParams>> params;
@@ -404,6 +607,7 @@ void Engine<Real>::build(const std::vector<Body> &bodies) {
   std::vector<uint8_t> flags(T);
   any_plastic_ = any_tear_ = false;
   n_snh_ = n_plastic_ = 0;
+  newton_body_.assign(B, 0);
   for (uint32_t b = 0; b < B; ++b) {
     const Body &bd = bodies[b];
     const size_t n0 = n0_[b], t0 = t0_[b];
@@ -438,6 +642,7 @@ void Engine<Real>::build(const std::vector<Body> &bodies) {
       tear[g] = bd.tear[e];
       flags[g] = bd.flags[e];
       any_plastic_ = any_plastic_ || (bd.flags[e] & FLAG_PLASTIC);
+      if ((bd.flags[e] & FLAG_PLASTIC) && !(bd.flags[e] & FLAG_SNH)) newton_body_[b] = 1;
       n_snh_ += (bd.flags[e] & FLAG_SNH) ? 1u : 0u;
       n_plastic_ += (bd.flags[e] & FLAG_PLASTIC) ? 1u : 0u;
       any_tear_ = any_tear_ || (bd.flags[e] & FLAG_TEAR);
@@ -494,6 +699,14 @@ void Engine<Real>::build(const std::vector<Body> &bodies) {
   d_.R = dev_copy(R);
   d_.W = dev_copy(W);
   d_.plastic = dev_copy(plastic);
+  {
+    std::vector<Real> lin(8 * size_t(T), Real(0));
+    std::fill(lin.begin(), lin.begin() + T, Real(1));
+    d_.lin = dev_copy(lin);
+  }
+  d_.loading = dev<uint32_t>(B);
+  d_.lowest = dev<double>(3 * N);
+  d_.nnorm = dev<double>(3 * size_t(d_.n_ntiles));
   d_.slot_f = dev<Real>(12 * T);
   d_.slot_d = dev<Real>(24 * T);
   d_.energy = dev<Real>(T);
@@ -512,7 +725,7 @@ void Engine<Real>::build(const std::vector<Body> &bodies) {
   HIPCHECK(hipMemset(d_.yflag, 0, std::max<size_t>(T, 1)));
   HIPCHECK(hipMemset(d_.tflag, 0, std::max<size_t>(T, 1)));
   HIPCHECK(hipMemset(d_.epartial, 0, std::max<size_t>(d_.n_ttiles, 1) * sizeof(double)));
-  scratch64_ = dev<double>(3 * N);
+  scratch64_ = dev<double>(6 * N);  // two node vectors (eval_forces, apply_system)
   hparams_slots_ = 8;
   hparams_ = pinned<StepParams>(hparams_slots_ * B);
   hflags_ = pinned<int>(B);
@@ -524,6 +737,10 @@ void Engine<Real>::build(const std::vector<Body> &bodies) {
   hepartial_ = pinned<double>(d_.n_ttiles);
   hyields_ = pinned<uint32_t>(B);
   hvec_ = pinned<Real>(3 * N + 12 * T + 9 * T);
+  hnnorm_ = pinned<double>(3 * size_t(d_.n_ntiles));
+  hloading_ = pinned<uint32_t>(B);
+  hnparams_ = pinned<StepParams>(NEWTON_SLOTS * B);
+  nslot_ = 0;
   {
     std::vector<uint32_t> nb(N);
     for (uint32_t b = 0; b < B; ++b)
@@ -822,6 +1039,141 @@ void Engine<Real>::upload_params(const std::vector<StepParams> &params, size_t s
                           hipMemcpyHostToDevice, stream_));
 }
 
+// parameters of one Newton operation from their own pinned ring; the ring
+// restarts after every synchronization (nslot_ = 0), and waits for the
+// stream when a round would reuse a slot still being copied
+template <class Real>
+void Engine<Real>::upload_newton(const std::vector<StepParams> &params) {
+  if (nslot_ == NEWTON_SLOTS) {
+    HIPCHECK(hipStreamSynchronize(stream_));
+    nslot_ = 0;
+  }
+  StepParams *h = hnparams_ + nslot_ * d_.B;
+  std::memcpy(h, params.data(), d_.B * sizeof(StepParams));
+  HIPCHECK(hipMemcpyAsync(const_cast<StepParams *>(d_.params), h, d_.B * sizeof(StepParams),
+                          hipMemcpyHostToDevice, stream_));
+  ++nslot_;
+}
+
+// the per-solve result ring holds every solve of a step; grown (after a
+// wait) when a Newton step needs more
+template <class Real>
+void Engine<Real>::ensure_solve_slot(uint32_t solve_index) {
+  if (solve_index < hsolve_slots_) return;
+  HIPCHECK(hipStreamSynchronize(stream_));
+  nslot_ = 0;
+  const size_t B = d_.B, grown = std::max<size_t>(2 * hsolve_slots_, solve_index + 1);
+  uint32_t *it = pinned<uint32_t>(grown * B);
+  double *rr = pinned<double>(grown * B), *tg = pinned<double>(grown * B);
+  std::memcpy(it, hiters_, hsolve_slots_ * B * sizeof(uint32_t));
+  std::memcpy(rr, hrr_, hsolve_slots_ * B * sizeof(double));
+  std::memcpy(tg, htarget_, hsolve_slots_ * B * sizeof(double));
+  hiters_ = it;
+  hrr_ = rr;
+  htarget_ = tg;
+  hsolve_slots_ = grown;
+}
+
+// The plastic Newton of `bodies` for one substep pass, all bodies at once:
+// each round collects every unfinished body's next operation, launches each
+// kind of operation once for the bodies that asked for it (per-body
+// parameters select them), waits, and hands every body its results.
+template <class Real>
+void Engine<Real>::newton_pass(const std::vector<uint32_t> &bodies, const std::vector<StepParams> &params,
+                               uint32_t cap, uint32_t limit, bool diag, uint32_t pass,
+                               std::vector<uint32_t> &solve_pass,
+                               std::vector<std::vector<uint8_t>> &solve_active, uint32_t &solve_index,
+                               talus_mi_report &rep) {
+  hipStream_t s = stream_;
+  const uint32_t B = d_.B;
+  const size_t K = bodies.size();
+  std::vector<NewtonBody> nb(K);
+  std::vector<NewtonOp> op(K);
+  for (size_t i = 0; i < K; ++i) {
+    const StepParams &P = params[bodies[i]];
+    nb[i].cap = cap;
+    nb[i].travel = P.beta * P.inner * P.inner;
+    op[i] = nb[i].next();
+  }
+  HIPCHECK(hipStreamSynchronize(s));
+  nslot_ = 0;
+  for (;;) {
+    std::map<int, std::vector<size_t>> groups;
+    for (size_t i = 0; i < K; ++i)
+      if (op[i].kind != NewtonOp::DONE) groups[op[i].key()].push_back(i);
+    if (groups.empty()) break;
+    HIPCHECK(hipMemsetAsync(d_.loading, 0, B * sizeof(uint32_t), s));
+    bool norms = false, loads = false;
+    for (const auto &group : groups) {
+      const NewtonOp o = op[group.second.front()];
+      std::vector<StepParams> q = params;
+      std::vector<uint8_t> mask(B, 0);
+      for (StepParams &p : q) p.active = 0;
+      for (size_t i : group.second) {
+        StepParams &p = q[bodies[i]];
+        p.active = 1;
+        p.newton = 1;
+        p.tangent = o.tangent;
+        p.elastic_trial = o.elastic;
+        p.margin = NewtonBody::MARGIN;
+        mask[bodies[i]] = 1;
+      }
+      upload_newton(q);
+      if (o.kind == NewtonOp::EVAL) {
+        if (!o.first) Launch<Real>::evaluate(d_, s);
+        Launch<Real>::assemble(d_, diag, true, s);
+        Launch<Real>::balance(d_, diag, !o.first, s);
+        Launch<Real>::newton_norms(d_, s);
+        norms = loads = true;
+      } else if (o.kind == NewtonOp::SOLVE) {
+        ensure_solve_slot(solve_index);
+        solve_pass.push_back(pass);
+        solve_active.push_back(mask);
+        solve(o.first, limit, solve_index++);
+        Launch<Real>::newton_norms(d_, s);
+        norms = true;
+      } else {
+        Launch<Real>::newton_vec(d_, o.vec, s);
+      }
+    }
+    if (norms)
+      HIPCHECK(hipMemcpyAsync(hnnorm_, d_.nnorm, 3 * size_t(d_.n_ntiles) * sizeof(double),
+                              hipMemcpyDeviceToHost, s));
+    if (loads) HIPCHECK(hipMemcpyAsync(hloading_, d_.loading, B * sizeof(uint32_t), hipMemcpyDeviceToHost, s));
+    HIPCHECK(hipStreamSynchronize(s));
+    nslot_ = 0;
+    for (size_t i = 0; i < K; ++i) {
+      if (op[i].kind == NewtonOp::DONE) continue;
+      const uint32_t b = bodies[i];
+      if (op[i].kind == NewtonOp::EVAL) {
+        double sq = 0.0;
+        for (uint32_t t = nt0_[b]; t < nt1_[b]; ++t) sq += hnnorm_[3 * size_t(t)];
+        nb[i].squared = sq;
+        nb[i].loading = hloading_[b];
+      } else if (op[i].kind == NewtonOp::SOLVE) {
+        double moved = 0.0, extent = 0.0;
+        for (uint32_t t = nt0_[b]; t < nt1_[b]; ++t) {
+          moved = std::max(moved, hnnorm_[3 * size_t(t) + 1]);
+          extent = std::max(extent, hnnorm_[3 * size_t(t) + 2]);
+        }
+        nb[i].moved = moved;
+        nb[i].extent = extent;
+      }
+      op[i] = nb[i].next();
+    }
+  }
+  for (const NewtonBody &n : nb) {
+    rep.plastic_evaluations += n.evaluations;
+    rep.plastic_solves += n.solves_all;
+    rep.plastic_backtracks += n.backtracks;
+    rep.plastic_linear_reads += n.linear_reads;
+    rep.plastic_handoffs += n.handoffs;
+    rep.plastic_fallbacks += n.fallbacks;
+    rep.plastic_rejections += n.rejections;
+    rep.plastic_capped += n.capped;
+  }
+}
+
 template <class Real>
 void Engine<Real>::all_active() {
   std::vector<StepParams> p(d_.B);
@@ -1000,7 +1352,14 @@ void Engine<Real>::step(const talus_mi_step_params &sp, talus_mi_report &rep,
   if (converge && !hnorms_) hnorms_ = pinned<double>(2 * size_t(d_.n_ntiles));
   if (converge && !d_norms_) d_norms_ = dev<double>(2 * size_t(d_.n_ntiles));
   std::vector<uint32_t> solve_pass;
+  std::vector<std::vector<uint8_t>> solve_active;  // bodies of each solve (empty: every body in the pass)
   uint32_t solve_index = 0;
+  // plastic Newton (plasticPasses > 1; not for converged rollouts)
+  const uint32_t cap = sp.plastic_passes;
+  const bool newton = cap > 1 && !converge &&
+                      std::any_of(newton_body_.begin(), newton_body_.end(), [](uint8_t f) { return f != 0; });
+  talus_mi_report newton_rep;
+  std::memset(&newton_rep, 0, sizeof newton_rep);
   const uint32_t refactors_before = refactors_;
   applications_ = 0;
   const size_t n3 = 3 * size_t(d_.N);
@@ -1043,34 +1402,63 @@ void Engine<Real>::step(const talus_mi_step_params &sp, talus_mi_report &rep,
       HIPCHECK(hipMemcpyAsync(record + 2 * n3, d_.a, n3 * sizeof(double), hipMemcpyDeviceToDevice, s));
     }
     Launch<Real>::predict(d_, s);
-    Launch<Real>::assemble(d_, diag, false, s);
-    Launch<Real>::balance(d_, diag, false, s);
-    solve_pass.push_back(pass);
-    solve(true, limit, solve_index++);
-    for (uint32_t read = 1; read < reads; ++read) {
-      Launch<Real>::evaluate(d_, s);
-      Launch<Real>::assemble(d_, diag, converge || read == 2, s);
-      Launch<Real>::balance(d_, diag, true, s);
-      solve_pass.push_back(pass);
-      solve(false, limit, solve_index++);
-      if (converge) {
-        // corrections until max |increment| <= 1e-12 max |a| in every body
-        Launch<Real>::increment_norms(d_, d_norms_, s);
-        HIPCHECK(hipMemcpyAsync(hnorms_, d_norms_, 2 * size_t(d_.n_ntiles) * sizeof(double),
-                                hipMemcpyDeviceToHost, s));
-        HIPCHECK(hipStreamSynchronize(s));
-        bool all = true;
-        for (uint32_t b = 0; b < B && all; ++b) {
-          if (pass >= substeps[b]) continue;
-          double dx = 0.0, a = 0.0;
-          for (uint32_t t = nt0_[b]; t < nt1_[b]; ++t) {
-            dx = std::max(dx, hnorms_[2 * t]);
-            a = std::max(a, hnorms_[2 * t + 1]);
-          }
-          all = dx <= 1.0e-12 * a;
-        }
-        if (all) break;
+    // bodies with yielding corotated lanes take the plastic Newton
+    std::vector<uint32_t> nbodies;
+    uint32_t in_pass = 0;
+    for (uint32_t b = 0; b < B; ++b) {
+      if (pass >= substeps[b]) continue;
+      ++in_pass;
+      if (newton && newton_body_[b]) nbodies.push_back(b);
+    }
+    std::vector<uint8_t> reg_mask;
+    if (!nbodies.empty()) {
+      std::vector<StepParams> reg = params;
+      reg_mask.assign(B, 0);
+      for (uint32_t b = 0; b < B; ++b) reg_mask[b] = pass < substeps[b] ? 1 : 0;
+      for (uint32_t b : nbodies) {
+        reg[b].active = 0;
+        reg_mask[b] = 0;
       }
+      upload_newton(reg);
+    }
+    if (nbodies.size() < in_pass) {
+      Launch<Real>::assemble(d_, diag, false, s);
+      Launch<Real>::balance(d_, diag, false, s);
+      solve_pass.push_back(pass);
+      solve_active.push_back(reg_mask);
+      ensure_solve_slot(solve_index);
+      solve(true, limit, solve_index++);
+      for (uint32_t read = 1; read < reads; ++read) {
+        Launch<Real>::evaluate(d_, s);
+        Launch<Real>::assemble(d_, diag, converge || read == 2, s);
+        Launch<Real>::balance(d_, diag, true, s);
+        solve_pass.push_back(pass);
+        solve_active.push_back(reg_mask);
+        ensure_solve_slot(solve_index);
+        solve(false, limit, solve_index++);
+        if (converge) {
+          // corrections until max |increment| <= 1e-12 max |a| in every body
+          Launch<Real>::increment_norms(d_, d_norms_, s);
+          HIPCHECK(hipMemcpyAsync(hnorms_, d_norms_, 2 * size_t(d_.n_ntiles) * sizeof(double),
+                                  hipMemcpyDeviceToHost, s));
+          HIPCHECK(hipStreamSynchronize(s));
+          bool all = true;
+          for (uint32_t b = 0; b < B && all; ++b) {
+            if (pass >= substeps[b]) continue;
+            double dx = 0.0, a = 0.0;
+            for (uint32_t t = nt0_[b]; t < nt1_[b]; ++t) {
+              dx = std::max(dx, hnorms_[2 * t]);
+              a = std::max(a, hnorms_[2 * t + 1]);
+            }
+            all = dx <= 1.0e-12 * a;
+          }
+          if (all) break;
+        }
+      }
+    }
+    if (!nbodies.empty()) {
+      newton_pass(nbodies, params, cap, limit, diag, pass, solve_pass, solve_active, solve_index, newton_rep);
+      upload_newton(params);
     }
     Launch<Real>::finish(d_, s);
     if (any_plastic_) Launch<Real>::commit(d_, s);
@@ -1118,8 +1506,10 @@ void Engine<Real>::step(const talus_mi_step_params &sp, talus_mi_report &rep,
   for (uint32_t si = 0; si < solve_index; ++si) {
     const uint32_t pass = solve_pass[si];
     const size_t slot = (si % hsolve_slots_) * B;
+    const std::vector<uint8_t> &mask = solve_active[si];
     for (uint32_t b = 0; b < B; ++b) {
       if (pass >= substeps[b]) continue;
+      if (!mask.empty() && !mask[b]) continue;
       rep.cg_iterations += hiters_[slot + b];
       worst[b] = std::max(worst[b], hiters_[slot + b]);
       const bool capped = hiters_[slot + b] >= limit && !(hrr_[slot + b] <= htarget_[slot + b]);
@@ -1144,6 +1534,14 @@ void Engine<Real>::step(const talus_mi_step_params &sp, talus_mi_report &rep,
     rep.yield_events += hyields_[b];
   }
   rep.plastic_growth = rep.yield_events ? 1u : 0u;
+  rep.plastic_evaluations = newton_rep.plastic_evaluations;
+  rep.plastic_solves = newton_rep.plastic_solves;
+  rep.plastic_backtracks = newton_rep.plastic_backtracks;
+  rep.plastic_linear_reads = newton_rep.plastic_linear_reads;
+  rep.plastic_handoffs = newton_rep.plastic_handoffs;
+  rep.plastic_fallbacks = newton_rep.plastic_fallbacks;
+  rep.plastic_rejections = newton_rep.plastic_rejections;
+  rep.plastic_capped = newton_rep.plastic_capped;
   if (use_factor_) {
     for (uint32_t b = 0; b < B; ++b)
       if (capped_body[b] || (worst[b] >= REFRESH_ITERS && steps_taken_ >= next_refresh_[b]))
@@ -1256,20 +1654,18 @@ void Engine<Real>::eval_forces(uint32_t b, const double *x, double *tf, double *
   if (R) rotations_get(b, R);
 }
 
+// the body's operator with the same kernels as the allocation-free entry
+// point, on the engine's FP64 staging and the body's own run of slots
 template <class Real>
 void Engine<Real>::apply_system(uint32_t b, double inertia, double step, const double *dv,
                                 double *y) {
   hipStream_t s = stream_;
   const size_t n0 = n0_[b], nn = nn_[b];
-  std::vector<Real> in(3 * size_t(d_.N), Real(0));
-  for (size_t i = 0; i < 3 * nn; ++i) in[3 * n0 + i] = Real(dv[i]);
-  HIPCHECK(hipMemcpyAsync(d_.p, in.data(), in.size() * sizeof(Real), hipMemcpyHostToDevice, s));
-  Launch<Real>::op_element_plain(d_, d_.p, s);
-  Launch<Real>::apply_plain(d_, d_.p, d_.q, inertia, step * step, s);
-  std::vector<Real> out(3 * nn);
-  HIPCHECK(hipMemcpyAsync(out.data(), d_.q + 3 * n0, 3 * nn * sizeof(Real), hipMemcpyDeviceToHost, s));
+  HIPCHECK(hipMemcpyAsync(scratch64_, dv, 3 * nn * sizeof(double), hipMemcpyHostToDevice, s));
+  apply_system_device(b, inertia, step, scratch64_, scratch64_ + 3 * size_t(d_.N),
+                      d_.slot_f + 3 * size_t(inc_start_[n0]), s);
+  HIPCHECK(hipMemcpyAsync(y, scratch64_ + 3 * size_t(d_.N), 3 * nn * sizeof(double), hipMemcpyDeviceToHost, s));
   HIPCHECK(hipStreamSynchronize(s));
-  for (size_t i = 0; i < 3 * nn; ++i) y[i] = double(out[i]);
 }
 
 template <class Real>
@@ -1650,6 +2046,14 @@ class GroupedEngine final : public EngineBase {
       rep.non_finite |= r.non_finite;
       rep.preconditioner_applications += r.preconditioner_applications;
       rep.device_seconds = std::max(rep.device_seconds, r.device_seconds);
+      rep.plastic_evaluations += r.plastic_evaluations;
+      rep.plastic_solves += r.plastic_solves;
+      rep.plastic_backtracks += r.plastic_backtracks;
+      rep.plastic_linear_reads += r.plastic_linear_reads;
+      rep.plastic_handoffs += r.plastic_handoffs;
+      rep.plastic_fallbacks += r.plastic_fallbacks;
+      rep.plastic_rejections += r.plastic_rejections;
+      rep.plastic_capped += r.plastic_capped;
     }
     // events in the one-engine order: by body, then tet, yield before tear
     const size_t first = events.size();
@@ -1892,6 +2296,7 @@ void talus_mi_default_params(talus_mi_step_params *p) {
   p->spectral_radius = 0.5f;
   p->preconditioner = TALUS_MI_BLOCK_JACOBI;
   p->flags = 0;
+  p->plastic_passes = 40;
 }
 
 int talus_mi_create(talus_mi_ctx **out, int device, uint32_t precision) {

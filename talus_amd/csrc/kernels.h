@@ -40,6 +40,13 @@ struct StepParams {
   double tolerance;
   uint32_t active;  // body takes part in this pass
   uint32_t limit;   // CG iteration cap
+  // plastic Newton of bodies with yielding corotated lanes (the step's
+  // plastic branch): the assembly stores the return maps' linearisations
+  // (newton), binds the consistent tangent in the operator and diagonal
+  // blocks (tangent), skips the return map (elastic_trial) and counts the
+  // tets loading past `margin`
+  uint32_t newton, tangent, elastic_trial, pad;
+  double margin;
 };
 
 template <class Real>
@@ -73,6 +80,10 @@ struct Dev {
   Real *R;
   Real *W;          // neo-Hookean volume frame cof(Fe) Fp^-T, 9 SoA planes
   double *plastic;  // 10 SoA planes
+  Real *lin;        // J2 linearisation {theta, thetaBar, n (6)}, 8 SoA planes
+  uint32_t *loading;  // [B] tets loading past the margin in the last assembly
+  double *lowest;   // 3N, the Newton's lowest-residual iterate
+  double *nnorm;    // 3 per node tile: sum |right|^2, max |dx|, max |x_eval|
   Real *slot_f, *slot_d, *energy;
   uint8_t *yflag, *tflag;
   // reductions
@@ -89,6 +100,15 @@ struct Dev {
 };
 
 enum : uint8_t { FLAG_SNH = 1, FLAG_PLASTIC = 2, FLAG_TEAR = 4 };
+
+// iterate updates of the plastic Newton, on the free nodes of active bodies
+enum NewtonVec : int {
+  NV_BACKTRACK = 0,  // dx *= 1/2, a -= dx (halve the last increment)
+  NV_SAVE = 1,       // lowest = a
+  NV_RESTORE = 2,    // a = lowest
+  NV_RESET = 3,      // a = carried, x0 = a, x_eval = x_w (acceleration 0 total)
+  NV_DX_FROM_A = 4,  // dx = a (the first solve's increment is the whole total)
+};
 
 // Adjoint of one converged substep (differentiable rollout): adjoint state,
 // accumulated gradients and the recorded substep (x_n, v_n, a_n, a').
@@ -134,6 +154,10 @@ struct Launch {
   static void gather_forces(const Dev<Real> &d, Real *out, hipStream_t s);
   // convergence of the corrections: per node tile max |dx| and max |a|
   static void increment_norms(const Dev<Real> &d, double *out, hipStream_t s);
+  // plastic Newton: per node tile |right|^2, max node |dx|, max node
+  // |x_eval| (d.nnorm); and the iterate updates of NewtonVec
+  static void newton_norms(const Dev<Real> &d, hipStream_t s);
+  static void newton_vec(const Dev<Real> &d, int mode, hipStream_t s);
   // adjoint of a converged substep (neo-Hookean Hessian)
   static void hvp(const Dev<Real> &d, const Real *dir, int parity, hipStream_t s);
   static void adj_prepare(const Dev<Real> &d, const AdjDev &a, hipStream_t s);

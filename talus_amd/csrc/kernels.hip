@@ -342,7 +342,8 @@ __global__ __launch_bounds__(TILE) void k_pcg_begin(Dev<Real> d) {
 // the linear stiffness; under stretch the volumetric part follows the
 // (lambda+mu)(J-1)^2 term's stiffening along cof(F).
 template <class Real>
-__device__ __forceinline__ void op_tet_values(const Dev<Real> &d, uint32_t e, const Real *dv, Real *y) {
+__device__ __forceinline__ void op_tet_values(const Dev<Real> &d, uint32_t e, const Real *dv, Real *y,
+                                              bool tangent = false) {
   Real dm[9], g[12], gs[12], Rm[9];
   load_dm(d, e, dm);
   shape_gradients(dm, g);
@@ -364,11 +365,25 @@ __device__ __forceinline__ void op_tet_values(const Dev<Real> &d, uint32_t e, co
   const Real lam = d.lam[e], mu = d.mu[e], V = d.vol[e];
   const Real tr = snh ? Real(0) : D[0] + D[4] + D[8];
   Real S[9];
+  if (tangent && !snh && (flags & FLAG_PLASTIC)) {
+    // the consistent J2 tangent (tangentOf) on the corotated strain sym(D')
+    Real lin[8], eps[9], sig[9];
 #pragma unroll
-  for (int r = 0; r < 3; ++r)
+    for (int k = 0; k < 8; ++k) lin[k] = d.lin[(size_t)k * d.T + e];
 #pragma unroll
-    for (int k = 0; k < 3; ++k)
-      S[r * 3 + k] = V * (mu * (D[r * 3 + k] + D[k * 3 + r]) + (r == k ? lam * tr : Real(0)));
+    for (int r = 0; r < 3; ++r)
+#pragma unroll
+      for (int k = 0; k < 3; ++k) eps[r * 3 + k] = Real(0.5) * (D[r * 3 + k] + D[k * 3 + r]);
+    j2_tangent_stress(lin, lam, mu, eps, sig);
+#pragma unroll
+    for (int k = 0; k < 9; ++k) S[k] = V * sig[k];
+  } else {
+#pragma unroll
+    for (int r = 0; r < 3; ++r)
+#pragma unroll
+      for (int k = 0; k < 3; ++k)
+        S[r * 3 + k] = V * (mu * (D[r * 3 + k] + D[k * 3 + r]) + (r == k ? lam * tr : Real(0)));
+  }
 #pragma unroll
   for (int i = 0; i < 4; ++i) {
     Real tmp[3];
@@ -398,10 +413,11 @@ __device__ __forceinline__ void op_tet_values(const Dev<Real> &d, uint32_t e, co
 }
 
 template <class Real>
-__device__ __forceinline__ void op_tet(const Dev<Real> &d, uint32_t e, const Real *dir, Real *y) {
+__device__ __forceinline__ void op_tet(const Dev<Real> &d, uint32_t e, const Real *dir, Real *y,
+                                       bool tangent = false) {
   Real dv[12];
   gather_vec(dir, d.conn[e], dv);
-  op_tet_values(d, e, dv, y);
+  op_tet_values(d, e, dv, y, tangent);
 }
 
 template <class Real>
@@ -411,7 +427,7 @@ __global__ __launch_bounds__(TILE) void k_op_element(Dev<Real> d, const Real *di
   const uint32_t e = d.tt_begin[t] + threadIdx.x;
   if (e >= d.tt_end[t]) return;
   Real y[12];
-  op_tet(d, e, dir, y);
+  op_tet(d, e, dir, y, d.params[b].tangent != 0u);
   store_slots(d.slot_f, d.slot_pos[e], y);
 }
 
@@ -620,7 +636,8 @@ __global__ __launch_bounds__(TILE) void k_assemble(Dev<Real> d) {
   constexpr bool MIXED = sizeof(Real) < 8;
   __shared__ double lds[TILE / 64];
   const uint32_t t = blockIdx.x, b = d.tt_body[t];
-  if (!d.params[b].active) return;
+  const StepParams &P = d.params[b];
+  if (!P.active) return;
   const uint32_t e = d.tt_begin[t] + threadIdx.x;
   double psi_total = 0.0;
   if (e < d.tt_end[t]) {
@@ -680,6 +697,7 @@ __global__ __launch_bounds__(TILE) void k_assemble(Dev<Real> d) {
     Real f[12];
     Real psi;
     Real Wm[9];
+    Real lin[8] = {Real(1), Real(0), Real(0), Real(0), Real(0), Real(0), Real(0), Real(0)};
     if (flags & FLAG_SNH) {
       const bool plastic = (flags & FLAG_PLASTIC) != 0;
       Real ep[9], He[9], Fe[9];
@@ -769,7 +787,23 @@ __global__ __launch_bounds__(TILE) void k_assemble(Dev<Real> d) {
 #pragma unroll
       for (int k = 0; k < 10; ++k) pl[k] = d.plastic[(size_t)k * d.T + e];
       if (flags & FLAG_PLASTIC) {
-        j2_return(eps, pl, lam, mu, Real(d.yield[e]), Real(d.hard[e]), S, (double *)nullptr);
+        // tetPointTrial (material.cc:1004-1022): the return map from the
+        // committed point, loading past the margin; its linearisation is the
+        // return map's when loading, else the elastic law
+        Real excess;
+        const bool returned = j2_return_lin(eps, pl, lam, mu, Real(d.yield[e]), Real(d.hard[e]),
+                                            P.elastic_trial == 0u, S, (double *)nullptr, lin, &excess);
+        const bool loading = returned && double(excess) > P.margin;
+        if (loading) atomicAdd(&d.loading[b], 1u);
+        if (!loading) {
+          lin[0] = Real(1);
+#pragma unroll
+          for (int k = 1; k < 8; ++k) lin[k] = Real(0);
+        }
+        if (P.newton) {
+#pragma unroll
+          for (int k = 0; k < 8; ++k) d.lin[(size_t)k * d.T + e] = lin[k];
+        }
       } else {
         const Real tr = eps[0] + eps[4] + eps[8];
 #pragma unroll
@@ -794,10 +828,21 @@ __global__ __launch_bounds__(TILE) void k_assemble(Dev<Real> d) {
     for (int k = 0; k < 9; ++k) d.R[(size_t)k * d.T + e] = Rm[k];
     if (DIAG) {
       // stiffnessDiagonal: R K_ii R^T = V((lambda+mu)(Rg)(Rg)^T + mu |g|^2 I);
-      // neo-Hookean: V(mu (Rg)(Rg)^T + lambda (Wg)(Wg)^T + mu |g|^2 I)
+      // neo-Hookean: V(mu (Rg)(Rg)^T + lambda (Wg)(Wg)^T + mu |g|^2 I);
+      // a yielding corotated tet under the consistent tangent (material.cc:
+      // 1033-1054): V((K + mu theta/3)(Rg)(Rg)^T + mu theta |g|^2 I
+      // - 2 mu thetaBar (R n g)(R n g)^T), K = lambda + 2 mu / 3, with the
+      // (R n) frame in place of W
       const bool snh = (flags & FLAG_SNH) != 0;
-      const Real lr = V * (snh ? mu : lam + mu), lw = snh ? V * lam : Real(0), vm = V * mu;
-      if (!snh) {
+      const bool tangent = P.tangent && !snh && (flags & FLAG_PLASTIC);
+      Real lr = V * (snh ? mu : lam + mu), lw = snh ? V * lam : Real(0), vm = V * mu;
+      if (tangent) {
+        const Real n[9] = {lin[2], lin[3], lin[4], lin[3], lin[5], lin[6], lin[4], lin[6], lin[7]};
+        lr = V * (lam + Real(2) * mu / Real(3) + mu * lin[0] / Real(3));
+        lw = Real(-2) * V * mu * lin[1];
+        vm = V * mu * lin[0];
+        mul3(Rm, n, Wm);  // (R n) g
+      } else if (!snh) {
 #pragma unroll
         for (int k = 0; k < 9; ++k) Wm[k] = Real(0);
       }
@@ -1018,6 +1063,89 @@ inline unsigned blocks_for(uint32_t n) { return (n + TILE - 1) / TILE; }
 // ------------------------------------------------- corrections converge
 
 // per node tile: max |dx| and max |a| over the tile's free nodes
+// Plastic Newton, per node tile of an active body: sum of |right|^2 over
+// its nodes (anchored rows are zero), the largest node increment |dx| and
+// the largest node position |x_eval| over its free nodes (the reference's
+// glm::length per node).
+template <class Real>
+__global__ __launch_bounds__(TILE) void k_newton_norms(Dev<Real> d) {
+  __shared__ double lds[3][TILE / 64];
+  const uint32_t t = blockIdx.x, b = d.nt_body[t];
+  if (!d.params[b].active) return;
+  const uint32_t n = d.nt_begin[t] + threadIdx.x;
+  double sq = 0.0, moved = 0.0, extent = 0.0;
+  if (n < d.nt_end[t]) {
+    double dd = 0.0, pp = 0.0;
+#pragma unroll
+    for (int c = 0; c < 3; ++c) {
+      const size_t i = 3 * (size_t)n + c;
+      const double r = d.right[i], dx = double(d.dx[i]), p = d.u_eval[i] + d.X[i];
+      sq += r * r;
+      dd += dx * dx;
+      pp += p * p;
+    }
+    if (!d.anchored[n]) {
+      moved = sqrt(dd);
+      extent = sqrt(pp);
+    }
+  }
+#pragma unroll
+  for (int off = 32; off > 0; off >>= 1) {
+    sq += __shfl_down(sq, off, 64);
+    moved = fmax(moved, __shfl_down(moved, off, 64));
+    extent = fmax(extent, __shfl_down(extent, off, 64));
+  }
+  const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+  if (lane == 0) {
+    lds[0][wave] = sq;
+    lds[1][wave] = moved;
+    lds[2][wave] = extent;
+  }
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    double s0 = 0.0, m1 = 0.0, m2 = 0.0;
+    for (int w = 0; w < TILE / 64; ++w) {
+      s0 += lds[0][w];
+      m1 = fmax(m1, lds[1][w]);
+      m2 = fmax(m2, lds[2][w]);
+    }
+    d.nnorm[3 * t] = s0;
+    d.nnorm[3 * t + 1] = m1;
+    d.nnorm[3 * t + 2] = m2;
+  }
+}
+
+// Iterate updates of the plastic Newton on active bodies (NewtonVec).
+template <class Real>
+__global__ __launch_bounds__(TILE) void k_newton_vec(Dev<Real> d, int mode) {
+  const uint32_t t = blockIdx.x, b = d.nt_body[t];
+  if (!d.params[b].active) return;
+  const uint32_t n = d.nt_begin[t] + threadIdx.x;
+  if (n >= d.nt_end[t]) return;
+  const bool free = !d.anchored[n];
+#pragma unroll
+  for (int c = 0; c < 3; ++c) {
+    const size_t i = 3 * (size_t)n + c;
+    switch (mode) {
+      case NV_BACKTRACK: {
+        const Real half = d.dx[i] * Real(0.5);
+        d.dx[i] = half;
+        d.a[i] -= double(half);
+        break;
+      }
+      case NV_SAVE: d.lowest[i] = d.a[i]; break;
+      case NV_RESTORE: d.a[i] = d.lowest[i]; break;
+      case NV_RESET:
+        d.a[i] = d.carried[i];
+        d.x0[i] = Real(d.carried[i]);
+        d.u_eval[i] = d.weighted[i] - d.X[i];
+        break;
+      case NV_DX_FROM_A: d.dx[i] = free ? Real(d.a[i]) : Real(0); break;
+      default: break;
+    }
+  }
+}
+
 template <class Real>
 __global__ __launch_bounds__(TILE) void k_increment_norms(Dev<Real> d, double *out) {
   __shared__ double lds[2][TILE / 64];
@@ -1351,6 +1479,14 @@ void Launch<Real>::tear(const Dev<Real> &d, hipStream_t s) {
 template <class Real>
 void Launch<Real>::increment_norms(const Dev<Real> &d, double *out, hipStream_t s) {
   TMI_LAUNCH(k_increment_norms<Real>, d.n_ntiles, s, d, out);
+}
+template <class Real>
+void Launch<Real>::newton_norms(const Dev<Real> &d, hipStream_t s) {
+  TMI_LAUNCH(k_newton_norms<Real>, d.n_ntiles, s, d);
+}
+template <class Real>
+void Launch<Real>::newton_vec(const Dev<Real> &d, int mode, hipStream_t s) {
+  TMI_LAUNCH(k_newton_vec<Real>, d.n_ntiles, s, d, mode);
 }
 template <class Real>
 void Launch<Real>::hvp(const Dev<Real> &d, const Real *dir, int parity, hipStream_t s) {

@@ -307,11 +307,15 @@ TM_HD void svd_rotation(const double *F, double *R) {
     for (int c = 0; c < 3; ++c) R[r * 3 + c] = u0[r] * v0[c] + u1[r] * v1[c] + u2[r] * v2[c];
 }
 
-// J2 radial return (returnMap) on the corotated small strain; plastic =
-// {eps_p[9], alpha}.  Writes the stress, returns true when plastic.
+// J2 radial return (returnMap, material.cc:594-630) on the corotated small
+// strain; plastic = {eps_p[9], alpha}.  Writes the stress, returns true when
+// the point returns to the yield surface; return_mapping = false is the
+// elastic trial.  lin, if given, receives the linearisation {theta,
+// thetaBar, n00, n01, n02, n11, n12, n22} (theta 1, thetaBar 0 when elastic)
+// and excess the relative overshoot (sigma_eq - radius) / radius.
 template <class R>
-TM_HD bool j2_return(const R *strain, const double *plastic, R lambda, R mu, R yield,
-                     R hardening, R *stress, double *next) {
+TM_HD bool j2_return_lin(const R *strain, const double *plastic, R lambda, R mu, R yield, R hardening,
+                         bool return_mapping, R *stress, double *next, R *lin, R *excess) {
   const R bulk = lambda + R(2) * mu / R(3);
   R el[9];
 #pragma unroll
@@ -330,7 +334,13 @@ TM_HD bool j2_return(const R *strain, const double *plastic, R lambda, R mu, R y
   if (next) {
     for (int i = 0; i < 10; ++i) next[i] = plastic[i];
   }
-  if (!(equivalent > radius) || !(norm > R(0))) {
+  if (lin) {
+    lin[0] = R(1);
+#pragma unroll
+    for (int i = 1; i < 8; ++i) lin[i] = R(0);
+  }
+  if (excess) *excess = R(0);
+  if (!return_mapping || !(equivalent > radius) || !(norm > R(0))) {
 #pragma unroll
     for (int i = 0; i < 9; ++i) stress[i] = trial[i] + ((i % 4 == 0) ? bulk * tr : R(0));
     return false;
@@ -344,7 +354,42 @@ TM_HD bool j2_return(const R *strain, const double *plastic, R lambda, R mu, R y
       next[i] += double(R(1.2247448713915890491) * gamma * trial[i] / norm);
     next[9] += double(gamma);
   }
+  if (lin) {
+    lin[0] = theta;
+    lin[1] = R(3) * mu / (R(3) * mu + hardening) - (R(1) - theta);
+    lin[2] = trial[0] / norm;
+    lin[3] = trial[1] / norm;
+    lin[4] = trial[2] / norm;
+    lin[5] = trial[4] / norm;
+    lin[6] = trial[5] / norm;
+    lin[7] = trial[8] / norm;
+  }
+  if (excess) *excess = (equivalent - radius) / radius;
   return true;
+}
+
+template <class R>
+TM_HD bool j2_return(const R *strain, const double *plastic, R lambda, R mu, R yield,
+                     R hardening, R *stress, double *next) {
+  return j2_return_lin(strain, plastic, lambda, mu, yield, hardening, true, stress, next, (R *)nullptr,
+                       (R *)nullptr);
+}
+
+// tangentOf (material.cc:636-647) for the linearisation lin of j2_return_lin:
+// the stress K tr(e) I + 2 mu theta dev(e) - 2 mu thetaBar (n:e) n of a
+// symmetric strain e.
+template <class R>
+TM_HD void j2_tangent_stress(const R *lin, R lambda, R mu, const R *e, R *out) {
+  const R bulk = lambda + R(2) * mu / R(3);
+  const R tr = e[0] + e[4] + e[8];
+  const R n[9] = {lin[2], lin[3], lin[4], lin[3], lin[5], lin[6], lin[4], lin[6], lin[7]};
+  R along = R(0);
+#pragma unroll
+  for (int i = 0; i < 9; ++i) along += n[i] * e[i];
+  const R shear = R(2) * mu * lin[0], flow = R(2) * mu * lin[1] * along;
+#pragma unroll
+  for (int i = 0; i < 9; ++i)
+    out[i] = shear * (e[i] - ((i % 4 == 0) ? tr / R(3) : R(0))) + ((i % 4 == 0) ? bulk * tr : R(0)) - flow * n[i];
 }
 
 // Hencky return (plasticflow.cc returnMap): returns true and the returned
