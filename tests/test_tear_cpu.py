@@ -186,3 +186,87 @@ def test_stretch_on_plain_arrays_equals_the_body_stretch_and_numpy():
     assert abs(s[0] - O.stretch(F)[0]) <= 1e-12
     with pytest.raises(ValueError):
         O.stretch_tets([[0, 1, 2, 9]], rest, rest)
+
+
+def _column(mesh, rest, hexes_x, which):
+    """Tets of the cells with x index 1 (the middle column of a 3-cell-long
+    strip): all of them, or those with three corners on the plane x = 2h
+    (whose crack plane is therefore x = 2h)."""
+    tet = mesh["tet_nodes"].reshape(-1, 4)
+    h = 0.1
+    cent = rest[tet].mean(axis=1)
+    col = np.flatnonzero(np.floor(cent[:, 0] / h).astype(int) == 1)
+    if which == "all":
+        return col
+    return np.array([e for e in col if np.isclose(rest[tet[e], 0], 2 * h).sum() == 3])
+
+
+@pytest.mark.parametrize("section", [(1, 1), (2, 1), (2, 2)])
+def test_tearing_a_cross_section_splits_its_plane_into_two_pieces(section):
+    """Hand-derived outcome of tearCells' rules (tearing.cc:292-513 on tets) on
+    a 3-cell strip stretched 1.5x along x, with the x = 0 face anchored: the
+    tets of the middle column whose three nearest corners lie on x = 2h
+    crack along that plane.  Corners closest to the crack plane split first
+    and the pinched ones follow, so exactly the nodes of the cross-section
+    x = 2h ((ny+1)(nz+1) of them) are duplicated, the strip falls into 2
+    pieces, the copies sit on the plane and are not anchored, and the
+    original ids stay on the side of negative rest offset (side 0 keeps the
+    node, the copy goes to the other side's cells)."""
+    ny, nz = section
+    body, mesh, mats = strip(cells=(3, ny, nz), voxel=0.1,
+                             supports=[((0.0, 0.5, 0.5), (0.01, 1.0, 1.0))])
+    rest = mesh["rest_positions"].astype(np.float64)
+    x = rest * np.array([1.5, 1.0, 1.0])
+    torn = _column(mesh, rest, 3, "plane")
+    assert torn.size == 2 * ny * nz  # two Kuhn tets per cell have three corners on x = 2h
+    new, origin = check_tear(mesh, mats, x, torn)
+    n0 = rest.shape[0]
+    plane = np.flatnonzero(np.isclose(rest[:, 0], 0.2))
+    assert plane.size == (ny + 1) * (nz + 1)
+    assert new["anchored"].size - n0 == plane.size
+    np.testing.assert_array_equal(np.sort(origin), plane)
+    assert np.allclose(new["rest_positions"][n0:, 0], 0.2)
+    assert pieces(new["tet_nodes"].reshape(-1, 4), new["anchored"].size) == 2
+    tet = new["tet_nodes"].reshape(-1, 4)
+    cent = rest[mesh["tet_nodes"].reshape(-1, 4)].mean(axis=1)[:, 0]
+    for e in range(tet.shape[0]):
+        on_plane = np.isin(mesh["tet_nodes"].reshape(-1, 4)[e], plane)
+        if not on_plane.any():
+            continue
+        if cent[e] < 0.2:
+            assert (tet[e][on_plane] < n0).all(), e  # originals on the low side
+        else:
+            assert (tet[e][on_plane] >= n0).all(), e  # copies on the high side
+
+
+@pytest.mark.parametrize("section", [(1, 1), (2, 1), (2, 2)])
+def test_tearing_a_whole_column_cuts_both_of_its_planes(section):
+    """Every tet of the middle column torn: cells whose nearest corners lie on
+    x = h crack there, the others on x = 2h, so both cross-sections are
+    duplicated (2 (ny+1)(nz+1) new nodes) and the strip falls into 3 pieces,
+    the middle column (6 ny nz tets, at least MIN_PIECE_CELLS) one of them."""
+    ny, nz = section
+    body, mesh, mats = strip(cells=(3, ny, nz), voxel=0.1,
+                             supports=[((0.0, 0.5, 0.5), (0.01, 1.0, 1.0))])
+    rest = mesh["rest_positions"].astype(np.float64)
+    x = rest * np.array([1.5, 1.0, 1.0])
+    new, origin = check_tear(mesh, mats, x, _column(mesh, rest, 3, "all"))
+    n0 = rest.shape[0]
+    assert new["anchored"].size - n0 == 2 * (ny + 1) * (nz + 1)
+    assert sorted(np.unique(np.round(new["rest_positions"][n0:, 0], 6))) == [0.1, 0.2]
+    assert pieces(new["tet_nodes"].reshape(-1, 4), new["anchored"].size) == 3
+
+
+def test_one_torn_tet_of_a_one_cell_strip_adds_four_nodes():
+    """The tet form of the reference's strip test (tests/dynamics_strainstep_
+    tests.cc:3760-3796: a one-cell-thick strip split adds exactly 4 nodes and
+    leaves 2 pieces): any single tet of the middle cell cracks the strip
+    through the whole 2 x 2-node cross-section nearest its corners."""
+    body, mesh, mats = strip(cells=(3, 1, 1), voxel=0.1)
+    rest = mesh["rest_positions"].astype(np.float64)
+    x = rest * np.array([1.5, 1.0, 1.0])
+    for e in _column(mesh, rest, 3, "all"):
+        new, origin = check_tear(mesh, mats, x, [int(e)])
+        assert new["anchored"].size - rest.shape[0] == 4
+        assert np.unique(np.round(rest[origin, 0], 6)).size == 1
+        assert pieces(new["tet_nodes"].reshape(-1, 4), new["anchored"].size) == 2
