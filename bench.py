@@ -49,6 +49,7 @@ sys.path.insert(0, ROOT)
 
 METRIC = "tet updates/sec: mixed-precision batched vs plain FP64, 1 GPU, ≤1e-5 rel force err"
 HBM_PEAK_GBS = 8000.0
+FP64_MFMA_PEAK_TFS = 78.6  # MI355X dense FP64 matrix peak
 FORCE_BAR = 1.0e-5  # the metric's relative force error bar, mixed against FP64
 KINDS = {"assemble": 0, "op_element": 1, "op_gather": 2, "solve_forward_lanes": 3,
          "solve_backward_lanes": 4, "solve_forward_rows": 5, "solve_backward_rows": 6,
@@ -525,6 +526,9 @@ def main(argv=None):
     ap.add_argument("--warmup", type=int, default=5)
     ap.add_argument("--workload", choices=["bodies", "cube"], default="bodies")
     ap.add_argument("--skip-fp64", action="store_true")
+    ap.add_argument("--run-steps", type=int, default=None,
+                    help="steps per batched run (default: the workload's, 120 or 240); a shorter run "
+                         "reports no metric value (for profiling)")
     ap.add_argument("--skip-plain", action="store_true")
     ap.add_argument("--skip-cpu", action="store_true")
     ap.add_argument("--no-tune", action="store_true")
@@ -601,7 +605,7 @@ def main(argv=None):
     log(f"[bench] workload {args.workload}: {len(meshes)} bodies, {T} tets, {N} nodes")
 
     w0 = max(args.warmup, 1)  # the first step (factorization, plan) is never timed
-    total = max(sc.steps, w0 + args.steps)
+    total = max(sc.steps if args.run_steps is None else args.run_steps, w0 + args.steps)
     window = [w0, w0 + args.steps]
     log(f"[bench] {total} steps per run, timed window [{window[0]}, {window[1]})")
 
@@ -705,6 +709,18 @@ def main(argv=None):
                                        "".join(map(str, ctx_m.plan()))),
                 "algorithmic_bytes": kd["bytes"], "avg_ms": kd["avg_ms"]}
     log(f"[bench] kernels {json.dumps(kernels)}")
+    # the supernodal factorization's kernels: FP64 MFMA tiles issued and the
+    # flops on the fronts' own entries, per kernel over one factorization
+    factor_kernels = {}
+    for name, k in ctx_m.factor_profile().items():
+        sec = k["ms"] * 1e-3
+        entry = {"launches": k["launches"], "workgroups": k["workgroups"], "ms": k["ms"]}
+        if k["mfma_flops"] > 0.0:
+            entry.update(mfma_tflops=k["mfma_flops"] / sec / 1e12, useful_tflops=k["useful_flops"] / sec / 1e12,
+                         mfma_frac=k["mfma_flops"] / sec / 1e12 / FP64_MFMA_PEAK_TFS,
+                         useful_frac=k["useful_flops"] / sec / 1e12 / FP64_MFMA_PEAK_TFS)
+        factor_kernels[name] = entry
+    log(f"[bench] factor kernels {json.dumps(factor_kernels)}")
 
     cpu = None
     if not args.skip_cpu and snaps is not None:
@@ -752,13 +768,17 @@ def main(argv=None):
         "precision": precision,
         "runs": runs,
         "kernels": kernels,
+        "factor_kernels": factor_kernels,
         "roofline": roofline,
         "cpu_baseline": cpu,
         "gpu_over_cpu": (value / cpu["value"]) if cpu else None,
     }
     # the metric holds only for a finite run with converged solves and the
-    # mixed kernel within 1e-5 of FP64: otherwise the line carries no value
-    failed = []
+    # mixed kernel within 1e-5 of FP64: otherwise the line carries no value;
+    # checks skipped on request (profiling runs) leave it without a value too
+    failed, unchecked = [], []
+    if total < sc.steps:
+        unchecked.append(f"run of {total} steps, the workload has {sc.steps}")
     for name, h in runs.items():
         if h["non_finite_steps"]:
             failed.append(f"{name}: {h['non_finite_steps']} non-finite step(s)")
@@ -767,11 +787,14 @@ def main(argv=None):
     if fp64_plain and (fp64_plain["non_finite_steps"] or fp64_plain["cg_capped"]):
         failed.append("fp64 plain: non-finite steps or capped solves")
     if rel_err is None:
-        failed.append("no FP64 run to check the force error against")
+        unchecked.append("no FP64 run to check the force error against (--skip-fp64)")
     elif not rel_err <= FORCE_BAR:
         failed.append(f"relative force error {rel_err} above {FORCE_BAR}")
-    if failed:
-        line["failed"] = failed
+    if failed or unchecked:
+        if failed:
+            line["failed"] = failed
+        if unchecked:
+            line["unchecked"] = unchecked
         line["value_if_valid"] = line["value"]
         line["value"] = None
     print(json.dumps(line), flush=True)
@@ -780,6 +803,8 @@ def main(argv=None):
     if failed:
         log("[bench] INVALID: " + "; ".join(failed))
         return 1
+    if unchecked:
+        log("[bench] no metric value: " + "; ".join(unchecked))
     return 0
 
 

@@ -322,6 +322,31 @@ typedef struct talus_mi_factor_info {
 /* Statistics of the current factor (zeros before the first factorization). */
 int talus_mi_factor_info_get(talus_mi_ctx *ctx, talus_mi_factor_info *info);
 
+/* Kernels of one factorization (SystemFactor::refresh, factor.cc:549-613,
+ * here the supernodal block Cholesky): element-pair assembly, extend-add,
+ * 64x64 diagonal Cholesky, panel TRSM, trailing SYRK/GEMM, L11^-1 and the
+ * packing of G = [L11^-1; L21 L11^-1]. */
+#define TALUS_MI_FK_ASSEMBLE 0u
+#define TALUS_MI_FK_EXTEND_ADD 1u
+#define TALUS_MI_FK_POTRF 2u
+#define TALUS_MI_FK_TRSM 3u
+#define TALUS_MI_FK_SYRK 4u
+#define TALUS_MI_FK_TRTRI 5u
+#define TALUS_MI_FK_PACK 6u
+typedef struct talus_mi_factor_kernel {
+  uint32_t kind;        /* TALUS_MI_FK_* */
+  uint32_t launches;
+  uint64_t workgroups;
+  double mfma_flops;    /* FP64 flops of the 64x64x64 MFMA tiles issued */
+  double useful_flops;  /* flops on the fronts' own entries (tiles clipped) */
+  double ms;            /* summed HIP-event time of the launches */
+} talus_mi_factor_kernel;
+/* Factors every body once (with the current step keys, or h = 1 and unit
+ * inertia before the first step) with a HIP event pair around every launch
+ * and returns one record per kernel kind, summed over its launches. */
+int talus_mi_factor_profile(talus_mi_ctx *ctx, talus_mi_factor_kernel *out, uint32_t capacity,
+                            uint32_t *count);
+
 /* ---- differentiation (no counterpart in the reference: the adjoint of
  * the implicit rollout, for gradient-based identification and design) ---- */
 

@@ -130,6 +130,15 @@ class FactorInfo(C.Structure):
                 ("factor_seconds", C.c_double)]
 
 
+class FactorKernel(C.Structure):
+    _fields_ = [("kind", C.c_uint32), ("launches", C.c_uint32), ("workgroups", C.c_uint64),
+                ("mfma_flops", C.c_double), ("useful_flops", C.c_double), ("ms", C.c_double)]
+
+
+FACTOR_KERNELS = {0: "assemble_pairs", 1: "extend_add", 2: "potrf", 3: "trsm", 4: "syrk", 5: "trtri",
+                  6: "pack"}
+
+
 class TuneReport(C.Structure):
     _fields_ = [("levels", C.c_uint32), ("lanes_forward", C.c_uint32),
                 ("lanes_backward", C.c_uint32), ("reserved", C.c_uint32),
@@ -228,6 +237,7 @@ _SIGNATURES = {
     "talus_mi_apply_system_noalloc": (C.c_int, [C.c_void_p, C.c_uint32, C.c_double, C.c_double,
                                                 C.c_void_p, C.c_void_p]),
     "talus_mi_time_kernel": (C.c_int, [C.c_void_p, C.c_uint32, C.c_uint32, _P(KernelTiming)]),
+    "talus_mi_factor_profile": (C.c_int, [C.c_void_p, _P(FactorKernel), C.c_uint32, _P(C.c_uint32)]),
     "talus_mi_factor_solve": (C.c_int, [C.c_void_p, C.c_uint32, C.c_double, C.c_double,
                                         _P(C.c_double), _P(C.c_double)]),
     "talus_mi_factor_info_get": (C.c_int, [C.c_void_p, _P(FactorInfo)]),
@@ -702,6 +712,16 @@ class Context:
         z = np.empty(3 * n)
         _check(lib().talus_mi_factor_solve(self._h, body, inertia, step, _dp(r), _dp(z)))
         return z.reshape(n, 3)
+
+    def factor_profile(self) -> dict:
+        """One factorization with HIP events around every launch: per kernel
+        kind its launches, workgroups, MFMA-issued and useful FP64 flops and
+        summed time (talus_mi_factor_profile)."""
+        out = (FactorKernel * 16)()
+        n = C.c_uint32()
+        _check(lib().talus_mi_factor_profile(self._h, out, 16, C.byref(n)))
+        return {FACTOR_KERNELS[out[i].kind]: {k: getattr(out[i], k) for k, _ in FactorKernel._fields_ if k != "kind"}
+                for i in range(n.value)}
 
     def factor_info(self) -> dict:
         i = FactorInfo()

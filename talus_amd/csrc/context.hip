@@ -312,6 +312,7 @@ class EngineBase {
   virtual void factor_solve(uint32_t b, double inertia, double step, const double *r,
                             double *z) = 0;
   virtual void factor_info(talus_mi_factor_info &info) = 0;
+  virtual void factor_profile(std::vector<talus_mi_factor_kernel> &out) = 0;
   virtual void tune(uint32_t repeats, talus_mi_tune_report &report) = 0;
   virtual void state_all_get(std::vector<double> &xva) = 0;
   virtual void state_all_set(const std::vector<double> &xva) = 0;
@@ -359,6 +360,7 @@ class Engine final : public EngineBase {
   void time_kernel(uint32_t kind, uint32_t repeats, talus_mi_kernel_timing &out) override;
   void factor_solve(uint32_t b, double inertia, double step, const double *r, double *z) override;
   void factor_info(talus_mi_factor_info &info) override;
+  void factor_profile(std::vector<talus_mi_factor_kernel> &out) override;
   void tune(uint32_t repeats, talus_mi_tune_report &report) override;
   void state_all_get(std::vector<double> &xva) override {
     const size_t n3 = 3 * size_t(d_.N);
@@ -435,8 +437,18 @@ class Engine final : public EngineBase {
   void all_active();
   // block Cholesky preconditioner
   void analyze_factor();
+  // per-launch HIP events of one factorization (factor_profile)
+  struct FactorProfile {
+    struct Launch {
+      uint32_t kind;
+      uint64_t workgroups;
+      double mfma, useful;
+      hipEvent_t a, b;
+    };
+    std::vector<Launch> launches;
+  };
   void factorize(const std::vector<double> &reach, const std::vector<double> &inertia,
-                 const std::vector<int> *mask = nullptr);
+                 const std::vector<int> *mask = nullptr, FactorProfile *prof = nullptr);
   void precondition(int parity);
   void forward_level(uint32_t l, int mode, int parity);
   void backward_level(uint32_t l, int mode, int parity);
@@ -831,7 +843,7 @@ void Engine<Real>::analyze_factor() {
 // inertia M + reach sum R K R^T with the committed rotations.
 template <class Real>
 void Engine<Real>::factorize(const std::vector<double> &reach, const std::vector<double> &inertia,
-                             const std::vector<int> *mask) {
+                             const std::vector<int> *mask, FactorProfile *prof) {
   if (!sym_ready_) analyze_factor();
   const std::vector<int> every(d_.B, 1);
   const std::vector<int> &fresh = mask ? *mask : every;
@@ -852,25 +864,107 @@ void Engine<Real>::factorize(const std::vector<double> &reach, const std::vector
     HIPCHECK(hipMemsetAsync(fd_.minv, 0, std::max<uint64_t>(t.minv_pool, 1) * sizeof(double), s));
   }
   HIPCHECK(hipMemsetAsync(fd_.failed, 0, d_.B * sizeof(int), s));
-  FactorLaunch<Real>::assemble(fd_, d_pair_target_, d_pair_ld_, d_pair_node_, d_pair_start_,
-                               d_contrib_, uint32_t(t.pair_target.size()), d_.dminv, d_.vol64,
-                               d_.lam64, d_.mu64, d_.R, d_.W, d_.plastic, d_.flags, d_.T, d_.mass,
-                               d_body_reach_,
-                               d_body_inertia_, d_pair_body_, s);
+  // every launch, with an event pair and its flops when profiled
+  constexpr double TILE3 = 2.0 * FNB * FNB * FNB;  // one 64x64x64 MFMA tile
+  auto dim = [](uint32_t total, uint32_t at) { return double(std::min<uint32_t>(FNB, total - at)); };
+  auto timed = [&](uint32_t kind, uint64_t workgroups, auto &&flops, auto &&launch) {
+    if (!prof || !workgroups) {
+      launch();
+      return;
+    }
+    typename FactorProfile::Launch L{kind, workgroups, 0.0, 0.0, nullptr, nullptr};
+    flops(L.mfma, L.useful);
+    HIPCHECK(hipEventCreate(&L.a));
+    HIPCHECK(hipEventCreate(&L.b));
+    HIPCHECK(hipEventRecord(L.a, s));
+    launch();
+    HIPCHECK(hipEventRecord(L.b, s));
+    prof->launches.push_back(L);
+  };
+  auto none = [](double &, double &) {};
+  timed(TALUS_MI_FK_ASSEMBLE, t.pair_target.size(), none, [&] {
+    FactorLaunch<Real>::assemble(fd_, d_pair_target_, d_pair_ld_, d_pair_node_, d_pair_start_, d_contrib_,
+                                 uint32_t(t.pair_target.size()), d_.dminv, d_.vol64, d_.lam64, d_.mu64, d_.R,
+                                 d_.W, d_.plastic, d_.flags, d_.T, d_.mass, d_body_reach_, d_body_inertia_,
+                                 d_pair_body_, s);
+  });
   size_t ea = 0, fac = 0;
   for (uint32_t l = 0; l < t.levels; ++l) {
     for (uint32_t r = 0; r < t.ea_rounds[l]; ++r, ++ea)
-      factor_extend_add(fd_, d_parent_, d_ea_ + t.ea_start[ea], t.ea_start[ea + 1] - t.ea_start[ea], s);
+      timed(TALUS_MI_FK_EXTEND_ADD, t.ea_start[ea + 1] - t.ea_start[ea], none, [&] {
+        factor_extend_add(fd_, d_parent_, d_ea_ + t.ea_start[ea], t.ea_start[ea + 1] - t.ea_start[ea], s);
+      });
     for (uint32_t p = 0; p < t.fac_panels[l]; ++p, ++fac) {
-      factor_potrf(fd_, d_potrf_ + t.potrf_start[fac], t.potrf_start[fac + 1] - t.potrf_start[fac], p, s);
-      factor_trsm(fd_, d_trsm_ + t.trsm_start[fac], t.trsm_start[fac + 1] - t.trsm_start[fac], p, s);
-      factor_syrk(fd_, d_syrk_ + t.syrk_start[fac], t.syrk_start[fac + 1] - t.syrk_start[fac], p, s);
+      const uint32_t P0 = p * FNB;
+      timed(
+          TALUS_MI_FK_POTRF, t.potrf_start[fac + 1] - t.potrf_start[fac],
+          [&](double &, double &useful) {
+            for (uint32_t q = t.potrf_start[fac]; q < t.potrf_start[fac + 1]; ++q) {
+              const double nb = dim(t.k[t.potrf_tasks[q]], P0);
+              useful += nb * nb * nb / 3.0;
+            }
+          },
+          [&] {
+            factor_potrf(fd_, d_potrf_ + t.potrf_start[fac], t.potrf_start[fac + 1] - t.potrf_start[fac], p, s);
+          });
+      timed(
+          TALUS_MI_FK_TRSM, t.trsm_start[fac + 1] - t.trsm_start[fac],
+          [&](double &mfma, double &useful) {
+            for (uint32_t q = t.trsm_start[fac]; q < t.trsm_start[fac + 1]; ++q) {
+              const uint64_t task = t.trsm_tasks[q];
+              const uint32_t fr = uint32_t(task), rt = uint32_t((task >> 32) & 0xffffu);
+              const double nb = dim(t.k[fr], P0);
+              const double nr = dim(t.m[fr], P0 + uint32_t(nb) + rt * FNB);
+              mfma += TILE3;
+              useful += 2.0 * nr * nb * nb;
+            }
+          },
+          [&] { factor_trsm(fd_, d_trsm_ + t.trsm_start[fac], t.trsm_start[fac + 1] - t.trsm_start[fac], p, s); });
+      timed(
+          TALUS_MI_FK_SYRK, t.syrk_start[fac + 1] - t.syrk_start[fac],
+          [&](double &mfma, double &useful) {
+            for (uint32_t q = t.syrk_start[fac]; q < t.syrk_start[fac + 1]; ++q) {
+              const uint64_t task = t.syrk_tasks[q];
+              const uint32_t fr = uint32_t(task), I = uint32_t((task >> 32) & 0xffffu), J = uint32_t(task >> 48);
+              const double nb = dim(t.k[fr], P0);
+              const uint32_t R = P0 + uint32_t(nb);
+              mfma += TILE3;
+              useful += 2.0 * dim(t.m[fr], R + I * FNB) * dim(t.m[fr], R + J * FNB) * nb;
+            }
+          },
+          [&] { factor_syrk(fd_, d_syrk_ + t.syrk_start[fac], t.syrk_start[fac + 1] - t.syrk_start[fac], p, s); });
     }
   }
   for (size_t i = 0; i + 1 < t.trtri_start.size(); ++i)
-    factor_trtri(fd_, d_trtri_ + t.trtri_start[i], t.trtri_start[i + 1] - t.trtri_start[i], s);
-  SolveLaunch<Real>::pack(fd_, d_w21_, uint32_t(t.w21_tasks.size()), d_pack_,
-                          uint32_t(t.pack_tasks.size()), d_gr_, d_gc_, s);
+    timed(
+        TALUS_MI_FK_TRTRI, t.trtri_start[i + 1] - t.trtri_start[i],
+        [&](double &mfma, double &useful) {
+          for (uint32_t q = t.trtri_start[i]; q < t.trtri_start[i + 1]; ++q) {
+            const uint64_t task = t.trtri_tasks[q];
+            const uint32_t fr = uint32_t(task), bi = uint32_t((task >> 32) & 0xffffu), bj = uint32_t(task >> 48);
+            const uint32_t k = t.k[fr];
+            const double ni = dim(k, bi * FNB), nj = dim(k, bj * FNB);
+            for (uint32_t c = bj; c < bi; ++c) useful += 2.0 * ni * dim(k, c * FNB) * nj;
+            useful += ni * ni * nj;  // the scalar L_ii^-1 product
+            mfma += double(bi - bj) * TILE3;
+          }
+        },
+        [&] { factor_trtri(fd_, d_trtri_ + t.trtri_start[i], t.trtri_start[i + 1] - t.trtri_start[i], s); });
+  timed(
+      TALUS_MI_FK_PACK, t.w21_tasks.size() + t.pack_tasks.size(),
+      [&](double &mfma, double &useful) {
+        for (const uint64_t task : t.w21_tasks) {
+          const uint32_t fr = uint32_t(task), I = uint32_t((task >> 32) & 0xffffu), J = uint32_t(task >> 48);
+          const uint32_t k = t.k[fr], u = t.m[fr] - k, kt = (k + FNB - 1) / FNB;
+          const double ni = dim(u, I * FNB), nj = dim(k, J * FNB);
+          for (uint32_t c = J; c < kt; ++c) useful += 2.0 * ni * dim(k, c * FNB) * nj;
+          mfma += double(kt - J) * TILE3;
+        }
+      },
+      [&] {
+        SolveLaunch<Real>::pack(fd_, d_w21_, uint32_t(t.w21_tasks.size()), d_pack_, uint32_t(t.pack_tasks.size()),
+                                d_gr_, d_gc_, s);
+      });
   HIPCHECK(hipMemcpyAsync(hfailed_, fd_.failed, d_.B * sizeof(int), hipMemcpyDeviceToHost, s));
   HIPCHECK(hipEventRecord(b, s));
   HIPCHECK(hipStreamSynchronize(s));
@@ -1913,6 +2007,35 @@ void Engine<Real>::rollout_backward(const talus_mi_step_params &sp,
 }
 
 template <class Real>
+void Engine<Real>::factor_profile(std::vector<talus_mi_factor_kernel> &out) {
+  const std::vector<double> ones(d_.B, 1.0);
+  const std::vector<double> reach = fac_reach_.size() == d_.B ? fac_reach_ : ones;
+  const std::vector<double> inertia = fac_inertia_.size() == d_.B ? fac_inertia_ : ones;
+  FactorProfile prof;
+  factorize(reach, inertia, nullptr, &prof);
+  out.clear();
+  for (const auto &L : prof.launches) {
+    float ms = 0.0f;
+    HIPCHECK(hipEventElapsedTime(&ms, L.a, L.b));
+    (void)hipEventDestroy(L.a);
+    (void)hipEventDestroy(L.b);
+    auto it = std::find_if(out.begin(), out.end(), [&](const talus_mi_factor_kernel &k) { return k.kind == L.kind; });
+    if (it == out.end()) {
+      talus_mi_factor_kernel k;
+      std::memset(&k, 0, sizeof k);
+      k.kind = L.kind;
+      out.push_back(k);
+      it = out.end() - 1;
+    }
+    ++it->launches;
+    it->workgroups += L.workgroups;
+    it->mfma_flops += L.mfma;
+    it->useful_flops += L.useful;
+    it->ms += double(ms);
+  }
+}
+
+template <class Real>
 void Engine<Real>::factor_info(talus_mi_factor_info &info) {
   std::memset(&info, 0, sizeof info);
   if (!sym_ready_) return;
@@ -2115,6 +2238,25 @@ class GroupedEngine final : public EngineBase {
     if (out.launches) {
       out.avg_ms = time / out.launches;
       out.bytes = bytes / out.launches;
+    }
+  }
+  void factor_profile(std::vector<talus_mi_factor_kernel> &out) override {
+    out.clear();
+    for (auto &e : eng_) {
+      std::vector<talus_mi_factor_kernel> one;
+      e->factor_profile(one);
+      for (const talus_mi_factor_kernel &k : one) {
+        auto it = std::find_if(out.begin(), out.end(), [&](const talus_mi_factor_kernel &o) { return o.kind == k.kind; });
+        if (it == out.end()) {
+          out.push_back(k);
+          continue;
+        }
+        it->launches += k.launches;
+        it->workgroups += k.workgroups;
+        it->mfma_flops += k.mfma_flops;
+        it->useful_flops += k.useful_flops;
+        it->ms += k.ms;
+      }
     }
   }
   void factor_info(talus_mi_factor_info &info) override {
@@ -2716,6 +2858,20 @@ int talus_mi_factor_solve(talus_mi_ctx *ctx, uint32_t body, double inertia, doub
   return guarded(ctx, [&]() -> int {
     ensure(ctx);
     ctx->engine->factor_solve(body, inertia, step, r, z);
+    return TALUS_MI_OK;
+  });
+}
+
+int talus_mi_factor_profile(talus_mi_ctx *ctx, talus_mi_factor_kernel *out, uint32_t capacity,
+                            uint32_t *count) {
+  if (!ctx || !count || (capacity && !out)) return fail(TALUS_MI_ERR_ARGUMENT, "bad arguments");
+  return guarded(ctx, [&]() -> int {
+    ensure(ctx);
+    if (ctx->bodies.empty()) return fail(TALUS_MI_ERR_STATE, "no bodies");
+    std::vector<talus_mi_factor_kernel> all;
+    ctx->engine->factor_profile(all);
+    *count = uint32_t(all.size());
+    for (size_t i = 0; i < all.size() && i < capacity; ++i) out[i] = all[i];
     return TALUS_MI_OK;
   });
 }
