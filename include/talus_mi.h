@@ -45,6 +45,10 @@ extern "C" {
 /* launch every PCG kernel directly instead of replaying the captured HIP
  * graph of one iteration (for comparison; results are identical) */
 #define TALUS_MI_DIRECT_LAUNCH 8u
+/* apply the block Cholesky in the frame it was factored in: no bodyRotation
+ * (factor.cc:615-646; by default the factor follows each body's best-fit
+ * rotation since its factorization, as the reference's) */
+#define TALUS_MI_FIXED_FRAME 32u
 
 /* elastic_model (ElasticModel, src/talus/material.h:8-11) */
 #define TALUS_MI_COROTATED 0u
@@ -318,6 +322,11 @@ typedef struct talus_mi_factor_info {
   double flops;            /* dense supernodal factorization flops */
   double factor_seconds;   /* device time of the last factorization */
 } talus_mi_factor_info;
+
+/* bodyRotation (factor.cc:615-646) of a body at its current positions: the
+ * polar factor of sum_n m_n (x_n - c)(r_n - c_r)^T with r the positions at
+ * its last factorization, row-major; the identity before a factorization. */
+int talus_mi_body_frame_get(talus_mi_ctx *ctx, uint32_t body, double *frame);
 
 /* Statistics of the current factor (zeros before the first factorization). */
 int talus_mi_factor_info_get(talus_mi_ctx *ctx, talus_mi_factor_info *info);

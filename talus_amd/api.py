@@ -28,6 +28,7 @@ RECORD_EVENTS = 2
 CONVERGE = 4
 DIRECT_LAUNCH = 8
 KERNEL_TIMING = 16
+FIXED_FRAME = 32
 EVENT_YIELD = 0
 EVENT_TEAR = 1
 COROTATED = 0
@@ -238,6 +239,7 @@ _SIGNATURES = {
                                                 C.c_void_p, C.c_void_p]),
     "talus_mi_time_kernel": (C.c_int, [C.c_void_p, C.c_uint32, C.c_uint32, _P(KernelTiming)]),
     "talus_mi_factor_profile": (C.c_int, [C.c_void_p, _P(FactorKernel), C.c_uint32, _P(C.c_uint32)]),
+    "talus_mi_body_frame_get": (C.c_int, [C.c_void_p, C.c_uint32, _P(C.c_double)]),
     "talus_mi_factor_solve": (C.c_int, [C.c_void_p, C.c_uint32, C.c_double, C.c_double,
                                         _P(C.c_double), _P(C.c_double)]),
     "talus_mi_factor_info_get": (C.c_int, [C.c_void_p, _P(FactorInfo)]),
@@ -712,6 +714,12 @@ class Context:
         z = np.empty(3 * n)
         _check(lib().talus_mi_factor_solve(self._h, body, inertia, step, _dp(r), _dp(z)))
         return z.reshape(n, 3)
+
+    def body_frame(self, body):
+        """bodyRotation of the body at its current positions (3x3)."""
+        R = np.empty(9)
+        _check(lib().talus_mi_body_frame_get(self._h, body, _dp(R)))
+        return R.reshape(3, 3)
 
     def factor_profile(self) -> dict:
         """One factorization with HIP events around every launch: per kernel

@@ -313,6 +313,7 @@ class EngineBase {
                             double *z) = 0;
   virtual void factor_info(talus_mi_factor_info &info) = 0;
   virtual void factor_profile(std::vector<talus_mi_factor_kernel> &out) = 0;
+  virtual void body_frame(uint32_t b, double *frame) = 0;
   virtual void tune(uint32_t repeats, talus_mi_tune_report &report) = 0;
   virtual void state_all_get(std::vector<double> &xva) = 0;
   virtual void state_all_set(const std::vector<double> &xva) = 0;
@@ -361,6 +362,7 @@ class Engine final : public EngineBase {
   void factor_solve(uint32_t b, double inertia, double step, const double *r, double *z) override;
   void factor_info(talus_mi_factor_info &info) override;
   void factor_profile(std::vector<talus_mi_factor_kernel> &out) override;
+  void body_frame(uint32_t b, double *frame) override;
   void tune(uint32_t repeats, talus_mi_tune_report &report) override;
   void state_all_get(std::vector<double> &xva) override {
     const size_t n3 = 3 * size_t(d_.N);
@@ -450,7 +452,7 @@ class Engine final : public EngineBase {
   void factorize(const std::vector<double> &reach, const std::vector<double> &inertia,
                  const std::vector<int> *mask = nullptr, FactorProfile *prof = nullptr);
   void precondition(int parity);
-  void forward_level(uint32_t l, int mode, int parity);
+  void forward_level(uint32_t l, int mode, int parity, const Real *rin = nullptr);
   void backward_level(uint32_t l, int mode, int parity);
   template <class T>
   T *upload(const std::vector<T> &h) { return dev_copy(h); }
@@ -480,6 +482,7 @@ class Engine final : public EngineBase {
   double *hnorms_ = nullptr, *d_norms_ = nullptr;  // corrections' convergence
   // plastic Newton: bodies with a yielding corotated tet, pinned staging of
   // its per-tile norms, loading counts and per-operation parameters
+  bool frames_ = false;  // the factor follows each body's rotation (bodyRotation)
   std::vector<uint8_t> newton_body_;
   double *hnnorm_ = nullptr;
   uint32_t *hloading_ = nullptr;
@@ -717,6 +720,15 @@ void Engine<Real>::build(const std::vector<Body> &bodies) {
     d_.lin = dev_copy(lin);
   }
   d_.loading = dev<uint32_t>(B);
+  d_.fref = dev_copy(x);
+  {
+    std::vector<double> eye(9 * size_t(B), 0.0);
+    for (uint32_t b = 0; b < B; ++b) eye[9 * b] = eye[9 * b + 4] = eye[9 * b + 8] = 1.0;
+    d_.frame = dev_copy(eye);
+  }
+  d_.fpart = dev<double>(9 * size_t(d_.n_ntiles));
+  d_.fcent = dev<double>(6 * size_t(B));
+  d_.rf = dev<Real>(3 * N);
   d_.lowest = dev<double>(3 * N);
   d_.nnorm = dev<double>(3 * size_t(d_.n_ntiles));
   d_.slot_f = dev<Real>(12 * T);
@@ -965,6 +977,8 @@ void Engine<Real>::factorize(const std::vector<double> &reach, const std::vector
         SolveLaunch<Real>::pack(fd_, d_w21_, uint32_t(t.w21_tasks.size()), d_pack_, uint32_t(t.pack_tasks.size()),
                                 d_gr_, d_gc_, s);
       });
+  // the refreshed bodies' positions become their factor's reference frame
+  Launch<Real>::frame_reference(d_, d_refresh_, s);
   HIPCHECK(hipMemcpyAsync(hfailed_, fd_.failed, d_.B * sizeof(int), hipMemcpyDeviceToHost, s));
   HIPCHECK(hipEventRecord(b, s));
   HIPCHECK(hipStreamSynchronize(s));
@@ -985,23 +999,24 @@ void Engine<Real>::factorize(const std::vector<double> &reach, const std::vector
 // row/column per lane, 2 64 rows/columns per workgroup split over its
 // wavefronts; the fronts too large for 1 and 2 always use 0).
 template <class Real>
-void Engine<Real>::forward_level(uint32_t l, int mode, int parity) {
+void Engine<Real>::forward_level(uint32_t l, int mode, int parity, const Real *rin) {
   const FactorTables &t = tab_;
+  if (!rin) rin = d_.r;
   if (mode == 1)
-    SolveLaunch<Real>::forward_lanes(fd_, d_gc_, d_fwl_ + t.fwl_start[l], t.fwl_start[l + 1] - t.fwl_start[l], d_.r,
+    SolveLaunch<Real>::forward_lanes(fd_, d_gc_, d_fwl_ + t.fwl_start[l], t.fwl_start[l + 1] - t.fwl_start[l], rin,
                                      d_.done, parity, d_.B, stream_);
   else if (mode == 2)
-    SolveLaunch<Real>::forward_split(fd_, d_gc_, d_fws_ + t.fws_start[l], t.fws_start[l + 1] - t.fws_start[l], d_.r,
+    SolveLaunch<Real>::forward_split(fd_, d_gc_, d_fws_ + t.fws_start[l], t.fws_start[l + 1] - t.fws_start[l], rin,
                                      d_.done, parity, d_.B, stream_);
   if (mode) {
     SolveLaunch<Real>::gather(fd_, d_gat_big_ + t.gat_big_start[l], t.gat_big_start[l + 1] - t.gat_big_start[l],
-                              d_.r, d_.done, parity, d_.B, stream_);
-    SolveLaunch<Real>::forward(fd_, d_gr_, d_fwr_ + t.fwr_start[l], t.fwr_start[l + 1] - t.fwr_start[l], d_.r,
+                              rin, d_.done, parity, d_.B, stream_);
+    SolveLaunch<Real>::forward(fd_, d_gr_, d_fwr_ + t.fwr_start[l], t.fwr_start[l + 1] - t.fwr_start[l], rin,
                                d_.done, parity, d_.B, stream_);
   } else {
     SolveLaunch<Real>::gather(fd_, d_gat_all_ + t.gat_all_start[l], t.gat_all_start[l + 1] - t.gat_all_start[l],
-                              d_.r, d_.done, parity, d_.B, stream_);
-    SolveLaunch<Real>::forward(fd_, d_gr_, d_fwd_ + t.fwd_start[l], t.fwd_start[l + 1] - t.fwd_start[l], d_.r,
+                              rin, d_.done, parity, d_.B, stream_);
+    SolveLaunch<Real>::forward(fd_, d_gr_, d_fwd_ + t.fwd_start[l], t.fwd_start[l + 1] - t.fwd_start[l], rin,
                                d_.done, parity, d_.B, stream_);
   }
 }
@@ -1028,8 +1043,8 @@ void Engine<Real>::backward_level(uint32_t l, int mode, int parity) {
   }
 }
 
-// z = (LL^T)^-1 r for the bodies still iterating (applyFactor,
-// factor.cc:648-704, without the body-frame rotation): per level from the
+// z = F (LL^T)^-1 F^T r for the bodies still iterating (applyFactor,
+// factor.cc:648-704, F the body frame, or the identity): per level from the
 // leaves, gather the right-hand side and the children's updates, then one
 // GEMV with G; per level from the root, one transposed GEMV with G against
 // [y; -x2], x2 read from the ancestors' solution at the front's update
@@ -1039,9 +1054,13 @@ void Engine<Real>::precondition(int parity) {
   hipStream_t s = stream_;
   const FactorTables &t = tab_;
   const uint32_t B = d_.B;
-  for (uint32_t l = 0; l < t.levels; ++l) forward_level(l, plan_[2 * l], parity);
+  // with bodyRotation the factor solves F^T r and the result turns by F
+  if (frames_) Launch<Real>::frame_in(d_, parity, s);
+  const Real *rin = frames_ ? d_.rf : d_.r;
+  for (uint32_t l = 0; l < t.levels; ++l) forward_level(l, plan_[2 * l], parity, rin);
   for (uint32_t l = t.levels; l-- > 0;) backward_level(l, plan_[2 * l + 1], parity);
-  SolveLaunch<Real>::finish(fd_, d_.z, d_.N, d_.anchored, d_node_body_, d_.done, parity, B, s);
+  SolveLaunch<Real>::finish(fd_, d_.z, d_.N, d_.anchored, d_node_body_, d_.done, parity, B,
+                            frames_ ? d_.frame : nullptr, s);
 }
 
 // Autotuner: times both kernel shapes of every level's forward and backward
@@ -1077,6 +1096,7 @@ void Engine<Real>::tune(uint32_t repeats, talus_mi_tune_report &rep) {
   };
   const FactorTables &t = tab_;
   rep.levels = t.levels;
+  frames_ = false;  // the solve kernels alone
   rep.before_seconds = time([&] { precondition(-1); });
   for (uint32_t l = 0; l < t.levels; ++l) {
     for (int dir = 0; dir < 2; ++dir) {
@@ -1485,6 +1505,9 @@ void Engine<Real>::step(const talus_mi_step_params &sp, talus_mi_report &rep,
     use_factor_ = factor && fac_ok_;
     const bool diag = !use_factor_;
     upload_params(params, 1 + pass);
+    // bodyRotation at the substep's start positions (solver.cc:3811-3813)
+    frames_ = use_factor_ && !(sp.flags & TALUS_MI_FIXED_FRAME);
+    if (frames_) Launch<Real>::frames(d_, s);
     double *record = nullptr;
     if (recorder_) {
       // x_n, v_n, a_n now, a' after the pass (adjoint replay)
@@ -1896,6 +1919,7 @@ void Engine<Real>::factor_solve(uint32_t b, double inertia, double step, const d
   hipStream_t s = stream_;
   std::vector<double> reach(d_.B, step * step), inert(d_.B, inertia);
   factorize(reach, inert);
+  frames_ = false;  // (LL^T)^-1 r of the factor just built
   const size_t n0 = n0_[b], nn = nn_[b];
   std::vector<Real> in(3 * size_t(d_.N), Real(0));
   for (size_t i = 0; i < 3 * nn; ++i) in[3 * n0 + i] = Real(r[i]);
@@ -2004,6 +2028,14 @@ void Engine<Real>::rollout_backward(const talus_mi_step_params &sp,
     throw;
   }
   for (void *p : mine) (void)hipFree(p);
+}
+
+template <class Real>
+void Engine<Real>::body_frame(uint32_t b, double *frame) {
+  all_active();
+  Launch<Real>::frames(d_, stream_);
+  HIPCHECK(hipMemcpyAsync(frame, d_.frame + 9 * size_t(b), 9 * sizeof(double), hipMemcpyDeviceToHost, stream_));
+  HIPCHECK(hipStreamSynchronize(stream_));
 }
 
 template <class Real>
@@ -2240,6 +2272,7 @@ class GroupedEngine final : public EngineBase {
       out.bytes = bytes / out.launches;
     }
   }
+  void body_frame(uint32_t b, double *frame) override { eng_[group_[b]]->body_frame(local_[b], frame); }
   void factor_profile(std::vector<talus_mi_factor_kernel> &out) override {
     out.clear();
     for (auto &e : eng_) {
@@ -2858,6 +2891,15 @@ int talus_mi_factor_solve(talus_mi_ctx *ctx, uint32_t body, double inertia, doub
   return guarded(ctx, [&]() -> int {
     ensure(ctx);
     ctx->engine->factor_solve(body, inertia, step, r, z);
+    return TALUS_MI_OK;
+  });
+}
+
+int talus_mi_body_frame_get(talus_mi_ctx *ctx, uint32_t body, double *frame) {
+  if (!ctx || !body_ok(ctx, body) || !frame) return fail(TALUS_MI_ERR_ARGUMENT, "bad arguments");
+  return guarded(ctx, [&]() -> int {
+    ensure(ctx);
+    ctx->engine->body_frame(body, frame);
     return TALUS_MI_OK;
   });
 }

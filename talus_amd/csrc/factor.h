@@ -228,9 +228,10 @@ struct SolveLaunch {
   static void backward_split(const FactorDev &f, const G *gr, const uint64_t *tasks,
                              uint32_t count, const int *done, int parity, uint32_t B,
                              hipStream_t s);
+  // z = F x (F per body, 9 doubles row-major, or the identity when null)
   static void finish(const FactorDev &f, Real *z, uint32_t N, const uint8_t *anchored,
                      const uint32_t *node_body, const int *done, int parity, uint32_t B,
-                     hipStream_t s);
+                     const double *frame, hipStream_t s);
 };
 
 }  // namespace tmi

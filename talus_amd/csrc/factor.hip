@@ -855,15 +855,28 @@ __global__ __launch_bounds__(NT) void k_solve_backward(FactorDev f, const typena
 }
 
 template <class Real>
+// z = F x for the free nodes (F the body frame of bodyRotation, or the
+// identity when frame is null), 0 for anchored ones
 __global__ __launch_bounds__(NT) void k_solve_finish(FactorDev f, Real *z, uint32_t N,
                                                      const uint8_t *anchored,
                                                      const uint32_t *node_body, const int *done,
-                                                     int parity, uint32_t B) {
+                                                     int parity, uint32_t B, const double *frame) {
   const uint32_t n = blockIdx.x * NT + threadIdx.x;
   if (n >= N || skip(done, parity, B, node_body[n])) return;
+  const double *x = f.xsol + 3 * size_t(n);
+  if (anchored[n]) {
 #pragma unroll
-  for (int c = 0; c < 3; ++c)
-    z[3 * size_t(n) + c] = anchored[n] ? Real(0) : Real(f.xsol[3 * size_t(n) + c]);
+    for (int c = 0; c < 3; ++c) z[3 * size_t(n) + c] = Real(0);
+    return;
+  }
+  if (!frame) {
+#pragma unroll
+    for (int c = 0; c < 3; ++c) z[3 * size_t(n) + c] = Real(x[c]);
+    return;
+  }
+  const double *F = frame + 9 * size_t(node_body[n]);
+#pragma unroll
+  for (int c = 0; c < 3; ++c) z[3 * size_t(n) + c] = Real(F[3 * c] * x[0] + F[3 * c + 1] * x[1] + F[3 * c + 2] * x[2]);
 }
 
 }  // namespace
@@ -966,8 +979,10 @@ void SolveLaunch<Real>::backward(const FactorDev &f, const G *gc, const uint64_t
 template <class Real>
 void SolveLaunch<Real>::finish(const FactorDev &f, Real *z, uint32_t N, const uint8_t *anchored,
                                const uint32_t *node_body, const int *done, int parity, uint32_t B,
-                               hipStream_t s) {
-  if (N) hipLaunchKernelGGL(k_solve_finish<Real>, dim3((N + NT - 1) / NT), dim3(NT), 0, s, f, z, N, anchored, node_body, done, parity, B);
+                               const double *frame, hipStream_t s) {
+  if (N)
+    hipLaunchKernelGGL(k_solve_finish<Real>, dim3((N + NT - 1) / NT), dim3(NT), 0, s, f, z, N, anchored, node_body,
+                       done, parity, B, frame);
 }
 
 template struct FactorLaunch<float>;

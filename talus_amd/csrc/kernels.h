@@ -84,6 +84,10 @@ struct Dev {
   uint32_t *loading;  // [B] tets loading past the margin in the last assembly
   double *lowest;   // 3N, the Newton's lowest-residual iterate
   double *nnorm;    // 3 per node tile: sum |right|^2, max |dx|, max |x_eval|
+  // bodyRotation: positions at each body's last factorization, the body
+  // frame F (9 per body), per node tile moments (9) and per body centroids (6)
+  double *fref, *frame, *fpart, *fcent;
+  Real *rf;         // F^T r, the preconditioner's input in the factor's frame
   Real *slot_f, *slot_d, *energy;
   uint8_t *yflag, *tflag;
   // reductions
@@ -157,6 +161,13 @@ struct Launch {
   // plastic Newton: per node tile |right|^2, max node |dx|, max node
   // |x_eval| (d.nnorm); and the iterate updates of NewtonVec
   static void newton_norms(const Dev<Real> &d, hipStream_t s);
+  // bodyRotation (factor.cc:615-646): the positions of the refreshed bodies
+  // become their factor's reference; the frame of every active body is the
+  // polar factor of its mass-weighted correlation of positions now and at
+  // the factorization; F^T r for the bodies still iterating
+  static void frame_reference(const Dev<Real> &d, const int *refresh, hipStream_t s);
+  static void frames(const Dev<Real> &d, hipStream_t s);
+  static void frame_in(const Dev<Real> &d, int parity, hipStream_t s);
   static void newton_vec(const Dev<Real> &d, int mode, hipStream_t s);
   // adjoint of a converged substep (neo-Hookean Hessian)
   static void hvp(const Dev<Real> &d, const Real *dir, int parity, hipStream_t s);

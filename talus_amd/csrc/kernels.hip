@@ -1115,6 +1115,98 @@ __global__ __launch_bounds__(TILE) void k_newton_norms(Dev<Real> d) {
   }
 }
 
+// bodyRotation (factor.cc:615-646).  The reference positions of the bodies
+// whose factor was rebuilt are their positions now.
+template <class Real>
+__global__ __launch_bounds__(TILE) void k_frame_reference(Dev<Real> d, const int *refresh) {
+  const uint32_t t = blockIdx.x, b = d.nt_body[t];
+  if (!refresh[b]) return;
+  const uint32_t n = d.nt_begin[t] + threadIdx.x;
+  if (n >= d.nt_end[t]) return;
+#pragma unroll
+  for (int c = 0; c < 3; ++c) d.fref[3 * (size_t)n + c] = d.x[3 * (size_t)n + c];
+}
+
+// Per node tile of an active body: mass, mass-weighted positions now and at
+// the factorization (7 sums; mode 0), or the mass-weighted correlation of
+// the centred positions, sum m (x - c)(r - c_r)^T (9 sums; mode 1).
+template <class Real>
+__global__ __launch_bounds__(TILE) void k_frame_moments(Dev<Real> d, int mode) {
+  __shared__ double lds[TILE / 64];
+  const uint32_t t = blockIdx.x, b = d.nt_body[t];
+  if (!d.params[b].active) return;
+  const uint32_t n = d.nt_begin[t] + threadIdx.x;
+  double v[9] = {0, 0, 0, 0, 0, 0, 0, 0, 0};
+  if (n < d.nt_end[t]) {
+    const double m = d.mass[n];
+    const double *x = d.x + 3 * (size_t)n, *r = d.fref + 3 * (size_t)n;
+    if (mode == 0) {
+      v[0] = m;
+#pragma unroll
+      for (int c = 0; c < 3; ++c) {
+        v[1 + c] = m * x[c];
+        v[4 + c] = m * r[c];
+      }
+    } else {
+      const double *cen = d.fcent + 6 * (size_t)b;
+#pragma unroll
+      for (int i = 0; i < 3; ++i)
+#pragma unroll
+        for (int j = 0; j < 3; ++j) v[3 * i + j] = m * (x[i] - cen[i]) * (r[j] - cen[3 + j]);
+    }
+  }
+  const int count = mode == 0 ? 7 : 9;
+  for (int k = 0; k < count; ++k) {
+    const double total = block_sum(v[k], lds);
+    if (threadIdx.x == 0) d.fpart[9 * (size_t)t + k] = total;
+    __syncthreads();
+  }
+}
+
+// One block per active body: its tiles' sums in tile order, then the
+// centroids (mode 0) or the frame, polarRotation of the correlation
+// (mode 1; identity when the correlation is singular, as the reference).
+template <class Real>
+__global__ __launch_bounds__(TILE) void k_frame_body(Dev<Real> d, int mode) {
+  __shared__ double lds[TILE / 64];
+  const uint32_t b = blockIdx.x;
+  if (!d.params[b].active) return;
+  const uint32_t t0 = d.body_nt0[b], t1 = d.body_nt1[b];
+  double sum[9];
+  const int count = mode == 0 ? 7 : 9;
+  for (int k = 0; k < count; ++k) {
+    double s = 0.0;
+    for (uint32_t t = t0 + threadIdx.x; t < t1; t += blockDim.x) s += d.fpart[9 * (size_t)t + k];
+    sum[k] = block_sum(s, lds);
+    __syncthreads();
+  }
+  if (threadIdx.x != 0) return;
+  if (mode == 0) {
+    const double m = sum[0] > 0.0 ? sum[0] : 1.0;
+#pragma unroll
+    for (int k = 0; k < 6; ++k) d.fcent[6 * (size_t)b + k] = sum[1 + k] / m;
+  } else {
+    double R[9];
+    polar(sum, R);
+#pragma unroll
+    for (int k = 0; k < 9; ++k) d.frame[9 * (size_t)b + k] = R[k];
+  }
+}
+
+// F^T r into rf for the nodes of bodies still iterating
+template <class Real>
+__global__ __launch_bounds__(TILE) void k_frame_in(Dev<Real> d, int parity) {
+  const uint32_t t = blockIdx.x, b = d.nt_body[t];
+  if (!d.params[b].active || (parity >= 0 && d.done[parity * d.B + b])) return;
+  const uint32_t n = d.nt_begin[t] + threadIdx.x;
+  if (n >= d.nt_end[t]) return;
+  const double *F = d.frame + 9 * (size_t)b;
+  const Real *r = d.r + 3 * (size_t)n;
+  const double r0 = double(r[0]), r1 = double(r[1]), r2 = double(r[2]);
+#pragma unroll
+  for (int c = 0; c < 3; ++c) d.rf[3 * (size_t)n + c] = Real(F[c] * r0 + F[3 + c] * r1 + F[6 + c] * r2);
+}
+
 // Iterate updates of the plastic Newton on active bodies (NewtonVec).
 template <class Real>
 __global__ __launch_bounds__(TILE) void k_newton_vec(Dev<Real> d, int mode) {
@@ -1483,6 +1575,21 @@ void Launch<Real>::increment_norms(const Dev<Real> &d, double *out, hipStream_t 
 template <class Real>
 void Launch<Real>::newton_norms(const Dev<Real> &d, hipStream_t s) {
   TMI_LAUNCH(k_newton_norms<Real>, d.n_ntiles, s, d);
+}
+template <class Real>
+void Launch<Real>::frame_reference(const Dev<Real> &d, const int *refresh, hipStream_t s) {
+  TMI_LAUNCH(k_frame_reference<Real>, d.n_ntiles, s, d, refresh);
+}
+template <class Real>
+void Launch<Real>::frames(const Dev<Real> &d, hipStream_t s) {
+  for (int mode = 0; mode < 2; ++mode) {
+    TMI_LAUNCH(k_frame_moments<Real>, d.n_ntiles, s, d, mode);
+    TMI_LAUNCH(k_frame_body<Real>, d.B, s, d, mode);
+  }
+}
+template <class Real>
+void Launch<Real>::frame_in(const Dev<Real> &d, int parity, hipStream_t s) {
+  TMI_LAUNCH(k_frame_in<Real>, d.n_ntiles, s, d, parity);
 }
 template <class Real>
 void Launch<Real>::newton_vec(const Dev<Real> &d, int mode, hipStream_t s) {
