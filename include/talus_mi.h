@@ -271,6 +271,14 @@ int talus_mi_eval_forces(talus_mi_ctx *ctx, uint32_t body, const double *x,
  * solver.cc:904-1048) with the committed rotations. */
 int talus_mi_apply_system(talus_mi_ctx *ctx, uint32_t body, double inertia,
                           double step, const double *d, double *y);
+/* The operator with the initial-stress (rotation-derivative) term of the
+ * static Newton (scratch.stressed, solver.cc:2623; initialStress and
+ * applyInitialStress, material.cc:1077-1174): at positions x of the body
+ * (which become its evaluation state: rotations and forces), y = inertia*M*d
+ * + step^2 sum_e (R K R^T + initialStress_e) d over its corotated tets
+ * (neo-Hookean tets add their operator only), anchored rows zero. */
+int talus_mi_apply_system_stressed(talus_mi_ctx *ctx, uint32_t body, const double *x, double inertia,
+                                   double step, const double *d, double *y);
 /* Size of the allocation-free path's per-GPU scratch array (MiB). */
 #define TALUS_MI_NOALLOC_SCRATCH_MIB 48
 /* talus_mi_apply_system without any allocation.  No counterpart in the

@@ -127,6 +127,7 @@ def lib():
         L.orc_j2_point.restype = C.c_int
         L.orc_j2_tangent.argtypes = [dp, C.c_double, C.c_double, dp, dp]
         L.orc_tangent_apply.argtypes = [C.c_void_p, dp, dp, dp]
+        L.orc_stressed_apply.argtypes = [C.c_void_p, dp, dp, dp]
         L.orc_tangent_apply.restype = C.c_uint32
         L.orc_eval_forces.argtypes = [C.c_void_p, dp, C.c_uint32, dp, dp, dp, dp]
         L.orc_apply_system.argtypes = [C.c_void_p, C.c_double, C.c_double, dp, dp, C.c_uint32]
@@ -414,6 +415,14 @@ class Body:
         y = np.empty_like(d)
         n = lib().orc_tangent_apply(self._h, _dp(x), _dp(d), _dp(y))
         return y.reshape(-1, 3), int(n)
+
+    def stressed_apply(self, x, d):
+        """(R K R^T + initialStress) d at positions x (h = 1, no inertia)."""
+        x = np.ascontiguousarray(x, dtype=np.float64).reshape(-1)
+        d = np.ascontiguousarray(d, dtype=np.float64).reshape(-1)
+        y = np.empty_like(d)
+        lib().orc_stressed_apply(self._h, _dp(x), _dp(d), _dp(y))
+        return y.reshape(-1, 3)
 
     def yield_flags(self):
         f = np.empty(self.tets, dtype=np.uint8)

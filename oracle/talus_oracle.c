@@ -1547,6 +1547,131 @@ static double substep(orc_body *b, const orc_settings *s, const rule_t *rule, do
   return elastic;
 }
 
+/* ---- initialStress / applyInitialStress (material.cc:1060-1174) ---- */
+
+static void cross_matrix(const double *v, double *m) {
+  m[0] = 0.0; m[1] = -v[2]; m[2] = v[1];
+  m[3] = v[2]; m[4] = 0.0; m[5] = -v[0];
+  m[6] = -v[1]; m[7] = v[0]; m[8] = 0.0;
+}
+
+/* rotationDerivative for a tet (material.cc:1077-1118, spreadOf 690-709):
+ * twist_i = R (sum_j K_ij [o_j]x) R^T + handed [f_i]x and
+ * turn_i = spin [handed R g_i]x with spin = R (tr S I - S)^-1 R^T,
+ * S = sym(R^T F); zero when F or tr S I - S is singular.  K_ij are the
+ * elastic stiffness blocks (block, material.cc:80-90), o_j the corotated
+ * offsets, f_i the element's corner forces.  twist, turn: 4 x 9 each. */
+static int initial_stress_tet(const double *x, const double *X, const double *dminv, double V,
+                              double lam, double mu, const double *R, const double *f,
+                              double *twist, double *turn) {
+  double g[4][3];
+  shape_gradients(dminv, g);
+  memset(twist, 0, 36 * sizeof(double));
+  memset(turn, 0, 36 * sizeof(double));
+  double F[9] = {0};
+  for (int c = 0; c < 4; ++c)
+    for (int r = 0; r < 3; ++r)
+      for (int k = 0; k < 3; ++k) F[r * 3 + k] += x[3 * c + r] * g[c][k];
+  if (fabs(orc_det3(F)) < 1.0e-12) return 0;
+  double RtF[9], S[9], spread[9], inv[9];
+  mul3_tn(R, F, RtF);
+  for (int r = 0; r < 3; ++r)
+    for (int k = 0; k < 3; ++k) S[r * 3 + k] = 0.5 * (RtF[r * 3 + k] + RtF[k * 3 + r]);
+  const double tr = trace3(S);
+  for (int i = 0; i < 9; ++i) spread[i] = (i % 4 == 0 ? tr : 0.0) - S[i];
+  if (!(orc_det3(spread) > 0.0) || !inv3(spread, inv)) return 0;
+  double Ri[9], spin[9];
+  mul3(R, inv, Ri);
+  mul3_nt(Ri, R, spin);
+  const double handed = orc_det3(R) < 0.0 ? -1.0 : 1.0;
+  double xc[3] = {0, 0, 0}, Xc[3] = {0, 0, 0};
+  for (int c = 0; c < 4; ++c)
+    for (int r = 0; r < 3; ++r) {
+      xc[r] += 0.25 * x[3 * c + r];
+      Xc[r] += 0.25 * X[3 * c + r];
+    }
+  double o[4][3];
+  for (int c = 0; c < 4; ++c) {
+    double arm[3];
+    for (int r = 0; r < 3; ++r) arm[r] = x[3 * c + r] - xc[r];
+    mtv3(R, arm, o[c]);
+    for (int r = 0; r < 3; ++r) o[c][r] -= X[3 * c + r] - Xc[r];
+  }
+  for (int i = 0; i < 4; ++i) {
+    double tw[9] = {0};
+    for (int j = 0; j < 4; ++j) {
+      double K[9], C[9], KC[9];
+      const double gg = g[i][0] * g[j][0] + g[i][1] * g[j][1] + g[i][2] * g[j][2];
+      for (int r = 0; r < 3; ++r)
+        for (int k = 0; k < 3; ++k)
+          K[r * 3 + k] = V * (lam * g[i][r] * g[j][k] + mu * g[j][r] * g[i][k] + (r == k ? mu * gg : 0.0));
+      cross_matrix(o[j], C);
+      mul3(K, C, KC);
+      for (int k = 0; k < 9; ++k) tw[k] += KC[k];
+    }
+    double Rt[9], RtR[9], Cf[9];
+    mul3(R, tw, Rt);
+    mul3_nt(Rt, R, RtR);
+    cross_matrix(&f[3 * i], Cf);
+    for (int k = 0; k < 9; ++k) twist[9 * i + k] = RtR[k] + handed * Cf[k];
+    double rg[3], Cg[9];
+    mv3(R, g[i], rg);
+    for (int r = 0; r < 3; ++r) rg[r] *= handed;
+    cross_matrix(rg, Cg);
+    mul3(spin, Cg, &turn[9 * i]);
+  }
+  return 1;
+}
+
+/* applyInitialStress (material.cc:1157-1174): turned = sum_k turn_k d_k,
+ * twisted = sum_k twist_k^T d_k, y_i += (twist_i turned + turn_i^T twisted)/2 */
+static void apply_initial_stress(const double *twist, const double *turn, const double *d, double *y) {
+  double turned[3] = {0, 0, 0}, twisted[3] = {0, 0, 0};
+  for (int k = 0; k < 4; ++k) {
+    double a[3], b[3];
+    mv3(&turn[9 * k], &d[3 * k], a);
+    mtv3(&twist[9 * k], &d[3 * k], b);
+    for (int r = 0; r < 3; ++r) {
+      turned[r] += a[r];
+      twisted[r] += b[r];
+    }
+  }
+  for (int i = 0; i < 4; ++i) {
+    double a[3], b[3];
+    mv3(&twist[9 * i], turned, a);
+    mtv3(&turn[9 * i], twisted, b);
+    for (int r = 0; r < 3; ++r) y[3 * i + r] += 0.5 * (a[r] + b[r]);
+  }
+}
+
+/* The system operator with the initial-stress term of the static Newton
+ * (scratch.stressed, solver.cc:2623; applySystem 947-948): at positions x,
+ * y = sum_e (R K R^T + initialStress) d over corotated tets (the elastic
+ * stiffness, the element forces of the trial from the committed plastic
+ * state), h = 1, no inertia; neo-Hookean tets add their operator only. */
+void orc_stressed_apply(orc_body *b, const double *x, const double *d, double *y) {
+  double *force = (double *)malloc(12 * (size_t)b->m.tets * sizeof(double) + 8);
+  orc_eval_forces(b, x, 1, force, NULL, NULL, NULL);
+  orc_apply_system(b, 0.0, 1.0, d, y, 1);
+  for (size_t e = 0; e < b->m.tets; ++e) {
+    if (b->snh[e]) continue;
+    double xe[12], Xe[12], de[12], tw[36], tu[36], ye[12] = {0};
+    gather4(b, x, e, xe);
+    gather4(b, b->m.rest, e, Xe);
+    gather4(b, d, e, de);
+    if (!initial_stress_tet(xe, Xe, &b->m.dminv[9 * e], b->m.volume[e], b->lambda[e], b->mu[e],
+                            &b->R[9 * e], &force[12 * e], tw, tu))
+      continue;
+    apply_initial_stress(tw, tu, de, ye);
+    for (int c = 0; c < 4; ++c) {
+      const uint32_t n = b->m.tet[4 * e + c];
+      if (b->m.anchored[n]) continue;
+      for (int r = 0; r < 3; ++r) y[3 * (size_t)n + r] += ye[3 * c + r];
+    }
+  }
+  free(force);
+}
+
 /* LOADING_MARGIN, PLASTIC_REDUCTION, PLASTIC_RESOLUTION (solver.cc:372-380) */
 #define LOADING_MARGIN 1.0e-3
 #define PLASTIC_REDUCTION 1.0e-2

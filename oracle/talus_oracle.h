@@ -158,6 +158,9 @@ void orc_j2_tangent(const double *lin11, double lambda, double mu, const double 
 /* The consistent-tangent operator K d of a Newton attempt at positions x
  * (h = 1, no inertia); returns the loading tets. */
 uint32_t orc_tangent_apply(orc_body *body, const double *x, const double *d, double *y);
+/* The operator with the initial-stress (rotation-derivative) term at x:
+ * y = sum over corotated tets of (R K R^T + initialStress) d, h = 1. */
+void orc_stressed_apply(orc_body *body, const double *x, const double *d, double *y);
 /* Yield flags of the last step (1 per tet). */
 void orc_yield_flags(const orc_body *body, uint8_t *flags);
 

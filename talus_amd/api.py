@@ -235,6 +235,8 @@ _SIGNATURES = {
                                        _P(C.c_double), _P(C.c_double), _P(C.c_double)]),
     "talus_mi_apply_system": (C.c_int, [C.c_void_p, C.c_uint32, C.c_double, C.c_double,
                                         _P(C.c_double), _P(C.c_double)]),
+    "talus_mi_apply_system_stressed": (C.c_int, [C.c_void_p, C.c_uint32, _P(C.c_double), C.c_double, C.c_double,
+                                                 _P(C.c_double), _P(C.c_double)]),
     "talus_mi_apply_system_noalloc": (C.c_int, [C.c_void_p, C.c_uint32, C.c_double, C.c_double,
                                                 C.c_void_p, C.c_void_p]),
     "talus_mi_time_kernel": (C.c_int, [C.c_void_p, C.c_uint32, C.c_uint32, _P(KernelTiming)]),
@@ -691,6 +693,15 @@ class Context:
         d = _f64(d, 3 * n)
         y = np.empty(3 * n)
         _check(lib().talus_mi_apply_system(self._h, body, inertia, step, _dp(d), _dp(y)))
+        return y.reshape(n, 3)
+
+    def apply_system_stressed(self, body, x, inertia, step, d):
+        """The operator with the initial-stress term at positions x
+        (talus_mi_apply_system_stressed)."""
+        n, _ = self.info(body)
+        x, d = _f64(x, 3 * n), _f64(d, 3 * n)
+        y = np.empty(3 * n)
+        _check(lib().talus_mi_apply_system_stressed(self._h, body, _dp(x), inertia, step, _dp(d), _dp(y)))
         return y.reshape(n, 3)
 
     def apply_system_noalloc(self, body, inertia, step, d_ptr: int, y_ptr: int):

@@ -308,6 +308,8 @@ class EngineBase {
   // only (no allocation, no host copy, no wait) on stream s
   virtual void apply_system_device(uint32_t b, double inertia, double step, const double *dv,
                                    double *y, void *slots, hipStream_t s) = 0;
+  virtual void apply_system_stressed(uint32_t b, const double *x, double inertia, double step,
+                                     const double *dv, double *y) = 0;
   virtual void time_kernel(uint32_t kind, uint32_t repeats, talus_mi_kernel_timing &out) = 0;
   virtual void factor_solve(uint32_t b, double inertia, double step, const double *r,
                             double *z) = 0;
@@ -358,6 +360,8 @@ class Engine final : public EngineBase {
     Launch<Real>::apply_body(d_, t0_[b], t0_[b] + tt_[b], n0_[b], n0_[b] + nn_[b], inc_start_[n0_[b]], dv, y,
                              inertia, step * step, static_cast<Real *>(slots), s);
   }
+  void apply_system_stressed(uint32_t b, const double *x, double inertia, double step, const double *dv,
+                             double *y) override;
   void time_kernel(uint32_t kind, uint32_t repeats, talus_mi_kernel_timing &out) override;
   void factor_solve(uint32_t b, double inertia, double step, const double *r, double *z) override;
   void factor_info(talus_mi_factor_info &info) override;
@@ -474,7 +478,7 @@ class Engine final : public EngineBase {
   Real *hvec_ = nullptr;  // 3N + 12T staging of Real
   double *scratch64_ = nullptr;
   bool any_plastic_ = false, any_tear_ = false;
-  uint64_t n_snh_ = 0, n_plastic_ = 0;
+  uint64_t n_snh_ = 0, n_plastic_ = 0, n_flowing_ = 0;  // flowing: plastic neo-Hookean
   uint32_t applications_ = 0;  // preconditioner applications this step
   // host copies of the incidence CSR: node ranges and the CSR position of
   // every (tet, corner) slot code 4e+c
@@ -621,7 +625,7 @@ void Engine<Real>::build(const std::vector<Body> &bodies) {
   std::vector<Real> vol(T), lam(T), mu(T), R(9 * T), W(9 * T);
   std::vector<uint8_t> flags(T);
   any_plastic_ = any_tear_ = false;
-  n_snh_ = n_plastic_ = 0;
+  n_snh_ = n_plastic_ = n_flowing_ = 0;
   newton_body_.assign(B, 0);
   for (uint32_t b = 0; b < B; ++b) {
     const Body &bd = bodies[b];
@@ -660,6 +664,7 @@ void Engine<Real>::build(const std::vector<Body> &bodies) {
       if ((bd.flags[e] & FLAG_PLASTIC) && !(bd.flags[e] & FLAG_SNH)) newton_body_[b] = 1;
       n_snh_ += (bd.flags[e] & FLAG_SNH) ? 1u : 0u;
       n_plastic_ += (bd.flags[e] & FLAG_PLASTIC) ? 1u : 0u;
+      n_flowing_ += ((bd.flags[e] & FLAG_SNH) && (bd.flags[e] & FLAG_PLASTIC)) ? 1u : 0u;
       any_tear_ = any_tear_ || (bd.flags[e] & FLAG_TEAR);
     }
   }
@@ -749,7 +754,7 @@ void Engine<Real>::build(const std::vector<Body> &bodies) {
   HIPCHECK(hipMemset(d_.yflag, 0, std::max<size_t>(T, 1)));
   HIPCHECK(hipMemset(d_.tflag, 0, std::max<size_t>(T, 1)));
   HIPCHECK(hipMemset(d_.epartial, 0, std::max<size_t>(d_.n_ttiles, 1) * sizeof(double)));
-  scratch64_ = dev<double>(6 * N);  // two node vectors (eval_forces, apply_system)
+  scratch64_ = dev<double>(9 * N);  // three node vectors (eval_forces, apply_system)
   hparams_slots_ = 8;
   hparams_ = pinned<StepParams>(hparams_slots_ * B);
   hflags_ = pinned<int>(B);
@@ -1785,6 +1790,27 @@ void Engine<Real>::apply_system(uint32_t b, double inertia, double step, const d
   HIPCHECK(hipStreamSynchronize(s));
 }
 
+// the operator with the initial-stress term at positions x of body b: the
+// element pass at x (rotations, corner forces), then the stressed operator
+template <class Real>
+void Engine<Real>::apply_system_stressed(uint32_t b, const double *x, double inertia, double step,
+                                         const double *dv, double *y) {
+  hipStream_t s = stream_;
+  const size_t n0 = n0_[b], nn = nn_[b], N3 = 3 * size_t(d_.N);
+  double *pos = scratch64_, *din = scratch64_ + N3, *yout = scratch64_ + 2 * N3;
+  HIPCHECK(hipMemcpyAsync(pos, d_.x, N3 * sizeof(double), hipMemcpyDeviceToDevice, s));
+  HIPCHECK(hipMemcpyAsync(pos + 3 * n0, x, 3 * nn * sizeof(double), hipMemcpyHostToDevice, s));
+  HIPCHECK(hipMemcpyAsync(din, dv, 3 * nn * sizeof(double), hipMemcpyHostToDevice, s));
+  all_active();
+  Launch<Real>::displacement(d_, pos, s);
+  Launch<Real>::assemble(d_, false, false, s);
+  Launch<Real>::apply_body_stressed(d_, t0_[b], t0_[b] + tt_[b], uint32_t(n0), uint32_t(n0 + nn), inc_start_[n0],
+                                    pos + 3 * n0, din, yout, inertia, step * step,
+                                    d_.slot_f + 3 * size_t(inc_start_[n0]), s);
+  HIPCHECK(hipMemcpyAsync(y, yout, 3 * nn * sizeof(double), hipMemcpyDeviceToHost, s));
+  HIPCHECK(hipStreamSynchronize(s));
+}
+
 template <class Real>
 void Engine<Real>::time_kernel(uint32_t kind, uint32_t repeats, talus_mi_kernel_timing &out) {
   std::memset(&out, 0, sizeof out);
@@ -1819,8 +1845,10 @@ void Engine<Real>::time_kernel(uint32_t kind, uint32_t repeats, talus_mi_kernel_
     const double snh = double(n_snh_), plastic = double(n_plastic_);
     if (kind == 0)
       out.bytes = T * (tet + 9.0 * sz + 12.0 * sz) + snh * 9.0 * sz + plastic * (80.0 + 16.0) + N * 3.0 * sz;
-    else if (kind == 1)  // the operator reads Dm^-1 in the element precision
-      out.bytes = T * (tet - 72.0 + 9.0 * sz + 9.0 * sz + 12.0 * sz) + snh * 9.0 * sz + N * 3.0 * sz;
+    else if (kind == 1)  // the operator reads Dm^-1 in the element precision, and the FP64
+                         // plastic offsets Fp^-1 - I of plastic neo-Hookean tets
+      out.bytes = T * (tet - 72.0 + 9.0 * sz + 9.0 * sz + 12.0 * sz) + snh * 9.0 * sz + double(n_flowing_) * 72.0 +
+                  N * 3.0 * sz;
     else
       out.bytes = T * 12.0 * sz + (N + 1.0) * 4.0 + N * (6.0 * sz + 8.0 + 1.0);
   } else if (kind <= 8) {
@@ -2251,6 +2279,10 @@ class GroupedEngine final : public EngineBase {
   void apply_system_device(uint32_t b, double inertia, double step, const double *dv, double *y, void *slots,
                            hipStream_t s) override {
     eng_[group_[b]]->apply_system_device(local_[b], inertia, step, dv, y, slots, s);
+  }
+  void apply_system_stressed(uint32_t b, const double *x, double inertia, double step, const double *dv,
+                             double *y) override {
+    eng_[group_[b]]->apply_system_stressed(local_[b], x, inertia, step, dv, y);
   }
   void factor_solve(uint32_t b, double inertia, double step, const double *r, double *z) override {
     eng_[group_[b]]->factor_solve(local_[b], inertia, step, r, z);
@@ -2849,6 +2881,16 @@ int talus_mi_apply_system(talus_mi_ctx *ctx, uint32_t body, double inertia, doub
   return guarded(ctx, [&]() -> int {
     ensure(ctx);
     ctx->engine->apply_system(body, inertia, step, d, y);
+    return TALUS_MI_OK;
+  });
+}
+
+int talus_mi_apply_system_stressed(talus_mi_ctx *ctx, uint32_t body, const double *x, double inertia,
+                                   double step, const double *d, double *y) {
+  if (!ctx || !body_ok(ctx, body) || !x || !d || !y) return fail(TALUS_MI_ERR_ARGUMENT, "bad arguments");
+  return guarded(ctx, [&]() -> int {
+    ensure(ctx);
+    ctx->engine->apply_system_stressed(body, x, inertia, step, d, y);
     return TALUS_MI_OK;
   });
 }

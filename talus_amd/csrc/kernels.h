@@ -155,6 +155,11 @@ struct Launch {
   static void apply_body(const Dev<Real> &d, uint32_t t0, uint32_t t1, uint32_t n0, uint32_t n1,
                          uint32_t k0, const double *din, double *yout, double inertia, double reach,
                          Real *slots, hipStream_t s);
+  // the same with the initial-stress term of corotated tets at positions xin
+  // (after an assembly at xin, whose corner forces the slots hold)
+  static void apply_body_stressed(const Dev<Real> &d, uint32_t t0, uint32_t t1, uint32_t n0, uint32_t n1,
+                                  uint32_t k0, const double *xin, const double *din, double *yout,
+                                  double inertia, double reach, Real *slots, hipStream_t s);
   static void gather_forces(const Dev<Real> &d, Real *out, hipStream_t s);
   // convergence of the corrections: per node tile max |dx| and max |a|
   static void increment_norms(const Dev<Real> &d, double *out, hipStream_t s);

@@ -1058,6 +1058,57 @@ __global__ __launch_bounds__(TILE) void k_apply_body_gather(Dev<Real> d, uint32_
   for (int c = 0; c < 3; ++c) yout[3 * l + c] = double(q[c]);
 }
 
+// The operator with the initial-stress term (the static Newton's
+// scratch.stressed, solver.cc:947-948) of one body at positions xin (FP64,
+// body-local), after k_assemble at those positions: each tet reads its
+// corner forces from its slots, adds initialStress d (FP64) to the
+// operator's R K R^T d for corotated tets, and writes the result back to
+// the same slots.
+template <class Real>
+__global__ __launch_bounds__(TILE) void k_stressed_element(Dev<Real> d, uint32_t t0, uint32_t t1, uint32_t n0,
+                                                           uint32_t k0, const double *xin, const double *din,
+                                                           Real *slots) {
+  const uint32_t e = t0 + blockIdx.x * TILE + threadIdx.x;
+  if (e >= t1) return;
+  const uint4 c = d.conn[e];
+  const uint32_t nn[4] = {c.x - n0, c.y - n0, c.z - n0, c.w - n0};
+  const uint32_t gn[4] = {c.x, c.y, c.z, c.w};
+  Real dv[12], y[12];
+#pragma unroll
+  for (int i = 0; i < 4; ++i)
+#pragma unroll
+    for (int r = 0; r < 3; ++r) dv[3 * i + r] = Real(din[3 * (size_t)nn[i] + r]);
+  op_tet_values(d, e, dv, y);
+  const uint4 pos = d.slot_pos[e];
+  const uint4 at = make_uint4(pos.x - k0, pos.y - k0, pos.z - k0, pos.w - k0);
+  if (!(d.flags[e] & FLAG_SNH)) {
+    const uint32_t ps[4] = {at.x, at.y, at.z, at.w};
+    double x[12], X[12], f[12], dd[12], dm[9], g[12], R[9], tw[36], tu[36], add[12];
+#pragma unroll
+    for (int i = 0; i < 4; ++i)
+#pragma unroll
+      for (int r = 0; r < 3; ++r) {
+        x[3 * i + r] = xin[3 * (size_t)nn[i] + r];
+        X[3 * i + r] = d.X[3 * (size_t)gn[i] + r];
+        f[3 * i + r] = double(slots[3 * (size_t)ps[i] + r]);
+        dd[3 * i + r] = din[3 * (size_t)nn[i] + r];
+        add[3 * i + r] = 0.0;
+      }
+#pragma unroll
+    for (int k = 0; k < 9; ++k) {
+      dm[k] = d.dminv[(size_t)k * d.T + e];
+      R[k] = double(d.R[(size_t)k * d.T + e]);
+    }
+    shape_gradients(dm, g);
+    if (initial_stress_tet(x, X, g, d.vol64[e], d.lam64[e], d.mu64[e], R, f, tw, tu)) {
+      apply_initial_stress(tw, tu, dd, add);
+#pragma unroll
+      for (int k = 0; k < 12; ++k) y[k] += Real(add[k]);
+    }
+  }
+  store_slots(slots, at, y);
+}
+
 inline unsigned blocks_for(uint32_t n) { return (n + TILE - 1) / TILE; }
 
 // ------------------------------------------------- corrections converge
@@ -1630,6 +1681,17 @@ void Launch<Real>::apply_body(const Dev<Real> &d, uint32_t t0, uint32_t t1, uint
                               uint32_t k0, const double *din, double *yout, double inertia,
                               double reach, Real *slots, hipStream_t s) {
   if (t1 > t0) TMI_LAUNCH(k_apply_body_element<Real>, blocks_for(t1 - t0), s, d, t0, t1, n0, k0, din, slots);
+  if (n1 > n0)
+    TMI_LAUNCH(k_apply_body_gather<Real>, blocks_for(n1 - n0), s, d, n0, n1, k0, slots, din, yout, inertia,
+               reach);
+}
+
+template <class Real>
+void Launch<Real>::apply_body_stressed(const Dev<Real> &d, uint32_t t0, uint32_t t1, uint32_t n0, uint32_t n1,
+                                       uint32_t k0, const double *xin, const double *din, double *yout,
+                                       double inertia, double reach, Real *slots, hipStream_t s) {
+  if (t1 > t0)
+    TMI_LAUNCH(k_stressed_element<Real>, blocks_for(t1 - t0), s, d, t0, t1, n0, k0, xin, din, slots);
   if (n1 > n0)
     TMI_LAUNCH(k_apply_body_gather<Real>, blocks_for(n1 - n0), s, d, n0, n1, k0, slots, din, yout, inertia,
                reach);

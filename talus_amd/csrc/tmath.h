@@ -392,6 +392,103 @@ TM_HD void j2_tangent_stress(const R *lin, R lambda, R mu, const R *e, R *out) {
     out[i] = shear * (e[i] - ((i % 4 == 0) ? tr / R(3) : R(0))) + ((i % 4 == 0) ? bulk * tr : R(0)) - flow * n[i];
 }
 
+// [v]x, row-major: [v]x w = v x w
+TM_HD void cross_matrix(const double *v, double *m) {
+  m[0] = 0.0;
+  m[1] = -v[2];
+  m[2] = v[1];
+  m[3] = v[2];
+  m[4] = 0.0;
+  m[5] = -v[0];
+  m[6] = -v[1];
+  m[7] = v[0];
+  m[8] = 0.0;
+}
+
+// initialStress of a corotated tet (rotationDerivative, material.cc:
+// 1077-1118, spreadOf 690-709), FP64: x, X corner positions now and at rest
+// (12 each), g the shape gradients, R the element rotation, f its corner
+// forces; twist_i = R (sum_j K_ij [o_j]x) R^T + handed [f_i]x,
+// turn_i = spin [handed R g_i]x, spin = R (tr S I - S)^-1 R^T,
+// S = sym(R^T F).  Returns false (no term) when F or tr S I - S is singular.
+TM_HD bool initial_stress_tet(const double *x, const double *X, const double *g, double V, double lam,
+                              double mu, const double *R, const double *f, double *twist, double *turn) {
+  double F[9] = {0, 0, 0, 0, 0, 0, 0, 0, 0};
+  for (int c = 0; c < 4; ++c)
+    for (int r = 0; r < 3; ++r)
+      for (int k = 0; k < 3; ++k) F[r * 3 + k] += x[3 * c + r] * g[3 * c + k];
+  if (fabs(det3(F)) < 1.0e-12) return false;
+  double RtF[9], spread[9], inv[9];
+  mul3_tn(R, F, RtF);
+  const double tr = RtF[0] + RtF[4] + RtF[8];
+  for (int r = 0; r < 3; ++r)
+    for (int k = 0; k < 3; ++k)
+      spread[r * 3 + k] = (r == k ? tr : 0.0) - 0.5 * (RtF[r * 3 + k] + RtF[k * 3 + r]);
+  if (!(det3(spread) > 0.0) || !inv3(spread, inv)) return false;
+  double Ri[9], spin[9];
+  mul3(R, inv, Ri);
+  mul3_nt(Ri, R, spin);
+  const double handed = det3(R) < 0.0 ? -1.0 : 1.0;
+  double xc[3] = {0, 0, 0}, Xc[3] = {0, 0, 0};
+  for (int c = 0; c < 4; ++c)
+    for (int r = 0; r < 3; ++r) {
+      xc[r] += 0.25 * x[3 * c + r];
+      Xc[r] += 0.25 * X[3 * c + r];
+    }
+  double o[12];
+  for (int c = 0; c < 4; ++c) {
+    double arm[3];
+    for (int r = 0; r < 3; ++r) arm[r] = x[3 * c + r] - xc[r];
+    mtv3(R, arm, &o[3 * c]);
+    for (int r = 0; r < 3; ++r) o[3 * c + r] -= X[3 * c + r] - Xc[r];
+  }
+  for (int i = 0; i < 4; ++i) {
+    double tw[9] = {0, 0, 0, 0, 0, 0, 0, 0, 0};
+    const double *gi = &g[3 * i];
+    for (int j = 0; j < 4; ++j) {
+      const double *gj = &g[3 * j];
+      const double gg = gi[0] * gj[0] + gi[1] * gj[1] + gi[2] * gj[2];
+      double K[9], C[9], KC[9];
+      for (int r = 0; r < 3; ++r)
+        for (int k = 0; k < 3; ++k) K[r * 3 + k] = V * (lam * gi[r] * gj[k] + mu * gj[r] * gi[k] + (r == k ? mu * gg : 0.0));
+      cross_matrix(&o[3 * j], C);
+      mul3(K, C, KC);
+      for (int k = 0; k < 9; ++k) tw[k] += KC[k];
+    }
+    double Rt[9], RtR[9], Cf[9], rg[3], Cg[9];
+    mul3(R, tw, Rt);
+    mul3_nt(Rt, R, RtR);
+    cross_matrix(&f[3 * i], Cf);
+    for (int k = 0; k < 9; ++k) twist[9 * i + k] = RtR[k] + handed * Cf[k];
+    mv3(R, gi, rg);
+    for (int r = 0; r < 3; ++r) rg[r] *= handed;
+    cross_matrix(rg, Cg);
+    mul3(spin, Cg, &turn[9 * i]);
+  }
+  return true;
+}
+
+// applyInitialStress (material.cc:1157-1174): y_i += (twist_i turned +
+// turn_i^T twisted) / 2 with turned = sum turn_k d_k, twisted = sum twist_k^T d_k
+TM_HD void apply_initial_stress(const double *twist, const double *turn, const double *d, double *y) {
+  double turned[3] = {0, 0, 0}, twisted[3] = {0, 0, 0};
+  for (int k = 0; k < 4; ++k) {
+    double a[3], b[3];
+    mv3(&turn[9 * k], &d[3 * k], a);
+    mtv3(&twist[9 * k], &d[3 * k], b);
+    for (int r = 0; r < 3; ++r) {
+      turned[r] += a[r];
+      twisted[r] += b[r];
+    }
+  }
+  for (int i = 0; i < 4; ++i) {
+    double a[3], b[3];
+    mv3(&twist[9 * i], turned, a);
+    mtv3(&turn[9 * i], twisted, b);
+    for (int r = 0; r < 3; ++r) y[3 * i + r] += 0.5 * (a[r] + b[r]);
+  }
+}
+
 // Hencky return (plasticflow.cc returnMap): returns true and the returned
 // elastic gradient when the trial stress exceeds `yield`.
 TM_HD bool hencky_return(const double *elastic, double mu, double yield, double *returned,

@@ -147,3 +147,45 @@ def test_newton_steps_a_yielding_cantilever_without_capping():
                 totals[k] = totals.get(k, 0) + v
     assert totals["yield_events"] > 0
     assert totals["plastic_capped"] == 0 and totals["plastic_rejections"] == 0
+
+
+def _corotated_strip():
+    mats = scenes.material_array(1, youngs_modulus=1e6, poissons_ratio=0.3, density=1000.0,
+                                 elastic_model=api.COROTATED)
+    return scenes.Body(resolution=(4, 2, 2), voxel=(0.1, 0.05, 0.05), materials=mats)
+
+
+def test_initial_stress_is_the_symmetric_rotation_derivative_of_the_forces():
+    """initialStress (material.cc:1077-1174): the corotated forces f depend on
+    x through the rotation too; -df/dx = R K R^T + A, and the initial-stress
+    operator is the symmetric part of A (central differences of the forces,
+    minus the operator with the rotation held).  It vanishes at rest and
+    under a pure rotation (no force, no offset)."""
+    from helpers import deformed
+
+    orc, m = oracle_body(_corotated_strip())
+    X = m["rest"]
+    x = deformed(X, seed=3, scale=0.02, twist=0.7)
+    rng = np.random.default_rng(2)
+    d1, d2 = rng.standard_normal((2,) + X.shape)
+    h = 1e-6
+
+    def jd(d):
+        return -(orc.eval_forces(x + h * d)[2] - orc.eval_forces(x - h * d)[2]) / (2 * h)
+
+    j1, j2 = jd(d1), jd(d2)
+    orc.eval_forces(x)
+    k1, k2 = orc.apply_system(0.0, 1.0, d1), orc.apply_system(0.0, 1.0, d2)
+    expect = 0.5 * ((d2 * (j1 - k1)).sum() + (d1 * (j2 - k2)).sum())
+    s1 = orc.stressed_apply(x, d1) - k1
+    s2 = orc.stressed_apply(x, d2) - k2
+    got = (d2 * s1).sum()
+    assert abs(got - (d1 * s2).sum()) <= 1e-9 * abs(got)  # symmetric
+    assert abs(got - expect) <= 1e-4 * abs(expect), (got, expect)
+    assert abs(expect) > 1e-4 * abs((d2 * k1).sum())  # the term is not negligible here
+    c = X.mean(axis=0)
+    Rz = np.array([[np.cos(0.9), -np.sin(0.9), 0.0], [np.sin(0.9), np.cos(0.9), 0.0], [0.0, 0.0, 1.0]])
+    for pos in (X, (X - c) @ Rz.T + c):
+        orc.eval_forces(pos)
+        k = orc.apply_system(0.0, 1.0, d1)
+        assert np.abs(orc.stressed_apply(pos, d1) - k).max() <= 1e-9 * np.abs(k).max()
