@@ -128,6 +128,7 @@ def lib():
         L.orc_j2_tangent.argtypes = [dp, C.c_double, C.c_double, dp, dp]
         L.orc_tangent_apply.argtypes = [C.c_void_p, dp, dp, dp]
         L.orc_stressed_apply.argtypes = [C.c_void_p, dp, dp, dp]
+        L.orc_damped_apply.argtypes = [C.c_void_p, dp, C.c_double, C.c_double, C.c_double, dp, dp]
         L.orc_tangent_apply.restype = C.c_uint32
         L.orc_eval_forces.argtypes = [C.c_void_p, dp, C.c_uint32, dp, dp, dp, dp]
         L.orc_apply_system.argtypes = [C.c_void_p, C.c_double, C.c_double, dp, dp, C.c_uint32]
@@ -422,6 +423,14 @@ class Body:
         d = np.ascontiguousarray(d, dtype=np.float64).reshape(-1)
         y = np.empty_like(d)
         lib().orc_stressed_apply(self._h, _dp(x), _dp(d), _dp(y))
+        return y.reshape(-1, 3)
+
+    def damped_apply(self, x, inertia, step, damp, d):
+        """inertia M d + step^2 K d + damp sum tau D d at positions x."""
+        x = np.ascontiguousarray(x, dtype=np.float64).reshape(-1)
+        d = np.ascontiguousarray(d, dtype=np.float64).reshape(-1)
+        y = np.empty_like(d)
+        lib().orc_damped_apply(self._h, _dp(x), inertia, step, damp, _dp(d), _dp(y))
         return y.reshape(-1, 3)
 
     def yield_flags(self):

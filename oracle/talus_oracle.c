@@ -866,6 +866,13 @@ struct orc_body {
   double *lin;
   uint8_t *loading;
   double *total, *lowest_it;
+  /* stiffness damping (factor.cc:755-855, solver.cc:625-902): per tet tau
+   * and rigid fit (arms 12, inverse inertia 9) at the last assembly, the
+   * operator's damping coefficient (1 - alpha_f) gamma h, per-slot damping
+   * values and diagonal blocks */
+  int damped;
+  double *tau, *fit, *slot_v, *slot_dd;
+  double damp, lag;
   uint32_t *inc_start, *inc_slot;
   /* scratch */
   double *slot_f, *slot_d, *tet_energy;
@@ -937,6 +944,16 @@ orc_body *orc_body_create(const orc_mesh *mesh, const orc_material *materials,
   b->total = (double *)calloc(3 * N + 1, sizeof(double));
   b->lowest_it = (double *)calloc(3 * N + 1, sizeof(double));
   b->return_mapping = 1;
+  b->tau = (double *)calloc(T + 1, sizeof(double));
+  for (size_t e = 0; e < T; ++e) {
+    const orc_material *mt = lane_of(materials, lanes, mesh->lane[e], &fb);
+    const double sd = (double)mt->stiffness_damping;
+    b->tau[e] = isfinite(sd) && sd > 0.0 ? sd : 0.0; /* stiffnessDamping, factor.cc:755-759 */
+    if (b->tau[e] > 0.0) b->damped = 1;
+  }
+  b->fit = (double *)calloc(21 * T + 1, sizeof(double));
+  b->slot_v = (double *)calloc(12 * T + 1, sizeof(double));
+  b->slot_dd = (double *)calloc(36 * T + 1, sizeof(double));
   for (size_t e = 0; e < T; ++e) {
     b->lin[11 * e] = 1.0;
     if (b->plastic_lane[e] && !b->snh[e]) b->newton = 1;
@@ -986,7 +1003,7 @@ void orc_body_free(orc_body *b) {
                   b->weighted, b->shifted, b->carried, b->drive, b->evaluated,
                   b->residual, b->direction, b->applied, b->conditioned,
                   b->increment, b->unconstrained, b->partial, b->ckpt, b->final_state,
-                  b->lin, b->loading, b->total, b->lowest_it};
+                  b->lin, b->loading, b->total, b->lowest_it, b->tau, b->fit, b->slot_v, b->slot_dd};
   for (size_t i = 0; i < sizeof ptrs / sizeof ptrs[0]; ++i) free(ptrs[i]);
   free(b);
 }
@@ -1062,6 +1079,90 @@ static void volume_frame(const double *F, const double *ep, double *W) {
   mul3_nt(C, Fi, W);
 }
 
+/* ---- stiffness damping (factor.cc:755-855; solver.cc:625-902) ---- */
+
+/* pseudoInverse (factor.cc:183-272) of a symmetric 3x3: eigenvalues above
+ * `floor` inverted, the others dropped */
+static void pseudo_inverse_sym3(const double *A, double floor, double *out) {
+  double w[3], V[9];
+  orc_eigen_sym3(A, w, V);
+  memset(out, 0, 9 * sizeof(double));
+  for (int k = 0; k < 3; ++k) {
+    if (!(w[k] > floor)) continue;
+    for (int r = 0; r < 3; ++r)
+      for (int c = 0; c < 3; ++c) out[r * 3 + c] += V[r * 3 + k] * V[c * 3 + k] / w[k];
+  }
+}
+
+/* rigidFit (factor.cc:783-817) of a tet's corners: arms from the centre and
+ * the inverse of sum (|a|^2 I - a a^T), a pseudo-inverse when nearly
+ * singular.  fit = {arms[12], inverse[9]} */
+static void rigid_fit(const double *corners, double *fit) {
+  double c[3] = {0, 0, 0};
+  for (int i = 0; i < 4; ++i)
+    for (int r = 0; r < 3; ++r) c[r] += corners[3 * i + r] / 4.0;
+  double I[9] = {0};
+  for (int i = 0; i < 4; ++i) {
+    double *a = &fit[3 * i];
+    for (int r = 0; r < 3; ++r) a[r] = corners[3 * i + r] - c[r];
+    const double aa = a[0] * a[0] + a[1] * a[1] + a[2] * a[2];
+    for (int r = 0; r < 3; ++r)
+      for (int k = 0; k < 3; ++k) I[r * 3 + k] += (r == k ? aa : 0.0) - a[r] * a[k];
+  }
+  const double third = (I[0] + I[4] + I[8]) / 3.0;
+  if (!(orc_det3(I) > 1.0e-6 * third * third * third) || !inv3(I, &fit[12]))
+    pseudo_inverse_sym3(I, 1.0e-6 * third, &fit[12]);
+}
+
+/* removeRigid (factor.cc:819-846): v_i -= mean + spin x arm_i,
+ * spin = inverse * sum arm_i x v_i */
+static void remove_rigid(const double *fit, double *v) {
+  double mean[3] = {0, 0, 0}, moment[3] = {0, 0, 0}, spin[3];
+  for (int i = 0; i < 4; ++i) {
+    const double *a = &fit[3 * i], *x = &v[3 * i];
+    for (int r = 0; r < 3; ++r) mean[r] += x[r] / 4.0;
+    moment[0] += a[1] * x[2] - a[2] * x[1];
+    moment[1] += a[2] * x[0] - a[0] * x[2];
+    moment[2] += a[0] * x[1] - a[1] * x[0];
+  }
+  mv3(&fit[12], moment, spin);
+  for (int i = 0; i < 4; ++i) {
+    const double *a = &fit[3 * i];
+    double *x = &v[3 * i];
+    const double sa[3] = {spin[1] * a[2] - spin[2] * a[1], spin[2] * a[0] - spin[0] * a[2],
+                          spin[0] * a[1] - spin[1] * a[0]};
+    for (int r = 0; r < 3; ++r) x[r] -= mean[r] + sa[r];
+  }
+}
+
+/* applyDamping (factor.cc:848-857): out = P R K R^T P v with P the rigid-
+ * mode removal of the fit and K the tet's elastic stiffness */
+static void damp_tet(const orc_body *b, size_t e, const double *v, double *out) {
+  const double *fit = &b->fit[21 * e], *R = &b->R[9 * e];
+  double rel[12], g[4][3], D[9] = {0};
+  memcpy(rel, v, sizeof rel);
+  remove_rigid(fit, rel);
+  shape_gradients(&b->m.dminv[9 * e], g);
+  for (int j = 0; j < 4; ++j) {
+    double dp[3];
+    mtv3(R, &rel[3 * j], dp);
+    for (int r = 0; r < 3; ++r)
+      for (int k = 0; k < 3; ++k) D[r * 3 + k] += dp[r] * g[j][k];
+  }
+  const double tr = trace3(D), V = b->m.volume[e];
+  double sg[9];
+  for (int r = 0; r < 3; ++r)
+    for (int k = 0; k < 3; ++k)
+      sg[r * 3 + k] = b->mu[e] * (D[r * 3 + k] + D[k * 3 + r]) + (r == k ? b->lambda[e] * tr : 0.0);
+  for (int i = 0; i < 4; ++i) {
+    double t[3];
+    mv3(sg, g[i], t);
+    mv3(R, t, &out[3 * i]);
+    for (int r = 0; r < 3; ++r) out[3 * i + r] *= V;
+  }
+  remove_rigid(fit, out);
+}
+
 /* Shape gradients of the plastic reference, Fp^-T g = (I + ep)^T g, for a
  * plastic neo-Hookean tet; the rest gradients otherwise. */
 static void plastic_gradients(const orc_body *b, size_t e, double g[4][3], double gs[4][3]) {
@@ -1131,6 +1232,19 @@ static void element_one(orc_body *b, const double *at, size_t e, int diag) {
   }
   memcpy(&b->R[9 * e], R, sizeof R);
   b->tet_energy[e] = energy;
+  if (b->tau[e] > 0.0) {
+    /* fitDamping at the evaluation point (solver.cc:681-699); the damping
+     * diagonal blocks of blockDamping / addDampingDiagonal (804-869) */
+    rigid_fit(x, &b->fit[21 * e]);
+    if (diag)
+      for (int i = 0; i < 4; ++i)
+        for (int axis = 0; axis < 3; ++axis) {
+          double unit[12] = {0}, col[12];
+          unit[3 * i + axis] = 1.0;
+          damp_tet(b, e, unit, col);
+          for (int r = 0; r < 3; ++r) b->slot_dd[36 * e + 9 * i + r * 3 + axis] = b->tau[e] * col[3 * i + r];
+        }
+  }
   if (diag) {
     double g[4][3];
     shape_gradients(dminv, g);
@@ -1191,7 +1305,8 @@ static void gather_range(void *p, size_t begin, size_t end) {
       const uint32_t s = b->inc_slot[k];
       for (int r = 0; r < 3; ++r) f[r] += b->slot_f[3 * (size_t)s + r];
       if (c->diag)
-        for (int i = 0; i < 9; ++i) d[i] += c->scale * b->slot_d[9 * (size_t)s + i];
+        for (int i = 0; i < 9; ++i)
+          d[i] += c->scale * b->slot_d[9 * (size_t)s + i] + (b->damped ? b->damp * b->slot_dd[9 * (size_t)s + i] : 0.0);
     }
     memcpy(&b->force[3 * n], f, sizeof f);
     if (c->diag) memcpy(&b->diag[9 * n], d, sizeof d);
@@ -1289,6 +1404,11 @@ static void op_element_range(void *p, size_t begin, size_t end) {
       if (snh) mv3(&b->W[9 * e], g[i], w);
       for (int r = 0; r < 3; ++r) b->slot_f[12 * e + 3 * i + r] = V * (y[r] + vol * w[r]);
     }
+    if (b->damped) {
+      double out[12] = {0};
+      if (b->tau[e] > 0.0) damp_tet(b, e, d, out);
+      for (int k = 0; k < 12; ++k) b->slot_v[12 * e + k] = b->tau[e] * out[k];
+    }
   }
 }
 
@@ -1300,13 +1420,15 @@ static void op_gather_range(void *p, size_t begin, size_t end) {
       c->y[3 * n] = c->y[3 * n + 1] = c->y[3 * n + 2] = 0.0;
       continue;
     }
-    double f[3] = {0, 0, 0};
+    double f[3] = {0, 0, 0}, v[3] = {0, 0, 0};
     for (uint32_t k = b->inc_start[n]; k < b->inc_start[n + 1]; ++k) {
       const uint32_t s = b->inc_slot[k];
       for (int r = 0; r < 3; ++r) f[r] += b->slot_f[3 * (size_t)s + r];
+      if (b->damped)
+        for (int r = 0; r < 3; ++r) v[r] += b->slot_v[3 * (size_t)s + r];
     }
     const double m = c->inertia * b->m.mass[n];
-    for (int r = 0; r < 3; ++r) c->y[3 * n + r] = m * c->d[3 * n + r] + c->scale * f[r];
+    for (int r = 0; r < 3; ++r) c->y[3 * n + r] = m * c->d[3 * n + r] + c->scale * f[r] + b->damp * v[r];
   }
 }
 
@@ -1392,8 +1514,39 @@ static rule_t rule_of(double spectral) {
 
 /* balance (solver.cc:3746-3806): right-hand side, inertia on the diagonal,
  * block-Jacobi inverse (guardPreconditioner, solver.cc:1574-1593). */
+/* The damping force of balance (solver.cc:3746-3783): viscous = sum_e tau_e
+ * D_e(v_w) at the alpha-point velocity v_w = v + lag (1 - gamma) a_n
+ * [+ lag gamma a], lag = (1 - alpha_f) h, anchored nodes at rest; into
+ * b->increment (free scratch between solves here). */
+static void viscous_force(orc_body *b, const rule_t *rule, const double *taken, double *viscous) {
+  const size_t N = b->m.nodes;
+  double *vw = (double *)malloc(3 * N * sizeof(double) + 8);
+  for (size_t n = 0; n < N; ++n)
+    for (int r = 0; r < 3; ++r) {
+      const size_t i = 3 * n + r;
+      vw[i] = b->m.anchored[n] ? 0.0
+                               : b->v[i] + b->lag * (1.0 - rule->gamma) * b->carried[i] +
+                                     (taken ? b->lag * rule->gamma * taken[i] : 0.0);
+    }
+  memset(viscous, 0, 3 * N * sizeof(double));
+  for (size_t e = 0; e < b->m.tets; ++e) {
+    if (!(b->tau[e] > 0.0)) continue;
+    double ve[12], out[12];
+    gather4(b, vw, e, ve);
+    damp_tet(b, e, ve, out);
+    for (int c = 0; c < 4; ++c)
+      for (int r = 0; r < 3; ++r) viscous[3 * (size_t)b->m.tet[4 * e + c] + r] += b->tau[e] * out[3 * c + r];
+  }
+  free(vw);
+}
+
 static void balance(orc_body *b, const rule_t *rule, double inertia,
                     const double *taken) {
+  double *viscous = NULL;
+  if (b->damped) {
+    viscous = (double *)malloc(3 * (size_t)b->m.nodes * sizeof(double) + 8);
+    viscous_force(b, rule, taken, viscous);
+  }
   for (size_t n = 0; n < b->m.nodes; ++n) {
     double *P = &b->precond[9 * n];
     if (b->m.anchored[n]) {
@@ -1410,6 +1563,7 @@ static void balance(orc_body *b, const rule_t *rule, double inertia,
       double rhs = b->force[3 * n + r] + b->drive[3 * n + r] -
                    rule->am * m * b->carried[3 * n + r];
       if (taken) rhs -= inertia * m * taken[3 * n + r];
+      if (viscous) rhs -= viscous[3 * n + r];
       b->right[3 * n + r] = rhs;
     }
     double *D = &b->diag[9 * n];
@@ -1418,6 +1572,7 @@ static void balance(orc_body *b, const rule_t *rule, double inertia,
     D[8] += inertia * m;
     if (!(orc_det3(D) > 0.0) || !inv3(D, P)) ident3(P);
   }
+  free(viscous);
 }
 
 /* predictorInverts (solver.cc:568-623) with FP64 determinants */
@@ -1491,6 +1646,9 @@ static double substep(orc_body *b, const orc_settings *s, const rule_t *rule, do
   const double step = sqrt((1.0 - rule->af) * rule->beta) * inner;
   const double carry = (0.5 - rule->beta) * inner * inner;
   const double reach = step * step;
+  /* damping.scale and the alpha-point lag (solver.cc:3586, 3750) */
+  b->damp = b->damped ? (1.0 - rule->af) * rule->gamma * inner : 0.0;
+  b->lag = (1.0 - rule->af) * inner;
   for (size_t n = 0; n < N; ++n)
     for (int r = 0; r < 3; ++r) {
       const size_t i = 3 * n + r;
@@ -1672,6 +1830,18 @@ void orc_stressed_apply(orc_body *b, const double *x, const double *d, double *y
   free(force);
 }
 
+/* The operator with stiffness damping at positions x (the fits and
+ * rotations of an assembly at x): y = inertia M d + step^2 K d + damp
+ * sum_e tau_e D_e d, anchored rows zero. */
+void orc_damped_apply(orc_body *b, const double *x, double inertia, double step, double damp,
+                      const double *d, double *y) {
+  orc_eval_forces(b, x, 1, NULL, NULL, NULL, NULL);
+  const double held = b->damp;
+  b->damp = damp;
+  orc_apply_system(b, inertia, step, d, y, 1);
+  b->damp = held;
+}
+
 /* LOADING_MARGIN, PLASTIC_REDUCTION, PLASTIC_RESOLUTION (solver.cc:372-380) */
 #define LOADING_MARGIN 1.0e-3
 #define PLASTIC_REDUCTION 1.0e-2
@@ -1755,6 +1925,8 @@ static double newton_substep(orc_body *b, const orc_settings *s, const rule_t *r
   const double carry = (0.5 - rule->beta) * inner * inner;
   const double reach = step * step;
   const double travel = rule->beta * inner * inner;
+  b->damp = b->damped ? (1.0 - rule->af) * rule->gamma * inner : 0.0;
+  b->lag = (1.0 - rule->af) * inner;
   for (size_t n = 0; n < N; ++n)
     for (int r = 0; r < 3; ++r) {
       const size_t i = 3 * n + r;

@@ -161,6 +161,10 @@ uint32_t orc_tangent_apply(orc_body *body, const double *x, const double *d, dou
 /* The operator with the initial-stress (rotation-derivative) term at x:
  * y = sum over corotated tets of (R K R^T + initialStress) d, h = 1. */
 void orc_stressed_apply(orc_body *body, const double *x, const double *d, double *y);
+/* The operator with stiffness damping at x: inertia M d + step^2 K d +
+ * damp sum_e tau_e P R K R^T P d (P the element's rigid-mode removal). */
+void orc_damped_apply(orc_body *body, const double *x, double inertia, double step, double damp,
+                      const double *d, double *y);
 /* Yield flags of the last step (1 per tet). */
 void orc_yield_flags(const orc_body *body, uint8_t *flags);
 
