@@ -6,12 +6,59 @@ refusals (a body beyond the scratch array, a context not built since its
 last edit)."""
 from __future__ import annotations
 
+import ctypes
+
 import numpy as np
 import pytest
 
 from talus_amd import api, scenes
 
 from helpers import deformed
+
+
+class DeviceBuffers:
+    """Device buffers for the entry point's inputs and outputs, from the HIP
+    runtime the library itself runs on (libamdhip64.so.7, loaded with it), so
+    the test holds no second runtime's state."""
+
+    H2D, D2H = 1, 2
+
+    def __init__(self):
+        api.lib()
+        self.hip = ctypes.CDLL("libamdhip64.so.7")
+        self.hip.hipMalloc.argtypes = [ctypes.POINTER(ctypes.c_void_p), ctypes.c_size_t]
+        self.hip.hipMemcpy.argtypes = [ctypes.c_void_p, ctypes.c_void_p, ctypes.c_size_t, ctypes.c_int]
+        self.hip.hipFree.argtypes = [ctypes.c_void_p]
+        self.held = []
+
+    def _ok(self, rc, what):
+        assert rc == 0, f"{what} returned HIP error {rc}"
+
+    def upload(self, a):
+        a = np.ascontiguousarray(a, dtype=np.float64)
+        p = ctypes.c_void_p()
+        self._ok(self.hip.hipMalloc(ctypes.byref(p), max(a.nbytes, 8)), "hipMalloc")
+        self.held.append(p.value)
+        self._ok(self.hip.hipMemcpy(p.value, a.ctypes.data, a.nbytes, self.H2D), "hipMemcpy")
+        return p.value
+
+    def download(self, ptr, n):
+        out = np.empty(n, dtype=np.float64)
+        self._ok(self.hip.hipDeviceSynchronize(), "hipDeviceSynchronize")
+        self._ok(self.hip.hipMemcpy(out.ctypes.data, ptr, out.nbytes, self.D2H), "hipMemcpy")
+        return out
+
+    def close(self):
+        for p in self.held:
+            self.hip.hipFree(p)
+        self.held.clear()
+
+
+@pytest.fixture
+def buffers():
+    held = DeviceBuffers()
+    yield held
+    held.close()
 
 
 def _body(cells, plastic):
@@ -41,9 +88,7 @@ def _prepare(ctx, body, seed):
 
 @pytest.mark.gpu
 @pytest.mark.parametrize("precision", [api.FP64, api.MIXED])
-def test_noalloc_operator_equals_apply_system_on_two_calls(precision):
-    import torch
-
+def test_noalloc_operator_equals_apply_system_on_two_calls(precision, buffers):
     with api.Context(0, precision) as ctx:
         big, xb = _prepare(ctx, _body((20, 5, 5), plastic=True), seed=3)  # 3,000 tets
         small, xs = _prepare(ctx, _body((3, 2, 2), plastic=False), seed=4)  # 72 tets
@@ -54,31 +99,27 @@ def test_noalloc_operator_equals_apply_system_on_two_calls(precision):
             n, _ = ctx.info(b)
             d = rng.standard_normal((n, 3))
             expect = ctx.apply_system(b, inertia, step, d)
-            dv = torch.tensor(d.reshape(-1), dtype=torch.float64, device="cuda")
-            y = torch.full_like(dv, float("nan"))
-            torch.cuda.synchronize()
-            ctx.apply_system_noalloc(b, inertia, step, dv.data_ptr(), y.data_ptr())
-            torch.cuda.synchronize()
-            got = y.cpu().numpy().reshape(n, 3)
+            dv = buffers.upload(d.reshape(-1))
+            y = buffers.upload(np.full(3 * n, np.nan))
+            ctx.apply_system_noalloc(b, inertia, step, dv, y)
+            got = buffers.download(y, 3 * n).reshape(n, 3)
             assert np.isfinite(expect).all() and np.abs(expect).max() > 0.0
             assert np.array_equal(got, expect)
 
 
 @pytest.mark.gpu
-def test_noalloc_operator_refuses_oversized_and_unbuilt_bodies():
-    import torch
-
-    dv = torch.zeros(3, dtype=torch.float64, device="cuda")
+def test_noalloc_operator_refuses_oversized_and_unbuilt_bodies(buffers):
+    dv = buffers.upload(np.zeros(3))
     with api.Context(0, api.FP64) as ctx:
         # 50 x 50 x 35 hexes = 525,000 tets: 12 doubles each exceed 48 MiB
         mesh, mats = scenes.build_mesh(_body((50, 50, 35), plastic=False))
         b = ctx.add_body(mesh, mats)
         with pytest.raises(api.TalusError, match="scratch bytes"):
-            ctx.apply_system_noalloc(b, 1.0, 0.01, dv.data_ptr(), dv.data_ptr())
+            ctx.apply_system_noalloc(b, 1.0, 0.01, dv, dv)
     with api.Context(0, api.FP64) as ctx:
         b, _ = _prepare(ctx, _body((3, 2, 2), plastic=False), seed=5)
         n, _ = ctx.info(b)
         ctx.apply_system(b, 1.0, 0.01, np.zeros((n, 3)))  # builds the device arrays
         ctx.add_body(*scenes.build_mesh(_body((2, 2, 2), plastic=False)))  # an edit, not built yet
         with pytest.raises(api.TalusError, match="not built"):
-            ctx.apply_system_noalloc(b, 1.0, 0.01, dv.data_ptr(), dv.data_ptr())
+            ctx.apply_system_noalloc(b, 1.0, 0.01, dv, dv)
