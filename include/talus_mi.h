@@ -279,6 +279,15 @@ int talus_mi_apply_system(talus_mi_ctx *ctx, uint32_t body, double inertia,
  * (neo-Hookean tets add their operator only), anchored rows zero. */
 int talus_mi_apply_system_stressed(talus_mi_ctx *ctx, uint32_t body, const double *x, double inertia,
                                    double step, const double *d, double *y);
+/* The operator with the stiffness-damping term (applySystem's addDamping,
+ * solver.cc:1006-1007; applyDamping, factor/factor.cc:848-857): at
+ * positions x of the body (which become its evaluation state: rotations and
+ * rigid fits), y = inertia*M*d + step^2 sum_e R K R^T d + damp sum_e tau_e
+ * P_e R K R^T P_e d, tau_e the tet's stiffness_damping and P_e the removal
+ * of its corners' rigid motion; anchored rows zero.  The step's damping
+ * coefficient is (1 - alpha_f) gamma h (solver.cc:3590). */
+int talus_mi_apply_system_damped(talus_mi_ctx *ctx, uint32_t body, const double *x, double inertia,
+                                 double step, double damp, const double *d, double *y);
 /* Size of the allocation-free path's per-GPU scratch array (MiB). */
 #define TALUS_MI_NOALLOC_SCRATCH_MIB 48
 /* talus_mi_apply_system without any allocation.  No counterpart in the

@@ -237,6 +237,8 @@ _SIGNATURES = {
                                         _P(C.c_double), _P(C.c_double)]),
     "talus_mi_apply_system_stressed": (C.c_int, [C.c_void_p, C.c_uint32, _P(C.c_double), C.c_double, C.c_double,
                                                  _P(C.c_double), _P(C.c_double)]),
+    "talus_mi_apply_system_damped": (C.c_int, [C.c_void_p, C.c_uint32, _P(C.c_double), C.c_double, C.c_double,
+                                               C.c_double, _P(C.c_double), _P(C.c_double)]),
     "talus_mi_apply_system_noalloc": (C.c_int, [C.c_void_p, C.c_uint32, C.c_double, C.c_double,
                                                 C.c_void_p, C.c_void_p]),
     "talus_mi_time_kernel": (C.c_int, [C.c_void_p, C.c_uint32, C.c_uint32, _P(KernelTiming)]),
@@ -693,6 +695,15 @@ class Context:
         d = _f64(d, 3 * n)
         y = np.empty(3 * n)
         _check(lib().talus_mi_apply_system(self._h, body, inertia, step, _dp(d), _dp(y)))
+        return y.reshape(n, 3)
+
+    def apply_system_damped(self, body, x, inertia, step, damp, d):
+        """The operator with the stiffness-damping term at positions x
+        (talus_mi_apply_system_damped)."""
+        n, _ = self.info(body)
+        x, d = _f64(x, 3 * n), _f64(d, 3 * n)
+        y = np.empty(3 * n)
+        _check(lib().talus_mi_apply_system_damped(self._h, body, _dp(x), inertia, step, damp, _dp(d), _dp(y)))
         return y.reshape(n, 3)
 
     def apply_system_stressed(self, body, x, inertia, step, d):

@@ -63,6 +63,7 @@ int prepare_body(Body &b) {
   b.hardening.resize(b.tets);
   b.plastic_max.resize(b.tets);
   b.tear.resize(b.tets);
+  b.tau.resize(b.tets);
   b.flags.resize(b.tets);
   b.dminv.resize(9 * size_t(b.tets));
   b.volume.resize(b.tets);
@@ -79,10 +80,13 @@ int prepare_body(Body &b) {
     b.hardening[e] = m.hardening_modulus > 0.0f ? double(m.hardening_modulus) : 0.0;
     b.plastic_max[e] = double(m.plastic_max);
     b.tear[e] = 1.0 + double(m.tear_strain);
+    // stiffnessDamping (factor.cc:755-759): finite and positive, else 0
+    b.tau[e] = std::isfinite(m.stiffness_damping) && m.stiffness_damping > 0.0f ? double(m.stiffness_damping) : 0.0;
     uint8_t f = 0;
     if (m.elastic_model == 1u) f |= FLAG_SNH;
     if (plastic) f |= FLAG_PLASTIC;
     if (std::isfinite(m.tear_strain)) f |= FLAG_TEAR;
+    if (b.tau[e] > 0.0) f |= FLAG_DAMP;
     b.flags[e] = f;
   }
   if (b.density_mass) {
@@ -310,6 +314,8 @@ class EngineBase {
                                    double *y, void *slots, hipStream_t s) = 0;
   virtual void apply_system_stressed(uint32_t b, const double *x, double inertia, double step,
                                      const double *dv, double *y) = 0;
+  virtual void apply_system_damped(uint32_t b, const double *x, double inertia, double step, double damp,
+                                   const double *dv, double *y) = 0;
   virtual void time_kernel(uint32_t kind, uint32_t repeats, talus_mi_kernel_timing &out) = 0;
   virtual void factor_solve(uint32_t b, double inertia, double step, const double *r,
                             double *z) = 0;
@@ -362,6 +368,8 @@ class Engine final : public EngineBase {
   }
   void apply_system_stressed(uint32_t b, const double *x, double inertia, double step, const double *dv,
                              double *y) override;
+  void apply_system_damped(uint32_t b, const double *x, double inertia, double step, double damp,
+                           const double *dv, double *y) override;
   void time_kernel(uint32_t kind, uint32_t repeats, talus_mi_kernel_timing &out) override;
   void factor_solve(uint32_t b, double inertia, double step, const double *r, double *z) override;
   void factor_info(talus_mi_factor_info &info) override;
@@ -453,7 +461,7 @@ class Engine final : public EngineBase {
     };
     std::vector<Launch> launches;
   };
-  void factorize(const std::vector<double> &reach, const std::vector<double> &inertia,
+  void factorize(const std::vector<double> &reach, const std::vector<double> &inertia, const std::vector<double> *damp,
                  const std::vector<int> *mask = nullptr, FactorProfile *prof = nullptr);
   void precondition(int parity);
   void forward_level(uint32_t l, int mode, int parity, const Real *rin = nullptr);
@@ -477,7 +485,7 @@ class Engine final : public EngineBase {
   uint32_t *hyields_ = nullptr;
   Real *hvec_ = nullptr;  // 3N + 12T staging of Real
   double *scratch64_ = nullptr;
-  bool any_plastic_ = false, any_tear_ = false;
+  bool any_plastic_ = false, any_tear_ = false, any_damped_ = false;
   uint64_t n_snh_ = 0, n_plastic_ = 0, n_flowing_ = 0;  // flowing: plastic neo-Hookean
   uint32_t applications_ = 0;  // preconditioner applications this step
   // host copies of the incidence CSR: node ranges and the CSR position of
@@ -521,7 +529,7 @@ class Engine final : public EngineBase {
   // factor state
   const std::vector<Body> *bodies_ = nullptr;
   bool sym_ready_ = false, fac_ready_ = false, fac_ok_ = false, use_factor_ = false;
-  std::vector<double> fac_reach_, fac_inertia_;
+  std::vector<double> fac_reach_, fac_inertia_, fac_damp_;
   FactorTables tab_;
   std::vector<uint8_t> plan_;  // per level: forward, backward kernel shape
   FactorDev fd_{};
@@ -536,7 +544,7 @@ class Engine final : public EngineBase {
            *d_gat_big_ = nullptr, *d_xg_all_ = nullptr, *d_xg_big_ = nullptr;
   using GT = typename GStore<Real>::type;  // storage of the solve matrix G
   GT *d_gr_ = nullptr, *d_gc_ = nullptr;
-  double *d_body_reach_ = nullptr, *d_body_inertia_ = nullptr;
+  double *d_body_reach_ = nullptr, *d_body_inertia_ = nullptr, *d_body_damp_ = nullptr;
   int *hfailed_ = nullptr;
   int *d_refresh_ = nullptr;
   // early refresh of single bodies' factors: a body whose solves needed at
@@ -622,9 +630,9 @@ void Engine<Real>::build(const std::vector<Body> &bodies) {
   std::vector<uint4> conn(T);
   std::vector<double> dminv(9 * T), vol64(T), lam64(T), mu64(T), rho(T), yield(T), hard(T), pmax(T),
       tear(T), plastic(10 * T);
-  std::vector<Real> vol(T), lam(T), mu(T), R(9 * T), W(9 * T);
+  std::vector<Real> vol(T), lam(T), mu(T), R(9 * T), W(9 * T), tau(T);
   std::vector<uint8_t> flags(T);
-  any_plastic_ = any_tear_ = false;
+  any_plastic_ = any_tear_ = any_damped_ = false;
   n_snh_ = n_plastic_ = n_flowing_ = 0;
   newton_body_.assign(B, 0);
   for (uint32_t b = 0; b < B; ++b) {
@@ -659,7 +667,9 @@ void Engine<Real>::build(const std::vector<Body> &bodies) {
       hard[g] = bd.hardening[e];
       pmax[g] = bd.plastic_max[e];
       tear[g] = bd.tear[e];
+      tau[g] = Real(bd.tau[e]);
       flags[g] = bd.flags[e];
+      any_damped_ = any_damped_ || (bd.flags[e] & FLAG_DAMP);
       any_plastic_ = any_plastic_ || (bd.flags[e] & FLAG_PLASTIC);
       if ((bd.flags[e] & FLAG_PLASTIC) && !(bd.flags[e] & FLAG_SNH)) newton_body_[b] = 1;
       n_snh_ += (bd.flags[e] & FLAG_SNH) ? 1u : 0u;
@@ -719,6 +729,11 @@ void Engine<Real>::build(const std::vector<Body> &bodies) {
   d_.R = dev_copy(R);
   d_.W = dev_copy(W);
   d_.plastic = dev_copy(plastic);
+  if (any_damped_) {
+    d_.tau = dev_copy(tau);
+    d_.fit = dev<Real>(size_t(FIT_PLANES) * T);
+    HIPCHECK(hipMemset(d_.fit, 0, size_t(FIT_PLANES) * T * sizeof(Real)));
+  }
   {
     std::vector<Real> lin(8 * size_t(T), Real(0));
     std::fill(lin.begin(), lin.begin() + T, Real(1));
@@ -853,6 +868,7 @@ void Engine<Real>::analyze_factor() {
   HIPCHECK(hipMemset(d_gc_, 0, std::max<uint64_t>(t.g_pool, 1) * sizeof(GT)));
   d_body_reach_ = dev<double>(d_.B);
   d_body_inertia_ = dev<double>(d_.B);
+  d_body_damp_ = dev<double>(d_.B);
   sym_ready_ = true;
 }
 
@@ -860,6 +876,7 @@ void Engine<Real>::analyze_factor() {
 // inertia M + reach sum R K R^T with the committed rotations.
 template <class Real>
 void Engine<Real>::factorize(const std::vector<double> &reach, const std::vector<double> &inertia,
+                             const std::vector<double> *damp,
                              const std::vector<int> *mask, FactorProfile *prof) {
   if (!sym_ready_) analyze_factor();
   const std::vector<int> every(d_.B, 1);
@@ -872,6 +889,17 @@ void Engine<Real>::factorize(const std::vector<double> &reach, const std::vector
   const FactorTables &t = tab_;
   HIPCHECK(hipMemcpyAsync(d_body_reach_, reach.data(), d_.B * sizeof(double), hipMemcpyHostToDevice, s));
   HIPCHECK(hipMemcpyAsync(d_body_inertia_, inertia.data(), d_.B * sizeof(double), hipMemcpyHostToDevice, s));
+  // the damping coefficient of each body, 0 for the factor of the plain
+  // operator (refreshFactor's damping argument, factor.cc:567-598)
+  fac_damp_ = damp ? *damp : std::vector<double>(d_.B, 0.0);
+  HIPCHECK(hipMemcpyAsync(d_body_damp_, fac_damp_.data(), d_.B * sizeof(double), hipMemcpyHostToDevice, s));
+  FactorDamping<Real> damping;
+  if (any_damped_) {
+    damping.x = d_.x;
+    damping.conn = d_.conn;
+    damping.tau = d_.tau;
+    damping.body_damp = d_body_damp_;
+  }
   HIPCHECK(hipMemcpyAsync(d_refresh_, fresh.data(), d_.B * sizeof(int), hipMemcpyHostToDevice, s));
   if (mask) {
     // partial refresh: only the flagged bodies' fronts are rebuilt
@@ -903,7 +931,7 @@ void Engine<Real>::factorize(const std::vector<double> &reach, const std::vector
     FactorLaunch<Real>::assemble(fd_, d_pair_target_, d_pair_ld_, d_pair_node_, d_pair_start_, d_contrib_,
                                  uint32_t(t.pair_target.size()), d_.dminv, d_.vol64, d_.lam64, d_.mu64, d_.R,
                                  d_.W, d_.plastic, d_.flags, d_.T, d_.mass, d_body_reach_, d_body_inertia_,
-                                 d_pair_body_, s);
+                                 d_pair_body_, damping, s);
   });
   size_t ea = 0, fac = 0;
   for (uint32_t l = 0; l < t.levels; ++l) {
@@ -1077,7 +1105,7 @@ void Engine<Real>::tune(uint32_t repeats, talus_mi_tune_report &rep) {
   if (!built_ || d_.B == 0) return;
   if (!fac_ready_) {
     std::vector<double> reach(d_.B, 1.0), inert(d_.B, 1.0);
-    factorize(reach, inert);
+    factorize(reach, inert, nullptr);
     fac_reach_.clear();  // the next step refactors with its own keys
   }
   const uint32_t reps = std::max(repeats, 1u);
@@ -1240,7 +1268,7 @@ void Engine<Real>::newton_pass(const std::vector<uint32_t> &bodies, const std::v
       upload_newton(q);
       if (o.kind == NewtonOp::EVAL) {
         if (!o.first) Launch<Real>::evaluate(d_, s);
-        Launch<Real>::assemble(d_, diag, true, s);
+        Launch<Real>::assemble(d_, diag, true, s, o.first ? VISCOUS_PREDICTOR : VISCOUS_TAKEN);
         Launch<Real>::balance(d_, diag, !o.first, s);
         Launch<Real>::newton_norms(d_, s);
         norms = loads = true;
@@ -1434,6 +1462,8 @@ void Engine<Real>::step(const talus_mi_step_params &sp, talus_mi_report &rep,
     p.tolerance = double(sp.tolerance);
     p.active = active ? 1u : 0u;
     p.limit = limit;
+    p.damp = (1.0 - af) * gamma * inner;
+    p.lag = (1.0 - af) * inner;
     return p;
   };
   // predictorInverts at the configured substep
@@ -1487,21 +1517,22 @@ void Engine<Real>::step(const talus_mi_step_params &sp, talus_mi_report &rep,
     if (factor) {
       // refresh when missing, when the step or inertia keys change, or after
       // a capped solve (solver.cc:3639-3655, 3839-3899)
-      std::vector<double> reach(B), inertia(B);
+      std::vector<double> reach(B), inertia(B), damp(B);
       for (uint32_t b = 0; b < B; ++b) {
         const StepParams q = params_for(substeps[b], true);
         reach[b] = q.reach;
         inertia[b] = q.inertia;
+        damp[b] = q.damp;
       }
-      if (!fac_ready_ || reach != fac_reach_ || inertia != fac_inertia_) {
+      if (!fac_ready_ || reach != fac_reach_ || inertia != fac_inertia_ || (any_damped_ && damp != fac_damp_)) {
         HIPCHECK(hipStreamSynchronize(s));
-        factorize(reach, inertia);
+        factorize(reach, inertia, &damp);
         std::fill(want_refresh_.begin(), want_refresh_.end(), 0);
       } else if (pass == 0 && std::any_of(want_refresh_.begin(), want_refresh_.end(), [](int w) { return w != 0; })) {
         // bodies whose solves were slow or capped last step: rebuild their
         // fronts only
         HIPCHECK(hipStreamSynchronize(s));
-        factorize(reach, inertia, &want_refresh_);
+        factorize(reach, inertia, &damp, &want_refresh_);
         for (uint32_t b = 0; b < B; ++b)
           if (want_refresh_[b]) next_refresh_[b] = steps_taken_ + REFRESH_COOLDOWN;
         std::fill(want_refresh_.begin(), want_refresh_.end(), 0);
@@ -1544,7 +1575,7 @@ void Engine<Real>::step(const talus_mi_step_params &sp, talus_mi_report &rep,
       upload_newton(reg);
     }
     if (nbodies.size() < in_pass) {
-      Launch<Real>::assemble(d_, diag, false, s);
+      Launch<Real>::assemble(d_, diag, false, s, VISCOUS_PREDICTOR);
       Launch<Real>::balance(d_, diag, false, s);
       solve_pass.push_back(pass);
       solve_active.push_back(reg_mask);
@@ -1552,7 +1583,7 @@ void Engine<Real>::step(const talus_mi_step_params &sp, talus_mi_report &rep,
       solve(true, limit, solve_index++);
       for (uint32_t read = 1; read < reads; ++read) {
         Launch<Real>::evaluate(d_, s);
-        Launch<Real>::assemble(d_, diag, converge || read == 2, s);
+        Launch<Real>::assemble(d_, diag, converge || read == 2, s, VISCOUS_TAKEN);
         Launch<Real>::balance(d_, diag, true, s);
         solve_pass.push_back(pass);
         solve_active.push_back(reg_mask);
@@ -1811,6 +1842,28 @@ void Engine<Real>::apply_system_stressed(uint32_t b, const double *x, double ine
   HIPCHECK(hipStreamSynchronize(s));
 }
 
+// the operator with the stiffness-damping term at positions x of body b:
+// the element pass at x (rotations, rigid fits), then inertia M d +
+// step^2 K d + damp sum_e tau_e C_e d
+template <class Real>
+void Engine<Real>::apply_system_damped(uint32_t b, const double *x, double inertia, double step, double damp,
+                                       const double *dv, double *y) {
+  hipStream_t s = stream_;
+  const size_t n0 = n0_[b], nn = nn_[b], N3 = 3 * size_t(d_.N);
+  double *pos = scratch64_, *din = scratch64_ + N3, *yout = scratch64_ + 2 * N3;
+  HIPCHECK(hipMemcpyAsync(pos, d_.x, N3 * sizeof(double), hipMemcpyDeviceToDevice, s));
+  HIPCHECK(hipMemcpyAsync(pos + 3 * n0, x, 3 * nn * sizeof(double), hipMemcpyHostToDevice, s));
+  HIPCHECK(hipMemcpyAsync(din, dv, 3 * nn * sizeof(double), hipMemcpyHostToDevice, s));
+  all_active();
+  Launch<Real>::displacement(d_, pos, s);
+  Launch<Real>::assemble(d_, false, false, s);
+  Launch<Real>::apply_body_damped(d_, t0_[b], t0_[b] + tt_[b], uint32_t(n0), uint32_t(n0 + nn), inc_start_[n0],
+                                  din, yout, inertia, step * step, damp,
+                                  d_.slot_f + 3 * size_t(inc_start_[n0]), s);
+  HIPCHECK(hipMemcpyAsync(y, yout, 3 * nn * sizeof(double), hipMemcpyDeviceToHost, s));
+  HIPCHECK(hipStreamSynchronize(s));
+}
+
 template <class Real>
 void Engine<Real>::time_kernel(uint32_t kind, uint32_t repeats, talus_mi_kernel_timing &out) {
   std::memset(&out, 0, sizeof out);
@@ -1854,7 +1907,7 @@ void Engine<Real>::time_kernel(uint32_t kind, uint32_t repeats, talus_mi_kernel_
   } else if (kind <= 8) {
     if (!fac_ready_) {
       std::vector<double> reach(d_.B, 1.0), inert(d_.B, 1.0);
-      factorize(reach, inert);
+      factorize(reach, inert, nullptr);
       fac_reach_.clear();  // the next step refactors with its own keys
     }
     const FactorTables &t = tab_;
@@ -1946,7 +1999,7 @@ void Engine<Real>::factor_solve(uint32_t b, double inertia, double step, const d
                                 double *z) {
   hipStream_t s = stream_;
   std::vector<double> reach(d_.B, step * step), inert(d_.B, inertia);
-  factorize(reach, inert);
+  factorize(reach, inert, nullptr);
   frames_ = false;  // (LL^T)^-1 r of the factor just built
   const size_t n0 = n0_[b], nn = nn_[b];
   std::vector<Real> in(3 * size_t(d_.N), Real(0));
@@ -2072,7 +2125,8 @@ void Engine<Real>::factor_profile(std::vector<talus_mi_factor_kernel> &out) {
   const std::vector<double> reach = fac_reach_.size() == d_.B ? fac_reach_ : ones;
   const std::vector<double> inertia = fac_inertia_.size() == d_.B ? fac_inertia_ : ones;
   FactorProfile prof;
-  factorize(reach, inertia, nullptr, &prof);
+  const std::vector<double> damp = fac_damp_.size() == d_.B ? fac_damp_ : std::vector<double>(d_.B, 0.0);
+  factorize(reach, inertia, &damp, nullptr, &prof);
   out.clear();
   for (const auto &L : prof.launches) {
     float ms = 0.0f;
@@ -2283,6 +2337,10 @@ class GroupedEngine final : public EngineBase {
   void apply_system_stressed(uint32_t b, const double *x, double inertia, double step, const double *dv,
                              double *y) override {
     eng_[group_[b]]->apply_system_stressed(local_[b], x, inertia, step, dv, y);
+  }
+  void apply_system_damped(uint32_t b, const double *x, double inertia, double step, double damp,
+                           const double *dv, double *y) override {
+    eng_[group_[b]]->apply_system_damped(local_[b], x, inertia, step, damp, dv, y);
   }
   void factor_solve(uint32_t b, double inertia, double step, const double *r, double *z) override {
     eng_[group_[b]]->factor_solve(local_[b], inertia, step, r, z);
@@ -2895,6 +2953,16 @@ int talus_mi_apply_system_stressed(talus_mi_ctx *ctx, uint32_t body, const doubl
   });
 }
 
+int talus_mi_apply_system_damped(talus_mi_ctx *ctx, uint32_t body, const double *x, double inertia,
+                                 double step, double damp, const double *d, double *y) {
+  if (!ctx || !body_ok(ctx, body) || !x || !d || !y) return fail(TALUS_MI_ERR_ARGUMENT, "bad arguments");
+  return guarded(ctx, [&]() -> int {
+    ensure(ctx);
+    ctx->engine->apply_system_damped(body, x, inertia, step, damp, d, y);
+    return TALUS_MI_OK;
+  });
+}
+
 int talus_mi_apply_system_noalloc(talus_mi_ctx *ctx, uint32_t body, double inertia, double step,
                                   const double *d, double *y) {
   if (!ctx || !body_ok(ctx, body) || !d || !y) return fail(TALUS_MI_ERR_ARGUMENT, "bad arguments");
@@ -3141,8 +3209,8 @@ int talus_mi_rollout_backward(talus_mi_ctx *ctx, uint32_t body, const double *x_
       return fail(TALUS_MI_ERR_STATE, "no rollout of the current bodies to differentiate");
     const tmi::Body &b = ctx->bodies[body];
     for (uint32_t e = 0; e < b.tets; ++e)
-      if ((b.flags[e] & tmi::FLAG_PLASTIC) || !(b.flags[e] & tmi::FLAG_SNH))
-        return fail(TALUS_MI_ERR_ARGUMENT, "the adjoint supports elastic neo-Hookean bodies only");
+      if ((b.flags[e] & (tmi::FLAG_PLASTIC | tmi::FLAG_DAMP)) || !(b.flags[e] & tmi::FLAG_SNH))
+        return fail(TALUS_MI_ERR_ARGUMENT, "the adjoint supports undamped elastic neo-Hookean bodies only");
     // the replays overwrite the per-tet operator state: keep it
     ctx->engine->download(ctx->bodies);
     ctx->engine->rollout_backward(ctx->ckpt_params, ctx->ckpt, ctx->ckpt_final, ctx->ckpt_steps,

@@ -172,6 +172,18 @@ struct FactorDev {
 // engine's element arrays; R is the committed rotation and W the
 // neo-Hookean volume frame, both in the element precision; plastic the FP64
 // plastic planes (Fp^-1 - I for plastic neo-Hookean tets).
+// Stiffness damping of the factor (factor.cc:320-375): the damping blocks
+// tau P R K R^T P of the damped tets, P fitted to the positions x at the
+// factorization, weighted by each body's damping coefficient; tau null when
+// no tet is damped.
+template <class Real>
+struct FactorDamping {
+  const double *x = nullptr;
+  const uint4 *conn = nullptr;
+  const Real *tau = nullptr;
+  const double *body_damp = nullptr;
+};
+
 template <class Real>
 struct FactorLaunch {
   static void assemble(const FactorDev &f, const uint64_t *pair_target, const uint32_t *pair_ld,
@@ -180,7 +192,8 @@ struct FactorLaunch {
                        const double *vol, const double *lam, const double *mu, const Real *R,
                        const Real *W, const double *plastic, const uint8_t *flags, uint32_t T,
                        const double *mass, const double *body_reach,
-                       const double *body_inertia, const uint32_t *pair_body, hipStream_t s);
+                       const double *body_inertia, const uint32_t *pair_body,
+                       const FactorDamping<Real> &damping, hipStream_t s);
 };
 
 void factor_extend_add(const FactorDev &f, const uint32_t *parent, const uint64_t *tasks,

@@ -39,10 +39,12 @@ __global__ __launch_bounds__(NT) void k_assemble_pairs(
     const uint32_t *pair_start, const uint32_t *contrib, uint32_t pairs, const double *dminv,
     const double *vol, const double *lam, const double *mu, const Real *R, const Real *W,
     const double *plastic, const uint8_t *flags, uint32_t T, const double *mass, const double *reach_of,
-    const double *inertia_of, const uint32_t *pair_body, const int *refresh) {
+    const double *inertia_of, const uint32_t *pair_body, const int *refresh, FactorDamping<Real> damping) {
   const uint32_t q = blockIdx.x * NT + threadIdx.x;
   if (q >= pairs || !refresh[pair_body[q]]) return;
   double B[9] = {0, 0, 0, 0, 0, 0, 0, 0, 0};
+  double Bd[9] = {0, 0, 0, 0, 0, 0, 0, 0, 0};  // sum tau C_ab of the damped tets
+  const double damp = damping.tau ? damping.body_damp[pair_body[q]] : 0.0;
   for (uint32_t c = pair_start[q]; c < pair_start[q + 1]; ++c) {
     const uint32_t code = contrib[c];
     const uint32_t e = code >> 4, a = (code >> 2) & 3u, b = code & 3u;
@@ -96,11 +98,32 @@ __global__ __launch_bounds__(NT) void k_assemble_pairs(
 #pragma unroll
         for (int s = 0; s < 3; ++s) B[r * 3 + s] += vl * wa[r] * wb[s];
     }
+    if (damp != 0.0 && (flags[e] & FLAG_DAMP)) {
+      // block (a, b) of tau C: column s is corner a of C e_(b, s)
+      // (dampingBlocks, factor.cc:273-297), with the rest gradients
+      double xc[12], fit[FIT_PLANES];
+      const uint4 c4 = damping.conn[e];
+      const uint32_t nn[4] = {c4.x, c4.y, c4.z, c4.w};
+#pragma unroll
+      for (int i = 0; i < 4; ++i)
+#pragma unroll
+        for (int r = 0; r < 3; ++r) xc[3 * i + r] = damping.x[3 * size_t(nn[i]) + r];
+      rigid_fit(xc, fit);
+      const double tau = double(damping.tau[e]);
+      for (int s = 0; s < 3; ++s) {
+        double unit[12], col[12];
+#pragma unroll
+        for (int k = 0; k < 12; ++k) unit[k] = k == int(3 * b) + s ? 1.0 : 0.0;
+        damp_tet(fit, Rm, g, V, lam[e], m, unit, col);
+#pragma unroll
+        for (int r = 0; r < 3; ++r) Bd[r * 3 + s] += tau * col[3 * a + r];
+      }
+    }
   }
   const uint32_t body = pair_body[q];
   const double reach = reach_of[body];
 #pragma unroll
-  for (int k = 0; k < 9; ++k) B[k] *= reach;
+  for (int k = 0; k < 9; ++k) B[k] = reach * B[k] + damp * Bd[k];
   if (pair_node[q] != ~0u) {
     const double im = inertia_of[body] * mass[pair_node[q]];
     B[0] += im;
@@ -892,12 +915,12 @@ void FactorLaunch<Real>::assemble(const FactorDev &f, const uint64_t *pair_targe
                                   const Real *W, const double *plastic, const uint8_t *flags,
                                   uint32_t T, const double *mass, const double *body_reach,
                                   const double *body_inertia, const uint32_t *pair_body,
-                                  hipStream_t s) {
+                                  const FactorDamping<Real> &damping, hipStream_t s) {
   if (!pairs) return;
   hipLaunchKernelGGL(k_assemble_pairs<Real>, dim3((pairs + NT - 1) / NT), dim3(NT), 0, s, f.pool,
                      pair_target, pair_ld, pair_node, pair_start, contrib, pairs, dminv, vol, lam,
                      mu, R, W, plastic, flags, T, mass, body_reach, body_inertia, pair_body,
-                     f.refresh);
+                     f.refresh, damping);
 }
 
 void factor_zero(const FactorDev &f, const uint64_t *tasks, uint32_t count, hipStream_t s) {

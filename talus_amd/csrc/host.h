@@ -37,6 +37,7 @@ struct Body {
   // per-tet material data; lame_set keeps lambda and mu as given
   // (talus_mi_lame_set) instead of deriving them from the lanes
   std::vector<double> lambda, mu, yield, hardening, plastic_max, tear;
+  std::vector<double> tau;  // stiffnessDamping per tet (0 undamped)
   std::vector<double> rho;  // density when masses follow rho V / 4, else 0
   bool lame_set = false;
   std::vector<uint8_t> flags;

@@ -47,7 +47,17 @@ struct StepParams {
   // tets loading past `margin`
   uint32_t newton, tangent, elastic_trial, pad;
   double margin;
+  // stiffness damping (solver.cc:3586-3592, 3746-3783): the operator's
+  // damping coefficient (1 - alpha_f) gamma h and the alpha-point lag
+  // (1 - alpha_f) h of the viscous velocity v + lag((1 - gamma) a_n + gamma a)
+  double damp, lag;
 };
+
+// The viscous force subtracted from an assembly's corner forces: none (an
+// evaluation of forces alone), at the predictor's velocity (balance without
+// an acceleration, solver.cc:3917), or with the current acceleration taken
+// (balance(&acceleration), 3931-3985).
+enum Viscous : int { VISCOUS_NONE = 0, VISCOUS_PREDICTOR = 1, VISCOUS_TAKEN = 2 };
 
 template <class Real>
 struct Dev {
@@ -80,6 +90,11 @@ struct Dev {
   Real *R;
   Real *W;          // neo-Hookean volume frame cof(Fe) Fp^-T, 9 SoA planes
   double *plastic;  // 10 SoA planes
+  // stiffness damping: tau per tet (0 undamped) and the rigid fit of each
+  // damped tet at its last assembly, FIT_PLANES SoA planes; both null when
+  // no tet is damped
+  const Real *tau;
+  Real *fit;
   Real *lin;        // J2 linearisation {theta, thetaBar, n (6)}, 8 SoA planes
   uint32_t *loading;  // [B] tets loading past the margin in the last assembly
   double *lowest;   // 3N, the Newton's lowest-residual iterate
@@ -103,7 +118,7 @@ struct Dev {
   uint32_t *yields;  // [B]
 };
 
-enum : uint8_t { FLAG_SNH = 1, FLAG_PLASTIC = 2, FLAG_TEAR = 4 };
+enum : uint8_t { FLAG_SNH = 1, FLAG_PLASTIC = 2, FLAG_TEAR = 4, FLAG_DAMP = 8 };
 
 // iterate updates of the plastic Newton, on the free nodes of active bodies
 enum NewtonVec : int {
@@ -131,7 +146,8 @@ struct Launch {
   static void predict(const Dev<Real> &d, hipStream_t s);
   static void predictor_inverts(const Dev<Real> &d, hipStream_t s);
   static void evaluate(const Dev<Real> &d, hipStream_t s);
-  static void assemble(const Dev<Real> &d, bool diag, bool energy, hipStream_t s);
+  static void assemble(const Dev<Real> &d, bool diag, bool energy, hipStream_t s,
+                       int viscous = VISCOUS_NONE);
   static void balance(const Dev<Real> &d, bool diag, bool taken, hipStream_t s);
   static void pcg_init(const Dev<Real> &d, bool with_x0, bool factor, hipStream_t s);
   static void pcg_rz(const Dev<Real> &d, bool start, int parity, hipStream_t s);
@@ -160,6 +176,11 @@ struct Launch {
   static void apply_body_stressed(const Dev<Real> &d, uint32_t t0, uint32_t t1, uint32_t n0, uint32_t n1,
                                   uint32_t k0, const double *xin, const double *din, double *yout,
                                   double inertia, double reach, Real *slots, hipStream_t s);
+  // the same with the damping term: inertia M d + reach K d + damp C d, C
+  // the damping operator of the fits of the last assembly
+  static void apply_body_damped(const Dev<Real> &d, uint32_t t0, uint32_t t1, uint32_t n0, uint32_t n1,
+                                uint32_t k0, const double *din, double *yout, double inertia, double reach,
+                                double damp, Real *slots, hipStream_t s);
   static void gather_forces(const Dev<Real> &d, Real *out, hipStream_t s);
   // convergence of the corrections: per node tile max |dx| and max |a|
   static void increment_norms(const Dev<Real> &d, double *out, hipStream_t s);

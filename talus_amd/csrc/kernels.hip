@@ -420,14 +420,38 @@ __device__ __forceinline__ void op_tet(const Dev<Real> &d, uint32_t e, const Rea
   op_tet_values(d, e, dv, y, tangent);
 }
 
+// the damping term of a damped tet, tau C d scaled into the operator's
+// slots: the gather multiplies the slots by reach, so they take
+// (damp / reach) tau C d (applySystem's addDamping, solver.cc:1006-1007)
 template <class Real>
+__device__ __forceinline__ void op_damping(const Dev<Real> &d, uint32_t e, const Real *dv, Real scale,
+                                           Real *y) {
+  Real fit[FIT_PLANES], Rm[9], dm[9], g[12], out[12];
+#pragma unroll
+  for (int k = 0; k < FIT_PLANES; ++k) fit[k] = d.fit[(size_t)k * d.T + e];
+#pragma unroll
+  for (int k = 0; k < 9; ++k) Rm[k] = d.R[(size_t)k * d.T + e];
+  load_dm(d, e, dm);
+  shape_gradients(dm, g);
+  damp_tet(fit, Rm, g, d.vol[e], d.lam[e], d.mu[e], dv, out);
+  const Real w = scale * d.tau[e];
+#pragma unroll
+  for (int k = 0; k < 12; ++k) y[k] += w * out[k];
+}
+
+template <class Real, bool DAMP>
 __global__ __launch_bounds__(TILE) void k_op_element(Dev<Real> d, const Real *dir, int parity) {
   const uint32_t t = blockIdx.x, b = d.tt_body[t];
   if (!d.params[b].active || (parity >= 0 && d.done[parity * d.B + b])) return;
   const uint32_t e = d.tt_begin[t] + threadIdx.x;
   if (e >= d.tt_end[t]) return;
-  Real y[12];
-  op_tet(d, e, dir, y, d.params[b].tangent != 0u);
+  Real dv[12], y[12];
+  gather_vec(dir, d.conn[e], dv);
+  op_tet_values(d, e, dv, y, d.params[b].tangent != 0u);
+  if (DAMP && (d.flags[e] & FLAG_DAMP)) {
+    const StepParams &P = d.params[b];
+    op_damping(d, e, dv, Real(P.damp / P.reach), y);
+  }
   store_slots(d.slot_f, d.slot_pos[e], y);
 }
 
@@ -631,8 +655,81 @@ __global__ __launch_bounds__(TILE) void k_kinetic(Dev<Real> d) {
 
 // ------------------------------------------------------- element assembly
 
-template <class Real, bool DIAG, bool ENERGY>
-__global__ __launch_bounds__(TILE) void k_assemble(Dev<Real> d) {
+// Stiffness damping of one tet at its assembly (DAMP instantiations only):
+// the rigid fit at the evaluation point (fitDamping, solver.cc:678-714),
+// the viscous force tau C v_w at the alpha-point velocity v_w = v +
+// lag((1 - gamma) a_n [+ gamma a]) (anchored corners at rest) subtracted
+// from the corner forces (balance, solver.cc:3746-3783), and with DIAG the
+// damping diagonal blocks tau C_ii (blockDamping / addDampingDiagonal,
+// solver.cc:802-880) scaled by damp / reach, so that the balance's
+// reach sum(slot_d) carries reach K_ii + damp tau C_ii.
+template <class Real>
+__device__ __forceinline__ void damp_assembly(const Dev<Real> &d, const StepParams &P, uint32_t e, uint4 c,
+                                              const double *u64, const Real *g, const Real *Rm, Real V,
+                                              Real lam, Real mu, int viscous, Real *f, Real *fit) {
+  const Real tau = d.tau[e];
+  double xc[12], fit64[FIT_PLANES];
+  gather_vec64(d.X, c, xc);
+#pragma unroll
+  for (int k = 0; k < 12; ++k) xc[k] += u64[k];
+  rigid_fit(xc, fit64);
+#pragma unroll
+  for (int k = 0; k < FIT_PLANES; ++k) {
+    fit[k] = Real(fit64[k]);
+    d.fit[(size_t)k * d.T + e] = fit[k];
+  }
+  if (viscous != VISCOUS_NONE) {
+    const uint32_t nn[4] = {c.x, c.y, c.z, c.w};
+    const double lc = P.lag * (1.0 - P.gamma), lt = viscous == VISCOUS_TAKEN ? P.lag * P.gamma : 0.0;
+    Real vw[12], out[12];
+#pragma unroll
+    for (int i = 0; i < 4; ++i) {
+      const bool held = d.anchored[nn[i]] != 0;
+#pragma unroll
+      for (int r = 0; r < 3; ++r) {
+        const size_t k = 3 * (size_t)nn[i] + r;
+        vw[3 * i + r] = held ? Real(0) : Real(d.v[k] + lc * d.carried[k] + lt * d.a[k]);
+      }
+    }
+    damp_tet(fit, Rm, g, V, lam, mu, vw, out);
+#pragma unroll
+    for (int k = 0; k < 12; ++k) f[k] -= tau * out[k];
+  }
+}
+
+template <class Real>
+__device__ __forceinline__ void damp_diagonal(const Dev<Real> &d, const StepParams &P, uint32_t e,
+                                              const Real *fit, const Real *g, const Real *Rm, Real V,
+                                              Real lam, Real mu) {
+  {
+    const Real w = d.tau[e] * Real(P.damp / P.reach);
+    const uint4 pos = d.slot_pos[e];
+    const uint32_t at[4] = {pos.x, pos.y, pos.z, pos.w};
+#pragma unroll
+    for (int i = 0; i < 4; ++i) {
+      Real B[9];
+#pragma unroll
+      for (int axis = 0; axis < 3; ++axis) {
+        Real unit[12], col[12];
+#pragma unroll
+        for (int k = 0; k < 12; ++k) unit[k] = Real(k == 3 * i + axis ? 1 : 0);
+        damp_tet(fit, Rm, g, V, lam, mu, unit, col);
+#pragma unroll
+        for (int r = 0; r < 3; ++r) B[r * 3 + axis] = col[3 * i + r];
+      }
+      Real *o = d.slot_d + 6 * (size_t)at[i];
+      o[0] += w * B[0];
+      o[1] += w * B[1];
+      o[2] += w * B[2];
+      o[3] += w * B[4];
+      o[4] += w * B[5];
+      o[5] += w * B[8];
+    }
+  }
+}
+
+template <class Real, bool DIAG, bool ENERGY, bool DAMP>
+__global__ __launch_bounds__(TILE) void k_assemble(Dev<Real> d, int viscous) {
   constexpr bool MIXED = sizeof(Real) < 8;
   __shared__ double lds[TILE / 64];
   const uint32_t t = blockIdx.x, b = d.tt_body[t];
@@ -823,6 +920,10 @@ __global__ __launch_bounds__(TILE) void k_assemble(Dev<Real> d) {
         f[3 * i + 2] = -V * w[2];
       }
     }
+    // the damping force needs the rest gradients of the Real law
+    const bool damped = DAMP && (flags & FLAG_DAMP);
+    Real fit[DAMP ? FIT_PLANES : 1];
+    if (damped) damp_assembly(d, P, e, c, u64, g, Rm, V, lam, mu, viscous, f, fit);
     store_slots(d.slot_f, d.slot_pos[e], f);
 #pragma unroll
     for (int k = 0; k < 9; ++k) d.R[(size_t)k * d.T + e] = Rm[k];
@@ -865,6 +966,7 @@ __global__ __launch_bounds__(TILE) void k_assemble(Dev<Real> d) {
         o[4] = lr * rg[1] * rg[2] + lw * wg[1] * wg[2];
         o[5] = lr * rg[2] * rg[2] + lw * wg[2] * wg[2] + vm * gg;
       }
+      if (damped) damp_diagonal(d, P, e, fit, g, Rm, V, lam, mu);
     }
     if (ENERGY) {
       d.energy[e] = psi;
@@ -1030,6 +1132,29 @@ __global__ __launch_bounds__(TILE) void k_apply_body_element(Dev<Real> d, uint32
 #pragma unroll
     for (int r = 0; r < 3; ++r) dv[3 * i + r] = Real(din[3 * (size_t)nn[i] + r]);
   op_tet_values(d, e, dv, y);
+  const uint4 pos = d.slot_pos[e];
+  store_slots(slots, make_uint4(pos.x - k0, pos.y - k0, pos.z - k0, pos.w - k0), y);
+}
+
+// The damped operator of one body (a test entry point): each tet writes
+// reach K d + damp tau C d to its slots, gathered with reach 1.
+template <class Real>
+__global__ __launch_bounds__(TILE) void k_apply_body_damped(Dev<Real> d, uint32_t t0, uint32_t t1,
+                                                            uint32_t n0, uint32_t k0, const double *din,
+                                                            double reach, double damp, Real *slots) {
+  const uint32_t e = t0 + blockIdx.x * TILE + threadIdx.x;
+  if (e >= t1) return;
+  const uint4 c = d.conn[e];
+  const uint32_t nn[4] = {c.x - n0, c.y - n0, c.z - n0, c.w - n0};
+  Real dv[12], y[12];
+#pragma unroll
+  for (int i = 0; i < 4; ++i)
+#pragma unroll
+    for (int r = 0; r < 3; ++r) dv[3 * i + r] = Real(din[3 * (size_t)nn[i] + r]);
+  op_tet_values(d, e, dv, y);
+#pragma unroll
+  for (int k = 0; k < 12; ++k) y[k] *= Real(reach);
+  if (d.fit && (d.flags[e] & FLAG_DAMP)) op_damping(d, e, dv, Real(damp), y);
   const uint4 pos = d.slot_pos[e];
   store_slots(slots, make_uint4(pos.x - k0, pos.y - k0, pos.z - k0, pos.w - k0), y);
 }
@@ -1556,11 +1681,18 @@ void Launch<Real>::evaluate(const Dev<Real> &d, hipStream_t s) {
   TMI_LAUNCH(k_evaluate<Real>, d.n_ntiles, s, d);
 }
 template <class Real>
-void Launch<Real>::assemble(const Dev<Real> &d, bool diag, bool energy, hipStream_t s) {
-  if (diag && energy) TMI_LAUNCH((k_assemble<Real, true, true>), d.n_ttiles, s, d);
-  else if (diag) TMI_LAUNCH((k_assemble<Real, true, false>), d.n_ttiles, s, d);
-  else if (energy) TMI_LAUNCH((k_assemble<Real, false, true>), d.n_ttiles, s, d);
-  else TMI_LAUNCH((k_assemble<Real, false, false>), d.n_ttiles, s, d);
+void Launch<Real>::assemble(const Dev<Real> &d, bool diag, bool energy, hipStream_t s, int viscous) {
+  if (d.fit) {
+    if (diag && energy) TMI_LAUNCH((k_assemble<Real, true, true, true>), d.n_ttiles, s, d, viscous);
+    else if (diag) TMI_LAUNCH((k_assemble<Real, true, false, true>), d.n_ttiles, s, d, viscous);
+    else if (energy) TMI_LAUNCH((k_assemble<Real, false, true, true>), d.n_ttiles, s, d, viscous);
+    else TMI_LAUNCH((k_assemble<Real, false, false, true>), d.n_ttiles, s, d, viscous);
+    return;
+  }
+  if (diag && energy) TMI_LAUNCH((k_assemble<Real, true, true, false>), d.n_ttiles, s, d, viscous);
+  else if (diag) TMI_LAUNCH((k_assemble<Real, true, false, false>), d.n_ttiles, s, d, viscous);
+  else if (energy) TMI_LAUNCH((k_assemble<Real, false, true, false>), d.n_ttiles, s, d, viscous);
+  else TMI_LAUNCH((k_assemble<Real, false, false, false>), d.n_ttiles, s, d, viscous);
 }
 template <class Real>
 void Launch<Real>::balance(const Dev<Real> &d, bool diag, bool taken, hipStream_t s) {
@@ -1587,7 +1719,8 @@ void Launch<Real>::pcg_begin(const Dev<Real> &d, hipStream_t s) {
 }
 template <class Real>
 void Launch<Real>::op_element(const Dev<Real> &d, const Real *dir, int parity, hipStream_t s) {
-  TMI_LAUNCH(k_op_element<Real>, d.n_ttiles, s, d, dir, parity);
+  if (d.fit) TMI_LAUNCH((k_op_element<Real, true>), d.n_ttiles, s, d, dir, parity);
+  else TMI_LAUNCH((k_op_element<Real, false>), d.n_ttiles, s, d, dir, parity);
 }
 template <class Real>
 void Launch<Real>::op_gather(const Dev<Real> &d, const Real *dir, Real *out, int parity,
@@ -1684,6 +1817,16 @@ void Launch<Real>::apply_body(const Dev<Real> &d, uint32_t t0, uint32_t t1, uint
   if (n1 > n0)
     TMI_LAUNCH(k_apply_body_gather<Real>, blocks_for(n1 - n0), s, d, n0, n1, k0, slots, din, yout, inertia,
                reach);
+}
+
+template <class Real>
+void Launch<Real>::apply_body_damped(const Dev<Real> &d, uint32_t t0, uint32_t t1, uint32_t n0, uint32_t n1,
+                                     uint32_t k0, const double *din, double *yout, double inertia, double reach,
+                                     double damp, Real *slots, hipStream_t s) {
+  if (t1 > t0)
+    TMI_LAUNCH(k_apply_body_damped<Real>, blocks_for(t1 - t0), s, d, t0, t1, n0, k0, din, reach, damp, slots);
+  if (n1 > n0)
+    TMI_LAUNCH(k_apply_body_gather<Real>, blocks_for(n1 - n0), s, d, n0, n1, k0, slots, din, yout, inertia, 1.0);
 }
 
 template <class Real>

@@ -578,4 +578,115 @@ TM_HD double stretch_of(const double *F, double *direction) {
   return sqrt(w[2] > 0.0 ? w[2] : 0.0);
 }
 
+// ------------------------------------------------------- stiffness damping
+// A tet's damping force is tau P R K R^T P v (applyDamping, factor/factor.cc:
+// 848-857): K the elastic stiffness of the rest gradients and Lame moduli,
+// R the committed rotation, P the removal of the corners' rigid motion.
+// The rigid fit of the corners (rigidFit, factor.cc:783-817): arms a_i from
+// the centroid and J^-1, J = sum (|a_i|^2 I - a_i a_i^T); a nearly singular
+// J (det J <= 1e-6 (tr J / 3)^3) takes the pseudo-inverse over eigenvalues
+// above 1e-6 tr J / 3.  fit = {arms (12), J^-1 upper triangle xx xy xz yy
+// yz zz (6)}.
+constexpr int FIT_PLANES = 18;
+
+TM_HD void rigid_fit(const double *corners, double *fit) {
+  double c[3] = {0.0, 0.0, 0.0};
+#pragma unroll
+  for (int i = 0; i < 4; ++i)
+#pragma unroll
+    for (int r = 0; r < 3; ++r) c[r] += corners[3 * i + r] * 0.25;
+  double J[9] = {0, 0, 0, 0, 0, 0, 0, 0, 0};
+#pragma unroll
+  for (int i = 0; i < 4; ++i) {
+    double *a = &fit[3 * i];
+#pragma unroll
+    for (int r = 0; r < 3; ++r) a[r] = corners[3 * i + r] - c[r];
+    const double aa = a[0] * a[0] + a[1] * a[1] + a[2] * a[2];
+#pragma unroll
+    for (int r = 0; r < 3; ++r)
+#pragma unroll
+      for (int k = 0; k < 3; ++k) J[r * 3 + k] += (r == k ? aa : 0.0) - a[r] * a[k];
+  }
+  const double third = (J[0] + J[4] + J[8]) / 3.0;
+  double inv[9];
+  if (!(det3(J) > 1.0e-6 * third * third * third) || !inv3(J, inv)) {
+    double w[3], V[9];
+    eigen_sym3(J, w, V);
+#pragma unroll
+    for (int k = 0; k < 9; ++k) inv[k] = 0.0;
+    for (int k = 0; k < 3; ++k) {
+      if (!(w[k] > 1.0e-6 * third)) continue;
+#pragma unroll
+      for (int r = 0; r < 3; ++r)
+#pragma unroll
+        for (int q = 0; q < 3; ++q) inv[r * 3 + q] += V[r * 3 + k] * V[q * 3 + k] / w[k];
+    }
+  }
+  fit[12] = inv[0];
+  fit[13] = inv[1];
+  fit[14] = inv[2];
+  fit[15] = inv[4];
+  fit[16] = inv[5];
+  fit[17] = inv[8];
+}
+
+// removeRigid (factor.cc:819-846): v_i -= mean(v) + s x a_i with the spin
+// s = J^-1 sum_i a_i x v_i
+template <class R>
+TM_HD void remove_rigid(const R *fit, R *v) {
+  R mean[3] = {R(0), R(0), R(0)}, m[3] = {R(0), R(0), R(0)};
+#pragma unroll
+  for (int i = 0; i < 4; ++i) {
+    const R *a = &fit[3 * i], *x = &v[3 * i];
+#pragma unroll
+    for (int r = 0; r < 3; ++r) mean[r] += x[r] * R(0.25);
+    m[0] += a[1] * x[2] - a[2] * x[1];
+    m[1] += a[2] * x[0] - a[0] * x[2];
+    m[2] += a[0] * x[1] - a[1] * x[0];
+  }
+  const R *J = &fit[12];
+  const R s[3] = {J[0] * m[0] + J[1] * m[1] + J[2] * m[2], J[1] * m[0] + J[3] * m[1] + J[4] * m[2],
+                  J[2] * m[0] + J[4] * m[1] + J[5] * m[2]};
+#pragma unroll
+  for (int i = 0; i < 4; ++i) {
+    const R *a = &fit[3 * i];
+    v[3 * i + 0] -= mean[0] + (s[1] * a[2] - s[2] * a[1]);
+    v[3 * i + 1] -= mean[1] + (s[2] * a[0] - s[0] * a[2]);
+    v[3 * i + 2] -= mean[2] + (s[0] * a[1] - s[1] * a[0]);
+  }
+}
+
+// out = P R K R^T P v for one tet: with D = sum_j (R^T (P v)_j) g_j^T,
+// out_i = V R (mu (D + D^T) + lambda tr(D) I) g_i, then P again
+template <class R>
+TM_HD void damp_tet(const R *fit, const R *Rm, const R *g, R V, R lam, R mu, const R *v, R *out) {
+  R rel[12];
+#pragma unroll
+  for (int k = 0; k < 12; ++k) rel[k] = v[k];
+  remove_rigid(fit, rel);
+  R D[9] = {R(0), R(0), R(0), R(0), R(0), R(0), R(0), R(0), R(0)};
+#pragma unroll
+  for (int j = 0; j < 4; ++j) {
+    R dp[3];
+    mtv3(Rm, &rel[3 * j], dp);
+#pragma unroll
+    for (int r = 0; r < 3; ++r)
+#pragma unroll
+      for (int k = 0; k < 3; ++k) D[r * 3 + k] += dp[r] * g[3 * j + k];
+  }
+  const R tr = D[0] + D[4] + D[8];
+  R S[9];
+#pragma unroll
+  for (int r = 0; r < 3; ++r)
+#pragma unroll
+    for (int k = 0; k < 3; ++k) S[r * 3 + k] = V * (mu * (D[r * 3 + k] + D[k * 3 + r]) + (r == k ? lam * tr : R(0)));
+#pragma unroll
+  for (int i = 0; i < 4; ++i) {
+    R t[3];
+    mv3(S, &g[3 * i], t);
+    mv3(Rm, t, &out[3 * i]);
+  }
+  remove_rigid(fit, out);
+}
+
 }  // namespace tmi
