@@ -613,6 +613,51 @@ int talus_mi_lattice_surface_get(talus_mi_ctx *ctx, uint32_t lattice, uint32_t *
                                  uint32_t capacity, uint32_t *count);
 /* latticePieces (resolution.cc:198-228): pieces of cells joined by nodes. */
 int talus_mi_lattice_pieces(talus_mi_ctx *ctx, uint32_t lattice, uint32_t *pieces);
+/* The FP64 stable neo-Hookean hex of tier switching and the energy ledger
+ * (src/talus/dynamics/neohookean/neohookean.cc, 8 Gauss points per cell;
+ * moduli per cell from its lane, moduliOf 82-90) at positions x of the
+ * lattice (3 per node; NULL: its current positions).  cell_energy: cellEnergy
+ * with corner 0's displacement taken out of every corner (as latticeEnergy
+ * does, ledger.cc:29-47), one per cell; node_gradient (3 per node) and
+ * node_hessian (9 per node, row-major): nodeTerms (153-196) summed over the
+ * node's cells.  Any output may be NULL. */
+int talus_mi_lattice_snh_terms(talus_mi_ctx *ctx, uint32_t lattice, const double *x, double *cell_energy,
+                               double *node_gradient, double *node_hessian);
+/* Energy (ledger.h:9-15). */
+typedef struct talus_mi_energy {
+  double kinetic;
+  double gravity;
+  double elastic;
+} talus_mi_energy;
+/* latticeEnergy (src/talus/dynamics/ledger/ledger.cc:9-52) at the lattice's
+ * current state: kinetic of free nodes, -m g.x of every node, the cells'
+ * neo-Hookean energies. */
+int talus_mi_lattice_energy(talus_mi_ctx *ctx, uint32_t lattice, const double *gravity,
+                            talus_mi_energy *out);
+/* RestBody (src/talus/dynamics/screen/screen.h:17-25): total mass, mass
+ * centre, inertia about it (row-major), principal axes (columns of a proper
+ * rotation) with their moments ascending, and the smallest rest extent. */
+typedef struct talus_mi_rest_body {
+  double total;
+  double centre[3];
+  double inertia[9];
+  double axes[9];
+  double moments[3];
+  double thickness;
+} talus_mi_rest_body;
+/* restBodyOf (screen.cc:26-83) of the lattice's rest mesh. */
+int talus_mi_lattice_rest_body(talus_mi_ctx *ctx, uint32_t lattice, talus_mi_rest_body *out);
+/* factorScreen + screenLoad (screen.cc:85-254): the displacement (3 per
+ * node) of the lattice's rest stiffness under node forces (3 per node), the
+ * lattice's anchored nodes and `held` fixed (per-DOF shift 1e6 x the largest
+ * stiffness diagonal; free nodes 1e-9 x largest / heaviest x node mass), the
+ * load relieved of its rigid acceleration and the rigid part removed from
+ * the displacement when no node is held, and dent = max |u| / thickness.
+ * Solved on the device by block-Jacobi PCG in FP64 to 1e-12 relative
+ * (iterations: the count taken, may be NULL); TALUS_MI_ERR_STATE when it
+ * does not converge. */
+int talus_mi_lattice_screen(talus_mi_ctx *ctx, uint32_t lattice, const uint32_t *held, uint32_t nheld,
+                            const double *forces, double *displacement, double *dent, uint32_t *iterations);
 /* One solveStrainstep over every lattice of the context, each its own
  * island (no contacts, joints or rigid partners): refreshFlowed of the cells
  * the last step's flow moved, per body islandSubsteps (strainstep.cc:551-567)

@@ -96,6 +96,19 @@ class Support(C.Structure):
         return s
 
 
+class Energy(C.Structure):
+    """talus_mi_energy (ledger.h Energy)."""
+
+    _fields_ = [("kinetic", C.c_double), ("gravity", C.c_double), ("elastic", C.c_double)]
+
+
+class RestBody(C.Structure):
+    """talus_mi_rest_body (screen.h RestBody)."""
+
+    _fields_ = [("total", C.c_double), ("centre", C.c_double * 3), ("inertia", C.c_double * 9),
+                ("axes", C.c_double * 9), ("moments", C.c_double * 3), ("thickness", C.c_double)]
+
+
 class StepParams(C.Structure):
     _fields_ = [
         ("dt", C.c_float), ("gravity", C.c_float * 3), ("substeps", C.c_uint32),
@@ -283,6 +296,12 @@ _SIGNATURES = {
     "talus_mi_lattice_cell_stretch": (C.c_int, [C.c_void_p, C.c_uint32, C.c_uint32, _P(C.c_double),
                                                 _P(C.c_double)]),
     "talus_mi_lattice_pieces": (C.c_int, [C.c_void_p, C.c_uint32, _P(C.c_uint32)]),
+    "talus_mi_lattice_snh_terms": (C.c_int, [C.c_void_p, C.c_uint32, _P(C.c_double), _P(C.c_double),
+                                             _P(C.c_double), _P(C.c_double)]),
+    "talus_mi_lattice_energy": (C.c_int, [C.c_void_p, C.c_uint32, _P(C.c_double), C.c_void_p]),
+    "talus_mi_lattice_rest_body": (C.c_int, [C.c_void_p, C.c_uint32, C.c_void_p]),
+    "talus_mi_lattice_screen": (C.c_int, [C.c_void_p, C.c_uint32, _P(C.c_uint32), C.c_uint32, _P(C.c_double),
+                                          _P(C.c_double), _P(C.c_double), _P(C.c_uint32)]),
     "talus_mi_lattice_lowest_get": (C.c_int, [C.c_void_p, C.c_uint32, _P(C.c_double)]),
     "talus_mi_lattice_surface_get": (C.c_int, [C.c_void_p, C.c_uint32, _P(C.c_uint32), C.c_uint32,
                                                _P(C.c_uint32)]),
@@ -898,6 +917,51 @@ class Context:
         _check(lib().talus_mi_lattice_surface_get(self._h, lattice, out.ctypes.data_as(C.POINTER(C.c_uint32)),
                                                   n.value, C.byref(n)))
         return out[: n.value]
+
+    def lattice_snh_terms(self, lattice, x=None):
+        """(cell energies, node gradients (n, 3), node Hessian blocks (n, 3, 3))
+        of the FP64 neo-Hookean hex at x (default: the current positions)
+        (talus_mi_lattice_snh_terms)."""
+        n = self.lattice_nodes(lattice)
+        cells = self.lattices[lattice][1]
+        xp = None
+        if x is not None:
+            x = np.ascontiguousarray(x, dtype=np.float64).reshape(-1)
+            assert x.size == 3 * n
+            xp = _dp(x)
+        e, g, h = np.empty(cells), np.empty(3 * n), np.empty(9 * n)
+        _check(lib().talus_mi_lattice_snh_terms(self._h, lattice, xp, _dp(e), _dp(g), _dp(h)))
+        return e, g.reshape(n, 3), h.reshape(n, 3, 3)
+
+    def lattice_energy(self, lattice, gravity=(0.0, -9.81, 0.0)):
+        """latticeEnergy: dict of kinetic, gravity, elastic (talus_mi_lattice_energy)."""
+        out = Energy()
+        g = np.ascontiguousarray(gravity, dtype=np.float64)
+        _check(lib().talus_mi_lattice_energy(self._h, lattice, _dp(g), C.byref(out)))
+        return {"kinetic": out.kinetic, "gravity": out.gravity, "elastic": out.elastic}
+
+    def lattice_rest_body(self, lattice):
+        """restBodyOf (talus_mi_lattice_rest_body)."""
+        rb = RestBody()
+        _check(lib().talus_mi_lattice_rest_body(self._h, lattice, C.byref(rb)))
+        return {"total": rb.total, "centre": np.array(rb.centre[:]),
+                "inertia": np.array(rb.inertia[:]).reshape(3, 3), "axes": np.array(rb.axes[:]).reshape(3, 3),
+                "moments": np.array(rb.moments[:]), "thickness": rb.thickness}
+
+    def lattice_screen(self, lattice, forces, held=()):
+        """factorScreen + screenLoad: (displacement (n, 3), dent, iterations)
+        (talus_mi_lattice_screen)."""
+        n = self.lattice_nodes(lattice)
+        f = np.ascontiguousarray(forces, dtype=np.float64).reshape(-1)
+        assert f.size == 3 * n
+        hl = np.ascontiguousarray(np.asarray(held, dtype=np.uint32).reshape(-1))
+        hp = hl.ctypes.data_as(_P(C.c_uint32)) if hl.size else None
+        u = np.empty(3 * n)
+        dent = C.c_double()
+        its = C.c_uint32()
+        _check(lib().talus_mi_lattice_screen(self._h, lattice, hp, hl.size, _dp(f), _dp(u), C.byref(dent),
+                                             C.byref(its)))
+        return u.reshape(n, 3), dent.value, int(its.value)
 
     def lattice_pieces(self, lattice) -> int:
         out = C.c_uint32()

@@ -32,6 +32,7 @@
 #include "host.h"
 #include "device.h"
 #include "hexlattice.h"
+#include "screen.h"
 #include "kernels.h"
 #include "tmath.h"
 
@@ -2480,6 +2481,8 @@ struct talus_mi_ctx {
   std::vector<talus_mi_event> events;
   uint32_t streams = 1;  // engines stepped concurrently (talus_mi_set_streams)
   std::unique_ptr<tmi::LatticeSet> lattices;  // strainstep hex lattices
+  // per lattice: the neo-Hookean hex terms and screen solve (built on use)
+  std::vector<std::unique_ptr<tmi::HexScreen>> screens;
 };
 
 namespace {
@@ -3241,7 +3244,63 @@ bool lattice_ok(talus_mi_ctx *ctx, uint32_t lattice) {
   return ctx->lattices && lattice < ctx->lattices->count();
 }
 
+// the lattice's screen, bound to its current mesh (host mirror downloaded)
+tmi::HexScreen &screen_of(talus_mi_ctx *ctx, uint32_t lattice, tmi::LatticeBody *&body) {
+  body = &ctx->lattices->body(lattice);
+  if (ctx->screens.size() < ctx->lattices->count()) ctx->screens.resize(ctx->lattices->count());
+  auto &slot = ctx->screens[lattice];
+  if (!slot) slot = std::make_unique<tmi::HexScreen>(ctx->stream);
+  slot->bind(*body);
+  return *slot;
+}
+
 }  // namespace
+
+int talus_mi_lattice_snh_terms(talus_mi_ctx *ctx, uint32_t lattice, const double *x, double *cell_energy,
+                               double *node_gradient, double *node_hessian) {
+  if (!ctx || !lattice_ok(ctx, lattice)) return fail(TALUS_MI_ERR_ARGUMENT, "bad arguments");
+  return guarded(ctx, [&]() -> int {
+    tmi::LatticeBody *b = nullptr;
+    tmi::HexScreen &sc = screen_of(ctx, lattice, b);
+    sc.terms(x ? x : b->x.data(), cell_energy, node_gradient, node_hessian);
+    return TALUS_MI_OK;
+  });
+}
+
+int talus_mi_lattice_energy(talus_mi_ctx *ctx, uint32_t lattice, const double *gravity, talus_mi_energy *out) {
+  if (!ctx || !lattice_ok(ctx, lattice) || !gravity || !out) return fail(TALUS_MI_ERR_ARGUMENT, "bad arguments");
+  return guarded(ctx, [&]() -> int {
+    tmi::LatticeBody *b = nullptr;
+    tmi::HexScreen &sc = screen_of(ctx, lattice, b);
+    *out = sc.energy(b->x.data(), b->v.data(), gravity);
+    return TALUS_MI_OK;
+  });
+}
+
+int talus_mi_lattice_rest_body(talus_mi_ctx *ctx, uint32_t lattice, talus_mi_rest_body *out) {
+  if (!ctx || !lattice_ok(ctx, lattice) || !out) return fail(TALUS_MI_ERR_ARGUMENT, "bad arguments");
+  return guarded(ctx, [&]() -> int {
+    tmi::rest_body_of(ctx->lattices->body(lattice), *out);
+    return TALUS_MI_OK;
+  });
+}
+
+int talus_mi_lattice_screen(talus_mi_ctx *ctx, uint32_t lattice, const uint32_t *held, uint32_t nheld,
+                            const double *forces, double *displacement, double *dent, uint32_t *iterations) {
+  if (!ctx || !lattice_ok(ctx, lattice) || !forces || !displacement || !dent || (nheld && !held))
+    return fail(TALUS_MI_ERR_ARGUMENT, "bad arguments");
+  return guarded(ctx, [&]() -> int {
+    tmi::LatticeBody *b = nullptr;
+    tmi::HexScreen &sc = screen_of(ctx, lattice, b);
+    for (uint32_t k = 0; k < nheld; ++k)
+      if (held[k] >= b->nodes) return fail(TALUS_MI_ERR_ARGUMENT, "held node out of range");
+    if (b->nodes == 0) return fail(TALUS_MI_ERR_ARGUMENT, "the lattice has no nodes");
+    const uint32_t limit = std::max<uint32_t>(2000u, 30u * b->nodes);
+    if (!sc.screen(*b, held, nheld, forces, displacement, dent, iterations, 1.0e-12, limit))
+      return fail(TALUS_MI_ERR_STATE, "the screen solve did not converge");
+    return TALUS_MI_OK;
+  });
+}
 
 int talus_mi_lattice_add(talus_mi_ctx *ctx, const talus_mi_hex_mesh *mesh,
                          const talus_mi_material *materials, uint32_t lanes, const double *offset,
