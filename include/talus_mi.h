@@ -658,6 +658,42 @@ int talus_mi_lattice_rest_body(talus_mi_ctx *ctx, uint32_t lattice, talus_mi_res
  * does not converge. */
 int talus_mi_lattice_screen(talus_mi_ctx *ctx, uint32_t lattice, const uint32_t *held, uint32_t nheld,
                             const double *forces, double *displacement, double *dent, uint32_t *iterations);
+/* A sampled signed distance field (SampledSdf, src/talus/field/sampledsdf/
+ * sampledsdf.h:19-32, without its brick summaries): samples x fastest,
+ * index x + resolution[0] (y + resolution[1] z); materials may be NULL. */
+typedef struct talus_mi_sdf {
+  float origin[3];
+  float voxel_size[3];
+  uint32_t resolution[3];
+  float *distances;
+  uint8_t *materials;
+} talus_mi_sdf;
+/* Pose (src/talus/pose.h:9-12): position and rotation quaternion w, x, y, z. */
+typedef struct talus_mi_pose {
+  float position[3];
+  float rotation[4];
+} talus_mi_pose;
+/* SampleBox (sampledsdf.h:34-37). */
+typedef struct talus_mi_sample_box {
+  uint32_t minimum[3];
+  uint32_t extent[3];
+} talus_mi_sample_box;
+/* bakeDeformedField (src/talus/deformable/deformedfield/deformedfield.cc:
+ * 15-102) of the lattice at its current positions: every sample of the
+ * rest field's lattice, placed in the world by the pose, finds the lattice
+ * cell holding it (probeDeformed, contact/contact.cc:1287-1425: Newton
+ * inversion of the deformed trilinear cell, crack faces capping the
+ * distance) and reads the rest field at the matching rest point; samples no
+ * cell reaches keep the rest value (or max(rest, pad) with
+ * carve_beyond_reach).  Samples that differ from `current` by more than
+ * 1e-7 are written into current->distances (in place, on the host) and
+ * `changed` receives their box (extent 0 when none); the caller's field
+ * refreshes its brick summaries over that box (field::write).  current must
+ * share rest's lattice (origin, voxel size, resolution), else
+ * TALUS_MI_ERR_ARGUMENT.  One device thread per sample. */
+int talus_mi_lattice_bake_field(talus_mi_ctx *ctx, uint32_t lattice, const talus_mi_sdf *rest,
+                                const talus_mi_pose *pose, int carve_beyond_reach, talus_mi_sdf *current,
+                                talus_mi_sample_box *changed);
 /* One solveStrainstep over every lattice of the context, each its own
  * island (no contacts, joints or rigid partners): refreshFlowed of the cells
  * the last step's flow moved, per body islandSubsteps (strainstep.cc:551-567)

@@ -109,6 +109,23 @@ class RestBody(C.Structure):
                 ("axes", C.c_double * 9), ("moments", C.c_double * 3), ("thickness", C.c_double)]
 
 
+class Sdf(C.Structure):
+    """talus_mi_sdf (SampledSdf without its brick summaries)."""
+
+    _fields_ = [("origin", C.c_float * 3), ("voxel_size", C.c_float * 3), ("resolution", C.c_uint32 * 3),
+                ("distances", C.POINTER(C.c_float)), ("materials", C.POINTER(C.c_uint8))]
+
+
+class Pose(C.Structure):
+    """talus_mi_pose: position and rotation quaternion w, x, y, z."""
+
+    _fields_ = [("position", C.c_float * 3), ("rotation", C.c_float * 4)]
+
+
+class SampleBox(C.Structure):
+    _fields_ = [("minimum", C.c_uint32 * 3), ("extent", C.c_uint32 * 3)]
+
+
 class StepParams(C.Structure):
     _fields_ = [
         ("dt", C.c_float), ("gravity", C.c_float * 3), ("substeps", C.c_uint32),
@@ -300,6 +317,8 @@ _SIGNATURES = {
                                              _P(C.c_double), _P(C.c_double)]),
     "talus_mi_lattice_energy": (C.c_int, [C.c_void_p, C.c_uint32, _P(C.c_double), C.c_void_p]),
     "talus_mi_lattice_rest_body": (C.c_int, [C.c_void_p, C.c_uint32, C.c_void_p]),
+    "talus_mi_lattice_bake_field": (C.c_int, [C.c_void_p, C.c_uint32, C.c_void_p, C.c_void_p, C.c_int, C.c_void_p,
+                                              C.c_void_p]),
     "talus_mi_lattice_screen": (C.c_int, [C.c_void_p, C.c_uint32, _P(C.c_uint32), C.c_uint32, _P(C.c_double),
                                           _P(C.c_double), _P(C.c_double), _P(C.c_uint32)]),
     "talus_mi_lattice_lowest_get": (C.c_int, [C.c_void_p, C.c_uint32, _P(C.c_double)]),
@@ -947,6 +966,32 @@ class Context:
         return {"total": rb.total, "centre": np.array(rb.centre[:]),
                 "inertia": np.array(rb.inertia[:]).reshape(3, 3), "axes": np.array(rb.axes[:]).reshape(3, 3),
                 "moments": np.array(rb.moments[:]), "thickness": rb.thickness}
+
+    def lattice_bake_field(self, lattice, rest, current=None, position=(0.0, 0.0, 0.0),
+                           rotation=(1.0, 0.0, 0.0, 0.0), carve=False):
+        """bakeDeformedField of the lattice at its current positions
+        (talus_mi_lattice_bake_field).  Fields are dicts of origin, voxel,
+        res (x fastest) and distances (float32); returns (the new current
+        distances, (box minimum, box extent))."""
+        def sdf(field, dist):
+            f = Sdf()
+            f.origin[:] = [float(v) for v in field["origin"]]
+            f.voxel_size[:] = [float(v) for v in field["voxel"]]
+            f.resolution[:] = [int(v) for v in field["res"]]
+            f.distances = dist.ctypes.data_as(_P(C.c_float))
+            f.materials = None
+            return f
+        rd = np.ascontiguousarray(rest["distances"], dtype=np.float32).reshape(-1)
+        cur = rest if current is None else current
+        cd = np.array(cur["distances"], dtype=np.float32).reshape(-1)
+        pose = Pose()
+        pose.position[:] = [float(v) for v in position]
+        pose.rotation[:] = [float(v) for v in rotation]
+        box = SampleBox()
+        rs, cs = sdf(rest, rd), sdf(cur, cd)
+        _check(lib().talus_mi_lattice_bake_field(self._h, lattice, C.byref(rs), C.byref(pose), int(carve),
+                                                 C.byref(cs), C.byref(box)))
+        return cd, (np.array(box.minimum[:], dtype=np.uint32), np.array(box.extent[:], dtype=np.uint32))
 
     def lattice_screen(self, lattice, forces, held=()):
         """factorScreen + screenLoad: (displacement (n, 3), dent, iterations)

@@ -17,7 +17,7 @@ LIB = os.path.join(HERE, "libtalus_mi.so")
 
 HIPCC = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")
 ARCH = "gfx950"
-SOURCES = ["kernels.hip", "factor.hip", "context.hip", "lattice.hip", "screen.hip", "mesh.cpp", "factor_symbolic.cpp",
+SOURCES = ["kernels.hip", "factor.hip", "context.hip", "lattice.hip", "screen.hip", "field.hip", "mesh.cpp", "factor_symbolic.cpp",
            "tear.cpp", "surface.cpp", "hex_build.cpp"]
 COMMON = [
     "-O3", "-fPIC", "-std=c++20", "-Wall", "-Wno-unused-function",

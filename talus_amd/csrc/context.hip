@@ -31,6 +31,7 @@
 #include "factor.h"
 #include "host.h"
 #include "device.h"
+#include "field.h"
 #include "hexlattice.h"
 #include "screen.h"
 #include "kernels.h"
@@ -3281,6 +3282,32 @@ int talus_mi_lattice_rest_body(talus_mi_ctx *ctx, uint32_t lattice, talus_mi_res
   if (!ctx || !lattice_ok(ctx, lattice) || !out) return fail(TALUS_MI_ERR_ARGUMENT, "bad arguments");
   return guarded(ctx, [&]() -> int {
     tmi::rest_body_of(ctx->lattices->body(lattice), *out);
+    return TALUS_MI_OK;
+  });
+}
+
+int talus_mi_lattice_bake_field(talus_mi_ctx *ctx, uint32_t lattice, const talus_mi_sdf *rest,
+                                const talus_mi_pose *pose, int carve_beyond_reach, talus_mi_sdf *current,
+                                talus_mi_sample_box *changed) {
+  if (!ctx || !lattice_ok(ctx, lattice) || !rest || !pose || !current || !changed || !rest->distances ||
+      !current->distances)
+    return fail(TALUS_MI_ERR_ARGUMENT, "bad arguments");
+  for (int k = 0; k < 3; ++k)
+    if (rest->resolution[k] < 2u) return fail(TALUS_MI_ERR_ARGUMENT, "a sampled field needs two samples per axis");
+  return guarded(ctx, [&]() -> int {
+    const tmi::LatticeBody &b = ctx->lattices->body(lattice);
+    // model(pose) (pose.h:14-16): the rotation of the quaternion, then the
+    // translation, in FP64 from the float pose
+    const double w = pose->rotation[0], x = pose->rotation[1], y = pose->rotation[2], z = pose->rotation[3];
+    const double model[12] = {1.0 - 2.0 * (y * y + z * z), 2.0 * (x * y - w * z), 2.0 * (x * z + w * y),
+                              double(pose->position[0]),
+                              2.0 * (x * y + w * z), 1.0 - 2.0 * (x * x + z * z), 2.0 * (y * z - w * x),
+                              double(pose->position[1]),
+                              2.0 * (x * z - w * y), 2.0 * (y * z + w * x), 1.0 - 2.0 * (x * x + y * y),
+                              double(pose->position[2])};
+    if (!tmi::bake_deformed_field(b, b.x.data(), model, *rest, carve_beyond_reach != 0, *current, *changed,
+                                  ctx->stream))
+      return fail(TALUS_MI_ERR_ARGUMENT, "the current field must share the rest field's lattice");
     return TALUS_MI_OK;
   });
 }
