@@ -207,9 +207,24 @@ typedef struct talus_mi_surface {
   uint32_t *edge_nodes;     /* 2*edges */
   uint32_t *surface_nodes;  /* nodes */
   uint8_t *crack_faces;     /* tets */
+  /* clusterSurface (src/talus/deformable/contact/contact.cc:540-597):
+   * triangle indices in Morton order of their rest centres (10 bits per
+   * axis over the centres' box), cut into clusters of 32; cluster c holds
+   * cluster_faces[cluster_starts[c] .. cluster_starts[c + 1]) */
+  uint32_t clusters;
+  uint32_t *cluster_faces;  /* triangles */
+  uint32_t *cluster_starts; /* clusters + 1 */
 } talus_mi_surface;
 int talus_mi_mesh_surface(const talus_mi_mesh *mesh, talus_mi_surface *out);
 void talus_mi_surface_free(talus_mi_surface *surface);
+
+struct talus_mi_sdf; /* the sampled field, defined with the lattices below */
+/* surfaceThickness of collectSurface (src/talus/deformable/solver/
+ * solver.cc:342-351): per triangle (3 node indices each) minus the mean of
+ * the lattice field's distance (field::distance, sampledsdf.cc:377-385, in
+ * float) at its three rest positions (3 per node). */
+int talus_mi_surface_thickness(const struct talus_mi_sdf *lattice, const double *rest, uint32_t triangles,
+                               const uint32_t *triangle_nodes, float *thickness);
 
 /* Tear topology update (tearCells, src/talus/dynamics/strainstep/
  * tearing.cc:485-513, on tets) of a host mesh at positions x (3 per node):

@@ -139,6 +139,8 @@ def lib():
         L.orc_surface.argtypes = [C.c_uint32, C.c_uint32, u32p, u32p, dp, u32p, u32p, u32p, u32p, u32p,
                                   u32p, u32p, C.POINTER(C.c_uint8)]
         L.orc_surface.restype = C.c_uint32
+        L.orc_cluster_surface.argtypes = [u32p, C.c_uint32, dp, u32p, u32p]
+        L.orc_cluster_surface.restype = C.c_uint32
         L.orc_tear_topology.argtypes = [C.c_uint32, C.c_uint32, u32p, dp, dp, C.POINTER(C.c_uint8),
                                         C.c_uint32, u32p, C.c_uint32, u32p]
         L.orc_tear_topology.restype = C.c_uint32
@@ -554,9 +556,23 @@ def surface(tet, rest, lane=None):
     nt = lib().orc_surface(N, T, u32(tet), None if lane is None else u32(lane), _dp(rest), u32(tri),
                            u32(tri_tet), u32(tri_lane), u32(edges), C.byref(ne), u32(surf), C.byref(ns),
                            crack.ctypes.data_as(C.POINTER(C.c_uint8)))
+    faces, starts = cluster_surface(tri[: 3 * nt], rest)
     return {"triangles": tri[: 3 * nt].reshape(-1, 3).copy(), "triangle_tet": tri_tet[:nt].copy(),
             "triangle_lane": tri_lane[:nt].copy(), "edges": edges[: 2 * ne.value].reshape(-1, 2).copy(),
-            "nodes": surf[: ns.value].copy(), "crack_faces": crack[:T].copy()}
+            "nodes": surf[: ns.value].copy(), "crack_faces": crack[:T].copy(),
+            "cluster_faces": faces, "cluster_starts": starts}
+
+
+def cluster_surface(triangles, rest):
+    """(faces, starts) of clusterSurface"""
+    tri = np.ascontiguousarray(triangles, dtype=np.uint32).reshape(-1)
+    rest = np.ascontiguousarray(rest, dtype=np.float64).reshape(-1)
+    n = tri.size // 3
+    faces = np.zeros(n + 1, dtype=np.uint32)
+    starts = np.zeros(n // 32 + 3, dtype=np.uint32)
+    u32 = lambda a: a.ctypes.data_as(C.POINTER(C.c_uint32))
+    c = lib().orc_cluster_surface(u32(tri), n, _dp(rest), u32(faces), u32(starts))
+    return faces[:n].copy(), starts[: c + 1 if n else 0].copy()
 
 
 def eigen_sym3(A):

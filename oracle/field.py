@@ -40,6 +40,9 @@ def lib():
         L.orc_bake_deformed.argtypes = [C.POINTER(HexMesh), u8p, dp, dp, C.POINTER(Sdf), C.POINTER(Sdf), C.c_int,
                                         u32p, u32p]
         L.orc_bake_deformed.restype = C.c_int
+        L.orc_sdf_distance.argtypes = [C.POINTER(Sdf), C.POINTER(C.c_float)]
+        L.orc_sdf_distance.restype = C.c_float
+        L.orc_surface_thickness.argtypes = [C.POINTER(Sdf), dp, C.c_uint32, u32p, C.POINTER(C.c_float)]
         _ready = True
     return L
 
@@ -71,6 +74,22 @@ def sdf_probe(field, point):
     f = _Field(field)
     p = np.ascontiguousarray(point, dtype=np.float64)
     return lib().orc_sdf_probe(C.byref(f.s), _dp(p))
+
+
+def sdf_distance(field, point):
+    f = _Field(field)
+    p = np.ascontiguousarray(point, dtype=np.float32)
+    return lib().orc_sdf_distance(C.byref(f.s), p.ctypes.data_as(C.POINTER(C.c_float)))
+
+
+def surface_thickness(field, rest, triangles):
+    f = _Field(field)
+    r = np.ascontiguousarray(rest, dtype=np.float64).reshape(-1)
+    t = np.ascontiguousarray(triangles, dtype=np.uint32).reshape(-1)
+    out = np.zeros(t.size // 3, dtype=np.float32)
+    lib().orc_surface_thickness(C.byref(f.s), _dp(r), t.size // 3, t.ctypes.data_as(C.POINTER(C.c_uint32)),
+                                out.ctypes.data_as(C.POINTER(C.c_float)))
+    return out
 
 
 def probe_deformed(mesh, field, position, point, crack=None):

@@ -504,3 +504,47 @@ int orc_bake_deformed(const orc_hex_mesh *mesh, const uint8_t *crack, const doub
   probe_free(&p);
   return 1;
 }
+
+float orc_sdf_distance(const orc_sdf *f, const float *p) {
+  float gap2 = 0.0f;
+  for (int k = 0; k < 3; ++k) {
+    const float hi = f->origin[k] + f->voxel[k] * (float)(f->res[k] - 1u);
+    const float c = p[k] < f->origin[k] ? f->origin[k] : (p[k] > hi ? hi : p[k]);
+    const float g = p[k] - c;
+    gap2 += g * g;
+  }
+  const float outside = sqrtf(gap2);
+  if (outside > 0.0f) return outside;
+  uint32_t base[3];
+  float t[3];
+  for (int k = 0; k < 3; ++k) {
+    const float lattice = (p[k] - f->origin[k]) / f->voxel[k];
+    uint32_t b = (uint32_t)floorf(lattice);
+    b = b < f->res[k] - 2u ? b : f->res[k] - 2u;
+    base[k] = b;
+    t[k] = lattice - (float)b;
+  }
+  float c[8];
+  for (uint32_t i = 0; i < 8; ++i)
+    c[i] = f->dist[sample_index(f, base[0] + (i & 1u), base[1] + ((i >> 1) & 1u), base[2] + ((i >> 2) & 1u))];
+  const float x00 = c[0] * (1.0f - t[0]) + c[1] * t[0];
+  const float x10 = c[2] * (1.0f - t[0]) + c[3] * t[0];
+  const float x01 = c[4] * (1.0f - t[0]) + c[5] * t[0];
+  const float x11 = c[6] * (1.0f - t[0]) + c[7] * t[0];
+  const float y0 = x00 * (1.0f - t[1]) + x10 * t[1];
+  const float y1 = x01 * (1.0f - t[1]) + x11 * t[1];
+  return y0 * (1.0f - t[2]) + y1 * t[2];
+}
+
+void orc_surface_thickness(const orc_sdf *f, const double *rest, uint32_t triangles, const uint32_t *tri,
+                           float *thickness) {
+  for (uint32_t t = 0; t < triangles; ++t) {
+    float sum = 0.0f;
+    for (int c = 0; c < 3; ++c) {
+      const double *r = &rest[3 * (size_t)tri[3 * t + c]];
+      const float p[3] = {(float)r[0], (float)r[1], (float)r[2]};
+      sum -= orc_sdf_distance(f, p);
+    }
+    thickness[t] = sum / 3.0f;
+  }
+}

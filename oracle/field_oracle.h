@@ -29,6 +29,14 @@ typedef struct orc_sdf {
  * clamped point's value plus the distance to it outside. */
 double orc_sdf_probe(const orc_sdf *field, const double *point);
 
+/* field::distance (sampledsdf.cc:377-385) in float: the distance to the
+ * lattice's box outside it, else the float trilinear sample. */
+float orc_sdf_distance(const orc_sdf *field, const float *point);
+/* surfaceThickness of collectSurface (solver.cc:342-351): per triangle
+ * minus the mean float distance at its three rest positions. */
+void orc_surface_thickness(const orc_sdf *field, const double *rest, uint32_t triangles, const uint32_t *tri,
+                           float *thickness);
+
 /* probeDeformed (contact.cc:1287-1425) of one point against the lattice
  * (rest positions of the mesh, hex nodes, crack bits per hex or NULL) placed
  * at `position`: 1 and the distance, the element and its local coordinates

@@ -3034,3 +3034,56 @@ uint32_t orc_surface(uint32_t nodes, uint32_t tets, const uint32_t *tet, const u
   free(faces);
   return nt;
 }
+
+/* clusterSurface (contact.cc:540-597; spreadBits 483-491: 10 bits to every
+ * third bit) */
+static uint32_t spread10(uint32_t v) {
+  v &= 0x3ffu;
+  v = (v | (v << 16)) & 0x030000ffu;
+  v = (v | (v << 8)) & 0x0300f00fu;
+  v = (v | (v << 4)) & 0x030c30c3u;
+  v = (v | (v << 2)) & 0x09249249u;
+  return v;
+}
+
+static int cmp_keyed(const void *a, const void *b) {
+  const uint64_t x = *(const uint64_t *)a, y = *(const uint64_t *)b;
+  return x < y ? -1 : (x > y ? 1 : 0);
+}
+
+uint32_t orc_cluster_surface(const uint32_t *tri, uint32_t ntri, const double *rest, uint32_t *faces,
+                             uint32_t *starts) {
+  if (ntri == 0) return 0;
+  float *centre = (float *)malloc(3 * (size_t)ntri * sizeof(float));
+  uint64_t *keyed = (uint64_t *)malloc((size_t)ntri * sizeof(uint64_t));
+  float low[3] = {FLT_MAX, FLT_MAX, FLT_MAX}, high[3] = {-FLT_MAX, -FLT_MAX, -FLT_MAX};
+  for (uint32_t f = 0; f < ntri; ++f)
+    for (int k = 0; k < 3; ++k) {
+      const float sum = (float)rest[3 * (size_t)tri[3 * f] + k] + (float)rest[3 * (size_t)tri[3 * f + 1] + k] +
+                        (float)rest[3 * (size_t)tri[3 * f + 2] + k];
+      const float c = sum / 3.0f;
+      centre[3 * f + k] = c;
+      low[k] = c < low[k] ? c : low[k];
+      high[k] = c > high[k] ? c : high[k];
+    }
+  float span[3];
+  for (int k = 0; k < 3; ++k) span[k] = high[k] - low[k] > FLT_MIN ? high[k] - low[k] : FLT_MIN;
+  for (uint32_t f = 0; f < ntri; ++f) {
+    uint32_t cell[3];
+    for (int k = 0; k < 3; ++k) {
+      float t = (centre[3 * f + k] - low[k]) / span[k];
+      t = t < 0.0f ? 0.0f : (t > 1.0f ? 1.0f : t);
+      cell[k] = (uint32_t)(t * 1023.0f);
+    }
+    const uint32_t key = spread10(cell[0]) | (spread10(cell[1]) << 1) | (spread10(cell[2]) << 2);
+    keyed[f] = ((uint64_t)key << 32) | f; /* key, then face */
+  }
+  qsort(keyed, ntri, sizeof(uint64_t), cmp_keyed);
+  for (uint32_t f = 0; f < ntri; ++f) faces[f] = (uint32_t)(keyed[f] & 0xffffffffu);
+  uint32_t c = 0;
+  for (uint32_t s = 0; s < ntri; s += 32) starts[c++] = s;
+  starts[c] = ntri;
+  free(centre);
+  free(keyed);
+  return c;
+}

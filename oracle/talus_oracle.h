@@ -204,6 +204,11 @@ int orc_rollout_backward(orc_body *body, const double *x_bar, const double *v_ba
 /* y = K(pos) d with K the neo-Hookean Hessian of the elastic energy. */
 void orc_hessian_apply(orc_body *body, const double *pos, const double *d, double *y);
 
+/* clusterSurface (contact.cc:540-597): the ntri triangles (3 nodes each)
+ * in Morton order of their rest centres (float arithmetic on the float rest
+ * samples), starts every 32 and the count last; returns the clusters. */
+uint32_t orc_cluster_surface(const uint32_t *tri, uint32_t ntri, const double *rest, uint32_t *faces,
+                             uint32_t *starts);
 /* Boundary surface and crack faces of a tet mesh (see talus_oracle.c). */
 uint32_t orc_surface(uint32_t nodes, uint32_t tets, const uint32_t *tet, const uint32_t *lane,
                      const double *rest, uint32_t *tri, uint32_t *tri_tet, uint32_t *tri_lane,

@@ -289,6 +289,8 @@ _SIGNATURES = {
     "talus_mi_plan_set": (C.c_int, [C.c_void_p, _P(C.c_uint8), C.c_uint32]),
     "talus_mi_set_streams": (C.c_int, [C.c_void_p, C.c_uint32]),
     "talus_mi_mesh_surface": (C.c_int, [C.c_void_p, C.c_void_p]),
+    "talus_mi_surface_thickness": (C.c_int, [C.c_void_p, _P(C.c_double), C.c_uint32, _P(C.c_uint32),
+                                             _P(C.c_float)]),
     "talus_mi_surface_get": (C.c_int, [C.c_void_p, C.c_uint32, C.c_void_p]),
     "talus_mi_surface_free": (None, [C.c_void_p]),
     "talus_mi_stream": (C.c_void_p, [C.c_void_p]),
@@ -485,7 +487,8 @@ class _Surface(C.Structure):
                 ("tets", C.c_uint32), ("triangle_nodes", C.POINTER(C.c_uint32)),
                 ("triangle_tet", C.POINTER(C.c_uint32)), ("triangle_lane", C.POINTER(C.c_uint32)),
                 ("edge_nodes", C.POINTER(C.c_uint32)), ("surface_nodes", C.POINTER(C.c_uint32)),
-                ("crack_faces", C.POINTER(C.c_uint8))]
+                ("crack_faces", C.POINTER(C.c_uint8)), ("clusters", C.c_uint32),
+                ("cluster_faces", C.POINTER(C.c_uint32)), ("cluster_starts", C.POINTER(C.c_uint32))]
 
 
 def _surface_out(s: _Surface) -> dict:
@@ -497,7 +500,26 @@ def _surface_out(s: _Surface) -> dict:
             "triangle_lane": arr(s.triangle_lane, s.triangles),
             "edges": arr(s.edge_nodes, 2 * s.edges).reshape(-1, 2),
             "nodes": arr(s.surface_nodes, s.nodes),
-            "crack_faces": arr(s.crack_faces, s.tets)}
+            "crack_faces": arr(s.crack_faces, s.tets),
+            "cluster_faces": arr(s.cluster_faces, s.triangles),
+            "cluster_starts": arr(s.cluster_starts, s.clusters + 1 if s.triangles else 0)}
+
+
+def surface_thickness(field, rest, triangles):
+    """surfaceThickness of a surface's triangles over a lattice field (a dict
+    of origin, voxel, res, distances) at rest positions (talus_mi_surface_thickness)."""
+    dist = np.ascontiguousarray(field["distances"], dtype=np.float32).reshape(-1)
+    f = Sdf()
+    f.origin[:] = [float(v) for v in field["origin"]]
+    f.voxel_size[:] = [float(v) for v in field["voxel"]]
+    f.resolution[:] = [int(v) for v in field["res"]]
+    f.distances = dist.ctypes.data_as(_P(C.c_float))
+    r = np.ascontiguousarray(rest, dtype=np.float64).reshape(-1)
+    t = np.ascontiguousarray(triangles, dtype=np.uint32).reshape(-1)
+    out = np.zeros(t.size // 3, dtype=np.float32)
+    _check(lib().talus_mi_surface_thickness(C.byref(f), _dp(r), t.size // 3, t.ctypes.data_as(_P(C.c_uint32)),
+                                            out.ctypes.data_as(_P(C.c_float))))
+    return out
 
 
 def mesh_surface(mesh) -> dict:

@@ -2638,8 +2638,11 @@ int export_surface(const tmi::SurfaceMesh &sm, uint32_t tets, talus_mi_surface *
   out->edge_nodes = copy(sm.edges);
   out->surface_nodes = copy(sm.nodes);
   out->crack_faces = copy(sm.crack);
+  out->clusters = sm.cluster_starts.empty() ? 0u : uint32_t(sm.cluster_starts.size() - 1);
+  out->cluster_faces = copy(sm.cluster_faces);
+  out->cluster_starts = copy(sm.cluster_starts);
   if (!out->triangle_nodes || !out->triangle_tet || !out->triangle_lane || !out->edge_nodes ||
-      !out->surface_nodes || !out->crack_faces) {
+      !out->surface_nodes || !out->crack_faces || !out->cluster_faces || !out->cluster_starts) {
     talus_mi_surface_free(out);
     return fail(TALUS_MI_ERR_MEMORY, "out of host memory");
   }
@@ -2673,6 +2676,16 @@ int talus_mi_surface_get(talus_mi_ctx *ctx, uint32_t body, talus_mi_surface *out
   });
 }
 
+int talus_mi_surface_thickness(const talus_mi_sdf *lattice, const double *rest, uint32_t triangles,
+                               const uint32_t *triangle_nodes, float *thickness) {
+  if (!lattice || !lattice->distances || (triangles && (!rest || !triangle_nodes || !thickness)))
+    return fail(TALUS_MI_ERR_ARGUMENT, "bad arguments");
+  for (int k = 0; k < 3; ++k)
+    if (lattice->resolution[k] < 2u) return fail(TALUS_MI_ERR_ARGUMENT, "a sampled field needs two samples per axis");
+  tmi::surface_thickness(*lattice, rest, triangles, triangle_nodes, thickness);
+  return TALUS_MI_OK;
+}
+
 void talus_mi_surface_free(talus_mi_surface *s) {
   if (!s) return;
   std::free(s->triangle_nodes);
@@ -2681,6 +2694,8 @@ void talus_mi_surface_free(talus_mi_surface *s) {
   std::free(s->edge_nodes);
   std::free(s->surface_nodes);
   std::free(s->crack_faces);
+  std::free(s->cluster_faces);
+  std::free(s->cluster_starts);
   std::memset(s, 0, sizeof *s);
 }
 

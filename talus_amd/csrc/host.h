@@ -63,9 +63,24 @@ struct SurfaceMesh {
   std::vector<uint32_t> edges;  // 2 per edge
   std::vector<uint32_t> nodes;
   std::vector<uint8_t> crack;   // per tet
+  // clusterSurface (contact.cc:540-597): triangles in Morton order of their
+  // rest centres, SURFACE_CLUSTER = 32 to a cluster; starts end with the
+  // triangle count
+  std::vector<uint32_t> cluster_faces, cluster_starts;
 };
 void mesh_surface(uint32_t nodes, uint32_t tets, const uint32_t *tet, const uint32_t *lane,
                   const double *rest, SurfaceMesh &out);
+// field::distance (sampledsdf.cc:377-385) in float: the distance to the
+// lattice's box outside it, else the trilinear sample
+float sdf_distance(const talus_mi_sdf &field, const float *point);
+// surfaceThickness of collectSurface (solver.cc:342-351): minus the mean
+// field distance at each triangle's three rest positions
+void surface_thickness(const talus_mi_sdf &field, const double *rest, uint32_t triangles,
+                       const uint32_t *triangle_nodes, float *thickness);
+// clusterSurface of outward triangles (3 node indices each) over rest
+// positions that hold float samples
+void cluster_surface(const std::vector<uint32_t> &triangles, const double *rest, std::vector<uint32_t> &faces,
+                     std::vector<uint32_t> &starts);
 
 // Tear topology update (tear.cpp): splits nodes around the torn tets
 // (ascending body-local indices) at the body's positions b.x; new nodes are

@@ -9,13 +9,17 @@
 // the tet's fourth node, so every triangle faces outward.  Triangles come in
 // ascending order of their sorted node triple, edges are the triangles'
 // sorted node pairs without repeats, surface nodes the triangles' nodes in
-// ascending order.  A crack face is a surface face whose three rest
+// ascending order, and in clusters of 32 along the Morton order of their
+// rest centres (clusterSurface, src/talus/deformable/contact/contact.cc:
+// 540-597).  A crack face is a surface face whose three rest
 // positions are those of another surface face: the two sides of a split made
 // by a tear (split nodes keep their origin's rest position).
 #include <algorithm>
 #include <array>
+#include <cmath>
 #include <cstdint>
 #include <cstring>
+#include <limits>
 #include <vector>
 
 #include "host.h"
@@ -112,6 +116,99 @@ void mesh_surface(uint32_t nodes, uint32_t tets, const uint32_t *tet, const uint
         out.crack[face->tet] = uint8_t(out.crack[face->tet] | (1u << face->which));
       }
     at = next;
+  }
+  cluster_surface(out.triangles, rest, out.cluster_faces, out.cluster_starts);
+}
+
+// 10 bits of v spread to every third bit (bit i to bit 3i)
+static uint32_t spread_bits(uint32_t v) {
+  v &= 0x3ffu;
+  v = (v | (v << 16)) & 0x030000ffu;
+  v = (v | (v << 8)) & 0x0300f00fu;
+  v = (v | (v << 4)) & 0x030c30c3u;
+  v = (v | (v << 2)) & 0x09249249u;
+  return v;
+}
+
+void cluster_surface(const std::vector<uint32_t> &triangles, const double *rest, std::vector<uint32_t> &faces,
+                     std::vector<uint32_t> &starts) {
+  constexpr uint32_t SURFACE_CLUSTER = 32;  // contact.h:81
+  const size_t T = triangles.size() / 3;
+  faces.clear();
+  starts.clear();
+  if (T == 0) return;
+  // the rest positions are the mesh's float samples: centres in float
+  std::vector<std::array<float, 3>> centre(T);
+  float low[3], high[3];
+  for (int k = 0; k < 3; ++k) {
+    low[k] = std::numeric_limits<float>::max();
+    high[k] = std::numeric_limits<float>::lowest();
+  }
+  for (size_t f = 0; f < T; ++f)
+    for (int k = 0; k < 3; ++k) {
+      const float sum = float(rest[3 * size_t(triangles[3 * f]) + k]) + float(rest[3 * size_t(triangles[3 * f + 1]) + k]) +
+                        float(rest[3 * size_t(triangles[3 * f + 2]) + k]);
+      centre[f][k] = sum / 3.0f;
+      low[k] = std::min(low[k], centre[f][k]);
+      high[k] = std::max(high[k], centre[f][k]);
+    }
+  float span[3];
+  for (int k = 0; k < 3; ++k) span[k] = std::max(high[k] - low[k], std::numeric_limits<float>::min());
+  std::vector<std::pair<uint32_t, uint32_t>> keyed(T);
+  for (size_t f = 0; f < T; ++f) {
+    uint32_t cell[3];
+    for (int k = 0; k < 3; ++k) {
+      const float t = std::clamp((centre[f][k] - low[k]) / span[k], 0.0f, 1.0f);
+      cell[k] = uint32_t(t * 1023.0f);
+    }
+    keyed[f] = {spread_bits(cell[0]) | (spread_bits(cell[1]) << 1) | (spread_bits(cell[2]) << 2), uint32_t(f)};
+  }
+  std::sort(keyed.begin(), keyed.end());
+  faces.reserve(T);
+  for (const auto &k : keyed) faces.push_back(k.second);
+  for (size_t s = 0; s < T; s += SURFACE_CLUSTER) starts.push_back(uint32_t(s));
+  starts.push_back(uint32_t(T));
+}
+
+float sdf_distance(const talus_mi_sdf &f, const float *p) {
+  float gap2 = 0.0f;
+  float clamped[3];
+  for (int k = 0; k < 3; ++k) {
+    const float hi = f.origin[k] + f.voxel_size[k] * float(f.resolution[k] - 1u);
+    clamped[k] = std::clamp(p[k], f.origin[k], hi);
+    const float g = p[k] - clamped[k];
+    gap2 += g * g;
+  }
+  const float outside = std::sqrt(gap2);
+  if (outside > 0.0f) return outside;
+  uint32_t base[3];
+  float t[3];
+  for (int k = 0; k < 3; ++k) {
+    const float lattice = (p[k] - f.origin[k]) / f.voxel_size[k];
+    base[k] = std::min(uint32_t(std::floor(lattice)), f.resolution[k] - 2u);
+    t[k] = lattice - float(base[k]);
+  }
+  float c[8];
+  for (uint32_t i = 0; i < 8; ++i) {
+    const size_t x = base[0] + (i & 1u), y = base[1] + ((i >> 1) & 1u), z = base[2] + ((i >> 2) & 1u);
+    c[i] = f.distances[x + size_t(f.resolution[0]) * (y + size_t(f.resolution[1]) * z)];
+  }
+  auto mix = [](float a, float b, float s) { return a * (1.0f - s) + b * s; };
+  const float x00 = mix(c[0], c[1], t[0]), x10 = mix(c[2], c[3], t[0]);
+  const float x01 = mix(c[4], c[5], t[0]), x11 = mix(c[6], c[7], t[0]);
+  return mix(mix(x00, x10, t[1]), mix(x01, x11, t[1]), t[2]);
+}
+
+void surface_thickness(const talus_mi_sdf &field, const double *rest, uint32_t triangles,
+                       const uint32_t *triangle_nodes, float *thickness) {
+  for (uint32_t f = 0; f < triangles; ++f) {
+    float sum = 0.0f;
+    for (int c = 0; c < 3; ++c) {
+      const double *r = &rest[3 * size_t(triangle_nodes[3 * size_t(f) + c])];
+      const float p[3] = {float(r[0]), float(r[1]), float(r[2])};
+      sum -= sdf_distance(field, p);
+    }
+    thickness[f] = sum / 3.0f;
   }
 }
 

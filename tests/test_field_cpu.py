@@ -166,3 +166,31 @@ def test_carving_beyond_reach_lifts_unplaced_samples_to_the_pad():
     assert np.array_equal(baked, np.maximum(field["distances"], pad))
     kept, (_, ext) = FD.bake_deformed(mesh, field, pos)
     assert np.array_equal(kept, field["distances"]) and (ext == 0).all()
+
+
+def test_surface_thickness_is_minus_the_mean_field_distance_at_the_corners():
+    """surfaceThickness (solver.cc:342-351) with field::distance in float:
+    a linear field is reproduced by the trilinear read, points beyond the
+    lattice read their distance to it; the library equals the oracle bit for
+    bit."""
+    from talus_amd import api
+    field = {"origin": np.array([-0.2, -0.1, -0.15], np.float32),
+             "voxel": np.array([0.05, 0.04, 0.05], np.float32), "res": np.array([9, 6, 7], np.uint32)}
+    a, c = np.array([0.4, -0.9, 0.2]), -0.02
+    field["distances"] = (sample_points(field) @ a + c).astype(np.float32)
+    rng = np.random.default_rng(3)
+    lo = field["origin"].astype(np.float64)
+    hi = lo + field["voxel"] * (field["res"] - 1)
+    rest = lo + rng.uniform(0.0, 1.0, (40, 3)) * (hi - lo)
+    rest[:3] = hi + np.array([[0.1, 0.0, 0.0], [0.0, 0.2, 0.0], [0.05, 0.05, 0.05]])
+    rest = rest.astype(np.float32).astype(np.float64)
+    tri = rng.integers(0, 40, (30, 3)).astype(np.uint32)
+    tri[0] = [0, 1, 2]
+    lib = api.surface_thickness(field, rest, tri)
+    orc = FD.surface_thickness(field, rest, tri)
+    assert np.array_equal(lib, orc)
+    inside = np.all(tri >= 3, axis=1)
+    expect = -(rest[tri] @ a + c).mean(axis=1)
+    assert np.abs(lib[inside] - expect[inside]).max() <= 1e-6
+    out = rest[:3] - np.clip(rest[:3], lo, hi)
+    assert lib[0] == pytest.approx(-np.linalg.norm(out, axis=1).mean(), rel=1e-5)
