@@ -441,6 +441,14 @@ int talus_mi_tune(talus_mi_ctx *ctx, uint32_t repeats, talus_mi_tune_report *rep
 int talus_mi_plan_get(talus_mi_ctx *ctx, uint8_t *plan, uint32_t capacity, uint32_t *count);
 int talus_mi_plan_set(talus_mi_ctx *ctx, const uint8_t *plan, uint32_t count);
 
+struct talus_mi_hex_mesh; /* the hex mesh, defined with the lattices below */
+/* collectSurface's hex faces (src/talus/deformable/solver/solver.cc:252-
+ * 303) with clusterSurface, for a host hex mesh: each cell face split in two
+ * triangles around its lowest node, kept when no other cell has it, turned
+ * outward; triangle_tet holds the cell, tets the cell count, surface_nodes
+ * the corners of unshared cell faces, crack_faces zero. */
+int talus_mi_hex_mesh_surface(const struct talus_mi_hex_mesh *mesh, talus_mi_surface *out);
+
 /* Surface of a context body's current mesh (after its tears), as
  * talus_mi_mesh_surface with the FP64 rest positions. */
 int talus_mi_surface_get(talus_mi_ctx *ctx, uint32_t body, talus_mi_surface *out);
@@ -626,6 +634,10 @@ int talus_mi_lattice_lowest_get(talus_mi_ctx *ctx, uint32_t lattice, double *low
  * written when nodes is not NULL and capacity holds them. */
 int talus_mi_lattice_surface_get(talus_mi_ctx *ctx, uint32_t lattice, uint32_t *nodes,
                                  uint32_t capacity, uint32_t *count);
+/* The hex surface (talus_mi_hex_mesh_surface) of a lattice's current mesh
+ * after its tears, with its surface nodes (surfaceNodes) and crack faces
+ * (bit 2 axis + side per cell). */
+int talus_mi_lattice_surface_mesh(talus_mi_ctx *ctx, uint32_t lattice, talus_mi_surface *out);
 /* latticePieces (resolution.cc:198-228): pieces of cells joined by nodes. */
 int talus_mi_lattice_pieces(talus_mi_ctx *ctx, uint32_t lattice, uint32_t *pieces);
 /* The FP64 stable neo-Hookean hex of tier switching and the energy ledger

@@ -43,6 +43,8 @@ def lib():
         L.orc_sdf_distance.argtypes = [C.POINTER(Sdf), C.POINTER(C.c_float)]
         L.orc_sdf_distance.restype = C.c_float
         L.orc_surface_thickness.argtypes = [C.POINTER(Sdf), dp, C.c_uint32, u32p, C.POINTER(C.c_float)]
+        L.orc_hex_surface.argtypes = [C.POINTER(HexMesh), u32p, u32p, u32p, u32p, u32p]
+        L.orc_hex_surface.restype = C.c_uint32
         _ready = True
     return L
 
@@ -80,6 +82,25 @@ def sdf_distance(field, point):
     f = _Field(field)
     p = np.ascontiguousarray(point, dtype=np.float32)
     return lib().orc_sdf_distance(C.byref(f.s), p.ctypes.data_as(C.POINTER(C.c_float)))
+
+
+def hex_surface(mesh):
+    """collectSurface's hex faces: triangles, their cells and lanes, edges
+    and the clusters of clusterSurface"""
+    from . import cluster_surface
+    mm = _Mesh(mesh)
+    H = len(mesh["hex"])
+    u32 = lambda a: a.ctypes.data_as(C.POINTER(C.c_uint32))
+    tri = np.zeros(36 * H + 3, np.uint32)
+    cell = np.zeros(12 * H + 1, np.uint32)
+    lane = np.zeros(12 * H + 1, np.uint32)
+    edges = np.zeros(72 * H + 2, np.uint32)
+    ne = C.c_uint32()
+    nt = lib().orc_hex_surface(C.byref(mm.m), u32(tri), u32(cell), u32(lane), u32(edges), C.byref(ne))
+    faces, starts = cluster_surface(tri[: 3 * nt], mesh["rest"])
+    return {"triangles": tri[: 3 * nt].reshape(-1, 3).copy(), "triangle_tet": cell[:nt].copy(),
+            "triangle_lane": lane[:nt].copy(), "edges": edges[: 2 * ne.value].reshape(-1, 2).copy(),
+            "cluster_faces": faces, "cluster_starts": starts}
 
 
 def surface_thickness(field, rest, triangles):

@@ -548,3 +548,99 @@ void orc_surface_thickness(const orc_sdf *f, const double *rest, uint32_t triang
     thickness[t] = sum / 3.0f;
   }
 }
+
+typedef struct {
+  uint32_t key[3], ordered[3], opposite, cell;
+} hex_tri_t;
+
+static int cmp_hex_tri(const void *a, const void *b) {
+  const hex_tri_t *x = (const hex_tri_t *)a, *y = (const hex_tri_t *)b;
+  for (int k = 0; k < 3; ++k)
+    if (x->key[k] != y->key[k]) return x->key[k] < y->key[k] ? -1 : 1;
+  return x->cell < y->cell ? -1 : (x->cell > y->cell ? 1 : 0);
+}
+
+static int cmp_pair(const void *a, const void *b) {
+  const uint32_t *x = (const uint32_t *)a, *y = (const uint32_t *)b;
+  if (x[0] != y[0]) return x[0] < y[0] ? -1 : 1;
+  return x[1] < y[1] ? -1 : (x[1] > y[1] ? 1 : 0);
+}
+
+static void sort3(uint32_t *v) {
+  for (int i = 0; i < 3; ++i)
+    for (int j = i + 1; j < 3; ++j)
+      if (v[j] < v[i]) {
+        const uint32_t t = v[i];
+        v[i] = v[j];
+        v[j] = t;
+      }
+}
+
+uint32_t orc_hex_surface(const orc_hex_mesh *m, uint32_t *tri, uint32_t *cell, uint32_t *lane, uint32_t *edges,
+                         uint32_t *nedges) {
+  static const int HEX_FACES[6][5] = {{0, 2, 6, 4, 1}, {1, 3, 7, 5, 0}, {0, 1, 5, 4, 2},
+                                      {2, 3, 7, 6, 0}, {0, 1, 3, 2, 4}, {4, 5, 7, 6, 0}};
+  const size_t H = m->hexes;
+  hex_tri_t *t = (hex_tri_t *)malloc((12 * H + 1) * sizeof(hex_tri_t));
+  size_t n = 0;
+  for (size_t c = 0; c < H; ++c)
+    for (int f = 0; f < 6; ++f) {
+      const uint32_t *h = &m->hex[8 * c];
+      const uint32_t quad[4] = {h[HEX_FACES[f][0]], h[HEX_FACES[f][1]], h[HEX_FACES[f][2]], h[HEX_FACES[f][3]]};
+      int low = 0;
+      for (int k = 1; k < 4; ++k)
+        if (quad[k] < quad[low]) low = k;
+      const int split[2][3] = {{low, (low + 1) % 4, (low + 2) % 4}, {low, (low + 2) % 4, (low + 3) % 4}};
+      for (int s = 0; s < 2; ++s) {
+        hex_tri_t *x = &t[n++];
+        for (int k = 0; k < 3; ++k) x->ordered[k] = x->key[k] = quad[split[s][k]];
+        sort3(x->key);
+        x->opposite = h[HEX_FACES[f][4]];
+        x->cell = (uint32_t)c;
+      }
+    }
+  qsort(t, n, sizeof(hex_tri_t), cmp_hex_tri);
+  uint32_t nt = 0, ne = 0;
+  for (size_t at = 0; at < n;) {
+    size_t next = at + 1;
+    while (next < n && memcmp(t[next].key, t[at].key, sizeof t[at].key) == 0) ++next;
+    if (next - at == 1) {
+      const hex_tri_t *x = &t[at];
+      const uint32_t a = x->ordered[0], b = x->ordered[1], c = x->ordered[2];
+      float u[3], w[3], o[3];
+      for (int k = 0; k < 3; ++k) {
+        const float ra = (float)m->rest[3 * (size_t)a + k];
+        u[k] = (float)m->rest[3 * (size_t)b + k] - ra;
+        w[k] = (float)m->rest[3 * (size_t)c + k] - ra;
+        o[k] = (float)m->rest[3 * (size_t)x->opposite + k] - ra;
+      }
+      const float nn[3] = {u[1] * w[2] - u[2] * w[1], u[2] * w[0] - u[0] * w[2], u[0] * w[1] - u[1] * w[0]};
+      const int inward = nn[0] * o[0] + nn[1] * o[1] + nn[2] * o[2] > 0.0f;
+      tri[3 * nt] = a;
+      tri[3 * nt + 1] = inward ? c : b;
+      tri[3 * nt + 2] = inward ? b : c;
+      cell[nt] = x->cell;
+      lane[nt] = m->lane ? m->lane[x->cell] : 0u;
+      ++nt;
+      const uint32_t *s = x->key;
+      const uint32_t pairs[3][2] = {{s[0], s[1]}, {s[0], s[2]}, {s[1], s[2]}};
+      for (int k = 0; k < 3; ++k) {
+        edges[2 * ne] = pairs[k][0];
+        edges[2 * ne + 1] = pairs[k][1];
+        ++ne;
+      }
+    }
+    at = next;
+  }
+  qsort(edges, ne, 2 * sizeof(uint32_t), cmp_pair);
+  uint32_t kept = 0;
+  for (uint32_t i = 0; i < ne; ++i)
+    if (kept == 0 || edges[2 * i] != edges[2 * (kept - 1)] || edges[2 * i + 1] != edges[2 * (kept - 1) + 1]) {
+      edges[2 * kept] = edges[2 * i];
+      edges[2 * kept + 1] = edges[2 * i + 1];
+      ++kept;
+    }
+  *nedges = kept;
+  free(t);
+  return nt;
+}

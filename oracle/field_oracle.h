@@ -37,6 +37,15 @@ float orc_sdf_distance(const orc_sdf *field, const float *point);
 void orc_surface_thickness(const orc_sdf *field, const double *rest, uint32_t triangles, const uint32_t *tri,
                            float *thickness);
 
+/* collectSurface's hex faces (solver.cc:252-303): each cell face split in
+ * two triangles around its lowest node, kept when no other cell has it,
+ * turned outward by the cell's opposite corner (float rest positions);
+ * triangles in ascending order of their sorted triple with their cell and
+ * lane, edges the sorted unique pairs.  Returns the triangle count; tri,
+ * cell, lane take up to 12 per cell, edges up to 36 per cell. */
+uint32_t orc_hex_surface(const orc_hex_mesh *mesh, uint32_t *tri, uint32_t *cell, uint32_t *lane,
+                         uint32_t *edges, uint32_t *nedges);
+
 /* probeDeformed (contact.cc:1287-1425) of one point against the lattice
  * (rest positions of the mesh, hex nodes, crack bits per hex or NULL) placed
  * at `position`: 1 and the distance, the element and its local coordinates

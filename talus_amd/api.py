@@ -289,6 +289,8 @@ _SIGNATURES = {
     "talus_mi_plan_set": (C.c_int, [C.c_void_p, _P(C.c_uint8), C.c_uint32]),
     "talus_mi_set_streams": (C.c_int, [C.c_void_p, C.c_uint32]),
     "talus_mi_mesh_surface": (C.c_int, [C.c_void_p, C.c_void_p]),
+    "talus_mi_hex_mesh_surface": (C.c_int, [C.c_void_p, C.c_void_p]),
+    "talus_mi_lattice_surface_mesh": (C.c_int, [C.c_void_p, C.c_uint32, C.c_void_p]),
     "talus_mi_surface_thickness": (C.c_int, [C.c_void_p, _P(C.c_double), C.c_uint32, _P(C.c_uint32),
                                              _P(C.c_float)]),
     "talus_mi_surface_get": (C.c_int, [C.c_void_p, C.c_uint32, C.c_void_p]),
@@ -520,6 +522,43 @@ def surface_thickness(field, rest, triangles):
     _check(lib().talus_mi_surface_thickness(C.byref(f), _dp(r), t.size // 3, t.ctypes.data_as(_P(C.c_uint32)),
                                             out.ctypes.data_as(_P(C.c_float))))
     return out
+
+
+def _hex_mesh_in(mesh, keep) -> _HexMesh:
+    """a talus_mi_hex_mesh over the dict's arrays (kept alive in `keep`)"""
+    keep.update({
+        "rest": np.ascontiguousarray(mesh["rest_positions"], dtype=np.float32).reshape(-1),
+        "mass": np.ascontiguousarray(mesh["node_mass"], dtype=np.float32),
+        "anchored": np.ascontiguousarray(mesh["anchored"], dtype=np.uint8),
+        "hex": np.ascontiguousarray(mesh["hex_nodes"], dtype=np.uint32).reshape(-1),
+        "volume": np.ascontiguousarray(mesh["volume"], dtype=np.float32),
+        "lane": np.ascontiguousarray(mesh["hex_material"], dtype=np.uint32),
+    })
+    m = _HexMesh()
+    m.nodes = keep["mass"].size
+    m.hexes = keep["volume"].size
+    m.rest_positions = keep["rest"].ctypes.data_as(C.POINTER(C.c_float))
+    m.node_mass = keep["mass"].ctypes.data_as(C.POINTER(C.c_float))
+    m.anchored = keep["anchored"].ctypes.data_as(C.POINTER(C.c_uint8))
+    m.hex_nodes = keep["hex"].ctypes.data_as(C.POINTER(C.c_uint32))
+    m.volume = keep["volume"].ctypes.data_as(C.POINTER(C.c_float))
+    m.hex_material = keep["lane"].ctypes.data_as(C.POINTER(C.c_uint32))
+    m.voxel_size[:] = [float(v) for v in mesh["voxel_size"]]
+    return m
+
+
+def hex_mesh_surface(mesh) -> dict:
+    """collectSurface's hex faces with clusters of a host hex mesh
+    (talus_mi_hex_mesh_surface); triangle_tet holds the cell."""
+    L = lib()
+    keep = {}
+    m = _hex_mesh_in(mesh, keep)
+    out = _Surface()
+    _check(L.talus_mi_hex_mesh_surface(C.byref(m), C.byref(out)))
+    try:
+        return _surface_out(out)
+    finally:
+        L.talus_mi_surface_free(C.byref(out))
 
 
 def mesh_surface(mesh) -> dict:
@@ -884,24 +923,8 @@ class Context:
 
     def add_lattice(self, mesh, materials, offset=(0.0, 0.0, 0.0), velocity=(0.0, 0.0, 0.0)) -> int:
         """addLattice: a hex lattice body at rest + offset moving at velocity."""
-        keep = {
-            "rest": np.ascontiguousarray(mesh["rest_positions"], dtype=np.float32).reshape(-1),
-            "mass": np.ascontiguousarray(mesh["node_mass"], dtype=np.float32),
-            "anchored": np.ascontiguousarray(mesh["anchored"], dtype=np.uint8),
-            "hex": np.ascontiguousarray(mesh["hex_nodes"], dtype=np.uint32).reshape(-1),
-            "volume": np.ascontiguousarray(mesh["volume"], dtype=np.float32),
-            "lane": np.ascontiguousarray(mesh["hex_material"], dtype=np.uint32),
-        }
-        m = _HexMesh()
-        m.nodes = keep["mass"].size
-        m.hexes = keep["volume"].size
-        m.rest_positions = keep["rest"].ctypes.data_as(C.POINTER(C.c_float))
-        m.node_mass = keep["mass"].ctypes.data_as(C.POINTER(C.c_float))
-        m.anchored = keep["anchored"].ctypes.data_as(C.POINTER(C.c_uint8))
-        m.hex_nodes = keep["hex"].ctypes.data_as(C.POINTER(C.c_uint32))
-        m.volume = keep["volume"].ctypes.data_as(C.POINTER(C.c_float))
-        m.hex_material = keep["lane"].ctypes.data_as(C.POINTER(C.c_uint32))
-        m.voxel_size[:] = [float(v) for v in mesh["voxel_size"]]
+        keep = {}
+        m = _hex_mesh_in(mesh, keep)
         mats = _materials(materials if isinstance(materials, np.ndarray) else list(materials))
         off, vel = _f64(offset, 3), _f64(velocity, 3)
         out = C.c_uint32()
@@ -1029,6 +1052,15 @@ class Context:
         _check(lib().talus_mi_lattice_screen(self._h, lattice, hp, hl.size, _dp(f), _dp(u), C.byref(dent),
                                              C.byref(its)))
         return u.reshape(n, 3), dent.value, int(its.value)
+
+    def lattice_surface_mesh(self, lattice) -> dict:
+        """the hex surface of the lattice's current mesh (talus_mi_lattice_surface_mesh)"""
+        out = _Surface()
+        _check(lib().talus_mi_lattice_surface_mesh(self._h, lattice, C.byref(out)))
+        try:
+            return _surface_out(out)
+        finally:
+            lib().talus_mi_surface_free(C.byref(out))
 
     def lattice_pieces(self, lattice) -> int:
         out = C.c_uint32()

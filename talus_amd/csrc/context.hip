@@ -2666,6 +2666,41 @@ int talus_mi_mesh_surface(const talus_mi_mesh *mesh, talus_mi_surface *out) {
   }
 }
 
+int talus_mi_hex_mesh_surface(const talus_mi_hex_mesh *mesh, talus_mi_surface *out) {
+  if (!mesh || !out || !mesh->rest_positions || (mesh->hexes && !mesh->hex_nodes))
+    return fail(TALUS_MI_ERR_ARGUMENT, "bad arguments");
+  try {
+    for (size_t i = 0; i < 8 * size_t(mesh->hexes); ++i)
+      if (mesh->hex_nodes[i] >= mesh->nodes) return fail(TALUS_MI_ERR_ARGUMENT, "hex node out of range");
+    std::vector<double> rest(mesh->rest_positions, mesh->rest_positions + 3 * size_t(mesh->nodes));
+    tmi::SurfaceMesh sm;
+    tmi::hex_surface(mesh->nodes, mesh->hexes, mesh->hex_nodes, mesh->hex_material, rest.data(), sm);
+    std::vector<uint8_t> on(mesh->nodes, 0);
+    for (const uint32_t n : sm.triangles) on[n] = 1;
+    for (uint32_t n = 0; n < mesh->nodes; ++n)
+      if (on[n]) sm.nodes.push_back(n);
+    sm.crack.assign(mesh->hexes, 0);
+    return export_surface(sm, mesh->hexes, out);
+  } catch (const std::bad_alloc &) {
+    return fail(TALUS_MI_ERR_MEMORY, "out of host memory");
+  }
+}
+
+int talus_mi_lattice_surface_mesh(talus_mi_ctx *ctx, uint32_t lattice, talus_mi_surface *out) {
+  if (!ctx || !out) return fail(TALUS_MI_ERR_ARGUMENT, "bad arguments");
+  return guarded(ctx, [&]() -> int {
+    if (!ctx->lattices || lattice >= ctx->lattices->count()) return fail(TALUS_MI_ERR_ARGUMENT, "no such lattice");
+    tmi::LatticeBody &b = ctx->lattices->body(lattice);
+    b.ensure_surface();
+    tmi::SurfaceMesh sm;
+    tmi::hex_surface(b.nodes, b.hexes, b.hex.data(), b.lane.data(), b.rest.data(), sm);
+    for (uint32_t n = 0; n < b.nodes; ++n)
+      if (b.surface[n]) sm.nodes.push_back(n);
+    sm.crack = b.crack.empty() ? std::vector<uint8_t>(b.hexes, 0) : b.crack;
+    return export_surface(sm, b.hexes, out);
+  });
+}
+
 int talus_mi_surface_get(talus_mi_ctx *ctx, uint32_t body, talus_mi_surface *out) {
   if (!ctx || !body_ok(ctx, body) || !out) return fail(TALUS_MI_ERR_ARGUMENT, "bad arguments");
   return guarded(ctx, [&]() -> int {

@@ -70,6 +70,14 @@ struct SurfaceMesh {
 };
 void mesh_surface(uint32_t nodes, uint32_t tets, const uint32_t *tet, const uint32_t *lane,
                   const double *rest, SurfaceMesh &out);
+// collectSurface's hex faces (solver.cc:252-303): each cell face split in
+// two triangles around its lowest node, a face kept when no other cell has
+// it, turned outward by the cell's opposite corner; triangles in ascending
+// order of their sorted node triple, triangle_tet the cell, edges the
+// triangles' sorted pairs without repeats, clusters as for tets.  Surface
+// nodes and crack faces are the caller's (LatticeBody).
+void hex_surface(uint32_t nodes, uint32_t hexes, const uint32_t *hex, const uint32_t *lane, const double *rest,
+                 SurfaceMesh &out);
 // field::distance (sampledsdf.cc:377-385) in float: the distance to the
 // lattice's box outside it, else the trilinear sample
 float sdf_distance(const talus_mi_sdf &field, const float *point);

@@ -170,6 +170,74 @@ void cluster_surface(const std::vector<uint32_t> &triangles, const double *rest,
   starts.push_back(uint32_t(T));
 }
 
+void hex_surface(uint32_t nodes, uint32_t hexes, const uint32_t *hex, const uint32_t *lane, const double *rest,
+                 SurfaceMesh &out) {
+  // quad corners in order and a corner of the opposite face
+  constexpr int QUAD[6][5] = {{0, 2, 6, 4, 1}, {1, 3, 7, 5, 0}, {0, 1, 5, 4, 2},
+                              {2, 3, 7, 6, 0}, {0, 1, 3, 2, 4}, {4, 5, 7, 6, 0}};
+  struct Tri {
+    std::array<uint32_t, 3> key, ordered;
+    uint32_t opposite, cell;
+  };
+  out = SurfaceMesh{};
+  std::vector<Tri> tris;
+  tris.reserve(12 * size_t(hexes));
+  for (uint32_t c = 0; c < hexes; ++c)
+    for (const auto &q : QUAD) {
+      const uint32_t *h = &hex[8 * size_t(c)];
+      const uint32_t quad[4] = {h[q[0]], h[q[1]], h[q[2]], h[q[3]]};
+      uint32_t low = 0;
+      for (uint32_t k = 1; k < 4; ++k)
+        if (quad[k] < quad[low]) low = k;
+      for (uint32_t half = 0; half < 2; ++half) {
+        Tri t;
+        t.ordered = {quad[low], quad[(low + 1 + half) % 4], quad[(low + 2 + half) % 4]};
+        t.key = t.ordered;
+        std::sort(t.key.begin(), t.key.end());
+        t.opposite = h[q[4]];
+        t.cell = c;
+        tris.push_back(t);
+      }
+    }
+  std::sort(tris.begin(), tris.end(), [](const Tri &a, const Tri &b) {
+    return a.key != b.key ? a.key < b.key : (a.cell < b.cell);
+  });
+  auto at_rest = [&](uint32_t n, int k) { return float(rest[3 * size_t(n) + k]); };
+  for (size_t at = 0; at < tris.size();) {
+    size_t next = at + 1;
+    while (next < tris.size() && tris[next].key == tris[at].key) ++next;
+    if (next - at == 1) {
+      const Tri &t = tris[at];
+      const uint32_t a = t.ordered[0], b = t.ordered[1], c = t.ordered[2];
+      float u[3], w[3], o[3];
+      for (int k = 0; k < 3; ++k) {
+        u[k] = at_rest(b, k) - at_rest(a, k);
+        w[k] = at_rest(c, k) - at_rest(a, k);
+        o[k] = at_rest(t.opposite, k) - at_rest(a, k);
+      }
+      const float n[3] = {u[1] * w[2] - u[2] * w[1], u[2] * w[0] - u[0] * w[2], u[0] * w[1] - u[1] * w[0]};
+      const bool inward = n[0] * o[0] + n[1] * o[1] + n[2] * o[2] > 0.0f;
+      if (inward)
+        out.triangles.insert(out.triangles.end(), {a, c, b});
+      else
+        out.triangles.insert(out.triangles.end(), {a, b, c});
+      out.triangle_tet.push_back(t.cell);
+      out.triangle_lane.push_back(lane ? lane[t.cell] : 0u);
+      const auto &s = t.key;
+      out.edges.insert(out.edges.end(), {s[0], s[1], s[0], s[2], s[1], s[2]});
+    }
+    at = next;
+  }
+  std::vector<std::array<uint32_t, 2>> pairs(out.edges.size() / 2);
+  for (size_t i = 0; i < pairs.size(); ++i) pairs[i] = {out.edges[2 * i], out.edges[2 * i + 1]};
+  std::sort(pairs.begin(), pairs.end());
+  pairs.erase(std::unique(pairs.begin(), pairs.end()), pairs.end());
+  out.edges.clear();
+  for (const auto &p : pairs) out.edges.insert(out.edges.end(), {p[0], p[1]});
+  (void)nodes;
+  cluster_surface(out.triangles, rest, out.cluster_faces, out.cluster_starts);
+}
+
 float sdf_distance(const talus_mi_sdf &f, const float *p) {
   float gap2 = 0.0f;
   float clamped[3];

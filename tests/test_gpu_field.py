@@ -119,9 +119,49 @@ def test_bake_reads_crack_faces_and_carves():
         cy, cx = divmod(rem, cells[0])
         for c in range(8):
             rest[h[c]] = -HALF + VOXEL * np.array([cx + (c & 1), cy + ((c >> 1) & 1), cz + ((c >> 2) & 1)])
+    for c in range(X.shape[0], n):
+        rest[c] = X[np.argmin(np.linalg.norm(X - rest[c], axis=1))]
     rest[:X.shape[0]] = X
     om = orc_mesh(dict(mesh, hex_nodes=hexes, node_mass=masses, anchored=np.zeros(x.shape[0], np.uint8)), rest)
     ob, obox = FD.bake_deformed(om, field, x, crack=crack)
     oc, _ = FD.bake_deformed(om, field, x, crack=crack, carve=True)
     check(baked, box, ob, obox)
     assert np.abs(carved - oc).max() <= 1e-6
+
+
+def test_torn_lattice_surface_mesh_matches_the_oracle_and_stays_closed():
+    """collectSurface's hex faces of a torn lattice: the library's surface of
+    the current mesh equals the oracle's on the same topology, encloses the
+    lattice's volume (crack faces come in opposite pairs), uses every edge
+    an even number of times, and carries the lattice's crack faces."""
+    from test_surface_cpu import area, edge_uses, enclosed_volume
+    mesh, mat = lattice(tear=0.05)
+    X = mesh["rest_positions"].astype(np.float64)
+    with api.Context() as ctx:
+        b = ctx.add_lattice(mesh, [mat])
+        ctx.set_lattice_state(b, x=X * np.array([1.3, 1.0, 1.0]))
+        made = ctx.lattice_tear(b, [20, 21, 52])
+        srf = ctx.lattice_surface_mesh(b)
+        hexes, masses, crack = ctx.lattice_topology(b)
+        n = ctx.lattice_nodes(b)
+    assert made > 0 and crack.any()
+    np.testing.assert_array_equal(srf["crack_faces"], crack)
+    cells = np.round(2 * HALF / VOXEL).astype(int)
+    rest = np.zeros((n, 3))
+    for i, h in enumerate(hexes):
+        cz, rem = divmod(i, cells[0] * cells[1])
+        cy, cx = divmod(rem, cells[0])
+        for c in range(8):
+            rest[h[c]] = -HALF + VOXEL * np.array([cx + (c & 1), cy + ((c >> 1) & 1), cz + ((c >> 2) & 1)])
+    # a copy keeps its origin's rest position: take the original node there
+    for c in range(X.shape[0], n):
+        rest[c] = X[np.argmin(np.linalg.norm(X - rest[c], axis=1))]
+    rest[:X.shape[0]] = X
+    om = orc_mesh(dict(mesh, hex_nodes=hexes, node_mass=masses, anchored=np.zeros(n, np.uint8)), rest)
+    orc = FD.hex_surface(om)
+    for key in orc:
+        np.testing.assert_array_equal(srf[key], orc[key], err_msg=key)
+    box = 8.0 * HALF.prod()
+    assert abs(enclosed_volume(srf, rest) - box) <= 1e-6 * box
+    assert area(srf["triangles"], rest) > 2 * 4 * (HALF[0] * HALF[1] + HALF[1] * HALF[2] + HALF[0] * HALF[2])
+    assert np.all(edge_uses(srf) % 2 == 0)

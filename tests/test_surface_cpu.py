@@ -153,3 +153,34 @@ def test_surface_clusters_follow_the_morton_order_of_the_face_centres(cells):
     np.testing.assert_array_equal(srf["cluster_faces"], expect)
     n = len(tri)
     np.testing.assert_array_equal(srf["cluster_starts"], np.append(np.arange(0, n, 32), n))
+
+
+@pytest.mark.parametrize("cells", [(1, 1, 1), (4, 2, 3)])
+def test_hex_surface_is_closed_outward_and_matches_the_oracle(cells):
+    """collectSurface's hex faces (solver.cc:252-303): two triangles per
+    unshared cell face, split at the lowest node, facing outward; the
+    surface of a box lattice encloses the box, covers its area, uses every
+    edge twice and has Euler characteristic 2; the library equals the
+    oracle, clusters included."""
+    from oracle import field as FD
+    voxel = (0.1, 0.08, 0.05)
+    mats = [api.PhysicalMaterial(youngs_modulus=1e5, density=1000.0, poissons_ratio=0.3, friction=0.7)]
+    mesh = api.build_hex_lattice([c + 1 for c in cells], voxel, (0.0, 0.0, 0.0), mats, [])
+    srf = api.hex_mesh_surface(mesh)
+    om = {"rest": mesh["rest_positions"].astype(np.float64), "mass": mesh["node_mass"],
+          "anchored": mesh["anchored"], "hex": mesh["hex_nodes"], "volume": mesh["volume"],
+          "lane": mesh["hex_material"], "voxel": mesh["voxel_size"]}
+    orc = FD.hex_surface(om)
+    for key in orc:
+        np.testing.assert_array_equal(srf[key], orc[key], err_msg=key)
+    rest = om["rest"]
+    L = np.array(cells) * np.array(voxel)
+    assert abs(enclosed_volume(srf, rest) - L.prod()) <= 1e-6 * L.prod()
+    assert abs(area(srf["triangles"], rest) - 2 * (L[0] * L[1] + L[1] * L[2] + L[0] * L[2])) <= 1e-6
+    assert np.all(edge_uses(srf) == 2)
+    V, E, F = srf["nodes"].size, srf["edges"].shape[0], srf["triangles"].shape[0]
+    assert V - E + F == 2 and F == 4 * (cells[0] * cells[1] + cells[1] * cells[2] + cells[0] * cells[2])
+    on_box = np.any(np.isclose(rest, rest.min(axis=0)) | np.isclose(rest, rest.max(axis=0)), axis=1)
+    np.testing.assert_array_equal(srf["nodes"], np.flatnonzero(on_box))
+    # every triangle is split at its quad's lowest node and starts there
+    assert all(min(t) == t[0] for t in srf["triangles"])
