@@ -161,6 +161,14 @@ typedef struct talus_mi_report {
   uint32_t plastic_capped;
 } talus_mi_report;
 
+/* The last talus_mi_step's StepReport of one body alone (deformable::step
+ * reports per body, solver.h:211-311): nodes, elements, substeps, CG
+ * iterations, capped and total solves, worst relative residual, energies,
+ * yield events, non-finite flag and the plastic Newton's counters;
+ * refactors, preconditioner applications and device time are the batch's
+ * and stay zero here.  All zero before the first step and after an edit. */
+int talus_mi_body_report(talus_mi_ctx *ctx, uint32_t body, talus_mi_report *out);
+
 #define TALUS_MI_EVENT_YIELD 0u
 #define TALUS_MI_EVENT_TEAR 1u
 

@@ -261,6 +261,7 @@ _SIGNATURES = {
                                         _P(C.c_double), _P(C.c_uint8)]),
     "talus_mi_step": (C.c_int, [C.c_void_p, _P(StepParams), _P(Report)]),
     "talus_mi_events": (C.c_int, [C.c_void_p, _P(Event), C.c_uint32, _P(C.c_uint32)]),
+    "talus_mi_body_report": (C.c_int, [C.c_void_p, C.c_uint32, _P(Report)]),
     "talus_mi_eval_forces": (C.c_int, [C.c_void_p, C.c_uint32, _P(C.c_double), _P(C.c_double),
                                        _P(C.c_double), _P(C.c_double), _P(C.c_double)]),
     "talus_mi_apply_system": (C.c_int, [C.c_void_p, C.c_uint32, C.c_double, C.c_double,
@@ -774,6 +775,12 @@ class Context:
         p = params if params is not None else default_params(**kw)
         r = Report()
         _check(lib().talus_mi_step(self._h, C.byref(p), C.byref(r)))
+        return r.as_dict()
+
+    def body_report(self, body) -> dict:
+        """the last step's report of one body (talus_mi_body_report)"""
+        r = Report()
+        _check(lib().talus_mi_body_report(self._h, body, C.byref(r)))
         return r.as_dict()
 
     def events(self, capacity=1 << 20):

@@ -298,6 +298,8 @@ class EngineBase {
   virtual void step(const talus_mi_step_params &sp, talus_mi_report &rep,
                     std::vector<talus_mi_event> &events, uint32_t step_index,
                     std::vector<uint8_t> &torn) = 0;
+  // the last step's report of body b alone (zero before the first step)
+  virtual void body_report(uint32_t b, talus_mi_report &out) const = 0;
   virtual void state_set(uint32_t b, const double *x, const double *v, const double *a) = 0;
   virtual void state_get(uint32_t b, double *x, double *v, double *a) = 0;
   virtual void rotations_get(uint32_t b, double *R) = 0;
@@ -372,6 +374,12 @@ class Engine final : public EngineBase {
                              double *y) override;
   void apply_system_damped(uint32_t b, const double *x, double inertia, double step, double damp,
                            const double *dv, double *y) override;
+  void body_report(uint32_t b, talus_mi_report &out) const override {
+    if (b < body_rep_.size())
+      out = body_rep_[b];
+    else
+      std::memset(&out, 0, sizeof out);
+  }
   void time_kernel(uint32_t kind, uint32_t repeats, talus_mi_kernel_timing &out) override;
   void factor_solve(uint32_t b, double inertia, double step, const double *r, double *z) override;
   void factor_info(talus_mi_factor_info &info) override;
@@ -554,6 +562,7 @@ class Engine final : public EngineBase {
   // (at most every REFRESH_COOLDOWN steps); a capped solve refreshes its body
   static constexpr uint32_t REFRESH_ITERS = 5, REFRESH_COOLDOWN = 4;
   std::vector<int> want_refresh_;
+  std::vector<talus_mi_report> body_rep_;  // the last step's report of each body
   std::vector<uint64_t> next_refresh_;
   uint64_t steps_taken_ = 0;
   uint32_t refactors_ = 0;
@@ -1311,7 +1320,17 @@ void Engine<Real>::newton_pass(const std::vector<uint32_t> &bodies, const std::v
       op[i] = nb[i].next();
     }
   }
-  for (const NewtonBody &n : nb) {
+  for (size_t i = 0; i < nb.size(); ++i) {
+    const NewtonBody &n = nb[i];
+    talus_mi_report &own = body_rep_[bodies[i]];
+    own.plastic_evaluations += n.evaluations;
+    own.plastic_solves += n.solves_all;
+    own.plastic_backtracks += n.backtracks;
+    own.plastic_linear_reads += n.linear_reads;
+    own.plastic_handoffs += n.handoffs;
+    own.plastic_fallbacks += n.fallbacks;
+    own.plastic_rejections += n.rejections;
+    own.plastic_capped += n.capped;
     rep.plastic_evaluations += n.evaluations;
     rep.plastic_solves += n.solves_all;
     rep.plastic_backtracks += n.backtracks;
@@ -1511,6 +1530,7 @@ void Engine<Real>::step(const talus_mi_step_params &sp, talus_mi_report &rep,
                       std::any_of(newton_body_.begin(), newton_body_.end(), [](uint8_t f) { return f != 0; });
   talus_mi_report newton_rep;
   std::memset(&newton_rep, 0, sizeof newton_rep);
+  body_rep_.assign(B, talus_mi_report{});
   const uint32_t refactors_before = refactors_;
   applications_ = 0;
   const size_t n3 = 3 * size_t(d_.N);
@@ -1665,15 +1685,20 @@ void Engine<Real>::step(const talus_mi_step_params &sp, talus_mi_report &rep,
     for (uint32_t b = 0; b < B; ++b) {
       if (pass >= substeps[b]) continue;
       if (!mask.empty() && !mask[b]) continue;
+      talus_mi_report &own = body_rep_[b];
       rep.cg_iterations += hiters_[slot + b];
+      own.cg_iterations += hiters_[slot + b];
       worst[b] = std::max(worst[b], hiters_[slot + b]);
       const bool capped = hiters_[slot + b] >= limit && !(hrr_[slot + b] <= htarget_[slot + b]);
       capped_body[b] |= capped ? 1 : 0;
       rep.cg_capped += capped ? 1u : 0u;
+      own.cg_capped += capped ? 1u : 0u;
       const double reference2 = htarget_[slot + b] / (sp.tolerance * double(sp.tolerance));
       const double rel = reference2 > 0.0 ? std::sqrt(hrr_[slot + b] / reference2) : 0.0;
       rep.cg_worst_relative = std::max(rep.cg_worst_relative, rel);
+      own.cg_worst_relative = std::max(own.cg_worst_relative, rel);
       ++rep.solves;
+      ++own.solves;
     }
   }
   for (uint32_t b = 0; b < B; ++b) {
@@ -1687,6 +1712,16 @@ void Engine<Real>::step(const talus_mi_step_params &sp, talus_mi_report &rep,
     rep.elastic_energy += elastic;
     rep.non_finite |= bad > 0.0 ? 1u : 0u;
     rep.yield_events += hyields_[b];
+    talus_mi_report &own = body_rep_[b];
+    own.bodies = 1;
+    own.nodes = nn_[b];
+    own.elements = tt_[b];
+    own.substeps = substeps[b];
+    own.kinetic_energy = kinetic;
+    own.elastic_energy = elastic;
+    own.non_finite = bad > 0.0 ? 1u : 0u;
+    own.yield_events = hyields_[b];
+    own.plastic_growth = hyields_[b] ? 1u : 0u;
   }
   rep.plastic_growth = rep.yield_events ? 1u : 0u;
   rep.plastic_evaluations = newton_rep.plastic_evaluations;
@@ -2344,6 +2379,9 @@ class GroupedEngine final : public EngineBase {
                            const double *dv, double *y) override {
     eng_[group_[b]]->apply_system_damped(local_[b], x, inertia, step, damp, dv, y);
   }
+  void body_report(uint32_t b, talus_mi_report &out) const override {
+    eng_[group_[b]]->body_report(local_[b], out);
+  }
   void factor_solve(uint32_t b, double inertia, double step, const double *r, double *z) override {
     eng_[group_[b]]->factor_solve(local_[b], inertia, step, r, z);
   }
@@ -2963,6 +3001,15 @@ int talus_mi_step(talus_mi_ctx *ctx, const talus_mi_step_params *params, talus_m
         ++ctx->version;
       }
     }
+    return TALUS_MI_OK;
+  });
+}
+
+int talus_mi_body_report(talus_mi_ctx *ctx, uint32_t body, talus_mi_report *out) {
+  if (!ctx || !body_ok(ctx, body) || !out) return fail(TALUS_MI_ERR_ARGUMENT, "bad arguments");
+  return guarded(ctx, [&]() -> int {
+    std::memset(out, 0, sizeof *out);
+    if (ctx->engine && !ctx->dirty) ctx->engine->body_report(body, *out);
     return TALUS_MI_OK;
   });
 }

@@ -210,3 +210,33 @@ def test_plastic_yield_events_match_oracle():
     for s in range(steps):
         assert np.array_equal(yg[s], yo[s]), s
     assert np.abs(po - orc.plastic()).max() <= 1e-8
+
+
+@pytest.mark.parametrize("streams", [1, 2])
+def test_body_reports_split_the_step_report(streams):
+    """talus_mi_body_report: each body's own StepReport (deformable::step
+    reports per body); the batch's counters are their sums, a body stepped
+    alone reports what it reports batched."""
+    sc = scenes.bodies(count=5, seed=12, lo=200, hi=2000)
+    params = api.default_params(substeps=2, preconditioner=api.FACTOR)
+    alone = []
+    for body in sc.bodies:
+        with api.Context(0, api.FP64) as ctx:
+            b, _ = product_body(ctx, body)
+            ctx.step(params)
+            alone.append(ctx.body_report(b))
+    with api.Context(0, api.FP64) as ctx:
+        ids = [product_body(ctx, body)[0] for body in sc.bodies]
+        ctx.set_streams(streams)
+        assert ctx.body_report(ids[0])["solves"] == 0  # before the first step
+        rep = ctx.step(params)
+        own = [ctx.body_report(b) for b in ids]
+    for key in ("cg_iterations", "cg_capped", "solves", "yield_events"):
+        assert rep[key] == sum(r[key] for r in own), key
+    for key in ("kinetic_energy", "elastic_energy"):
+        assert rep[key] == pytest.approx(sum(r[key] for r in own), rel=1e-12)
+    assert rep["cg_worst_relative"] == max(r["cg_worst_relative"] for r in own)
+    for r, a, body in zip(own, alone, sc.bodies):
+        assert r["bodies"] == 1 and r["elements"] == a["elements"] and r["substeps"] == 2
+        assert r["solves"] == a["solves"] == 6
+        assert r["kinetic_energy"] == pytest.approx(a["kinetic_energy"], rel=1e-9)
