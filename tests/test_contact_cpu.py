@@ -1,0 +1,363 @@
+"""Contact generation oracle (oracle/contact_oracle.c: findContacts of
+src/talus/deformable/contact/contact.cc with the float probe of
+src/talus/field/sampledsdf/sampledsdf.cc), pinned without a GPU by
+
+* the reference's own probe test of a sampled sphere (0.25 m radius on
+  [-0.35, 0.35]^3 with 71 samples per axis, probed at (0.3, 0, 0): distance
+  0.05 within 4e-4, tests/field_meshsdf_tests.cc:391-415);
+* what a linear field fixes: the trilinear probe of a plane is the plane
+  (distance and normal), outside the lattice the clamped value plus the
+  distance to the box, and its Lipschitz bound is sqrt(|n|^2 + 1);
+* the closest point of a triangle (barycentricOf) against brute force, and
+  the descent against the exact closest distance to a sphere;
+* closed forms of whole queries: a cube resting on a plane touches it at
+  its bottom nodes with gap = height - bias; a falling body and a rising
+  plane meet it after (d - bias) / speed, inside dt or not at all; skin
+  (surface thickness) adds to the bias; anchored nodes do not move; the
+  friction is the geometric mean of the two lanes' (0.5 beyond them);
+* culling (clusters and single faces, with the field's Lipschitz bound)
+  that changes no contact, while it does skip work, and de-duplication that
+  leaves no two contacts of a shape within 1e-2 voxel.
+"""
+from __future__ import annotations
+
+import numpy as np
+import pytest
+
+from oracle import contact as OC
+from talus_amd import api, scenes
+
+
+# ------------------------------------------------------------------ fields
+
+def grid_field(fn, lo, hi, voxel, materials=None):
+    """A sampled field of fn(x, y, z) on the lattice origin lo, spacing
+    voxel (float32 coordinates), up to hi."""
+    lo = np.asarray(lo, np.float32)
+    vox = np.float32(voxel)
+    res = (np.floor((np.asarray(hi, np.float64) - lo) / float(vox) + 1e-6) + 1).astype(np.int64)
+    axes = [lo[k] + vox * np.arange(res[k], dtype=np.float32) for k in range(3)]
+    z, y, x = np.meshgrid(axes[2], axes[1], axes[0], indexing="ij")
+    d = np.asarray(fn(x.astype(np.float64), y.astype(np.float64), z.astype(np.float64)), np.float32)
+    f = dict(origin=lo, voxel=(vox, vox, vox), res=res, distances=d.reshape(-1))
+    f["materials"] = (np.zeros(d.size, np.uint8) if materials is None
+                      else np.asarray(materials(x, y, z), np.uint8).reshape(-1))
+    return f
+
+
+def plane_field(height=0.0, lo=(-2.0, -1.0, -2.0), hi=(2.0, 1.0, 2.0), voxel=0.125, normal=(0.0, 1.0, 0.0),
+                materials=None):
+    n = np.asarray(normal, np.float64)
+    n = n / np.linalg.norm(n)
+    return grid_field(lambda x, y, z: n[0] * x + n[1] * y + n[2] * z - height, lo, hi, voxel, materials)
+
+
+def sphere_field(radius=0.25, lo=-0.35, hi=0.35, res=71):
+    voxel = (hi - lo) / (res - 1)
+    return grid_field(lambda x, y, z: np.sqrt(x * x + y * y + z * z) - radius, (lo, lo, lo), (hi, hi, hi), voxel)
+
+
+# ------------------------------------------------------------------ bodies
+
+def cube_body(cells=(4, 2, 4), voxel=0.125, origin=(-0.25, 0.0625, -0.25), friction=0.3, supports=()):
+    body = scenes.Body(resolution=tuple(c + 1 for c in cells), voxel=(voxel,) * 3, origin=origin,
+                       materials=scenes.material_array(1, friction=friction), supports=list(supports))
+    mesh, mats = scenes.build_mesh(body)
+    return body, mesh, mats
+
+
+def query(mesh, srf, shapes, dt=1.0 / 60.0, velocity=None, **kw):
+    pos = mesh["rest_positions"].astype(np.float32)
+    vel = np.zeros_like(pos) if velocity is None else np.broadcast_to(np.asarray(velocity, np.float32),
+                                                                      pos.shape).copy()
+    kw.setdefault("voxel", (0.125,) * 3)
+    kw.setdefault("anchored", mesh["anchored"])
+    return OC.find_contacts(srf, pos, vel, shapes, dt, **kw)
+
+
+def mesh_points(c, pos):
+    return np.einsum("ck,ckj->cj", c["weights"].astype(np.float64), pos[c["nodes"]].astype(np.float64))
+
+
+# ------------------------------------------------------------------- probe
+
+def test_sampled_sphere_probe_matches_the_reference_test():
+    f = sphere_field()
+    d, n = OC.field_probe(f, (0.3, 0.0, 0.0))
+    assert abs(float(d) - 0.05) <= 4.0e-4
+    assert np.degrees(np.arccos(min(1.0, float(n[0])))) <= 2.0
+
+
+@pytest.mark.parametrize("seed", [0, 1, 2])
+def test_probe_of_a_plane_is_the_plane(seed):
+    rng = np.random.default_rng(seed)
+    n = rng.normal(size=3)
+    n /= np.linalg.norm(n)
+    f = plane_field(height=0.1, lo=(-1, -1, -1), hi=(1, 1, 1), voxel=0.1, normal=n)
+    for p in rng.uniform(-0.95, 0.95, size=(40, 3)).astype(np.float32):
+        d, normal = OC.field_probe(f, p)
+        assert abs(float(d) - (n @ p.astype(np.float64) - 0.1)) <= 2e-6
+        assert np.abs(normal - n).max() <= 2e-6
+        assert float(OC.field_probe_distance(f, p)) == float(d)
+    for p in rng.uniform(-2.0, 2.0, size=(40, 3)).astype(np.float32):
+        c = np.clip(p.astype(np.float64), -1.0, 1.0)
+        gap = p - c
+        outside = np.linalg.norm(gap)
+        if outside < 1e-3:
+            continue
+        inner = n @ c - 0.1
+        d, normal = OC.field_probe(f, p)
+        assert abs(float(d) - (inner + outside)) <= 4e-6
+        blend = n * abs(inner) + gap
+        assert np.abs(normal - blend / np.linalg.norm(blend)).max() <= 4e-5
+
+
+def test_plane_lipschitz_and_a_random_field_bound():
+    f = plane_field(normal=(1.0, 2.0, 2.0), lo=(-1, -1, -1), hi=(1, 1, 1), voxel=0.25)
+    assert abs(OC.field_lipschitz(f) - np.sqrt(2.0)) <= 1e-6
+    rng = np.random.default_rng(4)
+    r = rng.normal(size=(6, 5, 7)).astype(np.float32)  # z, y, x
+    g = dict(origin=(0, 0, 0), voxel=(0.5, 0.25, 1.0), res=(7, 5, 6), distances=r.reshape(-1))
+    v = np.array([0.5, 0.25, 1.0], np.float32)
+    dx = np.abs(np.diff(r, axis=2))  # along x: (6, 5, 6)
+    dy = np.abs(np.diff(r, axis=1))
+    dz = np.abs(np.diff(r, axis=0))
+    sx = np.maximum.reduce([dx[:-1, :-1], dx[:-1, 1:], dx[1:, :-1], dx[1:, 1:]])
+    sy = np.maximum.reduce([dy[:-1, :, :-1], dy[:-1, :, 1:], dy[1:, :, :-1], dy[1:, :, 1:]])
+    sz = np.maximum.reduce([dz[:, :-1, :-1], dz[:, :-1, 1:], dz[:, 1:, :-1], dz[:, 1:, 1:]])
+    worst = np.sqrt((sx / v[0]) ** 2 + (sy / v[1]) ** 2 + (sz / v[2]) ** 2).max()
+    assert abs(OC.field_lipschitz(g) - np.sqrt(worst ** 2 + 1.0)) <= 1e-5 * np.sqrt(worst ** 2 + 1.0)
+
+
+def test_material_is_the_nearest_sample():
+    rng = np.random.default_rng(5)
+    f = plane_field(lo=(0, 0, 0), hi=(1, 1, 1), voxel=0.25, materials=lambda x, y, z: (
+        (4 * x).round() + 5 * (4 * y).round() + 25 * (4 * z).round()) % 200)
+    for p in rng.uniform(0.0, 1.0, size=(60, 3)):
+        frac = (p / 0.25) % 1.0
+        if np.any(np.abs(frac - 0.5) < 1e-3):
+            continue
+        q = np.round(p / 0.25)
+        expect = int((q[0] + 5 * q[1] + 25 * q[2]) % 200)
+        assert OC.field_material(f, p) == expect
+    assert OC.field_material(f, (1.5, 0.5, 0.5)) == 0
+
+
+# --------------------------------------------------------- closest points
+
+def closest_point_exact(tri, p):
+    """Closest point of a triangle to p in FP64 (plane projection inside,
+    else the best of the three edge segments)."""
+    a, b, c = tri
+    n = np.cross(b - a, c - a)
+    q = p - n * ((p - a) @ n) / (n @ n)
+    m = np.array([[b - a, c - a]]).reshape(2, 3)
+    s, t = np.linalg.solve(m @ m.T, m @ (q - a))
+    if s >= 0 and t >= 0 and s + t <= 1:
+        return q
+    best = None
+    for u, v in ((a, b), (b, c), (c, a)):
+        h = np.clip((p - u) @ (v - u) / ((v - u) @ (v - u)), 0.0, 1.0)
+        r = u + h * (v - u)
+        if best is None or np.linalg.norm(p - r) < np.linalg.norm(p - best):
+            best = r
+    return best
+
+
+@pytest.mark.parametrize("seed", range(4))
+def test_barycentric_is_the_closest_point(seed):
+    rng = np.random.default_rng(seed)
+    for _ in range(200):
+        tri = rng.normal(size=(3, 3)).astype(np.float32)
+        if rng.uniform() < 0.3:  # obtuse slivers
+            tri[2] = tri[0] + (tri[1] - tri[0]) * np.float32(rng.uniform(0.2, 0.8)) + \
+                np.float32(0.05) * rng.normal(size=3).astype(np.float32)
+        p = (rng.normal(size=3) * 2).astype(np.float32)
+        w = OC.barycentric(tri.reshape(-1), p)
+        assert w.min() >= 0.0 and abs(float(w.sum()) - 1.0) <= 1e-5
+        got = w.astype(np.float64) @ tri.astype(np.float64)
+        exact = closest_point_exact(tri.astype(np.float64), p.astype(np.float64))
+        assert np.linalg.norm(got - exact) <= 1e-4 * (1.0 + np.abs(tri).max())
+
+
+def test_closest_on_a_plane_is_the_lowest_corner():
+    f = plane_field(lo=(-1, -1, -1), hi=(1, 1, 1), voxel=0.125)
+    rng = np.random.default_rng(6)
+    for _ in range(50):
+        tri = rng.uniform(-0.5, 0.5, size=(3, 3)).astype(np.float32)
+        heights = np.sort(tri[:, 1])
+        if heights[1] - heights[0] < 1e-3:
+            continue
+        w, p, d, n = OC.closest_on_triangle(dict(field=f), tri)
+        k = int(np.argmin(tri[:, 1]))
+        assert w[k] == 1.0 and np.array_equal(p, tri[k])
+        assert abs(float(d) - float(tri[k, 1])) <= 1e-6
+        assert np.array_equal(n, np.array([0, 1, 0], np.float32))
+
+
+def test_closest_on_a_sphere_descends_to_the_nearest_point():
+    f = sphere_field()
+    rng = np.random.default_rng(7)
+    for _ in range(60):
+        c = rng.normal(size=3)
+        c *= 0.3 / np.linalg.norm(c)
+        tri = (c + 0.05 * rng.normal(size=(3, 3))).astype(np.float32)
+        w, p, d, n = OC.closest_on_triangle(dict(field=f), tri)
+        exact = np.linalg.norm(closest_point_exact(tri.astype(np.float64), np.zeros(3))) - 0.25
+        edge = max(np.linalg.norm(tri[i] - tri[j]) for i, j in ((0, 1), (1, 2), (2, 0)))
+        assert -5e-5 <= float(d) - exact <= 0.03 * edge
+        assert np.allclose(p, w.astype(np.float64) @ tri.astype(np.float64), atol=1e-6)
+
+
+@pytest.mark.parametrize("q0", [(1.0, 0.0, 0.0, 0.0), (0.8, 0.2, -0.4, 0.4)])
+def test_shape_advances_by_its_spin(q0):
+    q0 = np.asarray(q0, np.float64)
+    q0 /= np.linalg.norm(q0)
+    w = np.array([0.3, -1.2, 2.0])
+    shape = dict(field=plane_field(lo=(-1, -1, -1), hi=(1, 1, 1), voxel=0.5), center=(1, 2, 3), rotation=q0,
+                 linear=(0.5, -1.0, 2.0), angular=w)
+    t = 0.37
+    c, q = OC.advance_shape(shape, t)
+    assert np.allclose(c, np.array([1, 2, 3]) + t * np.array([0.5, -1.0, 2.0]), atol=1e-6)
+    a = np.linalg.norm(w) * t
+    turn = np.concatenate([[np.cos(a / 2)], np.sin(a / 2) * w / np.linalg.norm(w)])
+    pw, px, py, pz = turn
+    qw, qx, qy, qz = q0
+    expect = np.array([pw * qw - px * qx - py * qy - pz * qz, pw * qx + px * qw + py * qz - pz * qy,
+                       pw * qy - px * qz + py * qw + pz * qx, pw * qz + px * qy - py * qx + pz * qw])
+    assert np.abs(q - expect).max() <= 2e-6
+
+
+# ------------------------------------------------------------ whole query
+
+def test_resting_cube_touches_the_plane_at_its_bottom_nodes():
+    _, mesh, mats = cube_body()
+    srf = api.mesh_surface(mesh)
+    pos = mesh["rest_positions"]
+    for cull in (True, False):
+        c = query(mesh, srf, [dict(field=plane_field())], bias=0.01, threshold=0.1, cull=cull,
+                  friction=mats["friction"])
+        bottom = np.flatnonzero(pos[:, 1] == pos[:, 1].min())
+        assert len(c) == bottom.size == 25
+        corner = c["nodes"][np.arange(len(c)), np.argmax(c["weights"], axis=1)]
+        assert np.array_equal(np.sort(corner), bottom)
+        assert np.all(np.sort(c["weights"], axis=1) == np.array([0, 0, 1], np.float32))
+        assert np.all(c["normal"] == np.array([0, 1, 0], np.float32))
+        assert np.allclose(c["gap"], 0.0625 - 0.01, atol=1e-7)
+        assert np.all(c["bias"] == np.float32(0.01))
+        assert np.allclose(c["local_point"], pos[corner] * np.array([1, 0, 1]), atol=1e-7)
+        assert np.all(c["friction"] == np.float32(0.3)) and np.all(c["partner"] == 0)
+
+
+@pytest.mark.parametrize("speed,hits", [(4.0, True), (2.0, False)])
+def test_falling_body_meets_the_plane_after_the_closed_form_time(speed, hits):
+    # bottom 0.0625 above the plane, bias 0.01: impact after 0.0525 / speed,
+    # inside dt = 1/60 for 4 m/s (0.0131 s) and not for 2 m/s (0.0263 s)
+    _, mesh, mats = cube_body()
+    srf = api.mesh_surface(mesh)
+    c = query(mesh, srf, [dict(field=plane_field())], velocity=(0.0, -speed, 0.0), bias=0.01, threshold=0.02,
+              friction=mats["friction"])
+    if not hits:
+        assert len(c) == 0
+        return
+    assert len(c) == 25
+    assert np.allclose(c["gap"], 0.0625 - 0.01, atol=2e-6)
+    assert np.allclose(c["normal"], [0, 1, 0], atol=1e-6)
+
+
+def test_rising_plane_meets_a_resting_body():
+    _, mesh, mats = cube_body()
+    srf = api.mesh_surface(mesh)
+    pos = mesh["rest_positions"]
+    for u, hits in ((4.0, True), (2.0, False)):
+        c = query(mesh, srf, [dict(field=plane_field(), linear=(0.0, u, 0.0))], bias=0.01, threshold=0.02,
+                  friction=mats["friction"])
+        assert (len(c) == 25) == hits and (len(c) == 0) == (not hits)
+        if hits:
+            corner = c["nodes"][np.arange(len(c)), np.argmax(c["weights"], axis=1)]
+            assert np.allclose(c["gap"], 0.0625 - 0.01, atol=2e-6)
+            # the contact point in the shape's frame lies on its surface
+            assert np.allclose(c["local_point"][:, 1], 0.0, atol=2e-6)
+            assert np.allclose(c["local_point"][:, [0, 2]], pos[corner][:, [0, 2]], atol=2e-6)
+
+
+def test_skin_adds_to_the_bias_and_anchors_hold_still():
+    _, mesh, mats = cube_body(supports=[((0.0, 0.0625, 0.0), (1.0, 0.01, 1.0))])
+    srf = api.mesh_surface(mesh)
+    T = srf["triangles"].shape[0]
+    thick = np.full(T, 0.005, np.float32)
+    c = query(mesh, srf, [dict(field=plane_field())], bias=0.01, threshold=0.1, thickness=thick,
+              friction=mats["friction"])
+    assert len(c) == 25
+    assert np.allclose(c["bias"], 0.015) and np.allclose(c["gap"], 0.0625 - 0.015, atol=1e-7)
+    # an anchored body does not move: falling fast, it still does not reach
+    # a plane 0.3 below within dt, while the same body free does
+    assert mesh["anchored"].sum() == 25
+    _, low, _ = cube_body(origin=(-0.25, 0.3, -0.25), supports=[((0.0, 0.3, 0.0), (1.0, 1.0, 1.0))])
+    assert low["anchored"].all()
+    srf2 = api.mesh_surface(low)
+    c = query(low, srf2, [dict(field=plane_field())], velocity=(0.0, -30.0, 0.0), bias=0.0, threshold=0.01)
+    assert len(c) == 0
+    _, free, _ = cube_body(origin=(-0.25, 0.3, -0.25))
+    c = query(free, api.mesh_surface(free), [dict(field=plane_field())], velocity=(0.0, -30.0, 0.0), bias=0.01,
+              threshold=0.02)
+    # at 30 m/s the bottom layer (0.3 up) and the next (0.425 up) both reach
+    # the plane within dt: contacts at both heights, every bottom node once
+    low_gap = np.isclose(c["gap"], 0.3 - 0.01, atol=1e-6)
+    assert low_gap.sum() == 25 and np.all(low_gap | np.isclose(c["gap"], 0.425 - 0.01, atol=1e-6))
+
+
+def test_friction_is_the_geometric_mean_of_the_lanes():
+    _, mesh, mats = cube_body(friction=0.2)
+    srf = api.mesh_surface(mesh)
+    f1 = plane_field(materials=lambda x, y, z: np.ones_like(x))
+    c = query(mesh, srf, [dict(field=f1)], bias=0.0, threshold=0.1, friction=np.array([0.2, 0.8], np.float32))
+    assert np.allclose(c["friction"], 0.4)
+    # lanes beyond the body's materials read the default 0.5
+    c = query(mesh, srf, [dict(field=f1)], bias=0.0, threshold=0.1, friction=np.array([0.2], np.float32))
+    assert np.allclose(c["friction"], np.sqrt(0.1))
+
+
+def jittered_scene(seed):
+    rng = np.random.default_rng(seed)
+    _, mesh, mats = cube_body(cells=(6, 4, 5), voxel=0.1, origin=(-0.3, 0.05, -0.25))
+    pos = mesh["rest_positions"].astype(np.float32)
+    pos += (0.01 * rng.normal(size=pos.shape)).astype(np.float32)
+    vel = (rng.normal(size=pos.shape) * 0.3 + np.array([0.0, -1.5, 0.0])).astype(np.float32)
+    shapes = [dict(field=plane_field()),
+              dict(field=sphere_field(), center=(0.1, 0.62, 0.0), linear=(0.0, -2.0, 0.5)),
+              dict(field=sphere_field(), center=(3.0, 3.0, 3.0)),  # far away
+              dict(field=plane_field(normal=(1.0, 0.0, 0.0), height=0.0), center=(0.36, 0.0, 0.0),
+                   rotation=(np.cos(0.2), 0.0, 0.0, np.sin(0.2)))]
+    return mesh, mats, pos, vel, shapes
+
+
+@pytest.mark.parametrize("seed", [0, 1])
+def test_culling_changes_no_contact(seed):
+    mesh, mats, pos, vel, shapes = jittered_scene(seed)
+    srf = api.mesh_surface(mesh)
+    args = dict(bias=0.005, threshold=0.03, gravity=(0, -9.81, 0), voxel=(0.1,) * 3, anchored=mesh["anchored"],
+                friction=mats["friction"], stats=True)
+    a, sa, na = OC.find_contacts(srf, pos, vel, shapes, 1.0 / 60.0, cull=True, **args)
+    b, sb, nb = OC.find_contacts(srf, pos, vel, shapes, 1.0 / 60.0, cull=False, **args)
+    assert na == nb > 0 and a.tobytes() == b.tobytes()
+    assert sa["clusters_far"] > 0 and sa["faces_probed"] < sb["faces_probed"]
+    assert sa["candidates"] - sa["duplicates"] == na
+    assert set(np.unique(a["partner"])) >= {0, 1, 3}
+    # no two contacts of one shape within 1e-2 voxel
+    pts = mesh_points(a, pos)
+    for s in np.unique(a["partner"]):
+        q = pts[a["partner"] == s]
+        dist = np.linalg.norm(q[:, None] - q[None], axis=2) + np.eye(len(q))
+        assert dist.min() > 1e-3 - 1e-6
+
+
+def test_capacity_and_no_shapes():
+    mesh, mats, pos, vel, shapes = jittered_scene(0)
+    srf = api.mesh_surface(mesh)
+    full = OC.find_contacts(srf, pos, vel, shapes, 1 / 60, bias=0.005, threshold=0.03, voxel=(0.1,) * 3,
+                            friction=mats["friction"])
+    part, _, total = OC.find_contacts(srf, pos, vel, shapes, 1 / 60, bias=0.005, threshold=0.03, voxel=(0.1,) * 3,
+                                      friction=mats["friction"], capacity=3, stats=True)
+    assert total == len(full) and part.tobytes() == full[:3].tobytes()
+    assert len(OC.find_contacts(srf, pos, vel, [], 1 / 60)) == 0
