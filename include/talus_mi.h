@@ -741,6 +741,76 @@ int talus_mi_lattice_bake_field(talus_mi_ctx *ctx, uint32_t lattice, const talus
 int talus_mi_lattice_step(talus_mi_ctx *ctx, const talus_mi_lattice_params *params,
                           talus_mi_lattice_report *report);
 
+/* ---- contact generation against rigid partners' sampled fields ---- */
+
+/* Copies a sampled field (a rigid partner's shape, ShapeProbe::field,
+ * src/talus/deformable/contact/contact.h:19-26) into device memory, where it
+ * stays for every query until the context ends; materials NULL = lane 0
+ * everywhere.  Its probeLipschitz (sampledsdf.cc:424-430) is computed over
+ * every cell, as finishBricks (sampledsdf.cc:584-606) leaves it.  *id
+ * names it in talus_mi_shape. */
+int talus_mi_field_add(talus_mi_ctx *ctx, const talus_mi_sdf *field, uint32_t *id);
+/* Replaces field id's lattice and samples (after the caller rewrote them). */
+int talus_mi_field_set(talus_mi_ctx *ctx, uint32_t id, const talus_mi_sdf *field);
+/* probeLipschitz of field id: sqrt(g^2 + 1), g the largest cellLipschitz. */
+int talus_mi_field_lipschitz(talus_mi_ctx *ctx, uint32_t id, float *lipschitz);
+
+/* ShapeProbe (contact.h:19-26): a rigid partner placed by its centre of
+ * mass and rotation (w, x, y, z); a world point w maps to the field point
+ * conj(rotation) (w - center_of_mass) + field_offset. */
+typedef struct talus_mi_shape {
+  uint32_t field; /* talus_mi_field_add id */
+  float center_of_mass[3];
+  float rotation[4];
+  float field_offset[3];
+  float linear_velocity[3];
+  float angular_velocity[3];
+} talus_mi_shape;
+
+/* The arguments of findContacts (contact.h:85-94) besides the mesh, state
+ * and materials. */
+typedef struct talus_mi_contact_query {
+  float dt;
+  float gravity[3];
+  float bias;          /* ContactGaps::bias */
+  float threshold;     /* ContactGaps::threshold */
+  float voxel_size[3]; /* PreparedMesh::voxelSize: duplicates lie within 1e-2 of its largest side */
+  int32_t cull;
+} talus_mi_contact_query;
+
+/* SurfaceContact (contact.h:64-74). */
+typedef struct talus_mi_surface_contact {
+  uint32_t nodes[3];
+  float weights[3];
+  int32_t partner;
+  float normal[3];
+  float local_point[3];
+  float gap;
+  float bias;
+  float friction;
+  uint32_t triangle;
+} talus_mi_surface_contact;
+
+/* findContacts (src/talus/deformable/contact/contact.cc:599-880) of a tet
+ * body's boundary surface (talus_mi_surface_get: its triangles, lanes and
+ * clusters) at the body's state (positions and velocities rounded to float)
+ * against `count` shapes: for each shape in order, each surface triangle in
+ * order whose swept box meets the shape's and that culling (cull != 0:
+ * clusterFar per 32-triangle cluster, farFromShape per triangle, with the
+ * field's Lipschitz bound) does not rule out, the closest point of the
+ * triangle to the shape (seven starts, projected descent) and, when farther
+ * than threshold + skin, the advance to impact within dt; a contact within
+ * 1e-2 of the largest voxel side of an earlier contact of the same shape is
+ * dropped.  thickness (one per surface triangle: surfaceThickness,
+ * talus_mi_surface_thickness) may be NULL (zero skin).  Friction is
+ * sqrt(f(triangle lane) f(field material at the contact)), f a lane of the
+ * body's materials (0.5 beyond them).  Writes min(found, capacity)
+ * contacts; *found = how many there are.  Runs on the body's stream after
+ * its pending steps. */
+int talus_mi_find_contacts(talus_mi_ctx *ctx, uint32_t body, const talus_mi_shape *shapes, uint32_t count,
+                           const talus_mi_contact_query *query, const float *thickness,
+                           talus_mi_surface_contact *out, uint32_t capacity, uint32_t *found);
+
 #ifdef __cplusplus
 }
 #endif

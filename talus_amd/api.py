@@ -126,6 +126,59 @@ class SampleBox(C.Structure):
     _fields_ = [("minimum", C.c_uint32 * 3), ("extent", C.c_uint32 * 3)]
 
 
+class Shape(C.Structure):
+    """talus_mi_shape (ShapeProbe): a rigid partner's resident field placed
+    by its centre of mass and rotation (w, x, y, z)."""
+
+    _fields_ = [("field", C.c_uint32), ("center_of_mass", C.c_float * 3), ("rotation", C.c_float * 4),
+                ("field_offset", C.c_float * 3), ("linear_velocity", C.c_float * 3),
+                ("angular_velocity", C.c_float * 3)]
+
+
+class ContactQuery(C.Structure):
+    """talus_mi_contact_query: findContacts' dt, gravity, ContactGaps,
+    voxel size and cull flag."""
+
+    _fields_ = [("dt", C.c_float), ("gravity", C.c_float * 3), ("bias", C.c_float), ("threshold", C.c_float),
+                ("voxel_size", C.c_float * 3), ("cull", C.c_int32)]
+
+
+class SurfaceContact(C.Structure):
+    """talus_mi_surface_contact (SurfaceContact)."""
+
+    _fields_ = [("nodes", C.c_uint32 * 3), ("weights", C.c_float * 3), ("partner", C.c_int32),
+                ("normal", C.c_float * 3), ("local_point", C.c_float * 3), ("gap", C.c_float),
+                ("bias", C.c_float), ("friction", C.c_float), ("triangle", C.c_uint32)]
+
+
+# numpy view of talus_mi_surface_contact
+CONTACT_DTYPE = np.dtype([("nodes", "<u4", 3), ("weights", "<f4", 3), ("partner", "<i4"), ("normal", "<f4", 3),
+                          ("local_point", "<f4", 3), ("gap", "<f4"), ("bias", "<f4"), ("friction", "<f4"),
+                          ("triangle", "<u4")])
+assert CONTACT_DTYPE.itemsize == C.sizeof(SurfaceContact)
+
+
+def _sdf(field, keep):
+    """talus_mi_sdf over a field dict (origin, voxel, res, distances,
+    optional materials); arrays kept alive in ``keep``."""
+    dist = np.ascontiguousarray(field["distances"], dtype=np.float32).reshape(-1)
+    f = Sdf()
+    f.origin[:] = [float(v) for v in field["origin"]]
+    f.voxel_size[:] = [float(v) for v in field["voxel"]]
+    f.resolution[:] = [int(v) for v in field["res"]]
+    if dist.size != int(np.prod([int(v) for v in field["res"]])):
+        raise TalusError("a field needs one distance per sample")
+    f.distances = dist.ctypes.data_as(C.POINTER(C.c_float))
+    keep.append(dist)
+    if field.get("materials") is not None:
+        mat = np.ascontiguousarray(field["materials"], dtype=np.uint8).reshape(-1)
+        if mat.size != dist.size:
+            raise TalusError("a field needs one material per sample")
+        f.materials = mat.ctypes.data_as(C.POINTER(C.c_uint8))
+        keep.append(mat)
+    return f
+
+
 class StepParams(C.Structure):
     _fields_ = [
         ("dt", C.c_float), ("gravity", C.c_float * 3), ("substeps", C.c_uint32),
@@ -262,6 +315,11 @@ _SIGNATURES = {
     "talus_mi_step": (C.c_int, [C.c_void_p, _P(StepParams), _P(Report)]),
     "talus_mi_events": (C.c_int, [C.c_void_p, _P(Event), C.c_uint32, _P(C.c_uint32)]),
     "talus_mi_body_report": (C.c_int, [C.c_void_p, C.c_uint32, _P(Report)]),
+    "talus_mi_field_add": (C.c_int, [C.c_void_p, _P(Sdf), _P(C.c_uint32)]),
+    "talus_mi_field_set": (C.c_int, [C.c_void_p, C.c_uint32, _P(Sdf)]),
+    "talus_mi_field_lipschitz": (C.c_int, [C.c_void_p, C.c_uint32, _P(C.c_float)]),
+    "talus_mi_find_contacts": (C.c_int, [C.c_void_p, C.c_uint32, _P(Shape), C.c_uint32, _P(ContactQuery),
+                                         _P(C.c_float), _P(SurfaceContact), C.c_uint32, _P(C.c_uint32)]),
     "talus_mi_eval_forces": (C.c_int, [C.c_void_p, C.c_uint32, _P(C.c_double), _P(C.c_double),
                                        _P(C.c_double), _P(C.c_double), _P(C.c_double)]),
     "talus_mi_apply_system": (C.c_int, [C.c_void_p, C.c_uint32, C.c_double, C.c_double,
@@ -782,6 +840,65 @@ class Context:
         r = Report()
         _check(lib().talus_mi_body_report(self._h, body, C.byref(r)))
         return r.as_dict()
+
+    def field_add(self, field) -> int:
+        """Copies a sampled field (dict: origin, voxel, res, distances,
+        optional materials) to the device for contact queries
+        (talus_mi_field_add); returns its id."""
+        keep = []
+        f = _sdf(field, keep)
+        i = C.c_uint32()
+        _check(lib().talus_mi_field_add(self._h, C.byref(f), C.byref(i)))
+        return i.value
+
+    def field_set(self, fid, field):
+        """Replaces a resident field's samples (talus_mi_field_set)."""
+        keep = []
+        f = _sdf(field, keep)
+        _check(lib().talus_mi_field_set(self._h, fid, C.byref(f)))
+
+    def field_lipschitz(self, fid) -> float:
+        """probeLipschitz of a resident field (talus_mi_field_lipschitz)."""
+        out = C.c_float()
+        _check(lib().talus_mi_field_lipschitz(self._h, fid, C.byref(out)))
+        return out.value
+
+    def find_contacts(self, body, shapes, dt, bias=0.0, threshold=0.0, cull=True, gravity=(0.0, 0.0, 0.0),
+                      voxel=(1.0, 1.0, 1.0), thickness=None, capacity=None):
+        """findContacts of a body's surface at its state (talus_mi_find_contacts).
+        shapes: dicts with field (a field_add id) and optional center,
+        rotation (w, x, y, z), offset, linear, angular.  Returns the contacts
+        as a CONTACT_DTYPE array and how many there are in all."""
+        arr = (Shape * max(1, len(shapes)))()
+        for i, sh in enumerate(shapes):
+            s = arr[i]
+            s.field = int(sh["field"])
+            s.center_of_mass[:] = [float(v) for v in sh.get("center", (0, 0, 0))]
+            s.rotation[:] = [float(v) for v in sh.get("rotation", (1, 0, 0, 0))]
+            s.field_offset[:] = [float(v) for v in sh.get("offset", (0, 0, 0))]
+            s.linear_velocity[:] = [float(v) for v in sh.get("linear", (0, 0, 0))]
+            s.angular_velocity[:] = [float(v) for v in sh.get("angular", (0, 0, 0))]
+        q = ContactQuery()
+        q.dt, q.bias, q.threshold, q.cull = float(dt), float(bias), float(threshold), int(bool(cull))
+        q.gravity[:] = [float(v) for v in gravity]
+        q.voxel_size[:] = [float(v) for v in voxel]
+        th = None
+        if thickness is not None:
+            th = np.ascontiguousarray(thickness, dtype=np.float32).reshape(-1)
+            if th.size != self.surface(body)["triangles"].shape[0]:
+                raise TalusError("thickness needs one value per surface triangle")
+        thp = None if th is None else th.ctypes.data_as(_P(C.c_float))
+        found = C.c_uint32()
+        cap = 4096 if capacity is None else int(capacity)
+        while True:
+            out = (SurfaceContact * max(1, cap))()
+            _check(lib().talus_mi_find_contacts(self._h, body, arr, len(shapes), C.byref(q), thp, out, cap,
+                                                C.byref(found)))
+            if capacity is not None or found.value <= cap:
+                break
+            cap = found.value
+        n = min(cap, found.value)
+        return np.frombuffer(bytes(out), dtype=CONTACT_DTYPE)[:n].copy(), found.value
 
     def events(self, capacity=1 << 20):
         buf = (Event * capacity)()

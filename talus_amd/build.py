@@ -17,8 +17,8 @@ LIB = os.path.join(HERE, "libtalus_mi.so")
 
 HIPCC = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")
 ARCH = "gfx950"
-SOURCES = ["kernels.hip", "factor.hip", "context.hip", "lattice.hip", "screen.hip", "field.hip", "mesh.cpp", "factor_symbolic.cpp",
-           "tear.cpp", "surface.cpp", "hex_build.cpp"]
+SOURCES = ["kernels.hip", "factor.hip", "context.hip", "lattice.hip", "screen.hip", "field.hip", "contact.hip", "mesh.cpp",
+           "factor_symbolic.cpp", "tear.cpp", "surface.cpp", "hex_build.cpp"]
 COMMON = [
     "-O3", "-fPIC", "-std=c++20", "-Wall", "-Wno-unused-function",
     f"-I{os.path.join(ROOT, 'include')}",
@@ -27,13 +27,16 @@ COMMON = [
 # otherwise form v_pk_* FP32 ops from float code).
 DEVICE = [f"--offload-arch={ARCH}", "-Xclang", "-target-feature", "-Xclang", "-packed-fp32-ops",
           "-ffp-contract=fast"]
+# contact generation rounds every float operation as the reference does:
+# no contraction, correctly rounded division and square root
+EXTRA = {"contact.hip": ["-ffp-contract=off", "-fhip-fp32-correctly-rounded-divide-sqrt"]}
 
 
 def _compile(src: str) -> str:
     path = os.path.join(CSRC, src)
     obj = os.path.join(OBJ, src + ".o")
     if src.endswith(".hip"):
-        cmd = [HIPCC, "-x", "hip", *DEVICE, *COMMON, "-c", path, "-o", obj]
+        cmd = [HIPCC, "-x", "hip", *DEVICE, *EXTRA.get(src, []), *COMMON, "-c", path, "-o", obj]
     else:
         # host topology and symbolic code: no contraction, so results match
         # the oracle's -ffp-contract=off build operation for operation

@@ -28,6 +28,7 @@
 #include <thread>
 #include <vector>
 
+#include "contact.h"
 #include "factor.h"
 #include "host.h"
 #include "device.h"
@@ -300,6 +301,9 @@ class EngineBase {
                     std::vector<uint8_t> &torn) = 0;
   // the last step's report of body b alone (zero before the first step)
   virtual void body_report(uint32_t b, talus_mi_report &out) const = 0;
+  // device views of body b's positions and velocities (FP64, 3 per node)
+  // and the stream whose work writes them
+  virtual void device_state(uint32_t b, const double **x, const double **v, hipStream_t *s) const = 0;
   virtual void state_set(uint32_t b, const double *x, const double *v, const double *a) = 0;
   virtual void state_get(uint32_t b, double *x, double *v, double *a) = 0;
   virtual void rotations_get(uint32_t b, double *R) = 0;
@@ -379,6 +383,11 @@ class Engine final : public EngineBase {
       out = body_rep_[b];
     else
       std::memset(&out, 0, sizeof out);
+  }
+  void device_state(uint32_t b, const double **x, const double **v, hipStream_t *s) const override {
+    *x = d_.x + 3 * size_t(n0_[b]);
+    *v = d_.v + 3 * size_t(n0_[b]);
+    *s = stream_;
   }
   void time_kernel(uint32_t kind, uint32_t repeats, talus_mi_kernel_timing &out) override;
   void factor_solve(uint32_t b, double inertia, double step, const double *r, double *z) override;
@@ -2382,6 +2391,9 @@ class GroupedEngine final : public EngineBase {
   void body_report(uint32_t b, talus_mi_report &out) const override {
     eng_[group_[b]]->body_report(local_[b], out);
   }
+  void device_state(uint32_t b, const double **x, const double **v, hipStream_t *s) const override {
+    eng_[group_[b]]->device_state(local_[b], x, v, s);
+  }
   void factor_solve(uint32_t b, double inertia, double step, const double *r, double *z) override {
     eng_[group_[b]]->factor_solve(local_[b], inertia, step, r, z);
   }
@@ -2522,6 +2534,12 @@ struct talus_mi_ctx {
   std::unique_ptr<tmi::LatticeSet> lattices;  // strainstep hex lattices
   // per lattice: the neo-Hookean hex terms and screen solve (built on use)
   std::vector<std::unique_ptr<tmi::HexScreen>> screens;
+  // sampled fields resident for contact queries (talus_mi_field_add)
+  std::vector<std::unique_ptr<tmi::DeviceField>> fields;
+  // per body: its surface on the device for contact queries, and the
+  // context version it was built at
+  std::vector<std::unique_ptr<tmi::ContactFinder>> finders;
+  std::vector<uint64_t> finder_version;
 };
 
 namespace {
@@ -3602,6 +3620,71 @@ int talus_mi_lattice_step(talus_mi_ctx *ctx, const talus_mi_lattice_params *para
     talus_mi_lattice_report r{};
     if (ctx->lattices) ctx->lattices->step(*params, r);
     if (report) *report = r;
+    return TALUS_MI_OK;
+  });
+}
+
+// ---------------------------------------------------------- contact queries
+
+int talus_mi_field_add(talus_mi_ctx *ctx, const talus_mi_sdf *field, uint32_t *id) {
+  if (!ctx || !field || !id) return fail(TALUS_MI_ERR_ARGUMENT, "bad arguments");
+  return guarded(ctx, [&]() -> int {
+    auto f = std::make_unique<tmi::DeviceField>();
+    try {
+      f->set(*field, ctx->stream);
+    } catch (const std::invalid_argument &e) {
+      return fail(TALUS_MI_ERR_ARGUMENT, e.what());
+    }
+    ctx->fields.push_back(std::move(f));
+    *id = uint32_t(ctx->fields.size() - 1);
+    return TALUS_MI_OK;
+  });
+}
+
+int talus_mi_field_set(talus_mi_ctx *ctx, uint32_t id, const talus_mi_sdf *field) {
+  if (!ctx || !field || id >= ctx->fields.size()) return fail(TALUS_MI_ERR_ARGUMENT, "bad arguments");
+  return guarded(ctx, [&]() -> int {
+    try {
+      ctx->fields[id]->set(*field, ctx->stream);
+    } catch (const std::invalid_argument &e) {
+      return fail(TALUS_MI_ERR_ARGUMENT, e.what());
+    }
+    return TALUS_MI_OK;
+  });
+}
+
+int talus_mi_field_lipschitz(talus_mi_ctx *ctx, uint32_t id, float *lipschitz) {
+  if (!ctx || id >= ctx->fields.size() || !lipschitz) return fail(TALUS_MI_ERR_ARGUMENT, "bad arguments");
+  *lipschitz = ctx->fields[id]->lipschitz();
+  return TALUS_MI_OK;
+}
+
+int talus_mi_find_contacts(talus_mi_ctx *ctx, uint32_t body, const talus_mi_shape *shapes, uint32_t count,
+                           const talus_mi_contact_query *query, const float *thickness,
+                           talus_mi_surface_contact *out, uint32_t capacity, uint32_t *found) {
+  if (!ctx || !body_ok(ctx, body) || !query || !found || (count && !shapes) || (capacity && !out))
+    return fail(TALUS_MI_ERR_ARGUMENT, "bad arguments");
+  if (!(query->dt >= 0.0f) || !std::isfinite(query->dt)) return fail(TALUS_MI_ERR_ARGUMENT, "dt must be finite");
+  std::vector<const tmi::DeviceField *> fields(count);
+  for (uint32_t s = 0; s < count; ++s) {
+    if (shapes[s].field >= ctx->fields.size()) return fail(TALUS_MI_ERR_ARGUMENT, "unknown field");
+    fields[s] = ctx->fields[shapes[s].field].get();
+  }
+  return guarded(ctx, [&]() -> int {
+    ensure(ctx);
+    if (ctx->finders.size() != ctx->bodies.size()) {
+      ctx->finders.resize(ctx->bodies.size());
+      ctx->finder_version.resize(ctx->bodies.size(), ~0ull);
+    }
+    if (!ctx->finders[body] || ctx->finder_version[body] != ctx->version) {
+      ctx->finders[body].reset();
+      ctx->finders[body] = std::make_unique<tmi::ContactFinder>(ctx->bodies[body]);
+      ctx->finder_version[body] = ctx->version;
+    }
+    const double *x = nullptr, *v = nullptr;
+    hipStream_t s = nullptr;
+    ctx->engine->device_state(body, &x, &v, &s);
+    *found = ctx->finders[body]->find(x, v, s, shapes, fields.data(), count, *query, thickness, out, capacity);
     return TALUS_MI_OK;
   });
 }
