@@ -76,11 +76,17 @@ class ContactFinder {
   uint8_t *anchored_ = nullptr;
   float *friction_ = nullptr, *thickness_ = nullptr, *pos_ = nullptr, *vel_ = nullptr;
   void *clusters_ = nullptr;  // ClusterData per cluster
-  // grown on demand
+  uint32_t *count_ = nullptr;
+  // per (shape, triangle), grown with the number of shapes
   uint32_t shapes_cap_ = 0, out_cap_ = 0;
-  void *dshapes_ = nullptr, *cand_ = nullptr, *out_ = nullptr;
+  std::vector<void *> per_shape_;
+  contact_math::DevShape *dshapes_ = nullptr;
+  contact_math::ShapePre *pre_ = nullptr;
+  contact_math::Candidate *cand_ = nullptr;
   uint8_t *skip_ = nullptr, *flag_ = nullptr, *state_ = nullptr;
-  uint32_t *order_ = nullptr, *nbr_ = nullptr, *nbn_ = nullptr, *count_ = nullptr;
+  uint32_t *order_ = nullptr, *nbr_ = nullptr, *nbn_ = nullptr, *kount_ = nullptr;
+  float *pts_ = nullptr;
+  void *out_ = nullptr;
 };
 
 }  // namespace tmi

@@ -7,6 +7,7 @@
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
+#include <cmath>
 #include <cstring>
 #include <stdexcept>
 
@@ -20,7 +21,7 @@ namespace {
 using namespace contact_math;
 
 
-constexpr uint32_t NEIGHBOURS = 16u;         // earlier candidates kept per candidate
+constexpr uint32_t NEIGHBOURS = 8u;          // earlier candidates listed per candidate
 constexpr int SELECT_THREADS = 1024;
 enum : uint8_t { UNDECIDED = 0, ACCEPTED = 1, REJECTED = 2 };
 
@@ -41,10 +42,13 @@ struct ContactDev {
   ClusterData *clusters;
   uint8_t *skip, *flag, *state;
   Candidate *cand;
-  uint32_t *order, *nbr, *nbn, *count;
+  uint32_t *order, *nbr, *nbn, *count, *kount;
+  float *pts;  // candidates' mesh points in face order, 3 per candidate
   talus_mi_surface_contact *out;
   const DevShape *shapes;
-  float dt, g[3], bias, threshold, apart;
+  ShapePre *pre;  // per shape
+  float dt, g[3], bias, threshold;
+  float near2;  // the largest squared distance whose square root is <= apart
   int cull, clustered;
 };
 
@@ -66,6 +70,12 @@ __device__ inline float red_min(float v) {
 __device__ inline float red_max(float v) {
   for (int o = 16; o > 0; o >>= 1) v = mx(v, __shfl_xor(v, o, 32));
   return v;
+}
+
+// the shapes' swept boxes, radii and starting poses, once per query
+__global__ void k_contact_shapes(ContactDev d) {
+  const uint32_t s = blockIdx.x * blockDim.x + threadIdx.x;
+  if (s < d.S) d.pre[s] = shape_pre(d.shapes[s], d.dt);
 }
 
 // ClusterReach (contact.cc:640-703) and clusterFar (454-481) per shape: one
@@ -113,13 +123,12 @@ __global__ void __launch_bounds__(256) k_contact_clusters(ContactDev d) {
   }
   for (uint32_t s = lane; s < d.S; s += 32u) {
     const DevShape &sh = d.shapes[s];
-    const Bounds swept = swept_bounds(sh, d.dt);
-    const ShapeAt origin = advance_shape(sh, 0.0f);
+    const ShapePre &pre = d.pre[s];
     const float sb = d.bias + thick, st = d.threshold + thick;
-    bool far = !overlaps(reach, swept, st);
+    bool far = !overlaps(reach, pre.swept, st);
     if (!far)
-      far = out_of_reach(sh, distance_below(sh, origin, centre, spread, sh.lipschitz), speed, d.dt, sb, st,
-                         shape_radius(sh));
+      far = out_of_reach(sh, distance_below(sh, pre.origin, centre, spread, sh.lipschitz), speed, d.dt, sb, st,
+                         pre.radius);
     d.skip[size_t(s) * d.C + c] = far ? 1u : 0u;
   }
 }
@@ -138,7 +147,7 @@ __global__ void __launch_bounds__(256) k_contact_faces(ContactDev d) {
   q.tri = d.tri, q.lane = d.lane, q.pos = d.pos, q.vel = d.vel, q.friction = d.friction;
   q.lanes = d.lanes, q.dt = d.dt, q.bias = d.bias, q.threshold = d.threshold, q.cull = d.cull;
   const float thick = d.thickness ? d.thickness[face] : 0.0f;
-  if (face_contact(q, d.shapes[s], int32_t(s), face, thick, &d.cand[size_t(s) * d.T + face])) flag = 1;
+  if (face_contact(q, d.shapes[s], d.pre[s], int32_t(s), face, thick, &d.cand[size_t(s) * d.T + face])) flag = 1;
 }
 
 // exclusive prefix sum over the workgroup (SELECT_THREADS threads, 64-wide
@@ -163,60 +172,111 @@ __device__ uint32_t block_scan(uint32_t value, uint32_t *total) {
   return before + inclusive - value;
 }
 
-__device__ inline float gap_between(const Candidate &a, const Candidate &b) {
-  return len(mk(a.point[0], a.point[1], a.point[2]) - mk(b.point[0], b.point[1], b.point[2]));
+// the duplicate test (contact.cc:806-826): a candidate is kept when no
+// earlier KEPT candidate of its shape (face order) lies within `apart` -- a
+// sequential greedy rule, evaluated here in three steps: (1) each shape's
+// candidates compacted in face order with their mesh points, (2) every
+// candidate's earlier neighbours within `apart` found grid-wide, (3) the rule
+// resolved in rounds (a candidate decides once all its earlier neighbours
+// have; the first undecided one always can, so each round decides at least
+// one) and the kept contacts written shape by shape
+
+// (1) one workgroup per shape
+__global__ void __launch_bounds__(SELECT_THREADS) k_contact_compact(ContactDev d) {
+  const uint32_t s = blockIdx.x;
+  const size_t at = size_t(s) * d.T;
+  uint32_t K = 0;
+  for (uint32_t base = 0; base < d.T; base += SELECT_THREADS) {
+    const uint32_t f = base + threadIdx.x;
+    const uint32_t fl = f < d.T ? d.flag[at + f] : 0u;
+    uint32_t total;
+    const uint32_t rank = block_scan(fl, &total);
+    if (fl) {
+      const size_t k = at + K + rank;
+      d.order[k] = f;
+      const float *p = d.cand[at + f].point;
+      d.pts[3 * k] = p[0];
+      d.pts[3 * k + 1] = p[1];
+      d.pts[3 * k + 2] = p[2];
+      d.nbn[k] = 0;
+      d.state[k] = UNDECIDED;
+    }
+    K += total;
+  }
+  if (threadIdx.x == 0) d.kount[s] = K;
 }
 
-// the duplicate test (contact.cc:806-826) and the output, shape by shape:
-// candidates in face order; a candidate is kept when no earlier KEPT
-// candidate of its shape lies within `apart` -- the sequential greedy rule,
-// resolved in rounds (a candidate decides once every earlier neighbour has;
-// the first undecided one always can, so each round decides at least one)
-__global__ void __launch_bounds__(SELECT_THREADS) k_contact_select(ContactDev d) {
+__device__ inline F3 point_of(const float *pts, size_t i) { return mk(pts[3 * i], pts[3 * i + 1], pts[3 * i + 2]); }
+
+// |p - q| <= apart, decided as dot(p - q, p - q) <= near2: the square root
+// is correctly rounded and monotone, so this is the same decision
+__device__ inline bool near(F3 p, F3 q, float near2) {
+  const F3 e = p - q;
+  return dot(e, e) <= near2;
+}
+
+// (2) all earlier neighbours: the lower triangle of 256 x 256 tiles of
+// (candidate, earlier candidate) pairs, grid-stride over the tiles, the
+// earlier tile's points staged in LDS; a pair found appends to the
+// candidate's list (the first NEIGHBOURS listed, the count kept)
+__global__ void __launch_bounds__(256) k_contact_neighbours(ContactDev d) {
+  __shared__ float tile[3][256];
+  const uint32_t s = blockIdx.y;
+  const uint32_t K = d.kount[s];
+  const uint64_t nt = (uint64_t(K) + 255u) / 256u;
+  const uint64_t tiles = nt * (nt + 1) / 2;
+  const size_t at = size_t(s) * d.T;
+  for (uint64_t t = blockIdx.x; t < tiles; t += gridDim.x) {
+    uint64_t kt = uint64_t((sqrt(8.0 * double(t) + 1.0) - 1.0) * 0.5);
+    while (kt * (kt + 1) / 2 > t) --kt;
+    while ((kt + 1) * (kt + 2) / 2 <= t) ++kt;
+    const uint32_t jt = uint32_t(t - kt * (kt + 1) / 2);
+    const uint32_t j0 = jt * 256u, jn = min(256u, K - j0);
+    __syncthreads();
+    if (threadIdx.x < jn) {
+      const F3 q = point_of(d.pts, at + j0 + threadIdx.x);
+      tile[0][threadIdx.x] = q.x;
+      tile[1][threadIdx.x] = q.y;
+      tile[2][threadIdx.x] = q.z;
+    }
+    __syncthreads();
+    const uint32_t k = uint32_t(kt) * 256u + threadIdx.x;
+    if (k >= K) continue;
+    const F3 me = point_of(d.pts, at + k);
+    const uint32_t end = uint32_t(kt) == jt ? threadIdx.x : jn;
+    for (uint32_t i = 0; i < end; ++i) {
+      if (!near(mk(tile[0][i], tile[1][i], tile[2][i]), me, d.near2)) continue;
+      const uint32_t slot = atomicAdd(&d.nbn[at + k], 1u);
+      if (slot < NEIGHBOURS) d.nbr[(at + k) * NEIGHBOURS + slot] = j0 + i;
+    }
+  }
+}
+
+// (3) one workgroup for all shapes, in order
+__global__ void __launch_bounds__(SELECT_THREADS) k_contact_resolve(ContactDev d) {
   __shared__ uint32_t undecided;
   uint32_t written = 0;
   for (uint32_t s = 0; s < d.S; ++s) {
-    const Candidate *cand = d.cand + size_t(s) * d.T;
-    uint32_t K = 0;
-    for (uint32_t base = 0; base < d.T; base += SELECT_THREADS) {
-      const uint32_t f = base + threadIdx.x;
-      const uint32_t fl = f < d.T ? d.flag[size_t(s) * d.T + f] : 0u;
-      uint32_t total;
-      const uint32_t rank = block_scan(fl, &total);
-      if (fl) d.order[K + rank] = f;
-      K += total;
-    }
-    __syncthreads();
-    for (uint32_t k = threadIdx.x; k < K; k += SELECT_THREADS) {
-      const Candidate &me = cand[d.order[k]];
-      uint32_t n = 0;
-      for (uint32_t j = 0; j < k; ++j)
-        if (gap_between(cand[d.order[j]], me) <= d.apart) {
-          if (n < NEIGHBOURS) d.nbr[size_t(k) * NEIGHBOURS + n] = j;
-          ++n;
-        }
-      d.nbn[k] = n;
-      d.state[k] = n == 0 ? ACCEPTED : UNDECIDED;
-    }
-    __syncthreads();
+    const size_t at = size_t(s) * d.T;
+    const uint32_t K = d.kount[s];
+    volatile uint8_t *state = d.state + at;
     for (;;) {
       if (threadIdx.x == 0) undecided = 0;
       __syncthreads();
-      volatile uint8_t *state = d.state;
       for (uint32_t k = threadIdx.x; k < K; k += SELECT_THREADS) {
         if (state[k] != UNDECIDED) continue;
-        const uint32_t n = d.nbn[k];
+        const uint32_t n = d.nbn[at + k];
         bool kept_before = false, open = false;
         if (n <= NEIGHBOURS) {
           for (uint32_t i = 0; i < n; ++i) {
-            const uint8_t o = state[d.nbr[size_t(k) * NEIGHBOURS + i]];
+            const uint8_t o = state[d.nbr[(at + k) * NEIGHBOURS + i]];
             kept_before |= o == ACCEPTED;
             open |= o == UNDECIDED;
           }
         } else {
-          const Candidate &me = cand[d.order[k]];
+          const F3 me = point_of(d.pts, at + k);
           for (uint32_t j = 0; j < k; ++j)
-            if (gap_between(cand[d.order[j]], me) <= d.apart) {
+            if (near(point_of(d.pts, at + j), me, d.near2)) {
               const uint8_t o = state[j];
               kept_before |= o == ACCEPTED;
               open |= o == UNDECIDED;
@@ -237,11 +297,11 @@ __global__ void __launch_bounds__(SELECT_THREADS) k_contact_select(ContactDev d)
     uint32_t kept = 0;
     for (uint32_t base = 0; base < K; base += SELECT_THREADS) {
       const uint32_t k = base + threadIdx.x;
-      const uint32_t acc = k < K && d.state[k] == ACCEPTED ? 1u : 0u;
+      const uint32_t acc = k < K && state[k] == ACCEPTED ? 1u : 0u;
       uint32_t total;
       const uint32_t rank = block_scan(acc, &total);
-      const uint32_t at = written + kept + rank;
-      if (acc && at < d.capacity) d.out[at] = cand[d.order[k]].c;
+      const uint32_t o = written + kept + rank;
+      if (acc && o < d.capacity) d.out[o] = d.cand[at + d.order[at + k]].c;
       kept += total;
     }
     written += kept;
@@ -372,11 +432,6 @@ ContactFinder::ContactFinder(const Body &b) {
     pos_ = alloc<float>(3 * size_t(N_), owned_);
     vel_ = alloc<float>(3 * size_t(N_), owned_);
     clusters_ = alloc<ClusterData>(C_, owned_);
-    flag_ = nullptr;
-    order_ = alloc<uint32_t>(T_, owned_);
-    state_ = alloc<uint8_t>(T_, owned_);
-    nbr_ = alloc<uint32_t>(size_t(T_) * NEIGHBOURS, owned_);
-    nbn_ = alloc<uint32_t>(T_, owned_);
     count_ = alloc<uint32_t>(1, owned_);
   } catch (...) {
     for (void *p : owned_) (void)hipFree(p);
@@ -386,29 +441,27 @@ ContactFinder::ContactFinder(const Body &b) {
 
 ContactFinder::~ContactFinder() {
   for (void *p : owned_) (void)hipFree(p);
-  release(dshapes_);
-  release(cand_);
+  for (void *p : per_shape_) (void)hipFree(p);
   release(out_);
-  void *p = skip_;
-  release(p);
-  p = flag_;
-  release(p);
 }
 
 void ContactFinder::reserve(uint32_t shapes, uint32_t capacity) {
   if (shapes > shapes_cap_) {
-    release(dshapes_);
-    release(cand_);
-    void *p = skip_;
-    release(p);
-    p = flag_;
-    release(p);
-    skip_ = flag_ = nullptr;
+    for (void *p : per_shape_) HIPCHECK(hipFree(p));
+    per_shape_.clear();
     shapes_cap_ = 0;
-    HIPCHECK(hipMalloc(&dshapes_, shapes * sizeof(DevShape)));
-    HIPCHECK(hipMalloc(&cand_, std::max<size_t>(size_t(shapes) * T_, 1) * sizeof(Candidate)));
-    HIPCHECK(hipMalloc(&skip_, std::max<size_t>(size_t(shapes) * C_, 1)));
-    HIPCHECK(hipMalloc(&flag_, std::max<size_t>(size_t(shapes) * T_, 1)));
+    const size_t ST = std::max<size_t>(size_t(shapes) * T_, 1);
+    dshapes_ = alloc<DevShape>(shapes, per_shape_);
+    pre_ = alloc<ShapePre>(shapes, per_shape_);
+    cand_ = alloc<Candidate>(ST, per_shape_);
+    skip_ = alloc<uint8_t>(std::max<size_t>(size_t(shapes) * C_, 1), per_shape_);
+    flag_ = alloc<uint8_t>(ST, per_shape_);
+    state_ = alloc<uint8_t>(ST, per_shape_);
+    order_ = alloc<uint32_t>(ST, per_shape_);
+    nbn_ = alloc<uint32_t>(ST, per_shape_);
+    nbr_ = alloc<uint32_t>(ST * NEIGHBOURS, per_shape_);
+    pts_ = alloc<float>(3 * ST, per_shape_);
+    kount_ = alloc<uint32_t>(shapes, per_shape_);
     shapes_cap_ = shapes;
   }
   if (capacity > out_cap_) {
@@ -448,17 +501,31 @@ uint32_t ContactFinder::find(const double *x, const double *v, hipStream_t strea
   d.x = x, d.v = v, d.pos = pos_, d.vel = vel_;
   d.clusters = static_cast<ClusterData *>(clusters_);
   d.skip = skip_, d.flag = flag_, d.state = state_;
-  d.cand = static_cast<Candidate *>(cand_);
-  d.order = order_, d.nbr = nbr_, d.nbn = nbn_, d.count = count_;
+  d.cand = cand_;
+  d.order = order_, d.nbr = nbr_, d.nbn = nbn_, d.count = count_, d.kount = kount_, d.pts = pts_;
   d.out = static_cast<talus_mi_surface_contact *>(out_);
-  d.shapes = static_cast<const DevShape *>(dshapes_);
+  d.shapes = dshapes_;
+  d.pre = pre_;
   d.dt = q.dt;
   for (int k = 0; k < 3; ++k) d.g[k] = q.gravity[k];
   d.bias = q.bias, d.threshold = q.threshold;
-  d.apart = 1.0e-2f * std::max(std::max(q.voxel_size[0], q.voxel_size[1]), q.voxel_size[2]);
+  // duplicates lie within apart = 1e-2 of the largest voxel side
+  // (contact.cc:630); sqrtf is correctly rounded and monotone, so
+  // sqrtf(x) <= apart exactly when x <= near2
+  const float apart = 1.0e-2f * std::max(std::max(q.voxel_size[0], q.voxel_size[1]), q.voxel_size[2]);
+  float near2 = apart * apart;
+  if (!(apart >= 0.0f) || !std::isfinite(apart)) {
+    near2 = apart >= 0.0f ? INFINITY : -1.0f;
+  } else {
+    while (near2 > 0.0f && std::sqrt(near2) > apart) near2 = std::nextafter(near2, 0.0f);
+    while (std::sqrt(std::nextafter(near2, INFINITY)) <= apart) near2 = std::nextafter(near2, INFINITY);
+  }
+  d.near2 = near2;
   d.cull = q.cull ? 1 : 0;
   d.clustered = q.cull && C_ ? 1 : 0;
   hipLaunchKernelGGL(k_contact_state, dim3((N_ + 255) / 256), dim3(256), 0, stream, d);
+  HIPCHECK(hipGetLastError());
+  hipLaunchKernelGGL(k_contact_shapes, dim3((count + 63) / 64), dim3(64), 0, stream, d);
   HIPCHECK(hipGetLastError());
   if (d.clustered) {
     hipLaunchKernelGGL(k_contact_clusters, dim3((C_ + 7) / 8), dim3(256), 0, stream, d);
@@ -466,7 +533,11 @@ uint32_t ContactFinder::find(const double *x, const double *v, hipStream_t strea
   }
   hipLaunchKernelGGL(k_contact_faces, dim3((T_ + 255) / 256, count), dim3(256), 0, stream, d);
   HIPCHECK(hipGetLastError());
-  hipLaunchKernelGGL(k_contact_select, dim3(1), dim3(SELECT_THREADS), 0, stream, d);
+  hipLaunchKernelGGL(k_contact_compact, dim3(count), dim3(SELECT_THREADS), 0, stream, d);
+  HIPCHECK(hipGetLastError());
+  hipLaunchKernelGGL(k_contact_neighbours, dim3(256, count), dim3(256), 0, stream, d);
+  HIPCHECK(hipGetLastError());
+  hipLaunchKernelGGL(k_contact_resolve, dim3(1), dim3(SELECT_THREADS), 0, stream, d);
   HIPCHECK(hipGetLastError());
   uint32_t found = 0;
   HIPCHECK(hipMemcpyAsync(&found, count_, sizeof found, hipMemcpyDeviceToHost, stream));

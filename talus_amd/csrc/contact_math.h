@@ -404,6 +404,19 @@ TMI_HD inline float moving_speed(const Moving &m) {
   return fastest;
 }
 
+// what every triangle's test reads of a shape: its swept box over dt, its
+// radius and its pose at the start (sweptShapeBounds, shapeRadius,
+// advanceShape(shape, 0))
+struct ShapePre {
+  Bounds swept;
+  float radius;
+  ShapeAt origin;
+};
+
+TMI_HD inline ShapePre shape_pre(const DevShape &s, float dt) {
+  return ShapePre{swept_bounds(s, dt), shape_radius(s), advance_shape(s, 0.0f)};
+}
+
 // advanceToImpact (contact.cc:493-536)
 TMI_HD inline bool advance_to_impact(const DevShape &s, const Moving &m, float dt, float bias, float radius,
                                   Closest &closest, float &impact_time) {
@@ -440,8 +453,8 @@ struct FaceQuery {
 // the body of findContacts' face loop (contact.cc:739-869) for one triangle
 // and shape up to the duplicate test: false when the triangle makes no
 // contact, else the contact and its mesh point in *o
-TMI_HD inline bool face_contact(const FaceQuery &q, const DevShape &sh, int32_t partner, uint32_t face, float thick,
-                                Candidate *o) {
+TMI_HD inline bool face_contact(const FaceQuery &q, const DevShape &sh, const ShapePre &pre, int32_t partner,
+                                uint32_t face, float thick, Candidate *o) {
   const float sb = q.bias + thick, st = q.threshold + thick;
   Moving mv;
   Bounds reach = empty_bounds();
@@ -453,10 +466,9 @@ TMI_HD inline bool face_contact(const FaceQuery &q, const DevShape &sh, int32_t 
     include(reach, mv.start.c[k]);
     include(reach, mv.start.c[k] + mv.velocity[k] * q.dt);
   }
-  const Bounds swept = swept_bounds(sh, q.dt);
-  if (!overlaps(reach, swept, st)) return false;
-  const float radius = shape_radius(sh);
-  const ShapeAt origin = advance_shape(sh, 0.0f);
+  if (!overlaps(reach, pre.swept, st)) return false;
+  const float radius = pre.radius;
+  const ShapeAt &origin = pre.origin;
   if (q.cull) {
     const F3 centre = (mv.start.c[0] + mv.start.c[1] + mv.start.c[2]) / 3.0f;
     float spread = 0.0f;
