@@ -445,6 +445,67 @@ def lattice_run(workload, steps, warmup, cpu_seconds, threads, direct=False):
     return line
 
 
+def contact_run(queries, warmup, cpu_seconds):
+    """Contact generation (findContacts) of the 40^3 cube's surface (19,200
+    triangles) against five sampled-field partners (scenes.contact_cube) on
+    the GPU: queries/s over `queries` timed queries (each launches its
+    kernels, synchronizes and copies the contacts to the host).  The CPU
+    baseline runs the oracle's FP32 C restatement of the same query on one
+    host thread and compares its contacts with the GPU's byte for byte."""
+    from talus_amd import api, scenes
+
+    w = scenes.contact_cube(40)
+    q = dict(w["query"])
+    dt = q.pop("dt")
+    with api.Context(0, api.FP64) as ctx:
+        b = ctx.add_body(w["mesh"], w["materials"])
+        ctx.set_state(b, w["rest"], w["velocity"], np.zeros_like(w["rest"]))
+        ids = [ctx.field_add(f) for f in w["fields"]]
+        shapes = [dict(s, field=ids[s["field"]]) for s in w["shapes"]]
+        for _ in range(max(warmup, 1)):
+            got, found = ctx.find_contacts(b, shapes, dt, **q)
+        t0 = time.perf_counter()
+        for _ in range(queries):
+            got, found = ctx.find_contacts(b, shapes, dt, **q)
+        wall = time.perf_counter() - t0
+        x, v, _ = ctx.state(b)
+        srf = ctx.surface(b)
+    T = int(srf["triangles"].shape[0])
+    S = len(shapes)
+    value = queries / wall
+    line = {
+        "metric": "contact queries/sec (findContacts: tet surface vs sampled-field partners), 1 GPU",
+        "value": value, "unit": "queries/s", "n_gpus": 1, "steps": queries, "warmup": warmup,
+        "ms_per_step": 1e3 * wall / queries, "higher_is_better": True, "scaling": "weak", "vs_baseline": None,
+        "dtype": "f32", "data": "synthetic",
+        "config": {"workload": w["name"] + ": 40^3 cube (384,000 tets) falling onto a plane, three moving "
+                   "0.25 m spheres (one spinning) and one far away, bias 2 mm, threshold 2 cm, culling on",
+                   "triangles": T, "shapes": S, "contacts": int(found),
+                   "triangle_shape_pairs_per_s": T * S * value},
+    }
+    if cpu_seconds > 0:
+        from oracle import contact as OC
+
+        oshapes = [dict(s, field=w["fields"][s["field"]]) for s in w["shapes"]]
+        okw = dict(anchored=w["mesh"]["anchored"], friction=w["materials"]["friction"], **q)
+        xs, vs = x.astype(np.float32), v.astype(np.float32)
+        t0 = time.perf_counter()
+        n = 0
+        while True:
+            want = OC.find_contacts(srf, xs, vs, oshapes, dt, **okw)
+            n += 1
+            if time.perf_counter() - t0 > cpu_seconds:
+                break
+        cw = time.perf_counter() - t0
+        line["cpu_baseline"] = {"value": n / cw, "unit": "queries/s", "cores": 1, "kind": "port",
+                                "sample": f"{n} query(ies) of the same state with the oracle's FP32 C "
+                                          f"restatement on one host thread"}
+        line["gpu_over_cpu"] = value / line["cpu_baseline"]["value"]
+        line["parity"] = {"contacts_bit_equal": bool(got.tobytes() == want.tobytes()),
+                          "gpu_contacts": int(found), "oracle_contacts": int(len(want))}
+    return line
+
+
 def adjoint_check(steps, every, elements, seed=5):
     """Differentiation workload: the 1 x 0.1 x 0.1 m beam (103x9x9 hexes,
     50,058 tets, per-tet lognormal E, nu U[0.25,0.4]), a converged 60-step
@@ -555,6 +616,10 @@ def main(argv=None):
     ap.add_argument("--adjoint-elements", type=int, default=32)
     ap.add_argument("--lattice", choices=["cube", "bodies"], default=None,
                     help="strainstep hex lattices (FP64): element solves/s of the 40^3 cube or the 512-body batch")
+    ap.add_argument("--skip-contacts", action="store_true",
+                    help="leave the contact generation measurement out of the default line")
+    ap.add_argument("--contacts", action="store_true",
+                    help="contact generation queries/s on the 40^3 cube against five sampled-field partners")
     ap.add_argument("--copy-probe", action="store_true",
                     help="end with one 256 MiB device-to-device copy (calibrates PMC byte counters)")
     args = ap.parse_args(argv)
@@ -573,6 +638,11 @@ def main(argv=None):
                                      args.direct_launch)), flush=True)
         if args.copy_probe:
             copy_probe(256 << 20)
+        return 0
+
+    if args.contacts:
+        print(json.dumps(contact_run(max(args.steps, 1), args.warmup,
+                                     0.0 if args.skip_cpu else min(args.cpu_seconds, 3.0))), flush=True)
         return 0
 
     if args.adjoint:
@@ -733,6 +803,17 @@ def main(argv=None):
         cpu["single_thread"] = {"value": one["value"], "sample": one["sample"]}
         log(f"[bench] cpu 1 thread {one['value']:.4g} tet/s")
 
+    contacts = None
+    if not args.skip_contacts:
+        log("[bench] contact generation (contact_cube40)")
+        c = contact_run(50, 3, 0.0 if args.skip_cpu else 2.0)
+        contacts = {"workload": c["config"]["workload"], "queries_per_s": c["value"],
+                    "ms_per_query": c["ms_per_step"], "triangles": c["config"]["triangles"],
+                    "shapes": c["config"]["shapes"], "contacts": c["config"]["contacts"],
+                    "cpu_queries_per_s": c.get("cpu_baseline", {}).get("value"),
+                    "gpu_over_cpu": c.get("gpu_over_cpu"), "parity": c.get("parity")}
+        log(f"[bench] contacts {c['ms_per_step']:.3f} ms/query, {json.dumps(c.get('parity'))}")
+
     line = {
         "metric": METRIC,
         "value": value,
@@ -772,6 +853,7 @@ def main(argv=None):
         "roofline": roofline,
         "cpu_baseline": cpu,
         "gpu_over_cpu": (value / cpu["value"]) if cpu else None,
+        "contacts": contacts,
     }
     # the metric holds only for a finite run with converged solves and the
     # mixed kernel within 1e-5 of FP64: otherwise the line carries no value;

@@ -251,3 +251,64 @@ def build_mesh(body: Body):
             raise ValueError(f"per-tet materials: {mats.shape[0]} records for {t} tets")
         mesh["tet_material"] = np.arange(t, dtype=np.uint32)
     return mesh, mats
+
+
+# ------------------------------------------------ rigid partners' fields
+
+def sampled_field(fn, lo, hi, voxel, materials=None) -> dict:
+    """A sampled field (dict: origin, voxel, res, distances, materials) of
+    fn(x, y, z) on the lattice from lo with spacing voxel (float32
+    coordinates) up to hi; materials(x, y, z) gives each sample's lane."""
+    lo = np.asarray(lo, np.float32)
+    vox = np.float32(voxel)
+    res = (np.floor((np.asarray(hi, np.float64) - lo) / float(vox) + 1e-6) + 1).astype(np.int64)
+    axes = [lo[k] + vox * np.arange(res[k], dtype=np.float32) for k in range(3)]
+    z, y, x = np.meshgrid(axes[2], axes[1], axes[0], indexing="ij")
+    d = np.asarray(fn(x.astype(np.float64), y.astype(np.float64), z.astype(np.float64)), np.float32)
+    f = dict(origin=lo, voxel=(vox, vox, vox), res=res, distances=d.reshape(-1))
+    f["materials"] = (np.zeros(d.size, np.uint8) if materials is None
+                      else np.asarray(materials(x, y, z), np.uint8).reshape(-1))
+    return f
+
+
+def plane_field(height=0.0, lo=(-2.0, -1.0, -2.0), hi=(2.0, 1.0, 2.0), voxel=0.125, normal=(0.0, 1.0, 0.0),
+                materials=None) -> dict:
+    """The signed distance n . p - height of a plane, sampled."""
+    n = np.asarray(normal, np.float64)
+    n = n / np.linalg.norm(n)
+    return sampled_field(lambda x, y, z: n[0] * x + n[1] * y + n[2] * z - height, lo, hi, voxel, materials)
+
+
+def sphere_field(radius=0.25, lo=-0.35, hi=0.35, res=71) -> dict:
+    """A sphere about the origin, res samples per axis on [lo, hi]^3."""
+    voxel = (hi - lo) / (res - 1)
+    return sampled_field(lambda x, y, z: np.sqrt(x * x + y * y + z * z) - radius, (lo, lo, lo), (hi, hi, hi),
+                         voxel)
+
+
+def contact_cube(cells: int = 40, seed: int = 0) -> dict:
+    """Contact generation workload: the cells^3 cube (6 cells^3 tets, bottom
+    face pinned) falling at 0.5 m/s (+-0.05 noise) onto a plane 2 mm below
+    it while three 0.25 m sampled spheres (71^3 samples) move into its top
+    and two sides (one spinning at 2 rad/s) and a fourth stays far away;
+    dt 1/60 s, bias 2 mm, threshold 2 cm, gravity, duplicates within 1e-2 of
+    the cell size.  Returns mesh, materials, rest, velocity, fields, shapes
+    (field = index into fields) and the query arguments."""
+    sc = cube(cells)
+    mesh, mats = build_mesh(sc.bodies[0])
+    rest = mesh["rest_positions"].astype(np.float64)
+    rng = np.random.default_rng(seed)
+    vel = np.zeros_like(rest)
+    vel[:, 1] = -0.5
+    vel += 0.05 * rng.normal(size=vel.shape)
+    vel[mesh["anchored"] != 0] = 0.0
+    fields = [plane_field(height=-0.002, lo=(-0.5, -0.5, -0.5), hi=(1.5, 0.5, 1.5), voxel=0.05), sphere_field()]
+    shapes = [dict(field=0),
+              dict(field=1, center=(0.5, 1.27, 0.5), linear=(0.0, -1.0, 0.0)),
+              dict(field=1, center=(1.26, 0.5, 0.5), linear=(-1.0, 0.0, 0.0), angular=(0.0, 2.0, 0.0)),
+              dict(field=1, center=(0.5, 0.5, -0.26), linear=(0.0, 0.0, 1.0)),
+              dict(field=1, center=(-0.8, 0.5, 0.5))]
+    h = 1.0 / cells
+    query = dict(dt=1.0 / 60.0, bias=0.002, threshold=0.02, gravity=(0.0, -9.81, 0.0), voxel=(h, h, h))
+    return dict(mesh=mesh, materials=mats, rest=rest, velocity=vel, fields=fields, shapes=shapes, query=query,
+                name=f"contact_cube{cells}")

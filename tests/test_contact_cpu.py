@@ -30,31 +30,7 @@ from talus_amd import api, scenes
 
 # ------------------------------------------------------------------ fields
 
-def grid_field(fn, lo, hi, voxel, materials=None):
-    """A sampled field of fn(x, y, z) on the lattice origin lo, spacing
-    voxel (float32 coordinates), up to hi."""
-    lo = np.asarray(lo, np.float32)
-    vox = np.float32(voxel)
-    res = (np.floor((np.asarray(hi, np.float64) - lo) / float(vox) + 1e-6) + 1).astype(np.int64)
-    axes = [lo[k] + vox * np.arange(res[k], dtype=np.float32) for k in range(3)]
-    z, y, x = np.meshgrid(axes[2], axes[1], axes[0], indexing="ij")
-    d = np.asarray(fn(x.astype(np.float64), y.astype(np.float64), z.astype(np.float64)), np.float32)
-    f = dict(origin=lo, voxel=(vox, vox, vox), res=res, distances=d.reshape(-1))
-    f["materials"] = (np.zeros(d.size, np.uint8) if materials is None
-                      else np.asarray(materials(x, y, z), np.uint8).reshape(-1))
-    return f
-
-
-def plane_field(height=0.0, lo=(-2.0, -1.0, -2.0), hi=(2.0, 1.0, 2.0), voxel=0.125, normal=(0.0, 1.0, 0.0),
-                materials=None):
-    n = np.asarray(normal, np.float64)
-    n = n / np.linalg.norm(n)
-    return grid_field(lambda x, y, z: n[0] * x + n[1] * y + n[2] * z - height, lo, hi, voxel, materials)
-
-
-def sphere_field(radius=0.25, lo=-0.35, hi=0.35, res=71):
-    voxel = (hi - lo) / (res - 1)
-    return grid_field(lambda x, y, z: np.sqrt(x * x + y * y + z * z) - radius, (lo, lo, lo), (hi, hi, hi), voxel)
+grid_field, plane_field, sphere_field = scenes.sampled_field, scenes.plane_field, scenes.sphere_field
 
 
 # ------------------------------------------------------------------ bodies
