@@ -133,6 +133,32 @@ class Context {
                                 anchored.data()));
   }
 
+  // a rigid partner's sampled field, resident on the device from now on
+  uint32_t field_add(const talus_mi_sdf &field) {
+    uint32_t id = 0;
+    check(talus_mi_field_add(ctx_, &field, &id));
+    return id;
+  }
+  void field_set(uint32_t id, const talus_mi_sdf &field) { check(talus_mi_field_set(ctx_, id, &field)); }
+
+  // findContacts of a body's surface against the shapes; thickness (one per
+  // surface triangle) may be empty
+  std::vector<talus_mi_surface_contact> find_contacts(uint32_t body, const std::vector<talus_mi_shape> &shapes,
+                                                      const talus_mi_contact_query &query,
+                                                      const std::vector<float> &thickness = {}) {
+    std::vector<talus_mi_surface_contact> out(256);
+    uint32_t found = 0;
+    for (;;) {
+      check(talus_mi_find_contacts(ctx_, body, shapes.data(), uint32_t(shapes.size()), &query,
+                                   thickness.empty() ? nullptr : thickness.data(), out.data(),
+                                   uint32_t(out.size()), &found));
+      if (found <= out.size()) break;
+      out.resize(found);
+    }
+    out.resize(found);
+    return out;
+  }
+
  private:
   talus_mi_ctx *ctx_ = nullptr;
 };

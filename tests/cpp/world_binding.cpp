@@ -56,6 +56,26 @@ int main() {
     double sag = 0.0;
     for (size_t n = 0; n < s.x.size() / 3; ++n) sag = std::fmin(sag, s.x[3 * n + 1] - mesh.rest[3 * n + 1]);
     std::printf("stepped 10 steps nonfinite %u sag %.6e\n", report.non_finite, sag);
+    // a ground plane 30 cm below the beam as a rigid partner's sampled field
+    std::vector<float> ground(9 * 5 * 5);
+    for (uint32_t z = 0; z < 5; ++z)
+      for (uint32_t y = 0; y < 5; ++y)
+        for (uint32_t x = 0; x < 9; ++x) ground[x + 9 * (y + 5 * z)] = -0.5f + 0.25f * float(y) + 0.3f;
+    talus_mi_sdf field{{-0.5f, -0.5f, -0.5f}, {0.25f, 0.25f, 0.25f}, {9, 5, 5}, ground.data(), nullptr};
+    talus_mi_shape shape{};
+    shape.field = ctx.field_add(field);
+    shape.rotation[0] = 1.0f;
+    talus_mi_contact_query query{};
+    query.dt = params.dt;
+    query.threshold = 0.4f;
+    query.voxel_size[0] = query.voxel_size[1] = query.voxel_size[2] = 0.05f;
+    query.cull = 1;
+    const std::vector<talus_mi_surface_contact> contacts = ctx.find_contacts(body, {shape}, query);
+    float lowest = 1.0f;
+    for (const talus_mi_surface_contact &c : contacts) lowest = std::fmin(lowest, c.gap);
+    double bottom = 1.0;
+    for (size_t n = 0; n < s.x.size() / 3; ++n) bottom = std::fmin(bottom, s.x[3 * n + 1]);
+    std::printf("contacts %zu lowest gap %.6e lowest node %.6e\n", contacts.size(), lowest, bottom + 0.3);
   } catch (const talus_mi::Error &e) {
     std::printf("device unavailable code %d: %s\n", e.code, e.what());
   }

@@ -38,3 +38,16 @@ def test_cpp_binding_steps_on_the_device(program):
     m = re.search(r"stepped 10 steps nonfinite (\d+) sag (\S+)", out)
     assert m, out
     assert int(m.group(1)) == 0 and float(m.group(2)) < 0.0
+
+
+@pytest.mark.gpu
+def test_cpp_binding_finds_ground_contacts(program):
+    out = run(program)
+    m = re.search(r"contacts (\d+) lowest gap (\S+) lowest node (\S+)", out)
+    assert m, out
+    # a plane 0.3 m below the beam's bottom face (11 x 5 nodes): every bottom
+    # node touches it once, the sagging tip closest, at its own height above
+    # the plane (the closest point of a triangle to a plane is its lowest corner)
+    n, gap, node = int(m.group(1)), float(m.group(2)), float(m.group(3))
+    assert n >= 55 and 0.0 < gap <= 0.3
+    assert abs(gap - node) <= 1e-4
