@@ -396,12 +396,22 @@ __device__ __forceinline__ bool skip(const int *done, int parity, uint32_t B, ui
   return parity >= 0 && done[parity * B + body];
 }
 
+// A front's sizes and offsets, loaded together before the convergence test
+// (loads after the test's branch would wait for it)
+struct FrontMeta {
+  uint32_t k, m, body, rows_off;
+  uint64_t g_off, yoff, vec_off;
+  __device__ __forceinline__ FrontMeta(const FactorDev &f, uint32_t fr)
+      : k(f.k[fr]), m(f.m[fr]), body(f.body[fr]), rows_off(f.rows_off[fr]), g_off(f.g_off[fr]),
+        yoff(f.yoff[fr]), vec_off(f.vec_off[fr]) {}
+};
+
 // forward gather of row i of a front: r at the front's columns (i < k) plus
 // the children's update entries that map to row i, pulled in child order
 template <class Real>
-__device__ __forceinline__ double gather_row(const FactorDev &f, uint32_t fr, uint32_t i, uint32_t k,
+__device__ __forceinline__ double gather_row(const FactorDev &f, uint32_t rows_off, uint32_t i, uint32_t k,
                                              const Real *rin) {
-  const uint32_t nrow = f.rows_off[fr] + i / 3, d = i % 3;
+  const uint32_t nrow = rows_off + i / 3, d = i % 3;
   double v = i < k ? double(rin[3 * size_t(f.rows[nrow]) + d]) : 0.0;
   for (uint32_t q = f.src_start[nrow]; q < f.src_start[nrow + 1]; ++q) v += f.vec[f.src[q] + d];
   return v;
@@ -418,7 +428,7 @@ __global__ __launch_bounds__(NT) void k_solve_gather(FactorDev f, const uint64_t
   const uint32_t k = f.k[fr], m = f.m[fr];
   const uint32_t i = chunk * GCHUNK + threadIdx.x;
   if (i >= m) return;
-  f.vec[f.vec_off[fr] + i] = gather_row(f, fr, i, k, rin);
+  f.vec[f.vec_off[fr] + i] = gather_row(f, f.rows_off[fr], i, k, rin);
 }
 
 __global__ __launch_bounds__(NT) void k_solve_xgather(FactorDev f, const uint64_t *tasks, const int *done,
@@ -512,6 +522,36 @@ __device__ __forceinline__ void tile_dot(const G *g, uint64_t ts, const double *
     for (uint32_t q = 0; q < W; ++q) acc[q % 4] += gval(c.v[q]) * w[b * W + q];
   }
 }
+
+// The first PF tiles of a lane's run [b0, b1), loaded before the front
+// vector staged in LDS is needed: G does not depend on the vector, so these
+// loads stay in flight across the barrier (which waits for LDS, not for
+// outstanding global loads).  The loads are unconditional, clamped to
+// `last`, a tile of the lane's row or column that exists (a conditional load
+// would make the compiler wait for it where the branches join); dot() then
+// adds exactly what tile_dot(b0, b1) adds, in the same order.
+#ifndef TMI_PF
+#define TMI_PF 8
+#endif
+constexpr uint32_t PF = TMI_PF;
+template <class G>
+struct TilePrefetch {
+  Chunk<G> c[PF];
+  __device__ __forceinline__ void issue(const G *g, uint64_t ts, uint32_t b0, uint32_t last) {
+#pragma unroll
+    for (uint32_t u = 0; u < PF; ++u) c[u] = load_chunk(g + uint64_t(min(b0 + u, last)) * ts);
+  }
+  __device__ __forceinline__ void dot(const G *g, uint64_t ts, const double *w, uint32_t b0, uint32_t b1,
+                                      double *acc) const {
+    constexpr uint32_t W = Tile<G>::W;
+#pragma unroll
+    for (uint32_t u = 0; u < PF; ++u)
+      if (b0 + u < b1)
+#pragma unroll
+        for (uint32_t q = 0; q < W; ++q) acc[q % 4] += gval(c[u].v[q]) * w[(b0 + u) * W + q];
+    if (b0 + PF < b1) tile_dot(g, ts, w, b0 + PF, b1, acc);
+  }
+};
 
 // G = [M11; W21]: W21 tile (I, J) = sum_{q >= J} L21[I][q] M11[q][J]
 template <class Real>
@@ -682,27 +722,77 @@ __global__ __launch_bounds__(NT) void k_solve_forward_lanes(FactorDev f, const t
   __shared__ double vs[KLANES];
   const uint64_t t = tasks[blockIdx.x];
   const uint32_t fr = lo32(t), chunk = mid16(t);
-  if (skip(done, parity, B, f.body[fr])) return;
-  const uint32_t k = f.k[fr], m = f.m[fr];
+  const FrontMeta fm(f, fr);
+  if (skip(done, parity, B, fm.body)) return;
+  const uint32_t k = fm.k, m = fm.m;
   const uint32_t kp = (k + W - 1) / W * W;
-  double *v = f.vec + f.vec_off[fr];
-  const double *sc = f.gscale + f.yoff[fr];
-  for (uint32_t q = threadIdx.x; q < kp; q += NT) vs[q] = q < k ? gather_row(f, fr, q, k, rin) * sc[q] : 0.0;
-  __syncthreads();
+  double *v = f.vec + fm.vec_off;
+  const double *sc = f.gscale + fm.yoff;
   const uint32_t first = chunk * NT + (threadIdx.x & ~63u);
-  if (first >= m) return;
   const uint32_t i = chunk * NT + threadIdx.x;
-  const uint32_t nb = (min(k, first + 64u) + W - 1) / W;  // column tiles the wavefront needs
-  const G *g = gc + f.g_off[fr] + uint64_t(min(i, m - 1)) * W;
+  const uint32_t nb = first < m ? (min(k, first + 64u) + W - 1) / W : 0u;  // column tiles the wavefront needs
+  const G *g = gc + fm.g_off + uint64_t(min(i, m - 1)) * W;
+  // an update row's own right-hand side
+  const double own = (i >= k && i < m) ? gather_row(f, fm.rows_off, i, k, rin) : 0.0;
+  for (uint32_t q = threadIdx.x; q < kp; q += NT) vs[q] = q < k ? gather_row(f, fm.rows_off, q, k, rin) * sc[q] : 0.0;
+  TilePrefetch<G> pre;
+  pre.issue(g, uint64_t(m) * W, 0, (k - 1) / W);
+  __syncthreads();
+  if (first >= m) return;
   double acc[4] = {0.0, 0.0, 0.0, 0.0};
-  tile_dot(g, uint64_t(m) * W, vs, 0, nb, acc);
+  pre.dot(g, uint64_t(m) * W, vs, 0, nb, acc);
   if (i >= m) return;
   const double tot = (acc[0] + acc[1]) + (acc[2] + acc[3]);
   if (i < k)
-    f.y[f.yoff[fr] + i] = tot;
+    f.y[fm.yoff + i] = tot;
   else
-    v[i] = gather_row(f, fr, i, k, rin) - tot;
+    v[i] = own - tot;
 }
+
+// The backward kernels' LDS vector w = [y; -x2] from entry `from` to mp.
+// Staged in three parts so the tile prefetch overlaps the vector's loads:
+// load() reads the workgroup's first two rounds of entries (every entry of
+// most fronts) into registers, the caller issues the prefetch, store()
+// writes them (waiting for these loads, not for the tiles issued after
+// them: global loads complete in order), and rest() stages what is left.
+// The empty asm statements keep the compiler from moving loads across.
+__device__ __forceinline__ double w_entry(const double *y, const double *xsol, const uint32_t *rows, uint32_t k,
+                                          uint32_t m, uint32_t q) {
+  return q < k ? y[q] : (q < m ? -xsol[3 * size_t(rows[q / 3]) + q % 3] : 0.0);
+}
+struct WStage {
+  double w0, w1;
+  uint32_t q0, q1;
+  uint32_t orow;  // node of the lane's output column j
+  double oscale;  // s_j
+  __device__ __forceinline__ void load(const double *y, const double *xsol, const uint32_t *rows, uint32_t k,
+                                       uint32_t m, uint32_t mp, uint32_t from, uint32_t j, const double *gscale) {
+    q0 = from + threadIdx.x;
+    q1 = q0 + NT;
+    const uint32_t jc = min(j, k - 1);
+    orow = rows[jc / 3];
+    oscale = gscale[jc];
+    // unconditional loads at clamped indices, then selects: no branch, so
+    // the four value loads are in flight together
+    const uint32_t a0 = min(q0, m - 1), a1 = min(q1, m - 1);
+    const uint32_t r0 = rows[a0 / 3], r1 = rows[a1 / 3];
+    const double y0 = y[min(q0, k - 1)], y1 = y[min(q1, k - 1)];
+    const double x0 = xsol[3 * size_t(r0) + a0 % 3], x1 = xsol[3 * size_t(r1) + a1 % 3];
+    asm volatile("" ::: "memory");
+    w0 = q0 < k ? y0 : (q0 < m ? -x0 : 0.0);
+    w1 = q1 < k ? y1 : (q1 < m ? -x1 : 0.0);
+    asm volatile("" ::: "memory");
+  }
+  __device__ __forceinline__ void store(double *ws, uint32_t mp) const {
+    asm volatile("" ::: "memory");
+    if (q0 < mp) ws[q0] = w0;
+    if (q1 < mp) ws[q1] = w1;
+  }
+  __device__ __forceinline__ void rest(double *ws, const double *y, const double *xsol, const uint32_t *rows,
+                                       uint32_t k, uint32_t m, uint32_t mp) const {
+    for (uint32_t q = q1 + NT; q < mp; q += NT) ws[q] = w_entry(y, xsol, rows, k, m, q);
+  }
+};
 
 // backward for fronts with k <= KLANES and m <= MLANES: one lane per column
 // of G, along the column's row tiles of the row-tiled copy from the
@@ -717,23 +807,29 @@ __global__ __launch_bounds__(NT) void k_solve_backward_lanes(FactorDev f, const 
   __shared__ double ws[MLANES];
   const uint64_t t = tasks[blockIdx.x];
   const uint32_t fr = lo32(t), chunk = mid16(t);
-  if (skip(done, parity, B, f.body[fr])) return;
-  const uint32_t k = f.k[fr], m = f.m[fr];
+  const FrontMeta fm(f, fr);
+  if (skip(done, parity, B, fm.body)) return;
+  const uint32_t k = fm.k, m = fm.m;
   const uint32_t mp = (m + W - 1) / W * W;
-  const double *y = f.y + f.yoff[fr];
-  const uint32_t *rows = f.rows + f.rows_off[fr];
+  const double *y = f.y + fm.yoff;
+  const uint32_t *rows = f.rows + fm.rows_off;
   const uint32_t c0 = chunk * NT;
-  for (uint32_t q = c0 + threadIdx.x; q < mp; q += NT)
-    ws[q] = q < k ? y[q] : (q < m ? -f.xsol[3 * size_t(rows[q / 3]) + q % 3] : 0.0);
-  __syncthreads();
   const uint32_t first = c0 + (threadIdx.x & ~63u);
-  if (first >= k) return;
   const uint32_t j = c0 + threadIdx.x;
-  const G *g = gr + f.g_off[fr] + uint64_t(min(j, k - 1)) * W;
+  const G *g = gr + fm.g_off + uint64_t(min(j, k - 1)) * W;
+  const uint32_t tb0 = first / W, tb1 = first < k ? mp / W : tb0;
+  WStage st;
+  st.load(y, f.xsol, rows, k, m, mp, c0, j, f.gscale + fm.yoff);
+  TilePrefetch<G> pre;
+  pre.issue(g, uint64_t(k) * W, tb0, mp / W - 1);
+  st.store(ws, mp);
+  st.rest(ws, y, f.xsol, rows, k, m, mp);
+  __syncthreads();
+  if (first >= k) return;
   double acc[4] = {0.0, 0.0, 0.0, 0.0};
-  tile_dot(g, uint64_t(k) * W, ws, first / W, mp / W, acc);
+  pre.dot(g, uint64_t(k) * W, ws, tb0, tb1, acc);
   if (j >= k) return;
-  f.xsol[3 * size_t(rows[j / 3]) + j % 3] = ((acc[0] + acc[1]) + (acc[2] + acc[3])) * f.gscale[f.yoff[fr] + j];
+  f.xsol[3 * size_t(st.orow) + j % 3] = ((acc[0] + acc[1]) + (acc[2] + acc[3])) * st.oscale;
 }
 
 // forward, split shape (fronts with k <= KLANES): 64 rows of G per
@@ -750,29 +846,34 @@ __global__ __launch_bounds__(NT) void k_solve_forward_split(FactorDev f, const t
   __shared__ double red[NT / 64][64];
   const uint64_t t = tasks[blockIdx.x];
   const uint32_t fr = lo32(t), chunk = mid16(t);
-  if (skip(done, parity, B, f.body[fr])) return;
-  const uint32_t k = f.k[fr], m = f.m[fr];
-  double *v = f.vec + f.vec_off[fr];
+  const FrontMeta fm(f, fr);
+  if (skip(done, parity, B, fm.body)) return;
+  const uint32_t k = fm.k, m = fm.m;
+  double *v = f.vec + fm.vec_off;
   const uint32_t first = chunk * 64u;
   const uint32_t nb = (min(k, first + 64u) + W - 1) / W;
-  const double *sc = f.gscale + f.yoff[fr];
-  for (uint32_t q = threadIdx.x; q < nb * W; q += NT) vs[q] = q < k ? gather_row(f, fr, q, k, rin) * sc[q] : 0.0;
-  __syncthreads();
+  const double *sc = f.gscale + fm.yoff;
   const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
   const uint32_t i = first + lane;
   const uint32_t quarter = (nb + 3u) / 4u;
   const uint32_t b0 = min(nb, wave * quarter), b1 = min(nb, b0 + quarter);
-  const G *g = gc + f.g_off[fr] + uint64_t(min(i, m - 1)) * W;
+  const G *g = gc + fm.g_off + uint64_t(min(i, m - 1)) * W;
+  // an update row's own right-hand side
+  const double own = (wave == 0 && i >= k && i < m) ? gather_row(f, fm.rows_off, i, k, rin) : 0.0;
+  for (uint32_t q = threadIdx.x; q < nb * W; q += NT) vs[q] = q < k ? gather_row(f, fm.rows_off, q, k, rin) * sc[q] : 0.0;
+  TilePrefetch<G> pre;
+  pre.issue(g, uint64_t(m) * W, b0, (k - 1) / W);
+  __syncthreads();
   double acc[4] = {0.0, 0.0, 0.0, 0.0};
-  tile_dot(g, uint64_t(m) * W, vs, b0, b1, acc);
+  pre.dot(g, uint64_t(m) * W, vs, b0, b1, acc);
   red[wave][lane] = (acc[0] + acc[1]) + (acc[2] + acc[3]);
   __syncthreads();
   if (wave != 0 || i >= m) return;
   const double tot = ((red[0][lane] + red[1][lane]) + red[2][lane]) + red[3][lane];
   if (i < k)
-    f.y[f.yoff[fr] + i] = tot;
+    f.y[fm.yoff + i] = tot;
   else
-    v[i] = gather_row(f, fr, i, k, rin) - tot;
+    v[i] = own - tot;
 }
 
 // backward, split shape (k <= KLANES, m <= MLANES): 64 columns of G per
@@ -789,27 +890,31 @@ __global__ __launch_bounds__(NT) void k_solve_backward_split(FactorDev f, const 
   __shared__ double red[NT / 64][64];
   const uint64_t t = tasks[blockIdx.x];
   const uint32_t fr = lo32(t), chunk = mid16(t);
-  if (skip(done, parity, B, f.body[fr])) return;
-  const uint32_t k = f.k[fr], m = f.m[fr];
+  const FrontMeta fm(f, fr);
+  if (skip(done, parity, B, fm.body)) return;
+  const uint32_t k = fm.k, m = fm.m;
   const uint32_t mp = (m + W - 1) / W * W;
-  const double *y = f.y + f.yoff[fr];
-  const uint32_t *rowsf = f.rows + f.rows_off[fr];
+  const double *y = f.y + fm.yoff;
+  const uint32_t *rowsf = f.rows + fm.rows_off;
   const uint32_t first = chunk * 64u;
-  for (uint32_t q = first + threadIdx.x; q < mp; q += NT)
-    ws[q] = q < k ? y[q] : (q < m ? -f.xsol[3 * size_t(rowsf[q / 3]) + q % 3] : 0.0);
-  __syncthreads();
   const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
   const uint32_t j = first + lane;
   const uint32_t t0 = first / W, tiles = mp / W - t0, quarter = (tiles + 3u) / 4u;
   const uint32_t b0 = t0 + min(tiles, wave * quarter), b1 = t0 + min(tiles, wave * quarter + quarter);
-  const G *g = gr + f.g_off[fr] + uint64_t(min(j, k - 1)) * W;
+  const G *g = gr + fm.g_off + uint64_t(min(j, k - 1)) * W;
+  WStage st;
+  st.load(y, f.xsol, rowsf, k, m, mp, first, j, f.gscale + fm.yoff);
+  TilePrefetch<G> pre;
+  pre.issue(g, uint64_t(k) * W, b0, mp / W - 1);
+  st.store(ws, mp);
+  st.rest(ws, y, f.xsol, rowsf, k, m, mp);
+  __syncthreads();
   double acc[4] = {0.0, 0.0, 0.0, 0.0};
-  tile_dot(g, uint64_t(k) * W, ws, b0, b1, acc);
+  pre.dot(g, uint64_t(k) * W, ws, b0, b1, acc);
   red[wave][lane] = (acc[0] + acc[1]) + (acc[2] + acc[3]);
   __syncthreads();
   if (wave != 0 || j >= k) return;
-  f.xsol[3 * size_t(rowsf[j / 3]) + j % 3] =
-      (((red[0][lane] + red[1][lane]) + red[2][lane]) + red[3][lane]) * f.gscale[f.yoff[fr] + j];
+  f.xsol[3 * size_t(st.orow) + j % 3] = (((red[0][lane] + red[1][lane]) + red[2][lane]) + red[3][lane]) * st.oscale;
 }
 
 // backward, rows shape: x[col j] = sum_{j<=i<k} G[i][j] y[i]
