@@ -727,6 +727,37 @@ void Engine<Real>::build(const std::vector<Body> &bodies) {
   d_.p = dev<Real>(3 * N);
   d_.q = dev<Real>(3 * N);
   d_.conn = dev_copy(conn);
+  {
+    // each tet tile's nodes, once each in ascending order, and the tets'
+    // corners as positions in that list
+    std::vector<uint32_t> tn_start(1, 0), tn_node, scratch;
+    std::vector<uint2> conn_l(T);
+    uint32_t tn_max = 0;
+    for (size_t t = 0; t < ttb.size(); ++t) {
+      scratch.clear();
+      for (uint32_t e = ttbeg[t]; e < ttend[t]; ++e) {
+        const uint4 c = conn[e];
+        scratch.insert(scratch.end(), {c.x, c.y, c.z, c.w});
+      }
+      std::sort(scratch.begin(), scratch.end());
+      scratch.erase(std::unique(scratch.begin(), scratch.end()), scratch.end());
+      auto at = [&](uint32_t n) {
+        return uint32_t(std::lower_bound(scratch.begin(), scratch.end(), n) - scratch.begin());
+      };
+      for (uint32_t e = ttbeg[t]; e < ttend[t]; ++e) {
+        const uint4 c = conn[e];
+        conn_l[e] = make_uint2(at(c.x) | (at(c.y) << 16), at(c.z) | (at(c.w) << 16));
+      }
+      tn_node.insert(tn_node.end(), scratch.begin(), scratch.end());
+      tn_start.push_back(uint32_t(tn_node.size()));
+      tn_max = std::max(tn_max, uint32_t(scratch.size()));
+    }
+    static_assert(4 * TILE <= 65536, "tile node positions fit 16 bits");
+    d_.tn_start = dev_copy(tn_start);
+    d_.tn_node = dev_copy(tn_node);
+    d_.conn_l = dev_copy(conn_l);
+    d_.tn_max = tn_max;
+  }
   d_.dminv = dev_copy(dminv);
   if constexpr (sizeof(Real) == sizeof(double)) {
     d_.dminv_r = reinterpret_cast<const Real *>(d_.dminv);

@@ -80,6 +80,13 @@ struct Dev {
   Real *dx, *r, *z, *p, *q;
   // tets
   const uint4 *conn;
+  // per tet tile: its tets' nodes once each, ascending (tn_node[tn_start[t]
+  // .. tn_start[t+1]), at most tn_max per tile), and per tet its corners as
+  // 16-bit positions in its tile's list (conn_l), so an element kernel can
+  // stage the tile's node vectors in LDS and gather from there
+  const uint32_t *tn_start, *tn_node;
+  const uint2 *conn_l;
+  uint32_t tn_max;
   const double *dminv;
   const Real *dminv_r;  // Dm^-1 in the element precision (the operator's copy)
   const Real *vol, *lam, *mu;
