@@ -591,6 +591,10 @@ def main(argv=None):
                     help="steps per batched run (default: the workload's, 120 or 240); a shorter run "
                          "reports no metric value (for profiling)")
     ap.add_argument("--skip-plain", action="store_true")
+    ap.add_argument("--skip-refine", action="store_true",
+                    help="skip the FP64 residual measurement and the refined step after the run")
+    ap.add_argument("--refine", type=float, default=1e-9,
+                    help="refine_tolerance of the refined step after the run")
     ap.add_argument("--skip-cpu", action="store_true")
     ap.add_argument("--no-tune", action="store_true")
     ap.add_argument("--streams", type=int, default=3,
@@ -731,6 +735,27 @@ def main(argv=None):
     if reps_d:
         runs["fp64"] = health(reps_d)
     log(f"[bench] run health {json.dumps(runs)}")
+    # the FP64 residual |right - A a| / |right| of the mixed solves (the FP32
+    # PCG meets its tolerance on its own FP32 residual), measured on one more
+    # step after the run, then a step with FP64 iterative refinement
+    solve_precision = None
+    if not args.skip_refine:
+        measured = ctx_m.step(api.default_params(substeps=sub, preconditioner=api.FACTOR, refine_tolerance=1e-30,
+                                                 refine_rounds=0))
+        t0 = time.perf_counter()
+        refined = ctx_m.step(api.default_params(substeps=sub, preconditioner=api.FACTOR,
+                                                refine_tolerance=args.refine, refine_rounds=8))
+        refined_ms = 1e3 * (time.perf_counter() - t0)
+        solve_precision = {
+            "fp64_residual": measured["refine_worst_relative"],
+            "refined": {"tolerance": args.refine, "fp64_residual": refined["refine_worst_relative"],
+                        "unmet": refined["refine_unmet"], "corrections": refined["refinements"],
+                        "ms_per_step": refined_ms, "non_finite": refined["non_finite"]},
+            "steps": [total, total + 2]}
+        precision["solve"] = solve_precision
+        log(f"[bench] mixed solves' FP64 residual {measured['refine_worst_relative']:.3e}; refined to "
+            f"{refined['refine_worst_relative']:.3e} with {refined['refinements']} corrections, "
+            f"{refined_ms:.1f} ms/step")
 
     fp64_plain = None
     if not args.skip_plain and snaps is not None:

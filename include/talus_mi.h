@@ -125,6 +125,16 @@ typedef struct talus_mi_step_params {
                                40: Newton attempts per substep of a body with
                                yielding corotated lanes; 1 = the elastic
                                three-solve scheme */
+  /* FP64 iterative refinement of every elastic solve (no counterpart in the
+   * reference, whose CPU solve is FP32): 0 = off (default).  Otherwise,
+   * after each solve the residual right - A a is formed in FP64 (the
+   * operator in FP64 arithmetic on the FP64 rest data and acceleration) and
+   * correction solves A d = residual (the PCG at `tolerance` relative to the
+   * residual) are added until |right - A a| <= refine_tolerance |right| in
+   * every body or refine_rounds corrections were made.  Meant for the mixed
+   * path at tolerances below what FP32 vectors reach (about 1e-6). */
+  float refine_tolerance;
+  uint32_t refine_rounds;   /* default 8 */
 } talus_mi_step_params;
 
 /* deformable::StepReport subset (solver.h:211-311); filled per call. */
@@ -159,6 +169,12 @@ typedef struct talus_mi_report {
   uint32_t plastic_fallbacks;
   uint32_t plastic_rejections;
   uint32_t plastic_capped;
+  /* FP64 iterative refinement (refine_tolerance > 0): correction solves,
+   * summed over bodies and solves, and the largest final FP64 relative
+   * residual |right - A a| / |right| over bodies and solves */
+  uint32_t refinements;
+  uint32_t refine_unmet;    /* solves of a body left above refine_tolerance */
+  double refine_worst_relative;
 } talus_mi_report;
 
 /* The last talus_mi_step's StepReport of one body alone (deformable::step

@@ -186,6 +186,7 @@ class StepParams(C.Structure):
         ("tolerance", C.c_float), ("spectral_radius", C.c_float),
         ("preconditioner", C.c_uint32), ("flags", C.c_uint32),
         ("plastic_passes", C.c_uint32),
+        ("refine_tolerance", C.c_float), ("refine_rounds", C.c_uint32),
     ]
 
 
@@ -202,6 +203,8 @@ class Report(C.Structure):
         ("plastic_backtracks", C.c_uint32), ("plastic_linear_reads", C.c_uint32),
         ("plastic_handoffs", C.c_uint32), ("plastic_fallbacks", C.c_uint32),
         ("plastic_rejections", C.c_uint32), ("plastic_capped", C.c_uint32),
+        ("refinements", C.c_uint32), ("refine_unmet", C.c_uint32),
+        ("refine_worst_relative", C.c_double),
     ]
 
     def as_dict(self):

@@ -466,7 +466,10 @@ class Engine final : public EngineBase {
     built_ = false;
   }
   void upload_params(const std::vector<StepParams> &params, size_t slot);
-  void solve(bool first, uint32_t limit, uint32_t solve_index, bool tangent = false);
+  // rhs: the right-hand side (FP64, 3N) in place of d_.right (refinement's
+  // correction solves)
+  void solve(bool first, uint32_t limit, uint32_t solve_index, bool tangent = false,
+             double *rhs = nullptr);
   void all_active();
   // block Cholesky preconditioner
   void analyze_factor();
@@ -511,6 +514,22 @@ class Engine final : public EngineBase {
   // every (tet, corner) slot code 4e+c
   std::vector<uint32_t> inc_start_, slot_pos_;
   double *hnorms_ = nullptr, *d_norms_ = nullptr;  // corrections' convergence
+  // FP64 iterative refinement: pinned copies of the per-body test and
+  // relative residual, and the step's tallies
+  int *hrdone_ = nullptr;
+  double *hrrel_ = nullptr;
+  struct RefineTally {
+    uint32_t corrections = 0, unmet = 0;
+    double worst = 0.0;
+    std::vector<uint32_t> body_corrections, body_unmet;
+    std::vector<double> body_worst;
+  };
+  // refine the solve just made; increment: it solved for an increment added
+  // to a (d_.rbase keeps that increment's FP64 sum)
+  void refine(const talus_mi_step_params &sp, uint32_t limit, uint32_t pass, const std::vector<uint8_t> &mask,
+              bool increment, std::vector<uint32_t> &solve_pass, std::vector<std::vector<uint8_t>> &solve_active,
+              uint32_t &solve_index, RefineTally &tally);
+  void ensure_refine();
   // plastic Newton: bodies with a yielding corotated tet, pinned staging of
   // its per-tile norms, loading counts and per-operation parameters
   bool frames_ = false;  // the factor follows each body's rotation (bodyRotation)
@@ -588,6 +607,8 @@ void Engine<Real>::build(const std::vector<Body> &bodies) {
   bodies_ = &bodies;
   sym_ready_ = fac_ready_ = fac_ok_ = false;
   hnorms_ = d_norms_ = nullptr;
+  hrdone_ = nullptr;
+  hrrel_ = nullptr;
   want_refresh_.assign(bodies.size(), 0);
   next_refresh_.assign(bodies.size(), 0);
   fd_ = FactorDev{};
@@ -1396,7 +1417,7 @@ void Engine<Real>::all_active() {
 // solve) or zero (corrections) is the starting iterate; the iterate change
 // accumulates in dx and is added to a by apply_increment.
 template <class Real>
-void Engine<Real>::solve(bool first, uint32_t limit, uint32_t solve_index, bool tangent) {
+void Engine<Real>::solve(bool first, uint32_t limit, uint32_t solve_index, bool tangent, double *rhs) {
   hipStream_t s = stream_;
   const bool fac = use_factor_;
   // tangent: the neo-Hookean Hessian operator of the adjoint solve, zero
@@ -1408,6 +1429,10 @@ void Engine<Real>::solve(bool first, uint32_t limit, uint32_t solve_index, bool 
   if (first && !tangent) {
     op(d_.x0, -1);
     Launch<Real>::pcg_init(d_, true, fac, s);
+  } else if (rhs) {
+    Dev<Real> other = d_;
+    other.right = rhs;
+    Launch<Real>::pcg_init(other, false, fac, s);
   } else {
     Launch<Real>::pcg_init(d_, false, fac, s);
   }
@@ -1485,6 +1510,74 @@ void Engine<Real>::solve(bool first, uint32_t limit, uint32_t solve_index, bool 
   HIPCHECK(hipMemcpyAsync(htarget_ + slot, d_.target2, d_.B * sizeof(double), hipMemcpyDeviceToHost, s));
 }
 
+// FP64 iterative refinement of the solve just made: the FP64 residual of
+// every active body, then correction solves from zero on it (the PCG at
+// the step's tolerance relative to the residual, so each round cuts the
+// residual about by that factor) until every body meets refine_tolerance or
+// refine_rounds corrections were made.
+template <class Real>
+void Engine<Real>::ensure_refine() {
+  if (d_.slot64) return;
+  const uint32_t B = d_.B;
+  d_.slot64 = dev<double>(12 * size_t(d_.T));
+  d_.rres = dev<double>(3 * size_t(d_.N));
+  d_.rbase = dev<double>(3 * size_t(d_.N));
+  d_.rpart = dev<double>(2 * size_t(d_.n_ntiles));
+  d_.rrel = dev<double>(B);
+  d_.rdone = dev<int>(B);
+  hrdone_ = pinned<int>(B);
+  hrrel_ = pinned<double>(B);
+  ++graph_epoch_;  // captured iterations hold the old Dev
+}
+
+template <class Real>
+void Engine<Real>::refine(const talus_mi_step_params &sp, uint32_t limit, uint32_t pass,
+                          const std::vector<uint8_t> &mask, bool increment, std::vector<uint32_t> &solve_pass,
+                          std::vector<std::vector<uint8_t>> &solve_active, uint32_t &solve_index,
+                          RefineTally &tally) {
+  hipStream_t s = stream_;
+  const uint32_t B = d_.B;
+  ensure_refine();
+  if (tally.body_worst.size() != B) {
+    tally.body_corrections.assign(B, 0);
+    tally.body_unmet.assign(B, 0);
+    tally.body_worst.assign(B, 0.0);
+  }
+  const uint32_t rounds = sp.refine_rounds;
+  // a correction solve's unknown is its increment, kept as its own FP64 sum
+  if (increment) Launch<Real>::track_increment(d_, false, s);
+  for (uint32_t round = 0;; ++round) {
+    Launch<Real>::refine_residual(d_, increment ? d_.rbase : nullptr, double(sp.refine_tolerance), s);
+    HIPCHECK(hipMemcpyAsync(hrdone_, d_.rdone, B * sizeof(int), hipMemcpyDeviceToHost, s));
+    HIPCHECK(hipMemcpyAsync(hrrel_, d_.rrel, B * sizeof(double), hipMemcpyDeviceToHost, s));
+    HIPCHECK(hipStreamSynchronize(s));
+    bool all = true;
+    for (uint32_t b = 0; b < B && all; ++b) all = hrdone_[b] != 0;
+    if (all || round == rounds) {
+      for (uint32_t b = 0; b < B; ++b) {
+        if (!mask.empty() && !mask[b]) continue;
+        tally.worst = std::max(tally.worst, hrrel_[b]);
+        tally.body_worst[b] = std::max(tally.body_worst[b], hrrel_[b]);
+        if (!hrdone_[b]) {
+          ++tally.unmet;
+          ++tally.body_unmet[b];
+        }
+      }
+      return;
+    }
+    for (uint32_t b = 0; b < B; ++b)
+      if (!hrdone_[b]) {
+        ++tally.corrections;
+        ++tally.body_corrections[b];
+      }
+    solve_pass.push_back(pass);
+    solve_active.push_back(mask);
+    ensure_solve_slot(solve_index);
+    solve(false, limit, solve_index++, false, d_.rres);
+    if (increment) Launch<Real>::track_increment(d_, true, s);
+  }
+}
+
 template <class Real>
 void Engine<Real>::step(const talus_mi_step_params &sp, talus_mi_report &rep,
                         std::vector<talus_mi_event> &events, uint32_t step_index,
@@ -1551,6 +1644,10 @@ void Engine<Real>::step(const talus_mi_step_params &sp, talus_mi_report &rep,
   }
   const bool converge = (sp.flags & TALUS_MI_CONVERGE) != 0;
   graphs_ = (sp.flags & TALUS_MI_DIRECT_LAUNCH) == 0;
+  // FP64 iterative refinement of the elastic solves (not of converged
+  // rollouts, whose corrections already run to a fixed point)
+  const bool refining = sp.refine_tolerance > 0.0f && !converge;
+  RefineTally tally;
   const uint32_t reads = converge ? 64u : 3u;  // solves per pass at most
   if (reads * passes > hsolve_slots_) {
     HIPCHECK(hipStreamSynchronize(s));
@@ -1643,6 +1740,7 @@ void Engine<Real>::step(const talus_mi_step_params &sp, talus_mi_report &rep,
       solve_active.push_back(reg_mask);
       ensure_solve_slot(solve_index);
       solve(true, limit, solve_index++);
+      if (refining) refine(sp, limit, pass, reg_mask, false, solve_pass, solve_active, solve_index, tally);
       for (uint32_t read = 1; read < reads; ++read) {
         Launch<Real>::evaluate(d_, s);
         Launch<Real>::assemble(d_, diag, converge || read == 2, s, VISCOUS_TAKEN);
@@ -1651,6 +1749,7 @@ void Engine<Real>::step(const talus_mi_step_params &sp, talus_mi_report &rep,
         solve_active.push_back(reg_mask);
         ensure_solve_slot(solve_index);
         solve(false, limit, solve_index++);
+        if (refining) refine(sp, limit, pass, reg_mask, true, solve_pass, solve_active, solve_index, tally);
         if (converge) {
           // corrections until max |increment| <= 1e-12 max |a| in every body
           Launch<Real>::increment_norms(d_, d_norms_, s);
@@ -1772,6 +1871,15 @@ void Engine<Real>::step(const talus_mi_step_params &sp, talus_mi_report &rep,
   rep.plastic_fallbacks = newton_rep.plastic_fallbacks;
   rep.plastic_rejections = newton_rep.plastic_rejections;
   rep.plastic_capped = newton_rep.plastic_capped;
+  rep.refinements = tally.corrections;
+  rep.refine_unmet = tally.unmet;
+  rep.refine_worst_relative = tally.worst;
+  if (!tally.body_worst.empty())
+    for (uint32_t b = 0; b < B; ++b) {
+      body_rep_[b].refinements = tally.body_corrections[b];
+      body_rep_[b].refine_unmet = tally.body_unmet[b];
+      body_rep_[b].refine_worst_relative = tally.body_worst[b];
+    }
   if (use_factor_) {
     for (uint32_t b = 0; b < B; ++b)
       if (capped_body[b] || (worst[b] >= REFRESH_ITERS && steps_taken_ >= next_refresh_[b]))
@@ -2368,6 +2476,9 @@ class GroupedEngine final : public EngineBase {
       rep.plastic_fallbacks += r.plastic_fallbacks;
       rep.plastic_rejections += r.plastic_rejections;
       rep.plastic_capped += r.plastic_capped;
+      rep.refinements += r.refinements;
+      rep.refine_unmet += r.refine_unmet;
+      rep.refine_worst_relative = std::max(rep.refine_worst_relative, r.refine_worst_relative);
     }
     // events in the one-engine order: by body, then tet, yield before tear
     const size_t first = events.size();
@@ -2653,6 +2764,8 @@ void talus_mi_default_params(talus_mi_step_params *p) {
   p->preconditioner = TALUS_MI_BLOCK_JACOBI;
   p->flags = 0;
   p->plastic_passes = 40;
+  p->refine_tolerance = 0.0f;
+  p->refine_rounds = 8;
 }
 
 int talus_mi_create(talus_mi_ctx **out, int device, uint32_t precision) {
@@ -3329,6 +3442,9 @@ int talus_mi_rollout(talus_mi_ctx *ctx, const talus_mi_step_params *params, uint
       report->solves += r.solves;
       report->refactors += r.refactors;
       report->cg_worst_relative = std::max(report->cg_worst_relative, r.cg_worst_relative);
+      report->refinements += r.refinements;
+      report->refine_unmet += r.refine_unmet;
+      report->refine_worst_relative = std::max(report->refine_worst_relative, r.refine_worst_relative);
       report->elastic_energy = r.elastic_energy;
       report->kinetic_energy = r.kinetic_energy;
       report->yield_events += r.yield_events;

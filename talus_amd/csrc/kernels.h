@@ -123,6 +123,13 @@ struct Dev {
   double *rr_last;   // [B]
   int *inverts;      // [B]
   uint32_t *yields;  // [B]
+  // FP64 iterative refinement (allocated on first use): corner slots of the
+  // FP64 operator (12 per tet), the FP64 residual (3N), the FP64 sum of a
+  // correction solve's increments (3N), per node tile |residual|^2 and
+  // |right|^2, per body the residual relative to |right| and whether it
+  // meets the refinement tolerance
+  double *slot64, *rres, *rpart, *rrel, *rbase;
+  int *rdone;
 };
 
 enum : uint8_t { FLAG_SNH = 1, FLAG_PLASTIC = 2, FLAG_TEAR = 4, FLAG_DAMP = 8 };
@@ -164,6 +171,14 @@ struct Launch {
   static void pcg_update(const Dev<Real> &d, int parity, bool factor, hipStream_t s);
   static void pcg_direction(const Dev<Real> &d, int parity, hipStream_t s);
   static void apply_increment(const Dev<Real> &d, bool from_x0, hipStream_t s);
+  // FP64 iterative refinement: rres = right - A u with the operator in FP64
+  // arithmetic on the FP64 rest data, u the FP64 unknown of the last solve
+  // (a, or inc, the FP64 sum of the increments a correction solve added to
+  // a), rrel = |rres| / |right| per body, rdone = rrel <= tolerance (and
+  // rres zeroed for those bodies, so a correction solve leaves them alone)
+  static void refine_residual(const Dev<Real> &d, const double *inc, double tolerance, hipStream_t s);
+  // d.rbase = dx (add false) or d.rbase += dx, in FP64
+  static void track_increment(const Dev<Real> &d, bool add, hipStream_t s);
   static void finish(const Dev<Real> &d, hipStream_t s);
   static void commit(const Dev<Real> &d, hipStream_t s);
   static void tear(const Dev<Real> &d, hipStream_t s);

@@ -66,15 +66,15 @@ __device__ __forceinline__ void load_dm(const Dev<Real> &d, uint32_t e, Real *dm
 // Fp^-T g_i = (I + ep)^T g_i; the rest gradients for every other tet.  The
 // system operator's shear part is taken along them, so that it follows the
 // elastic response P(F Fp^-1) Fp^-T after the tet has flowed.
-template <class Real>
+template <class Real, class A = Real>
 __device__ __forceinline__ void plastic_gradients(const Dev<Real> &d, uint32_t e, bool flowing,
-                                                  const Real *g, Real *gs) {
+                                                  const A *g, A *gs) {
 #pragma unroll
   for (int i = 0; i < 12; ++i) gs[i] = g[i];
   if (!flowing) return;
-  Real ep[9];
+  A ep[9];
 #pragma unroll
-  for (int k = 0; k < 9; ++k) ep[k] = Real(d.plastic[(size_t)k * d.T + e]);
+  for (int k = 0; k < 9; ++k) ep[k] = A(d.plastic[(size_t)k * d.T + e]);
 #pragma unroll
   for (int i = 0; i < 4; ++i)
 #pragma unroll
@@ -341,20 +341,29 @@ __global__ __launch_bounds__(TILE) void k_pcg_begin(Dev<Real> d) {
 // with D = sum_j d_j g_j^T, so W:D = sum_j d_j . (W g_j).  At rest both are
 // the linear stiffness; under stretch the volumetric part follows the
 // (lambda+mu)(J-1)^2 term's stiffening along cof(F).
-template <class Real>
-__device__ __forceinline__ void op_tet_values(const Dev<Real> &d, uint32_t e, const Real *dv, Real *y,
+// A = Real is the operator of the PCG; A = double (with Real = float) is
+// the same operator in FP64 arithmetic with the FP64 rest data (Dm^-1, V,
+// lambda, mu), the residual of FP64 iterative refinement.
+template <class Real, class A = Real>
+__device__ __forceinline__ void op_tet_values(const Dev<Real> &d, uint32_t e, const A *dv, A *y,
                                               bool tangent = false) {
-  Real dm[9], g[12], gs[12], Rm[9];
-  load_dm(d, e, dm);
+  constexpr bool WIDE = sizeof(A) > sizeof(Real);
+  A dm[9], g[12], gs[12], Rm[9];
+  if constexpr (WIDE) {
+#pragma unroll
+    for (int k = 0; k < 9; ++k) dm[k] = d.dminv[(size_t)k * d.T + e];
+  } else {
+    load_dm(d, e, dm);
+  }
   shape_gradients(dm, g);
   const uint8_t flags = d.flags[e];
   plastic_gradients(d, e, (flags & FLAG_SNH) && (flags & FLAG_PLASTIC), g, gs);
 #pragma unroll
-  for (int k = 0; k < 9; ++k) Rm[k] = d.R[(size_t)k * d.T + e];
-  Real D[9] = {Real(0), Real(0), Real(0), Real(0), Real(0), Real(0), Real(0), Real(0), Real(0)};
+  for (int k = 0; k < 9; ++k) Rm[k] = A(d.R[(size_t)k * d.T + e]);
+  A D[9] = {A(0), A(0), A(0), A(0), A(0), A(0), A(0), A(0), A(0)};
 #pragma unroll
   for (int j = 0; j < 4; ++j) {
-    Real dp[3];
+    A dp[3];
     mtv3(Rm, &dv[3 * j], dp);
 #pragma unroll
     for (int r = 0; r < 3; ++r)
@@ -362,18 +371,19 @@ __device__ __forceinline__ void op_tet_values(const Dev<Real> &d, uint32_t e, co
       for (int k = 0; k < 3; ++k) D[r * 3 + k] += dp[r] * gs[3 * j + k];
   }
   const bool snh = (flags & FLAG_SNH) != 0;
-  const Real lam = d.lam[e], mu = d.mu[e], V = d.vol[e];
-  const Real tr = snh ? Real(0) : D[0] + D[4] + D[8];
-  Real S[9];
+  const A lam = WIDE ? A(d.lam64[e]) : A(d.lam[e]), mu = WIDE ? A(d.mu64[e]) : A(d.mu[e]),
+          V = WIDE ? A(d.vol64[e]) : A(d.vol[e]);
+  const A tr = snh ? A(0) : D[0] + D[4] + D[8];
+  A S[9];
   if (tangent && !snh && (flags & FLAG_PLASTIC)) {
     // the consistent J2 tangent (tangentOf) on the corotated strain sym(D')
-    Real lin[8], eps[9], sig[9];
+    A lin[8], eps[9], sig[9];
 #pragma unroll
-    for (int k = 0; k < 8; ++k) lin[k] = d.lin[(size_t)k * d.T + e];
+    for (int k = 0; k < 8; ++k) lin[k] = A(d.lin[(size_t)k * d.T + e]);
 #pragma unroll
     for (int r = 0; r < 3; ++r)
 #pragma unroll
-      for (int k = 0; k < 3; ++k) eps[r * 3 + k] = Real(0.5) * (D[r * 3 + k] + D[k * 3 + r]);
+      for (int k = 0; k < 3; ++k) eps[r * 3 + k] = A(0.5) * (D[r * 3 + k] + D[k * 3 + r]);
     j2_tangent_stress(lin, lam, mu, eps, sig);
 #pragma unroll
     for (int k = 0; k < 9; ++k) S[k] = V * sig[k];
@@ -382,29 +392,29 @@ __device__ __forceinline__ void op_tet_values(const Dev<Real> &d, uint32_t e, co
     for (int r = 0; r < 3; ++r)
 #pragma unroll
       for (int k = 0; k < 3; ++k)
-        S[r * 3 + k] = V * (mu * (D[r * 3 + k] + D[k * 3 + r]) + (r == k ? lam * tr : Real(0)));
+        S[r * 3 + k] = V * (mu * (D[r * 3 + k] + D[k * 3 + r]) + (r == k ? lam * tr : A(0)));
   }
 #pragma unroll
   for (int i = 0; i < 4; ++i) {
-    Real tmp[3];
+    A tmp[3];
     mv3(S, &gs[3 * i], tmp);
     mv3(Rm, tmp, &y[3 * i]);
   }
   if (snh) {
-    Real Wm[9];
+    A Wm[9];
 #pragma unroll
-    for (int k = 0; k < 9; ++k) Wm[k] = d.W[(size_t)k * d.T + e];
-    Real vol = Real(0);
+    for (int k = 0; k < 9; ++k) Wm[k] = A(d.W[(size_t)k * d.T + e]);
+    A vol = A(0);
 #pragma unroll
     for (int j = 0; j < 4; ++j) {
-      Real wg[3];
+      A wg[3];
       mv3(Wm, &g[3 * j], wg);
       vol += dv[3 * j] * wg[0] + dv[3 * j + 1] * wg[1] + dv[3 * j + 2] * wg[2];
     }
     vol *= V * lam;
 #pragma unroll
     for (int i = 0; i < 4; ++i) {
-      Real wg[3];
+      A wg[3];
       mv3(Wm, &g[3 * i], wg);
 #pragma unroll
       for (int r = 0; r < 3; ++r) y[3 * i + r] += vol * wg[r];
@@ -636,6 +646,127 @@ __global__ __launch_bounds__(TILE) void k_apply_increment(Dev<Real> d) {
   if (n >= d.nt_end[t] || d.anchored[n]) return;
 #pragma unroll
   for (int c = 0; c < 3; ++c) d.a[3 * (size_t)n + c] += double(d.dx[3 * (size_t)n + c]);
+}
+
+// FP64 iterative refinement.  The corner values of the operator applied to
+// the acceleration, in FP64 arithmetic (op_tet_values<Real, double>), into
+// the FP64 slots.
+// The unknown of the solve just made: the acceleration a (a solve from the
+// previous acceleration), or the FP64 sum inc of the increments a
+// correction solve and its refinements added to a (not a minus its old
+// value, which would cancel most of the increment's digits).
+__device__ __forceinline__ double unknown(const double *a, const double *inc, size_t i) {
+  return inc ? inc[i] : a[i];
+}
+
+// inc = dx (add = false) or inc += dx: the FP64 sum of a correction solve's
+// increments, exact sums of the FP32 (or FP64) increments up to FP64 rounding
+template <class Real>
+__global__ __launch_bounds__(TILE) void k_track_increment(Dev<Real> d, int add) {
+  const uint32_t t = blockIdx.x, b = d.nt_body[t];
+  if (!d.params[b].active) return;
+  const uint32_t n = d.nt_begin[t] + threadIdx.x;
+  if (n >= d.nt_end[t]) return;
+#pragma unroll
+  for (int c = 0; c < 3; ++c) {
+    const size_t i = 3 * (size_t)n + c;
+    const double dx = d.anchored[n] ? 0.0 : double(d.dx[i]);
+    d.rbase[i] = add ? d.rbase[i] + dx : dx;
+  }
+}
+
+template <class Real>
+__global__ __launch_bounds__(TILE) void k_op_element64(Dev<Real> d, const double *inc) {
+  const uint32_t t = blockIdx.x, b = d.tt_body[t];
+  if (!d.params[b].active) return;
+  const uint32_t e = d.tt_begin[t] + threadIdx.x;
+  if (e >= d.tt_end[t]) return;
+  double dv[12], y[12];
+  const uint4 c4 = d.conn[e];
+  const uint32_t nn[4] = {c4.x, c4.y, c4.z, c4.w};
+#pragma unroll
+  for (int i = 0; i < 4; ++i)
+#pragma unroll
+    for (int r = 0; r < 3; ++r) dv[3 * i + r] = unknown(d.a, inc, 3 * (size_t)nn[i] + r);
+  op_tet_values<Real, double>(d, e, dv, y, d.params[b].tangent != 0u);
+  const uint4 p = d.slot_pos[e];
+  const uint32_t pos[4] = {p.x, p.y, p.z, p.w};
+#pragma unroll
+  for (int c = 0; c < 4; ++c)
+#pragma unroll
+    for (int r = 0; r < 3; ++r) d.slot64[3 * (size_t)pos[c] + r] = y[3 * c + r];
+}
+
+// rres = right - (inertia M a + reach sum of the node's slots), anchored
+// rows 0; per node tile |rres|^2 and |right|^2
+template <class Real>
+__global__ __launch_bounds__(TILE) void k_residual64(Dev<Real> d, const double *inc) {
+  __shared__ double lds[TILE / 64];
+  const uint32_t t = blockIdx.x, b = d.nt_body[t];
+  const StepParams &P = d.params[b];
+  if (!P.active) return;
+  const uint32_t n = d.nt_begin[t] + threadIdx.x;
+  double rr = 0.0, bb = 0.0;
+  if (n < d.nt_end[t]) {
+    double q[3] = {0.0, 0.0, 0.0};
+    if (!d.anchored[n]) {
+      for (uint32_t k = d.inc_start[n]; k < d.inc_start[n + 1]; ++k)
+#pragma unroll
+        for (int c = 0; c < 3; ++c) q[c] += d.slot64[3 * (size_t)k + c];
+    }
+#pragma unroll
+    for (int c = 0; c < 3; ++c) {
+      const size_t i = 3 * (size_t)n + c;
+      double r = 0.0;
+      if (!d.anchored[n]) {
+        const double right = d.right[i];
+        r = right - (P.inertia * d.mass[n] * unknown(d.a, inc, i) + P.reach * q[c]);
+        bb += right * right;
+      }
+      d.rres[i] = r;
+      rr += r * r;
+    }
+  }
+  rr = block_sum(rr, lds);
+  bb = block_sum(bb, lds);
+  if (threadIdx.x == 0) {
+    d.rpart[2 * t] = rr;
+    d.rpart[2 * t + 1] = bb;
+  }
+}
+
+// one block per body: the relative FP64 residual, the refinement test, and
+// a zero right-hand side for the bodies that meet it
+template <class Real>
+__global__ __launch_bounds__(TILE) void k_refine_check(Dev<Real> d, double tolerance) {
+  __shared__ double lds[TILE / 64];
+  __shared__ int met;
+  const uint32_t b = blockIdx.x;
+  if (!d.params[b].active) {
+    if (threadIdx.x == 0) {
+      d.rdone[b] = 1;
+      d.rrel[b] = 0.0;
+    }
+    return;
+  }
+  const uint32_t t0 = d.body_nt0[b], t1 = d.body_nt1[b];
+  double rr = 0.0, bb = 0.0;
+  for (uint32_t t = t0 + threadIdx.x; t < t1; t += TILE) {
+    rr += d.rpart[2 * t];
+    bb += d.rpart[2 * t + 1];
+  }
+  rr = block_sum(rr, lds);
+  bb = block_sum(bb, lds);
+  if (threadIdx.x == 0) {
+    const double rel = bb > 0.0 ? sqrt(rr / bb) : (rr > 0.0 ? INFINITY : 0.0);
+    d.rrel[b] = rel;
+    met = rel <= tolerance ? 1 : 0;
+    d.rdone[b] = met;
+  }
+  __syncthreads();
+  if (!met) return;
+  const uint32_t n0 = d.nt_begin[t0], n1 = d.nt_end[t1 - 1];
+  for (uint32_t i = 3 * n0 + threadIdx.x; i < 3 * n1; i += TILE) d.rres[i] = 0.0;
 }
 
 template <class Real>
@@ -1772,6 +1903,16 @@ void Launch<Real>::pcg_direction(const Dev<Real> &d, int parity, hipStream_t s) 
 template <class Real>
 void Launch<Real>::apply_increment(const Dev<Real> &d, bool, hipStream_t s) {
   TMI_LAUNCH(k_apply_increment<Real>, d.n_ntiles, s, d);
+}
+template <class Real>
+void Launch<Real>::track_increment(const Dev<Real> &d, bool add, hipStream_t s) {
+  TMI_LAUNCH(k_track_increment<Real>, d.n_ntiles, s, d, add ? 1 : 0);
+}
+template <class Real>
+void Launch<Real>::refine_residual(const Dev<Real> &d, const double *inc, double tolerance, hipStream_t s) {
+  TMI_LAUNCH(k_op_element64<Real>, d.n_ttiles, s, d, inc);
+  TMI_LAUNCH(k_residual64<Real>, d.n_ntiles, s, d, inc);
+  TMI_LAUNCH(k_refine_check<Real>, d.B, s, d, tolerance);
 }
 template <class Real>
 void Launch<Real>::finish(const Dev<Real> &d, hipStream_t s) {
