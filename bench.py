@@ -51,6 +51,7 @@ METRIC = "tet updates/sec: mixed-precision batched vs plain FP64, 1 GPU, ≤1e-5
 HBM_PEAK_GBS = 8000.0
 FP64_MFMA_PEAK_TFS = 78.6  # MI355X dense FP64 matrix peak
 FORCE_BAR = 1.0e-5  # the metric's relative force error bar, mixed against FP64
+LEAF_CANDIDATES = (16, 24, 32, 48, 64)  # supernode leaf sizes the leaf tuner tries
 KINDS = {"assemble": 0, "op_element": 1, "op_gather": 2, "solve_forward_lanes": 3,
          "solve_backward_lanes": 4, "solve_forward_rows": 5, "solve_backward_rows": 6,
          "solve_forward_split": 7, "solve_backward_split": 8}
@@ -85,17 +86,22 @@ def workload(name):
 
 
 def run_batch(api, meshes, precision, params, steps, warmup, tune, plan=None, streams=1,
-              snapshot=False):
-    """Every body in one context, stepped `warmup` untimed steps, then `steps`
-    timed steps (the window).  Returns the context, body ids, the window's
-    wall time, the report of every step so far and, if asked, the state of
-    every body at the start of the window."""
+              snapshot=False, leaf=None, tune_leaf=False):
+    """Every body in one context (supernode leaf size `leaf`), stepped
+    `warmup` untimed steps, then `steps` timed steps (the window).  Returns
+    the context, body ids, the window's wall time, the report of every step
+    so far and, if asked, the state of every body at the start of the
+    window."""
     ctx = api.Context(0, precision)
     ids = [ctx.add_body(m, mats) for m, mats in meshes]
+    if leaf:
+        ctx.set_leaf(leaf)
     ctx.set_streams(streams)
     reps = [ctx.step(params)]
     tuned = None
-    if plan and len(plan) == len(ctx.plan()):
+    if tune_leaf:
+        tuned = ctx.tune_leaf(LEAF_CANDIDATES, 3)  # leaf size, then the per-level kernels
+    elif plan and len(plan) == len(ctx.plan()):
         ctx.set_plan([int(c) for c in plan])
     elif tune or plan:
         tuned = ctx.tune(3)  # no plan, or one made for another level structure
@@ -106,6 +112,14 @@ def run_batch(api, meshes, precision, params, steps, warmup, tune, plan=None, st
     timed = [ctx.step(params) for _ in range(steps)]
     wall = time.perf_counter() - t0
     return ctx, ids, wall, reps + timed, tuned, snap
+
+
+def read_plan(path):
+    """A saved solve plan: its digits and, when the file names it
+    ("leaf=N"), the supernode leaf size it was tuned for."""
+    tokens = open(path).read().split()
+    leaf = next((int(t[5:]) for t in tokens[1:] if t.startswith("leaf=")), None)
+    return tokens[0], leaf
 
 
 def body_snapshot(ctx, ids):
@@ -607,6 +621,10 @@ def main(argv=None):
     ap.add_argument("--retune", action="store_true",
                     help="autotune the mixed context even where profiles/solve_plan_<workload>.txt "
                          "holds a plan tuned earlier")
+    ap.add_argument("--tune-leaf", action="store_true",
+                    help="autotune the supernode leaf size too (talus_mi_tune_leaf), then the plan")
+    ap.add_argument("--leaf", type=int, default=None,
+                    help="supernode leaf size (default: the saved plan's, else the library's 64)")
     ap.add_argument("--cpu-seconds", type=float, default=15.0)
     ap.add_argument("--cpu-seconds-single", type=float, default=10.0)
     ap.add_argument("--cpu-threads", type=int, default=16)
@@ -666,10 +684,12 @@ def main(argv=None):
     plan_source = "fixed" if args.plan else "tuned"
     suffix = f"_s{args.streams}" if args.streams > 1 else ""
     saved = os.path.join(ROOT, "profiles", f"solve_plan_{args.workload}{suffix}.txt")
-    if args.plan is None and not args.retune and not args.no_tune and os.path.exists(saved):
-        with open(saved) as f:
-            args.plan = f.read().split()[0]
+    saved_leaf = None
+    if args.plan is None and not args.retune and not args.tune_leaf and not args.no_tune and os.path.exists(saved):
+        args.plan, saved_leaf = read_plan(saved)
         plan_source = f"profiles/solve_plan_{args.workload}{suffix}.txt (tuned earlier, re-applied)"
+    if args.leaf is None:
+        args.leaf = saved_leaf
 
     T = sum(int(m["tet_nodes"].shape[0]) for m, _ in meshes)
     N = sum(int(m["anchored"].shape[0]) for m, _ in meshes)
@@ -686,7 +706,8 @@ def main(argv=None):
     log("[bench] mixed batched")
     ctx_m, ids_m, wall_m, reps_m, tuned, _ = run_batch(api, meshes, api.MIXED, params, args.steps,
                                                       args.warmup, not args.no_tune, args.plan,
-                                                      args.streams)
+                                                      args.streams, leaf=args.leaf, tune_leaf=args.tune_leaf)
+    leaf = ctx_m.leaf()
     value = T * sub * args.steps / wall_m
     timed_m = reps_m[-args.steps:]
     plan_steps = "".join(map(str, ctx_m.plan()))
@@ -702,7 +723,7 @@ def main(argv=None):
         log("[bench] fp64 batched")
         ctx_d, ids_d, wall_d, reps_d, tuned_d, snaps = run_batch(api, meshes, api.FP64, params, args.steps,
                                                                  args.warmup, not args.no_tune, None,
-                                                                 args.streams, snapshot=True)
+                                                                 args.streams, snapshot=True, leaf=leaf)
         timed_d = reps_d[-args.steps:]
         fp64 = {"value": T * sub * args.steps / wall_d, "ms_per_step": 1e3 * wall_d / args.steps,
                 "cg_iterations_per_step": sum(r["cg_iterations"] for r in timed_d) / args.steps,
@@ -774,7 +795,10 @@ def main(argv=None):
     if args.streams > 1:
         ctx_m.set_streams(1)
         one = os.path.join(ROOT, "profiles", f"solve_plan_{args.workload}.txt")
-        plan1 = open(one).read().split()[0] if os.path.exists(one) and not args.retune else None
+        plan1 = None
+        if os.path.exists(one) and not args.retune and not args.tune_leaf:
+            plan1, leaf1 = read_plan(one)
+            plan1 = plan1 if leaf1 in (None, leaf) else None
         if plan1 and len(plan1) == len(ctx_m.plan()):
             ctx_m.set_plan([int(c) for c in plan1])
         else:
@@ -861,7 +885,7 @@ def main(argv=None):
             "cg_capped": sum(r["cg_capped"] for r in timed_m),
             "preconditioner_applications_per_step": apps,
             "refactors": sum(r["refactors"] for r in timed_m),
-            "solve_plan": plan_steps, "solve_plan_source": plan_source,
+            "solve_plan": plan_steps, "solve_plan_source": plan_source, "leaf": leaf,
             "kernel_table_plan": "".join(map(str, ctx_m.plan())),
             "tune": tuned,
         },

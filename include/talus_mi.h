@@ -455,12 +455,30 @@ typedef struct talus_mi_tune_report {
   uint32_t levels;
   uint32_t lanes_forward;  /* levels whose forward pass uses the lane shape */
   uint32_t lanes_backward;
-  uint32_t reserved;
+  uint32_t leaf;           /* supernode leaf size in use */
   double before_seconds;   /* one preconditioner application, previous plan */
   double after_seconds;    /* ... tuned plan */
 } talus_mi_tune_report;
 
 int talus_mi_tune(talus_mi_ctx *ctx, uint32_t repeats, talus_mi_tune_report *report);
+/* Supernode leaf size of the block Cholesky preconditioner: the nested
+ * dissection stops at fronts of at most `leaf` nodes (default 64).  Smaller
+ * leaves give less fill (fewer bytes per preconditioner application) and
+ * more elimination levels.  Changing it rebuilds the context's device
+ * data at the next call (the state is kept) and resets the solve plan.  No
+ * counterpart in the reference, whose SimplicialLLT has no supernodes. */
+#define TALUS_MI_LEAF_MIN 8u
+#define TALUS_MI_LEAF_MAX 512u
+int talus_mi_set_leaf(talus_mi_ctx *ctx, uint32_t leaf);
+int talus_mi_leaf_get(talus_mi_ctx *ctx, uint32_t *leaf);
+/* The autotuner over the leaf size: for each candidate, rebuild, factorize
+ * and tune the per-level solve kernels (talus_mi_tune), then keep the leaf
+ * whose tuned preconditioner application is fastest (HIP events, best of
+ * `repeats`).  times[i] (when not null) receives candidate i's application
+ * time in seconds; the report is the chosen leaf's talus_mi_tune report,
+ * with before_seconds the application time before the call. */
+int talus_mi_tune_leaf(talus_mi_ctx *ctx, const uint32_t *candidates, uint32_t count, uint32_t repeats,
+                       double *times, talus_mi_tune_report *report);
 /* Copies the plan (2*levels bytes) when capacity allows; *count = its size. */
 int talus_mi_plan_get(talus_mi_ctx *ctx, uint8_t *plan, uint32_t capacity, uint32_t *count);
 int talus_mi_plan_set(talus_mi_ctx *ctx, const uint8_t *plan, uint32_t count);

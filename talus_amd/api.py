@@ -228,7 +228,7 @@ FACTOR_KERNELS = {0: "assemble_pairs", 1: "extend_add", 2: "potrf", 3: "trsm", 4
 
 class TuneReport(C.Structure):
     _fields_ = [("levels", C.c_uint32), ("lanes_forward", C.c_uint32),
-                ("lanes_backward", C.c_uint32), ("reserved", C.c_uint32),
+                ("lanes_backward", C.c_uint32), ("leaf", C.c_uint32),
                 ("before_seconds", C.c_double), ("after_seconds", C.c_double)]
 
 
@@ -392,6 +392,10 @@ _SIGNATURES = {
                                                _P(C.c_uint32)]),
     "talus_mi_lattice_plastic_get": (C.c_int, [C.c_void_p, C.c_uint32, _P(C.c_double), _P(C.c_double),
                                                _P(C.c_uint32)]),
+    "talus_mi_tune_leaf": (C.c_int, [C.c_void_p, _P(C.c_uint32), C.c_uint32, C.c_uint32, _P(C.c_double),
+                                     _P(TuneReport)]),
+    "talus_mi_set_leaf": (C.c_int, [C.c_void_p, C.c_uint32]),
+    "talus_mi_leaf_get": (C.c_int, [C.c_void_p, _P(C.c_uint32)]),
     "talus_mi_lattice_plastic_set": (C.c_int, [C.c_void_p, C.c_uint32, _P(C.c_double), _P(C.c_double)]),
 }
 
@@ -1029,7 +1033,28 @@ class Context:
         """Autotunes the preconditioner solve's kernel shapes per level."""
         r = TuneReport()
         _check(lib().talus_mi_tune(self._h, repeats, C.byref(r)))
-        return {k: getattr(r, k) for k, _ in TuneReport._fields_ if k != "reserved"}
+        return {k: getattr(r, k) for k, _ in TuneReport._fields_}
+
+    def tune_leaf(self, candidates=(16, 24, 32, 48, 64, 96), repeats=3) -> dict:
+        """Autotunes the supernode leaf size (talus_mi_tune_leaf): the
+        chosen leaf's tune report plus each candidate's application time."""
+        cand = (C.c_uint32 * len(candidates))(*candidates)
+        times = (C.c_double * len(candidates))()
+        r = TuneReport()
+        _check(lib().talus_mi_tune_leaf(self._h, cand, len(candidates), repeats, times, C.byref(r)))
+        out = {k: getattr(r, k) for k, _ in TuneReport._fields_}
+        out["candidates"] = {int(c): t for c, t in zip(candidates, times)}
+        return out
+
+    def set_leaf(self, leaf: int):
+        """Supernode leaf size of the block Cholesky preconditioner
+        (talus_mi_set_leaf); rebuilds at the next call, keeps the state."""
+        _check(lib().talus_mi_set_leaf(self._h, leaf))
+
+    def leaf(self) -> int:
+        out = C.c_uint32()
+        _check(lib().talus_mi_leaf_get(self._h, C.byref(out)))
+        return out.value
 
     def plan(self) -> np.ndarray:
         n = C.c_uint32()

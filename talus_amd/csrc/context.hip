@@ -341,6 +341,8 @@ class EngineBase {
                                 double *mu_bar, double *rest_bar, double *x0_bar, double *v0_bar,
                                 double *a0_bar) = 0;
   virtual std::vector<uint8_t> plan() = 0;
+  // supernode leaf size of the next analysis (nodes per leaf front)
+  virtual void set_leaf(uint32_t leaf) = 0;
   virtual bool set_plan(const std::vector<uint8_t> &plan) = 0;
 };
 
@@ -415,6 +417,7 @@ class Engine final : public EngineBase {
                         uint32_t body, const double *x_bar, const double *v_bar, double tolerance,
                         double *lambda_bar, double *mu_bar, double *rest_bar, double *x0_bar,
                         double *v0_bar, double *a0_bar) override;
+  void set_leaf(uint32_t leaf) override { leaf_ = leaf; }
   std::vector<uint8_t> plan() override {
     if (!sym_ready_) analyze_factor();
     return plan_;
@@ -569,6 +572,7 @@ class Engine final : public EngineBase {
   bool sym_ready_ = false, fac_ready_ = false, fac_ok_ = false, use_factor_ = false;
   std::vector<double> fac_reach_, fac_inertia_, fac_damp_;
   FactorTables tab_;
+  uint32_t leaf_ = DEFAULT_LEAF;  // supernode leaf size (nodes)
   std::vector<uint8_t> plan_;  // per level: forward, backward kernel shape
   FactorDev fd_{};
   uint64_t *d_pair_target_ = nullptr;
@@ -874,7 +878,7 @@ template <class Real>
 void Engine<Real>::analyze_factor() {
   const std::vector<Body> &bodies = *bodies_;
   std::vector<Symbolic> sym(bodies.size());
-  for (size_t b = 0; b < bodies.size(); ++b) sym[b] = analyze(bodies[b], 64);
+  for (size_t b = 0; b < bodies.size(); ++b) sym[b] = analyze(bodies[b], leaf_);
   build_tables(sym, bodies, n0_, t0_, tab_);
   const FactorTables &t = tab_;
   fd_.fronts = t.fronts;
@@ -2613,6 +2617,9 @@ class GroupedEngine final : public EngineBase {
     unsupported();
   }
   // the groups' plans one after another
+  void set_leaf(uint32_t leaf) override {
+    for (auto &e : eng_) e->set_leaf(leaf);
+  }
   std::vector<uint8_t> plan() override {
     std::vector<uint8_t> out;
     for (uint32_t g = 0; g < eng_.size(); ++g) {
@@ -2673,6 +2680,7 @@ struct talus_mi_ctx {
   uint32_t steps = 0;
   std::vector<talus_mi_event> events;
   uint32_t streams = 1;  // engines stepped concurrently (talus_mi_set_streams)
+  uint32_t leaf = tmi::DEFAULT_LEAF;  // supernode leaf size (talus_mi_set_leaf)
   std::unique_ptr<tmi::LatticeSet> lattices;  // strainstep hex lattices
   // per lattice: the neo-Hookean hex terms and screen solve (built on use)
   std::vector<std::unique_ptr<tmi::HexScreen>> screens;
@@ -2716,8 +2724,11 @@ std::unique_ptr<tmi::EngineBase> make_engine(talus_mi_ctx *ctx) {
     if (precision == TALUS_MI_MIXED) return std::make_unique<tmi::Engine<float>>(s);
     return std::make_unique<tmi::Engine<double>>(s);
   };
-  if (ctx->streams <= 1) return one(ctx->stream);
-  return std::make_unique<tmi::GroupedEngine>(ctx->device, ctx->streams, ctx->stream, one);
+  std::unique_ptr<tmi::EngineBase> e;
+  if (ctx->streams <= 1) e = one(ctx->stream);
+  else e = std::make_unique<tmi::GroupedEngine>(ctx->device, ctx->streams, ctx->stream, one);
+  e->set_leaf(ctx->leaf);
+  return e;
 }
 
 void ensure(talus_mi_ctx *ctx) {
@@ -3307,6 +3318,7 @@ int talus_mi_tune(talus_mi_ctx *ctx, uint32_t repeats, talus_mi_tune_report *rep
     if (ctx->bodies.empty()) return TALUS_MI_OK;
     ensure(ctx);
     ctx->engine->tune(repeats, *report);
+    report->leaf = ctx->leaf;
     return TALUS_MI_OK;
   });
 }
@@ -3352,6 +3364,56 @@ int talus_mi_set_streams(talus_mi_ctx *ctx, uint32_t streams) {
     ++ctx->version;
     return TALUS_MI_OK;
   });
+}
+
+int talus_mi_set_leaf(talus_mi_ctx *ctx, uint32_t leaf) {
+  if (leaf < TALUS_MI_LEAF_MIN || leaf > TALUS_MI_LEAF_MAX)
+    return fail(TALUS_MI_ERR_ARGUMENT, "leaf must be " + std::to_string(TALUS_MI_LEAF_MIN) + ".." +
+                                           std::to_string(TALUS_MI_LEAF_MAX));
+  return guarded(ctx, [&]() -> int {
+    if (leaf == ctx->leaf) return TALUS_MI_OK;
+    if (!ctx->dirty) ctx->engine->download(ctx->bodies);  // keep the state of the current engine
+    ctx->leaf = leaf;
+    ctx->engine->set_leaf(leaf);
+    ctx->dirty = true;
+    ++ctx->version;
+    return TALUS_MI_OK;
+  });
+}
+
+int talus_mi_leaf_get(talus_mi_ctx *ctx, uint32_t *leaf) {
+  if (!ctx || !leaf) return fail(TALUS_MI_ERR_ARGUMENT, "null argument");
+  *leaf = ctx->leaf;
+  return TALUS_MI_OK;
+}
+
+int talus_mi_tune_leaf(talus_mi_ctx *ctx, const uint32_t *candidates, uint32_t count, uint32_t repeats,
+                       double *times, talus_mi_tune_report *report) {
+  if (!ctx || !report || (count && !candidates)) return fail(TALUS_MI_ERR_ARGUMENT, "null argument");
+  for (uint32_t i = 0; i < count; ++i)
+    if (candidates[i] < TALUS_MI_LEAF_MIN || candidates[i] > TALUS_MI_LEAF_MAX)
+      return fail(TALUS_MI_ERR_ARGUMENT, "leaf candidate out of range");
+  std::memset(report, 0, sizeof *report);
+  if (ctx->bodies.empty() || count == 0) return TALUS_MI_OK;
+  double before = 0.0;
+  int rc = talus_mi_tune(ctx, repeats, report);
+  if (rc != TALUS_MI_OK) return rc;
+  before = report->before_seconds;
+  uint32_t best = ctx->leaf;
+  double fastest = report->after_seconds;
+  for (uint32_t i = 0; i < count; ++i) {
+    if ((rc = talus_mi_set_leaf(ctx, candidates[i])) != TALUS_MI_OK) return rc;
+    if ((rc = talus_mi_tune(ctx, repeats, report)) != TALUS_MI_OK) return rc;
+    if (times) times[i] = report->after_seconds;
+    if (report->after_seconds < fastest) {
+      fastest = report->after_seconds;
+      best = candidates[i];
+    }
+  }
+  if ((rc = talus_mi_set_leaf(ctx, best)) != TALUS_MI_OK) return rc;
+  if ((rc = talus_mi_tune(ctx, repeats, report)) != TALUS_MI_OK) return rc;
+  report->before_seconds = before;
+  return TALUS_MI_OK;
 }
 
 }  // extern "C"

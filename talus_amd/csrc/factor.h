@@ -52,6 +52,7 @@ struct GStore<float> {
 };
 
 constexpr int FNB = 64;   // panel / tile width in DOF
+constexpr uint32_t DEFAULT_LEAF = 64;  // nodes per leaf front of the nested dissection
 constexpr uint32_t ROWS_GROUP = 8;  // rows-shape solve workgroup: 8 rows/columns, 4 wavefronts along the other dimension
 constexpr uint32_t GCHUNK = 256;     // front rows per gather workgroup
 constexpr uint32_t ZCHUNK = 65536;  // entries zeroed per workgroup before a factorization

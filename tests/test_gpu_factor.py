@@ -26,7 +26,7 @@ def dense_system(orc, m, inertia, step):
     return A[np.ix_(free, free)], free
 
 
-def _check_solve(body, x=None, bar=1e-10, precision=api.FP64, plan=None):
+def _check_solve(body, x=None, bar=1e-10, precision=api.FP64, plan=None, leaf=None):
     orc, m = oracle_body(body)
     if x is not None:
         orc.eval_forces(x)  # commits rotations and volume frames
@@ -40,6 +40,9 @@ def _check_solve(body, x=None, bar=1e-10, precision=api.FP64, plan=None):
     z_ref[free] = np.linalg.solve(A, r.reshape(-1)[free])
     with api.Context(0, precision) as ctx:
         b, _ = product_body(ctx, body)
+        if leaf is not None:
+            ctx.set_leaf(leaf)
+            assert ctx.leaf() == leaf
         ctx.set_rotations(b, R)
         ctx.set_frames(b, W)
         if plan is not None:
@@ -77,6 +80,61 @@ def test_factor_solve_either_kernel_shape(plan):
     solution on every level."""
     body = scenes.cube(9, seed=6).bodies[0]
     _check_solve(body, plan=plan)
+
+
+@pytest.mark.parametrize("leaf", [8, 16, 32, 128])
+def test_factor_solve_any_leaf_size(leaf):
+    """The supernode leaf size changes the fronts, never the solution."""
+    body = scenes.cube(9, seed=6).bodies[0]
+    info = _check_solve(body, leaf=leaf)
+    ref = _check_solve(body)
+    assert (info["fronts"] > ref["fronts"]) == (leaf < 64)
+
+
+def test_leaf_tuner_picks_a_candidate_and_its_plan_reproduces():
+    sc = scenes.bodies(count=6, seed=13, lo=300, hi=3000)
+    meshes = [scenes.build_mesh(b) for b in sc.bodies]
+    runs = []
+    chosen = None
+    for _ in range(2):
+        with api.Context(0, api.MIXED) as ctx:
+            ids = [ctx.add_body(m, mats) for m, mats in meshes]
+            if chosen is None:
+                rep = ctx.tune_leaf(candidates=(16, 32, 64), repeats=2)
+                assert rep["leaf"] in (16, 32, 64) and ctx.leaf() == rep["leaf"]
+                assert all(t > 0.0 for t in rep["candidates"].values())
+                assert rep["after_seconds"] <= min(rep["candidates"].values()) * 1.5
+                chosen = (rep["leaf"], ctx.plan())
+                assert chosen[1].size == 2 * rep["levels"]
+            else:
+                ctx.set_leaf(chosen[0])
+                ctx.set_plan(chosen[1])
+                assert np.array_equal(ctx.plan(), chosen[1])
+            for _ in range(3):
+                r = ctx.step(substeps=2, preconditioner=api.FACTOR)
+                assert not r["non_finite"] and r["cg_capped"] == 0
+            runs.append([ctx.state(b)[0] for b in ids])
+    for a, b in zip(*runs):
+        assert np.array_equal(a, b)
+
+
+def test_set_leaf_keeps_the_state():
+    sc = scenes.bodies(count=3, seed=5, lo=300, hi=2000)
+    meshes = [scenes.build_mesh(b) for b in sc.bodies]
+    with api.Context(0, api.FP64) as ctx:
+        ids = [ctx.add_body(m, mats) for m, mats in meshes]
+        for _ in range(2):
+            ctx.step(substeps=1, preconditioner=api.FACTOR)
+        before = [ctx.state(b) for b in ids]
+        ctx.set_leaf(24)
+        after = [ctx.state(b) for b in ids]
+        for p, q in zip(before, after):
+            for u, v in zip(p, q):
+                assert np.array_equal(u, v)
+        r = ctx.step(substeps=1, preconditioner=api.FACTOR)
+        assert not r["non_finite"] and r["cg_capped"] == 0
+        with pytest.raises(api.TalusError):
+            ctx.set_leaf(4)
 
 
 def test_tuned_plan_is_valid_and_reproducible():

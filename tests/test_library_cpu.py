@@ -98,3 +98,14 @@ def test_scenes_are_seeded():
     sc = scenes.bodies(count=16, seed=1)
     tets = [6 * (b.resolution[0] - 1) * (b.resolution[1] - 1) * (b.resolution[2] - 1) for b in sc.bodies]
     assert min(tets) >= 100 and max(tets) <= 30000
+
+
+def test_leaf_and_refinement_arguments_are_checked_before_any_device_call():
+    # -1 = TALUS_MI_ERR_ARGUMENT
+    L = api.lib()
+    assert L.talus_mi_set_leaf(None, 4) == -1
+    assert L.talus_mi_set_leaf(None, 1024) == -1
+    assert L.talus_mi_leaf_get(None, None) == -1
+    assert L.talus_mi_tune_leaf(None, None, 0, 1, None, None) == -1
+    p = api.default_params()
+    assert p.refine_tolerance == 0.0 and p.refine_rounds == 8
