@@ -520,13 +520,14 @@ def contact_run(queries, warmup, cpu_seconds):
     return line
 
 
-def adjoint_check(steps, every, elements, seed=5):
+def adjoint_check(steps, every, elements, seed=5, rest_coords=8):
     """Differentiation workload: the 1 x 0.1 x 0.1 m beam (103x9x9 hexes,
     50,058 tets, per-tet lognormal E, nu U[0.25,0.4]), a converged 60-step
     rollout with checkpoints every 10 steps, loss |x_tip - target|^2; the
     autograd gradients in E and nu of `elements` random tets and in 8 rest
-    coordinates against central finite differences (h = 1e-6 relative for
-    E, nu; 1e-7 m for rest positions) of the same FP64 rollout."""
+    coordinates (`rest_coords` of them) against central finite differences
+    (h = 1e-6 relative for E, nu; 1e-7 m for rest positions) of the same FP64
+    rollout."""
     import torch
 
     from talus_amd import adjoint, api, scenes
@@ -564,7 +565,7 @@ def adjoint_check(steps, every, elements, seed=5):
         rng = np.random.default_rng(seed)
         picks = rng.choice(E0.size, elements, replace=False)
         free = np.flatnonzero(np.repeat(mesh["anchored"] == 0, 3))
-        rest_picks = rng.choice(free, 8, replace=False)
+        rest_picks = rng.choice(free, rest_coords, replace=False)
         for kind, idxs in (("E", picks), ("nu", picks), ("rest", rest_picks)):
             errs, rels = [], []
             for idx in idxs:
