@@ -134,6 +134,17 @@ struct Dev {
 
 enum : uint8_t { FLAG_SNH = 1, FLAG_PLASTIC = 2, FLAG_TEAR = 4, FLAG_DAMP = 8 };
 
+// The work item (tile or task) of workgroup b in a grid of n.  Workgroups
+// are dealt to the GPU's 8 XCDs round-robin (workgroup b to XCD b % 8), each
+// XCD with its own L2; giving XCD x one contiguous run of items keeps what
+// neighbouring items share (nodes of adjacent tiles, a front's vector and
+// panels across its tasks) in one L2.  A bijection of [0, n).
+__device__ __forceinline__ uint32_t xcd_tile(uint32_t b, uint32_t n) {
+  constexpr uint32_t XCDS = 8;
+  const uint32_t q = n / XCDS, r = n % XCDS, x = b % XCDS, i = b / XCDS;
+  return x * q + min(x, r) + i;
+}
+
 // iterate updates of the plastic Newton, on the free nodes of active bodies
 enum NewtonVec : int {
   NV_BACKTRACK = 0,  // dx *= 1/2, a -= dx (halve the last increment)

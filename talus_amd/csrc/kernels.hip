@@ -480,7 +480,7 @@ template <class Real, bool DAMP>
 __global__ __launch_bounds__(TILE) void k_op_element(Dev<Real> d, const Real *dir, int parity) {
   extern __shared__ double staged_raw[];
   Real *staged = reinterpret_cast<Real *>(staged_raw);
-  const uint32_t t = blockIdx.x, b = d.tt_body[t];
+  const uint32_t t = xcd_tile(blockIdx.x, gridDim.x), b = d.tt_body[t];
   if (!d.params[b].active || (parity >= 0 && d.done[parity * d.B + b])) return;
   stage_tile_nodes(d.tn_start, d.tn_node, t, dir, staged);
   const uint32_t e = d.tt_begin[t] + threadIdx.x;
@@ -893,7 +893,7 @@ template <class Real, bool DIAG, bool ENERGY, bool DAMP>
 __global__ __launch_bounds__(TILE) void k_assemble(Dev<Real> d, int viscous) {
   constexpr bool MIXED = sizeof(Real) < 8;
   __shared__ double lds[TILE / 64];
-  const uint32_t t = blockIdx.x, b = d.tt_body[t];
+  const uint32_t t = xcd_tile(blockIdx.x, gridDim.x), b = d.tt_body[t];
   const StepParams &P = d.params[b];
   if (!P.active) return;
   const uint32_t e = d.tt_begin[t] + threadIdx.x;
