@@ -592,7 +592,7 @@ class Engine final : public EngineBase {
   // early refresh of single bodies' factors: a body whose solves needed at
   // least REFRESH_ITERS iterations in a step is refactored at the next step
   // (at most every REFRESH_COOLDOWN steps); a capped solve refreshes its body
-  static constexpr uint32_t REFRESH_ITERS = 5, REFRESH_COOLDOWN = 4;
+  static constexpr uint32_t REFRESH_ITERS = 4, REFRESH_COOLDOWN = 2;
   std::vector<int> want_refresh_;
   std::vector<talus_mi_report> body_rep_;  // the last step's report of each body
   std::vector<uint64_t> next_refresh_;
