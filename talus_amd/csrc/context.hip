@@ -476,6 +476,22 @@ class Engine final : public EngineBase {
   void all_active();
   // block Cholesky preconditioner
   void analyze_factor();
+  // Partial refreshes launch only the refreshed bodies' factor tasks: every
+  // task list holds each launch's tasks body by body (fronts are numbered
+  // body-major), so a body's tasks in a launch are one contiguous range.
+  // Compact copies of a partial refresh's lists are staged in pinned memory
+  // and uploaded once per refresh.
+  struct TaskSpan {
+    const void *dev;  // first task on the device
+    uint32_t count;
+  };
+  bool compactable_ = false;
+  uint64_t *hcompact64_ = nullptr, *dcompact64_ = nullptr;
+  uint32_t *hcompact32_ = nullptr, *dcompact32_ = nullptr;
+  size_t n64_ = 0, n32_ = 0;
+  const std::vector<int> *compact_mask_ = nullptr;
+  template <class T>
+  TaskSpan compact(const std::vector<T> &tasks, const T *dev, uint32_t s0, uint32_t s1);
   // per-launch HIP events of one factorization (factor_profile)
   struct FactorProfile {
     struct Launch {
@@ -907,6 +923,29 @@ void Engine<Real>::analyze_factor() {
   fd_.failed = dev<int>(d_.B);
   d_refresh_ = dev<int>(d_.B);
   fd_.refresh = d_refresh_;
+  {
+    // tasks of one launch are body-major in every list (partial refreshes
+    // rely on it); staging for compact lists
+    auto major = [&](const auto &tasks, const std::vector<uint32_t> &start) {
+      for (size_t r = 0; r + 1 < start.size(); ++r)
+        for (uint32_t q = start[r] + 1; q < start[r + 1]; ++q)
+          if (t.body[uint32_t(tasks[q])] < t.body[uint32_t(tasks[q - 1])]) return false;
+      return true;
+    };
+    const std::vector<uint32_t> one64 = {0u, uint32_t(t.w21_tasks.size())},
+                                one_pack = {0u, uint32_t(t.pack_tasks.size())},
+                                one_zero = {0u, uint32_t(t.zero_tasks.size())};
+    compactable_ = major(t.zero_tasks, one_zero) && major(t.ea_tasks, t.ea_start) &&
+                   major(t.potrf_tasks, t.potrf_start) && major(t.trsm_tasks, t.trsm_start) &&
+                   major(t.syrk_tasks, t.syrk_start) && major(t.trtri_tasks, t.trtri_start) &&
+                   major(t.w21_tasks, one64) && major(t.pack_tasks, one_pack);
+    const size_t n64 = t.zero_tasks.size() + t.ea_tasks.size() + t.trsm_tasks.size() + t.syrk_tasks.size() +
+                       t.trtri_tasks.size() + t.w21_tasks.size() + t.pack_tasks.size();
+    hcompact64_ = pinned<uint64_t>(n64);
+    dcompact64_ = dev<uint64_t>(n64);
+    hcompact32_ = pinned<uint32_t>(t.potrf_tasks.size());
+    dcompact32_ = dev<uint32_t>(t.potrf_tasks.size());
+  }
   d_zero_ = upload(t.zero_tasks);
   d_parent_ = upload(t.parent);
   d_pair_target_ = upload(t.pair_target);
@@ -950,6 +989,49 @@ void Engine<Real>::analyze_factor() {
 
 // Numeric factorization (refreshFactor, factor.cc:567-613) of
 // inertia M + reach sum R K R^T with the committed rotations.
+// The tasks of launch range [s0, s1) for the bodies flagged in
+// compact_mask_, appended to the pinned staging (their device copy lands at
+// the same offset once staged); every task when no mask is set.
+template <class Real>
+template <class T>
+typename Engine<Real>::TaskSpan Engine<Real>::compact(const std::vector<T> &tasks, const T *dev, uint32_t s0,
+                                                      uint32_t s1) {
+  if (!compact_mask_) return {dev + s0, s1 - s0};
+  T *host;
+  T *ddev;
+  size_t *n;
+  if constexpr (sizeof(T) == sizeof(uint64_t)) {
+    host = hcompact64_;
+    ddev = dcompact64_;
+    n = &n64_;
+  } else {
+    host = hcompact32_;
+    ddev = dcompact32_;
+    n = &n32_;
+  }
+  const size_t begin = *n;
+  const std::vector<int> &mask = *compact_mask_;
+  const FactorTables &t = tab_;
+  auto body_of = [&](uint32_t q) { return t.body[uint32_t(tasks[q])]; };
+  uint32_t q = s0;
+  while (q < s1) {
+    const uint32_t b = body_of(q);
+    // the end of body b's run
+    uint32_t lo = q, hi = s1;
+    while (lo < hi) {
+      const uint32_t mid = lo + (hi - lo) / 2;
+      if (body_of(mid) <= b) lo = mid + 1;
+      else hi = mid;
+    }
+    if (mask[b]) {
+      std::copy(tasks.begin() + q, tasks.begin() + lo, host + *n);
+      *n += lo - q;
+    }
+    q = lo;
+  }
+  return {ddev + begin, uint32_t(*n - begin)};
+}
+
 template <class Real>
 void Engine<Real>::factorize(const std::vector<double> &reach, const std::vector<double> &inertia,
                              const std::vector<double> *damp,
@@ -977,9 +1059,32 @@ void Engine<Real>::factorize(const std::vector<double> &reach, const std::vector
     damping.body_damp = d_body_damp_;
   }
   HIPCHECK(hipMemcpyAsync(d_refresh_, fresh.data(), d_.B * sizeof(int), hipMemcpyHostToDevice, s));
+  // the launches' task ranges: a partial refresh stages compact lists of the
+  // refreshed bodies' tasks and uploads them before the first launch
+  compact_mask_ = mask && compactable_ && !prof ? mask : nullptr;
+  n64_ = n32_ = 0;
+  const TaskSpan zero = compact(t.zero_tasks, d_zero_, 0u, uint32_t(t.zero_tasks.size()));
+  std::vector<TaskSpan> ea_span, potrf_span, trsm_span, syrk_span, trtri_span;
+  for (size_t r = 0; r + 1 < t.ea_start.size(); ++r) ea_span.push_back(compact(t.ea_tasks, d_ea_, t.ea_start[r], t.ea_start[r + 1]));
+  for (size_t r = 0; r + 1 < t.potrf_start.size(); ++r)
+    potrf_span.push_back(compact(t.potrf_tasks, d_potrf_, t.potrf_start[r], t.potrf_start[r + 1]));
+  for (size_t r = 0; r + 1 < t.trsm_start.size(); ++r)
+    trsm_span.push_back(compact(t.trsm_tasks, d_trsm_, t.trsm_start[r], t.trsm_start[r + 1]));
+  for (size_t r = 0; r + 1 < t.syrk_start.size(); ++r)
+    syrk_span.push_back(compact(t.syrk_tasks, d_syrk_, t.syrk_start[r], t.syrk_start[r + 1]));
+  for (size_t r = 0; r + 1 < t.trtri_start.size(); ++r)
+    trtri_span.push_back(compact(t.trtri_tasks, d_trtri_, t.trtri_start[r], t.trtri_start[r + 1]));
+  const TaskSpan w21 = compact(t.w21_tasks, d_w21_, 0u, uint32_t(t.w21_tasks.size()));
+  const TaskSpan pack = compact(t.pack_tasks, d_pack_, 0u, uint32_t(t.pack_tasks.size()));
+  if (compact_mask_) {
+    if (n64_) HIPCHECK(hipMemcpyAsync(dcompact64_, hcompact64_, n64_ * sizeof(uint64_t), hipMemcpyHostToDevice, s));
+    if (n32_) HIPCHECK(hipMemcpyAsync(dcompact32_, hcompact32_, n32_ * sizeof(uint32_t), hipMemcpyHostToDevice, s));
+  }
+  compact_mask_ = nullptr;
+  auto u64 = [](const TaskSpan &x) { return static_cast<const uint64_t *>(x.dev); };
   if (mask) {
     // partial refresh: only the flagged bodies' fronts are rebuilt
-    factor_zero(fd_, d_zero_, uint32_t(t.zero_tasks.size()), s);
+    factor_zero(fd_, u64(zero), zero.count, s);
   } else {
     HIPCHECK(hipMemsetAsync(fd_.pool, 0, std::max<uint64_t>(t.pool, 1) * sizeof(double), s));
     HIPCHECK(hipMemsetAsync(fd_.minv, 0, std::max<uint64_t>(t.minv_pool, 1) * sizeof(double), s));
@@ -1013,7 +1118,7 @@ void Engine<Real>::factorize(const std::vector<double> &reach, const std::vector
   for (uint32_t l = 0; l < t.levels; ++l) {
     for (uint32_t r = 0; r < t.ea_rounds[l]; ++r, ++ea)
       timed(TALUS_MI_FK_EXTEND_ADD, t.ea_start[ea + 1] - t.ea_start[ea], none, [&] {
-        factor_extend_add(fd_, d_parent_, d_ea_ + t.ea_start[ea], t.ea_start[ea + 1] - t.ea_start[ea], s);
+        factor_extend_add(fd_, d_parent_, u64(ea_span[ea]), ea_span[ea].count, s);
       });
     for (uint32_t p = 0; p < t.fac_panels[l]; ++p, ++fac) {
       const uint32_t P0 = p * FNB;
@@ -1026,7 +1131,7 @@ void Engine<Real>::factorize(const std::vector<double> &reach, const std::vector
             }
           },
           [&] {
-            factor_potrf(fd_, d_potrf_ + t.potrf_start[fac], t.potrf_start[fac + 1] - t.potrf_start[fac], p, s);
+            factor_potrf(fd_, static_cast<const uint32_t *>(potrf_span[fac].dev), potrf_span[fac].count, p, s);
           });
       timed(
           TALUS_MI_FK_TRSM, t.trsm_start[fac + 1] - t.trsm_start[fac],
@@ -1040,7 +1145,7 @@ void Engine<Real>::factorize(const std::vector<double> &reach, const std::vector
               useful += 2.0 * nr * nb * nb;
             }
           },
-          [&] { factor_trsm(fd_, d_trsm_ + t.trsm_start[fac], t.trsm_start[fac + 1] - t.trsm_start[fac], p, s); });
+          [&] { factor_trsm(fd_, u64(trsm_span[fac]), trsm_span[fac].count, p, s); });
       timed(
           TALUS_MI_FK_SYRK, t.syrk_start[fac + 1] - t.syrk_start[fac],
           [&](double &mfma, double &useful) {
@@ -1053,7 +1158,7 @@ void Engine<Real>::factorize(const std::vector<double> &reach, const std::vector
               useful += 2.0 * dim(t.m[fr], R + I * FNB) * dim(t.m[fr], R + J * FNB) * nb;
             }
           },
-          [&] { factor_syrk(fd_, d_syrk_ + t.syrk_start[fac], t.syrk_start[fac + 1] - t.syrk_start[fac], p, s); });
+          [&] { factor_syrk(fd_, u64(syrk_span[fac]), syrk_span[fac].count, p, s); });
     }
   }
   for (size_t i = 0; i + 1 < t.trtri_start.size(); ++i)
@@ -1070,7 +1175,7 @@ void Engine<Real>::factorize(const std::vector<double> &reach, const std::vector
             mfma += double(bi - bj) * TILE3;
           }
         },
-        [&] { factor_trtri(fd_, d_trtri_ + t.trtri_start[i], t.trtri_start[i + 1] - t.trtri_start[i], s); });
+        [&] { factor_trtri(fd_, u64(trtri_span[i]), trtri_span[i].count, s); });
   timed(
       TALUS_MI_FK_PACK, t.w21_tasks.size() + t.pack_tasks.size(),
       [&](double &mfma, double &useful) {
@@ -1083,8 +1188,7 @@ void Engine<Real>::factorize(const std::vector<double> &reach, const std::vector
         }
       },
       [&] {
-        SolveLaunch<Real>::pack(fd_, d_w21_, uint32_t(t.w21_tasks.size()), d_pack_, uint32_t(t.pack_tasks.size()),
-                                d_gr_, d_gc_, s);
+        SolveLaunch<Real>::pack(fd_, u64(w21), w21.count, u64(pack), pack.count, d_gr_, d_gc_, s);
       });
   // the refreshed bodies' positions become their factor's reference frame
   Launch<Real>::frame_reference(d_, d_refresh_, s);
