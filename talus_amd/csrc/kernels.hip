@@ -689,6 +689,23 @@ __global__ __launch_bounds__(TILE) void k_op_element64(Dev<Real> d, const double
 #pragma unroll
     for (int r = 0; r < 3; ++r) dv[3 * i + r] = unknown(d.a, inc, 3 * (size_t)nn[i] + r);
   op_tet_values<Real, double>(d, e, dv, y, d.params[b].tangent != 0u);
+  if (d.fit && (d.flags[e] & FLAG_DAMP)) {
+    // the damping term as the PCG's operator adds it (op_damping), in FP64
+    const StepParams &P = d.params[b];
+    double fit[FIT_PLANES], Rm[9], dm[9], g[12], out[12];
+#pragma unroll
+    for (int k = 0; k < FIT_PLANES; ++k) fit[k] = double(d.fit[(size_t)k * d.T + e]);
+#pragma unroll
+    for (int k = 0; k < 9; ++k) {
+      Rm[k] = double(d.R[(size_t)k * d.T + e]);
+      dm[k] = d.dminv[(size_t)k * d.T + e];
+    }
+    shape_gradients(dm, g);
+    damp_tet(fit, Rm, g, d.vol64[e], d.lam64[e], d.mu64[e], dv, out);
+    const double w = (P.damp / P.reach) * double(d.tau[e]);
+#pragma unroll
+    for (int k = 0; k < 12; ++k) y[k] += w * out[k];
+  }
   const uint4 p = d.slot_pos[e];
   const uint32_t pos[4] = {p.x, p.y, p.z, p.w};
 #pragma unroll

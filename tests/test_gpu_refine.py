@@ -117,3 +117,18 @@ def test_unmet_refinement_is_reported():
     assert r["refine_unmet"] >= 1
     assert r["refine_worst_relative"] > 1e-14
     assert not r["non_finite"]
+
+
+@pytest.mark.parametrize("precision", [api.MIXED, api.FP64])
+def test_refinement_of_a_damped_body(precision):
+    """A body with stiffness damping: the FP64 residual carries the damping
+    term the PCG's operator adds, so the corrections converge."""
+    body = scenes.cantilever().bodies[0]
+    body.materials = scenes.material_array(1, youngs_modulus=1e6, poissons_ratio=0.3, density=1000.0,
+                                           elastic_model=api.NEO_HOOKEAN, stiffness_damping=0.02)
+    _, reps = _product(body, precision, 2, 2, tolerance=1e-3, cg_iterations=400, preconditioner=api.FACTOR,
+                       refine_tolerance=1e-10, refine_rounds=12)
+    for r in reps:
+        assert r["refinements"] > 0 and r["refine_unmet"] == 0
+        assert r["refine_worst_relative"] <= 1e-10
+        assert not r["non_finite"]
