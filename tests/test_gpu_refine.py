@@ -126,9 +126,16 @@ def test_refinement_of_a_damped_body(precision):
     body = scenes.cantilever().bodies[0]
     body.materials = scenes.material_array(1, youngs_modulus=1e6, poissons_ratio=0.3, density=1000.0,
                                            elastic_model=api.NEO_HOOKEAN, stiffness_damping=0.02)
-    _, reps = _product(body, precision, 2, 2, tolerance=1e-3, cg_iterations=400, preconditioner=api.FACTOR,
+    x, reps = _product(body, precision, 2, 2, tolerance=1e-3, cg_iterations=400, preconditioner=api.FACTOR,
                        refine_tolerance=1e-10, refine_rounds=12)
     for r in reps:
         assert r["refinements"] > 0 and r["refine_unmet"] == 0
         assert r["refine_worst_relative"] <= 1e-10
         assert not r["non_finite"]
+    # the refined step lands where a tight FP64 solve of the damped system
+    # does (a residual without the damping term would converge elsewhere)
+    xt, _ = _product(body, api.FP64, 2, 2, tolerance=1e-11, cg_iterations=4000, factor_budget=4000,
+                     preconditioner=api.FACTOR)
+    _, _, rest = None, None, scenes.build_mesh(body)[0]["rest_positions"]
+    disp = np.abs(xt - rest).max()
+    assert np.abs(x - xt).max() / disp <= (1e-6 if precision == api.MIXED else 1e-9)
