@@ -177,12 +177,19 @@ __global__ __launch_bounds__(NT) void k_extend_add(FactorDev f, const uint32_t *
 
 // -------------------------------------------------------------------- POTRF
 
-// Cholesky of the 64x64 diagonal block of panel p in LDS by one wavefront
-// (lane i owns row i); writes L_pp to the front and L_pp^-1 to the (p,p)
-// block of M11.
+// Cholesky of the 64x64 diagonal block of panel p by one wavefront; writes
+// L_pp to the front and L_pp^-1 to the (p,p) block of M11.  Lane i keeps row
+// i of the block in registers (r[l] = A[i][l], l <= i) and every loop is
+// unrolled so the registers are indexed at compile time:
+//   factorization (right-looking): for each column j, the pivot comes from
+//   lane j by a shuffle, lanes below scale their r[j], and lane i updates
+//   r[l] -= r[j] L[l][j] for l > j with L[l][j] shuffled from lane l (lanes
+//   above the diagonal compute values they never use);
+//   inverse: L is stored to LDS, then lane c forms column c of L^-1 by
+//   forward substitution, x[q] in registers and L[i][q] read from LDS (the
+//   same address for every lane: a broadcast).
 __global__ __launch_bounds__(64) void k_potrf(FactorDev f, const uint32_t *tasks, uint32_t panel) {
-  __shared__ double a[FNB][LDP];
-  __shared__ double x[FNB][LDP];
+  __shared__ double Ls[FNB][LDP];
   const uint32_t fr = tasks[blockIdx.x];
   if (!stale(f, fr)) return;
   const uint32_t k = f.k[fr], m = f.m[fr];
@@ -190,44 +197,57 @@ __global__ __launch_bounds__(64) void k_potrf(FactorDev f, const uint32_t *tasks
   const int nb = int(min(uint32_t(FNB), k - P0));
   double *A = f.pool + f.off[fr];
   const int i = threadIdx.x;
-  for (int j = 0; j < FNB; ++j) a[i][j] = (i < nb && j < nb) ? A[uint64_t(P0 + j) * m + P0 + i] : 0.0;
-  __syncthreads();
+  double r[FNB];
+#pragma unroll
+  for (int l = 0; l < FNB; ++l)
+    r[l] = (i < nb && l < nb && l <= i) ? A[uint64_t(P0 + l) * m + P0 + i] : (l == i ? 1.0 : 0.0);
   bool bad = false;
-  for (int j = 0; j < nb; ++j) {
-    double d = a[j][j];
-    if (!(d > 0.0)) {
-      bad = true;
-      d = 1.0;
+#pragma unroll
+  for (int j = 0; j < FNB; ++j) {
+    if (j < nb) {
+      double d = __shfl(r[j], j, 64);
+      if (!(d > 0.0)) {
+        bad = true;
+        d = 1.0;
+      }
+      const double piv = sqrt(d);
+      if (i == j) r[j] = piv;
+      if (i > j) r[j] /= piv;
+#pragma unroll
+      for (int l = j + 1; l < FNB; ++l) {
+        const double lj = __shfl(r[j], l, 64);  // L[l][j]
+        if (l <= i) r[l] -= r[j] * lj;
+      }
     }
-    const double piv = sqrt(d);
-    __syncthreads();
-    if (i == j) a[j][j] = piv;
-    if (i > j && i < nb) a[i][j] /= piv;
-    __syncthreads();
-    if (i > j && i < nb) {
-      const double aij = a[i][j];
-      for (int l = j + 1; l <= i; ++l) a[i][l] -= aij * a[l][j];
-    }
-    __syncthreads();
   }
   if (bad && i == 0) f.failed[f.body[fr]] = 1;
-  // inverse of the lower factor, one column per lane
+#pragma unroll
+  for (int l = 0; l < FNB; ++l) Ls[i][l] = l <= i ? r[l] : 0.0;
   if (i < nb) {
-    for (int r = 0; r < i; ++r) x[r][i] = 0.0;
-    x[i][i] = 1.0 / a[i][i];
-    for (int r = i + 1; r < nb; ++r) {
-      double s = 0.0;
-      for (int q = i; q < r; ++q) s += a[r][q] * x[q][i];
-      x[r][i] = -s / a[r][r];
-    }
+#pragma unroll
+    for (int l = 0; l < FNB; ++l)
+      if (l <= i && l < nb) A[uint64_t(P0 + l) * m + P0 + i] = r[l];
   }
   __syncthreads();
+  // column c = i of L^-1: x[q] = 0 above the diagonal, 1 / L[c][c] on it,
+  // -(sum_{c<=q<t} L[t][q] x[q]) / L[t][t] below
+  double x[FNB];
+#pragma unroll
+  for (int t = 0; t < FNB; ++t) {
+    double s = 0.0;
+#pragma unroll
+    for (int q = 0; q < t; ++q)
+      if (q >= i) s += Ls[t][q] * x[q];
+    x[t] = t < i ? 0.0 : (t == i ? 1.0 : -s) / Ls[t][t];
+  }
   double *M = f.minv + f.minv_off[fr];
-  for (int j = 0; j < nb; ++j)
-    if (i >= j && i < nb) {
-      A[uint64_t(P0 + j) * m + P0 + i] = a[i][j];
-      M[uint64_t(P0 + j) * k + P0 + i] = x[i][j];
-    }
+  // M11 column i holds x; lanes write row t of the block together
+  __syncthreads();
+#pragma unroll
+  for (int t = 0; t < FNB; ++t) Ls[t][i] = x[t];
+  __syncthreads();
+  for (int c = 0; c < nb; ++c)
+    if (i >= c && i < nb) M[uint64_t(P0 + c) * k + P0 + i] = Ls[i][c];
 }
 
 // --------------------------------------------------------- 64x64 tile GEMM
