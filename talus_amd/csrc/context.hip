@@ -528,6 +528,7 @@ class Engine final : public EngineBase {
   double *scratch64_ = nullptr;
   bool any_plastic_ = false, any_tear_ = false, any_damped_ = false;
   uint64_t n_snh_ = 0, n_plastic_ = 0, n_flowing_ = 0;  // flowing: plastic neo-Hookean
+  uint64_t tn_total_ = 0;  // entries of the tet tiles' node lists
   uint32_t applications_ = 0;  // preconditioner applications this step
   // host copies of the incidence CSR: node ranges and the CSR position of
   // every (tet, corner) slot code 4e+c
@@ -798,6 +799,7 @@ void Engine<Real>::build(const std::vector<Body> &bodies) {
     d_.tn_node = dev_copy(tn_node);
     d_.conn_l = dev_copy(conn_l);
     d_.tn_max = tn_max;
+    tn_total_ = tn_node.size();
   }
   d_.dminv = dev_copy(dminv);
   if constexpr (sizeof(Real) == sizeof(double)) {
@@ -2191,10 +2193,12 @@ void Engine<Real>::time_kernel(uint32_t kind, uint32_t repeats, talus_mi_kernel_
     const double snh = double(n_snh_), plastic = double(n_plastic_);
     if (kind == 0)
       out.bytes = T * (tet + 9.0 * sz + 12.0 * sz) + snh * 9.0 * sz + plastic * (80.0 + 16.0) + N * 3.0 * sz;
-    else if (kind == 1)  // the operator reads Dm^-1 in the element precision, and the FP64
-                         // plastic offsets Fp^-1 - I of plastic neo-Hookean tets
-      out.bytes = T * (tet - 72.0 + 9.0 * sz + 9.0 * sz + 12.0 * sz) + snh * 9.0 * sz + double(n_flowing_) * 72.0 +
-                  N * 3.0 * sz;
+    else if (kind == 1)  // the operator reads Dm^-1 in the element precision, the FP64
+                         // plastic offsets Fp^-1 - I of plastic neo-Hookean tets, and
+                         // its corners as 8-byte tile positions plus each tile's node
+                         // list (4 bytes a node) instead of the 16-byte connectivity
+      out.bytes = T * (tet - 72.0 - 16.0 + 8.0 + 9.0 * sz + 9.0 * sz + 12.0 * sz) + snh * 9.0 * sz +
+                  double(n_flowing_) * 72.0 + 4.0 * double(tn_total_ + d_.n_ttiles + 1) + N * 3.0 * sz;
     else
       out.bytes = T * 12.0 * sz + (N + 1.0) * 4.0 + N * (6.0 * sz + 8.0 + 1.0);
   } else if (kind <= 8) {
