@@ -106,3 +106,33 @@ def test_one_plastic_pass_takes_the_elastic_scheme():
         x, _, _ = ctx.state(b)
     assert r["plastic_solves"] == ro["plastic_solves"] == 0 and r["solves"] == 3
     assert np.abs(x - orc.state()[0]).max() <= 1e-10
+
+
+@pytest.mark.parametrize("precision", [api.FP64, api.MIXED])
+def test_refinement_beside_newton_bodies(precision):
+    """FP64 refinement refines the three-solve bodies' solves and leaves the
+    Newton bodies' own solves alone: batched bodies still equal each body
+    stepped alone, and the yielding corotated body's state is the one it
+    reaches without refinement."""
+    bodies = _batch()
+    base = dict(substeps=2, preconditioner=api.FACTOR, tolerance=1e-6, cg_iterations=500, factor_budget=500)
+    refined = api.default_params(refine_tolerance=1e-10, **base)
+    plain = api.default_params(**base)
+    alone = []
+    for body in bodies:
+        with api.Context(0, precision) as ctx:
+            b, _ = product_body(ctx, body)
+            reps = [ctx.step(refined) for _ in range(4)]
+            alone.append(ctx.state(b)[0])
+    with api.Context(0, precision) as ctx:
+        ids = [product_body(ctx, body)[0] for body in bodies]
+        reps = [ctx.step(refined) for _ in range(4)]
+        for i, b in enumerate(ids):
+            assert np.array_equal(ctx.state(b)[0], alone[i]), i
+        assert all(r["refine_unmet"] == 0 for r in reps)
+        assert sum(r["refinements"] for r in reps) > 0 and sum(r["plastic_solves"] for r in reps) > 0
+    with api.Context(0, precision) as ctx:
+        b, _ = product_body(ctx, bodies[0])
+        for _ in range(4):
+            ctx.step(plain)
+        assert np.array_equal(ctx.state(b)[0], alone[0])
