@@ -561,6 +561,20 @@ struct TilePrefetch {
 #pragma unroll
     for (uint32_t u = 0; u < PF; ++u) c[u] = load_chunk(g + uint64_t(min(b0 + u, last)) * ts);
   }
+  // the same for a lane whose tiles outside [lo, hi] lie above the diagonal
+  // of M11 (row-tiled copy, lane = column j: b < lo = j / W; column-tiled
+  // copy, lane = row i < k: b > hi = i / W).  Those hold zeros and are read
+  // at tile zt instead, which the caller picks among the lane's zero tiles
+  // whenever there are any, so the lane sums the same values while the
+  // repeats of one tile hit in cache instead of fetching zeros from HBM
+  __device__ __forceinline__ void issue(const G *g, uint64_t ts, uint32_t b0, uint32_t last, uint32_t lo,
+                                        uint32_t hi, uint32_t zt) {
+#pragma unroll
+    for (uint32_t u = 0; u < PF; ++u) {
+      const uint32_t b = b0 + u;
+      c[u] = load_chunk(g + uint64_t(b < lo || b > hi ? zt : min(b, last)) * ts);
+    }
+  }
   __device__ __forceinline__ void dot(const G *g, uint64_t ts, const double *w, uint32_t b0, uint32_t b1,
                                       double *acc) const {
     constexpr uint32_t W = Tile<G>::W;
@@ -756,7 +770,7 @@ __global__ __launch_bounds__(NT) void k_solve_forward_lanes(FactorDev f, const t
   const double own = (i >= k && i < m) ? gather_row(f, fm.rows_off, i, k, rin) : 0.0;
   for (uint32_t q = threadIdx.x; q < kp; q += NT) vs[q] = q < k ? gather_row(f, fm.rows_off, q, k, rin) * sc[q] : 0.0;
   TilePrefetch<G> pre;
-  pre.issue(g, uint64_t(m) * W, 0, (k - 1) / W);
+  pre.issue(g, uint64_t(m) * W, 0, (k - 1) / W, 0, i < k ? i / W : (k - 1) / W, (k - 1) / W);
   __syncthreads();
   if (first >= m) return;
   double acc[4] = {0.0, 0.0, 0.0, 0.0};
@@ -841,7 +855,7 @@ __global__ __launch_bounds__(NT) void k_solve_backward_lanes(FactorDev f, const 
   WStage st;
   st.load(y, f.xsol, rows, k, m, mp, c0, j, f.gscale + fm.yoff);
   TilePrefetch<G> pre;
-  pre.issue(g, uint64_t(k) * W, tb0, mp / W - 1);
+  pre.issue(g, uint64_t(k) * W, tb0, mp / W - 1, min(j, k - 1) / W, mp / W - 1, tb0);
   st.store(ws, mp);
   st.rest(ws, y, f.xsol, rows, k, m, mp);
   __syncthreads();
@@ -882,7 +896,7 @@ __global__ __launch_bounds__(NT) void k_solve_forward_split(FactorDev f, const t
   const double own = (wave == 0 && i >= k && i < m) ? gather_row(f, fm.rows_off, i, k, rin) : 0.0;
   for (uint32_t q = threadIdx.x; q < nb * W; q += NT) vs[q] = q < k ? gather_row(f, fm.rows_off, q, k, rin) * sc[q] : 0.0;
   TilePrefetch<G> pre;
-  pre.issue(g, uint64_t(m) * W, b0, (k - 1) / W);
+  pre.issue(g, uint64_t(m) * W, b0, (k - 1) / W, 0, i < k ? i / W : (k - 1) / W, (k - 1) / W);
   __syncthreads();
   double acc[4] = {0.0, 0.0, 0.0, 0.0};
   pre.dot(g, uint64_t(m) * W, vs, b0, b1, acc);
@@ -925,7 +939,7 @@ __global__ __launch_bounds__(NT) void k_solve_backward_split(FactorDev f, const 
   WStage st;
   st.load(y, f.xsol, rowsf, k, m, mp, first, j, f.gscale + fm.yoff);
   TilePrefetch<G> pre;
-  pre.issue(g, uint64_t(k) * W, b0, mp / W - 1);
+  pre.issue(g, uint64_t(k) * W, b0, mp / W - 1, min(j, k - 1) / W, mp / W - 1, t0);
   st.store(ws, mp);
   st.rest(ws, y, f.xsol, rowsf, k, m, mp);
   __syncthreads();
